@@ -1,0 +1,390 @@
+// Cooperative (workgroup-per-shell-quartet) ERI kernel for gfx950, Boys device
+// function and the C ABI declared in include/gbasis_amd.h.
+//
+// Algorithm: Obara-Saika vertical recursion on A, Obara-Saika build of C,
+// contraction over primitive quartets into FP64 accumulators, Head-Gordon-Pople
+// horizontal recursion on D then B, factorial2 component norms, per-shell
+// output transform (norm_cont, component order, Cartesian->pure) and placement
+// into the canonical 8-fold packed array -- the stage order of
+// gbasis/integrals/_two_elec_int_improved.py:591-732 and
+// gbasis/base_four_symm.py:160-389.
+//
+// The recursion is driven by a per-class "program" (lists of independent items
+// per phase, built on the host).  A workgroup of 256 lanes processes PF
+// primitive quartets at once: every intermediate is stored slot-major with the
+// PF copies adjacent, so the 64 lanes of a wavefront that work on the same item
+// for consecutive copies touch consecutive LDS words.  Classes whose working set
+// does not fit in 64 KiB of LDS run the same code on a per-workgroup slice of a
+// global scratch buffer.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <string>
+
+#include "gbasis_amd.h"
+#include "gbx_common.h"
+
+namespace gbx {
+
+constexpr int kCoopThreads = 256;
+constexpr int kPhVrr = 0, kPhKvrr = 1, kPhAcc = 2, kPhHrrK = 3, kPhHrrB = 4;
+constexpr int kScalars = 17;  // per-primitive-quartet scalars held in LDS
+
+struct CoopArgs {
+  const int32_t* quart;
+  int64_t nquart;
+  const int32_t* items;
+  const int32_t* phases;
+  const int32_t* out_slots;
+  int32_t prim_slots, nacc, h_slots, nout, n_prim_ph, n_ph, L;
+  int32_t la, lb, lc, ld;
+  int32_t pf, mode;
+  double* out;
+  char* scratch;
+  int64_t stride;
+  int64_t sc_off, acc_off, un_off, h_off, f0_off, f1_off;
+};
+
+struct CoopLayout {
+  int64_t sc_off, acc_off, un_off, h_off, f0_off, f1_off, total;
+};
+
+inline int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
+
+inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf) {
+  const int64_t prim_slots = hdr[0], nacc = hdr[1], h_slots = hdr[2], nout = hdr[3];
+  CoopLayout l{};
+  l.sc_off = 0;
+  l.acc_off = align16(l.sc_off + kScalars * pf * tb_bytes);
+  l.un_off = align16(l.acc_off + nacc * 8);
+  const int64_t prim_bytes = prim_slots * pf * tb_bytes;
+  l.h_off = l.un_off;
+  l.f0_off = align16(l.h_off + h_slots * tb_bytes);
+  l.f1_off = align16(l.f0_off + nout * 8);
+  const int64_t epi_end = align16(l.f1_off + nout * 8);
+  l.total = std::max(align16(l.un_off + prim_bytes), epi_end);
+  return l;
+}
+
+__device__ inline uint64_t tri(uint64_t i, uint64_t j) {
+  return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+template <typename TB, bool kLDS>
+__global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, CoopArgs a) {
+  extern __shared__ __align__(16) char smem[];
+  char* base = kLDS ? smem : (a.scratch + static_cast<int64_t>(blockIdx.x) * a.stride);
+  TB* sc = reinterpret_cast<TB*>(base + a.sc_off);
+  double* acc = reinterpret_cast<double*>(base + a.acc_off);
+  TB* S = reinterpret_cast<TB*>(base + a.un_off);
+  TB* H = reinterpret_cast<TB*>(base + a.h_off);
+  double* F0 = reinterpret_cast<double*>(base + a.f0_off);
+  double* F1 = reinterpret_cast<double*>(base + a.f1_off);
+  __shared__ double s_red[kCoopThreads / 64];
+
+  const int tid = threadIdx.x;
+  const int PF = a.pf;
+  const int4* items4 = reinterpret_cast<const int4*>(a.items);
+  const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
+
+  for (int64_t qi = blockIdx.x; qi < a.nquart; qi += gridDim.x) {
+    const int bra = a.quart[2 * qi], ket = a.quart[2 * qi + 1];
+    const int4 pb = pinfo[bra];
+    const int4 pk = pinfo[ket];
+    const int nk = pk.w;
+    const int nprimq = pb.w * nk;
+
+    __syncthreads();  // previous quartet's epilogue is done with the workspace
+    for (int f = tid; f < a.nacc; f += kCoopThreads) acc[f] = 0.0;
+
+    for (int p0 = 0; p0 < nprimq; p0 += PF) {
+      const int ncopy = min(PF, nprimq - p0);
+      __syncthreads();
+      if (tid < ncopy) {
+        const int p = p0 + tid;
+        const int ib = p / nk, ik = p - ib * nk;
+        const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
+        const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
+        PrimQuartet<double> s;
+        const double Tval = prim_quartet_setup(rb, rk, s);
+        sc[0 * PF + tid] = static_cast<TB>(s.PA[0]);
+        sc[1 * PF + tid] = static_cast<TB>(s.PA[1]);
+        sc[2 * PF + tid] = static_cast<TB>(s.PA[2]);
+        sc[3 * PF + tid] = static_cast<TB>(s.WP[0]);
+        sc[4 * PF + tid] = static_cast<TB>(s.WP[1]);
+        sc[5 * PF + tid] = static_cast<TB>(s.WP[2]);
+        sc[6 * PF + tid] = static_cast<TB>(s.roz);
+        sc[7 * PF + tid] = static_cast<TB>(s.i2z);
+        sc[8 * PF + tid] = static_cast<TB>(s.QC[0]);
+        sc[9 * PF + tid] = static_cast<TB>(s.QC[1]);
+        sc[10 * PF + tid] = static_cast<TB>(s.QC[2]);
+        sc[11 * PF + tid] = static_cast<TB>(s.WQ[0]);
+        sc[12 * PF + tid] = static_cast<TB>(s.WQ[1]);
+        sc[13 * PF + tid] = static_cast<TB>(s.WQ[2]);
+        sc[14 * PF + tid] = static_cast<TB>(s.roe);
+        sc[15 * PF + tid] = static_cast<TB>(s.i2e);
+        sc[16 * PF + tid] = static_cast<TB>(s.i2s);
+        const double pref = s.pref;
+        boys_eval(Tval, a.L, t, [&](int m, double v) { S[m * PF + tid] = static_cast<TB>(v * pref); });
+      }
+      __syncthreads();
+      for (int ph = 0; ph < a.n_prim_ph; ++ph) {
+        const int kind = a.phases[3 * ph], i0 = a.phases[3 * ph + 1], i1 = a.phases[3 * ph + 2];
+        const int nwork = (i1 - i0) * PF;
+        for (int x = tid; x < nwork; x += kCoopThreads) {
+          const int it = i0 + x / PF;
+          const int c = x - (x / PF) * PF;
+          if (c >= ncopy) continue;
+          const int4 d0 = items4[2 * it];
+          const int4 d1 = items4[2 * it + 1];
+          const int code = d1.z;
+          const int dir = code & 15;
+          const TB k1 = static_cast<TB>((code >> 4) & 255);
+          TB v;
+          if (kind == kPhVrr) {
+            // [a+1_i]^(m) = PA_i [a]^(m) + WP_i [a]^(m+1) + a_i/(2z) ([a-1_i]^(m) - rho/z [a-1_i]^(m+1))
+            v = sc[dir * PF + c] * S[d0.y * PF + c] + sc[(3 + dir) * PF + c] * S[d0.z * PF + c];
+            if (d0.w >= 0)
+              v += k1 * sc[7 * PF + c] * (S[d0.w * PF + c] - sc[6 * PF + c] * S[d1.x * PF + c]);
+          } else {
+            // [a|c+1_i]^(m) = QC_i [a|c]^(m) + WQ_i [a|c]^(m+1)
+            //   + c_i/(2e) ([a|c-1_i]^(m) - rho/e [a|c-1_i]^(m+1)) + a_i/(2(z+e)) [a-1_i|c]^(m+1)
+            v = sc[(8 + dir) * PF + c] * S[d0.y * PF + c] +
+                sc[(11 + dir) * PF + c] * S[d0.z * PF + c];
+            if (d0.w >= 0)
+              v += k1 * sc[15 * PF + c] * (S[d0.w * PF + c] - sc[14 * PF + c] * S[d1.x * PF + c]);
+            if (d1.y >= 0) v += static_cast<TB>(code >> 12) * sc[16 * PF + c] * S[d1.y * PF + c];
+          }
+          S[d0.x * PF + c] = v;
+        }
+        __syncthreads();
+      }
+      {
+        const int i0 = a.phases[3 * a.n_prim_ph + 1];
+        for (int f = tid; f < a.nacc; f += kCoopThreads) {
+          const int src = a.items[8 * (i0 + f) + 1];
+          double sum = 0.0;
+          for (int c = 0; c < ncopy; ++c) sum += static_cast<double>(S[src * PF + c]);
+          acc[f] += sum;
+        }
+      }
+    }
+    __syncthreads();
+
+    // ---- horizontal recursion on the contracted values ----
+    const double ABx = t.pair_geom[4 * bra], ABy = t.pair_geom[4 * bra + 1],
+                 ABz = t.pair_geom[4 * bra + 2];
+    const double CDx = t.pair_geom[4 * ket], CDy = t.pair_geom[4 * ket + 1],
+                 CDz = t.pair_geom[4 * ket + 2];
+    for (int f = tid; f < a.nacc; f += kCoopThreads) H[f] = static_cast<TB>(acc[f]);
+    __syncthreads();
+    for (int ph = a.n_prim_ph + 1; ph < a.n_ph; ++ph) {
+      const int kind = a.phases[3 * ph], i0 = a.phases[3 * ph + 1], i1 = a.phases[3 * ph + 2];
+      const TB gx = static_cast<TB>(kind == kPhHrrK ? CDx : ABx);
+      const TB gy = static_cast<TB>(kind == kPhHrrK ? CDy : ABy);
+      const TB gz = static_cast<TB>(kind == kPhHrrK ? CDz : ABz);
+      for (int it = i0 + tid; it < i1; it += kCoopThreads) {
+        const int4 d0 = items4[2 * it];
+        const int dir = d0.w;
+        const TB g = dir == 0 ? gx : (dir == 1 ? gy : gz);
+        H[d0.x] = H[d0.y] + g * H[d0.z];
+      }
+      __syncthreads();
+    }
+
+    // ---- epilogue: factorial2 norms, output transforms, placement ----
+    const int nl[4] = {a.la, a.lb, a.lc, a.ld};
+    int nc[4];
+    for (int x = 0; x < 4; ++x) nc[x] = (nl[x] + 1) * (nl[x] + 2) / 2;
+    const double* f2a = t.f2norm + f2_offset(a.la);
+    const double* f2b = t.f2norm + f2_offset(a.lb);
+    const double* f2c = t.f2norm + f2_offset(a.lc);
+    const double* f2d = t.f2norm + f2_offset(a.ld);
+    for (int o = tid; o < a.nout; o += kCoopThreads) {
+      int r = o;
+      const int id = r % nc[3]; r /= nc[3];
+      const int ic = r % nc[2]; r /= nc[2];
+      const int ib = r % nc[1];
+      const int ia = r / nc[1];
+      F0[o] = static_cast<double>(H[a.out_slots[o]]) * f2a[ia] * f2b[ib] * f2c[ic] * f2d[id];
+    }
+    __syncthreads();
+
+    if (a.mode == GBX_MODE_SCHWARZ) {
+      double m = 0.0;
+      for (int o = tid; o < a.nout; o += kCoopThreads) m = fmax(m, fabs(F0[o]));
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      if ((tid & 63) == 0) s_red[tid >> 6] = m;
+      __syncthreads();
+      if (tid == 0) {
+        double mm = s_red[0];
+        for (int w = 1; w < kCoopThreads / 64; ++w) mm = fmax(mm, s_red[w]);
+        a.out[bra] = sqrt(mm);
+      }
+      continue;
+    }
+
+    const int sh[4] = {pb.x, pb.y, pk.x, pk.y};
+    int4 sinfo[4][2];
+    for (int x = 0; x < 4; ++x) {
+      const int4* si = reinterpret_cast<const int4*>(t.shell_info) + 2 * sh[x];
+      sinfo[x][0] = si[0];
+      sinfo[x][1] = si[1];
+    }
+    // dims of the current block; dense shells are transformed in place (ping-pong)
+    int dim[4] = {nc[0], nc[1], nc[2], nc[3]};
+    double* cur = F0;
+    double* nxt = F1;
+    for (int x = 0; x < 4; ++x) {
+      if (sinfo[x][0].z != 1) continue;  // kind 0: applied at placement
+      const int nout_x = sinfo[x][1].y;
+      const double* M = t.out_mat + sinfo[x][1].x;
+      int ndim[4] = {dim[0], dim[1], dim[2], dim[3]};
+      ndim[x] = nout_x;
+      int inner = 1;
+      for (int y = x + 1; y < 4; ++y) inner *= dim[y];
+      const int total = ndim[0] * ndim[1] * ndim[2] * ndim[3];
+      for (int e = tid; e < total; e += kCoopThreads) {
+        const int lo = e % inner;
+        const int jx = (e / inner) % nout_x;
+        const int hi = e / (inner * nout_x);
+        const double* src = cur + static_cast<int64_t>(hi) * dim[x] * inner + lo;
+        const double* mrow = M + jx * dim[x];
+        double s = 0.0;
+        for (int k = 0; k < dim[x]; ++k) s += mrow[k] * src[k * inner];
+        nxt[e] = s;
+      }
+      __syncthreads();
+      dim[x] = nout_x;
+      double* tmp = cur;
+      cur = nxt;
+      nxt = tmp;
+    }
+
+    const uint64_t obase = t.pair_out_base[bra];
+    const bool bra_diag = pb.x == pb.y, ket_diag = pk.x == pk.y, same_pair = bra == ket;
+    const int total = dim[0] * dim[1] * dim[2] * dim[3];
+    for (int e = tid; e < total; e += kCoopThreads) {
+      int idx[4];
+      int r = e;
+      idx[3] = r % dim[3]; r /= dim[3];
+      idx[2] = r % dim[2]; r /= dim[2];
+      idx[1] = r % dim[1];
+      idx[0] = r / dim[1];
+      double v = cur[e];
+      uint32_t fn[4];
+      for (int x = 0; x < 4; ++x) {
+        int pos = idx[x];
+        if (sinfo[x][0].z == 0) {
+          pos = t.comp_map[sinfo[x][0].w + idx[x]];
+          v *= t.comp_scale[sinfo[x][0].w + idx[x]];
+        }
+        fn[x] = static_cast<uint32_t>(sinfo[x][0].x + pos);
+      }
+      if (bra_diag && fn[0] < fn[1]) continue;
+      if (ket_diag && fn[2] < fn[3]) continue;
+      const uint64_t ij = tri(fn[0], fn[1]), kl = tri(fn[2], fn[3]);
+      if (same_pair && ij < kl) continue;
+      a.out[obase + tri(ij, kl)] = v;
+    }
+  }
+}
+
+template <typename TB, bool kLDS>
+int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, int lc, int ld,
+                const int32_t* d_quartets, int64_t nquart, int pf, int mode, double* d_out,
+                void* d_scratch, int64_t scratch_bytes, hipStream_t stream) {
+  const CoopLayout l = coop_layout(prog->header, sizeof(TB), pf);
+  CoopArgs a{};
+  a.quart = d_quartets;
+  a.nquart = nquart;
+  a.items = prog->items;
+  a.phases = prog->phases;
+  a.out_slots = prog->out_slots;
+  a.prim_slots = prog->header[0];
+  a.nacc = prog->header[1];
+  a.h_slots = prog->header[2];
+  a.nout = prog->header[3];
+  a.n_prim_ph = prog->header[4];
+  a.n_ph = prog->header[5];
+  a.L = prog->header[6];
+  a.la = la;
+  a.lb = lb;
+  a.lc = lc;
+  a.ld = ld;
+  a.pf = pf;
+  a.mode = mode;
+  a.out = d_out;
+  a.sc_off = l.sc_off;
+  a.acc_off = l.acc_off;
+  a.un_off = l.un_off;
+  a.h_off = l.h_off;
+  a.f0_off = l.f0_off;
+  a.f1_off = l.f1_off;
+  int64_t grid = nquart;
+  size_t shmem = 0;
+  if (kLDS) {
+    shmem = static_cast<size_t>(l.total);
+    grid = std::min<int64_t>(grid, int64_t(1) << 20);
+  } else {
+    a.scratch = static_cast<char*>(d_scratch);
+    a.stride = l.total;
+    const int64_t nblk = scratch_bytes / l.total;
+    if (nblk < 1) return fail(GBX_ERR_ARG, "gbx_eri_coop: scratch buffer smaller than one workgroup slice");
+    grid = std::min<int64_t>(grid, nblk);
+  }
+  hipLaunchKernelGGL((eri_coop_kernel<TB, kLDS>), dim3(static_cast<unsigned>(grid)),
+                     dim3(kCoopThreads), shmem, stream, *t, a);
+  return check_launch("eri_coop_kernel");
+}
+
+}  // namespace gbx
+
+using namespace gbx;
+
+extern "C" int gbx_coop_workspace(const gbx_program* prog, int32_t la, int32_t lb, int32_t lc,
+                                  int32_t ld, int32_t precision, int32_t prim_copies,
+                                  int64_t* lds_bytes, int64_t* global_bytes_per_block) {
+  if (!prog || prim_copies < 1 || prim_copies > kCoopThreads)
+    return fail(GBX_ERR_ARG, "gbx_coop_workspace: bad arguments");
+  (void)la; (void)lb; (void)lc; (void)ld;
+  const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
+  const bool fits = l.total <= kCoopLdsLimit;
+  if (lds_bytes) *lds_bytes = fits ? l.total : 0;
+  if (global_bytes_per_block) *global_bytes_per_block = fits ? 0 : l.total;
+  return GBX_OK;
+}
+
+extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_t la, int32_t lb,
+                            int32_t lc, int32_t ld, const int32_t* d_quartets, int64_t nquart,
+                            int32_t precision, int32_t prim_copies, int32_t mode, double* d_out,
+                            void* d_scratch, int64_t scratch_bytes, void* stream) {
+  if (!t || !prog || !d_out || (nquart > 0 && !d_quartets))
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: null argument");
+  if (prim_copies < 1 || prim_copies > kCoopThreads)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be in [1, 256]");
+  if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: class not in canonical orientation");
+  if (prog->header[6] != la + lb + lc + ld || prog->header[6] + kTaylorTerms > t->boys_nord)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: program / Boys table do not match the class");
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: bad mode");
+  if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: bad precision");
+  if (nquart == 0) return GBX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
+  const bool lds = l.total <= kCoopLdsLimit;
+  if (precision == GBX_FP64) {
+    return lds ? launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, prim_copies,
+                                           mode, d_out, d_scratch, scratch_bytes, s)
+               : launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart,
+                                            prim_copies, mode, d_out, d_scratch, scratch_bytes, s);
+  }
+  return lds ? launch_coop<float, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, prim_copies,
+                                        mode, d_out, d_scratch, scratch_bytes, s)
+             : launch_coop<float, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, prim_copies,
+                                         mode, d_out, d_scratch, scratch_bytes, s);
+}

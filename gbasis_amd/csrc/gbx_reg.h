@@ -1,0 +1,119 @@
+// Shared pieces of the per-class register kernels (one lane = one shell quartet):
+// launch arguments, a fixed-order Boys evaluation and the block store.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gbasis_amd.h"
+#include "gbx_common.h"
+
+namespace gbx {
+
+constexpr int kRegThreads = 256;
+
+struct RegArgs {
+  const int32_t* quart;  // (bra pair, ket pair) per quartet
+  int64_t nquart;
+  int32_t mode;          // GBX_MODE_PACKED or GBX_MODE_SCHWARZ
+  double* out;
+};
+
+// F_m(T), m = 0..L, into F[] (same algorithm as boys_eval, unrolled for fixed L).
+template <int L>
+__device__ inline void boys_eval_fixed(double T, const gbx_tables& t, double* F) {
+  const double tmax = t.boys_h * (t.boys_nt - 1);
+  const double emT = exp(-T);
+  if (T <= tmax) {
+    const int k = static_cast<int>(T / t.boys_h + 0.5);
+    const double x = k * t.boys_h - T;
+    const double* row = t.boys_table + static_cast<int64_t>(k) * t.boys_nord + L;
+    double h = row[7];
+    h = row[6] + x * h * (1.0 / 7.0);
+    h = row[5] + x * h * (1.0 / 6.0);
+    h = row[4] + x * h * (1.0 / 5.0);
+    h = row[3] + x * h * (1.0 / 4.0);
+    h = row[2] + x * h * (1.0 / 3.0);
+    h = row[1] + x * h * (1.0 / 2.0);
+    F[L] = row[0] + x * h;
+    const double twoT = 2.0 * T;
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) F[m] = (twoT * F[m + 1] + emT) * (1.0 / (2 * m + 1));
+  } else {
+    const double inv2T = 0.5 / T;
+    F[0] = 0.5 * sqrt(M_PI / T);
+#pragma unroll
+    for (int m = 0; m < L; ++m) F[m + 1] = (static_cast<double>(2 * m + 1) * F[m] - emT) * inv2T;
+  }
+}
+
+__device__ inline uint64_t tri_u64(uint64_t i, uint64_t j) {
+  return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
+}
+
+// Write one canonical-order Cartesian block (NA x NB x NC x ND, factorial2 norms
+// applied) through the per-shell maps (component position, norm_cont / sign scale)
+// into the canonical packed slots, skipping the duplicates of diagonal pairs and
+// of bra == ket.  In GBX_MODE_SCHWARZ write sqrt(max |block|) for the bra pair.
+template <int NA, int NB, int NC, int ND>
+__device__ inline void store_block(const gbx_tables& t, const RegArgs& a, int bra, int ket,
+                                   int4 pb, int4 pk, const double* out) {
+  constexpr int N = NA * NB * NC * ND;
+  if (a.mode == GBX_MODE_SCHWARZ) {
+    double m = 0.0;
+#pragma unroll
+    for (int i = 0; i < N; ++i) m = fmax(m, fabs(out[i]));
+    a.out[bra] = sqrt(m);
+    return;
+  }
+  const int4* si = reinterpret_cast<const int4*>(t.shell_info);
+  const int4 sa = si[2 * pb.x], sb = si[2 * pb.y], sc = si[2 * pk.x], sd = si[2 * pk.y];
+  uint32_t fa[NA], fb[NB], fc[NC], fd[ND];
+  double ca[NA], cb[NB], cc[NC], cd[ND];
+#pragma unroll
+  for (int i = 0; i < NA; ++i) {
+    fa[i] = static_cast<uint32_t>(sa.x + t.comp_map[sa.w + i]);
+    ca[i] = t.comp_scale[sa.w + i];
+  }
+#pragma unroll
+  for (int i = 0; i < NB; ++i) {
+    fb[i] = static_cast<uint32_t>(sb.x + t.comp_map[sb.w + i]);
+    cb[i] = t.comp_scale[sb.w + i];
+  }
+#pragma unroll
+  for (int i = 0; i < NC; ++i) {
+    fc[i] = static_cast<uint32_t>(sc.x + t.comp_map[sc.w + i]);
+    cc[i] = t.comp_scale[sc.w + i];
+  }
+#pragma unroll
+  for (int i = 0; i < ND; ++i) {
+    fd[i] = static_cast<uint32_t>(sd.x + t.comp_map[sd.w + i]);
+    cd[i] = t.comp_scale[sd.w + i];
+  }
+  const bool bdiag = pb.x == pb.y, kdiag = pk.x == pk.y, same = bra == ket;
+  double* base = a.out + t.pair_out_base[bra];
+#pragma unroll
+  for (int ia = 0; ia < NA; ++ia)
+#pragma unroll
+    for (int ib = 0; ib < NB; ++ib) {
+      if (bdiag && fa[ia] < fb[ib]) continue;
+      const uint64_t ij = tri_u64(fa[ia], fb[ib]);
+#pragma unroll
+      for (int ic = 0; ic < NC; ++ic)
+#pragma unroll
+        for (int id = 0; id < ND; ++id) {
+          if (kdiag && fc[ic] < fd[id]) continue;
+          const uint64_t kl = tri_u64(fc[ic], fd[id]);
+          if (same && ij < kl) continue;
+          base[tri_u64(ij, kl)] =
+              out[((ia * NB + ib) * NC + ic) * ND + id] * ca[ia] * cb[ib] * cc[ic] * cd[id];
+        }
+    }
+}
+
+int launch_reg_generated(const gbx_tables* t, int la, int lb, int lc, int ld, const RegArgs& a,
+                         int precision, hipStream_t s);
+bool has_reg_generated(int la, int lb, int lc, int ld);
+
+}  // namespace gbx

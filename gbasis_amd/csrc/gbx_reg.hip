@@ -1,0 +1,27 @@
+// C-ABI entry points of the per-class register kernels (the kernels themselves
+// are generated into gbx_reg_gen.hip by gbasis_amd/codegen.py).
+#include "gbx_reg.h"
+
+extern "C" int gbx_has_reg_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld) {
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > 7 || lb > 7 || lc > 7 || ld > 7) return 0;
+  return gbx::has_reg_generated(la, lb, lc, ld) ? 1 : 0;
+}
+
+extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                           const int32_t* d_quartets, int64_t nquart, int32_t precision,
+                           int32_t mode, double* d_out, void* stream) {
+  if (!t || !d_out || (nquart > 0 && !d_quartets))
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: null argument");
+  if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: bad precision");
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: bad mode");
+  if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
+    return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg: no register kernel for this class");
+  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table too short for this class");
+  if (nquart == 0) return GBX_OK;
+  gbx::RegArgs a{d_quartets, nquart, mode, d_out};
+  return gbx::launch_reg_generated(t, la, lb, lc, ld, a, precision,
+                                   static_cast<hipStream_t>(stream));
+}

@@ -1,0 +1,279 @@
+"""Device residency and kernel dispatch for the ERI path.
+
+``DeviceTables`` holds one batch of molecules' tables in HBM (torch tensors are
+only the allocation mechanism; every computation runs in the HIP kernels of
+``libgbasis_amd.so``).  ``EriEngine`` runs the three device stages:
+
+1. Schwarz bounds Q_ij = sqrt(max |(ij|ij)|) for every shell pair
+   (gbasis/integrals/_schwarz_screening.py:14-90), one launch per class;
+2. host screening / class bucketing of the unique quartets (tables.py);
+3. one launch per (class, primitive-count bin, precision) writing the
+   canonical packed array.
+"""
+
+import ctypes
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_SCHWARZ, GbxProgram,
+                   GbxTables, check, ptr)
+from .boys import BOYS_H, boys_table
+from .programs import class_program
+from .roofline import quartet_flops
+from .tables import BatchTables, MoleculeTables, f2norm_table
+
+DEFAULT_SCRATCH_BYTES = 1 << 30
+
+
+@dataclass
+class Bucket:
+    """One launch: quartets of one class / precision / primitive-count bin."""
+    cls: tuple
+    nprimq: int          # largest primitive-quartet count in the bucket
+    precision: int       # GBX_FP64 or GBX_FP32_BULK
+    dense: bool          # needs a dense Cartesian->pure output transform
+    quart: object        # device int32 (n, 2)
+    n: int               # shell quartets
+    eris: int            # canonical packed slots written
+    flops: float         # algorithmic FLOPs (roofline.quartet_flops)
+    path: str = ""       # "reg" or "coop" once launched
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("gbasis_amd: no HIP device visible; the ERI path runs only on the GPU")
+
+
+class ProgramCache:
+    """Class programs resident on the device, built once per class."""
+
+    def __init__(self, device):
+        self.device = device
+        self._cache = {}
+
+    def get(self, cls):
+        if cls not in self._cache:
+            prog = class_program(*cls)
+            header, items, phases = prog.encode()
+            d_items = torch.from_numpy(np.ascontiguousarray(items) if items.size else
+                                       np.zeros(8, np.int32)).to(self.device)
+            d_phases = torch.from_numpy(phases).to(self.device)
+            d_out = torch.from_numpy(prog.out_slots).to(self.device)
+            st = GbxProgram()
+            st.items = ptr(d_items)
+            st.phases = ptr(d_phases)
+            st.out_slots = ptr(d_out)
+            for i in range(16):
+                st.header[i] = int(header[i])
+            self._cache[cls] = (st, (d_items, d_phases, d_out), prog)
+        return self._cache[cls]
+
+
+class DeviceTables:
+    """A BatchTables object copied to the device, plus its gbx_tables struct."""
+
+    def __init__(self, batch: BatchTables, device="cuda"):
+        _require_gpu()
+        self.batch = batch
+        self.device = torch.device(device)
+
+        def up(a, dtype=None):
+            a = np.ascontiguousarray(a if dtype is None else a.astype(dtype))
+            return torch.from_numpy(a).to(self.device)
+
+        self.shell_info = up(batch.shell_info, np.int32)
+        self.comp_map = up(batch.comp_map, np.int32)
+        self.comp_scale = up(batch.comp_scale, np.float64)
+        self.out_mat = up(batch.out_mat, np.float64)
+        self.pair_info = up(batch.pair_info, np.int32)
+        self.pair_geom = up(batch.pair_geom, np.float64)
+        self.pair_out_base = up(batch.pair_out_base.view(np.int64))
+        self.prim_pairs = up(batch.prim_pairs, np.float64)
+        self.f2norm = up(f2norm_table(8))
+        tab = boys_table()
+        self.boys = up(tab)
+        st = GbxTables()
+        st.shell_info = ptr(self.shell_info)
+        st.comp_map = ptr(self.comp_map)
+        st.comp_scale = ptr(self.comp_scale)
+        st.out_mat = ptr(self.out_mat)
+        st.pair_info = ptr(self.pair_info)
+        st.pair_geom = ptr(self.pair_geom)
+        st.pair_out_base = ptr(self.pair_out_base)
+        st.prim_pairs = ptr(self.prim_pairs)
+        st.f2norm = ptr(self.f2norm)
+        st.boys_table = ptr(self.boys)
+        st.boys_nt = tab.shape[0]
+        st.boys_nord = tab.shape[1]
+        st.boys_h = BOYS_H
+        self.struct = st
+
+
+class EriEngine:
+    """Runs the ERI kernels for one device; holds programs and scratch."""
+
+    def __init__(self, device="cuda", scratch_bytes=DEFAULT_SCRATCH_BYTES):
+        _require_gpu()
+        self.lib = _lib.load()
+        self.device = torch.device(device)
+        self.programs = ProgramCache(self.device)
+        self.scratch_bytes = int(scratch_bytes)
+        self._scratch = None
+        self._reg = {}
+        self.path = "auto"   # "coop" forces the cooperative kernel for every class
+
+    def scratch(self):
+        if self._scratch is None:
+            self._scratch = torch.empty(self.scratch_bytes, dtype=torch.uint8, device=self.device)
+        return self._scratch
+
+    def _stream(self):
+        return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _plan(self, cls, nprimq, precision):
+        """Primitive quartets processed at once (PF) and whether LDS suffices."""
+        st, _, _ = self.programs.get(cls)
+        pf = 1
+        while pf < min(nprimq, 64):
+            pf *= 2
+        lds = ctypes.c_int64()
+        glb = ctypes.c_int64()
+        while True:
+            check(self.lib.gbx_coop_workspace(ctypes.byref(st), *cls, precision, pf,
+                                              ctypes.byref(lds), ctypes.byref(glb)),
+                  "gbx_coop_workspace")
+            if lds.value > 0 or pf == 1:
+                break
+            pf //= 2
+        if lds.value == 0:
+            # global-scratch mode: keep enough copies to occupy the workgroup
+            pf = max(1, min(16, 1 << int(np.ceil(np.log2(max(nprimq, 1))))))
+        return pf
+
+    def has_reg(self, cls):
+        if cls not in self._reg:
+            self._reg[cls] = bool(self.lib.gbx_has_reg_kernel(*cls))
+        return self._reg[cls]
+
+    def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,
+                     dense=False, path="auto"):
+        """One launch for a (class, bin) bucket; quartets is a device int32 (n,2).
+
+        Classes with a generated register kernel and no dense output transform run
+        one quartet per lane; all others run the cooperative kernel.
+        """
+        if path != "coop" and not dense and self.has_reg(cls):
+            check(self.lib.gbx_eri_reg(ctypes.byref(dev.struct), *cls, ptr(quartets),
+                                       quartets.shape[0], precision, mode, ptr(out),
+                                       self._stream()), f"gbx_eri_reg{cls}")
+            return "reg"
+        st, _, _ = self.programs.get(cls)
+        pf = self._plan(cls, nprimq, precision)
+        sc = self.scratch()
+        check(self.lib.gbx_eri_coop(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+                                    ptr(quartets), quartets.shape[0], precision, pf, mode,
+                                    ptr(out), ptr(sc), sc.numel(), self._stream()),
+              f"gbx_eri_coop{cls}")
+        return "coop"
+
+    def schwarz(self, dev: DeviceTables):
+        """Q per global shell pair (un-normalised, as the reference's screener)."""
+        b = dev.batch
+        npair = b.npair
+        q = torch.zeros(npair, dtype=torch.float64, device=self.device)
+        p = np.arange(npair, dtype=np.int32)
+        for cls, nprimq, _, quart in b.class_buckets(p, p):
+            dq = torch.from_numpy(quart).to(self.device)
+            # Q needs no output transform: the register kernels serve every class they cover
+            self.launch_class(dev, cls, dq, nprimq, GBX_FP64, GBX_MODE_SCHWARZ, q,
+                              path=self.path)
+        return q
+
+    def plan_quartets(self, dev: DeviceTables, q_pair=None, threshold=0.0, fp32_cutoff=0.0):
+        """Screened, bucketed quartet lists on the device.
+
+        Returns a list of Bucket.  Quartets whose bound Q_ij Q_kl is below
+        ``fp32_cutoff`` run with an FP32 recursion bulk; all others in FP64.
+        """
+        b = dev.batch
+        bras, kets = [], []
+        for mi, m in enumerate(b.mols):
+            po = b.mol_pair_off[mi]
+            qm = None if q_pair is None else q_pair[po:po + m.npair]
+            bra, ket = m.quartets(qm, threshold)
+            bras.append(bra.astype(np.int64) + po)
+            kets.append(ket.astype(np.int64) + po)
+        bra = np.concatenate(bras)
+        ket = np.concatenate(kets)
+        plans = []
+        if fp32_cutoff > 0:
+            if q_pair is None:
+                raise ValueError("mixed precision needs the Schwarz bounds")
+            qo = self._orig_bounds(b, q_pair)
+            bound = qo[bra] * qo[ket]
+            low = bound < fp32_cutoff
+            groups = [(GBX_FP32_BULK, low), (GBX_FP64, ~low)]
+        else:
+            groups = [(GBX_FP64, np.ones(bra.size, dtype=bool))]
+        nprim = b.pair_info[:, 3].astype(np.int64)
+        for prec, mask in groups:
+            if not mask.any():
+                continue
+            for cls, nprimq, dense, quart in b.class_buckets(bra[mask], ket[mask]):
+                qb, qk = quart[:, 0].astype(np.int64), quart[:, 1].astype(np.int64)
+                eris = int(b.unique_counts(qb, qk).sum())
+                flops = float(quartet_flops(cls, nprim[qb] * nprim[qk], pure=dense).sum())
+                plans.append(Bucket(cls, nprimq, prec, dense,
+                                    torch.from_numpy(quart).to(self.device), quart.shape[0],
+                                    eris, flops))
+        return plans
+
+    @staticmethod
+    def _orig_bounds(b, q_pair):
+        """Per global pair: bound of its original-shell pair (max over expanded pairs)."""
+        out = np.empty(b.npair)
+        for mi, m in enumerate(b.mols):
+            po = b.mol_pair_off[mi]
+            Qo = m.schwarz_orig(q_pair[po:po + m.npair])
+            oi, oj = m.pair_orig
+            out[po:po + m.npair] = Qo[oi, oj]
+        return out
+
+    def run(self, dev: DeviceTables, plans, out=None):
+        """Execute the plans into a zero-initialised packed array (allocated if None)."""
+        if out is None:
+            out = torch.zeros(dev.batch.n_slots, dtype=torch.float64, device=self.device)
+        for bk in plans:
+            bk.path = self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.precision,
+                                        GBX_MODE_PACKED, out, bk.dense, self.path)
+        return out
+
+    def unpack_dense(self, packed, nfunc, notation):
+        dense = torch.empty((nfunc,) * 4, dtype=torch.float64, device=self.device)
+        check(self.lib.gbx_unpack_dense(ptr(packed), nfunc, 1 if notation == "physicist" else 0,
+                                        ptr(dense), self._stream()), "gbx_unpack_dense")
+        return dense
+
+    def boys(self, dev: DeviceTables, T, mmax):
+        T = torch.as_tensor(T, dtype=torch.float64, device=self.device).contiguous()
+        out = torch.empty((T.numel(), mmax + 1), dtype=torch.float64, device=self.device)
+        check(self.lib.gbx_boys(ctypes.byref(dev.struct), ptr(T), T.numel(), mmax, ptr(out),
+                                self._stream()), "gbx_boys")
+        return out
+
+
+_ENGINES = {}
+
+
+def get_engine(device="cuda"):
+    key = str(torch.device(device))
+    if key not in _ENGINES:
+        _ENGINES[key] = EriEngine(device)
+    return _ENGINES[key]
+
+
+def molecule_tables(basis, coord_types=None):
+    return MoleculeTables(basis, coord_types)

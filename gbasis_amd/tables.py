@@ -1,0 +1,322 @@
+"""Host-side shell, shell-pair and shell-quartet tables.
+
+This is the host half of the ERI path.  It reads the duck-typed shell
+attributes the reference reads (gbasis/base_four_symm.py:160-389,
+gbasis/integrals/electron_repulsion.py:305-412): ``coord``, ``angmom``,
+``exps``, ``coeffs`` (K, M), ``norm_cont`` (M, L_cart),
+``angmom_components_cart``, ``angmom_components_sph`` and the per-shell
+coordinate type, and produces flat arrays for the device:
+
+* expanded shells: a generalised contraction with M columns becomes M
+  consecutive segmented shells, which reproduces the reference's m-major
+  function order (base_four_symm.py:207-213);
+* per-shell output maps: the kernels compute Cartesian components in the
+  canonical (reference default) order; each shell maps them to its own
+  component order, scales by ``norm_cont`` and, for pure shells with l >= 2,
+  applies the dense Cartesian->pure matrix of ``generate_transformation``;
+* shell pairs (bra orientation l_a >= l_b) with their primitive-pair records
+  {zeta, 1/(2 zeta), P, P-A, K', 0}, K' = c_a c_b N_a N_b exp(-ab/zeta |AB|^2)
+  sqrt(2) pi^(5/4) / zeta with N = (2a/pi)^(3/4) (4a)^(l/2)
+  (_two_elec_int_improved.py:44-84, :684-690);
+* screened, class-bucketed shell-quartet lists (the reference's unique
+  quartet loop, base_four_symm.py:189-194, with the Schwarz test of
+  _schwarz_screening.py:184-191).
+
+Packed output: every molecule owns a contiguous slice of canonical slots
+``ijkl = ij (ij+1)/2 + kl`` with ``ij = i (i+1)/2 + j``, i >= j, k >= l,
+ij >= kl over its output functions.
+"""
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from .programs import canonical_class, comps
+from .spherical import generate_transformation
+from .utils import factorial2
+
+SQRT2_PI_5_4 = np.sqrt(2.0) * np.pi ** 1.25
+
+
+def f2norm_table(lmax=8):
+    """1/sqrt(prod (2a_i-1)!!) for the canonical components of l = 0..lmax."""
+    vals = []
+    for l in range(lmax + 1):
+        c = np.array(comps(l))
+        vals.append(1.0 / np.sqrt(np.prod(factorial2(2 * c - 1), axis=1)))
+    return np.concatenate(vals)
+
+
+@dataclass
+class ExpandedShell:
+    orig: int           # index of the shell in the caller's basis
+    l: int
+    coord: np.ndarray
+    exps: np.ndarray
+    coeffs: np.ndarray  # (K,)
+    kind: int           # 0 = scaled permutation, 1 = dense Cartesian->pure matrix
+    nout: int
+    comp_map: np.ndarray    # (ncart,) int: output position of each canonical component
+    comp_scale: np.ndarray  # (ncart,) float
+    out_mat: np.ndarray = field(default=None)  # (nout, ncart) for kind 1
+    func_off: int = 0
+
+
+def _is_spherical(coord_type):
+    return coord_type in ("spherical", "p")
+
+
+def expand_shells(basis, coord_types):
+    """Expand generalised contractions and build per-shell output maps."""
+    out = []
+    func = 0
+    for idx, (shell, ctype) in enumerate(zip(basis, coord_types)):
+        l = int(shell.angmom)
+        cart = np.asarray(shell.angmom_components_cart).reshape(-1, 3)
+        ncart = (l + 1) * (l + 2) // 2
+        if cart.shape[0] != ncart:
+            raise ValueError(f"shell {idx}: {cart.shape[0]} Cartesian components for l={l}")
+        pos_of = {tuple(c): i for i, c in enumerate(cart.tolist())}
+        perm = np.array([pos_of[c] for c in comps(l)], dtype=np.int64)  # canonical -> shell order
+        coeffs = np.asarray(shell.coeffs, dtype=np.float64)
+        if coeffs.ndim == 1:
+            coeffs = coeffs[:, None]
+        norm_cont = np.asarray(shell.norm_cont, dtype=np.float64).reshape(coeffs.shape[1], ncart)
+        exps = np.asarray(shell.exps, dtype=np.float64)
+        coord = np.asarray(shell.coord, dtype=np.float64).reshape(3)
+        sph = _is_spherical(ctype)
+        if sph:
+            tmat = generate_transformation(l, cart, tuple(shell.angmom_components_sph), "left")
+        for m in range(coeffs.shape[1]):
+            nc = norm_cont[m]
+            if not sph:
+                es = ExpandedShell(idx, l, coord, exps, coeffs[:, m].copy(), 0, ncart,
+                                   perm.astype(np.int32), nc[perm].copy())
+            else:
+                dense = tmat[:, perm] * nc[perm][None, :]  # (nout, ncart canonical)
+                nz = np.count_nonzero(dense, axis=0)
+                if l <= 1 and np.all(nz == 1):
+                    rows = np.argmax(dense != 0, axis=0)
+                    es = ExpandedShell(idx, l, coord, exps, coeffs[:, m].copy(), 0, 2 * l + 1,
+                                       rows.astype(np.int32),
+                                       dense[rows, np.arange(ncart)].copy())
+                else:
+                    es = ExpandedShell(idx, l, coord, exps, coeffs[:, m].copy(), 1, 2 * l + 1,
+                                       np.arange(ncart, dtype=np.int32), np.ones(ncart),
+                                       np.ascontiguousarray(dense))
+            es.func_off = func
+            func += es.nout
+            out.append(es)
+    return out, func
+
+
+def prim_norm(exps, l):
+    return (2.0 * exps / np.pi) ** 0.75 * (4.0 * exps) ** (l / 2.0)
+
+
+class MoleculeTables:
+    """Shell, pair and primitive-pair arrays of one molecule (host, NumPy)."""
+
+    def __init__(self, basis, coord_types=None):
+        basis = list(basis)
+        if coord_types is None:
+            coord_types = [s.coord_type for s in basis]
+        self.n_orig = len(basis)
+        self.shells, self.nfunc = expand_shells(basis, coord_types)
+        ns = len(self.shells)
+        self.nshell = ns
+        # ---- shell pairs, I >= J over expanded shells, pair id I(I+1)/2 + J ----
+        ii, jj = np.tril_indices(ns)
+        order = np.argsort(ii * (ii + 1) // 2 + jj, kind="stable")
+        ii, jj = ii[order], jj[order]
+        self.npair = ii.size
+        ls = np.array([s.l for s in self.shells])
+        swap = ls[ii] < ls[jj]
+        sa = np.where(swap, jj, ii)
+        sb = np.where(swap, ii, jj)
+        self.pair_shell_a = sa.astype(np.int32)
+        self.pair_shell_b = sb.astype(np.int32)
+        self.pair_la = ls[sa]
+        self.pair_lb = ls[sb]
+        self.pair_orig = (np.array([s.orig for s in self.shells])[ii],
+                          np.array([s.orig for s in self.shells])[jj])
+        nprim = np.array([s.exps.size for s in self.shells])
+        self.pair_nprim = (nprim[sa] * nprim[sb]).astype(np.int32)
+        self.pair_prim_off = np.concatenate([[0], np.cumsum(self.pair_nprim)[:-1]]).astype(
+            np.int64)
+        coords = np.array([s.coord for s in self.shells])
+        self.pair_geom = np.zeros((self.npair, 4))
+        self.pair_geom[:, :3] = coords[sa] - coords[sb]
+        # ---- primitive-pair records ----
+        recs = np.zeros((int(self.pair_nprim.sum()), 10))
+        pos = 0
+        for a, b in zip(sa.tolist(), sb.tolist()):
+            A, B = self.shells[a], self.shells[b]
+            ea = A.exps[:, None]
+            eb = B.exps[None, :]
+            zeta = ea + eb
+            P = (ea[..., None] * A.coord + eb[..., None] * B.coord) / zeta[..., None]
+            ab2 = float(np.sum((A.coord - B.coord) ** 2))
+            ca = (A.coeffs * prim_norm(A.exps, A.l))[:, None]
+            cb = (B.coeffs * prim_norm(B.exps, B.l))[None, :]
+            K = ca * cb * np.exp(-(ea * eb / zeta) * ab2) * SQRT2_PI_5_4 / zeta
+            n = zeta.size
+            r = recs[pos:pos + n]
+            r[:, 0] = zeta.ravel()
+            r[:, 1] = (0.5 / zeta).ravel()
+            r[:, 2:5] = P.reshape(-1, 3)
+            r[:, 5:8] = (P - A.coord).reshape(-1, 3)
+            r[:, 8] = K.ravel()
+            pos += n
+        self.prim_pairs = recs
+        self.n_slots = _n_slots(self.nfunc)
+
+    # ------------------------------------------------------------------
+    def shell_arrays(self):
+        """shell_info (ns, 8) int32, comp_map, comp_scale, out_mat (flat)."""
+        info = np.zeros((self.nshell, 8), dtype=np.int32)
+        cmap, cscale, mats = [], [], []
+        moff = 0
+        coff = 0
+        for i, s in enumerate(self.shells):
+            info[i, 0] = s.func_off
+            info[i, 1] = s.l
+            info[i, 2] = s.kind
+            info[i, 3] = coff
+            info[i, 5] = s.nout
+            cmap.append(s.comp_map)
+            cscale.append(s.comp_scale)
+            coff += s.comp_map.size
+            if s.kind == 1:
+                info[i, 4] = moff
+                mats.append(s.out_mat.ravel())
+                moff += s.out_mat.size
+        return (info, np.concatenate(cmap).astype(np.int32), np.concatenate(cscale),
+                np.concatenate(mats) if mats else np.zeros(1))
+
+    def schwarz_orig(self, q_pair):
+        """Per original-shell-pair bound (max over the expanded pairs), (n_orig, n_orig)."""
+        Q = np.zeros((self.n_orig, self.n_orig))
+        oi, oj = self.pair_orig
+        np.maximum.at(Q, (oi, oj), q_pair)
+        Q = np.maximum(Q, Q.T)
+        return Q
+
+    def quartets(self, q_pair=None, threshold=0.0):
+        """Unique, unscreened shell quartets as (bra pair, ket pair) in canonical orientation.
+
+        Returns (bra, ket, cls_a, cls_b, cls_c, cls_d) int arrays.  With a threshold > 0,
+        quartet (ij|kl) is dropped iff Q_ij * Q_kl < threshold where Q is the bound of the
+        caller's (original) shells, as SchwarzScreener.is_significant.
+        """
+        P = self.npair
+        p, q = np.tril_indices(P)
+        if threshold > 0:
+            Qo = self.schwarz_orig(q_pair)
+            oi, oj = self.pair_orig
+            qp = Qo[oi, oj]
+            keep = qp[p] * qp[q] >= threshold
+            p, q = p[keep], q[keep]
+        la, lb = self.pair_la, self.pair_lb
+        key_p = (la[p] + lb[p]) * 16 + la[p]
+        key_q = (la[q] + lb[q]) * 16 + la[q]
+        sw = key_p < key_q
+        bra = np.where(sw, q, p).astype(np.int32)
+        ket = np.where(sw, p, q).astype(np.int32)
+        return bra, ket
+
+
+def _n_slots(nfunc):
+    npair = nfunc * (nfunc + 1) // 2
+    return npair * (npair + 1) // 2
+
+
+class BatchTables:
+    """Tables of several molecules concatenated into one device-resident set."""
+
+    def __init__(self, molecules):
+        self.mols = molecules
+        shell_off = pair_off = prim_off = 0
+        slot_off = 0
+        infos, cmaps, cscales, mats = [], [], [], []
+        pinfo, pgeom, pbase, prims = [], [], [], []
+        self.mol_pair_off, self.mol_slot_off = [], []
+        coff = moff = 0
+        for m in molecules:
+            info, cmap, cscale, mat = m.shell_arrays()
+            info = info.copy()
+            info[:, 3] += coff
+            info[:, 4] += moff
+            infos.append(info)
+            cmaps.append(cmap)
+            cscales.append(cscale)
+            mats.append(mat)
+            coff += cmap.size
+            moff += mat.size
+            pi = np.zeros((m.npair, 4), dtype=np.int64)
+            pi[:, 0] = m.pair_shell_a + shell_off
+            pi[:, 1] = m.pair_shell_b + shell_off
+            pi[:, 2] = m.pair_prim_off + prim_off
+            pi[:, 3] = m.pair_nprim
+            pinfo.append(pi)
+            pgeom.append(m.pair_geom)
+            pbase.append(np.full(m.npair, slot_off, dtype=np.uint64))
+            prims.append(m.prim_pairs)
+            self.mol_pair_off.append(pair_off)
+            self.mol_slot_off.append(slot_off)
+            shell_off += m.nshell
+            pair_off += m.npair
+            prim_off += m.prim_pairs.shape[0]
+            slot_off += m.n_slots
+        if prim_off >= 2 ** 31 or pair_off >= 2 ** 31:
+            raise ValueError("batch too large for 32-bit pair / primitive indices")
+        self.shell_info = np.concatenate(infos)
+        self.comp_map = np.concatenate(cmaps).astype(np.int32)
+        self.comp_scale = np.concatenate(cscales)
+        self.out_mat = np.concatenate(mats)
+        self.pair_info = np.concatenate(pinfo).astype(np.int32)
+        self.pair_geom = np.concatenate(pgeom)
+        self.pair_out_base = np.concatenate(pbase)
+        self.prim_pairs = np.concatenate(prims)
+        self.n_slots = slot_off
+        self.npair = pair_off
+        self.pair_la = np.concatenate([m.pair_la for m in molecules])
+        self.pair_lb = np.concatenate([m.pair_lb for m in molecules])
+        kind = self.shell_info[:, 2]
+        # pairs touching a pure shell with l >= 2 need the dense output transform
+        self.pair_dense = (kind[self.pair_info[:, 0]] == 1) | (kind[self.pair_info[:, 1]] == 1)
+
+    def unique_counts(self, bra, ket):
+        """Number of canonical packed slots each (bra, ket) quartet writes."""
+        nout = self.shell_info[:, 5].astype(np.int64)
+        sa, sb = self.pair_info[:, 0], self.pair_info[:, 1]
+        npf = np.where(sa == sb, nout[sa] * (nout[sa] + 1) // 2, nout[sa] * nout[sb])
+        nb, nk = npf[bra], npf[ket]
+        return np.where(bra == ket, nb * (nb + 1) // 2, nb * nk)
+
+    def class_buckets(self, bra, ket):
+        """Group global (bra, ket) quartets by class, dense-transform flag and
+        primitive-count bin.
+
+        Returns a list of (cls, nprimq_max, dense, quartets (n,2) int32) sorted by class.
+        """
+        la, lb = self.pair_la[bra], self.pair_lb[bra]
+        lc, ld = self.pair_la[ket], self.pair_lb[ket]
+        nprim = self.pair_info[:, 3].astype(np.int64)
+        npq = nprim[bra] * nprim[ket]
+        binid = np.ceil(np.log2(np.maximum(npq, 1))).astype(np.int64)
+        dense = (self.pair_dense[bra] | self.pair_dense[ket]).astype(np.int64)
+        key = ((((la * 8 + lb) * 8 + lc) * 8 + ld) * 2 + dense) * 64 + binid
+        order = np.lexsort((nprim[ket], nprim[bra], key))
+        key_s = key[order]
+        bounds = np.flatnonzero(np.diff(key_s)) + 1
+        starts = np.concatenate([[0], bounds])
+        ends = np.concatenate([bounds, [key_s.size]])
+        out = []
+        for s, e in zip(starts.tolist(), ends.tolist()):
+            idx = order[s:e]
+            k = int(key_s[s])
+            cls = ((k >> 16) & 7, (k >> 13) & 7, (k >> 10) & 7, (k >> 7) & 7)
+            quart = np.stack([bra[idx], ket[idx]], axis=1).astype(np.int32)
+            out.append((cls, int(npq[idx].max()), bool((k >> 6) & 1), quart))
+        return out

@@ -1,0 +1,121 @@
+/*
+ * gbasis_amd — MI355X (gfx950) electron-repulsion-integral library, C ABI.
+ *
+ * Every entry point takes device pointers, plain sizes and a HIP stream passed
+ * as void* (NULL = the null stream), returns 0 on success and a negative code on
+ * failure, and never throws.  gbx_last_error() returns the message of the last
+ * failure on the calling thread.  All buffers are owned by the caller.
+ *
+ * The Python host layer (gbasis_amd/electron_repulsion.py) builds the tables
+ * described below with NumPy, places them in device memory and binds these
+ * symbols with ctypes, in the style of the reference's own ctypes binding
+ * (gbasis/integrals/libcint.py:275-304).
+ */
+#ifndef GBASIS_AMD_H
+#define GBASIS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define GBX_OK 0
+#define GBX_ERR_ARG -1
+#define GBX_ERR_HIP -2
+#define GBX_ERR_UNSUPPORTED -3
+
+/* precision of the recursion bulk (Boys, prefactors, accumulation stay FP64) */
+#define GBX_FP64 0
+#define GBX_FP32_BULK 1
+
+/* output mode of the ERI kernels */
+#define GBX_MODE_PACKED 0  /* write canonical 8-fold packed slots        */
+#define GBX_MODE_SCHWARZ 1 /* write sqrt(max|(ij|ij)|) per bra pair       */
+
+/* Device-resident basis tables (layouts documented in DESIGN.md). */
+typedef struct gbx_tables {
+  const int32_t *shell_info;     /* 8 x int32 per shell: func_off, l, kind, map_off,
+                                    mat_off, nout, 0, 0 (kind 0 = scaled permutation,
+                                    kind 1 = dense Cartesian->pure matrix)            */
+  const int32_t *comp_map;       /* per Cartesian component: position in its shell  */
+  const double *comp_scale;      /* per Cartesian component: output scale           */
+  const double *out_mat;         /* dense per-shell matrices, row-major nout x ncart */
+  const int32_t *pair_info;      /* 4 x int32 per shell pair: shell_a, shell_b,
+                                    prim_off, nprim (bra orientation la >= lb)       */
+  const double *pair_geom;       /* 4 x double per pair: A-B (x,y,z), 0              */
+  const uint64_t *pair_out_base; /* per pair: first packed slot of its molecule      */
+  const double *prim_pairs;      /* 10 x double per primitive pair: zeta, 1/(2 zeta),
+                                    P(3), P-A(3), K', 0; K' = c_a c_b N_a N_b
+                                    exp(-ab/zeta |AB|^2) sqrt(2) pi^(5/4) / zeta     */
+  const double *f2norm;          /* 1/sqrt(prod (2a_i-1)!!) by l, then component     */
+  const double *boys_table;      /* boys_nt x boys_nord: F_n(k h)                    */
+  int32_t boys_nt;
+  int32_t boys_nord;
+  double boys_h;
+} gbx_tables;
+
+/* A class program (gbasis_amd/programs.py) resident on the device. */
+typedef struct gbx_program {
+  const int32_t *items;     /* 8 x int32 per item                               */
+  const int32_t *phases;    /* 3 x int32 per phase: kind, first item, end item  */
+  const int32_t *out_slots; /* per canonical output component: its H slot     */
+  int32_t header[16];    /* prim_slots, nacc, h_slots, nout, n_prim_phases,
+                            n_phases, L, n_items                              */
+} gbx_program;
+
+int gbx_version(void);
+const char *gbx_last_error(void);
+
+/* Boys function F_m(T), m = 0..mmax, for n values of T:
+ * out[i*(mmax+1)+m].  Replaces boys_function_standard
+ * (gbasis/integrals/boys_functions.py:26-63) evaluated on the device. */
+int gbx_boys(const gbx_tables *t, const double *d_T, int64_t n, int32_t mmax, double *d_out,
+             void *stream);
+
+/* Workspace sizes of the cooperative kernel for one class program. */
+int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                       int32_t precision, int32_t prim_copies, int64_t *lds_bytes,
+                       int64_t *global_bytes_per_block);
+
+/* One angular-momentum class of shell quartets with the cooperative
+ * (workgroup-per-quartet) kernel.  d_quartets holds nquart (bra pair, ket pair)
+ * int32 pairs in canonical orientation.  Replaces, for a batch of quartets,
+ * ElectronRepulsionIntegralImproved.construct_array_contraction
+ * (gbasis/integrals/electron_repulsion.py:305-412) ->
+ * compute_two_electron_integrals_os_hgp (gbasis/integrals/_two_elec_int_improved.py:591-732)
+ * plus the norm_cont / Cartesian->pure / 8-fold placement of
+ * gbasis/base_four_symm.py:160-389.  In GBX_MODE_SCHWARZ (bra == ket pair) it
+ * writes d_out[bra] = sqrt(max|(ij|ij)|) without norm_cont, as
+ * compute_schwarz_bound_shell_pair (gbasis/integrals/_schwarz_screening.py:14-59).
+ * d_scratch (scratch_bytes) is only used when the class does not fit in LDS. */
+int gbx_eri_coop(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb,
+                 int32_t lc, int32_t ld, const int32_t *d_quartets, int64_t nquart,
+                 int32_t precision, int32_t prim_copies, int32_t mode, double *d_out,
+                 void *d_scratch, int64_t scratch_bytes, void *stream);
+
+/* 1 if a generated register kernel (one lane per shell quartet) exists for the
+ * class (kernel orientation), else 0. */
+int gbx_has_reg_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld);
+
+/* One class of shell quartets with the generated register kernel: same
+ * contract as gbx_eri_coop (the reference interfaces it replaces are listed
+ * there), restricted to quartets whose four shells all have a scaled-
+ * permutation output map (Cartesian shells, pure shells with l <= 1). */
+int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                const int32_t *d_quartets, int64_t nquart, int32_t precision, int32_t mode,
+                double *d_out, void *stream);
+
+/* Expand the canonical 8-fold packed array of one molecule (nfunc output
+ * functions) to the dense (nfunc)^4 tensor; notation 0 = chemist (ab|cd),
+ * 1 = physicist <ab|cd> = (ac|bd).  Replaces the 8-fold block placement and
+ * concatenation of gbasis/base_four_symm.py:215-244 and the notation transpose
+ * of gbasis/integrals/electron_repulsion.py:279-280. */
+int gbx_unpack_dense(const double *d_packed, int64_t nfunc, int32_t notation, double *d_dense,
+                     void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBASIS_AMD_H */

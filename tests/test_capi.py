@@ -1,0 +1,34 @@
+"""The C-ABI library loads and exports every symbol include/gbasis_amd.h declares (CPU)."""
+import ctypes
+import os
+import re
+
+from gbasis_amd import _lib
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    text = open(os.path.join(ROOT, "include", "gbasis_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(gbx_[a-z_]+)\s*\(", text)))
+
+
+def test_header_symbols_exported_and_bound():
+    names = _declared()
+    assert "gbx_eri_coop" in names and "gbx_eri_reg" in names and "gbx_boys" in names
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in _lib.SIGNATURES, f"{n} not bound in gbasis_amd/_lib.py"
+
+
+def test_error_path_without_device():
+    lib = _lib.load()
+    assert lib.gbx_version() >= 1
+    # argument validation happens before any device call
+    rc = lib.gbx_eri_coop(None, None, 0, 0, 0, 0, None, 0, 0, 1, 0, None, None, 0, None)
+    assert rc == -1
+    assert b"null" in lib.gbx_last_error()
+    assert lib.gbx_has_reg_kernel(0, 0, 0, 0) == 1
+    assert lib.gbx_has_reg_kernel(4, 4, 4, 4) == 0

@@ -1,0 +1,142 @@
+"""GPU parity: HIP kernels against the oracle and the reference's fixtures."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from tests.molecules import random_basis
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name):
+    print(f"[gpu case] {name}", flush=True)
+
+
+def test_boys_device_vs_mpmath(gpu):
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from oracle.boys import boys_mpmath
+
+    _print_case("boys")
+    basis = random_basis(0, [0])
+    dev = DeviceTables(BatchTables([MoleculeTables(basis)]))
+    T = np.array([0.0, 1e-9, 0.05, 0.3, 1.7, 9.99, 25.0, 39.96, 40.0, 40.2, 55.0, 300.0, 1e4, 1e5])
+    F = get_engine().boys(dev, T, 16).cpu().numpy()
+    for i, t in enumerate(T):
+        for m in range(17):
+            ref = boys_mpmath(m, t)
+            assert abs(F[i, m] - ref) <= 1e-14 * abs(ref) + 1e-300, (t, m, F[i, m], ref)
+
+
+def test_horton_bec_fixture(gpu):
+    """Be+C STO-6G, physicist notation, vs the reference's HORTON fixture."""
+    from gbasis_amd.electron_repulsion import electron_repulsion_integral_improved
+    from gbasis_amd.parsers import make_contractions, parse_nwchem
+
+    _print_case("horton_bec")
+    d = parse_nwchem(os.path.join(HERE, "golden", "sto6g_be_c.nwchem"))
+    basis = make_contractions(d, ["Be", "C"], np.array([[0, 0, 0], [1.0, 0, 0]]), "cartesian")
+    ref = np.load(os.path.join(HERE, "golden", "horton_bec_cart_elec_repulsion.npy"))
+    got = electron_repulsion_integral_improved(basis)
+    np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-15)
+
+
+@pytest.mark.parametrize("ls,coord", [([0, 0, 1], "cartesian"), ([0, 1, 2], "cartesian"),
+                                      ([1, 2, 2, 0], "cartesian"), ([2, 1, 3], "cartesian"),
+                                      ([0, 1, 2, 3], "spherical"), ([4, 1], "spherical")])
+def test_dense_vs_oracle(gpu, ls, coord):
+    from gbasis_amd.electron_repulsion import electron_repulsion_integral
+    from oracle.assembly import dense_eri
+
+    _print_case(f"dense {ls} {coord}")
+    basis = random_basis(len(ls) * 11 + ls[-1], ls, kmax=3, coord_type=coord)
+    got = electron_repulsion_integral(basis, notation="chemist")
+    exact = dense_eri(basis, notation="chemist", exact=True)
+    ref64 = dense_eri(basis, notation="chemist")
+    scale = max(1.0, np.abs(exact).max())
+    # FP64 parity bar: 1e-12 Eh against the extended-precision evaluation
+    assert np.abs(got - exact).max() <= 1e-12 * scale
+    # the reference-orientation FP64 oracle carries its own rounding (<= ~1e-11 here)
+    assert np.abs(got - ref64).max() <= 1e-10 * scale
+
+
+def test_schwarz_bounds_and_screened_lists(gpu):
+    """Q_ij on the device vs the oracle's screener; screened quartet sets identical."""
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import batch_molecule
+    from oracle.assembly import schwarz_bounds
+
+    _print_case("schwarz")
+    basis = batch_molecule(2024, n_atoms=5)
+    eng = get_engine()
+    mol = MoleculeTables(basis)
+    dev = DeviceTables(BatchTables([mol]))
+    q = eng.schwarz(dev).cpu().numpy()
+    Qdev = mol.schwarz_orig(q)
+    Qref = schwarz_bounds(basis)
+    np.testing.assert_allclose(Qdev, Qref, rtol=1e-12, atol=0)
+    n = len(basis)
+    iu = np.array([(i, j) for i in range(n) for j in range(i, n)])
+    for thr in (1e-12, 1e-8, 1e-4):
+        keep_dev = set()
+        keep_ref = set()
+        for p, (i, j) in enumerate(iu):
+            for (k, l) in iu[p:]:
+                if Qdev[i, j] * Qdev[k, l] >= thr:
+                    keep_dev.add((i, j, k, l))
+                if Qref[i, j] * Qref[k, l] >= thr:
+                    keep_ref.add((i, j, k, l))
+        assert keep_dev == keep_ref
+
+
+def test_screened_dense_vs_oracle(gpu):
+    from gbasis_amd.electron_repulsion import electron_repulsion_integral_improved
+    from gbasis_amd.workloads import batch_molecule
+    from oracle.assembly import dense_eri
+
+    _print_case("screened dense")
+    basis = batch_molecule(77, n_atoms=4)
+    thr = 1e-6
+    got = electron_repulsion_integral_improved(basis, notation="physicist", schwarz_threshold=thr)
+    ref = dense_eri(basis, notation="physicist", schwarz_threshold=thr)
+    assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
+    assert np.count_nonzero(got == 0) == np.count_nonzero(ref == 0)
+
+
+def test_mixed_precision_error(gpu):
+    """FP32-bulk quartets (Q_ij Q_kl < cutoff) stay within 1e-8 Eh of FP64."""
+    from gbasis_amd.electron_repulsion import electron_repulsion_integral_packed
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("mixed")
+    basis = batch_molecule(31, n_atoms=8)
+    p64, n = electron_repulsion_integral_packed(basis, schwarz_threshold=1e-12)
+    pmx, _ = electron_repulsion_integral_packed(basis, schwarz_threshold=1e-12,
+                                                precision="mixed", fp32_cutoff=1e-3)
+    err = float((pmx - p64).abs().max())
+    print(f"mixed max err {err:.3e}", flush=True)
+    assert err <= 1e-8
+
+
+def test_register_vs_cooperative(gpu):
+    """Both kernel families give the same packed array (FP64)."""
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("reg vs coop")
+    eng = get_engine()
+    mol = MoleculeTables(batch_molecule(5, n_atoms=4))
+    dev = DeviceTables(BatchTables([mol]))
+    plans = eng.plan_quartets(dev)
+    a = eng.run(dev, plans)
+    eng.path = "coop"
+    try:
+        b = eng.run(dev, plans)
+    finally:
+        eng.path = "auto"
+    assert float((a - b).abs().max()) <= 1e-13 * max(1.0, float(a.abs().max()))

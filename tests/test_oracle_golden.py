@@ -1,0 +1,97 @@
+"""The oracle pinned against the reference's golden data and closed forms (CPU)."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from oracle.boys import boys_hyp1f1, boys_longdouble, boys_mpmath
+from oracle.os_hgp import os_hgp_quartet
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _prim(a, b, c, d, exps, coords, **kw):
+    A, B, C, D = (np.array(x) for x in coords)
+    return float(os_hgp_quartet(A, sum(a), [a], [exps[0]], [1.0], B, sum(b), [b], [exps[1]], [1.0],
+                                C, sum(c), [c], [exps[2]], [1.0], D, sum(d), [d], [exps[3]], [1.0],
+                                **kw).ravel()[0])
+
+
+def test_horton_primitive_values():
+    """53 HORTON normalised primitive ERIs (tests/test_two_elec_int.py:1111-1365)."""
+    data = json.load(open(os.path.join(HERE, "golden", "horton_primitive_eris.json")))
+    assert len(data["records"]) == 53
+    for r in data["records"]:
+        got = _prim(tuple(r["a"]), tuple(r["b"]), tuple(r["c"]), tuple(r["d"]), r["exps"],
+                    data["coords"])
+        assert abs(got - r["value"]) <= 1e-12 * abs(r["value"]), r
+
+
+def test_horton_primitive_values_extended_precision():
+    data = json.load(open(os.path.join(HERE, "golden", "horton_primitive_eris.json")))
+    for r in data["records"][::5]:
+        got = _prim(tuple(r["a"]), tuple(r["b"]), tuple(r["c"]), tuple(r["d"]), r["exps"],
+                    data["coords"], boys=boys_longdouble, dtype=np.longdouble)
+        assert abs(got - r["value"]) <= 1e-12 * abs(r["value"]), r
+
+
+def test_ssss_closed_form():
+    rng = np.random.default_rng(3)
+    for _ in range(5):
+        A, B, C, D = rng.uniform(-2, 2, (4, 3))
+        a, b, c, d = np.exp(rng.uniform(-2, 3, 4))
+        z, e = a + b, c + d
+        P, Q = (a * A + b * B) / z, (c * C + d * D) / e
+        rho = z * e / (z + e)
+        T = rho * np.sum((P - Q) ** 2)
+        F0 = 0.5 * np.sqrt(np.pi / T) * __import__("math").erf(np.sqrt(T))
+        N = np.prod([(2 * x / np.pi) ** 0.75 for x in (a, b, c, d)])
+        ref = (2 * np.pi ** 2.5 / (z * e * np.sqrt(z + e)) * np.exp(-a * b / z * np.sum((A - B) ** 2))
+               * np.exp(-c * d / e * np.sum((C - D) ** 2)) * F0 * N)
+        got = _prim((0, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0), [a, b, c, d], [A, B, C, D])
+        assert abs(got - ref) <= 1e-13 * abs(ref)
+
+
+def test_permutational_symmetry_and_schwarz():
+    """(ab|cd) = (ba|cd) = (cd|ab) and |(ab|cd)| <= sqrt((ab|ab)(cd|cd)) for p/d shells."""
+    from tests.molecules import random_basis
+    from oracle.assembly import dense_eri
+
+    basis = random_basis(11, [1, 2, 0], kmax=2)
+    g = dense_eri(basis, notation="chemist")
+    np.testing.assert_allclose(g, g.transpose(1, 0, 2, 3), atol=1e-13)
+    np.testing.assert_allclose(g, g.transpose(2, 3, 0, 1), atol=1e-13)
+    n = g.shape[0]
+    diag = np.einsum("ijij->ij", g)
+    assert np.all(np.abs(g) <= np.sqrt(np.abs(diag)[:, :, None, None] *
+                                       np.abs(diag)[None, None, :, :]) + 1e-12)
+
+
+def test_bec_sto6g_fixture():
+    """Be+C STO-6G, physicist notation, vs data_horton_bec_cart_elec_repulsion.npy."""
+    from gbasis_amd.parsers import make_contractions, parse_nwchem
+    from oracle.assembly import dense_eri
+
+    d = parse_nwchem(os.path.join(HERE, "golden", "sto6g_be_c.nwchem"))
+    basis = make_contractions(d, ["Be", "C"], np.array([[0, 0, 0], [1.0, 0, 0]]), "cartesian")
+    ref = np.load(os.path.join(HERE, "golden", "horton_bec_cart_elec_repulsion.npy"))
+    np.testing.assert_allclose(dense_eri(basis), ref, rtol=1e-10, atol=1e-15)
+
+
+@pytest.mark.parametrize("m", [0, 1, 4, 8, 12, 16])
+def test_boys_closed_forms(m):
+    T = np.array([0.0, 1e-6, 0.3, 2.0, 11.0, 35.0, 80.0, 1e3, 1e4])
+    ld = boys_longdouble(m, T)
+    for t, v in zip(T, ld):
+        assert abs(float(v) - boys_mpmath(m, t)) <= 1e-15 * boys_mpmath(m, t)
+    assert abs(float(boys_hyp1f1(m, 0.0)) - 1.0 / (2 * m + 1)) < 1e-16
+    # downward recurrence (2m+1) F_m = 2T F_{m+1} + e^{-T}
+    f_m, f_m1 = boys_longdouble(m, T), boys_longdouble(m + 1, T)
+    np.testing.assert_allclose((2 * m + 1) * f_m, 2 * T * f_m1 + np.exp(-T.astype(np.longdouble)),
+                               rtol=1e-17)
+    if m == 0:
+        from math import erf
+        for t in T[1:]:
+            assert abs(float(boys_longdouble(0, t)) - 0.5 * np.sqrt(np.pi / t) * erf(np.sqrt(t))) \
+                <= 2e-16

@@ -1,0 +1,95 @@
+"""Class programs (the recursion schedule both kernel families execute), run on the
+CPU in NumPy, against the oracle and an extended-precision evaluation."""
+import numpy as np
+import pytest
+
+from gbasis_amd.programs import PH_HRRK, PH_VRR, class_program, comps
+from oracle.boys import boys_hyp1f1
+from oracle.os_hgp import os_hgp_quartet
+
+
+def _f2(c):
+    def df(n):
+        r = 1
+        while n > 1:
+            r *= n
+            n -= 2
+        return r
+    return 1 / np.sqrt(df(2 * c[0] - 1) * df(2 * c[1] - 1) * df(2 * c[2] - 1))
+
+
+def run_program(cls, A, B, C, D, ex, co):
+    """Execute a class program exactly as the kernels do (FP64, NumPy)."""
+    la, lb, lc, ld = cls
+    prog = class_program(*cls)
+    L = prog.L
+    acc = np.zeros(prog.nacc)
+
+    def N(e, l):
+        return (2 * e / np.pi) ** 0.75 * (4 * e) ** (l / 2)
+
+    for i, a in enumerate(ex[0]):
+        for j, b in enumerate(ex[1]):
+            for k, c in enumerate(ex[2]):
+                for l, d in enumerate(ex[3]):
+                    z, e = a + b, c + d
+                    inv = 1 / (z + e)
+                    P, Q = (a * A + b * B) / z, (c * C + d * D) / e
+                    PQ = P - Q
+                    T = z * e * inv * PQ @ PQ
+                    kab = co[0][i] * co[1][j] * N(a, la) * N(b, lb) * np.exp(-a * b / z * np.sum((A - B) ** 2))
+                    kcd = co[2][k] * co[3][l] * N(c, lc) * N(d, ld) * np.exp(-c * d / e * np.sum((C - D) ** 2))
+                    S = np.zeros(prog.prim_slots)
+                    S[:L + 1] = boys_hyp1f1(np.arange(L + 1), T) * 2 * np.pi ** 2.5 / (z * e) * np.sqrt(inv) * kab * kcd
+                    PA, WP, roz, i2z = P - A, -e * inv * PQ, e * inv, 0.5 / z
+                    QC, WQ, roe, i2e, i2s = Q - C, z * inv * PQ, z * inv, 0.5 / e, 0.5 * inv
+                    for kind, items in prog.prim_phases:
+                        for dst, s0, s1, s2, s3, s4, code, _ in items:
+                            di, k1, k2 = code & 15, (code >> 4) & 255, (code >> 12) & 255
+                            if kind == PH_VRR:
+                                v = PA[di] * S[s0] + WP[di] * S[s1]
+                                if s2 >= 0:
+                                    v += k1 * i2z * (S[s2] - roz * S[s3])
+                            else:
+                                v = QC[di] * S[s0] + WQ[di] * S[s1]
+                                if s2 >= 0:
+                                    v += k1 * i2e * (S[s2] - roe * S[s3])
+                                if s4 >= 0:
+                                    v += k2 * i2s * S[s4]
+                            S[dst] = v
+                    for it in prog.acc_phase[1]:
+                        acc[it[0]] += S[it[1]]
+    H = np.zeros(prog.h_slots)
+    H[:prog.nacc] = acc
+    for kind, items in prog.hrr_phases:
+        g = (C - D) if kind == PH_HRRK else (A - B)
+        for it in items:
+            H[it[0]] = H[it[1]] + g[it[3]] * H[it[2]]
+    outs = [(a, b, c, d) for a in comps(la) for b in comps(lb) for c in comps(lc) for d in comps(ld)]
+    vals = np.array([H[s] * _f2(a) * _f2(b) * _f2(c) * _f2(d)
+                     for s, (a, b, c, d) in zip(prog.out_slots, outs)])
+    return vals.reshape([len(comps(x)) for x in cls])
+
+
+@pytest.mark.parametrize("cls", [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 0, 0), (1, 0, 1, 0),
+                                 (1, 1, 1, 1), (2, 0, 1, 0), (2, 1, 1, 1), (2, 2, 2, 1),
+                                 (3, 1, 2, 0), (2, 2, 2, 2), (3, 0, 3, 0)])
+def test_program_matches_oracle(cls):
+    rng = np.random.default_rng(sum(cls) * 7 + cls[0])
+    A, B, C, D = rng.uniform(-1, 1, (4, 3))
+    ex = [rng.uniform(0.2, 3, 2) for _ in range(4)]
+    co = [rng.uniform(0.1, 1, 2) for _ in range(4)]
+    la, lb, lc, ld = cls
+    got = run_program(cls, A, B, C, D, ex, co)
+    ref = os_hgp_quartet(A, la, comps(la), ex[0], co[0], B, lb, comps(lb), ex[1], co[1],
+                         C, lc, comps(lc), ex[2], co[2], D, ld, comps(ld), ex[3], co[3])
+    ref = ref[..., 0, 0, 0, 0]
+    assert np.abs(got - ref).max() <= 1e-13 * max(1.0, np.abs(ref).max())
+
+
+def test_program_sizes_monotone():
+    """Sanity of the schedule: intermediates grow with l, and every output is reachable."""
+    small, big = class_program(1, 1, 1, 1), class_program(2, 2, 2, 2)
+    assert big.prim_slots > small.prim_slots and big.h_slots > small.h_slots
+    for p in (small, big):
+        assert len(set(p.out_slots.tolist())) == p.nout
