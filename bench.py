@@ -3,9 +3,10 @@
 Workload (default): 64 molecules of 4-24 atoms (seeds 1000..1063), per atom
 1-3 s, 0-2 p, 0-1 d Cartesian shells, K = 1..6, exponents log-uniform in
 [0.05, 1e3], Schwarz threshold 1e-12, canonical 8-fold packed FP64 output.
-A step evaluates every unscreened shell quartet of the whole batch into the
-packed array, with the shell-pair tables, Schwarz bounds and quartet lists
-resident in HBM before timing starts.  It runs twice: plain FP64, and mixed
+A step builds the screened, class-bucketed quartet queue on the device and
+evaluates every unscreened shell quartet of the whole batch into the packed
+array; the shell and pair tables and the Schwarz bounds are resident in HBM
+before timing starts.  It runs twice: plain FP64, and mixed
 (FP32 recursion bulk for quartets with Q_ij Q_kl below the cutoff; Boys,
 prefactors and accumulation in FP64).  The headline value is the mixed form
 at max|err| <= 1e-8 Eh against the FP64 result.
@@ -40,35 +41,47 @@ def build_batch(n_mol):
     return BatchTables(mols)
 
 
-def time_plans(eng, dev, plans, out, steps, warmup):
-    """Run the plans warmup+steps times; per-bucket HIP event timings of the timed steps."""
+def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup):
+    """warmup + steps of: device queue build, then every ERI launch.
+
+    Returns (seconds per step, queue seconds per step, wall seconds per step,
+    per-bucket kernel seconds per step, plans of the last step).
+    """
     import torch
 
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
-        eng.run(dev, plans, out)
+        plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False)
+        eng.run(dev, plans, dev.out)
     torch.cuda.synchronize()
-    per_bucket = np.zeros(len(plans))
+    per_bucket = None
+    t_queue = 0.0
     t0 = time.perf_counter()
     start = torch.cuda.Event(enable_timing=True)
     stop = torch.cuda.Event(enable_timing=True)
     start.record(stream)
-    evs = []
+    recs = []
     for _ in range(steps):
+        q0 = torch.cuda.Event(enable_timing=True)
+        q0.record(stream)
+        plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False)
         marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(plans) + 1)]
         marks[0].record(stream)
         for i, bk in enumerate(plans):
-            bk.path = eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.precision, 0, out,
-                                       bk.dense, eng.path)
+            bk.path = eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.precision, 0,
+                                       dev.out, bk.dense, eng.path)
             marks[i + 1].record(stream)
-        evs.append(marks)
+        recs.append((q0, marks))
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    for marks in evs:
+    per_bucket = np.zeros(len(plans))
+    for q0, marks in recs:
+        t_queue += q0.elapsed_time(marks[0]) * 1e-3
         for i in range(len(plans)):
             per_bucket[i] += marks[i].elapsed_time(marks[i + 1]) * 1e-3
-    return start.elapsed_time(stop) * 1e-3 / steps, wall / steps, per_bucket / steps
+    return (start.elapsed_time(stop) * 1e-3 / steps, t_queue / steps, wall / steps,
+            per_bucket / steps, plans)
 
 
 def roofline_entry(plans, per_bucket):
@@ -164,7 +177,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--molecules", type=int, default=64)
     ap.add_argument("--threshold", type=float, default=1e-12)
-    ap.add_argument("--fp32-cutoff", type=float, default=1e-3)
+    ap.add_argument("--fp32-cutoff", type=float, default=1e-2)
     ap.add_argument("--path", choices=["auto", "coop"], default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -184,22 +197,30 @@ def main():
     eng.path = args.path
     dev = DeviceTables(batch)
     q = eng.schwarz(dev)
+    bound = eng.pair_bounds(dev, q)
     torch.cuda.synchronize()
-    qh = q.cpu().numpy()
-    plans64 = eng.plan_quartets(dev, qh, args.threshold, 0.0)
-    plansmx = eng.plan_quartets(dev, qh, args.threshold, args.fp32_cutoff)
+    plans64 = eng.plan_device(dev, bound, args.threshold, 0.0, stats=True)
+    plansmx = eng.plan_device(dev, bound, args.threshold, args.fp32_cutoff, stats=True)
     eris = sum(bk.eris for bk in plans64)
     nquart = sum(bk.n for bk in plans64)
     n_fp32 = sum(bk.n for bk in plansmx if bk.precision == 1)
     log(f"plans: {nquart} quartets, {eris} ERIs, {len(plans64)} buckets; "
         f"{n_fp32} quartets FP32-bulk in mixed ({time.perf_counter() - t0:.1f} s)")
 
-    out64 = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
-    t64, w64, pb64 = time_plans(eng, dev, plans64, out64, args.steps, args.warmup)
-    log(f"fp64: {t64 * 1e3:.1f} ms/step, {eris / t64:.3e} ERIs/s")
-    outmx = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
-    tmx, wmx, pbmx = time_plans(eng, dev, plansmx, outmx, args.steps, args.warmup)
-    log(f"mixed: {tmx * 1e3:.1f} ms/step, {eris / tmx:.3e} ERIs/s")
+    dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
+    t64, q64, w64, pb64, last64 = time_steps(eng, dev, bound, args.threshold, 0.0, args.steps,
+                                             args.warmup)
+    out64 = dev.out
+    log(f"fp64: {t64 * 1e3:.1f} ms/step (queue {q64 * 1e3:.1f} ms), {eris / t64:.3e} ERIs/s")
+    dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
+    tmx, qmx, wmx, pbmx, lastmx = time_steps(eng, dev, bound, args.threshold, args.fp32_cutoff,
+                                             args.steps, args.warmup)
+    outmx = dev.out
+    log(f"mixed: {tmx * 1e3:.1f} ms/step (queue {qmx * 1e3:.1f} ms), {eris / tmx:.3e} ERIs/s")
+    for bk, last in zip(plans64, last64):
+        bk.path = last.path
+    for bk, last in zip(plansmx, lastmx):
+        bk.path = last.path
     diff = outmx.sub_(out64).abs_()
     max_err = float(diff.max())
     rms_err = float(torch.linalg.vector_norm(diff)) / np.sqrt(diff.numel())
@@ -218,6 +239,7 @@ def main():
                    "schwarz_threshold": args.threshold, "fp32_cutoff": args.fp32_cutoff,
                    "fp32_quartets": n_fp32, "kernel_path": args.path},
         "fp64": {"value": eris / t64, "ms_per_step": t64 * 1e3},
+        "queue_build_ms": {"fp64": q64 * 1e3, "mixed": qmx * 1e3},
         "mixed_vs_fp64": {"max_abs_err": max_err, "rms_err": rms_err},
         "wall_ms_per_step": {"fp64": w64 * 1e3, "mixed": wmx * 1e3},
         "roofline": roof,

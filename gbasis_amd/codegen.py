@@ -5,8 +5,8 @@ programs.canonical_class) this emits one HIP kernel in which every lane
 evaluates one whole shell quartet: a loop over the bra and ket primitive pairs
 with the Boys function, the Obara-Saika recursions and the accumulation into
 FP64 registers written out as straight-line code, then the Head-Gordon-Pople
-horizontal recursion, the factorial2 norms and the placement into the packed
-array.  The straight-line code is produced from the same class program the
+horizontal recursion (always FP64, on the FP64 contracted values), the
+factorial2 norms and the placement into the packed array.  The straight-line code is produced from the same class program the
 cooperative kernel interprets (programs.ClassProgram), so both paths evaluate
 identical recursions; only the storage differs (registers instead of LDS).
 
@@ -124,16 +124,24 @@ def kernel_source(cls):
     w("    }")
     w("  }")
     # HRR
-    w("  const TB AB[3] = {static_cast<TB>(t.pair_geom[4 * bra]), "
-      "static_cast<TB>(t.pair_geom[4 * bra + 1]), static_cast<TB>(t.pair_geom[4 * bra + 2])};")
-    w("  const TB CD[3] = {static_cast<TB>(t.pair_geom[4 * ket]), "
-      "static_cast<TB>(t.pair_geom[4 * ket + 1]), static_cast<TB>(t.pair_geom[4 * ket + 2])};")
+    # largest contracted [e0|f0] (input of the horizontal recursion), for the
+    # precision bound of GBX_MODE_SCHWARZ
+    w("  double accmax = 0.0;")
+    w("  if (a.mode == GBX_MODE_SCHWARZ) {")
     for f in range(p.nacc):
-        w(f"  const TB h{f} = static_cast<TB>(acc{f});")
+        w(f"    accmax = fmax(accmax, fabs(acc{f}));")
+    w("  }")
+    # horizontal recursion on the FP64 contracted values
+    w("  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], "
+      "t.pair_geom[4 * bra + 2]};")
+    w("  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], "
+      "t.pair_geom[4 * ket + 2]};")
+    for f in range(p.nacc):
+        w(f"  const double h{f} = acc{f};")
     for kind, items in p.hrr_phases:
         g = "CD" if kind == PH_HRRK else "AB"
         for it in items:
-            w(f"  const TB h{it[0]} = h{it[1]} + {g}[{it[3]}] * h{it[2]};")
+            w(f"  const double h{it[0]} = h{it[1]} + {g}[{it[3]}] * h{it[2]};")
     # outputs with factorial2 norms
     outs = [(a, b, c, d) for a in comps(la) for b in comps(lb) for c in comps(lc)
             for d in comps(ld)]
@@ -142,10 +150,10 @@ def kernel_source(cls):
     for o_idx, ((a, b, c, d), hs) in enumerate(zip(outs, p.out_slots.tolist())):
         f2 = _f2(a) * _f2(b) * _f2(c) * _f2(d)
         if abs(f2 - 1.0) < 1e-300:
-            w(f"  out[{o_idx}] = static_cast<double>(h{hs});")
+            w(f"  out[{o_idx}] = h{hs};")
         else:
-            w(f"  out[{o_idx}] = static_cast<double>(h{hs}) * {f2!r};")
-    w(f"  store_block<{nca}, {ncb}, {ncc}, {ncd}>(t, a, bra, ket, pb, pk, out);")
+            w(f"  out[{o_idx}] = h{hs} * {f2!r};")
+    w(f"  store_block<{nca}, {ncb}, {ncc}, {ncd}>(t, a, bra, ket, pb, pk, out, accmax);")
     w("}")
     return "\n".join(o)
 

@@ -3,7 +3,7 @@
 //
 // Algorithm: Obara-Saika vertical recursion on A, Obara-Saika build of C,
 // contraction over primitive quartets into FP64 accumulators, Head-Gordon-Pople
-// horizontal recursion on D then B, factorial2 component norms, per-shell
+// horizontal recursion on D then B (FP64 in both precision modes), factorial2 component norms, per-shell
 // output transform (norm_cont, component order, Cartesian->pure) and placement
 // into the canonical 8-fold packed array -- the stage order of
 // gbasis/integrals/_two_elec_int_improved.py:591-732 and
@@ -61,7 +61,7 @@ inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf) {
   l.un_off = align16(l.acc_off + nacc * 8);
   const int64_t prim_bytes = prim_slots * pf * tb_bytes;
   l.h_off = l.un_off;
-  l.f0_off = align16(l.h_off + h_slots * tb_bytes);
+  l.f0_off = align16(l.h_off + h_slots * 8);
   l.f1_off = align16(l.f0_off + nout * 8);
   const int64_t epi_end = align16(l.f1_off + nout * 8);
   l.total = std::max(align16(l.un_off + prim_bytes), epi_end);
@@ -79,10 +79,11 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
   TB* sc = reinterpret_cast<TB*>(base + a.sc_off);
   double* acc = reinterpret_cast<double*>(base + a.acc_off);
   TB* S = reinterpret_cast<TB*>(base + a.un_off);
-  TB* H = reinterpret_cast<TB*>(base + a.h_off);
+  double* H = reinterpret_cast<double*>(base + a.h_off);
   double* F0 = reinterpret_cast<double*>(base + a.f0_off);
   double* F1 = reinterpret_cast<double*>(base + a.f1_off);
   __shared__ double s_red[kCoopThreads / 64];
+  __shared__ double s_red2[kCoopThreads / 64];
 
   const int tid = threadIdx.x;
   const int PF = a.pf;
@@ -178,17 +179,17 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
                  ABz = t.pair_geom[4 * bra + 2];
     const double CDx = t.pair_geom[4 * ket], CDy = t.pair_geom[4 * ket + 1],
                  CDz = t.pair_geom[4 * ket + 2];
-    for (int f = tid; f < a.nacc; f += kCoopThreads) H[f] = static_cast<TB>(acc[f]);
+    for (int f = tid; f < a.nacc; f += kCoopThreads) H[f] = acc[f];
     __syncthreads();
     for (int ph = a.n_prim_ph + 1; ph < a.n_ph; ++ph) {
       const int kind = a.phases[3 * ph], i0 = a.phases[3 * ph + 1], i1 = a.phases[3 * ph + 2];
-      const TB gx = static_cast<TB>(kind == kPhHrrK ? CDx : ABx);
-      const TB gy = static_cast<TB>(kind == kPhHrrK ? CDy : ABy);
-      const TB gz = static_cast<TB>(kind == kPhHrrK ? CDz : ABz);
+      const double gx = kind == kPhHrrK ? CDx : ABx;
+      const double gy = kind == kPhHrrK ? CDy : ABy;
+      const double gz = kind == kPhHrrK ? CDz : ABz;
       for (int it = i0 + tid; it < i1; it += kCoopThreads) {
         const int4 d0 = items4[2 * it];
         const int dir = d0.w;
-        const TB g = dir == 0 ? gx : (dir == 1 ? gy : gz);
+        const double g = dir == 0 ? gx : (dir == 1 ? gy : gz);
         H[d0.x] = H[d0.y] + g * H[d0.z];
       }
       __syncthreads();
@@ -208,20 +209,32 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
       const int ic = r % nc[2]; r /= nc[2];
       const int ib = r % nc[1];
       const int ia = r / nc[1];
-      F0[o] = static_cast<double>(H[a.out_slots[o]]) * f2a[ia] * f2b[ib] * f2c[ic] * f2d[id];
+      F0[o] = H[a.out_slots[o]] * f2a[ia] * f2b[ib] * f2c[ic] * f2d[id];
     }
     __syncthreads();
 
     if (a.mode == GBX_MODE_SCHWARZ) {
-      double m = 0.0;
+      // out[2 bra]: sqrt(max |(ij|ij)|); out[2 bra + 1]: sqrt(max |contracted [e0|f0]|)
+      double m = 0.0, m2 = 0.0;
       for (int o = tid; o < a.nout; o += kCoopThreads) m = fmax(m, fabs(F0[o]));
-      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-      if ((tid & 63) == 0) s_red[tid >> 6] = m;
+      for (int f = tid; f < a.nacc; f += kCoopThreads) m2 = fmax(m2, fabs(acc[f]));
+      for (int off = 32; off > 0; off >>= 1) {
+        m = fmax(m, __shfl_xor(m, off, 64));
+        m2 = fmax(m2, __shfl_xor(m2, off, 64));
+      }
+      if ((tid & 63) == 0) {
+        s_red[tid >> 6] = m;
+        s_red2[tid >> 6] = m2;
+      }
       __syncthreads();
       if (tid == 0) {
-        double mm = s_red[0];
-        for (int w = 1; w < kCoopThreads / 64; ++w) mm = fmax(mm, s_red[w]);
-        a.out[bra] = sqrt(mm);
+        double mm = s_red[0], mm2 = s_red2[0];
+        for (int w = 1; w < kCoopThreads / 64; ++w) {
+          mm = fmax(mm, s_red[w]);
+          mm2 = fmax(mm2, s_red2[w]);
+        }
+        a.out[2 * bra] = sqrt(mm);
+        a.out[2 * bra + 1] = sqrt(mm2);
       }
       continue;
     }

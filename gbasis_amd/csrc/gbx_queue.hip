@@ -1,0 +1,199 @@
+// Device-side construction of the screened, class-bucketed shell-quartet work
+// queue for a batch of molecules.
+//
+// Candidate quartets are the unique pairs-of-pairs (p >= q) of every molecule,
+// the enumeration of gbasis/base_four_symm.py:189-194.  A candidate is kept
+// iff Q_p * Q_q >= threshold (gbasis/integrals/_schwarz_screening.py:184-191;
+// threshold <= 0 keeps everything), oriented so the "heavier" pair is the
+// bra, marked FP32-bulk iff P_p * P_q < fp32_cutoff (P: bound of the contracted
+// [e0|f0], the input of the horizontal recursion), and assigned to a bucket
+//   key = (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + ceil(log2(nprim_bra*nprim_ket)).
+// Pass 1 counts quartets per bucket (optionally also packed slots and
+// primitive quartets per bucket); pass 2 writes (bra, ket) pairs at
+// bucket_offset[key] + an atomically reserved position.  Both passes
+// aggregate atomics per wavefront and key.  The order of quartets inside a
+// bucket is not deterministic; the integral of every packed slot is, since a
+// slot is produced by exactly one quartet with arithmetic that does not depend
+// on its neighbours.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "gbasis_amd.h"
+#include "gbx_common.h"
+
+namespace gbx {
+
+constexpr int kQueueThreads = 256;
+
+struct QueueArgs {
+  int32_t n_mol;
+  const int64_t* mol_cand_off;  // n_mol+1 prefix sums of P_m (P_m + 1) / 2
+  const int32_t* mol_pair_off;  // n_mol first global pair of each molecule
+  const int32_t* pair_key;      // per pair: la | lb << 3 | dense << 6 | nprim << 8
+  const int32_t* pair_npf;      // per pair: number of unique function pairs
+  const double* pair_bound;     // per pair: Schwarz bound Q of its original shell pair
+  const double* pair_pbound;    // per pair: bound of the contracted [e0|f0] (precision choice)
+  double threshold, fp32_cutoff;
+  int32_t* bucket_count;        // [kNumBuckets]
+  unsigned long long* bucket_eris;    // [kNumBuckets] or null
+  unsigned long long* bucket_primq;   // [kNumBuckets] or null
+  const int64_t* bucket_offset;       // pass 2: first slot of each bucket in out
+  int32_t* bucket_cursor;             // pass 2: [kNumBuckets], zeroed
+  int32_t* out;                       // pass 2: (bra, ket) pairs
+  int64_t n_cand;
+};
+
+__device__ inline int ceil_log2(int x) {
+  int b = 0;
+  while ((1 << b) < x) ++b;
+  return b;
+}
+
+// candidate t of a molecule with P pairs -> (p >= q), t = p (p + 1) / 2 + q
+__device__ inline void tri_decode(int64_t t, int64_t& p, int64_t& q) {
+  int64_t r = static_cast<int64_t>((sqrt(8.0 * static_cast<double>(t) + 1.0) - 1.0) * 0.5);
+  while (r * (r + 1) / 2 > t) --r;
+  while ((r + 1) * (r + 2) / 2 <= t) ++r;
+  p = r;
+  q = t - r * (r + 1) / 2;
+}
+
+template <bool kFill>
+__global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs a) {
+  const int64_t t = static_cast<int64_t>(blockIdx.x) * kQueueThreads + threadIdx.x;
+  bool valid = t < a.n_cand;
+  int key = -1;
+  int bra = 0, ket = 0;
+  uint64_t eris = 0, primq = 0;
+  if (valid) {
+    // molecule by binary search over the candidate prefix sums
+    int lo = 0, hi = a.n_mol - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (a.mol_cand_off[mid] <= t) lo = mid; else hi = mid - 1;
+    }
+    int64_t p, q;
+    tri_decode(t - a.mol_cand_off[lo], p, q);
+    const int gp = a.mol_pair_off[lo] + static_cast<int>(p);
+    const int gq = a.mol_pair_off[lo] + static_cast<int>(q);
+    const double bound = a.pair_bound[gp] * a.pair_bound[gq];
+    if (a.threshold > 0.0 && !(bound >= a.threshold)) {
+      valid = false;
+    } else {
+      const int kp = a.pair_key[gp], kq = a.pair_key[gq];
+      const int lap = kp & 7, lbp = (kp >> 3) & 7, laq = kq & 7, lbq = (kq >> 3) & 7;
+      const bool swap = ((lap + lbp) * 16 + lap) < ((laq + lbq) * 16 + laq);
+      bra = swap ? gq : gp;
+      ket = swap ? gp : gq;
+      const int kb = swap ? kq : kp, kk = swap ? kp : kq;
+      const int dense = ((kb >> 6) & 1) | ((kk >> 6) & 1);
+      const double pbound = a.pair_pbound[gp] * a.pair_pbound[gq];
+      const int fp32 = (a.fp32_cutoff > 0.0 && pbound < a.fp32_cutoff) ? 1 : 0;
+      const int np = (kb >> 8) * (kk >> 8);
+      const int cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7);
+      key = ((cls * 2 + dense) * 2 + fp32) * 16 + ceil_log2(np);
+      if (!kFill && a.bucket_eris) {
+        const uint64_t nb = static_cast<uint64_t>(a.pair_npf[bra]);
+        const uint64_t nk = static_cast<uint64_t>(a.pair_npf[ket]);
+        eris = bra == ket ? nb * (nb + 1) / 2 : nb * nk;
+        primq = static_cast<uint64_t>(np);
+      }
+    }
+  }
+  // wavefront-aggregated reservation per distinct key
+  const int lane = threadIdx.x & 63;
+  uint64_t pending = __ballot(valid);
+  while (pending) {
+    const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+    const int lkey = __shfl(key, leader, 64);
+    const bool mine = valid && key == lkey;
+    const uint64_t same = __ballot(mine);
+    const int n = __popcll(same);
+    const int rank = __popcll(same & ((uint64_t(1) << lane) - 1));
+    if (kFill) {
+      int base = 0;
+      if (lane == leader) base = atomicAdd(a.bucket_cursor + lkey, n);
+      base = __shfl(base, leader, 64);
+      if (mine) {
+        const int64_t pos = a.bucket_offset[lkey] + base + rank;
+        a.out[2 * pos] = bra;
+        a.out[2 * pos + 1] = ket;
+      }
+    } else {
+      if (lane == leader) atomicAdd(a.bucket_count + lkey, n);
+      if (mine && a.bucket_eris) {
+        atomicAdd(a.bucket_eris + lkey, static_cast<unsigned long long>(eris));
+        atomicAdd(a.bucket_primq + lkey, static_cast<unsigned long long>(primq));
+      }
+    }
+    pending &= ~same;
+  }
+}
+
+int launch_queue(const QueueArgs& a, bool fill, hipStream_t s) {
+  if (a.n_cand == 0) return GBX_OK;
+  const int64_t blocks = (a.n_cand + kQueueThreads - 1) / kQueueThreads;
+  if (blocks > 0x7fffffff) return fail(GBX_ERR_ARG, "gbx_quartets: too many candidates");
+  if (fill)
+    hipLaunchKernelGGL(quartet_queue_kernel<true>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(kQueueThreads), 0, s, a);
+  else
+    hipLaunchKernelGGL(quartet_queue_kernel<false>, dim3(static_cast<unsigned>(blocks)),
+                       dim3(kQueueThreads), 0, s, a);
+  return check_launch(fill ? "quartet_queue_kernel<fill>" : "quartet_queue_kernel<count>");
+}
+
+}  // namespace gbx
+
+extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
+                                  const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
+                                  const int32_t* d_pair_npf, const double* d_pair_bound,
+                                  const double* d_pair_pbound, double threshold, double fp32_cutoff, int64_t n_cand,
+                                  int32_t* d_bucket_count, uint64_t* d_bucket_eris,
+                                  uint64_t* d_bucket_primq, void* stream) {
+  if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
+      !d_pair_pbound || !d_bucket_count || n_cand < 0 ||
+      (d_bucket_eris && (!d_bucket_primq || !d_pair_npf)))
+    return gbx::fail(GBX_ERR_ARG, "gbx_quartets_count: bad arguments");
+  gbx::QueueArgs a{};
+  a.n_mol = n_mol;
+  a.mol_cand_off = d_mol_cand_off;
+  a.mol_pair_off = d_mol_pair_off;
+  a.pair_key = d_pair_key;
+  a.pair_npf = d_pair_npf;
+  a.pair_bound = d_pair_bound;
+  a.pair_pbound = d_pair_pbound;
+  a.threshold = threshold;
+  a.fp32_cutoff = fp32_cutoff;
+  a.bucket_count = d_bucket_count;
+  a.bucket_eris = reinterpret_cast<unsigned long long*>(d_bucket_eris);
+  a.bucket_primq = reinterpret_cast<unsigned long long*>(d_bucket_primq);
+  a.n_cand = n_cand;
+  return gbx::launch_queue(a, false, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int gbx_quartets_fill(int32_t n_mol, const int64_t* d_mol_cand_off,
+                                 const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
+                                 const double* d_pair_bound, const double* d_pair_pbound,
+                                 double threshold, double fp32_cutoff,
+                                 int64_t n_cand, const int64_t* d_bucket_offset,
+                                 int32_t* d_bucket_cursor, int32_t* d_quartets, void* stream) {
+  if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
+      !d_pair_pbound || !d_bucket_offset || !d_bucket_cursor || !d_quartets || n_cand < 0)
+    return gbx::fail(GBX_ERR_ARG, "gbx_quartets_fill: bad arguments");
+  gbx::QueueArgs a{};
+  a.n_mol = n_mol;
+  a.mol_cand_off = d_mol_cand_off;
+  a.mol_pair_off = d_mol_pair_off;
+  a.pair_key = d_pair_key;
+  a.pair_bound = d_pair_bound;
+  a.pair_pbound = d_pair_pbound;
+  a.threshold = threshold;
+  a.fp32_cutoff = fp32_cutoff;
+  a.bucket_offset = d_bucket_offset;
+  a.bucket_cursor = d_bucket_cursor;
+  a.out = d_quartets;
+  a.n_cand = n_cand;
+  return gbx::launch_queue(a, true, static_cast<hipStream_t>(stream));
+}

@@ -55,16 +55,18 @@ __device__ inline uint64_t tri_u64(uint64_t i, uint64_t j) {
 // Write one canonical-order Cartesian block (NA x NB x NC x ND, factorial2 norms
 // applied) through the per-shell maps (component position, norm_cont / sign scale)
 // into the canonical packed slots, skipping the duplicates of diagonal pairs and
-// of bra == ket.  In GBX_MODE_SCHWARZ write sqrt(max |block|) for the bra pair.
+// of bra == ket.  In GBX_MODE_SCHWARZ write, for the bra pair, sqrt(max |block|)
+// to out[2 bra] and sqrt(max |contracted [e0|f0]|) (accmax) to out[2 bra + 1].
 template <int NA, int NB, int NC, int ND>
 __device__ inline void store_block(const gbx_tables& t, const RegArgs& a, int bra, int ket,
-                                   int4 pb, int4 pk, const double* out) {
+                                   int4 pb, int4 pk, const double* out, double accmax) {
   constexpr int N = NA * NB * NC * ND;
   if (a.mode == GBX_MODE_SCHWARZ) {
     double m = 0.0;
 #pragma unroll
     for (int i = 0; i < N; ++i) m = fmax(m, fabs(out[i]));
-    a.out[bra] = sqrt(m);
+    a.out[2 * bra] = sqrt(m);
+    a.out[2 * bra + 1] = sqrt(accmax);
     return;
   }
   const int4* si = reinterpret_cast<const int4*>(t.shell_info);

@@ -62,11 +62,12 @@ def _coord_types(basis, coord_type=None):
 
 
 def electron_repulsion_integral_packed(basis, coord_types=None, schwarz_threshold=0.0,
-                                       precision="fp64", fp32_cutoff=1e-3, device="cuda"):
+                                       precision="fp64", fp32_cutoff=1e-2, device="cuda"):
     """Canonical 8-fold packed ERIs (float64 device tensor) of one basis.
 
-    ``precision="mixed"`` runs shell quartets with Q_ij Q_kl < ``fp32_cutoff`` with
-    an FP32 recursion bulk (Boys, prefactors and accumulation stay FP64).
+    ``precision="mixed"`` runs shell quartets whose contracted [e0|f0] bound
+    P_ij P_kl is below ``fp32_cutoff`` with an FP32 recursion bulk (Boys,
+    prefactors, accumulation and the horizontal recursion stay FP64).
     Returns ``(packed, nfunc)``.
     """
     _check_basis(basis)
@@ -76,9 +77,9 @@ def electron_repulsion_integral_packed(basis, coord_types=None, schwarz_threshol
     mol = MoleculeTables(basis, ctypes_)
     dev = DeviceTables(BatchTables([mol]), device)
     need_q = schwarz_threshold > 0 or precision == "mixed"
-    q = eng.schwarz(dev).cpu().numpy() if need_q else None
-    plans = eng.plan_quartets(dev, q, schwarz_threshold,
-                              fp32_cutoff if precision == "mixed" else 0.0)
+    bound = eng.pair_bounds(dev, eng.schwarz(dev)) if need_q else None
+    plans = eng.plan_device(dev, bound, schwarz_threshold,
+                            fp32_cutoff if precision == "mixed" else 0.0, stats=False)
     packed = eng.run(dev, plans)
     return packed, mol.nfunc
 

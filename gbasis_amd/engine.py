@@ -22,7 +22,7 @@ from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_SCHWARZ, G
                    GbxTables, check, ptr)
 from .boys import BOYS_H, boys_table
 from .programs import class_program
-from .roofline import quartet_flops
+from .roofline import class_flops, quartet_flops
 from .tables import BatchTables, MoleculeTables, f2norm_table
 
 DEFAULT_SCRATCH_BYTES = 1 << 30
@@ -81,7 +81,7 @@ class DeviceTables:
         self.device = torch.device(device)
 
         def up(a, dtype=None):
-            a = np.ascontiguousarray(a if dtype is None else a.astype(dtype))
+            a = np.array(a if dtype is None else a.astype(dtype), copy=True, order="C")
             return torch.from_numpy(a).to(self.device)
 
         self.shell_info = up(batch.shell_info, np.int32)
@@ -93,6 +93,10 @@ class DeviceTables:
         self.pair_out_base = up(batch.pair_out_base.view(np.int64))
         self.prim_pairs = up(batch.prim_pairs, np.float64)
         self.f2norm = up(f2norm_table(8))
+        self.pair_key = up(batch.pair_key, np.int32)
+        self.pair_npf = up(batch.pair_npf, np.int32)
+        self.mol_cand_off = up(batch.mol_cand_off, np.int64)
+        self.mol_pair_off = up(np.array(batch.mol_pair_off), np.int32)
         tab = boys_table()
         self.boys = up(tab)
         st = GbxTables()
@@ -180,10 +184,15 @@ class EriEngine:
         return "coop"
 
     def schwarz(self, dev: DeviceTables):
-        """Q per global shell pair (un-normalised, as the reference's screener)."""
+        """Bounds per global shell pair, device tensor (npair, 2).
+
+        Column 0: Q = sqrt(max |(ij|ij)|), un-normalised as the reference's screener.
+        Column 1: sqrt(max |contracted [e0|f0] of (ij|ij)|), the bound on the inputs of
+        the horizontal recursion that selects the FP32 recursion bulk.
+        """
         b = dev.batch
         npair = b.npair
-        q = torch.zeros(npair, dtype=torch.float64, device=self.device)
+        q = torch.zeros((npair, 2), dtype=torch.float64, device=self.device)
         p = np.arange(npair, dtype=np.int32)
         for cls, nprimq, _, quart in b.class_buckets(p, p):
             dq = torch.from_numpy(quart).to(self.device)
@@ -199,10 +208,12 @@ class EriEngine:
         ``fp32_cutoff`` run with an FP32 recursion bulk; all others in FP64.
         """
         b = dev.batch
+        if q_pair is not None:
+            q_pair = np.asarray(q_pair).reshape(b.npair, -1)
         bras, kets = [], []
         for mi, m in enumerate(b.mols):
             po = b.mol_pair_off[mi]
-            qm = None if q_pair is None else q_pair[po:po + m.npair]
+            qm = None if q_pair is None else q_pair[po:po + m.npair, 0]
             bra, ket = m.quartets(qm, threshold)
             bras.append(bra.astype(np.int64) + po)
             kets.append(ket.astype(np.int64) + po)
@@ -212,9 +223,8 @@ class EriEngine:
         if fp32_cutoff > 0:
             if q_pair is None:
                 raise ValueError("mixed precision needs the Schwarz bounds")
-            qo = self._orig_bounds(b, q_pair)
-            bound = qo[bra] * qo[ket]
-            low = bound < fp32_cutoff
+            po_ = self._orig_bounds(b, q_pair[:, -1])
+            low = po_[bra] * po_[ket] < fp32_cutoff
             groups = [(GBX_FP32_BULK, low), (GBX_FP64, ~low)]
         else:
             groups = [(GBX_FP64, np.ones(bra.size, dtype=bool))]
@@ -229,6 +239,75 @@ class EriEngine:
                 plans.append(Bucket(cls, nprimq, prec, dense,
                                     torch.from_numpy(quart).to(self.device), quart.shape[0],
                                     eris, flops))
+        return plans
+
+    def pair_bounds(self, dev: DeviceTables, q_pair):
+        """(screening bound, precision bound) per global pair, each of its original-shell
+        pair (max over the expanded pairs of a generalised contraction); device tensors."""
+        b = dev.batch
+        q = q_pair[:, 0].contiguous()
+        p = q_pair[:, 1].contiguous()
+        if not b.generalized:
+            return q, p
+        qh = q_pair.cpu().numpy()
+        return (torch.from_numpy(self._orig_bounds(b, qh[:, 0])).to(self.device),
+                torch.from_numpy(self._orig_bounds(b, qh[:, 1])).to(self.device))
+
+    def plan_device(self, dev: DeviceTables, bound=None, threshold=0.0, fp32_cutoff=0.0,
+                    stats=True):
+        """Build the screened, bucketed work queue on the device (two passes).
+
+        ``bound`` is the (screening, precision) pair of per-pair device bounds from
+        pair_bounds(); required when screening or mixed precision is requested.
+        Returns a list of Bucket (quartets are views of one device buffer).  With ``stats`` the packed-slot and primitive-quartet
+        counts per bucket are accumulated for FLOP / ERI bookkeeping.
+        """
+        b = dev.batch
+        if bound is None:
+            if threshold > 0 or fp32_cutoff > 0:
+                raise ValueError("screening / mixed precision need the Schwarz bounds")
+            ones = torch.ones(b.npair, dtype=torch.float64, device=self.device)
+            bound = (ones, ones)
+        qb, pbd = bound
+        nb = 262144
+        counts = torch.zeros(nb, dtype=torch.int32, device=self.device)
+        eris = torch.zeros(nb, dtype=torch.int64, device=self.device) if stats else None
+        primq = torch.zeros(nb, dtype=torch.int64, device=self.device) if stats else None
+        st = self._stream()
+        check(self.lib.gbx_quartets_count(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
+                                          ptr(dev.pair_key), ptr(dev.pair_npf), ptr(qb),
+                                          ptr(pbd), float(threshold), float(fp32_cutoff), b.n_cand,
+                                          ptr(counts), ptr(eris), ptr(primq), st),
+              "gbx_quartets_count")
+        ch = counts.cpu().numpy().astype(np.int64)
+        offs = np.concatenate([[0], np.cumsum(ch)[:-1]]).astype(np.int64)
+        total = int(ch.sum())
+        quart = torch.empty((max(total, 1), 2), dtype=torch.int32, device=self.device)
+        d_offs = torch.from_numpy(offs).to(self.device)
+        cursor = torch.zeros(nb, dtype=torch.int32, device=self.device)
+        check(self.lib.gbx_quartets_fill(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
+                                         ptr(dev.pair_key), ptr(qb), ptr(pbd), float(threshold),
+                                         float(fp32_cutoff), b.n_cand, ptr(d_offs), ptr(cursor),
+                                         ptr(quart), st), "gbx_quartets_fill")
+        eh = eris.cpu().numpy() if stats else None
+        ph = primq.cpu().numpy() if stats else None
+        plans = []
+        for key in np.flatnonzero(ch):
+            key = int(key)
+            binid = key & 15
+            fp32 = (key >> 4) & 1
+            dense = bool((key >> 5) & 1)
+            c = key >> 6
+            cls = ((c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7)
+            n = int(ch[key])
+            view = quart[offs[key]:offs[key] + n]
+            flops = 0.0
+            if stats:
+                fpr, fco = class_flops(*cls, pure=dense)
+                flops = float(ph[key]) * fpr + n * fco
+            plans.append(Bucket(cls, 1 << binid, GBX_FP32_BULK if fp32 else GBX_FP64, dense,
+                                view, n, int(eh[key]) if stats else 0, flops))
+        self._last_queue = quart
         return plans
 
     @staticmethod

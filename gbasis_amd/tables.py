@@ -285,6 +285,20 @@ class BatchTables:
         kind = self.shell_info[:, 2]
         # pairs touching a pure shell with l >= 2 need the dense output transform
         self.pair_dense = (kind[self.pair_info[:, 0]] == 1) | (kind[self.pair_info[:, 1]] == 1)
+        # per-pair key for the device queue builder: la | lb << 3 | dense << 6 | nprim << 8
+        self.pair_key = (self.pair_la.astype(np.int64) | (self.pair_lb.astype(np.int64) << 3)
+                         | (self.pair_dense.astype(np.int64) << 6)
+                         | (self.pair_info[:, 3].astype(np.int64) << 8)).astype(np.int32)
+        nout = self.shell_info[:, 5].astype(np.int64)
+        sa, sb = self.pair_info[:, 0], self.pair_info[:, 1]
+        self.pair_npf = np.where(sa == sb, nout[sa] * (nout[sa] + 1) // 2,
+                                 nout[sa] * nout[sb]).astype(np.int32)
+        npairs = np.array([m.npair for m in molecules], dtype=np.int64)
+        self.mol_cand_off = np.concatenate([[0], np.cumsum(npairs * (npairs + 1) // 2)])
+        self.n_cand = int(self.mol_cand_off[-1])
+        # generalised contractions: the Schwarz bound of an expanded pair is the max over
+        # the pairs of the same original shells, which the host folds in
+        self.generalized = any(m.nshell != m.n_orig for m in molecules)
 
     def unique_counts(self, bra, ket):
         """Number of canonical packed slots each (bra, ket) quartet writes."""

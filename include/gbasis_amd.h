@@ -25,13 +25,15 @@ extern "C" {
 #define GBX_ERR_HIP -2
 #define GBX_ERR_UNSUPPORTED -3
 
-/* precision of the recursion bulk (Boys, prefactors, accumulation stay FP64) */
+/* precision of the recursion bulk (Boys, prefactors, accumulation and the
+ * horizontal recursion stay FP64) */
 #define GBX_FP64 0
 #define GBX_FP32_BULK 1
 
 /* output mode of the ERI kernels */
 #define GBX_MODE_PACKED 0  /* write canonical 8-fold packed slots        */
-#define GBX_MODE_SCHWARZ 1 /* write sqrt(max|(ij|ij)|) per bra pair       */
+#define GBX_MODE_SCHWARZ 1 /* per bra pair p: d_out[2p] = sqrt(max|(ij|ij)|),
+                              d_out[2p+1] = sqrt(max|contracted [e0|f0]|)  */
 
 /* Device-resident basis tables (layouts documented in DESIGN.md). */
 typedef struct gbx_tables {
@@ -86,8 +88,10 @@ int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t 
  * compute_two_electron_integrals_os_hgp (gbasis/integrals/_two_elec_int_improved.py:591-732)
  * plus the norm_cont / Cartesian->pure / 8-fold placement of
  * gbasis/base_four_symm.py:160-389.  In GBX_MODE_SCHWARZ (bra == ket pair) it
- * writes d_out[bra] = sqrt(max|(ij|ij)|) without norm_cont, as
- * compute_schwarz_bound_shell_pair (gbasis/integrals/_schwarz_screening.py:14-59).
+ * writes d_out[2 bra] = sqrt(max|(ij|ij)|) without norm_cont, as
+ * compute_schwarz_bound_shell_pair (gbasis/integrals/_schwarz_screening.py:14-59),
+ * and d_out[2 bra + 1] = sqrt of the largest contracted input of the horizontal
+ * recursion, the bound that selects the FP32 recursion bulk.
  * d_scratch (scratch_bytes) is only used when the class does not fit in LDS. */
 int gbx_eri_coop(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb,
                  int32_t lc, int32_t ld, const int32_t *d_quartets, int64_t nquart,
@@ -105,6 +109,29 @@ int gbx_has_reg_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld);
 int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                 const int32_t *d_quartets, int64_t nquart, int32_t precision, int32_t mode,
                 double *d_out, void *stream);
+
+/* Work-queue construction on the device for a batch of molecules: the unique
+ * shell-quartet loop of gbasis/base_four_symm.py:189-194 with the Schwarz test
+ * of SchwarzScreener.is_significant (gbasis/integrals/_schwarz_screening.py:169-191),
+ * kept iff bound_p * bound_q >= threshold (threshold <= 0 keeps all), run with
+ * the FP32 recursion bulk iff pbound_p * pbound_q < fp32_cutoff (pbound: the
+ * second value GBX_MODE_SCHWARZ writes per pair).  Buckets
+ * are keyed (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + ceil(log2 nprimq),
+ * GBX_NUM_BUCKETS of them.  count: quartets per bucket (and, if d_bucket_eris is
+ * not NULL, packed slots and primitive quartets per bucket).  fill: writes the
+ * (bra, ket) pairs at d_bucket_offset[key] + position (d_bucket_cursor zeroed). */
+#define GBX_NUM_BUCKETS 262144
+int gbx_quartets_count(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
+                       const int32_t *d_pair_key, const int32_t *d_pair_npf,
+                       const double *d_pair_bound, const double *d_pair_pbound,
+                       double threshold, double fp32_cutoff,
+                       int64_t n_cand, int32_t *d_bucket_count, uint64_t *d_bucket_eris,
+                       uint64_t *d_bucket_primq, void *stream);
+int gbx_quartets_fill(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
+                      const int32_t *d_pair_key, const double *d_pair_bound,
+                      const double *d_pair_pbound, double threshold, double fp32_cutoff,
+                      int64_t n_cand, const int64_t *d_bucket_offset,
+                      int32_t *d_bucket_cursor, int32_t *d_quartets, void *stream);
 
 /* Expand the canonical 8-fold packed array of one molecule (nfunc output
  * functions) to the dense (nfunc)^4 tensor; notation 0 = chemist (ab|cd),

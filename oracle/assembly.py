@@ -33,20 +33,20 @@ def _engine(s1, s2, s3, s4, exact=False):
             s2.coord, s2.angmom, s2.angmom_components_cart, s2.exps, s2.coeffs,
             s3.coord, s3.angmom, s3.angmom_components_cart, s3.exps, s3.coeffs,
             s4.coord, s4.angmom, s4.angmom_components_cart, s4.exps, s4.coeffs,
-            boys=boys_longdouble, dtype=np.longdouble).astype(np.float64)
+            boys=boys_longdouble, dtype=np.longdouble, ket="vrr").astype(np.float64)
     return os_hgp_quartet(s1.coord, s1.angmom, s1.angmom_components_cart, s1.exps, s1.coeffs,
                           s2.coord, s2.angmom, s2.angmom_components_cart, s2.exps, s2.coeffs,
                           s3.coord, s3.angmom, s3.angmom_components_cart, s3.exps, s3.coeffs,
                           s4.coord, s4.angmom, s4.angmom_components_cart, s4.exps, s4.coeffs)
 
 
-def schwarz_bounds(basis):
+def schwarz_bounds(basis, exact=False):
     """Q (n_sh, n_sh): sqrt(max |(ij|ij)|), un-normalised, original orientation."""
     n = len(basis)
     Q = np.zeros((n, n))
     for i in range(n):
         for j in range(i, n):
-            blk = _engine(basis[i], basis[j], basis[i], basis[j])
+            blk = _engine(basis[i], basis[j], basis[i], basis[j], exact)
             Q[i, j] = Q[j, i] = np.sqrt(np.max(np.abs(blk)))
     return Q
 
@@ -54,11 +54,12 @@ def schwarz_bounds(basis):
 def contraction_block(s1, s2, s3, s4, exact=False):
     """(M1,L1,M2,L2,M3,L3,M4,L4) Cartesian block with the reference's l-reordering.
 
-    ``exact=True`` evaluates the same recursions in x87 extended precision with an
-    extended-precision Boys function: the reference's FP64 arithmetic loses up to
-    ~1e-12 relative in badly conditioned orientations (electron transfer of a large
-    ket angular momentum with zeta/eta >> 1), so the device kernels are also held
-    against this value.
+    ``exact=True`` evaluates the integrals in x87 extended precision with an
+    extended-precision Boys function and the bounded-coefficient Obara-Saika build
+    of the ket centre instead of the electron transfer: the transfer relation
+    multiplies by zeta/eta and, for exponent ranges like 0.06..685, loses all
+    digits in FP64 (errors of O(1) on values of O(0.1)) and ~1e-3 even in x87
+    precision, so it cannot serve as the accuracy reference.
     """
     bra = s1.angmom < s2.angmom
     if bra:

@@ -61,7 +61,7 @@ def os_hgp_quartet(coord_a, la, comps_a, exps_a, coeffs_a,
                    coord_b, lb, comps_b, exps_b, coeffs_b,
                    coord_c, lc, comps_c, exps_c, coeffs_c,
                    coord_d, ld, comps_d, exps_d, coeffs_d,
-                   boys=boys_hyp1f1, dtype=np.float64):
+                   boys=boys_hyp1f1, dtype=np.float64, ket="etr"):
     """Contracted Cartesian ERI block in chemists' notation.
 
     Returns an array of shape ``(n_a, n_b, n_c, n_d, M_a, M_b, M_c, M_d)`` with
@@ -125,7 +125,33 @@ def os_hgp_quartet(coord_a, la, comps_a, exps_a, coeffs_a,
     # 4. ETR: E[c][a], m = 0 only, |a| <= L - |c|
     E = {(0, 0, 0): {a: V[a][0] for n in range(L + 1) for a in _comps(n)}}
     coef = QC + zoe[:, None] * PA
-    for n in range(1, Lcd + 1):
+    if ket == "vrr":
+        # Obara-Saika two-centre build of C (bounded coefficients), for the
+        # numerical-stability checks: [a|c+1_i]^(m) = QC_i [a|c]^(m) + WQ_i [a|c]^(m+1)
+        #   + c_i/(2eta)([a|c-1_i]^(m) - rho/eta [a|c-1_i]^(m+1)) + a_i/(2(zeta+eta)) [a-1_i|c]^(m+1)
+        roe = rho / eta
+        WQ = roe[:, None] * PQ
+        h2s = 0.5 / (zeta + eta)
+        K = {(0, 0, 0): {a: V[a] for n in range(L + 1) for a in _comps(n)}}
+        for n in range(1, Lcd + 1):
+            for c in _comps(n):
+                i = _step(c)
+                c1 = _minus(c, i)
+                k1 = K[c1]
+                k2 = K[_minus(c, i, 2)] if c[i] > 1 else None
+                row = {}
+                for na in range(0, L - n + 1):
+                    for a in _comps(na):
+                        mm = L - n - na + 1
+                        val = QC[:, i] * k1[a][:mm] + WQ[:, i] * k1[a][1:mm + 1]
+                        if k2 is not None:
+                            val = val + (c[i] - 1) * h2e * (k2[a][:mm] - roe * k2[a][1:mm + 1])
+                        if a[i] > 0:
+                            val = val + a[i] * h2s * k1[_minus(a, i)][1:mm + 1]
+                        row[a] = val
+                K[c] = row
+        E = {c: {a: v[0] for a, v in row.items()} for c, row in K.items()}
+    for n in range(1, Lcd + 1 if ket == "etr" else 1):
         for c in _comps(n):
             i = _step(c)
             c1 = _minus(c, i)

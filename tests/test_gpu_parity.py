@@ -59,8 +59,10 @@ def test_dense_vs_oracle(gpu, ls, coord):
     scale = max(1.0, np.abs(exact).max())
     # FP64 parity bar: 1e-12 Eh against the extended-precision evaluation
     assert np.abs(got - exact).max() <= 1e-12 * scale
-    # the reference-orientation FP64 oracle carries its own rounding (<= ~1e-11 here)
-    assert np.abs(got - ref64).max() <= 1e-10 * scale
+    # against the reference's own FP64 algorithm: no further than that algorithm's own
+    # rounding error (its electron transfer loses up to ~3e-10 relative on these inputs)
+    ref_err = np.abs(ref64 - exact)
+    assert np.all(np.abs(got - ref64) <= ref_err + 1e-12 * scale)
 
 
 def test_schwarz_bounds_and_screened_lists(gpu):
@@ -75,21 +77,31 @@ def test_schwarz_bounds_and_screened_lists(gpu):
     eng = get_engine()
     mol = MoleculeTables(basis)
     dev = DeviceTables(BatchTables([mol]))
-    q = eng.schwarz(dev).cpu().numpy()
+    q = eng.schwarz(dev).cpu().numpy()[:, 0]
     Qdev = mol.schwarz_orig(q)
     Qref = schwarz_bounds(basis)
-    np.testing.assert_allclose(Qdev, Qref, rtol=1e-12, atol=0)
+    Qex = schwarz_bounds(basis, exact=True)
+    # Q^2 is an integral: hold it to the 1e-12 Eh bar against the extended-precision value,
+    # and to the reference's own FP64 error against the reference
+    assert np.all(np.abs(Qdev ** 2 - Qex ** 2) <= 1e-12 * np.maximum(1.0, Qex ** 2))
+    assert np.all(np.abs(Qdev ** 2 - Qref ** 2) <= np.abs(Qref ** 2 - Qex ** 2)
+                  + 1e-12 * np.maximum(1.0, Qex ** 2))
     n = len(basis)
     iu = np.array([(i, j) for i in range(n) for j in range(i, n)])
     for thr in (1e-12, 1e-8, 1e-4):
         keep_dev = set()
         keep_ref = set()
+        borderline = 0
         for p, (i, j) in enumerate(iu):
             for (k, l) in iu[p:]:
+                if abs(Qex[i, j] * Qex[k, l] - thr) <= 1e-9 * thr:
+                    borderline += 1  # within the FP64 rounding of either engine: not compared
+                    continue
                 if Qdev[i, j] * Qdev[k, l] >= thr:
                     keep_dev.add((i, j, k, l))
                 if Qref[i, j] * Qref[k, l] >= thr:
                     keep_ref.add((i, j, k, l))
+        assert borderline == 0
         assert keep_dev == keep_ref
 
 
@@ -99,12 +111,33 @@ def test_screened_dense_vs_oracle(gpu):
     from oracle.assembly import dense_eri
 
     _print_case("screened dense")
-    basis = batch_molecule(77, n_atoms=4)
+    basis = batch_molecule(77, n_atoms=3)
     thr = 1e-6
     got = electron_repulsion_integral_improved(basis, notation="physicist", schwarz_threshold=thr)
-    ref = dense_eri(basis, notation="physicist", schwarz_threshold=thr)
-    assert np.abs(got - ref).max() <= 1e-10 * max(1.0, np.abs(ref).max())
-    assert np.count_nonzero(got == 0) == np.count_nonzero(ref == 0)
+    exact = dense_eri(basis, notation="physicist", schwarz_threshold=thr, exact=True)
+    ref64 = dense_eri(basis, notation="physicist", schwarz_threshold=thr)
+    scale = max(1.0, np.abs(exact).max())
+    assert np.abs(got - exact).max() <= 1e-12 * scale
+    # the reference algorithm's own FP64 error on these exponents (0.05..1e3) is large
+    assert np.all(np.abs(got - ref64) <= np.abs(ref64 - exact) + 1e-12 * scale)
+    # screened shell quartets are exact zero blocks
+    from oracle.assembly import basis_offsets, schwarz_bounds
+
+    Q = schwarz_bounds(basis, exact=True)
+    off = basis_offsets(basis, [s.coord_type for s in basis])
+    chem = got.transpose(0, 2, 1, 3)
+    n = len(basis)
+    screened = 0
+    for i in range(n):
+        for j in range(n):
+            for k in range(n):
+                for l in range(n):
+                    if Q[i, j] * Q[k, l] < thr:
+                        screened += 1
+                        blk = chem[off[i]:off[i + 1], off[j]:off[j + 1], off[k]:off[k + 1],
+                                   off[l]:off[l + 1]]
+                        assert np.all(blk == 0.0)
+    assert screened > 0
 
 
 def test_mixed_precision_error(gpu):
@@ -116,7 +149,7 @@ def test_mixed_precision_error(gpu):
     basis = batch_molecule(31, n_atoms=8)
     p64, n = electron_repulsion_integral_packed(basis, schwarz_threshold=1e-12)
     pmx, _ = electron_repulsion_integral_packed(basis, schwarz_threshold=1e-12,
-                                                precision="mixed", fp32_cutoff=1e-3)
+                                                precision="mixed", fp32_cutoff=1e-2)
     err = float((pmx - p64).abs().max())
     print(f"mixed max err {err:.3e}", flush=True)
     assert err <= 1e-8
@@ -140,3 +173,31 @@ def test_register_vs_cooperative(gpu):
     finally:
         eng.path = "auto"
     assert float((a - b).abs().max()) <= 1e-13 * max(1.0, float(a.abs().max()))
+
+
+def test_device_queue_matches_host(gpu):
+    """Device-built screened quartet lists equal the host enumeration, bucket by bucket."""
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("device queue")
+    eng = get_engine()
+    mols = [MoleculeTables(batch_molecule(s, n_atoms=n)) for s, n in ((1, 3), (2, 6), (3, 5))]
+    b = BatchTables(mols)
+    dev = DeviceTables(b)
+    q = eng.schwarz(dev)
+    bound = eng.pair_bounds(dev, q)
+    qh = q.cpu().numpy()
+    for thr, cut in ((0.0, 0.0), (1e-10, 0.0), (1e-10, 1e-2)):
+        dplans = eng.plan_device(dev, bound if (thr or cut) else None, thr, cut)
+        hplans = eng.plan_quartets(dev, qh, thr, cut)
+        dd, hh = {}, {}
+        for bk in dplans:
+            dd.setdefault((bk.cls, bk.precision, bk.dense), set()).update(
+                map(tuple, bk.quart.cpu().numpy().tolist()))
+        for bk in hplans:
+            hh.setdefault((bk.cls, bk.precision, bk.dense), set()).update(
+                map(tuple, bk.quart.cpu().numpy().tolist()))
+        assert dd == hh
+        assert sum(bk.eris for bk in dplans) == sum(bk.eris for bk in hplans)
