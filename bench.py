@@ -84,6 +84,21 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup):
             per_bucket / steps, plans)
 
 
+def class_table(plans, per_bucket, top=16):
+    """Per (kernel path, class, precision): time, quartets, ERIs, algorithmic TFLOP/s."""
+    groups = {}
+    for bk, t in zip(plans, per_bucket):
+        key = f"{bk.path}:{''.join(map(str, bk.cls))}:{'f32' if bk.precision else 'f64'}"
+        g = groups.setdefault(key, [0.0, 0, 0, 0.0])
+        g[0] += t
+        g[1] += bk.n
+        g[2] += bk.eris
+        g[3] += bk.flops
+    rows = sorted(groups.items(), key=lambda kv: -kv[1][0])[:top]
+    return [{"kernel": k, "ms": v[0] * 1e3, "quartets": v[1], "eris": v[2],
+             "tflops": v[3] / v[0] / 1e12 if v[0] > 0 else 0.0} for k, v in rows]
+
+
 def roofline_entry(plans, per_bucket):
     """Dominant kernel (by total time): algorithmic FLOP/s against the FP64 vector peak."""
     from gbasis_amd.roofline import PEAK_FP32_VECTOR, PEAK_FP64_VECTOR
@@ -244,6 +259,8 @@ def main():
         "wall_ms_per_step": {"fp64": w64 * 1e3, "mixed": wmx * 1e3},
         "roofline": roof,
         "roofline_mixed": roof_mx,
+        "classes_fp64": class_table(plans64, pb64),
+        "classes_mixed": class_table(plansmx, pbmx),
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(batch, plans64)

@@ -10,12 +10,14 @@
 // gbasis/base_four_symm.py:160-389.
 //
 // The recursion is driven by a per-class "program" (lists of independent items
-// per phase, built on the host).  A workgroup of 256 lanes processes PF
-// primitive quartets at once: every intermediate is stored slot-major with the
-// PF copies adjacent, so the 64 lanes of a wavefront that work on the same item
-// for consecutive copies touch consecutive LDS words.  Classes whose working set
-// does not fit in 64 KiB of LDS run the same code on a per-workgroup slice of a
-// global scratch buffer.
+// per phase, built on the host, with storage slots reused once a value is dead).
+// One wavefront (a 64-lane workgroup) owns one shell quartet at a time and
+// processes PF primitive quartets at once: every intermediate is stored
+// slot-major with the PF copies adjacent, so lanes working on the same item for
+// consecutive copies touch consecutive LDS words, and a phase boundary is a
+// single-wave barrier.  Several such workgroups share a CU.  Classes whose
+// working set does not fit in 64 KiB of LDS run the same code on a
+// per-workgroup slice of a global scratch buffer.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,7 +30,7 @@
 
 namespace gbx {
 
-constexpr int kCoopThreads = 256;
+constexpr int kCoopThreads = 64;
 constexpr int kPhVrr = 0, kPhKvrr = 1, kPhAcc = 2, kPhHrrK = 3, kPhHrrB = 4;
 constexpr int kScalars = 17;  // per-primitive-quartet scalars held in LDS
 
@@ -361,11 +363,11 @@ extern "C" int gbx_coop_workspace(const gbx_program* prog, int32_t la, int32_t l
                                   int32_t ld, int32_t precision, int32_t prim_copies,
                                   int64_t* lds_bytes, int64_t* global_bytes_per_block) {
   if (!prog || prim_copies < 1 || prim_copies > kCoopThreads)
-    return fail(GBX_ERR_ARG, "gbx_coop_workspace: bad arguments");
+    return fail(GBX_ERR_ARG, "gbx_coop_workspace: prim_copies must be in [1, 64]");
   (void)la; (void)lb; (void)lc; (void)ld;
   const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
   const bool fits = l.total <= kCoopLdsLimit;
-  if (lds_bytes) *lds_bytes = fits ? l.total : 0;
+  if (lds_bytes) *lds_bytes = l.total;
   if (global_bytes_per_block) *global_bytes_per_block = fits ? 0 : l.total;
   return GBX_OK;
 }
@@ -377,7 +379,7 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
   if (!t || !prog || !d_out || (nquart > 0 && !d_quartets))
     return fail(GBX_ERR_ARG, "gbx_eri_coop: null argument");
   if (prim_copies < 1 || prim_copies > kCoopThreads)
-    return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be in [1, 256]");
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be in [1, 64]");
   if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: class not in canonical orientation");
   if (prog->header[6] != la + lb + lc + ld || prog->header[6] + kTaylorTerms > t->boys_nord)

@@ -61,7 +61,7 @@ class ProgramCache:
             d_items = torch.from_numpy(np.ascontiguousarray(items) if items.size else
                                        np.zeros(8, np.int32)).to(self.device)
             d_phases = torch.from_numpy(phases).to(self.device)
-            d_out = torch.from_numpy(prog.out_slots).to(self.device)
+            d_out = torch.from_numpy(prog.c_out_slots).to(self.device)
             st = GbxProgram()
             st.items = ptr(d_items)
             st.phases = ptr(d_phases)
@@ -127,6 +127,7 @@ class EriEngine:
         self.scratch_bytes = int(scratch_bytes)
         self._scratch = None
         self._reg = {}
+        self._pf = {}
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
 
     def scratch(self):
@@ -137,24 +138,37 @@ class EriEngine:
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
+    LDS_TARGET = 32768   # bytes per workgroup: ~5 single-wave workgroups per CU
+
     def _plan(self, cls, nprimq, precision):
-        """Primitive quartets processed at once (PF) and whether LDS suffices."""
+        """Primitive quartets in flight (PF) for the cooperative kernel.
+
+        Largest power of two <= min(nprimq, 64) whose LDS workspace stays within
+        LDS_TARGET; if even PF = 1 does not fit 64 KiB the kernel runs on global
+        scratch, where PF = min(nprimq, 16) keeps the wave busy.
+        """
+        if (cls, nprimq, precision) in self._pf:
+            return self._pf[(cls, nprimq, precision)]
         st, _, _ = self.programs.get(cls)
-        pf = 1
-        while pf < min(nprimq, 64):
-            pf *= 2
         lds = ctypes.c_int64()
         glb = ctypes.c_int64()
-        while True:
+
+        def need(pf):
             check(self.lib.gbx_coop_workspace(ctypes.byref(st), *cls, precision, pf,
                                               ctypes.byref(lds), ctypes.byref(glb)),
                   "gbx_coop_workspace")
-            if lds.value > 0 or pf == 1:
-                break
+            return lds.value, glb.value
+
+        pf = 1
+        while pf < min(nprimq, 64):
+            pf *= 2
+        while pf > 1 and need(pf)[0] > self.LDS_TARGET:
             pf //= 2
-        if lds.value == 0:
-            # global-scratch mode: keep enough copies to occupy the workgroup
-            pf = max(1, min(16, 1 << int(np.ceil(np.log2(max(nprimq, 1))))))
+        if need(pf)[1] > 0:
+            pf = 1
+            while pf < min(nprimq, 16):
+                pf *= 2
+        self._pf[(cls, nprimq, precision)] = pf
         return pf
 
     def has_reg(self, cls):

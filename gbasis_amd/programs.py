@@ -269,20 +269,92 @@ class ClassProgram:
         self.out_slots = np.array([hslot[g] for g in outs], dtype=np.int32)
         self.nout = len(outs)
 
+        self._compact()
+
+    def _compact(self):
+        """Slot reuse for the cooperative kernel's storage.
+
+        The program above is in single-assignment form (one slot per value), which
+        the register-kernel generator needs.  The cooperative kernel keeps the values
+        in LDS, so here every value gets a storage slot only from the phase that
+        defines it to the last phase that reads it; a slot read for the last time in
+        phase k is reused from phase k+1 on (items of one phase run concurrently).
+        """
+        # per-primitive-quartet store: fields 1..5 of VRR/KVRR items are sources
+        nph = len(self.prim_phases)
+        last = {}
+        for k, (_, items) in enumerate(self.prim_phases):
+            for it in items:
+                for src in it[1:6]:
+                    if src >= 0:
+                        last[src] = k
+        for it in self.acc_phase[1]:
+            last[it[1]] = nph
+        remap, nslots = _allocate([list(range(self.L + 1))] +
+                                  [[it[0] for it in items] for _, items in self.prim_phases],
+                                  last)
+        self.c_prim_phases = [(kind, [[remap[it[0]]] + [remap[x] if x >= 0 else -1
+                                                        for x in it[1:6]] + it[6:]
+                                      for it in items]) for kind, items in self.prim_phases]
+        self.c_acc_phase = (PH_ACC, [[it[0], remap[it[1]]] + it[2:] for it in self.acc_phase[1]])
+        self.c_prim_slots = nslots
+        # contracted store of the horizontal recursion: fields 1..2 are sources
+        last = {}
+        for k, (_, items) in enumerate(self.hrr_phases):
+            for it in items:
+                last[it[1]] = k
+                last[it[2]] = k
+        end = len(self.hrr_phases)
+        for o in self.out_slots.tolist():
+            last[o] = end
+        remap, nslots = _allocate([list(range(self.nacc))] +
+                                  [[it[0] for it in items] for _, items in self.hrr_phases], last)
+        self.c_hrr_phases = [(kind, [[remap[it[0]], remap[it[1]], remap[it[2]]] + it[3:]
+                                     for it in items]) for kind, items in self.hrr_phases]
+        self.c_h_slots = nslots
+        self.c_out_slots = np.array([remap[o] for o in self.out_slots.tolist()], dtype=np.int32)
+
     def encode(self):
-        """Flatten to (header int32[16], items int32[n*8], phase table int32[3*nph])."""
-        phases = self.prim_phases + [self.acc_phase] + self.hrr_phases
+        """Flatten (slot-reusing form) to (header int32[16], items int32[n*8], phases int32[3*nph])."""
+        phases = self.c_prim_phases + [self.c_acc_phase] + self.c_hrr_phases
         items = []
         table = []
         for kind, its in phases:
             table.append([kind, len(items), len(items) + len(its)])
             items.extend(its)
-        n_prim_ph = len(self.prim_phases)
+        n_prim_ph = len(self.c_prim_phases)
         header = np.zeros(16, dtype=np.int32)
-        header[:8] = [self.prim_slots, self.nacc, self.h_slots, self.nout, n_prim_ph,
+        header[:8] = [self.c_prim_slots, self.nacc, self.c_h_slots, self.nout, n_prim_ph,
                       len(phases), self.L, len(items)]
         flat = np.array(items, dtype=np.int32).reshape(-1) if items else np.zeros(0, np.int32)
         return header, flat, np.array(table, dtype=np.int32).reshape(-1)
+
+
+def _allocate(defs_by_phase, last_use):
+    """Linear-scan slot allocation; defs_by_phase[0] are the inputs (phase -1).
+
+    A value whose last read is in phase k frees its slot for phases > k.  Returns
+    (old value -> slot, number of slots).
+    """
+    import heapq
+
+    free = []
+    nslots = 0
+    remap = {}
+    releases = {}
+    for k, defs in enumerate(defs_by_phase):
+        phase = k - 1
+        for v in releases.pop(phase - 1, []):
+            heapq.heappush(free, remap[v])
+        for v in defs:
+            if free:
+                slot = heapq.heappop(free)
+            else:
+                slot = nslots
+                nslots += 1
+            remap[v] = slot
+            releases.setdefault(last_use.get(v, phase), []).append(v)
+    return remap, nslots
 
 
 @lru_cache(maxsize=None)

@@ -75,13 +75,16 @@ const char *gbx_last_error(void);
 int gbx_boys(const gbx_tables *t, const double *d_T, int64_t n, int32_t mmax, double *d_out,
              void *stream);
 
-/* Workspace sizes of the cooperative kernel for one class program. */
+/* Workspace of the cooperative kernel for one class program and prim_copies
+ * (1..64) primitive quartets in flight: *lds_bytes = bytes per workgroup;
+ * *global_bytes_per_block = the same when it exceeds 64 KiB (the kernel then
+ * works in a slice of d_scratch per workgroup), else 0. */
 int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                        int32_t precision, int32_t prim_copies, int64_t *lds_bytes,
                        int64_t *global_bytes_per_block);
 
 /* One angular-momentum class of shell quartets with the cooperative
- * (workgroup-per-quartet) kernel.  d_quartets holds nquart (bra pair, ket pair)
+ * (one 64-lane workgroup per shell quartet) kernel.  d_quartets holds nquart (bra pair, ket pair)
  * int32 pairs in canonical orientation.  Replaces, for a batch of quartets,
  * ElectronRepulsionIntegralImproved.construct_array_contraction
  * (gbasis/integrals/electron_repulsion.py:305-412) ->

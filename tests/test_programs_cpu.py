@@ -18,10 +18,20 @@ def _f2(c):
     return 1 / np.sqrt(df(2 * c[0] - 1) * df(2 * c[1] - 1) * df(2 * c[2] - 1))
 
 
-def run_program(cls, A, B, C, D, ex, co):
-    """Execute a class program exactly as the kernels do (FP64, NumPy)."""
+def run_program(cls, A, B, C, D, ex, co, compact=False):
+    """Execute a class program exactly as the kernels do (FP64, NumPy).
+
+    ``compact`` runs the slot-reusing form the cooperative kernel interprets, with
+    every phase evaluated as one parallel step (all reads before any write).
+    """
     la, lb, lc, ld = cls
     prog = class_program(*cls)
+    prim_phases = prog.c_prim_phases if compact else prog.prim_phases
+    acc_phase = prog.c_acc_phase if compact else prog.acc_phase
+    hrr_phases = prog.c_hrr_phases if compact else prog.hrr_phases
+    out_slots = prog.c_out_slots if compact else prog.out_slots
+    nslot = prog.c_prim_slots if compact else prog.prim_slots
+    hslot = prog.c_h_slots if compact else prog.h_slots
     L = prog.L
     acc = np.zeros(prog.nacc)
 
@@ -39,11 +49,12 @@ def run_program(cls, A, B, C, D, ex, co):
                     T = z * e * inv * PQ @ PQ
                     kab = co[0][i] * co[1][j] * N(a, la) * N(b, lb) * np.exp(-a * b / z * np.sum((A - B) ** 2))
                     kcd = co[2][k] * co[3][l] * N(c, lc) * N(d, ld) * np.exp(-c * d / e * np.sum((C - D) ** 2))
-                    S = np.zeros(prog.prim_slots)
+                    S = np.zeros(nslot)
                     S[:L + 1] = boys_hyp1f1(np.arange(L + 1), T) * 2 * np.pi ** 2.5 / (z * e) * np.sqrt(inv) * kab * kcd
                     PA, WP, roz, i2z = P - A, -e * inv * PQ, e * inv, 0.5 / z
                     QC, WQ, roe, i2e, i2s = Q - C, z * inv * PQ, z * inv, 0.5 / e, 0.5 * inv
-                    for kind, items in prog.prim_phases:
+                    for kind, items in prim_phases:
+                        writes = []
                         for dst, s0, s1, s2, s3, s4, code, _ in items:
                             di, k1, k2 = code & 15, (code >> 4) & 255, (code >> 12) & 255
                             if kind == PH_VRR:
@@ -56,31 +67,35 @@ def run_program(cls, A, B, C, D, ex, co):
                                     v += k1 * i2e * (S[s2] - roe * S[s3])
                                 if s4 >= 0:
                                     v += k2 * i2s * S[s4]
+                            writes.append((dst, v))
+                        for dst, v in writes:
                             S[dst] = v
-                    for it in prog.acc_phase[1]:
+                    for it in acc_phase[1]:
                         acc[it[0]] += S[it[1]]
-    H = np.zeros(prog.h_slots)
+    H = np.zeros(hslot)
     H[:prog.nacc] = acc
-    for kind, items in prog.hrr_phases:
+    for kind, items in hrr_phases:
         g = (C - D) if kind == PH_HRRK else (A - B)
-        for it in items:
-            H[it[0]] = H[it[1]] + g[it[3]] * H[it[2]]
+        writes = [(it[0], H[it[1]] + g[it[3]] * H[it[2]]) for it in items]
+        for dst, v in writes:
+            H[dst] = v
     outs = [(a, b, c, d) for a in comps(la) for b in comps(lb) for c in comps(lc) for d in comps(ld)]
     vals = np.array([H[s] * _f2(a) * _f2(b) * _f2(c) * _f2(d)
-                     for s, (a, b, c, d) in zip(prog.out_slots, outs)])
+                     for s, (a, b, c, d) in zip(out_slots, outs)])
     return vals.reshape([len(comps(x)) for x in cls])
 
 
 @pytest.mark.parametrize("cls", [(0, 0, 0, 0), (1, 0, 0, 0), (1, 1, 0, 0), (1, 0, 1, 0),
                                  (1, 1, 1, 1), (2, 0, 1, 0), (2, 1, 1, 1), (2, 2, 2, 1),
                                  (3, 1, 2, 0), (2, 2, 2, 2), (3, 0, 3, 0)])
-def test_program_matches_oracle(cls):
+@pytest.mark.parametrize("compact", [False, True])
+def test_program_matches_oracle(cls, compact):
     rng = np.random.default_rng(sum(cls) * 7 + cls[0])
     A, B, C, D = rng.uniform(-1, 1, (4, 3))
     ex = [rng.uniform(0.2, 3, 2) for _ in range(4)]
     co = [rng.uniform(0.1, 1, 2) for _ in range(4)]
     la, lb, lc, ld = cls
-    got = run_program(cls, A, B, C, D, ex, co)
+    got = run_program(cls, A, B, C, D, ex, co, compact)
     ref = os_hgp_quartet(A, la, comps(la), ex[0], co[0], B, lb, comps(lb), ex[1], co[1],
                          C, lc, comps(lc), ex[2], co[2], D, ld, comps(ld), ex[3], co[3])
     ref = ref[..., 0, 0, 0, 0]
@@ -93,3 +108,5 @@ def test_program_sizes_monotone():
     assert big.prim_slots > small.prim_slots and big.h_slots > small.h_slots
     for p in (small, big):
         assert len(set(p.out_slots.tolist())) == p.nout
+        assert len(set(p.c_out_slots.tolist())) == p.nout
+        assert p.c_prim_slots < p.prim_slots and p.c_h_slots < p.h_slots
