@@ -15,6 +15,7 @@ Run as a script (``build()`` does) to regenerate the source.
 
 import os
 import sys
+from functools import lru_cache
 
 if __package__ in (None, ""):
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
@@ -25,8 +26,112 @@ else:
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "gbx_reg_gen.hip")
 
 
+REG_BUDGET = 220     # FP64 values a lane may hold (accumulators + live intermediates)
+SPLIT_BUDGET = 200   # same, per split of a large class
+
+
+def _deps(p):
+    """value -> list of source values of its recursion item (single-assignment program)."""
+    deps = {}
+    for _, items in p.prim_phases:
+        for it in items:
+            deps[it[0]] = [x for x in it[1:6] if x >= 0]
+    return deps
+
+
+def schedule(p, finals):
+    """Register-pressure-aware order of the items that produce ``finals``.
+
+    Greedy list scheduling over the recursion DAG: among the ready items pick the
+    one that frees the most operands (their last reader) and creates no long-lived
+    value; every final is accumulated as soon as it is produced.  Returns
+    (ordered item destinations, peak number of live values).
+    """
+    deps = _deps(p)
+    need = set()
+    stack = list(finals)
+    while stack:
+        x = stack.pop()
+        if x in need:
+            continue
+        need.add(x)
+        stack.extend(deps.get(x, []))
+    users = {}
+    for d in need:
+        if d <= p.L:
+            continue
+        for src in set(deps[d]):
+            users.setdefault(src, set()).add(d)
+    remaining = {v: len(u) for v, u in users.items()}
+    pending = {d: sum(1 for src in set(deps[d]) if src > p.L) for d in need if d > p.L}
+    ready = {d for d, c in pending.items() if c == 0}
+    live = {m for m in range(p.L + 1) if m in need}
+    finals = set(finals)
+    order = []
+    peak = len(live)
+    while ready:
+        best, best_key = None, None
+        for d in ready:
+            freed = sum(1 for src in set(deps[d]) if remaining.get(src, 0) == 1)
+            created = 0 if (d in finals and d not in users) else 1
+            key = (freed - created, -d)
+            if best_key is None or key > best_key:
+                best, best_key = d, key
+        ready.discard(best)
+        order.append(best)
+        live.add(best)
+        peak = max(peak, len(live))
+        for src in set(deps[best]):
+            remaining[src] -= 1
+            if remaining[src] == 0:
+                live.discard(src)
+        if remaining.get(best, 0) == 0:
+            live.discard(best)
+        for u in users.get(best, ()):
+            pending[u] -= 1
+            if pending[u] == 0:
+                ready.add(u)
+    return order, peak
+
+
+def _finals_by_bra(p):
+    """Accumulator indices ordered by bra component first (splits share the ket build)."""
+    la, lb, lc, ld = p.cls
+    final_e = [(c, a) for nc in range(lc, lc + ld + 1) for c in comps(nc)
+               for na in range(la, la + lb + 1) for a in comps(na)]
+    idx = sorted(range(len(final_e)),
+                 key=lambda f: (sum(final_e[f][1]), comps(sum(final_e[f][1])).index(final_e[f][1]),
+                                sum(final_e[f][0])))
+    return idx
+
+
+@lru_cache(maxsize=None)
+def class_plan(cls):
+    """('fused', [(acc indices, order)]) or ('split', [(acc indices, order), ...])."""
+    p = class_program(*cls)
+    src_of = [it[1] for it in p.acc_phase[1]]          # acc index -> final value
+    all_f = list(range(p.nacc))
+    order, peak = schedule(p, [src_of[f] for f in all_f])
+    if p.nacc + peak <= REG_BUDGET:
+        return "fused", [(all_f, order)]
+    byb = _finals_by_bra(p)
+    for S in range(2, 33):
+        chunks = [byb[k * len(byb) // S:(k + 1) * len(byb) // S] for k in range(S)]
+        parts = []
+        ok = True
+        for ch in chunks:
+            o, pk = schedule(p, [src_of[f] for f in ch])
+            if len(ch) + pk > SPLIT_BUDGET:
+                ok = False
+                break
+            parts.append((ch, o))
+        if ok:
+            return "split", parts
+    raise RuntimeError(f"class {cls}: no split fits the register budget")
+
+
 def _reg_classes():
-    """Kernel-orientation classes with l <= 2 per shell whose working set fits registers."""
+    """Kernel-orientation classes with l <= 2 per shell (generated register kernels)."""
     out = []
     for la in range(3):
         for lb in range(la + 1):
@@ -34,14 +139,8 @@ def _reg_classes():
                 for ld in range(lc + 1):
                     if (la + lb, la) < (lc + ld, lc):
                         continue
-                    p = class_program(la, lb, lc, ld)
-                    if p.nacc <= REG_MAX_ACC and p.prim_slots <= REG_MAX_SLOTS:
-                        out.append((la, lb, lc, ld))
+                    out.append((la, lb, lc, ld))
     return out
-
-
-REG_MAX_ACC = 96
-REG_MAX_SLOTS = 260
 
 
 def _f2(comp):
@@ -54,23 +153,20 @@ def _f2(comp):
     return 1.0 / (df(2 * comp[0] - 1) * df(2 * comp[1] - 1) * df(2 * comp[2] - 1)) ** 0.5
 
 
-def kernel_source(cls):
-    la, lb, lc, ld = cls
-    p = class_program(*cls)
+def _item_table(p):
+    table = {}
+    for kind, items in p.prim_phases:
+        for it in items:
+            table[it[0]] = (kind, it)
+    return table
+
+
+def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids):
+    """Loop over bra x ket primitive pairs; straight-line recursion in ``order``;
+    every final value is added to its FP64 accumulator as soon as it exists."""
     L = p.L
-    name = f"eri_reg_{la}{lb}{lc}{ld}"
-    o = []
-    w = o.append
-    w(f"// class ({la}{lb}|{lc}{ld}): {p.prim_slots} primitive intermediates, "
-      f"{p.nacc} accumulators, {p.nout} outputs")
-    w("template <typename TB>")
-    w(f"__global__ __launch_bounds__(kRegThreads) void {name}(gbx_tables t, RegArgs a) {{")
-    w("  const int64_t q = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
-    w("  if (q >= a.nquart) return;")
-    w("  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];")
-    w("  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];")
-    w("  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];")
-    for f in range(p.nacc):
+    table = _item_table(p)
+    for f in acc_ids:
         w(f"  double acc{f} = 0.0;")
     w("  for (int ib = 0; ib < pb.w; ++ib) {")
     w("    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
@@ -80,50 +176,83 @@ def kernel_source(cls):
     w("      const double T = prim_quartet_setup(rb, rk, s);")
     w(f"      double F[{L + 1}];")
     w(f"      boys_eval_fixed<{L}>(T, t, F);")
+    used = set()
+    for d in order:
+        used.update(x for x in table[d][1][1:6] if x >= 0)
     for m in range(L + 1):
-        w(f"      const TB x{m} = static_cast<TB>(F[{m}] * s.pref);")
-    # coefficient multiples used by the items
-    k_vrr = set()
-    k_kv1 = set()
-    k_kv2 = set()
-    for kind, items in p.prim_phases:
-        for it in items:
-            code = it[6]
-            k1, k2 = (code >> 4) & 255, (code >> 12) & 255
-            if kind == PH_VRR and it[3] >= 0:
-                k_vrr.add(k1)
-            if kind == PH_KVRR:
-                if it[3] >= 0:
-                    k_kv1.add(k1)
-                if it[5] >= 0:
-                    k_kv2.add(k2)
+        if m in used or m in acc_of_value:
+            w(f"      const TB x{m} = static_cast<TB>(F[{m}] * s.pref);")
+    k_vrr, k_kv1, k_kv2 = set(), set(), set()
+    for d in order:
+        kind, it = table[d]
+        code = it[6]
+        k1, k2 = (code >> 4) & 255, (code >> 12) & 255
+        if kind == PH_VRR and it[3] >= 0:
+            k_vrr.add(k1)
+        if kind == PH_KVRR:
+            if it[3] >= 0:
+                k_kv1.add(k1)
+            if it[5] >= 0:
+                k_kv2.add(k2)
     for k in sorted(k_vrr):
         w(f"      const TB z{k} = static_cast<TB>({k}) * s.i2z;")
     for k in sorted(k_kv1):
         w(f"      const TB e{k} = static_cast<TB>({k}) * s.i2e;")
     for k in sorted(k_kv2):
         w(f"      const TB r{k} = static_cast<TB>({k}) * s.i2s;")
-    for kind, items in p.prim_phases:
-        for it in items:
-            dst, s0, s1, s2, s3, s4, code, _ = it
-            i = code & 15
-            k1, k2 = (code >> 4) & 255, (code >> 12) & 255
-            if kind == PH_VRR:
-                e = f"s.PA[{i}] * x{s0} + s.WP[{i}] * x{s1}"
-                if s2 >= 0:
-                    e += f" + z{k1} * (x{s2} - s.roz * x{s3})"
-            else:
-                e = f"s.QC[{i}] * x{s0} + s.WQ[{i}] * x{s1}"
-                if s2 >= 0:
-                    e += f" + e{k1} * (x{s2} - s.roe * x{s3})"
-                if s4 >= 0:
-                    e += f" + r{k2} * x{s4}"
-            w(f"      const TB x{dst} = {e};")
-    for it in p.acc_phase[1]:
-        w(f"      acc{it[0]} += static_cast<double>(x{it[1]});")
+    # finals that are Boys values themselves (classes with L = 0)
+    for m in range(L + 1):
+        for f in acc_of_value.get(m, ()):
+            w(f"      acc{f} += static_cast<double>(x{m});")
+    for d in order:
+        kind, it = table[d]
+        dst, s0, s1, s2, s3, s4, code, _ = it
+        i = code & 15
+        k1, k2 = (code >> 4) & 255, (code >> 12) & 255
+        if kind == PH_VRR:
+            e = f"s.PA[{i}] * x{s0} + s.WP[{i}] * x{s1}"
+            if s2 >= 0:
+                e += f" + z{k1} * (x{s2} - s.roz * x{s3})"
+        else:
+            e = f"s.QC[{i}] * x{s0} + s.WQ[{i}] * x{s1}"
+            if s2 >= 0:
+                e += f" + e{k1} * (x{s2} - s.roe * x{s3})"
+            if s4 >= 0:
+                e += f" + r{k2} * x{s4}"
+        w(f"      const TB x{dst} = {e};")
+        for f in acc_of_value.get(dst, ()):
+            w(f"      acc{f} += static_cast<double>(x{dst});")
     w("    }")
     w("  }")
-    # HRR
+
+
+def _head(w, name):
+    w("template <typename TB>")
+    w(f"__global__ __launch_bounds__(kRegThreads) void {name}(gbx_tables t, RegArgs a) {{")
+    w("  const int64_t q = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
+    w("  if (q >= a.nquart) return;")
+    w("  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];")
+    w("  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];")
+    w("  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];")
+
+
+def kernel_source(cls):
+    """Fused kernel: primitive loop, FP64 horizontal recursion, norms, placement."""
+    la, lb, lc, ld = cls
+    p = class_program(*cls)
+    kind, parts = class_plan(cls)
+    assert kind == "fused"
+    acc_ids, order = parts[0]
+    acc_of_value = {}
+    for f, it in enumerate(p.acc_phase[1]):
+        acc_of_value.setdefault(it[1], []).append(f)
+    name = f"eri_reg_{la}{lb}{lc}{ld}"
+    o = []
+    w = o.append
+    w(f"// class ({la}{lb}|{lc}{ld}): {len(order)} recursion values, {p.nacc} accumulators, "
+      f"{p.nout} outputs")
+    _head(w, name)
+    _emit_primitive_loop(w, p, order, acc_of_value, acc_ids)
     # largest contracted [e0|f0] (input of the horizontal recursion), for the
     # precision bound of GBX_MODE_SCHWARZ
     w("  double accmax = 0.0;")
@@ -138,11 +267,10 @@ def kernel_source(cls):
       "t.pair_geom[4 * ket + 2]};")
     for f in range(p.nacc):
         w(f"  const double h{f} = acc{f};")
-    for kind, items in p.hrr_phases:
-        g = "CD" if kind == PH_HRRK else "AB"
+    for kind_, items in p.hrr_phases:
+        g = "CD" if kind_ == PH_HRRK else "AB"
         for it in items:
             w(f"  const double h{it[0]} = h{it[1]} + {g}[{it[3]}] * h{it[2]};")
-    # outputs with factorial2 norms
     outs = [(a, b, c, d) for a in comps(la) for b in comps(lb) for c in comps(lc)
             for d in comps(ld)]
     nca, ncb, ncc, ncd = (len(comps(x)) for x in cls)
@@ -158,6 +286,30 @@ def kernel_source(cls):
     return "\n".join(o)
 
 
+def split_kernel_source(cls, k):
+    """Split kernel k: primitive loop for one subset of the accumulators, written to
+    a.acc[f * a.acc_stride + q] for the horizontal-recursion pass."""
+    la, lb, lc, ld = cls
+    p = class_program(*cls)
+    _, parts = class_plan(cls)
+    acc_ids, order = parts[k]
+    src_of = [it[1] for it in p.acc_phase[1]]
+    acc_of_value = {}
+    for f in acc_ids:
+        acc_of_value.setdefault(src_of[f], []).append(f)
+    name = f"eri_split_{la}{lb}{lc}{ld}_{k}"
+    o = []
+    w = o.append
+    w(f"// class ({la}{lb}|{lc}{ld}) split {k}/{len(parts)}: {len(order)} recursion values, "
+      f"{len(acc_ids)} accumulators")
+    _head(w, name)
+    _emit_primitive_loop(w, p, order, acc_of_value, acc_ids)
+    for f in acc_ids:
+        w(f"  a.acc[static_cast<int64_t>({f}) * a.acc_stride + q] = acc{f};")
+    w("}")
+    return "\n".join(o)
+
+
 def generate(path=OUT):
     classes = _reg_classes()
     parts = [
@@ -168,40 +320,75 @@ def generate(path=OUT):
         "namespace gbx {",
         "",
     ]
+    fused, split = [], []
     for cls in classes:
-        parts.append(kernel_source(cls))
+        kind, pl = class_plan(cls)
+        if kind == "fused":
+            fused.append(cls)
+            parts.append(kernel_source(cls))
+        else:
+            split.append((cls, len(pl)))
+            for k in range(len(pl)):
+                parts.append(split_kernel_source(cls, k))
         parts.append("")
+
+    def key(c):
+        return ((c[0] * 8 + c[1]) * 8 + c[2]) * 8 + c[3]
+
     parts.append("int launch_reg_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
                  "const RegArgs& a, int precision, hipStream_t s) {")
-    parts.append("  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;")
     parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
                  "kRegThreads);")
-    parts.append("  switch (key) {")
-    for cls in classes:
-        la, lb, lc, ld = cls
-        key = ((la * 8 + lb) * 8 + lc) * 8 + ld
-        name = f"eri_reg_{la}{lb}{lc}{ld}"
-        parts.append(f"    case {key}:")
+    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
+    for cls in fused:
+        name = f"eri_reg_{''.join(map(str, cls))}"
+        parts.append(f"    case {key(cls)}:")
         parts.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
                      "dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
         parts.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
                      "dim3(kRegThreads), 0, s, *t, a);")
-        parts.append("      return check_launch(\"" + name + "\");")
+        parts.append(f'      return check_launch("{name}");')
     parts.append("    default:")
-    parts.append("      return fail(GBX_ERR_UNSUPPORTED, \"gbx_eri_reg: no register kernel for "
-                 "this class\");")
+    parts.append('      return fail(GBX_ERR_UNSUPPORTED, "no fused register kernel for this class");')
+    parts.append("  }")
+    parts.append("}")
+    parts.append("")
+    parts.append("int launch_split_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
+                 "int k, const RegArgs& a, int precision, hipStream_t s) {")
+    parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
+                 "kRegThreads);")
+    parts.append("  switch ((((la * 8 + lb) * 8 + lc) * 8 + ld) * 64 + k) {")
+    for cls, n in split:
+        for k in range(n):
+            name = f"eri_split_{''.join(map(str, cls))}_{k}"
+            parts.append(f"    case {key(cls) * 64 + k}:")
+            parts.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
+                         "dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
+            parts.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
+                         "dim3(kRegThreads), 0, s, *t, a);")
+            parts.append(f'      return check_launch("{name}");')
+    parts.append("    default:")
+    parts.append('      return fail(GBX_ERR_UNSUPPORTED, "no split register kernel for this class");')
     parts.append("  }")
     parts.append("}")
     parts.append("")
     parts.append("bool has_reg_generated(int la, int lb, int lc, int ld) {")
-    parts.append("  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;")
-    parts.append("  switch (key) {")
-    for cls in classes:
-        la, lb, lc, ld = cls
-        parts.append(f"    case {((la * 8 + lb) * 8 + lc) * 8 + ld}:")
+    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
+    for cls in fused:
+        parts.append(f"    case {key(cls)}:")
     parts.append("      return true;")
     parts.append("    default:")
     parts.append("      return false;")
+    parts.append("  }")
+    parts.append("}")
+    parts.append("")
+    parts.append("int split_count_generated(int la, int lb, int lc, int ld) {")
+    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
+    for cls, n in split:
+        parts.append(f"    case {key(cls)}:")
+        parts.append(f"      return {n};")
+    parts.append("    default:")
+    parts.append("      return 0;")
     parts.append("  }")
     parts.append("}")
     parts.append("")
@@ -216,4 +403,4 @@ def generate(path=OUT):
 
 if __name__ == "__main__":
     cl = generate()
-    print(f"generated {len(cl)} register kernels -> {OUT}")
+    print(f"generated register kernels for {len(cl)} classes -> {OUT}")

@@ -18,6 +18,8 @@ struct RegArgs {
   int64_t nquart;
   int32_t mode;          // GBX_MODE_PACKED or GBX_MODE_SCHWARZ
   double* out;
+  double* acc;           // split kernels: contracted [e0|f0] out, acc[f * acc_stride + q]
+  int64_t acc_stride;
 };
 
 // F_m(T), m = 0..L, into F[] (same algorithm as boys_eval, unrolled for fixed L).
@@ -116,6 +118,9 @@ __device__ inline void store_block(const gbx_tables& t, const RegArgs& a, int br
 
 int launch_reg_generated(const gbx_tables* t, int la, int lb, int lc, int ld, const RegArgs& a,
                          int precision, hipStream_t s);
+int launch_split_generated(const gbx_tables* t, int la, int lb, int lc, int ld, int k,
+                           const RegArgs& a, int precision, hipStream_t s);
 bool has_reg_generated(int la, int lb, int lc, int ld);
+int split_count_generated(int la, int lb, int lc, int ld);
 
 }  // namespace gbx

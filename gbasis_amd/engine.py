@@ -127,6 +127,7 @@ class EriEngine:
         self.scratch_bytes = int(scratch_bytes)
         self._scratch = None
         self._reg = {}
+        self._split = {}
         self._pf = {}
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
 
@@ -171,6 +172,11 @@ class EriEngine:
         self._pf[(cls, nprimq, precision)] = pf
         return pf
 
+    def split_count(self, cls):
+        if cls not in self._split:
+            self._split[cls] = int(self.lib.gbx_split_count(*cls))
+        return self._split[cls]
+
     def has_reg(self, cls):
         if cls not in self._reg:
             self._reg[cls] = bool(self.lib.gbx_has_reg_kernel(*cls))
@@ -189,6 +195,20 @@ class EriEngine:
                                        self._stream()), f"gbx_eri_reg{cls}")
             return "reg"
         st, _, _ = self.programs.get(cls)
+        nsplit = self.split_count(cls)
+        if path != "coop" and nsplit > 0:
+            nq = quartets.shape[0]
+            nacc = st.header[1]
+            acc = torch.empty(nacc * nq, dtype=torch.float64, device=self.device)
+            for k in range(nsplit):
+                check(self.lib.gbx_eri_split(ctypes.byref(dev.struct), *cls, k, ptr(quartets), nq,
+                                             precision, ptr(acc), nq, self._stream()),
+                      f"gbx_eri_split{cls}[{k}]")
+            sc = self.scratch()
+            check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+                                       ptr(quartets), nq, mode, ptr(acc), nq, ptr(out), ptr(sc),
+                                       sc.numel(), self._stream()), f"gbx_eri_hrr{cls}")
+            return "split"
         pf = self._plan(cls, nprimq, precision)
         sc = self.scratch()
         check(self.lib.gbx_eri_coop(ctypes.byref(dev.struct), ctypes.byref(st), *cls,

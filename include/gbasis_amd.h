@@ -136,6 +136,29 @@ int gbx_quartets_fill(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_
                       int64_t n_cand, const int64_t *d_bucket_offset,
                       int32_t *d_bucket_cursor, int32_t *d_quartets, void *stream);
 
+/* Number of split register kernels for a class too large for one lane's
+ * registers (0 if the class has a fused kernel or none). */
+int gbx_split_count(int32_t la, int32_t lb, int32_t lc, int32_t ld);
+
+/* Split kernel `split` of a large class: the primitive loop (Boys, vertical
+ * recursion, ket build, FP64 accumulation -- the part of
+ * compute_two_electron_integrals_os_hgp, gbasis/integrals/_two_elec_int_improved.py:651-716,
+ * before the horizontal recursion) for one subset of the contracted [e0|f0],
+ * written to d_acc[f * acc_stride + q] (f: accumulator index of the class
+ * program, q: quartet index). */
+int gbx_eri_split(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                  int32_t split, const int32_t *d_quartets, int64_t nquart, int32_t precision,
+                  double *d_acc, int64_t acc_stride, void *stream);
+
+/* Horizontal recursion, norms, output transform and placement (or the Schwarz
+ * bounds in GBX_MODE_SCHWARZ) from contracted [e0|f0] produced by the split
+ * kernels: the rest of gbx_eri_coop (gbasis/integrals/_two_elec_int_improved.py:364-588,
+ * gbasis/base_four_symm.py:160-389). */
+int gbx_eri_hrr(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb, int32_t lc,
+                int32_t ld, const int32_t *d_quartets, int64_t nquart, int32_t mode,
+                const double *d_acc, int64_t acc_stride, double *d_out, void *d_scratch,
+                int64_t scratch_bytes, void *stream);
+
 /* Expand the canonical 8-fold packed array of one molecule (nfunc output
  * functions) to the dense (nfunc)^4 tensor; notation 0 = chemist (ab|cd),
  * 1 = physicist <ab|cd> = (ac|bd).  Replaces the 8-fold block placement and
