@@ -161,16 +161,26 @@ def _item_table(p):
     return table
 
 
-def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids):
+def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=False):
     """Loop over bra x ket primitive pairs; straight-line recursion in ``order``;
-    every final value is added to its FP64 accumulator as soon as it exists."""
+    every final value is added to its FP64 accumulator as soon as it exists.
+
+    ``strided``: the lane handles primitive quartets r, r + R, r + 2R, ... of the
+    flattened bra x ket range (R lanes share one shell quartet)."""
     L = p.L
     table = _item_table(p)
     for f in acc_ids:
         w(f"  double acc{f} = 0.0;")
-    w("  for (int ib = 0; ib < pb.w; ++ib) {")
-    w("    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
-    w("    for (int ik = 0; ik < pk.w; ++ik) {")
+    if strided:
+        w("  const int nprimq = pb.w * pk.w;")
+        w("  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {")
+        w("    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;")
+        w("    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
+        w("    {")
+    else:
+        w("  for (int ib = 0; ib < pb.w; ++ib) {")
+        w("    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
+        w("    for (int ik = 0; ik < pk.w; ++ik) {")
     w("      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;")
     w("      PrimQuartet<TB> s;")
     w("      const double T = prim_quartet_setup(rb, rk, s);")
@@ -226,11 +236,18 @@ def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids):
     w("  }")
 
 
-def _head(w, name):
+def _head(w, name, strided=False):
     w("template <typename TB>")
     w(f"__global__ __launch_bounds__(kRegThreads) void {name}(gbx_tables t, RegArgs a) {{")
-    w("  const int64_t q = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
-    w("  if (q >= a.nquart) return;")
+    if strided:
+        # lane -> (r, q): consecutive lanes take consecutive quartets of one stride slot r
+        w("  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
+        w("  if (gid >= a.nquart * a.lanes_per_quartet) return;")
+        w("  const int r = static_cast<int>(gid / a.nquart);")
+        w("  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;")
+    else:
+        w("  const int64_t q = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
+        w("  if (q >= a.nquart) return;")
     w("  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];")
     w("  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];")
     w("  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];")
@@ -287,8 +304,10 @@ def kernel_source(cls):
 
 
 def split_kernel_source(cls, k):
-    """Split kernel k: primitive loop for one subset of the accumulators, written to
-    a.acc[f * a.acc_stride + q] for the horizontal-recursion pass."""
+    """Split kernel k: primitive loop for one subset of the accumulators.  R lanes
+    (a.lanes_per_quartet) share a shell quartet, each summing a strided subset of
+    its primitive quartets; lane r writes its partial sums to
+    a.acc[(f * R + r) * a.acc_stride + q] for the horizontal-recursion pass."""
     la, lb, lc, ld = cls
     p = class_program(*cls)
     _, parts = class_plan(cls)
@@ -302,10 +321,11 @@ def split_kernel_source(cls, k):
     w = o.append
     w(f"// class ({la}{lb}|{lc}{ld}) split {k}/{len(parts)}: {len(order)} recursion values, "
       f"{len(acc_ids)} accumulators")
-    _head(w, name)
-    _emit_primitive_loop(w, p, order, acc_of_value, acc_ids)
+    _head(w, name, strided=True)
+    _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=True)
+    w("  const int64_t R = a.lanes_per_quartet;")
     for f in acc_ids:
-        w(f"  a.acc[static_cast<int64_t>({f}) * a.acc_stride + q] = acc{f};")
+        w(f"  a.acc[({f} * R + r) * a.acc_stride + q] = acc{f};")
     w("}")
     return "\n".join(o)
 
@@ -355,8 +375,8 @@ def generate(path=OUT):
     parts.append("")
     parts.append("int launch_split_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
                  "int k, const RegArgs& a, int precision, hipStream_t s) {")
-    parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
-                 "kRegThreads);")
+    parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart * a.lanes_per_quartet + "
+                 "kRegThreads - 1) / kRegThreads);")
     parts.append("  switch ((((la * 8 + lb) * 8 + lc) * 8 + ld) * 64 + k) {")
     for cls, n in split:
         for k in range(n):

@@ -47,8 +47,10 @@ struct CoopArgs {
   char* scratch;
   int64_t stride;
   int64_t sc_off, acc_off, un_off, h_off, f0_off, f1_off;
-  const double* acc_in;  // non-null: contracted [e0|f0] given, acc_in[f * acc_stride + qi]
+  const double* acc_in;  // non-null: partial contracted [e0|f0] given,
+                         //   acc_in[(f * acc_parts + r) * acc_stride + qi], summed over r
   int64_t acc_stride;
+  int32_t acc_parts;
 };
 
 struct CoopLayout {
@@ -103,7 +105,12 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
 
     __syncthreads();  // previous quartet's epilogue is done with the workspace
     if (a.acc_in) {
-      for (int f = tid; f < a.nacc; f += kCoopThreads) acc[f] = a.acc_in[f * a.acc_stride + qi];
+      for (int f = tid; f < a.nacc; f += kCoopThreads) {
+        double sum = 0.0;
+        for (int r = 0; r < a.acc_parts; ++r)
+          sum += a.acc_in[(static_cast<int64_t>(f) * a.acc_parts + r) * a.acc_stride + qi];
+        acc[f] = sum;
+      }
     } else {
       for (int f = tid; f < a.nacc; f += kCoopThreads) acc[f] = 0.0;
     }
@@ -317,7 +324,7 @@ template <typename TB, bool kLDS>
 int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, int lc, int ld,
                 const int32_t* d_quartets, int64_t nquart, int pf, int mode, double* d_out,
                 void* d_scratch, int64_t scratch_bytes, hipStream_t stream,
-                const double* acc_in = nullptr, int64_t acc_stride = 0) {
+                const double* acc_in = nullptr, int64_t acc_stride = 0, int acc_parts = 1) {
   const CoopLayout l = coop_layout(prog->header, sizeof(TB), pf);
   CoopArgs a{};
   a.quart = d_quartets;
@@ -347,6 +354,7 @@ int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, in
   a.f1_off = l.f1_off;
   a.acc_in = acc_in;
   a.acc_stride = acc_stride;
+  a.acc_parts = acc_parts;
   int64_t grid = nquart;
   size_t shmem = 0;
   if (kLDS) {
@@ -415,8 +423,9 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
 
 extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t la, int32_t lb,
                            int32_t lc, int32_t ld, const int32_t* d_quartets, int64_t nquart,
-                           int32_t mode, const double* d_acc, int64_t acc_stride, double* d_out,
-                           void* d_scratch, int64_t scratch_bytes, void* stream) {
+                           int32_t mode, const double* d_acc, int64_t acc_stride,
+                           int32_t acc_parts, double* d_out, void* d_scratch,
+                           int64_t scratch_bytes, void* stream) {
   if (!t || !prog || !d_out || !d_acc || (nquart > 0 && !d_quartets))
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: null argument");
   if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL ||
@@ -424,13 +433,14 @@ extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: program does not match the class");
   if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: bad mode");
-  if (acc_stride < nquart) return fail(GBX_ERR_ARG, "gbx_eri_hrr: acc_stride < nquart");
+  if (acc_stride < nquart || acc_parts < 1)
+    return fail(GBX_ERR_ARG, "gbx_eri_hrr: acc_stride < nquart or acc_parts < 1");
   if (nquart == 0) return GBX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const CoopLayout l = coop_layout(prog->header, 8, 1);
   if (l.total <= kCoopLdsLimit)
     return launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, 1, mode, d_out,
-                                     d_scratch, scratch_bytes, s, d_acc, acc_stride);
+                                     d_scratch, scratch_bytes, s, d_acc, acc_stride, acc_parts);
   return launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, 1, mode, d_out,
-                                    d_scratch, scratch_bytes, s, d_acc, acc_stride);
+                                    d_scratch, scratch_bytes, s, d_acc, acc_stride, acc_parts);
 }

@@ -18,8 +18,10 @@ struct RegArgs {
   int64_t nquart;
   int32_t mode;          // GBX_MODE_PACKED or GBX_MODE_SCHWARZ
   double* out;
-  double* acc;           // split kernels: contracted [e0|f0] out, acc[f * acc_stride + q]
+  double* acc;           // split kernels: partial contracted [e0|f0],
+                         //   acc[(f * lanes_per_quartet + r) * acc_stride + q]
   int64_t acc_stride;
+  int32_t lanes_per_quartet;  // split kernels: lanes sharing one shell quartet (R)
 };
 
 // F_m(T), m = 0..L, into F[] (same algorithm as boys_eval, unrolled for fixed L).

@@ -21,7 +21,7 @@ extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t 
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
-  gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0};
+  gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0, 1};
   return gbx::launch_reg_generated(t, la, lb, lc, ld, a, precision,
                                    static_cast<hipStream_t>(stream));
 }
@@ -33,7 +33,8 @@ extern "C" int gbx_split_count(int32_t la, int32_t lb, int32_t lc, int32_t ld) {
 
 extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                              int32_t split, const int32_t* d_quartets, int64_t nquart,
-                             int32_t precision, double* d_acc, int64_t acc_stride, void* stream) {
+                             int32_t precision, int32_t lanes_per_quartet, double* d_acc,
+                             int64_t acc_stride, void* stream) {
   if (!t || !d_acc || (nquart > 0 && !d_quartets))
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: null argument");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
@@ -42,10 +43,13 @@ extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_
   if (split < 0 || split >= n)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_split: no such split for this class");
   if (acc_stride < nquart) return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: acc_stride < nquart");
+  if (lanes_per_quartet < 1 || lanes_per_quartet > 4096)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: lanes_per_quartet must be in [1, 4096]");
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
-  gbx::RegArgs a{d_quartets, nquart, GBX_MODE_PACKED, nullptr, d_acc, acc_stride};
+  gbx::RegArgs a{d_quartets, nquart, GBX_MODE_PACKED, nullptr, d_acc, acc_stride,
+                 lanes_per_quartet};
   return gbx::launch_split_generated(t, la, lb, lc, ld, split, a, precision,
                                      static_cast<hipStream_t>(stream));
 }

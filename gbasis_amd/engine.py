@@ -172,6 +172,16 @@ class EriEngine:
         self._pf[(cls, nprimq, precision)] = pf
         return pf
 
+    TARGET_LANES = 1 << 18   # lanes in flight that keep all 1024 SIMDs of the GPU busy
+
+    def _lanes_per_quartet(self, nq, nprimq):
+        """R lanes per shell quartet for the split kernels: enough lanes to fill the GPU,
+        never more than the (upper bound of the) primitive quartets per shell quartet."""
+        R = 1
+        while R * 2 <= nprimq and nq * R * 2 <= self.TARGET_LANES:
+            R *= 2
+        return R
+
     def split_count(self, cls):
         if cls not in self._split:
             self._split[cls] = int(self.lib.gbx_split_count(*cls))
@@ -199,14 +209,15 @@ class EriEngine:
         if path != "coop" and nsplit > 0:
             nq = quartets.shape[0]
             nacc = st.header[1]
-            acc = torch.empty(nacc * nq, dtype=torch.float64, device=self.device)
+            R = self._lanes_per_quartet(nq, nprimq)
+            acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
             for k in range(nsplit):
                 check(self.lib.gbx_eri_split(ctypes.byref(dev.struct), *cls, k, ptr(quartets), nq,
-                                             precision, ptr(acc), nq, self._stream()),
+                                             precision, R, ptr(acc), nq, self._stream()),
                       f"gbx_eri_split{cls}[{k}]")
             sc = self.scratch()
             check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
-                                       ptr(quartets), nq, mode, ptr(acc), nq, ptr(out), ptr(sc),
+                                       ptr(quartets), nq, mode, ptr(acc), nq, R, ptr(out), ptr(sc),
                                        sc.numel(), self._stream()), f"gbx_eri_hrr{cls}")
             return "split"
         pf = self._plan(cls, nprimq, precision)

@@ -143,21 +143,23 @@ int gbx_split_count(int32_t la, int32_t lb, int32_t lc, int32_t ld);
 /* Split kernel `split` of a large class: the primitive loop (Boys, vertical
  * recursion, ket build, FP64 accumulation -- the part of
  * compute_two_electron_integrals_os_hgp, gbasis/integrals/_two_elec_int_improved.py:651-716,
- * before the horizontal recursion) for one subset of the contracted [e0|f0],
- * written to d_acc[f * acc_stride + q] (f: accumulator index of the class
- * program, q: quartet index). */
+ * before the horizontal recursion) for one subset of the contracted [e0|f0].
+ * lanes_per_quartet (R) lanes share each quartet, lane r summing primitive
+ * quartets r, r+R, ...; partial sums go to d_acc[(f * R + r) * acc_stride + q]
+ * (f: accumulator index of the class program, q: quartet index). */
 int gbx_eri_split(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                   int32_t split, const int32_t *d_quartets, int64_t nquart, int32_t precision,
-                  double *d_acc, int64_t acc_stride, void *stream);
+                  int32_t lanes_per_quartet, double *d_acc, int64_t acc_stride, void *stream);
 
 /* Horizontal recursion, norms, output transform and placement (or the Schwarz
  * bounds in GBX_MODE_SCHWARZ) from contracted [e0|f0] produced by the split
- * kernels: the rest of gbx_eri_coop (gbasis/integrals/_two_elec_int_improved.py:364-588,
+ * kernels (acc_parts partial sums per accumulator, summed here): the rest of
+ * gbx_eri_coop (gbasis/integrals/_two_elec_int_improved.py:364-588,
  * gbasis/base_four_symm.py:160-389). */
 int gbx_eri_hrr(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb, int32_t lc,
                 int32_t ld, const int32_t *d_quartets, int64_t nquart, int32_t mode,
-                const double *d_acc, int64_t acc_stride, double *d_out, void *d_scratch,
-                int64_t scratch_bytes, void *stream);
+                const double *d_acc, int64_t acc_stride, int32_t acc_parts, double *d_out,
+                void *d_scratch, int64_t scratch_bytes, void *stream);
 
 /* Expand the canonical 8-fold packed array of one molecule (nfunc output
  * functions) to the dense (nfunc)^4 tensor; notation 0 = chemist (ab|cd),
