@@ -41,7 +41,7 @@ def build_batch(n_mol):
     return BatchTables(mols)
 
 
-def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup):
+def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
     """warmup + steps of: device queue build, then every ERI launch.
 
     Returns (seconds per step, queue seconds per step, wall seconds per step,
@@ -52,6 +52,8 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup):
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
         plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False)
+        if tuner is not None:
+            tuner.tune(dev, plans, dev.out)
         eng.run(dev, plans, dev.out)
     torch.cuda.synchronize()
     per_bucket = None
@@ -65,11 +67,13 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup):
         q0 = torch.cuda.Event(enable_timing=True)
         q0.record(stream)
         plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False)
+        if tuner is not None:
+            tuner.apply(plans)
         marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(plans) + 1)]
         marks[0].record(stream)
         for i, bk in enumerate(plans):
-            bk.path = eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.precision, 0,
-                                       dev.out, bk.dense, eng.path)
+            bk.path = eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.launch_precision(), 0,
+                                       dev.out, bk.dense, eng.path, bk.lanes())
             marks[i + 1].record(stream)
         recs.append((q0, marks))
     stop.record(stream)
@@ -88,7 +92,7 @@ def class_table(plans, per_bucket, top=16):
     """Per (kernel path, class, precision): time, quartets, ERIs, algorithmic TFLOP/s."""
     groups = {}
     for bk, t in zip(plans, per_bucket):
-        key = f"{bk.path}:{''.join(map(str, bk.cls))}:{'f32' if bk.precision else 'f64'}"
+        key = f"{bk.path}:{''.join(map(str, bk.cls))}:{'f32' if bk.launch_precision() else 'f64'}"
         g = groups.setdefault(key, [0.0, 0, 0, 0.0])
         g[0] += t
         g[1] += bk.n
@@ -105,7 +109,7 @@ def roofline_entry(plans, per_bucket):
 
     groups = {}
     for bk, t in zip(plans, per_bucket):
-        key = (bk.path, bk.cls, bk.precision)
+        key = (bk.path, bk.cls, bk.launch_precision())
         g = groups.setdefault(key, [0.0, 0.0, 0])
         g[0] += t
         g[1] += bk.flops
@@ -195,6 +199,7 @@ def main():
     ap.add_argument("--fp32-cutoff", type=float, default=1e-2)
     ap.add_argument("--path", choices=["auto", "coop"], default="auto")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the ERI path on one GPU only; --gpus must be 1",
@@ -222,20 +227,23 @@ def main():
     log(f"plans: {nquart} quartets, {eris} ERIs, {len(plans64)} buckets; "
         f"{n_fp32} quartets FP32-bulk in mixed ({time.perf_counter() - t0:.1f} s)")
 
+    from gbasis_amd.autotune import Autotuner
+
+    tuner = None if args.no_tune else Autotuner(eng)
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     t64, q64, w64, pb64, last64 = time_steps(eng, dev, bound, args.threshold, 0.0, args.steps,
-                                             args.warmup)
+                                             args.warmup, tuner)
     out64 = dev.out
     log(f"fp64: {t64 * 1e3:.1f} ms/step (queue {q64 * 1e3:.1f} ms), {eris / t64:.3e} ERIs/s")
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     tmx, qmx, wmx, pbmx, lastmx = time_steps(eng, dev, bound, args.threshold, args.fp32_cutoff,
-                                             args.steps, args.warmup)
+                                             args.steps, args.warmup, tuner)
     outmx = dev.out
     log(f"mixed: {tmx * 1e3:.1f} ms/step (queue {qmx * 1e3:.1f} ms), {eris / tmx:.3e} ERIs/s")
     for bk, last in zip(plans64, last64):
-        bk.path = last.path
+        bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
     for bk, last in zip(plansmx, lastmx):
-        bk.path = last.path
+        bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
     diff = outmx.sub_(out64).abs_()
     max_err = float(diff.max())
     rms_err = float(torch.linalg.vector_norm(diff)) / np.sqrt(diff.numel())
@@ -252,7 +260,10 @@ def main():
                                "Cartesian, K 1-6, exps [0.05,1e3], packed 8-fold output",
                    "molecules": args.molecules, "shell_quartets": nquart, "eris": eris,
                    "schwarz_threshold": args.threshold, "fp32_cutoff": args.fp32_cutoff,
-                   "fp32_quartets": n_fp32, "kernel_path": args.path},
+                   "fp32_quartets": n_fp32, "kernel_path": args.path,
+                   "autotuned": not args.no_tune,
+                   "fp32_quartets_run_fp32": sum(bk.n for bk in plansmx
+                                                 if bk.launch_precision() == 1)},
         "fp64": {"value": eris / t64, "ms_per_step": t64 * 1e3},
         "queue_build_ms": {"fp64": q64 * 1e3, "mixed": qmx * 1e3},
         "mixed_vs_fp64": {"max_abs_err": max_err, "rms_err": rms_err},

@@ -1,0 +1,89 @@
+"""Measured choice of kernel variants per work bucket.
+
+For every bucket (class, primitive-count bin, precision class, dense flag) of a
+work queue the tuner times the admissible variants on the bucket's own
+quartets and keeps the fastest:
+
+* mixed precision: a bucket whose quartets may run with the FP32 recursion
+  bulk (bound below the cutoff) is timed in FP32 and in FP64.  Both meet the
+  error bound, so the faster one is kept.  (On CDNA4 non-packed FP32 FMA runs
+  at the FP64 vector rate, so FP32 wins only where it relieves registers or LDS.)
+* split classes: lanes per shell quartet R in {1, 2, 4, ..., 64};
+* cooperative classes: primitive quartets in flight PF.
+
+Choices are cached in memory per (class, bin, dense, precision class) and can be
+saved to / loaded from a JSON file.
+"""
+
+import json
+import os
+
+import torch
+
+from ._lib import GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED
+
+
+class Autotuner:
+    def __init__(self, engine, repeats=2):
+        self.eng = engine
+        self.repeats = repeats
+        self.choice = {}
+
+    @staticmethod
+    def key(bk):
+        return (tuple(bk.cls), int(bk.nprimq), bool(bk.dense), int(bk.precision))
+
+    def _time(self, fn):
+        s = torch.cuda.current_stream()
+        fn()
+        a = torch.cuda.Event(enable_timing=True)
+        b = torch.cuda.Event(enable_timing=True)
+        a.record(s)
+        for _ in range(self.repeats):
+            fn()
+        b.record(s)
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / self.repeats
+
+    def tune(self, dev, plans, out):
+        """Pick variants for every bucket of ``plans`` (timed on ``out``)."""
+        eng = self.eng
+        for bk in plans:
+            k = self.key(bk)
+            if k in self.choice:
+                continue
+            precisions = [GBX_FP64] if bk.precision == GBX_FP64 else [GBX_FP32_BULK, GBX_FP64]
+            best = None
+            for prec in precisions:
+                variants = [None]
+                if not bk.dense and eng.has_reg(bk.cls):
+                    variants = [None]
+                elif eng.split_count(bk.cls) > 0 and eng.path != "coop":
+                    variants = [("R", r) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq]
+                for var in variants:
+                    def run(prec=prec, var=var):
+                        eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, GBX_MODE_PACKED,
+                                         out, bk.dense, eng.path,
+                                         lanes=var[1] if var else None)
+                    t = self._time(run)
+                    if best is None or t < best[0]:
+                        best = (t, prec, var)
+            self.choice[k] = (best[1], best[2])
+        self.apply(plans)
+
+    def apply(self, plans):
+        for bk in plans:
+            c = self.choice.get(self.key(bk))
+            if c is not None:
+                bk.run_precision, bk.variant = c
+
+    def save(self, path):
+        with open(path, "w") as fh:
+            json.dump([[list(k), list(v[:1]) + [list(v[1]) if v[1] else None]]
+                       for k, v in self.choice.items()], fh)
+
+    def load(self, path):
+        if not os.path.exists(path):
+            return
+        for k, v in json.load(open(path)):
+            self.choice[(tuple(k[0]), k[1], k[2], k[3])] = (v[0], tuple(v[1]) if v[1] else None)

@@ -39,7 +39,15 @@ class Bucket:
     n: int               # shell quartets
     eris: int            # canonical packed slots written
     flops: float         # algorithmic FLOPs (roofline.quartet_flops)
-    path: str = ""       # "reg" or "coop" once launched
+    path: str = ""       # "reg", "split" or "coop" once launched
+    run_precision: int = -1   # precision actually launched (autotuner); -1: = precision
+    variant: tuple = None     # autotuned variant, e.g. ("R", lanes per quartet)
+
+    def launch_precision(self):
+        return self.precision if self.run_precision < 0 else self.run_precision
+
+    def lanes(self):
+        return self.variant[1] if self.variant and self.variant[0] == "R" else None
 
 
 def _require_gpu():
@@ -193,7 +201,7 @@ class EriEngine:
         return self._reg[cls]
 
     def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,
-                     dense=False, path="auto"):
+                     dense=False, path="auto", lanes=None):
         """One launch for a (class, bin) bucket; quartets is a device int32 (n,2).
 
         Classes with a generated register kernel and no dense output transform run
@@ -209,7 +217,7 @@ class EriEngine:
         if path != "coop" and nsplit > 0:
             nq = quartets.shape[0]
             nacc = st.header[1]
-            R = self._lanes_per_quartet(nq, nprimq)
+            R = lanes if lanes else self._lanes_per_quartet(nq, nprimq)
             acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
             for k in range(nsplit):
                 check(self.lib.gbx_eri_split(ctypes.byref(dev.struct), *cls, k, ptr(quartets), nq,
@@ -371,8 +379,8 @@ class EriEngine:
         if out is None:
             out = torch.zeros(dev.batch.n_slots, dtype=torch.float64, device=self.device)
         for bk in plans:
-            bk.path = self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.precision,
-                                        GBX_MODE_PACKED, out, bk.dense, self.path)
+            bk.path = self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.launch_precision(),
+                                        GBX_MODE_PACKED, out, bk.dense, self.path, bk.lanes())
         return out
 
     def unpack_dense(self, packed, nfunc, notation):
