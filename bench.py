@@ -200,6 +200,7 @@ def main():
     ap.add_argument("--path", choices=["auto", "coop"], default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
+    ap.add_argument("--tune-cache", default="", help="load (if present) / save autotuner choices")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the ERI path on one GPU only; --gpus must be 1",
@@ -230,6 +231,8 @@ def main():
     from gbasis_amd.autotune import Autotuner
 
     tuner = None if args.no_tune else Autotuner(eng)
+    if tuner is not None and args.tune_cache and tuner.load(args.tune_cache):
+        log(f"autotuner choices loaded from {args.tune_cache}")
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     t64, q64, w64, pb64, last64 = time_steps(eng, dev, bound, args.threshold, 0.0, args.steps,
                                              args.warmup, tuner)
@@ -240,10 +243,17 @@ def main():
                                              args.steps, args.warmup, tuner)
     outmx = dev.out
     log(f"mixed: {tmx * 1e3:.1f} ms/step (queue {qmx * 1e3:.1f} ms), {eris / tmx:.3e} ERIs/s")
+    if tuner is not None and args.tune_cache:
+        tuner.save(args.tune_cache)
+    # bookkeeping of the buckets actually launched (after the autotuner's FP32 mask)
+    plansmx = eng.plan_device(dev, bound, args.threshold, args.fp32_cutoff, stats=True)
+    if tuner is not None:
+        tuner.apply(plansmx)
     for bk, last in zip(plans64, last64):
         bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
     for bk, last in zip(plansmx, lastmx):
         bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
+    assert [bk.cls for bk in plansmx] == [bk.cls for bk in lastmx]
     diff = outmx.sub_(out64).abs_()
     max_err = float(diff.max())
     rms_err = float(torch.linalg.vector_norm(diff)) / np.sqrt(diff.numel())

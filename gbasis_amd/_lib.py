@@ -58,10 +58,10 @@ SIGNATURES = {
                                    c_i32, c_i32, c_i32, c_vp, c_i64, c_i32, c_vp, c_i64, c_i32, c_vp,
                                    c_vp, c_i64, c_vp]),
     "gbx_quartets_count": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
-                                          ctypes.c_double, ctypes.c_double, c_i64, c_vp, c_vp,
-                                          c_vp, c_vp]),
+                                          ctypes.c_double, ctypes.c_double, c_vp, c_i64, c_vp,
+                                          c_vp, c_vp, c_vp]),
     "gbx_quartets_fill": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double,
-                                         ctypes.c_double, c_i64, c_vp, c_vp, c_vp, c_vp]),
+                                         ctypes.c_double, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "gbx_eri_reg": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32, c_i32, c_vp,
                                    c_i64, c_i32, c_i32, c_vp, c_vp]),
 }

@@ -28,6 +28,7 @@ class Autotuner:
         self.eng = engine
         self.repeats = repeats
         self.choice = {}
+        self.fp32_win = {}
 
     @staticmethod
     def key(bk):
@@ -46,8 +47,14 @@ class Autotuner:
         return a.elapsed_time(b) / self.repeats
 
     def tune(self, dev, plans, out):
-        """Pick variants for every bucket of ``plans`` (timed on ``out``)."""
+        """Pick variants for every bucket of ``plans`` (timed on ``out``).
+
+        Classes whose FP32-eligible buckets run faster in FP64 lose the FP32
+        permission in the engine's queue mask, so their quartets join the FP64
+        launch from the next queue build on.
+        """
         eng = self.eng
+        fp32_win = dict(self.fp32_win)
         for bk in plans:
             k = self.key(bk)
             if k in self.choice:
@@ -69,6 +76,12 @@ class Autotuner:
                     if best is None or t < best[0]:
                         best = (t, prec, var)
             self.choice[k] = (best[1], best[2])
+            if bk.precision == GBX_FP32_BULK:
+                ck = (tuple(bk.cls), bool(bk.dense))
+                fp32_win[ck] = fp32_win.get(ck, True) and best[1] == GBX_FP32_BULK
+        if fp32_win != self.fp32_win:
+            self.fp32_win = fp32_win
+            eng.set_fp32_allowed(fp32_win)
         self.apply(plans)
 
     def apply(self, plans):
@@ -79,11 +92,17 @@ class Autotuner:
 
     def save(self, path):
         with open(path, "w") as fh:
-            json.dump([[list(k), list(v[:1]) + [list(v[1]) if v[1] else None]]
-                       for k, v in self.choice.items()], fh)
+            json.dump({"choice": [[list(k), v[0], list(v[1]) if v[1] else None]
+                                  for k, v in self.choice.items()],
+                       "fp32": [[list(k[0]), k[1], v] for k, v in self.fp32_win.items()]}, fh)
 
     def load(self, path):
+        """Load saved choices; returns True if a cache was found."""
         if not os.path.exists(path):
-            return
-        for k, v in json.load(open(path)):
-            self.choice[(tuple(k[0]), k[1], k[2], k[3])] = (v[0], tuple(v[1]) if v[1] else None)
+            return False
+        d = json.load(open(path))
+        for k, prec, var in d["choice"]:
+            self.choice[(tuple(k[0]), k[1], k[2], k[3])] = (prec, tuple(var) if var else None)
+        self.fp32_win = {(tuple(c), dense): v for c, dense, v in d["fp32"]}
+        self.eng.set_fp32_allowed(self.fp32_win)
+        return True

@@ -35,6 +35,7 @@ struct QueueArgs {
   const double* pair_bound;     // per pair: Schwarz bound Q of its original shell pair
   const double* pair_pbound;    // per pair: bound of the contracted [e0|f0] (precision choice)
   double threshold, fp32_cutoff;
+  const int8_t* fp32_ok;        // per (class * 2 + dense): FP32 bulk allowed; null = all
   int32_t* bucket_count;        // [kNumBuckets]
   unsigned long long* bucket_eris;    // [kNumBuckets] or null
   unsigned long long* bucket_primq;   // [kNumBuckets] or null
@@ -89,9 +90,10 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       const int kb = swap ? kq : kp, kk = swap ? kp : kq;
       const int dense = ((kb >> 6) & 1) | ((kk >> 6) & 1);
       const double pbound = a.pair_pbound[gp] * a.pair_pbound[gq];
-      const int fp32 = (a.fp32_cutoff > 0.0 && pbound < a.fp32_cutoff) ? 1 : 0;
       const int np = (kb >> 8) * (kk >> 8);
       const int cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7);
+      const int fp32 = (a.fp32_cutoff > 0.0 && pbound < a.fp32_cutoff &&
+                        (a.fp32_ok == nullptr || a.fp32_ok[cls * 2 + dense])) ? 1 : 0;
       key = ((cls * 2 + dense) * 2 + fp32) * 16 + ceil_log2(np);
       if (!kFill && a.bucket_eris) {
         const uint64_t nb = static_cast<uint64_t>(a.pair_npf[bra]);
@@ -149,7 +151,8 @@ int launch_queue(const QueueArgs& a, bool fill, hipStream_t s) {
 extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
                                   const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
                                   const int32_t* d_pair_npf, const double* d_pair_bound,
-                                  const double* d_pair_pbound, double threshold, double fp32_cutoff, int64_t n_cand,
+                                  const double* d_pair_pbound, double threshold,
+                                  double fp32_cutoff, const int8_t* d_fp32_ok, int64_t n_cand,
                                   int32_t* d_bucket_count, uint64_t* d_bucket_eris,
                                   uint64_t* d_bucket_primq, void* stream) {
   if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
@@ -166,6 +169,7 @@ extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
   a.pair_pbound = d_pair_pbound;
   a.threshold = threshold;
   a.fp32_cutoff = fp32_cutoff;
+  a.fp32_ok = d_fp32_ok;
   a.bucket_count = d_bucket_count;
   a.bucket_eris = reinterpret_cast<unsigned long long*>(d_bucket_eris);
   a.bucket_primq = reinterpret_cast<unsigned long long*>(d_bucket_primq);
@@ -176,7 +180,7 @@ extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
 extern "C" int gbx_quartets_fill(int32_t n_mol, const int64_t* d_mol_cand_off,
                                  const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
                                  const double* d_pair_bound, const double* d_pair_pbound,
-                                 double threshold, double fp32_cutoff,
+                                 double threshold, double fp32_cutoff, const int8_t* d_fp32_ok,
                                  int64_t n_cand, const int64_t* d_bucket_offset,
                                  int32_t* d_bucket_cursor, int32_t* d_quartets, void* stream) {
   if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
@@ -191,6 +195,7 @@ extern "C" int gbx_quartets_fill(int32_t n_mol, const int64_t* d_mol_cand_off,
   a.pair_pbound = d_pair_pbound;
   a.threshold = threshold;
   a.fp32_cutoff = fp32_cutoff;
+  a.fp32_ok = d_fp32_ok;
   a.bucket_offset = d_bucket_offset;
   a.bucket_cursor = d_bucket_cursor;
   a.out = d_quartets;

@@ -137,6 +137,7 @@ class EriEngine:
         self._reg = {}
         self._split = {}
         self._pf = {}
+        self.fp32_ok = None   # int8 device mask per class*2+dense (autotuner); None: all allowed
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
 
     def scratch(self):
@@ -327,10 +328,12 @@ class EriEngine:
         eris = torch.zeros(nb, dtype=torch.int64, device=self.device) if stats else None
         primq = torch.zeros(nb, dtype=torch.int64, device=self.device) if stats else None
         st = self._stream()
+        mask = self.fp32_ok
         check(self.lib.gbx_quartets_count(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
                                           ptr(dev.pair_key), ptr(dev.pair_npf), ptr(qb),
-                                          ptr(pbd), float(threshold), float(fp32_cutoff), b.n_cand,
-                                          ptr(counts), ptr(eris), ptr(primq), st),
+                                          ptr(pbd), float(threshold), float(fp32_cutoff),
+                                          ptr(mask), b.n_cand, ptr(counts), ptr(eris),
+                                          ptr(primq), st),
               "gbx_quartets_count")
         ch = counts.cpu().numpy().astype(np.int64)
         offs = np.concatenate([[0], np.cumsum(ch)[:-1]]).astype(np.int64)
@@ -340,26 +343,42 @@ class EriEngine:
         cursor = torch.zeros(nb, dtype=torch.int32, device=self.device)
         check(self.lib.gbx_quartets_fill(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
                                          ptr(dev.pair_key), ptr(qb), ptr(pbd), float(threshold),
-                                         float(fp32_cutoff), b.n_cand, ptr(d_offs), ptr(cursor),
-                                         ptr(quart), st), "gbx_quartets_fill")
+                                         float(fp32_cutoff), ptr(mask), b.n_cand, ptr(d_offs),
+                                         ptr(cursor), ptr(quart), st), "gbx_quartets_fill")
         eh = eris.cpu().numpy() if stats else None
         ph = primq.cpu().numpy() if stats else None
         plans = []
-        for key in np.flatnonzero(ch):
-            key = int(key)
-            binid = key & 15
+        keys = np.flatnonzero(ch)
+        # one launch per (class, dense, precision) for fused register classes: their bins
+        # are adjacent in the buffer (the bin is the lowest field of the bucket key)
+        groups = []
+        for key in keys.tolist():
+            head = key >> 4
+            c = key >> 6
+            cls = ((c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7)
+            dense = bool((key >> 5) & 1)
+            fused = (not dense) and self.path != "coop" and self.has_reg(cls)
+            if fused and groups and groups[-1][0] == head:
+                groups[-1][1].append(key)
+            else:
+                groups.append((head if fused else -1 - key, [key]))
+        for _, ks in groups:
+            key = ks[-1]
             fp32 = (key >> 4) & 1
             dense = bool((key >> 5) & 1)
             c = key >> 6
             cls = ((c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7)
-            n = int(ch[key])
-            view = quart[offs[key]:offs[key] + n]
+            n = int(sum(ch[k] for k in ks))
+            start = int(offs[ks[0]])
+            view = quart[start:start + n]
             flops = 0.0
+            eris_n = 0
             if stats:
                 fpr, fco = class_flops(*cls, pure=dense)
-                flops = float(ph[key]) * fpr + n * fco
-            plans.append(Bucket(cls, 1 << binid, GBX_FP32_BULK if fp32 else GBX_FP64, dense,
-                                view, n, int(eh[key]) if stats else 0, flops))
+                flops = float(sum(float(ph[k]) for k in ks)) * fpr + n * fco
+                eris_n = int(sum(int(eh[k]) for k in ks))
+            plans.append(Bucket(cls, 1 << (key & 15), GBX_FP32_BULK if fp32 else GBX_FP64, dense,
+                                view, n, eris_n, flops))
         self._last_queue = quart
         return plans
 
@@ -382,6 +401,14 @@ class EriEngine:
             bk.path = self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.launch_precision(),
                                         GBX_MODE_PACKED, out, bk.dense, self.path, bk.lanes())
         return out
+
+    def set_fp32_allowed(self, allowed):
+        """allowed: {(cls, dense): bool}; classes not listed keep FP32 allowed."""
+        mask = np.ones(8 ** 4 * 2, dtype=np.int8)
+        for (cls, dense), ok in allowed.items():
+            c = ((cls[0] * 8 + cls[1]) * 8 + cls[2]) * 8 + cls[3]
+            mask[c * 2 + int(dense)] = 1 if ok else 0
+        self.fp32_ok = torch.from_numpy(mask).to(self.device)
 
     def unpack_dense(self, packed, nfunc, notation):
         dense = torch.empty((nfunc,) * 4, dtype=torch.float64, device=self.device)

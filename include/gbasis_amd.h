@@ -118,7 +118,8 @@ int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t
  * of SchwarzScreener.is_significant (gbasis/integrals/_schwarz_screening.py:169-191),
  * kept iff bound_p * bound_q >= threshold (threshold <= 0 keeps all), run with
  * the FP32 recursion bulk iff pbound_p * pbound_q < fp32_cutoff (pbound: the
- * second value GBX_MODE_SCHWARZ writes per pair).  Buckets
+ * second value GBX_MODE_SCHWARZ writes per pair) and d_fp32_ok (NULL, or one
+ * int8 per class*2+dense, class = ((la*8+lb)*8+lc)*8+ld) allows it.  Buckets
  * are keyed (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + ceil(log2 nprimq),
  * GBX_NUM_BUCKETS of them.  count: quartets per bucket (and, if d_bucket_eris is
  * not NULL, packed slots and primitive quartets per bucket).  fill: writes the
@@ -127,13 +128,13 @@ int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t
 int gbx_quartets_count(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
                        const int32_t *d_pair_key, const int32_t *d_pair_npf,
                        const double *d_pair_bound, const double *d_pair_pbound,
-                       double threshold, double fp32_cutoff,
+                       double threshold, double fp32_cutoff, const int8_t *d_fp32_ok,
                        int64_t n_cand, int32_t *d_bucket_count, uint64_t *d_bucket_eris,
                        uint64_t *d_bucket_primq, void *stream);
 int gbx_quartets_fill(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
                       const int32_t *d_pair_key, const double *d_pair_bound,
                       const double *d_pair_pbound, double threshold, double fp32_cutoff,
-                      int64_t n_cand, const int64_t *d_bucket_offset,
+                      const int8_t *d_fp32_ok, int64_t n_cand, const int64_t *d_bucket_offset,
                       int32_t *d_bucket_cursor, int32_t *d_quartets, void *stream);
 
 /* Number of split register kernels for a class too large for one lane's
