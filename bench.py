@@ -72,8 +72,7 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
         marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(plans) + 1)]
         marks[0].record(stream)
         for i, bk in enumerate(plans):
-            bk.path = eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.launch_precision(), 0,
-                                       dev.out, bk.dense, eng.path, bk.lanes())
+            bk.path = eng.launch_bucket(dev, bk, dev.out)
             marks[i + 1].record(stream)
         recs.append((q0, marks))
     stop.record(stream)
@@ -104,7 +103,9 @@ def class_table(plans, per_bucket, top=16):
 
 
 def roofline_entry(plans, per_bucket):
-    """Dominant kernel (by total time): algorithmic FLOP/s against the FP64 vector peak."""
+    """Roofline of the dominant kernel: the fused register kernel (one launch per step)
+    with the largest time; algorithmic FLOPs per launch / its average launch time,
+    against the vector peak of its precision.  Also the aggregate of all ERI kernels."""
     from gbasis_amd.roofline import PEAK_FP32_VECTOR, PEAK_FP64_VECTOR
 
     groups = {}
@@ -114,7 +115,8 @@ def roofline_entry(plans, per_bucket):
         g[0] += t
         g[1] += bk.flops
         g[2] += 1
-    key, (t, f, nl) = max(groups.items(), key=lambda kv: kv[1][0])
+    fused = {k: v for k, v in groups.items() if k[0] == "reg"} or groups
+    key, (t, f, nl) = max(fused.items(), key=lambda kv: kv[1][0])
     path, cls, prec = key
     peak = PEAK_FP64_VECTOR if prec == 0 else PEAK_FP32_VECTOR
     ach = f / t
@@ -122,7 +124,8 @@ def roofline_entry(plans, per_bucket):
     total_t = float(per_bucket.sum())
     total_f = float(sum(bk.flops for bk in plans))
     return {"bound": "valu", "kernel": name, "achieved": ach / 1e12, "peak": peak / 1e12,
-            "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "launches": nl,
+            "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "launches_per_step": nl,
+            "flops_per_launch": f / nl, "avg_launch_ms": t / nl * 1e3,
             "kernel_time_share": t / total_t,
             "all_kernels": {"achieved": total_f / total_t / 1e12, "frac": total_f / total_t / peak}}
 

@@ -69,9 +69,8 @@ class Autotuner:
                     variants = [("R", r) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq]
                 for var in variants:
                     def run(prec=prec, var=var):
-                        eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, GBX_MODE_PACKED,
-                                         out, bk.dense, eng.path,
-                                         lanes=var[1] if var else None)
+                        eng.launch_bucket(dev, bk, out, precision=prec,
+                                          lanes=var[1] if var else None)
                     t = self._time(run)
                     if best is None or t < best[0]:
                         best = (t, prec, var)

@@ -246,8 +246,9 @@ def _head(w, name, strided=False):
         w("  const int r = static_cast<int>(gid / a.nquart);")
         w("  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;")
     else:
-        w("  const int64_t q = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
-        w("  if (q >= a.nquart) return;")
+        w("  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + "
+          "threadIdx.x);")
+        w("  if (q < 0) return;")
     w("  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];")
     w("  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];")
     w("  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];")

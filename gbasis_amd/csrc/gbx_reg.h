@@ -22,7 +22,26 @@ struct RegArgs {
                          //   acc[(f * lanes_per_quartet + r) * acc_stride + q]
   int64_t acc_stride;
   int32_t lanes_per_quartet;  // split kernels: lanes sharing one shell quartet (R)
+  const int64_t* seg_off;     // fused kernels, segmented launch: first quartet of each of
+  const int32_t* seg_cnt;     //   kSegments segments and its count (device); null: plain
 };
+
+constexpr int kSegments = 16;
+
+// Fused kernels: quartet index of this lane, or -1.  A segmented launch covers the
+// primitive-count bins of one class in one grid: lane gid takes quartet
+// seg_off[s] + (gid - sum_{s' < s} seg_cnt[s']).
+__device__ inline int64_t lane_quartet(const RegArgs& a, int64_t gid) {
+  if (a.seg_cnt == nullptr) return gid < a.nquart ? gid : -1;
+  int64_t base = 0;
+#pragma unroll
+  for (int s = 0; s < kSegments; ++s) {
+    const int64_t c = a.seg_cnt[s];
+    if (gid < base + c) return a.seg_off[s] + (gid - base);
+    base += c;
+  }
+  return -1;
+}
 
 // F_m(T), m = 0..L, into F[] (same algorithm as boys_eval, unrolled for fixed L).
 template <int L>

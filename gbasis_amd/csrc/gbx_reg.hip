@@ -21,7 +21,28 @@ extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t 
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
-  gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0, 1};
+  gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0, 1, nullptr, nullptr};
+  return gbx::launch_reg_generated(t, la, lb, lc, ld, a, precision,
+                                   static_cast<hipStream_t>(stream));
+}
+
+extern "C" int gbx_eri_reg_segments(const gbx_tables* t, int32_t la, int32_t lb, int32_t lc,
+                                    int32_t ld, const int32_t* d_quartets,
+                                    const int64_t* d_seg_off, const int32_t* d_seg_cnt,
+                                    int64_t max_quartets, int32_t precision, int32_t mode,
+                                    double* d_out, void* stream) {
+  if (!t || !d_out || !d_quartets || !d_seg_off || !d_seg_cnt || max_quartets < 0)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad argument");
+  if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad precision");
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad mode");
+  if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
+    return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg_segments: no register kernel for class");
+  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: Boys table too short for this class");
+  if (max_quartets == 0) return GBX_OK;
+  gbx::RegArgs a{d_quartets, max_quartets, mode, d_out, nullptr, 0, 1, d_seg_off, d_seg_cnt};
   return gbx::launch_reg_generated(t, la, lb, lc, ld, a, precision,
                                    static_cast<hipStream_t>(stream));
 }
@@ -49,7 +70,7 @@ extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
   gbx::RegArgs a{d_quartets, nquart, GBX_MODE_PACKED, nullptr, d_acc, acc_stride,
-                 lanes_per_quartet};
+                 lanes_per_quartet, nullptr, nullptr};
   return gbx::launch_split_generated(t, la, lb, lc, ld, split, a, precision,
                                      static_cast<hipStream_t>(stream));
 }

@@ -42,6 +42,7 @@ class Bucket:
     path: str = ""       # "reg", "split" or "coop" once launched
     run_precision: int = -1   # precision actually launched (autotuner); -1: = precision
     variant: tuple = None     # autotuned variant, e.g. ("R", lanes per quartet)
+    seg: tuple = None         # fused classes: (device segment offsets, device counts, max lanes)
 
     def launch_precision(self):
         return self.precision if self.run_precision < 0 else self.run_precision
@@ -138,6 +139,7 @@ class EriEngine:
         self._split = {}
         self._pf = {}
         self.fp32_ok = None   # int8 device mask per class*2+dense (autotuner); None: all allowed
+        self._queue_cache = {}
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
 
     def scratch(self):
@@ -201,17 +203,28 @@ class EriEngine:
             self._reg[cls] = bool(self.lib.gbx_has_reg_kernel(*cls))
         return self._reg[cls]
 
+    def launch_bucket(self, dev, bk, out, mode=GBX_MODE_PACKED, precision=None, lanes=None):
+        prec = bk.launch_precision() if precision is None else precision
+        return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
+                                 self.path, lanes if lanes is not None else bk.lanes(), bk.seg)
+
     def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,
-                     dense=False, path="auto", lanes=None):
+                     dense=False, path="auto", lanes=None, seg=None):
         """One launch for a (class, bin) bucket; quartets is a device int32 (n,2).
 
         Classes with a generated register kernel and no dense output transform run
         one quartet per lane; all others run the cooperative kernel.
         """
         if path != "coop" and not dense and self.has_reg(cls):
-            check(self.lib.gbx_eri_reg(ctypes.byref(dev.struct), *cls, ptr(quartets),
-                                       quartets.shape[0], precision, mode, ptr(out),
-                                       self._stream()), f"gbx_eri_reg{cls}")
+            if seg is not None:
+                check(self.lib.gbx_eri_reg_segments(ctypes.byref(dev.struct), *cls, ptr(quartets),
+                                                    ptr(seg[0]), ptr(seg[1]), seg[2], precision,
+                                                    mode, ptr(out), self._stream()),
+                      f"gbx_eri_reg_segments{cls}")
+            else:
+                check(self.lib.gbx_eri_reg(ctypes.byref(dev.struct), *cls, ptr(quartets),
+                                           quartets.shape[0], precision, mode, ptr(out),
+                                           self._stream()), f"gbx_eri_reg{cls}")
             return "reg"
         st, _, _ = self.programs.get(cls)
         nsplit = self.split_count(cls)
@@ -324,61 +337,75 @@ class EriEngine:
             bound = (ones, ones)
         qb, pbd = bound
         nb = 262144
-        counts = torch.zeros(nb, dtype=torch.int32, device=self.device)
-        eris = torch.zeros(nb, dtype=torch.int64, device=self.device) if stats else None
-        primq = torch.zeros(nb, dtype=torch.int64, device=self.device) if stats else None
         st = self._stream()
         mask = self.fp32_ok
-        check(self.lib.gbx_quartets_count(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
-                                          ptr(dev.pair_key), ptr(dev.pair_npf), ptr(qb),
-                                          ptr(pbd), float(threshold), float(fp32_cutoff),
-                                          ptr(mask), b.n_cand, ptr(counts), ptr(eris),
-                                          ptr(primq), st),
-              "gbx_quartets_count")
-        ch = counts.cpu().numpy().astype(np.int64)
-        offs = np.concatenate([[0], np.cumsum(ch)[:-1]]).astype(np.int64)
-        total = int(ch.sum())
-        quart = torch.empty((max(total, 1), 2), dtype=torch.int32, device=self.device)
-        d_offs = torch.from_numpy(offs).to(self.device)
+        if stats:
+            counts = torch.zeros(nb, dtype=torch.int32, device=self.device)
+            eris = torch.zeros(nb, dtype=torch.int64, device=self.device)
+            primq = torch.zeros(nb, dtype=torch.int64, device=self.device)
+            check(self.lib.gbx_quartets_count(len(b.mols), ptr(dev.mol_cand_off),
+                                              ptr(dev.mol_pair_off), ptr(dev.pair_key),
+                                              ptr(dev.pair_npf), ptr(qb), ptr(pbd),
+                                              float(threshold), float(fp32_cutoff), ptr(mask),
+                                              b.n_cand, ptr(counts), ptr(eris), ptr(primq), st),
+                  "gbx_quartets_count")
+            cap = counts.cpu().numpy().astype(np.int64)
+        else:
+            # one pass: room for every unscreened candidate of each bucket (both
+            # precision variants when mixed), exact counts read back from the cursors
+            cap = b.bucket_capacity().copy()
+            if fp32_cutoff > 0:
+                cap = cap.reshape(-1, 2, 16)
+                cap[:, 1, :] = cap[:, 0, :]
+                cap = cap.reshape(-1)
+        offs = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
+        total = int(cap.sum())
+        key_cache = (id(b), total, fp32_cutoff > 0, stats)
+        if self._queue_cache.get("key") != key_cache:
+            self._queue_cache = {"key": key_cache,
+                                 "quart": torch.empty((max(total, 1), 2), dtype=torch.int32,
+                                                      device=self.device),
+                                 "offs": torch.from_numpy(offs).to(self.device)}
+        quart = self._queue_cache["quart"]
+        d_offs = self._queue_cache["offs"]
         cursor = torch.zeros(nb, dtype=torch.int32, device=self.device)
         check(self.lib.gbx_quartets_fill(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
                                          ptr(dev.pair_key), ptr(qb), ptr(pbd), float(threshold),
                                          float(fp32_cutoff), ptr(mask), b.n_cand, ptr(d_offs),
                                          ptr(cursor), ptr(quart), st), "gbx_quartets_fill")
+        ch = cursor.cpu().numpy().astype(np.int64) if not stats else cap
+        if stats:
+            cursor = torch.from_numpy(cap.astype(np.int32)).to(self.device)
         eh = eris.cpu().numpy() if stats else None
         ph = primq.cpu().numpy() if stats else None
         plans = []
         keys = np.flatnonzero(ch)
-        # one launch per (class, dense, precision) for fused register classes: their bins
-        # are adjacent in the buffer (the bin is the lowest field of the bucket key)
-        groups = []
+        heads = {}
         for key in keys.tolist():
-            head = key >> 4
-            c = key >> 6
+            heads.setdefault(key >> 4, []).append(key)
+        for head, ks in heads.items():
+            fp32 = head & 1
+            dense = bool((head >> 1) & 1)
+            c = head >> 2
             cls = ((c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7)
-            dense = bool((key >> 5) & 1)
+            prec = GBX_FP32_BULK if fp32 else GBX_FP64
             fused = (not dense) and self.path != "coop" and self.has_reg(cls)
-            if fused and groups and groups[-1][0] == head:
-                groups[-1][1].append(key)
-            else:
-                groups.append((head if fused else -1 - key, [key]))
-        for _, ks in groups:
-            key = ks[-1]
-            fp32 = (key >> 4) & 1
-            dense = bool((key >> 5) & 1)
-            c = key >> 6
-            cls = ((c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7)
-            n = int(sum(ch[k] for k in ks))
-            start = int(offs[ks[0]])
-            view = quart[start:start + n]
-            flops = 0.0
-            eris_n = 0
-            if stats:
-                fpr, fco = class_flops(*cls, pure=dense)
-                flops = float(sum(float(ph[k]) for k in ks)) * fpr + n * fco
-                eris_n = int(sum(int(eh[k]) for k in ks))
-            plans.append(Bucket(cls, 1 << (key & 15), GBX_FP32_BULK if fp32 else GBX_FP64, dense,
-                                view, n, eris_n, flops))
+            fpr, fco = class_flops(*cls, pure=dense) if stats else (0, 0)
+            if fused:
+                base = head << 4
+                n = int(ch[ks].sum())
+                flops = float(ph[ks].sum()) * fpr + n * fco if stats else 0.0
+                eris_n = int(eh[ks].sum()) if stats else 0
+                seg = (d_offs[base:base + 16], cursor[base:base + 16], int(cap[base:base + 16].sum()))
+                plans.append(Bucket(cls, 1 << (ks[-1] & 15), prec, dense, quart, n, eris_n, flops,
+                                    seg=seg))
+                continue
+            for key in ks:
+                n = int(ch[key])
+                view = quart[offs[key]:offs[key] + n]
+                flops = float(ph[key]) * fpr + n * fco if stats else 0.0
+                plans.append(Bucket(cls, 1 << (key & 15), prec, dense, view, n,
+                                    int(eh[key]) if stats else 0, flops))
         self._last_queue = quart
         return plans
 
@@ -398,8 +425,7 @@ class EriEngine:
         if out is None:
             out = torch.zeros(dev.batch.n_slots, dtype=torch.float64, device=self.device)
         for bk in plans:
-            bk.path = self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, bk.launch_precision(),
-                                        GBX_MODE_PACKED, out, bk.dense, self.path, bk.lanes())
+            bk.path = self.launch_bucket(dev, bk, out)
         return out
 
     def set_fp32_allowed(self, allowed):

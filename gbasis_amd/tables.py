@@ -125,12 +125,17 @@ class MoleculeTables:
         self.shells, self.nfunc = expand_shells(basis, coord_types)
         ns = len(self.shells)
         self.nshell = ns
-        # ---- shell pairs, I >= J over expanded shells, pair id I(I+1)/2 + J ----
+        # ---- shell pairs I >= J over expanded shells, ordered by (l_a, l_b, primitive
+        # pairs): runs of equal pair classes make the device queue builder's
+        # candidates (p >= q) share bucket keys within a wavefront ----
         ii, jj = np.tril_indices(ns)
-        order = np.argsort(ii * (ii + 1) // 2 + jj, kind="stable")
+        ls = np.array([s.l for s in self.shells])
+        kp = np.array([s.exps.size for s in self.shells])
+        hi = np.maximum(ls[ii], ls[jj])
+        lo = np.minimum(ls[ii], ls[jj])
+        order = np.lexsort((jj, ii, kp[ii] * kp[jj], lo, hi))
         ii, jj = ii[order], jj[order]
         self.npair = ii.size
-        ls = np.array([s.l for s in self.shells])
         swap = ls[ii] < ls[jj]
         sa = np.where(swap, jj, ii)
         sb = np.where(swap, ii, jj)
@@ -299,6 +304,37 @@ class BatchTables:
         # generalised contractions: the Schwarz bound of an expanded pair is the max over
         # the pairs of the same original shells, which the host folds in
         self.generalized = any(m.nshell != m.n_orig for m in molecules)
+        self._capacity = None
+
+    def bucket_capacity(self):
+        """Unscreened candidate quartets per bucket key (precision bit 0): an upper bound
+        of every bucket the device queue builder can fill, computed from the pair-key
+        histogram of each molecule."""
+        if self._capacity is not None:
+            return self._capacity
+        cap = np.zeros(262144, dtype=np.int64)
+        for mi, m in enumerate(self.mols):
+            po = self.mol_pair_off[mi]
+            keys, counts = np.unique(self.pair_key[po:po + m.npair], return_counts=True)
+            keys = keys.astype(np.int64)
+            counts = counts.astype(np.int64)
+            ki, kj = np.meshgrid(np.arange(keys.size), np.arange(keys.size), indexing="ij")
+            sel = ki >= kj
+            ki, kj = ki[sel], kj[sel]
+            n = np.where(ki == kj, counts[ki] * (counts[ki] + 1) // 2, counts[ki] * counts[kj])
+            kp, kq = keys[ki], keys[kj]
+            la_p, lb_p, la_q, lb_q = kp & 7, (kp >> 3) & 7, kq & 7, (kq >> 3) & 7
+            swap = ((la_p + lb_p) * 16 + la_p) < ((la_q + lb_q) * 16 + la_q)
+            kb = np.where(swap, kq, kp)
+            kk = np.where(swap, kp, kq)
+            dense = ((kb >> 6) & 1) | ((kk >> 6) & 1)
+            npq = (kb >> 8) * (kk >> 8)
+            cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7)
+            binid = np.ceil(np.log2(np.maximum(npq, 1))).astype(np.int64)
+            key = ((cls * 2 + dense) * 2) * 16 + binid
+            np.add.at(cap, key, n)
+        self._capacity = cap
+        return cap
 
     def unique_counts(self, bra, ket):
         """Number of canonical packed slots each (bra, ket) quartet writes."""

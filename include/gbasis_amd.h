@@ -137,6 +137,16 @@ int gbx_quartets_fill(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_
                       const int8_t *d_fp32_ok, int64_t n_cand, const int64_t *d_bucket_offset,
                       int32_t *d_bucket_cursor, int32_t *d_quartets, void *stream);
 
+/* Segmented form of gbx_eri_reg: one grid of max_quartets lanes covers up to 16
+ * quartet segments of one class (the primitive-count bins of the device work
+ * queue); segment s holds d_seg_cnt[s] quartets starting at
+ * d_quartets[2 * d_seg_off[s]].  Counts are read on the device, so the launch
+ * needs no host round trip (max_quartets: an upper bound of their sum). */
+int gbx_eri_reg_segments(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                         const int32_t *d_quartets, const int64_t *d_seg_off,
+                         const int32_t *d_seg_cnt, int64_t max_quartets, int32_t precision,
+                         int32_t mode, double *d_out, void *stream);
+
 /* Number of split register kernels for a class too large for one lane's
  * registers (0 if the class has a fused kernel or none). */
 int gbx_split_count(int32_t la, int32_t lb, int32_t lc, int32_t ld);
