@@ -148,7 +148,7 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
         by_cls.setdefault(bk.cls, []).append(bk)
     per_cls_budget = budget_s / max(1, len(by_cls))
     for cls, bks in sorted(by_cls.items()):
-        quart = np.concatenate([bk.quart.cpu().numpy() for bk in bks])
+        quart = np.concatenate([bk.host_quartets() for bk in bks])
         n_cls = quart.shape[0]
         eris_cls = sum(bk.eris for bk in bks)
         times = []

@@ -50,6 +50,17 @@ class Bucket:
     def lanes(self):
         return self.variant[1] if self.variant and self.variant[0] == "R" else None
 
+    def host_quartets(self):
+        """(n, 2) int32 NumPy copy of this bucket's quartets (gathers the segments of a
+        segmented bucket)."""
+        if self.seg is None:
+            return self.quart.cpu().numpy()
+        offs = self.seg[0].cpu().numpy()
+        cnts = self.seg[1].cpu().numpy()
+        q = self.quart.cpu().numpy()
+        parts = [q[o:o + c] for o, c in zip(offs.tolist(), cnts.tolist()) if c > 0]
+        return np.concatenate(parts) if parts else np.zeros((0, 2), np.int32)
+
 
 def _require_gpu():
     if not torch.cuda.is_available():

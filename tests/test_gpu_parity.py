@@ -195,9 +195,9 @@ def test_device_queue_matches_host(gpu):
         dd, hh = {}, {}
         for bk in dplans:
             dd.setdefault((bk.cls, bk.precision, bk.dense), set()).update(
-                map(tuple, bk.quart.cpu().numpy().tolist()))
+                map(tuple, bk.host_quartets().tolist()))
         for bk in hplans:
             hh.setdefault((bk.cls, bk.precision, bk.dense), set()).update(
-                map(tuple, bk.quart.cpu().numpy().tolist()))
+                map(tuple, bk.host_quartets().tolist()))
         assert dd == hh
         assert sum(bk.eris for bk in dplans) == sum(bk.eris for bk in hplans)
