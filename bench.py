@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
     ap.add_argument("--tune-cache", default="", help="load (if present) / save autotuner choices")
+    ap.add_argument("--only", choices=["both", "fp64"], default="both",
+                    help="fp64: time the FP64 form only (for single-precision kernel traces)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the ERI path on one GPU only; --gpus must be 1",
@@ -241,6 +243,16 @@ def main():
                                              args.warmup, tuner)
     out64 = dev.out
     log(f"fp64: {t64 * 1e3:.1f} ms/step (queue {q64 * 1e3:.1f} ms), {eris / t64:.3e} ERIs/s")
+    if args.only == "fp64":
+        roof = roofline_entry(plans64, pb64)
+        print(json.dumps({"metric": METRIC, "value": eris / t64, "unit": "ERIs/s", "n_gpus": 1,
+                          "steps": args.steps, "warmup": args.warmup, "ms_per_step": t64 * 1e3,
+                          "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+                          "dtype": "f64", "data": "synthetic",
+                          "config": {"workload": "FP64 form only", "molecules": args.molecules,
+                                     "eris": eris}, "roofline": roof,
+                          "classes_fp64": class_table(plans64, pb64)}), flush=True)
+        return
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     tmx, qmx, wmx, pbmx, lastmx = time_steps(eng, dev, bound, args.threshold, args.fp32_cutoff,
                                              args.steps, args.warmup, tuner)
