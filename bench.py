@@ -204,6 +204,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
     ap.add_argument("--tune-cache", default="", help="load (if present) / save autotuner choices")
+    ap.add_argument("--copy-calibration-mb", type=int, default=0,
+                    help="run one plain device copy of this size first (HBM counter calibration)")
     ap.add_argument("--only", choices=["both", "fp64"], default="both",
                     help="fp64: time the FP64 form only (for single-precision kernel traces)")
     args = ap.parse_args()
@@ -215,6 +217,14 @@ def main():
     import torch
     from gbasis_amd.engine import DeviceTables, get_engine
 
+    if args.copy_calibration_mb > 0:
+        n = args.copy_calibration_mb * (1 << 20) // 8
+        src = torch.ones(n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"calibration copy: {n * 8} bytes read, {n * 8} bytes written")
+        del src, dst
     t0 = time.perf_counter()
     batch = build_batch(args.molecules)
     log(f"host tables: {len(batch.mols)} molecules, {batch.npair} pairs, "
