@@ -102,6 +102,20 @@ def class_table(plans, per_bucket, top=16):
              "tflops": v[3] / v[0] / 1e12 if v[0] > 0 else 0.0} for k, v in rows]
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "eri_batch64_hbm_counters.json")
+
+
+def recorded_traffic(kernel):
+    """HBM bytes per launch of ``kernel`` from the recorded FETCH_SIZE / WRITE_SIZE runs
+    (each scaled by the counter's reading for a plain copy of known size), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None, None
+    rec = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(kernel)
+    if rec is None:
+        return None, None
+    return rec["fetch_bytes"] + rec["write_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
+
+
 def roofline_entry(plans, per_bucket):
     """Roofline of the dominant kernel: the fused register kernel (one launch per step)
     with the largest time; algorithmic FLOPs per launch / its average launch time,
@@ -123,8 +137,10 @@ def roofline_entry(plans, per_bucket):
     name = f"eri_{path}_{''.join(map(str, cls))}<{'double' if prec == 0 else 'float'}>"
     total_t = float(per_bucket.sum())
     total_f = float(sum(bk.flops for bk in plans))
+    traffic, tsrc = recorded_traffic(name)
     return {"bound": "valu", "kernel": name, "achieved": ach / 1e12, "peak": peak / 1e12,
-            "unit": "TFLOP/s", "frac": ach / peak, "traffic": None, "launches_per_step": nl,
+            "unit": "TFLOP/s", "frac": ach / peak, "traffic": traffic, "traffic_source": tsrc,
+            "launches_per_step": nl,
             "flops_per_launch": f / nl, "avg_launch_ms": t / nl * 1e3,
             "kernel_time_share": t / total_t,
             "all_kernels": {"achieved": total_f / total_t / 1e12, "frac": total_f / total_t / peak}}
@@ -254,6 +270,8 @@ def main():
     out64 = dev.out
     log(f"fp64: {t64 * 1e3:.1f} ms/step (queue {q64 * 1e3:.1f} ms), {eris / t64:.3e} ERIs/s")
     if args.only == "fp64":
+        for bk, last in zip(plans64, last64):
+            bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
         roof = roofline_entry(plans64, pb64)
         print(json.dumps({"metric": METRIC, "value": eris / t64, "unit": "ERIs/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": t64 * 1e3,
