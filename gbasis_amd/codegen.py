@@ -241,14 +241,13 @@ def _head(w, name, strided=False):
     w(f"__global__ __launch_bounds__(kRegThreads) void {name}(gbx_tables t, RegArgs a) {{")
     if strided:
         # lane -> (r, q): consecutive lanes take consecutive quartets of one stride slot r
-        w("  const int64_t gid = static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x)) * "
-          "kRegThreads + threadIdx.x;")
+        w("  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
         w("  if (gid >= a.nquart * a.lanes_per_quartet) return;")
         w("  const int r = static_cast<int>(gid / a.nquart);")
         w("  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;")
     else:
-        w("  const int64_t q = lane_quartet(a, static_cast<int64_t>(xcd_block(blockIdx.x, gridDim.x))"
-          " * kRegThreads + threadIdx.x);")
+        w("  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + "
+          "threadIdx.x);")
         w("  if (q < 0) return;")
     w("  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];")
     w("  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];")

@@ -21,15 +21,6 @@ int fail(int code, const std::string& msg);
 // host: turn a pending launch error into a C-ABI error code
 int check_launch(const char* what);
 
-// Workgroups are dispatched round-robin over the 8 XCDs (each with its own L2).
-// Renumber them so that XCD x runs one contiguous range of logical workgroups:
-// neighbouring quartets (which share shell pairs and primitive-pair records)
-// then meet in the same L2.  Bijective for any grid size.
-__device__ inline unsigned xcd_block(unsigned b, unsigned nb) {
-  const unsigned q = nb >> 3, r = nb & 7, x = b & 7;
-  return x * q + (x < r ? x : r) + (b >> 3);
-}
-
 // offset of the factorial2 norms of angular momentum l in the f2norm table
 __host__ __device__ inline int f2_offset(int l) { return l * (l + 1) * (l + 2) / 6; }
 

@@ -96,7 +96,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
   const int4* items4 = reinterpret_cast<const int4*>(a.items);
   const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
 
-  for (int64_t qi = xcd_block(blockIdx.x, gridDim.x); qi < a.nquart; qi += gridDim.x) {
+  for (int64_t qi = blockIdx.x; qi < a.nquart; qi += gridDim.x) {
     const int bra = a.quart[2 * qi], ket = a.quart[2 * qi + 1];
     const int4 pb = pinfo[bra];
     const int4 pk = pinfo[ket];
