@@ -42,7 +42,8 @@ def build_batch(n_mol):
 
 
 def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
-    """warmup + steps of: device queue build, then every ERI launch.
+    """warmup + steps of: device queue build, then every ERI launch (the launches read
+    their quartet counts on the device: no host round trip inside a step).
 
     Returns (seconds per step, queue seconds per step, wall seconds per step,
     per-bucket kernel seconds per step, plans of the last step).
@@ -51,7 +52,7 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
 
     stream = torch.cuda.current_stream()
     for _ in range(warmup):
-        plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False)
+        plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False, device_counts=True)
         if tuner is not None:
             tuner.tune(dev, plans, dev.out)
         eng.run(dev, plans, dev.out)
@@ -66,7 +67,7 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
     for _ in range(steps):
         q0 = torch.cuda.Event(enable_timing=True)
         q0.record(stream)
-        plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False)
+        plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False, device_counts=True)
         if tuner is not None:
             tuner.apply(plans)
         marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(plans) + 1)]
@@ -85,6 +86,126 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
             per_bucket[i] += marks[i].elapsed_time(marks[i + 1]) * 1e-3
     return (start.elapsed_time(stop) * 1e-3 / steps, t_queue / steps, wall / steps,
             per_bucket / steps, plans)
+
+
+def time_graph(eng, dev, bound, threshold, cutoff, steps, tuner=None):
+    """The same step recorded once into a HIP graph and replayed ``steps`` times.
+    Returns (seconds per replay, the output of the last replay)."""
+    import torch
+
+    graph, out, _ = eng.capture(dev, bound, threshold, cutoff, out=dev.out, tuner=tuner)
+    graph.replay()
+    torch.cuda.synchronize()
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        graph.replay()
+    b.record()
+    torch.cuda.synchronize()
+    return a.elapsed_time(b) * 1e-3 / steps, out
+
+
+def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4):
+    """Serving workload: requests of 2-30 atoms (seeds 4000 + i, s/p/d, K <= 6),
+    precision FP64 or mixed with probability 1/2 each.  First the saturated rate
+    (n_sat requests all arriving at once), then Poisson arrivals at 50 % and 80 %
+    of it with n_requests each; latency = completion (packed result resident in
+    HBM) minus scheduled arrival, which includes the host table build."""
+    import torch
+    from gbasis_amd.engine import EriEngine
+    from gbasis_amd.serving import EriServer, latency_percentiles
+    from gbasis_amd.tables import BatchTables
+    from gbasis_amd.engine import DeviceTables
+    from gbasis_amd.workloads import serving_request
+
+    rng = np.random.default_rng(seed)
+    bases = [serving_request(4000 + i) for i in range(n_requests)]
+    precs = np.where(rng.random(n_requests) < 0.5, "mixed", "fp64")
+    srv = EriServer(threshold=threshold, fp32_cutoff=cutoff, n_streams=n_streams)
+    clock = srv.clock
+    # warm-up: programs and workspaces
+    for i in range(4):
+        srv.submit(bases[i], precs[i])
+    srv.drain(on_done=srv.release)
+
+    def run(count, rate):
+        arr = (np.cumsum(rng.exponential(1.0 / rate, count)) if rate else np.zeros(count))
+        reqs = []
+        t0 = clock()
+        i = ndone = 0
+        last_progress = last_log = t0
+        while i < count or srv.busy():
+            now = clock() - t0
+            while i < count and arr[i] <= now:
+                reqs.append(srv.submit(bases[i], str(precs[i]), t_arrival=t0 + arr[i]))
+                i += 1
+            done = srv.step()
+            for r in done:
+                srv.release(r)
+            ndone += len(done)
+            t = clock()
+            if done:
+                last_progress = t
+            elif t - last_progress > 60.0:
+                raise RuntimeError(f"serving made no progress for 60 s ({ndone}/{count} done)")
+            else:
+                time.sleep(20e-6)
+            if t - last_log > 10.0:
+                log(f"  serving: {ndone}/{count} requests done after {t - t0:.0f} s")
+                last_log = t
+        span = max(r.t_done for r in reqs) - t0
+        return reqs, span
+
+    def eris_of(reqs):
+        eng = EriEngine()
+        total = 0
+        mols = [r.tables for r in reqs]
+        for c in range(0, len(mols), 64):
+            dev = DeviceTables(BatchTables(mols[c:c + 64]))
+            bound = eng.pair_bounds(dev, eng.schwarz(dev))
+            total += sum(bk.eris for bk in eng.plan_device(dev, bound, threshold, 0.0))
+        torch.cuda.synchronize()
+        return total
+
+    sat_reqs, sat_span = run(n_sat, None)
+    sat_rate = n_sat / sat_span
+    sat_eris = eris_of(sat_reqs)
+    log(f"serving saturated: {sat_rate:.1f} req/s, {sat_eris / sat_span:.3e} ERIs/s "
+        f"({n_sat} requests, {srv.batches_launched} batches so far)")
+    out = {"workload": "requests of 2-30 atoms (s/p/d Cartesian, K 1-6, exps [0.05,1e3]), "
+                       "FP64 or mixed with probability 1/2, Schwarz threshold "
+                       f"{threshold:g}; result = packed ERIs resident in HBM",
+           "streams": n_streams, "saturated": {"requests": n_sat, "req_per_s": sat_rate,
+                                               "eris_per_s": sat_eris / sat_span}}
+    all_eris = None
+    for frac in (0.5, 0.8):
+        b0 = srv.batches_launched
+        reqs, span = run(n_requests, frac * sat_rate)
+        if all_eris is None:
+            all_eris = eris_of(reqs)
+        lat = latency_percentiles(reqs)
+        res = {"lambda_req_per_s": frac * sat_rate, "requests": n_requests,
+               "p50_ms": lat["p50"] * 1e3, "p99_ms": lat["p99"] * 1e3,
+               "sustained_req_per_s": n_requests / span,
+               "sustained_eris_per_s": all_eris / span,
+               "batches": srv.batches_launched - b0}
+        log(f"serving at {frac:.0%} ({frac * sat_rate:.1f} req/s): p50 {res['p50_ms']:.1f} ms, "
+            f"p99 {res['p99_ms']:.1f} ms, {res['sustained_eris_per_s']:.3e} ERIs/s")
+        out[f"at_{int(frac * 100)}pct"] = res
+    del srv
+    torch.cuda.empty_cache()
+    return out
+
+
+def attach_stats(timed, stats_plans):
+    """Copy quartet / ERI / FLOP counts of the counting pass onto the timed buckets
+    (matched by bucket key; a timed bucket with no counterpart held no quartets)."""
+    by_key = {bk.key: bk for bk in stats_plans}
+    for bk in timed:
+        s = by_key.get(bk.key)
+        bk.n, bk.eris, bk.flops = (s.n, s.eris, s.flops) if s is not None else (0, 0, 0.0)
+    return timed
 
 
 def class_table(plans, per_bucket, top=16):
@@ -224,6 +345,11 @@ def main():
                     help="run one plain device copy of this size first (HBM counter calibration)")
     ap.add_argument("--only", choices=["both", "fp64"], default="both",
                     help="fp64: time the FP64 form only (for single-precision kernel traces)")
+    ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
+    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-saturation", type=int, default=300)
+    ap.add_argument("--serving-streams", type=int, default=4)
+    ap.add_argument("--no-serving", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the ERI path on one GPU only; --gpus must be 1",
@@ -267,45 +393,51 @@ def main():
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     t64, q64, w64, pb64, last64 = time_steps(eng, dev, bound, args.threshold, 0.0, args.steps,
                                              args.warmup, tuner)
-    out64 = dev.out
     log(f"fp64: {t64 * 1e3:.1f} ms/step (queue {q64 * 1e3:.1f} ms), {eris / t64:.3e} ERIs/s")
+    g64 = None
+    if not args.no_graph:
+        g64, _ = time_graph(eng, dev, bound, args.threshold, 0.0, args.steps, tuner)
+        log(f"fp64 graph replay: {g64 * 1e3:.1f} ms/step, {eris / g64:.3e} ERIs/s")
+    out64 = dev.out
+    attach_stats(last64, plans64)
     if args.only == "fp64":
-        for bk, last in zip(plans64, last64):
-            bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
-        roof = roofline_entry(plans64, pb64)
+        roof = roofline_entry(last64, pb64)
         print(json.dumps({"metric": METRIC, "value": eris / t64, "unit": "ERIs/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": t64 * 1e3,
                           "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
                           "dtype": "f64", "data": "synthetic",
                           "config": {"workload": "FP64 form only", "molecules": args.molecules,
                                      "eris": eris}, "roofline": roof,
-                          "classes_fp64": class_table(plans64, pb64)}), flush=True)
+                          "graph_ms_per_step": None if g64 is None else g64 * 1e3,
+                          "classes_fp64": class_table(last64, pb64)}), flush=True)
         return
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     tmx, qmx, wmx, pbmx, lastmx = time_steps(eng, dev, bound, args.threshold, args.fp32_cutoff,
                                              args.steps, args.warmup, tuner)
-    outmx = dev.out
     log(f"mixed: {tmx * 1e3:.1f} ms/step (queue {qmx * 1e3:.1f} ms), {eris / tmx:.3e} ERIs/s")
+    gmx = None
+    if not args.no_graph:
+        gmx, _ = time_graph(eng, dev, bound, args.threshold, args.fp32_cutoff, args.steps, tuner)
+        log(f"mixed graph replay: {gmx * 1e3:.1f} ms/step, {eris / gmx:.3e} ERIs/s")
+    outmx = dev.out
     if tuner is not None and args.tune_cache:
         tuner.save(args.tune_cache)
     # bookkeeping of the buckets actually launched (after the autotuner's FP32 mask)
     plansmx = eng.plan_device(dev, bound, args.threshold, args.fp32_cutoff, stats=True)
-    if tuner is not None:
-        tuner.apply(plansmx)
-    for bk, last in zip(plans64, last64):
-        bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
-    for bk, last in zip(plansmx, lastmx):
-        bk.path, bk.run_precision, bk.variant = last.path, last.run_precision, last.variant
-    assert [bk.cls for bk in plansmx] == [bk.cls for bk in lastmx]
+    attach_stats(lastmx, plansmx)
     diff = outmx.sub_(out64).abs_()
     max_err = float(diff.max())
     rms_err = float(torch.linalg.vector_norm(diff)) / np.sqrt(diff.numel())
     del diff, outmx
-    roof = roofline_entry(plans64, pb64)
-    roof_mx = roofline_entry(plansmx, pbmx)
+    roof = roofline_entry(last64, pb64)
+    roof_mx = roofline_entry(lastmx, pbmx)
+    # the headline step is the HIP-graph replay of the whole step (queue fill and every
+    # launch) when captured, else the eager step; both are reported
+    tmx_head = gmx if gmx is not None else tmx
+    t64_head = g64 if g64 is not None else t64
     result = {
-        "metric": METRIC, "value": eris / tmx, "unit": "ERIs/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": tmx * 1e3,
+        "metric": METRIC, "value": eris / tmx_head, "unit": "ERIs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": tmx_head * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64 (Boys, prefactors, accumulation) + f32 (recursion bulk, low-bound quartets)",
         "data": "synthetic",
@@ -315,19 +447,29 @@ def main():
                    "schwarz_threshold": args.threshold, "fp32_cutoff": args.fp32_cutoff,
                    "fp32_quartets": n_fp32, "kernel_path": args.path,
                    "autotuned": not args.no_tune,
-                   "fp32_quartets_run_fp32": sum(bk.n for bk in plansmx
+                   "fp32_quartets_run_fp32": sum(bk.n for bk in lastmx
                                                  if bk.launch_precision() == 1)},
-        "fp64": {"value": eris / t64, "ms_per_step": t64 * 1e3},
+        "step_timed_as": "hip graph replay" if gmx is not None else "eager launches",
+        "fp64": {"value": eris / t64_head, "ms_per_step": t64_head * 1e3},
+        "eager_ms_per_step": {"fp64": t64 * 1e3, "mixed": tmx * 1e3},
+        "graph_ms_per_step": {"fp64": None if g64 is None else g64 * 1e3,
+                              "mixed": None if gmx is None else gmx * 1e3},
         "queue_build_ms": {"fp64": q64 * 1e3, "mixed": qmx * 1e3},
         "mixed_vs_fp64": {"max_abs_err": max_err, "rms_err": rms_err},
         "wall_ms_per_step": {"fp64": w64 * 1e3, "mixed": wmx * 1e3},
         "roofline": roof,
         "roofline_mixed": roof_mx,
-        "classes_fp64": class_table(plans64, pb64),
-        "classes_mixed": class_table(plansmx, pbmx),
+        "classes_fp64": class_table(last64, pb64),
+        "classes_mixed": class_table(lastmx, pbmx),
     }
     if not args.no_cpu:
         result["cpu_baseline"] = cpu_baseline(batch, plans64)
+    if not args.no_serving:
+        del dev, out64
+        torch.cuda.empty_cache()
+        result["serving"] = serving_bench(args.serving_requests, args.serving_saturation,
+                                          args.threshold, args.fp32_cutoff,
+                                          args.serving_streams)
     print(json.dumps(result), flush=True)
 
 

@@ -47,8 +47,8 @@ SIGNATURES = {
                                           c_i32, c_i32, ctypes.POINTER(c_i64),
                                           ctypes.POINTER(c_i64)]),
     "gbx_eri_coop": (ctypes.c_int, [ctypes.POINTER(GbxTables), ctypes.POINTER(GbxProgram), c_i32,
-                                    c_i32, c_i32, c_i32, c_vp, c_i64, c_i32, c_i32, c_i32, c_vp,
-                                    c_vp, c_i64, c_vp]),
+                                    c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32,
+                                    c_vp, c_vp, c_i64, c_vp]),
     "gbx_unpack_dense": (ctypes.c_int, [c_vp, c_i64, c_i32, c_vp, c_vp]),
     "gbx_has_reg_kernel": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i32]),
     "gbx_split_count": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i32]),
@@ -56,17 +56,18 @@ SIGNATURES = {
                                             c_i32, c_vp, c_vp, c_vp, c_i64, c_i32, c_i32, c_vp,
                                             c_vp]),
     "gbx_eri_split": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32, c_i32, c_i32,
-                                     c_vp, c_i64, c_i32, c_i32, c_vp, c_i64, c_vp]),
+                                     c_vp, c_i64, c_vp, c_i32, c_i32, c_vp, c_i64, c_vp]),
     "gbx_eri_hrr": (ctypes.c_int, [ctypes.POINTER(GbxTables), ctypes.POINTER(GbxProgram), c_i32,
-                                   c_i32, c_i32, c_i32, c_vp, c_i64, c_i32, c_vp, c_i64, c_i32, c_vp,
-                                   c_vp, c_i64, c_vp]),
+                                   c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i32, c_vp, c_i64, c_i32,
+                                   c_vp, c_vp, c_i64, c_vp]),
     "gbx_quartets_count": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
-                                          ctypes.c_double, ctypes.c_double, c_vp, c_i64, c_vp,
-                                          c_vp, c_vp, c_vp]),
+                                          ctypes.c_double, ctypes.c_double, c_vp, c_vp, c_i64,
+                                          c_vp, c_vp, c_vp, c_vp]),
     "gbx_quartets_fill": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double,
-                                         ctypes.c_double, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+                                         ctypes.c_double, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp,
+                                         c_vp]),
     "gbx_eri_reg": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32, c_i32, c_vp,
-                                   c_i64, c_i32, c_i32, c_vp, c_vp]),
+                                   c_i64, c_vp, c_i32, c_i32, c_vp, c_vp]),
 }
 
 _lib = None

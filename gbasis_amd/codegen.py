@@ -245,6 +245,7 @@ def _head(w, name, strided=False):
         w("  if (gid >= a.nquart * a.lanes_per_quartet) return;")
         w("  const int r = static_cast<int>(gid / a.nquart);")
         w("  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;")
+        w("  if (a.count && q >= *a.count) return;")
     else:
         w("  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + "
           "threadIdx.x);")

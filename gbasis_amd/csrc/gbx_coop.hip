@@ -37,6 +37,7 @@ constexpr int kScalars = 17;  // per-primitive-quartet scalars held in LDS
 struct CoopArgs {
   const int32_t* quart;
   int64_t nquart;
+  const int32_t* count;  // device quartet count (<= nquart); null: nquart
   const int32_t* items;
   const int32_t* phases;
   const int32_t* out_slots;
@@ -96,7 +97,8 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
   const int4* items4 = reinterpret_cast<const int4*>(a.items);
   const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
 
-  for (int64_t qi = blockIdx.x; qi < a.nquart; qi += gridDim.x) {
+  const int64_t nquart = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
+  for (int64_t qi = blockIdx.x; qi < nquart; qi += gridDim.x) {
     const int bra = a.quart[2 * qi], ket = a.quart[2 * qi + 1];
     const int4 pb = pinfo[bra];
     const int4 pk = pinfo[ket];
@@ -322,13 +324,15 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
 
 template <typename TB, bool kLDS>
 int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, int lc, int ld,
-                const int32_t* d_quartets, int64_t nquart, int pf, int mode, double* d_out,
+                const int32_t* d_quartets, int64_t nquart, const int32_t* d_count, int pf,
+                int mode, double* d_out,
                 void* d_scratch, int64_t scratch_bytes, hipStream_t stream,
                 const double* acc_in = nullptr, int64_t acc_stride = 0, int acc_parts = 1) {
   const CoopLayout l = coop_layout(prog->header, sizeof(TB), pf);
   CoopArgs a{};
   a.quart = d_quartets;
   a.nquart = nquart;
+  a.count = d_count;
   a.items = prog->items;
   a.phases = prog->phases;
   a.out_slots = prog->out_slots;
@@ -391,8 +395,9 @@ extern "C" int gbx_coop_workspace(const gbx_program* prog, int32_t la, int32_t l
 
 extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_t la, int32_t lb,
                             int32_t lc, int32_t ld, const int32_t* d_quartets, int64_t nquart,
-                            int32_t precision, int32_t prim_copies, int32_t mode, double* d_out,
-                            void* d_scratch, int64_t scratch_bytes, void* stream) {
+                            const int32_t* d_count, int32_t precision, int32_t prim_copies,
+                            int32_t mode, double* d_out, void* d_scratch, int64_t scratch_bytes,
+                            void* stream) {
   if (!t || !prog || !d_out || (nquart > 0 && !d_quartets))
     return fail(GBX_ERR_ARG, "gbx_eri_coop: null argument");
   if (prim_copies < 1 || prim_copies > kCoopThreads)
@@ -410,21 +415,21 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
   const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
   const bool lds = l.total <= kCoopLdsLimit;
   if (precision == GBX_FP64) {
-    return lds ? launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, prim_copies,
-                                           mode, d_out, d_scratch, scratch_bytes, s)
-               : launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart,
+    return lds ? launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
+                                           prim_copies, mode, d_out, d_scratch, scratch_bytes, s)
+               : launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
                                             prim_copies, mode, d_out, d_scratch, scratch_bytes, s);
   }
-  return lds ? launch_coop<float, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, prim_copies,
-                                        mode, d_out, d_scratch, scratch_bytes, s)
-             : launch_coop<float, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, prim_copies,
-                                         mode, d_out, d_scratch, scratch_bytes, s);
+  return lds ? launch_coop<float, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
+                                        prim_copies, mode, d_out, d_scratch, scratch_bytes, s)
+             : launch_coop<float, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
+                                         prim_copies, mode, d_out, d_scratch, scratch_bytes, s);
 }
 
 extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t la, int32_t lb,
                            int32_t lc, int32_t ld, const int32_t* d_quartets, int64_t nquart,
-                           int32_t mode, const double* d_acc, int64_t acc_stride,
-                           int32_t acc_parts, double* d_out, void* d_scratch,
+                           const int32_t* d_count, int32_t mode, const double* d_acc,
+                           int64_t acc_stride, int32_t acc_parts, double* d_out, void* d_scratch,
                            int64_t scratch_bytes, void* stream) {
   if (!t || !prog || !d_out || !d_acc || (nquart > 0 && !d_quartets))
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: null argument");
@@ -439,8 +444,10 @@ extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t
   hipStream_t s = static_cast<hipStream_t>(stream);
   const CoopLayout l = coop_layout(prog->header, 8, 1);
   if (l.total <= kCoopLdsLimit)
-    return launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, 1, mode, d_out,
-                                     d_scratch, scratch_bytes, s, d_acc, acc_stride, acc_parts);
-  return launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, 1, mode, d_out,
-                                    d_scratch, scratch_bytes, s, d_acc, acc_stride, acc_parts);
+    return launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count, 1, mode,
+                                     d_out, d_scratch, scratch_bytes, s, d_acc, acc_stride,
+                                     acc_parts);
+  return launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count, 1, mode,
+                                    d_out, d_scratch, scratch_bytes, s, d_acc, acc_stride,
+                                    acc_parts);
 }

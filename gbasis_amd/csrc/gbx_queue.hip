@@ -35,6 +35,7 @@ struct QueueArgs {
   const double* pair_bound;     // per pair: Schwarz bound Q of its original shell pair
   const double* pair_pbound;    // per pair: bound of the contracted [e0|f0] (precision choice)
   double threshold, fp32_cutoff;
+  const double* mol_fp32_cutoff;  // per molecule (overrides fp32_cutoff); null: fp32_cutoff
   const int8_t* fp32_ok;        // per (class * 2 + dense): FP32 bulk allowed; null = all
   int32_t* bucket_count;        // [kNumBuckets]
   unsigned long long* bucket_eris;    // [kNumBuckets] or null
@@ -92,7 +93,8 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       const double pbound = a.pair_pbound[gp] * a.pair_pbound[gq];
       const int np = (kb >> 8) * (kk >> 8);
       const int cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7);
-      const int fp32 = (a.fp32_cutoff > 0.0 && pbound < a.fp32_cutoff &&
+      const double cut = a.mol_fp32_cutoff ? a.mol_fp32_cutoff[lo] : a.fp32_cutoff;
+      const int fp32 = (cut > 0.0 && pbound < cut &&
                         (a.fp32_ok == nullptr || a.fp32_ok[cls * 2 + dense])) ? 1 : 0;
       key = ((cls * 2 + dense) * 2 + fp32) * 16 + ceil_log2(np);
       if (!kFill && a.bucket_eris) {
@@ -152,7 +154,8 @@ extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
                                   const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
                                   const int32_t* d_pair_npf, const double* d_pair_bound,
                                   const double* d_pair_pbound, double threshold,
-                                  double fp32_cutoff, const int8_t* d_fp32_ok, int64_t n_cand,
+                                  double fp32_cutoff, const double* d_mol_fp32_cutoff,
+                                  const int8_t* d_fp32_ok, int64_t n_cand,
                                   int32_t* d_bucket_count, uint64_t* d_bucket_eris,
                                   uint64_t* d_bucket_primq, void* stream) {
   if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
@@ -169,6 +172,7 @@ extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
   a.pair_pbound = d_pair_pbound;
   a.threshold = threshold;
   a.fp32_cutoff = fp32_cutoff;
+  a.mol_fp32_cutoff = d_mol_fp32_cutoff;
   a.fp32_ok = d_fp32_ok;
   a.bucket_count = d_bucket_count;
   a.bucket_eris = reinterpret_cast<unsigned long long*>(d_bucket_eris);
@@ -180,7 +184,8 @@ extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
 extern "C" int gbx_quartets_fill(int32_t n_mol, const int64_t* d_mol_cand_off,
                                  const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
                                  const double* d_pair_bound, const double* d_pair_pbound,
-                                 double threshold, double fp32_cutoff, const int8_t* d_fp32_ok,
+                                 double threshold, double fp32_cutoff,
+                                 const double* d_mol_fp32_cutoff, const int8_t* d_fp32_ok,
                                  int64_t n_cand, const int64_t* d_bucket_offset,
                                  int32_t* d_bucket_cursor, int32_t* d_quartets, void* stream) {
   if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
@@ -195,6 +200,7 @@ extern "C" int gbx_quartets_fill(int32_t n_mol, const int64_t* d_mol_cand_off,
   a.pair_pbound = d_pair_pbound;
   a.threshold = threshold;
   a.fp32_cutoff = fp32_cutoff;
+  a.mol_fp32_cutoff = d_mol_fp32_cutoff;
   a.fp32_ok = d_fp32_ok;
   a.bucket_offset = d_bucket_offset;
   a.bucket_cursor = d_bucket_cursor;

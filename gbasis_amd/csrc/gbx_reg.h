@@ -24,6 +24,8 @@ struct RegArgs {
   int32_t lanes_per_quartet;  // split kernels: lanes sharing one shell quartet (R)
   const int64_t* seg_off;     // fused kernels, segmented launch: first quartet of each of
   const int32_t* seg_cnt;     //   kSegments segments and its count (device); null: plain
+  const int32_t* count;       // plain launches: device quartet count (<= nquart, the
+                              //   capacity the grid is sized for); null: nquart
 };
 
 constexpr int kSegments = 16;
@@ -32,7 +34,10 @@ constexpr int kSegments = 16;
 // primitive-count bins of one class in one grid: lane gid takes quartet
 // seg_off[s] + (gid - sum_{s' < s} seg_cnt[s']).
 __device__ inline int64_t lane_quartet(const RegArgs& a, int64_t gid) {
-  if (a.seg_cnt == nullptr) return gid < a.nquart ? gid : -1;
+  if (a.seg_cnt == nullptr) {
+    const int64_t n = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
+    return gid < n ? gid : -1;
+  }
   int64_t base = 0;
 #pragma unroll
   for (int s = 0; s < kSegments; ++s) {

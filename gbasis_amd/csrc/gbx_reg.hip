@@ -8,8 +8,8 @@ extern "C" int gbx_has_reg_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld
 }
 
 extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
-                           const int32_t* d_quartets, int64_t nquart, int32_t precision,
-                           int32_t mode, double* d_out, void* stream) {
+                           const int32_t* d_quartets, int64_t nquart, const int32_t* d_count,
+                           int32_t precision, int32_t mode, double* d_out, void* stream) {
   if (!t || !d_out || (nquart > 0 && !d_quartets))
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: null argument");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
@@ -21,7 +21,7 @@ extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t 
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
-  gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0, 1, nullptr, nullptr};
+  gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0, 1, nullptr, nullptr, d_count};
   return gbx::launch_reg_generated(t, la, lb, lc, ld, a, precision,
                                    static_cast<hipStream_t>(stream));
 }
@@ -42,7 +42,8 @@ extern "C" int gbx_eri_reg_segments(const gbx_tables* t, int32_t la, int32_t lb,
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: Boys table too short for this class");
   if (max_quartets == 0) return GBX_OK;
-  gbx::RegArgs a{d_quartets, max_quartets, mode, d_out, nullptr, 0, 1, d_seg_off, d_seg_cnt};
+  gbx::RegArgs a{d_quartets, max_quartets, mode, d_out, nullptr, 0, 1, d_seg_off, d_seg_cnt,
+                 nullptr};
   return gbx::launch_reg_generated(t, la, lb, lc, ld, a, precision,
                                    static_cast<hipStream_t>(stream));
 }
@@ -54,7 +55,8 @@ extern "C" int gbx_split_count(int32_t la, int32_t lb, int32_t lc, int32_t ld) {
 
 extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                              int32_t split, const int32_t* d_quartets, int64_t nquart,
-                             int32_t precision, int32_t lanes_per_quartet, double* d_acc,
+                             const int32_t* d_count, int32_t precision,
+                             int32_t lanes_per_quartet, double* d_acc,
                              int64_t acc_stride, void* stream) {
   if (!t || !d_acc || (nquart > 0 && !d_quartets))
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: null argument");
@@ -70,7 +72,7 @@ extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
   gbx::RegArgs a{d_quartets, nquart, GBX_MODE_PACKED, nullptr, d_acc, acc_stride,
-                 lanes_per_quartet, nullptr, nullptr};
+                 lanes_per_quartet, nullptr, nullptr, d_count};
   return gbx::launch_split_generated(t, la, lb, lc, ld, split, a, precision,
                                      static_cast<hipStream_t>(stream));
 }

@@ -6,9 +6,14 @@ only the allocation mechanism; every computation runs in the HIP kernels of
 
 1. Schwarz bounds Q_ij = sqrt(max |(ij|ij)|) for every shell pair
    (gbasis/integrals/_schwarz_screening.py:14-90), one launch per class;
-2. host screening / class bucketing of the unique quartets (tables.py);
+2. the screened, class-bucketed work queue, built on the device
+   (``plan_device``; ``plan_quartets`` is the host enumeration kept for checks);
 3. one launch per (class, primitive-count bin, precision) writing the
    canonical packed array.
+
+With ``device_counts`` the launches of stage 3 read their quartet counts from
+the queue builder's cursors on the device, so a whole step is enqueued without
+a host round trip and can be recorded once into a HIP graph (``capture``).
 """
 
 import ctypes
@@ -43,6 +48,9 @@ class Bucket:
     run_precision: int = -1   # precision actually launched (autotuner); -1: = precision
     variant: tuple = None     # autotuned variant, e.g. ("R", lanes per quartet)
     seg: tuple = None         # fused classes: (device segment offsets, device counts, max lanes)
+    count: object = None      # device int32 count (n is then the capacity); None: n exact
+    key: tuple = None         # ("head", class/dense/precision head) for a fused launch,
+                              # ("key", bucket key) otherwise
 
     def launch_precision(self):
         return self.precision if self.run_precision < 0 else self.run_precision
@@ -54,7 +62,8 @@ class Bucket:
         """(n, 2) int32 NumPy copy of this bucket's quartets (gathers the segments of a
         segmented bucket)."""
         if self.seg is None:
-            return self.quart.cpu().numpy()
+            q = self.quart.cpu().numpy()
+            return q if self.count is None else q[:int(self.count.item())]
         offs = self.seg[0].cpu().numpy()
         cnts = self.seg[1].cpu().numpy()
         q = self.quart.cpu().numpy()
@@ -92,33 +101,69 @@ class ProgramCache:
         return self._cache[cls]
 
 
-class DeviceTables:
-    """A BatchTables object copied to the device, plus its gbx_tables struct."""
+def _h2d(a, device):
+    """Asynchronous host-to-device copy through pinned memory (stream-ordered on the
+    current stream; torch keeps the pinned block until the copy is done)."""
+    src = torch.from_numpy(np.ascontiguousarray(a))
+    return src.pin_memory().to(device, non_blocking=True)
 
-    def __init__(self, batch: BatchTables, device="cuda"):
+
+class DeviceTables:
+    """A BatchTables object copied to the device, plus its gbx_tables struct.
+
+    All tables travel in one pinned blob and one asynchronous copy on the
+    current stream.  ``mol_fp32_cutoff`` (optional, one value per molecule)
+    selects mixed precision per molecule.
+    """
+
+    ALIGN = 256
+
+    def __init__(self, batch: BatchTables, device="cuda", mol_fp32_cutoff=None):
         _require_gpu()
         self.batch = batch
         self.device = torch.device(device)
-
-        def up(a, dtype=None):
-            a = np.array(a if dtype is None else a.astype(dtype), copy=True, order="C")
-            return torch.from_numpy(a).to(self.device)
-
-        self.shell_info = up(batch.shell_info, np.int32)
-        self.comp_map = up(batch.comp_map, np.int32)
-        self.comp_scale = up(batch.comp_scale, np.float64)
-        self.out_mat = up(batch.out_mat, np.float64)
-        self.pair_info = up(batch.pair_info, np.int32)
-        self.pair_geom = up(batch.pair_geom, np.float64)
-        self.pair_out_base = up(batch.pair_out_base.view(np.int64))
-        self.prim_pairs = up(batch.prim_pairs, np.float64)
-        self.f2norm = up(f2norm_table(8))
-        self.pair_key = up(batch.pair_key, np.int32)
-        self.pair_npf = up(batch.pair_npf, np.int32)
-        self.mol_cand_off = up(batch.mol_cand_off, np.int64)
-        self.mol_pair_off = up(np.array(batch.mol_pair_off), np.int32)
         tab = boys_table()
-        self.boys = up(tab)
+        arrays = {
+            "shell_info": batch.shell_info.astype(np.int32),
+            "comp_map": batch.comp_map.astype(np.int32),
+            "comp_scale": batch.comp_scale.astype(np.float64),
+            "out_mat": batch.out_mat.astype(np.float64),
+            "pair_info": batch.pair_info.astype(np.int32),
+            "pair_geom": batch.pair_geom.astype(np.float64),
+            "pair_out_base": batch.pair_out_base.view(np.int64),
+            "prim_pairs": batch.prim_pairs.astype(np.float64),
+            "f2norm": f2norm_table(8),
+            "pair_key": batch.pair_key.astype(np.int32),
+            "pair_npf": batch.pair_npf.astype(np.int32),
+            "mol_cand_off": batch.mol_cand_off.astype(np.int64),
+            "mol_pair_off": np.asarray(batch.mol_pair_off, dtype=np.int32),
+            "boys": tab,
+        }
+        if mol_fp32_cutoff is not None:
+            arrays["mol_fp32_cutoff"] = np.asarray(mol_fp32_cutoff, dtype=np.float64).reshape(
+                len(batch.mols))
+        # bra == ket pair lists of the Schwarz pass, one per (class, primitive bin)
+        self.schwarz_lists = []
+        for i, (cls, nprimq, _, quart) in enumerate(batch.schwarz_buckets()):
+            arrays[f"sw{i}"] = quart
+            self.schwarz_lists.append((cls, nprimq, f"sw{i}"))
+        offs, total = {}, 0
+        for k, v in arrays.items():
+            offs[k] = total
+            total += -(-max(v.nbytes, 1) // self.ALIGN) * self.ALIGN
+        host = torch.empty(total, dtype=torch.uint8, pin_memory=True)
+        hv = host.numpy()
+        for k, v in arrays.items():
+            hv[offs[k]:offs[k] + v.nbytes] = np.ascontiguousarray(v).reshape(-1).view(np.uint8)
+        self.blob = host.to(self.device, non_blocking=True)
+        dt = {np.dtype(np.int32): torch.int32, np.dtype(np.int64): torch.int64,
+              np.dtype(np.float64): torch.float64}
+        for k, v in arrays.items():
+            view = self.blob[offs[k]:offs[k] + v.nbytes].view(dt[v.dtype]).view(v.shape)
+            setattr(self, k, view)
+        self.schwarz_lists = [(c, n, getattr(self, k)) for c, n, k in self.schwarz_lists]
+        if mol_fp32_cutoff is None:
+            self.mol_fp32_cutoff = None
         st = GbxTables()
         st.shell_info = ptr(self.shell_info)
         st.comp_map = ptr(self.comp_map)
@@ -134,6 +179,7 @@ class DeviceTables:
         st.boys_nord = tab.shape[1]
         st.boys_h = BOYS_H
         self.struct = st
+        self.queue = {}   # work-queue buffers of plan_device, per layout
 
 
 class EriEngine:
@@ -145,18 +191,40 @@ class EriEngine:
         self.device = torch.device(device)
         self.programs = ProgramCache(self.device)
         self.scratch_bytes = int(scratch_bytes)
-        self._scratch = None
+        self._scratch = {}
+        self._ws = {}
+        # stream_workspaces: split accumulators and work-queue buffers come from
+        # grow-only buffers per stream instead of fresh allocations (a server running
+        # batches on several streams then allocates nothing per batch)
+        self.stream_workspaces = False
         self._reg = {}
         self._split = {}
         self._pf = {}
         self.fp32_ok = None   # int8 device mask per class*2+dense (autotuner); None: all allowed
-        self._queue_cache = {}
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
+        # deterministic: variant choices depend only on (class, primitive bin, precision),
+        # never on how many quartets a launch holds, so the value of every packed slot
+        # is independent of the batch it runs in (serving)
+        self.deterministic = False
 
     def scratch(self):
-        if self._scratch is None:
-            self._scratch = torch.empty(self.scratch_bytes, dtype=torch.uint8, device=self.device)
-        return self._scratch
+        """Global scratch of the cooperative kernel, one buffer per stream."""
+        key = torch.cuda.current_stream(self.device).cuda_stream
+        if key not in self._scratch:
+            self._scratch[key] = torch.empty(self.scratch_bytes, dtype=torch.uint8,
+                                             device=self.device)
+        return self._scratch[key]
+
+    def workspace(self, name, numel, dtype):
+        """Grow-only device buffer ``name`` of the current stream, at least ``numel``
+        elements of ``dtype``; its contents are only valid until the next use."""
+        key = (torch.cuda.current_stream(self.device).cuda_stream, name)
+        buf = self._ws.get(key)
+        if buf is None or buf.numel() < numel or buf.dtype != dtype:
+            size = max(int(numel), int(1.25 * (buf.numel() if buf is not None else 0)), 1)
+            buf = torch.empty(size, dtype=dtype, device=self.device)
+            self._ws[key] = buf
+        return buf[:numel]
 
     def _stream(self):
         return ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
@@ -196,10 +264,17 @@ class EriEngine:
 
     TARGET_LANES = 1 << 18   # lanes in flight that keep all 1024 SIMDs of the GPU busy
 
+    DETERMINISTIC_LANES = 8
+
     def _lanes_per_quartet(self, nq, nprimq):
         """R lanes per shell quartet for the split kernels: enough lanes to fill the GPU,
-        never more than the (upper bound of the) primitive quartets per shell quartet."""
+        never more than the (upper bound of the) primitive quartets per shell quartet.
+        In deterministic mode R depends on the primitive bin only."""
         R = 1
+        if self.deterministic:
+            while R < min(nprimq, self.DETERMINISTIC_LANES):
+                R *= 2
+            return R
         while R * 2 <= nprimq and nq * R * 2 <= self.TARGET_LANES:
             R *= 2
         return R
@@ -217,14 +292,18 @@ class EriEngine:
     def launch_bucket(self, dev, bk, out, mode=GBX_MODE_PACKED, precision=None, lanes=None):
         prec = bk.launch_precision() if precision is None else precision
         return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
-                                 self.path, lanes if lanes is not None else bk.lanes(), bk.seg)
+                                 self.path, lanes if lanes is not None else bk.lanes(), bk.seg,
+                                 bk.count)
 
     def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,
-                     dense=False, path="auto", lanes=None, seg=None):
+                     dense=False, path="auto", lanes=None, seg=None, count=None):
         """One launch for a (class, bin) bucket; quartets is a device int32 (n,2).
 
         Classes with a generated register kernel and no dense output transform run
-        one quartet per lane; all others run the cooperative kernel.
+        one quartet per lane (split register kernels plus a horizontal-recursion pass
+        for the classes too large for one lane); all others run the cooperative
+        kernel.  ``count``: device int32 holding the number of quartets (n is then
+        the capacity).
         """
         if path != "coop" and not dense and self.has_reg(cls):
             if seg is not None:
@@ -234,8 +313,8 @@ class EriEngine:
                       f"gbx_eri_reg_segments{cls}")
             else:
                 check(self.lib.gbx_eri_reg(ctypes.byref(dev.struct), *cls, ptr(quartets),
-                                           quartets.shape[0], precision, mode, ptr(out),
-                                           self._stream()), f"gbx_eri_reg{cls}")
+                                           quartets.shape[0], ptr(count), precision, mode,
+                                           ptr(out), self._stream()), f"gbx_eri_reg{cls}")
             return "reg"
         st, _, _ = self.programs.get(cls)
         nsplit = self.split_count(cls)
@@ -243,21 +322,25 @@ class EriEngine:
             nq = quartets.shape[0]
             nacc = st.header[1]
             R = lanes if lanes else self._lanes_per_quartet(nq, nprimq)
-            acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
+            if self.stream_workspaces:
+                acc = self.workspace("acc", nacc * R * nq, torch.float64)
+            else:
+                acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
             for k in range(nsplit):
                 check(self.lib.gbx_eri_split(ctypes.byref(dev.struct), *cls, k, ptr(quartets), nq,
-                                             precision, R, ptr(acc), nq, self._stream()),
-                      f"gbx_eri_split{cls}[{k}]")
+                                             ptr(count), precision, R, ptr(acc), nq,
+                                             self._stream()), f"gbx_eri_split{cls}[{k}]")
             sc = self.scratch()
             check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
-                                       ptr(quartets), nq, mode, ptr(acc), nq, R, ptr(out), ptr(sc),
-                                       sc.numel(), self._stream()), f"gbx_eri_hrr{cls}")
+                                       ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
+                                       ptr(out), ptr(sc), sc.numel(), self._stream()),
+                  f"gbx_eri_hrr{cls}")
             return "split"
         pf = self._plan(cls, nprimq, precision)
         sc = self.scratch()
         check(self.lib.gbx_eri_coop(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
-                                    ptr(quartets), quartets.shape[0], precision, pf, mode,
-                                    ptr(out), ptr(sc), sc.numel(), self._stream()),
+                                    ptr(quartets), quartets.shape[0], ptr(count), precision, pf,
+                                    mode, ptr(out), ptr(sc), sc.numel(), self._stream()),
               f"gbx_eri_coop{cls}")
         return "coop"
 
@@ -268,12 +351,8 @@ class EriEngine:
         Column 1: sqrt(max |contracted [e0|f0] of (ij|ij)|), the bound on the inputs of
         the horizontal recursion that selects the FP32 recursion bulk.
         """
-        b = dev.batch
-        npair = b.npair
-        q = torch.zeros((npair, 2), dtype=torch.float64, device=self.device)
-        p = np.arange(npair, dtype=np.int32)
-        for cls, nprimq, _, quart in b.class_buckets(p, p):
-            dq = torch.from_numpy(quart).to(self.device)
+        q = torch.zeros((dev.batch.npair, 2), dtype=torch.float64, device=self.device)
+        for cls, nprimq, dq in dev.schwarz_lists:
             # Q needs no output transform: the register kernels serve every class they cover
             self.launch_class(dev, cls, dq, nprimq, GBX_FP64, GBX_MODE_SCHWARZ, q,
                               path=self.path)
@@ -332,17 +411,25 @@ class EriEngine:
                 torch.from_numpy(self._orig_bounds(b, qh[:, 1])).to(self.device))
 
     def plan_device(self, dev: DeviceTables, bound=None, threshold=0.0, fp32_cutoff=0.0,
-                    stats=True):
-        """Build the screened, bucketed work queue on the device (two passes).
+                    stats=True, device_counts=False):
+        """Build the screened, bucketed work queue on the device.
 
         ``bound`` is the (screening, precision) pair of per-pair device bounds from
-        pair_bounds(); required when screening or mixed precision is requested.
-        Returns a list of Bucket (quartets are views of one device buffer).  With ``stats`` the packed-slot and primitive-quartet
-        counts per bucket are accumulated for FLOP / ERI bookkeeping.
+        pair_bounds(); required when screening or mixed precision is requested
+        (mixed precision is on when ``fp32_cutoff`` > 0 or the tables carry
+        per-molecule cutoffs).  Returns a list of Bucket (quartets are views of one
+        device buffer cached on ``dev``).
+
+        stats=True: a counting pass first (exact capacities, packed-slot and
+        primitive-quartet counts per bucket for FLOP / ERI bookkeeping).
+        stats=False: one pass into capacities computed on the host from the pair
+        keys; the counts are then read back (device_counts=False) or left on the
+        device for the launches to read (device_counts=True: no host round trip).
         """
         b = dev.batch
+        mixed = fp32_cutoff > 0 or dev.mol_fp32_cutoff is not None
         if bound is None:
-            if threshold > 0 or fp32_cutoff > 0:
+            if threshold > 0 or mixed:
                 raise ValueError("screening / mixed precision need the Schwarz bounds")
             ones = torch.ones(b.npair, dtype=torch.float64, device=self.device)
             bound = (ones, ones)
@@ -350,6 +437,7 @@ class EriEngine:
         nb = 262144
         st = self._stream()
         mask = self.fp32_ok
+        mcut = dev.mol_fp32_cutoff
         if stats:
             counts = torch.zeros(nb, dtype=torch.int32, device=self.device)
             eris = torch.zeros(nb, dtype=torch.int64, device=self.device)
@@ -357,36 +445,46 @@ class EriEngine:
             check(self.lib.gbx_quartets_count(len(b.mols), ptr(dev.mol_cand_off),
                                               ptr(dev.mol_pair_off), ptr(dev.pair_key),
                                               ptr(dev.pair_npf), ptr(qb), ptr(pbd),
-                                              float(threshold), float(fp32_cutoff), ptr(mask),
-                                              b.n_cand, ptr(counts), ptr(eris), ptr(primq), st),
-                  "gbx_quartets_count")
+                                              float(threshold), float(fp32_cutoff), ptr(mcut),
+                                              ptr(mask), b.n_cand, ptr(counts), ptr(eris),
+                                              ptr(primq), st), "gbx_quartets_count")
             cap = counts.cpu().numpy().astype(np.int64)
+            layout = ("stats", cap.tobytes())
         else:
-            # one pass: room for every unscreened candidate of each bucket (both
-            # precision variants when mixed), exact counts read back from the cursors
+            # room for every unscreened candidate of each bucket (both precision
+            # variants when mixed)
+            layout = ("cap", mixed)
             cap = b.bucket_capacity().copy()
-            if fp32_cutoff > 0:
+            if mixed:
                 cap = cap.reshape(-1, 2, 16)
                 cap[:, 1, :] = cap[:, 0, :]
                 cap = cap.reshape(-1)
-        offs = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
-        total = int(cap.sum())
-        key_cache = (id(b), total, fp32_cutoff > 0, stats)
-        if self._queue_cache.get("key") != key_cache:
-            self._queue_cache = {"key": key_cache,
-                                 "quart": torch.empty((max(total, 1), 2), dtype=torch.int32,
-                                                      device=self.device),
-                                 "offs": torch.from_numpy(offs).to(self.device)}
-        quart = self._queue_cache["quart"]
-        d_offs = self._queue_cache["offs"]
-        cursor = torch.zeros(nb, dtype=torch.int32, device=self.device)
+        if layout not in dev.queue:
+            offs = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
+            total = max(int(cap.sum()), 1)
+            if self.stream_workspaces:
+                quart = self.workspace("queue", 2 * total, torch.int32).view(total, 2)
+                d_offs = self.workspace("queue_offs", nb, torch.int64)
+                d_offs.copy_(torch.from_numpy(offs).pin_memory(), non_blocking=True)
+                cursor = self.workspace("queue_cursor", nb, torch.int32)
+            else:
+                quart = torch.empty((total, 2), dtype=torch.int32, device=self.device)
+                d_offs = _h2d(offs, self.device)
+                cursor = torch.zeros(nb, dtype=torch.int32, device=self.device)
+            dev.queue[layout] = {"cap": cap, "offs": offs, "quart": quart, "d_offs": d_offs,
+                                 "cursor": cursor}
+        qc = dev.queue[layout]
+        quart, d_offs, cursor, offs = qc["quart"], qc["d_offs"], qc["cursor"], qc["offs"]
+        cursor.zero_()
         check(self.lib.gbx_quartets_fill(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
                                          ptr(dev.pair_key), ptr(qb), ptr(pbd), float(threshold),
-                                         float(fp32_cutoff), ptr(mask), b.n_cand, ptr(d_offs),
-                                         ptr(cursor), ptr(quart), st), "gbx_quartets_fill")
-        ch = cursor.cpu().numpy().astype(np.int64) if not stats else cap
-        if stats:
-            cursor = torch.from_numpy(cap.astype(np.int32)).to(self.device)
+                                         float(fp32_cutoff), ptr(mcut), ptr(mask), b.n_cand,
+                                         ptr(d_offs), ptr(cursor), ptr(quart), st),
+              "gbx_quartets_fill")
+        if stats or device_counts:
+            ch = cap
+        else:
+            ch = cursor.cpu().numpy().astype(np.int64)
         eh = eris.cpu().numpy() if stats else None
         ph = primq.cpu().numpy() if stats else None
         plans = []
@@ -407,18 +505,54 @@ class EriEngine:
                 n = int(ch[ks].sum())
                 flops = float(ph[ks].sum()) * fpr + n * fco if stats else 0.0
                 eris_n = int(eh[ks].sum()) if stats else 0
-                seg = (d_offs[base:base + 16], cursor[base:base + 16], int(cap[base:base + 16].sum()))
+                seg = (d_offs[base:base + 16], cursor[base:base + 16],
+                       int(cap[base:base + 16].sum()))
                 plans.append(Bucket(cls, 1 << (ks[-1] & 15), prec, dense, quart, n, eris_n, flops,
-                                    seg=seg))
+                                    seg=seg, key=("head", head)))
                 continue
             for key in ks:
                 n = int(ch[key])
                 view = quart[offs[key]:offs[key] + n]
                 flops = float(ph[key]) * fpr + n * fco if stats else 0.0
                 plans.append(Bucket(cls, 1 << (key & 15), prec, dense, view, n,
-                                    int(eh[key]) if stats else 0, flops))
-        self._last_queue = quart
+                                    int(eh[key]) if stats else 0, flops,
+                                    count=cursor[key:key + 1] if device_counts else None,
+                                    key=("key", key)))
         return plans
+
+    def capture(self, dev: DeviceTables, bound=None, threshold=0.0, fp32_cutoff=0.0,
+                out=None, tuner=None):
+        """Record one whole step -- the work-queue fill and every ERI launch -- into a
+        HIP graph (torch.cuda.CUDAGraph).  Counts stay on the device, so the graph
+        is valid for every replay on the same tables; replaying it recomputes the
+        packed array ``out`` in place.  Returns (graph, out, plans)."""
+        if out is None:
+            out = torch.zeros(dev.batch.n_slots, dtype=torch.float64, device=self.device)
+        # eager warm-up: allocates the queue buffers and per-stream workspaces
+        plans = self.plan_device(dev, bound, threshold, fp32_cutoff, stats=False,
+                                 device_counts=True)
+        if tuner is not None:
+            tuner.apply(plans)
+        self.run(dev, plans, out)
+        torch.cuda.synchronize(self.device)
+        graph = torch.cuda.CUDAGraph()
+        side = torch.cuda.Stream(self.device)
+        side.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(side):
+            plans = self.plan_device(dev, bound, threshold, fp32_cutoff, stats=False,
+                                     device_counts=True)
+            if tuner is not None:
+                tuner.apply(plans)
+            self.run(dev, plans, out)
+        torch.cuda.synchronize(self.device)
+        with torch.cuda.graph(graph, stream=side):
+            plans = self.plan_device(dev, bound, threshold, fp32_cutoff, stats=False,
+                                     device_counts=True)
+            if tuner is not None:
+                tuner.apply(plans)
+            for bk in plans:
+                bk.path = self.launch_bucket(dev, bk, out)
+        return graph, out, plans
 
     @staticmethod
     def _orig_bounds(b, q_pair):

@@ -152,27 +152,31 @@ class MoleculeTables:
         coords = np.array([s.coord for s in self.shells])
         self.pair_geom = np.zeros((self.npair, 4))
         self.pair_geom[:, :3] = coords[sa] - coords[sb]
-        # ---- primitive-pair records ----
-        recs = np.zeros((int(self.pair_nprim.sum()), 10))
-        pos = 0
-        for a, b in zip(sa.tolist(), sb.tolist()):
-            A, B = self.shells[a], self.shells[b]
-            ea = A.exps[:, None]
-            eb = B.exps[None, :]
-            zeta = ea + eb
-            P = (ea[..., None] * A.coord + eb[..., None] * B.coord) / zeta[..., None]
-            ab2 = float(np.sum((A.coord - B.coord) ** 2))
-            ca = (A.coeffs * prim_norm(A.exps, A.l))[:, None]
-            cb = (B.coeffs * prim_norm(B.exps, B.l))[None, :]
-            K = ca * cb * np.exp(-(ea * eb / zeta) * ab2) * SQRT2_PI_5_4 / zeta
-            n = zeta.size
-            r = recs[pos:pos + n]
-            r[:, 0] = zeta.ravel()
-            r[:, 1] = (0.5 / zeta).ravel()
-            r[:, 2:5] = P.reshape(-1, 3)
-            r[:, 5:8] = (P - A.coord).reshape(-1, 3)
-            r[:, 8] = K.ravel()
-            pos += n
+        # ---- primitive-pair records, all pairs at once: record r of pair p is the
+        # primitive combination (ia, ib) = divmod(r - first record of p, K_b) ----
+        nprim_s = nprim.astype(np.int64)
+        sh_off = np.concatenate([[0], np.cumsum(nprim_s)[:-1]])
+        e_all = np.concatenate([s.exps for s in self.shells])
+        cn_all = np.concatenate([s.coeffs * prim_norm(s.exps, s.l) for s in self.shells])
+        npp = self.pair_nprim.astype(np.int64)
+        rp = np.repeat(np.arange(self.npair), npp)
+        t = np.arange(rp.size) - self.pair_prim_off[rp]
+        kb = nprim_s[sb][rp]
+        ga = sh_off[sa][rp] + t // kb
+        gb = sh_off[sb][rp] + t % kb
+        ea, eb = e_all[ga], e_all[gb]
+        A, B = coords[sa][rp], coords[sb][rp]
+        zeta = ea + eb
+        P = (ea[:, None] * A + eb[:, None] * B) / zeta[:, None]
+        d = coords[sa] - coords[sb]
+        ab2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])[rp]
+        K = cn_all[ga] * cn_all[gb] * np.exp(-(ea * eb / zeta) * ab2) * SQRT2_PI_5_4 / zeta
+        recs = np.zeros((rp.size, 10))
+        recs[:, 0] = zeta
+        recs[:, 1] = 0.5 / zeta
+        recs[:, 2:5] = P
+        recs[:, 5:8] = P - A
+        recs[:, 8] = K
         self.prim_pairs = recs
         self.n_slots = _n_slots(self.nfunc)
 
@@ -335,6 +339,11 @@ class BatchTables:
             np.add.at(cap, key, n)
         self._capacity = cap
         return cap
+
+    def schwarz_buckets(self):
+        """(bra == ket) quartet lists of the Schwarz pass, bucketed as class_buckets."""
+        p = np.arange(self.npair, dtype=np.int32)
+        return self.class_buckets(p, p)
 
     def unique_counts(self, bra, ket):
         """Number of canonical packed slots each (bra, ket) quartet writes."""

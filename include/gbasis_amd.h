@@ -35,7 +35,7 @@ extern "C" {
 #define GBX_MODE_SCHWARZ 1 /* per bra pair p: d_out[2p] = sqrt(max|(ij|ij)|),
                               d_out[2p+1] = sqrt(max|contracted [e0|f0]|)  */
 
-/* Device-resident basis tables (layouts documented in DESIGN.md). */
+/* Device-resident basis tables of one batch of molecules. */
 typedef struct gbx_tables {
   const int32_t *shell_info;     /* 8 x int32 per shell: func_off, l, kind, map_off,
                                     mat_off, nout, 0, 0 (kind 0 = scaled permutation,
@@ -95,11 +95,15 @@ int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t 
  * compute_schwarz_bound_shell_pair (gbasis/integrals/_schwarz_screening.py:14-59),
  * and d_out[2 bra + 1] = sqrt of the largest contracted input of the horizontal
  * recursion, the bound that selects the FP32 recursion bulk.
- * d_scratch (scratch_bytes) is only used when the class does not fit in LDS. */
+ * d_scratch (scratch_bytes) is only used when the class does not fit in LDS.
+ * d_count: NULL, or a device int32 holding the number of quartets to evaluate
+ * (at most nquart, which is then only the capacity the launch is sized for), so
+ * a launch can follow the device queue builder with no host round trip; the
+ * same parameter of gbx_eri_reg, gbx_eri_split and gbx_eri_hrr means the same. */
 int gbx_eri_coop(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb,
                  int32_t lc, int32_t ld, const int32_t *d_quartets, int64_t nquart,
-                 int32_t precision, int32_t prim_copies, int32_t mode, double *d_out,
-                 void *d_scratch, int64_t scratch_bytes, void *stream);
+                 const int32_t *d_count, int32_t precision, int32_t prim_copies, int32_t mode,
+                 double *d_out, void *d_scratch, int64_t scratch_bytes, void *stream);
 
 /* 1 if a generated register kernel (one lane per shell quartet) exists for the
  * class (kernel orientation), else 0. */
@@ -110,8 +114,8 @@ int gbx_has_reg_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld);
  * there), restricted to quartets whose four shells all have a scaled-
  * permutation output map (Cartesian shells, pure shells with l <= 1). */
 int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
-                const int32_t *d_quartets, int64_t nquart, int32_t precision, int32_t mode,
-                double *d_out, void *stream);
+                const int32_t *d_quartets, int64_t nquart, const int32_t *d_count,
+                int32_t precision, int32_t mode, double *d_out, void *stream);
 
 /* Work-queue construction on the device for a batch of molecules: the unique
  * shell-quartet loop of gbasis/base_four_symm.py:189-194 with the Schwarz test
@@ -119,7 +123,9 @@ int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t
  * kept iff bound_p * bound_q >= threshold (threshold <= 0 keeps all), run with
  * the FP32 recursion bulk iff pbound_p * pbound_q < fp32_cutoff (pbound: the
  * second value GBX_MODE_SCHWARZ writes per pair) and d_fp32_ok (NULL, or one
- * int8 per class*2+dense, class = ((la*8+lb)*8+lc)*8+ld) allows it.  Buckets
+ * int8 per class*2+dense, class = ((la*8+lb)*8+lc)*8+ld) allows it;
+ * d_mol_fp32_cutoff (NULL, or one cutoff per molecule) replaces fp32_cutoff
+ * per molecule, so FP64 and mixed-precision requests can share one batch.  Buckets
  * are keyed (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + ceil(log2 nprimq),
  * GBX_NUM_BUCKETS of them.  count: quartets per bucket (and, if d_bucket_eris is
  * not NULL, packed slots and primitive quartets per bucket).  fill: writes the
@@ -128,13 +134,15 @@ int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t
 int gbx_quartets_count(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
                        const int32_t *d_pair_key, const int32_t *d_pair_npf,
                        const double *d_pair_bound, const double *d_pair_pbound,
-                       double threshold, double fp32_cutoff, const int8_t *d_fp32_ok,
-                       int64_t n_cand, int32_t *d_bucket_count, uint64_t *d_bucket_eris,
+                       double threshold, double fp32_cutoff, const double *d_mol_fp32_cutoff,
+                       const int8_t *d_fp32_ok, int64_t n_cand, int32_t *d_bucket_count,
+                       uint64_t *d_bucket_eris,
                        uint64_t *d_bucket_primq, void *stream);
 int gbx_quartets_fill(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
                       const int32_t *d_pair_key, const double *d_pair_bound,
                       const double *d_pair_pbound, double threshold, double fp32_cutoff,
-                      const int8_t *d_fp32_ok, int64_t n_cand, const int64_t *d_bucket_offset,
+                      const double *d_mol_fp32_cutoff, const int8_t *d_fp32_ok, int64_t n_cand,
+                      const int64_t *d_bucket_offset,
                       int32_t *d_bucket_cursor, int32_t *d_quartets, void *stream);
 
 /* Segmented form of gbx_eri_reg: one grid of max_quartets lanes covers up to 16
@@ -159,8 +167,9 @@ int gbx_split_count(int32_t la, int32_t lb, int32_t lc, int32_t ld);
  * quartets r, r+R, ...; partial sums go to d_acc[(f * R + r) * acc_stride + q]
  * (f: accumulator index of the class program, q: quartet index). */
 int gbx_eri_split(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
-                  int32_t split, const int32_t *d_quartets, int64_t nquart, int32_t precision,
-                  int32_t lanes_per_quartet, double *d_acc, int64_t acc_stride, void *stream);
+                  int32_t split, const int32_t *d_quartets, int64_t nquart,
+                  const int32_t *d_count, int32_t precision, int32_t lanes_per_quartet,
+                  double *d_acc, int64_t acc_stride, void *stream);
 
 /* Horizontal recursion, norms, output transform and placement (or the Schwarz
  * bounds in GBX_MODE_SCHWARZ) from contracted [e0|f0] produced by the split
@@ -168,8 +177,9 @@ int gbx_eri_split(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32
  * gbx_eri_coop (gbasis/integrals/_two_elec_int_improved.py:364-588,
  * gbasis/base_four_symm.py:160-389). */
 int gbx_eri_hrr(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb, int32_t lc,
-                int32_t ld, const int32_t *d_quartets, int64_t nquart, int32_t mode,
-                const double *d_acc, int64_t acc_stride, int32_t acc_parts, double *d_out,
+                int32_t ld, const int32_t *d_quartets, int64_t nquart, const int32_t *d_count,
+                int32_t mode, const double *d_acc, int64_t acc_stride, int32_t acc_parts,
+                double *d_out,
                 void *d_scratch, int64_t scratch_bytes, void *stream);
 
 /* Expand the canonical 8-fold packed array of one molecule (nfunc output

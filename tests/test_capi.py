@@ -8,10 +8,20 @@ from gbasis_amd import _lib
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
-def _declared():
+def _header():
     text = open(os.path.join(ROOT, "include", "gbasis_amd.h")).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return sorted(set(re.findall(r"\b(gbx_[a-z_]+)\s*\(", text)))
+    return re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+
+
+def _declared():
+    return sorted(set(re.findall(r"\b(gbx_[a-z_]+)\s*\(", _header())))
+
+
+def test_bound_argument_counts_match_header():
+    """Every ctypes signature has as many arguments as the C prototype."""
+    for name, args in re.findall(r"\b(gbx_[a-z_]+)\s*\(([^)]*)\)", _header()):
+        n = 0 if args.strip() in ("", "void") else args.count(",") + 1
+        assert len(_lib.SIGNATURES[name][1]) == n, name
 
 
 def test_header_symbols_exported_and_bound():
@@ -27,7 +37,7 @@ def test_error_path_without_device():
     lib = _lib.load()
     assert lib.gbx_version() >= 1
     # argument validation happens before any device call
-    rc = lib.gbx_eri_coop(None, None, 0, 0, 0, 0, None, 0, 0, 1, 0, None, None, 0, None)
+    rc = lib.gbx_eri_coop(None, None, 0, 0, 0, 0, None, 0, None, 0, 1, 0, None, None, 0, None)
     assert rc == -1
     assert b"null" in lib.gbx_last_error()
     assert lib.gbx_has_reg_kernel(0, 0, 0, 0) == 1

@@ -201,3 +201,95 @@ def test_device_queue_matches_host(gpu):
                 map(tuple, bk.host_quartets().tolist()))
         assert dd == hh
         assert sum(bk.eris for bk in dplans) == sum(bk.eris for bk in hplans)
+
+
+def test_device_counts_and_graph_replay(gpu):
+    """Launches that read their counts on the device match host-count launches, and a
+    HIP graph of the whole step (queue fill + every launch) reproduces the eager
+    result on every replay, with eager work between the replays."""
+    import torch
+
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("device counts + graph")
+    eng = get_engine()
+    mols = [MoleculeTables(batch_molecule(s, n_atoms=n)) for s, n in ((11, 4), (12, 6), (13, 3))]
+    dev = DeviceTables(BatchTables(mols))
+    bound = eng.pair_bounds(dev, eng.schwarz(dev))
+    for cut in (0.0, 1e-2):
+        ref = eng.run(dev, eng.plan_device(dev, bound, 1e-12, cut, stats=False))
+        out = torch.zeros_like(ref)
+        eng.run(dev, eng.plan_device(dev, bound, 1e-12, cut, stats=False, device_counts=True),
+                out)
+        tol = (1e-13 if cut == 0 else 1e-8) * max(1.0, float(ref.abs().max()))
+        assert float((out - ref).abs().max()) <= tol
+        graph, gout, _ = eng.capture(dev, bound, 1e-12, cut)
+        eager = gout.clone()
+        for rep in range(3):
+            # the step rewrites every unscreened slot; screened slots keep their zeros
+            gout.zero_()
+            graph.replay()
+            torch.cuda.synchronize()
+            assert torch.equal(gout, eager), rep
+            ref.mul_(1.0)  # eager work between replays
+            torch.cuda.synchronize()
+
+
+def test_per_molecule_fp32_cutoff(gpu):
+    """FP64 and mixed molecules in one batch give what each gives alone."""
+    import torch
+
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("per-molecule cutoff")
+    eng = get_engine()
+    bases = [batch_molecule(21, n_atoms=5), batch_molecule(22, n_atoms=4)]
+    mols = [MoleculeTables(b) for b in bases]
+    dev = DeviceTables(BatchTables(mols), mol_fp32_cutoff=[0.0, 1e-2])
+    bound = eng.pair_bounds(dev, eng.schwarz(dev))
+    both = eng.run(dev, eng.plan_device(dev, bound, 1e-12, 0.0, stats=False))
+    offs = dev.batch.mol_slot_off
+    for i, cut in enumerate((0.0, 1e-2)):
+        d1 = DeviceTables(BatchTables([mols[i]]))
+        b1 = eng.pair_bounds(d1, eng.schwarz(d1))
+        alone = eng.run(d1, eng.plan_device(d1, b1, 1e-12, cut, stats=False))
+        part = both[offs[i]:offs[i] + mols[i].n_slots]
+        assert float((part - alone).abs().max()) <= 1e-13 * max(1.0, float(alone.abs().max()))
+    assert torch.isfinite(both).all()
+
+
+def test_serving_alone_equals_mix(gpu):
+    """A request's packed ERIs are bit-identical alone and batched with others, and
+    match the FP64 path (1e-12) or the mixed-precision bound (1e-8)."""
+    import torch
+
+    from gbasis_amd.electron_repulsion import electron_repulsion_integral_packed
+    from gbasis_amd.serving import EriServer
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("serving alone == mix")
+    srv = EriServer(n_streams=2, hbm_budget=8 << 30, threshold=1e-12, fp32_cutoff=1e-2)
+    specs = [(batch_molecule(900 + i, n_atoms=n), p) for i, (n, p) in
+             enumerate(((3, "fp64"), (5, "mixed"), (2, "fp64"), (6, "mixed"), (4, "mixed"),
+                        (4, "fp64")))]
+    alone = []
+    for basis, prec in specs:
+        r = srv.submit(basis, prec)
+        srv.drain()
+        alone.append(r.result.clone())
+        srv.release(r)
+    assert srv.used == 0
+    reqs = [srv.submit(b, p, dispatch=False) for b, p in specs]
+    srv.drain()
+    assert srv.batches_launched >= len(specs) + 1
+    for a, r, (basis, prec) in zip(alone, reqs, specs):
+        assert torch.equal(a, r.result), r.id
+        ref, _ = electron_repulsion_integral_packed(basis, schwarz_threshold=1e-12)
+        tol = (1e-12 if prec == "fp64" else 1e-8) * max(1.0, float(ref.abs().max()))
+        assert float((r.result - ref).abs().max()) <= tol
+        srv.release(r)
+    assert srv.used == 0
