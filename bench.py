@@ -237,10 +237,11 @@ def recorded_traffic(kernel):
     return rec["fetch_bytes"] + rec["write_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
 
 
-def roofline_entry(plans, per_bucket):
+def roofline_entry(plans, per_bucket, prefer_reg=True):
     """Roofline of the dominant kernel: the fused register kernel (one launch per step)
-    with the largest time; algorithmic FLOPs per launch / its average launch time,
-    against the vector peak of its precision.  Also the aggregate of all ERI kernels."""
+    with the largest time (any kernel when ``prefer_reg`` is off or none ran);
+    algorithmic FLOPs per launch / its average launch time, against the vector peak
+    of its precision.  Also the aggregate of all ERI kernels."""
     from gbasis_amd.roofline import PEAK_FP32_VECTOR, PEAK_FP64_VECTOR
 
     groups = {}
@@ -250,7 +251,7 @@ def roofline_entry(plans, per_bucket):
         g[0] += t
         g[1] += bk.flops
         g[2] += 1
-    fused = {k: v for k, v in groups.items() if k[0] == "reg"} or groups
+    fused = ({k: v for k, v in groups.items() if k[0] == "reg"} if prefer_reg else {}) or groups
     key, (t, f, nl) = max(fused.items(), key=lambda kv: kv[1][0])
     path, cls, prec = key
     peak = PEAK_FP64_VECTOR if prec == 0 else PEAK_FP32_VECTOR
@@ -268,30 +269,42 @@ def roofline_entry(plans, per_bucket):
 
 
 def cpu_baseline(batch, plans_fp64, budget_s=20.0):
-    """Oracle (NumPy restatement of the reference engine) timed on a stratified sample."""
+    """Oracle (NumPy restatement of the reference engine) timed on a stratified sample.
+
+    Classes are visited from the cheapest (algorithmic FLOPs per quartet) up; each
+    gets up to 8 random quartets within its share of the budget.  Classes left
+    when the budget runs out are extrapolated with the measured oracle seconds per
+    algorithmic FLOP of the sampled ones."""
     from oracle.assembly import contraction_block
 
     rng = np.random.default_rng(0)
-    # map global pair -> (molecule, shells)
     pair_mol = np.zeros(batch.npair, dtype=np.int64)
     for mi, m in enumerate(batch.mols):
         pair_mol[batch.mol_pair_off[mi]:batch.mol_pair_off[mi] + m.npair] = mi
-    total_time_est = 0.0
-    total_eris = 0
-    sampled = 0
-    spent = 0.0
     by_cls = {}
     for bk in plans_fp64:
-        by_cls.setdefault(bk.cls, []).append(bk)
-    per_cls_budget = budget_s / max(1, len(by_cls))
-    for cls, bks in sorted(by_cls.items()):
+        if bk.n > 0:
+            by_cls.setdefault(bk.cls, []).append(bk)
+    stats = []
+    for cls, bks in by_cls.items():
+        n = sum(bk.n for bk in bks)
+        stats.append((sum(bk.flops for bk in bks) / n, cls, bks, n))
+    stats.sort(key=lambda t: t[0])
+    per_cls_budget = budget_s / max(1, len(stats))
+    est = spent = 0.0
+    sampled = skipped = 0
+    s_time = s_flops = 0.0
+    pending_flops = 0.0
+    total_eris = sum(bk.eris for bk in plans_fp64)
+    for fpq, cls, bks, n_cls in stats:
+        if spent > budget_s:
+            pending_flops += fpq * n_cls
+            skipped += 1
+            continue
         quart = np.concatenate([bk.host_quartets() for bk in bks])
-        n_cls = quart.shape[0]
-        eris_cls = sum(bk.eris for bk in bks)
         times = []
-        t_cls = 0.0
         for _ in range(8):
-            q = quart[rng.integers(0, n_cls)]
+            q = quart[rng.integers(0, quart.shape[0])]
             mi = pair_mol[q[0]]
             m = batch.mols[mi]
             po = batch.mol_pair_off[mi]
@@ -300,20 +313,24 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
             shells = [_OracleShell(m.shells[i]) for i in sh]
             t0 = time.perf_counter()
             contraction_block(*shells)
-            dt = time.perf_counter() - t0
-            times.append(dt)
-            t_cls += dt
-            if t_cls > per_cls_budget:
+            times.append(time.perf_counter() - t0)
+            if sum(times) > per_cls_budget:
                 break
-        spent += t_cls
+        spent += sum(times)
         sampled += len(times)
-        total_time_est += n_cls * float(np.mean(times))
-        total_eris += eris_cls
-    return {"value": total_eris / total_time_est, "unit": "ERIs/s", "cores": 1, "kind": "port",
-            "sample": f"{sampled} shell quartets stratified over {len(by_cls)} classes "
-                      f"({spent:.1f} s of CPU), extrapolated by class counts; oracle = NumPy "
-                      "restatement of the reference OS+HGP engine, one process, OpenBLAS "
-                      f"threads={os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
+        est += n_cls * float(np.mean(times))
+        s_time += sum(times)
+        s_flops += fpq * len(times)
+    if pending_flops > 0:
+        est += pending_flops * s_time / s_flops
+    return {"value": total_eris / est, "unit": "ERIs/s", "cores": 1, "kind": "port",
+            "sample": f"{sampled} shell quartets over {len(stats) - skipped} of {len(stats)} "
+                      f"classes ({spent:.1f} s of CPU), extrapolated by class counts"
+                      + (f" and, for the {skipped} costliest classes, by the algorithmic FLOP "
+                         "model at the sampled oracle rate" if skipped else "")
+                      + "; oracle = NumPy restatement of the reference OS+HGP engine, one "
+                      "process, OpenBLAS threads="
+                      f"{os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
 
 
 class _OracleShell:
@@ -327,6 +344,198 @@ class _OracleShell:
         self.exps = es.exps
         self.coeffs = es.coeffs[:, None]
         self.angmom_components_cart = default_cart_components(es.l)
+
+
+CONFIG_WORKLOADS = {
+    # name: (description, Schwarz threshold, dense output)
+    "dense_small": ("4 atoms in a 4 Bohr cube, Cartesian s,s,p,d per atom (N=44), K=3, exps "
+                    "[0.1,50], no screening, dense chemist output", 0.0, True),
+    "large_spdf": ("16 atoms in a 10 Bohr cube, Cartesian s,p,d,f per atom (N=320), K=4, exps "
+                   "[0.08,2e3], Schwarz 1e-10, packed output", 1e-10, False),
+    "high_l_pure": ("6 atoms in a 6 Bohr cube, pure s,p,d,f,g per atom (N=150), K=8, exps "
+                    "[0.02,1e5], Schwarz 1e-12, packed output", 1e-12, False),
+}
+SAMPLES_FILE = os.path.join(ROOT, "tests", "golden", "config_samples.npz")
+
+
+def _packed_index(i, j, k, l):
+    i, j = np.maximum(i, j), np.minimum(i, j)
+    k, l = np.maximum(k, l), np.minimum(k, l)
+    ij = i * (i + 1) // 2 + j
+    kl = k * (k + 1) // 2 + l
+    ij, kl = np.maximum(ij, kl), np.minimum(ij, kl)
+    return ij * (ij + 1) // 2 + kl
+
+
+def _canon_cls(ls):
+    la, lb, lc, ld = ls
+    a, b = max(la, lb), min(la, lb)
+    c, d = max(lc, ld), min(lc, ld)
+    return max((a, b, c, d), (c, d, a, b))
+
+
+def sample_errors(name, mol, basis, q_orig, threshold, outs):
+    """Per-class errors of the GPU outputs ``outs`` {label: packed device array} at the
+    oracle's sampled quartets (tests/golden/config_samples.npz), against the
+    extended-precision oracle; also the reference algorithm's own FP64 error there."""
+    import torch
+
+    if not os.path.exists(SAMPLES_FILE):
+        return None
+    z = np.load(SAMPLES_FILE)
+    if f"{name}_idx" not in z:
+        return None
+    idx, shapes = z[f"{name}_idx"], z[f"{name}_shape"]
+    exact, ref64 = z[f"{name}_exact"], z[f"{name}_fp64"]
+    first = {}
+    for es in mol.shells:
+        first.setdefault(es.orig, es.func_off)
+    rows = {}
+    pos = 0
+    screened = 0
+    for q, shp in zip(idx.tolist(), shapes.tolist()):
+        n = int(np.prod(shp))
+        ex, rf = exact[pos:pos + n], ref64[pos:pos + n]
+        pos += n
+        if threshold > 0 and q_orig[q[0], q[1]] * q_orig[q[2], q[3]] < threshold:
+            screened += 1
+            continue
+        a, b, c, d = np.meshgrid(*[first[x] + np.arange(m) for x, m in zip(q, shp)],
+                                 indexing="ij")
+        slots = torch.from_numpy(_packed_index(a, b, c, d).ravel()).to("cuda")
+        cls = "".join(map(str, _canon_cls([basis[x].angmom for x in q])))
+        r = rows.setdefault(cls, {"quartets": 0, "max_abs": 0.0,
+                                  "ref_fp64_err": 0.0,
+                                  **{f"{k}_err": 0.0 for k in outs}})
+        r["quartets"] += 1
+        r["max_abs"] = max(r["max_abs"], float(np.abs(ex).max()))
+        r["ref_fp64_err"] = max(r["ref_fp64_err"], float(np.abs(rf - ex).max()))
+        for k, out in outs.items():
+            got = out[slots].cpu().numpy()
+            r[f"{k}_err"] = max(r[f"{k}_err"], float(np.abs(got - ex).max()))
+    summary = {"sampled_quartets": int(idx.shape[0]), "screened_on_gpu": screened,
+               "classes": len(rows)}
+    for k in list(outs) + ["ref_fp64"]:
+        summary[f"max_{k}_err"] = max((r[f"{k}_err"] for r in rows.values()), default=0.0)
+    return {"summary": summary, "per_class": rows}
+
+
+def per_class_mixed_error(eng, dev, plans64, plansmx, n_slots):
+    """Max / RMS of mixed - FP64 over the packed slots of each class (each class run
+    alone into zeroed buffers)."""
+    import torch
+
+    a = torch.zeros(n_slots, dtype=torch.float64, device="cuda")
+    b = torch.zeros_like(a)
+    out = {}
+    for cls in sorted({bk.cls for bk in plans64}):
+        a.zero_()
+        b.zero_()
+        for bk in plans64:
+            if bk.cls == cls:
+                eng.launch_bucket(dev, bk, a)
+        for bk in plansmx:
+            if bk.cls == cls:
+                eng.launch_bucket(dev, bk, b)
+        nz = a != 0
+        d = (b - a).abs_()
+        cnt = int(nz.sum())
+        out["".join(map(str, cls))] = {
+            "slots": cnt, "max_abs_err": float(d.max()),
+            "rms_err": float(torch.sqrt((d[nz] ** 2).mean())) if cnt else 0.0,
+            "fp32_quartets": sum(bk.n for bk in plansmx
+                                 if bk.cls == cls and bk.launch_precision() == 1)}
+    del a, b
+    return out
+
+
+def config_bench(args):
+    """One single-molecule workload: FP64 and mixed steps, per-class errors, roofline,
+    CPU baseline.  Prints one JSON line."""
+    import torch
+    from gbasis_amd import workloads
+    from gbasis_amd.autotune import Autotuner
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+
+    name = args.workload
+    desc, thr, dense = CONFIG_WORKLOADS[name]
+    basis = getattr(workloads, name)()
+    mol = MoleculeTables(basis)
+    batch = BatchTables([mol])
+    eng = get_engine("cuda")
+    dev = DeviceTables(batch)
+    q = eng.schwarz(dev)
+    bound = eng.pair_bounds(dev, q)
+    q_orig = mol.schwarz_orig(q[:, 0].cpu().numpy())
+    plans64 = eng.plan_device(dev, bound, thr, 0.0, stats=True)
+    eris = sum(bk.eris for bk in plans64)
+    nquart = sum(bk.n for bk in plans64)
+    log(f"{name}: N={mol.nfunc}, {nquart} shell quartets, {eris} ERIs, {mol.n_slots} slots")
+    tuner = None if args.no_tune else Autotuner(eng)
+    dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
+    t64, q64, _, pb64, last64 = time_steps(eng, dev, bound, thr, 0.0, args.steps, args.warmup,
+                                           tuner)
+    g64 = None if args.no_graph else time_graph(eng, dev, bound, thr, 0.0, args.steps, tuner)[0]
+    out64 = dev.out
+    log(f"{name} fp64: {t64 * 1e3:.2f} ms/step, {eris / t64:.3e} ERIs/s"
+        + ("" if g64 is None else f"; graph {g64 * 1e3:.2f} ms/step"))
+    unpack_ms = None
+    if dense:
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+        eng.unpack_dense(out64, mol.nfunc, "chemist")
+        ev[0].record()
+        for _ in range(args.steps):
+            d = eng.unpack_dense(out64, mol.nfunc, "chemist")
+        ev[1].record()
+        torch.cuda.synchronize()
+        unpack_ms = ev[0].elapsed_time(ev[1]) / args.steps
+        del d
+    dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
+    tmx, qmx, _, pbmx, lastmx = time_steps(eng, dev, bound, thr, args.fp32_cutoff, args.steps,
+                                           args.warmup, tuner)
+    gmx = (None if args.no_graph else
+           time_graph(eng, dev, bound, thr, args.fp32_cutoff, args.steps, tuner)[0])
+    outmx = dev.out
+    log(f"{name} mixed: {tmx * 1e3:.2f} ms/step, {eris / tmx:.3e} ERIs/s"
+        + ("" if gmx is None else f"; graph {gmx * 1e3:.2f} ms/step"))
+    plansmx = eng.plan_device(dev, bound, thr, args.fp32_cutoff, stats=True)
+    if tuner is not None:
+        tuner.apply(plansmx)
+        tuner.apply(plans64)
+    attach_stats(last64, plans64)
+    attach_stats(lastmx, plansmx)
+    diff = (outmx - out64).abs_()
+    max_err = float(diff.max())
+    rms_err = float(torch.linalg.vector_norm(diff)) / np.sqrt(diff.numel())
+    del diff
+    samples = sample_errors(name, mol, basis, q_orig, thr, {"gpu_fp64": out64, "gpu_mixed": outmx})
+    per_cls = per_class_mixed_error(eng, dev, plans64, plansmx, batch.n_slots)
+    t64h = g64 if g64 is not None else t64
+    tmxh = gmx if gmx is not None else tmx
+    result = {
+        "metric": METRIC, "value": eris / tmxh, "unit": "ERIs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": tmxh * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (Boys, prefactors, accumulation) + f32 (recursion bulk, low-bound quartets)",
+        "data": "synthetic",
+        "config": {"workload": desc, "name": name, "nfunc": mol.nfunc, "shell_quartets": nquart,
+                   "eris": eris, "packed_slots": mol.n_slots, "schwarz_threshold": thr,
+                   "fp32_cutoff": args.fp32_cutoff},
+        "step_timed_as": "hip graph replay" if gmx is not None else "eager launches",
+        "fp64": {"value": eris / t64h, "ms_per_step": t64h * 1e3},
+        "eager_ms_per_step": {"fp64": t64 * 1e3, "mixed": tmx * 1e3},
+        "dense_unpack_ms": unpack_ms,
+        "mixed_vs_fp64": {"max_abs_err": max_err, "rms_err": rms_err, "per_class": per_cls},
+        "oracle_samples": samples,
+        "roofline": roofline_entry(last64, pb64, prefer_reg=False),
+        "roofline_mixed": roofline_entry(lastmx, pbmx, prefer_reg=False),
+        "classes_fp64": class_table(last64, pb64),
+        "classes_mixed": class_table(lastmx, pbmx),
+    }
+    if not args.no_cpu:
+        result["cpu_baseline"] = cpu_baseline(batch, plans64)
+    print(json.dumps(result), flush=True)
 
 
 def main():
@@ -350,6 +559,9 @@ def main():
     ap.add_argument("--serving-saturation", type=int, default=300)
     ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--workload", choices=["batch64"] + list(CONFIG_WORKLOADS), default="batch64",
+                    help="batch64: the 64-molecule batch plus serving (default); the others "
+                         "are single-molecule workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py: this build runs the ERI path on one GPU only; --gpus must be 1",
@@ -359,6 +571,9 @@ def main():
     import torch
     from gbasis_amd.engine import DeviceTables, get_engine
 
+    if args.workload != "batch64":
+        config_bench(args)
+        return
     if args.copy_calibration_mb > 0:
         n = args.copy_calibration_mb * (1 << 20) // 8
         src = torch.ones(n, dtype=torch.float64, device="cuda")

@@ -29,6 +29,16 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "gbx_reg_
 REG_BUDGET = 220     # FP64 values a lane may hold (accumulators + live intermediates)
 SPLIT_BUDGET = 200   # same, per split of a large class
 
+# Inner (ket primitive) loop unroll of the fused kernels, per class.  Each lane's
+# primitive loop is a serial chain (record load -> setup -> Boys table load ->
+# recursion); unrolling puts two independent chains in flight per wave where the
+# extra registers do not cost occupancy.  Classes not listed: 1.
+INNER_UNROLL = {}
+if os.environ.get("GBX_INNER_UNROLL_ALL"):
+    _U = int(os.environ["GBX_INNER_UNROLL_ALL"])
+else:
+    _U = 0
+
 
 def _deps(p):
     """value -> list of source values of its recursion item (single-assignment program)."""
@@ -180,6 +190,9 @@ def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=False):
     else:
         w("  for (int ib = 0; ib < pb.w; ++ib) {")
         w("    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
+        u = _U or INNER_UNROLL.get(p.cls, 1)
+        if u > 1:
+            w(f"#pragma unroll {u}")
         w("    for (int ik = 0; ik < pk.w; ++ik) {")
     w("      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;")
     w("      PrimQuartet<TB> s;")

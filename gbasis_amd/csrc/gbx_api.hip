@@ -78,7 +78,7 @@ extern "C" const char* gbx_last_error(void) { return gbx::g_last_error.c_str(); 
 extern "C" int gbx_boys(const gbx_tables* t, const double* d_T, int64_t n, int32_t mmax,
                         double* d_out, void* stream) {
   if (!t || (n > 0 && (!d_T || !d_out))) return gbx::fail(GBX_ERR_ARG, "gbx_boys: null argument");
-  if (mmax < 0 || mmax + gbx::kTaylorTerms > t->boys_nord)
+  if (mmax < 0 || mmax + gbx::kTaylorTerms > t->boys_nord - 1)
     return gbx::fail(GBX_ERR_ARG, "gbx_boys: mmax exceeds the Boys table");
   if (n == 0) return GBX_OK;
   const int threads = 256;

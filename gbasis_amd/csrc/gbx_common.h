@@ -46,7 +46,9 @@ template <typename T>
 __device__ inline double prim_quartet_setup(const double* __restrict__ rb,
                                             const double* __restrict__ rk, PrimQuartet<T>& s) {
   const double zeta = rb[0], i2z = rb[1], eta = rk[0], i2e = rk[1];
-  const double inv_s = 1.0 / (zeta + eta);
+  // one reciprocal square root gives both 1/(zeta+eta) and its square root
+  const double rs = rsqrt(zeta + eta);
+  const double inv_s = rs * rs;
   const double rho = zeta * eta * inv_s;
   const double PQx = rb[2] - rk[2], PQy = rb[3] - rk[3], PQz = rb[4] - rk[4];
   const double roz = eta * inv_s;
@@ -68,7 +70,7 @@ __device__ inline double prim_quartet_setup(const double* __restrict__ rb,
   s.roe = static_cast<T>(roe);
   s.i2e = static_cast<T>(i2e);
   s.i2s = static_cast<T>(0.5 * inv_s);
-  s.pref = rb[8] * rk[8] * sqrt(inv_s);
+  s.pref = rb[8] * rk[8] * rs;
   return rho * (PQx * PQx + PQy * PQy + PQz * PQz);
 }
 
@@ -79,25 +81,44 @@ __constant__ static const double kInvOdd[32] = {
     1.0 / 33,  1.0 / 35, 1.0 / 37, 1.0 / 39, 1.0 / 41, 1.0 / 43, 1.0 / 45, 1.0 / 47,
     1.0 / 49,  1.0 / 51, 1.0 / 53, 1.0 / 55, 1.0 / 57, 1.0 / 59, 1.0 / 61, 1.0 / 63};
 
+// e^x for |x| <= 0.05 (half a Boys grid step): Taylor polynomial of degree 8,
+// truncation error below 6e-18 relative.
+__device__ inline double exp_small(double x) {
+  double p = 1.0 / 40320.0;
+  p = p * x + 1.0 / 5040.0;
+  p = p * x + 1.0 / 720.0;
+  p = p * x + 1.0 / 120.0;
+  p = p * x + 1.0 / 24.0;
+  p = p * x + 1.0 / 6.0;
+  p = p * x + 0.5;
+  p = p * x + 1.0;
+  return p * x + 1.0;
+}
+
+constexpr double kHalfSqrtPi = 0.88622692545275801365;  // sqrt(pi) / 2
+
 // Boys function F_m(T), m = 0..L, delivered through out(m, value).
 //
 // For T on the tabulated range [0, (nt-1) h]: F_L from an 8-term Taylor
 // expansion about the nearest grid point T_k,
 //     F_L(T) = sum_j F_{L+j}(T_k) (T_k - T)^j / j!      (dF_n/dT = -F_{n+1}),
 // then the downward recurrence F_m = (2T F_{m+1} + e^{-T}) / (2m+1), which is
-// stable.  Beyond the table, F_0 = sqrt(pi/T)/2 (erf(sqrt T) = 1 to double
-// precision for T > 36) and the upward recurrence
-// F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T), stable for T > m.
-// The table holds F_n(k h) for n < nord, computed on the host in extended
-// precision from the series F_n(T) = e^{-T} sum_i (2T)^i / ((2n+1)(2n+3)...(2n+2i+1)).
+// stable, with e^{-T} = e^{-T_k} e^{T_k - T} (tabulated e^{-T_k}, exp_small).
+// Beyond the table, F_0 = sqrt(pi/T)/2 (erf(sqrt T) = 1 to double precision
+// for T > 36) and the upward recurrence F_{m+1} = ((2m+1) F_m - e^{-T}) / (2T),
+// stable for T > m.  Row k of the table is e^{-k h}, then F_n(k h) for
+// n < nord - 1, computed on the host in extended precision from the series
+// F_n(T) = e^{-T} sum_i (2T)^i / ((2n+1)(2n+3)...(2n+2i+1)).
 template <typename Out>
 __device__ inline void boys_eval(double T, int L, const gbx_tables& t, Out out) {
   const double tmax = t.boys_h * (t.boys_nt - 1);
-  const double emT = exp(-T);
+  const double inv_h = 1.0 / t.boys_h;  // uniform: computed once per kernel
   if (T <= tmax) {
-    const int k = static_cast<int>(T / t.boys_h + 0.5);
+    const int k = static_cast<int>(T * inv_h + 0.5);
     const double x = k * t.boys_h - T;
-    const double* row = t.boys_table + static_cast<int64_t>(k) * t.boys_nord + L;
+    const double* row = t.boys_table + static_cast<int64_t>(k) * t.boys_nord;
+    const double emT = row[0] * exp_small(x);
+    row += 1 + L;
     // Horner form of sum_j row[j] x^j / j!
     double h = row[7];
     h = row[6] + x * h * (1.0 / 7.0);
@@ -115,8 +136,10 @@ __device__ inline void boys_eval(double T, int L, const gbx_tables& t, Out out) 
       out(m, fm);
     }
   } else {
-    const double inv2T = 0.5 / T;
-    double fm = 0.5 * sqrt(M_PI / T);
+    const double emT = exp(-T);
+    const double r = rsqrt(T);
+    const double inv2T = 0.5 * r * r;
+    double fm = kHalfSqrtPi * r;
     out(0, fm);
     for (int m = 0; m < L; ++m) {
       fm = (static_cast<double>(2 * m + 1) * fm - emT) * inv2T;

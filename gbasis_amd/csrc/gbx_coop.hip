@@ -404,7 +404,7 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
     return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be in [1, 64]");
   if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: class not in canonical orientation");
-  if (prog->header[6] != la + lb + lc + ld || prog->header[6] + kTaylorTerms > t->boys_nord)
+  if (prog->header[6] != la + lb + lc + ld || prog->header[6] + kTaylorTerms > t->boys_nord - 1)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: program / Boys table do not match the class");
   if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: bad mode");

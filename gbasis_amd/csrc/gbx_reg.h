@@ -52,11 +52,13 @@ __device__ inline int64_t lane_quartet(const RegArgs& a, int64_t gid) {
 template <int L>
 __device__ inline void boys_eval_fixed(double T, const gbx_tables& t, double* F) {
   const double tmax = t.boys_h * (t.boys_nt - 1);
-  const double emT = exp(-T);
+  const double inv_h = 1.0 / t.boys_h;  // uniform: computed once per kernel
   if (T <= tmax) {
-    const int k = static_cast<int>(T / t.boys_h + 0.5);
+    const int k = static_cast<int>(T * inv_h + 0.5);
     const double x = k * t.boys_h - T;
-    const double* row = t.boys_table + static_cast<int64_t>(k) * t.boys_nord + L;
+    const double* row = t.boys_table + static_cast<int64_t>(k) * t.boys_nord;
+    const double emT = L > 0 ? row[0] * exp_small(x) : 0.0;
+    row += 1 + L;
     double h = row[7];
     h = row[6] + x * h * (1.0 / 7.0);
     h = row[5] + x * h * (1.0 / 6.0);
@@ -69,8 +71,10 @@ __device__ inline void boys_eval_fixed(double T, const gbx_tables& t, double* F)
 #pragma unroll
     for (int m = L - 1; m >= 0; --m) F[m] = (twoT * F[m + 1] + emT) * (1.0 / (2 * m + 1));
   } else {
-    const double inv2T = 0.5 / T;
-    F[0] = 0.5 * sqrt(M_PI / T);
+    const double emT = L > 0 ? exp(-T) : 0.0;
+    const double r = rsqrt(T);
+    const double inv2T = 0.5 * r * r;
+    F[0] = kHalfSqrtPi * r;
 #pragma unroll
     for (int m = 0; m < L; ++m) F[m + 1] = (static_cast<double>(2 * m + 1) * F[m] - emT) * inv2T;
   }

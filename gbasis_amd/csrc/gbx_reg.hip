@@ -18,7 +18,7 @@ extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t 
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg: no register kernel for this class");
-  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
+  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
   gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0, 1, nullptr, nullptr, d_count};
@@ -39,7 +39,7 @@ extern "C" int gbx_eri_reg_segments(const gbx_tables* t, int32_t la, int32_t lb,
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg_segments: no register kernel for class");
-  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
+  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: Boys table too short for this class");
   if (max_quartets == 0) return GBX_OK;
   gbx::RegArgs a{d_quartets, max_quartets, mode, d_out, nullptr, 0, 1, d_seg_off, d_seg_cnt,
@@ -68,7 +68,7 @@ extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_
   if (acc_stride < nquart) return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: acc_stride < nquart");
   if (lanes_per_quartet < 1 || lanes_per_quartet > 4096)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: lanes_per_quartet must be in [1, 4096]");
-  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord)
+  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: Boys table too short for this class");
   if (nquart == 0) return GBX_OK;
   gbx::RegArgs a{d_quartets, nquart, GBX_MODE_PACKED, nullptr, d_acc, acc_stride,

@@ -70,6 +70,8 @@ def electron_repulsion_integral_packed(basis, coord_types=None, schwarz_threshol
     prefactors, accumulation and the horizontal recursion stay FP64).
     Returns ``(packed, nfunc)``.
     """
+    if precision not in ("fp64", "mixed"):
+        raise ValueError("`precision` must be one of 'fp64' or 'mixed'")
     _check_basis(basis)
     basis = list(basis)
     ctypes_ = _coord_types(basis, coord_types)

@@ -51,7 +51,8 @@ typedef struct gbx_tables {
                                     P(3), P-A(3), K', 0; K' = c_a c_b N_a N_b
                                     exp(-ab/zeta |AB|^2) sqrt(2) pi^(5/4) / zeta     */
   const double *f2norm;          /* 1/sqrt(prod (2a_i-1)!!) by l, then component     */
-  const double *boys_table;      /* boys_nt x boys_nord: F_n(k h)                    */
+  const double *boys_table;      /* boys_nt x boys_nord, row k: e^{-k h}, then
+                                    F_n(k h) for n = 0 .. boys_nord - 2              */
   int32_t boys_nt;
   int32_t boys_nord;
   double boys_h;

@@ -1,0 +1,92 @@
+"""Write tests/golden/config_samples.npz: oracle blocks of sampled shell quartets of
+the dense s/s/p/d (N=44), large s/p/d/f (N=320) and high-l pure s-g (N=150)
+workloads.
+
+For every canonical angular-momentum class present among the unique shell
+quartets (i >= j, k >= l, ij >= kl) of a workload, up to ``per_class`` quartets
+are drawn (seeded), and their final normalised blocks (norm_cont, component
+order, Cartesian->pure as the shells ask) are evaluated by the oracle twice:
+in extended precision with the bounded-coefficient ket build ("exact"), and
+with the reference algorithm in FP64 ("fp64").  The GPU tests evaluate the same
+quartets and compare.  Run once in a container with the oracle's
+dependencies; takes a few minutes on 8 cores.
+"""
+
+import os
+import sys
+from multiprocessing import Pool
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from gbasis_amd import workloads  # noqa: E402
+from oracle.assembly import normalized_block  # noqa: E402
+
+CONFIGS = {
+    # name: (basis builder, samples per class)
+    "dense_small": (workloads.dense_small, 3),
+    "large_spdf": (workloads.large_spdf, 2),
+    "high_l_pure": (workloads.high_l_pure, 1),
+}
+
+
+def _canon(ls):
+    la, lb, lc, ld = ls
+    a, b = max(la, lb), min(la, lb)
+    c, d = max(lc, ld), min(lc, ld)
+    return max((a, b, c, d), (c, d, a, b))
+
+
+def pick(basis, per_class, seed):
+    n = len(basis)
+    pairs = [(i, j) for i in range(n) for j in range(i + 1)]
+    by_cls = {}
+    for p, (i, j) in enumerate(pairs):
+        for k, l in pairs[:p + 1]:
+            key = _canon([basis[x].angmom for x in (i, j, k, l)])
+            by_cls.setdefault(key, []).append((i, j, k, l))
+    rng = np.random.default_rng(seed)
+    out = []
+    for key in sorted(by_cls):
+        lst = by_cls[key]
+        for t in rng.choice(len(lst), size=min(per_class, len(lst)), replace=False):
+            out.append(lst[int(t)])
+    return out
+
+
+def _eval(args):
+    name, q = args
+    basis = CONFIGS[name][0]()
+    cts = [s.coord_type for s in basis]
+    ex = normalized_block(basis, cts, *q, exact=True)
+    f64 = normalized_block(basis, cts, *q, exact=False)
+    return name, q, ex, f64
+
+
+def main():
+    jobs = []
+    for seed, (name, (builder, per)) in enumerate(CONFIGS.items()):
+        for q in pick(builder(), per, 100 + seed):
+            jobs.append((name, q))
+    # heaviest first
+    jobs.sort(key=lambda j: -sum(CONFIGS[j[0]][0]()[x].angmom for x in j[1]))
+    res = {}
+    with Pool(8) as pool:
+        for name, q, ex, f64 in pool.imap_unordered(_eval, jobs):
+            res.setdefault(name, []).append((q, ex, f64))
+            print(name, q, ex.shape, flush=True)
+    arrays = {}
+    for name, items in res.items():
+        items.sort(key=lambda t: t[0])
+        arrays[f"{name}_idx"] = np.array([t[0] for t in items], dtype=np.int32)
+        arrays[f"{name}_shape"] = np.array([t[1].shape for t in items], dtype=np.int32)
+        arrays[f"{name}_exact"] = np.concatenate([t[1].ravel() for t in items])
+        arrays[f"{name}_fp64"] = np.concatenate([t[2].ravel() for t in items])
+    np.savez_compressed(os.path.join(HERE, "config_samples.npz"), **arrays)
+
+
+if __name__ == "__main__":
+    main()

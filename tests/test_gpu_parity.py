@@ -293,3 +293,54 @@ def test_serving_alone_equals_mix(gpu):
         assert float((r.result - ref).abs().max()) <= tol
         srv.release(r)
     assert srv.used == 0
+
+
+def _sample_blocks(name):
+    z = np.load(os.path.join(HERE, "golden", "config_samples.npz"))
+    idx, shapes = z[f"{name}_idx"], z[f"{name}_shape"]
+    exact, ref64 = z[f"{name}_exact"], z[f"{name}_fp64"]
+    pos = 0
+    for q, shp in zip(idx.tolist(), shapes.tolist()):
+        n = int(np.prod(shp))
+        yield q, shp, exact[pos:pos + n].reshape(shp), ref64[pos:pos + n].reshape(shp)
+        pos += n
+
+
+@pytest.mark.parametrize("name", ["dense_small", "large_spdf", "high_l_pure"])
+def test_workload_samples_vs_oracle(gpu, name):
+    """Sampled shell quartets of the dense s/s/p/d, large s/p/d/f and high-l pure
+    s-g workloads (every class present) against the extended-precision oracle
+    (tests/golden/config_samples.npz, made by tests/golden/make_config_samples.py):
+    FP64 within 1e-12 of max(1, |block|) and no further from the reference
+    algorithm's FP64 value than that value is from the exact one; mixed precision
+    within the 1e-8 bound.  Each quartet runs as a molecule of its own shells."""
+    from gbasis_amd import workloads
+    from gbasis_amd.electron_repulsion import electron_repulsion_integral_packed
+    from oracle.assembly import basis_offsets, packed_index
+
+    _print_case(f"workload samples {name}")
+    basis = getattr(workloads, name)()
+    worst = {"fp64": 0.0, "mixed": 0.0}
+    for q, shp, exact, ref64 in _sample_blocks(name):
+        uniq = sorted(set(q))
+        sub = [basis[i] for i in uniq]
+        off = basis_offsets(sub, [s.coord_type for s in sub])
+        loc = [uniq.index(i) for i in q]
+        a, b, c, d = np.meshgrid(*[off[x] + np.arange(m) for x, m in zip(loc, shp)],
+                                 indexing="ij")
+        slots = np.vectorize(packed_index)(a, b, c, d)
+        scale = max(1.0, float(np.abs(exact).max()))
+        for prec in ("fp64", "mixed"):
+            packed, _ = electron_repulsion_integral_packed(sub, precision=prec,
+                                                           schwarz_threshold=0.0,
+                                                           fp32_cutoff=1e-2)
+            got = packed.cpu().numpy()[slots]
+            err = float(np.abs(got - exact).max())
+            worst[prec] = max(worst[prec], err / scale)
+            if prec == "fp64":
+                assert err <= 1e-12 * scale, (q, err, scale)
+                assert np.all(np.abs(got - ref64) <= np.abs(ref64 - exact) + 1e-12 * scale), q
+            else:
+                assert err <= 1e-8, (q, err)
+    print(f"{name}: worst relative FP64 error {worst['fp64']:.2e}, mixed {worst['mixed']:.2e}",
+          flush=True)
