@@ -34,10 +34,14 @@ SPLIT_BUDGET = 200   # same, per split of a large class
 # recursion); unrolling puts two independent chains in flight per wave where the
 # extra registers do not cost occupancy.  Classes not listed: 1.
 INNER_UNROLL = {}
-if os.environ.get("GBX_INNER_UNROLL_ALL"):
-    _U = int(os.environ["GBX_INNER_UNROLL_ALL"])
-else:
-    _U = 0
+_U = int(os.environ.get("GBX_INNER_UNROLL_ALL", "0"))
+
+# Fused kernels: Boys table columns of the class staged in LDS once per workgroup
+# (table reads become LDS reads instead of divergent vector-memory loads), and
+# the primitive loops ordered with the bra pair innermost (the lanes of a wave
+# mostly share the bra pair, so its record loads hit few distinct addresses).
+BOYS_LDS = os.environ.get("GBX_BOYS_LDS", "1") == "1"
+LOOP_SWAP = os.environ.get("GBX_LOOP_SWAP", "1") == "1"
 
 
 def _deps(p):
@@ -171,7 +175,8 @@ def _item_table(p):
     return table
 
 
-def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=False):
+def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=False, lds=False,
+                         swap=False):
     """Loop over bra x ket primitive pairs; straight-line recursion in ``order``;
     every final value is added to its FP64 accumulator as soon as it exists.
 
@@ -187,6 +192,14 @@ def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=False):
         w("    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;")
         w("    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
         w("    {")
+    elif swap:
+        w("  for (int ik = 0; ik < pk.w; ++ik) {")
+        w("    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;")
+        u = _U or INNER_UNROLL.get(p.cls, 1)
+        if u > 1:
+            w(f"#pragma unroll {u}")
+        w("    for (int ib = 0; ib < pb.w; ++ib) {")
+        w("      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
     else:
         w("  for (int ib = 0; ib < pb.w; ++ib) {")
         w("    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;")
@@ -194,11 +207,15 @@ def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=False):
         if u > 1:
             w(f"#pragma unroll {u}")
         w("    for (int ik = 0; ik < pk.w; ++ik) {")
-    w("      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;")
+    if not swap or strided:
+        w("      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;")
     w("      PrimQuartet<TB> s;")
     w("      const double T = prim_quartet_setup(rb, rk, s);")
     w(f"      double F[{L + 1}];")
-    w(f"      boys_eval_fixed<{L}>(T, t, F);")
+    if lds:
+        w(f"      boys_eval_lds<{L}>(T, t, boys_lds, F);")
+    else:
+        w(f"      boys_eval_fixed<{L}>(T, t, F);")
     used = set()
     for d in order:
         used.update(x for x in table[d][1][1:6] if x >= 0)
@@ -249,9 +266,14 @@ def _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, strided=False):
     w("  }")
 
 
-def _head(w, name, strided=False):
+def _head(w, name, strided=False, lds_L=None):
     w("template <typename TB>")
     w(f"__global__ __launch_bounds__(kRegThreads) void {name}(gbx_tables t, RegArgs a) {{")
+    if lds_L is not None:
+        # every thread of the workgroup helps stage the table before any lane leaves
+        w("  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];")
+        w(f"  boys_stage_lds<{lds_L}>(t, boys_lds);")
+        w("  __syncthreads();")
     if strided:
         # lane -> (r, q): consecutive lanes take consecutive quartets of one stride slot r
         w("  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;")
@@ -283,8 +305,8 @@ def kernel_source(cls):
     w = o.append
     w(f"// class ({la}{lb}|{lc}{ld}): {len(order)} recursion values, {p.nacc} accumulators, "
       f"{p.nout} outputs")
-    _head(w, name)
-    _emit_primitive_loop(w, p, order, acc_of_value, acc_ids)
+    _head(w, name, lds_L=p.L if BOYS_LDS else None)
+    _emit_primitive_loop(w, p, order, acc_of_value, acc_ids, lds=BOYS_LDS, swap=LOOP_SWAP)
     # largest contracted [e0|f0] (input of the horizontal recursion), for the
     # precision bound of GBX_MODE_SCHWARZ
     w("  double accmax = 0.0;")

@@ -80,6 +80,52 @@ __device__ inline void boys_eval_fixed(double T, const gbx_tables& t, double* F)
   }
 }
 
+// Boys table staged in LDS by the fused kernels: for the class's L, row k holds
+// e^{-k h}, F_L(k h) .. F_{L+7}(k h) and one pad (16-byte aligned rows).
+constexpr int kBoysRows = 401;       // T = 0, h, ..., 40 with h = 0.1
+constexpr int kBoysLdsStride = 10;
+
+template <int L>
+__device__ inline void boys_stage_lds(const gbx_tables& t, double* lds) {
+  for (int i = threadIdx.x; i < kBoysRows * 9; i += kRegThreads) {
+    const int k = i / 9, j = i - k * 9;
+    lds[k * kBoysLdsStride + j] = t.boys_table[static_cast<int64_t>(k) * t.boys_nord +
+                                               (j == 0 ? 0 : L + j)];
+  }
+}
+
+// boys_eval_fixed with the table rows read from the LDS copy.
+template <int L>
+__device__ inline void boys_eval_lds(double T, const gbx_tables& t, const double* lds,
+                                     double* F) {
+  const double tmax = t.boys_h * (kBoysRows - 1);
+  const double inv_h = 1.0 / t.boys_h;
+  if (T <= tmax) {
+    const int k = static_cast<int>(T * inv_h + 0.5);
+    const double x = k * t.boys_h - T;
+    const double* row = lds + k * kBoysLdsStride;
+    const double emT = L > 0 ? row[0] * exp_small(x) : 0.0;
+    double h = row[8];
+    h = row[7] + x * h * (1.0 / 7.0);
+    h = row[6] + x * h * (1.0 / 6.0);
+    h = row[5] + x * h * (1.0 / 5.0);
+    h = row[4] + x * h * (1.0 / 4.0);
+    h = row[3] + x * h * (1.0 / 3.0);
+    h = row[2] + x * h * (1.0 / 2.0);
+    F[L] = row[1] + x * h;
+    const double twoT = 2.0 * T;
+#pragma unroll
+    for (int m = L - 1; m >= 0; --m) F[m] = (twoT * F[m + 1] + emT) * (1.0 / (2 * m + 1));
+  } else {
+    const double emT = L > 0 ? exp(-T) : 0.0;
+    const double r = rsqrt(T);
+    const double inv2T = 0.5 * r * r;
+    F[0] = kHalfSqrtPi * r;
+#pragma unroll
+    for (int m = 0; m < L; ++m) F[m + 1] = (static_cast<double>(2 * m + 1) * F[m] - emT) * inv2T;
+  }
+}
+
 __device__ inline uint64_t tri_u64(uint64_t i, uint64_t j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }

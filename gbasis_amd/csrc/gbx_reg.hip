@@ -18,8 +18,8 @@ extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t 
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg: no register kernel for this class");
-  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1)
-    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table too short for this class");
+  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1 || t->boys_nt != gbx::kBoysRows)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table does not fit this class / kernel");
   if (nquart == 0) return GBX_OK;
   gbx::RegArgs a{d_quartets, nquart, mode, d_out, nullptr, 0, 1, nullptr, nullptr, d_count};
   return gbx::launch_reg_generated(t, la, lb, lc, ld, a, precision,
@@ -39,8 +39,8 @@ extern "C" int gbx_eri_reg_segments(const gbx_tables* t, int32_t la, int32_t lb,
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg_segments: no register kernel for class");
-  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1)
-    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: Boys table too short for this class");
+  if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1 || t->boys_nt != gbx::kBoysRows)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: Boys table does not fit this class");
   if (max_quartets == 0) return GBX_OK;
   gbx::RegArgs a{d_quartets, max_quartets, mode, d_out, nullptr, 0, 1, d_seg_off, d_seg_cnt,
                  nullptr};

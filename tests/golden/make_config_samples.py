@@ -5,11 +5,13 @@ workloads.
 For every canonical angular-momentum class present among the unique shell
 quartets (i >= j, k >= l, ij >= kl) of a workload, up to ``per_class`` quartets
 are drawn (seeded), and their final normalised blocks (norm_cont, component
-order, Cartesian->pure as the shells ask) are evaluated by the oracle twice:
-in extended precision with the bounded-coefficient ket build ("exact"), and
-with the reference algorithm in FP64 ("fp64").  The GPU tests evaluate the same
-quartets and compare.  Run once in a container with the oracle's
-dependencies; takes a few minutes on 8 cores.
+order, Cartesian->pure as the shells ask) are evaluated by the oracle three
+times: in extended precision with the bounded-coefficient ket build ("exact"),
+with the reference algorithm in FP64 ("fp64"), and with the bounded-coefficient
+ket build in plain FP64 ("kvrr64": the FP64 rounding floor of the recursions
+the kernels evaluate).  The GPU tests evaluate the same quartets and compare.
+Run once in a container with the oracle's dependencies; takes about twenty
+minutes on 8 cores.
 """
 
 import os
@@ -23,7 +25,9 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
 from gbasis_amd import workloads  # noqa: E402
+import oracle.assembly as assembly  # noqa: E402
 from oracle.assembly import normalized_block  # noqa: E402
+from oracle.os_hgp import os_hgp_quartet  # noqa: E402
 
 CONFIGS = {
     # name: (basis builder, samples per class)
@@ -63,7 +67,24 @@ def _eval(args):
     cts = [s.coord_type for s in basis]
     ex = normalized_block(basis, cts, *q, exact=True)
     f64 = normalized_block(basis, cts, *q, exact=False)
-    return name, q, ex, f64
+    k64 = _kvrr64_block(basis, cts, q)
+    return name, q, ex, f64, k64
+
+
+def _kvrr64_engine(s1, s2, s3, s4, exact=False):
+    return os_hgp_quartet(*[x for s in (s1, s2, s3, s4)
+                            for x in (s.coord, s.angmom, s.angmom_components_cart, s.exps,
+                                      s.coeffs)], ket="vrr")
+
+
+def _kvrr64_block(basis, cts, q):
+    """normalized_block with the oracle engine swapped for the FP64 two-centre build."""
+    saved = assembly._engine
+    assembly._engine = _kvrr64_engine
+    try:
+        return normalized_block(basis, cts, *q, exact=False)
+    finally:
+        assembly._engine = saved
 
 
 def main():
@@ -75,8 +96,8 @@ def main():
     jobs.sort(key=lambda j: -sum(CONFIGS[j[0]][0]()[x].angmom for x in j[1]))
     res = {}
     with Pool(8) as pool:
-        for name, q, ex, f64 in pool.imap_unordered(_eval, jobs):
-            res.setdefault(name, []).append((q, ex, f64))
+        for name, q, ex, f64, k64 in pool.imap_unordered(_eval, jobs):
+            res.setdefault(name, []).append((q, ex, f64, k64))
             print(name, q, ex.shape, flush=True)
     arrays = {}
     for name, items in res.items():
@@ -85,6 +106,7 @@ def main():
         arrays[f"{name}_shape"] = np.array([t[1].shape for t in items], dtype=np.int32)
         arrays[f"{name}_exact"] = np.concatenate([t[1].ravel() for t in items])
         arrays[f"{name}_fp64"] = np.concatenate([t[2].ravel() for t in items])
+        arrays[f"{name}_kvrr64"] = np.concatenate([t[3].ravel() for t in items])
     np.savez_compressed(os.path.join(HERE, "config_samples.npz"), **arrays)
 
 
