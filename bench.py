@@ -51,12 +51,14 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
     import torch
 
     stream = torch.cuda.current_stream()
-    for _ in range(warmup):
+    for w in range(warmup):
+        t0 = time.perf_counter()
         plans = eng.plan_device(dev, bound, threshold, cutoff, stats=False, device_counts=True)
         if tuner is not None:
-            tuner.tune(dev, plans, dev.out)
+            tuner.tune(dev, plans, dev.out, log=log)
         eng.run(dev, plans, dev.out)
-    torch.cuda.synchronize()
+        torch.cuda.synchronize()
+        log(f"  warm-up step {w + 1}/{warmup}: {time.perf_counter() - t0:.1f} s")
     per_bucket = None
     t_queue = 0.0
     t0 = time.perf_counter()
@@ -72,10 +74,18 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
             tuner.apply(plans)
         marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(plans) + 1)]
         marks[0].record(stream)
-        for i, bk in enumerate(plans):
-            bk.path = eng.launch_bucket(dev, bk, dev.out)
-            marks[i + 1].record(stream)
+        if eng.launch_streams > 1:
+            eng.run(dev, plans, dev.out)
+            for m in marks[1:]:
+                m.record(stream)
+        else:
+            for i, bk in enumerate(plans):
+                bk.path = eng.launch_bucket(dev, bk, dev.out)
+                marks[i + 1].record(stream)
         recs.append((q0, marks))
+        if time.perf_counter() - t0 > 20.0:
+            torch.cuda.synchronize()
+            log(f"  step {len(recs)}/{steps} done after {time.perf_counter() - t0:.0f} s")
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -84,8 +94,19 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
         t_queue += q0.elapsed_time(marks[0]) * 1e-3
         for i in range(len(plans)):
             per_bucket[i] += marks[i].elapsed_time(marks[i + 1]) * 1e-3
+    per_bucket /= steps
+    if eng.launch_streams > 1:
+        # per-class times from one more step with the launches in sequence (not in the total)
+        marks = [torch.cuda.Event(enable_timing=True) for _ in range(len(plans) + 1)]
+        marks[0].record(stream)
+        for i, bk in enumerate(plans):
+            eng.launch_bucket(dev, bk, dev.out)
+            marks[i + 1].record(stream)
+        torch.cuda.synchronize()
+        per_bucket = np.array([marks[i].elapsed_time(marks[i + 1]) * 1e-3
+                               for i in range(len(plans))])
     return (start.elapsed_time(stop) * 1e-3 / steps, t_queue / steps, wall / steps,
-            per_bucket / steps, plans)
+            per_bucket, plans)
 
 
 def time_graph(eng, dev, bound, threshold, cutoff, steps, tuner=None):
@@ -428,7 +449,13 @@ def per_class_mixed_error(eng, dev, plans64, plansmx, n_slots):
     a = torch.zeros(n_slots, dtype=torch.float64, device="cuda")
     b = torch.zeros_like(a)
     out = {}
+    fp32_classes = {bk.cls for bk in plansmx if bk.launch_precision() == 1 and bk.n > 0}
     for cls in sorted({bk.cls for bk in plans64}):
+        if cls not in fp32_classes:
+            # no quartet of the class runs the FP32 bulk: mixed == FP64 bit for bit
+            out["".join(map(str, cls))] = {"slots": None, "max_abs_err": 0.0, "rms_err": 0.0,
+                                           "fp32_quartets": 0}
+            continue
         a.zero_()
         b.zero_()
         for bk in plans64:
@@ -464,6 +491,7 @@ def config_bench(args):
     mol = MoleculeTables(basis)
     batch = BatchTables([mol])
     eng = get_engine("cuda")
+    eng.launch_streams = args.launch_streams
     dev = DeviceTables(batch)
     q = eng.schwarz(dev)
     bound = eng.pair_bounds(dev, q)
@@ -559,6 +587,9 @@ def main():
     ap.add_argument("--serving-saturation", type=int, default=300)
     ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--launch-streams", type=int, default=1,
+                    help="spread the class launches of a step over this many streams (the "
+                         "per-class timings then need --launch-streams 1)")
     ap.add_argument("--workload", choices=["batch64"] + list(CONFIG_WORKLOADS), default="batch64",
                     help="batch64: the 64-molecule batch plus serving (default); the others "
                          "are single-molecule workloads")
@@ -588,6 +619,7 @@ def main():
         f"{batch.n_slots} slots ({time.perf_counter() - t0:.1f} s)")
     eng = get_engine("cuda")
     eng.path = args.path
+    eng.launch_streams = args.launch_streams
     dev = DeviceTables(batch)
     q = eng.schwarz(dev)
     bound = eng.pair_bounds(dev, q)

@@ -44,8 +44,11 @@ SIGNATURES = {
     "gbx_last_error": (ctypes.c_char_p, []),
     "gbx_boys": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_vp, c_i64, c_i32, c_vp, c_vp]),
     "gbx_coop_workspace": (ctypes.c_int, [ctypes.POINTER(GbxProgram), c_i32, c_i32, c_i32, c_i32,
-                                          c_i32, c_i32, ctypes.POINTER(c_i64),
+                                          c_i32, c_i32, c_i32, ctypes.POINTER(c_i64),
                                           ctypes.POINTER(c_i64)]),
+    "gbx_eri_coop_acc": (ctypes.c_int, [ctypes.POINTER(GbxTables), ctypes.POINTER(GbxProgram),
+                                        c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i32,
+                                        c_i32, c_i32, c_vp, c_i64, c_vp, c_i64, c_vp]),
     "gbx_eri_coop": (ctypes.c_int, [ctypes.POINTER(GbxTables), ctypes.POINTER(GbxProgram), c_i32,
                                     c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32,
                                     c_vp, c_vp, c_i64, c_vp]),

@@ -46,16 +46,22 @@ class Autotuner:
         torch.cuda.synchronize()
         return a.elapsed_time(b) / self.repeats
 
-    def tune(self, dev, plans, out):
+    def tune(self, dev, plans, out, log=None):
         """Pick variants for every bucket of ``plans`` (timed on ``out``).
 
         Classes whose FP32-eligible buckets run faster in FP64 lose the FP32
         permission in the engine's queue mask, so their quartets join the FP64
         launch from the next queue build on.
         """
+        import time
+
         eng = self.eng
         fp32_win = dict(self.fp32_win)
-        for bk in plans:
+        t_last = time.perf_counter()
+        for ib, bk in enumerate(plans):
+            if log is not None and time.perf_counter() - t_last > 20.0:
+                log(f"  autotuner: {ib}/{len(plans)} buckets")
+                t_last = time.perf_counter()
             k = self.key(bk)
             if k in self.choice:
                 continue

@@ -52,6 +52,10 @@ struct CoopArgs {
                          //   acc_in[(f * acc_parts + r) * acc_stride + qi], summed over r
   int64_t acc_stride;
   int32_t acc_parts;
+  double* acc_out;       // non-null: primitive pass only; work item (qi, r) sums primitive
+                         //   quartets [r * chunk, (r + 1) * chunk) of quartet qi and writes
+                         //   acc_out[(f * parts + r) * acc_stride + qi]
+  int32_t parts;
 };
 
 struct CoopLayout {
@@ -60,7 +64,7 @@ struct CoopLayout {
 
 inline int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
 
-inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf) {
+inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf, bool prim_only = false) {
   const int64_t prim_slots = hdr[0], nacc = hdr[1], h_slots = hdr[2], nout = hdr[3];
   CoopLayout l{};
   l.sc_off = 0;
@@ -71,7 +75,8 @@ inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf) {
   l.f0_off = align16(l.h_off + h_slots * 8);
   l.f1_off = align16(l.f0_off + nout * 8);
   const int64_t epi_end = align16(l.f1_off + nout * 8);
-  l.total = std::max(align16(l.un_off + prim_bytes), epi_end);
+  l.total = prim_only ? align16(l.un_off + prim_bytes)
+                      : std::max(align16(l.un_off + prim_bytes), epi_end);
   return l;
 }
 
@@ -98,12 +103,18 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
   const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
 
   const int64_t nquart = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
-  for (int64_t qi = blockIdx.x; qi < nquart; qi += gridDim.x) {
+  const int parts = a.acc_out ? a.parts : 1;
+  for (int64_t wi = blockIdx.x; wi < nquart * parts; wi += gridDim.x) {
+    const int64_t qi = wi % nquart;
+    const int part = static_cast<int>(wi / nquart);
     const int bra = a.quart[2 * qi], ket = a.quart[2 * qi + 1];
     const int4 pb = pinfo[bra];
     const int4 pk = pinfo[ket];
     const int nk = pk.w;
     const int nprimq = pb.w * nk;
+    const int chunk = (nprimq + parts - 1) / parts;
+    const int pbeg = min(nprimq, part * chunk);
+    const int pend = min(nprimq, pbeg + chunk);
 
     __syncthreads();  // previous quartet's epilogue is done with the workspace
     if (a.acc_in) {
@@ -117,8 +128,8 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
       for (int f = tid; f < a.nacc; f += kCoopThreads) acc[f] = 0.0;
     }
 
-    for (int p0 = 0; p0 < (a.acc_in ? 0 : nprimq); p0 += PF) {
-      const int ncopy = min(PF, nprimq - p0);
+    for (int p0 = pbeg; p0 < (a.acc_in ? pbeg : pend); p0 += PF) {
+      const int ncopy = min(PF, pend - p0);
       __syncthreads();
       if (tid < ncopy) {
         const int p = p0 + tid;
@@ -188,6 +199,12 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
           acc[f] += sum;
         }
       }
+    }
+    if (a.acc_out) {
+      // primitive pass only: each thread wrote acc[f] for its own f
+      for (int f = tid; f < a.nacc; f += kCoopThreads)
+        a.acc_out[(static_cast<int64_t>(f) * parts + part) * a.acc_stride + qi] = acc[f];
+      continue;
     }
     __syncthreads();
 
@@ -327,8 +344,9 @@ int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, in
                 const int32_t* d_quartets, int64_t nquart, const int32_t* d_count, int pf,
                 int mode, double* d_out,
                 void* d_scratch, int64_t scratch_bytes, hipStream_t stream,
-                const double* acc_in = nullptr, int64_t acc_stride = 0, int acc_parts = 1) {
-  const CoopLayout l = coop_layout(prog->header, sizeof(TB), pf);
+                const double* acc_in = nullptr, int64_t acc_stride = 0, int acc_parts = 1,
+                double* acc_out = nullptr, int parts = 1) {
+  const CoopLayout l = coop_layout(prog->header, sizeof(TB), pf, acc_out != nullptr);
   CoopArgs a{};
   a.quart = d_quartets;
   a.nquart = nquart;
@@ -359,7 +377,9 @@ int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, in
   a.acc_in = acc_in;
   a.acc_stride = acc_stride;
   a.acc_parts = acc_parts;
-  int64_t grid = nquart;
+  a.acc_out = acc_out;
+  a.parts = parts;
+  int64_t grid = nquart * (acc_out ? parts : 1);
   size_t shmem = 0;
   if (kLDS) {
     shmem = static_cast<size_t>(l.total);
@@ -382,11 +402,13 @@ using namespace gbx;
 
 extern "C" int gbx_coop_workspace(const gbx_program* prog, int32_t la, int32_t lb, int32_t lc,
                                   int32_t ld, int32_t precision, int32_t prim_copies,
-                                  int64_t* lds_bytes, int64_t* global_bytes_per_block) {
+                                  int32_t prim_only, int64_t* lds_bytes,
+                                  int64_t* global_bytes_per_block) {
   if (!prog || prim_copies < 1 || prim_copies > kCoopThreads)
     return fail(GBX_ERR_ARG, "gbx_coop_workspace: prim_copies must be in [1, 64]");
   (void)la; (void)lb; (void)lc; (void)ld;
-  const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
+  const CoopLayout l =
+      coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies, prim_only != 0);
   const bool fits = l.total <= kCoopLdsLimit;
   if (lds_bytes) *lds_bytes = l.total;
   if (global_bytes_per_block) *global_bytes_per_block = fits ? 0 : l.total;
@@ -404,7 +426,8 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
     return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be in [1, 64]");
   if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: class not in canonical orientation");
-  if (prog->header[6] != la + lb + lc + ld || prog->header[6] + kTaylorTerms > t->boys_nord - 1)
+  if (prog->header[6] != la + lb + lc + ld ||
+      prog->header[6] + kTaylorTerms > t->boys_nord - 1)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: program / Boys table do not match the class");
   if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: bad mode");
@@ -450,4 +473,45 @@ extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t
   return launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count, 1, mode,
                                     d_out, d_scratch, scratch_bytes, s, d_acc, acc_stride,
                                     acc_parts);
+}
+
+extern "C" int gbx_eri_coop_acc(const gbx_tables* t, const gbx_program* prog, int32_t la,
+                                int32_t lb, int32_t lc, int32_t ld, const int32_t* d_quartets,
+                                int64_t nquart, const int32_t* d_count, int32_t precision,
+                                int32_t prim_copies, int32_t parts, double* d_acc,
+                                int64_t acc_stride, void* d_scratch, int64_t scratch_bytes,
+                                void* stream) {
+  if (!t || !prog || !d_acc || (nquart > 0 && !d_quartets))
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: null argument");
+  if (prim_copies < 1 || prim_copies > kCoopThreads)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: prim_copies must be in [1, 64]");
+  if (parts < 1 || parts > 4096)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: parts must be in [1, 4096]");
+  if (acc_stride < nquart) return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: acc_stride < nquart");
+  if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: class not in canonical orientation");
+  if (prog->header[6] != la + lb + lc + ld ||
+      prog->header[6] + kTaylorTerms > t->boys_nord - 1)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: program / Boys table do not match the class");
+  if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: bad precision");
+  if (nquart == 0) return GBX_OK;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const CoopLayout l =
+      coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies, true);
+  const bool lds = l.total <= kCoopLdsLimit;
+  if (precision == GBX_FP64)
+    return lds ? launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
+                                           prim_copies, GBX_MODE_PACKED, nullptr, d_scratch,
+                                           scratch_bytes, s, nullptr, acc_stride, 1, d_acc, parts)
+               : launch_coop<double, false>(t, prog, la, lb, lc, ld, d_quartets, nquart,
+                                            d_count, prim_copies, GBX_MODE_PACKED, nullptr,
+                                            d_scratch, scratch_bytes, s, nullptr, acc_stride, 1,
+                                            d_acc, parts);
+  return lds ? launch_coop<float, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
+                                        prim_copies, GBX_MODE_PACKED, nullptr, d_scratch,
+                                        scratch_bytes, s, nullptr, acc_stride, 1, d_acc, parts)
+             : launch_coop<float, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
+                                         prim_copies, GBX_MODE_PACKED, nullptr, d_scratch,
+                                         scratch_bytes, s, nullptr, acc_stride, 1, d_acc, parts);
 }

@@ -30,7 +30,7 @@ from .programs import class_program
 from .roofline import class_flops, quartet_flops
 from .tables import BatchTables, MoleculeTables, f2norm_table
 
-DEFAULT_SCRATCH_BYTES = 1 << 30
+DEFAULT_SCRATCH_BYTES = 2 << 30
 
 
 @dataclass
@@ -193,6 +193,8 @@ class EriEngine:
         self.scratch_bytes = int(scratch_bytes)
         self._scratch = {}
         self._ws = {}
+        self._streams = {}
+        self.launch_streams = 1
         # stream_workspaces: split accumulators and work-queue buffers come from
         # grow-only buffers per stream instead of fresh allocations (a server running
         # batches on several streams then allocates nothing per batch)
@@ -200,7 +202,13 @@ class EriEngine:
         self._reg = {}
         self._split = {}
         self._pf = {}
-        self.fp32_ok = None   # int8 device mask per class*2+dense (autotuner); None: all allowed
+        # int8 device mask per class*2+dense: classes allowed the FP32 recursion bulk.
+        # By default only classes with every l <= 2: their FP32 error against FP64 is
+        # measured within the 1e-8 bound at the default cutoff; f and g shells with wide
+        # exponent ranges are not (errors near 1e-7 seen), and the autotuner only ever
+        # narrows this set.
+        self.fp32_ok = None
+        self.set_fp32_allowed({})
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
         # deterministic: variant choices depend only on (class, primitive bin, precision),
         # never on how many quartets a launch holds, so the value of every packed slot
@@ -231,36 +239,54 @@ class EriEngine:
 
     LDS_TARGET = 32768   # bytes per workgroup: ~5 single-wave workgroups per CU
 
-    def _plan(self, cls, nprimq, precision):
+    def _workspace(self, cls, precision, pf, prim_only):
+        st, _, _ = self.programs.get(cls)
+        lds = ctypes.c_int64()
+        glb = ctypes.c_int64()
+        check(self.lib.gbx_coop_workspace(ctypes.byref(st), *cls, precision, pf, int(prim_only),
+                                          ctypes.byref(lds), ctypes.byref(glb)),
+              "gbx_coop_workspace")
+        return lds.value, glb.value
+
+    def _plan(self, cls, nprimq, precision, prim_only=False):
         """Primitive quartets in flight (PF) for the cooperative kernel.
 
         Largest power of two <= min(nprimq, 64) whose LDS workspace stays within
         LDS_TARGET; if even PF = 1 does not fit 64 KiB the kernel runs on global
-        scratch, where PF = min(nprimq, 16) keeps the wave busy.
+        scratch, where PF = min(nprimq, 16) keeps the wave busy (PF = 1 for the
+        split primitive pass, whose parallelism comes from the parts).
         """
-        if (cls, nprimq, precision) in self._pf:
-            return self._pf[(cls, nprimq, precision)]
-        st, _, _ = self.programs.get(cls)
-        lds = ctypes.c_int64()
-        glb = ctypes.c_int64()
-
-        def need(pf):
-            check(self.lib.gbx_coop_workspace(ctypes.byref(st), *cls, precision, pf,
-                                              ctypes.byref(lds), ctypes.byref(glb)),
-                  "gbx_coop_workspace")
-            return lds.value, glb.value
-
+        key = (cls, nprimq, precision, prim_only)
+        if key in self._pf:
+            return self._pf[key]
         pf = 1
         while pf < min(nprimq, 64):
             pf *= 2
-        while pf > 1 and need(pf)[0] > self.LDS_TARGET:
+        while pf > 1 and self._workspace(cls, precision, pf, prim_only)[0] > self.LDS_TARGET:
             pf //= 2
-        if need(pf)[1] > 0:
+        if self._workspace(cls, precision, pf, prim_only)[1] > 0:
             pf = 1
-            while pf < min(nprimq, 16):
+            while not prim_only and pf < min(nprimq, 16):
                 pf *= 2
-        self._pf[(cls, nprimq, precision)] = pf
+        self._pf[key] = pf
         return pf
+
+    COOP_TARGET_BLOCKS = 8192   # workgroups that keep every CU busy on the primitive pass
+    COOP_MIN_CHUNK = 16         # primitive quartets per part at least
+
+    def _coop_parts(self, nq, nprimq):
+        """Parts per shell quartet of the cooperative primitive pass: enough workgroups to
+        fill the GPU, at least COOP_MIN_CHUNK primitive quartets each.  In deterministic
+        mode the count depends on the primitive bin only."""
+        cap = max(1, nprimq // self.COOP_MIN_CHUNK)
+        R = 1
+        if self.deterministic:
+            while R * 2 <= min(cap, 16):
+                R *= 2
+            return R
+        while R * 2 <= min(cap, 256) and nq * R < self.COOP_TARGET_BLOCKS:
+            R *= 2
+        return R
 
     TARGET_LANES = 1 << 18   # lanes in flight that keep all 1024 SIMDs of the GPU busy
 
@@ -336,13 +362,34 @@ class EriEngine:
                                        ptr(out), ptr(sc), sc.numel(), self._stream()),
                   f"gbx_eri_hrr{cls}")
             return "split"
-        pf = self._plan(cls, nprimq, precision)
+        nq = quartets.shape[0]
+        R = self._coop_parts(nq, nprimq)
+        full_fits = self._workspace(cls, precision, 1, False)[1] == 0
         sc = self.scratch()
-        check(self.lib.gbx_eri_coop(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
-                                    ptr(quartets), quartets.shape[0], ptr(count), precision, pf,
-                                    mode, ptr(out), ptr(sc), sc.numel(), self._stream()),
-              f"gbx_eri_coop{cls}")
-        return "coop"
+        if R == 1 and full_fits:
+            pf = self._plan(cls, nprimq, precision)
+            check(self.lib.gbx_eri_coop(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+                                        ptr(quartets), nq, ptr(count), precision, pf, mode,
+                                        ptr(out), ptr(sc), sc.numel(), self._stream()),
+                  f"gbx_eri_coop{cls}")
+            return "coop"
+        # two passes: the primitive work split over R workgroups per quartet (primitive
+        # workspace only), then the horizontal recursion and epilogue
+        pf = self._plan(cls, max(1, -(-nprimq // R)), precision, prim_only=True)
+        nacc = st.header[1]
+        if self.stream_workspaces:
+            acc = self.workspace("acc", nacc * R * nq, torch.float64)
+        else:
+            acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
+        check(self.lib.gbx_eri_coop_acc(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+                                        ptr(quartets), nq, ptr(count), precision, pf, R,
+                                        ptr(acc), nq, ptr(sc), sc.numel(), self._stream()),
+              f"gbx_eri_coop_acc{cls}")
+        check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+                                   ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
+                                   ptr(out), ptr(sc), sc.numel(), self._stream()),
+              f"gbx_eri_hrr{cls}")
+        return "coop2"
 
     def schwarz(self, dev: DeviceTables):
         """Bounds per global shell pair, device tensor (npair, 2).
@@ -550,8 +597,9 @@ class EriEngine:
                                      device_counts=True)
             if tuner is not None:
                 tuner.apply(plans)
-            for bk in plans:
-                bk.path = self.launch_bucket(dev, bk, out)
+            # with launch_streams > 1 the side streams fork from and join back into the
+            # capturing stream through events, so the graph holds the parallel branches
+            self.run(dev, plans, out)
         return graph, out, plans
 
     @staticmethod
@@ -565,20 +613,54 @@ class EriEngine:
             out[po:po + m.npair] = Qo[oi, oj]
         return out
 
-    def run(self, dev: DeviceTables, plans, out=None):
-        """Execute the plans into a zero-initialised packed array (allocated if None)."""
+    def _side_streams(self, n):
+        key = torch.cuda.current_stream(self.device).cuda_stream
+        lst = self._streams.setdefault(key, [])
+        while len(lst) < n:
+            lst.append(torch.cuda.Stream(self.device))   # default priority
+        return lst[:n]
+
+    def run(self, dev: DeviceTables, plans, out=None, n_streams=None):
+        """Execute the plans into a zero-initialised packed array (allocated if None).
+
+        With ``n_streams`` > 1 (default: ``self.launch_streams``) the buckets are spread
+        round-robin over that many side streams forked from and joined back into the
+        current stream: every bucket writes its own packed slots, so small launches of
+        different classes overlap on the device instead of running one after another."""
         if out is None:
             out = torch.zeros(dev.batch.n_slots, dtype=torch.float64, device=self.device)
-        for bk in plans:
-            bk.path = self.launch_bucket(dev, bk, out)
+        ns = self.launch_streams if n_streams is None else n_streams
+        if ns <= 1 or len(plans) <= 1:
+            for bk in plans:
+                bk.path = self.launch_bucket(dev, bk, out)
+            return out
+        cur = torch.cuda.current_stream(self.device)
+        side = self._side_streams(ns)
+        fork = torch.cuda.Event()
+        fork.record(cur)
+        for st in side:
+            st.wait_event(fork)
+        for i, bk in enumerate(plans):
+            with torch.cuda.stream(side[i % ns]):
+                bk.path = self.launch_bucket(dev, bk, out)
+        for st in side:
+            join = torch.cuda.Event()
+            join.record(st)
+            cur.wait_event(join)
         return out
 
+    FP32_MAX_L = 2
+
     def set_fp32_allowed(self, allowed):
-        """allowed: {(cls, dense): bool}; classes not listed keep FP32 allowed."""
-        mask = np.ones(8 ** 4 * 2, dtype=np.int8)
+        """allowed: {(cls, dense): bool}; classes not listed keep the default (FP32 allowed
+        iff every l <= FP32_MAX_L).  A listed class is never allowed beyond the default."""
+        c = np.arange(8 ** 4)
+        lmax = np.maximum.reduce([(c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7])
+        mask = np.repeat((lmax <= self.FP32_MAX_L).astype(np.int8), 2)
         for (cls, dense), ok in allowed.items():
-            c = ((cls[0] * 8 + cls[1]) * 8 + cls[2]) * 8 + cls[3]
-            mask[c * 2 + int(dense)] = 1 if ok else 0
+            ci = ((cls[0] * 8 + cls[1]) * 8 + cls[2]) * 8 + cls[3]
+            if not ok:
+                mask[ci * 2 + int(dense)] = 0
         self.fp32_ok = torch.from_numpy(mask).to(self.device)
 
     def unpack_dense(self, packed, nfunc, notation):

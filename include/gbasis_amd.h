@@ -77,12 +77,13 @@ int gbx_boys(const gbx_tables *t, const double *d_T, int64_t n, int32_t mmax, do
              void *stream);
 
 /* Workspace of the cooperative kernel for one class program and prim_copies
- * (1..64) primitive quartets in flight: *lds_bytes = bytes per workgroup;
- * *global_bytes_per_block = the same when it exceeds 64 KiB (the kernel then
- * works in a slice of d_scratch per workgroup), else 0. */
+ * (1..64) primitive quartets in flight (prim_only != 0: of the primitive pass
+ * gbx_eri_coop_acc): *lds_bytes = bytes per workgroup; *global_bytes_per_block
+ * = the same when it exceeds 64 KiB (the kernel then works in a slice of
+ * d_scratch per workgroup), else 0. */
 int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t lc, int32_t ld,
-                       int32_t precision, int32_t prim_copies, int64_t *lds_bytes,
-                       int64_t *global_bytes_per_block);
+                       int32_t precision, int32_t prim_copies, int32_t prim_only,
+                       int64_t *lds_bytes, int64_t *global_bytes_per_block);
 
 /* One angular-momentum class of shell quartets with the cooperative
  * (one 64-lane workgroup per shell quartet) kernel.  d_quartets holds nquart (bra pair, ket pair)
@@ -105,6 +106,21 @@ int gbx_eri_coop(const gbx_tables *t, const gbx_program *prog, int32_t la, int32
                  int32_t lc, int32_t ld, const int32_t *d_quartets, int64_t nquart,
                  const int32_t *d_count, int32_t precision, int32_t prim_copies, int32_t mode,
                  double *d_out, void *d_scratch, int64_t scratch_bytes, void *stream);
+
+/* Primitive pass of the cooperative kernel, split over `parts` workgroups per
+ * shell quartet: work item (q, r) sums the primitive quartets
+ * [r c, (r + 1) c), c = ceil(nprimq / parts), of quartet q -- Boys, vertical
+ * recursion, ket build, FP64 accumulation (gbasis/integrals/_two_elec_int_improved.py:651-716)
+ * -- and writes the partial contracted [e0|f0] to d_acc[(f * parts + r) * acc_stride + q].
+ * gbx_eri_hrr with acc_parts = parts finishes the quartets.  Spreads the
+ * primitive work of classes with few quartets and many primitive quartets (f
+ * and g shells, K = 8) over the GPU, and needs only the primitive part of the
+ * workspace. */
+int gbx_eri_coop_acc(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb,
+                     int32_t lc, int32_t ld, const int32_t *d_quartets, int64_t nquart,
+                     const int32_t *d_count, int32_t precision, int32_t prim_copies,
+                     int32_t parts, double *d_acc, int64_t acc_stride, void *d_scratch,
+                     int64_t scratch_bytes, void *stream);
 
 /* 1 if a generated register kernel (one lane per shell quartet) exists for the
  * class (kernel orientation), else 0. */

@@ -218,7 +218,8 @@ def test_device_counts_and_graph_replay(gpu):
     mols = [MoleculeTables(batch_molecule(s, n_atoms=n)) for s, n in ((11, 4), (12, 6), (13, 3))]
     dev = DeviceTables(BatchTables(mols))
     bound = eng.pair_bounds(dev, eng.schwarz(dev))
-    for cut in (0.0, 1e-2):
+    for cut, streams in ((0.0, 1), (1e-2, 1), (0.0, 4)):
+        eng.launch_streams = streams
         ref = eng.run(dev, eng.plan_device(dev, bound, 1e-12, cut, stats=False))
         out = torch.zeros_like(ref)
         eng.run(dev, eng.plan_device(dev, bound, 1e-12, cut, stats=False, device_counts=True),
@@ -235,6 +236,7 @@ def test_device_counts_and_graph_replay(gpu):
             assert torch.equal(gout, eager), rep
             ref.mul_(1.0)  # eager work between replays
             torch.cuda.synchronize()
+    eng.launch_streams = 1
 
 
 def test_per_molecule_fp32_cutoff(gpu):
@@ -307,6 +309,56 @@ def _sample_blocks(name):
         pos += n
 
 
+def _eval_sampled_quartets(basis, quartets, precision, fp32_cutoff=1e-2):
+    """Packed array of the whole molecule in which only the given shell quartets
+    (original shell indices) were evaluated: each is oriented and bucketed as the
+    work queue does and launched per class; with mixed precision a quartet runs
+    the FP32 bulk iff the product of its pairs' precision bounds is below the
+    cutoff, as in the queue builder."""
+    import torch
+
+    from gbasis_amd._lib import GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+
+    eng = get_engine()
+    mol = MoleculeTables(basis)
+    batch = BatchTables([mol])
+    dev = DeviceTables(batch)
+    pair_of = {}
+    oi, oj = mol.pair_orig
+    for pidx, (i, j) in enumerate(zip(oi.tolist(), oj.tolist())):
+        pair_of[(max(i, j), min(i, j))] = pidx
+    key = (batch.pair_la + batch.pair_lb) * 16 + batch.pair_la
+    bra, ket = [], []
+    for i, j, k, l in quartets:
+        p, q = pair_of[(max(i, j), min(i, j))], pair_of[(max(k, l), min(k, l))]
+        if key[p] < key[q]:
+            p, q = q, p
+        bra.append(p)
+        ket.append(q)
+    bra, ket = np.array(bra), np.array(ket)
+    prec = np.full(bra.size, GBX_FP64)
+    if precision == "mixed":
+        pb = eng.schwarz(dev)[:, 1].cpu().numpy()
+        # the class mask of the queue builder: FP32 only where the engine allows it
+        mask = eng.fp32_ok.cpu().numpy()
+        cls = ((batch.pair_la[bra] * 8 + batch.pair_lb[bra]) * 8 + batch.pair_la[ket]) * 8 \
+            + batch.pair_lb[ket]
+        dense = (batch.pair_dense[bra] | batch.pair_dense[ket]).astype(np.int64)
+        ok = mask[cls * 2 + dense] == 1
+        prec[(pb[bra] * pb[ket] < fp32_cutoff) & ok] = GBX_FP32_BULK
+    out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
+    for pr in (GBX_FP64, GBX_FP32_BULK):
+        sel = prec == pr
+        if not sel.any():
+            continue
+        for cls, nprimq, dense, quart in batch.class_buckets(bra[sel], ket[sel]):
+            eng.launch_class(dev, cls, torch.from_numpy(quart).cuda(), nprimq, int(pr),
+                             GBX_MODE_PACKED, out, dense)
+    return out.cpu().numpy()
+
+
 @pytest.mark.parametrize("name", ["dense_small", "large_spdf", "high_l_pure"])
 def test_workload_samples_vs_oracle(gpu, name):
     """Sampled shell quartets of the dense s/s/p/d, large s/p/d/f and high-l pure
@@ -325,31 +377,27 @@ def test_workload_samples_vs_oracle(gpu, name):
     that value is from the exact one (plus the same bar, and the rounding of that
     difference: the reference's transfer recursion can be off by orders of
     magnitude here).  Mixed precision: within
-    the 1e-8 bound.  Each quartet runs as a molecule of its own shells."""
+    the 1e-8 bound.  Only the sampled quartets are evaluated."""
     from gbasis_amd import workloads
-    from gbasis_amd.electron_repulsion import electron_repulsion_integral_packed
     from oracle.assembly import basis_offsets, packed_index
 
     _print_case(f"workload samples {name}")
     basis = getattr(workloads, name)()
+    samples = list(_sample_blocks(name))
+    quartets = [s[0] for s in samples]
+    got_all = {prec: _eval_sampled_quartets(basis, quartets, prec) for prec in ("fp64", "mixed")}
+    off = basis_offsets(basis, [s.coord_type for s in basis])
     strict = floor_only = 0
     worst = {"fp64": 0.0, "mixed": 0.0}
-    for q, shp, exact, ref64, kv64 in _sample_blocks(name):
-        uniq = sorted(set(q))
-        sub = [basis[i] for i in uniq]
-        off = basis_offsets(sub, [s.coord_type for s in sub])
-        loc = [uniq.index(i) for i in q]
-        a, b, c, d = np.meshgrid(*[off[x] + np.arange(m) for x, m in zip(loc, shp)],
+    for q, shp, exact, ref64, kv64 in samples:
+        a, b, c, d = np.meshgrid(*[off[x] + np.arange(m) for x, m in zip(q, shp)],
                                  indexing="ij")
         slots = np.vectorize(packed_index)(a, b, c, d)
         scale = max(1.0, float(np.abs(exact).max()))
         floor = 32.0 * float(np.abs(kv64 - exact).max())
         bar = max(1e-12 * scale, floor)
         for prec in ("fp64", "mixed"):
-            packed, _ = electron_repulsion_integral_packed(sub, precision=prec,
-                                                           schwarz_threshold=0.0,
-                                                           fp32_cutoff=1e-2)
-            got = packed.cpu().numpy()[slots]
+            got = got_all[prec][slots]
             err = float(np.abs(got - exact).max())
             worst[prec] = max(worst[prec], err)
             if prec == "fp64":
