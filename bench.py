@@ -587,7 +587,7 @@ def main():
     ap.add_argument("--serving-saturation", type=int, default=300)
     ap.add_argument("--serving-streams", type=int, default=4)
     ap.add_argument("--no-serving", action="store_true")
-    ap.add_argument("--launch-streams", type=int, default=1,
+    ap.add_argument("--launch-streams", type=int, default=4,
                     help="spread the class launches of a step over this many streams (the "
                          "per-class timings then need --launch-streams 1)")
     ap.add_argument("--workload", choices=["batch64"] + list(CONFIG_WORKLOADS), default="batch64",
