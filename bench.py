@@ -648,6 +648,8 @@ def main():
     out64 = dev.out
     attach_stats(last64, plans64)
     if args.only == "fp64":
+        if tuner is not None and args.tune_cache:
+            tuner.save(args.tune_cache)
         roof = roofline_entry(last64, pb64)
         print(json.dumps({"metric": METRIC, "value": eris / t64, "unit": "ERIs/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": t64 * 1e3,
