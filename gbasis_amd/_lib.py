@@ -63,6 +63,9 @@ SIGNATURES = {
     "gbx_eri_hrr": (ctypes.c_int, [ctypes.POINTER(GbxTables), ctypes.POINTER(GbxProgram), c_i32,
                                    c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i32, c_vp, c_i64, c_i32,
                                    c_vp, c_vp, c_i64, c_vp]),
+    "gbx_has_hrr_kernel": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i32]),
+    "gbx_eri_hrr_reg": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32, c_i32,
+                                       c_vp, c_i64, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
     "gbx_quartets_count": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                           ctypes.c_double, ctypes.c_double, c_vp, c_vp, c_i64,
                                           c_vp, c_vp, c_vp, c_vp]),

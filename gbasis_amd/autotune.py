@@ -74,11 +74,14 @@ class Autotuner:
                 if not bk.dense and eng.has_reg(bk.cls):
                     variants = [None]
                 elif eng.split_count(bk.cls) > 0 and eng.path != "coop":
-                    variants = [("R", r) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq]
+                    hrrs = (0, 1) if (eng.has_hrr(bk.cls) and not bk.dense) else (0,)
+                    variants = [("R", r, h) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq
+                                for h in hrrs]
                 for var in variants:
                     def run(prec=prec, var=var):
                         eng.launch_bucket(dev, bk, out, precision=prec,
-                                          lanes=var[1] if var else None)
+                                          lanes=var[1] if var else None,
+                                          hrr=bool(var[2]) if var and len(var) > 2 else None)
                     t = self._time(run)
                     if prec == GBX_FP32_BULK:
                         # keep FP32 only for a clear win: an FP64 bucket also merges with

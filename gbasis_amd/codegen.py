@@ -19,9 +19,10 @@ from functools import lru_cache
 
 if __package__ in (None, ""):
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
-    from gbasis_amd.programs import (PH_HRRK, PH_KVRR, PH_VRR, class_program, comps)
+    from gbasis_amd.programs import (PH_HRRK, PH_KVRR, PH_VRR, add, class_program, comps,
+                                     direction, sub)
 else:
-    from .programs import PH_HRRK, PH_KVRR, PH_VRR, class_program, comps
+    from .programs import PH_HRRK, PH_KVRR, PH_VRR, add, class_program, comps, direction, sub
 
 OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "gbx_reg_gen.hip")
 
@@ -40,6 +41,10 @@ _U = int(os.environ.get("GBX_INNER_UNROLL_ALL", "0"))
 # (table reads become LDS reads instead of divergent vector-memory loads), and
 # the primitive loops ordered with the bra pair innermost (the lanes of a wave
 # mostly share the bra pair, so its record loads hit few distinct addresses).
+# Split classes whose register HRR kernel would spill (two ket recursion levels keep
+# too many values live per column): they keep the cooperative HRR pass.
+HRR_REG_EXCLUDE = {(2, 2, 2, 2)}
+
 BOYS_LDS = os.environ.get("GBX_BOYS_LDS", "1") == "1"
 LOOP_SWAP = os.environ.get("GBX_LOOP_SWAP", "1") == "1"
 
@@ -367,6 +372,81 @@ def split_kernel_source(cls, k):
     return "\n".join(o)
 
 
+def hrr_kernel_source(cls):
+    """Horizontal-recursion kernel of a split class (FP64, packed output, shells with a
+    scaled-permutation output map): one lane per shell quartet.  The lane sums the R
+    partial accumulators of the split kernels and, one ket component pair (c, d) at a
+    time, runs the HRR on D then B for that column in registers and stores the
+    column's (a, b) block, so the live set stays one column wide."""
+    la, lb, lc, ld = cls
+    p = class_program(*cls)
+    acc_index = {}
+    # accumulator numbering of the class program: final_e order (c, a)
+    f = 0
+    for nc in range(lc, lc + ld + 1):
+        for c in comps(nc):
+            for na in range(la, la + lb + 1):
+                for a in comps(na):
+                    acc_index[(a, (0, 0, 0), c, (0, 0, 0))] = f
+                    f += 1
+    assert f == p.nacc
+    nca, ncb, ncc, ncd = (len(comps(x)) for x in cls)
+    name = f"eri_hrr_{la}{lb}{lc}{ld}"
+    o = []
+    w = o.append
+    w(f"// class ({la}{lb}|{lc}{ld}): horizontal recursion per ket column, {p.nacc} accumulators")
+    w(f"__global__ __launch_bounds__(kRegThreads) void {name}(gbx_tables t, RegArgs a) {{")
+    w("  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + "
+      "threadIdx.x);")
+    w("  if (q < 0) return;")
+    w("  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];")
+    w("  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];")
+    w("  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];")
+    w("  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], "
+      "t.pair_geom[4 * bra + 2]};")
+    w("  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], "
+      "t.pair_geom[4 * ket + 2]};")
+    w(f"  ColumnStore<{nca}, {ncb}, {ncc}, {ncd}> st(t, a, bra, ket, pb, pk);")
+    Z = (0, 0, 0)
+    for ic, c in enumerate(comps(lc)):
+        for id_, d in enumerate(comps(ld)):
+            # depth-first post-order from each output of the column: every value is
+            # computed right before its first use, accumulators are loaded there, and
+            # each output is stored as soon as it exists
+            names = {}
+            w("  {")
+            w(f"    const ColumnSlot cl = st.column({ic}, {id_});")
+
+            def emit(g):
+                if g in names:
+                    return names[g]
+                aa, bb, cc, dd = g
+                if bb != Z:
+                    i = direction(bb)
+                    x, y = emit((add(aa, i), sub(bb, i), cc, dd)), emit((aa, sub(bb, i), cc, dd))
+                    expr = f"{x} + AB[{i}] * {y}"
+                elif dd != Z:
+                    i = direction(dd)
+                    x, y = emit((aa, bb, add(cc, i), sub(dd, i))), emit((aa, bb, cc, sub(dd, i)))
+                    expr = f"{x} + CD[{i}] * {y}"
+                else:
+                    expr = f"sum_parts(a, {acc_index[g]}, q)"
+                nm = f"v{len(names)}"
+                names[g] = nm
+                w(f"    const double {nm} = {expr};")
+                return nm
+
+            for ia, aa in enumerate(comps(la)):
+                for ib, bb in enumerate(comps(lb)):
+                    v = emit((aa, bb, c, d))
+                    f2 = _f2(aa) * _f2(bb) * _f2(c) * _f2(d)
+                    val = v if abs(f2 - 1.0) < 1e-300 else f"{v} * {f2!r}"
+                    w(f"    st.put(cl, {ia}, {ib}, {val});")
+            w("  }")
+    w("}")
+    return "\n".join(o)
+
+
 def generate(path=OUT):
     classes = _reg_classes()
     parts = [
@@ -387,6 +467,8 @@ def generate(path=OUT):
             split.append((cls, len(pl)))
             for k in range(len(pl)):
                 parts.append(split_kernel_source(cls, k))
+            if cls not in HRR_REG_EXCLUDE:
+                parts.append(hrr_kernel_source(cls))
         parts.append("")
 
     def key(c):
@@ -426,6 +508,32 @@ def generate(path=OUT):
             parts.append(f'      return check_launch("{name}");')
     parts.append("    default:")
     parts.append('      return fail(GBX_ERR_UNSUPPORTED, "no split register kernel for this class");')
+    parts.append("  }")
+    parts.append("}")
+    parts.append("")
+    parts.append("int launch_hrr_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
+                 "const RegArgs& a, hipStream_t s) {")
+    parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
+                 "kRegThreads);")
+    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
+    hrr = [cls for cls, _ in split if cls not in HRR_REG_EXCLUDE]
+    for cls in hrr:
+        name = f"eri_hrr_{''.join(map(str, cls))}"
+        parts.append(f"    case {key(cls)}:")
+        parts.append(f"      hipLaunchKernelGGL({name}, dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
+        parts.append(f'      return check_launch("{name}");')
+    parts.append("    default:")
+    parts.append('      return fail(GBX_ERR_UNSUPPORTED, "no register HRR kernel for this class");')
+    parts.append("  }")
+    parts.append("}")
+    parts.append("")
+    parts.append("bool has_hrr_generated(int la, int lb, int lc, int ld) {")
+    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
+    for cls in hrr:
+        parts.append(f"    case {key(cls)}:")
+    parts.append("      return true;")
+    parts.append("    default:")
+    parts.append("      return false;")
     parts.append("  }")
     parts.append("}")
     parts.append("")

@@ -192,8 +192,78 @@ __device__ inline void store_block(const gbx_tables& t, const RegArgs& a, int br
     }
 }
 
+// Sum of the R partial accumulators f of quartet q written by the split kernels
+// (same order as the cooperative horizontal-recursion pass: r = 0, 1, ...).
+__device__ __forceinline__ double sum_parts(const RegArgs& a, int f, int64_t q) {
+  const int R = a.lanes_per_quartet;
+  const double* p = a.acc + static_cast<int64_t>(f) * R * a.acc_stride + q;
+  double s = 0.0;
+  for (int r = 0; r < R; ++r) s += p[static_cast<int64_t>(r) * a.acc_stride];
+  return s;
+}
+
+// Ket part of the placement of one column (fixed c, d components).
+struct ColumnSlot {
+  uint64_t kl;
+  double cc, cd;
+  bool skip;
+};
+
+// Placement of one ket column (fixed c, d components) of a canonical-order block,
+// element by element: the same maps, scales and duplicate rules as store_block.
+template <int NA, int NB, int NC, int ND>
+struct ColumnStore {
+  const int32_t* cmap;   // table pointers held by value (a reference to the kernel
+  const double* cscale;  // argument would force a private copy of it)
+  double* base;
+  int4 sc, sd;
+  bool bdiag, kdiag, same;
+  uint32_t fa[NA], fb[NB];
+  double ca[NA], cb[NB];
+
+  __device__ __forceinline__ ColumnStore(const gbx_tables& t, const RegArgs& a, int bra, int ket,
+                                         int4 pb, int4 pk)
+      : cmap(t.comp_map), cscale(t.comp_scale) {
+    const int4* si = reinterpret_cast<const int4*>(t.shell_info);
+    const int4 sa = si[2 * pb.x], sb = si[2 * pb.y];
+    sc = si[2 * pk.x];
+    sd = si[2 * pk.y];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) {
+      fa[i] = static_cast<uint32_t>(sa.x + cmap[sa.w + i]);
+      ca[i] = cscale[sa.w + i];
+    }
+#pragma unroll
+    for (int i = 0; i < NB; ++i) {
+      fb[i] = static_cast<uint32_t>(sb.x + cmap[sb.w + i]);
+      cb[i] = cscale[sb.w + i];
+    }
+    bdiag = pb.x == pb.y;
+    kdiag = pk.x == pk.y;
+    same = bra == ket;
+    base = a.out + t.pair_out_base[bra];
+  }
+
+  __device__ __forceinline__ ColumnSlot column(int ic, int id) const {
+    const uint32_t fc = static_cast<uint32_t>(sc.x + cmap[sc.w + ic]);
+    const uint32_t fd = static_cast<uint32_t>(sd.x + cmap[sd.w + id]);
+    return ColumnSlot{tri_u64(fc, fd), cscale[sc.w + ic], cscale[sd.w + id],
+                      kdiag && fc < fd};
+  }
+
+  __device__ __forceinline__ void put(const ColumnSlot& cl, int ia, int ib, double v) const {
+    if (cl.skip || (bdiag && fa[ia] < fb[ib])) return;
+    const uint64_t ij = tri_u64(fa[ia], fb[ib]);
+    if (same && ij < cl.kl) return;
+    base[tri_u64(ij, cl.kl)] = v * ca[ia] * cb[ib] * cl.cc * cl.cd;
+  }
+};
+
 int launch_reg_generated(const gbx_tables* t, int la, int lb, int lc, int ld, const RegArgs& a,
                          int precision, hipStream_t s);
+int launch_hrr_generated(const gbx_tables* t, int la, int lb, int lc, int ld, const RegArgs& a,
+                         hipStream_t s);
+bool has_hrr_generated(int la, int lb, int lc, int ld);
 int launch_split_generated(const gbx_tables* t, int la, int lb, int lc, int ld, int k,
                            const RegArgs& a, int precision, hipStream_t s);
 bool has_reg_generated(int la, int lb, int lc, int ld);

@@ -76,3 +76,24 @@ extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_
   return gbx::launch_split_generated(t, la, lb, lc, ld, split, a, precision,
                                      static_cast<hipStream_t>(stream));
 }
+
+extern "C" int gbx_has_hrr_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld) {
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > 7 || lb > 7 || lc > 7 || ld > 7) return 0;
+  return gbx::has_hrr_generated(la, lb, lc, ld) ? 1 : 0;
+}
+
+extern "C" int gbx_eri_hrr_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t lc,
+                               int32_t ld, const int32_t* d_quartets, int64_t nquart,
+                               const int32_t* d_count, const double* d_acc, int64_t acc_stride,
+                               int32_t acc_parts, double* d_out, void* stream) {
+  if (!t || !d_out || !d_acc || (nquart > 0 && !d_quartets))
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_hrr_reg: null argument");
+  if (gbx_has_hrr_kernel(la, lb, lc, ld) == 0)
+    return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_hrr_reg: no register HRR kernel for class");
+  if (acc_stride < nquart || acc_parts < 1 || acc_parts > 4096)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_hrr_reg: acc_stride < nquart or bad acc_parts");
+  if (nquart == 0) return GBX_OK;
+  gbx::RegArgs a{d_quartets, nquart, GBX_MODE_PACKED, d_out, const_cast<double*>(d_acc),
+                 acc_stride, acc_parts, nullptr, nullptr, d_count};
+  return gbx::launch_hrr_generated(t, la, lb, lc, ld, a, static_cast<hipStream_t>(stream));
+}

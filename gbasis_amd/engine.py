@@ -58,6 +58,13 @@ class Bucket:
     def lanes(self):
         return self.variant[1] if self.variant and self.variant[0] == "R" else None
 
+    def hrr(self):
+        """Autotuned finishing pass of a split class: True register HRR kernel, False the
+        cooperative pass, None the engine default."""
+        if self.variant and self.variant[0] == "R" and len(self.variant) > 2:
+            return bool(self.variant[2])
+        return None
+
     def host_quartets(self):
         """(n, 2) int32 NumPy copy of this bucket's quartets (gathers the segments of a
         segmented bucket)."""
@@ -195,11 +202,15 @@ class EriEngine:
         self._ws = {}
         self._streams = {}
         self.launch_streams = 1
+        # default finishing pass of the split classes (the autotuner picks per bucket):
+        # False = the cooperative HRR pass, True = the register HRR kernels
+        self.hrr_reg = False
         # stream_workspaces: split accumulators and work-queue buffers come from
         # grow-only buffers per stream instead of fresh allocations (a server running
         # batches on several streams then allocates nothing per batch)
         self.stream_workspaces = False
         self._reg = {}
+        self._hrr = {}
         self._split = {}
         self._pf = {}
         # int8 device mask per class*2+dense: classes allowed the FP32 recursion bulk.
@@ -310,19 +321,25 @@ class EriEngine:
             self._split[cls] = int(self.lib.gbx_split_count(*cls))
         return self._split[cls]
 
+    def has_hrr(self, cls):
+        if cls not in self._hrr:
+            self._hrr[cls] = bool(self.lib.gbx_has_hrr_kernel(*cls))
+        return self._hrr[cls]
+
     def has_reg(self, cls):
         if cls not in self._reg:
             self._reg[cls] = bool(self.lib.gbx_has_reg_kernel(*cls))
         return self._reg[cls]
 
-    def launch_bucket(self, dev, bk, out, mode=GBX_MODE_PACKED, precision=None, lanes=None):
+    def launch_bucket(self, dev, bk, out, mode=GBX_MODE_PACKED, precision=None, lanes=None,
+                      hrr=None):
         prec = bk.launch_precision() if precision is None else precision
         return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
                                  self.path, lanes if lanes is not None else bk.lanes(), bk.seg,
-                                 bk.count)
+                                 bk.count, hrr if hrr is not None else bk.hrr())
 
     def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,
-                     dense=False, path="auto", lanes=None, seg=None, count=None):
+                     dense=False, path="auto", lanes=None, seg=None, count=None, hrr=None):
         """One launch for a (class, bin) bucket; quartets is a device int32 (n,2).
 
         Classes with a generated register kernel and no dense output transform run
@@ -356,6 +373,12 @@ class EriEngine:
                 check(self.lib.gbx_eri_split(ctypes.byref(dev.struct), *cls, k, ptr(quartets), nq,
                                              ptr(count), precision, R, ptr(acc), nq,
                                              self._stream()), f"gbx_eri_split{cls}[{k}]")
+            use_reg = self.hrr_reg if hrr is None else hrr
+            if mode == GBX_MODE_PACKED and not dense and use_reg and self.has_hrr(cls):
+                check(self.lib.gbx_eri_hrr_reg(ctypes.byref(dev.struct), *cls, ptr(quartets), nq,
+                                               ptr(count), ptr(acc), nq, R, ptr(out),
+                                               self._stream()), f"gbx_eri_hrr_reg{cls}")
+                return "split"
             sc = self.scratch()
             check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
                                        ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,

@@ -199,6 +199,17 @@ int gbx_eri_hrr(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_
                 double *d_out,
                 void *d_scratch, int64_t scratch_bytes, void *stream);
 
+/* Register form of gbx_eri_hrr for the split classes gbx_has_hrr_kernel lists, packed output
+ * and shells with a scaled-permutation output map (Cartesian, pure l <= 1): one
+ * lane per quartet sums the acc_parts partials and runs the horizontal
+ * recursion one ket column at a time (gbasis/integrals/_two_elec_int_improved.py:364-588,
+ * gbasis/base_four_symm.py:160-244). */
+int gbx_has_hrr_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld);
+int gbx_eri_hrr_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                    const int32_t *d_quartets, int64_t nquart, const int32_t *d_count,
+                    const double *d_acc, int64_t acc_stride, int32_t acc_parts, double *d_out,
+                    void *stream);
+
 /* Expand the canonical 8-fold packed array of one molecule (nfunc output
  * functions) to the dense (nfunc)^4 tensor; notation 0 = chemist (ab|cd),
  * 1 = physicist <ab|cd> = (ac|bd).  Replaces the 8-fold block placement and

@@ -376,8 +376,8 @@ def test_workload_samples_vs_oracle(gpu, name):
     Also the GPU is never further from the reference algorithm's FP64 value than
     that value is from the exact one (plus the same bar, and the rounding of that
     difference: the reference's transfer recursion can be off by orders of
-    magnitude here).  Mixed precision: within
-    the 1e-8 bound.  Only the sampled quartets are evaluated."""
+    magnitude here).  Mixed precision: within 1e-8 of the FP64 result.  Only the
+    sampled quartets are evaluated."""
     from gbasis_amd import workloads
     from oracle.assembly import basis_offsets, packed_index
 
@@ -396,20 +396,45 @@ def test_workload_samples_vs_oracle(gpu, name):
         scale = max(1.0, float(np.abs(exact).max()))
         floor = 32.0 * float(np.abs(kv64 - exact).max())
         bar = max(1e-12 * scale, floor)
-        for prec in ("fp64", "mixed"):
-            got = got_all[prec][slots]
-            err = float(np.abs(got - exact).max())
-            worst[prec] = max(worst[prec], err)
-            if prec == "fp64":
-                assert err <= bar, (q, err, scale, floor)
-                ref_err = np.abs(ref64 - exact)
-                assert np.all(np.abs(got - ref64) <= ref_err * (1 + 1e-12) + bar), q
-                if err <= 1e-12 * scale:
-                    strict += 1
-                else:
-                    floor_only += 1
-            else:
-                assert err <= 1e-8, (q, err)
+        got64 = got_all["fp64"][slots]
+        err = float(np.abs(got64 - exact).max())
+        worst["fp64"] = max(worst["fp64"], err)
+        assert err <= bar, (q, err, scale, floor)
+        ref_err = np.abs(ref64 - exact)
+        assert np.all(np.abs(got64 - ref64) <= ref_err * (1 + 1e-12) + bar), q
+        if err <= 1e-12 * scale:
+            strict += 1
+        else:
+            floor_only += 1
+        # mixed precision: within 1e-8 of the FP64 result (the bound BASELINE states),
+        # hence within the FP64 bar + 1e-8 of the exact value
+        gotmx = got_all["mixed"][slots]
+        errmx = float(np.abs(gotmx - got64).max())
+        worst["mixed"] = max(worst["mixed"], errmx)
+        assert errmx <= 1e-8, (q, errmx)
+        assert float(np.abs(gotmx - exact).max()) <= bar + 1e-8, q
     print(f"{name}: {strict} quartets within 1e-12, {floor_only} within the FP64 floor only; "
-          f"worst absolute error FP64 {worst['fp64']:.2e}, mixed {worst['mixed']:.2e}",
+          f"worst absolute error FP64 {worst['fp64']:.2e}, mixed vs FP64 {worst['mixed']:.2e}",
           flush=True)
+
+
+def test_register_hrr_vs_cooperative_hrr(gpu):
+    """The split classes' register horizontal-recursion kernels give the packed array
+    of the cooperative horizontal-recursion pass (same recursion, FP64)."""
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("register HRR vs cooperative HRR")
+    eng = get_engine()
+    mols = [MoleculeTables(batch_molecule(s, n_atoms=6)) for s in (41, 42)]
+    dev = DeviceTables(BatchTables(mols))
+    plans = eng.plan_device(dev, None, 0.0, 0.0, stats=False)
+    assert any(eng.has_hrr(bk.cls) for bk in plans)
+    eng.hrr_reg = True
+    try:
+        a = eng.run(dev, plans)
+    finally:
+        eng.hrr_reg = False
+    b = eng.run(dev, plans)
+    assert float((a - b).abs().max()) <= 1e-13 * max(1.0, float(b.abs().max()))
