@@ -28,7 +28,7 @@ OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "csrc", "gbx_reg_
 
 
 REG_BUDGET = 220     # FP64 values a lane may hold (accumulators + live intermediates)
-SPLIT_BUDGET = 200   # same, per split of a large class
+SPLIT_BUDGET = int(os.environ.get("GBX_SPLIT_BUDGET", "200"))   # same, per split part
 
 # Inner (ket primitive) loop unroll of the fused kernels, per class.  Each lane's
 # primitive loop is a serial chain (record load -> setup -> Boys table load ->
@@ -41,6 +41,13 @@ _U = int(os.environ.get("GBX_INNER_UNROLL_ALL", "0"))
 # (table reads become LDS reads instead of divergent vector-memory loads), and
 # the primitive loops ordered with the bra pair innermost (the lanes of a wave
 # mostly share the bra pair, so its record loads hit few distinct addresses).
+# Classes run as split kernels even though a fused kernel fits the budget (the
+# fused kernel would spill a little); extra classes can be named in
+# GBX_FORCE_SPLIT as e.g. "2111,2120".
+FORCE_SPLIT = set()
+for _c in filter(None, os.environ.get("GBX_FORCE_SPLIT", "").split(",")):
+    FORCE_SPLIT.add(tuple(int(x) for x in _c))
+
 # Split classes whose register HRR kernel would spill (two ket recursion levels keep
 # too many values live per column): they keep the cooperative HRR pass.
 HRR_REG_EXCLUDE = {(2, 2, 2, 2)}
@@ -131,7 +138,7 @@ def class_plan(cls):
     src_of = [it[1] for it in p.acc_phase[1]]          # acc index -> final value
     all_f = list(range(p.nacc))
     order, peak = schedule(p, [src_of[f] for f in all_f])
-    if p.nacc + peak <= REG_BUDGET:
+    if p.nacc + peak <= REG_BUDGET and cls not in FORCE_SPLIT:
         return "fused", [(all_f, order)]
     byb = _finals_by_bra(p)
     for S in range(2, 33):
