@@ -545,7 +545,7 @@ def config_bench(args):
         "metric": METRIC, "value": eris / tmxh, "unit": "ERIs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": tmxh * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (Boys, prefactors, accumulation) + f32 (recursion bulk, low-bound quartets)",
+        "dtype": "f64+f32",
         "data": "synthetic",
         "config": {"workload": desc, "name": name, "nfunc": mol.nfunc, "shell_quartets": nquart,
                    "eris": eris, "packed_slots": mol.n_slots, "schwarz_threshold": thr,
@@ -688,7 +688,9 @@ def main():
         "metric": METRIC, "value": eris / tmx_head, "unit": "ERIs/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": tmx_head * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64 (Boys, prefactors, accumulation) + f32 (recursion bulk, low-bound quartets)",
+        "dtype": "f64+f32",
+        "dtype_detail": "f64 Boys, prefactors, accumulation and HRR; f32 recursion bulk of the "
+                        "low-bound quartets the autotuner keeps in f32",
         "data": "synthetic",
         "config": {"workload": f"batch of {args.molecules} molecules, 4-24 atoms, s/p/d "
                                "Cartesian, K 1-6, exps [0.05,1e3], packed 8-fold output",

@@ -11,7 +11,8 @@
 // Pass 1 counts quartets per bucket (optionally also packed slots and
 // primitive quartets per bucket); pass 2 writes (bra, ket) pairs at
 // bucket_offset[key] + an atomically reserved position.  Both passes
-// aggregate atomics per wavefront and key.  The order of quartets inside a
+// aggregate atomics per wavefront and key (the count pass also sums the
+// packed-slot and primitive-quartet statistics across the lanes of a key).  The order of quartets inside a
 // bucket is not deterministic; the integral of every packed slot is, since a
 // slot is produced by exactly one quartet with arithmetic that does not depend
 // on its neighbours.
@@ -126,9 +127,18 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       }
     } else {
       if (lane == leader) atomicAdd(a.bucket_count + lkey, n);
-      if (mine && a.bucket_eris) {
-        atomicAdd(a.bucket_eris + lkey, static_cast<unsigned long long>(eris));
-        atomicAdd(a.bucket_primq + lkey, static_cast<unsigned long long>(primq));
+      if (a.bucket_eris) {
+        // sums over the lanes of this key, one atomic per key and wavefront
+        unsigned long long e = mine ? static_cast<unsigned long long>(eris) : 0ull;
+        unsigned long long pq = mine ? static_cast<unsigned long long>(primq) : 0ull;
+        for (int off = 32; off > 0; off >>= 1) {
+          e += __shfl_xor(e, off, 64);
+          pq += __shfl_xor(pq, off, 64);
+        }
+        if (lane == leader) {
+          atomicAdd(a.bucket_eris + lkey, e);
+          atomicAdd(a.bucket_primq + lkey, pq);
+        }
       }
     }
     pending &= ~same;
