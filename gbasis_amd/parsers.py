@@ -87,3 +87,127 @@ def make_contractions(basis_dict, atoms, coords, coord_types, tol=1e-20, ovr=Fal
             basis.append(GeneralizedContractionShell(l, np.array(coord, dtype=float), coeffs,
                                                      exps, coord_types.pop(0), icenter=icenter))
     return tuple(basis)
+
+
+# ---------------------------------------------------------------------------
+# Gaussian formatted checkpoint (fchk) files.
+#
+# The reference reads fchk files through iodata (gbasis/wrappers.py:89-257,
+# ``from_iodata``): segmented shells whose contraction coefficients multiply
+# L2-normalised primitives and are used as given (``norm_cont`` = 1), with the
+# component orders the file defines.  The reader below restates that for the
+# fchk fields the ERI path needs.
+
+_FCHK_ARRAY = re.compile(r"^(.{40})\s+([IRCL])\s+N=\s*(\d+)\s*$")
+_FCHK_SCALAR = re.compile(r"^(.{40})\s+([IRCL])\s+(\S+)\s*$")
+
+# Gaussian's Cartesian component order per l <= 3, as exponent triples
+_GAUSSIAN_CART = {
+    0: [(0, 0, 0)],
+    1: [(1, 0, 0), (0, 1, 0), (0, 0, 1)],
+    2: [(2, 0, 0), (0, 2, 0), (0, 0, 2), (1, 1, 0), (1, 0, 1), (0, 1, 1)],
+    3: [(3, 0, 0), (0, 3, 0), (0, 0, 3), (1, 2, 0), (2, 1, 0), (2, 0, 1), (1, 0, 2), (0, 1, 2),
+        (0, 2, 1), (1, 1, 1)],
+}
+
+
+def _gaussian_cart(l):
+    if l in _GAUSSIAN_CART:
+        return np.array(_GAUSSIAN_CART[l])
+    # l >= 4: reverse alphabetical order (zzzz, yzzz, yyzz, ..., xxxx)
+    return np.array([(a, b, l - a - b) for a in range(l + 1) for b in range(l - a + 1)])
+
+
+def _gaussian_sph(l):
+    labels = ["c0"]
+    for m in range(1, l + 1):
+        labels += [f"c{m}", f"s{m}"]
+    return tuple(labels)
+
+
+def read_fchk(path):
+    """Fields of a Gaussian fchk file: ``{name: int | float | str | np.ndarray}``."""
+    out = {}
+    with open(path, "r") as fh:
+        lines = fh.read().splitlines()
+    i = 2  # title and job lines
+    while i < len(lines):
+        line = lines[i]
+        m = _FCHK_ARRAY.match(line)
+        if m:
+            name, kind, n = m.group(1).strip(), m.group(2), int(m.group(3))
+            vals = []
+            i += 1
+            while len(vals) < n and i < len(lines) and not _FCHK_ARRAY.match(lines[i]) \
+                    and not _FCHK_SCALAR.match(lines[i]):
+                vals.extend(lines[i].split())
+                i += 1
+            if kind == "I":
+                out[name] = np.array([int(v) for v in vals[:n]], dtype=np.int64)
+            elif kind == "R":
+                out[name] = np.array([_to_float(v) for v in vals[:n]], dtype=np.float64)
+            else:
+                out[name] = " ".join(vals)
+            continue
+        m = _FCHK_SCALAR.match(line)
+        if m:
+            name, kind, v = m.group(1).strip(), m.group(2), m.group(3)
+            out[name] = int(v) if kind == "I" else (_to_float(v) if kind == "R" else v)
+        i += 1
+    return out
+
+
+class FchkShell(GeneralizedContractionShell):
+    """A shell read from an fchk file: Gaussian's component orders, coefficients used
+    as given (``norm_cont`` = 1), as the reference's iodata shells
+    (gbasis/wrappers.py:133-227)."""
+
+    def assign_norm_cont(self):
+        ncart = (self.angmom + 1) * (self.angmom + 2) // 2
+        self.norm_cont = np.ones((self.coeffs.shape[1], ncart))
+
+    @property
+    def angmom_components_cart(self):
+        return _gaussian_cart(self.angmom)
+
+    @property
+    def angmom_components_sph(self):
+        return _gaussian_sph(self.angmom)
+
+
+def fchk_basis(fields):
+    """Shells of an fchk file (``read_fchk`` output) in the file's function order:
+    SP shells become an s and a p shell; negative shell types are pure."""
+    types = fields["Shell types"]
+    nprim = fields["Number of primitives per shell"]
+    atom = fields["Shell to atom map"] - 1
+    exps_all = fields["Primitive exponents"]
+    coef_all = fields["Contraction coefficients"]
+    coef_p = fields.get("P(S=P) Contraction coefficients")
+    xyz = fields["Coordinates of each shell"].reshape(-1, 3)
+    basis = []
+    p0 = 0
+    for s, (t, k) in enumerate(zip(types.tolist(), nprim.tolist())):
+        exps = exps_all[p0:p0 + k].copy()
+        coeffs = coef_all[p0:p0 + k].copy()
+        coord = xyz[s].astype(np.float64).copy()
+        if t == -1:
+            basis.append(FchkShell(0, coord, coeffs, exps, "cartesian", icenter=int(atom[s])))
+            basis.append(FchkShell(1, coord, coef_p[p0:p0 + k].copy(), exps.copy(), "cartesian",
+                                   icenter=int(atom[s])))
+        else:
+            l = abs(int(t))
+            kind = "spherical" if (t < 0 and l >= 2) else "cartesian"
+            basis.append(FchkShell(l, coord, coeffs, exps, kind, icenter=int(atom[s])))
+        p0 += k
+    return basis
+
+
+def fchk_rhf_orbitals(fields):
+    """(MO coefficients (nbasis, nmo), occupations (nmo,)) of a closed-shell fchk."""
+    nb = int(fields["Number of basis functions"])
+    nmo = int(fields["Number of independent functions"])
+    c = fields["Alpha MO coefficients"].reshape(nmo, nb).T
+    occ = np.zeros(nmo)
+    occ[:int(fields["Number of alpha electrons"])] = 2.0
+    return c, occ
