@@ -11,17 +11,20 @@
 //
 // The recursion is driven by a per-class "program" (lists of independent items
 // per phase, built on the host, with storage slots reused once a value is dead).
-// One wavefront (a 64-lane workgroup) owns one shell quartet at a time and
-// processes PF primitive quartets at once: every intermediate is stored
+// One workgroup (one wavefront, or several for large workspaces) owns one shell
+// quartet at a time and processes PF primitive quartets at once: every intermediate is stored
 // slot-major with the PF copies adjacent, so lanes working on the same item for
 // consecutive copies touch consecutive LDS words, and a phase boundary is a
-// single-wave barrier.  Several such workgroups share a CU.  Classes whose
-// working set does not fit in 64 KiB of LDS run the same code on a
-// per-workgroup slice of a global scratch buffer.
+// workgroup barrier.  Several such workgroups share a CU; workgroups whose
+// workspace exceeds 40 KiB run 4 to 16 wavefronts, so that the few of them that
+// fit in a CU's LDS still keep its four SIMDs busy.  Classes whose working set does not
+// fit in the LDS one workgroup may allocate (160 KiB on gfx950) run the same code
+// on a per-workgroup slice of a global scratch buffer.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 
@@ -84,8 +87,8 @@ __device__ inline uint64_t tri(uint64_t i, uint64_t j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
 
-template <typename TB, bool kLDS>
-__global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, CoopArgs a) {
+template <typename TB, bool kLDS, int NT>
+__global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) {
   extern __shared__ __align__(16) char smem[];
   char* base = kLDS ? smem : (a.scratch + static_cast<int64_t>(blockIdx.x) * a.stride);
   TB* sc = reinterpret_cast<TB*>(base + a.sc_off);
@@ -94,8 +97,8 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
   double* H = reinterpret_cast<double*>(base + a.h_off);
   double* F0 = reinterpret_cast<double*>(base + a.f0_off);
   double* F1 = reinterpret_cast<double*>(base + a.f1_off);
-  __shared__ double s_red[kCoopThreads / 64];
-  __shared__ double s_red2[kCoopThreads / 64];
+  __shared__ double s_red[NT / 64];
+  __shared__ double s_red2[NT / 64];
 
   const int tid = threadIdx.x;
   const int PF = a.pf;
@@ -118,14 +121,14 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
 
     __syncthreads();  // previous quartet's epilogue is done with the workspace
     if (a.acc_in) {
-      for (int f = tid; f < a.nacc; f += kCoopThreads) {
+      for (int f = tid; f < a.nacc; f += NT) {
         double sum = 0.0;
         for (int r = 0; r < a.acc_parts; ++r)
           sum += a.acc_in[(static_cast<int64_t>(f) * a.acc_parts + r) * a.acc_stride + qi];
         acc[f] = sum;
       }
     } else {
-      for (int f = tid; f < a.nacc; f += kCoopThreads) acc[f] = 0.0;
+      for (int f = tid; f < a.nacc; f += NT) acc[f] = 0.0;
     }
 
     for (int p0 = pbeg; p0 < (a.acc_in ? pbeg : pend); p0 += PF) {
@@ -162,7 +165,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
       for (int ph = 0; ph < a.n_prim_ph; ++ph) {
         const int kind = a.phases[3 * ph], i0 = a.phases[3 * ph + 1], i1 = a.phases[3 * ph + 2];
         const int nwork = (i1 - i0) * PF;
-        for (int x = tid; x < nwork; x += kCoopThreads) {
+        for (int x = tid; x < nwork; x += NT) {
           const int it = i0 + x / PF;
           const int c = x - (x / PF) * PF;
           if (c >= ncopy) continue;
@@ -192,7 +195,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
       }
       {
         const int i0 = a.phases[3 * a.n_prim_ph + 1];
-        for (int f = tid; f < a.nacc; f += kCoopThreads) {
+        for (int f = tid; f < a.nacc; f += NT) {
           const int src = a.items[8 * (i0 + f) + 1];
           double sum = 0.0;
           for (int c = 0; c < ncopy; ++c) sum += static_cast<double>(S[src * PF + c]);
@@ -202,7 +205,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
     }
     if (a.acc_out) {
       // primitive pass only: each thread wrote acc[f] for its own f
-      for (int f = tid; f < a.nacc; f += kCoopThreads)
+      for (int f = tid; f < a.nacc; f += NT)
         a.acc_out[(static_cast<int64_t>(f) * parts + part) * a.acc_stride + qi] = acc[f];
       continue;
     }
@@ -213,14 +216,14 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
                  ABz = t.pair_geom[4 * bra + 2];
     const double CDx = t.pair_geom[4 * ket], CDy = t.pair_geom[4 * ket + 1],
                  CDz = t.pair_geom[4 * ket + 2];
-    for (int f = tid; f < a.nacc; f += kCoopThreads) H[f] = acc[f];
+    for (int f = tid; f < a.nacc; f += NT) H[f] = acc[f];
     __syncthreads();
     for (int ph = a.n_prim_ph + 1; ph < a.n_ph; ++ph) {
       const int kind = a.phases[3 * ph], i0 = a.phases[3 * ph + 1], i1 = a.phases[3 * ph + 2];
       const double gx = kind == kPhHrrK ? CDx : ABx;
       const double gy = kind == kPhHrrK ? CDy : ABy;
       const double gz = kind == kPhHrrK ? CDz : ABz;
-      for (int it = i0 + tid; it < i1; it += kCoopThreads) {
+      for (int it = i0 + tid; it < i1; it += NT) {
         const int4 d0 = items4[2 * it];
         const int dir = d0.w;
         const double g = dir == 0 ? gx : (dir == 1 ? gy : gz);
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
     const double* f2b = t.f2norm + f2_offset(a.lb);
     const double* f2c = t.f2norm + f2_offset(a.lc);
     const double* f2d = t.f2norm + f2_offset(a.ld);
-    for (int o = tid; o < a.nout; o += kCoopThreads) {
+    for (int o = tid; o < a.nout; o += NT) {
       int r = o;
       const int id = r % nc[3]; r /= nc[3];
       const int ic = r % nc[2]; r /= nc[2];
@@ -250,8 +253,8 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
     if (a.mode == GBX_MODE_SCHWARZ) {
       // out[2 bra]: sqrt(max |(ij|ij)|); out[2 bra + 1]: sqrt(max |contracted [e0|f0]|)
       double m = 0.0, m2 = 0.0;
-      for (int o = tid; o < a.nout; o += kCoopThreads) m = fmax(m, fabs(F0[o]));
-      for (int f = tid; f < a.nacc; f += kCoopThreads) m2 = fmax(m2, fabs(acc[f]));
+      for (int o = tid; o < a.nout; o += NT) m = fmax(m, fabs(F0[o]));
+      for (int f = tid; f < a.nacc; f += NT) m2 = fmax(m2, fabs(acc[f]));
       for (int off = 32; off > 0; off >>= 1) {
         m = fmax(m, __shfl_xor(m, off, 64));
         m2 = fmax(m2, __shfl_xor(m2, off, 64));
@@ -263,7 +266,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
       __syncthreads();
       if (tid == 0) {
         double mm = s_red[0], mm2 = s_red2[0];
-        for (int w = 1; w < kCoopThreads / 64; ++w) {
+        for (int w = 1; w < NT / 64; ++w) {
           mm = fmax(mm, s_red[w]);
           mm2 = fmax(mm2, s_red2[w]);
         }
@@ -293,7 +296,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
       int inner = 1;
       for (int y = x + 1; y < 4; ++y) inner *= dim[y];
       const int total = ndim[0] * ndim[1] * ndim[2] * ndim[3];
-      for (int e = tid; e < total; e += kCoopThreads) {
+      for (int e = tid; e < total; e += NT) {
         const int lo = e % inner;
         const int jx = (e / inner) % nout_x;
         const int hi = e / (inner * nout_x);
@@ -313,7 +316,7 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
     const uint64_t obase = t.pair_out_base[bra];
     const bool bra_diag = pb.x == pb.y, ket_diag = pk.x == pk.y, same_pair = bra == ket;
     const int total = dim[0] * dim[1] * dim[2] * dim[3];
-    for (int e = tid; e < total; e += kCoopThreads) {
+    for (int e = tid; e < total; e += NT) {
       int idx[4];
       int r = e;
       idx[3] = r % dim[3]; r /= dim[3];
@@ -337,6 +340,68 @@ __global__ __launch_bounds__(kCoopThreads) void eri_coop_kernel(gbx_tables t, Co
       a.out[obase + tri(ij, kl)] = v;
     }
   }
+}
+
+// Largest dynamic LDS allocation of one workgroup on the current device (160 KiB on
+// gfx950), less the kernel's static LDS; cached per device.
+inline int64_t coop_lds_limit() {
+  static int64_t cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return kCoopLdsLimit;
+  if (cached[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess ||
+        v <= 0)
+      v = static_cast<int>(kCoopLdsLimit);
+    cached[dev] = std::max<int64_t>(kCoopLdsLimit, static_cast<int64_t>(v) - 256);
+  }
+  return cached[dev];
+}
+
+// Workgroups with LDS workspaces above this size run several wavefronts: few of them
+// fit on a CU, and one wavefront per workgroup would leave SIMDs idle.
+constexpr int64_t kCoopWideBytes = 40 * 1024;
+
+// Wavefronts per CU that a wide workgroup size aims for (environment variable
+// GBX_COOP_WAVES, 4..32, default 8).
+inline int coop_target_waves() {
+  static int cached = 0;
+  if (cached == 0) {
+    const char* e = std::getenv("GBX_COOP_WAVES");
+    const int v = e ? std::atoi(e) : 8;
+    cached = (v >= 4 && v <= 32) ? v : 8;
+  }
+  return cached;
+}
+
+// Threads per workgroup for an LDS workspace of `bytes`: one wavefront up to
+// kCoopWideBytes, else enough wavefronts that the workgroups fitting on a CU
+// together run about coop_target_waves() of them (a power of two, 256..1024).
+inline int coop_threads(int64_t bytes) {
+  if (bytes <= kCoopWideBytes) return 64;
+  const int64_t per_cu = std::max<int64_t>(1, coop_lds_limit() / bytes);
+  const int64_t waves = (coop_target_waves() + per_cu - 1) / per_cu;
+  int nt = 256;
+  while (nt < 1024 && nt < 64 * waves) nt *= 2;
+  return nt;
+}
+
+template <typename TB, bool kLDS, int NT>
+int launch_coop_nt(const CoopArgs& a, const gbx_tables* t, int64_t grid, size_t shmem,
+                   hipStream_t stream) {
+  if (shmem > 65536) {
+    static bool raised = false;
+    if (!raised) {
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&eri_coop_kernel<TB, kLDS, NT>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(coop_lds_limit())) != hipSuccess)
+        return fail(GBX_ERR_HIP, "eri_coop_kernel: could not raise the LDS limit");
+      raised = true;
+    }
+  }
+  hipLaunchKernelGGL((eri_coop_kernel<TB, kLDS, NT>), dim3(static_cast<unsigned>(grid)), dim3(NT),
+                     shmem, stream, *t, a);
+  return check_launch("eri_coop_kernel");
 }
 
 template <typename TB, bool kLDS>
@@ -381,19 +446,26 @@ int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, in
   a.parts = parts;
   int64_t grid = nquart * (acc_out ? parts : 1);
   size_t shmem = 0;
-  if (kLDS) {
+  if constexpr (kLDS) {
+    if (l.total > coop_lds_limit())
+      return fail(GBX_ERR_ARG, "gbx_eri_coop: workspace exceeds the LDS of one workgroup");
     shmem = static_cast<size_t>(l.total);
     grid = std::min<int64_t>(grid, int64_t(1) << 20);
+    switch (coop_threads(l.total)) {
+      case 1024: return launch_coop_nt<TB, kLDS, 1024>(a, t, grid, shmem, stream);
+      case 512: return launch_coop_nt<TB, kLDS, 512>(a, t, grid, shmem, stream);
+      case 256: return launch_coop_nt<TB, kLDS, 256>(a, t, grid, shmem, stream);
+      default: return launch_coop_nt<TB, kLDS, 64>(a, t, grid, shmem, stream);
+    }
   } else {
     a.scratch = static_cast<char*>(d_scratch);
     a.stride = l.total;
     const int64_t nblk = scratch_bytes / l.total;
-    if (nblk < 1) return fail(GBX_ERR_ARG, "gbx_eri_coop: scratch buffer smaller than one workgroup slice");
+    if (nblk < 1)
+      return fail(GBX_ERR_ARG, "gbx_eri_coop: scratch buffer smaller than one workgroup slice");
     grid = std::min<int64_t>(grid, nblk);
+    return launch_coop_nt<TB, kLDS, 64>(a, t, grid, shmem, stream);
   }
-  hipLaunchKernelGGL((eri_coop_kernel<TB, kLDS>), dim3(static_cast<unsigned>(grid)),
-                     dim3(kCoopThreads), shmem, stream, *t, a);
-  return check_launch("eri_coop_kernel");
 }
 
 }  // namespace gbx
@@ -409,7 +481,7 @@ extern "C" int gbx_coop_workspace(const gbx_program* prog, int32_t la, int32_t l
   (void)la; (void)lb; (void)lc; (void)ld;
   const CoopLayout l =
       coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies, prim_only != 0);
-  const bool fits = l.total <= kCoopLdsLimit;
+  const bool fits = l.total <= coop_lds_limit();
   if (lds_bytes) *lds_bytes = l.total;
   if (global_bytes_per_block) *global_bytes_per_block = fits ? 0 : l.total;
   return GBX_OK;
@@ -436,7 +508,7 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
   if (nquart == 0) return GBX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
-  const bool lds = l.total <= kCoopLdsLimit;
+  const bool lds = l.total <= coop_lds_limit();
   if (precision == GBX_FP64) {
     return lds ? launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
                                            prim_copies, mode, d_out, d_scratch, scratch_bytes, s)
@@ -466,7 +538,7 @@ extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t
   if (nquart == 0) return GBX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const CoopLayout l = coop_layout(prog->header, 8, 1);
-  if (l.total <= kCoopLdsLimit)
+  if (l.total <= coop_lds_limit())
     return launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count, 1, mode,
                                      d_out, d_scratch, scratch_bytes, s, d_acc, acc_stride,
                                      acc_parts);
@@ -499,7 +571,7 @@ extern "C" int gbx_eri_coop_acc(const gbx_tables* t, const gbx_program* prog, in
   hipStream_t s = static_cast<hipStream_t>(stream);
   const CoopLayout l =
       coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies, true);
-  const bool lds = l.total <= kCoopLdsLimit;
+  const bool lds = l.total <= coop_lds_limit();
   if (precision == GBX_FP64)
     return lds ? launch_coop<double, true>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
                                            prim_copies, GBX_MODE_PACKED, nullptr, d_scratch,

@@ -263,7 +263,8 @@ class EriEngine:
         """Primitive quartets in flight (PF) for the cooperative kernel.
 
         Largest power of two <= min(nprimq, 64) whose LDS workspace stays within
-        LDS_TARGET; if even PF = 1 does not fit 64 KiB the kernel runs on global
+        LDS_TARGET; if even PF = 1 does not fit the LDS of one workgroup (160 KiB on
+        gfx950; workspaces above 40 KiB run 4-16 wavefronts) the kernel runs on global
         scratch, where PF = min(nprimq, 16) keeps the wave busy (PF = 1 for the
         split primitive pass, whose parallelism comes from the parts).
         """
@@ -691,6 +692,23 @@ class EriEngine:
         check(self.lib.gbx_unpack_dense(ptr(packed), nfunc, 1 if notation == "physicist" else 0,
                                         ptr(dense), self._stream()), "gbx_unpack_dense")
         return dense
+
+    def jk(self, packed, nfunc, density):
+        """Coulomb and exchange matrices (device float64, n x n) of one molecule's
+        packed ERIs for a symmetric density."""
+        n = int(nfunc)
+        npair = n * (n + 1) // 2
+        if packed.numel() != npair * (npair + 1) // 2:
+            raise ValueError(f"packed array has {packed.numel()} slots, not the "
+                             f"{npair * (npair + 1) // 2} of {n} functions")
+        D = torch.as_tensor(density, dtype=torch.float64, device=self.device).contiguous()
+        if D.shape != (n, n):
+            raise ValueError(f"density must be ({n}, {n}), got {tuple(D.shape)}")
+        J = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        K = torch.empty((n, n), dtype=torch.float64, device=self.device)
+        check(self.lib.gbx_jk_packed(ptr(packed), n, ptr(D), ptr(J), ptr(K), self._stream()),
+              "gbx_jk_packed")
+        return J, K
 
     def boys(self, dev: DeviceTables, T, mmax):
         T = torch.as_tensor(T, dtype=torch.float64, device=self.device).contiguous()

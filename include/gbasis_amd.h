@@ -79,14 +79,16 @@ int gbx_boys(const gbx_tables *t, const double *d_T, int64_t n, int32_t mmax, do
 /* Workspace of the cooperative kernel for one class program and prim_copies
  * (1..64) primitive quartets in flight (prim_only != 0: of the primitive pass
  * gbx_eri_coop_acc): *lds_bytes = bytes per workgroup; *global_bytes_per_block
- * = the same when it exceeds 64 KiB (the kernel then works in a slice of
- * d_scratch per workgroup), else 0. */
+ * = the same when it exceeds the LDS one workgroup may allocate on the current
+ * device (160 KiB on gfx950; the kernel then works in a slice of d_scratch per
+ * workgroup), else 0. */
 int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                        int32_t precision, int32_t prim_copies, int32_t prim_only,
                        int64_t *lds_bytes, int64_t *global_bytes_per_block);
 
 /* One angular-momentum class of shell quartets with the cooperative
- * (one 64-lane workgroup per shell quartet) kernel.  d_quartets holds nquart (bra pair, ket pair)
+ * (one workgroup per shell quartet: 64 lanes, or 256-1024 for LDS workspaces
+ * above 40 KiB) kernel.  d_quartets holds nquart (bra pair, ket pair)
  * int32 pairs in canonical orientation.  Replaces, for a batch of quartets,
  * ElectronRepulsionIntegralImproved.construct_array_contraction
  * (gbasis/integrals/electron_repulsion.py:305-412) ->
@@ -217,6 +219,17 @@ int gbx_eri_hrr_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int
  * of gbasis/integrals/electron_repulsion.py:279-280. */
 int gbx_unpack_dense(const double *d_packed, int64_t nfunc, int32_t notation, double *d_dense,
                      void *stream);
+
+/* Coulomb and exchange matrices of one molecule straight from its canonical
+ * packed ERIs (nfunc <= 65536): d_J[m*n+n'] = sum_ls (mn'|ls) D_ls and
+ * d_K[m*n+l] = sum_ns (mn|ls) D_ns for a symmetric row-major density d_dens
+ * (n x n, n = nfunc); d_J and d_K (n x n each) are overwritten.  No reference
+ * entry point computes these: they replace the digestion its users write on top
+ * of the dense array, e.g. the Coulomb / exchange energies of
+ * notebooks/tutorial/JCP_paper.ipynb (cell 28) formed from
+ * electron_repulsion_integral(basis, transform=C.T, notation="chemist"). */
+int gbx_jk_packed(const double *d_packed, int64_t nfunc, const double *d_dens, double *d_J,
+                  double *d_K, void *stream);
 
 #ifdef __cplusplus
 }
