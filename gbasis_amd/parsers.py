@@ -62,6 +62,50 @@ def parse_nwchem(nwchem_basis_file):
     return out
 
 
+def parse_gbs(gbs_basis_file):
+    """Return ``{element: [(angmom, exps (K,), coeffs (K, M)), ...]}`` of a Gaussian94
+    basis file, the contract of gbasis/parsers.py:75-166: an "SP" shell gives one
+    s and one p entry, and a contraction directly following one of the same
+    angular momentum and exponents joins it as a further column (a generalised
+    contraction)."""
+    with open(gbs_basis_file, "r") as fh:
+        lines = fh.read().splitlines()
+    out = {}
+    atom = None
+    i = 0
+    while i < len(lines):
+        tok = lines[i].split()
+        i += 1
+        if not tok or tok[0].startswith("!"):
+            continue
+        if tok[0] == "****":
+            atom = None
+            continue
+        if atom is None:
+            if len(tok) == 2 and tok[0][0].isalpha():
+                atom = tok[0]
+                out.setdefault(atom, [])
+            continue
+        label = tok[0].lower()
+        if len(tok) < 2 or not all(c in _ANGMOM for c in label):
+            continue
+        nprim = int(tok[1])
+        rows = [[_to_float(x) for x in lines[i + r].split()] for r in range(nprim)]
+        i += nprim
+        exps = np.array([r[0] for r in rows])
+        coeffs = np.array([r[1:] for r in rows])
+        for col, ch in enumerate(label):
+            l = _ANGMOM[ch]
+            c = coeffs[:, col:col + 1]
+            prev = out[atom][-1] if out[atom] else None
+            if (prev is not None and prev[0] == l and prev[1].shape == exps.shape
+                    and np.allclose(prev[1], exps)):
+                out[atom][-1] = (l, exps, np.hstack([prev[2], c]))
+            else:
+                out[atom].append((l, exps, c))
+    return out
+
+
 def make_contractions(basis_dict, atoms, coords, coord_types, tol=1e-20, ovr=False):
     """Build shells for ``atoms`` at ``coords`` (``gbasis/parsers.py:169-247``)."""
     if not (isinstance(atoms, (list, tuple)) and all(isinstance(a, str) for a in atoms)):
