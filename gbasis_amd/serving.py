@@ -17,12 +17,18 @@ torch tensor shared between streams).
   grow-only per stream, so steady-state serving allocates and frees no device
   memory (a device free would synchronise the GPU and serialise the streams).
   A result view is valid until its request is released.
-* Continuous batching: every stream that has no batch in flight takes all
+* Continuous batching: every stream that has no batch in flight takes
   admitted requests that fit as one batch -- one table upload, one Schwarz pass,
   one device-built work queue whose launches read their counts on the device,
   every class launch -- so batches launched while others run overlap on the
   device.  FP64 and mixed requests share a batch through per-molecule FP32
   cutoffs.
+* Order: with policy "sjf" (default) the waiting request with the smallest
+  cost / (1 + age / age_tau) opens a batch (cost: packed slots, age: time since
+  arrival), and the batch takes the next requests by the same score whose cost
+  is at most twice the opener's -- small molecules do not wait behind large
+  ones, and a request waiting longer than max_wait (0.5 s) opens the next batch.
+  Policy "fifo" takes requests in arrival order.
 * Results do not depend on the company a request keeps: the engine runs in
   deterministic mode (variant choices depend on class and primitive bin only)
   and every packed slot is produced by one shell quartet whose arithmetic reads
@@ -125,7 +131,13 @@ class EriServer:
     """Admission, continuous batching and completion of ERI requests."""
 
     def __init__(self, device="cuda", threshold=1e-12, fp32_cutoff=1e-2, hbm_budget=None,
-                 n_streams=4, max_batch=8, clock=time.perf_counter):
+                 n_streams=4, max_batch=8, clock=time.perf_counter, policy="sjf",
+                 age_tau=0.1, max_wait=0.5):
+        if policy not in ("sjf", "fifo"):
+            raise ValueError("policy must be 'sjf' or 'fifo'")
+        self.policy = policy
+        self.age_tau = float(age_tau)
+        self.max_wait = float(max_wait)
         # an engine of its own: deterministic variant choices and per-stream workspaces
         self.eng = EriEngine(device)
         self.eng.deterministic = True
@@ -219,14 +231,12 @@ class EriServer:
         for i, b in enumerate(self.inflight):
             if b is not None or not self.pending:
                 continue
-            take, need, slots = [], 0, 0
-            while self.pending and len(take) < self.max_batch:
-                r = self.pending[0]
-                if self.used + need + r.footprint > self.budget:
-                    break
-                take.append(self.pending.popleft())
-                need += r.footprint
-                slots += r.tables.n_slots
+            take = select_batch(list(self.pending), self.clock(), self.budget - self.used,
+                                self.max_batch, self.policy, self.age_tau, self.max_wait)
+            for r in take:
+                self.pending.remove(r)
+            need = sum(r.footprint for r in take)
+            slots = sum(r.tables.n_slots for r in take)
             while take:
                 region = self.arena.alloc(slots)
                 if region is not None:
@@ -263,6 +273,39 @@ class EriServer:
             ev.record(stream)
         self.batches_launched += 1
         return _Batch(reqs, stream, dev, out, ev, plans, *region)
+
+
+def select_batch(pending, now, budget_left, max_batch, policy="sjf", age_tau=0.1,
+                 max_wait=0.5):
+    """The requests of the next batch, from ``pending`` (arrival order).  Objects need
+    ``footprint``, ``t_arrival`` and ``tables.n_slots``.  With "sjf" a request that
+    has waited longer than ``max_wait`` seconds opens the batch (oldest first)."""
+    take, need = [], 0
+    if policy == "fifo":
+        for r in pending:
+            if len(take) >= max_batch or need + r.footprint > budget_left:
+                break
+            take.append(r)
+            need += r.footprint
+        return take
+    order = sorted(pending, key=lambda r: r.tables.n_slots /
+                   (1.0 + max(0.0, now - r.t_arrival) / age_tau))
+    overdue = [r for r in pending if now - r.t_arrival > max_wait]
+    if overdue:
+        order.remove(overdue[0])
+        order.insert(0, overdue[0])
+    for r in order:
+        if len(take) >= max_batch:
+            break
+        if take and r.tables.n_slots > 2 * take[0].tables.n_slots:
+            continue
+        if need + r.footprint > budget_left:
+            if not take:
+                return []   # the best-scored request waits for memory
+            continue
+        take.append(r)
+        need += r.footprint
+    return take
 
 
 def latency_percentiles(reqs, qs=(50, 99)):

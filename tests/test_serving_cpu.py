@@ -38,3 +38,28 @@ def test_arena_random_alloc_release_never_overlaps():
             assert off == pos and n > 0
             pos += n
         assert pos == ar.numel
+
+
+class _Req:
+    def __init__(self, slots, t, fp=1):
+        self.footprint = fp
+        self.t_arrival = t
+
+        class _T:
+            n_slots = slots
+        self.tables = _T()
+
+
+def test_select_batch_fifo_and_sjf_with_aging():
+    from gbasis_amd.serving import select_batch
+
+    big, small1, small2, mid = _Req(10000, 0.9), _Req(10, 0.9), _Req(12, 0.95), _Req(15, 0.97)
+    pend = [big, small1, small2, mid]
+    assert select_batch(pend, 1.0, 100, 8, "fifo") == pend
+    assert select_batch(pend, 1.0, 2, 8, "fifo") == [big, small1]
+    # smallest cost first; the batch keeps to requests within 2x of its opener
+    assert select_batch(pend, 1.0, 100, 8, "sjf", age_tau=0.1) == [small1, small2, mid]
+    # a request waiting longer than max_wait opens the batch
+    assert select_batch(pend, 2.0, 100, 8, "sjf", age_tau=0.1, max_wait=0.5)[0] is big
+    # memory: the best-scored request waits rather than being overtaken
+    assert select_batch([_Req(10, 0.0, fp=5), _Req(20, 0.0)], 0.0, 4, 8, "sjf") == []
