@@ -19,6 +19,9 @@ GBX_FP64 = 0
 GBX_FP32_BULK = 1
 GBX_MODE_PACKED = 0
 GBX_MODE_SCHWARZ = 1
+GBX_POT_COULOMB = 0
+GBX_POT_ERF = 1
+GBX_POT_ERFC = 2
 
 c_i32 = ctypes.c_int32
 c_i64 = ctypes.c_int64
@@ -30,7 +33,8 @@ class GbxTables(ctypes.Structure):
         ("shell_info", c_vp), ("comp_map", c_vp), ("comp_scale", c_vp), ("out_mat", c_vp),
         ("pair_info", c_vp), ("pair_geom", c_vp), ("pair_out_base", c_vp), ("prim_pairs", c_vp),
         ("f2norm", c_vp), ("boys_table", c_vp), ("boys_nt", c_i32), ("boys_nord", c_i32),
-        ("boys_h", ctypes.c_double),
+        ("boys_h", ctypes.c_double), ("potential", c_i32), ("reserved", c_i32),
+        ("omega", ctypes.c_double),
     ]
 
 

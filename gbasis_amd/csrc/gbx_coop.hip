@@ -19,7 +19,8 @@
 // workspace exceeds 40 KiB run 4 to 16 wavefronts, so that the few of them that
 // fit in a CU's LDS still keep its four SIMDs busy.  Classes whose working set does not
 // fit in the LDS one workgroup may allocate (160 KiB on gfx950) run the same code
-// on a per-workgroup slice of a global scratch buffer.
+// with 256-lane workgroups on a per-workgroup slice of a global scratch buffer,
+// with few enough workgroups that the slices in use stay in the Infinity Cache.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -34,6 +35,7 @@
 namespace gbx {
 
 constexpr int kCoopThreads = 64;
+constexpr int kCoopGlobalThreads = 256;  // lanes of a global-scratch workgroup
 constexpr int kPhVrr = 0, kPhKvrr = 1, kPhAcc = 2, kPhHrrK = 3, kPhHrrB = 4;
 constexpr int kScalars = 17;  // per-primitive-quartet scalars held in LDS
 
@@ -63,16 +65,26 @@ struct CoopArgs {
 
 struct CoopLayout {
   int64_t sc_off, acc_off, un_off, h_off, f0_off, f1_off, total;
+  bool reg_acc;  // primitive pass keeps the accumulators in registers (no acc region)
 };
 
 inline int64_t align16(int64_t x) { return (x + 15) & ~int64_t(15); }
 
-inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf, bool prim_only = false) {
+// Register accumulators of the primitive pass: kRegAccThreads lanes with
+// kRegAccPerLane FP64 accumulators each.
+constexpr int kRegAccThreads = 512;
+constexpr int kRegAccPerLane = 24;
+
+inline int64_t coop_lds_limit();
+
+inline CoopLayout coop_layout_acc(const int32_t* hdr, int tb_bytes, int pf, bool prim_only,
+                                  bool reg_acc) {
   const int64_t prim_slots = hdr[0], nacc = hdr[1], h_slots = hdr[2], nout = hdr[3];
   CoopLayout l{};
+  l.reg_acc = reg_acc;
   l.sc_off = 0;
   l.acc_off = align16(l.sc_off + kScalars * pf * tb_bytes);
-  l.un_off = align16(l.acc_off + nacc * 8);
+  l.un_off = reg_acc ? l.acc_off : align16(l.acc_off + nacc * 8);
   const int64_t prim_bytes = prim_slots * pf * tb_bytes;
   l.h_off = l.un_off;
   l.f0_off = align16(l.h_off + h_slots * 8);
@@ -83,12 +95,26 @@ inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf, bool pri
   return l;
 }
 
+// Workspace of one workgroup.  The primitive pass (prim_only) moves its
+// accumulators to registers when they would push the workspace out of LDS and
+// the primitive store alone fits.
+inline CoopLayout coop_layout(const int32_t* hdr, int tb_bytes, int pf, bool prim_only = false) {
+  const CoopLayout l = coop_layout_acc(hdr, tb_bytes, pf, prim_only, false);
+  if (!prim_only || l.total <= coop_lds_limit() ||
+      hdr[1] > int64_t(kRegAccThreads) * kRegAccPerLane)
+    return l;
+  const CoopLayout r = coop_layout_acc(hdr, tb_bytes, pf, prim_only, true);
+  return r.total <= coop_lds_limit() ? r : l;
+}
+
 __device__ inline uint64_t tri(uint64_t i, uint64_t j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
 
-template <typename TB, bool kLDS, int NT>
+template <typename TB, bool kLDS, int NT, bool kRegAcc = false>
 __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) {
+  static_assert(!kRegAcc || NT == kRegAccThreads, "register accumulators need 512 lanes");
+  double accr[kRegAcc ? kRegAccPerLane : 1];
   extern __shared__ __align__(16) char smem[];
   char* base = kLDS ? smem : (a.scratch + static_cast<int64_t>(blockIdx.x) * a.stride);
   TB* sc = reinterpret_cast<TB*>(base + a.sc_off);
@@ -127,6 +153,9 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
           sum += a.acc_in[(static_cast<int64_t>(f) * a.acc_parts + r) * a.acc_stride + qi];
         acc[f] = sum;
       }
+    } else if constexpr (kRegAcc) {
+#pragma unroll
+      for (int u = 0; u < kRegAccPerLane; ++u) accr[u] = 0.0;
     } else {
       for (int f = tid; f < a.nacc; f += NT) acc[f] = 0.0;
     }
@@ -159,7 +188,27 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
         sc[15 * PF + tid] = static_cast<TB>(s.i2e);
         sc[16 * PF + tid] = static_cast<TB>(s.i2s);
         const double pref = s.pref;
-        boys_eval(Tval, a.L, t, [&](int m, double v) { S[m * PF + tid] = static_cast<TB>(v * pref); });
+        if (t.potential == GBX_POT_COULOMB || a.mode == GBX_MODE_SCHWARZ) {
+          boys_eval(Tval, a.L, t,
+                    [&](int m, double v) { S[m * PF + tid] = static_cast<TB>(v * pref); });
+        } else {
+          // range-separated operators (FP64 only, checked on the host):
+          // G_m = s^(m+1/2) F_m(s T), s = omega^2 / (omega^2 + rho)
+          const double w2 = t.omega * t.omega;
+          const double rho = rb[0] * rk[0] / (rb[0] + rk[0]);
+          const double sc = w2 / (w2 + rho);
+          const double sq = sqrt(sc);
+          const bool erfc = t.potential == GBX_POT_ERFC;
+          if (erfc)
+            boys_eval(Tval, a.L, t,
+                      [&](int m, double v) { S[m * PF + tid] = static_cast<TB>(v * pref); });
+          boys_eval(sc * Tval, a.L, t, [&](int m, double v) {
+            double pw = sq;
+            for (int i = 0; i < m; ++i) pw *= sc;
+            const double g = v * pw * pref;
+            S[m * PF + tid] = erfc ? static_cast<TB>(S[m * PF + tid] - g) : static_cast<TB>(g);
+          });
+        }
       }
       __syncthreads();
       for (int ph = 0; ph < a.n_prim_ph; ++ph) {
@@ -195,20 +244,44 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
       }
       {
         const int i0 = a.phases[3 * a.n_prim_ph + 1];
-        for (int f = tid; f < a.nacc; f += NT) {
-          const int src = a.items[8 * (i0 + f) + 1];
-          double sum = 0.0;
-          for (int c = 0; c < ncopy; ++c) sum += static_cast<double>(S[src * PF + c]);
-          acc[f] += sum;
+        if constexpr (kRegAcc) {
+#pragma unroll
+          for (int u = 0; u < kRegAccPerLane; ++u) {
+            const int f = tid + u * NT;
+            if (f < a.nacc) {
+              const int src = a.items[8 * (i0 + f) + 1];
+              double sum = 0.0;
+              for (int c = 0; c < ncopy; ++c) sum += static_cast<double>(S[src * PF + c]);
+              accr[u] += sum;
+            }
+          }
+        } else {
+          for (int f = tid; f < a.nacc; f += NT) {
+            const int src = a.items[8 * (i0 + f) + 1];
+            double sum = 0.0;
+            for (int c = 0; c < ncopy; ++c) sum += static_cast<double>(S[src * PF + c]);
+            acc[f] += sum;
+          }
         }
       }
     }
     if (a.acc_out) {
       // primitive pass only: each thread wrote acc[f] for its own f
-      for (int f = tid; f < a.nacc; f += NT)
-        a.acc_out[(static_cast<int64_t>(f) * parts + part) * a.acc_stride + qi] = acc[f];
+      if constexpr (kRegAcc) {
+#pragma unroll
+        for (int u = 0; u < kRegAccPerLane; ++u) {
+          const int f = tid + u * NT;
+          if (f < a.nacc)
+            a.acc_out[(static_cast<int64_t>(f) * parts + part) * a.acc_stride + qi] = accr[u];
+        }
+      } else {
+        for (int f = tid; f < a.nacc; f += NT)
+          a.acc_out[(static_cast<int64_t>(f) * parts + part) * a.acc_stride + qi] = acc[f];
+      }
       continue;
     }
+    // register accumulators serve the primitive pass only (acc_out is then set)
+    if constexpr (!kRegAcc) {
     __syncthreads();
 
     // ---- horizontal recursion on the contracted values ----
@@ -339,6 +412,7 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
       if (same_pair && ij < kl) continue;
       a.out[obase + tri(ij, kl)] = v;
     }
+    }  // !kRegAcc
   }
 }
 
@@ -358,9 +432,47 @@ inline int64_t coop_lds_limit() {
   return cached[dev];
 }
 
+// Compute units of the current device; cached per device.
+inline int64_t coop_num_cu() {
+  static int64_t cached[64] = {};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return 256;
+  if (cached[dev] == 0) {
+    int v = 0;
+    if (hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        v <= 0)
+      v = 256;
+    cached[dev] = v;
+  }
+  return cached[dev];
+}
+
+// Workgroups per CU of the global-scratch kernel (environment variable
+// GBX_COOP_GLOBAL_WGS, 1..16, default 2).  Its workspaces are 0.2-2 MB; a few
+// 256-lane workgroups per CU keep the scratch in flight within the 256 MB
+// Infinity Cache instead of streaming it through HBM.
+inline int coop_global_wgs_per_cu() {
+  static int cached = 0;
+  if (cached == 0) {
+    const char* e = std::getenv("GBX_COOP_GLOBAL_WGS");
+    const int v = e ? std::atoi(e) : 2;
+    cached = (v >= 1 && v <= 16) ? v : 2;
+  }
+  return cached;
+}
+
 // Workgroups with LDS workspaces above this size run several wavefronts: few of them
-// fit on a CU, and one wavefront per workgroup would leave SIMDs idle.
-constexpr int64_t kCoopWideBytes = 40 * 1024;
+// fit on a CU, and one wavefront per workgroup would leave SIMDs idle
+// (environment variable GBX_COOP_WIDE_KB, default 40).
+inline int64_t coop_wide_bytes() {
+  static int64_t cached = -1;
+  if (cached < 0) {
+    const char* e = std::getenv("GBX_COOP_WIDE_KB");
+    const int v = e ? std::atoi(e) : 40;
+    cached = int64_t(v >= 0 && v <= 160 ? v : 40) * 1024;
+  }
+  return cached;
+}
 
 // Wavefronts per CU that a wide workgroup size aims for (environment variable
 // GBX_COOP_WAVES, 4..32, default 8).
@@ -375,10 +487,10 @@ inline int coop_target_waves() {
 }
 
 // Threads per workgroup for an LDS workspace of `bytes`: one wavefront up to
-// kCoopWideBytes, else enough wavefronts that the workgroups fitting on a CU
+// coop_wide_bytes(), else enough wavefronts that the workgroups fitting on a CU
 // together run about coop_target_waves() of them (a power of two, 256..1024).
 inline int coop_threads(int64_t bytes) {
-  if (bytes <= kCoopWideBytes) return 64;
+  if (bytes <= coop_wide_bytes()) return 64;
   const int64_t per_cu = std::max<int64_t>(1, coop_lds_limit() / bytes);
   const int64_t waves = (coop_target_waves() + per_cu - 1) / per_cu;
   int nt = 256;
@@ -386,21 +498,21 @@ inline int coop_threads(int64_t bytes) {
   return nt;
 }
 
-template <typename TB, bool kLDS, int NT>
+template <typename TB, bool kLDS, int NT, bool kRegAcc = false>
 int launch_coop_nt(const CoopArgs& a, const gbx_tables* t, int64_t grid, size_t shmem,
                    hipStream_t stream) {
   if (shmem > 65536) {
     static bool raised = false;
     if (!raised) {
-      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&eri_coop_kernel<TB, kLDS, NT>),
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&eri_coop_kernel<TB, kLDS, NT, kRegAcc>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(coop_lds_limit())) != hipSuccess)
         return fail(GBX_ERR_HIP, "eri_coop_kernel: could not raise the LDS limit");
       raised = true;
     }
   }
-  hipLaunchKernelGGL((eri_coop_kernel<TB, kLDS, NT>), dim3(static_cast<unsigned>(grid)), dim3(NT),
-                     shmem, stream, *t, a);
+  hipLaunchKernelGGL((eri_coop_kernel<TB, kLDS, NT, kRegAcc>), dim3(static_cast<unsigned>(grid)),
+                     dim3(NT), shmem, stream, *t, a);
   return check_launch("eri_coop_kernel");
 }
 
@@ -451,6 +563,11 @@ int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, in
       return fail(GBX_ERR_ARG, "gbx_eri_coop: workspace exceeds the LDS of one workgroup");
     shmem = static_cast<size_t>(l.total);
     grid = std::min<int64_t>(grid, int64_t(1) << 20);
+    if (l.reg_acc) {
+      if (!acc_out)
+        return fail(GBX_ERR_ARG, "gbx_eri_coop: register accumulators need the primitive pass");
+      return launch_coop_nt<TB, kLDS, kRegAccThreads, true>(a, t, grid, shmem, stream);
+    }
     switch (coop_threads(l.total)) {
       case 1024: return launch_coop_nt<TB, kLDS, 1024>(a, t, grid, shmem, stream);
       case 512: return launch_coop_nt<TB, kLDS, 512>(a, t, grid, shmem, stream);
@@ -463,9 +580,22 @@ int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, in
     const int64_t nblk = scratch_bytes / l.total;
     if (nblk < 1)
       return fail(GBX_ERR_ARG, "gbx_eri_coop: scratch buffer smaller than one workgroup slice");
-    grid = std::min<int64_t>(grid, nblk);
-    return launch_coop_nt<TB, kLDS, 64>(a, t, grid, shmem, stream);
+    grid = std::min<int64_t>({grid, nblk, coop_global_wgs_per_cu() * coop_num_cu()});
+    return launch_coop_nt<TB, kLDS, kCoopGlobalThreads>(a, t, grid, shmem, stream);
   }
+}
+
+// The operator of a launch: Coulomb always; range-separated operators in FP64
+// packed launches with omega > 0.
+inline int check_potential(const gbx_tables* t, int mode, int precision, const char* who) {
+  if (t->potential == GBX_POT_COULOMB || mode == GBX_MODE_SCHWARZ) return GBX_OK;
+  if (t->potential != GBX_POT_ERF && t->potential != GBX_POT_ERFC)
+    return fail(GBX_ERR_ARG, std::string(who) + ": unknown potential");
+  if (!(t->omega > 0.0)) return fail(GBX_ERR_ARG, std::string(who) + ": omega must be > 0");
+  if (precision != GBX_FP64)
+    return fail(GBX_ERR_UNSUPPORTED,
+                std::string(who) + ": range-separated operators are evaluated in FP64 only");
+  return GBX_OK;
 }
 
 }  // namespace gbx
@@ -505,6 +635,7 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
     return fail(GBX_ERR_ARG, "gbx_eri_coop: bad mode");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: bad precision");
+  if (int rc = check_potential(t, mode, precision, "gbx_eri_coop")) return rc;
   if (nquart == 0) return GBX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
@@ -567,6 +698,7 @@ extern "C" int gbx_eri_coop_acc(const gbx_tables* t, const gbx_program* prog, in
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: program / Boys table do not match the class");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: bad precision");
+  if (int rc = check_potential(t, GBX_MODE_PACKED, precision, "gbx_eri_coop_acc")) return rc;
   if (nquart == 0) return GBX_OK;
   hipStream_t s = static_cast<hipStream_t>(stream);
   const CoopLayout l =

@@ -18,6 +18,8 @@ extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t 
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg: no register kernel for this class");
+  if (t->potential != GBX_POT_COULOMB && mode == GBX_MODE_PACKED)
+    return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg: register kernels evaluate 1/r only");
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1 || t->boys_nt != gbx::kBoysRows)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table does not fit this class / kernel");
   if (nquart == 0) return GBX_OK;
@@ -39,6 +41,9 @@ extern "C" int gbx_eri_reg_segments(const gbx_tables* t, int32_t la, int32_t lb,
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg_segments: no register kernel for class");
+  if (t->potential != GBX_POT_COULOMB && mode == GBX_MODE_PACKED)
+    return gbx::fail(GBX_ERR_UNSUPPORTED,
+                     "gbx_eri_reg_segments: register kernels evaluate 1/r only");
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1 || t->boys_nt != gbx::kBoysRows)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: Boys table does not fit this class");
   if (max_quartets == 0) return GBX_OK;
@@ -62,6 +67,8 @@ extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: null argument");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: bad precision");
+  if (t->potential != GBX_POT_COULOMB)
+    return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_split: register kernels evaluate 1/r only");
   const int n = gbx_split_count(la, lb, lc, ld);
   if (split < 0 || split >= n)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_split: no such split for this class");

@@ -62,22 +62,35 @@ def _coord_types(basis, coord_type=None):
 
 
 def electron_repulsion_integral_packed(basis, coord_types=None, schwarz_threshold=0.0,
-                                       precision="fp64", fp32_cutoff=1e-2, device="cuda"):
+                                       precision="fp64", fp32_cutoff=1e-2, device="cuda",
+                                       potential="coulomb", omega=None):
     """Canonical 8-fold packed ERIs (float64 device tensor) of one basis.
 
     ``precision="mixed"`` runs shell quartets whose contracted [e0|f0] bound
     P_ij P_kl is below ``fp32_cutoff`` with an FP32 recursion bulk (Boys,
     prefactors, accumulation and the horizontal recursion stay FP64).
+    ``potential`` "erf" / "erfc" with ``omega`` evaluates the range-separated
+    operators erf(omega r)/r and erfc(omega r)/r, the Boys functions of
+    gbasis/integrals/boys_functions.py:145-223 (FP64 only; Schwarz screening
+    keeps the Coulomb bounds, as the reference's screener,
+    gbasis/integrals/electron_repulsion.py:475).
     Returns ``(packed, nfunc)``.
     """
     if precision not in ("fp64", "mixed"):
         raise ValueError("`precision` must be one of 'fp64' or 'mixed'")
+    if potential not in ("coulomb", "erf", "erfc"):
+        raise ValueError("`potential` must be one of 'coulomb', 'erf' or 'erfc'")
+    if potential != "coulomb":
+        if omega is None:
+            raise ValueError(f"omega parameter is required for {potential}-attenuated potential")
+        if precision != "fp64":
+            raise ValueError("range-separated potentials are evaluated in FP64 only")
     _check_basis(basis)
     basis = list(basis)
     ctypes_ = _coord_types(basis, coord_types)
     eng = get_engine(device)
     mol = MoleculeTables(basis, ctypes_)
-    dev = DeviceTables(BatchTables([mol]), device)
+    dev = DeviceTables(BatchTables([mol]), device, potential=potential, omega=omega)
     need_q = schwarz_threshold > 0 or precision == "mixed"
     bound = eng.pair_bounds(dev, eng.schwarz(dev)) if need_q else None
     plans = eng.plan_device(dev, bound, schwarz_threshold,
@@ -86,11 +99,13 @@ def electron_repulsion_integral_packed(basis, coord_types=None, schwarz_threshol
     return packed, mol.nfunc
 
 
-def _dense(basis, coord_types, notation, schwarz_threshold, transform, device="cuda"):
+def _dense(basis, coord_types, notation, schwarz_threshold, transform, device="cuda",
+           potential="coulomb", omega=None):
     if notation not in _NOTATIONS:
         raise ValueError("`notation` must be one of 'physicist' or 'chemist'")
     packed, nfunc = electron_repulsion_integral_packed(basis, coord_types, schwarz_threshold,
-                                                       device=device)
+                                                       device=device, potential=potential,
+                                                       omega=omega)
     eng = get_engine(device)
     if transform is None:
         dense = eng.unpack_dense(packed, nfunc, notation)
@@ -119,18 +134,25 @@ def electron_repulsion_integral(basis, transform=None, notation="physicist"):
 
 
 def electron_repulsion_integral_improved(basis, transform=None, notation="physicist",
-                                         schwarz_threshold=0.0):
-    """Dense ERIs with Schwarz screening (electron_repulsion.py:415-504)."""
+                                         schwarz_threshold=0.0, potential="coulomb", omega=None):
+    """Dense ERIs with Schwarz screening (electron_repulsion.py:415-504).  The keyword
+    arguments ``potential`` ("coulomb", "erf", "erfc") and ``omega`` select the
+    operator, as a ``boys_func = get_boys_function(potential, omega)`` override of
+    the reference's ElectronRepulsionIntegralImproved does
+    (boys_functions.py:290-345)."""
     if notation not in _NOTATIONS:
         raise ValueError("`notation` must be one of 'physicist' or 'chemist'")
     _check_basis(basis)
-    return _dense(basis, None, notation, schwarz_threshold, transform)
+    return _dense(basis, None, notation, schwarz_threshold, transform, potential=potential,
+                  omega=omega)
 
 
 class ElectronRepulsionIntegral:
     """Class interface of gbasis/integrals/electron_repulsion.py:20-211 (chemists' notation)."""
 
     schwarz_threshold = 0.0
+    potential = "coulomb"   # or "erf" / "erfc" with omega (ElectronRepulsionIntegralImproved)
+    omega = None
 
     def __init__(self, contractions):
         contractions = tuple(contractions)
@@ -139,7 +161,8 @@ class ElectronRepulsionIntegral:
 
     def _build(self, coord_types, **kwargs):
         thr = kwargs.get("schwarz_threshold", self.schwarz_threshold)
-        return _dense(self.contractions, coord_types, "chemist", thr, None)
+        return _dense(self.contractions, coord_types, "chemist", thr, None,
+                      potential=self.potential, omega=self.omega)
 
     def construct_array_cartesian(self, **kwargs):
         return self._build(["cartesian"] * len(self.contractions), **kwargs)
@@ -155,7 +178,7 @@ class ElectronRepulsionIntegral:
     def construct_array_lincomb(self, transform, coord_type, **kwargs):
         thr = kwargs.get("schwarz_threshold", self.schwarz_threshold)
         return _dense(self.contractions, _coord_types(self.contractions, coord_type), "chemist",
-                      thr, transform)
+                      thr, transform, potential=self.potential, omega=self.omega)
 
 
 class ElectronRepulsionIntegralImproved(ElectronRepulsionIntegral):

@@ -23,8 +23,8 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_SCHWARZ, GbxProgram,
-                   GbxTables, check, ptr)
+from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_SCHWARZ, GBX_POT_COULOMB,
+                   GBX_POT_ERF, GBX_POT_ERFC, GbxProgram, GbxTables, check, ptr)
 from .boys import BOYS_H, boys_table
 from .programs import class_program
 from .roofline import class_flops, quartet_flops
@@ -125,8 +125,18 @@ class DeviceTables:
 
     ALIGN = 256
 
-    def __init__(self, batch: BatchTables, device="cuda", mol_fp32_cutoff=None):
+    POTENTIALS = {"coulomb": GBX_POT_COULOMB, "erf": GBX_POT_ERF, "erfc": GBX_POT_ERFC}
+
+    def __init__(self, batch: BatchTables, device="cuda", mol_fp32_cutoff=None,
+                 potential="coulomb", omega=None):
         _require_gpu()
+        if potential not in self.POTENTIALS:
+            raise ValueError(f"potential must be one of {tuple(self.POTENTIALS)}")
+        if potential != "coulomb" and (omega is None or not omega > 0):
+            raise ValueError("omega > 0 is required for the erf and erfc potentials")
+        if potential != "coulomb" and mol_fp32_cutoff is not None:
+            raise ValueError("range-separated potentials are evaluated in FP64 only")
+        self.potential = potential
         self.batch = batch
         self.device = torch.device(device)
         tab = boys_table()
@@ -185,6 +195,8 @@ class DeviceTables:
         st.boys_nt = tab.shape[0]
         st.boys_nord = tab.shape[1]
         st.boys_h = BOYS_H
+        st.potential = self.POTENTIALS[potential]
+        st.omega = float(omega) if omega is not None else 0.0
         self.struct = st
         self.queue = {}   # work-queue buffers of plan_device, per layout
 
@@ -349,6 +361,17 @@ class EriEngine:
         kernel.  ``count``: device int32 holding the number of quartets (n is then
         the capacity).
         """
+        if dev.potential != "coulomb":
+            # range-separated operators: cooperative kernels, FP64 (Schwarz bounds stay
+            # Coulomb inside the kernels)
+            path = "coop"
+            if precision != GBX_FP64:
+                raise ValueError("range-separated potentials are evaluated in FP64 only")
+        if seg is not None and (path == "coop" or dense or not self.has_reg(cls)):
+            # a fused launch covers several primitive bins of the queue: only the register
+            # kernels read its segment table
+            raise ValueError(f"fused launch of class {cls} planned for a register kernel "
+                             "cannot run on the cooperative kernel; plan again")
         if path != "coop" and not dense and self.has_reg(cls):
             if seg is not None:
                 check(self.lib.gbx_eri_reg_segments(ctypes.byref(dev.struct), *cls, ptr(quartets),
@@ -569,7 +592,8 @@ class EriEngine:
             c = head >> 2
             cls = ((c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7)
             prec = GBX_FP32_BULK if fp32 else GBX_FP64
-            fused = (not dense) and self.path != "coop" and self.has_reg(cls)
+            fused = ((not dense) and self.path != "coop" and dev.potential == "coulomb"
+                     and self.has_reg(cls))
             fpr, fco = class_flops(*cls, pure=dense) if stats else (0, 0)
             if fused:
                 base = head << 4

@@ -35,6 +35,13 @@ extern "C" {
 #define GBX_MODE_SCHWARZ 1 /* per bra pair p: d_out[2p] = sqrt(max|(ij|ij)|),
                               d_out[2p+1] = sqrt(max|contracted [e0|f0]|)  */
 
+/* two-electron operator (gbx_tables.potential); the range-separated ones use
+ * gbx_tables.omega, as gbasis/integrals/boys_functions.py:145-223 */
+#define GBX_POT_COULOMB 0 /* 1/r                                              */
+#define GBX_POT_ERF 1     /* erf(omega r)/r: G_m = s^(m+1/2) F_m(s T),
+                             s = omega^2 / (omega^2 + rho)                  */
+#define GBX_POT_ERFC 2    /* erfc(omega r)/r: F_m(T) - G_m                   */
+
 /* Device-resident basis tables of one batch of molecules. */
 typedef struct gbx_tables {
   const int32_t *shell_info;     /* 8 x int32 per shell: func_off, l, kind, map_off,
@@ -56,6 +63,15 @@ typedef struct gbx_tables {
   int32_t boys_nt;
   int32_t boys_nord;
   double boys_h;
+  int32_t potential;             /* GBX_POT_*: operator of GBX_MODE_PACKED launches;
+                                    GBX_MODE_SCHWARZ bounds are always Coulomb, as
+                                    the reference's screener
+                                    (gbasis/integrals/electron_repulsion.py:475).
+                                    Range-separated operators run on the
+                                    cooperative kernels in FP64 only; the
+                                    register kernels reject them.                 */
+  int32_t reserved;
+  double omega;                  /* range-separation parameter (> 0)            */
 } gbx_tables;
 
 /* A class program (gbasis_amd/programs.py) resident on the device. */

@@ -26,18 +26,20 @@ from .boys import boys_longdouble
 from .os_hgp import os_hgp_quartet
 
 
-def _engine(s1, s2, s3, s4, exact=False):
+def _engine(s1, s2, s3, s4, exact=False, potential="coulomb", omega=None):
     if exact:
         return os_hgp_quartet(
             s1.coord, s1.angmom, s1.angmom_components_cart, s1.exps, s1.coeffs,
             s2.coord, s2.angmom, s2.angmom_components_cart, s2.exps, s2.coeffs,
             s3.coord, s3.angmom, s3.angmom_components_cart, s3.exps, s3.coeffs,
             s4.coord, s4.angmom, s4.angmom_components_cart, s4.exps, s4.coeffs,
-            boys=boys_longdouble, dtype=np.longdouble, ket="vrr").astype(np.float64)
+            boys=boys_longdouble, dtype=np.longdouble, ket="vrr", potential=potential,
+            omega=omega).astype(np.float64)
     return os_hgp_quartet(s1.coord, s1.angmom, s1.angmom_components_cart, s1.exps, s1.coeffs,
                           s2.coord, s2.angmom, s2.angmom_components_cart, s2.exps, s2.coeffs,
                           s3.coord, s3.angmom, s3.angmom_components_cart, s3.exps, s3.coeffs,
-                          s4.coord, s4.angmom, s4.angmom_components_cart, s4.exps, s4.coeffs)
+                          s4.coord, s4.angmom, s4.angmom_components_cart, s4.exps, s4.coeffs,
+                          potential=potential, omega=omega)
 
 
 def schwarz_bounds(basis, exact=False):
@@ -51,7 +53,7 @@ def schwarz_bounds(basis, exact=False):
     return Q
 
 
-def contraction_block(s1, s2, s3, s4, exact=False):
+def contraction_block(s1, s2, s3, s4, exact=False, potential="coulomb", omega=None):
     """(M1,L1,M2,L2,M3,L3,M4,L4) Cartesian block with the reference's l-reordering.
 
     ``exact=True`` evaluates the integrals in x87 extended precision with an
@@ -71,7 +73,7 @@ def contraction_block(s1, s2, s3, s4, exact=False):
     if bk:
         s1, s3 = s3, s1
         s2, s4 = s4, s2
-    out = _engine(s1, s2, s3, s4, exact)
+    out = _engine(s1, s2, s3, s4, exact, potential, omega)
     if bk:
         out = np.transpose(out, (2, 3, 0, 1, 6, 7, 4, 5))
     if ket:
@@ -89,7 +91,8 @@ def _shell_out_matrix(shell, coord_type):
     return np.eye(len(shell.angmom_components_cart))
 
 
-def normalized_block(basis, coord_types, i, j, k, l, Q=None, threshold=0.0, exact=False):
+def normalized_block(basis, coord_types, i, j, k, l, Q=None, threshold=0.0, exact=False,
+                     potential="coulomb", omega=None):
     """Final (n1, n2, n3, n4) block (segmented contractions flattened m-major).
 
     Screened quartets (Q given and Q_ij Q_kl < threshold) are zero blocks.
@@ -99,7 +102,7 @@ def normalized_block(basis, coord_types, i, j, k, l, Q=None, threshold=0.0, exac
     if Q is not None and threshold > 0 and Q[i, j] * Q[k, l] < threshold:
         sizes = [s.coeffs.shape[1] * m.shape[0] for s, m in zip(shells, mats)]
         return np.zeros(sizes)
-    blk = contraction_block(*shells, exact=exact)
+    blk = contraction_block(*shells, exact=exact, potential=potential, omega=omega)
     for axis, s in enumerate(shells):
         nc = np.asarray(s.norm_cont)
         shape = [1] * 8
@@ -122,8 +125,11 @@ def basis_offsets(basis, coord_types):
     return off
 
 
-def dense_eri(basis, notation="physicist", schwarz_threshold=0.0, coord_types=None, exact=False):
-    """Dense (N,N,N,N) ERI tensor as the reference assembles it."""
+def dense_eri(basis, notation="physicist", schwarz_threshold=0.0, coord_types=None, exact=False,
+              potential="coulomb", omega=None):
+    """Dense (N,N,N,N) ERI tensor as the reference assembles it.  ``potential`` "erf" /
+    "erfc" with ``omega``: the range-separated operators (Schwarz bounds stay Coulomb,
+    gbasis/integrals/electron_repulsion.py:475)."""
     if notation not in ("physicist", "chemist"):
         raise ValueError("`notation` must be one of 'physicist' or 'chemist'")
     basis = list(basis)
@@ -136,7 +142,8 @@ def dense_eri(basis, notation="physicist", schwarz_threshold=0.0, coord_types=No
     pairs = list(it.combinations_with_replacement(range(len(basis)), 2))
     for p, (i, j) in enumerate(pairs):
         for (k, l) in pairs[p:]:
-            blk = normalized_block(basis, coord_types, i, j, k, l, Q, schwarz_threshold, exact)
+            blk = normalized_block(basis, coord_types, i, j, k, l, Q, schwarz_threshold, exact,
+                                   potential, omega)
             si, sj, sk, sl = (slice(off[x], off[x + 1]) for x in (i, j, k, l))
             out[si, sj, sk, sl] = blk
             out[si, sj, sl, sk] = blk.transpose(0, 1, 3, 2)

@@ -57,11 +57,26 @@ def _plus(comp, i):
     return tuple(c)
 
 
+def boys_potential(boys, orders, T, rho, potential="coulomb", omega=None, dtype=np.float64):
+    """Boys values of the operator: F_m(T) for 1/r; for erf(omega r)/r
+    G_m = s^(m+1/2) F_m(s T) with s = omega^2 / (omega^2 + rho), and F_m(T) - G_m for
+    erfc(omega r)/r (gbasis/integrals/boys_functions.py:145-223)."""
+    if potential == "coulomb":
+        return boys(orders, T).astype(dtype)
+    if potential not in ("erf", "erfc") or omega is None:
+        raise ValueError("potential must be 'coulomb', or 'erf' / 'erfc' with omega")
+    w2 = dtype(omega) * dtype(omega)
+    s = w2 / (w2 + rho)
+    g = s ** (np.asarray(orders, dtype=dtype) + dtype(0.5)) * boys(orders, s * T).astype(dtype)
+    return g if potential == "erf" else boys(orders, T).astype(dtype) - g
+
+
 def os_hgp_quartet(coord_a, la, comps_a, exps_a, coeffs_a,
                    coord_b, lb, comps_b, exps_b, coeffs_b,
                    coord_c, lc, comps_c, exps_c, coeffs_c,
                    coord_d, ld, comps_d, exps_d, coeffs_d,
-                   boys=boys_hyp1f1, dtype=np.float64, ket="etr"):
+                   boys=boys_hyp1f1, dtype=np.float64, ket="etr", potential="coulomb",
+                   omega=None):
     """Contracted Cartesian ERI block in chemists' notation.
 
     Returns an array of shape ``(n_a, n_b, n_c, n_d, M_a, M_b, M_c, M_d)`` with
@@ -107,7 +122,8 @@ def os_hgp_quartet(coord_a, la, comps_a, exps_a, coeffs_a,
     h2e = 0.5 / eta
 
     # 2-3. Boys * prefactor and VRR: V[a] has shape (L - |a| + 1, nprim)
-    fm = boys(np.arange(L + 1)[:, None], T[None, :]).astype(dtype) * pref[None, :]
+    fm = boys_potential(boys, np.arange(L + 1)[:, None], T[None, :], rho[None, :], potential,
+                        omega, dtype) * pref[None, :]
     V = {(0, 0, 0): fm}
     for n in range(1, L + 1):
         for a in _comps(n):

@@ -42,3 +42,19 @@ def test_error_path_without_device():
     assert b"null" in lib.gbx_last_error()
     assert lib.gbx_has_reg_kernel(0, 0, 0, 0) == 1
     assert lib.gbx_has_reg_kernel(4, 4, 4, 4) == 0
+
+
+def test_range_separated_argument_errors():
+    """Operator checks happen before any device work (CPU)."""
+    import pytest
+
+    from gbasis_amd.electron_repulsion import electron_repulsion_integral_packed
+    from tests.molecules import random_basis
+
+    basis = random_basis(0, [0, 1])
+    with pytest.raises(ValueError):
+        electron_repulsion_integral_packed(basis, potential="yukawa")
+    with pytest.raises(ValueError):
+        electron_repulsion_integral_packed(basis, potential="erf")
+    with pytest.raises(ValueError):
+        electron_repulsion_integral_packed(basis, potential="erfc", omega=0.3, precision="mixed")

@@ -65,6 +65,28 @@ def test_dense_vs_oracle(gpu, ls, coord):
     assert np.all(np.abs(got - ref64) <= ref_err + 1e-12 * scale)
 
 
+@pytest.mark.parametrize("potential,omega", [("erf", 0.4), ("erfc", 0.4), ("erf", 2.5)])
+def test_range_separated_vs_oracle(gpu, potential, omega):
+    """erf / erfc-attenuated operators (cooperative kernels, FP64) against the oracle's
+    extended-precision evaluation, and erf + erfc = 1/r on the device."""
+    from gbasis_amd.electron_repulsion import (electron_repulsion_integral,
+                                               electron_repulsion_integral_improved)
+    from oracle.assembly import dense_eri
+
+    _print_case(f"range-separated {potential} omega={omega}")
+    basis = random_basis(91, [0, 1, 2, 3], kmax=3, coord_type="spherical")
+    got = electron_repulsion_integral_improved(basis, notation="chemist", potential=potential,
+                                               omega=omega)
+    exact = dense_eri(basis, notation="chemist", exact=True, potential=potential, omega=omega)
+    scale = max(1.0, np.abs(exact).max())
+    assert np.abs(got - exact).max() <= 1e-12 * scale
+    other = "erfc" if potential == "erf" else "erf"
+    got2 = electron_repulsion_integral_improved(basis, notation="chemist", potential=other,
+                                                omega=omega)
+    coul = electron_repulsion_integral(basis, notation="chemist")
+    assert np.abs(got + got2 - coul).max() <= 1e-12 * scale
+
+
 def test_schwarz_bounds_and_screened_lists(gpu):
     """Q_ij on the device vs the oracle's screener; screened quartet sets identical."""
     from gbasis_amd.engine import DeviceTables, get_engine

@@ -95,3 +95,67 @@ def test_boys_closed_forms(m):
         for t in T[1:]:
             assert abs(float(boys_longdouble(0, t)) - 0.5 * np.sqrt(np.pi / t) * erf(np.sqrt(t))) \
                 <= 2e-16
+
+
+def _s_shell(exp, centre):
+    from gbasis_amd.contractions import GeneralizedContractionShell
+
+    return GeneralizedContractionShell(0, np.asarray(centre, dtype=float), np.array([1.0]),
+                                       np.array([exp]), "cartesian")
+
+
+@pytest.mark.parametrize("omega", [0.1, 0.7, 3.0])
+def test_erf_ssss_gaussian_charge_closed_form(omega):
+    """(ss|ss) of erf(omega r)/r for normalised s primitives equals the interaction of two
+    Gaussian charge clouds through a third Gaussian smearing of exponent omega^2:
+    Q1 Q2 erf(mu R) / R with 1/mu^2 = 1/p + 1/q + 1/omega^2 (R = |P - Q|), an independent
+    physical closed form of the oracle's G_m = s^(m+1/2) F_m(s T)."""
+    from math import erf, pi, sqrt
+
+    from oracle.assembly import dense_eri
+
+    a, b, c, d = 1.3, 0.45, 0.8, 2.1
+    A, B, C, D = np.array([0.1, 0.2, -0.3]), np.array([0.9, -0.4, 0.2]), \
+        np.array([-0.7, 1.1, 0.5]), np.array([0.3, 0.6, 1.4])
+    basis = [_s_shell(a, A), _s_shell(b, B), _s_shell(c, C), _s_shell(d, D)]
+    for potential in ("erf", "erfc", "coulomb"):
+        got = dense_eri(basis, "chemist", potential=potential,
+                        omega=omega if potential != "coulomb" else None)[0, 1, 2, 3]
+        p, q = a + b, c + d
+        P, Q = (a * A + b * B) / p, (c * C + d * D) / q
+        q1 = (2 * a / pi) ** 0.75 * (2 * b / pi) ** 0.75 * np.exp(-a * b / p * np.sum((A - B) ** 2)) \
+            * (pi / p) ** 1.5
+        q2 = (2 * c / pi) ** 0.75 * (2 * d / pi) ** 0.75 * np.exp(-c * d / q * np.sum((C - D) ** 2)) \
+            * (pi / q) ** 1.5
+        R = float(np.linalg.norm(P - Q))
+        mu_c = 1.0 / sqrt(1 / p + 1 / q)
+        mu_e = 1.0 / sqrt(1 / p + 1 / q + 1 / omega ** 2)
+        coul = q1 * q2 * erf(mu_c * R) / R
+        ref = {"coulomb": coul, "erf": q1 * q2 * erf(mu_e * R) / R,
+               "erfc": coul - q1 * q2 * erf(mu_e * R) / R}[potential]
+        assert abs(got - ref) <= 1e-14 * max(1.0, abs(coul)), (potential, got, ref)
+
+
+def test_erf_p_shell_from_centre_derivative():
+    """(p_x s|ss) of erf(omega r)/r from the s closed form differentiated in A_x:
+    for a normalised primitive, (p_x s|ss) = a^(-1/2) d/dA_x (ss|ss) -- pins the
+    recursion above the Boys level for the attenuated operator."""
+    from gbasis_amd.contractions import GeneralizedContractionShell
+    from oracle.assembly import dense_eri
+
+    omega, a, h = 0.6, 0.9, 1e-4
+    A = np.array([0.2, -0.1, 0.3])
+    others = [_s_shell(0.5, [1.0, 0.4, -0.2]), _s_shell(1.7, [-0.6, 0.8, 0.1]),
+              _s_shell(0.7, [0.4, -0.9, 0.6])]
+    pshell = GeneralizedContractionShell(1, A, np.array([1.0]), np.array([a]), "cartesian")
+    got = dense_eri([pshell] + others, "chemist", potential="erf", omega=omega)
+    for ax in range(3):
+        dA = np.zeros(3)
+        dA[ax] = h
+        fp = dense_eri([_s_shell(a, A + dA)] + others, "chemist", potential="erf",
+                       omega=omega)[0, 1, 2, 3]
+        fm = dense_eri([_s_shell(a, A - dA)] + others, "chemist", potential="erf",
+                       omega=omega)[0, 1, 2, 3]
+        ref = a ** -0.5 * (fp - fm) / (2 * h)
+        # p components in the default order x, y, z occupy functions 0..2
+        assert abs(got[ax, 3, 4, 5] - ref) <= 1e-8 * max(1.0, abs(ref)), (ax, got[ax, 3, 4, 5], ref)
