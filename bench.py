@@ -22,6 +22,7 @@ import sys
 import time
 
 import numpy as np
+from threadpoolctl import threadpool_limits
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
@@ -290,10 +291,11 @@ def roofline_entry(plans, per_bucket, prefer_reg=True):
 
 
 def cpu_baseline(batch, plans_fp64, budget_s=20.0):
-    """Oracle (NumPy restatement of the reference engine) timed on a stratified sample.
+    """Oracle (NumPy restatement of the reference engine) timed on a stratified sample,
+    on one core (BLAS limited to one thread).
 
     Classes are visited from the cheapest (algorithmic FLOPs per quartet) up; each
-    gets up to 8 random quartets within its share of the budget.  Classes left
+    gets up to 64 random quartets within its share of the budget.  Classes left
     when the budget runs out are extrapolated with the measured oracle seconds per
     algorithmic FLOP of the sampled ones."""
     from oracle.assembly import contraction_block
@@ -324,7 +326,7 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
             continue
         quart = np.concatenate([bk.host_quartets() for bk in bks])
         times = []
-        for _ in range(8):
+        for _ in range(64):
             q = quart[rng.integers(0, quart.shape[0])]
             mi = pair_mol[q[0]]
             m = batch.mols[mi]
@@ -350,8 +352,7 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
                       + (f" and, for the {skipped} costliest classes, by the algorithmic FLOP "
                          "model at the sampled oracle rate" if skipped else "")
                       + "; oracle = NumPy restatement of the reference OS+HGP engine, one "
-                      "process, OpenBLAS threads="
-                      f"{os.environ.get('OPENBLAS_NUM_THREADS', 'default')}"}
+                      "process, BLAS limited to one thread"}
 
 
 class _OracleShell:
@@ -562,7 +563,8 @@ def config_bench(args):
         "classes_mixed": class_table(lastmx, pbmx),
     }
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(batch, plans64)
+        with threadpool_limits(limits=1):
+            result["cpu_baseline"] = cpu_baseline(batch, plans64)
     print(json.dumps(result), flush=True)
 
 
@@ -714,7 +716,8 @@ def main():
         "classes_mixed": class_table(lastmx, pbmx),
     }
     if not args.no_cpu:
-        result["cpu_baseline"] = cpu_baseline(batch, plans64)
+        with threadpool_limits(limits=1):
+            result["cpu_baseline"] = cpu_baseline(batch, plans64)
     if not args.no_serving:
         del dev, out64
         torch.cuda.empty_cache()

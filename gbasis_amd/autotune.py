@@ -8,7 +8,8 @@ quartets and keeps the fastest:
   bulk (bound below the cutoff) is timed in FP32 and in FP64.  Both meet the
   error bound, so the faster one is kept.  (On CDNA4 non-packed FP32 FMA runs
   at the FP64 vector rate, so FP32 wins only where it relieves registers or LDS.)
-* split classes: lanes per shell quartet R in {1, 2, 4, ..., 64};
+* split classes: lanes per shell quartet R in {1, 2, 4, ..., 64} with either
+  horizontal-recursion pass, or the cooperative kernel instead;
 * cooperative classes: primitive quartets in flight PF.
 
 Choices are cached in memory per (class, bin, dense, precision class) and can be
@@ -76,12 +77,14 @@ class Autotuner:
                 elif eng.split_count(bk.cls) > 0 and eng.path != "coop":
                     hrrs = (0, 1) if (eng.has_hrr(bk.cls) and not bk.dense) else (0,)
                     variants = [("R", r, h) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq
-                                for h in hrrs]
+                                for h in hrrs] + [("coop",)]
                 for var in variants:
                     def run(prec=prec, var=var):
+                        coop = bool(var) and var[0] == "coop"
                         eng.launch_bucket(dev, bk, out, precision=prec,
-                                          lanes=var[1] if var else None,
-                                          hrr=bool(var[2]) if var and len(var) > 2 else None)
+                                          lanes=var[1] if var and not coop else None,
+                                          hrr=bool(var[2]) if var and len(var) > 2 else None,
+                                          path="coop" if coop else None)
                     t = self._time(run)
                     if prec == GBX_FP32_BULK:
                         # keep FP32 only for a clear win: an FP64 bucket also merges with

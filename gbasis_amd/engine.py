@@ -46,7 +46,8 @@ class Bucket:
     flops: float         # algorithmic FLOPs (roofline.quartet_flops)
     path: str = ""       # "reg", "split" or "coop" once launched
     run_precision: int = -1   # precision actually launched (autotuner); -1: = precision
-    variant: tuple = None     # autotuned variant, e.g. ("R", lanes per quartet)
+    variant: tuple = None     # autotuned variant: ("R", lanes per quartet, register HRR)
+                              # or ("coop",) for the cooperative kernel
     seg: tuple = None         # fused classes: (device segment offsets, device counts, max lanes)
     count: object = None      # device int32 count (n is then the capacity); None: n exact
     key: tuple = None         # ("head", class/dense/precision head) for a fused launch,
@@ -345,10 +346,12 @@ class EriEngine:
         return self._reg[cls]
 
     def launch_bucket(self, dev, bk, out, mode=GBX_MODE_PACKED, precision=None, lanes=None,
-                      hrr=None):
+                      hrr=None, path=None):
         prec = bk.launch_precision() if precision is None else precision
+        if path is None:
+            path = "coop" if bk.variant and bk.variant[0] == "coop" else self.path
         return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
-                                 self.path, lanes if lanes is not None else bk.lanes(), bk.seg,
+                                 path, lanes if lanes is not None else bk.lanes(), bk.seg,
                                  bk.count, hrr if hrr is not None else bk.hrr())
 
     def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,

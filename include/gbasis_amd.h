@@ -103,8 +103,7 @@ int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t 
                        int64_t *lds_bytes, int64_t *global_bytes_per_block);
 
 /* One angular-momentum class of shell quartets with the cooperative
- * (one workgroup per shell quartet: 64 lanes, or 256-1024 for LDS workspaces
- * above 40 KiB) kernel.  d_quartets holds nquart (bra pair, ket pair)
+ * (one workgroup of 256-1024 lanes per shell quartet) kernel.  d_quartets holds nquart (bra pair, ket pair)
  * int32 pairs in canonical orientation.  Replaces, for a batch of quartets,
  * ElectronRepulsionIntegralImproved.construct_array_contraction
  * (gbasis/integrals/electron_repulsion.py:305-412) ->

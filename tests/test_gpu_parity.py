@@ -4,6 +4,7 @@ import os
 
 import numpy as np
 import pytest
+import torch
 
 from tests.molecules import random_basis
 
@@ -195,6 +196,36 @@ def test_register_vs_cooperative(gpu):
     finally:
         eng.path = "auto"
     assert float((a - b).abs().max()) <= 1e-13 * max(1.0, float(a.abs().max()))
+
+
+def test_autotuned_variants_match_default(gpu):
+    """Every variant the autotuner may pick gives the default launch's packed array:
+    the tuned plan, and the split classes forced onto the cooperative kernel."""
+    from gbasis_amd.autotune import Autotuner
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import batch_molecule
+
+    _print_case("autotuned variants")
+    eng = get_engine()
+    mol = MoleculeTables(batch_molecule(11, n_atoms=5))
+    dev = DeviceTables(BatchTables([mol]))
+    plans = eng.plan_device(dev, None, 0.0, 0.0, stats=True)
+    ref = eng.run(dev, plans)
+    scale = max(1.0, float(ref.abs().max()))
+    tuner = Autotuner(eng, repeats=1)
+    scratch = torch.zeros_like(ref)
+    tuner.tune(dev, plans, scratch)
+    got = eng.run(dev, plans)
+    assert float((got - ref).abs().max()) <= 1e-13 * scale
+    forced = 0
+    for bk in plans:
+        if eng.split_count(bk.cls) > 0 and not bk.dense:
+            bk.variant = ("coop",)
+            forced += 1
+    assert forced > 0
+    got = eng.run(dev, plans)
+    assert float((got - ref).abs().max()) <= 1e-13 * scale
 
 
 def test_device_queue_matches_host(gpu):
