@@ -128,6 +128,11 @@ def time_graph(eng, dev, bound, threshold, cutoff, steps, tuner=None):
     return a.elapsed_time(b) * 1e-3 / steps, out
 
 
+def _host_busy(srv, span):
+    """Fraction of the wall span the serving thread spent building tables / enqueueing."""
+    return {k: v / span for k, v in srv.host_seconds.items()}
+
+
 def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4):
     """Serving workload: requests of 2-30 atoms (seeds 4000 + i, s/p/d, K <= 6),
     precision FP64 or mixed with probability 1/2 each.  First the saturated rate
@@ -190,19 +195,24 @@ def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4):
         torch.cuda.synchronize()
         return total
 
+    srv.host_seconds = {"tables": 0.0, "launch": 0.0}
     sat_reqs, sat_span = run(n_sat, None)
     sat_rate = n_sat / sat_span
     sat_eris = eris_of(sat_reqs)
+    hb = _host_busy(srv, sat_span)
     log(f"serving saturated: {sat_rate:.1f} req/s, {sat_eris / sat_span:.3e} ERIs/s "
-        f"({n_sat} requests, {srv.batches_launched} batches so far)")
+        f"({n_sat} requests, {srv.batches_launched} batches so far); host busy "
+        f"{hb['tables']:.0%} tables + {hb['launch']:.0%} launch")
     out = {"workload": "requests of 2-30 atoms (s/p/d Cartesian, K 1-6, exps [0.05,1e3]), "
                        "FP64 or mixed with probability 1/2, Schwarz threshold "
                        f"{threshold:g}; result = packed ERIs resident in HBM",
            "streams": n_streams, "saturated": {"requests": n_sat, "req_per_s": sat_rate,
                                                "eris_per_s": sat_eris / sat_span}}
     all_eris = None
+    out["saturated"]["host_busy"] = _host_busy(srv, sat_span)
     for frac in (0.5, 0.8):
         b0 = srv.batches_launched
+        srv.host_seconds = {"tables": 0.0, "launch": 0.0}
         reqs, span = run(n_requests, frac * sat_rate)
         if all_eris is None:
             all_eris = eris_of(reqs)
@@ -211,9 +221,10 @@ def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4):
                "p50_ms": lat["p50"] * 1e3, "p99_ms": lat["p99"] * 1e3,
                "sustained_req_per_s": n_requests / span,
                "sustained_eris_per_s": all_eris / span,
-               "batches": srv.batches_launched - b0}
+               "batches": srv.batches_launched - b0, "host_busy": _host_busy(srv, span)}
         log(f"serving at {frac:.0%} ({frac * sat_rate:.1f} req/s): p50 {res['p50_ms']:.1f} ms, "
-            f"p99 {res['p99_ms']:.1f} ms, {res['sustained_eris_per_s']:.3e} ERIs/s")
+            f"p99 {res['p99_ms']:.1f} ms, {res['sustained_eris_per_s']:.3e} ERIs/s; host busy "
+            f"{res['host_busy']['tables']:.0%} tables + {res['host_busy']['launch']:.0%} launch")
         out[f"at_{int(frac * 100)}pct"] = res
     del srv
     torch.cuda.empty_cache()

@@ -161,13 +161,17 @@ class EriServer:
         self.clock = clock
         self._next_id = 0
         self.batches_launched = 0
+        # host seconds spent building request tables (submit) and enqueueing batches
+        self.host_seconds = {"tables": 0.0, "launch": 0.0}
 
     # ------------------------------------------------------------------
     def submit(self, basis, precision="fp64", t_arrival=None, dispatch=True):
         """Queue a request (host tables are built here) and, with ``dispatch``, launch
         what fits on idle streams.  Completions are reported by ``step`` only."""
         t = self.clock() if t_arrival is None else t_arrival
+        h0 = time.perf_counter()
         req = EriRequest(self._next_id, basis, precision, t)
+        self.host_seconds["tables"] += time.perf_counter() - h0
         self._next_id += 1
         if req.footprint > self.budget:
             raise ValueError(f"request needs {req.footprint} bytes of HBM, more than the "
@@ -249,7 +253,9 @@ class EriServer:
             if not take:
                 return   # head of the queue waits for memory to be released
             self.used += need
+            h0 = time.perf_counter()
             self.inflight[i] = self._launch(take, self.streams[i], region)
+            self.host_seconds["launch"] += time.perf_counter() - h0
 
     def _launch(self, reqs, stream, region):
         eng = self.eng
