@@ -193,20 +193,21 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
                     [&](int m, double v) { S[m * PF + tid] = static_cast<TB>(v * pref); });
         } else {
           // range-separated operators (FP64 only, checked on the host):
-          // G_m = s^(m+1/2) F_m(s T), s = omega^2 / (omega^2 + rho)
+          // G_m = att^(m+1/2) F_m(att T), att = omega^2 / (omega^2 + rho)
           const double w2 = t.omega * t.omega;
           const double rho = rb[0] * rk[0] / (rb[0] + rk[0]);
-          const double sc = w2 / (w2 + rho);
-          const double sq = sqrt(sc);
-          const bool erfc = t.potential == GBX_POT_ERFC;
-          if (erfc)
+          const double att = w2 / (w2 + rho);
+          const double att_sqrt = sqrt(att);
+          const bool complement = t.potential == GBX_POT_ERFC;
+          if (complement)
             boys_eval(Tval, a.L, t,
                       [&](int m, double v) { S[m * PF + tid] = static_cast<TB>(v * pref); });
-          boys_eval(sc * Tval, a.L, t, [&](int m, double v) {
-            double pw = sq;
-            for (int i = 0; i < m; ++i) pw *= sc;
+          boys_eval(att * Tval, a.L, t, [&](int m, double v) {
+            double pw = att_sqrt;
+            for (int i = 0; i < m; ++i) pw *= att;
             const double g = v * pw * pref;
-            S[m * PF + tid] = erfc ? static_cast<TB>(S[m * PF + tid] - g) : static_cast<TB>(g);
+            S[m * PF + tid] =
+                complement ? static_cast<TB>(S[m * PF + tid] - g) : static_cast<TB>(g);
           });
         }
       }
