@@ -38,6 +38,7 @@ constexpr int kCoopThreads = 64;
 constexpr int kCoopGlobalThreads = 256;  // lanes of a global-scratch workgroup
 constexpr int kPhVrr = 0, kPhKvrr = 1, kPhAcc = 2, kPhHrrK = 3, kPhHrrB = 4;
 constexpr int kScalars = 17;  // per-primitive-quartet scalars held in LDS
+constexpr int kNoSlot = 0xFFFF;  // absent source of a recursion item
 
 struct CoopArgs {
   const int32_t* quart;
@@ -127,7 +128,8 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
   __shared__ double s_red2[NT / 64];
 
   const int tid = threadIdx.x;
-  const int PF = a.pf;
+  const int PF = a.pf;  // a power of two (checked on the host)
+  const int pf_shift = 31 - __builtin_clz(static_cast<unsigned>(PF));
   const int4* items4 = reinterpret_cast<const int4*>(a.items);
   const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
 
@@ -212,34 +214,53 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
         }
       }
       __syncthreads();
+      // NT is a multiple of PF, so a thread works on the same primitive copy c in every
+      // phase: the recursion scalars of the current phase kind live in registers
+      // (A-centre VRR: P-A, W-P, rho/zeta, 1/(2 zeta); C-centre build: Q-C, W-Q,
+      // rho/eta, 1/(2 eta), 1/(2 (zeta + eta)))
+      const int c = tid & (PF - 1);
+      const bool active = c < ncopy;
+      TB r_p[3] = {0, 0, 0}, r_w[3] = {0, 0, 0};
+      TB r_ro = 0, r_i2 = 0, r_i2s = 0;
+      int loaded = -1;
       for (int ph = 0; ph < a.n_prim_ph; ++ph) {
         const int kind = a.phases[3 * ph], i0 = a.phases[3 * ph + 1], i1 = a.phases[3 * ph + 2];
-        const int nwork = (i1 - i0) * PF;
-        for (int x = tid; x < nwork; x += NT) {
-          const int it = i0 + x / PF;
-          const int c = x - (x / PF) * PF;
-          if (c >= ncopy) continue;
-          const int4 d0 = items4[2 * it];
-          const int4 d1 = items4[2 * it + 1];
-          const int code = d1.z;
-          const int dir = code & 15;
-          const TB k1 = static_cast<TB>((code >> 4) & 255);
-          TB v;
-          if (kind == kPhVrr) {
-            // [a+1_i]^(m) = PA_i [a]^(m) + WP_i [a]^(m+1) + a_i/(2z) ([a-1_i]^(m) - rho/z [a-1_i]^(m+1))
-            v = sc[dir * PF + c] * S[d0.y * PF + c] + sc[(3 + dir) * PF + c] * S[d0.z * PF + c];
-            if (d0.w >= 0)
-              v += k1 * sc[7 * PF + c] * (S[d0.w * PF + c] - sc[6 * PF + c] * S[d1.x * PF + c]);
-          } else {
-            // [a|c+1_i]^(m) = QC_i [a|c]^(m) + WQ_i [a|c]^(m+1)
-            //   + c_i/(2e) ([a|c-1_i]^(m) - rho/e [a|c-1_i]^(m+1)) + a_i/(2(z+e)) [a-1_i|c]^(m+1)
-            v = sc[(8 + dir) * PF + c] * S[d0.y * PF + c] +
-                sc[(11 + dir) * PF + c] * S[d0.z * PF + c];
-            if (d0.w >= 0)
-              v += k1 * sc[15 * PF + c] * (S[d0.w * PF + c] - sc[14 * PF + c] * S[d1.x * PF + c]);
-            if (d1.y >= 0) v += static_cast<TB>(code >> 12) * sc[16 * PF + c] * S[d1.y * PF + c];
+        const int nwork = (i1 - i0) << pf_shift;
+        if (active) {
+          if (kind != loaded) {
+            const int o = kind == kPhVrr ? 0 : 8;
+#pragma unroll
+            for (int i = 0; i < 3; ++i) {
+              r_p[i] = sc[(o + i) * PF + c];
+              r_w[i] = sc[(o + 3 + i) * PF + c];
+            }
+            r_ro = sc[(o + 6) * PF + c];
+            r_i2 = sc[(o + 7) * PF + c];
+            r_i2s = sc[16 * PF + c];
+            loaded = kind;
           }
-          S[d0.x * PF + c] = v;
+          for (int x = tid; x < nwork; x += NT) {
+            const int it = i0 + (x >> pf_shift);
+            // one 16-byte item: 16-bit slots dst | s0, s1 | s2, s3 | s4 (0xFFFF: none), code
+            const int4 d = items4[it];
+            const uint32_t w0 = static_cast<uint32_t>(d.x), w1 = static_cast<uint32_t>(d.y),
+                           w2 = static_cast<uint32_t>(d.z);
+            const int dst = w0 & 0xFFFF, s0 = w0 >> 16, s1 = w1 & 0xFFFF, s2 = w1 >> 16;
+            const int s3 = w2 & 0xFFFF, s4 = w2 >> 16;
+            const int code = d.w;
+            const int dir = code & 15;
+            const TB k1 = static_cast<TB>((code >> 4) & 255);
+            const TB pd = dir == 0 ? r_p[0] : (dir == 1 ? r_p[1] : r_p[2]);
+            const TB wd = dir == 0 ? r_w[0] : (dir == 1 ? r_w[1] : r_w[2]);
+            // A centre: [a+1_i]^(m) = PA_i [a]^(m) + WP_i [a]^(m+1)
+            //   + a_i/(2z) ([a-1_i]^(m) - rho/z [a-1_i]^(m+1))
+            // C centre: [a|c+1_i]^(m) = QC_i [a|c]^(m) + WQ_i [a|c]^(m+1)
+            //   + c_i/(2e) ([a|c-1_i]^(m) - rho/e [a|c-1_i]^(m+1)) + a_i/(2(z+e)) [a-1_i|c]^(m+1)
+            TB v = pd * S[s0 * PF + c] + wd * S[s1 * PF + c];
+            if (s2 != kNoSlot) v += k1 * r_i2 * (S[s2 * PF + c] - r_ro * S[s3 * PF + c]);
+            if (s4 != kNoSlot) v += static_cast<TB>(code >> 12) * r_i2s * S[s4 * PF + c];
+            S[dst * PF + c] = v;
+          }
         }
         __syncthreads();
       }
@@ -250,7 +271,7 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
           for (int u = 0; u < kRegAccPerLane; ++u) {
             const int f = tid + u * NT;
             if (f < a.nacc) {
-              const int src = a.items[8 * (i0 + f) + 1];
+              const int src = a.items[4 * (i0 + f) + 1];
               double sum = 0.0;
               for (int c = 0; c < ncopy; ++c) sum += static_cast<double>(S[src * PF + c]);
               accr[u] += sum;
@@ -258,7 +279,7 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
           }
         } else {
           for (int f = tid; f < a.nacc; f += NT) {
-            const int src = a.items[8 * (i0 + f) + 1];
+            const int src = a.items[4 * (i0 + f) + 1];
             double sum = 0.0;
             for (int c = 0; c < ncopy; ++c) sum += static_cast<double>(S[src * PF + c]);
             acc[f] += sum;
@@ -298,7 +319,7 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
       const double gy = kind == kPhHrrK ? CDy : ABy;
       const double gz = kind == kPhHrrK ? CDz : ABz;
       for (int it = i0 + tid; it < i1; it += NT) {
-        const int4 d0 = items4[2 * it];
+        const int4 d0 = items4[it];  // dst, src1, src2, direction
         const int dir = d0.w;
         const double g = dir == 0 ? gx : (dir == 1 ? gy : gz);
         H[d0.x] = H[d0.y] + g * H[d0.z];
@@ -627,8 +648,8 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
                             void* stream) {
   if (!t || !prog || !d_out || (nquart > 0 && !d_quartets))
     return fail(GBX_ERR_ARG, "gbx_eri_coop: null argument");
-  if (prim_copies < 1 || prim_copies > kCoopThreads)
-    return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be in [1, 64]");
+  if (prim_copies < 1 || prim_copies > kCoopThreads || (prim_copies & (prim_copies - 1)))
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be a power of two in [1, 64]");
   if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: class not in canonical orientation");
   if (prog->header[6] != la + lb + lc + ld ||
@@ -689,8 +710,8 @@ extern "C" int gbx_eri_coop_acc(const gbx_tables* t, const gbx_program* prog, in
                                 void* stream) {
   if (!t || !prog || !d_acc || (nquart > 0 && !d_quartets))
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: null argument");
-  if (prim_copies < 1 || prim_copies > kCoopThreads)
-    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: prim_copies must be in [1, 64]");
+  if (prim_copies < 1 || prim_copies > kCoopThreads || (prim_copies & (prim_copies - 1)))
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: prim_copies must be a power of two in [1, 64]");
   if (parts < 1 || parts > 4096)
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: parts must be in [1, 4096]");
   if (acc_stride < nquart) return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: acc_stride < nquart");

@@ -96,7 +96,7 @@ class ProgramCache:
             prog = class_program(*cls)
             header, items, phases = prog.encode()
             d_items = torch.from_numpy(np.ascontiguousarray(items) if items.size else
-                                       np.zeros(8, np.int32)).to(self.device)
+                                       np.zeros(4, np.int32)).to(self.device)
             d_phases = torch.from_numpy(phases).to(self.device)
             d_out = torch.from_numpy(prog.c_out_slots).to(self.device)
             st = GbxProgram()

@@ -315,18 +315,41 @@ class ClassProgram:
         self.c_out_slots = np.array([remap[o] for o in self.out_slots.tolist()], dtype=np.int32)
 
     def encode(self):
-        """Flatten (slot-reusing form) to (header int32[16], items int32[n*8], phases int32[3*nph])."""
+        """Flatten (slot-reusing form) to (header int32[16], items int32[n*4],
+        phases int32[3*nph]).
+
+        One item is four int32 words.  Recursion items (VRR / ket build) pack their
+        16-bit storage slots: dst | s0 << 16, s1 | s2 << 16, s3 | s4 << 16, code
+        (0xFFFF marks an absent source).  Accumulation items are (acc index, slot,
+        0, 0); horizontal-recursion items (dst, src1, src2, direction) with 32-bit
+        slots.
+        """
+        if self.c_prim_slots >= 0xFFFF:
+            raise ValueError(f"class {self.cls}: {self.c_prim_slots} primitive slots exceed "
+                             "the 16-bit item encoding")
+
+        def pack(lo, hi):
+            return (lo & 0xFFFF) | ((hi & 0xFFFF) << 16)
+
         phases = self.c_prim_phases + [self.c_acc_phase] + self.c_hrr_phases
-        items = []
+        rows = []
         table = []
         for kind, its in phases:
-            table.append([kind, len(items), len(items) + len(its)])
-            items.extend(its)
+            table.append([kind, len(rows), len(rows) + len(its)])
+            for it in its:
+                if kind in (PH_VRR, PH_KVRR):
+                    dst, s0, s1, s2, s3, s4, code = it[:7]
+                    rows.append([pack(dst, s0), pack(s1, s2), pack(s3, s4), code])
+                elif kind == PH_ACC:
+                    rows.append([it[0], it[1], 0, 0])
+                else:
+                    rows.append([it[0], it[1], it[2], it[3]])
         n_prim_ph = len(self.c_prim_phases)
         header = np.zeros(16, dtype=np.int32)
         header[:8] = [self.c_prim_slots, self.nacc, self.c_h_slots, self.nout, n_prim_ph,
-                      len(phases), self.L, len(items)]
-        flat = np.array(items, dtype=np.int32).reshape(-1) if items else np.zeros(0, np.int32)
+                      len(phases), self.L, len(rows)]
+        flat = (np.array(rows, dtype=np.uint32).view(np.int32).reshape(-1) if rows
+                else np.zeros(0, np.int32))
         return header, flat, np.array(table, dtype=np.int32).reshape(-1)
 
 

@@ -76,7 +76,10 @@ typedef struct gbx_tables {
 
 /* A class program (gbasis_amd/programs.py) resident on the device. */
 typedef struct gbx_program {
-  const int32_t *items;     /* 8 x int32 per item                               */
+  const int32_t *items;     /* 4 x int32 per item: recursion items pack 16-bit
+                               slots (dst | s0 << 16, s1 | s2 << 16, s3 | s4 << 16,
+                               code; 0xFFFF = no source); accumulation items
+                               (acc, slot, 0, 0); HRR items (dst, src1, src2, dir) */
   const int32_t *phases;    /* 3 x int32 per phase: kind, first item, end item  */
   const int32_t *out_slots; /* per canonical output component: its H slot     */
   int32_t header[16];    /* prim_slots, nacc, h_slots, nout, n_prim_phases,

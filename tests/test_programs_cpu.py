@@ -110,3 +110,28 @@ def test_program_sizes_monotone():
         assert len(set(p.out_slots.tolist())) == p.nout
         assert len(set(p.c_out_slots.tolist())) == p.nout
         assert p.c_prim_slots < p.prim_slots and p.c_h_slots < p.h_slots
+
+
+@pytest.mark.parametrize("cls", [(0, 0, 0, 0), (1, 0, 1, 0), (2, 2, 2, 1), (3, 3, 3, 3),
+                                 (4, 4, 4, 4)])
+def test_encoded_items_round_trip(cls):
+    """The 16-byte items the cooperative kernel reads decode to the compact program."""
+    from gbasis_amd.programs import PH_ACC, PH_KVRR
+
+    p = class_program(*cls)
+    header, items, phases = p.encode()
+    w = items.view(np.uint32).reshape(-1, 4)
+    assert header[7] == w.shape[0]
+    progs = p.c_prim_phases + [p.c_acc_phase] + p.c_hrr_phases
+    for (kind, its), (k2, i0, i1) in zip(progs, phases.reshape(-1, 3)):
+        assert kind == k2 and i1 - i0 == len(its)
+        for it, row in zip(its, w[i0:i1]):
+            if kind in (PH_VRR, PH_KVRR):
+                f = [row[0] & 0xFFFF, row[0] >> 16, row[1] & 0xFFFF, row[1] >> 16,
+                     row[2] & 0xFFFF, row[2] >> 16]
+                dec = [int(v) if v != 0xFFFF else -1 for v in f] + [int(row[3])]
+                assert dec == list(it[:7])
+            elif kind == PH_ACC:
+                assert [int(row[0]), int(row[1])] == list(it[:2])
+            else:
+                assert [int(x) for x in row] == list(it[:4])
