@@ -293,9 +293,13 @@ class ClassProgram:
         remap, nslots = _allocate([list(range(self.L + 1))] +
                                   [[it[0] for it in items] for _, items in self.prim_phases],
                                   last)
-        self.c_prim_phases = [(kind, [[remap[it[0]]] + [remap[x] if x >= 0 else -1
-                                                        for x in it[1:6]] + it[6:]
-                                      for it in items]) for kind, items in self.prim_phases]
+        # items of a phase are independent: group them by which optional sources they
+        # read, so that the lanes of a wavefront mostly take the same path
+        self.c_prim_phases = [(kind, sorted([[remap[it[0]]] + [remap[x] if x >= 0 else -1
+                                                               for x in it[1:6]] + it[6:]
+                                             for it in items],
+                                            key=lambda r: (r[3] >= 0, r[5] >= 0)))
+                              for kind, items in self.prim_phases]
         self.c_acc_phase = (PH_ACC, [[it[0], remap[it[1]]] + it[2:] for it in self.acc_phase[1]])
         self.c_prim_slots = nslots
         # contracted store of the horizontal recursion: fields 1..2 are sources
