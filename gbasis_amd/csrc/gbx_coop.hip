@@ -288,17 +288,22 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
       }
     }
     if (a.acc_out) {
-      // primitive pass only: each thread wrote acc[f] for its own f
+      // primitive pass only: each thread wrote acc[f] for its own f; the accumulation
+      // item of f names its row of the class's accumulators (a program may produce a
+      // subset of them)
+      const int32_t* acc_items = a.items + 4 * a.phases[3 * a.n_prim_ph + 1];
       if constexpr (kRegAcc) {
 #pragma unroll
         for (int u = 0; u < kRegAccPerLane; ++u) {
           const int f = tid + u * NT;
           if (f < a.nacc)
-            a.acc_out[(static_cast<int64_t>(f) * parts + part) * a.acc_stride + qi] = accr[u];
+            a.acc_out[(static_cast<int64_t>(acc_items[4 * f]) * parts + part) * a.acc_stride +
+                      qi] = accr[u];
         }
       } else {
         for (int f = tid; f < a.nacc; f += NT)
-          a.acc_out[(static_cast<int64_t>(f) * parts + part) * a.acc_stride + qi] = acc[f];
+          a.acc_out[(static_cast<int64_t>(acc_items[4 * f]) * parts + part) * a.acc_stride +
+                    qi] = acc[f];
       }
       continue;
     }

@@ -26,7 +26,7 @@ from . import _lib
 from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_SCHWARZ, GBX_POT_COULOMB,
                    GBX_POT_ERF, GBX_POT_ERFC, GbxProgram, GbxTables, check, ptr)
 from .boys import BOYS_H, boys_table
-from .programs import class_program
+from .programs import class_program, class_subprograms
 from .roofline import class_flops, quartet_flops
 from .tables import BatchTables, MoleculeTables, f2norm_table
 
@@ -91,9 +91,12 @@ class ProgramCache:
         self.device = device
         self._cache = {}
 
-    def get(self, cls):
-        if cls not in self._cache:
-            prog = class_program(*cls)
+    def get(self, cls, sub=None):
+        """(device struct, device arrays, host program) of a class, or with
+        ``sub = (parts, k)`` of the k-th of its ``parts`` primitive-pass programs."""
+        key = cls if sub is None else (cls, sub)
+        if key not in self._cache:
+            prog = class_program(*cls) if sub is None else class_subprograms(*cls, sub[0])[sub[1]]
             header, items, phases = prog.encode()
             d_items = torch.from_numpy(np.ascontiguousarray(items) if items.size else
                                        np.zeros(4, np.int32)).to(self.device)
@@ -105,8 +108,8 @@ class ProgramCache:
             st.out_slots = ptr(d_out)
             for i in range(16):
                 st.header[i] = int(header[i])
-            self._cache[cls] = (st, (d_items, d_phases, d_out), prog)
-        return self._cache[cls]
+            self._cache[key] = (st, (d_items, d_phases, d_out), prog)
+        return self._cache[key]
 
 
 def _h2d(a, device):
@@ -263,8 +266,8 @@ class EriEngine:
 
     LDS_TARGET = 32768   # bytes per workgroup: ~5 single-wave workgroups per CU
 
-    def _workspace(self, cls, precision, pf, prim_only):
-        st, _, _ = self.programs.get(cls)
+    def _workspace(self, cls, precision, pf, prim_only, sub=None):
+        st, _, _ = self.programs.get(cls, sub)
         lds = ctypes.c_int64()
         glb = ctypes.c_int64()
         check(self.lib.gbx_coop_workspace(ctypes.byref(st), *cls, precision, pf, int(prim_only),
@@ -272,7 +275,7 @@ class EriEngine:
               "gbx_coop_workspace")
         return lds.value, glb.value
 
-    def _plan(self, cls, nprimq, precision, prim_only=False):
+    def _plan(self, cls, nprimq, precision, prim_only=False, sub=None):
         """Primitive quartets in flight (PF) for the cooperative kernel.
 
         Largest power of two <= min(nprimq, 64) whose LDS workspace stays within
@@ -281,20 +284,38 @@ class EriEngine:
         scratch, where PF = min(nprimq, 16) keeps the wave busy (PF = 1 for the
         split primitive pass, whose parallelism comes from the parts).
         """
-        key = (cls, nprimq, precision, prim_only)
+        key = (cls, nprimq, precision, prim_only, sub)
         if key in self._pf:
             return self._pf[key]
         pf = 1
         while pf < min(nprimq, 64):
             pf *= 2
-        while pf > 1 and self._workspace(cls, precision, pf, prim_only)[0] > self.LDS_TARGET:
+        while pf > 1 and self._workspace(cls, precision, pf, prim_only, sub)[0] > self.LDS_TARGET:
             pf //= 2
-        if self._workspace(cls, precision, pf, prim_only)[1] > 0:
+        if self._workspace(cls, precision, pf, prim_only, sub)[1] > 0:
             pf = 1
             while not prim_only and pf < min(nprimq, 16):
                 pf *= 2
         self._pf[key] = pf
         return pf
+
+    SUB_PARTS = (2, 3, 4, 6, 8)
+
+    def _prim_parts(self, cls, precision):
+        """Number of primitive-pass programs of a class: 1 if its primitive store fits
+        one workgroup's LDS, else the fewest accumulator subsets whose stores all fit
+        (1 again, on global scratch, if none does)."""
+        key = ("parts", cls, precision)
+        if key not in self._pf:
+            parts = 1
+            if self._workspace(cls, precision, 1, True)[1] > 0:
+                for S in self.SUB_PARTS:
+                    if all(self._workspace(cls, precision, 1, True, (S, k))[1] == 0
+                           for k in range(S)):
+                        parts = S
+                        break
+            self._pf[key] = parts
+        return self._pf[key]
 
     COOP_TARGET_BLOCKS = 8192   # workgroups that keep every CU busy on the primitive pass
     COOP_MIN_CHUNK = 16         # primitive quartets per part at least
@@ -424,17 +445,23 @@ class EriEngine:
                   f"gbx_eri_coop{cls}")
             return "coop"
         # two passes: the primitive work split over R workgroups per quartet (primitive
-        # workspace only), then the horizontal recursion and epilogue
-        pf = self._plan(cls, max(1, -(-nprimq // R)), precision, prim_only=True)
+        # workspace only), then the horizontal recursion and epilogue.  A class whose
+        # primitive store exceeds the LDS runs its primitive pass as several programs
+        # over subsets of the accumulators, each of which fits.
         nacc = st.header[1]
         if self.stream_workspaces:
             acc = self.workspace("acc", nacc * R * nq, torch.float64)
         else:
             acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
-        check(self.lib.gbx_eri_coop_acc(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
-                                        ptr(quartets), nq, ptr(count), precision, pf, R,
-                                        ptr(acc), nq, ptr(sc), sc.numel(), self._stream()),
-              f"gbx_eri_coop_acc{cls}")
+        parts = self._prim_parts(cls, precision)
+        subs = [None] if parts == 1 else [(parts, k) for k in range(parts)]
+        for sub in subs:
+            stp = st if sub is None else self.programs.get(cls, sub)[0]
+            pf = self._plan(cls, max(1, -(-nprimq // R)), precision, prim_only=True, sub=sub)
+            check(self.lib.gbx_eri_coop_acc(ctypes.byref(dev.struct), ctypes.byref(stp), *cls,
+                                            ptr(quartets), nq, ptr(count), precision, pf, R,
+                                            ptr(acc), nq, ptr(sc), sc.numel(), self._stream()),
+                  f"gbx_eri_coop_acc{cls}")
         check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
                                    ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
                                    ptr(out), ptr(sc), sc.numel(), self._stream()),

@@ -280,7 +280,12 @@ class ClassProgram:
         defines it to the last phase that reads it; a slot read for the last time in
         phase k is reused from phase k+1 on (items of one phase run concurrently).
         """
-        # per-primitive-quartet store: fields 1..5 of VRR/KVRR items are sources
+        self._compact_prim()
+        self._compact_hrr()
+
+    def _compact_prim(self):
+        """Slot reuse of the per-primitive-quartet store (fields 1..5 of VRR/KVRR
+        items are sources)."""
         nph = len(self.prim_phases)
         last = {}
         for k, (_, items) in enumerate(self.prim_phases):
@@ -302,7 +307,10 @@ class ClassProgram:
                               for kind, items in self.prim_phases]
         self.c_acc_phase = (PH_ACC, [[it[0], remap[it[1]]] + it[2:] for it in self.acc_phase[1]])
         self.c_prim_slots = nslots
-        # contracted store of the horizontal recursion: fields 1..2 are sources
+
+    def _compact_hrr(self):
+        """Slot reuse of the contracted store of the horizontal recursion (fields 1..2
+        are sources)."""
         last = {}
         for k, (_, items) in enumerate(self.hrr_phases):
             for it in items:
@@ -387,3 +395,57 @@ def _allocate(defs_by_phase, last_use):
 @lru_cache(maxsize=None)
 def class_program(la, lb, lc, ld):
     return ClassProgram(la, lb, lc, ld)
+
+
+def _finals_bra_order(p):
+    """Accumulator indices ordered by bra component, then ket degree: neighbours share
+    most of their ket-build values."""
+    la, lb, lc, ld = p.cls
+    final_e = [(c, a) for nc in range(lc, lc + ld + 1) for c in comps(nc)
+               for na in range(la, la + lb + 1) for a in comps(na)]
+    return sorted(range(len(final_e)),
+                  key=lambda f: (deg(final_e[f][1]), comps(deg(final_e[f][1])).index(final_e[f][1]),
+                                 deg(final_e[f][0])))
+
+
+def _subprogram(p, finals):
+    """Copy of program p whose primitive part produces only the accumulators in
+    ``finals`` (global accumulator indices kept in the accumulation items), with its
+    own slot reuse.  The horizontal-recursion part is p's."""
+    import copy
+
+    q = copy.copy(p)
+    deps = {}
+    for _, items in p.prim_phases:
+        for it in items:
+            deps[it[0]] = [x for x in it[1:6] if x >= 0]
+    keep = set(finals)
+    acc_items = [it for it in p.acc_phase[1] if it[0] in keep]
+    need = set()
+    stack = [it[1] for it in acc_items]
+    while stack:
+        x = stack.pop()
+        if x in need:
+            continue
+        need.add(x)
+        stack.extend(deps.get(x, []))
+    phases = [(k, [it for it in items if it[0] in need]) for k, items in p.prim_phases]
+    q.prim_phases = [(k, its) for k, its in phases if its]
+    q.acc_phase = (PH_ACC, acc_items)
+    q.nacc = len(acc_items)
+    q._compact_prim()
+    # a primitive-pass program: no horizontal recursion of its own
+    q.hrr_phases = q.c_hrr_phases = []
+    q.h_slots = q.c_h_slots = 0
+    return q
+
+
+@lru_cache(maxsize=None)
+def class_subprograms(la, lb, lc, ld, parts):
+    """The primitive part of a class split into ``parts`` programs over contiguous runs
+    of the bra-ordered accumulators (for classes whose primitive store does not fit
+    one workgroup's LDS: each part recomputes the values its accumulators need)."""
+    p = class_program(la, lb, lc, ld)
+    order = _finals_bra_order(p)
+    n = len(order)
+    return tuple(_subprogram(p, order[k * n // parts:(k + 1) * n // parts]) for k in range(parts))

@@ -135,7 +135,9 @@ int gbx_eri_coop(const gbx_tables *t, const gbx_program *prog, int32_t la, int32
  * gbx_eri_hrr with acc_parts = parts finishes the quartets.  Spreads the
  * primitive work of classes with few quartets and many primitive quartets (f
  * and g shells, K = 8) over the GPU, and needs only the primitive part of the
- * workspace. */
+ * workspace.  prog may be a primitive-pass program that produces a subset of the
+ * class's accumulators (its accumulation items name their rows f); several such
+ * programs together fill d_acc for classes whose primitive store exceeds the LDS. */
 int gbx_eri_coop_acc(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb,
                      int32_t lc, int32_t ld, const int32_t *d_quartets, int64_t nquart,
                      const int32_t *d_count, int32_t precision, int32_t prim_copies,

@@ -135,3 +135,53 @@ def test_encoded_items_round_trip(cls):
                 assert [int(row[0]), int(row[1])] == list(it[:2])
             else:
                 assert [int(x) for x in row] == list(it[:4])
+
+
+def _prim_acc(prog, boys, sc):
+    """Accumulators of one primitive quartet from a compact program: boys = S[0..L],
+    sc = (PA, WP, roz, i2z, QC, WQ, roe, i2e, i2s).  Returns {acc row: value}."""
+    PA, WP, roz, i2z, QC, WQ, roe, i2e, i2s = sc
+    S = np.zeros(prog.c_prim_slots)
+    S[:prog.L + 1] = boys
+    for kind, items in prog.c_prim_phases:
+        writes = []
+        for dst, s0, s1, s2, s3, s4, code, _ in items:
+            di, k1, k2 = code & 15, (code >> 4) & 255, (code >> 12) & 255
+            if kind == PH_VRR:
+                v = PA[di] * S[s0] + WP[di] * S[s1]
+                if s2 >= 0:
+                    v += k1 * i2z * (S[s2] - roz * S[s3])
+            else:
+                v = QC[di] * S[s0] + WQ[di] * S[s1]
+                if s2 >= 0:
+                    v += k1 * i2e * (S[s2] - roe * S[s3])
+                if s4 >= 0:
+                    v += k2 * i2s * S[s4]
+            writes.append((dst, v))
+        for dst, v in writes:
+            S[dst] = v
+    return {it[0]: S[it[1]] for it in prog.c_acc_phase[1]}
+
+
+@pytest.mark.parametrize("cls,parts", [((2, 1, 1, 0), 2), ((3, 2, 2, 1), 3), ((2, 2, 2, 2), 4)])
+def test_subprograms_cover_the_accumulators(cls, parts):
+    """Primitive-pass programs over accumulator subsets give every accumulator once, with
+    the value of the full program."""
+    from gbasis_amd.programs import class_subprograms
+
+    rng = np.random.default_rng(sum(cls) + parts)
+    full = class_program(*cls)
+    boys = rng.uniform(0.1, 1.0, full.L + 1)
+    sc = (rng.normal(size=3), rng.normal(size=3), 0.3, 0.7, rng.normal(size=3),
+          rng.normal(size=3), 0.4, 0.6, 0.25)
+    ref = _prim_acc(full, boys, sc)
+    got = {}
+    subs = class_subprograms(*cls, parts)
+    for q in subs:
+        part = _prim_acc(q, boys, sc)
+        assert not set(part) & set(got)
+        got.update(part)
+        assert q.c_prim_slots <= full.prim_slots
+    assert sorted(got) == list(range(full.nacc))
+    for f, v in ref.items():
+        assert abs(got[f] - v) <= 1e-12 * max(1.0, abs(v)), f
