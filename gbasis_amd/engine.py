@@ -202,6 +202,15 @@ class DeviceTables:
         st.potential = self.POTENTIALS[potential]
         st.omega = float(omega) if omega is not None else 0.0
         self.struct = st
+        # Schwarz bounds are Coulomb whatever the operator (as the reference's screener)
+        if potential == "coulomb":
+            self.struct_coulomb = st
+        else:
+            sc = GbxTables()
+            ctypes.pointer(sc)[0] = st
+            sc.potential = GBX_POT_COULOMB
+            sc.omega = 0.0
+            self.struct_coulomb = sc
         self.queue = {}   # work-queue buffers of plan_device, per layout
 
 
@@ -385,9 +394,9 @@ class EriEngine:
         kernel.  ``count``: device int32 holding the number of quartets (n is then
         the capacity).
         """
-        if dev.potential != "coulomb":
-            # range-separated operators: cooperative kernels, FP64 (Schwarz bounds stay
-            # Coulomb inside the kernels)
+        tab = dev.struct if mode == GBX_MODE_PACKED else dev.struct_coulomb
+        if dev.potential != "coulomb" and mode == GBX_MODE_PACKED:
+            # range-separated operators: cooperative kernels, FP64
             path = "coop"
             if precision != GBX_FP64:
                 raise ValueError("range-separated potentials are evaluated in FP64 only")
@@ -398,12 +407,12 @@ class EriEngine:
                              "cannot run on the cooperative kernel; plan again")
         if path != "coop" and not dense and self.has_reg(cls):
             if seg is not None:
-                check(self.lib.gbx_eri_reg_segments(ctypes.byref(dev.struct), *cls, ptr(quartets),
+                check(self.lib.gbx_eri_reg_segments(ctypes.byref(tab), *cls, ptr(quartets),
                                                     ptr(seg[0]), ptr(seg[1]), seg[2], precision,
                                                     mode, ptr(out), self._stream()),
                       f"gbx_eri_reg_segments{cls}")
             else:
-                check(self.lib.gbx_eri_reg(ctypes.byref(dev.struct), *cls, ptr(quartets),
+                check(self.lib.gbx_eri_reg(ctypes.byref(tab), *cls, ptr(quartets),
                                            quartets.shape[0], ptr(count), precision, mode,
                                            ptr(out), self._stream()), f"gbx_eri_reg{cls}")
             return "reg"
@@ -418,17 +427,17 @@ class EriEngine:
             else:
                 acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
             for k in range(nsplit):
-                check(self.lib.gbx_eri_split(ctypes.byref(dev.struct), *cls, k, ptr(quartets), nq,
+                check(self.lib.gbx_eri_split(ctypes.byref(tab), *cls, k, ptr(quartets), nq,
                                              ptr(count), precision, R, ptr(acc), nq,
                                              self._stream()), f"gbx_eri_split{cls}[{k}]")
             use_reg = self.hrr_reg if hrr is None else hrr
             if mode == GBX_MODE_PACKED and not dense and use_reg and self.has_hrr(cls):
-                check(self.lib.gbx_eri_hrr_reg(ctypes.byref(dev.struct), *cls, ptr(quartets), nq,
+                check(self.lib.gbx_eri_hrr_reg(ctypes.byref(tab), *cls, ptr(quartets), nq,
                                                ptr(count), ptr(acc), nq, R, ptr(out),
                                                self._stream()), f"gbx_eri_hrr_reg{cls}")
                 return "split"
             sc = self.scratch()
-            check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+            check(self.lib.gbx_eri_hrr(ctypes.byref(tab), ctypes.byref(st), *cls,
                                        ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
                                        ptr(out), ptr(sc), sc.numel(), self._stream()),
                   f"gbx_eri_hrr{cls}")
@@ -439,7 +448,7 @@ class EriEngine:
         sc = self.scratch()
         if R == 1 and full_fits:
             pf = self._plan(cls, nprimq, precision)
-            check(self.lib.gbx_eri_coop(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+            check(self.lib.gbx_eri_coop(ctypes.byref(tab), ctypes.byref(st), *cls,
                                         ptr(quartets), nq, ptr(count), precision, pf, mode,
                                         ptr(out), ptr(sc), sc.numel(), self._stream()),
                   f"gbx_eri_coop{cls}")
@@ -458,11 +467,11 @@ class EriEngine:
         for sub in subs:
             stp = st if sub is None else self.programs.get(cls, sub)[0]
             pf = self._plan(cls, max(1, -(-nprimq // R)), precision, prim_only=True, sub=sub)
-            check(self.lib.gbx_eri_coop_acc(ctypes.byref(dev.struct), ctypes.byref(stp), *cls,
+            check(self.lib.gbx_eri_coop_acc(ctypes.byref(tab), ctypes.byref(stp), *cls,
                                             ptr(quartets), nq, ptr(count), precision, pf, R,
                                             ptr(acc), nq, ptr(sc), sc.numel(), self._stream()),
                   f"gbx_eri_coop_acc{cls}")
-        check(self.lib.gbx_eri_hrr(ctypes.byref(dev.struct), ctypes.byref(st), *cls,
+        check(self.lib.gbx_eri_hrr(ctypes.byref(tab), ctypes.byref(st), *cls,
                                    ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
                                    ptr(out), ptr(sc), sc.numel(), self._stream()),
               f"gbx_eri_hrr{cls}")

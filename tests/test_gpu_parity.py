@@ -86,6 +86,15 @@ def test_range_separated_vs_oracle(gpu, potential, omega):
                                                 omega=omega)
     coul = electron_repulsion_integral(basis, notation="chemist")
     assert np.abs(got + got2 - coul).max() <= 1e-12 * scale
+    # Schwarz screening keeps the Coulomb bounds, as the reference's screener
+    thr = 1e-4
+    got_s = electron_repulsion_integral_improved(basis, notation="chemist", potential=potential,
+                                                 omega=omega, schwarz_threshold=thr)
+    ref_s = dense_eri(basis, notation="chemist", potential=potential, omega=omega,
+                      schwarz_threshold=thr, exact=True)
+    # a block screened by one bound and kept by the other would differ by its value
+    assert np.abs(got_s - ref_s).max() <= 1e-12 * scale
+    assert int((got_s == 0.0).sum()) > int((got == 0.0).sum())
 
 
 def test_schwarz_bounds_and_screened_lists(gpu):
