@@ -51,9 +51,20 @@ def _code(direction_, k1=0, k2=0):
     return direction_ | (k1 << 4) | (k2 << 12)
 
 
+@lru_cache(maxsize=None)
+def _comps(l):
+    return tuple((x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1))
+
+
 def comps(l):
     """Canonical Cartesian component order (same as the reference default)."""
-    return [(x, y, l - x - y) for x in range(l, -1, -1) for y in range(l - x, -1, -1)]
+    return list(_comps(l))
+
+
+@lru_cache(maxsize=None)
+def comp_index(c):
+    """Position of component c in comps(deg(c))."""
+    return _comps(c[0] + c[1] + c[2]).index(c)
 
 
 def ncart(l):
@@ -158,12 +169,12 @@ class ClassProgram:
         for m in range(L + 1):
             slot[("V", Z3, m)] = m
         v_items = sorted((x for x in need_v if x[0] != Z3),
-                         key=lambda x: (deg(x[0]), x[1], comps(deg(x[0])).index(x[0])))
+                         key=lambda x: (deg(x[0]), x[1], comp_index(x[0])))
         for a, m in v_items:
             slot[("V", a, m)] = len(slot)
         k_items = sorted((x for x in need_k if x[0] != Z3),
-                         key=lambda x: (deg(x[0]), comps(deg(x[0])).index(x[0]), deg(x[1]),
-                                        comps(deg(x[1])).index(x[1]), x[2]))
+                         key=lambda x: (deg(x[0]), comp_index(x[0]), deg(x[1]),
+                                        comp_index(x[1]), x[2]))
         for c, a, m in k_items:
             slot[("K", c, a, m)] = len(slot)
         self.prim_slots = len(slot)
@@ -404,7 +415,7 @@ def _finals_bra_order(p):
     final_e = [(c, a) for nc in range(lc, lc + ld + 1) for c in comps(nc)
                for na in range(la, la + lb + 1) for a in comps(na)]
     return sorted(range(len(final_e)),
-                  key=lambda f: (deg(final_e[f][1]), comps(deg(final_e[f][1])).index(final_e[f][1]),
+                  key=lambda f: (deg(final_e[f][1]), comp_index(final_e[f][1]),
                                  deg(final_e[f][0])))
 
 
