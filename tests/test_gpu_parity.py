@@ -86,8 +86,10 @@ def test_range_separated_vs_oracle(gpu, potential, omega):
                                                 omega=omega)
     coul = electron_repulsion_integral(basis, notation="chemist")
     assert np.abs(got + got2 - coul).max() <= 1e-12 * scale
-    # Schwarz screening keeps the Coulomb bounds, as the reference's screener
-    thr = 1e-4
+    # Schwarz screening keeps the Coulomb bounds, as the reference's screener; at 1e-2
+    # some quartets of this basis are screened (smallest Q products 4.8e-3 .. 1.2e-2, none
+    # within 2 % of the threshold)
+    thr = 1e-2
     got_s = electron_repulsion_integral_improved(basis, notation="chemist", potential=potential,
                                                  omega=omega, schwarz_threshold=thr)
     ref_s = dense_eri(basis, notation="chemist", potential=potential, omega=omega,
