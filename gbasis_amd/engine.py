@@ -26,7 +26,7 @@ from . import _lib
 from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_SCHWARZ, GBX_POT_COULOMB,
                    GBX_POT_ERF, GBX_POT_ERFC, GbxProgram, GbxTables, check, ptr)
 from .boys import BOYS_H, boys_table
-from .programs import class_program, class_subprograms
+from .programs import encoded_program
 from .roofline import class_flops, quartet_flops
 from .tables import BatchTables, MoleculeTables, f2norm_table
 
@@ -92,23 +92,22 @@ class ProgramCache:
         self._cache = {}
 
     def get(self, cls, sub=None):
-        """(device struct, device arrays, host program) of a class, or with
+        """(device struct, device arrays, header) of a class, or with
         ``sub = (parts, k)`` of the k-th of its ``parts`` primitive-pass programs."""
         key = cls if sub is None else (cls, sub)
         if key not in self._cache:
-            prog = class_program(*cls) if sub is None else class_subprograms(*cls, sub[0])[sub[1]]
-            header, items, phases = prog.encode()
+            header, items, phases, out_slots = encoded_program(cls, sub)
             d_items = torch.from_numpy(np.ascontiguousarray(items) if items.size else
                                        np.zeros(4, np.int32)).to(self.device)
-            d_phases = torch.from_numpy(phases).to(self.device)
-            d_out = torch.from_numpy(prog.c_out_slots).to(self.device)
+            d_phases = torch.from_numpy(np.ascontiguousarray(phases)).to(self.device)
+            d_out = torch.from_numpy(np.ascontiguousarray(out_slots)).to(self.device)
             st = GbxProgram()
             st.items = ptr(d_items)
             st.phases = ptr(d_phases)
             st.out_slots = ptr(d_out)
             for i in range(16):
                 st.header[i] = int(header[i])
-            self._cache[key] = (st, (d_items, d_phases, d_out), prog)
+            self._cache[key] = (st, (d_items, d_phases, d_out), header)
         return self._cache[key]
 
 

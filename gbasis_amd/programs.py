@@ -460,3 +460,60 @@ def class_subprograms(la, lb, lc, ld, parts):
     order = _finals_bra_order(p)
     n = len(order)
     return tuple(_subprogram(p, order[k * n // parts:(k + 1) * n // parts]) for k in range(parts))
+
+
+# ---------------------------------------------------------------------------
+# Encoded programs on disk.  Building a class program is pure Python and takes
+# seconds for g shells ((gg|gg): about 7 s); the encoded arrays are cached per
+# class in a directory (GBX_PROGRAM_CACHE, default gbasis_amd/_program_cache),
+# keyed by a hash of this file so that a changed builder never reads stale
+# programs.  Set GBX_PROGRAM_CACHE to an empty string to disable the cache.
+
+def _builder_tag():
+    import hashlib
+    import os
+
+    with open(os.path.abspath(__file__), "rb") as fh:
+        return hashlib.sha1(fh.read()).hexdigest()[:12]
+
+
+@lru_cache(maxsize=None)
+def _cache_dir():
+    import os
+
+    d = os.environ.get("GBX_PROGRAM_CACHE")
+    if d == "":
+        return None
+    if d is None:
+        d = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_program_cache")
+    return os.path.join(d, _builder_tag())
+
+
+def encoded_program(cls, sub=None):
+    """(header, items, phases, out_slots) of a class program (``sub = (parts, k)``: of
+    the k-th of its primitive-pass programs), from the disk cache when present."""
+    import os
+    import tempfile
+
+    d = _cache_dir()
+    name = "_".join(map(str, cls)) + ("" if sub is None else f"_p{sub[0]}_{sub[1]}") + ".npz"
+    path = None if d is None else os.path.join(d, name)
+    if path is not None and os.path.exists(path):
+        try:
+            with np.load(path, allow_pickle=False) as z:
+                return z["header"], z["items"], z["phases"], z["out_slots"]
+        except (OSError, ValueError, KeyError):
+            pass  # unreadable entry: rebuild it below
+    prog = class_program(*cls) if sub is None else class_subprograms(*cls, sub[0])[sub[1]]
+    header, items, phases = prog.encode()
+    out_slots = np.asarray(prog.c_out_slots, dtype=np.int32)
+    if path is not None:
+        try:
+            os.makedirs(d, exist_ok=True)
+            fd, tmp = tempfile.mkstemp(dir=d, suffix=".npz")
+            with os.fdopen(fd, "wb") as fh:
+                np.savez(fh, header=header, items=items, phases=phases, out_slots=out_slots)
+            os.replace(tmp, path)
+        except OSError:
+            pass  # a read-only location only costs the rebuild next time
+    return header, items, phases, out_slots

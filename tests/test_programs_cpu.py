@@ -185,3 +185,25 @@ def test_subprograms_cover_the_accumulators(cls, parts):
     assert sorted(got) == list(range(full.nacc))
     for f, v in ref.items():
         assert abs(got[f] - v) <= 1e-12 * max(1.0, abs(v)), f
+
+
+def test_encoded_program_disk_cache(tmp_path, monkeypatch):
+    """The on-disk cache returns exactly the arrays the builder encodes."""
+    from gbasis_amd import programs
+
+    monkeypatch.setenv("GBX_PROGRAM_CACHE", str(tmp_path))
+    programs._cache_dir.cache_clear()
+    try:
+        for cls, sub in (((2, 1, 2, 0), None), ((3, 2, 2, 1), (3, 1))):
+            first = programs.encoded_program(cls, sub)
+            again = programs.encoded_program(cls, sub)     # read back from disk
+            p = class_program(*cls) if sub is None else \
+                programs.class_subprograms(*cls, sub[0])[sub[1]]
+            h, it, ph = p.encode()
+            for a in (first, again):
+                assert np.array_equal(a[0], h) and np.array_equal(a[1], it)
+                assert np.array_equal(a[2], ph) and np.array_equal(a[3], p.c_out_slots)
+        assert len(list(tmp_path.rglob("*.npz"))) == 2
+    finally:
+        monkeypatch.delenv("GBX_PROGRAM_CACHE")
+        programs._cache_dir.cache_clear()
