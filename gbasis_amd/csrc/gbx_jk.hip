@@ -125,7 +125,8 @@ using namespace gbx;
 
 extern "C" int gbx_jk_packed(const double* d_packed, int64_t nfunc, const double* d_dens,
                              double* d_J, double* d_K, void* stream) {
-  if (!d_packed || !d_dens || !d_J || !d_K) return fail(GBX_ERR_ARG, "gbx_jk_packed: null argument");
+  if (!d_packed || !d_dens || !d_J || !d_K)
+    return fail(GBX_ERR_ARG, "gbx_jk_packed: null argument");
   // 65536 functions keep every slot index below 2^62
   if (nfunc < 1 || nfunc > 65536)
     return fail(GBX_ERR_ARG, "gbx_jk_packed: nfunc must be in [1, 65536]");
@@ -141,7 +142,8 @@ extern "C" int gbx_jk_packed(const double* d_packed, int64_t nfunc, const double
       hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
     return fail(GBX_ERR_HIP, "gbx_jk_packed: device query failed");
 
-  const unsigned zgrid = static_cast<unsigned>(std::min<int64_t>((nn + kJkThreads - 1) / kJkThreads, 4096));
+  const unsigned zgrid =
+      static_cast<unsigned>(std::min<int64_t>((nn + kJkThreads - 1) / kJkThreads, 4096));
   hipLaunchKernelGGL(jk_zero_kernel, dim3(zgrid), dim3(kJkThreads), 0, s, d_J, d_K, nn);
   int rc = check_launch("jk_zero_kernel");
   if (rc != GBX_OK) return rc;
@@ -150,12 +152,14 @@ extern "C" int gbx_jk_packed(const double* d_packed, int64_t nfunc, const double
   const uint64_t want = (nslot + 512 * kJkThreads - 1) / (512 * kJkThreads);
   const size_t shmem = static_cast<size_t>(2 * nn) * sizeof(double);
   if (shmem + 256 <= static_cast<size_t>(lds)) {
-    const unsigned grid = static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(want, 4 * uint64_t(ncu))));
+    const unsigned grid =
+        static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(want, 4 * uint64_t(ncu))));
     if (shmem > 65536) {
       static bool raised = false;
       if (!raised) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&jk_digest_lds_kernel),
-                                hipFuncAttributeMaxDynamicSharedMemorySize, lds - 256) != hipSuccess)
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                lds - 256) != hipSuccess)
           return fail(GBX_ERR_HIP, "gbx_jk_packed: could not raise the LDS limit");
         raised = true;
       }
@@ -164,14 +168,16 @@ extern "C" int gbx_jk_packed(const double* d_packed, int64_t nfunc, const double
                        nslot, d_dens, n, d_J, d_K);
     rc = check_launch("jk_digest_lds_kernel");
   } else {
-    const unsigned grid = static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(want * 16, 64 * uint64_t(ncu))));
+    const unsigned grid = static_cast<unsigned>(
+        std::max<uint64_t>(1, std::min<uint64_t>(want * 16, 64 * uint64_t(ncu))));
     hipLaunchKernelGGL(jk_digest_global_kernel, dim3(grid), dim3(kJkThreads), 0, s, d_packed,
                        nslot, d_dens, n, d_J, d_K);
     rc = check_launch("jk_digest_global_kernel");
   }
   if (rc != GBX_OK) return rc;
   const uint64_t ntri = static_cast<uint64_t>(n) * (n + 1) / 2;
-  const unsigned sgrid = static_cast<unsigned>(std::min<uint64_t>((ntri + kJkThreads - 1) / kJkThreads, 4096));
+  const unsigned sgrid =
+      static_cast<unsigned>(std::min<uint64_t>((ntri + kJkThreads - 1) / kJkThreads, 4096));
   hipLaunchKernelGGL(jk_symmetrize_kernel, dim3(sgrid), dim3(kJkThreads), 0, s, d_J, d_K, n);
   return check_launch("jk_symmetrize_kernel");
 }

@@ -21,8 +21,10 @@ from .engine import get_engine
 def coulomb_exchange_packed(packed, nfunc, density, device="cuda"):
     """(J, K) as device float64 tensors from one molecule's packed ERIs (device)."""
     D = np.asarray(density, dtype=np.float64) if not torch.is_tensor(density) else density
-    if not torch.is_tensor(D) and not np.allclose(D, D.T, rtol=0, atol=1e-12 * max(1.0, np.abs(D).max())):
-        raise ValueError("density must be symmetric")
+    if not torch.is_tensor(D):
+        tol = 1e-12 * max(1.0, float(np.abs(D).max())) if D.size else 0.0
+        if not np.allclose(D, D.T, rtol=0, atol=tol):
+            raise ValueError("density must be symmetric")
     return get_engine(device).jk(packed, nfunc, D)
 
 
