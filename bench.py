@@ -310,7 +310,7 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
     on one core (BLAS limited to one thread).
 
     Classes are visited from the cheapest (algorithmic FLOPs per quartet) up; each
-    gets up to 64 random quartets within its share of the budget.  Classes left
+    gets up to 256 random quartets within its share of the budget.  Classes left
     when the budget runs out are extrapolated with the measured oracle seconds per
     algorithmic FLOP of the sampled ones."""
     from oracle.assembly import contraction_block
@@ -341,7 +341,7 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
             continue
         quart = np.concatenate([bk.host_quartets() for bk in bks])
         times = []
-        for _ in range(64):
+        for _ in range(256):
             q = quart[rng.integers(0, quart.shape[0])]
             mi = pair_mol[q[0]]
             m = batch.mols[mi]
