@@ -470,7 +470,7 @@ def test_workload_samples_vs_oracle(gpu, name):
             strict += 1
         else:
             floor_only += 1
-        # mixed precision: within 1e-8 of the FP64 result (the bound BASELINE states),
+        # mixed precision: within 1e-8 Eh of the FP64 result (the mixed-precision bound),
         # hence within the FP64 bar + 1e-8 of the exact value
         gotmx = got_all["mixed"][slots]
         errmx = float(np.abs(gotmx - got64).max())
