@@ -133,7 +133,7 @@ def _host_busy(srv, span):
     return {k: v / span for k, v in srv.host_seconds.items()}
 
 
-def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4):
+def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_batch=8):
     """Serving workload: requests of 2-30 atoms (seeds 4000 + i, s/p/d, K <= 6),
     precision FP64 or mixed with probability 1/2 each.  First the saturated rate
     (n_sat requests all arriving at once), then Poisson arrivals at 50 % and 80 %
@@ -149,7 +149,8 @@ def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4):
     rng = np.random.default_rng(seed)
     bases = [serving_request(4000 + i) for i in range(n_requests)]
     precs = np.where(rng.random(n_requests) < 0.5, "mixed", "fp64")
-    srv = EriServer(threshold=threshold, fp32_cutoff=cutoff, n_streams=n_streams)
+    srv = EriServer(threshold=threshold, fp32_cutoff=cutoff, n_streams=n_streams,
+                    max_batch=max_batch)
     clock = srv.clock
     # warm-up: programs and workspaces
     for i in range(4):
@@ -603,6 +604,7 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--serving-saturation", type=int, default=300)
     ap.add_argument("--serving-streams", type=int, default=4)
+    ap.add_argument("--serving-max-batch", type=int, default=8)
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--launch-streams", type=int, default=4,
                     help="spread the class launches of a step over this many streams (the "
@@ -738,7 +740,8 @@ def main():
         torch.cuda.empty_cache()
         result["serving"] = serving_bench(args.serving_requests, args.serving_saturation,
                                           args.threshold, args.fp32_cutoff,
-                                          args.serving_streams)
+                                          args.serving_streams,
+                                          max_batch=args.serving_max_batch)
     print(json.dumps(result), flush=True)
 
 
