@@ -13,13 +13,18 @@ Only ``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
 How each piece is pinned (so that a dropped term, a wrong sign or index, or a
 transposed operand fails a test):
   * boys.py        closed forms F_m(0)=1/(2m+1), F_0 via erf, the downward
-                   recurrence, and mpmath at 50 digits (tests/test_oracle_boys.py).
+                   recurrence, and mpmath at 50 digits.
   * os_hgp.py      the reference test-suite's HORTON primitive values for s..f
                    components at a fixed geometry, the closed-form (ss|ss), an
-                   independent McMurchie-Davidson brute force for small l, and
-                   the 8-fold permutational symmetry (tests/test_oracle_eri.py).
+                   independent McMurchie-Davidson evaluation in mpmath at random
+                   geometries (components up to d), and the 8-fold
+                   permutational symmetry.
+  * erf / erfc     (boys_potential) the closed form of Gaussian charge clouds
+                   screened by a Gaussian of exponent omega^2, and its centre
+                   derivative for a p function.
   * assembly.py    the reference fixture data_horton_bec_cart_elec_repulsion.npy
                    (Be+C STO-6G, physicist notation) and the Schwarz inequality.
   * spherical tables (product module gbasis_amd.spherical, used here) are pinned
                    by HORTON cartpure matrices (tests/test_spherical.py).
+The tests of boys.py, os_hgp.py and assembly.py are in tests/test_oracle_golden.py.
 """

@@ -159,3 +159,93 @@ def test_erf_p_shell_from_centre_derivative():
         ref = a ** -0.5 * (fp - fm) / (2 * h)
         # p components in the default order x, y, z occupy functions 0..2
         assert abs(got[ax, 3, 4, 5] - ref) <= 1e-8 * max(1.0, abs(ref)), (ax, got[ax, 3, 4, 5], ref)
+
+
+def _md_primitive(a, b, c, d, ea, eb, ec, ed, A, B, C, D):
+    """Unnormalised Cartesian primitive (ab|cd) by McMurchie-Davidson (Hermite
+    expansion coefficients E and Hermite Coulomb integrals R, Helgaker, Jorgensen and
+    Olsen, Molecular Electronic-Structure Theory, ch. 9), in mpmath at 40 digits: an
+    evaluation that shares no recursion with the Obara-Saika oracle."""
+    import mpmath
+
+    mpmath.mp.dps = 40
+    mpf = mpmath.mpf
+
+    def E(i, j, t, Qx, p, mu, XPA, XPB):
+        if t < 0 or t > i + j:
+            return mpf(0)
+        if i == j == t == 0:
+            return mpmath.exp(-mu * Qx * Qx)
+        if i > 0:
+            return (E(i - 1, j, t - 1, Qx, p, mu, XPA, XPB) / (2 * p)
+                    + XPA * E(i - 1, j, t, Qx, p, mu, XPA, XPB)
+                    + (t + 1) * E(i - 1, j, t + 1, Qx, p, mu, XPA, XPB))
+        return (E(i, j - 1, t - 1, Qx, p, mu, XPA, XPB) / (2 * p)
+                + XPB * E(i, j - 1, t, Qx, p, mu, XPA, XPB)
+                + (t + 1) * E(i, j - 1, t + 1, Qx, p, mu, XPA, XPB))
+
+    def boys(n, T):
+        return mpmath.hyp1f1(n + mpf(1) / 2, n + mpf(3) / 2, -T) / (2 * n + 1)
+
+    memo = {}
+
+    def R(t, u, v, n, alpha, X, Y, Z):
+        key = (t, u, v, n)
+        if key in memo:
+            return memo[key]
+        if t < 0 or u < 0 or v < 0:
+            return mpf(0)
+        if t == u == v == 0:
+            val = (-2 * alpha) ** n * boys(n, alpha * (X * X + Y * Y + Z * Z))
+        elif t > 0:
+            val = (t - 1) * R(t - 2, u, v, n + 1, alpha, X, Y, Z) + X * R(t - 1, u, v, n + 1, alpha, X, Y, Z)
+        elif u > 0:
+            val = (u - 1) * R(t, u - 2, v, n + 1, alpha, X, Y, Z) + Y * R(t, u - 1, v, n + 1, alpha, X, Y, Z)
+        else:
+            val = (v - 1) * R(t, u, v - 2, n + 1, alpha, X, Y, Z) + Z * R(t, u, v - 1, n + 1, alpha, X, Y, Z)
+        memo[key] = val
+        return val
+
+    A, B, C, D = ([mpf(float(x)) for x in v] for v in (A, B, C, D))
+    ea, eb, ec, ed = (mpf(float(x)) for x in (ea, eb, ec, ed))
+    p, q = ea + eb, ec + ed
+    P = [(ea * A[k] + eb * B[k]) / p for k in range(3)]
+    Q = [(ec * C[k] + ed * D[k]) / q for k in range(3)]
+    alpha = p * q / (p + q)
+    Eab = [[E(a[k], b[k], t, A[k] - B[k], p, ea * eb / p, P[k] - A[k], P[k] - B[k])
+            for t in range(a[k] + b[k] + 1)] for k in range(3)]
+    Ecd = [[E(c[k], d[k], t, C[k] - D[k], q, ec * ed / q, Q[k] - C[k], Q[k] - D[k])
+            for t in range(c[k] + d[k] + 1)] for k in range(3)]
+    X, Y, Z = (P[k] - Q[k] for k in range(3))
+    total = mpf(0)
+    for t, et in enumerate(Eab[0]):
+        for u, eu in enumerate(Eab[1]):
+            for v, ev in enumerate(Eab[2]):
+                for tau, f1 in enumerate(Ecd[0]):
+                    for nu, f2 in enumerate(Ecd[1]):
+                        for phi, f3 in enumerate(Ecd[2]):
+                            total += ((-1) ** (tau + nu + phi) * et * eu * ev * f1 * f2 * f3
+                                      * R(t + tau, u + nu, v + phi, 0, alpha, X, Y, Z))
+    return float(2 * mpmath.pi ** mpf(2.5) / (p * q * mpmath.sqrt(p + q)) * total)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_oracle_vs_mcmurchie_davidson(seed):
+    """Random geometry, exponents and components up to d on every centre: both ket
+    builds of the oracle against the McMurchie-Davidson brute force."""
+    from oracle.os_hgp import _comps
+
+    rng = np.random.default_rng(40 + seed)
+    ls = rng.integers(0, 3, 4)
+    comp = [_comps(int(l))[rng.integers(0, (l + 1) * (l + 2) // 2)] for l in ls]
+    exps = np.exp(rng.uniform(np.log(0.2), np.log(8.0), 4))
+    coords = rng.uniform(-1.5, 1.5, (4, 3))
+    ref = _md_primitive(*comp, *exps, *coords)
+    norm = 1.0
+    for (x, y, z), e in zip(comp, exps):
+        l = x + y + z
+        df = np.prod([np.prod(np.arange(2 * k - 1, 0, -2)) if k > 0 else 1.0 for k in (x, y, z)])
+        norm *= (2 * e / np.pi) ** 0.75 * (4 * e) ** (l / 2) / np.sqrt(df)
+    for ket in ("etr", "vrr"):
+        got = _prim(*comp, exps, coords, ket=ket)
+        assert abs(got - ref * norm) <= 1e-11 * max(1e-3, abs(ref * norm)), (ket, comp, got, ref * norm)
