@@ -249,9 +249,20 @@ def fchk_basis(fields):
 
 def fchk_rhf_orbitals(fields):
     """(MO coefficients (nbasis, nmo), occupations (nmo,)) of a closed-shell fchk."""
+    c, occ = fchk_orbitals(fields, "alpha")
+    return c, 2.0 * occ
+
+
+def fchk_orbitals(fields, spin="alpha"):
+    """(MO coefficients (nbasis, nmo), occupations 0/1 (nmo,)) of one spin of an fchk;
+    a restricted file has no beta set, and its beta orbitals are the alpha ones."""
+    if spin not in ("alpha", "beta"):
+        raise ValueError("spin must be 'alpha' or 'beta'")
     nb = int(fields["Number of basis functions"])
     nmo = int(fields["Number of independent functions"])
-    c = fields["Alpha MO coefficients"].reshape(nmo, nb).T
+    key = "Beta MO coefficients" if spin == "beta" and "Beta MO coefficients" in fields \
+        else "Alpha MO coefficients"
+    c = fields[key].reshape(nmo, nb).T
     occ = np.zeros(nmo)
-    occ[:int(fields["Number of alpha electrons"])] = 2.0
+    occ[:int(fields[f"Number of {spin} electrons"])] = 1.0
     return c, occ

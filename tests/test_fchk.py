@@ -134,3 +134,20 @@ def test_fchk_ethylene_coulomb_exchange(gpu):
     # the two published values differ by 5e-9 (J) and 1.1e-8 (K)
     assert abs(j_coul - J_GBASIS) < 2e-8 and abs(j_coul - J_LIBCINT) < 2e-8
     assert abs(k_ex - K_GBASIS) < 2e-8 and abs(k_ex - K_LIBCINT) < 2e-8
+
+
+def test_fchk_unrestricted_pure_f_orthonormal():
+    """Water (uwB97XD/def2-TZVPD, notebooks/tutorial/water.fchk): pure d and f shells;
+    alpha and beta orbitals orthonormal under the shells as built, which pins the pure
+    component order c0, c1, s1, c2, s2, c3, s3 and the f normalisation."""
+    from gbasis_amd.parsers import fchk_orbitals
+
+    f = read_fchk(os.path.join(HERE, "golden", "water.fchk"))
+    basis = fchk_basis(f)
+    assert sorted({(s.angmom, s.coord_type) for s in basis}) == [
+        (0, "cartesian"), (1, "cartesian"), (2, "spherical"), (3, "spherical")]
+    S = _overlap_matrix(basis)
+    for spin in ("alpha", "beta"):
+        c, occ = fchk_orbitals(f, spin)
+        assert c.shape == (58, 58) and occ.sum() == 5
+        assert np.abs(c.T @ S @ c - np.eye(58)).max() < 1e-6
