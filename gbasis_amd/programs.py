@@ -40,6 +40,7 @@ Item encoding (8 int32 per item; code = dir | k1 << 4 | k2 << 12):
 """
 
 from functools import lru_cache
+from itertools import groupby
 
 import numpy as np
 
@@ -71,6 +72,7 @@ def ncart(l):
     return (l + 1) * (l + 2) // 2
 
 
+@lru_cache(maxsize=None)
 def direction(comp):
     for i in (2, 1, 0):
         if comp[i] > 0:
@@ -78,12 +80,14 @@ def direction(comp):
     return -1
 
 
+@lru_cache(maxsize=None)
 def sub(c, i, k=1):
     c = list(c)
     c[i] -= k
     return tuple(c)
 
 
+@lru_cache(maxsize=None)
 def add(c, i):
     c = list(c)
     c[i] += 1
@@ -183,12 +187,10 @@ class ClassProgram:
             return slot[("V", a, m)] if c == Z3 else slot[("K", c, a, m)]
 
         phases = []
-        # VRR phases by degree of a
-        for n in range(1, L + 1):
+        # VRR phases by degree of a (v_items are sorted by it)
+        for _, group in groupby(v_items, key=lambda x: deg(x[0])):
             items = []
-            for a, m in v_items:
-                if deg(a) != n:
-                    continue
+            for a, m in group:
                 i = direction(a)
                 a1 = sub(a, i)
                 s2 = s3 = -1
@@ -199,12 +201,10 @@ class ClassProgram:
                               s2, s3, -1, _code(i, a[i] - 1), 0])
             if items:
                 phases.append((PH_VRR, items))
-        # ket-centre phases by degree of c
-        for n in range(1, Lcd + 1):
+        # ket-centre phases by degree of c (k_items are sorted by it)
+        for _, group in groupby(k_items, key=lambda x: deg(x[0])):
             items = []
-            for c, a, m in k_items:
-                if deg(c) != n:
-                    continue
+            for c, a, m in group:
                 i = direction(c)
                 c1 = sub(c, i)
                 s2 = s3 = s4 = -1
@@ -253,23 +253,19 @@ class ClassProgram:
         for g in ket_items + bra_items:
             if g not in hslot:
                 hslot[g] = len(hslot)
-        for n in range(1, ld + 1):
+        for _, group in groupby(ket_items, key=lambda g: deg(g[3])):
             items = []
-            for g in ket_items:
+            for g in group:
                 a, b, c, d = g
-                if deg(d) != n:
-                    continue
                 i = direction(d)
                 items.append([hslot[g], hslot[(a, b, add(c, i), sub(d, i))],
                               hslot[(a, b, c, sub(d, i))], i, 0, 0, 0, 0])
             if items:
                 hphases.append((PH_HRRK, items))
-        for n in range(1, lb + 1):
+        for _, group in groupby(bra_items, key=lambda g: deg(g[1])):
             items = []
-            for g in bra_items:
+            for g in group:
                 a, b, c, d = g
-                if deg(b) != n:
-                    continue
                 i = direction(b)
                 items.append([hslot[g], hslot[(add(a, i), sub(b, i), c, d)],
                               hslot[(a, sub(b, i), c, d)], i, 0, 0, 0, 0])
