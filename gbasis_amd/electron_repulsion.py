@@ -180,6 +180,44 @@ class ElectronRepulsionIntegral:
         return _dense(self.contractions, _coord_types(self.contractions, coord_type), "chemist",
                       thr, transform, potential=self.potential, omega=self.omega)
 
+    @classmethod
+    def construct_array_contraction(cls, cont_one, cont_two, cont_three, cont_four, **kwargs):
+        """Cartesian block (M_1, L_1, M_2, L_2, M_3, L_3, M_4, L_4) of one shell quartet in
+        chemists' notation, in each shell's Cartesian component order, without
+        ``norm_cont`` (electron_repulsion.py:87-211, :306-412).
+
+        The quartet is evaluated on the device as the four shells' dense array, from
+        which the block is cut.  The reference's optional ``screener`` keyword is
+        accepted and not used: the block is always computed.
+        """
+        conts = (cont_one, cont_two, cont_three, cont_four)
+        names = ("cont_one", "cont_two", "cont_three", "cont_four")
+        for name, c in zip(names, conts):
+            if not isinstance(c, GeneralizedContractionShell):
+                raise TypeError(f"`{name}` must be a `GeneralizedContractionShell` instance.")
+        shells = [_UnitNormShell(c) for c in conts]
+        dense = _dense(shells, ["cartesian"] * 4, "chemist", 0.0, None,
+                       potential=cls.potential, omega=cls.omega)
+        sizes = [c.coeffs.shape[1] * len(c.angmom_components_cart) for c in conts]
+        off = np.concatenate([[0], np.cumsum(sizes)])
+        blk = dense[off[0]:off[1], off[1]:off[2], off[2]:off[3], off[3]:off[4]]
+        shape = []
+        for c in conts:
+            shape += [c.coeffs.shape[1], len(c.angmom_components_cart)]
+        return np.ascontiguousarray(blk).reshape(shape)
+
+
+class _UnitNormShell:
+    """A shell seen with ``norm_cont`` = 1 (the contraction block of the reference
+    excludes it)."""
+
+    def __init__(self, shell):
+        self._s = shell
+        self.norm_cont = np.ones((shell.coeffs.shape[1], len(shell.angmom_components_cart)))
+
+    def __getattr__(self, name):
+        return getattr(self._s, name)
+
 
 class ElectronRepulsionIntegralImproved(ElectronRepulsionIntegral):
     """Class interface of gbasis/integrals/electron_repulsion.py:284-412."""

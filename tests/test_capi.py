@@ -58,3 +58,23 @@ def test_range_separated_argument_errors():
         electron_repulsion_integral_packed(basis, potential="erf")
     with pytest.raises(ValueError):
         electron_repulsion_integral_packed(basis, potential="erfc", omega=0.3, precision="mixed")
+
+
+def test_construct_array_contraction_type_checks():
+    """As tests/test_electron_repulsion.py:20-35 of the reference: non-shell arguments
+    raise TypeError before any device work."""
+    import numpy as np
+    import pytest
+
+    from gbasis_amd.contractions import GeneralizedContractionShell
+    from gbasis_amd.electron_repulsion import (ElectronRepulsionIntegral,
+                                               ElectronRepulsionIntegralImproved)
+
+    cont = GeneralizedContractionShell(0, np.zeros(3), np.array([1.0]), np.array([0.1]),
+                                       "spherical")
+    for cls in (ElectronRepulsionIntegral, ElectronRepulsionIntegralImproved):
+        for k in range(4):
+            args = [cont] * 4
+            args[k] = None
+            with pytest.raises(TypeError):
+                cls.construct_array_contraction(*args)

@@ -99,6 +99,29 @@ def test_range_separated_vs_oracle(gpu, potential, omega):
     assert int((got_s == 0.0).sum()) > int((got == 0.0).sum())
 
 
+def test_construct_array_contraction_vs_oracle(gpu):
+    """The per-quartet class method: (M1, L1, ..., M4, L4) Cartesian blocks without
+    norm_cont, in the shells' component order, for reordered and generalised cases."""
+    from gbasis_amd.contractions import GeneralizedContractionShell
+    from gbasis_amd.electron_repulsion import ElectronRepulsionIntegralImproved
+    from oracle.assembly import contraction_block
+
+    _print_case("construct_array_contraction")
+    rng = np.random.default_rng(12)
+
+    def shell(l, m):
+        return GeneralizedContractionShell(l, rng.uniform(-1, 1, 3), rng.uniform(0.2, 1.0, (3, m)),
+                                           np.exp(rng.uniform(np.log(0.3), np.log(6.0), 3)),
+                                           "cartesian")
+
+    for ls in ([1, 0, 2, 0], [0, 2, 1, 1], [2, 2, 0, 1]):
+        conts = [shell(l, 1 + (i == 1)) for i, l in enumerate(ls)]
+        got = ElectronRepulsionIntegralImproved.construct_array_contraction(*conts)
+        ref = contraction_block(*conts, exact=True)
+        assert got.shape == ref.shape
+        assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), ls
+
+
 def test_schwarz_bounds_and_screened_lists(gpu):
     """Q_ij on the device vs the oracle's screener; screened quartet sets identical."""
     from gbasis_amd.engine import DeviceTables, get_engine
