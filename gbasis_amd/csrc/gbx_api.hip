@@ -31,7 +31,12 @@ __global__ void boys_kernel(gbx_tables t, const double* __restrict__ T, int64_t 
   const int64_t i = static_cast<int64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double* o = out + i * (mmax + 1);
-  boys_eval(T[i], mmax, t, [&](int m, double v) { o[m] = v; });
+  const double Ti = T[i];
+  if (!(Ti >= 0.0)) {  // negative or NaN: no table row; the ERI kernels never form one
+    for (int m = 0; m <= mmax; ++m) o[m] = __builtin_nan("");
+    return;
+  }
+  boys_eval(Ti, mmax, t, [&](int m, double v) { o[m] = v; });
 }
 
 __device__ inline uint64_t tri_idx(uint64_t i, uint64_t j) {

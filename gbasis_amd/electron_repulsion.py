@@ -23,6 +23,7 @@ FP32-bulk mixed precision.
 import numpy as np
 import torch
 
+from .boys_functions import get_boys_function
 from .contractions import GeneralizedContractionShell
 from .engine import DeviceTables, get_engine
 from .tables import BatchTables, MoleculeTables
@@ -153,6 +154,23 @@ class ElectronRepulsionIntegral:
     schwarz_threshold = 0.0
     potential = "coulomb"   # or "erf" / "erfc" with omega (ElectronRepulsionIntegralImproved)
     omega = None
+    # the reference's Boys hook (electron_repulsion.py:84,303): a callable from
+    # get_boys_function; its ``operator`` tag selects what the device evaluates
+    boys_func = get_boys_function("coulomb")
+
+    @classmethod
+    def _operator(cls):
+        """(potential, omega): a non-Coulomb ``boys_func`` wins over the ``potential``
+        and ``omega`` attributes.  Any other callable cannot run inside the device
+        kernels and is rejected."""
+        tag = getattr(cls.boys_func, "operator", None)
+        if tag is None:
+            raise NotImplementedError(
+                "`boys_func` must come from gbasis_amd.boys_functions.get_boys_function: the "
+                "device kernels evaluate the Boys function themselves")
+        if tag[0] != "coulomb":
+            return tag
+        return cls.potential, cls.omega
 
     def __init__(self, contractions):
         contractions = tuple(contractions)
@@ -161,8 +179,9 @@ class ElectronRepulsionIntegral:
 
     def _build(self, coord_types, **kwargs):
         thr = kwargs.get("schwarz_threshold", self.schwarz_threshold)
-        return _dense(self.contractions, coord_types, "chemist", thr, None,
-                      potential=self.potential, omega=self.omega)
+        pot, om = self._operator()
+        return _dense(self.contractions, coord_types, "chemist", thr, None, potential=pot,
+                      omega=om)
 
     def construct_array_cartesian(self, **kwargs):
         return self._build(["cartesian"] * len(self.contractions), **kwargs)
@@ -177,8 +196,9 @@ class ElectronRepulsionIntegral:
 
     def construct_array_lincomb(self, transform, coord_type, **kwargs):
         thr = kwargs.get("schwarz_threshold", self.schwarz_threshold)
+        pot, om = self._operator()
         return _dense(self.contractions, _coord_types(self.contractions, coord_type), "chemist",
-                      thr, transform, potential=self.potential, omega=self.omega)
+                      thr, transform, potential=pot, omega=om)
 
     @classmethod
     def construct_array_contraction(cls, cont_one, cont_two, cont_three, cont_four, **kwargs):
@@ -196,8 +216,8 @@ class ElectronRepulsionIntegral:
             if not isinstance(c, GeneralizedContractionShell):
                 raise TypeError(f"`{name}` must be a `GeneralizedContractionShell` instance.")
         shells = [_UnitNormShell(c) for c in conts]
-        dense = _dense(shells, ["cartesian"] * 4, "chemist", 0.0, None,
-                       potential=cls.potential, omega=cls.omega)
+        pot, om = cls._operator()
+        dense = _dense(shells, ["cartesian"] * 4, "chemist", 0.0, None, potential=pot, omega=om)
         sizes = [c.coeffs.shape[1] * len(c.angmom_components_cart) for c in conts]
         off = np.concatenate([[0], np.cumsum(sizes)])
         blk = dense[off[0]:off[1], off[1]:off[2], off[2]:off[3], off[3]:off[4]]

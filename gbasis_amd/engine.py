@@ -772,12 +772,35 @@ class EriEngine:
               "gbx_jk_packed")
         return J, K
 
-    def boys(self, dev: DeviceTables, T, mmax):
+    def boys(self, dev, T, mmax):
+        """F_m(T), m = 0..mmax, as a device (n, mmax + 1) tensor.  ``dev``: the tables
+        whose Boys table to use, or None for the engine's own copy."""
         T = torch.as_tensor(T, dtype=torch.float64, device=self.device).contiguous()
         out = torch.empty((T.numel(), mmax + 1), dtype=torch.float64, device=self.device)
-        check(self.lib.gbx_boys(ctypes.byref(dev.struct), ptr(T), T.numel(), mmax, ptr(out),
+        st = dev.struct if dev is not None else self._boys_struct()
+        check(self.lib.gbx_boys(ctypes.byref(st), ptr(T), T.numel(), mmax, ptr(out),
                                 self._stream()), "gbx_boys")
         return out
+
+    def _boys_struct(self):
+        """Tables struct with only the Boys table set (for standalone Boys values)."""
+        if getattr(self, "_boys_tab", None) is None:
+            tab = boys_table()
+            self._boys_dev = torch.tensor(np.asarray(tab), dtype=torch.float64,
+                                         device=self.device)
+            st = GbxTables()
+            st.boys_table = ptr(self._boys_dev)
+            st.boys_nt = tab.shape[0]
+            st.boys_nord = tab.shape[1]
+            st.boys_h = BOYS_H
+            self._boys_tab = st
+        return self._boys_tab
+
+    @property
+    def boys_max_order(self):
+        """Largest order the device Boys evaluation serves (table columns less the
+        Taylor terms)."""
+        return boys_table().shape[1] - 1 - 8
 
 
 _ENGINES = {}

@@ -91,7 +91,9 @@ const char *gbx_last_error(void);
 
 /* Boys function F_m(T), m = 0..mmax, for n values of T:
  * out[i*(mmax+1)+m].  Replaces boys_function_standard
- * (gbasis/integrals/boys_functions.py:26-63) evaluated on the device. */
+ * (gbasis/integrals/boys_functions.py:26-63) evaluated on the device.
+ * mmax + 8 must be below t->boys_nord (orders up to 24 with the shipped
+ * table); a negative or NaN T gives NaN for every order. */
 int gbx_boys(const gbx_tables *t, const double *d_T, int64_t n, int32_t mmax, double *d_out,
              void *stream);
 
