@@ -1,5 +1,6 @@
 """Boys functions evaluated on the device: the interface of
 gbasis/integrals/boys_functions.py (boys_function_standard :26-63,
+boys_function_all_orders :66-99, boys_function_recursion :102-142,
 boys_function_erf :145-182, boys_function_erfc :185-223,
 get_boys_function :290-345).
 
@@ -7,7 +8,9 @@ Values come from the device kernel the ERI kernels use (tabulated Taylor
 expansion, downward recursion, asymptotic branch beyond the table), within
 2.2e-15 relative of mpmath for orders up to 16 and T up to 1e5.  Orders up to
 ``get_engine().boys_max_order`` (24) are served; higher orders raise
-ValueError (there is no host fallback).
+ValueError (there is no host fallback).  The reference's mpmath evaluation
+(boys_function_mpmath :226-280) is a validation tool, not a device path; the
+tests use the oracle's mpmath Boys function in its place.
 """
 
 import numpy as np
@@ -35,6 +38,30 @@ def boys_function_standard(orders, weighted_dist, device="cuda"):
     tf = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
     F = eng.boys(None, tf, mmax).cpu().numpy()
     return F[np.arange(tf.size), o.reshape(-1).astype(np.int64)].reshape(o.shape)
+
+
+def boys_function_all_orders(m_max, weighted_dist, device="cuda"):
+    """F_m(T) for m = 0..m_max, shape (m_max + 1,) + T.shape (boys_functions.py:66-99).
+    One device pass gives every order (downward recursion from the top order)."""
+    T = np.asarray(weighted_dist, dtype=np.float64)
+    m_max = int(m_max)
+    if m_max < 0:
+        raise ValueError("m_max must be non-negative")
+    if np.any(T < 0):
+        raise ValueError("Boys arguments must be non-negative")
+    if T.size == 0:
+        return np.zeros((m_max + 1,) + T.shape)
+    eng = get_engine(device)
+    if m_max > eng.boys_max_order:
+        raise ValueError(f"Boys order {m_max} exceeds the device table (at most "
+                         f"{eng.boys_max_order})")
+    F = eng.boys(None, np.ascontiguousarray(T).reshape(-1), m_max).cpu().numpy()
+    return np.ascontiguousarray(F.T).reshape((m_max + 1,) + T.shape)
+
+
+# the reference's downward-recursion variant (boys_functions.py:102-142) computes the
+# same values; the device evaluation recurs downwards already
+boys_function_recursion = boys_function_all_orders
 
 
 def boys_function_erf(orders, weighted_dist, rho, omega, device="cuda"):

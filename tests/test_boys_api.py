@@ -16,6 +16,12 @@ def test_argument_errors():
     with pytest.raises(ValueError):
         boys_function_standard(np.array([0]), -np.ones(1))
     assert boys_function_standard(np.zeros((0, 1), dtype=int), np.ones(3)).shape == (0, 3)
+    from gbasis_amd.boys_functions import boys_function_all_orders
+    with pytest.raises(ValueError):
+        boys_function_all_orders(-1, np.ones(2))
+    with pytest.raises(ValueError):
+        boys_function_all_orders(3, -np.ones(2))
+    assert boys_function_all_orders(4, np.zeros((0, 2))).shape == (5, 0, 2)
 
 
 def test_factory_argument_errors():
@@ -69,6 +75,12 @@ def test_boys_functions_on_device(gpu):
     from gbasis_amd.engine import get_engine
     v = get_engine().boys(None, np.array([-1.0, np.nan, 2.0]), 3).cpu().numpy()
     assert np.all(np.isnan(v[:2])) and np.all(np.isfinite(v[2]))
+    from gbasis_amd.boys_functions import boys_function_all_orders
+    Ta = np.array([[0.0, 2.5], [45.0, 1e3]])
+    A = boys_function_all_orders(20, Ta)
+    assert A.shape == (21, 2, 2)
+    np.testing.assert_array_equal(A, boys_function_standard(np.arange(21)[:, None, None],
+                                                            Ta[None]))
     f = get_boys_function("erfc", omega=0.8)
     np.testing.assert_allclose(f(orders, Tr, rho=rho), boys_function_erfc(orders, Tr, rho, 0.8))
 
