@@ -614,7 +614,8 @@ def main():
                          "are single-molecule workloads")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this build runs the ERI path on one GPU only; --gpus must be 1",
+        print("bench.py: gbasis_amd measures the ERI path on one GPU only (multi-GPU "
+              "sharding is a design, not an implementation); --gpus must be 1",
               file=sys.stderr)
         sys.exit(2)
 
