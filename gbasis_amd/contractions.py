@@ -61,8 +61,10 @@ class GeneralizedContractionShell:
     def coord(self, coord):
         if not (isinstance(coord, np.ndarray) and coord.size == 3):
             raise TypeError("Coordinate must be given as a `numpy` array with 3 elements.")
+        if coord.dtype == int:      # integer coordinates are taken as floats
+            coord = coord.astype(float)
         if coord.dtype != float:
-            raise TypeError("Coordinate must be given as a `numpy` array of floats.")
+            raise TypeError("The data type of the coordinate must be int or float.")
         self._coord = coord
 
     @property

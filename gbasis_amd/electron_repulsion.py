@@ -23,6 +23,7 @@ FP32-bulk mixed precision.
 import numpy as np
 import torch
 
+from .base_four_symm import BaseFourIndexSymmetric
 from .boys_functions import get_boys_function
 from .contractions import GeneralizedContractionShell
 from .engine import DeviceTables, get_engine
@@ -148,8 +149,12 @@ def electron_repulsion_integral_improved(basis, transform=None, notation="physic
                   omega=omega)
 
 
-class ElectronRepulsionIntegral:
-    """Class interface of gbasis/integrals/electron_repulsion.py:20-211 (chemists' notation)."""
+class ElectronRepulsionIntegral(BaseFourIndexSymmetric):
+    """Class interface of gbasis/integrals/electron_repulsion.py:20-211 (chemists' notation).
+
+    Every quartet is evaluated on the device.  A subclass that supplies its own
+    ``construct_array_contraction`` gets the reference's per-quartet assembly
+    (``BaseFourIndexSymmetric``) around it instead."""
 
     schwarz_threshold = 0.0
     potential = "coulomb"   # or "erf" / "erfc" with omega (ElectronRepulsionIntegralImproved)
@@ -175,9 +180,19 @@ class ElectronRepulsionIntegral:
     def __init__(self, contractions):
         contractions = tuple(contractions)
         _check_basis(contractions)
-        self.contractions = contractions
+        self._axes_contractions = (contractions,)
+
+    def _plugin(self):
+        """True when a subclass replaces the device ``construct_array_contraction``."""
+        for klass in type(self).__mro__:
+            if "construct_array_contraction" in klass.__dict__:
+                return klass.__dict__["construct_array_contraction"] is not \
+                    ElectronRepulsionIntegral.__dict__["construct_array_contraction"]
+        return False
 
     def _build(self, coord_types, **kwargs):
+        if self._plugin():
+            return self._assemble([ct in ("spherical", "p") for ct in coord_types], kwargs)
         thr = kwargs.get("schwarz_threshold", self.schwarz_threshold)
         pot, om = self._operator()
         return _dense(self.contractions, coord_types, "chemist", thr, None, potential=pot,
@@ -195,6 +210,9 @@ class ElectronRepulsionIntegral:
         return self._build(_coord_types(self.contractions, coord_types), **kwargs)
 
     def construct_array_lincomb(self, transform, coord_type, **kwargs):
+        if self._plugin():
+            return BaseFourIndexSymmetric.construct_array_lincomb(self, transform, coord_type,
+                                                                  **kwargs)
         thr = kwargs.get("schwarz_threshold", self.schwarz_threshold)
         pot, om = self._operator()
         return _dense(self.contractions, _coord_types(self.contractions, coord_type), "chemist",
