@@ -40,6 +40,9 @@ def boys_function_standard(orders, weighted_dist, device="cuda"):
     return F[np.arange(tf.size), o.reshape(-1).astype(np.int64)].reshape(o.shape)
 
 
+boys_function_standard.operator = ("coulomb", None)
+
+
 def boys_function_all_orders(m_max, weighted_dist, device="cuda"):
     """F_m(T) for m = 0..m_max, shape (m_max + 1,) + T.shape (boys_functions.py:66-99).
     One device pass gives every order (downward recursion from the top order)."""

@@ -233,16 +233,22 @@ class ElectronRepulsionIntegral(BaseFourIndexSymmetric):
         for name, c in zip(names, conts):
             if not isinstance(c, GeneralizedContractionShell):
                 raise TypeError(f"`{name}` must be a `GeneralizedContractionShell` instance.")
-        shells = [_UnitNormShell(c) for c in conts]
         pot, om = cls._operator()
-        dense = _dense(shells, ["cartesian"] * 4, "chemist", 0.0, None, potential=pot, omega=om)
-        sizes = [c.coeffs.shape[1] * len(c.angmom_components_cart) for c in conts]
-        off = np.concatenate([[0], np.cumsum(sizes)])
-        blk = dense[off[0]:off[1], off[1]:off[2], off[2]:off[3], off[3]:off[4]]
-        shape = []
-        for c in conts:
-            shape += [c.coeffs.shape[1], len(c.angmom_components_cart)]
-        return np.ascontiguousarray(blk).reshape(shape)
+        return _quartet_block([_UnitNormShell(c) for c in conts], pot, om)
+
+
+def _quartet_block(shells, potential="coulomb", omega=None):
+    """(M1, L1, M2, L2, M3, L3, M4, L4) chemists' block of four shells whose
+    ``norm_cont`` is 1, cut from their dense array evaluated on the device."""
+    dense = _dense(shells, ["cartesian"] * 4, "chemist", 0.0, None, potential=potential,
+                   omega=omega)
+    sizes = [s.coeffs.shape[1] * len(s.angmom_components_cart) for s in shells]
+    off = np.concatenate([[0], np.cumsum(sizes)])
+    blk = dense[off[0]:off[1], off[1]:off[2], off[2]:off[3], off[3]:off[4]]
+    shape = []
+    for s in shells:
+        shape += [s.coeffs.shape[1], len(s.angmom_components_cart)]
+    return np.ascontiguousarray(blk).reshape(shape)
 
 
 class _UnitNormShell:

@@ -122,6 +122,51 @@ def test_construct_array_contraction_vs_oracle(gpu):
         assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), ls
 
 
+def test_engine_functions_vs_oracle(gpu):
+    """compute_two_electron_integrals_os_hgp and the first engine's entry points on raw
+    quartet data: (L_a, L_b, L_c, L_d, M_a, ..., M_d) against the oracle's quartet
+    (extended precision), a permuted component order, and the all-s form."""
+    from gbasis_amd.boys_functions import boys_function_standard, get_boys_function
+    from gbasis_amd.two_elec_int import (_compute_two_elec_integrals,
+                                         _compute_two_elec_integrals_angmom_zero,
+                                         compute_two_electron_integrals_os_hgp)
+    from gbasis_amd.utils import default_cart_components
+    from oracle.boys import boys_longdouble
+    from oracle.os_hgp import os_hgp_quartet
+
+    _print_case("engine functions")
+    rng = np.random.default_rng(31)
+
+    def raw(l, m):
+        return (rng.uniform(-1, 1, 3), l, default_cart_components(l),
+                np.exp(rng.uniform(np.log(0.3), np.log(6.0), 3)), rng.uniform(0.2, 1.0, (3, m)))
+
+    for ls in ([2, 1, 1, 0], [1, 1, 1, 1], [2, 0, 2, 2]):
+        q = [raw(l, 1 + (i == 2)) for i, l in enumerate(ls)]
+        flat = [x for a in q for x in a]
+        ref = os_hgp_quartet(*flat, boys=boys_longdouble, dtype=np.longdouble,
+                             ket="vrr").astype(np.float64)
+        got = compute_two_electron_integrals_os_hgp(boys_function_standard, *flat)
+        assert got.shape == ref.shape
+        tol = 1e-12 * max(1.0, np.abs(ref).max())
+        assert np.abs(got - ref).max() <= tol, ls
+        old = _compute_two_elec_integrals(get_boys_function("coulomb"), *flat)
+        assert np.abs(old - ref).max() <= tol, ls
+        # a reversed component order on centre a permutes the first axis
+        flat_p = list(flat)
+        flat_p[2] = flat[2][::-1]
+        got_p = compute_two_electron_integrals_os_hgp(boys_function_standard, *flat_p)
+        assert np.abs(got_p - got[::-1]).max() <= tol, ls
+    q = [raw(0, 2) for _ in range(4)]
+    args = [x for a in q for x in (a[0], a[3], a[4])]
+    flat = [x for a in q for x in a]
+    ref = os_hgp_quartet(*flat, boys=boys_longdouble, dtype=np.longdouble,
+                         ket="vrr").astype(np.float64)
+    got = _compute_two_elec_integrals_angmom_zero(boys_function_standard, *args)
+    assert got.shape == (1, 1, 1, 1, 2, 2, 2, 2)
+    assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+
+
 def test_schwarz_bounds_and_screened_lists(gpu):
     """Q_ij on the device vs the oracle's screener; screened quartet sets identical."""
     from gbasis_amd.engine import DeviceTables, get_engine
