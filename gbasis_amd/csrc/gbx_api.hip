@@ -67,6 +67,8 @@ extern "C" int gbx_unpack_dense(const double* d_packed, int64_t nfunc, int32_t n
                                 double* d_dense, void* stream) {
   if (nfunc < 0 || (nfunc > 0 && (!d_packed || !d_dense)) || (notation != 0 && notation != 1))
     return gbx::fail(GBX_ERR_ARG, "gbx_unpack_dense: bad arguments");
+  // nfunc^4 elements must fit int64 (and any dense array that large is far beyond memory)
+  if (nfunc > 32768) return gbx::fail(GBX_ERR_ARG, "gbx_unpack_dense: nfunc exceeds 32768");
   if (nfunc == 0) return GBX_OK;
   const int64_t total = nfunc * nfunc * nfunc * nfunc;
   const int threads = 256;

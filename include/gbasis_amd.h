@@ -238,7 +238,7 @@ int gbx_eri_hrr_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int
  * functions) to the dense (nfunc)^4 tensor; notation 0 = chemist (ab|cd),
  * 1 = physicist <ab|cd> = (ac|bd).  Replaces the 8-fold block placement and
  * concatenation of gbasis/base_four_symm.py:215-244 and the notation transpose
- * of gbasis/integrals/electron_repulsion.py:279-280. */
+ * of gbasis/integrals/electron_repulsion.py:279-280.  nfunc <= 32768. */
 int gbx_unpack_dense(const double *d_packed, int64_t nfunc, int32_t notation, double *d_dense,
                      void *stream);
 

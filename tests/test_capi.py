@@ -42,6 +42,16 @@ def test_error_path_without_device():
     assert b"null" in lib.gbx_last_error()
     assert lib.gbx_has_reg_kernel(0, 0, 0, 0) == 1
     assert lib.gbx_has_reg_kernel(4, 4, 4, 4) == 0
+    fake = ctypes.c_void_p(0x1000)       # never dereferenced: validation fails first
+    assert lib.gbx_unpack_dense(None, 4, 0, None, None) == -1
+    assert lib.gbx_unpack_dense(fake, 4, 2, fake, None) == -1          # notation
+    assert lib.gbx_unpack_dense(fake, 40000, 0, fake, None) == -1      # n^4 overflow
+    assert b"32768" in lib.gbx_last_error()
+    assert lib.gbx_unpack_dense(None, 0, 0, None, None) == 0           # empty
+    assert lib.gbx_jk_packed(fake, 0, fake, fake, fake, None) == -1
+    assert lib.gbx_jk_packed(fake, 70000, fake, fake, fake, None) == -1
+    assert lib.gbx_jk_packed(None, 4, fake, fake, fake, None) == -1
+    assert lib.gbx_boys(None, fake, 1, 0, fake, None) == -1
 
 
 def test_range_separated_argument_errors():
