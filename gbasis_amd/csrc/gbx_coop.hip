@@ -531,13 +531,16 @@ template <typename TB, bool kLDS, int NT, bool kRegAcc = false>
 int launch_coop_nt(const CoopArgs& a, const gbx_tables* t, int64_t grid, size_t shmem,
                    hipStream_t stream) {
   if (shmem > 65536) {
-    static bool raised = false;
-    if (!raised) {
+    // the attribute is set once per device (it is per-device state)
+    static bool raised[64] = {};
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(GBX_ERR_HIP, "eri_coop_kernel: no device");
+    if (dev < 0 || dev >= 64 || !raised[dev]) {
       if (hipFuncSetAttribute(reinterpret_cast<const void*>(&eri_coop_kernel<TB, kLDS, NT, kRegAcc>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(coop_lds_limit())) != hipSuccess)
         return fail(GBX_ERR_HIP, "eri_coop_kernel: could not raise the LDS limit");
-      raised = true;
+      if (dev >= 0 && dev < 64) raised[dev] = true;
     }
   }
   hipLaunchKernelGGL((eri_coop_kernel<TB, kLDS, NT, kRegAcc>), dim3(static_cast<unsigned>(grid)),

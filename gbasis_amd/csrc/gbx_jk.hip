@@ -155,13 +155,13 @@ extern "C" int gbx_jk_packed(const double* d_packed, int64_t nfunc, const double
     const unsigned grid =
         static_cast<unsigned>(std::max<uint64_t>(1, std::min<uint64_t>(want, 4 * uint64_t(ncu))));
     if (shmem > 65536) {
-      static bool raised = false;
-      if (!raised) {
+      static bool raised[64] = {};  // per device: the attribute is per-device state
+      if (dev < 0 || dev >= 64 || !raised[dev]) {
         if (hipFuncSetAttribute(reinterpret_cast<const void*>(&jk_digest_lds_kernel),
                                 hipFuncAttributeMaxDynamicSharedMemorySize,
                                 lds - 256) != hipSuccess)
           return fail(GBX_ERR_HIP, "gbx_jk_packed: could not raise the LDS limit");
-        raised = true;
+        if (dev >= 0 && dev < 64) raised[dev] = true;
       }
     }
     hipLaunchKernelGGL(jk_digest_lds_kernel, dim3(grid), dim3(kJkThreads), shmem, s, d_packed,

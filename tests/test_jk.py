@@ -65,10 +65,11 @@ def test_jk_argument_errors_without_device():
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n", [1, 5, 23, 110])
+@pytest.mark.parametrize("n", [1, 5, 23, 90, 110])
 def test_jk_random_packed_vs_einsum(gpu, n):
-    """n = 110 exceeds the LDS-private accumulators (n <= 101 on gfx950) and takes the
-    global-atomic kernel."""
+    """n = 90 needs more than 64 KiB of LDS-private accumulators (the raised LDS
+    limit); n = 110 exceeds them (n <= 101 on gfx950) and takes the global-atomic
+    kernel."""
     import torch
     from gbasis_amd.jk import coulomb_exchange_packed
 
