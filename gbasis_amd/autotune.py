@@ -21,7 +21,7 @@ import os
 
 import torch
 
-from ._lib import GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED
+from ._lib import GBX_FP32_BULK, GBX_FP64
 
 
 class Autotuner:

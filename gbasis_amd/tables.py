@@ -31,7 +31,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .programs import canonical_class, comps
+from .programs import comps
 from .spherical import generate_transformation
 from .utils import factorial2
 

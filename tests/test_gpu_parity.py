@@ -1,5 +1,4 @@
 """GPU parity: HIP kernels against the oracle and the reference's fixtures."""
-import json
 import os
 
 import numpy as np

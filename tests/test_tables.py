@@ -84,8 +84,6 @@ def test_canonical_orientation_and_buckets():
 
 def test_norm_cont_closed_form_is_unit_norm():
     """norm_cont normalises each contracted Cartesian function (closed-form overlap)."""
-    from oracle.assembly import dense_eri  # noqa: F401  (import check only)
-
     rng = np.random.default_rng(1)
     for l in range(4):
         exps = np.exp(rng.uniform(-2, 3, 4))
