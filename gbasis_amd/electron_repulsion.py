@@ -26,6 +26,7 @@ import torch
 from .base_four_symm import BaseFourIndexSymmetric
 from .boys_functions import get_boys_function
 from .contractions import GeneralizedContractionShell
+from .point_charge import PointChargeIntegral
 from .engine import DeviceTables, get_engine
 from .tables import BatchTables, MoleculeTables
 
@@ -160,8 +161,8 @@ class ElectronRepulsionIntegral(BaseFourIndexSymmetric):
     potential = "coulomb"   # or "erf" / "erfc" with omega (ElectronRepulsionIntegralImproved)
     omega = None
     # the reference's Boys hook (electron_repulsion.py:84,303): a callable from
-    # get_boys_function; its ``operator`` tag selects what the device evaluates
-    boys_func = get_boys_function("coulomb")
+    # gbasis_amd.boys_functions; its ``operator`` tag selects what the device evaluates
+    boys_func = PointChargeIntegral.boys_func
 
     @classmethod
     def _operator(cls):
@@ -265,3 +266,5 @@ class _UnitNormShell:
 
 class ElectronRepulsionIntegralImproved(ElectronRepulsionIntegral):
     """Class interface of gbasis/integrals/electron_repulsion.py:284-412."""
+
+    boys_func = get_boys_function("coulomb")

@@ -91,7 +91,15 @@ def test_boys_func_class_hook():
     from gbasis_amd.electron_repulsion import (ElectronRepulsionIntegral,
                                                ElectronRepulsionIntegralImproved)
 
+    from gbasis_amd.point_charge import PointChargeIntegral
+
     assert ElectronRepulsionIntegral._operator() == ("coulomb", None)
+    assert ElectronRepulsionIntegralImproved._operator() == ("coulomb", None)
+
+    class Old(ElectronRepulsionIntegralImproved):    # as the reference's screening tests
+        boys_func = PointChargeIntegral.boys_func
+
+    assert Old._operator() == ("coulomb", None)
 
     class Erfc(ElectronRepulsionIntegralImproved):
         boys_func = get_boys_function("erfc_coulomb", omega=0.3)
