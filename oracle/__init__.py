@@ -23,7 +23,11 @@ transposed operand fails a test):
                    screened by a Gaussian of exponent omega^2, and its centre
                    derivative for a p function.
   * assembly.py    the reference fixture data_horton_bec_cart_elec_repulsion.npy
-                   (Be+C STO-6G, physicist notation) and the Schwarz inequality.
+                   (Be+C STO-6G, physicist notation), the Schwarz inequality, and
+                   vectors written by the reference itself
+                   (tests/golden/reference_vectors.npz): its Schwarz matrices
+                   (identical screened lists) and sampled dense FP64 ERIs, within
+                   the reference's own FP64 error (tests/test_reference_vectors.py).
   * spherical tables (product module gbasis_amd.spherical, used here) are pinned
                    by HORTON cartpure matrices (tests/test_spherical.py).
 The tests of boys.py, os_hgp.py and assembly.py are in tests/test_oracle_golden.py.
