@@ -121,6 +121,46 @@ def test_construct_array_contraction_vs_oracle(gpu):
         assert np.abs(got - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), ls
 
 
+@pytest.mark.gpu
+def test_class_interface_vs_oracle(gpu):
+    """ElectronRepulsionIntegral[Improved](basis).construct_array_{cartesian, spherical,
+    mix, lincomb} on the device against the oracle's extended-precision dense arrays
+    (chemists' notation, base_four_symm.py:160-616)."""
+    from gbasis_amd.contractions import GeneralizedContractionShell
+    from gbasis_amd.electron_repulsion import (ElectronRepulsionIntegral,
+                                               ElectronRepulsionIntegralImproved)
+    from oracle.assembly import dense_eri
+
+    _print_case("class interface")
+    rng = np.random.default_rng(77)
+    basis = [GeneralizedContractionShell(l, rng.uniform(-1, 1, 3), rng.uniform(0.2, 1.0, (2, m)),
+                                         np.exp(rng.uniform(np.log(0.3), np.log(4.0), 2)),
+                                         "spherical")
+             for l, m in ((0, 1), (1, 2), (2, 1))]
+
+    def ref(types):
+        return dense_eri(basis, notation="chemist", coord_types=types, exact=True)
+
+    def close(a, b):
+        assert a.shape == b.shape
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(b).max())
+
+    cart = ref(["cartesian"] * 3)
+    mix_types = ["cartesian", "spherical", "spherical"]
+    mix = ref(mix_types)
+    for cls in (ElectronRepulsionIntegral, ElectronRepulsionIntegralImproved):
+        obj = cls(basis)
+        close(obj.construct_array_cartesian(), cart)
+        close(obj.construct_array_spherical(), ref(["spherical"] * 3))
+        close(obj.construct_array_mix(mix_types), mix)
+        T = rng.standard_normal((4, mix.shape[0]))
+        close(obj.construct_array_lincomb(T, mix_types),
+              np.einsum("ai,bj,ck,dl,ijkl->abcd", T, T, T, T, mix))
+    close(ElectronRepulsionIntegralImproved(basis).construct_array_cartesian(
+        schwarz_threshold=1e-12), cart)
+
+
+@pytest.mark.gpu
 def test_engine_functions_vs_oracle(gpu):
     """compute_two_electron_integrals_os_hgp and the first engine's entry points on raw
     quartet data: (L_a, L_b, L_c, L_d, M_a, ..., M_d) against the oracle's quartet
