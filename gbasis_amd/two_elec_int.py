@@ -77,7 +77,11 @@ def _compute_two_elec_integrals(boys_func,
                                 coord_b, angmom_b, angmom_components_b, exps_b, coeffs_b,
                                 coord_c, angmom_c, angmom_components_c, exps_c, coeffs_c,
                                 coord_d, angmom_d, angmom_components_d, exps_d, coeffs_d):
-    """The first engine's entry point (same integrals, same layout)."""
+    """The first engine's entry point (same integrals, same layout).  Four s shells
+    raise ValueError, as the reference's does (tests/test_two_elec_int.py:1466-1491):
+    they belong to ``_compute_two_elec_integrals_angmom_zero``."""
+    if angmom_a == angmom_b == angmom_c == angmom_d == 0:
+        raise ValueError("four s shells: use _compute_two_elec_integrals_angmom_zero")
     return compute_two_electron_integrals_os_hgp(
         boys_func, coord_a, angmom_a, angmom_components_a, exps_a, coeffs_a,
         coord_b, angmom_b, angmom_components_b, exps_b, coeffs_b,

@@ -34,3 +34,12 @@ def test_inconsistent_components_and_coefficients():
     q[4] = np.ones((3, 1))                      # three coefficients, two exponents
     with pytest.raises(ValueError):
         compute_two_electron_integrals_os_hgp(boys_function_standard, *q)
+
+
+def test_first_engine_rejects_four_s_shells():
+    """As the reference's test_compute_two_elec_integrals_prim: the general function of
+    the first engine refuses four s shells (they go to the closed-form function)."""
+    from gbasis_amd.two_elec_int import _compute_two_elec_integrals
+
+    with pytest.raises(ValueError):
+        _compute_two_elec_integrals(boys_function_standard, *_quartet(0))
