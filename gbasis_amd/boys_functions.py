@@ -23,21 +23,23 @@ def boys_function_standard(orders, weighted_dist, device="cuda"):
     ``weighted_dist``."""
     orders = np.asarray(orders)
     T = np.asarray(weighted_dist, dtype=np.float64)
-    o, t = np.broadcast_arrays(orders, T)
+    o = np.broadcast_to(orders, np.broadcast_shapes(orders.shape, T.shape))
     if o.size == 0:
         return np.zeros(o.shape)
     if np.any(o < 0) or np.any(o != np.floor(o)):
         raise ValueError("Boys orders must be non-negative integers")
-    if np.any(t < 0):
+    if np.any(T < 0):
         raise ValueError("Boys arguments must be non-negative")
     eng = get_engine(device)
     mmax = int(o.max())
     if mmax > eng.boys_max_order:
         raise ValueError(f"Boys order {mmax} exceeds the device table (at most "
                          f"{eng.boys_max_order})")
-    tf = np.ascontiguousarray(t, dtype=np.float64).reshape(-1)
+    # each element of weighted_dist once, with all orders; then pick (argument, order)
+    tf = np.ascontiguousarray(T, dtype=np.float64).reshape(-1)
     F = eng.boys(None, tf, mmax).cpu().numpy()
-    return F[np.arange(tf.size), o.reshape(-1).astype(np.int64)].reshape(o.shape)
+    which = np.broadcast_to(np.arange(tf.size).reshape(T.shape), o.shape)
+    return F[which, o.astype(np.int64)]
 
 
 boys_function_standard.operator = ("coulomb", None)

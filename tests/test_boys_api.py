@@ -114,3 +114,36 @@ def test_boys_func_class_hook():
     assert Attr._operator() == ("erf", 0.7)
     with pytest.raises(NotImplementedError):
         Custom._operator()
+
+
+def test_broadcasting_and_order_selection(monkeypatch):
+    """Host logic of boys_function_standard / _all_orders with a stand-in device call
+    that returns F[i, m] = 100 T_i + m: each argument is evaluated once, and the
+    broadcast (orders, weighted_dist) pairs pick the right row and column."""
+    import torch
+
+    import gbasis_amd.boys_functions as bf
+
+    calls = []
+
+    class FakeEngine:
+        boys_max_order = 24
+
+        def boys(self, dev, T, mmax):
+            calls.append(np.asarray(T).size)
+            T = torch.as_tensor(np.asarray(T))
+            return 100 * T[:, None] + torch.arange(mmax + 1, dtype=torch.float64)[None, :]
+
+    monkeypatch.setattr(bf, "get_engine", lambda device="cuda": FakeEngine())
+    orders = np.arange(4)[:, None, None]
+    T = np.array([[0.5, 1.5, 2.5], [3.5, 4.5, 5.5]])[None]
+    got = bf.boys_function_standard(orders, T)
+    assert got.shape == (4, 2, 3) and calls == [6]
+    np.testing.assert_array_equal(got, 100 * T + orders)
+    np.testing.assert_array_equal(bf.boys_function_standard(2, 7.0), 702.0)
+    allo = bf.boys_function_all_orders(3, T[0])
+    np.testing.assert_array_equal(allo, 100 * T + orders)
+    s = 0.4 ** 2 / (0.4 ** 2 + 0.8)
+    np.testing.assert_allclose(bf.get_boys_function("erf", omega=0.4)(np.array([1]), np.array([2.0]),
+                                                                       rho=0.8),
+                               s ** 1.5 * (100 * s * 2.0 + 1))
