@@ -504,6 +504,7 @@ def encoded_program(cls, sub=None):
     header, items, phases = prog.encode()
     out_slots = np.asarray(prog.c_out_slots, dtype=np.int32)
     if path is not None:
+        tmp = None
         try:
             os.makedirs(d, exist_ok=True)
             fd, tmp = tempfile.mkstemp(dir=d, suffix=".npz")
@@ -511,5 +512,10 @@ def encoded_program(cls, sub=None):
                 np.savez(fh, header=header, items=items, phases=phases, out_slots=out_slots)
             os.replace(tmp, path)
         except OSError:
-            pass  # a read-only location only costs the rebuild next time
+            # a read-only or full location only costs the rebuild next time
+            if tmp is not None and os.path.exists(tmp):
+                try:
+                    os.remove(tmp)
+                except OSError:
+                    pass
     return header, items, phases, out_slots
