@@ -7,7 +7,10 @@
 // threshold <= 0 keeps everything), oriented so the "heavier" pair is the
 // bra, marked FP32-bulk iff P_p * P_q < fp32_cutoff (P: bound of the contracted
 // [e0|f0], the input of the horizontal recursion), and assigned to a bucket
-//   key = (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + ceil(log2(nprim_bra*nprim_ket)).
+//   key = (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + bin,
+//   bin = min(ceil(log2(nprim_bra*nprim_ket)), 15)
+// (the bin only groups quartets of similar cost; kernels read the true primitive
+// counts from the pair table, so the last bin holds every count above 2^14).
 // Pass 1 counts quartets per bucket (optionally also packed slots and
 // primitive quartets per bucket); pass 2 writes (bra, ket) pairs at
 // bucket_offset[key] + an atomically reserved position.  Both passes
@@ -47,9 +50,11 @@ struct QueueArgs {
   int64_t n_cand;
 };
 
-__device__ inline int ceil_log2(int x) {
+// primitive-count bin of the bucket key: ceil(log2(x)), at most kMaxBin
+constexpr int kMaxBin = 15;
+__device__ inline int prim_bin(int x) {
   int b = 0;
-  while ((1 << b) < x) ++b;
+  while (b < kMaxBin && (1 << b) < x) ++b;
   return b;
 }
 
@@ -97,7 +102,7 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       const double cut = a.mol_fp32_cutoff ? a.mol_fp32_cutoff[lo] : a.fp32_cutoff;
       const int fp32 = (cut > 0.0 && pbound < cut &&
                         (a.fp32_ok == nullptr || a.fp32_ok[cls * 2 + dense])) ? 1 : 0;
-      key = ((cls * 2 + dense) * 2 + fp32) * 16 + ceil_log2(np);
+      key = ((cls * 2 + dense) * 2 + fp32) * 16 + prim_bin(np);
       if (!kFill && a.bucket_eris) {
         const uint64_t nb = static_cast<uint64_t>(a.pair_npf[bra]);
         const uint64_t nk = static_cast<uint64_t>(a.pair_npf[ket]);

@@ -187,8 +187,7 @@ class ElectronRepulsionIntegral(BaseFourIndexSymmetric):
         """True when a subclass replaces the device ``construct_array_contraction``."""
         for klass in type(self).__mro__:
             if "construct_array_contraction" in klass.__dict__:
-                return klass.__dict__["construct_array_contraction"] is not \
-                    ElectronRepulsionIntegral.__dict__["construct_array_contraction"]
+                return klass.__dict__["construct_array_contraction"] not in _DEVICE_CONTRACTION
         return False
 
     def _build(self, coord_types, **kwargs):
@@ -226,8 +225,8 @@ class ElectronRepulsionIntegral(BaseFourIndexSymmetric):
         ``norm_cont`` (electron_repulsion.py:87-211, :306-412).
 
         The quartet is evaluated on the device as the four shells' dense array, from
-        which the block is cut.  The reference's optional ``screener`` keyword is
-        accepted and not used: the block is always computed.
+        which the block is cut.  As in the reference, this engine takes no screening
+        keywords (``ElectronRepulsionIntegralImproved`` does).
         """
         conts = (cont_one, cont_two, cont_three, cont_four)
         names = ("cont_one", "cont_two", "cont_three", "cont_four")
@@ -268,3 +267,34 @@ class ElectronRepulsionIntegralImproved(ElectronRepulsionIntegral):
     """Class interface of gbasis/integrals/electron_repulsion.py:284-412."""
 
     boys_func = get_boys_function("coulomb")
+
+    @classmethod
+    def construct_array_contraction(cls, cont_one, cont_two, cont_three, cont_four, **kwargs):
+        """Cartesian block of one shell quartet, as the base class, with the Schwarz test of
+        electron_repulsion.py:341-361: given both ``screener`` (a SchwarzScreener) and
+        ``contraction_index_map`` ({id(shell): shell index}), a quartet whose four shells
+        are all in the map and that ``screener.is_significant(i, j, k, l)`` rejects is
+        returned as a zero block without any evaluation; ``is_significant`` updates the
+        screener's counters as in the reference."""
+        conts = (cont_one, cont_two, cont_three, cont_four)
+        names = ("cont_one", "cont_two", "cont_three", "cont_four")
+        for name, c in zip(names, conts):
+            if not isinstance(c, GeneralizedContractionShell):
+                raise TypeError(f"`{name}` must be a `GeneralizedContractionShell` instance.")
+        screener = kwargs.get("screener")
+        index_map = kwargs.get("contraction_index_map")
+        if screener is not None and index_map is not None:
+            ids = [index_map.get(id(c)) for c in conts]
+            if all(x is not None for x in ids) and not screener.is_significant(*ids):
+                shape = []
+                for c in conts:
+                    shape += [c.coeffs.shape[1], len(c.angmom_components_cart)]
+                return np.zeros(shape)
+        return ElectronRepulsionIntegral.construct_array_contraction.__func__(
+            cls, cont_one, cont_two, cont_three, cont_four)
+
+
+# the device implementations of the per-quartet hook (a subclass replacing them runs the
+# host per-quartet assembly instead)
+_DEVICE_CONTRACTION = (ElectronRepulsionIntegral.__dict__["construct_array_contraction"],
+                       ElectronRepulsionIntegralImproved.__dict__["construct_array_contraction"])

@@ -235,6 +235,22 @@ class MoleculeTables:
         return bra, ket
 
 
+MAX_PRIM_BIN = 15
+
+
+def prim_bin(nprimq):
+    """Primitive-count bin of the device work-queue key: ceil(log2(n)), at most 15 (4 bits
+    of the key; the last bin holds every count above 2^14, the kernels read the true
+    counts from the pair table)."""
+    n = np.maximum(np.asarray(nprimq, dtype=np.int64), 1)
+    b = np.zeros(n.shape, dtype=np.int64)
+    while True:
+        grow = (b < MAX_PRIM_BIN) & ((np.int64(1) << b) < n)
+        if not grow.any():
+            return b
+        b += grow
+
+
 def _n_slots(nfunc):
     npair = nfunc * (nfunc + 1) // 2
     return npair * (npair + 1) // 2
@@ -334,8 +350,7 @@ class BatchTables:
             dense = ((kb >> 6) & 1) | ((kk >> 6) & 1)
             npq = (kb >> 8) * (kk >> 8)
             cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7)
-            binid = np.ceil(np.log2(np.maximum(npq, 1))).astype(np.int64)
-            key = ((cls * 2 + dense) * 2) * 16 + binid
+            key = ((cls * 2 + dense) * 2) * 16 + prim_bin(npq)
             np.add.at(cap, key, n)
         self._capacity = cap
         return cap

@@ -27,6 +27,12 @@ from .os_hgp import os_hgp_quartet
 
 
 def _engine(s1, s2, s3, s4, exact=False, potential="coulomb", omega=None):
+    if exact == "quad":
+        if potential != "coulomb":
+            raise ValueError("the binary128 engine evaluates the Coulomb operator only")
+        from .hp import hp_engine
+
+        return hp_engine(s1, s2, s3, s4, prec="quad", mode=1, tau=0.0)
     if exact:
         return os_hgp_quartet(
             s1.coord, s1.angmom, s1.angmom_components_cart, s1.exps, s1.coeffs,
@@ -55,6 +61,11 @@ def schwarz_bounds(basis, exact=False):
 
 def contraction_block(s1, s2, s3, s4, exact=False, potential="coulomb", omega=None):
     """(M1,L1,M2,L2,M3,L3,M4,L4) Cartesian block with the reference's l-reordering.
+
+    ``exact="quad"`` evaluates them in IEEE binary128 with every primitive pair built on
+    the centre of its larger exponent (oracle/eri_hp.c), the accuracy reference for
+    shells up to g with exponents up to 1e5, where the horizontal recursion of the
+    reference's orientation amplifies x87 rounding beyond 1e-12.
 
     ``exact=True`` evaluates the integrals in x87 extended precision with an
     extended-precision Boys function and the bounded-coefficient Obara-Saika build

@@ -65,6 +65,26 @@ def test_dense_vs_oracle(gpu, ls, coord):
     assert np.all(np.abs(got - ref64) <= ref_err + 1e-12 * scale)
 
 
+def test_deep_contractions_vs_oracle(gpu):
+    """Shells of K = 14 primitives: their 38416 primitive quartets per (ss|ss) lie beyond
+    the largest primitive-count bin of the work-queue key, which must keep them FP64 and
+    in their own class (screened and mixed-precision runs included)."""
+    from gbasis_amd.electron_repulsion import (electron_repulsion_integral_improved,
+                                               electron_repulsion_integral_packed)
+    from oracle.assembly import dense_eri
+
+    _print_case("deep contractions K=14")
+    basis = random_basis(14, [0, 0, 1], kmin=14, kmax=14, emin=0.05, emax=500.0)
+    got = electron_repulsion_integral_improved(basis, notation="chemist",
+                                               schwarz_threshold=1e-12)
+    exact = dense_eri(basis, notation="chemist", exact=True)
+    scale = max(1.0, np.abs(exact).max())
+    assert np.abs(got - exact).max() <= 1e-12 * scale
+    fp64, _ = electron_repulsion_integral_packed(basis, precision="fp64")
+    mixed, _ = electron_repulsion_integral_packed(basis, precision="mixed")
+    assert float((mixed - fp64).abs().max()) <= 1e-8
+
+
 @pytest.mark.parametrize("potential,omega", [("erf", 0.4), ("erfc", 0.4), ("erf", 2.5)])
 def test_range_separated_vs_oracle(gpu, potential, omega):
     """erf / erfc-attenuated operators (cooperative kernels, FP64) against the oracle's

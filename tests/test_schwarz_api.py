@@ -48,3 +48,67 @@ def test_compute_schwarz_bounds_vs_oracle(gpu):
     ref = schwarz_bounds(basis, exact=True)
     assert got.shape == ref.shape and np.array_equal(got, got.T)
     assert np.all(np.abs(got ** 2 - ref ** 2) <= 1e-12 * np.maximum(1.0, ref ** 2))
+
+
+def test_construct_array_contraction_screened_block():
+    """ElectronRepulsionIntegralImproved.construct_array_contraction with the reference's
+    screener keywords (electron_repulsion.py:341-361): a rejected quartet is a zero block
+    of the (M1, L1, ..., M4, L4) shape, returned without evaluation, and the screener's
+    counters move; shells missing from the index map are never screened."""
+    from gbasis_amd.electron_repulsion import ElectronRepulsionIntegralImproved
+
+    a = GeneralizedContractionShell(1, np.zeros(3), np.ones((2, 2)), np.array([1.0, 0.3]),
+                                    "cartesian")
+    b = _s([0, 0, 9.0], [0.2])
+    Q = np.array([[1.0, 1e-8], [1e-8, 1.0]])
+    scr = SchwarzScreener(None, bounds=Q, threshold=1e-12)
+    imap = {id(a): 0, id(b): 1}
+    blk = ElectronRepulsionIntegralImproved.construct_array_contraction(
+        a, b, b, a, screener=scr, contraction_index_map=imap)
+    assert blk.shape == (2, 3, 1, 1, 1, 1, 2, 3) and not blk.any()
+    blk = ElectronRepulsionIntegralImproved.construct_array_contraction(
+        b, a, a, b, screener=scr, contraction_index_map=imap)
+    assert blk.shape == (1, 1, 2, 3, 2, 3, 1, 1) and not blk.any()
+    st = scr.get_statistics()
+    assert st["n_screened"] == 2 and st["n_computed"] == 0
+    # the type checks still come first
+    with pytest.raises(TypeError):
+        ElectronRepulsionIntegralImproved.construct_array_contraction(
+            a, None, b, a, screener=scr, contraction_index_map=imap)
+    assert scr.get_statistics()["total"] == 2
+
+
+@pytest.mark.gpu
+def test_construct_array_contraction_screener_vs_oracle(gpu):
+    """Quartets of a basis through the per-quartet class method with a SchwarzScreener at a
+    threshold that screens: the zero blocks are exactly the quartets the oracle's bounds
+    screen, the others equal the oracle's blocks, and the counters add up."""
+    from gbasis_amd.electron_repulsion import ElectronRepulsionIntegralImproved
+    from oracle.assembly import contraction_block, schwarz_bounds
+    from tests.molecules import random_basis
+
+    print("[gpu case] construct_array_contraction with a screener", flush=True)
+    basis = random_basis(23, [0, 1, 0, 1], kmax=2, box=12.0)
+    ref_q = schwarz_bounds(basis, exact=True)
+    # the bound products of these quartets are 1e-30..1e-20 or 5e-9..1.2: far from the
+    # threshold on both sides
+    thr = 1e-12
+    scr = SchwarzScreener(basis, threshold=thr)
+    imap = {id(s): i for i, s in enumerate(basis)}
+    n = len(basis)
+    screened = computed = 0
+    for i, j, k, l in ((0, 1, 2, 3), (0, 0, 3, 3), (1, 2, 1, 3), (2, 3, 0, 1), (0, 2, 1, 3),
+                       (3, 3, 3, 3), (1, 1, 2, 2)):
+        assert max(i, j, k, l) < n
+        blk = ElectronRepulsionIntegralImproved.construct_array_contraction(
+            basis[i], basis[j], basis[k], basis[l], screener=scr, contraction_index_map=imap)
+        if ref_q[i, j] * ref_q[k, l] < thr:
+            screened += 1
+            assert not blk.any(), (i, j, k, l)
+        else:
+            computed += 1
+            ref = contraction_block(basis[i], basis[j], basis[k], basis[l], exact=True)
+            assert np.abs(blk - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max()), (i, j, k, l)
+    st = scr.get_statistics()
+    assert (st["n_screened"], st["n_computed"]) == (screened, computed)
+    assert screened > 0 and computed > 0

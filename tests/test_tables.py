@@ -105,3 +105,26 @@ def test_norm_cont_closed_form_is_unit_norm():
                 iy = np.sqrt(np.pi / a)
                 tot += c[p] * c[q] * ix * iy * iy
         assert abs(tot - 1.0) < 1e-10
+
+
+def test_bucket_capacity_deep_contractions():
+    """Shells with K = 14 primitives (cc-pVTZ S/Cl, cc-pV5Z C s shells): 14^4 = 38416
+    primitive quartets exceed the 4-bit primitive-count bin, which must stay in the last
+    bin and never carry into the precision or output-transform bits of the key.  The
+    capacities cover every unscreened candidate exactly once."""
+    from gbasis_amd.tables import MAX_PRIM_BIN, prim_bin
+
+    assert prim_bin(np.array([1, 2, 3, 16384, 16385, 38416, 2 ** 30])).tolist() == \
+        [0, 1, 2, 14, 15, 15, 15]
+    basis = random_basis(5, [0, 0, 1, 0], kmin=14, kmax=14, box=4.0)
+    m = MoleculeTables(basis)
+    b = BatchTables([m])
+    cap = b.bucket_capacity()
+    keys = np.flatnonzero(cap)
+    assert int(cap.sum()) == b.n_cand
+    binid, fp32, dense = keys & 15, (keys >> 4) & 1, (keys >> 5) & 1
+    cls = keys >> 6
+    assert np.all(binid == MAX_PRIM_BIN) and not fp32.any() and not dense.any()
+    got = {(int(c >> 9) & 7, int(c >> 6) & 7, int(c >> 3) & 7, int(c) & 7) for c in cls}
+    assert got == {(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0),
+                   (1, 1, 1, 1)}
