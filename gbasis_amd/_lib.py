@@ -19,6 +19,7 @@ GBX_FP64 = 0
 GBX_FP32_BULK = 1
 GBX_MODE_PACKED = 0
 GBX_MODE_SCHWARZ = 1
+GBX_MODE_PACKED_ADD = 2
 GBX_POT_COULOMB = 0
 GBX_POT_ERF = 1
 GBX_POT_ERFC = 2
@@ -70,13 +71,14 @@ SIGNATURES = {
                                    c_vp, c_vp, c_i64, c_vp]),
     "gbx_has_hrr_kernel": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i32]),
     "gbx_eri_hrr_reg": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32, c_i32,
-                                       c_vp, c_i64, c_vp, c_vp, c_i64, c_i32, c_vp, c_vp]),
-    "gbx_quartets_count": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                       c_vp, c_i64, c_vp, c_vp, c_i64, c_i32, c_i32, c_vp,
+                                       c_vp]),
+    "gbx_quartets_count": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
                                           ctypes.c_double, ctypes.c_double, c_vp, c_vp, c_i64,
                                           c_vp, c_vp, c_vp, c_vp]),
-    "gbx_quartets_fill": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, ctypes.c_double,
-                                         ctypes.c_double, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp,
-                                         c_vp]),
+    "gbx_quartets_fill": (ctypes.c_int, [c_i32, c_vp, c_vp, c_vp, c_vp, c_vp, c_vp,
+                                         ctypes.c_double, ctypes.c_double, c_vp, c_vp, c_i64,
+                                         c_vp, c_vp, c_vp, c_vp]),
     "gbx_eri_reg": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32, c_i32, c_vp,
                                    c_i64, c_vp, c_i32, c_i32, c_vp, c_vp]),
 }

@@ -1,4 +1,5 @@
-"""Generator of the per-class register kernels (gbasis_amd/csrc/gbx_reg_gen.hip).
+"""Generator of the per-class register kernels (gbasis_amd/csrc/gbx_reg_gen_<k>.hip, with
+the dispatch in gbasis_amd/csrc/gbx_reg_gen.hip).
 
 For each angular-momentum class in REG_CLASSES (kernel orientation, see
 programs.canonical_class) this emits one HIP kernel in which every lane
@@ -157,7 +158,10 @@ def class_plan(cls):
 
 
 def _reg_classes():
-    """Kernel-orientation classes with l <= 2 per shell (generated register kernels)."""
+    """Classes with l <= 2 per shell that get generated register kernels: the kernel
+    orientations of the shell quartets (la >= lb, lc >= ld, bra heavier), then the
+    orientations of their sub-quartets whose bra or ket entry is built on its second
+    centre (entries (lb, la) with lb > 0, and likewise on the ket)."""
     out = []
     for la in range(3):
         for lb in range(la + 1):
@@ -166,7 +170,12 @@ def _reg_classes():
                     if (la + lb, la) < (lc + ld, lc):
                         continue
                     out.append((la, lb, lc, ld))
-    return out
+    extra = set()
+    for la, lb, lc, ld in out:
+        for bra in ((la, lb), (lb, la)) if lb > 0 else ((la, lb),):
+            for ket in ((lc, ld), (ld, lc)) if ld > 0 else ((lc, ld),):
+                extra.add(bra + ket)
+    return out + sorted(extra - set(out))
 
 
 def _f2(comp):
@@ -454,122 +463,188 @@ def hrr_kernel_source(cls):
     return "\n".join(o)
 
 
-def generate(path=OUT):
-    classes = _reg_classes()
-    parts = [
-        "// GENERATED by gbasis_amd/codegen.py -- do not edit.",
-        "// Per-class register kernels: one lane = one shell quartet.",
-        '#include "gbx_reg.h"',
-        "",
-        "namespace gbx {",
-        "",
-    ]
-    fused, split = [], []
+N_PARTS = 8   # translation units of the generated kernels (compiled in parallel)
+NOT_HERE = 1  # return value of a part's launcher for a class it does not hold
+
+
+def _key(c):
+    return ((c[0] * 8 + c[1]) * 8 + c[2]) * 8 + c[3]
+
+
+def _part_sources(classes):
+    """Kernel sources of every class, grouped into N_PARTS translation units of similar
+    size, each with its own launchers (which return NOT_HERE for other classes)."""
+    units = []
     for cls in classes:
         kind, pl = class_plan(cls)
+        src = []
         if kind == "fused":
-            fused.append(cls)
-            parts.append(kernel_source(cls))
+            src.append(kernel_source(cls))
         else:
-            split.append((cls, len(pl)))
             for k in range(len(pl)):
-                parts.append(split_kernel_source(cls, k))
+                src.append(split_kernel_source(cls, k))
             if cls not in HRR_REG_EXCLUDE:
-                parts.append(hrr_kernel_source(cls))
-        parts.append("")
-
-    def key(c):
-        return ((c[0] * 8 + c[1]) * 8 + c[2]) * 8 + c[3]
-
-    parts.append("int launch_reg_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
-                 "const RegArgs& a, int precision, hipStream_t s) {")
-    parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
+                src.append(hrr_kernel_source(cls))
+        units.append((cls, kind, len(pl), "\n".join(src)))
+    bins = [[] for _ in range(N_PARTS)]
+    size = [0] * N_PARTS
+    for u in sorted(units, key=lambda u: -len(u[3])):
+        k = size.index(min(size))
+        bins[k].append(u)
+        size[k] += len(u[3])
+    out = []
+    for k, members in enumerate(bins):
+        members.sort(key=lambda u: _key(u[0]))
+        o = ["// GENERATED by gbasis_amd/codegen.py -- do not edit.",
+             f"// Per-class register kernels, part {k} of {N_PARTS}: one lane = one shell quartet.",
+             '#include "gbx_reg.h"', "", "namespace gbx {", ""]
+        o += [u[3] + "\n" for u in members]
+        o.append(f"int launch_reg_part{k}(const gbx_tables* t, int key, const RegArgs& a, "
+                 "int precision, hipStream_t s) {")
+        o.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
                  "kRegThreads);")
-    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
-    for cls in fused:
-        name = f"eri_reg_{''.join(map(str, cls))}"
-        parts.append(f"    case {key(cls)}:")
-        parts.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
+        o.append("  (void)grid;")
+        o.append("  switch (key) {")
+        for cls, kind, _, _ in members:
+            if kind != "fused":
+                continue
+            name = f"eri_reg_{''.join(map(str, cls))}"
+            o.append(f"    case {_key(cls)}:")
+            o.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
                      "dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
-        parts.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
+            o.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
                      "dim3(kRegThreads), 0, s, *t, a);")
-        parts.append(f'      return check_launch("{name}");')
-    parts.append("    default:")
-    parts.append('      return fail(GBX_ERR_UNSUPPORTED, "no fused register kernel for this class");')
-    parts.append("  }")
-    parts.append("}")
-    parts.append("")
-    parts.append("int launch_split_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
-                 "int k, const RegArgs& a, int precision, hipStream_t s) {")
-    parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart * a.lanes_per_quartet + "
+            o.append(f'      return check_launch("{name}");')
+        o.append("    default:")
+        o.append(f"      return {NOT_HERE};")
+        o.append("  }")
+        o.append("}")
+        o.append("")
+        o.append(f"int launch_split_part{k}(const gbx_tables* t, int key, int split, "
+                 "const RegArgs& a, int precision, hipStream_t s) {")
+        o.append("  const unsigned grid = static_cast<unsigned>((a.nquart * a.lanes_per_quartet + "
                  "kRegThreads - 1) / kRegThreads);")
-    parts.append("  switch ((((la * 8 + lb) * 8 + lc) * 8 + ld) * 64 + k) {")
-    for cls, n in split:
-        for k in range(n):
-            name = f"eri_split_{''.join(map(str, cls))}_{k}"
-            parts.append(f"    case {key(cls) * 64 + k}:")
-            parts.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
+        o.append("  (void)grid;")
+        o.append("  switch (key * 64 + split) {")
+        for cls, kind, n, _ in members:
+            if kind == "fused":
+                continue
+            for j in range(n):
+                name = f"eri_split_{''.join(map(str, cls))}_{j}"
+                o.append(f"    case {_key(cls) * 64 + j}:")
+                o.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
                          "dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
-            parts.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
+                o.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
                          "dim3(kRegThreads), 0, s, *t, a);")
-            parts.append(f'      return check_launch("{name}");')
-    parts.append("    default:")
-    parts.append('      return fail(GBX_ERR_UNSUPPORTED, "no split register kernel for this class");')
-    parts.append("  }")
-    parts.append("}")
-    parts.append("")
-    parts.append("int launch_hrr_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
-                 "const RegArgs& a, hipStream_t s) {")
-    parts.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
+                o.append(f'      return check_launch("{name}");')
+        o.append("    default:")
+        o.append(f"      return {NOT_HERE};")
+        o.append("  }")
+        o.append("}")
+        o.append("")
+        o.append(f"int launch_hrr_part{k}(const gbx_tables* t, int key, const RegArgs& a, "
+                 "hipStream_t s) {")
+        o.append("  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / "
                  "kRegThreads);")
-    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
-    hrr = [cls for cls, _ in split if cls not in HRR_REG_EXCLUDE]
-    for cls in hrr:
-        name = f"eri_hrr_{''.join(map(str, cls))}"
-        parts.append(f"    case {key(cls)}:")
-        parts.append(f"      hipLaunchKernelGGL({name}, dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
-        parts.append(f'      return check_launch("{name}");')
-    parts.append("    default:")
-    parts.append('      return fail(GBX_ERR_UNSUPPORTED, "no register HRR kernel for this class");')
-    parts.append("  }")
-    parts.append("}")
-    parts.append("")
-    parts.append("bool has_hrr_generated(int la, int lb, int lc, int ld) {")
-    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
-    for cls in hrr:
-        parts.append(f"    case {key(cls)}:")
-    parts.append("      return true;")
-    parts.append("    default:")
-    parts.append("      return false;")
-    parts.append("  }")
-    parts.append("}")
-    parts.append("")
-    parts.append("bool has_reg_generated(int la, int lb, int lc, int ld) {")
-    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
-    for cls in fused:
-        parts.append(f"    case {key(cls)}:")
-    parts.append("      return true;")
-    parts.append("    default:")
-    parts.append("      return false;")
-    parts.append("  }")
-    parts.append("}")
-    parts.append("")
-    parts.append("int split_count_generated(int la, int lb, int lc, int ld) {")
-    parts.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
-    for cls, n in split:
-        parts.append(f"    case {key(cls)}:")
-        parts.append(f"      return {n};")
-    parts.append("    default:")
-    parts.append("      return 0;")
-    parts.append("  }")
-    parts.append("}")
-    parts.append("")
-    parts.append("}  // namespace gbx")
-    src = "\n".join(parts) + "\n"
+        o.append("  (void)grid;")
+        o.append("  switch (key) {")
+        for cls, kind, _, _ in members:
+            if kind == "fused" or cls in HRR_REG_EXCLUDE:
+                continue
+            name = f"eri_hrr_{''.join(map(str, cls))}"
+            o.append(f"    case {_key(cls)}:")
+            o.append(f"      hipLaunchKernelGGL({name}, dim3(grid), dim3(kRegThreads), 0, s, "
+                     "*t, a);")
+            o.append(f'      return check_launch("{name}");')
+        o.append("    default:")
+        o.append(f"      return {NOT_HERE};")
+        o.append("  }")
+        o.append("}")
+        o.append("")
+        o.append("}  // namespace gbx")
+        out.append("\n".join(o) + "\n")
+    return out
+
+
+def _write(path, src):
     if os.path.exists(path) and open(path).read() == src:
-        return classes
+        return
     with open(path, "w") as fh:
         fh.write(src)
+
+
+def generate(path=OUT):
+    """Write the kernel parts gbx_reg_gen_<k>.hip and the dispatch file ``path``."""
+    classes = _reg_classes()
+    base = path[:-len(".hip")]
+    for k, src in enumerate(_part_sources(classes)):
+        _write(f"{base}_{k}.hip", src)
+    fused = [c for c in classes if class_plan(c)[0] == "fused"]
+    split = [(c, len(class_plan(c)[1])) for c in classes if class_plan(c)[0] != "fused"]
+    hrr = [c for c, _ in split if c not in HRR_REG_EXCLUDE]
+    m = ["// GENERATED by gbasis_amd/codegen.py -- do not edit.",
+         f"// Dispatch over the {N_PARTS} parts of the generated register kernels.",
+         '#include "gbx_reg.h"', "", "namespace gbx {", ""]
+    for k in range(N_PARTS):
+        m.append(f"int launch_reg_part{k}(const gbx_tables* t, int key, const RegArgs& a, "
+                 "int precision, hipStream_t s);")
+        m.append(f"int launch_split_part{k}(const gbx_tables* t, int key, int split, "
+                 "const RegArgs& a, int precision, hipStream_t s);")
+        m.append(f"int launch_hrr_part{k}(const gbx_tables* t, int key, const RegArgs& a, "
+                 "hipStream_t s);")
+    m.append("")
+    m.append("int launch_reg_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
+             "const RegArgs& a, int precision, hipStream_t s) {")
+    m.append("  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;")
+    m.append("  int rc;")
+    for k in range(N_PARTS):
+        m.append(f"  if ((rc = launch_reg_part{k}(t, key, a, precision, s)) != {NOT_HERE}) "
+                 "return rc;")
+    m.append('  return fail(GBX_ERR_UNSUPPORTED, "no fused register kernel for this class");')
+    m.append("}")
+    m.append("")
+    m.append("int launch_split_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
+             "int k, const RegArgs& a, int precision, hipStream_t s) {")
+    m.append("  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;")
+    m.append("  int rc;")
+    for k in range(N_PARTS):
+        m.append(f"  if ((rc = launch_split_part{k}(t, key, k, a, precision, s)) != {NOT_HERE}) "
+                 "return rc;")
+    m.append('  return fail(GBX_ERR_UNSUPPORTED, "no split register kernel for this class");')
+    m.append("}")
+    m.append("")
+    m.append("int launch_hrr_generated(const gbx_tables* t, int la, int lb, int lc, int ld, "
+             "const RegArgs& a, hipStream_t s) {")
+    m.append("  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;")
+    m.append("  int rc;")
+    for k in range(N_PARTS):
+        m.append(f"  if ((rc = launch_hrr_part{k}(t, key, a, s)) != {NOT_HERE}) return rc;")
+    m.append('  return fail(GBX_ERR_UNSUPPORTED, "no register HRR kernel for this class");')
+    m.append("}")
+    m.append("")
+    for fname, members in (("has_hrr_generated", hrr), ("has_reg_generated", fused)):
+        m.append(f"bool {fname}(int la, int lb, int lc, int ld) {{")
+        m.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
+        for cls in members:
+            m.append(f"    case {_key(cls)}:")
+        m.append("      return true;")
+        m.append("    default:")
+        m.append("      return false;")
+        m.append("  }")
+        m.append("}")
+        m.append("")
+    m.append("int split_count_generated(int la, int lb, int lc, int ld) {")
+    m.append("  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {")
+    for cls, n in split:
+        m.append(f"    case {_key(cls)}:")
+        m.append(f"      return {n};")
+    m.append("    default:")
+    m.append("      return 0;")
+    m.append("  }")
+    m.append("}")
+    m.append("")
+    m.append("}  // namespace gbx")
+    _write(path, "\n".join(m) + "\n")
     return classes
 
 

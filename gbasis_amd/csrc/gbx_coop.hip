@@ -414,7 +414,11 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
     }
 
     const uint64_t obase = t.pair_out_base[bra];
-    const bool bra_diag = pb.x == pb.y, ket_diag = pk.x == pk.y, same_pair = bra == ket;
+    // bra and ket entries on the same two shells (a shell pair's (ij|ij) quartet or one
+    // of its sub-quartets): only the ij >= kl half is written
+    const bool bra_diag = pb.x == pb.y, ket_diag = pk.x == pk.y;
+    const bool same_pair = (pb.x == pk.x && pb.y == pk.y) || (pb.x == pk.y && pb.y == pk.x);
+    const bool add = a.mode == GBX_MODE_PACKED_ADD;
     const int total = dim[0] * dim[1] * dim[2] * dim[3];
     for (int e = tid; e < total; e += NT) {
       int idx[4];
@@ -437,7 +441,10 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
       if (ket_diag && fn[2] < fn[3]) continue;
       const uint64_t ij = tri(fn[0], fn[1]), kl = tri(fn[2], fn[3]);
       if (same_pair && ij < kl) continue;
-      a.out[obase + tri(ij, kl)] = v;
+      if (add)
+        a.out[obase + tri(ij, kl)] += v;
+      else
+        a.out[obase + tri(ij, kl)] = v;
     }
     }  // !kRegAcc
   }
@@ -617,6 +624,13 @@ int launch_coop(const gbx_tables* t, const gbx_program* prog, int la, int lb, in
   }
 }
 
+// Angular momenta of a class in the entries' own order (either shell of a pair may
+// carry the larger l: sub-pairs built on their second centre are reversed).
+inline bool class_in_range(int la, int lb, int lc, int ld) {
+  return la >= 0 && lb >= 0 && lc >= 0 && ld >= 0 && la <= kMaxShellL && lb <= kMaxShellL &&
+         lc <= kMaxShellL && ld <= kMaxShellL;
+}
+
 // The operator of a launch: Coulomb always; range-separated operators in FP64
 // packed launches with omega > 0.
 inline int check_potential(const gbx_tables* t, int mode, int precision, const char* who) {
@@ -658,12 +672,12 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
     return fail(GBX_ERR_ARG, "gbx_eri_coop: null argument");
   if (prim_copies < 1 || prim_copies > kCoopThreads || (prim_copies & (prim_copies - 1)))
     return fail(GBX_ERR_ARG, "gbx_eri_coop: prim_copies must be a power of two in [1, 64]");
-  if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
-    return fail(GBX_ERR_ARG, "gbx_eri_coop: class not in canonical orientation");
+  if (!class_in_range(la, lb, lc, ld))
+    return fail(GBX_ERR_ARG, "gbx_eri_coop: angular momentum out of range");
   if (prog->header[6] != la + lb + lc + ld ||
       prog->header[6] + kTaylorTerms > t->boys_nord - 1)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: program / Boys table do not match the class");
-  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_PACKED_ADD && mode != GBX_MODE_SCHWARZ)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: bad mode");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
     return fail(GBX_ERR_ARG, "gbx_eri_coop: bad precision");
@@ -691,10 +705,9 @@ extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t
                            int64_t scratch_bytes, void* stream) {
   if (!t || !prog || !d_out || !d_acc || (nquart > 0 && !d_quartets))
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: null argument");
-  if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL ||
-      prog->header[6] != la + lb + lc + ld)
+  if (!class_in_range(la, lb, lc, ld) || prog->header[6] != la + lb + lc + ld)
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: program does not match the class");
-  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_PACKED_ADD && mode != GBX_MODE_SCHWARZ)
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: bad mode");
   if (acc_stride < nquart || acc_parts < 1)
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: acc_stride < nquart or acc_parts < 1");
@@ -723,8 +736,8 @@ extern "C" int gbx_eri_coop_acc(const gbx_tables* t, const gbx_program* prog, in
   if (parts < 1 || parts > 4096)
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: parts must be in [1, 4096]");
   if (acc_stride < nquart) return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: acc_stride < nquart");
-  if (la < lb || lc < ld || la > kMaxShellL || lc > kMaxShellL)
-    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: class not in canonical orientation");
+  if (!class_in_range(la, lb, lc, ld))
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: angular momentum out of range");
   if (prog->header[6] != la + lb + lc + ld ||
       prog->header[6] + kTaylorTerms > t->boys_nord - 1)
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: program / Boys table do not match the class");

@@ -5,20 +5,25 @@
 // the enumeration of gbasis/base_four_symm.py:189-194.  A candidate is kept
 // iff Q_p * Q_q >= threshold (gbasis/integrals/_schwarz_screening.py:184-191;
 // threshold <= 0 keeps everything), oriented so the "heavier" pair is the
-// bra, marked FP32-bulk iff P_p * P_q < fp32_cutoff (P: bound of the contracted
-// [e0|f0], the input of the horizontal recursion), and assigned to a bucket
-//   key = (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + bin,
+// bra, and marked FP32-bulk iff P_p * P_q < fp32_cutoff (P: bound of the
+// contracted [e0|f0], the input of the horizontal recursion).  A kept quartet
+// becomes up to four sub-quartets, one per phase, from the pairs' sub-pair
+// entries pair_sub[p] = (first-centre part A_p, second-centre part B_p or -1):
+//   phase 0 (A_p|A_q), always; phase 1 (B_p|A_q); phase 2 (A_p|B_q); phase 3 (B_p|B_q)
+// (the primitive pairs of a pair split by the centre they are built on; the
+// quartet's integrals are the sum of its sub-quartets').  Each goes to a bucket
+//   key = ((((phase*4096 + cls)*2 + dense)*2 + fp32)*16 + bin,
+//   cls = ((la*8+lb)*8+lc)*8+ld of the sub-quartet's own entries,
 //   bin = min(ceil(log2(nprim_bra*nprim_ket)), 15)
-// (the bin only groups quartets of similar cost; kernels read the true primitive
-// counts from the pair table, so the last bin holds every count above 2^14).
-// Pass 1 counts quartets per bucket (optionally also packed slots and
-// primitive quartets per bucket); pass 2 writes (bra, ket) pairs at
-// bucket_offset[key] + an atomically reserved position.  Both passes
-// aggregate atomics per wavefront and key (the count pass also sums the
-// packed-slot and primitive-quartet statistics across the lanes of a key).  The order of quartets inside a
-// bucket is not deterministic; the integral of every packed slot is, since a
-// slot is produced by exactly one quartet with arithmetic that does not depend
-// on its neighbours.
+// (the bin only groups sub-quartets of similar cost; kernels read the true
+// primitive counts from the pair table, so the last bin holds every count above
+// 2^14).  Pass 1 counts sub-quartets per bucket (optionally also packed slots,
+// counted once per quartet in phase 0, and primitive quartets per bucket); pass 2
+// writes (bra, ket) entries at bucket_offset[key] + an atomically reserved
+// position.  Both passes aggregate atomics per wavefront and key.  The order of
+// sub-quartets inside a bucket is not deterministic; the integral of every packed
+// slot is, since a slot gets exactly one sub-quartet per phase, with arithmetic
+// that does not depend on its neighbours, and the phases run in order.
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
@@ -34,18 +39,20 @@ struct QueueArgs {
   int32_t n_mol;
   const int64_t* mol_cand_off;  // n_mol+1 prefix sums of P_m (P_m + 1) / 2
   const int32_t* mol_pair_off;  // n_mol first global pair of each molecule
-  const int32_t* pair_key;      // per pair: la | lb << 3 | dense << 6 | nprim << 8
+  const int32_t* pair_key;      // per pair entry: la | lb << 3 | dense << 6 | nprim << 8
+  const int32_t* pair_sub;      // per pair: (first-centre entry, second-centre entry | -1)
   const int32_t* pair_npf;      // per pair: number of unique function pairs
   const double* pair_bound;     // per pair: Schwarz bound Q of its original shell pair
   const double* pair_pbound;    // per pair: bound of the contracted [e0|f0] (precision choice)
   double threshold, fp32_cutoff;
   const double* mol_fp32_cutoff;  // per molecule (overrides fp32_cutoff); null: fp32_cutoff
   const int8_t* fp32_ok;        // per (class * 2 + dense): FP32 bulk allowed; null = all
-  int32_t* bucket_count;        // [kNumBuckets]
-  unsigned long long* bucket_eris;    // [kNumBuckets] or null
-  unsigned long long* bucket_primq;   // [kNumBuckets] or null
-  const int64_t* bucket_offset;       // pass 2: first slot of each bucket in out
-  int32_t* bucket_cursor;             // pass 2: [kNumBuckets], zeroed
+  int32_t* bucket_count;        // [GBX_NUM_BUCKETS]
+  unsigned long long* bucket_eris;    // [GBX_NUM_BUCKETS] or null
+  unsigned long long* bucket_primq;   // [GBX_NUM_BUCKETS] or null
+  const int64_t* bucket_offset;       // pass 2: first slot of each bucket in out, and
+                                      //   the end of the last ([GBX_NUM_BUCKETS + 1])
+  int32_t* bucket_cursor;             // pass 2: [GBX_NUM_BUCKETS], zeroed
   int32_t* out;                       // pass 2: (bra, ket) pairs
   int64_t n_cand;
 };
@@ -71,9 +78,9 @@ template <bool kFill>
 __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs a) {
   const int64_t t = static_cast<int64_t>(blockIdx.x) * kQueueThreads + threadIdx.x;
   bool valid = t < a.n_cand;
-  int key = -1;
-  int bra = 0, ket = 0;
-  uint64_t eris = 0, primq = 0;
+  int bra = 0, ket = 0, dense = 0, fp32 = 0;
+  int sb[2] = {-1, -1}, sk[2] = {-1, -1};
+  uint64_t eris = 0;
   if (valid) {
     // molecule by binary search over the candidate prefix sums
     int lo = 0, hi = a.n_mol - 1;
@@ -95,58 +102,78 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       bra = swap ? gq : gp;
       ket = swap ? gp : gq;
       const int kb = swap ? kq : kp, kk = swap ? kp : kq;
-      const int dense = ((kb >> 6) & 1) | ((kk >> 6) & 1);
+      dense = ((kb >> 6) & 1) | ((kk >> 6) & 1);
       const double pbound = a.pair_pbound[gp] * a.pair_pbound[gq];
-      const int np = (kb >> 8) * (kk >> 8);
       const int cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7);
       const double cut = a.mol_fp32_cutoff ? a.mol_fp32_cutoff[lo] : a.fp32_cutoff;
-      const int fp32 = (cut > 0.0 && pbound < cut &&
-                        (a.fp32_ok == nullptr || a.fp32_ok[cls * 2 + dense])) ? 1 : 0;
-      key = ((cls * 2 + dense) * 2 + fp32) * 16 + prim_bin(np);
+      fp32 = (cut > 0.0 && pbound < cut &&
+              (a.fp32_ok == nullptr || a.fp32_ok[cls * 2 + dense])) ? 1 : 0;
+      sb[0] = a.pair_sub[2 * bra];
+      sb[1] = a.pair_sub[2 * bra + 1];
+      sk[0] = a.pair_sub[2 * ket];
+      sk[1] = a.pair_sub[2 * ket + 1];
       if (!kFill && a.bucket_eris) {
         const uint64_t nb = static_cast<uint64_t>(a.pair_npf[bra]);
         const uint64_t nk = static_cast<uint64_t>(a.pair_npf[ket]);
         eris = bra == ket ? nb * (nb + 1) / 2 : nb * nk;
-        primq = static_cast<uint64_t>(np);
       }
     }
   }
-  // wavefront-aggregated reservation per distinct key
   const int lane = threadIdx.x & 63;
-  uint64_t pending = __ballot(valid);
-  while (pending) {
-    const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
-    const int lkey = __shfl(key, leader, 64);
-    const bool mine = valid && key == lkey;
-    const uint64_t same = __ballot(mine);
-    const int n = __popcll(same);
-    const int rank = __popcll(same & ((uint64_t(1) << lane) - 1));
-    if (kFill) {
-      int base = 0;
-      if (lane == leader) base = atomicAdd(a.bucket_cursor + lkey, n);
-      base = __shfl(base, leader, 64);
-      if (mine) {
-        const int64_t pos = a.bucket_offset[lkey] + base + rank;
-        a.out[2 * pos] = bra;
-        a.out[2 * pos + 1] = ket;
-      }
-    } else {
-      if (lane == leader) atomicAdd(a.bucket_count + lkey, n);
-      if (a.bucket_eris) {
-        // sums over the lanes of this key, one atomic per key and wavefront
-        unsigned long long e = mine ? static_cast<unsigned long long>(eris) : 0ull;
-        unsigned long long pq = mine ? static_cast<unsigned long long>(primq) : 0ull;
-        for (int off = 32; off > 0; off >>= 1) {
-          e += __shfl_xor(e, off, 64);
-          pq += __shfl_xor(pq, off, 64);
-        }
-        if (lane == leader) {
-          atomicAdd(a.bucket_eris + lkey, e);
-          atomicAdd(a.bucket_primq + lkey, pq);
-        }
-      }
+  for (int phase = 0; phase < 4; ++phase) {
+    const int eb = sb[phase & 1], ek = sk[phase >> 1];
+    const bool emit = valid && eb >= 0 && ek >= 0;
+    int key = -1;
+    uint64_t primq = 0;
+    if (emit) {
+      const int kb = a.pair_key[eb], kk = a.pair_key[ek];
+      const int cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7);
+      const int np = (kb >> 8) * (kk >> 8);
+      key = (((phase * 4096 + cls) * 2 + dense) * 2 + fp32) * 16 + prim_bin(np);
+      primq = static_cast<uint64_t>(np);
     }
-    pending &= ~same;
+    // wavefront-aggregated reservation per distinct key
+    uint64_t pending = __ballot(emit);
+    while (pending) {
+      const int leader = __ffsll(static_cast<unsigned long long>(pending)) - 1;
+      const int lkey = __shfl(key, leader, 64);
+      const bool mine = emit && key == lkey;
+      const uint64_t same = __ballot(mine);
+      const int n = __popcll(same);
+      const int rank = __popcll(same & ((uint64_t(1) << lane) - 1));
+      if (kFill) {
+        int base = 0;
+        if (lane == leader) base = atomicAdd(a.bucket_cursor + lkey, n);
+        base = __shfl(base, leader, 64);
+        if (mine) {
+          // a bucket sized too small never writes into its neighbour: its cursor then
+          // exceeds its capacity, which the host can detect
+          const int64_t pos = a.bucket_offset[lkey] + base + rank;
+          if (pos < a.bucket_offset[lkey + 1]) {
+            a.out[2 * pos] = eb;
+            a.out[2 * pos + 1] = ek;
+          }
+        }
+      } else {
+        if (lane == leader) atomicAdd(a.bucket_count + lkey, n);
+        if (a.bucket_eris) {
+          // sums over the lanes of this key, one atomic per key and wavefront; the
+          // packed slots of a quartet count once, in phase 0
+          unsigned long long e = (mine && phase == 0) ? static_cast<unsigned long long>(eris)
+                                                       : 0ull;
+          unsigned long long pq = mine ? static_cast<unsigned long long>(primq) : 0ull;
+          for (int off = 32; off > 0; off >>= 1) {
+            e += __shfl_xor(e, off, 64);
+            pq += __shfl_xor(pq, off, 64);
+          }
+          if (lane == leader) {
+            atomicAdd(a.bucket_eris + lkey, e);
+            atomicAdd(a.bucket_primq + lkey, pq);
+          }
+        }
+      }
+      pending &= ~same;
+    }
   }
 }
 
@@ -167,14 +194,15 @@ int launch_queue(const QueueArgs& a, bool fill, hipStream_t s) {
 
 extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
                                   const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
-                                  const int32_t* d_pair_npf, const double* d_pair_bound,
-                                  const double* d_pair_pbound, double threshold,
+                                  const int32_t* d_pair_sub, const int32_t* d_pair_npf,
+                                  const double* d_pair_bound, const double* d_pair_pbound,
+                                  double threshold,
                                   double fp32_cutoff, const double* d_mol_fp32_cutoff,
                                   const int8_t* d_fp32_ok, int64_t n_cand,
                                   int32_t* d_bucket_count, uint64_t* d_bucket_eris,
                                   uint64_t* d_bucket_primq, void* stream) {
-  if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
-      !d_pair_pbound || !d_bucket_count || n_cand < 0 ||
+  if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_sub ||
+      !d_pair_bound || !d_pair_pbound || !d_bucket_count || n_cand < 0 ||
       (d_bucket_eris && (!d_bucket_primq || !d_pair_npf)))
     return gbx::fail(GBX_ERR_ARG, "gbx_quartets_count: bad arguments");
   gbx::QueueArgs a{};
@@ -182,6 +210,7 @@ extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
   a.mol_cand_off = d_mol_cand_off;
   a.mol_pair_off = d_mol_pair_off;
   a.pair_key = d_pair_key;
+  a.pair_sub = d_pair_sub;
   a.pair_npf = d_pair_npf;
   a.pair_bound = d_pair_bound;
   a.pair_pbound = d_pair_pbound;
@@ -198,19 +227,22 @@ extern "C" int gbx_quartets_count(int32_t n_mol, const int64_t* d_mol_cand_off,
 
 extern "C" int gbx_quartets_fill(int32_t n_mol, const int64_t* d_mol_cand_off,
                                  const int32_t* d_mol_pair_off, const int32_t* d_pair_key,
-                                 const double* d_pair_bound, const double* d_pair_pbound,
-                                 double threshold, double fp32_cutoff,
+                                 const int32_t* d_pair_sub, const double* d_pair_bound,
+                                 const double* d_pair_pbound, double threshold,
+                                 double fp32_cutoff,
                                  const double* d_mol_fp32_cutoff, const int8_t* d_fp32_ok,
                                  int64_t n_cand, const int64_t* d_bucket_offset,
                                  int32_t* d_bucket_cursor, int32_t* d_quartets, void* stream) {
-  if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_bound ||
-      !d_pair_pbound || !d_bucket_offset || !d_bucket_cursor || !d_quartets || n_cand < 0)
+  if (n_mol < 1 || !d_mol_cand_off || !d_mol_pair_off || !d_pair_key || !d_pair_sub ||
+      !d_pair_bound || !d_pair_pbound || !d_bucket_offset || !d_bucket_cursor || !d_quartets ||
+      n_cand < 0)
     return gbx::fail(GBX_ERR_ARG, "gbx_quartets_fill: bad arguments");
   gbx::QueueArgs a{};
   a.n_mol = n_mol;
   a.mol_cand_off = d_mol_cand_off;
   a.mol_pair_off = d_mol_pair_off;
   a.pair_key = d_pair_key;
+  a.pair_sub = d_pair_sub;
   a.pair_bound = d_pair_bound;
   a.pair_pbound = d_pair_pbound;
   a.threshold = threshold;

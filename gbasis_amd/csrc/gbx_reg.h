@@ -16,14 +16,16 @@ constexpr int kRegThreads = 256;
 struct RegArgs {
   const int32_t* quart;  // (bra pair, ket pair) per quartet
   int64_t nquart;
-  int32_t mode;          // GBX_MODE_PACKED or GBX_MODE_SCHWARZ
+  int32_t mode;          // GBX_MODE_PACKED, GBX_MODE_PACKED_ADD or GBX_MODE_SCHWARZ
   double* out;
   double* acc;           // split kernels: partial contracted [e0|f0],
                          //   acc[(f * lanes_per_quartet + r) * acc_stride + q]
   int64_t acc_stride;
   int32_t lanes_per_quartet;  // split kernels: lanes sharing one shell quartet (R)
   const int64_t* seg_off;     // fused kernels, segmented launch: first quartet of each of
-  const int32_t* seg_cnt;     //   kSegments segments and its count (device); null: plain
+                              //   kSegments segments, and the end of the last (capacity of
+                              //   segment s: seg_off[s + 1] - seg_off[s])
+  const int32_t* seg_cnt;     //   and its count (device); null: plain launch
   const int32_t* count;       // plain launches: device quartet count (<= nquart, the
                               //   capacity the grid is sized for); null: nquart
 };
@@ -32,7 +34,7 @@ constexpr int kSegments = 16;
 
 // Fused kernels: quartet index of this lane, or -1.  A segmented launch covers the
 // primitive-count bins of one class in one grid: lane gid takes quartet
-// seg_off[s] + (gid - sum_{s' < s} seg_cnt[s']).
+// seg_off[s] + (gid - sum_{s' < s} n[s']), n[s] = min(seg_cnt[s], capacity of s).
 __device__ inline int64_t lane_quartet(const RegArgs& a, int64_t gid) {
   if (a.seg_cnt == nullptr) {
     const int64_t n = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
@@ -41,7 +43,7 @@ __device__ inline int64_t lane_quartet(const RegArgs& a, int64_t gid) {
   int64_t base = 0;
 #pragma unroll
   for (int s = 0; s < kSegments; ++s) {
-    const int64_t c = a.seg_cnt[s];
+    const int64_t c = min(static_cast<int64_t>(a.seg_cnt[s]), a.seg_off[s + 1] - a.seg_off[s]);
     if (gid < base + c) return a.seg_off[s] + (gid - base);
     base += c;
   }
@@ -130,10 +132,24 @@ __device__ inline uint64_t tri_u64(uint64_t i, uint64_t j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
 
+// Bra and ket entries cover the same two shells (the quartet (ij|ij) of a shell pair, or
+// one of its sub-quartets): the packed placement then writes only the ij >= kl half.
+__device__ inline bool same_shells(int4 pb, int4 pk) {
+  return (pb.x == pk.x && pb.y == pk.y) || (pb.x == pk.y && pb.y == pk.x);
+}
+
+// Store or (GBX_MODE_PACKED_ADD) add one packed value.
+__device__ __forceinline__ void put_packed(double* p, double v, bool add) {
+  if (add)
+    *p += v;
+  else
+    *p = v;
+}
+
 // Write one canonical-order Cartesian block (NA x NB x NC x ND, factorial2 norms
 // applied) through the per-shell maps (component position, norm_cont / sign scale)
-// into the canonical packed slots, skipping the duplicates of diagonal pairs and
-// of bra == ket.  In GBX_MODE_SCHWARZ write, for the bra pair, sqrt(max |block|)
+// into the canonical packed slots (stored, or added in GBX_MODE_PACKED_ADD),
+// skipping the duplicates of diagonal pairs and of bra and ket on the same shells.  In GBX_MODE_SCHWARZ write, for the bra pair, sqrt(max |block|)
 // to out[2 bra] and sqrt(max |contracted [e0|f0]|) (accmax) to out[2 bra + 1].
 template <int NA, int NB, int NC, int ND>
 __device__ inline void store_block(const gbx_tables& t, const RegArgs& a, int bra, int ket,
@@ -171,7 +187,8 @@ __device__ inline void store_block(const gbx_tables& t, const RegArgs& a, int br
     fd[i] = static_cast<uint32_t>(sd.x + t.comp_map[sd.w + i]);
     cd[i] = t.comp_scale[sd.w + i];
   }
-  const bool bdiag = pb.x == pb.y, kdiag = pk.x == pk.y, same = bra == ket;
+  const bool bdiag = pb.x == pb.y, kdiag = pk.x == pk.y, same = same_shells(pb, pk);
+  const bool add = a.mode == GBX_MODE_PACKED_ADD;
   double* base = a.out + t.pair_out_base[bra];
 #pragma unroll
   for (int ia = 0; ia < NA; ++ia)
@@ -186,8 +203,9 @@ __device__ inline void store_block(const gbx_tables& t, const RegArgs& a, int br
           if (kdiag && fc[ic] < fd[id]) continue;
           const uint64_t kl = tri_u64(fc[ic], fd[id]);
           if (same && ij < kl) continue;
-          base[tri_u64(ij, kl)] =
-              out[((ia * NB + ib) * NC + ic) * ND + id] * ca[ia] * cb[ib] * cc[ic] * cd[id];
+          put_packed(base + tri_u64(ij, kl),
+                     out[((ia * NB + ib) * NC + ic) * ND + id] * ca[ia] * cb[ib] * cc[ic] * cd[id],
+                     add);
         }
     }
 }
@@ -217,7 +235,7 @@ struct ColumnStore {
   const double* cscale;  // argument would force a private copy of it)
   double* base;
   int4 sc, sd;
-  bool bdiag, kdiag, same;
+  bool bdiag, kdiag, same, add;
   uint32_t fa[NA], fb[NB];
   double ca[NA], cb[NB];
 
@@ -240,7 +258,8 @@ struct ColumnStore {
     }
     bdiag = pb.x == pb.y;
     kdiag = pk.x == pk.y;
-    same = bra == ket;
+    same = same_shells(pb, pk);
+    add = a.mode == GBX_MODE_PACKED_ADD;
     base = a.out + t.pair_out_base[bra];
   }
 
@@ -255,7 +274,7 @@ struct ColumnStore {
     if (cl.skip || (bdiag && fa[ia] < fb[ib])) return;
     const uint64_t ij = tri_u64(fa[ia], fb[ib]);
     if (same && ij < cl.kl) return;
-    base[tri_u64(ij, cl.kl)] = v * ca[ia] * cb[ib] * cl.cc * cl.cd;
+    put_packed(base + tri_u64(ij, cl.kl), v * ca[ia] * cb[ib] * cl.cc * cl.cd, add);
   }
 };
 

@@ -14,11 +14,11 @@ extern "C" int gbx_eri_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t 
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: null argument");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: bad precision");
-  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_PACKED_ADD && mode != GBX_MODE_SCHWARZ)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg: no register kernel for this class");
-  if (t->potential != GBX_POT_COULOMB && mode == GBX_MODE_PACKED)
+  if (t->potential != GBX_POT_COULOMB && mode != GBX_MODE_SCHWARZ)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg: register kernels evaluate 1/r only");
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1 || t->boys_nt != gbx::kBoysRows)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg: Boys table does not fit this class / kernel");
@@ -37,11 +37,11 @@ extern "C" int gbx_eri_reg_segments(const gbx_tables* t, int32_t la, int32_t lb,
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad argument");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad precision");
-  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_SCHWARZ)
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_PACKED_ADD && mode != GBX_MODE_SCHWARZ)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_reg_segments: bad mode");
   if (gbx_has_reg_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_reg_segments: no register kernel for class");
-  if (t->potential != GBX_POT_COULOMB && mode == GBX_MODE_PACKED)
+  if (t->potential != GBX_POT_COULOMB && mode != GBX_MODE_SCHWARZ)
     return gbx::fail(GBX_ERR_UNSUPPORTED,
                      "gbx_eri_reg_segments: register kernels evaluate 1/r only");
   if (la + lb + lc + ld + gbx::kTaylorTerms > t->boys_nord - 1 || t->boys_nt != gbx::kBoysRows)
@@ -92,15 +92,17 @@ extern "C" int gbx_has_hrr_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld
 extern "C" int gbx_eri_hrr_reg(const gbx_tables* t, int32_t la, int32_t lb, int32_t lc,
                                int32_t ld, const int32_t* d_quartets, int64_t nquart,
                                const int32_t* d_count, const double* d_acc, int64_t acc_stride,
-                               int32_t acc_parts, double* d_out, void* stream) {
+                               int32_t acc_parts, int32_t mode, double* d_out, void* stream) {
   if (!t || !d_out || !d_acc || (nquart > 0 && !d_quartets))
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_hrr_reg: null argument");
   if (gbx_has_hrr_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_hrr_reg: no register HRR kernel for class");
   if (acc_stride < nquart || acc_parts < 1 || acc_parts > 4096)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_hrr_reg: acc_stride < nquart or bad acc_parts");
+  if (mode != GBX_MODE_PACKED && mode != GBX_MODE_PACKED_ADD)
+    return gbx::fail(GBX_ERR_ARG, "gbx_eri_hrr_reg: bad mode");
   if (nquart == 0) return GBX_OK;
-  gbx::RegArgs a{d_quartets, nquart, GBX_MODE_PACKED, d_out, const_cast<double*>(d_acc),
+  gbx::RegArgs a{d_quartets, nquart, mode, d_out, const_cast<double*>(d_acc),
                  acc_stride, acc_parts, nullptr, nullptr, d_count};
   return gbx::launch_hrr_generated(t, la, lb, lc, ld, a, static_cast<hipStream_t>(stream));
 }

@@ -1,30573 +1,74 @@
 // GENERATED by gbasis_amd/codegen.py -- do not edit.
-// Per-class register kernels: one lane = one shell quartet.
+// Dispatch over the 8 parts of the generated register kernels.
 #include "gbx_reg.h"
 
 namespace gbx {
 
-// class (00|00): 0 recursion values, 1 accumulators, 1 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_0000(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<0>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[1];
-      boys_eval_lds<0>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      acc0 += static_cast<double>(x0);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  double out[1];
-  out[0] = h0;
-  store_block<1, 1, 1, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (10|00): 3 recursion values, 3 accumulators, 3 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_1000(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<1>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[2];
-      boys_eval_lds<1>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = s.PA[0] * x0 + s.WP[0] * x1;
-      acc0 += static_cast<double>(x2);
-      const TB x3 = s.PA[1] * x0 + s.WP[1] * x1;
-      acc1 += static_cast<double>(x3);
-      const TB x4 = s.PA[2] * x0 + s.WP[2] * x1;
-      acc2 += static_cast<double>(x4);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  double out[3];
-  out[0] = h0;
-  out[1] = h1;
-  out[2] = h2;
-  store_block<3, 1, 1, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (10|10): 15 recursion values, 9 accumulators, 9 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_1010(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<2>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[3];
-      boys_eval_lds<2>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB x3 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x4 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x5 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x6 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.QC[0] * x3 + s.WQ[0] * x6 + r1 * x1;
-      acc0 += static_cast<double>(x9);
-      const TB x12 = s.QC[1] * x3 + s.WQ[1] * x6;
-      acc3 += static_cast<double>(x12);
-      const TB x15 = s.QC[2] * x3 + s.WQ[2] * x6;
-      acc6 += static_cast<double>(x15);
-      const TB x7 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x8 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x10 = s.QC[0] * x4 + s.WQ[0] * x7;
-      acc1 += static_cast<double>(x10);
-      const TB x11 = s.QC[0] * x5 + s.WQ[0] * x8;
-      acc2 += static_cast<double>(x11);
-      const TB x13 = s.QC[1] * x4 + s.WQ[1] * x7 + r1 * x1;
-      acc4 += static_cast<double>(x13);
-      const TB x16 = s.QC[2] * x4 + s.WQ[2] * x7;
-      acc7 += static_cast<double>(x16);
-      const TB x17 = s.QC[2] * x5 + s.WQ[2] * x8 + r1 * x1;
-      acc8 += static_cast<double>(x17);
-      const TB x14 = s.QC[1] * x5 + s.WQ[1] * x8;
-      acc5 += static_cast<double>(x14);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  double out[9];
-  out[0] = h0;
-  out[1] = h3;
-  out[2] = h6;
-  out[3] = h1;
-  out[4] = h4;
-  out[5] = h7;
-  out[6] = h2;
-  out[7] = h5;
-  out[8] = h8;
-  store_block<3, 1, 3, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (11|00): 12 recursion values, 9 accumulators, 9 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_1100(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<2>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[3];
-      boys_eval_lds<2>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB x3 = s.PA[0] * x0 + s.WP[0] * x1;
-      acc0 += static_cast<double>(x3);
-      const TB x4 = s.PA[1] * x0 + s.WP[1] * x1;
-      acc1 += static_cast<double>(x4);
-      const TB x5 = s.PA[2] * x0 + s.WP[2] * x1;
-      acc2 += static_cast<double>(x5);
-      const TB x6 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.PA[0] * x3 + s.WP[0] * x6 + z1 * (x0 - s.roz * x1);
-      acc3 += static_cast<double>(x9);
-      const TB x10 = s.PA[1] * x3 + s.WP[1] * x6;
-      acc4 += static_cast<double>(x10);
-      const TB x11 = s.PA[2] * x3 + s.WP[2] * x6;
-      acc5 += static_cast<double>(x11);
-      const TB x7 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x8 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x14 = s.PA[2] * x5 + s.WP[2] * x8 + z1 * (x0 - s.roz * x1);
-      acc8 += static_cast<double>(x14);
-      const TB x12 = s.PA[1] * x4 + s.WP[1] * x7 + z1 * (x0 - s.roz * x1);
-      acc6 += static_cast<double>(x12);
-      const TB x13 = s.PA[2] * x4 + s.WP[2] * x7;
-      acc7 += static_cast<double>(x13);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = h8 + AB[2] * h2;
-  const double h10 = h7 + AB[1] * h2;
-  const double h11 = h5 + AB[0] * h2;
-  const double h12 = h7 + AB[2] * h1;
-  const double h13 = h6 + AB[1] * h1;
-  const double h14 = h4 + AB[0] * h1;
-  const double h15 = h5 + AB[2] * h0;
-  const double h16 = h4 + AB[1] * h0;
-  const double h17 = h3 + AB[0] * h0;
-  double out[9];
-  out[0] = h17;
-  out[1] = h16;
-  out[2] = h15;
-  out[3] = h14;
-  out[4] = h13;
-  out[5] = h12;
-  out[6] = h11;
-  out[7] = h10;
-  out[8] = h9;
-  store_block<3, 3, 1, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (11|10): 48 recursion values, 27 accumulators, 27 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_1110(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<3>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[4];
-      boys_eval_lds<3>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x4 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x5 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x6 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x7 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x25 = s.QC[0] * x4 + s.WQ[0] * x7 + r1 * x1;
-      acc0 += static_cast<double>(x25);
-      const TB x34 = s.QC[1] * x4 + s.WQ[1] * x7;
-      acc9 += static_cast<double>(x34);
-      const TB x43 = s.QC[2] * x4 + s.WQ[2] * x7;
-      acc18 += static_cast<double>(x43);
-      const TB x8 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x26 = s.QC[0] * x5 + s.WQ[0] * x8;
-      acc1 += static_cast<double>(x26);
-      const TB x35 = s.QC[1] * x5 + s.WQ[1] * x8 + r1 * x1;
-      acc10 += static_cast<double>(x35);
-      const TB x44 = s.QC[2] * x5 + s.WQ[2] * x8;
-      acc19 += static_cast<double>(x44);
-      const TB x9 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x27 = s.QC[0] * x6 + s.WQ[0] * x9;
-      acc2 += static_cast<double>(x27);
-      const TB x36 = s.QC[1] * x6 + s.WQ[1] * x9;
-      acc11 += static_cast<double>(x36);
-      const TB x45 = s.QC[2] * x6 + s.WQ[2] * x9 + r1 * x1;
-      acc20 += static_cast<double>(x45);
-      const TB x18 = s.PA[2] * x6 + s.WP[2] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x10 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x11 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x12 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x24 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x33 = s.QC[0] * x18 + s.WQ[0] * x24;
-      acc8 += static_cast<double>(x33);
-      const TB x42 = s.QC[1] * x18 + s.WQ[1] * x24;
-      acc17 += static_cast<double>(x42);
-      const TB x51 = s.QC[2] * x18 + s.WQ[2] * x24 + r2 * x9;
-      acc26 += static_cast<double>(x51);
-      const TB x13 = s.PA[0] * x4 + s.WP[0] * x7 + z1 * (x0 - s.roz * x1);
-      const TB x16 = s.PA[1] * x5 + s.WP[1] * x8 + z1 * (x0 - s.roz * x1);
-      const TB x17 = s.PA[2] * x5 + s.WP[2] * x8;
-      const TB x14 = s.PA[1] * x4 + s.WP[1] * x7;
-      const TB x15 = s.PA[2] * x4 + s.WP[2] * x7;
-      const TB x19 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x1 - s.roz * x2);
-      const TB x22 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x23 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x28 = s.QC[0] * x13 + s.WQ[0] * x19 + r2 * x7;
-      acc3 += static_cast<double>(x28);
-      const TB x31 = s.QC[0] * x16 + s.WQ[0] * x22;
-      acc6 += static_cast<double>(x31);
-      const TB x32 = s.QC[0] * x17 + s.WQ[0] * x23;
-      acc7 += static_cast<double>(x32);
-      const TB x37 = s.QC[1] * x13 + s.WQ[1] * x19;
-      acc12 += static_cast<double>(x37);
-      const TB x46 = s.QC[2] * x13 + s.WQ[2] * x19;
-      acc21 += static_cast<double>(x46);
-      const TB x40 = s.QC[1] * x16 + s.WQ[1] * x22 + r2 * x8;
-      acc15 += static_cast<double>(x40);
-      const TB x49 = s.QC[2] * x16 + s.WQ[2] * x22;
-      acc24 += static_cast<double>(x49);
-      const TB x41 = s.QC[1] * x17 + s.WQ[1] * x23 + r1 * x9;
-      acc16 += static_cast<double>(x41);
-      const TB x50 = s.QC[2] * x17 + s.WQ[2] * x23 + r1 * x8;
-      acc25 += static_cast<double>(x50);
-      const TB x20 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x29 = s.QC[0] * x14 + s.WQ[0] * x20 + r1 * x8;
-      acc4 += static_cast<double>(x29);
-      const TB x21 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x30 = s.QC[0] * x15 + s.WQ[0] * x21 + r1 * x9;
-      acc5 += static_cast<double>(x30);
-      const TB x38 = s.QC[1] * x14 + s.WQ[1] * x20 + r1 * x7;
-      acc13 += static_cast<double>(x38);
-      const TB x47 = s.QC[2] * x14 + s.WQ[2] * x20;
-      acc22 += static_cast<double>(x47);
-      const TB x48 = s.QC[2] * x15 + s.WQ[2] * x21 + r1 * x7;
-      acc23 += static_cast<double>(x48);
-      const TB x39 = s.QC[1] * x15 + s.WQ[1] * x21;
-      acc14 += static_cast<double>(x39);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = h26 + AB[2] * h20;
-  const double h28 = h17 + AB[2] * h11;
-  const double h29 = h8 + AB[2] * h2;
-  const double h30 = h25 + AB[1] * h20;
-  const double h31 = h16 + AB[1] * h11;
-  const double h32 = h7 + AB[1] * h2;
-  const double h33 = h23 + AB[0] * h20;
-  const double h34 = h14 + AB[0] * h11;
-  const double h35 = h5 + AB[0] * h2;
-  const double h36 = h25 + AB[2] * h19;
-  const double h37 = h16 + AB[2] * h10;
-  const double h38 = h7 + AB[2] * h1;
-  const double h39 = h24 + AB[1] * h19;
-  const double h40 = h15 + AB[1] * h10;
-  const double h41 = h6 + AB[1] * h1;
-  const double h42 = h22 + AB[0] * h19;
-  const double h43 = h13 + AB[0] * h10;
-  const double h44 = h4 + AB[0] * h1;
-  const double h45 = h23 + AB[2] * h18;
-  const double h46 = h14 + AB[2] * h9;
-  const double h47 = h5 + AB[2] * h0;
-  const double h48 = h22 + AB[1] * h18;
-  const double h49 = h13 + AB[1] * h9;
-  const double h50 = h4 + AB[1] * h0;
-  const double h51 = h21 + AB[0] * h18;
-  const double h52 = h12 + AB[0] * h9;
-  const double h53 = h3 + AB[0] * h0;
-  double out[27];
-  out[0] = h53;
-  out[1] = h52;
-  out[2] = h51;
-  out[3] = h50;
-  out[4] = h49;
-  out[5] = h48;
-  out[6] = h47;
-  out[7] = h46;
-  out[8] = h45;
-  out[9] = h44;
-  out[10] = h43;
-  out[11] = h42;
-  out[12] = h41;
-  out[13] = h40;
-  out[14] = h39;
-  out[15] = h38;
-  out[16] = h37;
-  out[17] = h36;
-  out[18] = h35;
-  out[19] = h34;
-  out[20] = h33;
-  out[21] = h32;
-  out[22] = h31;
-  out[23] = h30;
-  out[24] = h29;
-  out[25] = h28;
-  out[26] = h27;
-  store_block<3, 3, 3, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (11|11): 141 recursion values, 81 accumulators, 81 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_1111(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<4>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  double acc31 = 0.0;
-  double acc32 = 0.0;
-  double acc33 = 0.0;
-  double acc34 = 0.0;
-  double acc35 = 0.0;
-  double acc36 = 0.0;
-  double acc37 = 0.0;
-  double acc38 = 0.0;
-  double acc39 = 0.0;
-  double acc40 = 0.0;
-  double acc41 = 0.0;
-  double acc42 = 0.0;
-  double acc43 = 0.0;
-  double acc44 = 0.0;
-  double acc45 = 0.0;
-  double acc46 = 0.0;
-  double acc47 = 0.0;
-  double acc48 = 0.0;
-  double acc49 = 0.0;
-  double acc50 = 0.0;
-  double acc51 = 0.0;
-  double acc52 = 0.0;
-  double acc53 = 0.0;
-  double acc54 = 0.0;
-  double acc55 = 0.0;
-  double acc56 = 0.0;
-  double acc57 = 0.0;
-  double acc58 = 0.0;
-  double acc59 = 0.0;
-  double acc60 = 0.0;
-  double acc61 = 0.0;
-  double acc62 = 0.0;
-  double acc63 = 0.0;
-  double acc64 = 0.0;
-  double acc65 = 0.0;
-  double acc66 = 0.0;
-  double acc67 = 0.0;
-  double acc68 = 0.0;
-  double acc69 = 0.0;
-  double acc70 = 0.0;
-  double acc71 = 0.0;
-  double acc72 = 0.0;
-  double acc73 = 0.0;
-  double acc74 = 0.0;
-  double acc75 = 0.0;
-  double acc76 = 0.0;
-  double acc77 = 0.0;
-  double acc78 = 0.0;
-  double acc79 = 0.0;
-  double acc80 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[5];
-      boys_eval_lds<4>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x5 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x6 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x7 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x8 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x10 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x11 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x12 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x13 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x14 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x15 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x16 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x34 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x17 = s.PA[0] * x5 + s.WP[0] * x8 + z1 * (x0 - s.roz * x1);
-      const TB x18 = s.PA[1] * x5 + s.WP[1] * x8;
-      const TB x19 = s.PA[2] * x5 + s.WP[2] * x8;
-      const TB x20 = s.PA[1] * x6 + s.WP[1] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x22 = s.PA[2] * x7 + s.WP[2] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x21 = s.PA[2] * x6 + s.WP[2] * x9;
-      const TB x23 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x24 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x25 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x26 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x27 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x28 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x29 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x2 - s.roz * x3);
-      const TB x32 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x33 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x30 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x31 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x35 = s.QC[0] * x1 + s.WQ[0] * x2;
-      const TB x36 = s.QC[0] * x5 + s.WQ[0] * x8 + r1 * x1;
-      acc0 += static_cast<double>(x36);
-      const TB x37 = s.QC[0] * x8 + s.WQ[0] * x11 + r1 * x2;
-      const TB x92 = s.QC[0] * x36 + s.WQ[0] * x37 + e1 * (x5 - s.roe * x8) + r1 * x35;
-      acc27 += static_cast<double>(x92);
-      const TB x101 = s.QC[1] * x36 + s.WQ[1] * x37;
-      acc36 += static_cast<double>(x101);
-      const TB x110 = s.QC[2] * x36 + s.WQ[2] * x37;
-      acc45 += static_cast<double>(x110);
-      const TB x38 = s.QC[0] * x6 + s.WQ[0] * x9;
-      acc1 += static_cast<double>(x38);
-      const TB x39 = s.QC[0] * x9 + s.WQ[0] * x12;
-      const TB x93 = s.QC[0] * x38 + s.WQ[0] * x39 + e1 * (x6 - s.roe * x9);
-      acc28 += static_cast<double>(x93);
-      const TB x102 = s.QC[1] * x38 + s.WQ[1] * x39 + r1 * x35;
-      acc37 += static_cast<double>(x102);
-      const TB x111 = s.QC[2] * x38 + s.WQ[2] * x39;
-      acc46 += static_cast<double>(x111);
-      const TB x40 = s.QC[0] * x7 + s.WQ[0] * x10;
-      acc2 += static_cast<double>(x40);
-      const TB x41 = s.QC[0] * x10 + s.WQ[0] * x13;
-      const TB x112 = s.QC[2] * x40 + s.WQ[2] * x41 + r1 * x35;
-      acc47 += static_cast<double>(x112);
-      const TB x94 = s.QC[0] * x40 + s.WQ[0] * x41 + e1 * (x7 - s.roe * x10);
-      acc29 += static_cast<double>(x94);
-      const TB x103 = s.QC[1] * x40 + s.WQ[1] * x41;
-      acc38 += static_cast<double>(x103);
-      const TB x42 = s.QC[0] * x17 + s.WQ[0] * x23 + r2 * x8;
-      acc3 += static_cast<double>(x42);
-      const TB x43 = s.QC[0] * x23 + s.WQ[0] * x29 + r2 * x11;
-      const TB x95 = s.QC[0] * x42 + s.WQ[0] * x43 + e1 * (x17 - s.roe * x23) + r2 * x37;
-      acc30 += static_cast<double>(x95);
-      const TB x104 = s.QC[1] * x42 + s.WQ[1] * x43;
-      acc39 += static_cast<double>(x104);
-      const TB x113 = s.QC[2] * x42 + s.WQ[2] * x43;
-      acc48 += static_cast<double>(x113);
-      const TB x44 = s.QC[0] * x18 + s.WQ[0] * x24 + r1 * x9;
-      acc4 += static_cast<double>(x44);
-      const TB x45 = s.QC[0] * x24 + s.WQ[0] * x30 + r1 * x12;
-      const TB x96 = s.QC[0] * x44 + s.WQ[0] * x45 + e1 * (x18 - s.roe * x24) + r1 * x39;
-      acc31 += static_cast<double>(x96);
-      const TB x105 = s.QC[1] * x44 + s.WQ[1] * x45 + r1 * x37;
-      acc40 += static_cast<double>(x105);
-      const TB x114 = s.QC[2] * x44 + s.WQ[2] * x45;
-      acc49 += static_cast<double>(x114);
-      const TB x46 = s.QC[0] * x19 + s.WQ[0] * x25 + r1 * x10;
-      acc5 += static_cast<double>(x46);
-      const TB x47 = s.QC[0] * x25 + s.WQ[0] * x31 + r1 * x13;
-      const TB x115 = s.QC[2] * x46 + s.WQ[2] * x47 + r1 * x37;
-      acc50 += static_cast<double>(x115);
-      const TB x97 = s.QC[0] * x46 + s.WQ[0] * x47 + e1 * (x19 - s.roe * x25) + r1 * x41;
-      acc32 += static_cast<double>(x97);
-      const TB x106 = s.QC[1] * x46 + s.WQ[1] * x47;
-      acc41 += static_cast<double>(x106);
-      const TB x48 = s.QC[0] * x20 + s.WQ[0] * x26;
-      acc6 += static_cast<double>(x48);
-      const TB x49 = s.QC[0] * x26 + s.WQ[0] * x32;
-      const TB x98 = s.QC[0] * x48 + s.WQ[0] * x49 + e1 * (x20 - s.roe * x26);
-      acc33 += static_cast<double>(x98);
-      const TB x107 = s.QC[1] * x48 + s.WQ[1] * x49 + r2 * x39;
-      acc42 += static_cast<double>(x107);
-      const TB x116 = s.QC[2] * x48 + s.WQ[2] * x49;
-      acc51 += static_cast<double>(x116);
-      const TB x50 = s.QC[0] * x21 + s.WQ[0] * x27;
-      acc7 += static_cast<double>(x50);
-      const TB x51 = s.QC[0] * x27 + s.WQ[0] * x33;
-      const TB x117 = s.QC[2] * x50 + s.WQ[2] * x51 + r1 * x39;
-      acc52 += static_cast<double>(x117);
-      const TB x99 = s.QC[0] * x50 + s.WQ[0] * x51 + e1 * (x21 - s.roe * x27);
-      acc34 += static_cast<double>(x99);
-      const TB x108 = s.QC[1] * x50 + s.WQ[1] * x51 + r1 * x41;
-      acc43 += static_cast<double>(x108);
-      const TB x52 = s.QC[0] * x22 + s.WQ[0] * x28;
-      acc8 += static_cast<double>(x52);
-      const TB x53 = s.QC[0] * x28 + s.WQ[0] * x34;
-      const TB x118 = s.QC[2] * x52 + s.WQ[2] * x53 + r2 * x41;
-      acc53 += static_cast<double>(x118);
-      const TB x100 = s.QC[0] * x52 + s.WQ[0] * x53 + e1 * (x22 - s.roe * x28);
-      acc35 += static_cast<double>(x100);
-      const TB x109 = s.QC[1] * x52 + s.WQ[1] * x53;
-      acc44 += static_cast<double>(x109);
-      const TB x54 = s.QC[1] * x1 + s.WQ[1] * x2;
-      const TB x55 = s.QC[1] * x5 + s.WQ[1] * x8;
-      acc9 += static_cast<double>(x55);
-      const TB x56 = s.QC[1] * x8 + s.WQ[1] * x11;
-      const TB x119 = s.QC[1] * x55 + s.WQ[1] * x56 + e1 * (x5 - s.roe * x8);
-      acc54 += static_cast<double>(x119);
-      const TB x128 = s.QC[2] * x55 + s.WQ[2] * x56;
-      acc63 += static_cast<double>(x128);
-      const TB x57 = s.QC[1] * x6 + s.WQ[1] * x9 + r1 * x1;
-      acc10 += static_cast<double>(x57);
-      const TB x58 = s.QC[1] * x9 + s.WQ[1] * x12 + r1 * x2;
-      const TB x120 = s.QC[1] * x57 + s.WQ[1] * x58 + e1 * (x6 - s.roe * x9) + r1 * x54;
-      acc55 += static_cast<double>(x120);
-      const TB x129 = s.QC[2] * x57 + s.WQ[2] * x58;
-      acc64 += static_cast<double>(x129);
-      const TB x59 = s.QC[1] * x7 + s.WQ[1] * x10;
-      acc11 += static_cast<double>(x59);
-      const TB x60 = s.QC[1] * x10 + s.WQ[1] * x13;
-      const TB x130 = s.QC[2] * x59 + s.WQ[2] * x60 + r1 * x54;
-      acc65 += static_cast<double>(x130);
-      const TB x121 = s.QC[1] * x59 + s.WQ[1] * x60 + e1 * (x7 - s.roe * x10);
-      acc56 += static_cast<double>(x121);
-      const TB x61 = s.QC[1] * x17 + s.WQ[1] * x23;
-      acc12 += static_cast<double>(x61);
-      const TB x62 = s.QC[1] * x23 + s.WQ[1] * x29;
-      const TB x81 = s.QC[2] * x23 + s.WQ[2] * x29;
-      const TB x122 = s.QC[1] * x61 + s.WQ[1] * x62 + e1 * (x17 - s.roe * x23);
-      acc57 += static_cast<double>(x122);
-      const TB x131 = s.QC[2] * x61 + s.WQ[2] * x62;
-      acc66 += static_cast<double>(x131);
-      const TB x63 = s.QC[1] * x18 + s.WQ[1] * x24 + r1 * x8;
-      acc13 += static_cast<double>(x63);
-      const TB x64 = s.QC[1] * x24 + s.WQ[1] * x30 + r1 * x11;
-      const TB x83 = s.QC[2] * x24 + s.WQ[2] * x30;
-      const TB x123 = s.QC[1] * x63 + s.WQ[1] * x64 + e1 * (x18 - s.roe * x24) + r1 * x56;
-      acc58 += static_cast<double>(x123);
-      const TB x132 = s.QC[2] * x63 + s.WQ[2] * x64;
-      acc67 += static_cast<double>(x132);
-      const TB x65 = s.QC[1] * x19 + s.WQ[1] * x25;
-      acc14 += static_cast<double>(x65);
-      const TB x66 = s.QC[1] * x25 + s.WQ[1] * x31;
-      const TB x133 = s.QC[2] * x65 + s.WQ[2] * x66 + r1 * x56;
-      acc68 += static_cast<double>(x133);
-      const TB x124 = s.QC[1] * x65 + s.WQ[1] * x66 + e1 * (x19 - s.roe * x25);
-      acc59 += static_cast<double>(x124);
-      const TB x85 = s.QC[2] * x25 + s.WQ[2] * x31 + r1 * x11;
-      const TB x75 = s.QC[2] * x8 + s.WQ[2] * x11;
-      const TB x67 = s.QC[1] * x20 + s.WQ[1] * x26 + r2 * x9;
-      acc15 += static_cast<double>(x67);
-      const TB x68 = s.QC[1] * x26 + s.WQ[1] * x32 + r2 * x12;
-      const TB x87 = s.QC[2] * x26 + s.WQ[2] * x32;
-      const TB x125 = s.QC[1] * x67 + s.WQ[1] * x68 + e1 * (x20 - s.roe * x26) + r2 * x58;
-      acc60 += static_cast<double>(x125);
-      const TB x134 = s.QC[2] * x67 + s.WQ[2] * x68;
-      acc69 += static_cast<double>(x134);
-      const TB x69 = s.QC[1] * x21 + s.WQ[1] * x27 + r1 * x10;
-      acc16 += static_cast<double>(x69);
-      const TB x70 = s.QC[1] * x27 + s.WQ[1] * x33 + r1 * x13;
-      const TB x135 = s.QC[2] * x69 + s.WQ[2] * x70 + r1 * x58;
-      acc70 += static_cast<double>(x135);
-      const TB x126 = s.QC[1] * x69 + s.WQ[1] * x70 + e1 * (x21 - s.roe * x27) + r1 * x60;
-      acc61 += static_cast<double>(x126);
-      const TB x89 = s.QC[2] * x27 + s.WQ[2] * x33 + r1 * x12;
-      const TB x77 = s.QC[2] * x9 + s.WQ[2] * x12;
-      const TB x71 = s.QC[1] * x22 + s.WQ[1] * x28;
-      acc17 += static_cast<double>(x71);
-      const TB x72 = s.QC[1] * x28 + s.WQ[1] * x34;
-      const TB x136 = s.QC[2] * x71 + s.WQ[2] * x72 + r2 * x60;
-      acc71 += static_cast<double>(x136);
-      const TB x127 = s.QC[1] * x71 + s.WQ[1] * x72 + e1 * (x22 - s.roe * x28);
-      acc62 += static_cast<double>(x127);
-      const TB x91 = s.QC[2] * x28 + s.WQ[2] * x34 + r2 * x13;
-      const TB x79 = s.QC[2] * x10 + s.WQ[2] * x13 + r1 * x2;
-      const TB x73 = s.QC[2] * x1 + s.WQ[2] * x2;
-      const TB x78 = s.QC[2] * x7 + s.WQ[2] * x10 + r1 * x1;
-      acc20 += static_cast<double>(x78);
-      const TB x139 = s.QC[2] * x78 + s.WQ[2] * x79 + e1 * (x7 - s.roe * x10) + r1 * x73;
-      acc74 += static_cast<double>(x139);
-      const TB x90 = s.QC[2] * x22 + s.WQ[2] * x28 + r2 * x10;
-      acc26 += static_cast<double>(x90);
-      const TB x145 = s.QC[2] * x90 + s.WQ[2] * x91 + e1 * (x22 - s.roe * x28) + r2 * x79;
-      acc80 += static_cast<double>(x145);
-      const TB x74 = s.QC[2] * x5 + s.WQ[2] * x8;
-      acc18 += static_cast<double>(x74);
-      const TB x137 = s.QC[2] * x74 + s.WQ[2] * x75 + e1 * (x5 - s.roe * x8);
-      acc72 += static_cast<double>(x137);
-      const TB x84 = s.QC[2] * x19 + s.WQ[2] * x25 + r1 * x8;
-      acc23 += static_cast<double>(x84);
-      const TB x142 = s.QC[2] * x84 + s.WQ[2] * x85 + e1 * (x19 - s.roe * x25) + r1 * x75;
-      acc77 += static_cast<double>(x142);
-      const TB x76 = s.QC[2] * x6 + s.WQ[2] * x9;
-      acc19 += static_cast<double>(x76);
-      const TB x138 = s.QC[2] * x76 + s.WQ[2] * x77 + e1 * (x6 - s.roe * x9);
-      acc73 += static_cast<double>(x138);
-      const TB x88 = s.QC[2] * x21 + s.WQ[2] * x27 + r1 * x9;
-      acc25 += static_cast<double>(x88);
-      const TB x144 = s.QC[2] * x88 + s.WQ[2] * x89 + e1 * (x21 - s.roe * x27) + r1 * x77;
-      acc79 += static_cast<double>(x144);
-      const TB x80 = s.QC[2] * x17 + s.WQ[2] * x23;
-      acc21 += static_cast<double>(x80);
-      const TB x140 = s.QC[2] * x80 + s.WQ[2] * x81 + e1 * (x17 - s.roe * x23);
-      acc75 += static_cast<double>(x140);
-      const TB x82 = s.QC[2] * x18 + s.WQ[2] * x24;
-      acc22 += static_cast<double>(x82);
-      const TB x141 = s.QC[2] * x82 + s.WQ[2] * x83 + e1 * (x18 - s.roe * x24);
-      acc76 += static_cast<double>(x141);
-      const TB x86 = s.QC[2] * x20 + s.WQ[2] * x26;
-      acc24 += static_cast<double>(x86);
-      const TB x143 = s.QC[2] * x86 + s.WQ[2] * x87 + e1 * (x20 - s.roe * x26);
-      acc78 += static_cast<double>(x143);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-    accmax = fmax(accmax, fabs(acc31));
-    accmax = fmax(accmax, fabs(acc32));
-    accmax = fmax(accmax, fabs(acc33));
-    accmax = fmax(accmax, fabs(acc34));
-    accmax = fmax(accmax, fabs(acc35));
-    accmax = fmax(accmax, fabs(acc36));
-    accmax = fmax(accmax, fabs(acc37));
-    accmax = fmax(accmax, fabs(acc38));
-    accmax = fmax(accmax, fabs(acc39));
-    accmax = fmax(accmax, fabs(acc40));
-    accmax = fmax(accmax, fabs(acc41));
-    accmax = fmax(accmax, fabs(acc42));
-    accmax = fmax(accmax, fabs(acc43));
-    accmax = fmax(accmax, fabs(acc44));
-    accmax = fmax(accmax, fabs(acc45));
-    accmax = fmax(accmax, fabs(acc46));
-    accmax = fmax(accmax, fabs(acc47));
-    accmax = fmax(accmax, fabs(acc48));
-    accmax = fmax(accmax, fabs(acc49));
-    accmax = fmax(accmax, fabs(acc50));
-    accmax = fmax(accmax, fabs(acc51));
-    accmax = fmax(accmax, fabs(acc52));
-    accmax = fmax(accmax, fabs(acc53));
-    accmax = fmax(accmax, fabs(acc54));
-    accmax = fmax(accmax, fabs(acc55));
-    accmax = fmax(accmax, fabs(acc56));
-    accmax = fmax(accmax, fabs(acc57));
-    accmax = fmax(accmax, fabs(acc58));
-    accmax = fmax(accmax, fabs(acc59));
-    accmax = fmax(accmax, fabs(acc60));
-    accmax = fmax(accmax, fabs(acc61));
-    accmax = fmax(accmax, fabs(acc62));
-    accmax = fmax(accmax, fabs(acc63));
-    accmax = fmax(accmax, fabs(acc64));
-    accmax = fmax(accmax, fabs(acc65));
-    accmax = fmax(accmax, fabs(acc66));
-    accmax = fmax(accmax, fabs(acc67));
-    accmax = fmax(accmax, fabs(acc68));
-    accmax = fmax(accmax, fabs(acc69));
-    accmax = fmax(accmax, fabs(acc70));
-    accmax = fmax(accmax, fabs(acc71));
-    accmax = fmax(accmax, fabs(acc72));
-    accmax = fmax(accmax, fabs(acc73));
-    accmax = fmax(accmax, fabs(acc74));
-    accmax = fmax(accmax, fabs(acc75));
-    accmax = fmax(accmax, fabs(acc76));
-    accmax = fmax(accmax, fabs(acc77));
-    accmax = fmax(accmax, fabs(acc78));
-    accmax = fmax(accmax, fabs(acc79));
-    accmax = fmax(accmax, fabs(acc80));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = acc31;
-  const double h32 = acc32;
-  const double h33 = acc33;
-  const double h34 = acc34;
-  const double h35 = acc35;
-  const double h36 = acc36;
-  const double h37 = acc37;
-  const double h38 = acc38;
-  const double h39 = acc39;
-  const double h40 = acc40;
-  const double h41 = acc41;
-  const double h42 = acc42;
-  const double h43 = acc43;
-  const double h44 = acc44;
-  const double h45 = acc45;
-  const double h46 = acc46;
-  const double h47 = acc47;
-  const double h48 = acc48;
-  const double h49 = acc49;
-  const double h50 = acc50;
-  const double h51 = acc51;
-  const double h52 = acc52;
-  const double h53 = acc53;
-  const double h54 = acc54;
-  const double h55 = acc55;
-  const double h56 = acc56;
-  const double h57 = acc57;
-  const double h58 = acc58;
-  const double h59 = acc59;
-  const double h60 = acc60;
-  const double h61 = acc61;
-  const double h62 = acc62;
-  const double h63 = acc63;
-  const double h64 = acc64;
-  const double h65 = acc65;
-  const double h66 = acc66;
-  const double h67 = acc67;
-  const double h68 = acc68;
-  const double h69 = acc69;
-  const double h70 = acc70;
-  const double h71 = acc71;
-  const double h72 = acc72;
-  const double h73 = acc73;
-  const double h74 = acc74;
-  const double h75 = acc75;
-  const double h76 = acc76;
-  const double h77 = acc77;
-  const double h78 = acc78;
-  const double h79 = acc79;
-  const double h80 = acc80;
-  const double h81 = h74 + CD[2] * h20;
-  const double h82 = h65 + CD[1] * h20;
-  const double h83 = h47 + CD[0] * h20;
-  const double h84 = h65 + CD[2] * h11;
-  const double h85 = h56 + CD[1] * h11;
-  const double h86 = h38 + CD[0] * h11;
-  const double h87 = h47 + CD[2] * h2;
-  const double h88 = h38 + CD[1] * h2;
-  const double h89 = h29 + CD[0] * h2;
-  const double h90 = h80 + CD[2] * h26;
-  const double h91 = h71 + CD[1] * h26;
-  const double h92 = h53 + CD[0] * h26;
-  const double h93 = h71 + CD[2] * h17;
-  const double h94 = h62 + CD[1] * h17;
-  const double h95 = h44 + CD[0] * h17;
-  const double h96 = h53 + CD[2] * h8;
-  const double h97 = h44 + CD[1] * h8;
-  const double h98 = h35 + CD[0] * h8;
-  const double h99 = h73 + CD[2] * h19;
-  const double h100 = h64 + CD[1] * h19;
-  const double h101 = h46 + CD[0] * h19;
-  const double h102 = h64 + CD[2] * h10;
-  const double h103 = h55 + CD[1] * h10;
-  const double h104 = h37 + CD[0] * h10;
-  const double h105 = h46 + CD[2] * h1;
-  const double h106 = h37 + CD[1] * h1;
-  const double h107 = h28 + CD[0] * h1;
-  const double h108 = h79 + CD[2] * h25;
-  const double h109 = h70 + CD[1] * h25;
-  const double h110 = h52 + CD[0] * h25;
-  const double h111 = h70 + CD[2] * h16;
-  const double h112 = h61 + CD[1] * h16;
-  const double h113 = h43 + CD[0] * h16;
-  const double h114 = h52 + CD[2] * h7;
-  const double h115 = h43 + CD[1] * h7;
-  const double h116 = h34 + CD[0] * h7;
-  const double h117 = h78 + CD[2] * h24;
-  const double h118 = h69 + CD[1] * h24;
-  const double h119 = h51 + CD[0] * h24;
-  const double h120 = h69 + CD[2] * h15;
-  const double h121 = h60 + CD[1] * h15;
-  const double h122 = h42 + CD[0] * h15;
-  const double h123 = h51 + CD[2] * h6;
-  const double h124 = h42 + CD[1] * h6;
-  const double h125 = h33 + CD[0] * h6;
-  const double h126 = h72 + CD[2] * h18;
-  const double h127 = h63 + CD[1] * h18;
-  const double h128 = h45 + CD[0] * h18;
-  const double h129 = h63 + CD[2] * h9;
-  const double h130 = h54 + CD[1] * h9;
-  const double h131 = h36 + CD[0] * h9;
-  const double h132 = h45 + CD[2] * h0;
-  const double h133 = h36 + CD[1] * h0;
-  const double h134 = h27 + CD[0] * h0;
-  const double h135 = h77 + CD[2] * h23;
-  const double h136 = h68 + CD[1] * h23;
-  const double h137 = h50 + CD[0] * h23;
-  const double h138 = h68 + CD[2] * h14;
-  const double h139 = h59 + CD[1] * h14;
-  const double h140 = h41 + CD[0] * h14;
-  const double h141 = h50 + CD[2] * h5;
-  const double h142 = h41 + CD[1] * h5;
-  const double h143 = h32 + CD[0] * h5;
-  const double h144 = h76 + CD[2] * h22;
-  const double h145 = h67 + CD[1] * h22;
-  const double h146 = h49 + CD[0] * h22;
-  const double h147 = h67 + CD[2] * h13;
-  const double h148 = h58 + CD[1] * h13;
-  const double h149 = h40 + CD[0] * h13;
-  const double h150 = h49 + CD[2] * h4;
-  const double h151 = h40 + CD[1] * h4;
-  const double h152 = h31 + CD[0] * h4;
-  const double h153 = h75 + CD[2] * h21;
-  const double h154 = h66 + CD[1] * h21;
-  const double h155 = h48 + CD[0] * h21;
-  const double h156 = h66 + CD[2] * h12;
-  const double h157 = h57 + CD[1] * h12;
-  const double h158 = h39 + CD[0] * h12;
-  const double h159 = h48 + CD[2] * h3;
-  const double h160 = h39 + CD[1] * h3;
-  const double h161 = h30 + CD[0] * h3;
-  const double h162 = h90 + AB[2] * h81;
-  const double h163 = h91 + AB[2] * h82;
-  const double h164 = h92 + AB[2] * h83;
-  const double h165 = h93 + AB[2] * h84;
-  const double h166 = h94 + AB[2] * h85;
-  const double h167 = h95 + AB[2] * h86;
-  const double h168 = h96 + AB[2] * h87;
-  const double h169 = h97 + AB[2] * h88;
-  const double h170 = h98 + AB[2] * h89;
-  const double h171 = h108 + AB[1] * h81;
-  const double h172 = h109 + AB[1] * h82;
-  const double h173 = h110 + AB[1] * h83;
-  const double h174 = h111 + AB[1] * h84;
-  const double h175 = h112 + AB[1] * h85;
-  const double h176 = h113 + AB[1] * h86;
-  const double h177 = h114 + AB[1] * h87;
-  const double h178 = h115 + AB[1] * h88;
-  const double h179 = h116 + AB[1] * h89;
-  const double h180 = h135 + AB[0] * h81;
-  const double h181 = h136 + AB[0] * h82;
-  const double h182 = h137 + AB[0] * h83;
-  const double h183 = h138 + AB[0] * h84;
-  const double h184 = h139 + AB[0] * h85;
-  const double h185 = h140 + AB[0] * h86;
-  const double h186 = h141 + AB[0] * h87;
-  const double h187 = h142 + AB[0] * h88;
-  const double h188 = h143 + AB[0] * h89;
-  const double h189 = h108 + AB[2] * h99;
-  const double h190 = h109 + AB[2] * h100;
-  const double h191 = h110 + AB[2] * h101;
-  const double h192 = h111 + AB[2] * h102;
-  const double h193 = h112 + AB[2] * h103;
-  const double h194 = h113 + AB[2] * h104;
-  const double h195 = h114 + AB[2] * h105;
-  const double h196 = h115 + AB[2] * h106;
-  const double h197 = h116 + AB[2] * h107;
-  const double h198 = h117 + AB[1] * h99;
-  const double h199 = h118 + AB[1] * h100;
-  const double h200 = h119 + AB[1] * h101;
-  const double h201 = h120 + AB[1] * h102;
-  const double h202 = h121 + AB[1] * h103;
-  const double h203 = h122 + AB[1] * h104;
-  const double h204 = h123 + AB[1] * h105;
-  const double h205 = h124 + AB[1] * h106;
-  const double h206 = h125 + AB[1] * h107;
-  const double h207 = h144 + AB[0] * h99;
-  const double h208 = h145 + AB[0] * h100;
-  const double h209 = h146 + AB[0] * h101;
-  const double h210 = h147 + AB[0] * h102;
-  const double h211 = h148 + AB[0] * h103;
-  const double h212 = h149 + AB[0] * h104;
-  const double h213 = h150 + AB[0] * h105;
-  const double h214 = h151 + AB[0] * h106;
-  const double h215 = h152 + AB[0] * h107;
-  const double h216 = h135 + AB[2] * h126;
-  const double h217 = h136 + AB[2] * h127;
-  const double h218 = h137 + AB[2] * h128;
-  const double h219 = h138 + AB[2] * h129;
-  const double h220 = h139 + AB[2] * h130;
-  const double h221 = h140 + AB[2] * h131;
-  const double h222 = h141 + AB[2] * h132;
-  const double h223 = h142 + AB[2] * h133;
-  const double h224 = h143 + AB[2] * h134;
-  const double h225 = h144 + AB[1] * h126;
-  const double h226 = h145 + AB[1] * h127;
-  const double h227 = h146 + AB[1] * h128;
-  const double h228 = h147 + AB[1] * h129;
-  const double h229 = h148 + AB[1] * h130;
-  const double h230 = h149 + AB[1] * h131;
-  const double h231 = h150 + AB[1] * h132;
-  const double h232 = h151 + AB[1] * h133;
-  const double h233 = h152 + AB[1] * h134;
-  const double h234 = h153 + AB[0] * h126;
-  const double h235 = h154 + AB[0] * h127;
-  const double h236 = h155 + AB[0] * h128;
-  const double h237 = h156 + AB[0] * h129;
-  const double h238 = h157 + AB[0] * h130;
-  const double h239 = h158 + AB[0] * h131;
-  const double h240 = h159 + AB[0] * h132;
-  const double h241 = h160 + AB[0] * h133;
-  const double h242 = h161 + AB[0] * h134;
-  double out[81];
-  out[0] = h242;
-  out[1] = h241;
-  out[2] = h240;
-  out[3] = h239;
-  out[4] = h238;
-  out[5] = h237;
-  out[6] = h236;
-  out[7] = h235;
-  out[8] = h234;
-  out[9] = h233;
-  out[10] = h232;
-  out[11] = h231;
-  out[12] = h230;
-  out[13] = h229;
-  out[14] = h228;
-  out[15] = h227;
-  out[16] = h226;
-  out[17] = h225;
-  out[18] = h224;
-  out[19] = h223;
-  out[20] = h222;
-  out[21] = h221;
-  out[22] = h220;
-  out[23] = h219;
-  out[24] = h218;
-  out[25] = h217;
-  out[26] = h216;
-  out[27] = h215;
-  out[28] = h214;
-  out[29] = h213;
-  out[30] = h212;
-  out[31] = h211;
-  out[32] = h210;
-  out[33] = h209;
-  out[34] = h208;
-  out[35] = h207;
-  out[36] = h206;
-  out[37] = h205;
-  out[38] = h204;
-  out[39] = h203;
-  out[40] = h202;
-  out[41] = h201;
-  out[42] = h200;
-  out[43] = h199;
-  out[44] = h198;
-  out[45] = h197;
-  out[46] = h196;
-  out[47] = h195;
-  out[48] = h194;
-  out[49] = h193;
-  out[50] = h192;
-  out[51] = h191;
-  out[52] = h190;
-  out[53] = h189;
-  out[54] = h188;
-  out[55] = h187;
-  out[56] = h186;
-  out[57] = h185;
-  out[58] = h184;
-  out[59] = h183;
-  out[60] = h182;
-  out[61] = h181;
-  out[62] = h180;
-  out[63] = h179;
-  out[64] = h178;
-  out[65] = h177;
-  out[66] = h176;
-  out[67] = h175;
-  out[68] = h174;
-  out[69] = h173;
-  out[70] = h172;
-  out[71] = h171;
-  out[72] = h170;
-  out[73] = h169;
-  out[74] = h168;
-  out[75] = h167;
-  out[76] = h166;
-  out[77] = h165;
-  out[78] = h164;
-  out[79] = h163;
-  out[80] = h162;
-  store_block<3, 3, 3, 3>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (20|00): 12 recursion values, 6 accumulators, 6 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2000(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<2>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[3];
-      boys_eval_lds<2>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB x3 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x4 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x5 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x6 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.PA[0] * x3 + s.WP[0] * x6 + z1 * (x0 - s.roz * x1);
-      acc0 += static_cast<double>(x9);
-      const TB x10 = s.PA[1] * x3 + s.WP[1] * x6;
-      acc1 += static_cast<double>(x10);
-      const TB x11 = s.PA[2] * x3 + s.WP[2] * x6;
-      acc2 += static_cast<double>(x11);
-      const TB x7 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x8 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x14 = s.PA[2] * x5 + s.WP[2] * x8 + z1 * (x0 - s.roz * x1);
-      acc5 += static_cast<double>(x14);
-      const TB x12 = s.PA[1] * x4 + s.WP[1] * x7 + z1 * (x0 - s.roz * x1);
-      acc3 += static_cast<double>(x12);
-      const TB x13 = s.PA[2] * x4 + s.WP[2] * x7;
-      acc4 += static_cast<double>(x13);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  double out[6];
-  out[0] = h0 * 0.5773502691896258;
-  out[1] = h1;
-  out[2] = h2;
-  out[3] = h3 * 0.5773502691896258;
-  out[4] = h4;
-  out[5] = h5 * 0.5773502691896258;
-  store_block<6, 1, 1, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (20|10): 39 recursion values, 18 accumulators, 18 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2010(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<3>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[4];
-      boys_eval_lds<3>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x4 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x5 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x6 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x7 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x8 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x9 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x18 = s.PA[2] * x6 + s.WP[2] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x10 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x11 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x12 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x24 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x30 = s.QC[0] * x18 + s.WQ[0] * x24;
-      acc5 += static_cast<double>(x30);
-      const TB x36 = s.QC[1] * x18 + s.WQ[1] * x24;
-      acc11 += static_cast<double>(x36);
-      const TB x42 = s.QC[2] * x18 + s.WQ[2] * x24 + r2 * x9;
-      acc17 += static_cast<double>(x42);
-      const TB x13 = s.PA[0] * x4 + s.WP[0] * x7 + z1 * (x0 - s.roz * x1);
-      const TB x16 = s.PA[1] * x5 + s.WP[1] * x8 + z1 * (x0 - s.roz * x1);
-      const TB x17 = s.PA[2] * x5 + s.WP[2] * x8;
-      const TB x14 = s.PA[1] * x4 + s.WP[1] * x7;
-      const TB x15 = s.PA[2] * x4 + s.WP[2] * x7;
-      const TB x19 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x1 - s.roz * x2);
-      const TB x22 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x23 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x25 = s.QC[0] * x13 + s.WQ[0] * x19 + r2 * x7;
-      acc0 += static_cast<double>(x25);
-      const TB x28 = s.QC[0] * x16 + s.WQ[0] * x22;
-      acc3 += static_cast<double>(x28);
-      const TB x29 = s.QC[0] * x17 + s.WQ[0] * x23;
-      acc4 += static_cast<double>(x29);
-      const TB x31 = s.QC[1] * x13 + s.WQ[1] * x19;
-      acc6 += static_cast<double>(x31);
-      const TB x37 = s.QC[2] * x13 + s.WQ[2] * x19;
-      acc12 += static_cast<double>(x37);
-      const TB x34 = s.QC[1] * x16 + s.WQ[1] * x22 + r2 * x8;
-      acc9 += static_cast<double>(x34);
-      const TB x40 = s.QC[2] * x16 + s.WQ[2] * x22;
-      acc15 += static_cast<double>(x40);
-      const TB x35 = s.QC[1] * x17 + s.WQ[1] * x23 + r1 * x9;
-      acc10 += static_cast<double>(x35);
-      const TB x41 = s.QC[2] * x17 + s.WQ[2] * x23 + r1 * x8;
-      acc16 += static_cast<double>(x41);
-      const TB x20 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x26 = s.QC[0] * x14 + s.WQ[0] * x20 + r1 * x8;
-      acc1 += static_cast<double>(x26);
-      const TB x21 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x27 = s.QC[0] * x15 + s.WQ[0] * x21 + r1 * x9;
-      acc2 += static_cast<double>(x27);
-      const TB x32 = s.QC[1] * x14 + s.WQ[1] * x20 + r1 * x7;
-      acc7 += static_cast<double>(x32);
-      const TB x38 = s.QC[2] * x14 + s.WQ[2] * x20;
-      acc13 += static_cast<double>(x38);
-      const TB x39 = s.QC[2] * x15 + s.WQ[2] * x21 + r1 * x7;
-      acc14 += static_cast<double>(x39);
-      const TB x33 = s.QC[1] * x15 + s.WQ[1] * x21;
-      acc8 += static_cast<double>(x33);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  double out[18];
-  out[0] = h0 * 0.5773502691896258;
-  out[1] = h6 * 0.5773502691896258;
-  out[2] = h12 * 0.5773502691896258;
-  out[3] = h1;
-  out[4] = h7;
-  out[5] = h13;
-  out[6] = h2;
-  out[7] = h8;
-  out[8] = h14;
-  out[9] = h3 * 0.5773502691896258;
-  out[10] = h9 * 0.5773502691896258;
-  out[11] = h15 * 0.5773502691896258;
-  out[12] = h4;
-  out[13] = h10;
-  out[14] = h16;
-  out[15] = h5 * 0.5773502691896258;
-  out[16] = h11 * 0.5773502691896258;
-  out[17] = h17 * 0.5773502691896258;
-  store_block<6, 1, 3, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (20|11): 111 recursion values, 54 accumulators, 54 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2011(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<4>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  double acc31 = 0.0;
-  double acc32 = 0.0;
-  double acc33 = 0.0;
-  double acc34 = 0.0;
-  double acc35 = 0.0;
-  double acc36 = 0.0;
-  double acc37 = 0.0;
-  double acc38 = 0.0;
-  double acc39 = 0.0;
-  double acc40 = 0.0;
-  double acc41 = 0.0;
-  double acc42 = 0.0;
-  double acc43 = 0.0;
-  double acc44 = 0.0;
-  double acc45 = 0.0;
-  double acc46 = 0.0;
-  double acc47 = 0.0;
-  double acc48 = 0.0;
-  double acc49 = 0.0;
-  double acc50 = 0.0;
-  double acc51 = 0.0;
-  double acc52 = 0.0;
-  double acc53 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[5];
-      boys_eval_lds<4>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x5 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x6 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x7 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x8 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x10 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x22 = s.PA[2] * x7 + s.WP[2] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x11 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x12 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x13 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x14 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x15 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x16 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x34 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x17 = s.PA[0] * x5 + s.WP[0] * x8 + z1 * (x0 - s.roz * x1);
-      const TB x20 = s.PA[1] * x6 + s.WP[1] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x21 = s.PA[2] * x6 + s.WP[2] * x9;
-      const TB x18 = s.PA[1] * x5 + s.WP[1] * x8;
-      const TB x19 = s.PA[2] * x5 + s.WP[2] * x8;
-      const TB x23 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x24 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x25 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x26 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x28 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x27 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x29 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x2 - s.roz * x3);
-      const TB x32 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x33 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x30 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x31 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x35 = s.QC[0] * x8 + s.WQ[0] * x11 + r1 * x2;
-      const TB x36 = s.QC[0] * x9 + s.WQ[0] * x12;
-      const TB x37 = s.QC[0] * x10 + s.WQ[0] * x13;
-      const TB x38 = s.QC[0] * x17 + s.WQ[0] * x23 + r2 * x8;
-      acc0 += static_cast<double>(x38);
-      const TB x39 = s.QC[0] * x23 + s.WQ[0] * x29 + r2 * x11;
-      const TB x80 = s.QC[0] * x38 + s.WQ[0] * x39 + e1 * (x17 - s.roe * x23) + r2 * x35;
-      acc18 += static_cast<double>(x80);
-      const TB x86 = s.QC[1] * x38 + s.WQ[1] * x39;
-      acc24 += static_cast<double>(x86);
-      const TB x92 = s.QC[2] * x38 + s.WQ[2] * x39;
-      acc30 += static_cast<double>(x92);
-      const TB x40 = s.QC[0] * x18 + s.WQ[0] * x24 + r1 * x9;
-      acc1 += static_cast<double>(x40);
-      const TB x41 = s.QC[0] * x24 + s.WQ[0] * x30 + r1 * x12;
-      const TB x81 = s.QC[0] * x40 + s.WQ[0] * x41 + e1 * (x18 - s.roe * x24) + r1 * x36;
-      acc19 += static_cast<double>(x81);
-      const TB x87 = s.QC[1] * x40 + s.WQ[1] * x41 + r1 * x35;
-      acc25 += static_cast<double>(x87);
-      const TB x93 = s.QC[2] * x40 + s.WQ[2] * x41;
-      acc31 += static_cast<double>(x93);
-      const TB x42 = s.QC[0] * x19 + s.WQ[0] * x25 + r1 * x10;
-      acc2 += static_cast<double>(x42);
-      const TB x43 = s.QC[0] * x25 + s.WQ[0] * x31 + r1 * x13;
-      const TB x94 = s.QC[2] * x42 + s.WQ[2] * x43 + r1 * x35;
-      acc32 += static_cast<double>(x94);
-      const TB x82 = s.QC[0] * x42 + s.WQ[0] * x43 + e1 * (x19 - s.roe * x25) + r1 * x37;
-      acc20 += static_cast<double>(x82);
-      const TB x88 = s.QC[1] * x42 + s.WQ[1] * x43;
-      acc26 += static_cast<double>(x88);
-      const TB x44 = s.QC[0] * x20 + s.WQ[0] * x26;
-      acc3 += static_cast<double>(x44);
-      const TB x45 = s.QC[0] * x26 + s.WQ[0] * x32;
-      const TB x83 = s.QC[0] * x44 + s.WQ[0] * x45 + e1 * (x20 - s.roe * x26);
-      acc21 += static_cast<double>(x83);
-      const TB x89 = s.QC[1] * x44 + s.WQ[1] * x45 + r2 * x36;
-      acc27 += static_cast<double>(x89);
-      const TB x95 = s.QC[2] * x44 + s.WQ[2] * x45;
-      acc33 += static_cast<double>(x95);
-      const TB x46 = s.QC[0] * x21 + s.WQ[0] * x27;
-      acc4 += static_cast<double>(x46);
-      const TB x47 = s.QC[0] * x27 + s.WQ[0] * x33;
-      const TB x96 = s.QC[2] * x46 + s.WQ[2] * x47 + r1 * x36;
-      acc34 += static_cast<double>(x96);
-      const TB x84 = s.QC[0] * x46 + s.WQ[0] * x47 + e1 * (x21 - s.roe * x27);
-      acc22 += static_cast<double>(x84);
-      const TB x90 = s.QC[1] * x46 + s.WQ[1] * x47 + r1 * x37;
-      acc28 += static_cast<double>(x90);
-      const TB x48 = s.QC[0] * x22 + s.WQ[0] * x28;
-      acc5 += static_cast<double>(x48);
-      const TB x49 = s.QC[0] * x28 + s.WQ[0] * x34;
-      const TB x97 = s.QC[2] * x48 + s.WQ[2] * x49 + r2 * x37;
-      acc35 += static_cast<double>(x97);
-      const TB x85 = s.QC[0] * x48 + s.WQ[0] * x49 + e1 * (x22 - s.roe * x28);
-      acc23 += static_cast<double>(x85);
-      const TB x91 = s.QC[1] * x48 + s.WQ[1] * x49;
-      acc29 += static_cast<double>(x91);
-      const TB x50 = s.QC[1] * x8 + s.WQ[1] * x11;
-      const TB x51 = s.QC[1] * x9 + s.WQ[1] * x12 + r1 * x2;
-      const TB x67 = s.QC[2] * x10 + s.WQ[2] * x13 + r1 * x2;
-      const TB x52 = s.QC[1] * x10 + s.WQ[1] * x13;
-      const TB x53 = s.QC[1] * x17 + s.WQ[1] * x23;
-      acc6 += static_cast<double>(x53);
-      const TB x54 = s.QC[1] * x23 + s.WQ[1] * x29;
-      const TB x69 = s.QC[2] * x23 + s.WQ[2] * x29;
-      const TB x98 = s.QC[1] * x53 + s.WQ[1] * x54 + e1 * (x17 - s.roe * x23);
-      acc36 += static_cast<double>(x98);
-      const TB x104 = s.QC[2] * x53 + s.WQ[2] * x54;
-      acc42 += static_cast<double>(x104);
-      const TB x55 = s.QC[1] * x18 + s.WQ[1] * x24 + r1 * x8;
-      acc7 += static_cast<double>(x55);
-      const TB x56 = s.QC[1] * x24 + s.WQ[1] * x30 + r1 * x11;
-      const TB x71 = s.QC[2] * x24 + s.WQ[2] * x30;
-      const TB x99 = s.QC[1] * x55 + s.WQ[1] * x56 + e1 * (x18 - s.roe * x24) + r1 * x50;
-      acc37 += static_cast<double>(x99);
-      const TB x105 = s.QC[2] * x55 + s.WQ[2] * x56;
-      acc43 += static_cast<double>(x105);
-      const TB x57 = s.QC[1] * x19 + s.WQ[1] * x25;
-      acc8 += static_cast<double>(x57);
-      const TB x58 = s.QC[1] * x25 + s.WQ[1] * x31;
-      const TB x106 = s.QC[2] * x57 + s.WQ[2] * x58 + r1 * x50;
-      acc44 += static_cast<double>(x106);
-      const TB x100 = s.QC[1] * x57 + s.WQ[1] * x58 + e1 * (x19 - s.roe * x25);
-      acc38 += static_cast<double>(x100);
-      const TB x73 = s.QC[2] * x25 + s.WQ[2] * x31 + r1 * x11;
-      const TB x65 = s.QC[2] * x8 + s.WQ[2] * x11;
-      const TB x72 = s.QC[2] * x19 + s.WQ[2] * x25 + r1 * x8;
-      acc14 += static_cast<double>(x72);
-      const TB x112 = s.QC[2] * x72 + s.WQ[2] * x73 + e1 * (x19 - s.roe * x25) + r1 * x65;
-      acc50 += static_cast<double>(x112);
-      const TB x59 = s.QC[1] * x20 + s.WQ[1] * x26 + r2 * x9;
-      acc9 += static_cast<double>(x59);
-      const TB x60 = s.QC[1] * x26 + s.WQ[1] * x32 + r2 * x12;
-      const TB x75 = s.QC[2] * x26 + s.WQ[2] * x32;
-      const TB x101 = s.QC[1] * x59 + s.WQ[1] * x60 + e1 * (x20 - s.roe * x26) + r2 * x51;
-      acc39 += static_cast<double>(x101);
-      const TB x107 = s.QC[2] * x59 + s.WQ[2] * x60;
-      acc45 += static_cast<double>(x107);
-      const TB x61 = s.QC[1] * x21 + s.WQ[1] * x27 + r1 * x10;
-      acc10 += static_cast<double>(x61);
-      const TB x78 = s.QC[2] * x22 + s.WQ[2] * x28 + r2 * x10;
-      acc17 += static_cast<double>(x78);
-      const TB x62 = s.QC[1] * x27 + s.WQ[1] * x33 + r1 * x13;
-      const TB x108 = s.QC[2] * x61 + s.WQ[2] * x62 + r1 * x51;
-      acc46 += static_cast<double>(x108);
-      const TB x102 = s.QC[1] * x61 + s.WQ[1] * x62 + e1 * (x21 - s.roe * x27) + r1 * x52;
-      acc40 += static_cast<double>(x102);
-      const TB x77 = s.QC[2] * x27 + s.WQ[2] * x33 + r1 * x12;
-      const TB x66 = s.QC[2] * x9 + s.WQ[2] * x12;
-      const TB x76 = s.QC[2] * x21 + s.WQ[2] * x27 + r1 * x9;
-      acc16 += static_cast<double>(x76);
-      const TB x114 = s.QC[2] * x76 + s.WQ[2] * x77 + e1 * (x21 - s.roe * x27) + r1 * x66;
-      acc52 += static_cast<double>(x114);
-      const TB x79 = s.QC[2] * x28 + s.WQ[2] * x34 + r2 * x13;
-      const TB x115 = s.QC[2] * x78 + s.WQ[2] * x79 + e1 * (x22 - s.roe * x28) + r2 * x67;
-      acc53 += static_cast<double>(x115);
-      const TB x64 = s.QC[1] * x28 + s.WQ[1] * x34;
-      const TB x63 = s.QC[1] * x22 + s.WQ[1] * x28;
-      acc11 += static_cast<double>(x63);
-      const TB x103 = s.QC[1] * x63 + s.WQ[1] * x64 + e1 * (x22 - s.roe * x28);
-      acc41 += static_cast<double>(x103);
-      const TB x109 = s.QC[2] * x63 + s.WQ[2] * x64 + r2 * x52;
-      acc47 += static_cast<double>(x109);
-      const TB x68 = s.QC[2] * x17 + s.WQ[2] * x23;
-      acc12 += static_cast<double>(x68);
-      const TB x110 = s.QC[2] * x68 + s.WQ[2] * x69 + e1 * (x17 - s.roe * x23);
-      acc48 += static_cast<double>(x110);
-      const TB x70 = s.QC[2] * x18 + s.WQ[2] * x24;
-      acc13 += static_cast<double>(x70);
-      const TB x111 = s.QC[2] * x70 + s.WQ[2] * x71 + e1 * (x18 - s.roe * x24);
-      acc49 += static_cast<double>(x111);
-      const TB x74 = s.QC[2] * x20 + s.WQ[2] * x26;
-      acc15 += static_cast<double>(x74);
-      const TB x113 = s.QC[2] * x74 + s.WQ[2] * x75 + e1 * (x20 - s.roe * x26);
-      acc51 += static_cast<double>(x113);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-    accmax = fmax(accmax, fabs(acc31));
-    accmax = fmax(accmax, fabs(acc32));
-    accmax = fmax(accmax, fabs(acc33));
-    accmax = fmax(accmax, fabs(acc34));
-    accmax = fmax(accmax, fabs(acc35));
-    accmax = fmax(accmax, fabs(acc36));
-    accmax = fmax(accmax, fabs(acc37));
-    accmax = fmax(accmax, fabs(acc38));
-    accmax = fmax(accmax, fabs(acc39));
-    accmax = fmax(accmax, fabs(acc40));
-    accmax = fmax(accmax, fabs(acc41));
-    accmax = fmax(accmax, fabs(acc42));
-    accmax = fmax(accmax, fabs(acc43));
-    accmax = fmax(accmax, fabs(acc44));
-    accmax = fmax(accmax, fabs(acc45));
-    accmax = fmax(accmax, fabs(acc46));
-    accmax = fmax(accmax, fabs(acc47));
-    accmax = fmax(accmax, fabs(acc48));
-    accmax = fmax(accmax, fabs(acc49));
-    accmax = fmax(accmax, fabs(acc50));
-    accmax = fmax(accmax, fabs(acc51));
-    accmax = fmax(accmax, fabs(acc52));
-    accmax = fmax(accmax, fabs(acc53));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = acc31;
-  const double h32 = acc32;
-  const double h33 = acc33;
-  const double h34 = acc34;
-  const double h35 = acc35;
-  const double h36 = acc36;
-  const double h37 = acc37;
-  const double h38 = acc38;
-  const double h39 = acc39;
-  const double h40 = acc40;
-  const double h41 = acc41;
-  const double h42 = acc42;
-  const double h43 = acc43;
-  const double h44 = acc44;
-  const double h45 = acc45;
-  const double h46 = acc46;
-  const double h47 = acc47;
-  const double h48 = acc48;
-  const double h49 = acc49;
-  const double h50 = acc50;
-  const double h51 = acc51;
-  const double h52 = acc52;
-  const double h53 = acc53;
-  const double h54 = h53 + CD[2] * h17;
-  const double h55 = h47 + CD[1] * h17;
-  const double h56 = h35 + CD[0] * h17;
-  const double h57 = h47 + CD[2] * h11;
-  const double h58 = h41 + CD[1] * h11;
-  const double h59 = h29 + CD[0] * h11;
-  const double h60 = h35 + CD[2] * h5;
-  const double h61 = h29 + CD[1] * h5;
-  const double h62 = h23 + CD[0] * h5;
-  const double h63 = h52 + CD[2] * h16;
-  const double h64 = h46 + CD[1] * h16;
-  const double h65 = h34 + CD[0] * h16;
-  const double h66 = h46 + CD[2] * h10;
-  const double h67 = h40 + CD[1] * h10;
-  const double h68 = h28 + CD[0] * h10;
-  const double h69 = h34 + CD[2] * h4;
-  const double h70 = h28 + CD[1] * h4;
-  const double h71 = h22 + CD[0] * h4;
-  const double h72 = h51 + CD[2] * h15;
-  const double h73 = h45 + CD[1] * h15;
-  const double h74 = h33 + CD[0] * h15;
-  const double h75 = h45 + CD[2] * h9;
-  const double h76 = h39 + CD[1] * h9;
-  const double h77 = h27 + CD[0] * h9;
-  const double h78 = h33 + CD[2] * h3;
-  const double h79 = h27 + CD[1] * h3;
-  const double h80 = h21 + CD[0] * h3;
-  const double h81 = h50 + CD[2] * h14;
-  const double h82 = h44 + CD[1] * h14;
-  const double h83 = h32 + CD[0] * h14;
-  const double h84 = h44 + CD[2] * h8;
-  const double h85 = h38 + CD[1] * h8;
-  const double h86 = h26 + CD[0] * h8;
-  const double h87 = h32 + CD[2] * h2;
-  const double h88 = h26 + CD[1] * h2;
-  const double h89 = h20 + CD[0] * h2;
-  const double h90 = h49 + CD[2] * h13;
-  const double h91 = h43 + CD[1] * h13;
-  const double h92 = h31 + CD[0] * h13;
-  const double h93 = h43 + CD[2] * h7;
-  const double h94 = h37 + CD[1] * h7;
-  const double h95 = h25 + CD[0] * h7;
-  const double h96 = h31 + CD[2] * h1;
-  const double h97 = h25 + CD[1] * h1;
-  const double h98 = h19 + CD[0] * h1;
-  const double h99 = h48 + CD[2] * h12;
-  const double h100 = h42 + CD[1] * h12;
-  const double h101 = h30 + CD[0] * h12;
-  const double h102 = h42 + CD[2] * h6;
-  const double h103 = h36 + CD[1] * h6;
-  const double h104 = h24 + CD[0] * h6;
-  const double h105 = h30 + CD[2] * h0;
-  const double h106 = h24 + CD[1] * h0;
-  const double h107 = h18 + CD[0] * h0;
-  double out[54];
-  out[0] = h107 * 0.5773502691896258;
-  out[1] = h106 * 0.5773502691896258;
-  out[2] = h105 * 0.5773502691896258;
-  out[3] = h104 * 0.5773502691896258;
-  out[4] = h103 * 0.5773502691896258;
-  out[5] = h102 * 0.5773502691896258;
-  out[6] = h101 * 0.5773502691896258;
-  out[7] = h100 * 0.5773502691896258;
-  out[8] = h99 * 0.5773502691896258;
-  out[9] = h98;
-  out[10] = h97;
-  out[11] = h96;
-  out[12] = h95;
-  out[13] = h94;
-  out[14] = h93;
-  out[15] = h92;
-  out[16] = h91;
-  out[17] = h90;
-  out[18] = h89;
-  out[19] = h88;
-  out[20] = h87;
-  out[21] = h86;
-  out[22] = h85;
-  out[23] = h84;
-  out[24] = h83;
-  out[25] = h82;
-  out[26] = h81;
-  out[27] = h80 * 0.5773502691896258;
-  out[28] = h79 * 0.5773502691896258;
-  out[29] = h78 * 0.5773502691896258;
-  out[30] = h77 * 0.5773502691896258;
-  out[31] = h76 * 0.5773502691896258;
-  out[32] = h75 * 0.5773502691896258;
-  out[33] = h74 * 0.5773502691896258;
-  out[34] = h73 * 0.5773502691896258;
-  out[35] = h72 * 0.5773502691896258;
-  out[36] = h71;
-  out[37] = h70;
-  out[38] = h69;
-  out[39] = h68;
-  out[40] = h67;
-  out[41] = h66;
-  out[42] = h65;
-  out[43] = h64;
-  out[44] = h63;
-  out[45] = h62 * 0.5773502691896258;
-  out[46] = h61 * 0.5773502691896258;
-  out[47] = h60 * 0.5773502691896258;
-  out[48] = h59 * 0.5773502691896258;
-  out[49] = h58 * 0.5773502691896258;
-  out[50] = h57 * 0.5773502691896258;
-  out[51] = h56 * 0.5773502691896258;
-  out[52] = h55 * 0.5773502691896258;
-  out[53] = h54 * 0.5773502691896258;
-  store_block<6, 1, 3, 3>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (20|20): 111 recursion values, 36 accumulators, 36 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2020(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<4>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  double acc31 = 0.0;
-  double acc32 = 0.0;
-  double acc33 = 0.0;
-  double acc34 = 0.0;
-  double acc35 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[5];
-      boys_eval_lds<4>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x5 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x6 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x7 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x8 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x10 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x22 = s.PA[2] * x7 + s.WP[2] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x11 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x12 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x13 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x14 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x15 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x16 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x34 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x17 = s.PA[0] * x5 + s.WP[0] * x8 + z1 * (x0 - s.roz * x1);
-      const TB x20 = s.PA[1] * x6 + s.WP[1] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x21 = s.PA[2] * x6 + s.WP[2] * x9;
-      const TB x18 = s.PA[1] * x5 + s.WP[1] * x8;
-      const TB x19 = s.PA[2] * x5 + s.WP[2] * x8;
-      const TB x23 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x24 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x25 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x26 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x28 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x27 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x29 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x2 - s.roz * x3);
-      const TB x32 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x33 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x30 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x31 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x35 = s.QC[0] * x8 + s.WQ[0] * x11 + r1 * x2;
-      const TB x36 = s.QC[0] * x9 + s.WQ[0] * x12;
-      const TB x37 = s.QC[0] * x10 + s.WQ[0] * x13;
-      const TB x38 = s.QC[0] * x17 + s.WQ[0] * x23 + r2 * x8;
-      const TB x39 = s.QC[0] * x23 + s.WQ[0] * x29 + r2 * x11;
-      const TB x80 = s.QC[0] * x38 + s.WQ[0] * x39 + e1 * (x17 - s.roe * x23) + r2 * x35;
-      acc0 += static_cast<double>(x80);
-      const TB x86 = s.QC[1] * x38 + s.WQ[1] * x39;
-      acc6 += static_cast<double>(x86);
-      const TB x92 = s.QC[2] * x38 + s.WQ[2] * x39;
-      acc12 += static_cast<double>(x92);
-      const TB x40 = s.QC[0] * x18 + s.WQ[0] * x24 + r1 * x9;
-      const TB x41 = s.QC[0] * x24 + s.WQ[0] * x30 + r1 * x12;
-      const TB x81 = s.QC[0] * x40 + s.WQ[0] * x41 + e1 * (x18 - s.roe * x24) + r1 * x36;
-      acc1 += static_cast<double>(x81);
-      const TB x87 = s.QC[1] * x40 + s.WQ[1] * x41 + r1 * x35;
-      acc7 += static_cast<double>(x87);
-      const TB x93 = s.QC[2] * x40 + s.WQ[2] * x41;
-      acc13 += static_cast<double>(x93);
-      const TB x42 = s.QC[0] * x19 + s.WQ[0] * x25 + r1 * x10;
-      const TB x43 = s.QC[0] * x25 + s.WQ[0] * x31 + r1 * x13;
-      const TB x94 = s.QC[2] * x42 + s.WQ[2] * x43 + r1 * x35;
-      acc14 += static_cast<double>(x94);
-      const TB x82 = s.QC[0] * x42 + s.WQ[0] * x43 + e1 * (x19 - s.roe * x25) + r1 * x37;
-      acc2 += static_cast<double>(x82);
-      const TB x88 = s.QC[1] * x42 + s.WQ[1] * x43;
-      acc8 += static_cast<double>(x88);
-      const TB x44 = s.QC[0] * x20 + s.WQ[0] * x26;
-      const TB x45 = s.QC[0] * x26 + s.WQ[0] * x32;
-      const TB x83 = s.QC[0] * x44 + s.WQ[0] * x45 + e1 * (x20 - s.roe * x26);
-      acc3 += static_cast<double>(x83);
-      const TB x89 = s.QC[1] * x44 + s.WQ[1] * x45 + r2 * x36;
-      acc9 += static_cast<double>(x89);
-      const TB x95 = s.QC[2] * x44 + s.WQ[2] * x45;
-      acc15 += static_cast<double>(x95);
-      const TB x46 = s.QC[0] * x21 + s.WQ[0] * x27;
-      const TB x47 = s.QC[0] * x27 + s.WQ[0] * x33;
-      const TB x96 = s.QC[2] * x46 + s.WQ[2] * x47 + r1 * x36;
-      acc16 += static_cast<double>(x96);
-      const TB x84 = s.QC[0] * x46 + s.WQ[0] * x47 + e1 * (x21 - s.roe * x27);
-      acc4 += static_cast<double>(x84);
-      const TB x90 = s.QC[1] * x46 + s.WQ[1] * x47 + r1 * x37;
-      acc10 += static_cast<double>(x90);
-      const TB x48 = s.QC[0] * x22 + s.WQ[0] * x28;
-      const TB x49 = s.QC[0] * x28 + s.WQ[0] * x34;
-      const TB x97 = s.QC[2] * x48 + s.WQ[2] * x49 + r2 * x37;
-      acc17 += static_cast<double>(x97);
-      const TB x85 = s.QC[0] * x48 + s.WQ[0] * x49 + e1 * (x22 - s.roe * x28);
-      acc5 += static_cast<double>(x85);
-      const TB x91 = s.QC[1] * x48 + s.WQ[1] * x49;
-      acc11 += static_cast<double>(x91);
-      const TB x50 = s.QC[1] * x8 + s.WQ[1] * x11;
-      const TB x51 = s.QC[1] * x9 + s.WQ[1] * x12 + r1 * x2;
-      const TB x67 = s.QC[2] * x10 + s.WQ[2] * x13 + r1 * x2;
-      const TB x52 = s.QC[1] * x10 + s.WQ[1] * x13;
-      const TB x53 = s.QC[1] * x17 + s.WQ[1] * x23;
-      const TB x54 = s.QC[1] * x23 + s.WQ[1] * x29;
-      const TB x69 = s.QC[2] * x23 + s.WQ[2] * x29;
-      const TB x98 = s.QC[1] * x53 + s.WQ[1] * x54 + e1 * (x17 - s.roe * x23);
-      acc18 += static_cast<double>(x98);
-      const TB x104 = s.QC[2] * x53 + s.WQ[2] * x54;
-      acc24 += static_cast<double>(x104);
-      const TB x55 = s.QC[1] * x18 + s.WQ[1] * x24 + r1 * x8;
-      const TB x56 = s.QC[1] * x24 + s.WQ[1] * x30 + r1 * x11;
-      const TB x71 = s.QC[2] * x24 + s.WQ[2] * x30;
-      const TB x99 = s.QC[1] * x55 + s.WQ[1] * x56 + e1 * (x18 - s.roe * x24) + r1 * x50;
-      acc19 += static_cast<double>(x99);
-      const TB x105 = s.QC[2] * x55 + s.WQ[2] * x56;
-      acc25 += static_cast<double>(x105);
-      const TB x57 = s.QC[1] * x19 + s.WQ[1] * x25;
-      const TB x58 = s.QC[1] * x25 + s.WQ[1] * x31;
-      const TB x106 = s.QC[2] * x57 + s.WQ[2] * x58 + r1 * x50;
-      acc26 += static_cast<double>(x106);
-      const TB x100 = s.QC[1] * x57 + s.WQ[1] * x58 + e1 * (x19 - s.roe * x25);
-      acc20 += static_cast<double>(x100);
-      const TB x73 = s.QC[2] * x25 + s.WQ[2] * x31 + r1 * x11;
-      const TB x65 = s.QC[2] * x8 + s.WQ[2] * x11;
-      const TB x72 = s.QC[2] * x19 + s.WQ[2] * x25 + r1 * x8;
-      const TB x112 = s.QC[2] * x72 + s.WQ[2] * x73 + e1 * (x19 - s.roe * x25) + r1 * x65;
-      acc32 += static_cast<double>(x112);
-      const TB x59 = s.QC[1] * x20 + s.WQ[1] * x26 + r2 * x9;
-      const TB x60 = s.QC[1] * x26 + s.WQ[1] * x32 + r2 * x12;
-      const TB x75 = s.QC[2] * x26 + s.WQ[2] * x32;
-      const TB x101 = s.QC[1] * x59 + s.WQ[1] * x60 + e1 * (x20 - s.roe * x26) + r2 * x51;
-      acc21 += static_cast<double>(x101);
-      const TB x107 = s.QC[2] * x59 + s.WQ[2] * x60;
-      acc27 += static_cast<double>(x107);
-      const TB x61 = s.QC[1] * x21 + s.WQ[1] * x27 + r1 * x10;
-      const TB x78 = s.QC[2] * x22 + s.WQ[2] * x28 + r2 * x10;
-      const TB x62 = s.QC[1] * x27 + s.WQ[1] * x33 + r1 * x13;
-      const TB x108 = s.QC[2] * x61 + s.WQ[2] * x62 + r1 * x51;
-      acc28 += static_cast<double>(x108);
-      const TB x102 = s.QC[1] * x61 + s.WQ[1] * x62 + e1 * (x21 - s.roe * x27) + r1 * x52;
-      acc22 += static_cast<double>(x102);
-      const TB x77 = s.QC[2] * x27 + s.WQ[2] * x33 + r1 * x12;
-      const TB x66 = s.QC[2] * x9 + s.WQ[2] * x12;
-      const TB x76 = s.QC[2] * x21 + s.WQ[2] * x27 + r1 * x9;
-      const TB x114 = s.QC[2] * x76 + s.WQ[2] * x77 + e1 * (x21 - s.roe * x27) + r1 * x66;
-      acc34 += static_cast<double>(x114);
-      const TB x79 = s.QC[2] * x28 + s.WQ[2] * x34 + r2 * x13;
-      const TB x115 = s.QC[2] * x78 + s.WQ[2] * x79 + e1 * (x22 - s.roe * x28) + r2 * x67;
-      acc35 += static_cast<double>(x115);
-      const TB x64 = s.QC[1] * x28 + s.WQ[1] * x34;
-      const TB x63 = s.QC[1] * x22 + s.WQ[1] * x28;
-      const TB x103 = s.QC[1] * x63 + s.WQ[1] * x64 + e1 * (x22 - s.roe * x28);
-      acc23 += static_cast<double>(x103);
-      const TB x109 = s.QC[2] * x63 + s.WQ[2] * x64 + r2 * x52;
-      acc29 += static_cast<double>(x109);
-      const TB x68 = s.QC[2] * x17 + s.WQ[2] * x23;
-      const TB x110 = s.QC[2] * x68 + s.WQ[2] * x69 + e1 * (x17 - s.roe * x23);
-      acc30 += static_cast<double>(x110);
-      const TB x70 = s.QC[2] * x18 + s.WQ[2] * x24;
-      const TB x111 = s.QC[2] * x70 + s.WQ[2] * x71 + e1 * (x18 - s.roe * x24);
-      acc31 += static_cast<double>(x111);
-      const TB x74 = s.QC[2] * x20 + s.WQ[2] * x26;
-      const TB x113 = s.QC[2] * x74 + s.WQ[2] * x75 + e1 * (x20 - s.roe * x26);
-      acc33 += static_cast<double>(x113);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-    accmax = fmax(accmax, fabs(acc31));
-    accmax = fmax(accmax, fabs(acc32));
-    accmax = fmax(accmax, fabs(acc33));
-    accmax = fmax(accmax, fabs(acc34));
-    accmax = fmax(accmax, fabs(acc35));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = acc31;
-  const double h32 = acc32;
-  const double h33 = acc33;
-  const double h34 = acc34;
-  const double h35 = acc35;
-  double out[36];
-  out[0] = h0 * 0.3333333333333334;
-  out[1] = h6 * 0.5773502691896258;
-  out[2] = h12 * 0.5773502691896258;
-  out[3] = h18 * 0.3333333333333334;
-  out[4] = h24 * 0.5773502691896258;
-  out[5] = h30 * 0.3333333333333334;
-  out[6] = h1 * 0.5773502691896258;
-  out[7] = h7;
-  out[8] = h13;
-  out[9] = h19 * 0.5773502691896258;
-  out[10] = h25;
-  out[11] = h31 * 0.5773502691896258;
-  out[12] = h2 * 0.5773502691896258;
-  out[13] = h8;
-  out[14] = h14;
-  out[15] = h20 * 0.5773502691896258;
-  out[16] = h26;
-  out[17] = h32 * 0.5773502691896258;
-  out[18] = h3 * 0.3333333333333334;
-  out[19] = h9 * 0.5773502691896258;
-  out[20] = h15 * 0.5773502691896258;
-  out[21] = h21 * 0.3333333333333334;
-  out[22] = h27 * 0.5773502691896258;
-  out[23] = h33 * 0.3333333333333334;
-  out[24] = h4 * 0.5773502691896258;
-  out[25] = h10;
-  out[26] = h16;
-  out[27] = h22 * 0.5773502691896258;
-  out[28] = h28;
-  out[29] = h34 * 0.5773502691896258;
-  out[30] = h5 * 0.3333333333333334;
-  out[31] = h11 * 0.5773502691896258;
-  out[32] = h17 * 0.5773502691896258;
-  out[33] = h23 * 0.3333333333333334;
-  out[34] = h29 * 0.5773502691896258;
-  out[35] = h35 * 0.3333333333333334;
-  store_block<6, 1, 6, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (21|00): 31 recursion values, 16 accumulators, 18 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2100(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<3>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[4];
-      boys_eval_lds<3>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB x4 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x5 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x6 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x7 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x8 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x9 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x10 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x11 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x12 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x24 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x13 = s.PA[0] * x4 + s.WP[0] * x7 + z1 * (x0 - s.roz * x1);
-      acc0 += static_cast<double>(x13);
-      const TB x14 = s.PA[1] * x4 + s.WP[1] * x7;
-      acc1 += static_cast<double>(x14);
-      const TB x15 = s.PA[2] * x4 + s.WP[2] * x7;
-      acc2 += static_cast<double>(x15);
-      const TB x16 = s.PA[1] * x5 + s.WP[1] * x8 + z1 * (x0 - s.roz * x1);
-      acc3 += static_cast<double>(x16);
-      const TB x18 = s.PA[2] * x6 + s.WP[2] * x9 + z1 * (x0 - s.roz * x1);
-      acc5 += static_cast<double>(x18);
-      const TB x34 = s.PA[2] * x18 + s.WP[2] * x24 + z2 * (x6 - s.roz * x9);
-      acc15 += static_cast<double>(x34);
-      const TB x17 = s.PA[2] * x5 + s.WP[2] * x8;
-      acc4 += static_cast<double>(x17);
-      const TB x19 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x1 - s.roz * x2);
-      const TB x22 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x23 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x33 = s.PA[2] * x17 + s.WP[2] * x23 + z1 * (x5 - s.roz * x8);
-      acc14 += static_cast<double>(x33);
-      const TB x31 = s.PA[1] * x16 + s.WP[1] * x22 + z2 * (x5 - s.roz * x8);
-      acc12 += static_cast<double>(x31);
-      const TB x32 = s.PA[2] * x16 + s.WP[2] * x22;
-      acc13 += static_cast<double>(x32);
-      const TB x25 = s.PA[0] * x13 + s.WP[0] * x19 + z2 * (x4 - s.roz * x7);
-      acc6 += static_cast<double>(x25);
-      const TB x26 = s.PA[1] * x13 + s.WP[1] * x19;
-      acc7 += static_cast<double>(x26);
-      const TB x27 = s.PA[2] * x13 + s.WP[2] * x19;
-      acc8 += static_cast<double>(x27);
-      const TB x20 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x21 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x30 = s.PA[2] * x15 + s.WP[2] * x21 + z1 * (x4 - s.roz * x7);
-      acc11 += static_cast<double>(x30);
-      const TB x28 = s.PA[1] * x14 + s.WP[1] * x20 + z1 * (x4 - s.roz * x7);
-      acc9 += static_cast<double>(x28);
-      const TB x29 = s.PA[2] * x14 + s.WP[2] * x20;
-      acc10 += static_cast<double>(x29);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = h15 + AB[2] * h5;
-  const double h17 = h14 + AB[1] * h5;
-  const double h18 = h11 + AB[0] * h5;
-  const double h19 = h14 + AB[2] * h4;
-  const double h20 = h13 + AB[1] * h4;
-  const double h21 = h10 + AB[0] * h4;
-  const double h22 = h13 + AB[2] * h3;
-  const double h23 = h12 + AB[1] * h3;
-  const double h24 = h9 + AB[0] * h3;
-  const double h25 = h11 + AB[2] * h2;
-  const double h26 = h10 + AB[1] * h2;
-  const double h27 = h8 + AB[0] * h2;
-  const double h28 = h10 + AB[2] * h1;
-  const double h29 = h9 + AB[1] * h1;
-  const double h30 = h7 + AB[0] * h1;
-  const double h31 = h8 + AB[2] * h0;
-  const double h32 = h7 + AB[1] * h0;
-  const double h33 = h6 + AB[0] * h0;
-  double out[18];
-  out[0] = h33 * 0.5773502691896258;
-  out[1] = h32 * 0.5773502691896258;
-  out[2] = h31 * 0.5773502691896258;
-  out[3] = h30;
-  out[4] = h29;
-  out[5] = h28;
-  out[6] = h27;
-  out[7] = h26;
-  out[8] = h25;
-  out[9] = h24 * 0.5773502691896258;
-  out[10] = h23 * 0.5773502691896258;
-  out[11] = h22 * 0.5773502691896258;
-  out[12] = h21;
-  out[13] = h20;
-  out[14] = h19;
-  out[15] = h18 * 0.5773502691896258;
-  out[16] = h17 * 0.5773502691896258;
-  out[17] = h16 * 0.5773502691896258;
-  store_block<6, 3, 1, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (21|10): 98 recursion values, 48 accumulators, 54 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2110(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<4>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  double acc31 = 0.0;
-  double acc32 = 0.0;
-  double acc33 = 0.0;
-  double acc34 = 0.0;
-  double acc35 = 0.0;
-  double acc36 = 0.0;
-  double acc37 = 0.0;
-  double acc38 = 0.0;
-  double acc39 = 0.0;
-  double acc40 = 0.0;
-  double acc41 = 0.0;
-  double acc42 = 0.0;
-  double acc43 = 0.0;
-  double acc44 = 0.0;
-  double acc45 = 0.0;
-  double acc46 = 0.0;
-  double acc47 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[5];
-      boys_eval_lds<4>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x5 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x6 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x7 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x8 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x10 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x11 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x12 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x13 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x14 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x15 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x16 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x34 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x17 = s.PA[0] * x5 + s.WP[0] * x8 + z1 * (x0 - s.roz * x1);
-      const TB x18 = s.PA[1] * x5 + s.WP[1] * x8;
-      const TB x19 = s.PA[2] * x5 + s.WP[2] * x8;
-      const TB x20 = s.PA[1] * x6 + s.WP[1] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x22 = s.PA[2] * x7 + s.WP[2] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x21 = s.PA[2] * x6 + s.WP[2] * x9;
-      const TB x23 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x55 = s.QC[0] * x17 + s.WQ[0] * x23 + r2 * x8;
-      acc0 += static_cast<double>(x55);
-      const TB x71 = s.QC[1] * x17 + s.WQ[1] * x23;
-      acc16 += static_cast<double>(x71);
-      const TB x87 = s.QC[2] * x17 + s.WQ[2] * x23;
-      acc32 += static_cast<double>(x87);
-      const TB x24 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x56 = s.QC[0] * x18 + s.WQ[0] * x24 + r1 * x9;
-      acc1 += static_cast<double>(x56);
-      const TB x72 = s.QC[1] * x18 + s.WQ[1] * x24 + r1 * x8;
-      acc17 += static_cast<double>(x72);
-      const TB x88 = s.QC[2] * x18 + s.WQ[2] * x24;
-      acc33 += static_cast<double>(x88);
-      const TB x25 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x57 = s.QC[0] * x19 + s.WQ[0] * x25 + r1 * x10;
-      acc2 += static_cast<double>(x57);
-      const TB x73 = s.QC[1] * x19 + s.WQ[1] * x25;
-      acc18 += static_cast<double>(x73);
-      const TB x89 = s.QC[2] * x19 + s.WQ[2] * x25 + r1 * x8;
-      acc34 += static_cast<double>(x89);
-      const TB x40 = s.PA[2] * x19 + s.WP[2] * x25 + z1 * (x5 - s.roz * x8);
-      const TB x26 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x28 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x54 = s.PA[2] * x28 + s.WP[2] * x34 + z2 * (x10 - s.roz * x13);
-      const TB x44 = s.PA[2] * x22 + s.WP[2] * x28 + z2 * (x7 - s.roz * x10);
-      const TB x58 = s.QC[0] * x20 + s.WQ[0] * x26;
-      acc3 += static_cast<double>(x58);
-      const TB x60 = s.QC[0] * x22 + s.WQ[0] * x28;
-      acc5 += static_cast<double>(x60);
-      const TB x70 = s.QC[0] * x44 + s.WQ[0] * x54;
-      acc15 += static_cast<double>(x70);
-      const TB x74 = s.QC[1] * x20 + s.WQ[1] * x26 + r2 * x9;
-      acc19 += static_cast<double>(x74);
-      const TB x76 = s.QC[1] * x22 + s.WQ[1] * x28;
-      acc21 += static_cast<double>(x76);
-      const TB x92 = s.QC[2] * x22 + s.WQ[2] * x28 + r2 * x10;
-      acc37 += static_cast<double>(x92);
-      const TB x86 = s.QC[1] * x44 + s.WQ[1] * x54;
-      acc31 += static_cast<double>(x86);
-      const TB x102 = s.QC[2] * x44 + s.WQ[2] * x54 + r3 * x28;
-      acc47 += static_cast<double>(x102);
-      const TB x90 = s.QC[2] * x20 + s.WQ[2] * x26;
-      acc35 += static_cast<double>(x90);
-      const TB x27 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x75 = s.QC[1] * x21 + s.WQ[1] * x27 + r1 * x10;
-      acc20 += static_cast<double>(x75);
-      const TB x59 = s.QC[0] * x21 + s.WQ[0] * x27;
-      acc4 += static_cast<double>(x59);
-      const TB x91 = s.QC[2] * x21 + s.WQ[2] * x27 + r1 * x9;
-      acc36 += static_cast<double>(x91);
-      const TB x43 = s.PA[2] * x21 + s.WP[2] * x27 + z1 * (x6 - s.roz * x9);
-      const TB x41 = s.PA[1] * x20 + s.WP[1] * x26 + z2 * (x6 - s.roz * x9);
-      const TB x42 = s.PA[2] * x20 + s.WP[2] * x26;
-      const TB x29 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x2 - s.roz * x3);
-      const TB x32 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x33 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x53 = s.PA[2] * x27 + s.WP[2] * x33 + z1 * (x9 - s.roz * x12);
-      const TB x51 = s.PA[1] * x26 + s.WP[1] * x32 + z2 * (x9 - s.roz * x12);
-      const TB x52 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x67 = s.QC[0] * x41 + s.WQ[0] * x51;
-      acc12 += static_cast<double>(x67);
-      const TB x68 = s.QC[0] * x42 + s.WQ[0] * x52;
-      acc13 += static_cast<double>(x68);
-      const TB x69 = s.QC[0] * x43 + s.WQ[0] * x53;
-      acc14 += static_cast<double>(x69);
-      const TB x83 = s.QC[1] * x41 + s.WQ[1] * x51 + r3 * x26;
-      acc28 += static_cast<double>(x83);
-      const TB x99 = s.QC[2] * x41 + s.WQ[2] * x51;
-      acc44 += static_cast<double>(x99);
-      const TB x84 = s.QC[1] * x42 + s.WQ[1] * x52 + r2 * x27;
-      acc29 += static_cast<double>(x84);
-      const TB x100 = s.QC[2] * x42 + s.WQ[2] * x52 + r1 * x26;
-      acc45 += static_cast<double>(x100);
-      const TB x85 = s.QC[1] * x43 + s.WQ[1] * x53 + r1 * x28;
-      acc30 += static_cast<double>(x85);
-      const TB x101 = s.QC[2] * x43 + s.WQ[2] * x53 + r2 * x27;
-      acc46 += static_cast<double>(x101);
-      const TB x30 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x31 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x50 = s.PA[2] * x25 + s.WP[2] * x31 + z1 * (x8 - s.roz * x11);
-      const TB x66 = s.QC[0] * x40 + s.WQ[0] * x50 + r1 * x28;
-      acc11 += static_cast<double>(x66);
-      const TB x82 = s.QC[1] * x40 + s.WQ[1] * x50;
-      acc27 += static_cast<double>(x82);
-      const TB x98 = s.QC[2] * x40 + s.WQ[2] * x50 + r2 * x25;
-      acc43 += static_cast<double>(x98);
-      const TB x35 = s.PA[0] * x17 + s.WP[0] * x23 + z2 * (x5 - s.roz * x8);
-      const TB x38 = s.PA[1] * x18 + s.WP[1] * x24 + z1 * (x5 - s.roz * x8);
-      const TB x39 = s.PA[2] * x18 + s.WP[2] * x24;
-      const TB x36 = s.PA[1] * x17 + s.WP[1] * x23;
-      const TB x37 = s.PA[2] * x17 + s.WP[2] * x23;
-      const TB x45 = s.PA[0] * x23 + s.WP[0] * x29 + z2 * (x8 - s.roz * x11);
-      const TB x48 = s.PA[1] * x24 + s.WP[1] * x30 + z1 * (x8 - s.roz * x11);
-      const TB x64 = s.QC[0] * x38 + s.WQ[0] * x48 + r1 * x26;
-      acc9 += static_cast<double>(x64);
-      const TB x49 = s.PA[2] * x24 + s.WP[2] * x30;
-      const TB x65 = s.QC[0] * x39 + s.WQ[0] * x49 + r1 * x27;
-      acc10 += static_cast<double>(x65);
-      const TB x61 = s.QC[0] * x35 + s.WQ[0] * x45 + r3 * x23;
-      acc6 += static_cast<double>(x61);
-      const TB x77 = s.QC[1] * x35 + s.WQ[1] * x45;
-      acc22 += static_cast<double>(x77);
-      const TB x93 = s.QC[2] * x35 + s.WQ[2] * x45;
-      acc38 += static_cast<double>(x93);
-      const TB x80 = s.QC[1] * x38 + s.WQ[1] * x48 + r2 * x24;
-      acc25 += static_cast<double>(x80);
-      const TB x96 = s.QC[2] * x38 + s.WQ[2] * x48;
-      acc41 += static_cast<double>(x96);
-      const TB x81 = s.QC[1] * x39 + s.WQ[1] * x49 + r1 * x25;
-      acc26 += static_cast<double>(x81);
-      const TB x97 = s.QC[2] * x39 + s.WQ[2] * x49 + r1 * x24;
-      acc42 += static_cast<double>(x97);
-      const TB x46 = s.PA[1] * x23 + s.WP[1] * x29;
-      const TB x62 = s.QC[0] * x36 + s.WQ[0] * x46 + r2 * x24;
-      acc7 += static_cast<double>(x62);
-      const TB x47 = s.PA[2] * x23 + s.WP[2] * x29;
-      const TB x63 = s.QC[0] * x37 + s.WQ[0] * x47 + r2 * x25;
-      acc8 += static_cast<double>(x63);
-      const TB x78 = s.QC[1] * x36 + s.WQ[1] * x46 + r1 * x23;
-      acc23 += static_cast<double>(x78);
-      const TB x94 = s.QC[2] * x36 + s.WQ[2] * x46;
-      acc39 += static_cast<double>(x94);
-      const TB x95 = s.QC[2] * x37 + s.WQ[2] * x47 + r1 * x23;
-      acc40 += static_cast<double>(x95);
-      const TB x79 = s.QC[1] * x37 + s.WQ[1] * x47;
-      acc24 += static_cast<double>(x79);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-    accmax = fmax(accmax, fabs(acc31));
-    accmax = fmax(accmax, fabs(acc32));
-    accmax = fmax(accmax, fabs(acc33));
-    accmax = fmax(accmax, fabs(acc34));
-    accmax = fmax(accmax, fabs(acc35));
-    accmax = fmax(accmax, fabs(acc36));
-    accmax = fmax(accmax, fabs(acc37));
-    accmax = fmax(accmax, fabs(acc38));
-    accmax = fmax(accmax, fabs(acc39));
-    accmax = fmax(accmax, fabs(acc40));
-    accmax = fmax(accmax, fabs(acc41));
-    accmax = fmax(accmax, fabs(acc42));
-    accmax = fmax(accmax, fabs(acc43));
-    accmax = fmax(accmax, fabs(acc44));
-    accmax = fmax(accmax, fabs(acc45));
-    accmax = fmax(accmax, fabs(acc46));
-    accmax = fmax(accmax, fabs(acc47));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = acc31;
-  const double h32 = acc32;
-  const double h33 = acc33;
-  const double h34 = acc34;
-  const double h35 = acc35;
-  const double h36 = acc36;
-  const double h37 = acc37;
-  const double h38 = acc38;
-  const double h39 = acc39;
-  const double h40 = acc40;
-  const double h41 = acc41;
-  const double h42 = acc42;
-  const double h43 = acc43;
-  const double h44 = acc44;
-  const double h45 = acc45;
-  const double h46 = acc46;
-  const double h47 = acc47;
-  const double h48 = h47 + AB[2] * h37;
-  const double h49 = h31 + AB[2] * h21;
-  const double h50 = h15 + AB[2] * h5;
-  const double h51 = h46 + AB[1] * h37;
-  const double h52 = h30 + AB[1] * h21;
-  const double h53 = h14 + AB[1] * h5;
-  const double h54 = h43 + AB[0] * h37;
-  const double h55 = h27 + AB[0] * h21;
-  const double h56 = h11 + AB[0] * h5;
-  const double h57 = h46 + AB[2] * h36;
-  const double h58 = h30 + AB[2] * h20;
-  const double h59 = h14 + AB[2] * h4;
-  const double h60 = h45 + AB[1] * h36;
-  const double h61 = h29 + AB[1] * h20;
-  const double h62 = h13 + AB[1] * h4;
-  const double h63 = h42 + AB[0] * h36;
-  const double h64 = h26 + AB[0] * h20;
-  const double h65 = h10 + AB[0] * h4;
-  const double h66 = h45 + AB[2] * h35;
-  const double h67 = h29 + AB[2] * h19;
-  const double h68 = h13 + AB[2] * h3;
-  const double h69 = h44 + AB[1] * h35;
-  const double h70 = h28 + AB[1] * h19;
-  const double h71 = h12 + AB[1] * h3;
-  const double h72 = h41 + AB[0] * h35;
-  const double h73 = h25 + AB[0] * h19;
-  const double h74 = h9 + AB[0] * h3;
-  const double h75 = h43 + AB[2] * h34;
-  const double h76 = h27 + AB[2] * h18;
-  const double h77 = h11 + AB[2] * h2;
-  const double h78 = h42 + AB[1] * h34;
-  const double h79 = h26 + AB[1] * h18;
-  const double h80 = h10 + AB[1] * h2;
-  const double h81 = h40 + AB[0] * h34;
-  const double h82 = h24 + AB[0] * h18;
-  const double h83 = h8 + AB[0] * h2;
-  const double h84 = h42 + AB[2] * h33;
-  const double h85 = h26 + AB[2] * h17;
-  const double h86 = h10 + AB[2] * h1;
-  const double h87 = h41 + AB[1] * h33;
-  const double h88 = h25 + AB[1] * h17;
-  const double h89 = h9 + AB[1] * h1;
-  const double h90 = h39 + AB[0] * h33;
-  const double h91 = h23 + AB[0] * h17;
-  const double h92 = h7 + AB[0] * h1;
-  const double h93 = h40 + AB[2] * h32;
-  const double h94 = h24 + AB[2] * h16;
-  const double h95 = h8 + AB[2] * h0;
-  const double h96 = h39 + AB[1] * h32;
-  const double h97 = h23 + AB[1] * h16;
-  const double h98 = h7 + AB[1] * h0;
-  const double h99 = h38 + AB[0] * h32;
-  const double h100 = h22 + AB[0] * h16;
-  const double h101 = h6 + AB[0] * h0;
-  double out[54];
-  out[0] = h101 * 0.5773502691896258;
-  out[1] = h100 * 0.5773502691896258;
-  out[2] = h99 * 0.5773502691896258;
-  out[3] = h98 * 0.5773502691896258;
-  out[4] = h97 * 0.5773502691896258;
-  out[5] = h96 * 0.5773502691896258;
-  out[6] = h95 * 0.5773502691896258;
-  out[7] = h94 * 0.5773502691896258;
-  out[8] = h93 * 0.5773502691896258;
-  out[9] = h92;
-  out[10] = h91;
-  out[11] = h90;
-  out[12] = h89;
-  out[13] = h88;
-  out[14] = h87;
-  out[15] = h86;
-  out[16] = h85;
-  out[17] = h84;
-  out[18] = h83;
-  out[19] = h82;
-  out[20] = h81;
-  out[21] = h80;
-  out[22] = h79;
-  out[23] = h78;
-  out[24] = h77;
-  out[25] = h76;
-  out[26] = h75;
-  out[27] = h74 * 0.5773502691896258;
-  out[28] = h73 * 0.5773502691896258;
-  out[29] = h72 * 0.5773502691896258;
-  out[30] = h71 * 0.5773502691896258;
-  out[31] = h70 * 0.5773502691896258;
-  out[32] = h69 * 0.5773502691896258;
-  out[33] = h68 * 0.5773502691896258;
-  out[34] = h67 * 0.5773502691896258;
-  out[35] = h66 * 0.5773502691896258;
-  out[36] = h65;
-  out[37] = h64;
-  out[38] = h63;
-  out[39] = h62;
-  out[40] = h61;
-  out[41] = h60;
-  out[42] = h59;
-  out[43] = h58;
-  out[44] = h57;
-  out[45] = h56 * 0.5773502691896258;
-  out[46] = h55 * 0.5773502691896258;
-  out[47] = h54 * 0.5773502691896258;
-  out[48] = h53 * 0.5773502691896258;
-  out[49] = h52 * 0.5773502691896258;
-  out[50] = h51 * 0.5773502691896258;
-  out[51] = h50 * 0.5773502691896258;
-  out[52] = h49 * 0.5773502691896258;
-  out[53] = h48 * 0.5773502691896258;
-  store_block<6, 3, 3, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (21|11): 270 recursion values, 144 accumulators, 162 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2111(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<5>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  double acc31 = 0.0;
-  double acc32 = 0.0;
-  double acc33 = 0.0;
-  double acc34 = 0.0;
-  double acc35 = 0.0;
-  double acc36 = 0.0;
-  double acc37 = 0.0;
-  double acc38 = 0.0;
-  double acc39 = 0.0;
-  double acc40 = 0.0;
-  double acc41 = 0.0;
-  double acc42 = 0.0;
-  double acc43 = 0.0;
-  double acc44 = 0.0;
-  double acc45 = 0.0;
-  double acc46 = 0.0;
-  double acc47 = 0.0;
-  double acc48 = 0.0;
-  double acc49 = 0.0;
-  double acc50 = 0.0;
-  double acc51 = 0.0;
-  double acc52 = 0.0;
-  double acc53 = 0.0;
-  double acc54 = 0.0;
-  double acc55 = 0.0;
-  double acc56 = 0.0;
-  double acc57 = 0.0;
-  double acc58 = 0.0;
-  double acc59 = 0.0;
-  double acc60 = 0.0;
-  double acc61 = 0.0;
-  double acc62 = 0.0;
-  double acc63 = 0.0;
-  double acc64 = 0.0;
-  double acc65 = 0.0;
-  double acc66 = 0.0;
-  double acc67 = 0.0;
-  double acc68 = 0.0;
-  double acc69 = 0.0;
-  double acc70 = 0.0;
-  double acc71 = 0.0;
-  double acc72 = 0.0;
-  double acc73 = 0.0;
-  double acc74 = 0.0;
-  double acc75 = 0.0;
-  double acc76 = 0.0;
-  double acc77 = 0.0;
-  double acc78 = 0.0;
-  double acc79 = 0.0;
-  double acc80 = 0.0;
-  double acc81 = 0.0;
-  double acc82 = 0.0;
-  double acc83 = 0.0;
-  double acc84 = 0.0;
-  double acc85 = 0.0;
-  double acc86 = 0.0;
-  double acc87 = 0.0;
-  double acc88 = 0.0;
-  double acc89 = 0.0;
-  double acc90 = 0.0;
-  double acc91 = 0.0;
-  double acc92 = 0.0;
-  double acc93 = 0.0;
-  double acc94 = 0.0;
-  double acc95 = 0.0;
-  double acc96 = 0.0;
-  double acc97 = 0.0;
-  double acc98 = 0.0;
-  double acc99 = 0.0;
-  double acc100 = 0.0;
-  double acc101 = 0.0;
-  double acc102 = 0.0;
-  double acc103 = 0.0;
-  double acc104 = 0.0;
-  double acc105 = 0.0;
-  double acc106 = 0.0;
-  double acc107 = 0.0;
-  double acc108 = 0.0;
-  double acc109 = 0.0;
-  double acc110 = 0.0;
-  double acc111 = 0.0;
-  double acc112 = 0.0;
-  double acc113 = 0.0;
-  double acc114 = 0.0;
-  double acc115 = 0.0;
-  double acc116 = 0.0;
-  double acc117 = 0.0;
-  double acc118 = 0.0;
-  double acc119 = 0.0;
-  double acc120 = 0.0;
-  double acc121 = 0.0;
-  double acc122 = 0.0;
-  double acc123 = 0.0;
-  double acc124 = 0.0;
-  double acc125 = 0.0;
-  double acc126 = 0.0;
-  double acc127 = 0.0;
-  double acc128 = 0.0;
-  double acc129 = 0.0;
-  double acc130 = 0.0;
-  double acc131 = 0.0;
-  double acc132 = 0.0;
-  double acc133 = 0.0;
-  double acc134 = 0.0;
-  double acc135 = 0.0;
-  double acc136 = 0.0;
-  double acc137 = 0.0;
-  double acc138 = 0.0;
-  double acc139 = 0.0;
-  double acc140 = 0.0;
-  double acc141 = 0.0;
-  double acc142 = 0.0;
-  double acc143 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[6];
-      boys_eval_lds<5>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x6 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x7 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x8 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x9 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x10 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x11 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x12 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x13 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x14 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x15 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x16 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x17 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x18 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x19 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x20 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x44 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x21 = s.PA[0] * x6 + s.WP[0] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x22 = s.PA[1] * x6 + s.WP[1] * x9;
-      const TB x23 = s.PA[2] * x6 + s.WP[2] * x9;
-      const TB x24 = s.PA[1] * x7 + s.WP[1] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x26 = s.PA[2] * x8 + s.WP[2] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x25 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x27 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x28 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x29 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x30 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x54 = s.PA[2] * x26 + s.WP[2] * x32 + z2 * (x8 - s.roz * x11);
-      const TB x31 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x33 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x34 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x35 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x36 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x37 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x38 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x74 = s.PA[2] * x38 + s.WP[2] * x44 + z2 * (x14 - s.roz * x17);
-      const TB x39 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x3 - s.roz * x4);
-      const TB x42 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x43 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x73 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x13 - s.roz * x16);
-      const TB x71 = s.PA[1] * x36 + s.WP[1] * x42 + z2 * (x13 - s.roz * x16);
-      const TB x72 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x40 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x41 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x70 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x12 - s.roz * x15);
-      const TB x45 = s.PA[0] * x21 + s.WP[0] * x27 + z2 * (x6 - s.roz * x9);
-      const TB x46 = s.PA[1] * x21 + s.WP[1] * x27;
-      const TB x47 = s.PA[2] * x21 + s.WP[2] * x27;
-      const TB x48 = s.PA[1] * x22 + s.WP[1] * x28 + z1 * (x6 - s.roz * x9);
-      const TB x50 = s.PA[2] * x23 + s.WP[2] * x29 + z1 * (x6 - s.roz * x9);
-      const TB x49 = s.PA[2] * x22 + s.WP[2] * x28;
-      const TB x51 = s.PA[1] * x24 + s.WP[1] * x30 + z2 * (x7 - s.roz * x10);
-      const TB x53 = s.PA[2] * x25 + s.WP[2] * x31 + z1 * (x7 - s.roz * x10);
-      const TB x52 = s.PA[2] * x24 + s.WP[2] * x30;
-      const TB x55 = s.PA[0] * x27 + s.WP[0] * x33 + z2 * (x9 - s.roz * x12);
-      const TB x56 = s.PA[1] * x27 + s.WP[1] * x33;
-      const TB x57 = s.PA[2] * x27 + s.WP[2] * x33;
-      const TB x58 = s.PA[1] * x28 + s.WP[1] * x34 + z1 * (x9 - s.roz * x12);
-      const TB x59 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x60 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x9 - s.roz * x12);
-      const TB x61 = s.PA[1] * x30 + s.WP[1] * x36 + z2 * (x10 - s.roz * x13);
-      const TB x62 = s.PA[2] * x30 + s.WP[2] * x36;
-      const TB x63 = s.PA[2] * x31 + s.WP[2] * x37 + z1 * (x10 - s.roz * x13);
-      const TB x64 = s.PA[2] * x32 + s.WP[2] * x38 + z2 * (x11 - s.roz * x14);
-      const TB x65 = s.PA[0] * x33 + s.WP[0] * x39 + z2 * (x12 - s.roz * x15);
-      const TB x68 = s.PA[1] * x34 + s.WP[1] * x40 + z1 * (x12 - s.roz * x15);
-      const TB x69 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x66 = s.PA[1] * x33 + s.WP[1] * x39;
-      const TB x67 = s.PA[2] * x33 + s.WP[2] * x39;
-      const TB x75 = s.QC[0] * x9 + s.WQ[0] * x12 + r1 * x2;
-      const TB x76 = s.QC[0] * x10 + s.WQ[0] * x13;
-      const TB x77 = s.QC[0] * x11 + s.WQ[0] * x14;
-      const TB x78 = s.QC[0] * x21 + s.WQ[0] * x27 + r2 * x9;
-      acc0 += static_cast<double>(x78);
-      const TB x79 = s.QC[0] * x27 + s.WQ[0] * x33 + r2 * x12;
-      const TB x180 = s.QC[0] * x78 + s.WQ[0] * x79 + e1 * (x21 - s.roe * x27) + r2 * x75;
-      acc48 += static_cast<double>(x180);
-      const TB x196 = s.QC[1] * x78 + s.WQ[1] * x79;
-      acc64 += static_cast<double>(x196);
-      const TB x212 = s.QC[2] * x78 + s.WQ[2] * x79;
-      acc80 += static_cast<double>(x212);
-      const TB x80 = s.QC[0] * x22 + s.WQ[0] * x28 + r1 * x10;
-      acc1 += static_cast<double>(x80);
-      const TB x81 = s.QC[0] * x28 + s.WQ[0] * x34 + r1 * x13;
-      const TB x181 = s.QC[0] * x80 + s.WQ[0] * x81 + e1 * (x22 - s.roe * x28) + r1 * x76;
-      acc49 += static_cast<double>(x181);
-      const TB x197 = s.QC[1] * x80 + s.WQ[1] * x81 + r1 * x75;
-      acc65 += static_cast<double>(x197);
-      const TB x213 = s.QC[2] * x80 + s.WQ[2] * x81;
-      acc81 += static_cast<double>(x213);
-      const TB x82 = s.QC[0] * x23 + s.WQ[0] * x29 + r1 * x11;
-      acc2 += static_cast<double>(x82);
-      const TB x83 = s.QC[0] * x29 + s.WQ[0] * x35 + r1 * x14;
-      const TB x214 = s.QC[2] * x82 + s.WQ[2] * x83 + r1 * x75;
-      acc82 += static_cast<double>(x214);
-      const TB x182 = s.QC[0] * x82 + s.WQ[0] * x83 + e1 * (x23 - s.roe * x29) + r1 * x77;
-      acc50 += static_cast<double>(x182);
-      const TB x198 = s.QC[1] * x82 + s.WQ[1] * x83;
-      acc66 += static_cast<double>(x198);
-      const TB x84 = s.QC[0] * x24 + s.WQ[0] * x30;
-      acc3 += static_cast<double>(x84);
-      const TB x85 = s.QC[0] * x30 + s.WQ[0] * x36;
-      const TB x183 = s.QC[0] * x84 + s.WQ[0] * x85 + e1 * (x24 - s.roe * x30);
-      acc51 += static_cast<double>(x183);
-      const TB x199 = s.QC[1] * x84 + s.WQ[1] * x85 + r2 * x76;
-      acc67 += static_cast<double>(x199);
-      const TB x215 = s.QC[2] * x84 + s.WQ[2] * x85;
-      acc83 += static_cast<double>(x215);
-      const TB x86 = s.QC[0] * x25 + s.WQ[0] * x31;
-      acc4 += static_cast<double>(x86);
-      const TB x87 = s.QC[0] * x31 + s.WQ[0] * x37;
-      const TB x216 = s.QC[2] * x86 + s.WQ[2] * x87 + r1 * x76;
-      acc84 += static_cast<double>(x216);
-      const TB x184 = s.QC[0] * x86 + s.WQ[0] * x87 + e1 * (x25 - s.roe * x31);
-      acc52 += static_cast<double>(x184);
-      const TB x200 = s.QC[1] * x86 + s.WQ[1] * x87 + r1 * x77;
-      acc68 += static_cast<double>(x200);
-      const TB x88 = s.QC[0] * x26 + s.WQ[0] * x32;
-      acc5 += static_cast<double>(x88);
-      const TB x89 = s.QC[0] * x32 + s.WQ[0] * x38;
-      const TB x217 = s.QC[2] * x88 + s.WQ[2] * x89 + r2 * x77;
-      acc85 += static_cast<double>(x217);
-      const TB x185 = s.QC[0] * x88 + s.WQ[0] * x89 + e1 * (x26 - s.roe * x32);
-      acc53 += static_cast<double>(x185);
-      const TB x201 = s.QC[1] * x88 + s.WQ[1] * x89;
-      acc69 += static_cast<double>(x201);
-      const TB x90 = s.QC[0] * x45 + s.WQ[0] * x55 + r3 * x27;
-      acc6 += static_cast<double>(x90);
-      const TB x91 = s.QC[0] * x55 + s.WQ[0] * x65 + r3 * x33;
-      const TB x186 = s.QC[0] * x90 + s.WQ[0] * x91 + e1 * (x45 - s.roe * x55) + r3 * x79;
-      acc54 += static_cast<double>(x186);
-      const TB x202 = s.QC[1] * x90 + s.WQ[1] * x91;
-      acc70 += static_cast<double>(x202);
-      const TB x218 = s.QC[2] * x90 + s.WQ[2] * x91;
-      acc86 += static_cast<double>(x218);
-      const TB x92 = s.QC[0] * x46 + s.WQ[0] * x56 + r2 * x28;
-      acc7 += static_cast<double>(x92);
-      const TB x93 = s.QC[0] * x56 + s.WQ[0] * x66 + r2 * x34;
-      const TB x187 = s.QC[0] * x92 + s.WQ[0] * x93 + e1 * (x46 - s.roe * x56) + r2 * x81;
-      acc55 += static_cast<double>(x187);
-      const TB x203 = s.QC[1] * x92 + s.WQ[1] * x93 + r1 * x79;
-      acc71 += static_cast<double>(x203);
-      const TB x219 = s.QC[2] * x92 + s.WQ[2] * x93;
-      acc87 += static_cast<double>(x219);
-      const TB x94 = s.QC[0] * x47 + s.WQ[0] * x57 + r2 * x29;
-      acc8 += static_cast<double>(x94);
-      const TB x95 = s.QC[0] * x57 + s.WQ[0] * x67 + r2 * x35;
-      const TB x220 = s.QC[2] * x94 + s.WQ[2] * x95 + r1 * x79;
-      acc88 += static_cast<double>(x220);
-      const TB x188 = s.QC[0] * x94 + s.WQ[0] * x95 + e1 * (x47 - s.roe * x57) + r2 * x83;
-      acc56 += static_cast<double>(x188);
-      const TB x204 = s.QC[1] * x94 + s.WQ[1] * x95;
-      acc72 += static_cast<double>(x204);
-      const TB x96 = s.QC[0] * x48 + s.WQ[0] * x58 + r1 * x30;
-      acc9 += static_cast<double>(x96);
-      const TB x97 = s.QC[0] * x58 + s.WQ[0] * x68 + r1 * x36;
-      const TB x189 = s.QC[0] * x96 + s.WQ[0] * x97 + e1 * (x48 - s.roe * x58) + r1 * x85;
-      acc57 += static_cast<double>(x189);
-      const TB x205 = s.QC[1] * x96 + s.WQ[1] * x97 + r2 * x81;
-      acc73 += static_cast<double>(x205);
-      const TB x221 = s.QC[2] * x96 + s.WQ[2] * x97;
-      acc89 += static_cast<double>(x221);
-      const TB x98 = s.QC[0] * x49 + s.WQ[0] * x59 + r1 * x31;
-      acc10 += static_cast<double>(x98);
-      const TB x99 = s.QC[0] * x59 + s.WQ[0] * x69 + r1 * x37;
-      const TB x222 = s.QC[2] * x98 + s.WQ[2] * x99 + r1 * x81;
-      acc90 += static_cast<double>(x222);
-      const TB x190 = s.QC[0] * x98 + s.WQ[0] * x99 + e1 * (x49 - s.roe * x59) + r1 * x87;
-      acc58 += static_cast<double>(x190);
-      const TB x206 = s.QC[1] * x98 + s.WQ[1] * x99 + r1 * x83;
-      acc74 += static_cast<double>(x206);
-      const TB x100 = s.QC[0] * x50 + s.WQ[0] * x60 + r1 * x32;
-      acc11 += static_cast<double>(x100);
-      const TB x101 = s.QC[0] * x60 + s.WQ[0] * x70 + r1 * x38;
-      const TB x223 = s.QC[2] * x100 + s.WQ[2] * x101 + r2 * x83;
-      acc91 += static_cast<double>(x223);
-      const TB x191 = s.QC[0] * x100 + s.WQ[0] * x101 + e1 * (x50 - s.roe * x60) + r1 * x89;
-      acc59 += static_cast<double>(x191);
-      const TB x207 = s.QC[1] * x100 + s.WQ[1] * x101;
-      acc75 += static_cast<double>(x207);
-      const TB x102 = s.QC[0] * x51 + s.WQ[0] * x61;
-      acc12 += static_cast<double>(x102);
-      const TB x103 = s.QC[0] * x61 + s.WQ[0] * x71;
-      const TB x192 = s.QC[0] * x102 + s.WQ[0] * x103 + e1 * (x51 - s.roe * x61);
-      acc60 += static_cast<double>(x192);
-      const TB x208 = s.QC[1] * x102 + s.WQ[1] * x103 + r3 * x85;
-      acc76 += static_cast<double>(x208);
-      const TB x224 = s.QC[2] * x102 + s.WQ[2] * x103;
-      acc92 += static_cast<double>(x224);
-      const TB x104 = s.QC[0] * x52 + s.WQ[0] * x62;
-      acc13 += static_cast<double>(x104);
-      const TB x105 = s.QC[0] * x62 + s.WQ[0] * x72;
-      const TB x225 = s.QC[2] * x104 + s.WQ[2] * x105 + r1 * x85;
-      acc93 += static_cast<double>(x225);
-      const TB x193 = s.QC[0] * x104 + s.WQ[0] * x105 + e1 * (x52 - s.roe * x62);
-      acc61 += static_cast<double>(x193);
-      const TB x209 = s.QC[1] * x104 + s.WQ[1] * x105 + r2 * x87;
-      acc77 += static_cast<double>(x209);
-      const TB x106 = s.QC[0] * x53 + s.WQ[0] * x63;
-      acc14 += static_cast<double>(x106);
-      const TB x107 = s.QC[0] * x63 + s.WQ[0] * x73;
-      const TB x226 = s.QC[2] * x106 + s.WQ[2] * x107 + r2 * x87;
-      acc94 += static_cast<double>(x226);
-      const TB x194 = s.QC[0] * x106 + s.WQ[0] * x107 + e1 * (x53 - s.roe * x63);
-      acc62 += static_cast<double>(x194);
-      const TB x210 = s.QC[1] * x106 + s.WQ[1] * x107 + r1 * x89;
-      acc78 += static_cast<double>(x210);
-      const TB x108 = s.QC[0] * x54 + s.WQ[0] * x64;
-      acc15 += static_cast<double>(x108);
-      const TB x109 = s.QC[0] * x64 + s.WQ[0] * x74;
-      const TB x227 = s.QC[2] * x108 + s.WQ[2] * x109 + r3 * x89;
-      acc95 += static_cast<double>(x227);
-      const TB x195 = s.QC[0] * x108 + s.WQ[0] * x109 + e1 * (x54 - s.roe * x64);
-      acc63 += static_cast<double>(x195);
-      const TB x211 = s.QC[1] * x108 + s.WQ[1] * x109;
-      acc79 += static_cast<double>(x211);
-      const TB x110 = s.QC[1] * x9 + s.WQ[1] * x12;
-      const TB x111 = s.QC[1] * x10 + s.WQ[1] * x13 + r1 * x2;
-      const TB x147 = s.QC[2] * x11 + s.WQ[2] * x14 + r1 * x2;
-      const TB x112 = s.QC[1] * x11 + s.WQ[1] * x14;
-      const TB x113 = s.QC[1] * x21 + s.WQ[1] * x27;
-      acc16 += static_cast<double>(x113);
-      const TB x114 = s.QC[1] * x27 + s.WQ[1] * x33;
-      const TB x228 = s.QC[1] * x113 + s.WQ[1] * x114 + e1 * (x21 - s.roe * x27);
-      acc96 += static_cast<double>(x228);
-      const TB x244 = s.QC[2] * x113 + s.WQ[2] * x114;
-      acc112 += static_cast<double>(x244);
-      const TB x115 = s.QC[1] * x22 + s.WQ[1] * x28 + r1 * x9;
-      acc17 += static_cast<double>(x115);
-      const TB x116 = s.QC[1] * x28 + s.WQ[1] * x34 + r1 * x12;
-      const TB x229 = s.QC[1] * x115 + s.WQ[1] * x116 + e1 * (x22 - s.roe * x28) + r1 * x110;
-      acc97 += static_cast<double>(x229);
-      const TB x245 = s.QC[2] * x115 + s.WQ[2] * x116;
-      acc113 += static_cast<double>(x245);
-      const TB x117 = s.QC[1] * x23 + s.WQ[1] * x29;
-      acc18 += static_cast<double>(x117);
-      const TB x118 = s.QC[1] * x29 + s.WQ[1] * x35;
-      const TB x246 = s.QC[2] * x117 + s.WQ[2] * x118 + r1 * x110;
-      acc114 += static_cast<double>(x246);
-      const TB x230 = s.QC[1] * x117 + s.WQ[1] * x118 + e1 * (x23 - s.roe * x29);
-      acc98 += static_cast<double>(x230);
-      const TB x119 = s.QC[1] * x24 + s.WQ[1] * x30 + r2 * x10;
-      acc19 += static_cast<double>(x119);
-      const TB x120 = s.QC[1] * x30 + s.WQ[1] * x36 + r2 * x13;
-      const TB x231 = s.QC[1] * x119 + s.WQ[1] * x120 + e1 * (x24 - s.roe * x30) + r2 * x111;
-      acc99 += static_cast<double>(x231);
-      const TB x247 = s.QC[2] * x119 + s.WQ[2] * x120;
-      acc115 += static_cast<double>(x247);
-      const TB x121 = s.QC[1] * x25 + s.WQ[1] * x31 + r1 * x11;
-      acc20 += static_cast<double>(x121);
-      const TB x158 = s.QC[2] * x26 + s.WQ[2] * x32 + r2 * x11;
-      acc37 += static_cast<double>(x158);
-      const TB x122 = s.QC[1] * x31 + s.WQ[1] * x37 + r1 * x14;
-      const TB x248 = s.QC[2] * x121 + s.WQ[2] * x122 + r1 * x111;
-      acc116 += static_cast<double>(x248);
-      const TB x232 = s.QC[1] * x121 + s.WQ[1] * x122 + e1 * (x25 - s.roe * x31) + r1 * x112;
-      acc100 += static_cast<double>(x232);
-      const TB x159 = s.QC[2] * x32 + s.WQ[2] * x38 + r2 * x14;
-      const TB x265 = s.QC[2] * x158 + s.WQ[2] * x159 + e1 * (x26 - s.roe * x32) + r2 * x147;
-      acc133 += static_cast<double>(x265);
-      const TB x123 = s.QC[1] * x26 + s.WQ[1] * x32;
-      acc21 += static_cast<double>(x123);
-      const TB x124 = s.QC[1] * x32 + s.WQ[1] * x38;
-      const TB x233 = s.QC[1] * x123 + s.WQ[1] * x124 + e1 * (x26 - s.roe * x32);
-      acc101 += static_cast<double>(x233);
-      const TB x249 = s.QC[2] * x123 + s.WQ[2] * x124 + r2 * x112;
-      acc117 += static_cast<double>(x249);
-      const TB x125 = s.QC[1] * x45 + s.WQ[1] * x55;
-      acc22 += static_cast<double>(x125);
-      const TB x126 = s.QC[1] * x55 + s.WQ[1] * x65;
-      const TB x161 = s.QC[2] * x55 + s.WQ[2] * x65;
-      const TB x234 = s.QC[1] * x125 + s.WQ[1] * x126 + e1 * (x45 - s.roe * x55);
-      acc102 += static_cast<double>(x234);
-      const TB x250 = s.QC[2] * x125 + s.WQ[2] * x126;
-      acc118 += static_cast<double>(x250);
-      const TB x127 = s.QC[1] * x46 + s.WQ[1] * x56 + r1 * x27;
-      acc23 += static_cast<double>(x127);
-      const TB x128 = s.QC[1] * x56 + s.WQ[1] * x66 + r1 * x33;
-      const TB x163 = s.QC[2] * x56 + s.WQ[2] * x66;
-      const TB x235 = s.QC[1] * x127 + s.WQ[1] * x128 + e1 * (x46 - s.roe * x56) + r1 * x114;
-      acc103 += static_cast<double>(x235);
-      const TB x251 = s.QC[2] * x127 + s.WQ[2] * x128;
-      acc119 += static_cast<double>(x251);
-      const TB x129 = s.QC[1] * x47 + s.WQ[1] * x57;
-      acc24 += static_cast<double>(x129);
-      const TB x130 = s.QC[1] * x57 + s.WQ[1] * x67;
-      const TB x252 = s.QC[2] * x129 + s.WQ[2] * x130 + r1 * x114;
-      acc120 += static_cast<double>(x252);
-      const TB x236 = s.QC[1] * x129 + s.WQ[1] * x130 + e1 * (x47 - s.roe * x57);
-      acc104 += static_cast<double>(x236);
-      const TB x165 = s.QC[2] * x57 + s.WQ[2] * x67 + r1 * x33;
-      const TB x149 = s.QC[2] * x27 + s.WQ[2] * x33;
-      const TB x131 = s.QC[1] * x48 + s.WQ[1] * x58 + r2 * x28;
-      acc25 += static_cast<double>(x131);
-      const TB x132 = s.QC[1] * x58 + s.WQ[1] * x68 + r2 * x34;
-      const TB x167 = s.QC[2] * x58 + s.WQ[2] * x68;
-      const TB x237 = s.QC[1] * x131 + s.WQ[1] * x132 + e1 * (x48 - s.roe * x58) + r2 * x116;
-      acc105 += static_cast<double>(x237);
-      const TB x253 = s.QC[2] * x131 + s.WQ[2] * x132;
-      acc121 += static_cast<double>(x253);
-      const TB x133 = s.QC[1] * x49 + s.WQ[1] * x59 + r1 * x29;
-      acc26 += static_cast<double>(x133);
-      const TB x134 = s.QC[1] * x59 + s.WQ[1] * x69 + r1 * x35;
-      const TB x254 = s.QC[2] * x133 + s.WQ[2] * x134 + r1 * x116;
-      acc122 += static_cast<double>(x254);
-      const TB x238 = s.QC[1] * x133 + s.WQ[1] * x134 + e1 * (x49 - s.roe * x59) + r1 * x118;
-      acc106 += static_cast<double>(x238);
-      const TB x169 = s.QC[2] * x59 + s.WQ[2] * x69 + r1 * x34;
-      const TB x151 = s.QC[2] * x28 + s.WQ[2] * x34;
-      const TB x135 = s.QC[1] * x50 + s.WQ[1] * x60;
-      acc27 += static_cast<double>(x135);
-      const TB x136 = s.QC[1] * x60 + s.WQ[1] * x70;
-      const TB x255 = s.QC[2] * x135 + s.WQ[2] * x136 + r2 * x118;
-      acc123 += static_cast<double>(x255);
-      const TB x239 = s.QC[1] * x135 + s.WQ[1] * x136 + e1 * (x50 - s.roe * x60);
-      acc107 += static_cast<double>(x239);
-      const TB x171 = s.QC[2] * x60 + s.WQ[2] * x70 + r2 * x35;
-      const TB x153 = s.QC[2] * x29 + s.WQ[2] * x35 + r1 * x12;
-      const TB x145 = s.QC[2] * x9 + s.WQ[2] * x12;
-      const TB x152 = s.QC[2] * x23 + s.WQ[2] * x29 + r1 * x9;
-      acc34 += static_cast<double>(x152);
-      const TB x262 = s.QC[2] * x152 + s.WQ[2] * x153 + e1 * (x23 - s.roe * x29) + r1 * x145;
-      acc130 += static_cast<double>(x262);
-      const TB x170 = s.QC[2] * x50 + s.WQ[2] * x60 + r2 * x29;
-      acc43 += static_cast<double>(x170);
-      const TB x271 = s.QC[2] * x170 + s.WQ[2] * x171 + e1 * (x50 - s.roe * x60) + r2 * x153;
-      acc139 += static_cast<double>(x271);
-      const TB x137 = s.QC[1] * x51 + s.WQ[1] * x61 + r3 * x30;
-      acc28 += static_cast<double>(x137);
-      const TB x138 = s.QC[1] * x61 + s.WQ[1] * x71 + r3 * x36;
-      const TB x173 = s.QC[2] * x61 + s.WQ[2] * x71;
-      const TB x240 = s.QC[1] * x137 + s.WQ[1] * x138 + e1 * (x51 - s.roe * x61) + r3 * x120;
-      acc108 += static_cast<double>(x240);
-      const TB x256 = s.QC[2] * x137 + s.WQ[2] * x138;
-      acc124 += static_cast<double>(x256);
-      const TB x139 = s.QC[1] * x52 + s.WQ[1] * x62 + r2 * x31;
-      acc29 += static_cast<double>(x139);
-      const TB x140 = s.QC[1] * x62 + s.WQ[1] * x72 + r2 * x37;
-      const TB x257 = s.QC[2] * x139 + s.WQ[2] * x140 + r1 * x120;
-      acc125 += static_cast<double>(x257);
-      const TB x241 = s.QC[1] * x139 + s.WQ[1] * x140 + e1 * (x52 - s.roe * x62) + r2 * x122;
-      acc109 += static_cast<double>(x241);
-      const TB x175 = s.QC[2] * x62 + s.WQ[2] * x72 + r1 * x36;
-      const TB x155 = s.QC[2] * x30 + s.WQ[2] * x36;
-      const TB x141 = s.QC[1] * x53 + s.WQ[1] * x63 + r1 * x32;
-      acc30 += static_cast<double>(x141);
-      const TB x178 = s.QC[2] * x54 + s.WQ[2] * x64 + r3 * x32;
-      acc47 += static_cast<double>(x178);
-      const TB x142 = s.QC[1] * x63 + s.WQ[1] * x73 + r1 * x38;
-      const TB x258 = s.QC[2] * x141 + s.WQ[2] * x142 + r2 * x122;
-      acc126 += static_cast<double>(x258);
-      const TB x242 = s.QC[1] * x141 + s.WQ[1] * x142 + e1 * (x53 - s.roe * x63) + r1 * x124;
-      acc110 += static_cast<double>(x242);
-      const TB x177 = s.QC[2] * x63 + s.WQ[2] * x73 + r2 * x37;
-      const TB x157 = s.QC[2] * x31 + s.WQ[2] * x37 + r1 * x13;
-      const TB x146 = s.QC[2] * x10 + s.WQ[2] * x13;
-      const TB x156 = s.QC[2] * x25 + s.WQ[2] * x31 + r1 * x10;
-      acc36 += static_cast<double>(x156);
-      const TB x264 = s.QC[2] * x156 + s.WQ[2] * x157 + e1 * (x25 - s.roe * x31) + r1 * x146;
-      acc132 += static_cast<double>(x264);
-      const TB x176 = s.QC[2] * x53 + s.WQ[2] * x63 + r2 * x31;
-      acc46 += static_cast<double>(x176);
-      const TB x274 = s.QC[2] * x176 + s.WQ[2] * x177 + e1 * (x53 - s.roe * x63) + r2 * x157;
-      acc142 += static_cast<double>(x274);
-      const TB x179 = s.QC[2] * x64 + s.WQ[2] * x74 + r3 * x38;
-      const TB x275 = s.QC[2] * x178 + s.WQ[2] * x179 + e1 * (x54 - s.roe * x64) + r3 * x159;
-      acc143 += static_cast<double>(x275);
-      const TB x144 = s.QC[1] * x64 + s.WQ[1] * x74;
-      const TB x143 = s.QC[1] * x54 + s.WQ[1] * x64;
-      acc31 += static_cast<double>(x143);
-      const TB x243 = s.QC[1] * x143 + s.WQ[1] * x144 + e1 * (x54 - s.roe * x64);
-      acc111 += static_cast<double>(x243);
-      const TB x259 = s.QC[2] * x143 + s.WQ[2] * x144 + r3 * x124;
-      acc127 += static_cast<double>(x259);
-      const TB x148 = s.QC[2] * x21 + s.WQ[2] * x27;
-      acc32 += static_cast<double>(x148);
-      const TB x260 = s.QC[2] * x148 + s.WQ[2] * x149 + e1 * (x21 - s.roe * x27);
-      acc128 += static_cast<double>(x260);
-      const TB x164 = s.QC[2] * x47 + s.WQ[2] * x57 + r1 * x27;
-      acc40 += static_cast<double>(x164);
-      const TB x268 = s.QC[2] * x164 + s.WQ[2] * x165 + e1 * (x47 - s.roe * x57) + r1 * x149;
-      acc136 += static_cast<double>(x268);
-      const TB x150 = s.QC[2] * x22 + s.WQ[2] * x28;
-      acc33 += static_cast<double>(x150);
-      const TB x261 = s.QC[2] * x150 + s.WQ[2] * x151 + e1 * (x22 - s.roe * x28);
-      acc129 += static_cast<double>(x261);
-      const TB x168 = s.QC[2] * x49 + s.WQ[2] * x59 + r1 * x28;
-      acc42 += static_cast<double>(x168);
-      const TB x270 = s.QC[2] * x168 + s.WQ[2] * x169 + e1 * (x49 - s.roe * x59) + r1 * x151;
-      acc138 += static_cast<double>(x270);
-      const TB x154 = s.QC[2] * x24 + s.WQ[2] * x30;
-      acc35 += static_cast<double>(x154);
-      const TB x263 = s.QC[2] * x154 + s.WQ[2] * x155 + e1 * (x24 - s.roe * x30);
-      acc131 += static_cast<double>(x263);
-      const TB x174 = s.QC[2] * x52 + s.WQ[2] * x62 + r1 * x30;
-      acc45 += static_cast<double>(x174);
-      const TB x273 = s.QC[2] * x174 + s.WQ[2] * x175 + e1 * (x52 - s.roe * x62) + r1 * x155;
-      acc141 += static_cast<double>(x273);
-      const TB x160 = s.QC[2] * x45 + s.WQ[2] * x55;
-      acc38 += static_cast<double>(x160);
-      const TB x266 = s.QC[2] * x160 + s.WQ[2] * x161 + e1 * (x45 - s.roe * x55);
-      acc134 += static_cast<double>(x266);
-      const TB x162 = s.QC[2] * x46 + s.WQ[2] * x56;
-      acc39 += static_cast<double>(x162);
-      const TB x267 = s.QC[2] * x162 + s.WQ[2] * x163 + e1 * (x46 - s.roe * x56);
-      acc135 += static_cast<double>(x267);
-      const TB x166 = s.QC[2] * x48 + s.WQ[2] * x58;
-      acc41 += static_cast<double>(x166);
-      const TB x269 = s.QC[2] * x166 + s.WQ[2] * x167 + e1 * (x48 - s.roe * x58);
-      acc137 += static_cast<double>(x269);
-      const TB x172 = s.QC[2] * x51 + s.WQ[2] * x61;
-      acc44 += static_cast<double>(x172);
-      const TB x272 = s.QC[2] * x172 + s.WQ[2] * x173 + e1 * (x51 - s.roe * x61);
-      acc140 += static_cast<double>(x272);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-    accmax = fmax(accmax, fabs(acc31));
-    accmax = fmax(accmax, fabs(acc32));
-    accmax = fmax(accmax, fabs(acc33));
-    accmax = fmax(accmax, fabs(acc34));
-    accmax = fmax(accmax, fabs(acc35));
-    accmax = fmax(accmax, fabs(acc36));
-    accmax = fmax(accmax, fabs(acc37));
-    accmax = fmax(accmax, fabs(acc38));
-    accmax = fmax(accmax, fabs(acc39));
-    accmax = fmax(accmax, fabs(acc40));
-    accmax = fmax(accmax, fabs(acc41));
-    accmax = fmax(accmax, fabs(acc42));
-    accmax = fmax(accmax, fabs(acc43));
-    accmax = fmax(accmax, fabs(acc44));
-    accmax = fmax(accmax, fabs(acc45));
-    accmax = fmax(accmax, fabs(acc46));
-    accmax = fmax(accmax, fabs(acc47));
-    accmax = fmax(accmax, fabs(acc48));
-    accmax = fmax(accmax, fabs(acc49));
-    accmax = fmax(accmax, fabs(acc50));
-    accmax = fmax(accmax, fabs(acc51));
-    accmax = fmax(accmax, fabs(acc52));
-    accmax = fmax(accmax, fabs(acc53));
-    accmax = fmax(accmax, fabs(acc54));
-    accmax = fmax(accmax, fabs(acc55));
-    accmax = fmax(accmax, fabs(acc56));
-    accmax = fmax(accmax, fabs(acc57));
-    accmax = fmax(accmax, fabs(acc58));
-    accmax = fmax(accmax, fabs(acc59));
-    accmax = fmax(accmax, fabs(acc60));
-    accmax = fmax(accmax, fabs(acc61));
-    accmax = fmax(accmax, fabs(acc62));
-    accmax = fmax(accmax, fabs(acc63));
-    accmax = fmax(accmax, fabs(acc64));
-    accmax = fmax(accmax, fabs(acc65));
-    accmax = fmax(accmax, fabs(acc66));
-    accmax = fmax(accmax, fabs(acc67));
-    accmax = fmax(accmax, fabs(acc68));
-    accmax = fmax(accmax, fabs(acc69));
-    accmax = fmax(accmax, fabs(acc70));
-    accmax = fmax(accmax, fabs(acc71));
-    accmax = fmax(accmax, fabs(acc72));
-    accmax = fmax(accmax, fabs(acc73));
-    accmax = fmax(accmax, fabs(acc74));
-    accmax = fmax(accmax, fabs(acc75));
-    accmax = fmax(accmax, fabs(acc76));
-    accmax = fmax(accmax, fabs(acc77));
-    accmax = fmax(accmax, fabs(acc78));
-    accmax = fmax(accmax, fabs(acc79));
-    accmax = fmax(accmax, fabs(acc80));
-    accmax = fmax(accmax, fabs(acc81));
-    accmax = fmax(accmax, fabs(acc82));
-    accmax = fmax(accmax, fabs(acc83));
-    accmax = fmax(accmax, fabs(acc84));
-    accmax = fmax(accmax, fabs(acc85));
-    accmax = fmax(accmax, fabs(acc86));
-    accmax = fmax(accmax, fabs(acc87));
-    accmax = fmax(accmax, fabs(acc88));
-    accmax = fmax(accmax, fabs(acc89));
-    accmax = fmax(accmax, fabs(acc90));
-    accmax = fmax(accmax, fabs(acc91));
-    accmax = fmax(accmax, fabs(acc92));
-    accmax = fmax(accmax, fabs(acc93));
-    accmax = fmax(accmax, fabs(acc94));
-    accmax = fmax(accmax, fabs(acc95));
-    accmax = fmax(accmax, fabs(acc96));
-    accmax = fmax(accmax, fabs(acc97));
-    accmax = fmax(accmax, fabs(acc98));
-    accmax = fmax(accmax, fabs(acc99));
-    accmax = fmax(accmax, fabs(acc100));
-    accmax = fmax(accmax, fabs(acc101));
-    accmax = fmax(accmax, fabs(acc102));
-    accmax = fmax(accmax, fabs(acc103));
-    accmax = fmax(accmax, fabs(acc104));
-    accmax = fmax(accmax, fabs(acc105));
-    accmax = fmax(accmax, fabs(acc106));
-    accmax = fmax(accmax, fabs(acc107));
-    accmax = fmax(accmax, fabs(acc108));
-    accmax = fmax(accmax, fabs(acc109));
-    accmax = fmax(accmax, fabs(acc110));
-    accmax = fmax(accmax, fabs(acc111));
-    accmax = fmax(accmax, fabs(acc112));
-    accmax = fmax(accmax, fabs(acc113));
-    accmax = fmax(accmax, fabs(acc114));
-    accmax = fmax(accmax, fabs(acc115));
-    accmax = fmax(accmax, fabs(acc116));
-    accmax = fmax(accmax, fabs(acc117));
-    accmax = fmax(accmax, fabs(acc118));
-    accmax = fmax(accmax, fabs(acc119));
-    accmax = fmax(accmax, fabs(acc120));
-    accmax = fmax(accmax, fabs(acc121));
-    accmax = fmax(accmax, fabs(acc122));
-    accmax = fmax(accmax, fabs(acc123));
-    accmax = fmax(accmax, fabs(acc124));
-    accmax = fmax(accmax, fabs(acc125));
-    accmax = fmax(accmax, fabs(acc126));
-    accmax = fmax(accmax, fabs(acc127));
-    accmax = fmax(accmax, fabs(acc128));
-    accmax = fmax(accmax, fabs(acc129));
-    accmax = fmax(accmax, fabs(acc130));
-    accmax = fmax(accmax, fabs(acc131));
-    accmax = fmax(accmax, fabs(acc132));
-    accmax = fmax(accmax, fabs(acc133));
-    accmax = fmax(accmax, fabs(acc134));
-    accmax = fmax(accmax, fabs(acc135));
-    accmax = fmax(accmax, fabs(acc136));
-    accmax = fmax(accmax, fabs(acc137));
-    accmax = fmax(accmax, fabs(acc138));
-    accmax = fmax(accmax, fabs(acc139));
-    accmax = fmax(accmax, fabs(acc140));
-    accmax = fmax(accmax, fabs(acc141));
-    accmax = fmax(accmax, fabs(acc142));
-    accmax = fmax(accmax, fabs(acc143));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = acc31;
-  const double h32 = acc32;
-  const double h33 = acc33;
-  const double h34 = acc34;
-  const double h35 = acc35;
-  const double h36 = acc36;
-  const double h37 = acc37;
-  const double h38 = acc38;
-  const double h39 = acc39;
-  const double h40 = acc40;
-  const double h41 = acc41;
-  const double h42 = acc42;
-  const double h43 = acc43;
-  const double h44 = acc44;
-  const double h45 = acc45;
-  const double h46 = acc46;
-  const double h47 = acc47;
-  const double h48 = acc48;
-  const double h49 = acc49;
-  const double h50 = acc50;
-  const double h51 = acc51;
-  const double h52 = acc52;
-  const double h53 = acc53;
-  const double h54 = acc54;
-  const double h55 = acc55;
-  const double h56 = acc56;
-  const double h57 = acc57;
-  const double h58 = acc58;
-  const double h59 = acc59;
-  const double h60 = acc60;
-  const double h61 = acc61;
-  const double h62 = acc62;
-  const double h63 = acc63;
-  const double h64 = acc64;
-  const double h65 = acc65;
-  const double h66 = acc66;
-  const double h67 = acc67;
-  const double h68 = acc68;
-  const double h69 = acc69;
-  const double h70 = acc70;
-  const double h71 = acc71;
-  const double h72 = acc72;
-  const double h73 = acc73;
-  const double h74 = acc74;
-  const double h75 = acc75;
-  const double h76 = acc76;
-  const double h77 = acc77;
-  const double h78 = acc78;
-  const double h79 = acc79;
-  const double h80 = acc80;
-  const double h81 = acc81;
-  const double h82 = acc82;
-  const double h83 = acc83;
-  const double h84 = acc84;
-  const double h85 = acc85;
-  const double h86 = acc86;
-  const double h87 = acc87;
-  const double h88 = acc88;
-  const double h89 = acc89;
-  const double h90 = acc90;
-  const double h91 = acc91;
-  const double h92 = acc92;
-  const double h93 = acc93;
-  const double h94 = acc94;
-  const double h95 = acc95;
-  const double h96 = acc96;
-  const double h97 = acc97;
-  const double h98 = acc98;
-  const double h99 = acc99;
-  const double h100 = acc100;
-  const double h101 = acc101;
-  const double h102 = acc102;
-  const double h103 = acc103;
-  const double h104 = acc104;
-  const double h105 = acc105;
-  const double h106 = acc106;
-  const double h107 = acc107;
-  const double h108 = acc108;
-  const double h109 = acc109;
-  const double h110 = acc110;
-  const double h111 = acc111;
-  const double h112 = acc112;
-  const double h113 = acc113;
-  const double h114 = acc114;
-  const double h115 = acc115;
-  const double h116 = acc116;
-  const double h117 = acc117;
-  const double h118 = acc118;
-  const double h119 = acc119;
-  const double h120 = acc120;
-  const double h121 = acc121;
-  const double h122 = acc122;
-  const double h123 = acc123;
-  const double h124 = acc124;
-  const double h125 = acc125;
-  const double h126 = acc126;
-  const double h127 = acc127;
-  const double h128 = acc128;
-  const double h129 = acc129;
-  const double h130 = acc130;
-  const double h131 = acc131;
-  const double h132 = acc132;
-  const double h133 = acc133;
-  const double h134 = acc134;
-  const double h135 = acc135;
-  const double h136 = acc136;
-  const double h137 = acc137;
-  const double h138 = acc138;
-  const double h139 = acc139;
-  const double h140 = acc140;
-  const double h141 = acc141;
-  const double h142 = acc142;
-  const double h143 = acc143;
-  const double h144 = h133 + CD[2] * h37;
-  const double h145 = h117 + CD[1] * h37;
-  const double h146 = h85 + CD[0] * h37;
-  const double h147 = h117 + CD[2] * h21;
-  const double h148 = h101 + CD[1] * h21;
-  const double h149 = h69 + CD[0] * h21;
-  const double h150 = h85 + CD[2] * h5;
-  const double h151 = h69 + CD[1] * h5;
-  const double h152 = h53 + CD[0] * h5;
-  const double h153 = h143 + CD[2] * h47;
-  const double h154 = h127 + CD[1] * h47;
-  const double h155 = h95 + CD[0] * h47;
-  const double h156 = h127 + CD[2] * h31;
-  const double h157 = h111 + CD[1] * h31;
-  const double h158 = h79 + CD[0] * h31;
-  const double h159 = h95 + CD[2] * h15;
-  const double h160 = h79 + CD[1] * h15;
-  const double h161 = h63 + CD[0] * h15;
-  const double h162 = h132 + CD[2] * h36;
-  const double h163 = h116 + CD[1] * h36;
-  const double h164 = h84 + CD[0] * h36;
-  const double h165 = h116 + CD[2] * h20;
-  const double h166 = h100 + CD[1] * h20;
-  const double h167 = h68 + CD[0] * h20;
-  const double h168 = h84 + CD[2] * h4;
-  const double h169 = h68 + CD[1] * h4;
-  const double h170 = h52 + CD[0] * h4;
-  const double h171 = h142 + CD[2] * h46;
-  const double h172 = h126 + CD[1] * h46;
-  const double h173 = h94 + CD[0] * h46;
-  const double h174 = h126 + CD[2] * h30;
-  const double h175 = h110 + CD[1] * h30;
-  const double h176 = h78 + CD[0] * h30;
-  const double h177 = h94 + CD[2] * h14;
-  const double h178 = h78 + CD[1] * h14;
-  const double h179 = h62 + CD[0] * h14;
-  const double h180 = h131 + CD[2] * h35;
-  const double h181 = h115 + CD[1] * h35;
-  const double h182 = h83 + CD[0] * h35;
-  const double h183 = h115 + CD[2] * h19;
-  const double h184 = h99 + CD[1] * h19;
-  const double h185 = h67 + CD[0] * h19;
-  const double h186 = h83 + CD[2] * h3;
-  const double h187 = h67 + CD[1] * h3;
-  const double h188 = h51 + CD[0] * h3;
-  const double h189 = h141 + CD[2] * h45;
-  const double h190 = h125 + CD[1] * h45;
-  const double h191 = h93 + CD[0] * h45;
-  const double h192 = h125 + CD[2] * h29;
-  const double h193 = h109 + CD[1] * h29;
-  const double h194 = h77 + CD[0] * h29;
-  const double h195 = h93 + CD[2] * h13;
-  const double h196 = h77 + CD[1] * h13;
-  const double h197 = h61 + CD[0] * h13;
-  const double h198 = h140 + CD[2] * h44;
-  const double h199 = h124 + CD[1] * h44;
-  const double h200 = h92 + CD[0] * h44;
-  const double h201 = h124 + CD[2] * h28;
-  const double h202 = h108 + CD[1] * h28;
-  const double h203 = h76 + CD[0] * h28;
-  const double h204 = h92 + CD[2] * h12;
-  const double h205 = h76 + CD[1] * h12;
-  const double h206 = h60 + CD[0] * h12;
-  const double h207 = h130 + CD[2] * h34;
-  const double h208 = h114 + CD[1] * h34;
-  const double h209 = h82 + CD[0] * h34;
-  const double h210 = h114 + CD[2] * h18;
-  const double h211 = h98 + CD[1] * h18;
-  const double h212 = h66 + CD[0] * h18;
-  const double h213 = h82 + CD[2] * h2;
-  const double h214 = h66 + CD[1] * h2;
-  const double h215 = h50 + CD[0] * h2;
-  const double h216 = h139 + CD[2] * h43;
-  const double h217 = h123 + CD[1] * h43;
-  const double h218 = h91 + CD[0] * h43;
-  const double h219 = h123 + CD[2] * h27;
-  const double h220 = h107 + CD[1] * h27;
-  const double h221 = h75 + CD[0] * h27;
-  const double h222 = h91 + CD[2] * h11;
-  const double h223 = h75 + CD[1] * h11;
-  const double h224 = h59 + CD[0] * h11;
-  const double h225 = h129 + CD[2] * h33;
-  const double h226 = h113 + CD[1] * h33;
-  const double h227 = h81 + CD[0] * h33;
-  const double h228 = h113 + CD[2] * h17;
-  const double h229 = h97 + CD[1] * h17;
-  const double h230 = h65 + CD[0] * h17;
-  const double h231 = h81 + CD[2] * h1;
-  const double h232 = h65 + CD[1] * h1;
-  const double h233 = h49 + CD[0] * h1;
-  const double h234 = h138 + CD[2] * h42;
-  const double h235 = h122 + CD[1] * h42;
-  const double h236 = h90 + CD[0] * h42;
-  const double h237 = h122 + CD[2] * h26;
-  const double h238 = h106 + CD[1] * h26;
-  const double h239 = h74 + CD[0] * h26;
-  const double h240 = h90 + CD[2] * h10;
-  const double h241 = h74 + CD[1] * h10;
-  const double h242 = h58 + CD[0] * h10;
-  const double h243 = h137 + CD[2] * h41;
-  const double h244 = h121 + CD[1] * h41;
-  const double h245 = h89 + CD[0] * h41;
-  const double h246 = h121 + CD[2] * h25;
-  const double h247 = h105 + CD[1] * h25;
-  const double h248 = h73 + CD[0] * h25;
-  const double h249 = h89 + CD[2] * h9;
-  const double h250 = h73 + CD[1] * h9;
-  const double h251 = h57 + CD[0] * h9;
-  const double h252 = h128 + CD[2] * h32;
-  const double h253 = h112 + CD[1] * h32;
-  const double h254 = h80 + CD[0] * h32;
-  const double h255 = h112 + CD[2] * h16;
-  const double h256 = h96 + CD[1] * h16;
-  const double h257 = h64 + CD[0] * h16;
-  const double h258 = h80 + CD[2] * h0;
-  const double h259 = h64 + CD[1] * h0;
-  const double h260 = h48 + CD[0] * h0;
-  const double h261 = h136 + CD[2] * h40;
-  const double h262 = h120 + CD[1] * h40;
-  const double h263 = h88 + CD[0] * h40;
-  const double h264 = h120 + CD[2] * h24;
-  const double h265 = h104 + CD[1] * h24;
-  const double h266 = h72 + CD[0] * h24;
-  const double h267 = h88 + CD[2] * h8;
-  const double h268 = h72 + CD[1] * h8;
-  const double h269 = h56 + CD[0] * h8;
-  const double h270 = h135 + CD[2] * h39;
-  const double h271 = h119 + CD[1] * h39;
-  const double h272 = h87 + CD[0] * h39;
-  const double h273 = h119 + CD[2] * h23;
-  const double h274 = h103 + CD[1] * h23;
-  const double h275 = h71 + CD[0] * h23;
-  const double h276 = h87 + CD[2] * h7;
-  const double h277 = h71 + CD[1] * h7;
-  const double h278 = h55 + CD[0] * h7;
-  const double h279 = h134 + CD[2] * h38;
-  const double h280 = h118 + CD[1] * h38;
-  const double h281 = h86 + CD[0] * h38;
-  const double h282 = h118 + CD[2] * h22;
-  const double h283 = h102 + CD[1] * h22;
-  const double h284 = h70 + CD[0] * h22;
-  const double h285 = h86 + CD[2] * h6;
-  const double h286 = h70 + CD[1] * h6;
-  const double h287 = h54 + CD[0] * h6;
-  const double h288 = h153 + AB[2] * h144;
-  const double h289 = h154 + AB[2] * h145;
-  const double h290 = h155 + AB[2] * h146;
-  const double h291 = h156 + AB[2] * h147;
-  const double h292 = h157 + AB[2] * h148;
-  const double h293 = h158 + AB[2] * h149;
-  const double h294 = h159 + AB[2] * h150;
-  const double h295 = h160 + AB[2] * h151;
-  const double h296 = h161 + AB[2] * h152;
-  const double h297 = h171 + AB[1] * h144;
-  const double h298 = h172 + AB[1] * h145;
-  const double h299 = h173 + AB[1] * h146;
-  const double h300 = h174 + AB[1] * h147;
-  const double h301 = h175 + AB[1] * h148;
-  const double h302 = h176 + AB[1] * h149;
-  const double h303 = h177 + AB[1] * h150;
-  const double h304 = h178 + AB[1] * h151;
-  const double h305 = h179 + AB[1] * h152;
-  const double h306 = h216 + AB[0] * h144;
-  const double h307 = h217 + AB[0] * h145;
-  const double h308 = h218 + AB[0] * h146;
-  const double h309 = h219 + AB[0] * h147;
-  const double h310 = h220 + AB[0] * h148;
-  const double h311 = h221 + AB[0] * h149;
-  const double h312 = h222 + AB[0] * h150;
-  const double h313 = h223 + AB[0] * h151;
-  const double h314 = h224 + AB[0] * h152;
-  const double h315 = h171 + AB[2] * h162;
-  const double h316 = h172 + AB[2] * h163;
-  const double h317 = h173 + AB[2] * h164;
-  const double h318 = h174 + AB[2] * h165;
-  const double h319 = h175 + AB[2] * h166;
-  const double h320 = h176 + AB[2] * h167;
-  const double h321 = h177 + AB[2] * h168;
-  const double h322 = h178 + AB[2] * h169;
-  const double h323 = h179 + AB[2] * h170;
-  const double h324 = h189 + AB[1] * h162;
-  const double h325 = h190 + AB[1] * h163;
-  const double h326 = h191 + AB[1] * h164;
-  const double h327 = h192 + AB[1] * h165;
-  const double h328 = h193 + AB[1] * h166;
-  const double h329 = h194 + AB[1] * h167;
-  const double h330 = h195 + AB[1] * h168;
-  const double h331 = h196 + AB[1] * h169;
-  const double h332 = h197 + AB[1] * h170;
-  const double h333 = h234 + AB[0] * h162;
-  const double h334 = h235 + AB[0] * h163;
-  const double h335 = h236 + AB[0] * h164;
-  const double h336 = h237 + AB[0] * h165;
-  const double h337 = h238 + AB[0] * h166;
-  const double h338 = h239 + AB[0] * h167;
-  const double h339 = h240 + AB[0] * h168;
-  const double h340 = h241 + AB[0] * h169;
-  const double h341 = h242 + AB[0] * h170;
-  const double h342 = h189 + AB[2] * h180;
-  const double h343 = h190 + AB[2] * h181;
-  const double h344 = h191 + AB[2] * h182;
-  const double h345 = h192 + AB[2] * h183;
-  const double h346 = h193 + AB[2] * h184;
-  const double h347 = h194 + AB[2] * h185;
-  const double h348 = h195 + AB[2] * h186;
-  const double h349 = h196 + AB[2] * h187;
-  const double h350 = h197 + AB[2] * h188;
-  const double h351 = h198 + AB[1] * h180;
-  const double h352 = h199 + AB[1] * h181;
-  const double h353 = h200 + AB[1] * h182;
-  const double h354 = h201 + AB[1] * h183;
-  const double h355 = h202 + AB[1] * h184;
-  const double h356 = h203 + AB[1] * h185;
-  const double h357 = h204 + AB[1] * h186;
-  const double h358 = h205 + AB[1] * h187;
-  const double h359 = h206 + AB[1] * h188;
-  const double h360 = h243 + AB[0] * h180;
-  const double h361 = h244 + AB[0] * h181;
-  const double h362 = h245 + AB[0] * h182;
-  const double h363 = h246 + AB[0] * h183;
-  const double h364 = h247 + AB[0] * h184;
-  const double h365 = h248 + AB[0] * h185;
-  const double h366 = h249 + AB[0] * h186;
-  const double h367 = h250 + AB[0] * h187;
-  const double h368 = h251 + AB[0] * h188;
-  const double h369 = h216 + AB[2] * h207;
-  const double h370 = h217 + AB[2] * h208;
-  const double h371 = h218 + AB[2] * h209;
-  const double h372 = h219 + AB[2] * h210;
-  const double h373 = h220 + AB[2] * h211;
-  const double h374 = h221 + AB[2] * h212;
-  const double h375 = h222 + AB[2] * h213;
-  const double h376 = h223 + AB[2] * h214;
-  const double h377 = h224 + AB[2] * h215;
-  const double h378 = h234 + AB[1] * h207;
-  const double h379 = h235 + AB[1] * h208;
-  const double h380 = h236 + AB[1] * h209;
-  const double h381 = h237 + AB[1] * h210;
-  const double h382 = h238 + AB[1] * h211;
-  const double h383 = h239 + AB[1] * h212;
-  const double h384 = h240 + AB[1] * h213;
-  const double h385 = h241 + AB[1] * h214;
-  const double h386 = h242 + AB[1] * h215;
-  const double h387 = h261 + AB[0] * h207;
-  const double h388 = h262 + AB[0] * h208;
-  const double h389 = h263 + AB[0] * h209;
-  const double h390 = h264 + AB[0] * h210;
-  const double h391 = h265 + AB[0] * h211;
-  const double h392 = h266 + AB[0] * h212;
-  const double h393 = h267 + AB[0] * h213;
-  const double h394 = h268 + AB[0] * h214;
-  const double h395 = h269 + AB[0] * h215;
-  const double h396 = h234 + AB[2] * h225;
-  const double h397 = h235 + AB[2] * h226;
-  const double h398 = h236 + AB[2] * h227;
-  const double h399 = h237 + AB[2] * h228;
-  const double h400 = h238 + AB[2] * h229;
-  const double h401 = h239 + AB[2] * h230;
-  const double h402 = h240 + AB[2] * h231;
-  const double h403 = h241 + AB[2] * h232;
-  const double h404 = h242 + AB[2] * h233;
-  const double h405 = h243 + AB[1] * h225;
-  const double h406 = h244 + AB[1] * h226;
-  const double h407 = h245 + AB[1] * h227;
-  const double h408 = h246 + AB[1] * h228;
-  const double h409 = h247 + AB[1] * h229;
-  const double h410 = h248 + AB[1] * h230;
-  const double h411 = h249 + AB[1] * h231;
-  const double h412 = h250 + AB[1] * h232;
-  const double h413 = h251 + AB[1] * h233;
-  const double h414 = h270 + AB[0] * h225;
-  const double h415 = h271 + AB[0] * h226;
-  const double h416 = h272 + AB[0] * h227;
-  const double h417 = h273 + AB[0] * h228;
-  const double h418 = h274 + AB[0] * h229;
-  const double h419 = h275 + AB[0] * h230;
-  const double h420 = h276 + AB[0] * h231;
-  const double h421 = h277 + AB[0] * h232;
-  const double h422 = h278 + AB[0] * h233;
-  const double h423 = h261 + AB[2] * h252;
-  const double h424 = h262 + AB[2] * h253;
-  const double h425 = h263 + AB[2] * h254;
-  const double h426 = h264 + AB[2] * h255;
-  const double h427 = h265 + AB[2] * h256;
-  const double h428 = h266 + AB[2] * h257;
-  const double h429 = h267 + AB[2] * h258;
-  const double h430 = h268 + AB[2] * h259;
-  const double h431 = h269 + AB[2] * h260;
-  const double h432 = h270 + AB[1] * h252;
-  const double h433 = h271 + AB[1] * h253;
-  const double h434 = h272 + AB[1] * h254;
-  const double h435 = h273 + AB[1] * h255;
-  const double h436 = h274 + AB[1] * h256;
-  const double h437 = h275 + AB[1] * h257;
-  const double h438 = h276 + AB[1] * h258;
-  const double h439 = h277 + AB[1] * h259;
-  const double h440 = h278 + AB[1] * h260;
-  const double h441 = h279 + AB[0] * h252;
-  const double h442 = h280 + AB[0] * h253;
-  const double h443 = h281 + AB[0] * h254;
-  const double h444 = h282 + AB[0] * h255;
-  const double h445 = h283 + AB[0] * h256;
-  const double h446 = h284 + AB[0] * h257;
-  const double h447 = h285 + AB[0] * h258;
-  const double h448 = h286 + AB[0] * h259;
-  const double h449 = h287 + AB[0] * h260;
-  double out[162];
-  out[0] = h449 * 0.5773502691896258;
-  out[1] = h448 * 0.5773502691896258;
-  out[2] = h447 * 0.5773502691896258;
-  out[3] = h446 * 0.5773502691896258;
-  out[4] = h445 * 0.5773502691896258;
-  out[5] = h444 * 0.5773502691896258;
-  out[6] = h443 * 0.5773502691896258;
-  out[7] = h442 * 0.5773502691896258;
-  out[8] = h441 * 0.5773502691896258;
-  out[9] = h440 * 0.5773502691896258;
-  out[10] = h439 * 0.5773502691896258;
-  out[11] = h438 * 0.5773502691896258;
-  out[12] = h437 * 0.5773502691896258;
-  out[13] = h436 * 0.5773502691896258;
-  out[14] = h435 * 0.5773502691896258;
-  out[15] = h434 * 0.5773502691896258;
-  out[16] = h433 * 0.5773502691896258;
-  out[17] = h432 * 0.5773502691896258;
-  out[18] = h431 * 0.5773502691896258;
-  out[19] = h430 * 0.5773502691896258;
-  out[20] = h429 * 0.5773502691896258;
-  out[21] = h428 * 0.5773502691896258;
-  out[22] = h427 * 0.5773502691896258;
-  out[23] = h426 * 0.5773502691896258;
-  out[24] = h425 * 0.5773502691896258;
-  out[25] = h424 * 0.5773502691896258;
-  out[26] = h423 * 0.5773502691896258;
-  out[27] = h422;
-  out[28] = h421;
-  out[29] = h420;
-  out[30] = h419;
-  out[31] = h418;
-  out[32] = h417;
-  out[33] = h416;
-  out[34] = h415;
-  out[35] = h414;
-  out[36] = h413;
-  out[37] = h412;
-  out[38] = h411;
-  out[39] = h410;
-  out[40] = h409;
-  out[41] = h408;
-  out[42] = h407;
-  out[43] = h406;
-  out[44] = h405;
-  out[45] = h404;
-  out[46] = h403;
-  out[47] = h402;
-  out[48] = h401;
-  out[49] = h400;
-  out[50] = h399;
-  out[51] = h398;
-  out[52] = h397;
-  out[53] = h396;
-  out[54] = h395;
-  out[55] = h394;
-  out[56] = h393;
-  out[57] = h392;
-  out[58] = h391;
-  out[59] = h390;
-  out[60] = h389;
-  out[61] = h388;
-  out[62] = h387;
-  out[63] = h386;
-  out[64] = h385;
-  out[65] = h384;
-  out[66] = h383;
-  out[67] = h382;
-  out[68] = h381;
-  out[69] = h380;
-  out[70] = h379;
-  out[71] = h378;
-  out[72] = h377;
-  out[73] = h376;
-  out[74] = h375;
-  out[75] = h374;
-  out[76] = h373;
-  out[77] = h372;
-  out[78] = h371;
-  out[79] = h370;
-  out[80] = h369;
-  out[81] = h368 * 0.5773502691896258;
-  out[82] = h367 * 0.5773502691896258;
-  out[83] = h366 * 0.5773502691896258;
-  out[84] = h365 * 0.5773502691896258;
-  out[85] = h364 * 0.5773502691896258;
-  out[86] = h363 * 0.5773502691896258;
-  out[87] = h362 * 0.5773502691896258;
-  out[88] = h361 * 0.5773502691896258;
-  out[89] = h360 * 0.5773502691896258;
-  out[90] = h359 * 0.5773502691896258;
-  out[91] = h358 * 0.5773502691896258;
-  out[92] = h357 * 0.5773502691896258;
-  out[93] = h356 * 0.5773502691896258;
-  out[94] = h355 * 0.5773502691896258;
-  out[95] = h354 * 0.5773502691896258;
-  out[96] = h353 * 0.5773502691896258;
-  out[97] = h352 * 0.5773502691896258;
-  out[98] = h351 * 0.5773502691896258;
-  out[99] = h350 * 0.5773502691896258;
-  out[100] = h349 * 0.5773502691896258;
-  out[101] = h348 * 0.5773502691896258;
-  out[102] = h347 * 0.5773502691896258;
-  out[103] = h346 * 0.5773502691896258;
-  out[104] = h345 * 0.5773502691896258;
-  out[105] = h344 * 0.5773502691896258;
-  out[106] = h343 * 0.5773502691896258;
-  out[107] = h342 * 0.5773502691896258;
-  out[108] = h341;
-  out[109] = h340;
-  out[110] = h339;
-  out[111] = h338;
-  out[112] = h337;
-  out[113] = h336;
-  out[114] = h335;
-  out[115] = h334;
-  out[116] = h333;
-  out[117] = h332;
-  out[118] = h331;
-  out[119] = h330;
-  out[120] = h329;
-  out[121] = h328;
-  out[122] = h327;
-  out[123] = h326;
-  out[124] = h325;
-  out[125] = h324;
-  out[126] = h323;
-  out[127] = h322;
-  out[128] = h321;
-  out[129] = h320;
-  out[130] = h319;
-  out[131] = h318;
-  out[132] = h317;
-  out[133] = h316;
-  out[134] = h315;
-  out[135] = h314 * 0.5773502691896258;
-  out[136] = h313 * 0.5773502691896258;
-  out[137] = h312 * 0.5773502691896258;
-  out[138] = h311 * 0.5773502691896258;
-  out[139] = h310 * 0.5773502691896258;
-  out[140] = h309 * 0.5773502691896258;
-  out[141] = h308 * 0.5773502691896258;
-  out[142] = h307 * 0.5773502691896258;
-  out[143] = h306 * 0.5773502691896258;
-  out[144] = h305 * 0.5773502691896258;
-  out[145] = h304 * 0.5773502691896258;
-  out[146] = h303 * 0.5773502691896258;
-  out[147] = h302 * 0.5773502691896258;
-  out[148] = h301 * 0.5773502691896258;
-  out[149] = h300 * 0.5773502691896258;
-  out[150] = h299 * 0.5773502691896258;
-  out[151] = h298 * 0.5773502691896258;
-  out[152] = h297 * 0.5773502691896258;
-  out[153] = h296 * 0.5773502691896258;
-  out[154] = h295 * 0.5773502691896258;
-  out[155] = h294 * 0.5773502691896258;
-  out[156] = h293 * 0.5773502691896258;
-  out[157] = h292 * 0.5773502691896258;
-  out[158] = h291 * 0.5773502691896258;
-  out[159] = h290 * 0.5773502691896258;
-  out[160] = h289 * 0.5773502691896258;
-  out[161] = h288 * 0.5773502691896258;
-  store_block<6, 3, 3, 3>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (21|20): 270 recursion values, 96 accumulators, 108 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2120(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<5>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  double acc31 = 0.0;
-  double acc32 = 0.0;
-  double acc33 = 0.0;
-  double acc34 = 0.0;
-  double acc35 = 0.0;
-  double acc36 = 0.0;
-  double acc37 = 0.0;
-  double acc38 = 0.0;
-  double acc39 = 0.0;
-  double acc40 = 0.0;
-  double acc41 = 0.0;
-  double acc42 = 0.0;
-  double acc43 = 0.0;
-  double acc44 = 0.0;
-  double acc45 = 0.0;
-  double acc46 = 0.0;
-  double acc47 = 0.0;
-  double acc48 = 0.0;
-  double acc49 = 0.0;
-  double acc50 = 0.0;
-  double acc51 = 0.0;
-  double acc52 = 0.0;
-  double acc53 = 0.0;
-  double acc54 = 0.0;
-  double acc55 = 0.0;
-  double acc56 = 0.0;
-  double acc57 = 0.0;
-  double acc58 = 0.0;
-  double acc59 = 0.0;
-  double acc60 = 0.0;
-  double acc61 = 0.0;
-  double acc62 = 0.0;
-  double acc63 = 0.0;
-  double acc64 = 0.0;
-  double acc65 = 0.0;
-  double acc66 = 0.0;
-  double acc67 = 0.0;
-  double acc68 = 0.0;
-  double acc69 = 0.0;
-  double acc70 = 0.0;
-  double acc71 = 0.0;
-  double acc72 = 0.0;
-  double acc73 = 0.0;
-  double acc74 = 0.0;
-  double acc75 = 0.0;
-  double acc76 = 0.0;
-  double acc77 = 0.0;
-  double acc78 = 0.0;
-  double acc79 = 0.0;
-  double acc80 = 0.0;
-  double acc81 = 0.0;
-  double acc82 = 0.0;
-  double acc83 = 0.0;
-  double acc84 = 0.0;
-  double acc85 = 0.0;
-  double acc86 = 0.0;
-  double acc87 = 0.0;
-  double acc88 = 0.0;
-  double acc89 = 0.0;
-  double acc90 = 0.0;
-  double acc91 = 0.0;
-  double acc92 = 0.0;
-  double acc93 = 0.0;
-  double acc94 = 0.0;
-  double acc95 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[6];
-      boys_eval_lds<5>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x6 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x7 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x8 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x9 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x10 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x11 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x12 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x13 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x14 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x15 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x16 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x17 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x18 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x19 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x20 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x44 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x21 = s.PA[0] * x6 + s.WP[0] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x22 = s.PA[1] * x6 + s.WP[1] * x9;
-      const TB x23 = s.PA[2] * x6 + s.WP[2] * x9;
-      const TB x24 = s.PA[1] * x7 + s.WP[1] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x26 = s.PA[2] * x8 + s.WP[2] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x25 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x27 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x28 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x29 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x30 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x54 = s.PA[2] * x26 + s.WP[2] * x32 + z2 * (x8 - s.roz * x11);
-      const TB x31 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x33 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x34 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x35 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x36 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x37 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x38 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x74 = s.PA[2] * x38 + s.WP[2] * x44 + z2 * (x14 - s.roz * x17);
-      const TB x39 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x3 - s.roz * x4);
-      const TB x42 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x43 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x73 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x13 - s.roz * x16);
-      const TB x71 = s.PA[1] * x36 + s.WP[1] * x42 + z2 * (x13 - s.roz * x16);
-      const TB x72 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x40 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x41 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x70 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x12 - s.roz * x15);
-      const TB x45 = s.PA[0] * x21 + s.WP[0] * x27 + z2 * (x6 - s.roz * x9);
-      const TB x46 = s.PA[1] * x21 + s.WP[1] * x27;
-      const TB x47 = s.PA[2] * x21 + s.WP[2] * x27;
-      const TB x48 = s.PA[1] * x22 + s.WP[1] * x28 + z1 * (x6 - s.roz * x9);
-      const TB x50 = s.PA[2] * x23 + s.WP[2] * x29 + z1 * (x6 - s.roz * x9);
-      const TB x49 = s.PA[2] * x22 + s.WP[2] * x28;
-      const TB x51 = s.PA[1] * x24 + s.WP[1] * x30 + z2 * (x7 - s.roz * x10);
-      const TB x53 = s.PA[2] * x25 + s.WP[2] * x31 + z1 * (x7 - s.roz * x10);
-      const TB x52 = s.PA[2] * x24 + s.WP[2] * x30;
-      const TB x55 = s.PA[0] * x27 + s.WP[0] * x33 + z2 * (x9 - s.roz * x12);
-      const TB x56 = s.PA[1] * x27 + s.WP[1] * x33;
-      const TB x57 = s.PA[2] * x27 + s.WP[2] * x33;
-      const TB x58 = s.PA[1] * x28 + s.WP[1] * x34 + z1 * (x9 - s.roz * x12);
-      const TB x59 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x60 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x9 - s.roz * x12);
-      const TB x61 = s.PA[1] * x30 + s.WP[1] * x36 + z2 * (x10 - s.roz * x13);
-      const TB x62 = s.PA[2] * x30 + s.WP[2] * x36;
-      const TB x63 = s.PA[2] * x31 + s.WP[2] * x37 + z1 * (x10 - s.roz * x13);
-      const TB x64 = s.PA[2] * x32 + s.WP[2] * x38 + z2 * (x11 - s.roz * x14);
-      const TB x65 = s.PA[0] * x33 + s.WP[0] * x39 + z2 * (x12 - s.roz * x15);
-      const TB x68 = s.PA[1] * x34 + s.WP[1] * x40 + z1 * (x12 - s.roz * x15);
-      const TB x69 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x66 = s.PA[1] * x33 + s.WP[1] * x39;
-      const TB x67 = s.PA[2] * x33 + s.WP[2] * x39;
-      const TB x75 = s.QC[0] * x9 + s.WQ[0] * x12 + r1 * x2;
-      const TB x76 = s.QC[0] * x10 + s.WQ[0] * x13;
-      const TB x77 = s.QC[0] * x11 + s.WQ[0] * x14;
-      const TB x78 = s.QC[0] * x21 + s.WQ[0] * x27 + r2 * x9;
-      const TB x79 = s.QC[0] * x27 + s.WQ[0] * x33 + r2 * x12;
-      const TB x180 = s.QC[0] * x78 + s.WQ[0] * x79 + e1 * (x21 - s.roe * x27) + r2 * x75;
-      acc0 += static_cast<double>(x180);
-      const TB x196 = s.QC[1] * x78 + s.WQ[1] * x79;
-      acc16 += static_cast<double>(x196);
-      const TB x212 = s.QC[2] * x78 + s.WQ[2] * x79;
-      acc32 += static_cast<double>(x212);
-      const TB x80 = s.QC[0] * x22 + s.WQ[0] * x28 + r1 * x10;
-      const TB x81 = s.QC[0] * x28 + s.WQ[0] * x34 + r1 * x13;
-      const TB x181 = s.QC[0] * x80 + s.WQ[0] * x81 + e1 * (x22 - s.roe * x28) + r1 * x76;
-      acc1 += static_cast<double>(x181);
-      const TB x197 = s.QC[1] * x80 + s.WQ[1] * x81 + r1 * x75;
-      acc17 += static_cast<double>(x197);
-      const TB x213 = s.QC[2] * x80 + s.WQ[2] * x81;
-      acc33 += static_cast<double>(x213);
-      const TB x82 = s.QC[0] * x23 + s.WQ[0] * x29 + r1 * x11;
-      const TB x83 = s.QC[0] * x29 + s.WQ[0] * x35 + r1 * x14;
-      const TB x214 = s.QC[2] * x82 + s.WQ[2] * x83 + r1 * x75;
-      acc34 += static_cast<double>(x214);
-      const TB x182 = s.QC[0] * x82 + s.WQ[0] * x83 + e1 * (x23 - s.roe * x29) + r1 * x77;
-      acc2 += static_cast<double>(x182);
-      const TB x198 = s.QC[1] * x82 + s.WQ[1] * x83;
-      acc18 += static_cast<double>(x198);
-      const TB x84 = s.QC[0] * x24 + s.WQ[0] * x30;
-      const TB x85 = s.QC[0] * x30 + s.WQ[0] * x36;
-      const TB x183 = s.QC[0] * x84 + s.WQ[0] * x85 + e1 * (x24 - s.roe * x30);
-      acc3 += static_cast<double>(x183);
-      const TB x199 = s.QC[1] * x84 + s.WQ[1] * x85 + r2 * x76;
-      acc19 += static_cast<double>(x199);
-      const TB x215 = s.QC[2] * x84 + s.WQ[2] * x85;
-      acc35 += static_cast<double>(x215);
-      const TB x86 = s.QC[0] * x25 + s.WQ[0] * x31;
-      const TB x87 = s.QC[0] * x31 + s.WQ[0] * x37;
-      const TB x216 = s.QC[2] * x86 + s.WQ[2] * x87 + r1 * x76;
-      acc36 += static_cast<double>(x216);
-      const TB x184 = s.QC[0] * x86 + s.WQ[0] * x87 + e1 * (x25 - s.roe * x31);
-      acc4 += static_cast<double>(x184);
-      const TB x200 = s.QC[1] * x86 + s.WQ[1] * x87 + r1 * x77;
-      acc20 += static_cast<double>(x200);
-      const TB x88 = s.QC[0] * x26 + s.WQ[0] * x32;
-      const TB x89 = s.QC[0] * x32 + s.WQ[0] * x38;
-      const TB x217 = s.QC[2] * x88 + s.WQ[2] * x89 + r2 * x77;
-      acc37 += static_cast<double>(x217);
-      const TB x185 = s.QC[0] * x88 + s.WQ[0] * x89 + e1 * (x26 - s.roe * x32);
-      acc5 += static_cast<double>(x185);
-      const TB x201 = s.QC[1] * x88 + s.WQ[1] * x89;
-      acc21 += static_cast<double>(x201);
-      const TB x90 = s.QC[0] * x45 + s.WQ[0] * x55 + r3 * x27;
-      const TB x91 = s.QC[0] * x55 + s.WQ[0] * x65 + r3 * x33;
-      const TB x186 = s.QC[0] * x90 + s.WQ[0] * x91 + e1 * (x45 - s.roe * x55) + r3 * x79;
-      acc6 += static_cast<double>(x186);
-      const TB x202 = s.QC[1] * x90 + s.WQ[1] * x91;
-      acc22 += static_cast<double>(x202);
-      const TB x218 = s.QC[2] * x90 + s.WQ[2] * x91;
-      acc38 += static_cast<double>(x218);
-      const TB x92 = s.QC[0] * x46 + s.WQ[0] * x56 + r2 * x28;
-      const TB x93 = s.QC[0] * x56 + s.WQ[0] * x66 + r2 * x34;
-      const TB x187 = s.QC[0] * x92 + s.WQ[0] * x93 + e1 * (x46 - s.roe * x56) + r2 * x81;
-      acc7 += static_cast<double>(x187);
-      const TB x203 = s.QC[1] * x92 + s.WQ[1] * x93 + r1 * x79;
-      acc23 += static_cast<double>(x203);
-      const TB x219 = s.QC[2] * x92 + s.WQ[2] * x93;
-      acc39 += static_cast<double>(x219);
-      const TB x94 = s.QC[0] * x47 + s.WQ[0] * x57 + r2 * x29;
-      const TB x95 = s.QC[0] * x57 + s.WQ[0] * x67 + r2 * x35;
-      const TB x220 = s.QC[2] * x94 + s.WQ[2] * x95 + r1 * x79;
-      acc40 += static_cast<double>(x220);
-      const TB x188 = s.QC[0] * x94 + s.WQ[0] * x95 + e1 * (x47 - s.roe * x57) + r2 * x83;
-      acc8 += static_cast<double>(x188);
-      const TB x204 = s.QC[1] * x94 + s.WQ[1] * x95;
-      acc24 += static_cast<double>(x204);
-      const TB x96 = s.QC[0] * x48 + s.WQ[0] * x58 + r1 * x30;
-      const TB x97 = s.QC[0] * x58 + s.WQ[0] * x68 + r1 * x36;
-      const TB x189 = s.QC[0] * x96 + s.WQ[0] * x97 + e1 * (x48 - s.roe * x58) + r1 * x85;
-      acc9 += static_cast<double>(x189);
-      const TB x205 = s.QC[1] * x96 + s.WQ[1] * x97 + r2 * x81;
-      acc25 += static_cast<double>(x205);
-      const TB x221 = s.QC[2] * x96 + s.WQ[2] * x97;
-      acc41 += static_cast<double>(x221);
-      const TB x98 = s.QC[0] * x49 + s.WQ[0] * x59 + r1 * x31;
-      const TB x99 = s.QC[0] * x59 + s.WQ[0] * x69 + r1 * x37;
-      const TB x222 = s.QC[2] * x98 + s.WQ[2] * x99 + r1 * x81;
-      acc42 += static_cast<double>(x222);
-      const TB x190 = s.QC[0] * x98 + s.WQ[0] * x99 + e1 * (x49 - s.roe * x59) + r1 * x87;
-      acc10 += static_cast<double>(x190);
-      const TB x206 = s.QC[1] * x98 + s.WQ[1] * x99 + r1 * x83;
-      acc26 += static_cast<double>(x206);
-      const TB x100 = s.QC[0] * x50 + s.WQ[0] * x60 + r1 * x32;
-      const TB x101 = s.QC[0] * x60 + s.WQ[0] * x70 + r1 * x38;
-      const TB x223 = s.QC[2] * x100 + s.WQ[2] * x101 + r2 * x83;
-      acc43 += static_cast<double>(x223);
-      const TB x191 = s.QC[0] * x100 + s.WQ[0] * x101 + e1 * (x50 - s.roe * x60) + r1 * x89;
-      acc11 += static_cast<double>(x191);
-      const TB x207 = s.QC[1] * x100 + s.WQ[1] * x101;
-      acc27 += static_cast<double>(x207);
-      const TB x102 = s.QC[0] * x51 + s.WQ[0] * x61;
-      const TB x103 = s.QC[0] * x61 + s.WQ[0] * x71;
-      const TB x192 = s.QC[0] * x102 + s.WQ[0] * x103 + e1 * (x51 - s.roe * x61);
-      acc12 += static_cast<double>(x192);
-      const TB x208 = s.QC[1] * x102 + s.WQ[1] * x103 + r3 * x85;
-      acc28 += static_cast<double>(x208);
-      const TB x224 = s.QC[2] * x102 + s.WQ[2] * x103;
-      acc44 += static_cast<double>(x224);
-      const TB x104 = s.QC[0] * x52 + s.WQ[0] * x62;
-      const TB x105 = s.QC[0] * x62 + s.WQ[0] * x72;
-      const TB x225 = s.QC[2] * x104 + s.WQ[2] * x105 + r1 * x85;
-      acc45 += static_cast<double>(x225);
-      const TB x193 = s.QC[0] * x104 + s.WQ[0] * x105 + e1 * (x52 - s.roe * x62);
-      acc13 += static_cast<double>(x193);
-      const TB x209 = s.QC[1] * x104 + s.WQ[1] * x105 + r2 * x87;
-      acc29 += static_cast<double>(x209);
-      const TB x106 = s.QC[0] * x53 + s.WQ[0] * x63;
-      const TB x107 = s.QC[0] * x63 + s.WQ[0] * x73;
-      const TB x226 = s.QC[2] * x106 + s.WQ[2] * x107 + r2 * x87;
-      acc46 += static_cast<double>(x226);
-      const TB x194 = s.QC[0] * x106 + s.WQ[0] * x107 + e1 * (x53 - s.roe * x63);
-      acc14 += static_cast<double>(x194);
-      const TB x210 = s.QC[1] * x106 + s.WQ[1] * x107 + r1 * x89;
-      acc30 += static_cast<double>(x210);
-      const TB x108 = s.QC[0] * x54 + s.WQ[0] * x64;
-      const TB x109 = s.QC[0] * x64 + s.WQ[0] * x74;
-      const TB x227 = s.QC[2] * x108 + s.WQ[2] * x109 + r3 * x89;
-      acc47 += static_cast<double>(x227);
-      const TB x195 = s.QC[0] * x108 + s.WQ[0] * x109 + e1 * (x54 - s.roe * x64);
-      acc15 += static_cast<double>(x195);
-      const TB x211 = s.QC[1] * x108 + s.WQ[1] * x109;
-      acc31 += static_cast<double>(x211);
-      const TB x110 = s.QC[1] * x9 + s.WQ[1] * x12;
-      const TB x111 = s.QC[1] * x10 + s.WQ[1] * x13 + r1 * x2;
-      const TB x147 = s.QC[2] * x11 + s.WQ[2] * x14 + r1 * x2;
-      const TB x112 = s.QC[1] * x11 + s.WQ[1] * x14;
-      const TB x113 = s.QC[1] * x21 + s.WQ[1] * x27;
-      const TB x114 = s.QC[1] * x27 + s.WQ[1] * x33;
-      const TB x228 = s.QC[1] * x113 + s.WQ[1] * x114 + e1 * (x21 - s.roe * x27);
-      acc48 += static_cast<double>(x228);
-      const TB x244 = s.QC[2] * x113 + s.WQ[2] * x114;
-      acc64 += static_cast<double>(x244);
-      const TB x115 = s.QC[1] * x22 + s.WQ[1] * x28 + r1 * x9;
-      const TB x116 = s.QC[1] * x28 + s.WQ[1] * x34 + r1 * x12;
-      const TB x229 = s.QC[1] * x115 + s.WQ[1] * x116 + e1 * (x22 - s.roe * x28) + r1 * x110;
-      acc49 += static_cast<double>(x229);
-      const TB x245 = s.QC[2] * x115 + s.WQ[2] * x116;
-      acc65 += static_cast<double>(x245);
-      const TB x117 = s.QC[1] * x23 + s.WQ[1] * x29;
-      const TB x118 = s.QC[1] * x29 + s.WQ[1] * x35;
-      const TB x246 = s.QC[2] * x117 + s.WQ[2] * x118 + r1 * x110;
-      acc66 += static_cast<double>(x246);
-      const TB x230 = s.QC[1] * x117 + s.WQ[1] * x118 + e1 * (x23 - s.roe * x29);
-      acc50 += static_cast<double>(x230);
-      const TB x119 = s.QC[1] * x24 + s.WQ[1] * x30 + r2 * x10;
-      const TB x120 = s.QC[1] * x30 + s.WQ[1] * x36 + r2 * x13;
-      const TB x231 = s.QC[1] * x119 + s.WQ[1] * x120 + e1 * (x24 - s.roe * x30) + r2 * x111;
-      acc51 += static_cast<double>(x231);
-      const TB x247 = s.QC[2] * x119 + s.WQ[2] * x120;
-      acc67 += static_cast<double>(x247);
-      const TB x121 = s.QC[1] * x25 + s.WQ[1] * x31 + r1 * x11;
-      const TB x158 = s.QC[2] * x26 + s.WQ[2] * x32 + r2 * x11;
-      const TB x122 = s.QC[1] * x31 + s.WQ[1] * x37 + r1 * x14;
-      const TB x248 = s.QC[2] * x121 + s.WQ[2] * x122 + r1 * x111;
-      acc68 += static_cast<double>(x248);
-      const TB x232 = s.QC[1] * x121 + s.WQ[1] * x122 + e1 * (x25 - s.roe * x31) + r1 * x112;
-      acc52 += static_cast<double>(x232);
-      const TB x159 = s.QC[2] * x32 + s.WQ[2] * x38 + r2 * x14;
-      const TB x265 = s.QC[2] * x158 + s.WQ[2] * x159 + e1 * (x26 - s.roe * x32) + r2 * x147;
-      acc85 += static_cast<double>(x265);
-      const TB x123 = s.QC[1] * x26 + s.WQ[1] * x32;
-      const TB x124 = s.QC[1] * x32 + s.WQ[1] * x38;
-      const TB x233 = s.QC[1] * x123 + s.WQ[1] * x124 + e1 * (x26 - s.roe * x32);
-      acc53 += static_cast<double>(x233);
-      const TB x249 = s.QC[2] * x123 + s.WQ[2] * x124 + r2 * x112;
-      acc69 += static_cast<double>(x249);
-      const TB x125 = s.QC[1] * x45 + s.WQ[1] * x55;
-      const TB x126 = s.QC[1] * x55 + s.WQ[1] * x65;
-      const TB x161 = s.QC[2] * x55 + s.WQ[2] * x65;
-      const TB x234 = s.QC[1] * x125 + s.WQ[1] * x126 + e1 * (x45 - s.roe * x55);
-      acc54 += static_cast<double>(x234);
-      const TB x250 = s.QC[2] * x125 + s.WQ[2] * x126;
-      acc70 += static_cast<double>(x250);
-      const TB x127 = s.QC[1] * x46 + s.WQ[1] * x56 + r1 * x27;
-      const TB x128 = s.QC[1] * x56 + s.WQ[1] * x66 + r1 * x33;
-      const TB x163 = s.QC[2] * x56 + s.WQ[2] * x66;
-      const TB x235 = s.QC[1] * x127 + s.WQ[1] * x128 + e1 * (x46 - s.roe * x56) + r1 * x114;
-      acc55 += static_cast<double>(x235);
-      const TB x251 = s.QC[2] * x127 + s.WQ[2] * x128;
-      acc71 += static_cast<double>(x251);
-      const TB x129 = s.QC[1] * x47 + s.WQ[1] * x57;
-      const TB x130 = s.QC[1] * x57 + s.WQ[1] * x67;
-      const TB x252 = s.QC[2] * x129 + s.WQ[2] * x130 + r1 * x114;
-      acc72 += static_cast<double>(x252);
-      const TB x236 = s.QC[1] * x129 + s.WQ[1] * x130 + e1 * (x47 - s.roe * x57);
-      acc56 += static_cast<double>(x236);
-      const TB x165 = s.QC[2] * x57 + s.WQ[2] * x67 + r1 * x33;
-      const TB x149 = s.QC[2] * x27 + s.WQ[2] * x33;
-      const TB x131 = s.QC[1] * x48 + s.WQ[1] * x58 + r2 * x28;
-      const TB x132 = s.QC[1] * x58 + s.WQ[1] * x68 + r2 * x34;
-      const TB x167 = s.QC[2] * x58 + s.WQ[2] * x68;
-      const TB x237 = s.QC[1] * x131 + s.WQ[1] * x132 + e1 * (x48 - s.roe * x58) + r2 * x116;
-      acc57 += static_cast<double>(x237);
-      const TB x253 = s.QC[2] * x131 + s.WQ[2] * x132;
-      acc73 += static_cast<double>(x253);
-      const TB x133 = s.QC[1] * x49 + s.WQ[1] * x59 + r1 * x29;
-      const TB x134 = s.QC[1] * x59 + s.WQ[1] * x69 + r1 * x35;
-      const TB x254 = s.QC[2] * x133 + s.WQ[2] * x134 + r1 * x116;
-      acc74 += static_cast<double>(x254);
-      const TB x238 = s.QC[1] * x133 + s.WQ[1] * x134 + e1 * (x49 - s.roe * x59) + r1 * x118;
-      acc58 += static_cast<double>(x238);
-      const TB x169 = s.QC[2] * x59 + s.WQ[2] * x69 + r1 * x34;
-      const TB x151 = s.QC[2] * x28 + s.WQ[2] * x34;
-      const TB x135 = s.QC[1] * x50 + s.WQ[1] * x60;
-      const TB x136 = s.QC[1] * x60 + s.WQ[1] * x70;
-      const TB x255 = s.QC[2] * x135 + s.WQ[2] * x136 + r2 * x118;
-      acc75 += static_cast<double>(x255);
-      const TB x239 = s.QC[1] * x135 + s.WQ[1] * x136 + e1 * (x50 - s.roe * x60);
-      acc59 += static_cast<double>(x239);
-      const TB x171 = s.QC[2] * x60 + s.WQ[2] * x70 + r2 * x35;
-      const TB x153 = s.QC[2] * x29 + s.WQ[2] * x35 + r1 * x12;
-      const TB x145 = s.QC[2] * x9 + s.WQ[2] * x12;
-      const TB x152 = s.QC[2] * x23 + s.WQ[2] * x29 + r1 * x9;
-      const TB x262 = s.QC[2] * x152 + s.WQ[2] * x153 + e1 * (x23 - s.roe * x29) + r1 * x145;
-      acc82 += static_cast<double>(x262);
-      const TB x170 = s.QC[2] * x50 + s.WQ[2] * x60 + r2 * x29;
-      const TB x271 = s.QC[2] * x170 + s.WQ[2] * x171 + e1 * (x50 - s.roe * x60) + r2 * x153;
-      acc91 += static_cast<double>(x271);
-      const TB x137 = s.QC[1] * x51 + s.WQ[1] * x61 + r3 * x30;
-      const TB x138 = s.QC[1] * x61 + s.WQ[1] * x71 + r3 * x36;
-      const TB x173 = s.QC[2] * x61 + s.WQ[2] * x71;
-      const TB x240 = s.QC[1] * x137 + s.WQ[1] * x138 + e1 * (x51 - s.roe * x61) + r3 * x120;
-      acc60 += static_cast<double>(x240);
-      const TB x256 = s.QC[2] * x137 + s.WQ[2] * x138;
-      acc76 += static_cast<double>(x256);
-      const TB x139 = s.QC[1] * x52 + s.WQ[1] * x62 + r2 * x31;
-      const TB x140 = s.QC[1] * x62 + s.WQ[1] * x72 + r2 * x37;
-      const TB x257 = s.QC[2] * x139 + s.WQ[2] * x140 + r1 * x120;
-      acc77 += static_cast<double>(x257);
-      const TB x241 = s.QC[1] * x139 + s.WQ[1] * x140 + e1 * (x52 - s.roe * x62) + r2 * x122;
-      acc61 += static_cast<double>(x241);
-      const TB x175 = s.QC[2] * x62 + s.WQ[2] * x72 + r1 * x36;
-      const TB x155 = s.QC[2] * x30 + s.WQ[2] * x36;
-      const TB x141 = s.QC[1] * x53 + s.WQ[1] * x63 + r1 * x32;
-      const TB x178 = s.QC[2] * x54 + s.WQ[2] * x64 + r3 * x32;
-      const TB x142 = s.QC[1] * x63 + s.WQ[1] * x73 + r1 * x38;
-      const TB x258 = s.QC[2] * x141 + s.WQ[2] * x142 + r2 * x122;
-      acc78 += static_cast<double>(x258);
-      const TB x242 = s.QC[1] * x141 + s.WQ[1] * x142 + e1 * (x53 - s.roe * x63) + r1 * x124;
-      acc62 += static_cast<double>(x242);
-      const TB x177 = s.QC[2] * x63 + s.WQ[2] * x73 + r2 * x37;
-      const TB x157 = s.QC[2] * x31 + s.WQ[2] * x37 + r1 * x13;
-      const TB x146 = s.QC[2] * x10 + s.WQ[2] * x13;
-      const TB x156 = s.QC[2] * x25 + s.WQ[2] * x31 + r1 * x10;
-      const TB x264 = s.QC[2] * x156 + s.WQ[2] * x157 + e1 * (x25 - s.roe * x31) + r1 * x146;
-      acc84 += static_cast<double>(x264);
-      const TB x176 = s.QC[2] * x53 + s.WQ[2] * x63 + r2 * x31;
-      const TB x274 = s.QC[2] * x176 + s.WQ[2] * x177 + e1 * (x53 - s.roe * x63) + r2 * x157;
-      acc94 += static_cast<double>(x274);
-      const TB x179 = s.QC[2] * x64 + s.WQ[2] * x74 + r3 * x38;
-      const TB x275 = s.QC[2] * x178 + s.WQ[2] * x179 + e1 * (x54 - s.roe * x64) + r3 * x159;
-      acc95 += static_cast<double>(x275);
-      const TB x144 = s.QC[1] * x64 + s.WQ[1] * x74;
-      const TB x143 = s.QC[1] * x54 + s.WQ[1] * x64;
-      const TB x243 = s.QC[1] * x143 + s.WQ[1] * x144 + e1 * (x54 - s.roe * x64);
-      acc63 += static_cast<double>(x243);
-      const TB x259 = s.QC[2] * x143 + s.WQ[2] * x144 + r3 * x124;
-      acc79 += static_cast<double>(x259);
-      const TB x148 = s.QC[2] * x21 + s.WQ[2] * x27;
-      const TB x260 = s.QC[2] * x148 + s.WQ[2] * x149 + e1 * (x21 - s.roe * x27);
-      acc80 += static_cast<double>(x260);
-      const TB x164 = s.QC[2] * x47 + s.WQ[2] * x57 + r1 * x27;
-      const TB x268 = s.QC[2] * x164 + s.WQ[2] * x165 + e1 * (x47 - s.roe * x57) + r1 * x149;
-      acc88 += static_cast<double>(x268);
-      const TB x150 = s.QC[2] * x22 + s.WQ[2] * x28;
-      const TB x261 = s.QC[2] * x150 + s.WQ[2] * x151 + e1 * (x22 - s.roe * x28);
-      acc81 += static_cast<double>(x261);
-      const TB x168 = s.QC[2] * x49 + s.WQ[2] * x59 + r1 * x28;
-      const TB x270 = s.QC[2] * x168 + s.WQ[2] * x169 + e1 * (x49 - s.roe * x59) + r1 * x151;
-      acc90 += static_cast<double>(x270);
-      const TB x154 = s.QC[2] * x24 + s.WQ[2] * x30;
-      const TB x263 = s.QC[2] * x154 + s.WQ[2] * x155 + e1 * (x24 - s.roe * x30);
-      acc83 += static_cast<double>(x263);
-      const TB x174 = s.QC[2] * x52 + s.WQ[2] * x62 + r1 * x30;
-      const TB x273 = s.QC[2] * x174 + s.WQ[2] * x175 + e1 * (x52 - s.roe * x62) + r1 * x155;
-      acc93 += static_cast<double>(x273);
-      const TB x160 = s.QC[2] * x45 + s.WQ[2] * x55;
-      const TB x266 = s.QC[2] * x160 + s.WQ[2] * x161 + e1 * (x45 - s.roe * x55);
-      acc86 += static_cast<double>(x266);
-      const TB x162 = s.QC[2] * x46 + s.WQ[2] * x56;
-      const TB x267 = s.QC[2] * x162 + s.WQ[2] * x163 + e1 * (x46 - s.roe * x56);
-      acc87 += static_cast<double>(x267);
-      const TB x166 = s.QC[2] * x48 + s.WQ[2] * x58;
-      const TB x269 = s.QC[2] * x166 + s.WQ[2] * x167 + e1 * (x48 - s.roe * x58);
-      acc89 += static_cast<double>(x269);
-      const TB x172 = s.QC[2] * x51 + s.WQ[2] * x61;
-      const TB x272 = s.QC[2] * x172 + s.WQ[2] * x173 + e1 * (x51 - s.roe * x61);
-      acc92 += static_cast<double>(x272);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-    accmax = fmax(accmax, fabs(acc31));
-    accmax = fmax(accmax, fabs(acc32));
-    accmax = fmax(accmax, fabs(acc33));
-    accmax = fmax(accmax, fabs(acc34));
-    accmax = fmax(accmax, fabs(acc35));
-    accmax = fmax(accmax, fabs(acc36));
-    accmax = fmax(accmax, fabs(acc37));
-    accmax = fmax(accmax, fabs(acc38));
-    accmax = fmax(accmax, fabs(acc39));
-    accmax = fmax(accmax, fabs(acc40));
-    accmax = fmax(accmax, fabs(acc41));
-    accmax = fmax(accmax, fabs(acc42));
-    accmax = fmax(accmax, fabs(acc43));
-    accmax = fmax(accmax, fabs(acc44));
-    accmax = fmax(accmax, fabs(acc45));
-    accmax = fmax(accmax, fabs(acc46));
-    accmax = fmax(accmax, fabs(acc47));
-    accmax = fmax(accmax, fabs(acc48));
-    accmax = fmax(accmax, fabs(acc49));
-    accmax = fmax(accmax, fabs(acc50));
-    accmax = fmax(accmax, fabs(acc51));
-    accmax = fmax(accmax, fabs(acc52));
-    accmax = fmax(accmax, fabs(acc53));
-    accmax = fmax(accmax, fabs(acc54));
-    accmax = fmax(accmax, fabs(acc55));
-    accmax = fmax(accmax, fabs(acc56));
-    accmax = fmax(accmax, fabs(acc57));
-    accmax = fmax(accmax, fabs(acc58));
-    accmax = fmax(accmax, fabs(acc59));
-    accmax = fmax(accmax, fabs(acc60));
-    accmax = fmax(accmax, fabs(acc61));
-    accmax = fmax(accmax, fabs(acc62));
-    accmax = fmax(accmax, fabs(acc63));
-    accmax = fmax(accmax, fabs(acc64));
-    accmax = fmax(accmax, fabs(acc65));
-    accmax = fmax(accmax, fabs(acc66));
-    accmax = fmax(accmax, fabs(acc67));
-    accmax = fmax(accmax, fabs(acc68));
-    accmax = fmax(accmax, fabs(acc69));
-    accmax = fmax(accmax, fabs(acc70));
-    accmax = fmax(accmax, fabs(acc71));
-    accmax = fmax(accmax, fabs(acc72));
-    accmax = fmax(accmax, fabs(acc73));
-    accmax = fmax(accmax, fabs(acc74));
-    accmax = fmax(accmax, fabs(acc75));
-    accmax = fmax(accmax, fabs(acc76));
-    accmax = fmax(accmax, fabs(acc77));
-    accmax = fmax(accmax, fabs(acc78));
-    accmax = fmax(accmax, fabs(acc79));
-    accmax = fmax(accmax, fabs(acc80));
-    accmax = fmax(accmax, fabs(acc81));
-    accmax = fmax(accmax, fabs(acc82));
-    accmax = fmax(accmax, fabs(acc83));
-    accmax = fmax(accmax, fabs(acc84));
-    accmax = fmax(accmax, fabs(acc85));
-    accmax = fmax(accmax, fabs(acc86));
-    accmax = fmax(accmax, fabs(acc87));
-    accmax = fmax(accmax, fabs(acc88));
-    accmax = fmax(accmax, fabs(acc89));
-    accmax = fmax(accmax, fabs(acc90));
-    accmax = fmax(accmax, fabs(acc91));
-    accmax = fmax(accmax, fabs(acc92));
-    accmax = fmax(accmax, fabs(acc93));
-    accmax = fmax(accmax, fabs(acc94));
-    accmax = fmax(accmax, fabs(acc95));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = acc31;
-  const double h32 = acc32;
-  const double h33 = acc33;
-  const double h34 = acc34;
-  const double h35 = acc35;
-  const double h36 = acc36;
-  const double h37 = acc37;
-  const double h38 = acc38;
-  const double h39 = acc39;
-  const double h40 = acc40;
-  const double h41 = acc41;
-  const double h42 = acc42;
-  const double h43 = acc43;
-  const double h44 = acc44;
-  const double h45 = acc45;
-  const double h46 = acc46;
-  const double h47 = acc47;
-  const double h48 = acc48;
-  const double h49 = acc49;
-  const double h50 = acc50;
-  const double h51 = acc51;
-  const double h52 = acc52;
-  const double h53 = acc53;
-  const double h54 = acc54;
-  const double h55 = acc55;
-  const double h56 = acc56;
-  const double h57 = acc57;
-  const double h58 = acc58;
-  const double h59 = acc59;
-  const double h60 = acc60;
-  const double h61 = acc61;
-  const double h62 = acc62;
-  const double h63 = acc63;
-  const double h64 = acc64;
-  const double h65 = acc65;
-  const double h66 = acc66;
-  const double h67 = acc67;
-  const double h68 = acc68;
-  const double h69 = acc69;
-  const double h70 = acc70;
-  const double h71 = acc71;
-  const double h72 = acc72;
-  const double h73 = acc73;
-  const double h74 = acc74;
-  const double h75 = acc75;
-  const double h76 = acc76;
-  const double h77 = acc77;
-  const double h78 = acc78;
-  const double h79 = acc79;
-  const double h80 = acc80;
-  const double h81 = acc81;
-  const double h82 = acc82;
-  const double h83 = acc83;
-  const double h84 = acc84;
-  const double h85 = acc85;
-  const double h86 = acc86;
-  const double h87 = acc87;
-  const double h88 = acc88;
-  const double h89 = acc89;
-  const double h90 = acc90;
-  const double h91 = acc91;
-  const double h92 = acc92;
-  const double h93 = acc93;
-  const double h94 = acc94;
-  const double h95 = acc95;
-  const double h96 = h95 + AB[2] * h85;
-  const double h97 = h79 + AB[2] * h69;
-  const double h98 = h63 + AB[2] * h53;
-  const double h99 = h47 + AB[2] * h37;
-  const double h100 = h31 + AB[2] * h21;
-  const double h101 = h15 + AB[2] * h5;
-  const double h102 = h94 + AB[1] * h85;
-  const double h103 = h78 + AB[1] * h69;
-  const double h104 = h62 + AB[1] * h53;
-  const double h105 = h46 + AB[1] * h37;
-  const double h106 = h30 + AB[1] * h21;
-  const double h107 = h14 + AB[1] * h5;
-  const double h108 = h91 + AB[0] * h85;
-  const double h109 = h75 + AB[0] * h69;
-  const double h110 = h59 + AB[0] * h53;
-  const double h111 = h43 + AB[0] * h37;
-  const double h112 = h27 + AB[0] * h21;
-  const double h113 = h11 + AB[0] * h5;
-  const double h114 = h94 + AB[2] * h84;
-  const double h115 = h78 + AB[2] * h68;
-  const double h116 = h62 + AB[2] * h52;
-  const double h117 = h46 + AB[2] * h36;
-  const double h118 = h30 + AB[2] * h20;
-  const double h119 = h14 + AB[2] * h4;
-  const double h120 = h93 + AB[1] * h84;
-  const double h121 = h77 + AB[1] * h68;
-  const double h122 = h61 + AB[1] * h52;
-  const double h123 = h45 + AB[1] * h36;
-  const double h124 = h29 + AB[1] * h20;
-  const double h125 = h13 + AB[1] * h4;
-  const double h126 = h90 + AB[0] * h84;
-  const double h127 = h74 + AB[0] * h68;
-  const double h128 = h58 + AB[0] * h52;
-  const double h129 = h42 + AB[0] * h36;
-  const double h130 = h26 + AB[0] * h20;
-  const double h131 = h10 + AB[0] * h4;
-  const double h132 = h93 + AB[2] * h83;
-  const double h133 = h77 + AB[2] * h67;
-  const double h134 = h61 + AB[2] * h51;
-  const double h135 = h45 + AB[2] * h35;
-  const double h136 = h29 + AB[2] * h19;
-  const double h137 = h13 + AB[2] * h3;
-  const double h138 = h92 + AB[1] * h83;
-  const double h139 = h76 + AB[1] * h67;
-  const double h140 = h60 + AB[1] * h51;
-  const double h141 = h44 + AB[1] * h35;
-  const double h142 = h28 + AB[1] * h19;
-  const double h143 = h12 + AB[1] * h3;
-  const double h144 = h89 + AB[0] * h83;
-  const double h145 = h73 + AB[0] * h67;
-  const double h146 = h57 + AB[0] * h51;
-  const double h147 = h41 + AB[0] * h35;
-  const double h148 = h25 + AB[0] * h19;
-  const double h149 = h9 + AB[0] * h3;
-  const double h150 = h91 + AB[2] * h82;
-  const double h151 = h75 + AB[2] * h66;
-  const double h152 = h59 + AB[2] * h50;
-  const double h153 = h43 + AB[2] * h34;
-  const double h154 = h27 + AB[2] * h18;
-  const double h155 = h11 + AB[2] * h2;
-  const double h156 = h90 + AB[1] * h82;
-  const double h157 = h74 + AB[1] * h66;
-  const double h158 = h58 + AB[1] * h50;
-  const double h159 = h42 + AB[1] * h34;
-  const double h160 = h26 + AB[1] * h18;
-  const double h161 = h10 + AB[1] * h2;
-  const double h162 = h88 + AB[0] * h82;
-  const double h163 = h72 + AB[0] * h66;
-  const double h164 = h56 + AB[0] * h50;
-  const double h165 = h40 + AB[0] * h34;
-  const double h166 = h24 + AB[0] * h18;
-  const double h167 = h8 + AB[0] * h2;
-  const double h168 = h90 + AB[2] * h81;
-  const double h169 = h74 + AB[2] * h65;
-  const double h170 = h58 + AB[2] * h49;
-  const double h171 = h42 + AB[2] * h33;
-  const double h172 = h26 + AB[2] * h17;
-  const double h173 = h10 + AB[2] * h1;
-  const double h174 = h89 + AB[1] * h81;
-  const double h175 = h73 + AB[1] * h65;
-  const double h176 = h57 + AB[1] * h49;
-  const double h177 = h41 + AB[1] * h33;
-  const double h178 = h25 + AB[1] * h17;
-  const double h179 = h9 + AB[1] * h1;
-  const double h180 = h87 + AB[0] * h81;
-  const double h181 = h71 + AB[0] * h65;
-  const double h182 = h55 + AB[0] * h49;
-  const double h183 = h39 + AB[0] * h33;
-  const double h184 = h23 + AB[0] * h17;
-  const double h185 = h7 + AB[0] * h1;
-  const double h186 = h88 + AB[2] * h80;
-  const double h187 = h72 + AB[2] * h64;
-  const double h188 = h56 + AB[2] * h48;
-  const double h189 = h40 + AB[2] * h32;
-  const double h190 = h24 + AB[2] * h16;
-  const double h191 = h8 + AB[2] * h0;
-  const double h192 = h87 + AB[1] * h80;
-  const double h193 = h71 + AB[1] * h64;
-  const double h194 = h55 + AB[1] * h48;
-  const double h195 = h39 + AB[1] * h32;
-  const double h196 = h23 + AB[1] * h16;
-  const double h197 = h7 + AB[1] * h0;
-  const double h198 = h86 + AB[0] * h80;
-  const double h199 = h70 + AB[0] * h64;
-  const double h200 = h54 + AB[0] * h48;
-  const double h201 = h38 + AB[0] * h32;
-  const double h202 = h22 + AB[0] * h16;
-  const double h203 = h6 + AB[0] * h0;
-  double out[108];
-  out[0] = h203 * 0.3333333333333334;
-  out[1] = h202 * 0.5773502691896258;
-  out[2] = h201 * 0.5773502691896258;
-  out[3] = h200 * 0.3333333333333334;
-  out[4] = h199 * 0.5773502691896258;
-  out[5] = h198 * 0.3333333333333334;
-  out[6] = h197 * 0.3333333333333334;
-  out[7] = h196 * 0.5773502691896258;
-  out[8] = h195 * 0.5773502691896258;
-  out[9] = h194 * 0.3333333333333334;
-  out[10] = h193 * 0.5773502691896258;
-  out[11] = h192 * 0.3333333333333334;
-  out[12] = h191 * 0.3333333333333334;
-  out[13] = h190 * 0.5773502691896258;
-  out[14] = h189 * 0.5773502691896258;
-  out[15] = h188 * 0.3333333333333334;
-  out[16] = h187 * 0.5773502691896258;
-  out[17] = h186 * 0.3333333333333334;
-  out[18] = h185 * 0.5773502691896258;
-  out[19] = h184;
-  out[20] = h183;
-  out[21] = h182 * 0.5773502691896258;
-  out[22] = h181;
-  out[23] = h180 * 0.5773502691896258;
-  out[24] = h179 * 0.5773502691896258;
-  out[25] = h178;
-  out[26] = h177;
-  out[27] = h176 * 0.5773502691896258;
-  out[28] = h175;
-  out[29] = h174 * 0.5773502691896258;
-  out[30] = h173 * 0.5773502691896258;
-  out[31] = h172;
-  out[32] = h171;
-  out[33] = h170 * 0.5773502691896258;
-  out[34] = h169;
-  out[35] = h168 * 0.5773502691896258;
-  out[36] = h167 * 0.5773502691896258;
-  out[37] = h166;
-  out[38] = h165;
-  out[39] = h164 * 0.5773502691896258;
-  out[40] = h163;
-  out[41] = h162 * 0.5773502691896258;
-  out[42] = h161 * 0.5773502691896258;
-  out[43] = h160;
-  out[44] = h159;
-  out[45] = h158 * 0.5773502691896258;
-  out[46] = h157;
-  out[47] = h156 * 0.5773502691896258;
-  out[48] = h155 * 0.5773502691896258;
-  out[49] = h154;
-  out[50] = h153;
-  out[51] = h152 * 0.5773502691896258;
-  out[52] = h151;
-  out[53] = h150 * 0.5773502691896258;
-  out[54] = h149 * 0.3333333333333334;
-  out[55] = h148 * 0.5773502691896258;
-  out[56] = h147 * 0.5773502691896258;
-  out[57] = h146 * 0.3333333333333334;
-  out[58] = h145 * 0.5773502691896258;
-  out[59] = h144 * 0.3333333333333334;
-  out[60] = h143 * 0.3333333333333334;
-  out[61] = h142 * 0.5773502691896258;
-  out[62] = h141 * 0.5773502691896258;
-  out[63] = h140 * 0.3333333333333334;
-  out[64] = h139 * 0.5773502691896258;
-  out[65] = h138 * 0.3333333333333334;
-  out[66] = h137 * 0.3333333333333334;
-  out[67] = h136 * 0.5773502691896258;
-  out[68] = h135 * 0.5773502691896258;
-  out[69] = h134 * 0.3333333333333334;
-  out[70] = h133 * 0.5773502691896258;
-  out[71] = h132 * 0.3333333333333334;
-  out[72] = h131 * 0.5773502691896258;
-  out[73] = h130;
-  out[74] = h129;
-  out[75] = h128 * 0.5773502691896258;
-  out[76] = h127;
-  out[77] = h126 * 0.5773502691896258;
-  out[78] = h125 * 0.5773502691896258;
-  out[79] = h124;
-  out[80] = h123;
-  out[81] = h122 * 0.5773502691896258;
-  out[82] = h121;
-  out[83] = h120 * 0.5773502691896258;
-  out[84] = h119 * 0.5773502691896258;
-  out[85] = h118;
-  out[86] = h117;
-  out[87] = h116 * 0.5773502691896258;
-  out[88] = h115;
-  out[89] = h114 * 0.5773502691896258;
-  out[90] = h113 * 0.3333333333333334;
-  out[91] = h112 * 0.5773502691896258;
-  out[92] = h111 * 0.5773502691896258;
-  out[93] = h110 * 0.3333333333333334;
-  out[94] = h109 * 0.5773502691896258;
-  out[95] = h108 * 0.3333333333333334;
-  out[96] = h107 * 0.3333333333333334;
-  out[97] = h106 * 0.5773502691896258;
-  out[98] = h105 * 0.5773502691896258;
-  out[99] = h104 * 0.3333333333333334;
-  out[100] = h103 * 0.5773502691896258;
-  out[101] = h102 * 0.3333333333333334;
-  out[102] = h101 * 0.3333333333333334;
-  out[103] = h100 * 0.5773502691896258;
-  out[104] = h99 * 0.5773502691896258;
-  out[105] = h98 * 0.3333333333333334;
-  out[106] = h97 * 0.5773502691896258;
-  out[107] = h96 * 0.3333333333333334;
-  store_block<6, 3, 6, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (21|21) split 0/3: 234 recursion values, 85 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2121_0(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc16 = 0.0;
-  double acc32 = 0.0;
-  double acc48 = 0.0;
-  double acc64 = 0.0;
-  double acc80 = 0.0;
-  double acc96 = 0.0;
-  double acc112 = 0.0;
-  double acc128 = 0.0;
-  double acc144 = 0.0;
-  double acc160 = 0.0;
-  double acc176 = 0.0;
-  double acc192 = 0.0;
-  double acc208 = 0.0;
-  double acc224 = 0.0;
-  double acc240 = 0.0;
-  double acc1 = 0.0;
-  double acc17 = 0.0;
-  double acc33 = 0.0;
-  double acc49 = 0.0;
-  double acc65 = 0.0;
-  double acc81 = 0.0;
-  double acc97 = 0.0;
-  double acc113 = 0.0;
-  double acc129 = 0.0;
-  double acc145 = 0.0;
-  double acc161 = 0.0;
-  double acc177 = 0.0;
-  double acc193 = 0.0;
-  double acc209 = 0.0;
-  double acc225 = 0.0;
-  double acc241 = 0.0;
-  double acc2 = 0.0;
-  double acc18 = 0.0;
-  double acc34 = 0.0;
-  double acc50 = 0.0;
-  double acc66 = 0.0;
-  double acc82 = 0.0;
-  double acc98 = 0.0;
-  double acc114 = 0.0;
-  double acc130 = 0.0;
-  double acc146 = 0.0;
-  double acc162 = 0.0;
-  double acc178 = 0.0;
-  double acc194 = 0.0;
-  double acc210 = 0.0;
-  double acc226 = 0.0;
-  double acc242 = 0.0;
-  double acc3 = 0.0;
-  double acc19 = 0.0;
-  double acc35 = 0.0;
-  double acc51 = 0.0;
-  double acc67 = 0.0;
-  double acc83 = 0.0;
-  double acc99 = 0.0;
-  double acc115 = 0.0;
-  double acc131 = 0.0;
-  double acc147 = 0.0;
-  double acc163 = 0.0;
-  double acc179 = 0.0;
-  double acc195 = 0.0;
-  double acc211 = 0.0;
-  double acc227 = 0.0;
-  double acc243 = 0.0;
-  double acc4 = 0.0;
-  double acc20 = 0.0;
-  double acc36 = 0.0;
-  double acc52 = 0.0;
-  double acc68 = 0.0;
-  double acc84 = 0.0;
-  double acc100 = 0.0;
-  double acc116 = 0.0;
-  double acc132 = 0.0;
-  double acc148 = 0.0;
-  double acc164 = 0.0;
-  double acc180 = 0.0;
-  double acc196 = 0.0;
-  double acc212 = 0.0;
-  double acc228 = 0.0;
-  double acc244 = 0.0;
-  double acc5 = 0.0;
-  double acc21 = 0.0;
-  double acc37 = 0.0;
-  double acc53 = 0.0;
-  double acc69 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x8 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x20 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x25 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x28 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x27 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x31 = s.PA[0] * x10 + s.WP[0] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x37 = s.PA[0] * x13 + s.WP[0] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x43 = s.PA[0] * x16 + s.WP[0] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x46 = s.PA[1] * x17 + s.WP[1] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x47 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x45 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x95 = s.QC[0] * x2 + s.WQ[0] * x3;
-      const TB x96 = s.QC[0] * x10 + s.WQ[0] * x13 + r1 * x2;
-      const TB x97 = s.QC[0] * x13 + s.WQ[0] * x16 + r1 * x3;
-      const TB x98 = s.QC[0] * x11 + s.WQ[0] * x14;
-      const TB x99 = s.QC[0] * x14 + s.WQ[0] * x17;
-      const TB x100 = s.QC[0] * x12 + s.WQ[0] * x15;
-      const TB x101 = s.QC[0] * x15 + s.WQ[0] * x18;
-      const TB x102 = s.QC[0] * x25 + s.WQ[0] * x31 + r2 * x10;
-      const TB x103 = s.QC[0] * x31 + s.WQ[0] * x37 + r2 * x13;
-      const TB x104 = s.QC[0] * x37 + s.WQ[0] * x43 + r2 * x16;
-      const TB x105 = s.QC[0] * x26 + s.WQ[0] * x32 + r1 * x11;
-      const TB x106 = s.QC[0] * x32 + s.WQ[0] * x38 + r1 * x14;
-      const TB x107 = s.QC[0] * x38 + s.WQ[0] * x44 + r1 * x17;
-      const TB x108 = s.QC[0] * x27 + s.WQ[0] * x33 + r1 * x12;
-      const TB x109 = s.QC[0] * x33 + s.WQ[0] * x39 + r1 * x15;
-      const TB x110 = s.QC[0] * x39 + s.WQ[0] * x45 + r1 * x18;
-      const TB x111 = s.QC[0] * x28 + s.WQ[0] * x34;
-      const TB x112 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x113 = s.QC[0] * x40 + s.WQ[0] * x46;
-      const TB x114 = s.QC[0] * x29 + s.WQ[0] * x35;
-      const TB x115 = s.QC[0] * x35 + s.WQ[0] * x41;
-      const TB x116 = s.QC[0] * x41 + s.WQ[0] * x47;
-      const TB x117 = s.QC[0] * x30 + s.WQ[0] * x36;
-      const TB x118 = s.QC[0] * x36 + s.WQ[0] * x42;
-      const TB x173 = s.QC[1] * x36 + s.WQ[1] * x42;
-      const TB x273 = s.QC[0] * x117 + s.WQ[0] * x118 + e1 * (x30 - s.roe * x36);
-      acc5 += static_cast<double>(x273);
-      const TB x308 = s.QC[1] * x117 + s.WQ[1] * x118;
-      acc21 += static_cast<double>(x308);
-      const TB x343 = s.QC[2] * x117 + s.WQ[2] * x118 + r2 * x100;
-      acc37 += static_cast<double>(x343);
-      const TB x150 = s.QC[1] * x2 + s.WQ[1] * x3;
-      const TB x153 = s.QC[1] * x11 + s.WQ[1] * x14 + r1 * x2;
-      const TB x154 = s.QC[1] * x14 + s.WQ[1] * x17 + r1 * x3;
-      const TB x366 = s.QC[1] * x153 + s.WQ[1] * x154 + e1 * (x11 - s.roe * x14) + r1 * x150;
-      const TB x151 = s.QC[1] * x10 + s.WQ[1] * x13;
-      const TB x152 = s.QC[1] * x13 + s.WQ[1] * x16;
-      const TB x155 = s.QC[1] * x12 + s.WQ[1] * x15;
-      const TB x156 = s.QC[1] * x15 + s.WQ[1] * x18;
-      const TB x171 = s.QC[1] * x41 + s.WQ[1] * x47 + r1 * x18;
-      const TB x226 = s.QC[2] * x41 + s.WQ[2] * x47 + r1 * x17;
-      const TB x157 = s.QC[1] * x25 + s.WQ[1] * x31;
-      const TB x158 = s.QC[1] * x31 + s.WQ[1] * x37;
-      const TB x159 = s.QC[1] * x37 + s.WQ[1] * x43;
-      const TB x214 = s.QC[2] * x37 + s.WQ[2] * x43;
-      const TB x160 = s.QC[1] * x26 + s.WQ[1] * x32 + r1 * x10;
-      const TB x161 = s.QC[1] * x32 + s.WQ[1] * x38 + r1 * x13;
-      const TB x162 = s.QC[1] * x38 + s.WQ[1] * x44 + r1 * x16;
-      const TB x217 = s.QC[2] * x38 + s.WQ[2] * x44;
-      const TB x163 = s.QC[1] * x27 + s.WQ[1] * x33;
-      const TB x164 = s.QC[1] * x33 + s.WQ[1] * x39;
-      const TB x165 = s.QC[1] * x39 + s.WQ[1] * x45;
-      const TB x220 = s.QC[2] * x39 + s.WQ[2] * x45 + r1 * x16;
-      const TB x207 = s.QC[2] * x13 + s.WQ[2] * x16;
-      const TB x166 = s.QC[1] * x28 + s.WQ[1] * x34 + r2 * x11;
-      const TB x167 = s.QC[1] * x34 + s.WQ[1] * x40 + r2 * x14;
-      const TB x168 = s.QC[1] * x40 + s.WQ[1] * x46 + r2 * x17;
-      const TB x209 = s.QC[2] * x14 + s.WQ[2] * x17;
-      const TB x223 = s.QC[2] * x40 + s.WQ[2] * x46;
-      const TB x169 = s.QC[1] * x29 + s.WQ[1] * x35 + r1 * x12;
-      const TB x170 = s.QC[1] * x35 + s.WQ[1] * x41 + r1 * x15;
-      const TB x172 = s.QC[1] * x30 + s.WQ[1] * x36;
-      const TB x378 = s.QC[1] * x172 + s.WQ[1] * x173 + e1 * (x30 - s.roe * x36);
-      acc53 += static_cast<double>(x378);
-      const TB x413 = s.QC[2] * x172 + s.WQ[2] * x173 + r2 * x155;
-      acc69 += static_cast<double>(x413);
-      const TB x206 = s.QC[2] * x10 + s.WQ[2] * x13;
-      const TB x208 = s.QC[2] * x11 + s.WQ[2] * x14;
-      const TB x212 = s.QC[2] * x25 + s.WQ[2] * x31;
-      const TB x213 = s.QC[2] * x31 + s.WQ[2] * x37;
-      const TB x439 = s.QC[2] * x213 + s.WQ[2] * x214 + e1 * (x31 - s.roe * x37);
-      const TB x215 = s.QC[2] * x26 + s.WQ[2] * x32;
-      const TB x216 = s.QC[2] * x32 + s.WQ[2] * x38;
-      const TB x441 = s.QC[2] * x216 + s.WQ[2] * x217 + e1 * (x32 - s.roe * x38);
-      const TB x218 = s.QC[2] * x27 + s.WQ[2] * x33 + r1 * x10;
-      const TB x219 = s.QC[2] * x33 + s.WQ[2] * x39 + r1 * x13;
-      const TB x443 = s.QC[2] * x219 + s.WQ[2] * x220 + e1 * (x33 - s.roe * x39) + r1 * x207;
-      const TB x435 = s.QC[2] * x206 + s.WQ[2] * x207 + e1 * (x10 - s.roe * x13);
-      const TB x442 = s.QC[2] * x218 + s.WQ[2] * x219 + e1 * (x27 - s.roe * x33) + r1 * x206;
-      acc82 += static_cast<double>(x442);
-      const TB x616 = s.QC[2] * x442 + s.WQ[2] * x443 + e2 * (x218 - s.roe * x219) + r1 * x435;
-      acc242 += static_cast<double>(x616);
-      const TB x221 = s.QC[2] * x28 + s.WQ[2] * x34;
-      const TB x222 = s.QC[2] * x34 + s.WQ[2] * x40;
-      const TB x445 = s.QC[2] * x222 + s.WQ[2] * x223 + e1 * (x34 - s.roe * x40);
-      const TB x224 = s.QC[2] * x29 + s.WQ[2] * x35 + r1 * x11;
-      const TB x225 = s.QC[2] * x35 + s.WQ[2] * x41 + r1 * x14;
-      const TB x447 = s.QC[2] * x225 + s.WQ[2] * x226 + e1 * (x35 - s.roe * x41) + r1 * x209;
-      const TB x436 = s.QC[2] * x208 + s.WQ[2] * x209 + e1 * (x11 - s.roe * x14);
-      const TB x261 = s.QC[0] * x98 + s.WQ[0] * x99 + e1 * (x11 - s.roe * x14);
-      const TB x446 = s.QC[2] * x224 + s.WQ[2] * x225 + e1 * (x29 - s.roe * x35) + r1 * x208;
-      acc84 += static_cast<double>(x446);
-      const TB x618 = s.QC[2] * x446 + s.WQ[2] * x447 + e2 * (x224 - s.roe * x225) + r1 * x436;
-      acc244 += static_cast<double>(x618);
-      const TB x260 = s.QC[0] * x96 + s.WQ[0] * x97 + e1 * (x10 - s.roe * x13) + r1 * x95;
-      const TB x365 = s.QC[1] * x151 + s.WQ[1] * x152 + e1 * (x10 - s.roe * x13);
-      const TB x296 = s.QC[1] * x98 + s.WQ[1] * x99 + r1 * x95;
-      const TB x262 = s.QC[0] * x100 + s.WQ[0] * x101 + e1 * (x12 - s.roe * x15);
-      const TB x367 = s.QC[1] * x155 + s.WQ[1] * x156 + e1 * (x12 - s.roe * x15);
-      const TB x376 = s.QC[1] * x169 + s.WQ[1] * x170 + e1 * (x29 - s.roe * x35) + r1 * x155;
-      acc52 += static_cast<double>(x376);
-      const TB x271 = s.QC[0] * x114 + s.WQ[0] * x115 + e1 * (x29 - s.roe * x35);
-      acc4 += static_cast<double>(x271);
-      const TB x377 = s.QC[1] * x170 + s.WQ[1] * x171 + e1 * (x35 - s.roe * x41) + r1 * x156;
-      const TB x272 = s.QC[0] * x115 + s.WQ[0] * x116 + e1 * (x35 - s.roe * x41);
-      const TB x570 = s.QC[1] * x376 + s.WQ[1] * x377 + e2 * (x169 - s.roe * x170) + r1 * x367;
-      acc196 += static_cast<double>(x570);
-      const TB x586 = s.QC[2] * x376 + s.WQ[2] * x377 + r1 * x366;
-      acc212 += static_cast<double>(x586);
-      const TB x412 = s.QC[2] * x170 + s.WQ[2] * x171 + r1 * x154;
-      const TB x474 = s.QC[0] * x271 + s.WQ[0] * x272 + e2 * (x114 - s.roe * x115);
-      acc100 += static_cast<double>(x474);
-      const TB x490 = s.QC[1] * x271 + s.WQ[1] * x272 + r1 * x262;
-      acc116 += static_cast<double>(x490);
-      const TB x506 = s.QC[2] * x271 + s.WQ[2] * x272 + r1 * x261;
-      acc132 += static_cast<double>(x506);
-      const TB x263 = s.QC[0] * x102 + s.WQ[0] * x103 + e1 * (x25 - s.roe * x31) + r2 * x96;
-      acc0 += static_cast<double>(x263);
-      const TB x264 = s.QC[0] * x103 + s.WQ[0] * x104 + e1 * (x31 - s.roe * x37) + r2 * x97;
-      const TB x369 = s.QC[1] * x158 + s.WQ[1] * x159 + e1 * (x31 - s.roe * x37);
-      const TB x404 = s.QC[2] * x158 + s.WQ[2] * x159;
-      const TB x470 = s.QC[0] * x263 + s.WQ[0] * x264 + e2 * (x102 - s.roe * x103) + r2 * x260;
-      acc96 += static_cast<double>(x470);
-      const TB x486 = s.QC[1] * x263 + s.WQ[1] * x264;
-      acc112 += static_cast<double>(x486);
-      const TB x502 = s.QC[2] * x263 + s.WQ[2] * x264;
-      acc128 += static_cast<double>(x502);
-      const TB x265 = s.QC[0] * x105 + s.WQ[0] * x106 + e1 * (x26 - s.roe * x32) + r1 * x98;
-      acc1 += static_cast<double>(x265);
-      const TB x266 = s.QC[0] * x106 + s.WQ[0] * x107 + e1 * (x32 - s.roe * x38) + r1 * x99;
-      const TB x371 = s.QC[1] * x161 + s.WQ[1] * x162 + e1 * (x32 - s.roe * x38) + r1 * x152;
-      const TB x406 = s.QC[2] * x161 + s.WQ[2] * x162;
-      const TB x471 = s.QC[0] * x265 + s.WQ[0] * x266 + e2 * (x105 - s.roe * x106) + r1 * x261;
-      acc97 += static_cast<double>(x471);
-      const TB x487 = s.QC[1] * x265 + s.WQ[1] * x266 + r1 * x260;
-      acc113 += static_cast<double>(x487);
-      const TB x503 = s.QC[2] * x265 + s.WQ[2] * x266;
-      acc129 += static_cast<double>(x503);
-      const TB x267 = s.QC[0] * x108 + s.WQ[0] * x109 + e1 * (x27 - s.roe * x33) + r1 * x100;
-      acc2 += static_cast<double>(x267);
-      const TB x372 = s.QC[1] * x163 + s.WQ[1] * x164 + e1 * (x27 - s.roe * x33);
-      acc50 += static_cast<double>(x372);
-      const TB x268 = s.QC[0] * x109 + s.WQ[0] * x110 + e1 * (x33 - s.roe * x39) + r1 * x101;
-      const TB x373 = s.QC[1] * x164 + s.WQ[1] * x165 + e1 * (x33 - s.roe * x39);
-      const TB x472 = s.QC[0] * x267 + s.WQ[0] * x268 + e2 * (x108 - s.roe * x109) + r1 * x262;
-      acc98 += static_cast<double>(x472);
-      const TB x504 = s.QC[2] * x267 + s.WQ[2] * x268 + r1 * x260;
-      acc130 += static_cast<double>(x504);
-      const TB x488 = s.QC[1] * x267 + s.WQ[1] * x268;
-      acc114 += static_cast<double>(x488);
-      const TB x408 = s.QC[2] * x164 + s.WQ[2] * x165 + r1 * x152;
-      const TB x400 = s.QC[2] * x151 + s.WQ[2] * x152;
-      const TB x568 = s.QC[1] * x372 + s.WQ[1] * x373 + e2 * (x163 - s.roe * x164);
-      acc194 += static_cast<double>(x568);
-      const TB x584 = s.QC[2] * x372 + s.WQ[2] * x373 + r1 * x365;
-      acc210 += static_cast<double>(x584);
-      const TB x269 = s.QC[0] * x111 + s.WQ[0] * x112 + e1 * (x28 - s.roe * x34);
-      acc3 += static_cast<double>(x269);
-      const TB x270 = s.QC[0] * x112 + s.WQ[0] * x113 + e1 * (x34 - s.roe * x40);
-      const TB x489 = s.QC[1] * x269 + s.WQ[1] * x270 + r2 * x261;
-      acc115 += static_cast<double>(x489);
-      const TB x375 = s.QC[1] * x167 + s.WQ[1] * x168 + e1 * (x34 - s.roe * x40) + r2 * x154;
-      const TB x401 = s.QC[2] * x153 + s.WQ[2] * x154;
-      const TB x410 = s.QC[2] * x167 + s.WQ[2] * x168;
-      const TB x473 = s.QC[0] * x269 + s.WQ[0] * x270 + e2 * (x111 - s.roe * x112);
-      acc99 += static_cast<double>(x473);
-      const TB x505 = s.QC[2] * x269 + s.WQ[2] * x270;
-      acc131 += static_cast<double>(x505);
-      const TB x295 = s.QC[1] * x96 + s.WQ[1] * x97;
-      const TB x297 = s.QC[1] * x100 + s.WQ[1] * x101;
-      const TB x306 = s.QC[1] * x114 + s.WQ[1] * x115 + r1 * x100;
-      acc20 += static_cast<double>(x306);
-      const TB x307 = s.QC[1] * x115 + s.WQ[1] * x116 + r1 * x101;
-      const TB x522 = s.QC[1] * x306 + s.WQ[1] * x307 + e1 * (x114 - s.roe * x115) + r1 * x297;
-      acc148 += static_cast<double>(x522);
-      const TB x538 = s.QC[2] * x306 + s.WQ[2] * x307 + r1 * x296;
-      acc164 += static_cast<double>(x538);
-      const TB x342 = s.QC[2] * x115 + s.WQ[2] * x116 + r1 * x99;
-      const TB x298 = s.QC[1] * x102 + s.WQ[1] * x103;
-      acc16 += static_cast<double>(x298);
-      const TB x299 = s.QC[1] * x103 + s.WQ[1] * x104;
-      const TB x334 = s.QC[2] * x103 + s.WQ[2] * x104;
-      const TB x518 = s.QC[1] * x298 + s.WQ[1] * x299 + e1 * (x102 - s.roe * x103);
-      acc144 += static_cast<double>(x518);
-      const TB x534 = s.QC[2] * x298 + s.WQ[2] * x299;
-      acc160 += static_cast<double>(x534);
-      const TB x300 = s.QC[1] * x105 + s.WQ[1] * x106 + r1 * x96;
-      acc17 += static_cast<double>(x300);
-      const TB x301 = s.QC[1] * x106 + s.WQ[1] * x107 + r1 * x97;
-      const TB x336 = s.QC[2] * x106 + s.WQ[2] * x107;
-      const TB x519 = s.QC[1] * x300 + s.WQ[1] * x301 + e1 * (x105 - s.roe * x106) + r1 * x295;
-      acc145 += static_cast<double>(x519);
-      const TB x535 = s.QC[2] * x300 + s.WQ[2] * x301;
-      acc161 += static_cast<double>(x535);
-      const TB x302 = s.QC[1] * x108 + s.WQ[1] * x109;
-      acc18 += static_cast<double>(x302);
-      const TB x303 = s.QC[1] * x109 + s.WQ[1] * x110;
-      const TB x536 = s.QC[2] * x302 + s.WQ[2] * x303 + r1 * x295;
-      acc162 += static_cast<double>(x536);
-      const TB x520 = s.QC[1] * x302 + s.WQ[1] * x303 + e1 * (x108 - s.roe * x109);
-      acc146 += static_cast<double>(x520);
-      const TB x338 = s.QC[2] * x109 + s.WQ[2] * x110 + r1 * x97;
-      const TB x330 = s.QC[2] * x96 + s.WQ[2] * x97;
-      const TB x337 = s.QC[2] * x108 + s.WQ[2] * x109 + r1 * x96;
-      acc34 += static_cast<double>(x337);
-      const TB x552 = s.QC[2] * x337 + s.WQ[2] * x338 + e1 * (x108 - s.roe * x109) + r1 * x330;
-      acc178 += static_cast<double>(x552);
-      const TB x304 = s.QC[1] * x111 + s.WQ[1] * x112 + r2 * x98;
-      acc19 += static_cast<double>(x304);
-      const TB x305 = s.QC[1] * x112 + s.WQ[1] * x113 + r2 * x99;
-      const TB x521 = s.QC[1] * x304 + s.WQ[1] * x305 + e1 * (x111 - s.roe * x112) + r2 * x296;
-      acc147 += static_cast<double>(x521);
-      const TB x537 = s.QC[2] * x304 + s.WQ[2] * x305;
-      acc163 += static_cast<double>(x537);
-      const TB x331 = s.QC[2] * x98 + s.WQ[2] * x99;
-      const TB x340 = s.QC[2] * x112 + s.WQ[2] * x113;
-      const TB x341 = s.QC[2] * x114 + s.WQ[2] * x115 + r1 * x98;
-      acc36 += static_cast<double>(x341);
-      const TB x554 = s.QC[2] * x341 + s.WQ[2] * x342 + e1 * (x114 - s.roe * x115) + r1 * x331;
-      acc180 += static_cast<double>(x554);
-      const TB x333 = s.QC[2] * x102 + s.WQ[2] * x103;
-      acc32 += static_cast<double>(x333);
-      const TB x550 = s.QC[2] * x333 + s.WQ[2] * x334 + e1 * (x102 - s.roe * x103);
-      acc176 += static_cast<double>(x550);
-      const TB x335 = s.QC[2] * x105 + s.WQ[2] * x106;
-      acc33 += static_cast<double>(x335);
-      const TB x551 = s.QC[2] * x335 + s.WQ[2] * x336 + e1 * (x105 - s.roe * x106);
-      acc177 += static_cast<double>(x551);
-      const TB x339 = s.QC[2] * x111 + s.WQ[2] * x112;
-      acc35 += static_cast<double>(x339);
-      const TB x553 = s.QC[2] * x339 + s.WQ[2] * x340 + e1 * (x111 - s.roe * x112);
-      acc179 += static_cast<double>(x553);
-      const TB x368 = s.QC[1] * x157 + s.WQ[1] * x158 + e1 * (x25 - s.roe * x31);
-      acc48 += static_cast<double>(x368);
-      const TB x438 = s.QC[2] * x212 + s.WQ[2] * x213 + e1 * (x25 - s.roe * x31);
-      acc80 += static_cast<double>(x438);
-      const TB x614 = s.QC[2] * x438 + s.WQ[2] * x439 + e2 * (x212 - s.roe * x213);
-      acc240 += static_cast<double>(x614);
-      const TB x566 = s.QC[1] * x368 + s.WQ[1] * x369 + e2 * (x157 - s.roe * x158);
-      acc192 += static_cast<double>(x566);
-      const TB x582 = s.QC[2] * x368 + s.WQ[2] * x369;
-      acc208 += static_cast<double>(x582);
-      const TB x370 = s.QC[1] * x160 + s.WQ[1] * x161 + e1 * (x26 - s.roe * x32) + r1 * x151;
-      acc49 += static_cast<double>(x370);
-      const TB x440 = s.QC[2] * x215 + s.WQ[2] * x216 + e1 * (x26 - s.roe * x32);
-      acc81 += static_cast<double>(x440);
-      const TB x615 = s.QC[2] * x440 + s.WQ[2] * x441 + e2 * (x215 - s.roe * x216);
-      acc241 += static_cast<double>(x615);
-      const TB x567 = s.QC[1] * x370 + s.WQ[1] * x371 + e2 * (x160 - s.roe * x161) + r1 * x365;
-      acc193 += static_cast<double>(x567);
-      const TB x583 = s.QC[2] * x370 + s.WQ[2] * x371;
-      acc209 += static_cast<double>(x583);
-      const TB x407 = s.QC[2] * x163 + s.WQ[2] * x164 + r1 * x151;
-      acc66 += static_cast<double>(x407);
-      const TB x600 = s.QC[2] * x407 + s.WQ[2] * x408 + e1 * (x163 - s.roe * x164) + r1 * x400;
-      acc226 += static_cast<double>(x600);
-      const TB x374 = s.QC[1] * x166 + s.WQ[1] * x167 + e1 * (x28 - s.roe * x34) + r2 * x153;
-      acc51 += static_cast<double>(x374);
-      const TB x444 = s.QC[2] * x221 + s.WQ[2] * x222 + e1 * (x28 - s.roe * x34);
-      acc83 += static_cast<double>(x444);
-      const TB x617 = s.QC[2] * x444 + s.WQ[2] * x445 + e2 * (x221 - s.roe * x222);
-      acc243 += static_cast<double>(x617);
-      const TB x569 = s.QC[1] * x374 + s.WQ[1] * x375 + e2 * (x166 - s.roe * x167) + r2 * x366;
-      acc195 += static_cast<double>(x569);
-      const TB x585 = s.QC[2] * x374 + s.WQ[2] * x375;
-      acc211 += static_cast<double>(x585);
-      const TB x411 = s.QC[2] * x169 + s.WQ[2] * x170 + r1 * x153;
-      acc68 += static_cast<double>(x411);
-      const TB x602 = s.QC[2] * x411 + s.WQ[2] * x412 + e1 * (x169 - s.roe * x170) + r1 * x401;
-      acc228 += static_cast<double>(x602);
-      const TB x403 = s.QC[2] * x157 + s.WQ[2] * x158;
-      acc64 += static_cast<double>(x403);
-      const TB x598 = s.QC[2] * x403 + s.WQ[2] * x404 + e1 * (x157 - s.roe * x158);
-      acc224 += static_cast<double>(x598);
-      const TB x405 = s.QC[2] * x160 + s.WQ[2] * x161;
-      acc65 += static_cast<double>(x405);
-      const TB x599 = s.QC[2] * x405 + s.WQ[2] * x406 + e1 * (x160 - s.roe * x161);
-      acc225 += static_cast<double>(x599);
-      const TB x409 = s.QC[2] * x166 + s.WQ[2] * x167;
-      acc67 += static_cast<double>(x409);
-      const TB x601 = s.QC[2] * x409 + s.WQ[2] * x410 + e1 * (x166 - s.roe * x167);
-      acc227 += static_cast<double>(x601);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(0 * R + r) * a.acc_stride + q] = acc0;
-  a.acc[(16 * R + r) * a.acc_stride + q] = acc16;
-  a.acc[(32 * R + r) * a.acc_stride + q] = acc32;
-  a.acc[(48 * R + r) * a.acc_stride + q] = acc48;
-  a.acc[(64 * R + r) * a.acc_stride + q] = acc64;
-  a.acc[(80 * R + r) * a.acc_stride + q] = acc80;
-  a.acc[(96 * R + r) * a.acc_stride + q] = acc96;
-  a.acc[(112 * R + r) * a.acc_stride + q] = acc112;
-  a.acc[(128 * R + r) * a.acc_stride + q] = acc128;
-  a.acc[(144 * R + r) * a.acc_stride + q] = acc144;
-  a.acc[(160 * R + r) * a.acc_stride + q] = acc160;
-  a.acc[(176 * R + r) * a.acc_stride + q] = acc176;
-  a.acc[(192 * R + r) * a.acc_stride + q] = acc192;
-  a.acc[(208 * R + r) * a.acc_stride + q] = acc208;
-  a.acc[(224 * R + r) * a.acc_stride + q] = acc224;
-  a.acc[(240 * R + r) * a.acc_stride + q] = acc240;
-  a.acc[(1 * R + r) * a.acc_stride + q] = acc1;
-  a.acc[(17 * R + r) * a.acc_stride + q] = acc17;
-  a.acc[(33 * R + r) * a.acc_stride + q] = acc33;
-  a.acc[(49 * R + r) * a.acc_stride + q] = acc49;
-  a.acc[(65 * R + r) * a.acc_stride + q] = acc65;
-  a.acc[(81 * R + r) * a.acc_stride + q] = acc81;
-  a.acc[(97 * R + r) * a.acc_stride + q] = acc97;
-  a.acc[(113 * R + r) * a.acc_stride + q] = acc113;
-  a.acc[(129 * R + r) * a.acc_stride + q] = acc129;
-  a.acc[(145 * R + r) * a.acc_stride + q] = acc145;
-  a.acc[(161 * R + r) * a.acc_stride + q] = acc161;
-  a.acc[(177 * R + r) * a.acc_stride + q] = acc177;
-  a.acc[(193 * R + r) * a.acc_stride + q] = acc193;
-  a.acc[(209 * R + r) * a.acc_stride + q] = acc209;
-  a.acc[(225 * R + r) * a.acc_stride + q] = acc225;
-  a.acc[(241 * R + r) * a.acc_stride + q] = acc241;
-  a.acc[(2 * R + r) * a.acc_stride + q] = acc2;
-  a.acc[(18 * R + r) * a.acc_stride + q] = acc18;
-  a.acc[(34 * R + r) * a.acc_stride + q] = acc34;
-  a.acc[(50 * R + r) * a.acc_stride + q] = acc50;
-  a.acc[(66 * R + r) * a.acc_stride + q] = acc66;
-  a.acc[(82 * R + r) * a.acc_stride + q] = acc82;
-  a.acc[(98 * R + r) * a.acc_stride + q] = acc98;
-  a.acc[(114 * R + r) * a.acc_stride + q] = acc114;
-  a.acc[(130 * R + r) * a.acc_stride + q] = acc130;
-  a.acc[(146 * R + r) * a.acc_stride + q] = acc146;
-  a.acc[(162 * R + r) * a.acc_stride + q] = acc162;
-  a.acc[(178 * R + r) * a.acc_stride + q] = acc178;
-  a.acc[(194 * R + r) * a.acc_stride + q] = acc194;
-  a.acc[(210 * R + r) * a.acc_stride + q] = acc210;
-  a.acc[(226 * R + r) * a.acc_stride + q] = acc226;
-  a.acc[(242 * R + r) * a.acc_stride + q] = acc242;
-  a.acc[(3 * R + r) * a.acc_stride + q] = acc3;
-  a.acc[(19 * R + r) * a.acc_stride + q] = acc19;
-  a.acc[(35 * R + r) * a.acc_stride + q] = acc35;
-  a.acc[(51 * R + r) * a.acc_stride + q] = acc51;
-  a.acc[(67 * R + r) * a.acc_stride + q] = acc67;
-  a.acc[(83 * R + r) * a.acc_stride + q] = acc83;
-  a.acc[(99 * R + r) * a.acc_stride + q] = acc99;
-  a.acc[(115 * R + r) * a.acc_stride + q] = acc115;
-  a.acc[(131 * R + r) * a.acc_stride + q] = acc131;
-  a.acc[(147 * R + r) * a.acc_stride + q] = acc147;
-  a.acc[(163 * R + r) * a.acc_stride + q] = acc163;
-  a.acc[(179 * R + r) * a.acc_stride + q] = acc179;
-  a.acc[(195 * R + r) * a.acc_stride + q] = acc195;
-  a.acc[(211 * R + r) * a.acc_stride + q] = acc211;
-  a.acc[(227 * R + r) * a.acc_stride + q] = acc227;
-  a.acc[(243 * R + r) * a.acc_stride + q] = acc243;
-  a.acc[(4 * R + r) * a.acc_stride + q] = acc4;
-  a.acc[(20 * R + r) * a.acc_stride + q] = acc20;
-  a.acc[(36 * R + r) * a.acc_stride + q] = acc36;
-  a.acc[(52 * R + r) * a.acc_stride + q] = acc52;
-  a.acc[(68 * R + r) * a.acc_stride + q] = acc68;
-  a.acc[(84 * R + r) * a.acc_stride + q] = acc84;
-  a.acc[(100 * R + r) * a.acc_stride + q] = acc100;
-  a.acc[(116 * R + r) * a.acc_stride + q] = acc116;
-  a.acc[(132 * R + r) * a.acc_stride + q] = acc132;
-  a.acc[(148 * R + r) * a.acc_stride + q] = acc148;
-  a.acc[(164 * R + r) * a.acc_stride + q] = acc164;
-  a.acc[(180 * R + r) * a.acc_stride + q] = acc180;
-  a.acc[(196 * R + r) * a.acc_stride + q] = acc196;
-  a.acc[(212 * R + r) * a.acc_stride + q] = acc212;
-  a.acc[(228 * R + r) * a.acc_stride + q] = acc228;
-  a.acc[(244 * R + r) * a.acc_stride + q] = acc244;
-  a.acc[(5 * R + r) * a.acc_stride + q] = acc5;
-  a.acc[(21 * R + r) * a.acc_stride + q] = acc21;
-  a.acc[(37 * R + r) * a.acc_stride + q] = acc37;
-  a.acc[(53 * R + r) * a.acc_stride + q] = acc53;
-  a.acc[(69 * R + r) * a.acc_stride + q] = acc69;
-}
-// class (21|21) split 1/3: 281 recursion values, 85 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2121_1(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc85 = 0.0;
-  double acc101 = 0.0;
-  double acc117 = 0.0;
-  double acc133 = 0.0;
-  double acc149 = 0.0;
-  double acc165 = 0.0;
-  double acc181 = 0.0;
-  double acc197 = 0.0;
-  double acc213 = 0.0;
-  double acc229 = 0.0;
-  double acc245 = 0.0;
-  double acc6 = 0.0;
-  double acc22 = 0.0;
-  double acc38 = 0.0;
-  double acc54 = 0.0;
-  double acc70 = 0.0;
-  double acc86 = 0.0;
-  double acc102 = 0.0;
-  double acc118 = 0.0;
-  double acc134 = 0.0;
-  double acc150 = 0.0;
-  double acc166 = 0.0;
-  double acc182 = 0.0;
-  double acc198 = 0.0;
-  double acc214 = 0.0;
-  double acc230 = 0.0;
-  double acc246 = 0.0;
-  double acc7 = 0.0;
-  double acc23 = 0.0;
-  double acc39 = 0.0;
-  double acc55 = 0.0;
-  double acc71 = 0.0;
-  double acc87 = 0.0;
-  double acc103 = 0.0;
-  double acc119 = 0.0;
-  double acc135 = 0.0;
-  double acc151 = 0.0;
-  double acc167 = 0.0;
-  double acc183 = 0.0;
-  double acc199 = 0.0;
-  double acc215 = 0.0;
-  double acc231 = 0.0;
-  double acc247 = 0.0;
-  double acc8 = 0.0;
-  double acc24 = 0.0;
-  double acc40 = 0.0;
-  double acc56 = 0.0;
-  double acc72 = 0.0;
-  double acc88 = 0.0;
-  double acc104 = 0.0;
-  double acc120 = 0.0;
-  double acc136 = 0.0;
-  double acc152 = 0.0;
-  double acc168 = 0.0;
-  double acc184 = 0.0;
-  double acc200 = 0.0;
-  double acc216 = 0.0;
-  double acc232 = 0.0;
-  double acc248 = 0.0;
-  double acc9 = 0.0;
-  double acc25 = 0.0;
-  double acc41 = 0.0;
-  double acc57 = 0.0;
-  double acc73 = 0.0;
-  double acc89 = 0.0;
-  double acc105 = 0.0;
-  double acc121 = 0.0;
-  double acc137 = 0.0;
-  double acc153 = 0.0;
-  double acc169 = 0.0;
-  double acc185 = 0.0;
-  double acc201 = 0.0;
-  double acc217 = 0.0;
-  double acc233 = 0.0;
-  double acc249 = 0.0;
-  double acc10 = 0.0;
-  double acc26 = 0.0;
-  double acc42 = 0.0;
-  double acc58 = 0.0;
-  double acc74 = 0.0;
-  double acc90 = 0.0;
-  double acc106 = 0.0;
-  double acc122 = 0.0;
-  double acc138 = 0.0;
-  double acc154 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x22 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x25 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x20 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x21 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x48 = s.PA[2] * x18 + s.WP[2] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x49 = s.PA[0] * x19 + s.WP[0] * x22 + z1 * (x4 - s.roz * x5);
-      const TB x50 = s.PA[1] * x19 + s.WP[1] * x22;
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x31 = s.PA[0] * x10 + s.WP[0] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x37 = s.PA[0] * x13 + s.WP[0] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x43 = s.PA[0] * x16 + s.WP[0] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x46 = s.PA[1] * x17 + s.WP[1] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x47 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x45 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x55 = s.PA[0] * x25 + s.WP[0] * x31 + z2 * (x7 - s.roz * x10);
-      const TB x58 = s.PA[1] * x26 + s.WP[1] * x32 + z1 * (x7 - s.roz * x10);
-      const TB x59 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x56 = s.PA[1] * x25 + s.WP[1] * x31;
-      const TB x57 = s.PA[2] * x25 + s.WP[2] * x31;
-      const TB x65 = s.PA[0] * x31 + s.WP[0] * x37 + z2 * (x10 - s.roz * x13);
-      const TB x68 = s.PA[1] * x32 + s.WP[1] * x38 + z1 * (x10 - s.roz * x13);
-      const TB x66 = s.PA[1] * x31 + s.WP[1] * x37;
-      const TB x67 = s.PA[2] * x31 + s.WP[2] * x37;
-      const TB x69 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x75 = s.PA[0] * x37 + s.WP[0] * x43 + z2 * (x13 - s.roz * x16);
-      const TB x76 = s.PA[1] * x37 + s.WP[1] * x43;
-      const TB x77 = s.PA[2] * x37 + s.WP[2] * x43;
-      const TB x78 = s.PA[1] * x38 + s.WP[1] * x44 + z1 * (x13 - s.roz * x16);
-      const TB x79 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x85 = s.PA[0] * x43 + s.WP[0] * x49 + z2 * (x16 - s.roz * x19);
-      const TB x88 = s.PA[1] * x44 + s.WP[1] * x50 + z1 * (x16 - s.roz * x19);
-      const TB x89 = s.PA[2] * x44 + s.WP[2] * x50;
-      const TB x134 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x47;
-      const TB x116 = s.QC[0] * x41 + s.WQ[0] * x47;
-      const TB x86 = s.PA[1] * x43 + s.WP[1] * x49;
-      const TB x87 = s.PA[2] * x43 + s.WP[2] * x49;
-      const TB x95 = s.QC[0] * x2 + s.WQ[0] * x3;
-      const TB x97 = s.QC[0] * x13 + s.WQ[0] * x16 + r1 * x3;
-      const TB x99 = s.QC[0] * x14 + s.WQ[0] * x17;
-      const TB x106 = s.QC[0] * x32 + s.WQ[0] * x38 + r1 * x14;
-      const TB x107 = s.QC[0] * x38 + s.WQ[0] * x44 + r1 * x17;
-      const TB x266 = s.QC[0] * x106 + s.WQ[0] * x107 + e1 * (x32 - s.roe * x38) + r1 * x99;
-      const TB x100 = s.QC[0] * x12 + s.WQ[0] * x15;
-      const TB x101 = s.QC[0] * x15 + s.WQ[0] * x18;
-      const TB x332 = s.QC[2] * x100 + s.WQ[2] * x101 + r1 * x95;
-      const TB x103 = s.QC[0] * x31 + s.WQ[0] * x37 + r2 * x13;
-      const TB x104 = s.QC[0] * x37 + s.WQ[0] * x43 + r2 * x16;
-      const TB x109 = s.QC[0] * x33 + s.WQ[0] * x39 + r1 * x15;
-      const TB x110 = s.QC[0] * x39 + s.WQ[0] * x45 + r1 * x18;
-      const TB x128 = s.QC[0] * x77 + s.WQ[0] * x87 + r2 * x45;
-      const TB x112 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x113 = s.QC[0] * x40 + s.WQ[0] * x46;
-      const TB x131 = s.QC[0] * x78 + s.WQ[0] * x88 + r1 * x46;
-      const TB x115 = s.QC[0] * x35 + s.WQ[0] * x41;
-      const TB x117 = s.QC[0] * x30 + s.WQ[0] * x36;
-      const TB x118 = s.QC[0] * x36 + s.WQ[0] * x42;
-      const TB x119 = s.QC[0] * x42 + s.WQ[0] * x48;
-      const TB x120 = s.QC[0] * x55 + s.WQ[0] * x65 + r3 * x31;
-      const TB x121 = s.QC[0] * x65 + s.WQ[0] * x75 + r3 * x37;
-      const TB x122 = s.QC[0] * x75 + s.WQ[0] * x85 + r3 * x43;
-      const TB x123 = s.QC[0] * x56 + s.WQ[0] * x66 + r2 * x32;
-      const TB x124 = s.QC[0] * x66 + s.WQ[0] * x76 + r2 * x38;
-      const TB x125 = s.QC[0] * x76 + s.WQ[0] * x86 + r2 * x44;
-      const TB x126 = s.QC[0] * x57 + s.WQ[0] * x67 + r2 * x33;
-      const TB x127 = s.QC[0] * x67 + s.WQ[0] * x77 + r2 * x39;
-      const TB x129 = s.QC[0] * x58 + s.WQ[0] * x68 + r1 * x34;
-      const TB x270 = s.QC[0] * x112 + s.WQ[0] * x113 + e1 * (x34 - s.roe * x40);
-      const TB x130 = s.QC[0] * x68 + s.WQ[0] * x78 + r1 * x40;
-      const TB x281 = s.QC[0] * x129 + s.WQ[0] * x130 + e1 * (x58 - s.roe * x68) + r1 * x112;
-      acc9 += static_cast<double>(x281);
-      const TB x282 = s.QC[0] * x130 + s.WQ[0] * x131 + e1 * (x68 - s.roe * x78) + r1 * x113;
-      const TB x479 = s.QC[0] * x281 + s.WQ[0] * x282 + e2 * (x129 - s.roe * x130) + r1 * x270;
-      acc105 += static_cast<double>(x479);
-      const TB x495 = s.QC[1] * x281 + s.WQ[1] * x282 + r2 * x266;
-      acc121 += static_cast<double>(x495);
-      const TB x511 = s.QC[2] * x281 + s.WQ[2] * x282;
-      acc137 += static_cast<double>(x511);
-      const TB x132 = s.QC[0] * x59 + s.WQ[0] * x69 + r1 * x35;
-      const TB x272 = s.QC[0] * x115 + s.WQ[0] * x116 + e1 * (x35 - s.roe * x41);
-      const TB x133 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x41;
-      const TB x283 = s.QC[0] * x132 + s.WQ[0] * x133 + e1 * (x59 - s.roe * x69) + r1 * x115;
-      acc10 += static_cast<double>(x283);
-      const TB x284 = s.QC[0] * x133 + s.WQ[0] * x134 + e1 * (x69 - s.roe * x79) + r1 * x116;
-      const TB x480 = s.QC[0] * x283 + s.WQ[0] * x284 + e2 * (x132 - s.roe * x133) + r1 * x272;
-      acc106 += static_cast<double>(x480);
-      const TB x319 = s.QC[1] * x133 + s.WQ[1] * x134 + r1 * x110;
-      const TB x353 = s.QC[2] * x132 + s.WQ[2] * x133 + r1 * x106;
-      acc42 += static_cast<double>(x353);
-      const TB x512 = s.QC[2] * x283 + s.WQ[2] * x284 + r1 * x266;
-      acc138 += static_cast<double>(x512);
-      const TB x150 = s.QC[1] * x2 + s.WQ[1] * x3;
-      const TB x152 = s.QC[1] * x13 + s.WQ[1] * x16;
-      const TB x161 = s.QC[1] * x32 + s.WQ[1] * x38 + r1 * x13;
-      const TB x162 = s.QC[1] * x38 + s.WQ[1] * x44 + r1 * x16;
-      const TB x186 = s.QC[1] * x78 + s.WQ[1] * x88 + r2 * x44;
-      const TB x241 = s.QC[2] * x78 + s.WQ[2] * x88;
-      const TB x371 = s.QC[1] * x161 + s.WQ[1] * x162 + e1 * (x32 - s.roe * x38) + r1 * x152;
-      const TB x155 = s.QC[1] * x12 + s.WQ[1] * x15;
-      const TB x156 = s.QC[1] * x15 + s.WQ[1] * x18;
-      const TB x402 = s.QC[2] * x155 + s.WQ[2] * x156 + r1 * x150;
-      const TB x158 = s.QC[1] * x31 + s.WQ[1] * x37;
-      const TB x159 = s.QC[1] * x37 + s.WQ[1] * x43;
-      const TB x164 = s.QC[1] * x33 + s.WQ[1] * x39;
-      const TB x172 = s.QC[1] * x30 + s.WQ[1] * x36;
-      const TB x173 = s.QC[1] * x36 + s.WQ[1] * x42;
-      const TB x174 = s.QC[1] * x42 + s.WQ[1] * x48;
-      const TB x229 = s.QC[2] * x42 + s.WQ[2] * x48 + r2 * x18;
-      const TB x211 = s.QC[2] * x15 + s.WQ[2] * x18 + r1 * x3;
-      const TB x205 = s.QC[2] * x2 + s.WQ[2] * x3;
-      const TB x210 = s.QC[2] * x12 + s.WQ[2] * x15 + r1 * x2;
-      const TB x437 = s.QC[2] * x210 + s.WQ[2] * x211 + e1 * (x12 - s.roe * x15) + r1 * x205;
-      const TB x175 = s.QC[1] * x55 + s.WQ[1] * x65;
-      const TB x176 = s.QC[1] * x65 + s.WQ[1] * x75;
-      const TB x177 = s.QC[1] * x75 + s.WQ[1] * x85;
-      const TB x232 = s.QC[2] * x75 + s.WQ[2] * x85;
-      const TB x178 = s.QC[1] * x56 + s.WQ[1] * x66 + r1 * x31;
-      const TB x179 = s.QC[1] * x66 + s.WQ[1] * x76 + r1 * x37;
-      const TB x180 = s.QC[1] * x76 + s.WQ[1] * x86 + r1 * x43;
-      const TB x235 = s.QC[2] * x76 + s.WQ[2] * x86;
-      const TB x181 = s.QC[1] * x57 + s.WQ[1] * x67;
-      const TB x182 = s.QC[1] * x67 + s.WQ[1] * x77;
-      const TB x183 = s.QC[1] * x77 + s.WQ[1] * x87;
-      const TB x238 = s.QC[2] * x77 + s.WQ[2] * x87 + r1 * x43;
-      const TB x214 = s.QC[2] * x37 + s.WQ[2] * x43;
-      const TB x184 = s.QC[1] * x58 + s.WQ[1] * x68 + r2 * x32;
-      const TB x185 = s.QC[1] * x68 + s.WQ[1] * x78 + r2 * x38;
-      const TB x387 = s.QC[1] * x185 + s.WQ[1] * x186 + e1 * (x68 - s.roe * x78) + r2 * x162;
-      const TB x422 = s.QC[2] * x185 + s.WQ[2] * x186;
-      const TB x187 = s.QC[1] * x59 + s.WQ[1] * x69 + r1 * x33;
-      const TB x268 = s.QC[0] * x109 + s.WQ[0] * x110 + e1 * (x33 - s.roe * x39) + r1 * x101;
-      const TB x496 = s.QC[1] * x283 + s.WQ[1] * x284 + r1 * x268;
-      acc122 += static_cast<double>(x496);
-      const TB x188 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x39;
-      const TB x388 = s.QC[1] * x187 + s.WQ[1] * x188 + e1 * (x59 - s.roe * x69) + r1 * x164;
-      acc58 += static_cast<double>(x388);
-      const TB x423 = s.QC[2] * x187 + s.WQ[2] * x188 + r1 * x161;
-      acc74 += static_cast<double>(x423);
-      const TB x243 = s.QC[2] * x69 + s.WQ[2] * x79 + r1 * x38;
-      const TB x216 = s.QC[2] * x32 + s.WQ[2] * x38;
-      const TB x242 = s.QC[2] * x59 + s.WQ[2] * x69 + r1 * x32;
-      const TB x458 = s.QC[2] * x242 + s.WQ[2] * x243 + e1 * (x59 - s.roe * x69) + r1 * x216;
-      acc90 += static_cast<double>(x458);
-      const TB x386 = s.QC[1] * x184 + s.WQ[1] * x185 + e1 * (x58 - s.roe * x68) + r2 * x161;
-      acc57 += static_cast<double>(x386);
-      const TB x575 = s.QC[1] * x386 + s.WQ[1] * x387 + e2 * (x184 - s.roe * x185) + r2 * x371;
-      acc201 += static_cast<double>(x575);
-      const TB x591 = s.QC[2] * x386 + s.WQ[2] * x387;
-      acc217 += static_cast<double>(x591);
-      const TB x213 = s.QC[2] * x31 + s.WQ[2] * x37;
-      const TB x227 = s.QC[2] * x30 + s.WQ[2] * x36 + r2 * x12;
-      const TB x228 = s.QC[2] * x36 + s.WQ[2] * x42 + r2 * x15;
-      const TB x449 = s.QC[2] * x228 + s.WQ[2] * x229 + e1 * (x36 - s.roe * x42) + r2 * x211;
-      const TB x448 = s.QC[2] * x227 + s.WQ[2] * x228 + e1 * (x30 - s.roe * x36) + r2 * x210;
-      acc85 += static_cast<double>(x448);
-      const TB x619 = s.QC[2] * x448 + s.WQ[2] * x449 + e2 * (x227 - s.roe * x228) + r2 * x437;
-      acc245 += static_cast<double>(x619);
-      const TB x230 = s.QC[2] * x55 + s.WQ[2] * x65;
-      const TB x231 = s.QC[2] * x65 + s.WQ[2] * x75;
-      const TB x451 = s.QC[2] * x231 + s.WQ[2] * x232 + e1 * (x65 - s.roe * x75);
-      const TB x233 = s.QC[2] * x56 + s.WQ[2] * x66;
-      const TB x234 = s.QC[2] * x66 + s.WQ[2] * x76;
-      const TB x453 = s.QC[2] * x234 + s.WQ[2] * x235 + e1 * (x66 - s.roe * x76);
-      const TB x236 = s.QC[2] * x57 + s.WQ[2] * x67 + r1 * x31;
-      const TB x237 = s.QC[2] * x67 + s.WQ[2] * x77 + r1 * x37;
-      const TB x455 = s.QC[2] * x237 + s.WQ[2] * x238 + e1 * (x67 - s.roe * x77) + r1 * x214;
-      const TB x439 = s.QC[2] * x213 + s.WQ[2] * x214 + e1 * (x31 - s.roe * x37);
-      const TB x454 = s.QC[2] * x236 + s.WQ[2] * x237 + e1 * (x57 - s.roe * x67) + r1 * x213;
-      acc88 += static_cast<double>(x454);
-      const TB x622 = s.QC[2] * x454 + s.WQ[2] * x455 + e2 * (x236 - s.roe * x237) + r1 * x439;
-      acc248 += static_cast<double>(x622);
-      const TB x239 = s.QC[2] * x58 + s.WQ[2] * x68;
-      const TB x240 = s.QC[2] * x68 + s.WQ[2] * x78;
-      const TB x457 = s.QC[2] * x240 + s.WQ[2] * x241 + e1 * (x68 - s.roe * x78);
-      const TB x456 = s.QC[2] * x239 + s.WQ[2] * x240 + e1 * (x58 - s.roe * x68);
-      acc89 += static_cast<double>(x456);
-      const TB x623 = s.QC[2] * x456 + s.WQ[2] * x457 + e2 * (x239 - s.roe * x240);
-      acc249 += static_cast<double>(x623);
-      const TB x262 = s.QC[0] * x100 + s.WQ[0] * x101 + e1 * (x12 - s.roe * x15);
-      const TB x367 = s.QC[1] * x155 + s.WQ[1] * x156 + e1 * (x12 - s.roe * x15);
-      const TB x413 = s.QC[2] * x172 + s.WQ[2] * x173 + r2 * x155;
-      const TB x414 = s.QC[2] * x173 + s.WQ[2] * x174 + r2 * x156;
-      const TB x603 = s.QC[2] * x413 + s.WQ[2] * x414 + e1 * (x172 - s.roe * x173) + r2 * x402;
-      acc229 += static_cast<double>(x603);
-      const TB x379 = s.QC[1] * x173 + s.WQ[1] * x174 + e1 * (x36 - s.roe * x42);
-      const TB x274 = s.QC[0] * x118 + s.WQ[0] * x119 + e1 * (x36 - s.roe * x42);
-      const TB x264 = s.QC[0] * x103 + s.WQ[0] * x104 + e1 * (x31 - s.roe * x37) + r2 * x97;
-      const TB x369 = s.QC[1] * x158 + s.WQ[1] * x159 + e1 * (x31 - s.roe * x37);
-      const TB x301 = s.QC[1] * x106 + s.WQ[1] * x107 + r1 * x97;
-      const TB x273 = s.QC[0] * x117 + s.WQ[0] * x118 + e1 * (x30 - s.roe * x36);
-      const TB x378 = s.QC[1] * x172 + s.WQ[1] * x173 + e1 * (x30 - s.roe * x36);
-      const TB x571 = s.QC[1] * x378 + s.WQ[1] * x379 + e2 * (x172 - s.roe * x173);
-      acc197 += static_cast<double>(x571);
-      const TB x587 = s.QC[2] * x378 + s.WQ[2] * x379 + r2 * x367;
-      acc213 += static_cast<double>(x587);
-      const TB x507 = s.QC[2] * x273 + s.WQ[2] * x274 + r2 * x262;
-      acc133 += static_cast<double>(x507);
-      const TB x475 = s.QC[0] * x273 + s.WQ[0] * x274 + e2 * (x117 - s.roe * x118);
-      acc101 += static_cast<double>(x475);
-      const TB x491 = s.QC[1] * x273 + s.WQ[1] * x274;
-      acc117 += static_cast<double>(x491);
-      const TB x275 = s.QC[0] * x120 + s.WQ[0] * x121 + e1 * (x55 - s.roe * x65) + r3 * x103;
-      acc6 += static_cast<double>(x275);
-      const TB x276 = s.QC[0] * x121 + s.WQ[0] * x122 + e1 * (x65 - s.roe * x75) + r3 * x104;
-      const TB x381 = s.QC[1] * x176 + s.WQ[1] * x177 + e1 * (x65 - s.roe * x75);
-      const TB x416 = s.QC[2] * x176 + s.WQ[2] * x177;
-      const TB x476 = s.QC[0] * x275 + s.WQ[0] * x276 + e2 * (x120 - s.roe * x121) + r3 * x264;
-      acc102 += static_cast<double>(x476);
-      const TB x492 = s.QC[1] * x275 + s.WQ[1] * x276;
-      acc118 += static_cast<double>(x492);
-      const TB x508 = s.QC[2] * x275 + s.WQ[2] * x276;
-      acc134 += static_cast<double>(x508);
-      const TB x277 = s.QC[0] * x123 + s.WQ[0] * x124 + e1 * (x56 - s.roe * x66) + r2 * x106;
-      acc7 += static_cast<double>(x277);
-      const TB x316 = s.QC[1] * x129 + s.WQ[1] * x130 + r2 * x106;
-      acc25 += static_cast<double>(x316);
-      const TB x278 = s.QC[0] * x124 + s.WQ[0] * x125 + e1 * (x66 - s.roe * x76) + r2 * x107;
-      const TB x477 = s.QC[0] * x277 + s.WQ[0] * x278 + e2 * (x123 - s.roe * x124) + r2 * x266;
-      acc103 += static_cast<double>(x477);
-      const TB x317 = s.QC[1] * x130 + s.WQ[1] * x131 + r2 * x107;
-      const TB x527 = s.QC[1] * x316 + s.WQ[1] * x317 + e1 * (x129 - s.roe * x130) + r2 * x301;
-      acc153 += static_cast<double>(x527);
-      const TB x543 = s.QC[2] * x316 + s.WQ[2] * x317;
-      acc169 += static_cast<double>(x543);
-      const TB x352 = s.QC[2] * x130 + s.WQ[2] * x131;
-      const TB x383 = s.QC[1] * x179 + s.WQ[1] * x180 + e1 * (x66 - s.roe * x76) + r1 * x159;
-      const TB x418 = s.QC[2] * x179 + s.WQ[2] * x180;
-      const TB x493 = s.QC[1] * x277 + s.WQ[1] * x278 + r1 * x264;
-      acc119 += static_cast<double>(x493);
-      const TB x509 = s.QC[2] * x277 + s.WQ[2] * x278;
-      acc135 += static_cast<double>(x509);
-      const TB x279 = s.QC[0] * x126 + s.WQ[0] * x127 + e1 * (x57 - s.roe * x67) + r2 * x109;
-      acc8 += static_cast<double>(x279);
-      const TB x384 = s.QC[1] * x181 + s.WQ[1] * x182 + e1 * (x57 - s.roe * x67);
-      acc56 += static_cast<double>(x384);
-      const TB x280 = s.QC[0] * x127 + s.WQ[0] * x128 + e1 * (x67 - s.roe * x77) + r2 * x110;
-      const TB x385 = s.QC[1] * x182 + s.WQ[1] * x183 + e1 * (x67 - s.roe * x77);
-      const TB x478 = s.QC[0] * x279 + s.WQ[0] * x280 + e2 * (x126 - s.roe * x127) + r2 * x268;
-      acc104 += static_cast<double>(x478);
-      const TB x510 = s.QC[2] * x279 + s.WQ[2] * x280 + r1 * x264;
-      acc136 += static_cast<double>(x510);
-      const TB x494 = s.QC[1] * x279 + s.WQ[1] * x280;
-      acc120 += static_cast<double>(x494);
-      const TB x303 = s.QC[1] * x109 + s.WQ[1] * x110;
-      const TB x318 = s.QC[1] * x132 + s.WQ[1] * x133 + r1 * x109;
-      acc26 += static_cast<double>(x318);
-      const TB x528 = s.QC[1] * x318 + s.WQ[1] * x319 + e1 * (x132 - s.roe * x133) + r1 * x303;
-      acc154 += static_cast<double>(x528);
-      const TB x420 = s.QC[2] * x182 + s.WQ[2] * x183 + r1 * x159;
-      const TB x404 = s.QC[2] * x158 + s.WQ[2] * x159;
-      const TB x574 = s.QC[1] * x384 + s.WQ[1] * x385 + e2 * (x181 - s.roe * x182);
-      acc200 += static_cast<double>(x574);
-      const TB x590 = s.QC[2] * x384 + s.WQ[2] * x385 + r1 * x369;
-      acc216 += static_cast<double>(x590);
-      const TB x297 = s.QC[1] * x100 + s.WQ[1] * x101;
-      const TB x343 = s.QC[2] * x117 + s.WQ[2] * x118 + r2 * x100;
-      const TB x344 = s.QC[2] * x118 + s.WQ[2] * x119 + r2 * x101;
-      const TB x555 = s.QC[2] * x343 + s.WQ[2] * x344 + e1 * (x117 - s.roe * x118) + r2 * x332;
-      acc181 += static_cast<double>(x555);
-      const TB x309 = s.QC[1] * x118 + s.WQ[1] * x119;
-      const TB x299 = s.QC[1] * x103 + s.WQ[1] * x104;
-      const TB x308 = s.QC[1] * x117 + s.WQ[1] * x118;
-      const TB x523 = s.QC[1] * x308 + s.WQ[1] * x309 + e1 * (x117 - s.roe * x118);
-      acc149 += static_cast<double>(x523);
-      const TB x539 = s.QC[2] * x308 + s.WQ[2] * x309 + r2 * x297;
-      acc165 += static_cast<double>(x539);
-      const TB x310 = s.QC[1] * x120 + s.WQ[1] * x121;
-      acc22 += static_cast<double>(x310);
-      const TB x311 = s.QC[1] * x121 + s.WQ[1] * x122;
-      const TB x346 = s.QC[2] * x121 + s.WQ[2] * x122;
-      const TB x524 = s.QC[1] * x310 + s.WQ[1] * x311 + e1 * (x120 - s.roe * x121);
-      acc150 += static_cast<double>(x524);
-      const TB x540 = s.QC[2] * x310 + s.WQ[2] * x311;
-      acc166 += static_cast<double>(x540);
-      const TB x312 = s.QC[1] * x123 + s.WQ[1] * x124 + r1 * x103;
-      acc23 += static_cast<double>(x312);
-      const TB x313 = s.QC[1] * x124 + s.WQ[1] * x125 + r1 * x104;
-      const TB x348 = s.QC[2] * x124 + s.WQ[2] * x125;
-      const TB x525 = s.QC[1] * x312 + s.WQ[1] * x313 + e1 * (x123 - s.roe * x124) + r1 * x299;
-      acc151 += static_cast<double>(x525);
-      const TB x541 = s.QC[2] * x312 + s.WQ[2] * x313;
-      acc167 += static_cast<double>(x541);
-      const TB x314 = s.QC[1] * x126 + s.WQ[1] * x127;
-      acc24 += static_cast<double>(x314);
-      const TB x315 = s.QC[1] * x127 + s.WQ[1] * x128;
-      const TB x542 = s.QC[2] * x314 + s.WQ[2] * x315 + r1 * x299;
-      acc168 += static_cast<double>(x542);
-      const TB x526 = s.QC[1] * x314 + s.WQ[1] * x315 + e1 * (x126 - s.roe * x127);
-      acc152 += static_cast<double>(x526);
-      const TB x350 = s.QC[2] * x127 + s.WQ[2] * x128 + r1 * x104;
-      const TB x334 = s.QC[2] * x103 + s.WQ[2] * x104;
-      const TB x349 = s.QC[2] * x126 + s.WQ[2] * x127 + r1 * x103;
-      acc40 += static_cast<double>(x349);
-      const TB x558 = s.QC[2] * x349 + s.WQ[2] * x350 + e1 * (x126 - s.roe * x127) + r1 * x334;
-      acc184 += static_cast<double>(x558);
-      const TB x345 = s.QC[2] * x120 + s.WQ[2] * x121;
-      acc38 += static_cast<double>(x345);
-      const TB x556 = s.QC[2] * x345 + s.WQ[2] * x346 + e1 * (x120 - s.roe * x121);
-      acc182 += static_cast<double>(x556);
-      const TB x347 = s.QC[2] * x123 + s.WQ[2] * x124;
-      acc39 += static_cast<double>(x347);
-      const TB x557 = s.QC[2] * x347 + s.WQ[2] * x348 + e1 * (x123 - s.roe * x124);
-      acc183 += static_cast<double>(x557);
-      const TB x351 = s.QC[2] * x129 + s.WQ[2] * x130;
-      acc41 += static_cast<double>(x351);
-      const TB x559 = s.QC[2] * x351 + s.WQ[2] * x352 + e1 * (x129 - s.roe * x130);
-      acc185 += static_cast<double>(x559);
-      const TB x380 = s.QC[1] * x175 + s.WQ[1] * x176 + e1 * (x55 - s.roe * x65);
-      acc54 += static_cast<double>(x380);
-      const TB x450 = s.QC[2] * x230 + s.WQ[2] * x231 + e1 * (x55 - s.roe * x65);
-      acc86 += static_cast<double>(x450);
-      const TB x620 = s.QC[2] * x450 + s.WQ[2] * x451 + e2 * (x230 - s.roe * x231);
-      acc246 += static_cast<double>(x620);
-      const TB x572 = s.QC[1] * x380 + s.WQ[1] * x381 + e2 * (x175 - s.roe * x176);
-      acc198 += static_cast<double>(x572);
-      const TB x588 = s.QC[2] * x380 + s.WQ[2] * x381;
-      acc214 += static_cast<double>(x588);
-      const TB x382 = s.QC[1] * x178 + s.WQ[1] * x179 + e1 * (x56 - s.roe * x66) + r1 * x158;
-      acc55 += static_cast<double>(x382);
-      const TB x452 = s.QC[2] * x233 + s.WQ[2] * x234 + e1 * (x56 - s.roe * x66);
-      acc87 += static_cast<double>(x452);
-      const TB x621 = s.QC[2] * x452 + s.WQ[2] * x453 + e2 * (x233 - s.roe * x234);
-      acc247 += static_cast<double>(x621);
-      const TB x573 = s.QC[1] * x382 + s.WQ[1] * x383 + e2 * (x178 - s.roe * x179) + r1 * x369;
-      acc199 += static_cast<double>(x573);
-      const TB x589 = s.QC[2] * x382 + s.WQ[2] * x383;
-      acc215 += static_cast<double>(x589);
-      const TB x419 = s.QC[2] * x181 + s.WQ[2] * x182 + r1 * x158;
-      acc72 += static_cast<double>(x419);
-      const TB x606 = s.QC[2] * x419 + s.WQ[2] * x420 + e1 * (x181 - s.roe * x182) + r1 * x404;
-      acc232 += static_cast<double>(x606);
-      const TB x415 = s.QC[2] * x175 + s.WQ[2] * x176;
-      acc70 += static_cast<double>(x415);
-      const TB x604 = s.QC[2] * x415 + s.WQ[2] * x416 + e1 * (x175 - s.roe * x176);
-      acc230 += static_cast<double>(x604);
-      const TB x417 = s.QC[2] * x178 + s.WQ[2] * x179;
-      acc71 += static_cast<double>(x417);
-      const TB x605 = s.QC[2] * x417 + s.WQ[2] * x418 + e1 * (x178 - s.roe * x179);
-      acc231 += static_cast<double>(x605);
-      const TB x421 = s.QC[2] * x184 + s.WQ[2] * x185;
-      acc73 += static_cast<double>(x421);
-      const TB x607 = s.QC[2] * x421 + s.WQ[2] * x422 + e1 * (x184 - s.roe * x185);
-      acc233 += static_cast<double>(x607);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(85 * R + r) * a.acc_stride + q] = acc85;
-  a.acc[(101 * R + r) * a.acc_stride + q] = acc101;
-  a.acc[(117 * R + r) * a.acc_stride + q] = acc117;
-  a.acc[(133 * R + r) * a.acc_stride + q] = acc133;
-  a.acc[(149 * R + r) * a.acc_stride + q] = acc149;
-  a.acc[(165 * R + r) * a.acc_stride + q] = acc165;
-  a.acc[(181 * R + r) * a.acc_stride + q] = acc181;
-  a.acc[(197 * R + r) * a.acc_stride + q] = acc197;
-  a.acc[(213 * R + r) * a.acc_stride + q] = acc213;
-  a.acc[(229 * R + r) * a.acc_stride + q] = acc229;
-  a.acc[(245 * R + r) * a.acc_stride + q] = acc245;
-  a.acc[(6 * R + r) * a.acc_stride + q] = acc6;
-  a.acc[(22 * R + r) * a.acc_stride + q] = acc22;
-  a.acc[(38 * R + r) * a.acc_stride + q] = acc38;
-  a.acc[(54 * R + r) * a.acc_stride + q] = acc54;
-  a.acc[(70 * R + r) * a.acc_stride + q] = acc70;
-  a.acc[(86 * R + r) * a.acc_stride + q] = acc86;
-  a.acc[(102 * R + r) * a.acc_stride + q] = acc102;
-  a.acc[(118 * R + r) * a.acc_stride + q] = acc118;
-  a.acc[(134 * R + r) * a.acc_stride + q] = acc134;
-  a.acc[(150 * R + r) * a.acc_stride + q] = acc150;
-  a.acc[(166 * R + r) * a.acc_stride + q] = acc166;
-  a.acc[(182 * R + r) * a.acc_stride + q] = acc182;
-  a.acc[(198 * R + r) * a.acc_stride + q] = acc198;
-  a.acc[(214 * R + r) * a.acc_stride + q] = acc214;
-  a.acc[(230 * R + r) * a.acc_stride + q] = acc230;
-  a.acc[(246 * R + r) * a.acc_stride + q] = acc246;
-  a.acc[(7 * R + r) * a.acc_stride + q] = acc7;
-  a.acc[(23 * R + r) * a.acc_stride + q] = acc23;
-  a.acc[(39 * R + r) * a.acc_stride + q] = acc39;
-  a.acc[(55 * R + r) * a.acc_stride + q] = acc55;
-  a.acc[(71 * R + r) * a.acc_stride + q] = acc71;
-  a.acc[(87 * R + r) * a.acc_stride + q] = acc87;
-  a.acc[(103 * R + r) * a.acc_stride + q] = acc103;
-  a.acc[(119 * R + r) * a.acc_stride + q] = acc119;
-  a.acc[(135 * R + r) * a.acc_stride + q] = acc135;
-  a.acc[(151 * R + r) * a.acc_stride + q] = acc151;
-  a.acc[(167 * R + r) * a.acc_stride + q] = acc167;
-  a.acc[(183 * R + r) * a.acc_stride + q] = acc183;
-  a.acc[(199 * R + r) * a.acc_stride + q] = acc199;
-  a.acc[(215 * R + r) * a.acc_stride + q] = acc215;
-  a.acc[(231 * R + r) * a.acc_stride + q] = acc231;
-  a.acc[(247 * R + r) * a.acc_stride + q] = acc247;
-  a.acc[(8 * R + r) * a.acc_stride + q] = acc8;
-  a.acc[(24 * R + r) * a.acc_stride + q] = acc24;
-  a.acc[(40 * R + r) * a.acc_stride + q] = acc40;
-  a.acc[(56 * R + r) * a.acc_stride + q] = acc56;
-  a.acc[(72 * R + r) * a.acc_stride + q] = acc72;
-  a.acc[(88 * R + r) * a.acc_stride + q] = acc88;
-  a.acc[(104 * R + r) * a.acc_stride + q] = acc104;
-  a.acc[(120 * R + r) * a.acc_stride + q] = acc120;
-  a.acc[(136 * R + r) * a.acc_stride + q] = acc136;
-  a.acc[(152 * R + r) * a.acc_stride + q] = acc152;
-  a.acc[(168 * R + r) * a.acc_stride + q] = acc168;
-  a.acc[(184 * R + r) * a.acc_stride + q] = acc184;
-  a.acc[(200 * R + r) * a.acc_stride + q] = acc200;
-  a.acc[(216 * R + r) * a.acc_stride + q] = acc216;
-  a.acc[(232 * R + r) * a.acc_stride + q] = acc232;
-  a.acc[(248 * R + r) * a.acc_stride + q] = acc248;
-  a.acc[(9 * R + r) * a.acc_stride + q] = acc9;
-  a.acc[(25 * R + r) * a.acc_stride + q] = acc25;
-  a.acc[(41 * R + r) * a.acc_stride + q] = acc41;
-  a.acc[(57 * R + r) * a.acc_stride + q] = acc57;
-  a.acc[(73 * R + r) * a.acc_stride + q] = acc73;
-  a.acc[(89 * R + r) * a.acc_stride + q] = acc89;
-  a.acc[(105 * R + r) * a.acc_stride + q] = acc105;
-  a.acc[(121 * R + r) * a.acc_stride + q] = acc121;
-  a.acc[(137 * R + r) * a.acc_stride + q] = acc137;
-  a.acc[(153 * R + r) * a.acc_stride + q] = acc153;
-  a.acc[(169 * R + r) * a.acc_stride + q] = acc169;
-  a.acc[(185 * R + r) * a.acc_stride + q] = acc185;
-  a.acc[(201 * R + r) * a.acc_stride + q] = acc201;
-  a.acc[(217 * R + r) * a.acc_stride + q] = acc217;
-  a.acc[(233 * R + r) * a.acc_stride + q] = acc233;
-  a.acc[(249 * R + r) * a.acc_stride + q] = acc249;
-  a.acc[(10 * R + r) * a.acc_stride + q] = acc10;
-  a.acc[(26 * R + r) * a.acc_stride + q] = acc26;
-  a.acc[(42 * R + r) * a.acc_stride + q] = acc42;
-  a.acc[(58 * R + r) * a.acc_stride + q] = acc58;
-  a.acc[(74 * R + r) * a.acc_stride + q] = acc74;
-  a.acc[(90 * R + r) * a.acc_stride + q] = acc90;
-  a.acc[(106 * R + r) * a.acc_stride + q] = acc106;
-  a.acc[(122 * R + r) * a.acc_stride + q] = acc122;
-  a.acc[(138 * R + r) * a.acc_stride + q] = acc138;
-  a.acc[(154 * R + r) * a.acc_stride + q] = acc154;
-}
-// class (21|21) split 2/3: 315 recursion values, 86 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2121_2(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc170 = 0.0;
-  double acc186 = 0.0;
-  double acc202 = 0.0;
-  double acc218 = 0.0;
-  double acc234 = 0.0;
-  double acc250 = 0.0;
-  double acc11 = 0.0;
-  double acc27 = 0.0;
-  double acc43 = 0.0;
-  double acc59 = 0.0;
-  double acc75 = 0.0;
-  double acc91 = 0.0;
-  double acc107 = 0.0;
-  double acc123 = 0.0;
-  double acc139 = 0.0;
-  double acc155 = 0.0;
-  double acc171 = 0.0;
-  double acc187 = 0.0;
-  double acc203 = 0.0;
-  double acc219 = 0.0;
-  double acc235 = 0.0;
-  double acc251 = 0.0;
-  double acc12 = 0.0;
-  double acc28 = 0.0;
-  double acc44 = 0.0;
-  double acc60 = 0.0;
-  double acc76 = 0.0;
-  double acc92 = 0.0;
-  double acc108 = 0.0;
-  double acc124 = 0.0;
-  double acc140 = 0.0;
-  double acc156 = 0.0;
-  double acc172 = 0.0;
-  double acc188 = 0.0;
-  double acc204 = 0.0;
-  double acc220 = 0.0;
-  double acc236 = 0.0;
-  double acc252 = 0.0;
-  double acc13 = 0.0;
-  double acc29 = 0.0;
-  double acc45 = 0.0;
-  double acc61 = 0.0;
-  double acc77 = 0.0;
-  double acc93 = 0.0;
-  double acc109 = 0.0;
-  double acc125 = 0.0;
-  double acc141 = 0.0;
-  double acc157 = 0.0;
-  double acc173 = 0.0;
-  double acc189 = 0.0;
-  double acc205 = 0.0;
-  double acc221 = 0.0;
-  double acc237 = 0.0;
-  double acc253 = 0.0;
-  double acc14 = 0.0;
-  double acc30 = 0.0;
-  double acc46 = 0.0;
-  double acc62 = 0.0;
-  double acc78 = 0.0;
-  double acc94 = 0.0;
-  double acc110 = 0.0;
-  double acc126 = 0.0;
-  double acc142 = 0.0;
-  double acc158 = 0.0;
-  double acc174 = 0.0;
-  double acc190 = 0.0;
-  double acc206 = 0.0;
-  double acc222 = 0.0;
-  double acc238 = 0.0;
-  double acc254 = 0.0;
-  double acc15 = 0.0;
-  double acc31 = 0.0;
-  double acc47 = 0.0;
-  double acc63 = 0.0;
-  double acc79 = 0.0;
-  double acc95 = 0.0;
-  double acc111 = 0.0;
-  double acc127 = 0.0;
-  double acc143 = 0.0;
-  double acc159 = 0.0;
-  double acc175 = 0.0;
-  double acc191 = 0.0;
-  double acc207 = 0.0;
-  double acc223 = 0.0;
-  double acc239 = 0.0;
-  double acc255 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x8 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x20 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x21 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x22 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x23 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x24 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x54 = s.PA[2] * x21 + s.WP[2] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x52 = s.PA[1] * x20 + s.WP[1] * x23 + z1 * (x4 - s.roz * x5);
-      const TB x53 = s.PA[2] * x20 + s.WP[2] * x23;
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x27 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x28 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x59 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x60 = s.PA[2] * x27 + s.WP[2] * x33 + z1 * (x7 - s.roz * x10);
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x64 = s.PA[2] * x30 + s.WP[2] * x36 + z2 * (x9 - s.roz * x12);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x63 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x8 - s.roz * x11);
-      const TB x61 = s.PA[1] * x28 + s.WP[1] * x34 + z2 * (x8 - s.roz * x11);
-      const TB x62 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x70 = s.PA[2] * x33 + s.WP[2] * x39 + z1 * (x10 - s.roz * x13);
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x74 = s.PA[2] * x36 + s.WP[2] * x42 + z2 * (x12 - s.roz * x15);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x45 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x46 = s.PA[1] * x17 + s.WP[1] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x48 = s.PA[2] * x18 + s.WP[2] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x94 = s.PA[2] * x48 + s.WP[2] * x54 + z2 * (x18 - s.roz * x21);
-      const TB x47 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x93 = s.PA[2] * x47 + s.WP[2] * x53 + z1 * (x17 - s.roz * x20);
-      const TB x91 = s.PA[1] * x46 + s.WP[1] * x52 + z2 * (x17 - s.roz * x20);
-      const TB x92 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x50 = s.PA[1] * x19 + s.WP[1] * x22;
-      const TB x51 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x90 = s.PA[2] * x45 + s.WP[2] * x51 + z1 * (x16 - s.roz * x19);
-      const TB x89 = s.PA[2] * x44 + s.WP[2] * x50;
-      const TB x69 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x71 = s.PA[1] * x34 + s.WP[1] * x40 + z2 * (x11 - s.roz * x14);
-      const TB x73 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x11 - s.roz * x14);
-      const TB x72 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x79 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x80 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x13 - s.roz * x16);
-      const TB x81 = s.PA[1] * x40 + s.WP[1] * x46 + z2 * (x14 - s.roz * x17);
-      const TB x82 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x83 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x14 - s.roz * x17);
-      const TB x84 = s.PA[2] * x42 + s.WP[2] * x48 + z2 * (x15 - s.roz * x18);
-      const TB x97 = s.QC[0] * x13 + s.WQ[0] * x16 + r1 * x3;
-      const TB x99 = s.QC[0] * x14 + s.WQ[0] * x17;
-      const TB x101 = s.QC[0] * x15 + s.WQ[0] * x18;
-      const TB x106 = s.QC[0] * x32 + s.WQ[0] * x38 + r1 * x14;
-      const TB x107 = s.QC[0] * x38 + s.WQ[0] * x44 + r1 * x17;
-      const TB x109 = s.QC[0] * x33 + s.WQ[0] * x39 + r1 * x15;
-      const TB x110 = s.QC[0] * x39 + s.WQ[0] * x45 + r1 * x18;
-      const TB x112 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x113 = s.QC[0] * x40 + s.WQ[0] * x46;
-      const TB x115 = s.QC[0] * x35 + s.WQ[0] * x41;
-      const TB x116 = s.QC[0] * x41 + s.WQ[0] * x47;
-      const TB x118 = s.QC[0] * x36 + s.WQ[0] * x42;
-      const TB x119 = s.QC[0] * x42 + s.WQ[0] * x48;
-      const TB x132 = s.QC[0] * x59 + s.WQ[0] * x69 + r1 * x35;
-      const TB x133 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x41;
-      const TB x134 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x47;
-      const TB x135 = s.QC[0] * x60 + s.WQ[0] * x70 + r1 * x36;
-      const TB x136 = s.QC[0] * x70 + s.WQ[0] * x80 + r1 * x42;
-      const TB x137 = s.QC[0] * x80 + s.WQ[0] * x90 + r1 * x48;
-      const TB x138 = s.QC[0] * x61 + s.WQ[0] * x71;
-      const TB x139 = s.QC[0] * x71 + s.WQ[0] * x81;
-      const TB x140 = s.QC[0] * x81 + s.WQ[0] * x91;
-      const TB x141 = s.QC[0] * x62 + s.WQ[0] * x72;
-      const TB x142 = s.QC[0] * x72 + s.WQ[0] * x82;
-      const TB x143 = s.QC[0] * x82 + s.WQ[0] * x92;
-      const TB x144 = s.QC[0] * x63 + s.WQ[0] * x73;
-      const TB x145 = s.QC[0] * x73 + s.WQ[0] * x83;
-      const TB x146 = s.QC[0] * x83 + s.WQ[0] * x93;
-      const TB x147 = s.QC[0] * x64 + s.WQ[0] * x74;
-      const TB x148 = s.QC[0] * x74 + s.WQ[0] * x84;
-      const TB x149 = s.QC[0] * x84 + s.WQ[0] * x94;
-      const TB x152 = s.QC[1] * x13 + s.WQ[1] * x16;
-      const TB x154 = s.QC[1] * x14 + s.WQ[1] * x17 + r1 * x3;
-      const TB x211 = s.QC[2] * x15 + s.WQ[2] * x18 + r1 * x3;
-      const TB x156 = s.QC[1] * x15 + s.WQ[1] * x18;
-      const TB x161 = s.QC[1] * x32 + s.WQ[1] * x38 + r1 * x13;
-      const TB x162 = s.QC[1] * x38 + s.WQ[1] * x44 + r1 * x16;
-      const TB x164 = s.QC[1] * x33 + s.WQ[1] * x39;
-      const TB x165 = s.QC[1] * x39 + s.WQ[1] * x45;
-      const TB x167 = s.QC[1] * x34 + s.WQ[1] * x40 + r2 * x14;
-      const TB x168 = s.QC[1] * x40 + s.WQ[1] * x46 + r2 * x17;
-      const TB x170 = s.QC[1] * x35 + s.WQ[1] * x41 + r1 * x15;
-      const TB x228 = s.QC[2] * x36 + s.WQ[2] * x42 + r2 * x15;
-      const TB x171 = s.QC[1] * x41 + s.WQ[1] * x47 + r1 * x18;
-      const TB x229 = s.QC[2] * x42 + s.WQ[2] * x48 + r2 * x18;
-      const TB x449 = s.QC[2] * x228 + s.WQ[2] * x229 + e1 * (x36 - s.roe * x42) + r2 * x211;
-      const TB x173 = s.QC[1] * x36 + s.WQ[1] * x42;
-      const TB x174 = s.QC[1] * x42 + s.WQ[1] * x48;
-      const TB x187 = s.QC[1] * x59 + s.WQ[1] * x69 + r1 * x33;
-      const TB x188 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x39;
-      const TB x189 = s.QC[1] * x79 + s.WQ[1] * x89 + r1 * x45;
-      const TB x244 = s.QC[2] * x79 + s.WQ[2] * x89 + r1 * x44;
-      const TB x217 = s.QC[2] * x38 + s.WQ[2] * x44;
-      const TB x190 = s.QC[1] * x60 + s.WQ[1] * x70;
-      const TB x191 = s.QC[1] * x70 + s.WQ[1] * x80;
-      const TB x192 = s.QC[1] * x80 + s.WQ[1] * x90;
-      const TB x247 = s.QC[2] * x80 + s.WQ[2] * x90 + r2 * x45;
-      const TB x220 = s.QC[2] * x39 + s.WQ[2] * x45 + r1 * x16;
-      const TB x207 = s.QC[2] * x13 + s.WQ[2] * x16;
-      const TB x219 = s.QC[2] * x33 + s.WQ[2] * x39 + r1 * x13;
-      const TB x443 = s.QC[2] * x219 + s.WQ[2] * x220 + e1 * (x33 - s.roe * x39) + r1 * x207;
-      const TB x193 = s.QC[1] * x61 + s.WQ[1] * x71 + r3 * x34;
-      const TB x194 = s.QC[1] * x71 + s.WQ[1] * x81 + r3 * x40;
-      const TB x195 = s.QC[1] * x81 + s.WQ[1] * x91 + r3 * x46;
-      const TB x250 = s.QC[2] * x81 + s.WQ[2] * x91;
-      const TB x196 = s.QC[1] * x62 + s.WQ[1] * x72 + r2 * x35;
-      const TB x197 = s.QC[1] * x72 + s.WQ[1] * x82 + r2 * x41;
-      const TB x198 = s.QC[1] * x82 + s.WQ[1] * x92 + r2 * x47;
-      const TB x253 = s.QC[2] * x82 + s.WQ[2] * x92 + r1 * x46;
-      const TB x223 = s.QC[2] * x40 + s.WQ[2] * x46;
-      const TB x199 = s.QC[1] * x63 + s.WQ[1] * x73 + r1 * x36;
-      const TB x200 = s.QC[1] * x73 + s.WQ[1] * x83 + r1 * x42;
-      const TB x201 = s.QC[1] * x83 + s.WQ[1] * x93 + r1 * x48;
-      const TB x256 = s.QC[2] * x83 + s.WQ[2] * x93 + r2 * x47;
-      const TB x226 = s.QC[2] * x41 + s.WQ[2] * x47 + r1 * x17;
-      const TB x209 = s.QC[2] * x14 + s.WQ[2] * x17;
-      const TB x225 = s.QC[2] * x35 + s.WQ[2] * x41 + r1 * x14;
-      const TB x259 = s.QC[2] * x84 + s.WQ[2] * x94 + r3 * x48;
-      const TB x204 = s.QC[1] * x84 + s.WQ[1] * x94;
-      const TB x447 = s.QC[2] * x225 + s.WQ[2] * x226 + e1 * (x35 - s.roe * x41) + r1 * x209;
-      const TB x202 = s.QC[1] * x64 + s.WQ[1] * x74;
-      const TB x203 = s.QC[1] * x74 + s.WQ[1] * x84;
-      const TB x216 = s.QC[2] * x32 + s.WQ[2] * x38;
-      const TB x222 = s.QC[2] * x34 + s.WQ[2] * x40;
-      const TB x242 = s.QC[2] * x59 + s.WQ[2] * x69 + r1 * x32;
-      const TB x243 = s.QC[2] * x69 + s.WQ[2] * x79 + r1 * x38;
-      const TB x459 = s.QC[2] * x243 + s.WQ[2] * x244 + e1 * (x69 - s.roe * x79) + r1 * x217;
-      const TB x389 = s.QC[1] * x188 + s.WQ[1] * x189 + e1 * (x69 - s.roe * x79) + r1 * x165;
-      const TB x424 = s.QC[2] * x188 + s.WQ[2] * x189 + r1 * x162;
-      const TB x441 = s.QC[2] * x216 + s.WQ[2] * x217 + e1 * (x32 - s.roe * x38);
-      const TB x371 = s.QC[1] * x161 + s.WQ[1] * x162 + e1 * (x32 - s.roe * x38) + r1 * x152;
-      const TB x406 = s.QC[2] * x161 + s.WQ[2] * x162;
-      const TB x408 = s.QC[2] * x164 + s.WQ[2] * x165 + r1 * x152;
-      const TB x423 = s.QC[2] * x187 + s.WQ[2] * x188 + r1 * x161;
-      const TB x608 = s.QC[2] * x423 + s.WQ[2] * x424 + e1 * (x187 - s.roe * x188) + r1 * x406;
-      acc234 += static_cast<double>(x608);
-      const TB x458 = s.QC[2] * x242 + s.WQ[2] * x243 + e1 * (x59 - s.roe * x69) + r1 * x216;
-      const TB x624 = s.QC[2] * x458 + s.WQ[2] * x459 + e2 * (x242 - s.roe * x243) + r1 * x441;
-      acc250 += static_cast<double>(x624);
-      const TB x388 = s.QC[1] * x187 + s.WQ[1] * x188 + e1 * (x59 - s.roe * x69) + r1 * x164;
-      const TB x592 = s.QC[2] * x388 + s.WQ[2] * x389 + r1 * x371;
-      acc218 += static_cast<double>(x592);
-      const TB x245 = s.QC[2] * x60 + s.WQ[2] * x70 + r2 * x33;
-      const TB x246 = s.QC[2] * x70 + s.WQ[2] * x80 + r2 * x39;
-      const TB x461 = s.QC[2] * x246 + s.WQ[2] * x247 + e1 * (x70 - s.roe * x80) + r2 * x220;
-      const TB x460 = s.QC[2] * x245 + s.WQ[2] * x246 + e1 * (x60 - s.roe * x70) + r2 * x219;
-      acc91 += static_cast<double>(x460);
-      const TB x625 = s.QC[2] * x460 + s.WQ[2] * x461 + e2 * (x245 - s.roe * x246) + r2 * x443;
-      acc251 += static_cast<double>(x625);
-      const TB x248 = s.QC[2] * x61 + s.WQ[2] * x71;
-      const TB x249 = s.QC[2] * x71 + s.WQ[2] * x81;
-      const TB x463 = s.QC[2] * x249 + s.WQ[2] * x250 + e1 * (x71 - s.roe * x81);
-      const TB x251 = s.QC[2] * x62 + s.WQ[2] * x72 + r1 * x34;
-      const TB x252 = s.QC[2] * x72 + s.WQ[2] * x82 + r1 * x40;
-      const TB x465 = s.QC[2] * x252 + s.WQ[2] * x253 + e1 * (x72 - s.roe * x82) + r1 * x223;
-      const TB x445 = s.QC[2] * x222 + s.WQ[2] * x223 + e1 * (x34 - s.roe * x40);
-      const TB x464 = s.QC[2] * x251 + s.WQ[2] * x252 + e1 * (x62 - s.roe * x72) + r1 * x222;
-      acc93 += static_cast<double>(x464);
-      const TB x627 = s.QC[2] * x464 + s.WQ[2] * x465 + e2 * (x251 - s.roe * x252) + r1 * x445;
-      acc253 += static_cast<double>(x627);
-      const TB x254 = s.QC[2] * x63 + s.WQ[2] * x73 + r2 * x35;
-      const TB x255 = s.QC[2] * x73 + s.WQ[2] * x83 + r2 * x41;
-      const TB x467 = s.QC[2] * x255 + s.WQ[2] * x256 + e1 * (x73 - s.roe * x83) + r2 * x226;
-      const TB x466 = s.QC[2] * x254 + s.WQ[2] * x255 + e1 * (x63 - s.roe * x73) + r2 * x225;
-      acc94 += static_cast<double>(x466);
-      const TB x628 = s.QC[2] * x466 + s.WQ[2] * x467 + e2 * (x254 - s.roe * x255) + r2 * x447;
-      acc254 += static_cast<double>(x628);
-      const TB x257 = s.QC[2] * x64 + s.WQ[2] * x74 + r3 * x36;
-      const TB x258 = s.QC[2] * x74 + s.WQ[2] * x84 + r3 * x42;
-      const TB x469 = s.QC[2] * x258 + s.WQ[2] * x259 + e1 * (x74 - s.roe * x84) + r3 * x229;
-      const TB x468 = s.QC[2] * x257 + s.WQ[2] * x258 + e1 * (x64 - s.roe * x74) + r3 * x228;
-      acc95 += static_cast<double>(x468);
-      const TB x629 = s.QC[2] * x468 + s.WQ[2] * x469 + e2 * (x257 - s.roe * x258) + r3 * x449;
-      acc255 += static_cast<double>(x629);
-      const TB x268 = s.QC[0] * x109 + s.WQ[0] * x110 + e1 * (x33 - s.roe * x39) + r1 * x101;
-      const TB x373 = s.QC[1] * x164 + s.WQ[1] * x165 + e1 * (x33 - s.roe * x39);
-      const TB x576 = s.QC[1] * x388 + s.WQ[1] * x389 + e2 * (x187 - s.roe * x188) + r1 * x373;
-      acc202 += static_cast<double>(x576);
-      const TB x425 = s.QC[2] * x190 + s.WQ[2] * x191 + r2 * x164;
-      acc75 += static_cast<double>(x425);
-      const TB x426 = s.QC[2] * x191 + s.WQ[2] * x192 + r2 * x165;
-      const TB x609 = s.QC[2] * x425 + s.WQ[2] * x426 + e1 * (x190 - s.roe * x191) + r2 * x408;
-      acc235 += static_cast<double>(x609);
-      const TB x391 = s.QC[1] * x191 + s.WQ[1] * x192 + e1 * (x70 - s.roe * x80);
-      const TB x286 = s.QC[0] * x136 + s.WQ[0] * x137 + e1 * (x70 - s.roe * x80) + r1 * x119;
-      const TB x270 = s.QC[0] * x112 + s.WQ[0] * x113 + e1 * (x34 - s.roe * x40);
-      const TB x375 = s.QC[1] * x167 + s.WQ[1] * x168 + e1 * (x34 - s.roe * x40) + r2 * x154;
-      const TB x412 = s.QC[2] * x170 + s.WQ[2] * x171 + r1 * x154;
-      const TB x272 = s.QC[0] * x115 + s.WQ[0] * x116 + e1 * (x35 - s.roe * x41);
-      const TB x377 = s.QC[1] * x170 + s.WQ[1] * x171 + e1 * (x35 - s.roe * x41) + r1 * x156;
-      const TB x414 = s.QC[2] * x173 + s.WQ[2] * x174 + r2 * x156;
-      const TB x274 = s.QC[0] * x118 + s.WQ[0] * x119 + e1 * (x36 - s.roe * x42);
-      const TB x379 = s.QC[1] * x173 + s.WQ[1] * x174 + e1 * (x36 - s.roe * x42);
-      const TB x285 = s.QC[0] * x135 + s.WQ[0] * x136 + e1 * (x60 - s.roe * x70) + r1 * x118;
-      acc11 += static_cast<double>(x285);
-      const TB x390 = s.QC[1] * x190 + s.WQ[1] * x191 + e1 * (x60 - s.roe * x70);
-      acc59 += static_cast<double>(x390);
-      const TB x577 = s.QC[1] * x390 + s.WQ[1] * x391 + e2 * (x190 - s.roe * x191);
-      acc203 += static_cast<double>(x577);
-      const TB x593 = s.QC[2] * x390 + s.WQ[2] * x391 + r2 * x373;
-      acc219 += static_cast<double>(x593);
-      const TB x513 = s.QC[2] * x285 + s.WQ[2] * x286 + r2 * x268;
-      acc139 += static_cast<double>(x513);
-      const TB x481 = s.QC[0] * x285 + s.WQ[0] * x286 + e2 * (x135 - s.roe * x136) + r1 * x274;
-      acc107 += static_cast<double>(x481);
-      const TB x497 = s.QC[1] * x285 + s.WQ[1] * x286;
-      acc123 += static_cast<double>(x497);
-      const TB x287 = s.QC[0] * x138 + s.WQ[0] * x139 + e1 * (x61 - s.roe * x71);
-      acc12 += static_cast<double>(x287);
-      const TB x288 = s.QC[0] * x139 + s.WQ[0] * x140 + e1 * (x71 - s.roe * x81);
-      const TB x393 = s.QC[1] * x194 + s.WQ[1] * x195 + e1 * (x71 - s.roe * x81) + r3 * x168;
-      const TB x428 = s.QC[2] * x194 + s.WQ[2] * x195;
-      const TB x482 = s.QC[0] * x287 + s.WQ[0] * x288 + e2 * (x138 - s.roe * x139);
-      acc108 += static_cast<double>(x482);
-      const TB x498 = s.QC[1] * x287 + s.WQ[1] * x288 + r3 * x270;
-      acc124 += static_cast<double>(x498);
-      const TB x514 = s.QC[2] * x287 + s.WQ[2] * x288;
-      acc140 += static_cast<double>(x514);
-      const TB x289 = s.QC[0] * x141 + s.WQ[0] * x142 + e1 * (x62 - s.roe * x72);
-      acc13 += static_cast<double>(x289);
-      const TB x394 = s.QC[1] * x196 + s.WQ[1] * x197 + e1 * (x62 - s.roe * x72) + r2 * x170;
-      acc61 += static_cast<double>(x394);
-      const TB x431 = s.QC[2] * x199 + s.WQ[2] * x200 + r2 * x170;
-      acc78 += static_cast<double>(x431);
-      const TB x290 = s.QC[0] * x142 + s.WQ[0] * x143 + e1 * (x72 - s.roe * x82);
-      const TB x395 = s.QC[1] * x197 + s.WQ[1] * x198 + e1 * (x72 - s.roe * x82) + r2 * x171;
-      const TB x515 = s.QC[2] * x289 + s.WQ[2] * x290 + r1 * x270;
-      acc141 += static_cast<double>(x515);
-      const TB x430 = s.QC[2] * x197 + s.WQ[2] * x198 + r1 * x168;
-      const TB x410 = s.QC[2] * x167 + s.WQ[2] * x168;
-      const TB x432 = s.QC[2] * x200 + s.WQ[2] * x201 + r2 * x171;
-      const TB x612 = s.QC[2] * x431 + s.WQ[2] * x432 + e1 * (x199 - s.roe * x200) + r2 * x412;
-      acc238 += static_cast<double>(x612);
-      const TB x397 = s.QC[1] * x200 + s.WQ[1] * x201 + e1 * (x73 - s.roe * x83) + r1 * x174;
-      const TB x292 = s.QC[0] * x145 + s.WQ[0] * x146 + e1 * (x73 - s.roe * x83);
-      const TB x434 = s.QC[2] * x203 + s.WQ[2] * x204 + r3 * x174;
-      const TB x399 = s.QC[1] * x203 + s.WQ[1] * x204 + e1 * (x74 - s.roe * x84);
-      const TB x294 = s.QC[0] * x148 + s.WQ[0] * x149 + e1 * (x74 - s.roe * x84);
-      const TB x483 = s.QC[0] * x289 + s.WQ[0] * x290 + e2 * (x141 - s.roe * x142);
-      acc109 += static_cast<double>(x483);
-      const TB x499 = s.QC[1] * x289 + s.WQ[1] * x290 + r2 * x272;
-      acc125 += static_cast<double>(x499);
-      const TB x579 = s.QC[1] * x394 + s.WQ[1] * x395 + e2 * (x196 - s.roe * x197) + r2 * x377;
-      acc205 += static_cast<double>(x579);
-      const TB x595 = s.QC[2] * x394 + s.WQ[2] * x395 + r1 * x375;
-      acc221 += static_cast<double>(x595);
-      const TB x291 = s.QC[0] * x144 + s.WQ[0] * x145 + e1 * (x63 - s.roe * x73);
-      acc14 += static_cast<double>(x291);
-      const TB x396 = s.QC[1] * x199 + s.WQ[1] * x200 + e1 * (x63 - s.roe * x73) + r1 * x173;
-      acc62 += static_cast<double>(x396);
-      const TB x580 = s.QC[1] * x396 + s.WQ[1] * x397 + e2 * (x199 - s.roe * x200) + r1 * x379;
-      acc206 += static_cast<double>(x580);
-      const TB x596 = s.QC[2] * x396 + s.WQ[2] * x397 + r2 * x377;
-      acc222 += static_cast<double>(x596);
-      const TB x516 = s.QC[2] * x291 + s.WQ[2] * x292 + r2 * x272;
-      acc142 += static_cast<double>(x516);
-      const TB x433 = s.QC[2] * x202 + s.WQ[2] * x203 + r3 * x173;
-      acc79 += static_cast<double>(x433);
-      const TB x613 = s.QC[2] * x433 + s.WQ[2] * x434 + e1 * (x202 - s.roe * x203) + r3 * x414;
-      acc239 += static_cast<double>(x613);
-      const TB x484 = s.QC[0] * x291 + s.WQ[0] * x292 + e2 * (x144 - s.roe * x145);
-      acc110 += static_cast<double>(x484);
-      const TB x500 = s.QC[1] * x291 + s.WQ[1] * x292 + r1 * x274;
-      acc126 += static_cast<double>(x500);
-      const TB x293 = s.QC[0] * x147 + s.WQ[0] * x148 + e1 * (x64 - s.roe * x74);
-      acc15 += static_cast<double>(x293);
-      const TB x398 = s.QC[1] * x202 + s.WQ[1] * x203 + e1 * (x64 - s.roe * x74);
-      acc63 += static_cast<double>(x398);
-      const TB x581 = s.QC[1] * x398 + s.WQ[1] * x399 + e2 * (x202 - s.roe * x203);
-      acc207 += static_cast<double>(x581);
-      const TB x597 = s.QC[2] * x398 + s.WQ[2] * x399 + r3 * x379;
-      acc223 += static_cast<double>(x597);
-      const TB x517 = s.QC[2] * x293 + s.WQ[2] * x294 + r3 * x274;
-      acc143 += static_cast<double>(x517);
-      const TB x485 = s.QC[0] * x293 + s.WQ[0] * x294 + e2 * (x147 - s.roe * x148);
-      acc111 += static_cast<double>(x485);
-      const TB x501 = s.QC[1] * x293 + s.WQ[1] * x294;
-      acc127 += static_cast<double>(x501);
-      const TB x301 = s.QC[1] * x106 + s.WQ[1] * x107 + r1 * x97;
-      const TB x338 = s.QC[2] * x109 + s.WQ[2] * x110 + r1 * x97;
-      const TB x303 = s.QC[1] * x109 + s.WQ[1] * x110;
-      const TB x305 = s.QC[1] * x112 + s.WQ[1] * x113 + r2 * x99;
-      const TB x342 = s.QC[2] * x115 + s.WQ[2] * x116 + r1 * x99;
-      const TB x307 = s.QC[1] * x115 + s.WQ[1] * x116 + r1 * x101;
-      const TB x344 = s.QC[2] * x118 + s.WQ[2] * x119 + r2 * x101;
-      const TB x309 = s.QC[1] * x118 + s.WQ[1] * x119;
-      const TB x318 = s.QC[1] * x132 + s.WQ[1] * x133 + r1 * x109;
-      const TB x355 = s.QC[2] * x135 + s.WQ[2] * x136 + r2 * x109;
-      acc43 += static_cast<double>(x355);
-      const TB x319 = s.QC[1] * x133 + s.WQ[1] * x134 + r1 * x110;
-      const TB x544 = s.QC[2] * x318 + s.WQ[2] * x319 + r1 * x301;
-      acc170 += static_cast<double>(x544);
-      const TB x354 = s.QC[2] * x133 + s.WQ[2] * x134 + r1 * x107;
-      const TB x336 = s.QC[2] * x106 + s.WQ[2] * x107;
-      const TB x353 = s.QC[2] * x132 + s.WQ[2] * x133 + r1 * x106;
-      const TB x560 = s.QC[2] * x353 + s.WQ[2] * x354 + e1 * (x132 - s.roe * x133) + r1 * x336;
-      acc186 += static_cast<double>(x560);
-      const TB x356 = s.QC[2] * x136 + s.WQ[2] * x137 + r2 * x110;
-      const TB x561 = s.QC[2] * x355 + s.WQ[2] * x356 + e1 * (x135 - s.roe * x136) + r2 * x338;
-      acc187 += static_cast<double>(x561);
-      const TB x321 = s.QC[1] * x136 + s.WQ[1] * x137;
-      const TB x320 = s.QC[1] * x135 + s.WQ[1] * x136;
-      acc27 += static_cast<double>(x320);
-      const TB x529 = s.QC[1] * x320 + s.WQ[1] * x321 + e1 * (x135 - s.roe * x136);
-      acc155 += static_cast<double>(x529);
-      const TB x545 = s.QC[2] * x320 + s.WQ[2] * x321 + r2 * x303;
-      acc171 += static_cast<double>(x545);
-      const TB x322 = s.QC[1] * x138 + s.WQ[1] * x139 + r3 * x112;
-      acc28 += static_cast<double>(x322);
-      const TB x323 = s.QC[1] * x139 + s.WQ[1] * x140 + r3 * x113;
-      const TB x358 = s.QC[2] * x139 + s.WQ[2] * x140;
-      const TB x530 = s.QC[1] * x322 + s.WQ[1] * x323 + e1 * (x138 - s.roe * x139) + r3 * x305;
-      acc156 += static_cast<double>(x530);
-      const TB x546 = s.QC[2] * x322 + s.WQ[2] * x323;
-      acc172 += static_cast<double>(x546);
-      const TB x324 = s.QC[1] * x141 + s.WQ[1] * x142 + r2 * x115;
-      acc29 += static_cast<double>(x324);
-      const TB x361 = s.QC[2] * x144 + s.WQ[2] * x145 + r2 * x115;
-      acc46 += static_cast<double>(x361);
-      const TB x325 = s.QC[1] * x142 + s.WQ[1] * x143 + r2 * x116;
-      const TB x547 = s.QC[2] * x324 + s.WQ[2] * x325 + r1 * x305;
-      acc173 += static_cast<double>(x547);
-      const TB x531 = s.QC[1] * x324 + s.WQ[1] * x325 + e1 * (x141 - s.roe * x142) + r2 * x307;
-      acc157 += static_cast<double>(x531);
-      const TB x360 = s.QC[2] * x142 + s.WQ[2] * x143 + r1 * x113;
-      const TB x340 = s.QC[2] * x112 + s.WQ[2] * x113;
-      const TB x359 = s.QC[2] * x141 + s.WQ[2] * x142 + r1 * x112;
-      acc45 += static_cast<double>(x359);
-      const TB x563 = s.QC[2] * x359 + s.WQ[2] * x360 + e1 * (x141 - s.roe * x142) + r1 * x340;
-      acc189 += static_cast<double>(x563);
-      const TB x362 = s.QC[2] * x145 + s.WQ[2] * x146 + r2 * x116;
-      const TB x564 = s.QC[2] * x361 + s.WQ[2] * x362 + e1 * (x144 - s.roe * x145) + r2 * x342;
-      acc190 += static_cast<double>(x564);
-      const TB x327 = s.QC[1] * x145 + s.WQ[1] * x146 + r1 * x119;
-      const TB x364 = s.QC[2] * x148 + s.WQ[2] * x149 + r3 * x119;
-      const TB x329 = s.QC[1] * x148 + s.WQ[1] * x149;
-      const TB x326 = s.QC[1] * x144 + s.WQ[1] * x145 + r1 * x118;
-      acc30 += static_cast<double>(x326);
-      const TB x532 = s.QC[1] * x326 + s.WQ[1] * x327 + e1 * (x144 - s.roe * x145) + r1 * x309;
-      acc158 += static_cast<double>(x532);
-      const TB x548 = s.QC[2] * x326 + s.WQ[2] * x327 + r2 * x307;
-      acc174 += static_cast<double>(x548);
-      const TB x363 = s.QC[2] * x147 + s.WQ[2] * x148 + r3 * x118;
-      acc47 += static_cast<double>(x363);
-      const TB x565 = s.QC[2] * x363 + s.WQ[2] * x364 + e1 * (x147 - s.roe * x148) + r3 * x344;
-      acc191 += static_cast<double>(x565);
-      const TB x328 = s.QC[1] * x147 + s.WQ[1] * x148;
-      acc31 += static_cast<double>(x328);
-      const TB x533 = s.QC[1] * x328 + s.WQ[1] * x329 + e1 * (x147 - s.roe * x148);
-      acc159 += static_cast<double>(x533);
-      const TB x549 = s.QC[2] * x328 + s.WQ[2] * x329 + r3 * x309;
-      acc175 += static_cast<double>(x549);
-      const TB x357 = s.QC[2] * x138 + s.WQ[2] * x139;
-      acc44 += static_cast<double>(x357);
-      const TB x562 = s.QC[2] * x357 + s.WQ[2] * x358 + e1 * (x138 - s.roe * x139);
-      acc188 += static_cast<double>(x562);
-      const TB x392 = s.QC[1] * x193 + s.WQ[1] * x194 + e1 * (x61 - s.roe * x71) + r3 * x167;
-      acc60 += static_cast<double>(x392);
-      const TB x462 = s.QC[2] * x248 + s.WQ[2] * x249 + e1 * (x61 - s.roe * x71);
-      acc92 += static_cast<double>(x462);
-      const TB x626 = s.QC[2] * x462 + s.WQ[2] * x463 + e2 * (x248 - s.roe * x249);
-      acc252 += static_cast<double>(x626);
-      const TB x578 = s.QC[1] * x392 + s.WQ[1] * x393 + e2 * (x193 - s.roe * x194) + r3 * x375;
-      acc204 += static_cast<double>(x578);
-      const TB x594 = s.QC[2] * x392 + s.WQ[2] * x393;
-      acc220 += static_cast<double>(x594);
-      const TB x429 = s.QC[2] * x196 + s.WQ[2] * x197 + r1 * x167;
-      acc77 += static_cast<double>(x429);
-      const TB x611 = s.QC[2] * x429 + s.WQ[2] * x430 + e1 * (x196 - s.roe * x197) + r1 * x410;
-      acc237 += static_cast<double>(x611);
-      const TB x427 = s.QC[2] * x193 + s.WQ[2] * x194;
-      acc76 += static_cast<double>(x427);
-      const TB x610 = s.QC[2] * x427 + s.WQ[2] * x428 + e1 * (x193 - s.roe * x194);
-      acc236 += static_cast<double>(x610);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(170 * R + r) * a.acc_stride + q] = acc170;
-  a.acc[(186 * R + r) * a.acc_stride + q] = acc186;
-  a.acc[(202 * R + r) * a.acc_stride + q] = acc202;
-  a.acc[(218 * R + r) * a.acc_stride + q] = acc218;
-  a.acc[(234 * R + r) * a.acc_stride + q] = acc234;
-  a.acc[(250 * R + r) * a.acc_stride + q] = acc250;
-  a.acc[(11 * R + r) * a.acc_stride + q] = acc11;
-  a.acc[(27 * R + r) * a.acc_stride + q] = acc27;
-  a.acc[(43 * R + r) * a.acc_stride + q] = acc43;
-  a.acc[(59 * R + r) * a.acc_stride + q] = acc59;
-  a.acc[(75 * R + r) * a.acc_stride + q] = acc75;
-  a.acc[(91 * R + r) * a.acc_stride + q] = acc91;
-  a.acc[(107 * R + r) * a.acc_stride + q] = acc107;
-  a.acc[(123 * R + r) * a.acc_stride + q] = acc123;
-  a.acc[(139 * R + r) * a.acc_stride + q] = acc139;
-  a.acc[(155 * R + r) * a.acc_stride + q] = acc155;
-  a.acc[(171 * R + r) * a.acc_stride + q] = acc171;
-  a.acc[(187 * R + r) * a.acc_stride + q] = acc187;
-  a.acc[(203 * R + r) * a.acc_stride + q] = acc203;
-  a.acc[(219 * R + r) * a.acc_stride + q] = acc219;
-  a.acc[(235 * R + r) * a.acc_stride + q] = acc235;
-  a.acc[(251 * R + r) * a.acc_stride + q] = acc251;
-  a.acc[(12 * R + r) * a.acc_stride + q] = acc12;
-  a.acc[(28 * R + r) * a.acc_stride + q] = acc28;
-  a.acc[(44 * R + r) * a.acc_stride + q] = acc44;
-  a.acc[(60 * R + r) * a.acc_stride + q] = acc60;
-  a.acc[(76 * R + r) * a.acc_stride + q] = acc76;
-  a.acc[(92 * R + r) * a.acc_stride + q] = acc92;
-  a.acc[(108 * R + r) * a.acc_stride + q] = acc108;
-  a.acc[(124 * R + r) * a.acc_stride + q] = acc124;
-  a.acc[(140 * R + r) * a.acc_stride + q] = acc140;
-  a.acc[(156 * R + r) * a.acc_stride + q] = acc156;
-  a.acc[(172 * R + r) * a.acc_stride + q] = acc172;
-  a.acc[(188 * R + r) * a.acc_stride + q] = acc188;
-  a.acc[(204 * R + r) * a.acc_stride + q] = acc204;
-  a.acc[(220 * R + r) * a.acc_stride + q] = acc220;
-  a.acc[(236 * R + r) * a.acc_stride + q] = acc236;
-  a.acc[(252 * R + r) * a.acc_stride + q] = acc252;
-  a.acc[(13 * R + r) * a.acc_stride + q] = acc13;
-  a.acc[(29 * R + r) * a.acc_stride + q] = acc29;
-  a.acc[(45 * R + r) * a.acc_stride + q] = acc45;
-  a.acc[(61 * R + r) * a.acc_stride + q] = acc61;
-  a.acc[(77 * R + r) * a.acc_stride + q] = acc77;
-  a.acc[(93 * R + r) * a.acc_stride + q] = acc93;
-  a.acc[(109 * R + r) * a.acc_stride + q] = acc109;
-  a.acc[(125 * R + r) * a.acc_stride + q] = acc125;
-  a.acc[(141 * R + r) * a.acc_stride + q] = acc141;
-  a.acc[(157 * R + r) * a.acc_stride + q] = acc157;
-  a.acc[(173 * R + r) * a.acc_stride + q] = acc173;
-  a.acc[(189 * R + r) * a.acc_stride + q] = acc189;
-  a.acc[(205 * R + r) * a.acc_stride + q] = acc205;
-  a.acc[(221 * R + r) * a.acc_stride + q] = acc221;
-  a.acc[(237 * R + r) * a.acc_stride + q] = acc237;
-  a.acc[(253 * R + r) * a.acc_stride + q] = acc253;
-  a.acc[(14 * R + r) * a.acc_stride + q] = acc14;
-  a.acc[(30 * R + r) * a.acc_stride + q] = acc30;
-  a.acc[(46 * R + r) * a.acc_stride + q] = acc46;
-  a.acc[(62 * R + r) * a.acc_stride + q] = acc62;
-  a.acc[(78 * R + r) * a.acc_stride + q] = acc78;
-  a.acc[(94 * R + r) * a.acc_stride + q] = acc94;
-  a.acc[(110 * R + r) * a.acc_stride + q] = acc110;
-  a.acc[(126 * R + r) * a.acc_stride + q] = acc126;
-  a.acc[(142 * R + r) * a.acc_stride + q] = acc142;
-  a.acc[(158 * R + r) * a.acc_stride + q] = acc158;
-  a.acc[(174 * R + r) * a.acc_stride + q] = acc174;
-  a.acc[(190 * R + r) * a.acc_stride + q] = acc190;
-  a.acc[(206 * R + r) * a.acc_stride + q] = acc206;
-  a.acc[(222 * R + r) * a.acc_stride + q] = acc222;
-  a.acc[(238 * R + r) * a.acc_stride + q] = acc238;
-  a.acc[(254 * R + r) * a.acc_stride + q] = acc254;
-  a.acc[(15 * R + r) * a.acc_stride + q] = acc15;
-  a.acc[(31 * R + r) * a.acc_stride + q] = acc31;
-  a.acc[(47 * R + r) * a.acc_stride + q] = acc47;
-  a.acc[(63 * R + r) * a.acc_stride + q] = acc63;
-  a.acc[(79 * R + r) * a.acc_stride + q] = acc79;
-  a.acc[(95 * R + r) * a.acc_stride + q] = acc95;
-  a.acc[(111 * R + r) * a.acc_stride + q] = acc111;
-  a.acc[(127 * R + r) * a.acc_stride + q] = acc127;
-  a.acc[(143 * R + r) * a.acc_stride + q] = acc143;
-  a.acc[(159 * R + r) * a.acc_stride + q] = acc159;
-  a.acc[(175 * R + r) * a.acc_stride + q] = acc175;
-  a.acc[(191 * R + r) * a.acc_stride + q] = acc191;
-  a.acc[(207 * R + r) * a.acc_stride + q] = acc207;
-  a.acc[(223 * R + r) * a.acc_stride + q] = acc223;
-  a.acc[(239 * R + r) * a.acc_stride + q] = acc239;
-  a.acc[(255 * R + r) * a.acc_stride + q] = acc255;
-}
-// class (21|21): horizontal recursion per ket column, 256 accumulators
-__global__ __launch_bounds__(kRegThreads) void eri_hrr_2121(gbx_tables t, RegArgs a) {
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  ColumnStore<6, 3, 6, 3> st(t, a, bra, ket, pb, pk);
-  {
-    const ColumnSlot cl = st.column(0, 0);
-    const double v0 = sum_parts(a, 102, q);
-    const double v1 = sum_parts(a, 6, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 96, q);
-    const double v4 = sum_parts(a, 0, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 103, q);
-    const double v8 = sum_parts(a, 7, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 104, q);
-    const double v12 = sum_parts(a, 8, q);
-    const double v13 = v11 + CD[0] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 97, q);
-    const double v16 = sum_parts(a, 1, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 105, q);
-    const double v20 = sum_parts(a, 9, q);
-    const double v21 = v19 + CD[0] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 106, q);
-    const double v24 = sum_parts(a, 10, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 98, q);
-    const double v28 = sum_parts(a, 2, q);
-    const double v29 = v27 + CD[0] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 107, q);
-    const double v33 = sum_parts(a, 11, q);
-    const double v34 = v32 + CD[0] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 99, q);
-    const double v37 = sum_parts(a, 3, q);
-    const double v38 = v36 + CD[0] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 108, q);
-    const double v41 = sum_parts(a, 12, q);
-    const double v42 = v40 + CD[0] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 109, q);
-    const double v45 = sum_parts(a, 13, q);
-    const double v46 = v44 + CD[0] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 100, q);
-    const double v49 = sum_parts(a, 4, q);
-    const double v50 = v48 + CD[0] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 110, q);
-    const double v54 = sum_parts(a, 14, q);
-    const double v55 = v53 + CD[0] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 101, q);
-    const double v58 = sum_parts(a, 5, q);
-    const double v59 = v57 + CD[0] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 111, q);
-    const double v63 = sum_parts(a, 15, q);
-    const double v64 = v62 + CD[0] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(0, 1);
-    const double v0 = sum_parts(a, 118, q);
-    const double v1 = sum_parts(a, 6, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 112, q);
-    const double v4 = sum_parts(a, 0, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 119, q);
-    const double v8 = sum_parts(a, 7, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 120, q);
-    const double v12 = sum_parts(a, 8, q);
-    const double v13 = v11 + CD[1] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 113, q);
-    const double v16 = sum_parts(a, 1, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 121, q);
-    const double v20 = sum_parts(a, 9, q);
-    const double v21 = v19 + CD[1] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 122, q);
-    const double v24 = sum_parts(a, 10, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 114, q);
-    const double v28 = sum_parts(a, 2, q);
-    const double v29 = v27 + CD[1] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 123, q);
-    const double v33 = sum_parts(a, 11, q);
-    const double v34 = v32 + CD[1] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 115, q);
-    const double v37 = sum_parts(a, 3, q);
-    const double v38 = v36 + CD[1] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 124, q);
-    const double v41 = sum_parts(a, 12, q);
-    const double v42 = v40 + CD[1] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 125, q);
-    const double v45 = sum_parts(a, 13, q);
-    const double v46 = v44 + CD[1] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 116, q);
-    const double v49 = sum_parts(a, 4, q);
-    const double v50 = v48 + CD[1] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 126, q);
-    const double v54 = sum_parts(a, 14, q);
-    const double v55 = v53 + CD[1] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 117, q);
-    const double v58 = sum_parts(a, 5, q);
-    const double v59 = v57 + CD[1] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 127, q);
-    const double v63 = sum_parts(a, 15, q);
-    const double v64 = v62 + CD[1] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(0, 2);
-    const double v0 = sum_parts(a, 134, q);
-    const double v1 = sum_parts(a, 6, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 128, q);
-    const double v4 = sum_parts(a, 0, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 135, q);
-    const double v8 = sum_parts(a, 7, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 136, q);
-    const double v12 = sum_parts(a, 8, q);
-    const double v13 = v11 + CD[2] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 129, q);
-    const double v16 = sum_parts(a, 1, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 137, q);
-    const double v20 = sum_parts(a, 9, q);
-    const double v21 = v19 + CD[2] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 138, q);
-    const double v24 = sum_parts(a, 10, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 130, q);
-    const double v28 = sum_parts(a, 2, q);
-    const double v29 = v27 + CD[2] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 139, q);
-    const double v33 = sum_parts(a, 11, q);
-    const double v34 = v32 + CD[2] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 131, q);
-    const double v37 = sum_parts(a, 3, q);
-    const double v38 = v36 + CD[2] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 140, q);
-    const double v41 = sum_parts(a, 12, q);
-    const double v42 = v40 + CD[2] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 141, q);
-    const double v45 = sum_parts(a, 13, q);
-    const double v46 = v44 + CD[2] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 132, q);
-    const double v49 = sum_parts(a, 4, q);
-    const double v50 = v48 + CD[2] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 142, q);
-    const double v54 = sum_parts(a, 14, q);
-    const double v55 = v53 + CD[2] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 133, q);
-    const double v58 = sum_parts(a, 5, q);
-    const double v59 = v57 + CD[2] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 143, q);
-    const double v63 = sum_parts(a, 15, q);
-    const double v64 = v62 + CD[2] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 0);
-    const double v0 = sum_parts(a, 118, q);
-    const double v1 = sum_parts(a, 22, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 112, q);
-    const double v4 = sum_parts(a, 16, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 119, q);
-    const double v8 = sum_parts(a, 23, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 120, q);
-    const double v12 = sum_parts(a, 24, q);
-    const double v13 = v11 + CD[0] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 113, q);
-    const double v16 = sum_parts(a, 17, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 121, q);
-    const double v20 = sum_parts(a, 25, q);
-    const double v21 = v19 + CD[0] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 122, q);
-    const double v24 = sum_parts(a, 26, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 114, q);
-    const double v28 = sum_parts(a, 18, q);
-    const double v29 = v27 + CD[0] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 123, q);
-    const double v33 = sum_parts(a, 27, q);
-    const double v34 = v32 + CD[0] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 115, q);
-    const double v37 = sum_parts(a, 19, q);
-    const double v38 = v36 + CD[0] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 124, q);
-    const double v41 = sum_parts(a, 28, q);
-    const double v42 = v40 + CD[0] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 125, q);
-    const double v45 = sum_parts(a, 29, q);
-    const double v46 = v44 + CD[0] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 116, q);
-    const double v49 = sum_parts(a, 20, q);
-    const double v50 = v48 + CD[0] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 126, q);
-    const double v54 = sum_parts(a, 30, q);
-    const double v55 = v53 + CD[0] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 117, q);
-    const double v58 = sum_parts(a, 21, q);
-    const double v59 = v57 + CD[0] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 127, q);
-    const double v63 = sum_parts(a, 31, q);
-    const double v64 = v62 + CD[0] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 1);
-    const double v0 = sum_parts(a, 150, q);
-    const double v1 = sum_parts(a, 22, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 144, q);
-    const double v4 = sum_parts(a, 16, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 151, q);
-    const double v8 = sum_parts(a, 23, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 152, q);
-    const double v12 = sum_parts(a, 24, q);
-    const double v13 = v11 + CD[1] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 145, q);
-    const double v16 = sum_parts(a, 17, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 153, q);
-    const double v20 = sum_parts(a, 25, q);
-    const double v21 = v19 + CD[1] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 154, q);
-    const double v24 = sum_parts(a, 26, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 146, q);
-    const double v28 = sum_parts(a, 18, q);
-    const double v29 = v27 + CD[1] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 155, q);
-    const double v33 = sum_parts(a, 27, q);
-    const double v34 = v32 + CD[1] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 147, q);
-    const double v37 = sum_parts(a, 19, q);
-    const double v38 = v36 + CD[1] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 156, q);
-    const double v41 = sum_parts(a, 28, q);
-    const double v42 = v40 + CD[1] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 157, q);
-    const double v45 = sum_parts(a, 29, q);
-    const double v46 = v44 + CD[1] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 148, q);
-    const double v49 = sum_parts(a, 20, q);
-    const double v50 = v48 + CD[1] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 158, q);
-    const double v54 = sum_parts(a, 30, q);
-    const double v55 = v53 + CD[1] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 149, q);
-    const double v58 = sum_parts(a, 21, q);
-    const double v59 = v57 + CD[1] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 159, q);
-    const double v63 = sum_parts(a, 31, q);
-    const double v64 = v62 + CD[1] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 2);
-    const double v0 = sum_parts(a, 166, q);
-    const double v1 = sum_parts(a, 22, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 160, q);
-    const double v4 = sum_parts(a, 16, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 167, q);
-    const double v8 = sum_parts(a, 23, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 168, q);
-    const double v12 = sum_parts(a, 24, q);
-    const double v13 = v11 + CD[2] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 161, q);
-    const double v16 = sum_parts(a, 17, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 169, q);
-    const double v20 = sum_parts(a, 25, q);
-    const double v21 = v19 + CD[2] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 170, q);
-    const double v24 = sum_parts(a, 26, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 162, q);
-    const double v28 = sum_parts(a, 18, q);
-    const double v29 = v27 + CD[2] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 171, q);
-    const double v33 = sum_parts(a, 27, q);
-    const double v34 = v32 + CD[2] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 163, q);
-    const double v37 = sum_parts(a, 19, q);
-    const double v38 = v36 + CD[2] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 172, q);
-    const double v41 = sum_parts(a, 28, q);
-    const double v42 = v40 + CD[2] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 173, q);
-    const double v45 = sum_parts(a, 29, q);
-    const double v46 = v44 + CD[2] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 164, q);
-    const double v49 = sum_parts(a, 20, q);
-    const double v50 = v48 + CD[2] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 174, q);
-    const double v54 = sum_parts(a, 30, q);
-    const double v55 = v53 + CD[2] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 165, q);
-    const double v58 = sum_parts(a, 21, q);
-    const double v59 = v57 + CD[2] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 175, q);
-    const double v63 = sum_parts(a, 31, q);
-    const double v64 = v62 + CD[2] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 0);
-    const double v0 = sum_parts(a, 134, q);
-    const double v1 = sum_parts(a, 38, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 128, q);
-    const double v4 = sum_parts(a, 32, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 135, q);
-    const double v8 = sum_parts(a, 39, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 136, q);
-    const double v12 = sum_parts(a, 40, q);
-    const double v13 = v11 + CD[0] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 129, q);
-    const double v16 = sum_parts(a, 33, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 137, q);
-    const double v20 = sum_parts(a, 41, q);
-    const double v21 = v19 + CD[0] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 138, q);
-    const double v24 = sum_parts(a, 42, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 130, q);
-    const double v28 = sum_parts(a, 34, q);
-    const double v29 = v27 + CD[0] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 139, q);
-    const double v33 = sum_parts(a, 43, q);
-    const double v34 = v32 + CD[0] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 131, q);
-    const double v37 = sum_parts(a, 35, q);
-    const double v38 = v36 + CD[0] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 140, q);
-    const double v41 = sum_parts(a, 44, q);
-    const double v42 = v40 + CD[0] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 141, q);
-    const double v45 = sum_parts(a, 45, q);
-    const double v46 = v44 + CD[0] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 132, q);
-    const double v49 = sum_parts(a, 36, q);
-    const double v50 = v48 + CD[0] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 142, q);
-    const double v54 = sum_parts(a, 46, q);
-    const double v55 = v53 + CD[0] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 133, q);
-    const double v58 = sum_parts(a, 37, q);
-    const double v59 = v57 + CD[0] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 143, q);
-    const double v63 = sum_parts(a, 47, q);
-    const double v64 = v62 + CD[0] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 1);
-    const double v0 = sum_parts(a, 166, q);
-    const double v1 = sum_parts(a, 38, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 160, q);
-    const double v4 = sum_parts(a, 32, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 167, q);
-    const double v8 = sum_parts(a, 39, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 168, q);
-    const double v12 = sum_parts(a, 40, q);
-    const double v13 = v11 + CD[1] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 161, q);
-    const double v16 = sum_parts(a, 33, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 169, q);
-    const double v20 = sum_parts(a, 41, q);
-    const double v21 = v19 + CD[1] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 170, q);
-    const double v24 = sum_parts(a, 42, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 162, q);
-    const double v28 = sum_parts(a, 34, q);
-    const double v29 = v27 + CD[1] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 171, q);
-    const double v33 = sum_parts(a, 43, q);
-    const double v34 = v32 + CD[1] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 163, q);
-    const double v37 = sum_parts(a, 35, q);
-    const double v38 = v36 + CD[1] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 172, q);
-    const double v41 = sum_parts(a, 44, q);
-    const double v42 = v40 + CD[1] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 173, q);
-    const double v45 = sum_parts(a, 45, q);
-    const double v46 = v44 + CD[1] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 164, q);
-    const double v49 = sum_parts(a, 36, q);
-    const double v50 = v48 + CD[1] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 174, q);
-    const double v54 = sum_parts(a, 46, q);
-    const double v55 = v53 + CD[1] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 165, q);
-    const double v58 = sum_parts(a, 37, q);
-    const double v59 = v57 + CD[1] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 175, q);
-    const double v63 = sum_parts(a, 47, q);
-    const double v64 = v62 + CD[1] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 2);
-    const double v0 = sum_parts(a, 182, q);
-    const double v1 = sum_parts(a, 38, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 176, q);
-    const double v4 = sum_parts(a, 32, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 183, q);
-    const double v8 = sum_parts(a, 39, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 184, q);
-    const double v12 = sum_parts(a, 40, q);
-    const double v13 = v11 + CD[2] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 177, q);
-    const double v16 = sum_parts(a, 33, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 185, q);
-    const double v20 = sum_parts(a, 41, q);
-    const double v21 = v19 + CD[2] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 186, q);
-    const double v24 = sum_parts(a, 42, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 178, q);
-    const double v28 = sum_parts(a, 34, q);
-    const double v29 = v27 + CD[2] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 187, q);
-    const double v33 = sum_parts(a, 43, q);
-    const double v34 = v32 + CD[2] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 179, q);
-    const double v37 = sum_parts(a, 35, q);
-    const double v38 = v36 + CD[2] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 188, q);
-    const double v41 = sum_parts(a, 44, q);
-    const double v42 = v40 + CD[2] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 189, q);
-    const double v45 = sum_parts(a, 45, q);
-    const double v46 = v44 + CD[2] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 180, q);
-    const double v49 = sum_parts(a, 36, q);
-    const double v50 = v48 + CD[2] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 190, q);
-    const double v54 = sum_parts(a, 46, q);
-    const double v55 = v53 + CD[2] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 181, q);
-    const double v58 = sum_parts(a, 37, q);
-    const double v59 = v57 + CD[2] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 191, q);
-    const double v63 = sum_parts(a, 47, q);
-    const double v64 = v62 + CD[2] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(3, 0);
-    const double v0 = sum_parts(a, 150, q);
-    const double v1 = sum_parts(a, 54, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 144, q);
-    const double v4 = sum_parts(a, 48, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 151, q);
-    const double v8 = sum_parts(a, 55, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 152, q);
-    const double v12 = sum_parts(a, 56, q);
-    const double v13 = v11 + CD[0] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 145, q);
-    const double v16 = sum_parts(a, 49, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 153, q);
-    const double v20 = sum_parts(a, 57, q);
-    const double v21 = v19 + CD[0] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 154, q);
-    const double v24 = sum_parts(a, 58, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 146, q);
-    const double v28 = sum_parts(a, 50, q);
-    const double v29 = v27 + CD[0] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 155, q);
-    const double v33 = sum_parts(a, 59, q);
-    const double v34 = v32 + CD[0] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 147, q);
-    const double v37 = sum_parts(a, 51, q);
-    const double v38 = v36 + CD[0] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 156, q);
-    const double v41 = sum_parts(a, 60, q);
-    const double v42 = v40 + CD[0] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 157, q);
-    const double v45 = sum_parts(a, 61, q);
-    const double v46 = v44 + CD[0] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 148, q);
-    const double v49 = sum_parts(a, 52, q);
-    const double v50 = v48 + CD[0] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 158, q);
-    const double v54 = sum_parts(a, 62, q);
-    const double v55 = v53 + CD[0] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 149, q);
-    const double v58 = sum_parts(a, 53, q);
-    const double v59 = v57 + CD[0] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 159, q);
-    const double v63 = sum_parts(a, 63, q);
-    const double v64 = v62 + CD[0] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(3, 1);
-    const double v0 = sum_parts(a, 198, q);
-    const double v1 = sum_parts(a, 54, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 192, q);
-    const double v4 = sum_parts(a, 48, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 199, q);
-    const double v8 = sum_parts(a, 55, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 200, q);
-    const double v12 = sum_parts(a, 56, q);
-    const double v13 = v11 + CD[1] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 193, q);
-    const double v16 = sum_parts(a, 49, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 201, q);
-    const double v20 = sum_parts(a, 57, q);
-    const double v21 = v19 + CD[1] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 202, q);
-    const double v24 = sum_parts(a, 58, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 194, q);
-    const double v28 = sum_parts(a, 50, q);
-    const double v29 = v27 + CD[1] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 203, q);
-    const double v33 = sum_parts(a, 59, q);
-    const double v34 = v32 + CD[1] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 195, q);
-    const double v37 = sum_parts(a, 51, q);
-    const double v38 = v36 + CD[1] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 204, q);
-    const double v41 = sum_parts(a, 60, q);
-    const double v42 = v40 + CD[1] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 205, q);
-    const double v45 = sum_parts(a, 61, q);
-    const double v46 = v44 + CD[1] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 196, q);
-    const double v49 = sum_parts(a, 52, q);
-    const double v50 = v48 + CD[1] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 206, q);
-    const double v54 = sum_parts(a, 62, q);
-    const double v55 = v53 + CD[1] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 197, q);
-    const double v58 = sum_parts(a, 53, q);
-    const double v59 = v57 + CD[1] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 207, q);
-    const double v63 = sum_parts(a, 63, q);
-    const double v64 = v62 + CD[1] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(3, 2);
-    const double v0 = sum_parts(a, 214, q);
-    const double v1 = sum_parts(a, 54, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 208, q);
-    const double v4 = sum_parts(a, 48, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 215, q);
-    const double v8 = sum_parts(a, 55, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 216, q);
-    const double v12 = sum_parts(a, 56, q);
-    const double v13 = v11 + CD[2] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 209, q);
-    const double v16 = sum_parts(a, 49, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 217, q);
-    const double v20 = sum_parts(a, 57, q);
-    const double v21 = v19 + CD[2] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 218, q);
-    const double v24 = sum_parts(a, 58, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 210, q);
-    const double v28 = sum_parts(a, 50, q);
-    const double v29 = v27 + CD[2] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 219, q);
-    const double v33 = sum_parts(a, 59, q);
-    const double v34 = v32 + CD[2] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 211, q);
-    const double v37 = sum_parts(a, 51, q);
-    const double v38 = v36 + CD[2] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 220, q);
-    const double v41 = sum_parts(a, 60, q);
-    const double v42 = v40 + CD[2] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 221, q);
-    const double v45 = sum_parts(a, 61, q);
-    const double v46 = v44 + CD[2] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 212, q);
-    const double v49 = sum_parts(a, 52, q);
-    const double v50 = v48 + CD[2] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 222, q);
-    const double v54 = sum_parts(a, 62, q);
-    const double v55 = v53 + CD[2] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 213, q);
-    const double v58 = sum_parts(a, 53, q);
-    const double v59 = v57 + CD[2] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 223, q);
-    const double v63 = sum_parts(a, 63, q);
-    const double v64 = v62 + CD[2] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(4, 0);
-    const double v0 = sum_parts(a, 166, q);
-    const double v1 = sum_parts(a, 70, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 160, q);
-    const double v4 = sum_parts(a, 64, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 167, q);
-    const double v8 = sum_parts(a, 71, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 168, q);
-    const double v12 = sum_parts(a, 72, q);
-    const double v13 = v11 + CD[0] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 161, q);
-    const double v16 = sum_parts(a, 65, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 169, q);
-    const double v20 = sum_parts(a, 73, q);
-    const double v21 = v19 + CD[0] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 170, q);
-    const double v24 = sum_parts(a, 74, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 162, q);
-    const double v28 = sum_parts(a, 66, q);
-    const double v29 = v27 + CD[0] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 171, q);
-    const double v33 = sum_parts(a, 75, q);
-    const double v34 = v32 + CD[0] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 163, q);
-    const double v37 = sum_parts(a, 67, q);
-    const double v38 = v36 + CD[0] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 172, q);
-    const double v41 = sum_parts(a, 76, q);
-    const double v42 = v40 + CD[0] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 173, q);
-    const double v45 = sum_parts(a, 77, q);
-    const double v46 = v44 + CD[0] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 164, q);
-    const double v49 = sum_parts(a, 68, q);
-    const double v50 = v48 + CD[0] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 174, q);
-    const double v54 = sum_parts(a, 78, q);
-    const double v55 = v53 + CD[0] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 165, q);
-    const double v58 = sum_parts(a, 69, q);
-    const double v59 = v57 + CD[0] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 175, q);
-    const double v63 = sum_parts(a, 79, q);
-    const double v64 = v62 + CD[0] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(4, 1);
-    const double v0 = sum_parts(a, 214, q);
-    const double v1 = sum_parts(a, 70, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 208, q);
-    const double v4 = sum_parts(a, 64, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 215, q);
-    const double v8 = sum_parts(a, 71, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 216, q);
-    const double v12 = sum_parts(a, 72, q);
-    const double v13 = v11 + CD[1] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 209, q);
-    const double v16 = sum_parts(a, 65, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 217, q);
-    const double v20 = sum_parts(a, 73, q);
-    const double v21 = v19 + CD[1] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 218, q);
-    const double v24 = sum_parts(a, 74, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 210, q);
-    const double v28 = sum_parts(a, 66, q);
-    const double v29 = v27 + CD[1] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 219, q);
-    const double v33 = sum_parts(a, 75, q);
-    const double v34 = v32 + CD[1] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 211, q);
-    const double v37 = sum_parts(a, 67, q);
-    const double v38 = v36 + CD[1] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 220, q);
-    const double v41 = sum_parts(a, 76, q);
-    const double v42 = v40 + CD[1] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 221, q);
-    const double v45 = sum_parts(a, 77, q);
-    const double v46 = v44 + CD[1] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 212, q);
-    const double v49 = sum_parts(a, 68, q);
-    const double v50 = v48 + CD[1] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 222, q);
-    const double v54 = sum_parts(a, 78, q);
-    const double v55 = v53 + CD[1] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 213, q);
-    const double v58 = sum_parts(a, 69, q);
-    const double v59 = v57 + CD[1] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 223, q);
-    const double v63 = sum_parts(a, 79, q);
-    const double v64 = v62 + CD[1] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(4, 2);
-    const double v0 = sum_parts(a, 230, q);
-    const double v1 = sum_parts(a, 70, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 224, q);
-    const double v4 = sum_parts(a, 64, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.5773502691896258);
-    const double v7 = sum_parts(a, 231, q);
-    const double v8 = sum_parts(a, 71, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.5773502691896258);
-    const double v11 = sum_parts(a, 232, q);
-    const double v12 = sum_parts(a, 72, q);
-    const double v13 = v11 + CD[2] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.5773502691896258);
-    const double v15 = sum_parts(a, 225, q);
-    const double v16 = sum_parts(a, 65, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18);
-    const double v19 = sum_parts(a, 233, q);
-    const double v20 = sum_parts(a, 73, q);
-    const double v21 = v19 + CD[2] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22);
-    const double v23 = sum_parts(a, 234, q);
-    const double v24 = sum_parts(a, 74, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26);
-    const double v27 = sum_parts(a, 226, q);
-    const double v28 = sum_parts(a, 66, q);
-    const double v29 = v27 + CD[2] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31);
-    const double v32 = sum_parts(a, 235, q);
-    const double v33 = sum_parts(a, 75, q);
-    const double v34 = v32 + CD[2] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35);
-    const double v36 = sum_parts(a, 227, q);
-    const double v37 = sum_parts(a, 67, q);
-    const double v38 = v36 + CD[2] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.5773502691896258);
-    const double v40 = sum_parts(a, 236, q);
-    const double v41 = sum_parts(a, 76, q);
-    const double v42 = v40 + CD[2] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.5773502691896258);
-    const double v44 = sum_parts(a, 237, q);
-    const double v45 = sum_parts(a, 77, q);
-    const double v46 = v44 + CD[2] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.5773502691896258);
-    const double v48 = sum_parts(a, 228, q);
-    const double v49 = sum_parts(a, 68, q);
-    const double v50 = v48 + CD[2] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52);
-    const double v53 = sum_parts(a, 238, q);
-    const double v54 = sum_parts(a, 78, q);
-    const double v55 = v53 + CD[2] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56);
-    const double v57 = sum_parts(a, 229, q);
-    const double v58 = sum_parts(a, 69, q);
-    const double v59 = v57 + CD[2] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.5773502691896258);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.5773502691896258);
-    const double v62 = sum_parts(a, 239, q);
-    const double v63 = sum_parts(a, 79, q);
-    const double v64 = v62 + CD[2] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.5773502691896258);
-  }
-  {
-    const ColumnSlot cl = st.column(5, 0);
-    const double v0 = sum_parts(a, 182, q);
-    const double v1 = sum_parts(a, 86, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 176, q);
-    const double v4 = sum_parts(a, 80, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 183, q);
-    const double v8 = sum_parts(a, 87, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 184, q);
-    const double v12 = sum_parts(a, 88, q);
-    const double v13 = v11 + CD[0] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 177, q);
-    const double v16 = sum_parts(a, 81, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 185, q);
-    const double v20 = sum_parts(a, 89, q);
-    const double v21 = v19 + CD[0] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 186, q);
-    const double v24 = sum_parts(a, 90, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 178, q);
-    const double v28 = sum_parts(a, 82, q);
-    const double v29 = v27 + CD[0] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 187, q);
-    const double v33 = sum_parts(a, 91, q);
-    const double v34 = v32 + CD[0] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 179, q);
-    const double v37 = sum_parts(a, 83, q);
-    const double v38 = v36 + CD[0] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 188, q);
-    const double v41 = sum_parts(a, 92, q);
-    const double v42 = v40 + CD[0] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 189, q);
-    const double v45 = sum_parts(a, 93, q);
-    const double v46 = v44 + CD[0] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 180, q);
-    const double v49 = sum_parts(a, 84, q);
-    const double v50 = v48 + CD[0] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 190, q);
-    const double v54 = sum_parts(a, 94, q);
-    const double v55 = v53 + CD[0] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 181, q);
-    const double v58 = sum_parts(a, 85, q);
-    const double v59 = v57 + CD[0] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 191, q);
-    const double v63 = sum_parts(a, 95, q);
-    const double v64 = v62 + CD[0] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(5, 1);
-    const double v0 = sum_parts(a, 230, q);
-    const double v1 = sum_parts(a, 86, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 224, q);
-    const double v4 = sum_parts(a, 80, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 231, q);
-    const double v8 = sum_parts(a, 87, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 232, q);
-    const double v12 = sum_parts(a, 88, q);
-    const double v13 = v11 + CD[1] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 225, q);
-    const double v16 = sum_parts(a, 81, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 233, q);
-    const double v20 = sum_parts(a, 89, q);
-    const double v21 = v19 + CD[1] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 234, q);
-    const double v24 = sum_parts(a, 90, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 226, q);
-    const double v28 = sum_parts(a, 82, q);
-    const double v29 = v27 + CD[1] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 235, q);
-    const double v33 = sum_parts(a, 91, q);
-    const double v34 = v32 + CD[1] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 227, q);
-    const double v37 = sum_parts(a, 83, q);
-    const double v38 = v36 + CD[1] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 236, q);
-    const double v41 = sum_parts(a, 92, q);
-    const double v42 = v40 + CD[1] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 237, q);
-    const double v45 = sum_parts(a, 93, q);
-    const double v46 = v44 + CD[1] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 228, q);
-    const double v49 = sum_parts(a, 84, q);
-    const double v50 = v48 + CD[1] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 238, q);
-    const double v54 = sum_parts(a, 94, q);
-    const double v55 = v53 + CD[1] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 229, q);
-    const double v58 = sum_parts(a, 85, q);
-    const double v59 = v57 + CD[1] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 239, q);
-    const double v63 = sum_parts(a, 95, q);
-    const double v64 = v62 + CD[1] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(5, 2);
-    const double v0 = sum_parts(a, 246, q);
-    const double v1 = sum_parts(a, 86, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 240, q);
-    const double v4 = sum_parts(a, 80, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    st.put(cl, 0, 0, v6 * 0.3333333333333334);
-    const double v7 = sum_parts(a, 247, q);
-    const double v8 = sum_parts(a, 87, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v9 + AB[1] * v5;
-    st.put(cl, 0, 1, v10 * 0.3333333333333334);
-    const double v11 = sum_parts(a, 248, q);
-    const double v12 = sum_parts(a, 88, q);
-    const double v13 = v11 + CD[2] * v12;
-    const double v14 = v13 + AB[2] * v5;
-    st.put(cl, 0, 2, v14 * 0.3333333333333334);
-    const double v15 = sum_parts(a, 241, q);
-    const double v16 = sum_parts(a, 81, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v9 + AB[0] * v17;
-    st.put(cl, 1, 0, v18 * 0.5773502691896258);
-    const double v19 = sum_parts(a, 249, q);
-    const double v20 = sum_parts(a, 89, q);
-    const double v21 = v19 + CD[2] * v20;
-    const double v22 = v21 + AB[1] * v17;
-    st.put(cl, 1, 1, v22 * 0.5773502691896258);
-    const double v23 = sum_parts(a, 250, q);
-    const double v24 = sum_parts(a, 90, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v25 + AB[2] * v17;
-    st.put(cl, 1, 2, v26 * 0.5773502691896258);
-    const double v27 = sum_parts(a, 242, q);
-    const double v28 = sum_parts(a, 82, q);
-    const double v29 = v27 + CD[2] * v28;
-    const double v30 = v13 + AB[0] * v29;
-    st.put(cl, 2, 0, v30 * 0.5773502691896258);
-    const double v31 = v25 + AB[1] * v29;
-    st.put(cl, 2, 1, v31 * 0.5773502691896258);
-    const double v32 = sum_parts(a, 251, q);
-    const double v33 = sum_parts(a, 91, q);
-    const double v34 = v32 + CD[2] * v33;
-    const double v35 = v34 + AB[2] * v29;
-    st.put(cl, 2, 2, v35 * 0.5773502691896258);
-    const double v36 = sum_parts(a, 243, q);
-    const double v37 = sum_parts(a, 83, q);
-    const double v38 = v36 + CD[2] * v37;
-    const double v39 = v21 + AB[0] * v38;
-    st.put(cl, 3, 0, v39 * 0.3333333333333334);
-    const double v40 = sum_parts(a, 252, q);
-    const double v41 = sum_parts(a, 92, q);
-    const double v42 = v40 + CD[2] * v41;
-    const double v43 = v42 + AB[1] * v38;
-    st.put(cl, 3, 1, v43 * 0.3333333333333334);
-    const double v44 = sum_parts(a, 253, q);
-    const double v45 = sum_parts(a, 93, q);
-    const double v46 = v44 + CD[2] * v45;
-    const double v47 = v46 + AB[2] * v38;
-    st.put(cl, 3, 2, v47 * 0.3333333333333334);
-    const double v48 = sum_parts(a, 244, q);
-    const double v49 = sum_parts(a, 84, q);
-    const double v50 = v48 + CD[2] * v49;
-    const double v51 = v25 + AB[0] * v50;
-    st.put(cl, 4, 0, v51 * 0.5773502691896258);
-    const double v52 = v46 + AB[1] * v50;
-    st.put(cl, 4, 1, v52 * 0.5773502691896258);
-    const double v53 = sum_parts(a, 254, q);
-    const double v54 = sum_parts(a, 94, q);
-    const double v55 = v53 + CD[2] * v54;
-    const double v56 = v55 + AB[2] * v50;
-    st.put(cl, 4, 2, v56 * 0.5773502691896258);
-    const double v57 = sum_parts(a, 245, q);
-    const double v58 = sum_parts(a, 85, q);
-    const double v59 = v57 + CD[2] * v58;
-    const double v60 = v34 + AB[0] * v59;
-    st.put(cl, 5, 0, v60 * 0.3333333333333334);
-    const double v61 = v55 + AB[1] * v59;
-    st.put(cl, 5, 1, v61 * 0.3333333333333334);
-    const double v62 = sum_parts(a, 255, q);
-    const double v63 = sum_parts(a, 95, q);
-    const double v64 = v62 + CD[2] * v63;
-    const double v65 = v64 + AB[2] * v59;
-    st.put(cl, 5, 2, v65 * 0.3333333333333334);
-  }
-}
-
-// class (22|00): 65 recursion values, 31 accumulators, 36 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2200(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<4>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[5];
-      boys_eval_lds<4>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB x5 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x6 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x7 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x8 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x9 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x10 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x11 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x12 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x13 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x14 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x15 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x16 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x34 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x17 = s.PA[0] * x5 + s.WP[0] * x8 + z1 * (x0 - s.roz * x1);
-      acc0 += static_cast<double>(x17);
-      const TB x18 = s.PA[1] * x5 + s.WP[1] * x8;
-      acc1 += static_cast<double>(x18);
-      const TB x19 = s.PA[2] * x5 + s.WP[2] * x8;
-      acc2 += static_cast<double>(x19);
-      const TB x20 = s.PA[1] * x6 + s.WP[1] * x9 + z1 * (x0 - s.roz * x1);
-      acc3 += static_cast<double>(x20);
-      const TB x22 = s.PA[2] * x7 + s.WP[2] * x10 + z1 * (x0 - s.roz * x1);
-      acc5 += static_cast<double>(x22);
-      const TB x21 = s.PA[2] * x6 + s.WP[2] * x9;
-      acc4 += static_cast<double>(x21);
-      const TB x23 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x1 - s.roz * x2);
-      const TB x24 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x25 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x26 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x28 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x54 = s.PA[2] * x28 + s.WP[2] * x34 + z2 * (x10 - s.roz * x13);
-      const TB x44 = s.PA[2] * x22 + s.WP[2] * x28 + z2 * (x7 - s.roz * x10);
-      acc15 += static_cast<double>(x44);
-      const TB x69 = s.PA[2] * x44 + s.WP[2] * x54 + z3 * (x22 - s.roz * x28);
-      acc30 += static_cast<double>(x69);
-      const TB x27 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x29 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x2 - s.roz * x3);
-      const TB x32 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x33 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x53 = s.PA[2] * x27 + s.WP[2] * x33 + z1 * (x9 - s.roz * x12);
-      const TB x51 = s.PA[1] * x26 + s.WP[1] * x32 + z2 * (x9 - s.roz * x12);
-      const TB x52 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x30 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x31 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x50 = s.PA[2] * x25 + s.WP[2] * x31 + z1 * (x8 - s.roz * x11);
-      const TB x35 = s.PA[0] * x17 + s.WP[0] * x23 + z2 * (x5 - s.roz * x8);
-      acc6 += static_cast<double>(x35);
-      const TB x36 = s.PA[1] * x17 + s.WP[1] * x23;
-      acc7 += static_cast<double>(x36);
-      const TB x37 = s.PA[2] * x17 + s.WP[2] * x23;
-      acc8 += static_cast<double>(x37);
-      const TB x38 = s.PA[1] * x18 + s.WP[1] * x24 + z1 * (x5 - s.roz * x8);
-      acc9 += static_cast<double>(x38);
-      const TB x40 = s.PA[2] * x19 + s.WP[2] * x25 + z1 * (x5 - s.roz * x8);
-      acc11 += static_cast<double>(x40);
-      const TB x64 = s.PA[2] * x40 + s.WP[2] * x50 + z2 * (x19 - s.roz * x25);
-      acc25 += static_cast<double>(x64);
-      const TB x39 = s.PA[2] * x18 + s.WP[2] * x24;
-      acc10 += static_cast<double>(x39);
-      const TB x41 = s.PA[1] * x20 + s.WP[1] * x26 + z2 * (x6 - s.roz * x9);
-      acc12 += static_cast<double>(x41);
-      const TB x43 = s.PA[2] * x21 + s.WP[2] * x27 + z1 * (x6 - s.roz * x9);
-      acc14 += static_cast<double>(x43);
-      const TB x68 = s.PA[2] * x43 + s.WP[2] * x53 + z2 * (x21 - s.roz * x27);
-      acc29 += static_cast<double>(x68);
-      const TB x65 = s.PA[1] * x41 + s.WP[1] * x51 + z3 * (x20 - s.roz * x26);
-      acc26 += static_cast<double>(x65);
-      const TB x66 = s.PA[2] * x41 + s.WP[2] * x51;
-      acc27 += static_cast<double>(x66);
-      const TB x42 = s.PA[2] * x20 + s.WP[2] * x26;
-      acc13 += static_cast<double>(x42);
-      const TB x67 = s.PA[2] * x42 + s.WP[2] * x52 + z1 * (x20 - s.roz * x26);
-      acc28 += static_cast<double>(x67);
-      const TB x45 = s.PA[0] * x23 + s.WP[0] * x29 + z2 * (x8 - s.roz * x11);
-      const TB x48 = s.PA[1] * x24 + s.WP[1] * x30 + z1 * (x8 - s.roz * x11);
-      const TB x49 = s.PA[2] * x24 + s.WP[2] * x30;
-      const TB x63 = s.PA[2] * x39 + s.WP[2] * x49 + z1 * (x18 - s.roz * x24);
-      acc24 += static_cast<double>(x63);
-      const TB x61 = s.PA[1] * x38 + s.WP[1] * x48 + z2 * (x18 - s.roz * x24);
-      acc22 += static_cast<double>(x61);
-      const TB x62 = s.PA[2] * x38 + s.WP[2] * x48;
-      acc23 += static_cast<double>(x62);
-      const TB x55 = s.PA[0] * x35 + s.WP[0] * x45 + z3 * (x17 - s.roz * x23);
-      acc16 += static_cast<double>(x55);
-      const TB x56 = s.PA[1] * x35 + s.WP[1] * x45;
-      acc17 += static_cast<double>(x56);
-      const TB x57 = s.PA[2] * x35 + s.WP[2] * x45;
-      acc18 += static_cast<double>(x57);
-      const TB x46 = s.PA[1] * x23 + s.WP[1] * x29;
-      const TB x47 = s.PA[2] * x23 + s.WP[2] * x29;
-      const TB x60 = s.PA[2] * x37 + s.WP[2] * x47 + z1 * (x17 - s.roz * x23);
-      acc21 += static_cast<double>(x60);
-      const TB x58 = s.PA[1] * x36 + s.WP[1] * x46 + z1 * (x17 - s.roz * x23);
-      acc19 += static_cast<double>(x58);
-      const TB x59 = s.PA[2] * x36 + s.WP[2] * x46;
-      acc20 += static_cast<double>(x59);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = h15 + AB[2] * h5;
-  const double h32 = h14 + AB[1] * h5;
-  const double h33 = h11 + AB[0] * h5;
-  const double h34 = h30 + AB[2] * h15;
-  const double h35 = h29 + AB[1] * h15;
-  const double h36 = h25 + AB[0] * h15;
-  const double h37 = h14 + AB[2] * h4;
-  const double h38 = h13 + AB[1] * h4;
-  const double h39 = h10 + AB[0] * h4;
-  const double h40 = h29 + AB[2] * h14;
-  const double h41 = h28 + AB[1] * h14;
-  const double h42 = h24 + AB[0] * h14;
-  const double h43 = h13 + AB[2] * h3;
-  const double h44 = h12 + AB[1] * h3;
-  const double h45 = h9 + AB[0] * h3;
-  const double h46 = h28 + AB[2] * h13;
-  const double h47 = h27 + AB[1] * h13;
-  const double h48 = h23 + AB[0] * h13;
-  const double h49 = h26 + AB[1] * h12;
-  const double h50 = h22 + AB[0] * h12;
-  const double h51 = h11 + AB[2] * h2;
-  const double h52 = h10 + AB[1] * h2;
-  const double h53 = h8 + AB[0] * h2;
-  const double h54 = h25 + AB[2] * h11;
-  const double h55 = h24 + AB[1] * h11;
-  const double h56 = h21 + AB[0] * h11;
-  const double h57 = h10 + AB[2] * h1;
-  const double h58 = h9 + AB[1] * h1;
-  const double h59 = h7 + AB[0] * h1;
-  const double h60 = h24 + AB[2] * h10;
-  const double h61 = h23 + AB[1] * h10;
-  const double h62 = h20 + AB[0] * h10;
-  const double h63 = h22 + AB[1] * h9;
-  const double h64 = h19 + AB[0] * h9;
-  const double h65 = h8 + AB[2] * h0;
-  const double h66 = h7 + AB[1] * h0;
-  const double h67 = h6 + AB[0] * h0;
-  const double h68 = h21 + AB[2] * h8;
-  const double h69 = h20 + AB[1] * h8;
-  const double h70 = h18 + AB[0] * h8;
-  const double h71 = h19 + AB[1] * h7;
-  const double h72 = h17 + AB[0] * h7;
-  const double h73 = h16 + AB[0] * h6;
-  const double h74 = h34 + AB[2] * h31;
-  const double h75 = h35 + AB[2] * h32;
-  const double h76 = h41 + AB[1] * h32;
-  const double h77 = h36 + AB[2] * h33;
-  const double h78 = h42 + AB[1] * h33;
-  const double h79 = h56 + AB[0] * h33;
-  const double h80 = h40 + AB[2] * h37;
-  const double h81 = h41 + AB[2] * h38;
-  const double h82 = h47 + AB[1] * h38;
-  const double h83 = h42 + AB[2] * h39;
-  const double h84 = h48 + AB[1] * h39;
-  const double h85 = h62 + AB[0] * h39;
-  const double h86 = h46 + AB[2] * h43;
-  const double h87 = h47 + AB[2] * h44;
-  const double h88 = h49 + AB[1] * h44;
-  const double h89 = h48 + AB[2] * h45;
-  const double h90 = h50 + AB[1] * h45;
-  const double h91 = h64 + AB[0] * h45;
-  const double h92 = h54 + AB[2] * h51;
-  const double h93 = h55 + AB[2] * h52;
-  const double h94 = h61 + AB[1] * h52;
-  const double h95 = h56 + AB[2] * h53;
-  const double h96 = h62 + AB[1] * h53;
-  const double h97 = h70 + AB[0] * h53;
-  const double h98 = h60 + AB[2] * h57;
-  const double h99 = h61 + AB[2] * h58;
-  const double h100 = h63 + AB[1] * h58;
-  const double h101 = h62 + AB[2] * h59;
-  const double h102 = h64 + AB[1] * h59;
-  const double h103 = h72 + AB[0] * h59;
-  const double h104 = h68 + AB[2] * h65;
-  const double h105 = h69 + AB[2] * h66;
-  const double h106 = h71 + AB[1] * h66;
-  const double h107 = h70 + AB[2] * h67;
-  const double h108 = h72 + AB[1] * h67;
-  const double h109 = h73 + AB[0] * h67;
-  double out[36];
-  out[0] = h109 * 0.3333333333333334;
-  out[1] = h108 * 0.5773502691896258;
-  out[2] = h107 * 0.5773502691896258;
-  out[3] = h106 * 0.3333333333333334;
-  out[4] = h105 * 0.5773502691896258;
-  out[5] = h104 * 0.3333333333333334;
-  out[6] = h103 * 0.5773502691896258;
-  out[7] = h102;
-  out[8] = h101;
-  out[9] = h100 * 0.5773502691896258;
-  out[10] = h99;
-  out[11] = h98 * 0.5773502691896258;
-  out[12] = h97 * 0.5773502691896258;
-  out[13] = h96;
-  out[14] = h95;
-  out[15] = h94 * 0.5773502691896258;
-  out[16] = h93;
-  out[17] = h92 * 0.5773502691896258;
-  out[18] = h91 * 0.3333333333333334;
-  out[19] = h90 * 0.5773502691896258;
-  out[20] = h89 * 0.5773502691896258;
-  out[21] = h88 * 0.3333333333333334;
-  out[22] = h87 * 0.5773502691896258;
-  out[23] = h86 * 0.3333333333333334;
-  out[24] = h85 * 0.5773502691896258;
-  out[25] = h84;
-  out[26] = h83;
-  out[27] = h82 * 0.5773502691896258;
-  out[28] = h81;
-  out[29] = h80 * 0.5773502691896258;
-  out[30] = h79 * 0.3333333333333334;
-  out[31] = h78 * 0.5773502691896258;
-  out[32] = h77 * 0.5773502691896258;
-  out[33] = h76 * 0.3333333333333334;
-  out[34] = h75 * 0.5773502691896258;
-  out[35] = h74 * 0.3333333333333334;
-  store_block<6, 6, 1, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (22|10): 192 recursion values, 93 accumulators, 108 outputs
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_reg_2210(gbx_tables t, RegArgs a) {
-  __shared__ __align__(16) double boys_lds[kBoysRows * kBoysLdsStride];
-  boys_stage_lds<5>(t, boys_lds);
-  __syncthreads();
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc1 = 0.0;
-  double acc2 = 0.0;
-  double acc3 = 0.0;
-  double acc4 = 0.0;
-  double acc5 = 0.0;
-  double acc6 = 0.0;
-  double acc7 = 0.0;
-  double acc8 = 0.0;
-  double acc9 = 0.0;
-  double acc10 = 0.0;
-  double acc11 = 0.0;
-  double acc12 = 0.0;
-  double acc13 = 0.0;
-  double acc14 = 0.0;
-  double acc15 = 0.0;
-  double acc16 = 0.0;
-  double acc17 = 0.0;
-  double acc18 = 0.0;
-  double acc19 = 0.0;
-  double acc20 = 0.0;
-  double acc21 = 0.0;
-  double acc22 = 0.0;
-  double acc23 = 0.0;
-  double acc24 = 0.0;
-  double acc25 = 0.0;
-  double acc26 = 0.0;
-  double acc27 = 0.0;
-  double acc28 = 0.0;
-  double acc29 = 0.0;
-  double acc30 = 0.0;
-  double acc31 = 0.0;
-  double acc32 = 0.0;
-  double acc33 = 0.0;
-  double acc34 = 0.0;
-  double acc35 = 0.0;
-  double acc36 = 0.0;
-  double acc37 = 0.0;
-  double acc38 = 0.0;
-  double acc39 = 0.0;
-  double acc40 = 0.0;
-  double acc41 = 0.0;
-  double acc42 = 0.0;
-  double acc43 = 0.0;
-  double acc44 = 0.0;
-  double acc45 = 0.0;
-  double acc46 = 0.0;
-  double acc47 = 0.0;
-  double acc48 = 0.0;
-  double acc49 = 0.0;
-  double acc50 = 0.0;
-  double acc51 = 0.0;
-  double acc52 = 0.0;
-  double acc53 = 0.0;
-  double acc54 = 0.0;
-  double acc55 = 0.0;
-  double acc56 = 0.0;
-  double acc57 = 0.0;
-  double acc58 = 0.0;
-  double acc59 = 0.0;
-  double acc60 = 0.0;
-  double acc61 = 0.0;
-  double acc62 = 0.0;
-  double acc63 = 0.0;
-  double acc64 = 0.0;
-  double acc65 = 0.0;
-  double acc66 = 0.0;
-  double acc67 = 0.0;
-  double acc68 = 0.0;
-  double acc69 = 0.0;
-  double acc70 = 0.0;
-  double acc71 = 0.0;
-  double acc72 = 0.0;
-  double acc73 = 0.0;
-  double acc74 = 0.0;
-  double acc75 = 0.0;
-  double acc76 = 0.0;
-  double acc77 = 0.0;
-  double acc78 = 0.0;
-  double acc79 = 0.0;
-  double acc80 = 0.0;
-  double acc81 = 0.0;
-  double acc82 = 0.0;
-  double acc83 = 0.0;
-  double acc84 = 0.0;
-  double acc85 = 0.0;
-  double acc86 = 0.0;
-  double acc87 = 0.0;
-  double acc88 = 0.0;
-  double acc89 = 0.0;
-  double acc90 = 0.0;
-  double acc91 = 0.0;
-  double acc92 = 0.0;
-  for (int ik = 0; ik < pk.w; ++ik) {
-    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    for (int ib = 0; ib < pb.w; ++ib) {
-      const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[6];
-      boys_eval_lds<5>(T, t, boys_lds, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x6 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x7 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x8 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x9 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x10 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x11 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x12 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x13 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x14 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x15 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x16 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x17 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x18 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x19 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x20 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x44 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x21 = s.PA[0] * x6 + s.WP[0] * x9 + z1 * (x0 - s.roz * x1);
-      const TB x22 = s.PA[1] * x6 + s.WP[1] * x9;
-      const TB x23 = s.PA[2] * x6 + s.WP[2] * x9;
-      const TB x24 = s.PA[1] * x7 + s.WP[1] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x26 = s.PA[2] * x8 + s.WP[2] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x25 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x27 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x1 - s.roz * x2);
-      const TB x105 = s.QC[0] * x21 + s.WQ[0] * x27 + r2 * x9;
-      acc0 += static_cast<double>(x105);
-      const TB x136 = s.QC[1] * x21 + s.WQ[1] * x27;
-      acc31 += static_cast<double>(x136);
-      const TB x167 = s.QC[2] * x21 + s.WQ[2] * x27;
-      acc62 += static_cast<double>(x167);
-      const TB x28 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x106 = s.QC[0] * x22 + s.WQ[0] * x28 + r1 * x10;
-      acc1 += static_cast<double>(x106);
-      const TB x137 = s.QC[1] * x22 + s.WQ[1] * x28 + r1 * x9;
-      acc32 += static_cast<double>(x137);
-      const TB x168 = s.QC[2] * x22 + s.WQ[2] * x28;
-      acc63 += static_cast<double>(x168);
-      const TB x29 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x107 = s.QC[0] * x23 + s.WQ[0] * x29 + r1 * x11;
-      acc2 += static_cast<double>(x107);
-      const TB x138 = s.QC[1] * x23 + s.WQ[1] * x29;
-      acc33 += static_cast<double>(x138);
-      const TB x169 = s.QC[2] * x23 + s.WQ[2] * x29 + r1 * x9;
-      acc64 += static_cast<double>(x169);
-      const TB x30 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x54 = s.PA[2] * x26 + s.WP[2] * x32 + z2 * (x8 - s.roz * x11);
-      const TB x108 = s.QC[0] * x24 + s.WQ[0] * x30;
-      acc3 += static_cast<double>(x108);
-      const TB x110 = s.QC[0] * x26 + s.WQ[0] * x32;
-      acc5 += static_cast<double>(x110);
-      const TB x139 = s.QC[1] * x24 + s.WQ[1] * x30 + r2 * x10;
-      acc34 += static_cast<double>(x139);
-      const TB x141 = s.QC[1] * x26 + s.WQ[1] * x32;
-      acc36 += static_cast<double>(x141);
-      const TB x170 = s.QC[2] * x24 + s.WQ[2] * x30;
-      acc65 += static_cast<double>(x170);
-      const TB x172 = s.QC[2] * x26 + s.WQ[2] * x32 + r2 * x11;
-      acc67 += static_cast<double>(x172);
-      const TB x31 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x109 = s.QC[0] * x25 + s.WQ[0] * x31;
-      acc4 += static_cast<double>(x109);
-      const TB x140 = s.QC[1] * x25 + s.WQ[1] * x31 + r1 * x11;
-      acc35 += static_cast<double>(x140);
-      const TB x171 = s.QC[2] * x25 + s.WQ[2] * x31 + r1 * x10;
-      acc66 += static_cast<double>(x171);
-      const TB x33 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x2 - s.roz * x3);
-      const TB x34 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x35 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x36 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x38 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x74 = s.PA[2] * x38 + s.WP[2] * x44 + z2 * (x14 - s.roz * x17);
-      const TB x64 = s.PA[2] * x32 + s.WP[2] * x38 + z2 * (x11 - s.roz * x14);
-      const TB x104 = s.PA[2] * x64 + s.WP[2] * x74 + z3 * (x32 - s.roz * x38);
-      const TB x89 = s.PA[2] * x54 + s.WP[2] * x64 + z3 * (x26 - s.roz * x32);
-      const TB x120 = s.QC[0] * x54 + s.WQ[0] * x64;
-      acc15 += static_cast<double>(x120);
-      const TB x135 = s.QC[0] * x89 + s.WQ[0] * x104;
-      acc30 += static_cast<double>(x135);
-      const TB x151 = s.QC[1] * x54 + s.WQ[1] * x64;
-      acc46 += static_cast<double>(x151);
-      const TB x182 = s.QC[2] * x54 + s.WQ[2] * x64 + r3 * x32;
-      acc77 += static_cast<double>(x182);
-      const TB x166 = s.QC[1] * x89 + s.WQ[1] * x104;
-      acc61 += static_cast<double>(x166);
-      const TB x197 = s.QC[2] * x89 + s.WQ[2] * x104 + r4 * x64;
-      acc92 += static_cast<double>(x197);
-      const TB x37 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x39 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x3 - s.roz * x4);
-      const TB x42 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x43 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x73 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x13 - s.roz * x16);
-      const TB x71 = s.PA[1] * x36 + s.WP[1] * x42 + z2 * (x13 - s.roz * x16);
-      const TB x72 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x40 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x41 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x70 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x12 - s.roz * x15);
-      const TB x45 = s.PA[0] * x21 + s.WP[0] * x27 + z2 * (x6 - s.roz * x9);
-      const TB x46 = s.PA[1] * x21 + s.WP[1] * x27;
-      const TB x47 = s.PA[2] * x21 + s.WP[2] * x27;
-      const TB x48 = s.PA[1] * x22 + s.WP[1] * x28 + z1 * (x6 - s.roz * x9);
-      const TB x50 = s.PA[2] * x23 + s.WP[2] * x29 + z1 * (x6 - s.roz * x9);
-      const TB x49 = s.PA[2] * x22 + s.WP[2] * x28;
-      const TB x51 = s.PA[1] * x24 + s.WP[1] * x30 + z2 * (x7 - s.roz * x10);
-      const TB x53 = s.PA[2] * x25 + s.WP[2] * x31 + z1 * (x7 - s.roz * x10);
-      const TB x52 = s.PA[2] * x24 + s.WP[2] * x30;
-      const TB x55 = s.PA[0] * x27 + s.WP[0] * x33 + z2 * (x9 - s.roz * x12);
-      const TB x111 = s.QC[0] * x45 + s.WQ[0] * x55 + r3 * x27;
-      acc6 += static_cast<double>(x111);
-      const TB x142 = s.QC[1] * x45 + s.WQ[1] * x55;
-      acc37 += static_cast<double>(x142);
-      const TB x173 = s.QC[2] * x45 + s.WQ[2] * x55;
-      acc68 += static_cast<double>(x173);
-      const TB x56 = s.PA[1] * x27 + s.WP[1] * x33;
-      const TB x112 = s.QC[0] * x46 + s.WQ[0] * x56 + r2 * x28;
-      acc7 += static_cast<double>(x112);
-      const TB x143 = s.QC[1] * x46 + s.WQ[1] * x56 + r1 * x27;
-      acc38 += static_cast<double>(x143);
-      const TB x174 = s.QC[2] * x46 + s.WQ[2] * x56;
-      acc69 += static_cast<double>(x174);
-      const TB x57 = s.PA[2] * x27 + s.WP[2] * x33;
-      const TB x113 = s.QC[0] * x47 + s.WQ[0] * x57 + r2 * x29;
-      acc8 += static_cast<double>(x113);
-      const TB x144 = s.QC[1] * x47 + s.WQ[1] * x57;
-      acc39 += static_cast<double>(x144);
-      const TB x175 = s.QC[2] * x47 + s.WQ[2] * x57 + r1 * x27;
-      acc70 += static_cast<double>(x175);
-      const TB x80 = s.PA[2] * x47 + s.WP[2] * x57 + z1 * (x21 - s.roz * x27);
-      const TB x58 = s.PA[1] * x28 + s.WP[1] * x34 + z1 * (x9 - s.roz * x12);
-      const TB x60 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x9 - s.roz * x12);
-      const TB x99 = s.PA[2] * x60 + s.WP[2] * x70 + z2 * (x29 - s.roz * x35);
-      const TB x84 = s.PA[2] * x50 + s.WP[2] * x60 + z2 * (x23 - s.roz * x29);
-      const TB x114 = s.QC[0] * x48 + s.WQ[0] * x58 + r1 * x30;
-      acc9 += static_cast<double>(x114);
-      const TB x116 = s.QC[0] * x50 + s.WQ[0] * x60 + r1 * x32;
-      acc11 += static_cast<double>(x116);
-      const TB x130 = s.QC[0] * x84 + s.WQ[0] * x99 + r1 * x64;
-      acc25 += static_cast<double>(x130);
-      const TB x145 = s.QC[1] * x48 + s.WQ[1] * x58 + r2 * x28;
-      acc40 += static_cast<double>(x145);
-      const TB x147 = s.QC[1] * x50 + s.WQ[1] * x60;
-      acc42 += static_cast<double>(x147);
-      const TB x178 = s.QC[2] * x50 + s.WQ[2] * x60 + r2 * x29;
-      acc73 += static_cast<double>(x178);
-      const TB x161 = s.QC[1] * x84 + s.WQ[1] * x99;
-      acc56 += static_cast<double>(x161);
-      const TB x192 = s.QC[2] * x84 + s.WQ[2] * x99 + r3 * x60;
-      acc87 += static_cast<double>(x192);
-      const TB x176 = s.QC[2] * x48 + s.WQ[2] * x58;
-      acc71 += static_cast<double>(x176);
-      const TB x59 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x146 = s.QC[1] * x49 + s.WQ[1] * x59 + r1 * x29;
-      acc41 += static_cast<double>(x146);
-      const TB x115 = s.QC[0] * x49 + s.WQ[0] * x59 + r1 * x31;
-      acc10 += static_cast<double>(x115);
-      const TB x177 = s.QC[2] * x49 + s.WQ[2] * x59 + r1 * x28;
-      acc72 += static_cast<double>(x177);
-      const TB x83 = s.PA[2] * x49 + s.WP[2] * x59 + z1 * (x22 - s.roz * x28);
-      const TB x81 = s.PA[1] * x48 + s.WP[1] * x58 + z2 * (x22 - s.roz * x28);
-      const TB x82 = s.PA[2] * x48 + s.WP[2] * x58;
-      const TB x61 = s.PA[1] * x30 + s.WP[1] * x36 + z2 * (x10 - s.roz * x13);
-      const TB x63 = s.PA[2] * x31 + s.WP[2] * x37 + z1 * (x10 - s.roz * x13);
-      const TB x103 = s.PA[2] * x63 + s.WP[2] * x73 + z2 * (x31 - s.roz * x37);
-      const TB x150 = s.QC[1] * x53 + s.WQ[1] * x63 + r1 * x32;
-      acc45 += static_cast<double>(x150);
-      const TB x88 = s.PA[2] * x53 + s.WP[2] * x63 + z2 * (x25 - s.roz * x31);
-      const TB x165 = s.QC[1] * x88 + s.WQ[1] * x103 + r1 * x64;
-      acc60 += static_cast<double>(x165);
-      const TB x117 = s.QC[0] * x51 + s.WQ[0] * x61;
-      acc12 += static_cast<double>(x117);
-      const TB x119 = s.QC[0] * x53 + s.WQ[0] * x63;
-      acc14 += static_cast<double>(x119);
-      const TB x181 = s.QC[2] * x53 + s.WQ[2] * x63 + r2 * x31;
-      acc76 += static_cast<double>(x181);
-      const TB x134 = s.QC[0] * x88 + s.WQ[0] * x103;
-      acc29 += static_cast<double>(x134);
-      const TB x196 = s.QC[2] * x88 + s.WQ[2] * x103 + r3 * x63;
-      acc91 += static_cast<double>(x196);
-      const TB x148 = s.QC[1] * x51 + s.WQ[1] * x61 + r3 * x30;
-      acc43 += static_cast<double>(x148);
-      const TB x179 = s.QC[2] * x51 + s.WQ[2] * x61;
-      acc74 += static_cast<double>(x179);
-      const TB x62 = s.PA[2] * x30 + s.WP[2] * x36;
-      const TB x149 = s.QC[1] * x52 + s.WQ[1] * x62 + r2 * x31;
-      acc44 += static_cast<double>(x149);
-      const TB x102 = s.PA[2] * x62 + s.WP[2] * x72 + z1 * (x30 - s.roz * x36);
-      const TB x100 = s.PA[1] * x61 + s.WP[1] * x71 + z3 * (x30 - s.roz * x36);
-      const TB x101 = s.PA[2] * x61 + s.WP[2] * x71;
-      const TB x118 = s.QC[0] * x52 + s.WQ[0] * x62;
-      acc13 += static_cast<double>(x118);
-      const TB x180 = s.QC[2] * x52 + s.WQ[2] * x62 + r1 * x30;
-      acc75 += static_cast<double>(x180);
-      const TB x87 = s.PA[2] * x52 + s.WP[2] * x62 + z1 * (x24 - s.roz * x30);
-      const TB x85 = s.PA[1] * x51 + s.WP[1] * x61 + z3 * (x24 - s.roz * x30);
-      const TB x86 = s.PA[2] * x51 + s.WP[2] * x61;
-      const TB x131 = s.QC[0] * x85 + s.WQ[0] * x100;
-      acc26 += static_cast<double>(x131);
-      const TB x132 = s.QC[0] * x86 + s.WQ[0] * x101;
-      acc27 += static_cast<double>(x132);
-      const TB x133 = s.QC[0] * x87 + s.WQ[0] * x102;
-      acc28 += static_cast<double>(x133);
-      const TB x162 = s.QC[1] * x85 + s.WQ[1] * x100 + r4 * x61;
-      acc57 += static_cast<double>(x162);
-      const TB x193 = s.QC[2] * x85 + s.WQ[2] * x100;
-      acc88 += static_cast<double>(x193);
-      const TB x163 = s.QC[1] * x86 + s.WQ[1] * x101 + r3 * x62;
-      acc58 += static_cast<double>(x163);
-      const TB x194 = s.QC[2] * x86 + s.WQ[2] * x101 + r1 * x61;
-      acc89 += static_cast<double>(x194);
-      const TB x164 = s.QC[1] * x87 + s.WQ[1] * x102 + r2 * x63;
-      acc59 += static_cast<double>(x164);
-      const TB x195 = s.QC[2] * x87 + s.WQ[2] * x102 + r2 * x62;
-      acc90 += static_cast<double>(x195);
-      const TB x65 = s.PA[0] * x33 + s.WP[0] * x39 + z2 * (x12 - s.roz * x15);
-      const TB x68 = s.PA[1] * x34 + s.WP[1] * x40 + z1 * (x12 - s.roz * x15);
-      const TB x69 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x98 = s.PA[2] * x59 + s.WP[2] * x69 + z1 * (x28 - s.roz * x34);
-      const TB x96 = s.PA[1] * x58 + s.WP[1] * x68 + z2 * (x28 - s.roz * x34);
-      const TB x127 = s.QC[0] * x81 + s.WQ[0] * x96 + r1 * x61;
-      acc22 += static_cast<double>(x127);
-      const TB x129 = s.QC[0] * x83 + s.WQ[0] * x98 + r1 * x63;
-      acc24 += static_cast<double>(x129);
-      const TB x97 = s.PA[2] * x58 + s.WP[2] * x68;
-      const TB x128 = s.QC[0] * x82 + s.WQ[0] * x97 + r1 * x62;
-      acc23 += static_cast<double>(x128);
-      const TB x158 = s.QC[1] * x81 + s.WQ[1] * x96 + r3 * x58;
-      acc53 += static_cast<double>(x158);
-      const TB x189 = s.QC[2] * x81 + s.WQ[2] * x96;
-      acc84 += static_cast<double>(x189);
-      const TB x159 = s.QC[1] * x82 + s.WQ[1] * x97 + r2 * x59;
-      acc54 += static_cast<double>(x159);
-      const TB x190 = s.QC[2] * x82 + s.WQ[2] * x97 + r1 * x58;
-      acc85 += static_cast<double>(x190);
-      const TB x160 = s.QC[1] * x83 + s.WQ[1] * x98 + r1 * x60;
-      acc55 += static_cast<double>(x160);
-      const TB x191 = s.QC[2] * x83 + s.WQ[2] * x98 + r2 * x59;
-      acc86 += static_cast<double>(x191);
-      const TB x66 = s.PA[1] * x33 + s.WP[1] * x39;
-      const TB x67 = s.PA[2] * x33 + s.WP[2] * x39;
-      const TB x95 = s.PA[2] * x57 + s.WP[2] * x67 + z1 * (x27 - s.roz * x33);
-      const TB x126 = s.QC[0] * x80 + s.WQ[0] * x95 + r2 * x60;
-      acc21 += static_cast<double>(x126);
-      const TB x157 = s.QC[1] * x80 + s.WQ[1] * x95;
-      acc52 += static_cast<double>(x157);
-      const TB x188 = s.QC[2] * x80 + s.WQ[2] * x95 + r2 * x57;
-      acc83 += static_cast<double>(x188);
-      const TB x75 = s.PA[0] * x45 + s.WP[0] * x55 + z3 * (x21 - s.roz * x27);
-      const TB x78 = s.PA[1] * x46 + s.WP[1] * x56 + z1 * (x21 - s.roz * x27);
-      const TB x79 = s.PA[2] * x46 + s.WP[2] * x56;
-      const TB x76 = s.PA[1] * x45 + s.WP[1] * x55;
-      const TB x77 = s.PA[2] * x45 + s.WP[2] * x55;
-      const TB x90 = s.PA[0] * x55 + s.WP[0] * x65 + z3 * (x27 - s.roz * x33);
-      const TB x93 = s.PA[1] * x56 + s.WP[1] * x66 + z1 * (x27 - s.roz * x33);
-      const TB x124 = s.QC[0] * x78 + s.WQ[0] * x93 + r2 * x58;
-      acc19 += static_cast<double>(x124);
-      const TB x94 = s.PA[2] * x56 + s.WP[2] * x66;
-      const TB x125 = s.QC[0] * x79 + s.WQ[0] * x94 + r2 * x59;
-      acc20 += static_cast<double>(x125);
-      const TB x121 = s.QC[0] * x75 + s.WQ[0] * x90 + r4 * x55;
-      acc16 += static_cast<double>(x121);
-      const TB x152 = s.QC[1] * x75 + s.WQ[1] * x90;
-      acc47 += static_cast<double>(x152);
-      const TB x183 = s.QC[2] * x75 + s.WQ[2] * x90;
-      acc78 += static_cast<double>(x183);
-      const TB x155 = s.QC[1] * x78 + s.WQ[1] * x93 + r2 * x56;
-      acc50 += static_cast<double>(x155);
-      const TB x186 = s.QC[2] * x78 + s.WQ[2] * x93;
-      acc81 += static_cast<double>(x186);
-      const TB x156 = s.QC[1] * x79 + s.WQ[1] * x94 + r1 * x57;
-      acc51 += static_cast<double>(x156);
-      const TB x187 = s.QC[2] * x79 + s.WQ[2] * x94 + r1 * x56;
-      acc82 += static_cast<double>(x187);
-      const TB x91 = s.PA[1] * x55 + s.WP[1] * x65;
-      const TB x122 = s.QC[0] * x76 + s.WQ[0] * x91 + r3 * x56;
-      acc17 += static_cast<double>(x122);
-      const TB x92 = s.PA[2] * x55 + s.WP[2] * x65;
-      const TB x123 = s.QC[0] * x77 + s.WQ[0] * x92 + r3 * x57;
-      acc18 += static_cast<double>(x123);
-      const TB x153 = s.QC[1] * x76 + s.WQ[1] * x91 + r1 * x55;
-      acc48 += static_cast<double>(x153);
-      const TB x184 = s.QC[2] * x76 + s.WQ[2] * x91;
-      acc79 += static_cast<double>(x184);
-      const TB x185 = s.QC[2] * x77 + s.WQ[2] * x92 + r1 * x55;
-      acc80 += static_cast<double>(x185);
-      const TB x154 = s.QC[1] * x77 + s.WQ[1] * x92;
-      acc49 += static_cast<double>(x154);
-    }
-  }
-  double accmax = 0.0;
-  if (a.mode == GBX_MODE_SCHWARZ) {
-    accmax = fmax(accmax, fabs(acc0));
-    accmax = fmax(accmax, fabs(acc1));
-    accmax = fmax(accmax, fabs(acc2));
-    accmax = fmax(accmax, fabs(acc3));
-    accmax = fmax(accmax, fabs(acc4));
-    accmax = fmax(accmax, fabs(acc5));
-    accmax = fmax(accmax, fabs(acc6));
-    accmax = fmax(accmax, fabs(acc7));
-    accmax = fmax(accmax, fabs(acc8));
-    accmax = fmax(accmax, fabs(acc9));
-    accmax = fmax(accmax, fabs(acc10));
-    accmax = fmax(accmax, fabs(acc11));
-    accmax = fmax(accmax, fabs(acc12));
-    accmax = fmax(accmax, fabs(acc13));
-    accmax = fmax(accmax, fabs(acc14));
-    accmax = fmax(accmax, fabs(acc15));
-    accmax = fmax(accmax, fabs(acc16));
-    accmax = fmax(accmax, fabs(acc17));
-    accmax = fmax(accmax, fabs(acc18));
-    accmax = fmax(accmax, fabs(acc19));
-    accmax = fmax(accmax, fabs(acc20));
-    accmax = fmax(accmax, fabs(acc21));
-    accmax = fmax(accmax, fabs(acc22));
-    accmax = fmax(accmax, fabs(acc23));
-    accmax = fmax(accmax, fabs(acc24));
-    accmax = fmax(accmax, fabs(acc25));
-    accmax = fmax(accmax, fabs(acc26));
-    accmax = fmax(accmax, fabs(acc27));
-    accmax = fmax(accmax, fabs(acc28));
-    accmax = fmax(accmax, fabs(acc29));
-    accmax = fmax(accmax, fabs(acc30));
-    accmax = fmax(accmax, fabs(acc31));
-    accmax = fmax(accmax, fabs(acc32));
-    accmax = fmax(accmax, fabs(acc33));
-    accmax = fmax(accmax, fabs(acc34));
-    accmax = fmax(accmax, fabs(acc35));
-    accmax = fmax(accmax, fabs(acc36));
-    accmax = fmax(accmax, fabs(acc37));
-    accmax = fmax(accmax, fabs(acc38));
-    accmax = fmax(accmax, fabs(acc39));
-    accmax = fmax(accmax, fabs(acc40));
-    accmax = fmax(accmax, fabs(acc41));
-    accmax = fmax(accmax, fabs(acc42));
-    accmax = fmax(accmax, fabs(acc43));
-    accmax = fmax(accmax, fabs(acc44));
-    accmax = fmax(accmax, fabs(acc45));
-    accmax = fmax(accmax, fabs(acc46));
-    accmax = fmax(accmax, fabs(acc47));
-    accmax = fmax(accmax, fabs(acc48));
-    accmax = fmax(accmax, fabs(acc49));
-    accmax = fmax(accmax, fabs(acc50));
-    accmax = fmax(accmax, fabs(acc51));
-    accmax = fmax(accmax, fabs(acc52));
-    accmax = fmax(accmax, fabs(acc53));
-    accmax = fmax(accmax, fabs(acc54));
-    accmax = fmax(accmax, fabs(acc55));
-    accmax = fmax(accmax, fabs(acc56));
-    accmax = fmax(accmax, fabs(acc57));
-    accmax = fmax(accmax, fabs(acc58));
-    accmax = fmax(accmax, fabs(acc59));
-    accmax = fmax(accmax, fabs(acc60));
-    accmax = fmax(accmax, fabs(acc61));
-    accmax = fmax(accmax, fabs(acc62));
-    accmax = fmax(accmax, fabs(acc63));
-    accmax = fmax(accmax, fabs(acc64));
-    accmax = fmax(accmax, fabs(acc65));
-    accmax = fmax(accmax, fabs(acc66));
-    accmax = fmax(accmax, fabs(acc67));
-    accmax = fmax(accmax, fabs(acc68));
-    accmax = fmax(accmax, fabs(acc69));
-    accmax = fmax(accmax, fabs(acc70));
-    accmax = fmax(accmax, fabs(acc71));
-    accmax = fmax(accmax, fabs(acc72));
-    accmax = fmax(accmax, fabs(acc73));
-    accmax = fmax(accmax, fabs(acc74));
-    accmax = fmax(accmax, fabs(acc75));
-    accmax = fmax(accmax, fabs(acc76));
-    accmax = fmax(accmax, fabs(acc77));
-    accmax = fmax(accmax, fabs(acc78));
-    accmax = fmax(accmax, fabs(acc79));
-    accmax = fmax(accmax, fabs(acc80));
-    accmax = fmax(accmax, fabs(acc81));
-    accmax = fmax(accmax, fabs(acc82));
-    accmax = fmax(accmax, fabs(acc83));
-    accmax = fmax(accmax, fabs(acc84));
-    accmax = fmax(accmax, fabs(acc85));
-    accmax = fmax(accmax, fabs(acc86));
-    accmax = fmax(accmax, fabs(acc87));
-    accmax = fmax(accmax, fabs(acc88));
-    accmax = fmax(accmax, fabs(acc89));
-    accmax = fmax(accmax, fabs(acc90));
-    accmax = fmax(accmax, fabs(acc91));
-    accmax = fmax(accmax, fabs(acc92));
-  }
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  const double h0 = acc0;
-  const double h1 = acc1;
-  const double h2 = acc2;
-  const double h3 = acc3;
-  const double h4 = acc4;
-  const double h5 = acc5;
-  const double h6 = acc6;
-  const double h7 = acc7;
-  const double h8 = acc8;
-  const double h9 = acc9;
-  const double h10 = acc10;
-  const double h11 = acc11;
-  const double h12 = acc12;
-  const double h13 = acc13;
-  const double h14 = acc14;
-  const double h15 = acc15;
-  const double h16 = acc16;
-  const double h17 = acc17;
-  const double h18 = acc18;
-  const double h19 = acc19;
-  const double h20 = acc20;
-  const double h21 = acc21;
-  const double h22 = acc22;
-  const double h23 = acc23;
-  const double h24 = acc24;
-  const double h25 = acc25;
-  const double h26 = acc26;
-  const double h27 = acc27;
-  const double h28 = acc28;
-  const double h29 = acc29;
-  const double h30 = acc30;
-  const double h31 = acc31;
-  const double h32 = acc32;
-  const double h33 = acc33;
-  const double h34 = acc34;
-  const double h35 = acc35;
-  const double h36 = acc36;
-  const double h37 = acc37;
-  const double h38 = acc38;
-  const double h39 = acc39;
-  const double h40 = acc40;
-  const double h41 = acc41;
-  const double h42 = acc42;
-  const double h43 = acc43;
-  const double h44 = acc44;
-  const double h45 = acc45;
-  const double h46 = acc46;
-  const double h47 = acc47;
-  const double h48 = acc48;
-  const double h49 = acc49;
-  const double h50 = acc50;
-  const double h51 = acc51;
-  const double h52 = acc52;
-  const double h53 = acc53;
-  const double h54 = acc54;
-  const double h55 = acc55;
-  const double h56 = acc56;
-  const double h57 = acc57;
-  const double h58 = acc58;
-  const double h59 = acc59;
-  const double h60 = acc60;
-  const double h61 = acc61;
-  const double h62 = acc62;
-  const double h63 = acc63;
-  const double h64 = acc64;
-  const double h65 = acc65;
-  const double h66 = acc66;
-  const double h67 = acc67;
-  const double h68 = acc68;
-  const double h69 = acc69;
-  const double h70 = acc70;
-  const double h71 = acc71;
-  const double h72 = acc72;
-  const double h73 = acc73;
-  const double h74 = acc74;
-  const double h75 = acc75;
-  const double h76 = acc76;
-  const double h77 = acc77;
-  const double h78 = acc78;
-  const double h79 = acc79;
-  const double h80 = acc80;
-  const double h81 = acc81;
-  const double h82 = acc82;
-  const double h83 = acc83;
-  const double h84 = acc84;
-  const double h85 = acc85;
-  const double h86 = acc86;
-  const double h87 = acc87;
-  const double h88 = acc88;
-  const double h89 = acc89;
-  const double h90 = acc90;
-  const double h91 = acc91;
-  const double h92 = acc92;
-  const double h93 = h77 + AB[2] * h67;
-  const double h94 = h46 + AB[2] * h36;
-  const double h95 = h15 + AB[2] * h5;
-  const double h96 = h76 + AB[1] * h67;
-  const double h97 = h45 + AB[1] * h36;
-  const double h98 = h14 + AB[1] * h5;
-  const double h99 = h73 + AB[0] * h67;
-  const double h100 = h42 + AB[0] * h36;
-  const double h101 = h11 + AB[0] * h5;
-  const double h102 = h92 + AB[2] * h77;
-  const double h103 = h61 + AB[2] * h46;
-  const double h104 = h30 + AB[2] * h15;
-  const double h105 = h91 + AB[1] * h77;
-  const double h106 = h60 + AB[1] * h46;
-  const double h107 = h29 + AB[1] * h15;
-  const double h108 = h87 + AB[0] * h77;
-  const double h109 = h56 + AB[0] * h46;
-  const double h110 = h25 + AB[0] * h15;
-  const double h111 = h76 + AB[2] * h66;
-  const double h112 = h45 + AB[2] * h35;
-  const double h113 = h14 + AB[2] * h4;
-  const double h114 = h75 + AB[1] * h66;
-  const double h115 = h44 + AB[1] * h35;
-  const double h116 = h13 + AB[1] * h4;
-  const double h117 = h72 + AB[0] * h66;
-  const double h118 = h41 + AB[0] * h35;
-  const double h119 = h10 + AB[0] * h4;
-  const double h120 = h91 + AB[2] * h76;
-  const double h121 = h60 + AB[2] * h45;
-  const double h122 = h29 + AB[2] * h14;
-  const double h123 = h90 + AB[1] * h76;
-  const double h124 = h59 + AB[1] * h45;
-  const double h125 = h28 + AB[1] * h14;
-  const double h126 = h86 + AB[0] * h76;
-  const double h127 = h55 + AB[0] * h45;
-  const double h128 = h24 + AB[0] * h14;
-  const double h129 = h75 + AB[2] * h65;
-  const double h130 = h44 + AB[2] * h34;
-  const double h131 = h13 + AB[2] * h3;
-  const double h132 = h74 + AB[1] * h65;
-  const double h133 = h43 + AB[1] * h34;
-  const double h134 = h12 + AB[1] * h3;
-  const double h135 = h71 + AB[0] * h65;
-  const double h136 = h40 + AB[0] * h34;
-  const double h137 = h9 + AB[0] * h3;
-  const double h138 = h90 + AB[2] * h75;
-  const double h139 = h59 + AB[2] * h44;
-  const double h140 = h28 + AB[2] * h13;
-  const double h141 = h89 + AB[1] * h75;
-  const double h142 = h58 + AB[1] * h44;
-  const double h143 = h27 + AB[1] * h13;
-  const double h144 = h85 + AB[0] * h75;
-  const double h145 = h54 + AB[0] * h44;
-  const double h146 = h23 + AB[0] * h13;
-  const double h147 = h88 + AB[1] * h74;
-  const double h148 = h57 + AB[1] * h43;
-  const double h149 = h26 + AB[1] * h12;
-  const double h150 = h84 + AB[0] * h74;
-  const double h151 = h53 + AB[0] * h43;
-  const double h152 = h22 + AB[0] * h12;
-  const double h153 = h73 + AB[2] * h64;
-  const double h154 = h42 + AB[2] * h33;
-  const double h155 = h11 + AB[2] * h2;
-  const double h156 = h72 + AB[1] * h64;
-  const double h157 = h41 + AB[1] * h33;
-  const double h158 = h10 + AB[1] * h2;
-  const double h159 = h70 + AB[0] * h64;
-  const double h160 = h39 + AB[0] * h33;
-  const double h161 = h8 + AB[0] * h2;
-  const double h162 = h87 + AB[2] * h73;
-  const double h163 = h56 + AB[2] * h42;
-  const double h164 = h25 + AB[2] * h11;
-  const double h165 = h86 + AB[1] * h73;
-  const double h166 = h55 + AB[1] * h42;
-  const double h167 = h24 + AB[1] * h11;
-  const double h168 = h83 + AB[0] * h73;
-  const double h169 = h52 + AB[0] * h42;
-  const double h170 = h21 + AB[0] * h11;
-  const double h171 = h72 + AB[2] * h63;
-  const double h172 = h41 + AB[2] * h32;
-  const double h173 = h10 + AB[2] * h1;
-  const double h174 = h71 + AB[1] * h63;
-  const double h175 = h40 + AB[1] * h32;
-  const double h176 = h9 + AB[1] * h1;
-  const double h177 = h69 + AB[0] * h63;
-  const double h178 = h38 + AB[0] * h32;
-  const double h179 = h7 + AB[0] * h1;
-  const double h180 = h86 + AB[2] * h72;
-  const double h181 = h55 + AB[2] * h41;
-  const double h182 = h24 + AB[2] * h10;
-  const double h183 = h85 + AB[1] * h72;
-  const double h184 = h54 + AB[1] * h41;
-  const double h185 = h23 + AB[1] * h10;
-  const double h186 = h82 + AB[0] * h72;
-  const double h187 = h51 + AB[0] * h41;
-  const double h188 = h20 + AB[0] * h10;
-  const double h189 = h84 + AB[1] * h71;
-  const double h190 = h53 + AB[1] * h40;
-  const double h191 = h22 + AB[1] * h9;
-  const double h192 = h81 + AB[0] * h71;
-  const double h193 = h50 + AB[0] * h40;
-  const double h194 = h19 + AB[0] * h9;
-  const double h195 = h70 + AB[2] * h62;
-  const double h196 = h39 + AB[2] * h31;
-  const double h197 = h8 + AB[2] * h0;
-  const double h198 = h69 + AB[1] * h62;
-  const double h199 = h38 + AB[1] * h31;
-  const double h200 = h7 + AB[1] * h0;
-  const double h201 = h68 + AB[0] * h62;
-  const double h202 = h37 + AB[0] * h31;
-  const double h203 = h6 + AB[0] * h0;
-  const double h204 = h83 + AB[2] * h70;
-  const double h205 = h52 + AB[2] * h39;
-  const double h206 = h21 + AB[2] * h8;
-  const double h207 = h82 + AB[1] * h70;
-  const double h208 = h51 + AB[1] * h39;
-  const double h209 = h20 + AB[1] * h8;
-  const double h210 = h80 + AB[0] * h70;
-  const double h211 = h49 + AB[0] * h39;
-  const double h212 = h18 + AB[0] * h8;
-  const double h213 = h81 + AB[1] * h69;
-  const double h214 = h50 + AB[1] * h38;
-  const double h215 = h19 + AB[1] * h7;
-  const double h216 = h79 + AB[0] * h69;
-  const double h217 = h48 + AB[0] * h38;
-  const double h218 = h17 + AB[0] * h7;
-  const double h219 = h78 + AB[0] * h68;
-  const double h220 = h47 + AB[0] * h37;
-  const double h221 = h16 + AB[0] * h6;
-  const double h222 = h102 + AB[2] * h93;
-  const double h223 = h103 + AB[2] * h94;
-  const double h224 = h104 + AB[2] * h95;
-  const double h225 = h105 + AB[2] * h96;
-  const double h226 = h106 + AB[2] * h97;
-  const double h227 = h107 + AB[2] * h98;
-  const double h228 = h123 + AB[1] * h96;
-  const double h229 = h124 + AB[1] * h97;
-  const double h230 = h125 + AB[1] * h98;
-  const double h231 = h108 + AB[2] * h99;
-  const double h232 = h109 + AB[2] * h100;
-  const double h233 = h110 + AB[2] * h101;
-  const double h234 = h126 + AB[1] * h99;
-  const double h235 = h127 + AB[1] * h100;
-  const double h236 = h128 + AB[1] * h101;
-  const double h237 = h168 + AB[0] * h99;
-  const double h238 = h169 + AB[0] * h100;
-  const double h239 = h170 + AB[0] * h101;
-  const double h240 = h120 + AB[2] * h111;
-  const double h241 = h121 + AB[2] * h112;
-  const double h242 = h122 + AB[2] * h113;
-  const double h243 = h123 + AB[2] * h114;
-  const double h244 = h124 + AB[2] * h115;
-  const double h245 = h125 + AB[2] * h116;
-  const double h246 = h141 + AB[1] * h114;
-  const double h247 = h142 + AB[1] * h115;
-  const double h248 = h143 + AB[1] * h116;
-  const double h249 = h126 + AB[2] * h117;
-  const double h250 = h127 + AB[2] * h118;
-  const double h251 = h128 + AB[2] * h119;
-  const double h252 = h144 + AB[1] * h117;
-  const double h253 = h145 + AB[1] * h118;
-  const double h254 = h146 + AB[1] * h119;
-  const double h255 = h186 + AB[0] * h117;
-  const double h256 = h187 + AB[0] * h118;
-  const double h257 = h188 + AB[0] * h119;
-  const double h258 = h138 + AB[2] * h129;
-  const double h259 = h139 + AB[2] * h130;
-  const double h260 = h140 + AB[2] * h131;
-  const double h261 = h141 + AB[2] * h132;
-  const double h262 = h142 + AB[2] * h133;
-  const double h263 = h143 + AB[2] * h134;
-  const double h264 = h147 + AB[1] * h132;
-  const double h265 = h148 + AB[1] * h133;
-  const double h266 = h149 + AB[1] * h134;
-  const double h267 = h144 + AB[2] * h135;
-  const double h268 = h145 + AB[2] * h136;
-  const double h269 = h146 + AB[2] * h137;
-  const double h270 = h150 + AB[1] * h135;
-  const double h271 = h151 + AB[1] * h136;
-  const double h272 = h152 + AB[1] * h137;
-  const double h273 = h192 + AB[0] * h135;
-  const double h274 = h193 + AB[0] * h136;
-  const double h275 = h194 + AB[0] * h137;
-  const double h276 = h162 + AB[2] * h153;
-  const double h277 = h163 + AB[2] * h154;
-  const double h278 = h164 + AB[2] * h155;
-  const double h279 = h165 + AB[2] * h156;
-  const double h280 = h166 + AB[2] * h157;
-  const double h281 = h167 + AB[2] * h158;
-  const double h282 = h183 + AB[1] * h156;
-  const double h283 = h184 + AB[1] * h157;
-  const double h284 = h185 + AB[1] * h158;
-  const double h285 = h168 + AB[2] * h159;
-  const double h286 = h169 + AB[2] * h160;
-  const double h287 = h170 + AB[2] * h161;
-  const double h288 = h186 + AB[1] * h159;
-  const double h289 = h187 + AB[1] * h160;
-  const double h290 = h188 + AB[1] * h161;
-  const double h291 = h210 + AB[0] * h159;
-  const double h292 = h211 + AB[0] * h160;
-  const double h293 = h212 + AB[0] * h161;
-  const double h294 = h180 + AB[2] * h171;
-  const double h295 = h181 + AB[2] * h172;
-  const double h296 = h182 + AB[2] * h173;
-  const double h297 = h183 + AB[2] * h174;
-  const double h298 = h184 + AB[2] * h175;
-  const double h299 = h185 + AB[2] * h176;
-  const double h300 = h189 + AB[1] * h174;
-  const double h301 = h190 + AB[1] * h175;
-  const double h302 = h191 + AB[1] * h176;
-  const double h303 = h186 + AB[2] * h177;
-  const double h304 = h187 + AB[2] * h178;
-  const double h305 = h188 + AB[2] * h179;
-  const double h306 = h192 + AB[1] * h177;
-  const double h307 = h193 + AB[1] * h178;
-  const double h308 = h194 + AB[1] * h179;
-  const double h309 = h216 + AB[0] * h177;
-  const double h310 = h217 + AB[0] * h178;
-  const double h311 = h218 + AB[0] * h179;
-  const double h312 = h204 + AB[2] * h195;
-  const double h313 = h205 + AB[2] * h196;
-  const double h314 = h206 + AB[2] * h197;
-  const double h315 = h207 + AB[2] * h198;
-  const double h316 = h208 + AB[2] * h199;
-  const double h317 = h209 + AB[2] * h200;
-  const double h318 = h213 + AB[1] * h198;
-  const double h319 = h214 + AB[1] * h199;
-  const double h320 = h215 + AB[1] * h200;
-  const double h321 = h210 + AB[2] * h201;
-  const double h322 = h211 + AB[2] * h202;
-  const double h323 = h212 + AB[2] * h203;
-  const double h324 = h216 + AB[1] * h201;
-  const double h325 = h217 + AB[1] * h202;
-  const double h326 = h218 + AB[1] * h203;
-  const double h327 = h219 + AB[0] * h201;
-  const double h328 = h220 + AB[0] * h202;
-  const double h329 = h221 + AB[0] * h203;
-  double out[108];
-  out[0] = h329 * 0.3333333333333334;
-  out[1] = h328 * 0.3333333333333334;
-  out[2] = h327 * 0.3333333333333334;
-  out[3] = h326 * 0.5773502691896258;
-  out[4] = h325 * 0.5773502691896258;
-  out[5] = h324 * 0.5773502691896258;
-  out[6] = h323 * 0.5773502691896258;
-  out[7] = h322 * 0.5773502691896258;
-  out[8] = h321 * 0.5773502691896258;
-  out[9] = h320 * 0.3333333333333334;
-  out[10] = h319 * 0.3333333333333334;
-  out[11] = h318 * 0.3333333333333334;
-  out[12] = h317 * 0.5773502691896258;
-  out[13] = h316 * 0.5773502691896258;
-  out[14] = h315 * 0.5773502691896258;
-  out[15] = h314 * 0.3333333333333334;
-  out[16] = h313 * 0.3333333333333334;
-  out[17] = h312 * 0.3333333333333334;
-  out[18] = h311 * 0.5773502691896258;
-  out[19] = h310 * 0.5773502691896258;
-  out[20] = h309 * 0.5773502691896258;
-  out[21] = h308;
-  out[22] = h307;
-  out[23] = h306;
-  out[24] = h305;
-  out[25] = h304;
-  out[26] = h303;
-  out[27] = h302 * 0.5773502691896258;
-  out[28] = h301 * 0.5773502691896258;
-  out[29] = h300 * 0.5773502691896258;
-  out[30] = h299;
-  out[31] = h298;
-  out[32] = h297;
-  out[33] = h296 * 0.5773502691896258;
-  out[34] = h295 * 0.5773502691896258;
-  out[35] = h294 * 0.5773502691896258;
-  out[36] = h293 * 0.5773502691896258;
-  out[37] = h292 * 0.5773502691896258;
-  out[38] = h291 * 0.5773502691896258;
-  out[39] = h290;
-  out[40] = h289;
-  out[41] = h288;
-  out[42] = h287;
-  out[43] = h286;
-  out[44] = h285;
-  out[45] = h284 * 0.5773502691896258;
-  out[46] = h283 * 0.5773502691896258;
-  out[47] = h282 * 0.5773502691896258;
-  out[48] = h281;
-  out[49] = h280;
-  out[50] = h279;
-  out[51] = h278 * 0.5773502691896258;
-  out[52] = h277 * 0.5773502691896258;
-  out[53] = h276 * 0.5773502691896258;
-  out[54] = h275 * 0.3333333333333334;
-  out[55] = h274 * 0.3333333333333334;
-  out[56] = h273 * 0.3333333333333334;
-  out[57] = h272 * 0.5773502691896258;
-  out[58] = h271 * 0.5773502691896258;
-  out[59] = h270 * 0.5773502691896258;
-  out[60] = h269 * 0.5773502691896258;
-  out[61] = h268 * 0.5773502691896258;
-  out[62] = h267 * 0.5773502691896258;
-  out[63] = h266 * 0.3333333333333334;
-  out[64] = h265 * 0.3333333333333334;
-  out[65] = h264 * 0.3333333333333334;
-  out[66] = h263 * 0.5773502691896258;
-  out[67] = h262 * 0.5773502691896258;
-  out[68] = h261 * 0.5773502691896258;
-  out[69] = h260 * 0.3333333333333334;
-  out[70] = h259 * 0.3333333333333334;
-  out[71] = h258 * 0.3333333333333334;
-  out[72] = h257 * 0.5773502691896258;
-  out[73] = h256 * 0.5773502691896258;
-  out[74] = h255 * 0.5773502691896258;
-  out[75] = h254;
-  out[76] = h253;
-  out[77] = h252;
-  out[78] = h251;
-  out[79] = h250;
-  out[80] = h249;
-  out[81] = h248 * 0.5773502691896258;
-  out[82] = h247 * 0.5773502691896258;
-  out[83] = h246 * 0.5773502691896258;
-  out[84] = h245;
-  out[85] = h244;
-  out[86] = h243;
-  out[87] = h242 * 0.5773502691896258;
-  out[88] = h241 * 0.5773502691896258;
-  out[89] = h240 * 0.5773502691896258;
-  out[90] = h239 * 0.3333333333333334;
-  out[91] = h238 * 0.3333333333333334;
-  out[92] = h237 * 0.3333333333333334;
-  out[93] = h236 * 0.5773502691896258;
-  out[94] = h235 * 0.5773502691896258;
-  out[95] = h234 * 0.5773502691896258;
-  out[96] = h233 * 0.5773502691896258;
-  out[97] = h232 * 0.5773502691896258;
-  out[98] = h231 * 0.5773502691896258;
-  out[99] = h230 * 0.3333333333333334;
-  out[100] = h229 * 0.3333333333333334;
-  out[101] = h228 * 0.3333333333333334;
-  out[102] = h227 * 0.5773502691896258;
-  out[103] = h226 * 0.5773502691896258;
-  out[104] = h225 * 0.5773502691896258;
-  out[105] = h224 * 0.3333333333333334;
-  out[106] = h223 * 0.3333333333333334;
-  out[107] = h222 * 0.3333333333333334;
-  store_block<6, 6, 3, 1>(t, a, bra, ket, pb, pk, out, accmax);
-}
-
-// class (22|11) split 0/3: 179 recursion values, 93 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2211_0(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc31 = 0.0;
-  double acc62 = 0.0;
-  double acc93 = 0.0;
-  double acc124 = 0.0;
-  double acc155 = 0.0;
-  double acc186 = 0.0;
-  double acc217 = 0.0;
-  double acc248 = 0.0;
-  double acc1 = 0.0;
-  double acc32 = 0.0;
-  double acc63 = 0.0;
-  double acc94 = 0.0;
-  double acc125 = 0.0;
-  double acc156 = 0.0;
-  double acc187 = 0.0;
-  double acc218 = 0.0;
-  double acc249 = 0.0;
-  double acc2 = 0.0;
-  double acc33 = 0.0;
-  double acc64 = 0.0;
-  double acc95 = 0.0;
-  double acc126 = 0.0;
-  double acc157 = 0.0;
-  double acc188 = 0.0;
-  double acc219 = 0.0;
-  double acc250 = 0.0;
-  double acc3 = 0.0;
-  double acc34 = 0.0;
-  double acc65 = 0.0;
-  double acc96 = 0.0;
-  double acc127 = 0.0;
-  double acc158 = 0.0;
-  double acc189 = 0.0;
-  double acc220 = 0.0;
-  double acc251 = 0.0;
-  double acc4 = 0.0;
-  double acc35 = 0.0;
-  double acc66 = 0.0;
-  double acc97 = 0.0;
-  double acc128 = 0.0;
-  double acc159 = 0.0;
-  double acc190 = 0.0;
-  double acc221 = 0.0;
-  double acc252 = 0.0;
-  double acc5 = 0.0;
-  double acc36 = 0.0;
-  double acc67 = 0.0;
-  double acc98 = 0.0;
-  double acc129 = 0.0;
-  double acc160 = 0.0;
-  double acc191 = 0.0;
-  double acc222 = 0.0;
-  double acc253 = 0.0;
-  double acc6 = 0.0;
-  double acc37 = 0.0;
-  double acc68 = 0.0;
-  double acc99 = 0.0;
-  double acc130 = 0.0;
-  double acc161 = 0.0;
-  double acc192 = 0.0;
-  double acc223 = 0.0;
-  double acc254 = 0.0;
-  double acc7 = 0.0;
-  double acc38 = 0.0;
-  double acc69 = 0.0;
-  double acc100 = 0.0;
-  double acc131 = 0.0;
-  double acc162 = 0.0;
-  double acc193 = 0.0;
-  double acc224 = 0.0;
-  double acc255 = 0.0;
-  double acc8 = 0.0;
-  double acc39 = 0.0;
-  double acc70 = 0.0;
-  double acc101 = 0.0;
-  double acc132 = 0.0;
-  double acc163 = 0.0;
-  double acc194 = 0.0;
-  double acc225 = 0.0;
-  double acc256 = 0.0;
-  double acc9 = 0.0;
-  double acc40 = 0.0;
-  double acc71 = 0.0;
-  double acc102 = 0.0;
-  double acc133 = 0.0;
-  double acc164 = 0.0;
-  double acc195 = 0.0;
-  double acc226 = 0.0;
-  double acc257 = 0.0;
-  double acc10 = 0.0;
-  double acc41 = 0.0;
-  double acc72 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x8 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x43 = s.PA[0] * x16 + s.WP[0] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x25 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x28 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x27 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x31 = s.PA[0] * x10 + s.WP[0] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x37 = s.PA[0] * x13 + s.WP[0] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x78 = s.PA[1] * x38 + s.WP[1] * x44 + z1 * (x13 - s.roz * x16);
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x75 = s.PA[0] * x37 + s.WP[0] * x43 + z2 * (x13 - s.roz * x16);
-      const TB x55 = s.PA[0] * x25 + s.WP[0] * x31 + z2 * (x7 - s.roz * x10);
-      const TB x58 = s.PA[1] * x26 + s.WP[1] * x32 + z1 * (x7 - s.roz * x10);
-      const TB x56 = s.PA[1] * x25 + s.WP[1] * x31;
-      const TB x57 = s.PA[2] * x25 + s.WP[2] * x31;
-      const TB x59 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x65 = s.PA[0] * x31 + s.WP[0] * x37 + z2 * (x10 - s.roz * x13);
-      const TB x66 = s.PA[1] * x31 + s.WP[1] * x37;
-      const TB x67 = s.PA[2] * x31 + s.WP[2] * x37;
-      const TB x68 = s.PA[1] * x32 + s.WP[1] * x38 + z1 * (x10 - s.roz * x13);
-      const TB x69 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x163 = s.QC[0] * x59 + s.WQ[0] * x69 + r1 * x35;
-      acc10 += static_cast<double>(x163);
-      const TB x228 = s.QC[1] * x59 + s.WQ[1] * x69 + r1 * x33;
-      acc41 += static_cast<double>(x228);
-      const TB x293 = s.QC[2] * x59 + s.WQ[2] * x69 + r1 * x32;
-      acc72 += static_cast<double>(x293);
-      const TB x76 = s.PA[1] * x37 + s.WP[1] * x43;
-      const TB x77 = s.PA[2] * x37 + s.WP[2] * x43;
-      const TB x140 = s.QC[0] * x10 + s.WQ[0] * x13 + r1 * x2;
-      const TB x141 = s.QC[0] * x11 + s.WQ[0] * x14;
-      const TB x142 = s.QC[0] * x12 + s.WQ[0] * x15;
-      const TB x143 = s.QC[0] * x25 + s.WQ[0] * x31 + r2 * x10;
-      acc0 += static_cast<double>(x143);
-      const TB x144 = s.QC[0] * x31 + s.WQ[0] * x37 + r2 * x13;
-      const TB x335 = s.QC[0] * x143 + s.WQ[0] * x144 + e1 * (x25 - s.roe * x31) + r2 * x140;
-      acc93 += static_cast<double>(x335);
-      const TB x366 = s.QC[1] * x143 + s.WQ[1] * x144;
-      acc124 += static_cast<double>(x366);
-      const TB x397 = s.QC[2] * x143 + s.WQ[2] * x144;
-      acc155 += static_cast<double>(x397);
-      const TB x145 = s.QC[0] * x26 + s.WQ[0] * x32 + r1 * x11;
-      acc1 += static_cast<double>(x145);
-      const TB x146 = s.QC[0] * x32 + s.WQ[0] * x38 + r1 * x14;
-      const TB x336 = s.QC[0] * x145 + s.WQ[0] * x146 + e1 * (x26 - s.roe * x32) + r1 * x141;
-      acc94 += static_cast<double>(x336);
-      const TB x367 = s.QC[1] * x145 + s.WQ[1] * x146 + r1 * x140;
-      acc125 += static_cast<double>(x367);
-      const TB x398 = s.QC[2] * x145 + s.WQ[2] * x146;
-      acc156 += static_cast<double>(x398);
-      const TB x147 = s.QC[0] * x27 + s.WQ[0] * x33 + r1 * x12;
-      acc2 += static_cast<double>(x147);
-      const TB x148 = s.QC[0] * x33 + s.WQ[0] * x39 + r1 * x15;
-      const TB x399 = s.QC[2] * x147 + s.WQ[2] * x148 + r1 * x140;
-      acc157 += static_cast<double>(x399);
-      const TB x337 = s.QC[0] * x147 + s.WQ[0] * x148 + e1 * (x27 - s.roe * x33) + r1 * x142;
-      acc95 += static_cast<double>(x337);
-      const TB x368 = s.QC[1] * x147 + s.WQ[1] * x148;
-      acc126 += static_cast<double>(x368);
-      const TB x149 = s.QC[0] * x28 + s.WQ[0] * x34;
-      acc3 += static_cast<double>(x149);
-      const TB x150 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x338 = s.QC[0] * x149 + s.WQ[0] * x150 + e1 * (x28 - s.roe * x34);
-      acc96 += static_cast<double>(x338);
-      const TB x369 = s.QC[1] * x149 + s.WQ[1] * x150 + r2 * x141;
-      acc127 += static_cast<double>(x369);
-      const TB x400 = s.QC[2] * x149 + s.WQ[2] * x150;
-      acc158 += static_cast<double>(x400);
-      const TB x151 = s.QC[0] * x29 + s.WQ[0] * x35;
-      acc4 += static_cast<double>(x151);
-      const TB x152 = s.QC[0] * x35 + s.WQ[0] * x41;
-      const TB x401 = s.QC[2] * x151 + s.WQ[2] * x152 + r1 * x141;
-      acc159 += static_cast<double>(x401);
-      const TB x339 = s.QC[0] * x151 + s.WQ[0] * x152 + e1 * (x29 - s.roe * x35);
-      acc97 += static_cast<double>(x339);
-      const TB x370 = s.QC[1] * x151 + s.WQ[1] * x152 + r1 * x142;
-      acc128 += static_cast<double>(x370);
-      const TB x153 = s.QC[0] * x30 + s.WQ[0] * x36;
-      acc5 += static_cast<double>(x153);
-      const TB x154 = s.QC[0] * x36 + s.WQ[0] * x42;
-      const TB x402 = s.QC[2] * x153 + s.WQ[2] * x154 + r2 * x142;
-      acc160 += static_cast<double>(x402);
-      const TB x340 = s.QC[0] * x153 + s.WQ[0] * x154 + e1 * (x30 - s.roe * x36);
-      acc98 += static_cast<double>(x340);
-      const TB x371 = s.QC[1] * x153 + s.WQ[1] * x154;
-      acc129 += static_cast<double>(x371);
-      const TB x155 = s.QC[0] * x55 + s.WQ[0] * x65 + r3 * x31;
-      acc6 += static_cast<double>(x155);
-      const TB x156 = s.QC[0] * x65 + s.WQ[0] * x75 + r3 * x37;
-      const TB x341 = s.QC[0] * x155 + s.WQ[0] * x156 + e1 * (x55 - s.roe * x65) + r3 * x144;
-      acc99 += static_cast<double>(x341);
-      const TB x372 = s.QC[1] * x155 + s.WQ[1] * x156;
-      acc130 += static_cast<double>(x372);
-      const TB x403 = s.QC[2] * x155 + s.WQ[2] * x156;
-      acc161 += static_cast<double>(x403);
-      const TB x157 = s.QC[0] * x56 + s.WQ[0] * x66 + r2 * x32;
-      acc7 += static_cast<double>(x157);
-      const TB x158 = s.QC[0] * x66 + s.WQ[0] * x76 + r2 * x38;
-      const TB x342 = s.QC[0] * x157 + s.WQ[0] * x158 + e1 * (x56 - s.roe * x66) + r2 * x146;
-      acc100 += static_cast<double>(x342);
-      const TB x373 = s.QC[1] * x157 + s.WQ[1] * x158 + r1 * x144;
-      acc131 += static_cast<double>(x373);
-      const TB x404 = s.QC[2] * x157 + s.WQ[2] * x158;
-      acc162 += static_cast<double>(x404);
-      const TB x159 = s.QC[0] * x57 + s.WQ[0] * x67 + r2 * x33;
-      acc8 += static_cast<double>(x159);
-      const TB x160 = s.QC[0] * x67 + s.WQ[0] * x77 + r2 * x39;
-      const TB x343 = s.QC[0] * x159 + s.WQ[0] * x160 + e1 * (x57 - s.roe * x67) + r2 * x148;
-      acc101 += static_cast<double>(x343);
-      const TB x405 = s.QC[2] * x159 + s.WQ[2] * x160 + r1 * x144;
-      acc163 += static_cast<double>(x405);
-      const TB x374 = s.QC[1] * x159 + s.WQ[1] * x160;
-      acc132 += static_cast<double>(x374);
-      const TB x161 = s.QC[0] * x58 + s.WQ[0] * x68 + r1 * x34;
-      acc9 += static_cast<double>(x161);
-      const TB x162 = s.QC[0] * x68 + s.WQ[0] * x78 + r1 * x40;
-      const TB x344 = s.QC[0] * x161 + s.WQ[0] * x162 + e1 * (x58 - s.roe * x68) + r1 * x150;
-      acc102 += static_cast<double>(x344);
-      const TB x375 = s.QC[1] * x161 + s.WQ[1] * x162 + r2 * x146;
-      acc133 += static_cast<double>(x375);
-      const TB x406 = s.QC[2] * x161 + s.WQ[2] * x162;
-      acc164 += static_cast<double>(x406);
-      const TB x205 = s.QC[1] * x10 + s.WQ[1] * x13;
-      const TB x206 = s.QC[1] * x11 + s.WQ[1] * x14 + r1 * x2;
-      const TB x272 = s.QC[2] * x12 + s.WQ[2] * x15 + r1 * x2;
-      const TB x207 = s.QC[1] * x12 + s.WQ[1] * x15;
-      const TB x208 = s.QC[1] * x25 + s.WQ[1] * x31;
-      acc31 += static_cast<double>(x208);
-      const TB x209 = s.QC[1] * x31 + s.WQ[1] * x37;
-      const TB x428 = s.QC[1] * x208 + s.WQ[1] * x209 + e1 * (x25 - s.roe * x31);
-      acc186 += static_cast<double>(x428);
-      const TB x459 = s.QC[2] * x208 + s.WQ[2] * x209;
-      acc217 += static_cast<double>(x459);
-      const TB x210 = s.QC[1] * x26 + s.WQ[1] * x32 + r1 * x10;
-      acc32 += static_cast<double>(x210);
-      const TB x211 = s.QC[1] * x32 + s.WQ[1] * x38 + r1 * x13;
-      const TB x429 = s.QC[1] * x210 + s.WQ[1] * x211 + e1 * (x26 - s.roe * x32) + r1 * x205;
-      acc187 += static_cast<double>(x429);
-      const TB x460 = s.QC[2] * x210 + s.WQ[2] * x211;
-      acc218 += static_cast<double>(x460);
-      const TB x212 = s.QC[1] * x27 + s.WQ[1] * x33;
-      acc33 += static_cast<double>(x212);
-      const TB x213 = s.QC[1] * x33 + s.WQ[1] * x39;
-      const TB x461 = s.QC[2] * x212 + s.WQ[2] * x213 + r1 * x205;
-      acc219 += static_cast<double>(x461);
-      const TB x430 = s.QC[1] * x212 + s.WQ[1] * x213 + e1 * (x27 - s.roe * x33);
-      acc188 += static_cast<double>(x430);
-      const TB x278 = s.QC[2] * x33 + s.WQ[2] * x39 + r1 * x13;
-      const TB x270 = s.QC[2] * x10 + s.WQ[2] * x13;
-      const TB x277 = s.QC[2] * x27 + s.WQ[2] * x33 + r1 * x10;
-      acc64 += static_cast<double>(x277);
-      const TB x492 = s.QC[2] * x277 + s.WQ[2] * x278 + e1 * (x27 - s.roe * x33) + r1 * x270;
-      acc250 += static_cast<double>(x492);
-      const TB x214 = s.QC[1] * x28 + s.WQ[1] * x34 + r2 * x11;
-      acc34 += static_cast<double>(x214);
-      const TB x215 = s.QC[1] * x34 + s.WQ[1] * x40 + r2 * x14;
-      const TB x280 = s.QC[2] * x34 + s.WQ[2] * x40;
-      const TB x431 = s.QC[1] * x214 + s.WQ[1] * x215 + e1 * (x28 - s.roe * x34) + r2 * x206;
-      acc189 += static_cast<double>(x431);
-      const TB x462 = s.QC[2] * x214 + s.WQ[2] * x215;
-      acc220 += static_cast<double>(x462);
-      const TB x216 = s.QC[1] * x29 + s.WQ[1] * x35 + r1 * x12;
-      acc35 += static_cast<double>(x216);
-      const TB x283 = s.QC[2] * x30 + s.WQ[2] * x36 + r2 * x12;
-      acc67 += static_cast<double>(x283);
-      const TB x217 = s.QC[1] * x35 + s.WQ[1] * x41 + r1 * x15;
-      const TB x463 = s.QC[2] * x216 + s.WQ[2] * x217 + r1 * x206;
-      acc221 += static_cast<double>(x463);
-      const TB x432 = s.QC[1] * x216 + s.WQ[1] * x217 + e1 * (x29 - s.roe * x35) + r1 * x207;
-      acc190 += static_cast<double>(x432);
-      const TB x282 = s.QC[2] * x35 + s.WQ[2] * x41 + r1 * x14;
-      const TB x271 = s.QC[2] * x11 + s.WQ[2] * x14;
-      const TB x281 = s.QC[2] * x29 + s.WQ[2] * x35 + r1 * x11;
-      acc66 += static_cast<double>(x281);
-      const TB x494 = s.QC[2] * x281 + s.WQ[2] * x282 + e1 * (x29 - s.roe * x35) + r1 * x271;
-      acc252 += static_cast<double>(x494);
-      const TB x284 = s.QC[2] * x36 + s.WQ[2] * x42 + r2 * x15;
-      const TB x495 = s.QC[2] * x283 + s.WQ[2] * x284 + e1 * (x30 - s.roe * x36) + r2 * x272;
-      acc253 += static_cast<double>(x495);
-      const TB x219 = s.QC[1] * x36 + s.WQ[1] * x42;
-      const TB x218 = s.QC[1] * x30 + s.WQ[1] * x36;
-      acc36 += static_cast<double>(x218);
-      const TB x433 = s.QC[1] * x218 + s.WQ[1] * x219 + e1 * (x30 - s.roe * x36);
-      acc191 += static_cast<double>(x433);
-      const TB x464 = s.QC[2] * x218 + s.WQ[2] * x219 + r2 * x207;
-      acc222 += static_cast<double>(x464);
-      const TB x220 = s.QC[1] * x55 + s.WQ[1] * x65;
-      acc37 += static_cast<double>(x220);
-      const TB x221 = s.QC[1] * x65 + s.WQ[1] * x75;
-      const TB x286 = s.QC[2] * x65 + s.WQ[2] * x75;
-      const TB x434 = s.QC[1] * x220 + s.WQ[1] * x221 + e1 * (x55 - s.roe * x65);
-      acc192 += static_cast<double>(x434);
-      const TB x465 = s.QC[2] * x220 + s.WQ[2] * x221;
-      acc223 += static_cast<double>(x465);
-      const TB x222 = s.QC[1] * x56 + s.WQ[1] * x66 + r1 * x31;
-      acc38 += static_cast<double>(x222);
-      const TB x223 = s.QC[1] * x66 + s.WQ[1] * x76 + r1 * x37;
-      const TB x288 = s.QC[2] * x66 + s.WQ[2] * x76;
-      const TB x435 = s.QC[1] * x222 + s.WQ[1] * x223 + e1 * (x56 - s.roe * x66) + r1 * x209;
-      acc193 += static_cast<double>(x435);
-      const TB x466 = s.QC[2] * x222 + s.WQ[2] * x223;
-      acc224 += static_cast<double>(x466);
-      const TB x224 = s.QC[1] * x57 + s.WQ[1] * x67;
-      acc39 += static_cast<double>(x224);
-      const TB x225 = s.QC[1] * x67 + s.WQ[1] * x77;
-      const TB x467 = s.QC[2] * x224 + s.WQ[2] * x225 + r1 * x209;
-      acc225 += static_cast<double>(x467);
-      const TB x436 = s.QC[1] * x224 + s.WQ[1] * x225 + e1 * (x57 - s.roe * x67);
-      acc194 += static_cast<double>(x436);
-      const TB x290 = s.QC[2] * x67 + s.WQ[2] * x77 + r1 * x37;
-      const TB x274 = s.QC[2] * x31 + s.WQ[2] * x37;
-      const TB x226 = s.QC[1] * x58 + s.WQ[1] * x68 + r2 * x32;
-      acc40 += static_cast<double>(x226);
-      const TB x227 = s.QC[1] * x68 + s.WQ[1] * x78 + r2 * x38;
-      const TB x437 = s.QC[1] * x226 + s.WQ[1] * x227 + e1 * (x58 - s.roe * x68) + r2 * x211;
-      acc195 += static_cast<double>(x437);
-      const TB x468 = s.QC[2] * x226 + s.WQ[2] * x227;
-      acc226 += static_cast<double>(x468);
-      const TB x276 = s.QC[2] * x32 + s.WQ[2] * x38;
-      const TB x292 = s.QC[2] * x68 + s.WQ[2] * x78;
-      const TB x273 = s.QC[2] * x25 + s.WQ[2] * x31;
-      acc62 += static_cast<double>(x273);
-      const TB x490 = s.QC[2] * x273 + s.WQ[2] * x274 + e1 * (x25 - s.roe * x31);
-      acc248 += static_cast<double>(x490);
-      const TB x289 = s.QC[2] * x57 + s.WQ[2] * x67 + r1 * x31;
-      acc70 += static_cast<double>(x289);
-      const TB x498 = s.QC[2] * x289 + s.WQ[2] * x290 + e1 * (x57 - s.roe * x67) + r1 * x274;
-      acc256 += static_cast<double>(x498);
-      const TB x275 = s.QC[2] * x26 + s.WQ[2] * x32;
-      acc63 += static_cast<double>(x275);
-      const TB x491 = s.QC[2] * x275 + s.WQ[2] * x276 + e1 * (x26 - s.roe * x32);
-      acc249 += static_cast<double>(x491);
-      const TB x279 = s.QC[2] * x28 + s.WQ[2] * x34;
-      acc65 += static_cast<double>(x279);
-      const TB x493 = s.QC[2] * x279 + s.WQ[2] * x280 + e1 * (x28 - s.roe * x34);
-      acc251 += static_cast<double>(x493);
-      const TB x285 = s.QC[2] * x55 + s.WQ[2] * x65;
-      acc68 += static_cast<double>(x285);
-      const TB x496 = s.QC[2] * x285 + s.WQ[2] * x286 + e1 * (x55 - s.roe * x65);
-      acc254 += static_cast<double>(x496);
-      const TB x287 = s.QC[2] * x56 + s.WQ[2] * x66;
-      acc69 += static_cast<double>(x287);
-      const TB x497 = s.QC[2] * x287 + s.WQ[2] * x288 + e1 * (x56 - s.roe * x66);
-      acc255 += static_cast<double>(x497);
-      const TB x291 = s.QC[2] * x58 + s.WQ[2] * x68;
-      acc71 += static_cast<double>(x291);
-      const TB x499 = s.QC[2] * x291 + s.WQ[2] * x292 + e1 * (x58 - s.roe * x68);
-      acc257 += static_cast<double>(x499);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(0 * R + r) * a.acc_stride + q] = acc0;
-  a.acc[(31 * R + r) * a.acc_stride + q] = acc31;
-  a.acc[(62 * R + r) * a.acc_stride + q] = acc62;
-  a.acc[(93 * R + r) * a.acc_stride + q] = acc93;
-  a.acc[(124 * R + r) * a.acc_stride + q] = acc124;
-  a.acc[(155 * R + r) * a.acc_stride + q] = acc155;
-  a.acc[(186 * R + r) * a.acc_stride + q] = acc186;
-  a.acc[(217 * R + r) * a.acc_stride + q] = acc217;
-  a.acc[(248 * R + r) * a.acc_stride + q] = acc248;
-  a.acc[(1 * R + r) * a.acc_stride + q] = acc1;
-  a.acc[(32 * R + r) * a.acc_stride + q] = acc32;
-  a.acc[(63 * R + r) * a.acc_stride + q] = acc63;
-  a.acc[(94 * R + r) * a.acc_stride + q] = acc94;
-  a.acc[(125 * R + r) * a.acc_stride + q] = acc125;
-  a.acc[(156 * R + r) * a.acc_stride + q] = acc156;
-  a.acc[(187 * R + r) * a.acc_stride + q] = acc187;
-  a.acc[(218 * R + r) * a.acc_stride + q] = acc218;
-  a.acc[(249 * R + r) * a.acc_stride + q] = acc249;
-  a.acc[(2 * R + r) * a.acc_stride + q] = acc2;
-  a.acc[(33 * R + r) * a.acc_stride + q] = acc33;
-  a.acc[(64 * R + r) * a.acc_stride + q] = acc64;
-  a.acc[(95 * R + r) * a.acc_stride + q] = acc95;
-  a.acc[(126 * R + r) * a.acc_stride + q] = acc126;
-  a.acc[(157 * R + r) * a.acc_stride + q] = acc157;
-  a.acc[(188 * R + r) * a.acc_stride + q] = acc188;
-  a.acc[(219 * R + r) * a.acc_stride + q] = acc219;
-  a.acc[(250 * R + r) * a.acc_stride + q] = acc250;
-  a.acc[(3 * R + r) * a.acc_stride + q] = acc3;
-  a.acc[(34 * R + r) * a.acc_stride + q] = acc34;
-  a.acc[(65 * R + r) * a.acc_stride + q] = acc65;
-  a.acc[(96 * R + r) * a.acc_stride + q] = acc96;
-  a.acc[(127 * R + r) * a.acc_stride + q] = acc127;
-  a.acc[(158 * R + r) * a.acc_stride + q] = acc158;
-  a.acc[(189 * R + r) * a.acc_stride + q] = acc189;
-  a.acc[(220 * R + r) * a.acc_stride + q] = acc220;
-  a.acc[(251 * R + r) * a.acc_stride + q] = acc251;
-  a.acc[(4 * R + r) * a.acc_stride + q] = acc4;
-  a.acc[(35 * R + r) * a.acc_stride + q] = acc35;
-  a.acc[(66 * R + r) * a.acc_stride + q] = acc66;
-  a.acc[(97 * R + r) * a.acc_stride + q] = acc97;
-  a.acc[(128 * R + r) * a.acc_stride + q] = acc128;
-  a.acc[(159 * R + r) * a.acc_stride + q] = acc159;
-  a.acc[(190 * R + r) * a.acc_stride + q] = acc190;
-  a.acc[(221 * R + r) * a.acc_stride + q] = acc221;
-  a.acc[(252 * R + r) * a.acc_stride + q] = acc252;
-  a.acc[(5 * R + r) * a.acc_stride + q] = acc5;
-  a.acc[(36 * R + r) * a.acc_stride + q] = acc36;
-  a.acc[(67 * R + r) * a.acc_stride + q] = acc67;
-  a.acc[(98 * R + r) * a.acc_stride + q] = acc98;
-  a.acc[(129 * R + r) * a.acc_stride + q] = acc129;
-  a.acc[(160 * R + r) * a.acc_stride + q] = acc160;
-  a.acc[(191 * R + r) * a.acc_stride + q] = acc191;
-  a.acc[(222 * R + r) * a.acc_stride + q] = acc222;
-  a.acc[(253 * R + r) * a.acc_stride + q] = acc253;
-  a.acc[(6 * R + r) * a.acc_stride + q] = acc6;
-  a.acc[(37 * R + r) * a.acc_stride + q] = acc37;
-  a.acc[(68 * R + r) * a.acc_stride + q] = acc68;
-  a.acc[(99 * R + r) * a.acc_stride + q] = acc99;
-  a.acc[(130 * R + r) * a.acc_stride + q] = acc130;
-  a.acc[(161 * R + r) * a.acc_stride + q] = acc161;
-  a.acc[(192 * R + r) * a.acc_stride + q] = acc192;
-  a.acc[(223 * R + r) * a.acc_stride + q] = acc223;
-  a.acc[(254 * R + r) * a.acc_stride + q] = acc254;
-  a.acc[(7 * R + r) * a.acc_stride + q] = acc7;
-  a.acc[(38 * R + r) * a.acc_stride + q] = acc38;
-  a.acc[(69 * R + r) * a.acc_stride + q] = acc69;
-  a.acc[(100 * R + r) * a.acc_stride + q] = acc100;
-  a.acc[(131 * R + r) * a.acc_stride + q] = acc131;
-  a.acc[(162 * R + r) * a.acc_stride + q] = acc162;
-  a.acc[(193 * R + r) * a.acc_stride + q] = acc193;
-  a.acc[(224 * R + r) * a.acc_stride + q] = acc224;
-  a.acc[(255 * R + r) * a.acc_stride + q] = acc255;
-  a.acc[(8 * R + r) * a.acc_stride + q] = acc8;
-  a.acc[(39 * R + r) * a.acc_stride + q] = acc39;
-  a.acc[(70 * R + r) * a.acc_stride + q] = acc70;
-  a.acc[(101 * R + r) * a.acc_stride + q] = acc101;
-  a.acc[(132 * R + r) * a.acc_stride + q] = acc132;
-  a.acc[(163 * R + r) * a.acc_stride + q] = acc163;
-  a.acc[(194 * R + r) * a.acc_stride + q] = acc194;
-  a.acc[(225 * R + r) * a.acc_stride + q] = acc225;
-  a.acc[(256 * R + r) * a.acc_stride + q] = acc256;
-  a.acc[(9 * R + r) * a.acc_stride + q] = acc9;
-  a.acc[(40 * R + r) * a.acc_stride + q] = acc40;
-  a.acc[(71 * R + r) * a.acc_stride + q] = acc71;
-  a.acc[(102 * R + r) * a.acc_stride + q] = acc102;
-  a.acc[(133 * R + r) * a.acc_stride + q] = acc133;
-  a.acc[(164 * R + r) * a.acc_stride + q] = acc164;
-  a.acc[(195 * R + r) * a.acc_stride + q] = acc195;
-  a.acc[(226 * R + r) * a.acc_stride + q] = acc226;
-  a.acc[(257 * R + r) * a.acc_stride + q] = acc257;
-  a.acc[(10 * R + r) * a.acc_stride + q] = acc10;
-  a.acc[(41 * R + r) * a.acc_stride + q] = acc41;
-  a.acc[(72 * R + r) * a.acc_stride + q] = acc72;
-}
-// class (22|11) split 1/3: 235 recursion values, 93 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2211_1(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc103 = 0.0;
-  double acc134 = 0.0;
-  double acc165 = 0.0;
-  double acc196 = 0.0;
-  double acc227 = 0.0;
-  double acc258 = 0.0;
-  double acc11 = 0.0;
-  double acc42 = 0.0;
-  double acc73 = 0.0;
-  double acc104 = 0.0;
-  double acc135 = 0.0;
-  double acc166 = 0.0;
-  double acc197 = 0.0;
-  double acc228 = 0.0;
-  double acc259 = 0.0;
-  double acc12 = 0.0;
-  double acc43 = 0.0;
-  double acc74 = 0.0;
-  double acc105 = 0.0;
-  double acc136 = 0.0;
-  double acc167 = 0.0;
-  double acc198 = 0.0;
-  double acc229 = 0.0;
-  double acc260 = 0.0;
-  double acc13 = 0.0;
-  double acc44 = 0.0;
-  double acc75 = 0.0;
-  double acc106 = 0.0;
-  double acc137 = 0.0;
-  double acc168 = 0.0;
-  double acc199 = 0.0;
-  double acc230 = 0.0;
-  double acc261 = 0.0;
-  double acc14 = 0.0;
-  double acc45 = 0.0;
-  double acc76 = 0.0;
-  double acc107 = 0.0;
-  double acc138 = 0.0;
-  double acc169 = 0.0;
-  double acc200 = 0.0;
-  double acc231 = 0.0;
-  double acc262 = 0.0;
-  double acc15 = 0.0;
-  double acc46 = 0.0;
-  double acc77 = 0.0;
-  double acc108 = 0.0;
-  double acc139 = 0.0;
-  double acc170 = 0.0;
-  double acc201 = 0.0;
-  double acc232 = 0.0;
-  double acc263 = 0.0;
-  double acc16 = 0.0;
-  double acc47 = 0.0;
-  double acc78 = 0.0;
-  double acc109 = 0.0;
-  double acc140 = 0.0;
-  double acc171 = 0.0;
-  double acc202 = 0.0;
-  double acc233 = 0.0;
-  double acc264 = 0.0;
-  double acc17 = 0.0;
-  double acc48 = 0.0;
-  double acc79 = 0.0;
-  double acc110 = 0.0;
-  double acc141 = 0.0;
-  double acc172 = 0.0;
-  double acc203 = 0.0;
-  double acc234 = 0.0;
-  double acc265 = 0.0;
-  double acc18 = 0.0;
-  double acc49 = 0.0;
-  double acc80 = 0.0;
-  double acc111 = 0.0;
-  double acc142 = 0.0;
-  double acc173 = 0.0;
-  double acc204 = 0.0;
-  double acc235 = 0.0;
-  double acc266 = 0.0;
-  double acc19 = 0.0;
-  double acc50 = 0.0;
-  double acc81 = 0.0;
-  double acc112 = 0.0;
-  double acc143 = 0.0;
-  double acc174 = 0.0;
-  double acc205 = 0.0;
-  double acc236 = 0.0;
-  double acc267 = 0.0;
-  double acc20 = 0.0;
-  double acc51 = 0.0;
-  double acc82 = 0.0;
-  double acc113 = 0.0;
-  double acc144 = 0.0;
-  double acc175 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x22 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x8 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x49 = s.PA[0] * x19 + s.WP[0] * x22 + z1 * (x4 - s.roz * x5);
-      const TB x20 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x21 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x48 = s.PA[2] * x18 + s.WP[2] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x25 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x27 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x28 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x31 = s.PA[0] * x10 + s.WP[0] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x59 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x60 = s.PA[2] * x27 + s.WP[2] * x33 + z1 * (x7 - s.roz * x10);
-      const TB x55 = s.PA[0] * x25 + s.WP[0] * x31 + z2 * (x7 - s.roz * x10);
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x64 = s.PA[2] * x30 + s.WP[2] * x36 + z2 * (x9 - s.roz * x12);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x63 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x8 - s.roz * x11);
-      const TB x61 = s.PA[1] * x28 + s.WP[1] * x34 + z2 * (x8 - s.roz * x11);
-      const TB x62 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x37 = s.PA[0] * x13 + s.WP[0] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x84 = s.PA[2] * x42 + s.WP[2] * x48 + z2 * (x15 - s.roz * x18);
-      const TB x74 = s.PA[2] * x36 + s.WP[2] * x42 + z2 * (x12 - s.roz * x15);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x43 = s.PA[0] * x16 + s.WP[0] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x46 = s.PA[1] * x17 + s.WP[1] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x47 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x83 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x14 - s.roz * x17);
-      const TB x81 = s.PA[1] * x40 + s.WP[1] * x46 + z2 * (x14 - s.roz * x17);
-      const TB x82 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x45 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x80 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x13 - s.roz * x16);
-      const TB x85 = s.PA[0] * x43 + s.WP[0] * x49 + z2 * (x16 - s.roz * x19);
-      const TB x86 = s.PA[1] * x43 + s.WP[1] * x49;
-      const TB x56 = s.PA[1] * x25 + s.WP[1] * x31;
-      const TB x65 = s.PA[0] * x31 + s.WP[0] * x37 + z2 * (x10 - s.roz * x13);
-      const TB x66 = s.PA[1] * x31 + s.WP[1] * x37;
-      const TB x67 = s.PA[2] * x31 + s.WP[2] * x37;
-      const TB x68 = s.PA[1] * x32 + s.WP[1] * x38 + z1 * (x10 - s.roz * x13);
-      const TB x70 = s.PA[2] * x33 + s.WP[2] * x39 + z1 * (x10 - s.roz * x13);
-      const TB x69 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x71 = s.PA[1] * x34 + s.WP[1] * x40 + z2 * (x11 - s.roz * x14);
-      const TB x73 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x11 - s.roz * x14);
-      const TB x72 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x75 = s.PA[0] * x37 + s.WP[0] * x43 + z2 * (x13 - s.roz * x16);
-      const TB x78 = s.PA[1] * x38 + s.WP[1] * x44 + z1 * (x13 - s.roz * x16);
-      const TB x79 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x76 = s.PA[1] * x37 + s.WP[1] * x43;
-      const TB x77 = s.PA[2] * x37 + s.WP[2] * x43;
-      const TB x95 = s.PA[0] * x55 + s.WP[0] * x65 + z3 * (x25 - s.roz * x31);
-      const TB x98 = s.PA[1] * x56 + s.WP[1] * x66 + z1 * (x25 - s.roz * x31);
-      const TB x99 = s.PA[2] * x56 + s.WP[2] * x66;
-      const TB x96 = s.PA[1] * x55 + s.WP[1] * x65;
-      const TB x97 = s.PA[2] * x55 + s.WP[2] * x65;
-      const TB x110 = s.PA[0] * x65 + s.WP[0] * x75 + z3 * (x31 - s.roz * x37);
-      const TB x113 = s.PA[1] * x66 + s.WP[1] * x76 + z1 * (x31 - s.roz * x37);
-      const TB x111 = s.PA[1] * x65 + s.WP[1] * x75;
-      const TB x112 = s.PA[2] * x65 + s.WP[2] * x75;
-      const TB x114 = s.PA[2] * x66 + s.WP[2] * x76;
-      const TB x248 = s.QC[1] * x99 + s.WQ[1] * x114 + r1 * x67;
-      acc51 += static_cast<double>(x248);
-      const TB x313 = s.QC[2] * x99 + s.WQ[2] * x114 + r1 * x66;
-      acc82 += static_cast<double>(x313);
-      const TB x125 = s.PA[0] * x75 + s.WP[0] * x85 + z3 * (x37 - s.roz * x43);
-      const TB x128 = s.PA[1] * x76 + s.WP[1] * x86 + z1 * (x37 - s.roz * x43);
-      const TB x129 = s.PA[2] * x76 + s.WP[2] * x86;
-      const TB x184 = s.QC[0] * x114 + s.WQ[0] * x129 + r2 * x79;
-      const TB x126 = s.PA[1] * x75 + s.WP[1] * x85;
-      const TB x127 = s.PA[2] * x75 + s.WP[2] * x85;
-      const TB x146 = s.QC[0] * x32 + s.WQ[0] * x38 + r1 * x14;
-      const TB x148 = s.QC[0] * x33 + s.WQ[0] * x39 + r1 * x15;
-      const TB x150 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x152 = s.QC[0] * x35 + s.WQ[0] * x41;
-      const TB x154 = s.QC[0] * x36 + s.WQ[0] * x42;
-      const TB x156 = s.QC[0] * x65 + s.WQ[0] * x75 + r3 * x37;
-      const TB x223 = s.QC[1] * x66 + s.WQ[1] * x76 + r1 * x37;
-      const TB x158 = s.QC[0] * x66 + s.WQ[0] * x76 + r2 * x38;
-      const TB x160 = s.QC[0] * x67 + s.WQ[0] * x77 + r2 * x39;
-      const TB x179 = s.QC[0] * x97 + s.WQ[0] * x112 + r3 * x67;
-      acc18 += static_cast<double>(x179);
-      const TB x180 = s.QC[0] * x112 + s.WQ[0] * x127 + r3 * x77;
-      const TB x353 = s.QC[0] * x179 + s.WQ[0] * x180 + e1 * (x97 - s.roe * x112) + r3 * x160;
-      acc111 += static_cast<double>(x353);
-      const TB x384 = s.QC[1] * x179 + s.WQ[1] * x180;
-      acc142 += static_cast<double>(x384);
-      const TB x415 = s.QC[2] * x179 + s.WQ[2] * x180 + r1 * x156;
-      acc173 += static_cast<double>(x415);
-      const TB x162 = s.QC[0] * x68 + s.WQ[0] * x78 + r1 * x40;
-      const TB x181 = s.QC[0] * x98 + s.WQ[0] * x113 + r2 * x68;
-      acc19 += static_cast<double>(x181);
-      const TB x182 = s.QC[0] * x113 + s.WQ[0] * x128 + r2 * x78;
-      const TB x354 = s.QC[0] * x181 + s.WQ[0] * x182 + e1 * (x98 - s.roe * x113) + r2 * x162;
-      acc112 += static_cast<double>(x354);
-      const TB x385 = s.QC[1] * x181 + s.WQ[1] * x182 + r2 * x158;
-      acc143 += static_cast<double>(x385);
-      const TB x416 = s.QC[2] * x181 + s.WQ[2] * x182;
-      acc174 += static_cast<double>(x416);
-      const TB x163 = s.QC[0] * x59 + s.WQ[0] * x69 + r1 * x35;
-      const TB x164 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x41;
-      const TB x407 = s.QC[2] * x163 + s.WQ[2] * x164 + r1 * x146;
-      acc165 += static_cast<double>(x407);
-      const TB x345 = s.QC[0] * x163 + s.WQ[0] * x164 + e1 * (x59 - s.roe * x69) + r1 * x152;
-      acc103 += static_cast<double>(x345);
-      const TB x376 = s.QC[1] * x163 + s.WQ[1] * x164 + r1 * x148;
-      acc134 += static_cast<double>(x376);
-      const TB x165 = s.QC[0] * x60 + s.WQ[0] * x70 + r1 * x36;
-      acc11 += static_cast<double>(x165);
-      const TB x166 = s.QC[0] * x70 + s.WQ[0] * x80 + r1 * x42;
-      const TB x408 = s.QC[2] * x165 + s.WQ[2] * x166 + r2 * x148;
-      acc166 += static_cast<double>(x408);
-      const TB x346 = s.QC[0] * x165 + s.WQ[0] * x166 + e1 * (x60 - s.roe * x70) + r1 * x154;
-      acc104 += static_cast<double>(x346);
-      const TB x377 = s.QC[1] * x165 + s.WQ[1] * x166;
-      acc135 += static_cast<double>(x377);
-      const TB x167 = s.QC[0] * x61 + s.WQ[0] * x71;
-      acc12 += static_cast<double>(x167);
-      const TB x168 = s.QC[0] * x71 + s.WQ[0] * x81;
-      const TB x347 = s.QC[0] * x167 + s.WQ[0] * x168 + e1 * (x61 - s.roe * x71);
-      acc105 += static_cast<double>(x347);
-      const TB x378 = s.QC[1] * x167 + s.WQ[1] * x168 + r3 * x150;
-      acc136 += static_cast<double>(x378);
-      const TB x409 = s.QC[2] * x167 + s.WQ[2] * x168;
-      acc167 += static_cast<double>(x409);
-      const TB x169 = s.QC[0] * x62 + s.WQ[0] * x72;
-      acc13 += static_cast<double>(x169);
-      const TB x170 = s.QC[0] * x72 + s.WQ[0] * x82;
-      const TB x410 = s.QC[2] * x169 + s.WQ[2] * x170 + r1 * x150;
-      acc168 += static_cast<double>(x410);
-      const TB x348 = s.QC[0] * x169 + s.WQ[0] * x170 + e1 * (x62 - s.roe * x72);
-      acc106 += static_cast<double>(x348);
-      const TB x379 = s.QC[1] * x169 + s.WQ[1] * x170 + r2 * x152;
-      acc137 += static_cast<double>(x379);
-      const TB x171 = s.QC[0] * x63 + s.WQ[0] * x73;
-      acc14 += static_cast<double>(x171);
-      const TB x172 = s.QC[0] * x73 + s.WQ[0] * x83;
-      const TB x411 = s.QC[2] * x171 + s.WQ[2] * x172 + r2 * x152;
-      acc169 += static_cast<double>(x411);
-      const TB x349 = s.QC[0] * x171 + s.WQ[0] * x172 + e1 * (x63 - s.roe * x73);
-      acc107 += static_cast<double>(x349);
-      const TB x380 = s.QC[1] * x171 + s.WQ[1] * x172 + r1 * x154;
-      acc138 += static_cast<double>(x380);
-      const TB x173 = s.QC[0] * x64 + s.WQ[0] * x74;
-      acc15 += static_cast<double>(x173);
-      const TB x174 = s.QC[0] * x74 + s.WQ[0] * x84;
-      const TB x412 = s.QC[2] * x173 + s.WQ[2] * x174 + r3 * x154;
-      acc170 += static_cast<double>(x412);
-      const TB x350 = s.QC[0] * x173 + s.WQ[0] * x174 + e1 * (x64 - s.roe * x74);
-      acc108 += static_cast<double>(x350);
-      const TB x381 = s.QC[1] * x173 + s.WQ[1] * x174;
-      acc139 += static_cast<double>(x381);
-      const TB x175 = s.QC[0] * x95 + s.WQ[0] * x110 + r4 * x65;
-      acc16 += static_cast<double>(x175);
-      const TB x176 = s.QC[0] * x110 + s.WQ[0] * x125 + r4 * x75;
-      const TB x351 = s.QC[0] * x175 + s.WQ[0] * x176 + e1 * (x95 - s.roe * x110) + r4 * x156;
-      acc109 += static_cast<double>(x351);
-      const TB x382 = s.QC[1] * x175 + s.WQ[1] * x176;
-      acc140 += static_cast<double>(x382);
-      const TB x413 = s.QC[2] * x175 + s.WQ[2] * x176;
-      acc171 += static_cast<double>(x413);
-      const TB x177 = s.QC[0] * x96 + s.WQ[0] * x111 + r3 * x66;
-      acc17 += static_cast<double>(x177);
-      const TB x246 = s.QC[1] * x98 + s.WQ[1] * x113 + r2 * x66;
-      acc50 += static_cast<double>(x246);
-      const TB x178 = s.QC[0] * x111 + s.WQ[0] * x126 + r3 * x76;
-      const TB x383 = s.QC[1] * x177 + s.WQ[1] * x178 + r1 * x156;
-      acc141 += static_cast<double>(x383);
-      const TB x247 = s.QC[1] * x113 + s.WQ[1] * x128 + r2 * x76;
-      const TB x447 = s.QC[1] * x246 + s.WQ[1] * x247 + e1 * (x98 - s.roe * x113) + r2 * x223;
-      acc205 += static_cast<double>(x447);
-      const TB x478 = s.QC[2] * x246 + s.WQ[2] * x247;
-      acc236 += static_cast<double>(x478);
-      const TB x312 = s.QC[2] * x113 + s.WQ[2] * x128;
-      const TB x352 = s.QC[0] * x177 + s.WQ[0] * x178 + e1 * (x96 - s.roe * x111) + r3 * x158;
-      acc110 += static_cast<double>(x352);
-      const TB x414 = s.QC[2] * x177 + s.WQ[2] * x178;
-      acc172 += static_cast<double>(x414);
-      const TB x183 = s.QC[0] * x99 + s.WQ[0] * x114 + r2 * x69;
-      acc20 += static_cast<double>(x183);
-      const TB x355 = s.QC[0] * x183 + s.WQ[0] * x184 + e1 * (x99 - s.roe * x114) + r2 * x164;
-      acc113 += static_cast<double>(x355);
-      const TB x386 = s.QC[1] * x183 + s.WQ[1] * x184 + r1 * x160;
-      acc144 += static_cast<double>(x386);
-      const TB x417 = s.QC[2] * x183 + s.WQ[2] * x184 + r1 * x158;
-      acc175 += static_cast<double>(x417);
-      const TB x211 = s.QC[1] * x32 + s.WQ[1] * x38 + r1 * x13;
-      const TB x278 = s.QC[2] * x33 + s.WQ[2] * x39 + r1 * x13;
-      const TB x213 = s.QC[1] * x33 + s.WQ[1] * x39;
-      const TB x215 = s.QC[1] * x34 + s.WQ[1] * x40 + r2 * x14;
-      const TB x282 = s.QC[2] * x35 + s.WQ[2] * x41 + r1 * x14;
-      const TB x217 = s.QC[1] * x35 + s.WQ[1] * x41 + r1 * x15;
-      const TB x284 = s.QC[2] * x36 + s.WQ[2] * x42 + r2 * x15;
-      const TB x219 = s.QC[1] * x36 + s.WQ[1] * x42;
-      const TB x221 = s.QC[1] * x65 + s.WQ[1] * x75;
-      const TB x228 = s.QC[1] * x59 + s.WQ[1] * x69 + r1 * x33;
-      const TB x295 = s.QC[2] * x60 + s.WQ[2] * x70 + r2 * x33;
-      acc73 += static_cast<double>(x295);
-      const TB x229 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x39;
-      const TB x469 = s.QC[2] * x228 + s.WQ[2] * x229 + r1 * x211;
-      acc227 += static_cast<double>(x469);
-      const TB x438 = s.QC[1] * x228 + s.WQ[1] * x229 + e1 * (x59 - s.roe * x69) + r1 * x213;
-      acc196 += static_cast<double>(x438);
-      const TB x294 = s.QC[2] * x69 + s.WQ[2] * x79 + r1 * x38;
-      const TB x276 = s.QC[2] * x32 + s.WQ[2] * x38;
-      const TB x293 = s.QC[2] * x59 + s.WQ[2] * x69 + r1 * x32;
-      const TB x500 = s.QC[2] * x293 + s.WQ[2] * x294 + e1 * (x59 - s.roe * x69) + r1 * x276;
-      acc258 += static_cast<double>(x500);
-      const TB x296 = s.QC[2] * x70 + s.WQ[2] * x80 + r2 * x39;
-      const TB x501 = s.QC[2] * x295 + s.WQ[2] * x296 + e1 * (x60 - s.roe * x70) + r2 * x278;
-      acc259 += static_cast<double>(x501);
-      const TB x231 = s.QC[1] * x70 + s.WQ[1] * x80;
-      const TB x230 = s.QC[1] * x60 + s.WQ[1] * x70;
-      acc42 += static_cast<double>(x230);
-      const TB x439 = s.QC[1] * x230 + s.WQ[1] * x231 + e1 * (x60 - s.roe * x70);
-      acc197 += static_cast<double>(x439);
-      const TB x470 = s.QC[2] * x230 + s.WQ[2] * x231 + r2 * x213;
-      acc228 += static_cast<double>(x470);
-      const TB x232 = s.QC[1] * x61 + s.WQ[1] * x71 + r3 * x34;
-      acc43 += static_cast<double>(x232);
-      const TB x233 = s.QC[1] * x71 + s.WQ[1] * x81 + r3 * x40;
-      const TB x298 = s.QC[2] * x71 + s.WQ[2] * x81;
-      const TB x440 = s.QC[1] * x232 + s.WQ[1] * x233 + e1 * (x61 - s.roe * x71) + r3 * x215;
-      acc198 += static_cast<double>(x440);
-      const TB x471 = s.QC[2] * x232 + s.WQ[2] * x233;
-      acc229 += static_cast<double>(x471);
-      const TB x234 = s.QC[1] * x62 + s.WQ[1] * x72 + r2 * x35;
-      acc44 += static_cast<double>(x234);
-      const TB x301 = s.QC[2] * x63 + s.WQ[2] * x73 + r2 * x35;
-      acc76 += static_cast<double>(x301);
-      const TB x235 = s.QC[1] * x72 + s.WQ[1] * x82 + r2 * x41;
-      const TB x472 = s.QC[2] * x234 + s.WQ[2] * x235 + r1 * x215;
-      acc230 += static_cast<double>(x472);
-      const TB x441 = s.QC[1] * x234 + s.WQ[1] * x235 + e1 * (x62 - s.roe * x72) + r2 * x217;
-      acc199 += static_cast<double>(x441);
-      const TB x300 = s.QC[2] * x72 + s.WQ[2] * x82 + r1 * x40;
-      const TB x280 = s.QC[2] * x34 + s.WQ[2] * x40;
-      const TB x299 = s.QC[2] * x62 + s.WQ[2] * x72 + r1 * x34;
-      acc75 += static_cast<double>(x299);
-      const TB x503 = s.QC[2] * x299 + s.WQ[2] * x300 + e1 * (x62 - s.roe * x72) + r1 * x280;
-      acc261 += static_cast<double>(x503);
-      const TB x302 = s.QC[2] * x73 + s.WQ[2] * x83 + r2 * x41;
-      const TB x504 = s.QC[2] * x301 + s.WQ[2] * x302 + e1 * (x63 - s.roe * x73) + r2 * x282;
-      acc262 += static_cast<double>(x504);
-      const TB x237 = s.QC[1] * x73 + s.WQ[1] * x83 + r1 * x42;
-      const TB x304 = s.QC[2] * x74 + s.WQ[2] * x84 + r3 * x42;
-      const TB x239 = s.QC[1] * x74 + s.WQ[1] * x84;
-      const TB x236 = s.QC[1] * x63 + s.WQ[1] * x73 + r1 * x36;
-      acc45 += static_cast<double>(x236);
-      const TB x442 = s.QC[1] * x236 + s.WQ[1] * x237 + e1 * (x63 - s.roe * x73) + r1 * x219;
-      acc200 += static_cast<double>(x442);
-      const TB x473 = s.QC[2] * x236 + s.WQ[2] * x237 + r2 * x217;
-      acc231 += static_cast<double>(x473);
-      const TB x303 = s.QC[2] * x64 + s.WQ[2] * x74 + r3 * x36;
-      acc77 += static_cast<double>(x303);
-      const TB x505 = s.QC[2] * x303 + s.WQ[2] * x304 + e1 * (x64 - s.roe * x74) + r3 * x284;
-      acc263 += static_cast<double>(x505);
-      const TB x238 = s.QC[1] * x64 + s.WQ[1] * x74;
-      acc46 += static_cast<double>(x238);
-      const TB x443 = s.QC[1] * x238 + s.WQ[1] * x239 + e1 * (x64 - s.roe * x74);
-      acc201 += static_cast<double>(x443);
-      const TB x474 = s.QC[2] * x238 + s.WQ[2] * x239 + r3 * x219;
-      acc232 += static_cast<double>(x474);
-      const TB x240 = s.QC[1] * x95 + s.WQ[1] * x110;
-      acc47 += static_cast<double>(x240);
-      const TB x241 = s.QC[1] * x110 + s.WQ[1] * x125;
-      const TB x306 = s.QC[2] * x110 + s.WQ[2] * x125;
-      const TB x444 = s.QC[1] * x240 + s.WQ[1] * x241 + e1 * (x95 - s.roe * x110);
-      acc202 += static_cast<double>(x444);
-      const TB x475 = s.QC[2] * x240 + s.WQ[2] * x241;
-      acc233 += static_cast<double>(x475);
-      const TB x242 = s.QC[1] * x96 + s.WQ[1] * x111 + r1 * x65;
-      acc48 += static_cast<double>(x242);
-      const TB x243 = s.QC[1] * x111 + s.WQ[1] * x126 + r1 * x75;
-      const TB x308 = s.QC[2] * x111 + s.WQ[2] * x126;
-      const TB x445 = s.QC[1] * x242 + s.WQ[1] * x243 + e1 * (x96 - s.roe * x111) + r1 * x221;
-      acc203 += static_cast<double>(x445);
-      const TB x476 = s.QC[2] * x242 + s.WQ[2] * x243;
-      acc234 += static_cast<double>(x476);
-      const TB x244 = s.QC[1] * x97 + s.WQ[1] * x112;
-      acc49 += static_cast<double>(x244);
-      const TB x245 = s.QC[1] * x112 + s.WQ[1] * x127;
-      const TB x477 = s.QC[2] * x244 + s.WQ[2] * x245 + r1 * x221;
-      acc235 += static_cast<double>(x477);
-      const TB x446 = s.QC[1] * x244 + s.WQ[1] * x245 + e1 * (x97 - s.roe * x112);
-      acc204 += static_cast<double>(x446);
-      const TB x310 = s.QC[2] * x112 + s.WQ[2] * x127 + r1 * x75;
-      const TB x286 = s.QC[2] * x65 + s.WQ[2] * x75;
-      const TB x309 = s.QC[2] * x97 + s.WQ[2] * x112 + r1 * x65;
-      acc80 += static_cast<double>(x309);
-      const TB x508 = s.QC[2] * x309 + s.WQ[2] * x310 + e1 * (x97 - s.roe * x112) + r1 * x286;
-      acc266 += static_cast<double>(x508);
-      const TB x297 = s.QC[2] * x61 + s.WQ[2] * x71;
-      acc74 += static_cast<double>(x297);
-      const TB x502 = s.QC[2] * x297 + s.WQ[2] * x298 + e1 * (x61 - s.roe * x71);
-      acc260 += static_cast<double>(x502);
-      const TB x305 = s.QC[2] * x95 + s.WQ[2] * x110;
-      acc78 += static_cast<double>(x305);
-      const TB x506 = s.QC[2] * x305 + s.WQ[2] * x306 + e1 * (x95 - s.roe * x110);
-      acc264 += static_cast<double>(x506);
-      const TB x307 = s.QC[2] * x96 + s.WQ[2] * x111;
-      acc79 += static_cast<double>(x307);
-      const TB x507 = s.QC[2] * x307 + s.WQ[2] * x308 + e1 * (x96 - s.roe * x111);
-      acc265 += static_cast<double>(x507);
-      const TB x311 = s.QC[2] * x98 + s.WQ[2] * x113;
-      acc81 += static_cast<double>(x311);
-      const TB x509 = s.QC[2] * x311 + s.WQ[2] * x312 + e1 * (x98 - s.roe * x113);
-      acc267 += static_cast<double>(x509);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(103 * R + r) * a.acc_stride + q] = acc103;
-  a.acc[(134 * R + r) * a.acc_stride + q] = acc134;
-  a.acc[(165 * R + r) * a.acc_stride + q] = acc165;
-  a.acc[(196 * R + r) * a.acc_stride + q] = acc196;
-  a.acc[(227 * R + r) * a.acc_stride + q] = acc227;
-  a.acc[(258 * R + r) * a.acc_stride + q] = acc258;
-  a.acc[(11 * R + r) * a.acc_stride + q] = acc11;
-  a.acc[(42 * R + r) * a.acc_stride + q] = acc42;
-  a.acc[(73 * R + r) * a.acc_stride + q] = acc73;
-  a.acc[(104 * R + r) * a.acc_stride + q] = acc104;
-  a.acc[(135 * R + r) * a.acc_stride + q] = acc135;
-  a.acc[(166 * R + r) * a.acc_stride + q] = acc166;
-  a.acc[(197 * R + r) * a.acc_stride + q] = acc197;
-  a.acc[(228 * R + r) * a.acc_stride + q] = acc228;
-  a.acc[(259 * R + r) * a.acc_stride + q] = acc259;
-  a.acc[(12 * R + r) * a.acc_stride + q] = acc12;
-  a.acc[(43 * R + r) * a.acc_stride + q] = acc43;
-  a.acc[(74 * R + r) * a.acc_stride + q] = acc74;
-  a.acc[(105 * R + r) * a.acc_stride + q] = acc105;
-  a.acc[(136 * R + r) * a.acc_stride + q] = acc136;
-  a.acc[(167 * R + r) * a.acc_stride + q] = acc167;
-  a.acc[(198 * R + r) * a.acc_stride + q] = acc198;
-  a.acc[(229 * R + r) * a.acc_stride + q] = acc229;
-  a.acc[(260 * R + r) * a.acc_stride + q] = acc260;
-  a.acc[(13 * R + r) * a.acc_stride + q] = acc13;
-  a.acc[(44 * R + r) * a.acc_stride + q] = acc44;
-  a.acc[(75 * R + r) * a.acc_stride + q] = acc75;
-  a.acc[(106 * R + r) * a.acc_stride + q] = acc106;
-  a.acc[(137 * R + r) * a.acc_stride + q] = acc137;
-  a.acc[(168 * R + r) * a.acc_stride + q] = acc168;
-  a.acc[(199 * R + r) * a.acc_stride + q] = acc199;
-  a.acc[(230 * R + r) * a.acc_stride + q] = acc230;
-  a.acc[(261 * R + r) * a.acc_stride + q] = acc261;
-  a.acc[(14 * R + r) * a.acc_stride + q] = acc14;
-  a.acc[(45 * R + r) * a.acc_stride + q] = acc45;
-  a.acc[(76 * R + r) * a.acc_stride + q] = acc76;
-  a.acc[(107 * R + r) * a.acc_stride + q] = acc107;
-  a.acc[(138 * R + r) * a.acc_stride + q] = acc138;
-  a.acc[(169 * R + r) * a.acc_stride + q] = acc169;
-  a.acc[(200 * R + r) * a.acc_stride + q] = acc200;
-  a.acc[(231 * R + r) * a.acc_stride + q] = acc231;
-  a.acc[(262 * R + r) * a.acc_stride + q] = acc262;
-  a.acc[(15 * R + r) * a.acc_stride + q] = acc15;
-  a.acc[(46 * R + r) * a.acc_stride + q] = acc46;
-  a.acc[(77 * R + r) * a.acc_stride + q] = acc77;
-  a.acc[(108 * R + r) * a.acc_stride + q] = acc108;
-  a.acc[(139 * R + r) * a.acc_stride + q] = acc139;
-  a.acc[(170 * R + r) * a.acc_stride + q] = acc170;
-  a.acc[(201 * R + r) * a.acc_stride + q] = acc201;
-  a.acc[(232 * R + r) * a.acc_stride + q] = acc232;
-  a.acc[(263 * R + r) * a.acc_stride + q] = acc263;
-  a.acc[(16 * R + r) * a.acc_stride + q] = acc16;
-  a.acc[(47 * R + r) * a.acc_stride + q] = acc47;
-  a.acc[(78 * R + r) * a.acc_stride + q] = acc78;
-  a.acc[(109 * R + r) * a.acc_stride + q] = acc109;
-  a.acc[(140 * R + r) * a.acc_stride + q] = acc140;
-  a.acc[(171 * R + r) * a.acc_stride + q] = acc171;
-  a.acc[(202 * R + r) * a.acc_stride + q] = acc202;
-  a.acc[(233 * R + r) * a.acc_stride + q] = acc233;
-  a.acc[(264 * R + r) * a.acc_stride + q] = acc264;
-  a.acc[(17 * R + r) * a.acc_stride + q] = acc17;
-  a.acc[(48 * R + r) * a.acc_stride + q] = acc48;
-  a.acc[(79 * R + r) * a.acc_stride + q] = acc79;
-  a.acc[(110 * R + r) * a.acc_stride + q] = acc110;
-  a.acc[(141 * R + r) * a.acc_stride + q] = acc141;
-  a.acc[(172 * R + r) * a.acc_stride + q] = acc172;
-  a.acc[(203 * R + r) * a.acc_stride + q] = acc203;
-  a.acc[(234 * R + r) * a.acc_stride + q] = acc234;
-  a.acc[(265 * R + r) * a.acc_stride + q] = acc265;
-  a.acc[(18 * R + r) * a.acc_stride + q] = acc18;
-  a.acc[(49 * R + r) * a.acc_stride + q] = acc49;
-  a.acc[(80 * R + r) * a.acc_stride + q] = acc80;
-  a.acc[(111 * R + r) * a.acc_stride + q] = acc111;
-  a.acc[(142 * R + r) * a.acc_stride + q] = acc142;
-  a.acc[(173 * R + r) * a.acc_stride + q] = acc173;
-  a.acc[(204 * R + r) * a.acc_stride + q] = acc204;
-  a.acc[(235 * R + r) * a.acc_stride + q] = acc235;
-  a.acc[(266 * R + r) * a.acc_stride + q] = acc266;
-  a.acc[(19 * R + r) * a.acc_stride + q] = acc19;
-  a.acc[(50 * R + r) * a.acc_stride + q] = acc50;
-  a.acc[(81 * R + r) * a.acc_stride + q] = acc81;
-  a.acc[(112 * R + r) * a.acc_stride + q] = acc112;
-  a.acc[(143 * R + r) * a.acc_stride + q] = acc143;
-  a.acc[(174 * R + r) * a.acc_stride + q] = acc174;
-  a.acc[(205 * R + r) * a.acc_stride + q] = acc205;
-  a.acc[(236 * R + r) * a.acc_stride + q] = acc236;
-  a.acc[(267 * R + r) * a.acc_stride + q] = acc267;
-  a.acc[(20 * R + r) * a.acc_stride + q] = acc20;
-  a.acc[(51 * R + r) * a.acc_stride + q] = acc51;
-  a.acc[(82 * R + r) * a.acc_stride + q] = acc82;
-  a.acc[(113 * R + r) * a.acc_stride + q] = acc113;
-  a.acc[(144 * R + r) * a.acc_stride + q] = acc144;
-  a.acc[(175 * R + r) * a.acc_stride + q] = acc175;
-}
-// class (22|11) split 2/3: 270 recursion values, 93 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2211_2(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc206 = 0.0;
-  double acc237 = 0.0;
-  double acc268 = 0.0;
-  double acc21 = 0.0;
-  double acc52 = 0.0;
-  double acc83 = 0.0;
-  double acc114 = 0.0;
-  double acc145 = 0.0;
-  double acc176 = 0.0;
-  double acc207 = 0.0;
-  double acc238 = 0.0;
-  double acc269 = 0.0;
-  double acc22 = 0.0;
-  double acc53 = 0.0;
-  double acc84 = 0.0;
-  double acc115 = 0.0;
-  double acc146 = 0.0;
-  double acc177 = 0.0;
-  double acc208 = 0.0;
-  double acc239 = 0.0;
-  double acc270 = 0.0;
-  double acc23 = 0.0;
-  double acc54 = 0.0;
-  double acc85 = 0.0;
-  double acc116 = 0.0;
-  double acc147 = 0.0;
-  double acc178 = 0.0;
-  double acc209 = 0.0;
-  double acc240 = 0.0;
-  double acc271 = 0.0;
-  double acc24 = 0.0;
-  double acc55 = 0.0;
-  double acc86 = 0.0;
-  double acc117 = 0.0;
-  double acc148 = 0.0;
-  double acc179 = 0.0;
-  double acc210 = 0.0;
-  double acc241 = 0.0;
-  double acc272 = 0.0;
-  double acc25 = 0.0;
-  double acc56 = 0.0;
-  double acc87 = 0.0;
-  double acc118 = 0.0;
-  double acc149 = 0.0;
-  double acc180 = 0.0;
-  double acc211 = 0.0;
-  double acc242 = 0.0;
-  double acc273 = 0.0;
-  double acc26 = 0.0;
-  double acc57 = 0.0;
-  double acc88 = 0.0;
-  double acc119 = 0.0;
-  double acc150 = 0.0;
-  double acc181 = 0.0;
-  double acc212 = 0.0;
-  double acc243 = 0.0;
-  double acc274 = 0.0;
-  double acc27 = 0.0;
-  double acc58 = 0.0;
-  double acc89 = 0.0;
-  double acc120 = 0.0;
-  double acc151 = 0.0;
-  double acc182 = 0.0;
-  double acc213 = 0.0;
-  double acc244 = 0.0;
-  double acc275 = 0.0;
-  double acc28 = 0.0;
-  double acc59 = 0.0;
-  double acc90 = 0.0;
-  double acc121 = 0.0;
-  double acc152 = 0.0;
-  double acc183 = 0.0;
-  double acc214 = 0.0;
-  double acc245 = 0.0;
-  double acc276 = 0.0;
-  double acc29 = 0.0;
-  double acc60 = 0.0;
-  double acc91 = 0.0;
-  double acc122 = 0.0;
-  double acc153 = 0.0;
-  double acc184 = 0.0;
-  double acc215 = 0.0;
-  double acc246 = 0.0;
-  double acc277 = 0.0;
-  double acc30 = 0.0;
-  double acc61 = 0.0;
-  double acc92 = 0.0;
-  double acc123 = 0.0;
-  double acc154 = 0.0;
-  double acc185 = 0.0;
-  double acc216 = 0.0;
-  double acc247 = 0.0;
-  double acc278 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x8 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x20 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x21 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x22 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x23 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x24 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x54 = s.PA[2] * x21 + s.WP[2] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x25 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x27 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x28 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x31 = s.PA[0] * x10 + s.WP[0] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x64 = s.PA[2] * x30 + s.WP[2] * x36 + z2 * (x9 - s.roz * x12);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x37 = s.PA[0] * x13 + s.WP[0] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x74 = s.PA[2] * x36 + s.WP[2] * x42 + z2 * (x12 - s.roz * x15);
-      const TB x109 = s.PA[2] * x64 + s.WP[2] * x74 + z3 * (x30 - s.roz * x36);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x43 = s.PA[0] * x16 + s.WP[0] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x45 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x46 = s.PA[1] * x17 + s.WP[1] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x48 = s.PA[2] * x18 + s.WP[2] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x94 = s.PA[2] * x48 + s.WP[2] * x54 + z2 * (x18 - s.roz * x21);
-      const TB x84 = s.PA[2] * x42 + s.WP[2] * x48 + z2 * (x15 - s.roz * x18);
-      const TB x139 = s.PA[2] * x84 + s.WP[2] * x94 + z3 * (x42 - s.roz * x48);
-      const TB x124 = s.PA[2] * x74 + s.WP[2] * x84 + z3 * (x36 - s.roz * x42);
-      const TB x47 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x49 = s.PA[0] * x19 + s.WP[0] * x22 + z1 * (x4 - s.roz * x5);
-      const TB x52 = s.PA[1] * x20 + s.WP[1] * x23 + z1 * (x4 - s.roz * x5);
-      const TB x53 = s.PA[2] * x20 + s.WP[2] * x23;
-      const TB x93 = s.PA[2] * x47 + s.WP[2] * x53 + z1 * (x17 - s.roz * x20);
-      const TB x91 = s.PA[1] * x46 + s.WP[1] * x52 + z2 * (x17 - s.roz * x20);
-      const TB x92 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x50 = s.PA[1] * x19 + s.WP[1] * x22;
-      const TB x51 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x90 = s.PA[2] * x45 + s.WP[2] * x51 + z1 * (x16 - s.roz * x19);
-      const TB x88 = s.PA[1] * x44 + s.WP[1] * x50 + z1 * (x16 - s.roz * x19);
-      const TB x89 = s.PA[2] * x44 + s.WP[2] * x50;
-      const TB x56 = s.PA[1] * x25 + s.WP[1] * x31;
-      const TB x57 = s.PA[2] * x25 + s.WP[2] * x31;
-      const TB x58 = s.PA[1] * x26 + s.WP[1] * x32 + z1 * (x7 - s.roz * x10);
-      const TB x60 = s.PA[2] * x27 + s.WP[2] * x33 + z1 * (x7 - s.roz * x10);
-      const TB x59 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x61 = s.PA[1] * x28 + s.WP[1] * x34 + z2 * (x8 - s.roz * x11);
-      const TB x63 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x8 - s.roz * x11);
-      const TB x62 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x66 = s.PA[1] * x31 + s.WP[1] * x37;
-      const TB x99 = s.PA[2] * x56 + s.WP[2] * x66;
-      const TB x67 = s.PA[2] * x31 + s.WP[2] * x37;
-      const TB x100 = s.PA[2] * x57 + s.WP[2] * x67 + z1 * (x25 - s.roz * x31);
-      const TB x68 = s.PA[1] * x32 + s.WP[1] * x38 + z1 * (x10 - s.roz * x13);
-      const TB x70 = s.PA[2] * x33 + s.WP[2] * x39 + z1 * (x10 - s.roz * x13);
-      const TB x104 = s.PA[2] * x60 + s.WP[2] * x70 + z2 * (x27 - s.roz * x33);
-      const TB x69 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x103 = s.PA[2] * x59 + s.WP[2] * x69 + z1 * (x26 - s.roz * x32);
-      const TB x101 = s.PA[1] * x58 + s.WP[1] * x68 + z2 * (x26 - s.roz * x32);
-      const TB x102 = s.PA[2] * x58 + s.WP[2] * x68;
-      const TB x71 = s.PA[1] * x34 + s.WP[1] * x40 + z2 * (x11 - s.roz * x14);
-      const TB x73 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x11 - s.roz * x14);
-      const TB x108 = s.PA[2] * x63 + s.WP[2] * x73 + z2 * (x29 - s.roz * x35);
-      const TB x72 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x107 = s.PA[2] * x62 + s.WP[2] * x72 + z1 * (x28 - s.roz * x34);
-      const TB x105 = s.PA[1] * x61 + s.WP[1] * x71 + z3 * (x28 - s.roz * x34);
-      const TB x106 = s.PA[2] * x61 + s.WP[2] * x71;
-      const TB x76 = s.PA[1] * x37 + s.WP[1] * x43;
-      const TB x77 = s.PA[2] * x37 + s.WP[2] * x43;
-      const TB x115 = s.PA[2] * x67 + s.WP[2] * x77 + z1 * (x31 - s.roz * x37);
-      const TB x78 = s.PA[1] * x38 + s.WP[1] * x44 + z1 * (x13 - s.roz * x16);
-      const TB x80 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x13 - s.roz * x16);
-      const TB x134 = s.PA[2] * x80 + s.WP[2] * x90 + z2 * (x39 - s.roz * x45);
-      const TB x119 = s.PA[2] * x70 + s.WP[2] * x80 + z2 * (x33 - s.roz * x39);
-      const TB x79 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x133 = s.PA[2] * x79 + s.WP[2] * x89 + z1 * (x38 - s.roz * x44);
-      const TB x131 = s.PA[1] * x78 + s.WP[1] * x88 + z2 * (x38 - s.roz * x44);
-      const TB x132 = s.PA[2] * x78 + s.WP[2] * x88;
-      const TB x81 = s.PA[1] * x40 + s.WP[1] * x46 + z2 * (x14 - s.roz * x17);
-      const TB x83 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x14 - s.roz * x17);
-      const TB x138 = s.PA[2] * x83 + s.WP[2] * x93 + z2 * (x41 - s.roz * x47);
-      const TB x123 = s.PA[2] * x73 + s.WP[2] * x83 + z2 * (x35 - s.roz * x41);
-      const TB x82 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x137 = s.PA[2] * x82 + s.WP[2] * x92 + z1 * (x40 - s.roz * x46);
-      const TB x135 = s.PA[1] * x81 + s.WP[1] * x91 + z3 * (x40 - s.roz * x46);
-      const TB x136 = s.PA[2] * x81 + s.WP[2] * x91;
-      const TB x86 = s.PA[1] * x43 + s.WP[1] * x49;
-      const TB x87 = s.PA[2] * x43 + s.WP[2] * x49;
-      const TB x130 = s.PA[2] * x77 + s.WP[2] * x87 + z1 * (x37 - s.roz * x43);
-      const TB x129 = s.PA[2] * x76 + s.WP[2] * x86;
-      const TB x114 = s.PA[2] * x66 + s.WP[2] * x76;
-      const TB x116 = s.PA[1] * x68 + s.WP[1] * x78 + z2 * (x32 - s.roz * x38);
-      const TB x118 = s.PA[2] * x69 + s.WP[2] * x79 + z1 * (x32 - s.roz * x38);
-      const TB x117 = s.PA[2] * x68 + s.WP[2] * x78;
-      const TB x120 = s.PA[1] * x71 + s.WP[1] * x81 + z3 * (x34 - s.roz * x40);
-      const TB x122 = s.PA[2] * x72 + s.WP[2] * x82 + z1 * (x34 - s.roz * x40);
-      const TB x121 = s.PA[2] * x71 + s.WP[2] * x81;
-      const TB x160 = s.QC[0] * x67 + s.WQ[0] * x77 + r2 * x39;
-      const TB x162 = s.QC[0] * x68 + s.WQ[0] * x78 + r1 * x40;
-      const TB x164 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x41;
-      const TB x166 = s.QC[0] * x70 + s.WQ[0] * x80 + r1 * x42;
-      const TB x168 = s.QC[0] * x71 + s.WQ[0] * x81;
-      const TB x170 = s.QC[0] * x72 + s.WQ[0] * x82;
-      const TB x172 = s.QC[0] * x73 + s.WQ[0] * x83;
-      const TB x174 = s.QC[0] * x74 + s.WQ[0] * x84;
-      const TB x185 = s.QC[0] * x100 + s.WQ[0] * x115 + r2 * x70;
-      acc21 += static_cast<double>(x185);
-      const TB x186 = s.QC[0] * x115 + s.WQ[0] * x130 + r2 * x80;
-      const TB x418 = s.QC[2] * x185 + s.WQ[2] * x186 + r2 * x160;
-      acc176 += static_cast<double>(x418);
-      const TB x356 = s.QC[0] * x185 + s.WQ[0] * x186 + e1 * (x100 - s.roe * x115) + r2 * x166;
-      acc114 += static_cast<double>(x356);
-      const TB x387 = s.QC[1] * x185 + s.WQ[1] * x186;
-      acc145 += static_cast<double>(x387);
-      const TB x187 = s.QC[0] * x101 + s.WQ[0] * x116 + r1 * x71;
-      acc22 += static_cast<double>(x187);
-      const TB x188 = s.QC[0] * x116 + s.WQ[0] * x131 + r1 * x81;
-      const TB x357 = s.QC[0] * x187 + s.WQ[0] * x188 + e1 * (x101 - s.roe * x116) + r1 * x168;
-      acc115 += static_cast<double>(x357);
-      const TB x388 = s.QC[1] * x187 + s.WQ[1] * x188 + r3 * x162;
-      acc146 += static_cast<double>(x388);
-      const TB x419 = s.QC[2] * x187 + s.WQ[2] * x188;
-      acc177 += static_cast<double>(x419);
-      const TB x189 = s.QC[0] * x102 + s.WQ[0] * x117 + r1 * x72;
-      acc23 += static_cast<double>(x189);
-      const TB x190 = s.QC[0] * x117 + s.WQ[0] * x132 + r1 * x82;
-      const TB x420 = s.QC[2] * x189 + s.WQ[2] * x190 + r1 * x162;
-      acc178 += static_cast<double>(x420);
-      const TB x358 = s.QC[0] * x189 + s.WQ[0] * x190 + e1 * (x102 - s.roe * x117) + r1 * x170;
-      acc116 += static_cast<double>(x358);
-      const TB x389 = s.QC[1] * x189 + s.WQ[1] * x190 + r2 * x164;
-      acc147 += static_cast<double>(x389);
-      const TB x191 = s.QC[0] * x103 + s.WQ[0] * x118 + r1 * x73;
-      acc24 += static_cast<double>(x191);
-      const TB x192 = s.QC[0] * x118 + s.WQ[0] * x133 + r1 * x83;
-      const TB x421 = s.QC[2] * x191 + s.WQ[2] * x192 + r2 * x164;
-      acc179 += static_cast<double>(x421);
-      const TB x359 = s.QC[0] * x191 + s.WQ[0] * x192 + e1 * (x103 - s.roe * x118) + r1 * x172;
-      acc117 += static_cast<double>(x359);
-      const TB x390 = s.QC[1] * x191 + s.WQ[1] * x192 + r1 * x166;
-      acc148 += static_cast<double>(x390);
-      const TB x193 = s.QC[0] * x104 + s.WQ[0] * x119 + r1 * x74;
-      acc25 += static_cast<double>(x193);
-      const TB x194 = s.QC[0] * x119 + s.WQ[0] * x134 + r1 * x84;
-      const TB x422 = s.QC[2] * x193 + s.WQ[2] * x194 + r3 * x166;
-      acc180 += static_cast<double>(x422);
-      const TB x360 = s.QC[0] * x193 + s.WQ[0] * x194 + e1 * (x104 - s.roe * x119) + r1 * x174;
-      acc118 += static_cast<double>(x360);
-      const TB x391 = s.QC[1] * x193 + s.WQ[1] * x194;
-      acc149 += static_cast<double>(x391);
-      const TB x195 = s.QC[0] * x105 + s.WQ[0] * x120;
-      acc26 += static_cast<double>(x195);
-      const TB x196 = s.QC[0] * x120 + s.WQ[0] * x135;
-      const TB x361 = s.QC[0] * x195 + s.WQ[0] * x196 + e1 * (x105 - s.roe * x120);
-      acc119 += static_cast<double>(x361);
-      const TB x392 = s.QC[1] * x195 + s.WQ[1] * x196 + r4 * x168;
-      acc150 += static_cast<double>(x392);
-      const TB x423 = s.QC[2] * x195 + s.WQ[2] * x196;
-      acc181 += static_cast<double>(x423);
-      const TB x197 = s.QC[0] * x106 + s.WQ[0] * x121;
-      acc27 += static_cast<double>(x197);
-      const TB x198 = s.QC[0] * x121 + s.WQ[0] * x136;
-      const TB x424 = s.QC[2] * x197 + s.WQ[2] * x198 + r1 * x168;
-      acc182 += static_cast<double>(x424);
-      const TB x362 = s.QC[0] * x197 + s.WQ[0] * x198 + e1 * (x106 - s.roe * x121);
-      acc120 += static_cast<double>(x362);
-      const TB x393 = s.QC[1] * x197 + s.WQ[1] * x198 + r3 * x170;
-      acc151 += static_cast<double>(x393);
-      const TB x199 = s.QC[0] * x107 + s.WQ[0] * x122;
-      acc28 += static_cast<double>(x199);
-      const TB x200 = s.QC[0] * x122 + s.WQ[0] * x137;
-      const TB x425 = s.QC[2] * x199 + s.WQ[2] * x200 + r2 * x170;
-      acc183 += static_cast<double>(x425);
-      const TB x363 = s.QC[0] * x199 + s.WQ[0] * x200 + e1 * (x107 - s.roe * x122);
-      acc121 += static_cast<double>(x363);
-      const TB x394 = s.QC[1] * x199 + s.WQ[1] * x200 + r2 * x172;
-      acc152 += static_cast<double>(x394);
-      const TB x201 = s.QC[0] * x108 + s.WQ[0] * x123;
-      acc29 += static_cast<double>(x201);
-      const TB x202 = s.QC[0] * x123 + s.WQ[0] * x138;
-      const TB x426 = s.QC[2] * x201 + s.WQ[2] * x202 + r3 * x172;
-      acc184 += static_cast<double>(x426);
-      const TB x364 = s.QC[0] * x201 + s.WQ[0] * x202 + e1 * (x108 - s.roe * x123);
-      acc122 += static_cast<double>(x364);
-      const TB x395 = s.QC[1] * x201 + s.WQ[1] * x202 + r1 * x174;
-      acc153 += static_cast<double>(x395);
-      const TB x203 = s.QC[0] * x109 + s.WQ[0] * x124;
-      acc30 += static_cast<double>(x203);
-      const TB x204 = s.QC[0] * x124 + s.WQ[0] * x139;
-      const TB x427 = s.QC[2] * x203 + s.WQ[2] * x204 + r4 * x174;
-      acc185 += static_cast<double>(x427);
-      const TB x365 = s.QC[0] * x203 + s.WQ[0] * x204 + e1 * (x109 - s.roe * x124);
-      acc123 += static_cast<double>(x365);
-      const TB x396 = s.QC[1] * x203 + s.WQ[1] * x204;
-      acc154 += static_cast<double>(x396);
-      const TB x223 = s.QC[1] * x66 + s.WQ[1] * x76 + r1 * x37;
-      const TB x290 = s.QC[2] * x67 + s.WQ[2] * x77 + r1 * x37;
-      const TB x225 = s.QC[1] * x67 + s.WQ[1] * x77;
-      const TB x227 = s.QC[1] * x68 + s.WQ[1] * x78 + r2 * x38;
-      const TB x294 = s.QC[2] * x69 + s.WQ[2] * x79 + r1 * x38;
-      const TB x229 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x39;
-      const TB x296 = s.QC[2] * x70 + s.WQ[2] * x80 + r2 * x39;
-      const TB x231 = s.QC[1] * x70 + s.WQ[1] * x80;
-      const TB x233 = s.QC[1] * x71 + s.WQ[1] * x81 + r3 * x40;
-      const TB x300 = s.QC[2] * x72 + s.WQ[2] * x82 + r1 * x40;
-      const TB x235 = s.QC[1] * x72 + s.WQ[1] * x82 + r2 * x41;
-      const TB x302 = s.QC[2] * x73 + s.WQ[2] * x83 + r2 * x41;
-      const TB x237 = s.QC[1] * x73 + s.WQ[1] * x83 + r1 * x42;
-      const TB x304 = s.QC[2] * x74 + s.WQ[2] * x84 + r3 * x42;
-      const TB x239 = s.QC[1] * x74 + s.WQ[1] * x84;
-      const TB x248 = s.QC[1] * x99 + s.WQ[1] * x114 + r1 * x67;
-      const TB x315 = s.QC[2] * x100 + s.WQ[2] * x115 + r2 * x67;
-      acc83 += static_cast<double>(x315);
-      const TB x249 = s.QC[1] * x114 + s.WQ[1] * x129 + r1 * x77;
-      const TB x479 = s.QC[2] * x248 + s.WQ[2] * x249 + r1 * x223;
-      acc237 += static_cast<double>(x479);
-      const TB x448 = s.QC[1] * x248 + s.WQ[1] * x249 + e1 * (x99 - s.roe * x114) + r1 * x225;
-      acc206 += static_cast<double>(x448);
-      const TB x314 = s.QC[2] * x114 + s.WQ[2] * x129 + r1 * x76;
-      const TB x288 = s.QC[2] * x66 + s.WQ[2] * x76;
-      const TB x313 = s.QC[2] * x99 + s.WQ[2] * x114 + r1 * x66;
-      const TB x510 = s.QC[2] * x313 + s.WQ[2] * x314 + e1 * (x99 - s.roe * x114) + r1 * x288;
-      acc268 += static_cast<double>(x510);
-      const TB x316 = s.QC[2] * x115 + s.WQ[2] * x130 + r2 * x77;
-      const TB x511 = s.QC[2] * x315 + s.WQ[2] * x316 + e1 * (x100 - s.roe * x115) + r2 * x290;
-      acc269 += static_cast<double>(x511);
-      const TB x251 = s.QC[1] * x115 + s.WQ[1] * x130;
-      const TB x250 = s.QC[1] * x100 + s.WQ[1] * x115;
-      acc52 += static_cast<double>(x250);
-      const TB x449 = s.QC[1] * x250 + s.WQ[1] * x251 + e1 * (x100 - s.roe * x115);
-      acc207 += static_cast<double>(x449);
-      const TB x480 = s.QC[2] * x250 + s.WQ[2] * x251 + r2 * x225;
-      acc238 += static_cast<double>(x480);
-      const TB x252 = s.QC[1] * x101 + s.WQ[1] * x116 + r3 * x68;
-      acc53 += static_cast<double>(x252);
-      const TB x253 = s.QC[1] * x116 + s.WQ[1] * x131 + r3 * x78;
-      const TB x318 = s.QC[2] * x116 + s.WQ[2] * x131;
-      const TB x450 = s.QC[1] * x252 + s.WQ[1] * x253 + e1 * (x101 - s.roe * x116) + r3 * x227;
-      acc208 += static_cast<double>(x450);
-      const TB x481 = s.QC[2] * x252 + s.WQ[2] * x253;
-      acc239 += static_cast<double>(x481);
-      const TB x254 = s.QC[1] * x102 + s.WQ[1] * x117 + r2 * x69;
-      acc54 += static_cast<double>(x254);
-      const TB x321 = s.QC[2] * x103 + s.WQ[2] * x118 + r2 * x69;
-      acc86 += static_cast<double>(x321);
-      const TB x255 = s.QC[1] * x117 + s.WQ[1] * x132 + r2 * x79;
-      const TB x482 = s.QC[2] * x254 + s.WQ[2] * x255 + r1 * x227;
-      acc240 += static_cast<double>(x482);
-      const TB x451 = s.QC[1] * x254 + s.WQ[1] * x255 + e1 * (x102 - s.roe * x117) + r2 * x229;
-      acc209 += static_cast<double>(x451);
-      const TB x320 = s.QC[2] * x117 + s.WQ[2] * x132 + r1 * x78;
-      const TB x292 = s.QC[2] * x68 + s.WQ[2] * x78;
-      const TB x319 = s.QC[2] * x102 + s.WQ[2] * x117 + r1 * x68;
-      acc85 += static_cast<double>(x319);
-      const TB x513 = s.QC[2] * x319 + s.WQ[2] * x320 + e1 * (x102 - s.roe * x117) + r1 * x292;
-      acc271 += static_cast<double>(x513);
-      const TB x322 = s.QC[2] * x118 + s.WQ[2] * x133 + r2 * x79;
-      const TB x514 = s.QC[2] * x321 + s.WQ[2] * x322 + e1 * (x103 - s.roe * x118) + r2 * x294;
-      acc272 += static_cast<double>(x514);
-      const TB x257 = s.QC[1] * x118 + s.WQ[1] * x133 + r1 * x80;
-      const TB x324 = s.QC[2] * x119 + s.WQ[2] * x134 + r3 * x80;
-      const TB x259 = s.QC[1] * x119 + s.WQ[1] * x134;
-      const TB x256 = s.QC[1] * x103 + s.WQ[1] * x118 + r1 * x70;
-      acc55 += static_cast<double>(x256);
-      const TB x452 = s.QC[1] * x256 + s.WQ[1] * x257 + e1 * (x103 - s.roe * x118) + r1 * x231;
-      acc210 += static_cast<double>(x452);
-      const TB x483 = s.QC[2] * x256 + s.WQ[2] * x257 + r2 * x229;
-      acc241 += static_cast<double>(x483);
-      const TB x323 = s.QC[2] * x104 + s.WQ[2] * x119 + r3 * x70;
-      acc87 += static_cast<double>(x323);
-      const TB x515 = s.QC[2] * x323 + s.WQ[2] * x324 + e1 * (x104 - s.roe * x119) + r3 * x296;
-      acc273 += static_cast<double>(x515);
-      const TB x258 = s.QC[1] * x104 + s.WQ[1] * x119;
-      acc56 += static_cast<double>(x258);
-      const TB x453 = s.QC[1] * x258 + s.WQ[1] * x259 + e1 * (x104 - s.roe * x119);
-      acc211 += static_cast<double>(x453);
-      const TB x484 = s.QC[2] * x258 + s.WQ[2] * x259 + r3 * x231;
-      acc242 += static_cast<double>(x484);
-      const TB x260 = s.QC[1] * x105 + s.WQ[1] * x120 + r4 * x71;
-      acc57 += static_cast<double>(x260);
-      const TB x261 = s.QC[1] * x120 + s.WQ[1] * x135 + r4 * x81;
-      const TB x326 = s.QC[2] * x120 + s.WQ[2] * x135;
-      const TB x454 = s.QC[1] * x260 + s.WQ[1] * x261 + e1 * (x105 - s.roe * x120) + r4 * x233;
-      acc212 += static_cast<double>(x454);
-      const TB x485 = s.QC[2] * x260 + s.WQ[2] * x261;
-      acc243 += static_cast<double>(x485);
-      const TB x262 = s.QC[1] * x106 + s.WQ[1] * x121 + r3 * x72;
-      acc58 += static_cast<double>(x262);
-      const TB x329 = s.QC[2] * x107 + s.WQ[2] * x122 + r2 * x72;
-      acc90 += static_cast<double>(x329);
-      const TB x263 = s.QC[1] * x121 + s.WQ[1] * x136 + r3 * x82;
-      const TB x486 = s.QC[2] * x262 + s.WQ[2] * x263 + r1 * x233;
-      acc244 += static_cast<double>(x486);
-      const TB x455 = s.QC[1] * x262 + s.WQ[1] * x263 + e1 * (x106 - s.roe * x121) + r3 * x235;
-      acc213 += static_cast<double>(x455);
-      const TB x328 = s.QC[2] * x121 + s.WQ[2] * x136 + r1 * x81;
-      const TB x298 = s.QC[2] * x71 + s.WQ[2] * x81;
-      const TB x327 = s.QC[2] * x106 + s.WQ[2] * x121 + r1 * x71;
-      acc89 += static_cast<double>(x327);
-      const TB x517 = s.QC[2] * x327 + s.WQ[2] * x328 + e1 * (x106 - s.roe * x121) + r1 * x298;
-      acc275 += static_cast<double>(x517);
-      const TB x330 = s.QC[2] * x122 + s.WQ[2] * x137 + r2 * x82;
-      const TB x518 = s.QC[2] * x329 + s.WQ[2] * x330 + e1 * (x107 - s.roe * x122) + r2 * x300;
-      acc276 += static_cast<double>(x518);
-      const TB x265 = s.QC[1] * x122 + s.WQ[1] * x137 + r2 * x83;
-      const TB x332 = s.QC[2] * x123 + s.WQ[2] * x138 + r3 * x83;
-      const TB x267 = s.QC[1] * x123 + s.WQ[1] * x138 + r1 * x84;
-      const TB x334 = s.QC[2] * x124 + s.WQ[2] * x139 + r4 * x84;
-      const TB x269 = s.QC[1] * x124 + s.WQ[1] * x139;
-      const TB x264 = s.QC[1] * x107 + s.WQ[1] * x122 + r2 * x73;
-      acc59 += static_cast<double>(x264);
-      const TB x456 = s.QC[1] * x264 + s.WQ[1] * x265 + e1 * (x107 - s.roe * x122) + r2 * x237;
-      acc214 += static_cast<double>(x456);
-      const TB x487 = s.QC[2] * x264 + s.WQ[2] * x265 + r2 * x235;
-      acc245 += static_cast<double>(x487);
-      const TB x331 = s.QC[2] * x108 + s.WQ[2] * x123 + r3 * x73;
-      acc91 += static_cast<double>(x331);
-      const TB x519 = s.QC[2] * x331 + s.WQ[2] * x332 + e1 * (x108 - s.roe * x123) + r3 * x302;
-      acc277 += static_cast<double>(x519);
-      const TB x266 = s.QC[1] * x108 + s.WQ[1] * x123 + r1 * x74;
-      acc60 += static_cast<double>(x266);
-      const TB x457 = s.QC[1] * x266 + s.WQ[1] * x267 + e1 * (x108 - s.roe * x123) + r1 * x239;
-      acc215 += static_cast<double>(x457);
-      const TB x488 = s.QC[2] * x266 + s.WQ[2] * x267 + r3 * x237;
-      acc246 += static_cast<double>(x488);
-      const TB x333 = s.QC[2] * x109 + s.WQ[2] * x124 + r4 * x74;
-      acc92 += static_cast<double>(x333);
-      const TB x520 = s.QC[2] * x333 + s.WQ[2] * x334 + e1 * (x109 - s.roe * x124) + r4 * x304;
-      acc278 += static_cast<double>(x520);
-      const TB x268 = s.QC[1] * x109 + s.WQ[1] * x124;
-      acc61 += static_cast<double>(x268);
-      const TB x458 = s.QC[1] * x268 + s.WQ[1] * x269 + e1 * (x109 - s.roe * x124);
-      acc216 += static_cast<double>(x458);
-      const TB x489 = s.QC[2] * x268 + s.WQ[2] * x269 + r4 * x239;
-      acc247 += static_cast<double>(x489);
-      const TB x317 = s.QC[2] * x101 + s.WQ[2] * x116;
-      acc84 += static_cast<double>(x317);
-      const TB x512 = s.QC[2] * x317 + s.WQ[2] * x318 + e1 * (x101 - s.roe * x116);
-      acc270 += static_cast<double>(x512);
-      const TB x325 = s.QC[2] * x105 + s.WQ[2] * x120;
-      acc88 += static_cast<double>(x325);
-      const TB x516 = s.QC[2] * x325 + s.WQ[2] * x326 + e1 * (x105 - s.roe * x120);
-      acc274 += static_cast<double>(x516);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(206 * R + r) * a.acc_stride + q] = acc206;
-  a.acc[(237 * R + r) * a.acc_stride + q] = acc237;
-  a.acc[(268 * R + r) * a.acc_stride + q] = acc268;
-  a.acc[(21 * R + r) * a.acc_stride + q] = acc21;
-  a.acc[(52 * R + r) * a.acc_stride + q] = acc52;
-  a.acc[(83 * R + r) * a.acc_stride + q] = acc83;
-  a.acc[(114 * R + r) * a.acc_stride + q] = acc114;
-  a.acc[(145 * R + r) * a.acc_stride + q] = acc145;
-  a.acc[(176 * R + r) * a.acc_stride + q] = acc176;
-  a.acc[(207 * R + r) * a.acc_stride + q] = acc207;
-  a.acc[(238 * R + r) * a.acc_stride + q] = acc238;
-  a.acc[(269 * R + r) * a.acc_stride + q] = acc269;
-  a.acc[(22 * R + r) * a.acc_stride + q] = acc22;
-  a.acc[(53 * R + r) * a.acc_stride + q] = acc53;
-  a.acc[(84 * R + r) * a.acc_stride + q] = acc84;
-  a.acc[(115 * R + r) * a.acc_stride + q] = acc115;
-  a.acc[(146 * R + r) * a.acc_stride + q] = acc146;
-  a.acc[(177 * R + r) * a.acc_stride + q] = acc177;
-  a.acc[(208 * R + r) * a.acc_stride + q] = acc208;
-  a.acc[(239 * R + r) * a.acc_stride + q] = acc239;
-  a.acc[(270 * R + r) * a.acc_stride + q] = acc270;
-  a.acc[(23 * R + r) * a.acc_stride + q] = acc23;
-  a.acc[(54 * R + r) * a.acc_stride + q] = acc54;
-  a.acc[(85 * R + r) * a.acc_stride + q] = acc85;
-  a.acc[(116 * R + r) * a.acc_stride + q] = acc116;
-  a.acc[(147 * R + r) * a.acc_stride + q] = acc147;
-  a.acc[(178 * R + r) * a.acc_stride + q] = acc178;
-  a.acc[(209 * R + r) * a.acc_stride + q] = acc209;
-  a.acc[(240 * R + r) * a.acc_stride + q] = acc240;
-  a.acc[(271 * R + r) * a.acc_stride + q] = acc271;
-  a.acc[(24 * R + r) * a.acc_stride + q] = acc24;
-  a.acc[(55 * R + r) * a.acc_stride + q] = acc55;
-  a.acc[(86 * R + r) * a.acc_stride + q] = acc86;
-  a.acc[(117 * R + r) * a.acc_stride + q] = acc117;
-  a.acc[(148 * R + r) * a.acc_stride + q] = acc148;
-  a.acc[(179 * R + r) * a.acc_stride + q] = acc179;
-  a.acc[(210 * R + r) * a.acc_stride + q] = acc210;
-  a.acc[(241 * R + r) * a.acc_stride + q] = acc241;
-  a.acc[(272 * R + r) * a.acc_stride + q] = acc272;
-  a.acc[(25 * R + r) * a.acc_stride + q] = acc25;
-  a.acc[(56 * R + r) * a.acc_stride + q] = acc56;
-  a.acc[(87 * R + r) * a.acc_stride + q] = acc87;
-  a.acc[(118 * R + r) * a.acc_stride + q] = acc118;
-  a.acc[(149 * R + r) * a.acc_stride + q] = acc149;
-  a.acc[(180 * R + r) * a.acc_stride + q] = acc180;
-  a.acc[(211 * R + r) * a.acc_stride + q] = acc211;
-  a.acc[(242 * R + r) * a.acc_stride + q] = acc242;
-  a.acc[(273 * R + r) * a.acc_stride + q] = acc273;
-  a.acc[(26 * R + r) * a.acc_stride + q] = acc26;
-  a.acc[(57 * R + r) * a.acc_stride + q] = acc57;
-  a.acc[(88 * R + r) * a.acc_stride + q] = acc88;
-  a.acc[(119 * R + r) * a.acc_stride + q] = acc119;
-  a.acc[(150 * R + r) * a.acc_stride + q] = acc150;
-  a.acc[(181 * R + r) * a.acc_stride + q] = acc181;
-  a.acc[(212 * R + r) * a.acc_stride + q] = acc212;
-  a.acc[(243 * R + r) * a.acc_stride + q] = acc243;
-  a.acc[(274 * R + r) * a.acc_stride + q] = acc274;
-  a.acc[(27 * R + r) * a.acc_stride + q] = acc27;
-  a.acc[(58 * R + r) * a.acc_stride + q] = acc58;
-  a.acc[(89 * R + r) * a.acc_stride + q] = acc89;
-  a.acc[(120 * R + r) * a.acc_stride + q] = acc120;
-  a.acc[(151 * R + r) * a.acc_stride + q] = acc151;
-  a.acc[(182 * R + r) * a.acc_stride + q] = acc182;
-  a.acc[(213 * R + r) * a.acc_stride + q] = acc213;
-  a.acc[(244 * R + r) * a.acc_stride + q] = acc244;
-  a.acc[(275 * R + r) * a.acc_stride + q] = acc275;
-  a.acc[(28 * R + r) * a.acc_stride + q] = acc28;
-  a.acc[(59 * R + r) * a.acc_stride + q] = acc59;
-  a.acc[(90 * R + r) * a.acc_stride + q] = acc90;
-  a.acc[(121 * R + r) * a.acc_stride + q] = acc121;
-  a.acc[(152 * R + r) * a.acc_stride + q] = acc152;
-  a.acc[(183 * R + r) * a.acc_stride + q] = acc183;
-  a.acc[(214 * R + r) * a.acc_stride + q] = acc214;
-  a.acc[(245 * R + r) * a.acc_stride + q] = acc245;
-  a.acc[(276 * R + r) * a.acc_stride + q] = acc276;
-  a.acc[(29 * R + r) * a.acc_stride + q] = acc29;
-  a.acc[(60 * R + r) * a.acc_stride + q] = acc60;
-  a.acc[(91 * R + r) * a.acc_stride + q] = acc91;
-  a.acc[(122 * R + r) * a.acc_stride + q] = acc122;
-  a.acc[(153 * R + r) * a.acc_stride + q] = acc153;
-  a.acc[(184 * R + r) * a.acc_stride + q] = acc184;
-  a.acc[(215 * R + r) * a.acc_stride + q] = acc215;
-  a.acc[(246 * R + r) * a.acc_stride + q] = acc246;
-  a.acc[(277 * R + r) * a.acc_stride + q] = acc277;
-  a.acc[(30 * R + r) * a.acc_stride + q] = acc30;
-  a.acc[(61 * R + r) * a.acc_stride + q] = acc61;
-  a.acc[(92 * R + r) * a.acc_stride + q] = acc92;
-  a.acc[(123 * R + r) * a.acc_stride + q] = acc123;
-  a.acc[(154 * R + r) * a.acc_stride + q] = acc154;
-  a.acc[(185 * R + r) * a.acc_stride + q] = acc185;
-  a.acc[(216 * R + r) * a.acc_stride + q] = acc216;
-  a.acc[(247 * R + r) * a.acc_stride + q] = acc247;
-  a.acc[(278 * R + r) * a.acc_stride + q] = acc278;
-}
-// class (22|11): horizontal recursion per ket column, 279 accumulators
-__global__ __launch_bounds__(kRegThreads) void eri_hrr_2211(gbx_tables t, RegArgs a) {
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  ColumnStore<6, 6, 3, 3> st(t, a, bra, ket, pb, pk);
-  {
-    const ColumnSlot cl = st.column(0, 0);
-    const double v0 = sum_parts(a, 109, q);
-    const double v1 = sum_parts(a, 16, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 99, q);
-    const double v4 = sum_parts(a, 6, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 93, q);
-    const double v8 = sum_parts(a, 0, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 110, q);
-    const double v13 = sum_parts(a, 17, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 100, q);
-    const double v16 = sum_parts(a, 7, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 111, q);
-    const double v21 = sum_parts(a, 18, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 101, q);
-    const double v24 = sum_parts(a, 8, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 112, q);
-    const double v29 = sum_parts(a, 19, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 113, q);
-    const double v35 = sum_parts(a, 20, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 114, q);
-    const double v40 = sum_parts(a, 21, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 94, q);
-    const double v46 = sum_parts(a, 1, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 102, q);
-    const double v51 = sum_parts(a, 9, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 103, q);
-    const double v56 = sum_parts(a, 10, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 115, q);
-    const double v61 = sum_parts(a, 22, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 116, q);
-    const double v67 = sum_parts(a, 23, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 117, q);
-    const double v72 = sum_parts(a, 24, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 95, q);
-    const double v78 = sum_parts(a, 2, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 104, q);
-    const double v84 = sum_parts(a, 11, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 118, q);
-    const double v93 = sum_parts(a, 25, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 96, q);
-    const double v99 = sum_parts(a, 3, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 105, q);
-    const double v104 = sum_parts(a, 12, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 106, q);
-    const double v109 = sum_parts(a, 13, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 119, q);
-    const double v114 = sum_parts(a, 26, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 120, q);
-    const double v120 = sum_parts(a, 27, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 121, q);
-    const double v125 = sum_parts(a, 28, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 97, q);
-    const double v131 = sum_parts(a, 4, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 107, q);
-    const double v137 = sum_parts(a, 14, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 122, q);
-    const double v146 = sum_parts(a, 29, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 98, q);
-    const double v152 = sum_parts(a, 5, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 108, q);
-    const double v158 = sum_parts(a, 15, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 123, q);
-    const double v167 = sum_parts(a, 30, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(0, 1);
-    const double v0 = sum_parts(a, 140, q);
-    const double v1 = sum_parts(a, 16, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 130, q);
-    const double v4 = sum_parts(a, 6, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 124, q);
-    const double v8 = sum_parts(a, 0, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 141, q);
-    const double v13 = sum_parts(a, 17, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 131, q);
-    const double v16 = sum_parts(a, 7, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 142, q);
-    const double v21 = sum_parts(a, 18, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 132, q);
-    const double v24 = sum_parts(a, 8, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 143, q);
-    const double v29 = sum_parts(a, 19, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 144, q);
-    const double v35 = sum_parts(a, 20, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 145, q);
-    const double v40 = sum_parts(a, 21, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 125, q);
-    const double v46 = sum_parts(a, 1, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 133, q);
-    const double v51 = sum_parts(a, 9, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 134, q);
-    const double v56 = sum_parts(a, 10, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 146, q);
-    const double v61 = sum_parts(a, 22, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 147, q);
-    const double v67 = sum_parts(a, 23, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 148, q);
-    const double v72 = sum_parts(a, 24, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 126, q);
-    const double v78 = sum_parts(a, 2, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 135, q);
-    const double v84 = sum_parts(a, 11, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 149, q);
-    const double v93 = sum_parts(a, 25, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 127, q);
-    const double v99 = sum_parts(a, 3, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 136, q);
-    const double v104 = sum_parts(a, 12, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 137, q);
-    const double v109 = sum_parts(a, 13, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 150, q);
-    const double v114 = sum_parts(a, 26, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 151, q);
-    const double v120 = sum_parts(a, 27, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 152, q);
-    const double v125 = sum_parts(a, 28, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 128, q);
-    const double v131 = sum_parts(a, 4, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 138, q);
-    const double v137 = sum_parts(a, 14, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 153, q);
-    const double v146 = sum_parts(a, 29, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 129, q);
-    const double v152 = sum_parts(a, 5, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 139, q);
-    const double v158 = sum_parts(a, 15, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 154, q);
-    const double v167 = sum_parts(a, 30, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(0, 2);
-    const double v0 = sum_parts(a, 171, q);
-    const double v1 = sum_parts(a, 16, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 161, q);
-    const double v4 = sum_parts(a, 6, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 155, q);
-    const double v8 = sum_parts(a, 0, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 172, q);
-    const double v13 = sum_parts(a, 17, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 162, q);
-    const double v16 = sum_parts(a, 7, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 173, q);
-    const double v21 = sum_parts(a, 18, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 163, q);
-    const double v24 = sum_parts(a, 8, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 174, q);
-    const double v29 = sum_parts(a, 19, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 175, q);
-    const double v35 = sum_parts(a, 20, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 176, q);
-    const double v40 = sum_parts(a, 21, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 156, q);
-    const double v46 = sum_parts(a, 1, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 164, q);
-    const double v51 = sum_parts(a, 9, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 165, q);
-    const double v56 = sum_parts(a, 10, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 177, q);
-    const double v61 = sum_parts(a, 22, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 178, q);
-    const double v67 = sum_parts(a, 23, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 179, q);
-    const double v72 = sum_parts(a, 24, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 157, q);
-    const double v78 = sum_parts(a, 2, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 166, q);
-    const double v84 = sum_parts(a, 11, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 180, q);
-    const double v93 = sum_parts(a, 25, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 158, q);
-    const double v99 = sum_parts(a, 3, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 167, q);
-    const double v104 = sum_parts(a, 12, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 168, q);
-    const double v109 = sum_parts(a, 13, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 181, q);
-    const double v114 = sum_parts(a, 26, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 182, q);
-    const double v120 = sum_parts(a, 27, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 183, q);
-    const double v125 = sum_parts(a, 28, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 159, q);
-    const double v131 = sum_parts(a, 4, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 169, q);
-    const double v137 = sum_parts(a, 14, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 184, q);
-    const double v146 = sum_parts(a, 29, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 160, q);
-    const double v152 = sum_parts(a, 5, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 170, q);
-    const double v158 = sum_parts(a, 15, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 185, q);
-    const double v167 = sum_parts(a, 30, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 0);
-    const double v0 = sum_parts(a, 140, q);
-    const double v1 = sum_parts(a, 47, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 130, q);
-    const double v4 = sum_parts(a, 37, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 124, q);
-    const double v8 = sum_parts(a, 31, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 141, q);
-    const double v13 = sum_parts(a, 48, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 131, q);
-    const double v16 = sum_parts(a, 38, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 142, q);
-    const double v21 = sum_parts(a, 49, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 132, q);
-    const double v24 = sum_parts(a, 39, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 143, q);
-    const double v29 = sum_parts(a, 50, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 144, q);
-    const double v35 = sum_parts(a, 51, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 145, q);
-    const double v40 = sum_parts(a, 52, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 125, q);
-    const double v46 = sum_parts(a, 32, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 133, q);
-    const double v51 = sum_parts(a, 40, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 134, q);
-    const double v56 = sum_parts(a, 41, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 146, q);
-    const double v61 = sum_parts(a, 53, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 147, q);
-    const double v67 = sum_parts(a, 54, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 148, q);
-    const double v72 = sum_parts(a, 55, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 126, q);
-    const double v78 = sum_parts(a, 33, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 135, q);
-    const double v84 = sum_parts(a, 42, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 149, q);
-    const double v93 = sum_parts(a, 56, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 127, q);
-    const double v99 = sum_parts(a, 34, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 136, q);
-    const double v104 = sum_parts(a, 43, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 137, q);
-    const double v109 = sum_parts(a, 44, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 150, q);
-    const double v114 = sum_parts(a, 57, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 151, q);
-    const double v120 = sum_parts(a, 58, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 152, q);
-    const double v125 = sum_parts(a, 59, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 128, q);
-    const double v131 = sum_parts(a, 35, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 138, q);
-    const double v137 = sum_parts(a, 45, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 153, q);
-    const double v146 = sum_parts(a, 60, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 129, q);
-    const double v152 = sum_parts(a, 36, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 139, q);
-    const double v158 = sum_parts(a, 46, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 154, q);
-    const double v167 = sum_parts(a, 61, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 1);
-    const double v0 = sum_parts(a, 202, q);
-    const double v1 = sum_parts(a, 47, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 192, q);
-    const double v4 = sum_parts(a, 37, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 186, q);
-    const double v8 = sum_parts(a, 31, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 203, q);
-    const double v13 = sum_parts(a, 48, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 193, q);
-    const double v16 = sum_parts(a, 38, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 204, q);
-    const double v21 = sum_parts(a, 49, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 194, q);
-    const double v24 = sum_parts(a, 39, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 205, q);
-    const double v29 = sum_parts(a, 50, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 206, q);
-    const double v35 = sum_parts(a, 51, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 207, q);
-    const double v40 = sum_parts(a, 52, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 187, q);
-    const double v46 = sum_parts(a, 32, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 195, q);
-    const double v51 = sum_parts(a, 40, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 196, q);
-    const double v56 = sum_parts(a, 41, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 208, q);
-    const double v61 = sum_parts(a, 53, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 209, q);
-    const double v67 = sum_parts(a, 54, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 210, q);
-    const double v72 = sum_parts(a, 55, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 188, q);
-    const double v78 = sum_parts(a, 33, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 197, q);
-    const double v84 = sum_parts(a, 42, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 211, q);
-    const double v93 = sum_parts(a, 56, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 189, q);
-    const double v99 = sum_parts(a, 34, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 198, q);
-    const double v104 = sum_parts(a, 43, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 199, q);
-    const double v109 = sum_parts(a, 44, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 212, q);
-    const double v114 = sum_parts(a, 57, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 213, q);
-    const double v120 = sum_parts(a, 58, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 214, q);
-    const double v125 = sum_parts(a, 59, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 190, q);
-    const double v131 = sum_parts(a, 35, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 200, q);
-    const double v137 = sum_parts(a, 45, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 215, q);
-    const double v146 = sum_parts(a, 60, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 191, q);
-    const double v152 = sum_parts(a, 36, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 201, q);
-    const double v158 = sum_parts(a, 46, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 216, q);
-    const double v167 = sum_parts(a, 61, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 2);
-    const double v0 = sum_parts(a, 233, q);
-    const double v1 = sum_parts(a, 47, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 223, q);
-    const double v4 = sum_parts(a, 37, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 217, q);
-    const double v8 = sum_parts(a, 31, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 234, q);
-    const double v13 = sum_parts(a, 48, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 224, q);
-    const double v16 = sum_parts(a, 38, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 235, q);
-    const double v21 = sum_parts(a, 49, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 225, q);
-    const double v24 = sum_parts(a, 39, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 236, q);
-    const double v29 = sum_parts(a, 50, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 237, q);
-    const double v35 = sum_parts(a, 51, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 238, q);
-    const double v40 = sum_parts(a, 52, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 218, q);
-    const double v46 = sum_parts(a, 32, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 226, q);
-    const double v51 = sum_parts(a, 40, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 227, q);
-    const double v56 = sum_parts(a, 41, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 239, q);
-    const double v61 = sum_parts(a, 53, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 240, q);
-    const double v67 = sum_parts(a, 54, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 241, q);
-    const double v72 = sum_parts(a, 55, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 219, q);
-    const double v78 = sum_parts(a, 33, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 228, q);
-    const double v84 = sum_parts(a, 42, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 242, q);
-    const double v93 = sum_parts(a, 56, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 220, q);
-    const double v99 = sum_parts(a, 34, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 229, q);
-    const double v104 = sum_parts(a, 43, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 230, q);
-    const double v109 = sum_parts(a, 44, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 243, q);
-    const double v114 = sum_parts(a, 57, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 244, q);
-    const double v120 = sum_parts(a, 58, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 245, q);
-    const double v125 = sum_parts(a, 59, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 221, q);
-    const double v131 = sum_parts(a, 35, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 231, q);
-    const double v137 = sum_parts(a, 45, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 246, q);
-    const double v146 = sum_parts(a, 60, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 222, q);
-    const double v152 = sum_parts(a, 36, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 232, q);
-    const double v158 = sum_parts(a, 46, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 247, q);
-    const double v167 = sum_parts(a, 61, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 0);
-    const double v0 = sum_parts(a, 171, q);
-    const double v1 = sum_parts(a, 78, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 161, q);
-    const double v4 = sum_parts(a, 68, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 155, q);
-    const double v8 = sum_parts(a, 62, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 172, q);
-    const double v13 = sum_parts(a, 79, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 162, q);
-    const double v16 = sum_parts(a, 69, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 173, q);
-    const double v21 = sum_parts(a, 80, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 163, q);
-    const double v24 = sum_parts(a, 70, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 174, q);
-    const double v29 = sum_parts(a, 81, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 175, q);
-    const double v35 = sum_parts(a, 82, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 176, q);
-    const double v40 = sum_parts(a, 83, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 156, q);
-    const double v46 = sum_parts(a, 63, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 164, q);
-    const double v51 = sum_parts(a, 71, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 165, q);
-    const double v56 = sum_parts(a, 72, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 177, q);
-    const double v61 = sum_parts(a, 84, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 178, q);
-    const double v67 = sum_parts(a, 85, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 179, q);
-    const double v72 = sum_parts(a, 86, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 157, q);
-    const double v78 = sum_parts(a, 64, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 166, q);
-    const double v84 = sum_parts(a, 73, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 180, q);
-    const double v93 = sum_parts(a, 87, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 158, q);
-    const double v99 = sum_parts(a, 65, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 167, q);
-    const double v104 = sum_parts(a, 74, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 168, q);
-    const double v109 = sum_parts(a, 75, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 181, q);
-    const double v114 = sum_parts(a, 88, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 182, q);
-    const double v120 = sum_parts(a, 89, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 183, q);
-    const double v125 = sum_parts(a, 90, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 159, q);
-    const double v131 = sum_parts(a, 66, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 169, q);
-    const double v137 = sum_parts(a, 76, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 184, q);
-    const double v146 = sum_parts(a, 91, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 160, q);
-    const double v152 = sum_parts(a, 67, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 170, q);
-    const double v158 = sum_parts(a, 77, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 185, q);
-    const double v167 = sum_parts(a, 92, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 1);
-    const double v0 = sum_parts(a, 233, q);
-    const double v1 = sum_parts(a, 78, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 223, q);
-    const double v4 = sum_parts(a, 68, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 217, q);
-    const double v8 = sum_parts(a, 62, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 234, q);
-    const double v13 = sum_parts(a, 79, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 224, q);
-    const double v16 = sum_parts(a, 69, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 235, q);
-    const double v21 = sum_parts(a, 80, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 225, q);
-    const double v24 = sum_parts(a, 70, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 236, q);
-    const double v29 = sum_parts(a, 81, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 237, q);
-    const double v35 = sum_parts(a, 82, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 238, q);
-    const double v40 = sum_parts(a, 83, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 218, q);
-    const double v46 = sum_parts(a, 63, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 226, q);
-    const double v51 = sum_parts(a, 71, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 227, q);
-    const double v56 = sum_parts(a, 72, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 239, q);
-    const double v61 = sum_parts(a, 84, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 240, q);
-    const double v67 = sum_parts(a, 85, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 241, q);
-    const double v72 = sum_parts(a, 86, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 219, q);
-    const double v78 = sum_parts(a, 64, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 228, q);
-    const double v84 = sum_parts(a, 73, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 242, q);
-    const double v93 = sum_parts(a, 87, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 220, q);
-    const double v99 = sum_parts(a, 65, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 229, q);
-    const double v104 = sum_parts(a, 74, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 230, q);
-    const double v109 = sum_parts(a, 75, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 243, q);
-    const double v114 = sum_parts(a, 88, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 244, q);
-    const double v120 = sum_parts(a, 89, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 245, q);
-    const double v125 = sum_parts(a, 90, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 221, q);
-    const double v131 = sum_parts(a, 66, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 231, q);
-    const double v137 = sum_parts(a, 76, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 246, q);
-    const double v146 = sum_parts(a, 91, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 222, q);
-    const double v152 = sum_parts(a, 67, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 232, q);
-    const double v158 = sum_parts(a, 77, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 247, q);
-    const double v167 = sum_parts(a, 92, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 2);
-    const double v0 = sum_parts(a, 264, q);
-    const double v1 = sum_parts(a, 78, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 254, q);
-    const double v4 = sum_parts(a, 68, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 248, q);
-    const double v8 = sum_parts(a, 62, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 265, q);
-    const double v13 = sum_parts(a, 79, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 255, q);
-    const double v16 = sum_parts(a, 69, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 266, q);
-    const double v21 = sum_parts(a, 80, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 256, q);
-    const double v24 = sum_parts(a, 70, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 267, q);
-    const double v29 = sum_parts(a, 81, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 268, q);
-    const double v35 = sum_parts(a, 82, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 269, q);
-    const double v40 = sum_parts(a, 83, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 249, q);
-    const double v46 = sum_parts(a, 63, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 257, q);
-    const double v51 = sum_parts(a, 71, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 258, q);
-    const double v56 = sum_parts(a, 72, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 270, q);
-    const double v61 = sum_parts(a, 84, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 271, q);
-    const double v67 = sum_parts(a, 85, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 272, q);
-    const double v72 = sum_parts(a, 86, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 250, q);
-    const double v78 = sum_parts(a, 64, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 259, q);
-    const double v84 = sum_parts(a, 73, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 273, q);
-    const double v93 = sum_parts(a, 87, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 251, q);
-    const double v99 = sum_parts(a, 65, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 260, q);
-    const double v104 = sum_parts(a, 74, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 261, q);
-    const double v109 = sum_parts(a, 75, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 274, q);
-    const double v114 = sum_parts(a, 88, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 275, q);
-    const double v120 = sum_parts(a, 89, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 276, q);
-    const double v125 = sum_parts(a, 90, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 252, q);
-    const double v131 = sum_parts(a, 66, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 262, q);
-    const double v137 = sum_parts(a, 76, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 277, q);
-    const double v146 = sum_parts(a, 91, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 253, q);
-    const double v152 = sum_parts(a, 67, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 263, q);
-    const double v158 = sum_parts(a, 77, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 278, q);
-    const double v167 = sum_parts(a, 92, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-}
-
-// class (22|20) split 0/2: 263 recursion values, 93 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2220_0(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc31 = 0.0;
-  double acc62 = 0.0;
-  double acc93 = 0.0;
-  double acc124 = 0.0;
-  double acc155 = 0.0;
-  double acc1 = 0.0;
-  double acc32 = 0.0;
-  double acc63 = 0.0;
-  double acc94 = 0.0;
-  double acc125 = 0.0;
-  double acc156 = 0.0;
-  double acc2 = 0.0;
-  double acc33 = 0.0;
-  double acc64 = 0.0;
-  double acc95 = 0.0;
-  double acc126 = 0.0;
-  double acc157 = 0.0;
-  double acc3 = 0.0;
-  double acc34 = 0.0;
-  double acc65 = 0.0;
-  double acc96 = 0.0;
-  double acc127 = 0.0;
-  double acc158 = 0.0;
-  double acc4 = 0.0;
-  double acc35 = 0.0;
-  double acc66 = 0.0;
-  double acc97 = 0.0;
-  double acc128 = 0.0;
-  double acc159 = 0.0;
-  double acc5 = 0.0;
-  double acc36 = 0.0;
-  double acc67 = 0.0;
-  double acc98 = 0.0;
-  double acc129 = 0.0;
-  double acc160 = 0.0;
-  double acc6 = 0.0;
-  double acc37 = 0.0;
-  double acc68 = 0.0;
-  double acc99 = 0.0;
-  double acc130 = 0.0;
-  double acc161 = 0.0;
-  double acc7 = 0.0;
-  double acc38 = 0.0;
-  double acc69 = 0.0;
-  double acc100 = 0.0;
-  double acc131 = 0.0;
-  double acc162 = 0.0;
-  double acc8 = 0.0;
-  double acc39 = 0.0;
-  double acc70 = 0.0;
-  double acc101 = 0.0;
-  double acc132 = 0.0;
-  double acc163 = 0.0;
-  double acc9 = 0.0;
-  double acc40 = 0.0;
-  double acc71 = 0.0;
-  double acc102 = 0.0;
-  double acc133 = 0.0;
-  double acc164 = 0.0;
-  double acc10 = 0.0;
-  double acc41 = 0.0;
-  double acc72 = 0.0;
-  double acc103 = 0.0;
-  double acc134 = 0.0;
-  double acc165 = 0.0;
-  double acc11 = 0.0;
-  double acc42 = 0.0;
-  double acc73 = 0.0;
-  double acc104 = 0.0;
-  double acc135 = 0.0;
-  double acc166 = 0.0;
-  double acc12 = 0.0;
-  double acc43 = 0.0;
-  double acc74 = 0.0;
-  double acc105 = 0.0;
-  double acc136 = 0.0;
-  double acc167 = 0.0;
-  double acc13 = 0.0;
-  double acc44 = 0.0;
-  double acc75 = 0.0;
-  double acc106 = 0.0;
-  double acc137 = 0.0;
-  double acc168 = 0.0;
-  double acc14 = 0.0;
-  double acc45 = 0.0;
-  double acc76 = 0.0;
-  double acc107 = 0.0;
-  double acc138 = 0.0;
-  double acc169 = 0.0;
-  double acc15 = 0.0;
-  double acc46 = 0.0;
-  double acc77 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x8 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x20 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x21 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x48 = s.PA[2] * x18 + s.WP[2] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x25 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x27 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x28 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x31 = s.PA[0] * x10 + s.WP[0] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x64 = s.PA[2] * x30 + s.WP[2] * x36 + z2 * (x9 - s.roz * x12);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x37 = s.PA[0] * x13 + s.WP[0] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x84 = s.PA[2] * x42 + s.WP[2] * x48 + z2 * (x15 - s.roz * x18);
-      const TB x43 = s.PA[0] * x16 + s.WP[0] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x46 = s.PA[1] * x17 + s.WP[1] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x47 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x83 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x14 - s.roz * x17);
-      const TB x81 = s.PA[1] * x40 + s.WP[1] * x46 + z2 * (x14 - s.roz * x17);
-      const TB x82 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x45 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x80 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x13 - s.roz * x16);
-      const TB x55 = s.PA[0] * x25 + s.WP[0] * x31 + z2 * (x7 - s.roz * x10);
-      const TB x56 = s.PA[1] * x25 + s.WP[1] * x31;
-      const TB x57 = s.PA[2] * x25 + s.WP[2] * x31;
-      const TB x58 = s.PA[1] * x26 + s.WP[1] * x32 + z1 * (x7 - s.roz * x10);
-      const TB x60 = s.PA[2] * x27 + s.WP[2] * x33 + z1 * (x7 - s.roz * x10);
-      const TB x59 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x61 = s.PA[1] * x28 + s.WP[1] * x34 + z2 * (x8 - s.roz * x11);
-      const TB x63 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x8 - s.roz * x11);
-      const TB x62 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x65 = s.PA[0] * x31 + s.WP[0] * x37 + z2 * (x10 - s.roz * x13);
-      const TB x66 = s.PA[1] * x31 + s.WP[1] * x37;
-      const TB x67 = s.PA[2] * x31 + s.WP[2] * x37;
-      const TB x68 = s.PA[1] * x32 + s.WP[1] * x38 + z1 * (x10 - s.roz * x13);
-      const TB x69 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x70 = s.PA[2] * x33 + s.WP[2] * x39 + z1 * (x10 - s.roz * x13);
-      const TB x71 = s.PA[1] * x34 + s.WP[1] * x40 + z2 * (x11 - s.roz * x14);
-      const TB x72 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x73 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x11 - s.roz * x14);
-      const TB x74 = s.PA[2] * x36 + s.WP[2] * x42 + z2 * (x12 - s.roz * x15);
-      const TB x174 = s.QC[0] * x74 + s.WQ[0] * x84;
-      const TB x75 = s.PA[0] * x37 + s.WP[0] * x43 + z2 * (x13 - s.roz * x16);
-      const TB x78 = s.PA[1] * x38 + s.WP[1] * x44 + z1 * (x13 - s.roz * x16);
-      const TB x79 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x76 = s.PA[1] * x37 + s.WP[1] * x43;
-      const TB x77 = s.PA[2] * x37 + s.WP[2] * x43;
-      const TB x140 = s.QC[0] * x10 + s.WQ[0] * x13 + r1 * x2;
-      const TB x141 = s.QC[0] * x11 + s.WQ[0] * x14;
-      const TB x142 = s.QC[0] * x12 + s.WQ[0] * x15;
-      const TB x143 = s.QC[0] * x25 + s.WQ[0] * x31 + r2 * x10;
-      const TB x144 = s.QC[0] * x31 + s.WQ[0] * x37 + r2 * x13;
-      const TB x335 = s.QC[0] * x143 + s.WQ[0] * x144 + e1 * (x25 - s.roe * x31) + r2 * x140;
-      acc0 += static_cast<double>(x335);
-      const TB x366 = s.QC[1] * x143 + s.WQ[1] * x144;
-      acc31 += static_cast<double>(x366);
-      const TB x397 = s.QC[2] * x143 + s.WQ[2] * x144;
-      acc62 += static_cast<double>(x397);
-      const TB x145 = s.QC[0] * x26 + s.WQ[0] * x32 + r1 * x11;
-      const TB x146 = s.QC[0] * x32 + s.WQ[0] * x38 + r1 * x14;
-      const TB x336 = s.QC[0] * x145 + s.WQ[0] * x146 + e1 * (x26 - s.roe * x32) + r1 * x141;
-      acc1 += static_cast<double>(x336);
-      const TB x367 = s.QC[1] * x145 + s.WQ[1] * x146 + r1 * x140;
-      acc32 += static_cast<double>(x367);
-      const TB x398 = s.QC[2] * x145 + s.WQ[2] * x146;
-      acc63 += static_cast<double>(x398);
-      const TB x147 = s.QC[0] * x27 + s.WQ[0] * x33 + r1 * x12;
-      const TB x148 = s.QC[0] * x33 + s.WQ[0] * x39 + r1 * x15;
-      const TB x399 = s.QC[2] * x147 + s.WQ[2] * x148 + r1 * x140;
-      acc64 += static_cast<double>(x399);
-      const TB x337 = s.QC[0] * x147 + s.WQ[0] * x148 + e1 * (x27 - s.roe * x33) + r1 * x142;
-      acc2 += static_cast<double>(x337);
-      const TB x368 = s.QC[1] * x147 + s.WQ[1] * x148;
-      acc33 += static_cast<double>(x368);
-      const TB x149 = s.QC[0] * x28 + s.WQ[0] * x34;
-      const TB x150 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x338 = s.QC[0] * x149 + s.WQ[0] * x150 + e1 * (x28 - s.roe * x34);
-      acc3 += static_cast<double>(x338);
-      const TB x369 = s.QC[1] * x149 + s.WQ[1] * x150 + r2 * x141;
-      acc34 += static_cast<double>(x369);
-      const TB x400 = s.QC[2] * x149 + s.WQ[2] * x150;
-      acc65 += static_cast<double>(x400);
-      const TB x151 = s.QC[0] * x29 + s.WQ[0] * x35;
-      const TB x152 = s.QC[0] * x35 + s.WQ[0] * x41;
-      const TB x401 = s.QC[2] * x151 + s.WQ[2] * x152 + r1 * x141;
-      acc66 += static_cast<double>(x401);
-      const TB x339 = s.QC[0] * x151 + s.WQ[0] * x152 + e1 * (x29 - s.roe * x35);
-      acc4 += static_cast<double>(x339);
-      const TB x370 = s.QC[1] * x151 + s.WQ[1] * x152 + r1 * x142;
-      acc35 += static_cast<double>(x370);
-      const TB x153 = s.QC[0] * x30 + s.WQ[0] * x36;
-      const TB x154 = s.QC[0] * x36 + s.WQ[0] * x42;
-      const TB x402 = s.QC[2] * x153 + s.WQ[2] * x154 + r2 * x142;
-      acc67 += static_cast<double>(x402);
-      const TB x340 = s.QC[0] * x153 + s.WQ[0] * x154 + e1 * (x30 - s.roe * x36);
-      acc5 += static_cast<double>(x340);
-      const TB x371 = s.QC[1] * x153 + s.WQ[1] * x154;
-      acc36 += static_cast<double>(x371);
-      const TB x155 = s.QC[0] * x55 + s.WQ[0] * x65 + r3 * x31;
-      const TB x156 = s.QC[0] * x65 + s.WQ[0] * x75 + r3 * x37;
-      const TB x341 = s.QC[0] * x155 + s.WQ[0] * x156 + e1 * (x55 - s.roe * x65) + r3 * x144;
-      acc6 += static_cast<double>(x341);
-      const TB x372 = s.QC[1] * x155 + s.WQ[1] * x156;
-      acc37 += static_cast<double>(x372);
-      const TB x403 = s.QC[2] * x155 + s.WQ[2] * x156;
-      acc68 += static_cast<double>(x403);
-      const TB x157 = s.QC[0] * x56 + s.WQ[0] * x66 + r2 * x32;
-      const TB x158 = s.QC[0] * x66 + s.WQ[0] * x76 + r2 * x38;
-      const TB x342 = s.QC[0] * x157 + s.WQ[0] * x158 + e1 * (x56 - s.roe * x66) + r2 * x146;
-      acc7 += static_cast<double>(x342);
-      const TB x373 = s.QC[1] * x157 + s.WQ[1] * x158 + r1 * x144;
-      acc38 += static_cast<double>(x373);
-      const TB x404 = s.QC[2] * x157 + s.WQ[2] * x158;
-      acc69 += static_cast<double>(x404);
-      const TB x159 = s.QC[0] * x57 + s.WQ[0] * x67 + r2 * x33;
-      const TB x160 = s.QC[0] * x67 + s.WQ[0] * x77 + r2 * x39;
-      const TB x405 = s.QC[2] * x159 + s.WQ[2] * x160 + r1 * x144;
-      acc70 += static_cast<double>(x405);
-      const TB x343 = s.QC[0] * x159 + s.WQ[0] * x160 + e1 * (x57 - s.roe * x67) + r2 * x148;
-      acc8 += static_cast<double>(x343);
-      const TB x374 = s.QC[1] * x159 + s.WQ[1] * x160;
-      acc39 += static_cast<double>(x374);
-      const TB x161 = s.QC[0] * x58 + s.WQ[0] * x68 + r1 * x34;
-      const TB x162 = s.QC[0] * x68 + s.WQ[0] * x78 + r1 * x40;
-      const TB x344 = s.QC[0] * x161 + s.WQ[0] * x162 + e1 * (x58 - s.roe * x68) + r1 * x150;
-      acc9 += static_cast<double>(x344);
-      const TB x375 = s.QC[1] * x161 + s.WQ[1] * x162 + r2 * x146;
-      acc40 += static_cast<double>(x375);
-      const TB x406 = s.QC[2] * x161 + s.WQ[2] * x162;
-      acc71 += static_cast<double>(x406);
-      const TB x163 = s.QC[0] * x59 + s.WQ[0] * x69 + r1 * x35;
-      const TB x164 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x41;
-      const TB x407 = s.QC[2] * x163 + s.WQ[2] * x164 + r1 * x146;
-      acc72 += static_cast<double>(x407);
-      const TB x345 = s.QC[0] * x163 + s.WQ[0] * x164 + e1 * (x59 - s.roe * x69) + r1 * x152;
-      acc10 += static_cast<double>(x345);
-      const TB x376 = s.QC[1] * x163 + s.WQ[1] * x164 + r1 * x148;
-      acc41 += static_cast<double>(x376);
-      const TB x165 = s.QC[0] * x60 + s.WQ[0] * x70 + r1 * x36;
-      const TB x166 = s.QC[0] * x70 + s.WQ[0] * x80 + r1 * x42;
-      const TB x408 = s.QC[2] * x165 + s.WQ[2] * x166 + r2 * x148;
-      acc73 += static_cast<double>(x408);
-      const TB x346 = s.QC[0] * x165 + s.WQ[0] * x166 + e1 * (x60 - s.roe * x70) + r1 * x154;
-      acc11 += static_cast<double>(x346);
-      const TB x377 = s.QC[1] * x165 + s.WQ[1] * x166;
-      acc42 += static_cast<double>(x377);
-      const TB x167 = s.QC[0] * x61 + s.WQ[0] * x71;
-      const TB x168 = s.QC[0] * x71 + s.WQ[0] * x81;
-      const TB x347 = s.QC[0] * x167 + s.WQ[0] * x168 + e1 * (x61 - s.roe * x71);
-      acc12 += static_cast<double>(x347);
-      const TB x378 = s.QC[1] * x167 + s.WQ[1] * x168 + r3 * x150;
-      acc43 += static_cast<double>(x378);
-      const TB x409 = s.QC[2] * x167 + s.WQ[2] * x168;
-      acc74 += static_cast<double>(x409);
-      const TB x169 = s.QC[0] * x62 + s.WQ[0] * x72;
-      const TB x170 = s.QC[0] * x72 + s.WQ[0] * x82;
-      const TB x410 = s.QC[2] * x169 + s.WQ[2] * x170 + r1 * x150;
-      acc75 += static_cast<double>(x410);
-      const TB x348 = s.QC[0] * x169 + s.WQ[0] * x170 + e1 * (x62 - s.roe * x72);
-      acc13 += static_cast<double>(x348);
-      const TB x379 = s.QC[1] * x169 + s.WQ[1] * x170 + r2 * x152;
-      acc44 += static_cast<double>(x379);
-      const TB x171 = s.QC[0] * x63 + s.WQ[0] * x73;
-      const TB x172 = s.QC[0] * x73 + s.WQ[0] * x83;
-      const TB x411 = s.QC[2] * x171 + s.WQ[2] * x172 + r2 * x152;
-      acc76 += static_cast<double>(x411);
-      const TB x349 = s.QC[0] * x171 + s.WQ[0] * x172 + e1 * (x63 - s.roe * x73);
-      acc14 += static_cast<double>(x349);
-      const TB x380 = s.QC[1] * x171 + s.WQ[1] * x172 + r1 * x154;
-      acc45 += static_cast<double>(x380);
-      const TB x173 = s.QC[0] * x64 + s.WQ[0] * x74;
-      const TB x350 = s.QC[0] * x173 + s.WQ[0] * x174 + e1 * (x64 - s.roe * x74);
-      acc15 += static_cast<double>(x350);
-      const TB x412 = s.QC[2] * x173 + s.WQ[2] * x174 + r3 * x154;
-      acc77 += static_cast<double>(x412);
-      const TB x381 = s.QC[1] * x173 + s.WQ[1] * x174;
-      acc46 += static_cast<double>(x381);
-      const TB x205 = s.QC[1] * x10 + s.WQ[1] * x13;
-      const TB x206 = s.QC[1] * x11 + s.WQ[1] * x14 + r1 * x2;
-      const TB x272 = s.QC[2] * x12 + s.WQ[2] * x15 + r1 * x2;
-      const TB x207 = s.QC[1] * x12 + s.WQ[1] * x15;
-      const TB x208 = s.QC[1] * x25 + s.WQ[1] * x31;
-      const TB x209 = s.QC[1] * x31 + s.WQ[1] * x37;
-      const TB x428 = s.QC[1] * x208 + s.WQ[1] * x209 + e1 * (x25 - s.roe * x31);
-      acc93 += static_cast<double>(x428);
-      const TB x459 = s.QC[2] * x208 + s.WQ[2] * x209;
-      acc124 += static_cast<double>(x459);
-      const TB x210 = s.QC[1] * x26 + s.WQ[1] * x32 + r1 * x10;
-      const TB x211 = s.QC[1] * x32 + s.WQ[1] * x38 + r1 * x13;
-      const TB x429 = s.QC[1] * x210 + s.WQ[1] * x211 + e1 * (x26 - s.roe * x32) + r1 * x205;
-      acc94 += static_cast<double>(x429);
-      const TB x460 = s.QC[2] * x210 + s.WQ[2] * x211;
-      acc125 += static_cast<double>(x460);
-      const TB x212 = s.QC[1] * x27 + s.WQ[1] * x33;
-      const TB x213 = s.QC[1] * x33 + s.WQ[1] * x39;
-      const TB x461 = s.QC[2] * x212 + s.WQ[2] * x213 + r1 * x205;
-      acc126 += static_cast<double>(x461);
-      const TB x430 = s.QC[1] * x212 + s.WQ[1] * x213 + e1 * (x27 - s.roe * x33);
-      acc95 += static_cast<double>(x430);
-      const TB x214 = s.QC[1] * x28 + s.WQ[1] * x34 + r2 * x11;
-      const TB x215 = s.QC[1] * x34 + s.WQ[1] * x40 + r2 * x14;
-      const TB x431 = s.QC[1] * x214 + s.WQ[1] * x215 + e1 * (x28 - s.roe * x34) + r2 * x206;
-      acc96 += static_cast<double>(x431);
-      const TB x462 = s.QC[2] * x214 + s.WQ[2] * x215;
-      acc127 += static_cast<double>(x462);
-      const TB x216 = s.QC[1] * x29 + s.WQ[1] * x35 + r1 * x12;
-      const TB x283 = s.QC[2] * x30 + s.WQ[2] * x36 + r2 * x12;
-      const TB x217 = s.QC[1] * x35 + s.WQ[1] * x41 + r1 * x15;
-      const TB x463 = s.QC[2] * x216 + s.WQ[2] * x217 + r1 * x206;
-      acc128 += static_cast<double>(x463);
-      const TB x432 = s.QC[1] * x216 + s.WQ[1] * x217 + e1 * (x29 - s.roe * x35) + r1 * x207;
-      acc97 += static_cast<double>(x432);
-      const TB x284 = s.QC[2] * x36 + s.WQ[2] * x42 + r2 * x15;
-      const TB x495 = s.QC[2] * x283 + s.WQ[2] * x284 + e1 * (x30 - s.roe * x36) + r2 * x272;
-      acc160 += static_cast<double>(x495);
-      const TB x218 = s.QC[1] * x30 + s.WQ[1] * x36;
-      const TB x219 = s.QC[1] * x36 + s.WQ[1] * x42;
-      const TB x433 = s.QC[1] * x218 + s.WQ[1] * x219 + e1 * (x30 - s.roe * x36);
-      acc98 += static_cast<double>(x433);
-      const TB x464 = s.QC[2] * x218 + s.WQ[2] * x219 + r2 * x207;
-      acc129 += static_cast<double>(x464);
-      const TB x236 = s.QC[1] * x63 + s.WQ[1] * x73 + r1 * x36;
-      const TB x237 = s.QC[1] * x73 + s.WQ[1] * x83 + r1 * x42;
-      const TB x442 = s.QC[1] * x236 + s.WQ[1] * x237 + e1 * (x63 - s.roe * x73) + r1 * x219;
-      acc107 += static_cast<double>(x442);
-      const TB x473 = s.QC[2] * x236 + s.WQ[2] * x237 + r2 * x217;
-      acc138 += static_cast<double>(x473);
-      const TB x302 = s.QC[2] * x73 + s.WQ[2] * x83 + r2 * x41;
-      const TB x220 = s.QC[1] * x55 + s.WQ[1] * x65;
-      const TB x221 = s.QC[1] * x65 + s.WQ[1] * x75;
-      const TB x286 = s.QC[2] * x65 + s.WQ[2] * x75;
-      const TB x434 = s.QC[1] * x220 + s.WQ[1] * x221 + e1 * (x55 - s.roe * x65);
-      acc99 += static_cast<double>(x434);
-      const TB x465 = s.QC[2] * x220 + s.WQ[2] * x221;
-      acc130 += static_cast<double>(x465);
-      const TB x222 = s.QC[1] * x56 + s.WQ[1] * x66 + r1 * x31;
-      const TB x223 = s.QC[1] * x66 + s.WQ[1] * x76 + r1 * x37;
-      const TB x288 = s.QC[2] * x66 + s.WQ[2] * x76;
-      const TB x435 = s.QC[1] * x222 + s.WQ[1] * x223 + e1 * (x56 - s.roe * x66) + r1 * x209;
-      acc100 += static_cast<double>(x435);
-      const TB x466 = s.QC[2] * x222 + s.WQ[2] * x223;
-      acc131 += static_cast<double>(x466);
-      const TB x224 = s.QC[1] * x57 + s.WQ[1] * x67;
-      const TB x225 = s.QC[1] * x67 + s.WQ[1] * x77;
-      const TB x467 = s.QC[2] * x224 + s.WQ[2] * x225 + r1 * x209;
-      acc132 += static_cast<double>(x467);
-      const TB x436 = s.QC[1] * x224 + s.WQ[1] * x225 + e1 * (x57 - s.roe * x67);
-      acc101 += static_cast<double>(x436);
-      const TB x290 = s.QC[2] * x67 + s.WQ[2] * x77 + r1 * x37;
-      const TB x274 = s.QC[2] * x31 + s.WQ[2] * x37;
-      const TB x226 = s.QC[1] * x58 + s.WQ[1] * x68 + r2 * x32;
-      const TB x227 = s.QC[1] * x68 + s.WQ[1] * x78 + r2 * x38;
-      const TB x292 = s.QC[2] * x68 + s.WQ[2] * x78;
-      const TB x437 = s.QC[1] * x226 + s.WQ[1] * x227 + e1 * (x58 - s.roe * x68) + r2 * x211;
-      acc102 += static_cast<double>(x437);
-      const TB x468 = s.QC[2] * x226 + s.WQ[2] * x227;
-      acc133 += static_cast<double>(x468);
-      const TB x228 = s.QC[1] * x59 + s.WQ[1] * x69 + r1 * x33;
-      const TB x229 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x39;
-      const TB x469 = s.QC[2] * x228 + s.WQ[2] * x229 + r1 * x211;
-      acc134 += static_cast<double>(x469);
-      const TB x438 = s.QC[1] * x228 + s.WQ[1] * x229 + e1 * (x59 - s.roe * x69) + r1 * x213;
-      acc103 += static_cast<double>(x438);
-      const TB x294 = s.QC[2] * x69 + s.WQ[2] * x79 + r1 * x38;
-      const TB x276 = s.QC[2] * x32 + s.WQ[2] * x38;
-      const TB x230 = s.QC[1] * x60 + s.WQ[1] * x70;
-      const TB x231 = s.QC[1] * x70 + s.WQ[1] * x80;
-      const TB x470 = s.QC[2] * x230 + s.WQ[2] * x231 + r2 * x213;
-      acc135 += static_cast<double>(x470);
-      const TB x439 = s.QC[1] * x230 + s.WQ[1] * x231 + e1 * (x60 - s.roe * x70);
-      acc104 += static_cast<double>(x439);
-      const TB x296 = s.QC[2] * x70 + s.WQ[2] * x80 + r2 * x39;
-      const TB x278 = s.QC[2] * x33 + s.WQ[2] * x39 + r1 * x13;
-      const TB x270 = s.QC[2] * x10 + s.WQ[2] * x13;
-      const TB x277 = s.QC[2] * x27 + s.WQ[2] * x33 + r1 * x10;
-      const TB x492 = s.QC[2] * x277 + s.WQ[2] * x278 + e1 * (x27 - s.roe * x33) + r1 * x270;
-      acc157 += static_cast<double>(x492);
-      const TB x295 = s.QC[2] * x60 + s.WQ[2] * x70 + r2 * x33;
-      const TB x501 = s.QC[2] * x295 + s.WQ[2] * x296 + e1 * (x60 - s.roe * x70) + r2 * x278;
-      acc166 += static_cast<double>(x501);
-      const TB x232 = s.QC[1] * x61 + s.WQ[1] * x71 + r3 * x34;
-      const TB x233 = s.QC[1] * x71 + s.WQ[1] * x81 + r3 * x40;
-      const TB x298 = s.QC[2] * x71 + s.WQ[2] * x81;
-      const TB x440 = s.QC[1] * x232 + s.WQ[1] * x233 + e1 * (x61 - s.roe * x71) + r3 * x215;
-      acc105 += static_cast<double>(x440);
-      const TB x471 = s.QC[2] * x232 + s.WQ[2] * x233;
-      acc136 += static_cast<double>(x471);
-      const TB x234 = s.QC[1] * x62 + s.WQ[1] * x72 + r2 * x35;
-      const TB x235 = s.QC[1] * x72 + s.WQ[1] * x82 + r2 * x41;
-      const TB x441 = s.QC[1] * x234 + s.WQ[1] * x235 + e1 * (x62 - s.roe * x72) + r2 * x217;
-      acc106 += static_cast<double>(x441);
-      const TB x472 = s.QC[2] * x234 + s.WQ[2] * x235 + r1 * x215;
-      acc137 += static_cast<double>(x472);
-      const TB x282 = s.QC[2] * x35 + s.WQ[2] * x41 + r1 * x14;
-      const TB x271 = s.QC[2] * x11 + s.WQ[2] * x14;
-      const TB x281 = s.QC[2] * x29 + s.WQ[2] * x35 + r1 * x11;
-      const TB x494 = s.QC[2] * x281 + s.WQ[2] * x282 + e1 * (x29 - s.roe * x35) + r1 * x271;
-      acc159 += static_cast<double>(x494);
-      const TB x300 = s.QC[2] * x72 + s.WQ[2] * x82 + r1 * x40;
-      const TB x280 = s.QC[2] * x34 + s.WQ[2] * x40;
-      const TB x301 = s.QC[2] * x63 + s.WQ[2] * x73 + r2 * x35;
-      const TB x504 = s.QC[2] * x301 + s.WQ[2] * x302 + e1 * (x63 - s.roe * x73) + r2 * x282;
-      acc169 += static_cast<double>(x504);
-      const TB x273 = s.QC[2] * x25 + s.WQ[2] * x31;
-      const TB x490 = s.QC[2] * x273 + s.WQ[2] * x274 + e1 * (x25 - s.roe * x31);
-      acc155 += static_cast<double>(x490);
-      const TB x289 = s.QC[2] * x57 + s.WQ[2] * x67 + r1 * x31;
-      const TB x498 = s.QC[2] * x289 + s.WQ[2] * x290 + e1 * (x57 - s.roe * x67) + r1 * x274;
-      acc163 += static_cast<double>(x498);
-      const TB x275 = s.QC[2] * x26 + s.WQ[2] * x32;
-      const TB x491 = s.QC[2] * x275 + s.WQ[2] * x276 + e1 * (x26 - s.roe * x32);
-      acc156 += static_cast<double>(x491);
-      const TB x293 = s.QC[2] * x59 + s.WQ[2] * x69 + r1 * x32;
-      const TB x500 = s.QC[2] * x293 + s.WQ[2] * x294 + e1 * (x59 - s.roe * x69) + r1 * x276;
-      acc165 += static_cast<double>(x500);
-      const TB x279 = s.QC[2] * x28 + s.WQ[2] * x34;
-      const TB x493 = s.QC[2] * x279 + s.WQ[2] * x280 + e1 * (x28 - s.roe * x34);
-      acc158 += static_cast<double>(x493);
-      const TB x299 = s.QC[2] * x62 + s.WQ[2] * x72 + r1 * x34;
-      const TB x503 = s.QC[2] * x299 + s.WQ[2] * x300 + e1 * (x62 - s.roe * x72) + r1 * x280;
-      acc168 += static_cast<double>(x503);
-      const TB x285 = s.QC[2] * x55 + s.WQ[2] * x65;
-      const TB x496 = s.QC[2] * x285 + s.WQ[2] * x286 + e1 * (x55 - s.roe * x65);
-      acc161 += static_cast<double>(x496);
-      const TB x287 = s.QC[2] * x56 + s.WQ[2] * x66;
-      const TB x497 = s.QC[2] * x287 + s.WQ[2] * x288 + e1 * (x56 - s.roe * x66);
-      acc162 += static_cast<double>(x497);
-      const TB x291 = s.QC[2] * x58 + s.WQ[2] * x68;
-      const TB x499 = s.QC[2] * x291 + s.WQ[2] * x292 + e1 * (x58 - s.roe * x68);
-      acc164 += static_cast<double>(x499);
-      const TB x297 = s.QC[2] * x61 + s.WQ[2] * x71;
-      const TB x502 = s.QC[2] * x297 + s.WQ[2] * x298 + e1 * (x61 - s.roe * x71);
-      acc167 += static_cast<double>(x502);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(0 * R + r) * a.acc_stride + q] = acc0;
-  a.acc[(31 * R + r) * a.acc_stride + q] = acc31;
-  a.acc[(62 * R + r) * a.acc_stride + q] = acc62;
-  a.acc[(93 * R + r) * a.acc_stride + q] = acc93;
-  a.acc[(124 * R + r) * a.acc_stride + q] = acc124;
-  a.acc[(155 * R + r) * a.acc_stride + q] = acc155;
-  a.acc[(1 * R + r) * a.acc_stride + q] = acc1;
-  a.acc[(32 * R + r) * a.acc_stride + q] = acc32;
-  a.acc[(63 * R + r) * a.acc_stride + q] = acc63;
-  a.acc[(94 * R + r) * a.acc_stride + q] = acc94;
-  a.acc[(125 * R + r) * a.acc_stride + q] = acc125;
-  a.acc[(156 * R + r) * a.acc_stride + q] = acc156;
-  a.acc[(2 * R + r) * a.acc_stride + q] = acc2;
-  a.acc[(33 * R + r) * a.acc_stride + q] = acc33;
-  a.acc[(64 * R + r) * a.acc_stride + q] = acc64;
-  a.acc[(95 * R + r) * a.acc_stride + q] = acc95;
-  a.acc[(126 * R + r) * a.acc_stride + q] = acc126;
-  a.acc[(157 * R + r) * a.acc_stride + q] = acc157;
-  a.acc[(3 * R + r) * a.acc_stride + q] = acc3;
-  a.acc[(34 * R + r) * a.acc_stride + q] = acc34;
-  a.acc[(65 * R + r) * a.acc_stride + q] = acc65;
-  a.acc[(96 * R + r) * a.acc_stride + q] = acc96;
-  a.acc[(127 * R + r) * a.acc_stride + q] = acc127;
-  a.acc[(158 * R + r) * a.acc_stride + q] = acc158;
-  a.acc[(4 * R + r) * a.acc_stride + q] = acc4;
-  a.acc[(35 * R + r) * a.acc_stride + q] = acc35;
-  a.acc[(66 * R + r) * a.acc_stride + q] = acc66;
-  a.acc[(97 * R + r) * a.acc_stride + q] = acc97;
-  a.acc[(128 * R + r) * a.acc_stride + q] = acc128;
-  a.acc[(159 * R + r) * a.acc_stride + q] = acc159;
-  a.acc[(5 * R + r) * a.acc_stride + q] = acc5;
-  a.acc[(36 * R + r) * a.acc_stride + q] = acc36;
-  a.acc[(67 * R + r) * a.acc_stride + q] = acc67;
-  a.acc[(98 * R + r) * a.acc_stride + q] = acc98;
-  a.acc[(129 * R + r) * a.acc_stride + q] = acc129;
-  a.acc[(160 * R + r) * a.acc_stride + q] = acc160;
-  a.acc[(6 * R + r) * a.acc_stride + q] = acc6;
-  a.acc[(37 * R + r) * a.acc_stride + q] = acc37;
-  a.acc[(68 * R + r) * a.acc_stride + q] = acc68;
-  a.acc[(99 * R + r) * a.acc_stride + q] = acc99;
-  a.acc[(130 * R + r) * a.acc_stride + q] = acc130;
-  a.acc[(161 * R + r) * a.acc_stride + q] = acc161;
-  a.acc[(7 * R + r) * a.acc_stride + q] = acc7;
-  a.acc[(38 * R + r) * a.acc_stride + q] = acc38;
-  a.acc[(69 * R + r) * a.acc_stride + q] = acc69;
-  a.acc[(100 * R + r) * a.acc_stride + q] = acc100;
-  a.acc[(131 * R + r) * a.acc_stride + q] = acc131;
-  a.acc[(162 * R + r) * a.acc_stride + q] = acc162;
-  a.acc[(8 * R + r) * a.acc_stride + q] = acc8;
-  a.acc[(39 * R + r) * a.acc_stride + q] = acc39;
-  a.acc[(70 * R + r) * a.acc_stride + q] = acc70;
-  a.acc[(101 * R + r) * a.acc_stride + q] = acc101;
-  a.acc[(132 * R + r) * a.acc_stride + q] = acc132;
-  a.acc[(163 * R + r) * a.acc_stride + q] = acc163;
-  a.acc[(9 * R + r) * a.acc_stride + q] = acc9;
-  a.acc[(40 * R + r) * a.acc_stride + q] = acc40;
-  a.acc[(71 * R + r) * a.acc_stride + q] = acc71;
-  a.acc[(102 * R + r) * a.acc_stride + q] = acc102;
-  a.acc[(133 * R + r) * a.acc_stride + q] = acc133;
-  a.acc[(164 * R + r) * a.acc_stride + q] = acc164;
-  a.acc[(10 * R + r) * a.acc_stride + q] = acc10;
-  a.acc[(41 * R + r) * a.acc_stride + q] = acc41;
-  a.acc[(72 * R + r) * a.acc_stride + q] = acc72;
-  a.acc[(103 * R + r) * a.acc_stride + q] = acc103;
-  a.acc[(134 * R + r) * a.acc_stride + q] = acc134;
-  a.acc[(165 * R + r) * a.acc_stride + q] = acc165;
-  a.acc[(11 * R + r) * a.acc_stride + q] = acc11;
-  a.acc[(42 * R + r) * a.acc_stride + q] = acc42;
-  a.acc[(73 * R + r) * a.acc_stride + q] = acc73;
-  a.acc[(104 * R + r) * a.acc_stride + q] = acc104;
-  a.acc[(135 * R + r) * a.acc_stride + q] = acc135;
-  a.acc[(166 * R + r) * a.acc_stride + q] = acc166;
-  a.acc[(12 * R + r) * a.acc_stride + q] = acc12;
-  a.acc[(43 * R + r) * a.acc_stride + q] = acc43;
-  a.acc[(74 * R + r) * a.acc_stride + q] = acc74;
-  a.acc[(105 * R + r) * a.acc_stride + q] = acc105;
-  a.acc[(136 * R + r) * a.acc_stride + q] = acc136;
-  a.acc[(167 * R + r) * a.acc_stride + q] = acc167;
-  a.acc[(13 * R + r) * a.acc_stride + q] = acc13;
-  a.acc[(44 * R + r) * a.acc_stride + q] = acc44;
-  a.acc[(75 * R + r) * a.acc_stride + q] = acc75;
-  a.acc[(106 * R + r) * a.acc_stride + q] = acc106;
-  a.acc[(137 * R + r) * a.acc_stride + q] = acc137;
-  a.acc[(168 * R + r) * a.acc_stride + q] = acc168;
-  a.acc[(14 * R + r) * a.acc_stride + q] = acc14;
-  a.acc[(45 * R + r) * a.acc_stride + q] = acc45;
-  a.acc[(76 * R + r) * a.acc_stride + q] = acc76;
-  a.acc[(107 * R + r) * a.acc_stride + q] = acc107;
-  a.acc[(138 * R + r) * a.acc_stride + q] = acc138;
-  a.acc[(169 * R + r) * a.acc_stride + q] = acc169;
-  a.acc[(15 * R + r) * a.acc_stride + q] = acc15;
-  a.acc[(46 * R + r) * a.acc_stride + q] = acc46;
-  a.acc[(77 * R + r) * a.acc_stride + q] = acc77;
-}
-// class (22|20) split 1/2: 350 recursion values, 93 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2220_1(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc108 = 0.0;
-  double acc139 = 0.0;
-  double acc170 = 0.0;
-  double acc16 = 0.0;
-  double acc47 = 0.0;
-  double acc78 = 0.0;
-  double acc109 = 0.0;
-  double acc140 = 0.0;
-  double acc171 = 0.0;
-  double acc17 = 0.0;
-  double acc48 = 0.0;
-  double acc79 = 0.0;
-  double acc110 = 0.0;
-  double acc141 = 0.0;
-  double acc172 = 0.0;
-  double acc18 = 0.0;
-  double acc49 = 0.0;
-  double acc80 = 0.0;
-  double acc111 = 0.0;
-  double acc142 = 0.0;
-  double acc173 = 0.0;
-  double acc19 = 0.0;
-  double acc50 = 0.0;
-  double acc81 = 0.0;
-  double acc112 = 0.0;
-  double acc143 = 0.0;
-  double acc174 = 0.0;
-  double acc20 = 0.0;
-  double acc51 = 0.0;
-  double acc82 = 0.0;
-  double acc113 = 0.0;
-  double acc144 = 0.0;
-  double acc175 = 0.0;
-  double acc21 = 0.0;
-  double acc52 = 0.0;
-  double acc83 = 0.0;
-  double acc114 = 0.0;
-  double acc145 = 0.0;
-  double acc176 = 0.0;
-  double acc22 = 0.0;
-  double acc53 = 0.0;
-  double acc84 = 0.0;
-  double acc115 = 0.0;
-  double acc146 = 0.0;
-  double acc177 = 0.0;
-  double acc23 = 0.0;
-  double acc54 = 0.0;
-  double acc85 = 0.0;
-  double acc116 = 0.0;
-  double acc147 = 0.0;
-  double acc178 = 0.0;
-  double acc24 = 0.0;
-  double acc55 = 0.0;
-  double acc86 = 0.0;
-  double acc117 = 0.0;
-  double acc148 = 0.0;
-  double acc179 = 0.0;
-  double acc25 = 0.0;
-  double acc56 = 0.0;
-  double acc87 = 0.0;
-  double acc118 = 0.0;
-  double acc149 = 0.0;
-  double acc180 = 0.0;
-  double acc26 = 0.0;
-  double acc57 = 0.0;
-  double acc88 = 0.0;
-  double acc119 = 0.0;
-  double acc150 = 0.0;
-  double acc181 = 0.0;
-  double acc27 = 0.0;
-  double acc58 = 0.0;
-  double acc89 = 0.0;
-  double acc120 = 0.0;
-  double acc151 = 0.0;
-  double acc182 = 0.0;
-  double acc28 = 0.0;
-  double acc59 = 0.0;
-  double acc90 = 0.0;
-  double acc121 = 0.0;
-  double acc152 = 0.0;
-  double acc183 = 0.0;
-  double acc29 = 0.0;
-  double acc60 = 0.0;
-  double acc91 = 0.0;
-  double acc122 = 0.0;
-  double acc153 = 0.0;
-  double acc184 = 0.0;
-  double acc30 = 0.0;
-  double acc61 = 0.0;
-  double acc92 = 0.0;
-  double acc123 = 0.0;
-  double acc154 = 0.0;
-  double acc185 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[7];
-      boys_eval_fixed<6>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x7 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x8 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x9 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x10 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x11 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x12 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x13 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x14 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x15 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x16 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x17 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x18 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x19 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x20 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x21 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x22 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x23 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x24 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x54 = s.PA[2] * x21 + s.WP[2] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x25 = s.PA[0] * x7 + s.WP[0] * x10 + z1 * (x0 - s.roz * x1);
-      const TB x26 = s.PA[1] * x7 + s.WP[1] * x10;
-      const TB x27 = s.PA[2] * x7 + s.WP[2] * x10;
-      const TB x28 = s.PA[1] * x8 + s.WP[1] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x30 = s.PA[2] * x9 + s.WP[2] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x31 = s.PA[0] * x10 + s.WP[0] * x13 + z1 * (x1 - s.roz * x2);
-      const TB x32 = s.PA[1] * x10 + s.WP[1] * x13;
-      const TB x33 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x34 = s.PA[1] * x11 + s.WP[1] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[2] * x12 + s.WP[2] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x64 = s.PA[2] * x30 + s.WP[2] * x36 + z2 * (x9 - s.roz * x12);
-      const TB x35 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x37 = s.PA[0] * x13 + s.WP[0] * x16 + z1 * (x2 - s.roz * x3);
-      const TB x38 = s.PA[1] * x13 + s.WP[1] * x16;
-      const TB x39 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x40 = s.PA[1] * x14 + s.WP[1] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[2] * x15 + s.WP[2] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x74 = s.PA[2] * x36 + s.WP[2] * x42 + z2 * (x12 - s.roz * x15);
-      const TB x109 = s.PA[2] * x64 + s.WP[2] * x74 + z3 * (x30 - s.roz * x36);
-      const TB x41 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x43 = s.PA[0] * x16 + s.WP[0] * x19 + z1 * (x3 - s.roz * x4);
-      const TB x44 = s.PA[1] * x16 + s.WP[1] * x19;
-      const TB x45 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x46 = s.PA[1] * x17 + s.WP[1] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x48 = s.PA[2] * x18 + s.WP[2] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x94 = s.PA[2] * x48 + s.WP[2] * x54 + z2 * (x18 - s.roz * x21);
-      const TB x84 = s.PA[2] * x42 + s.WP[2] * x48 + z2 * (x15 - s.roz * x18);
-      const TB x139 = s.PA[2] * x84 + s.WP[2] * x94 + z3 * (x42 - s.roz * x48);
-      const TB x284 = s.QC[2] * x36 + s.WQ[2] * x42 + r2 * x15;
-      const TB x47 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x49 = s.PA[0] * x19 + s.WP[0] * x22 + z1 * (x4 - s.roz * x5);
-      const TB x52 = s.PA[1] * x20 + s.WP[1] * x23 + z1 * (x4 - s.roz * x5);
-      const TB x53 = s.PA[2] * x20 + s.WP[2] * x23;
-      const TB x93 = s.PA[2] * x47 + s.WP[2] * x53 + z1 * (x17 - s.roz * x20);
-      const TB x91 = s.PA[1] * x46 + s.WP[1] * x52 + z2 * (x17 - s.roz * x20);
-      const TB x92 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x50 = s.PA[1] * x19 + s.WP[1] * x22;
-      const TB x51 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x90 = s.PA[2] * x45 + s.WP[2] * x51 + z1 * (x16 - s.roz * x19);
-      const TB x55 = s.PA[0] * x25 + s.WP[0] * x31 + z2 * (x7 - s.roz * x10);
-      const TB x56 = s.PA[1] * x25 + s.WP[1] * x31;
-      const TB x57 = s.PA[2] * x25 + s.WP[2] * x31;
-      const TB x58 = s.PA[1] * x26 + s.WP[1] * x32 + z1 * (x7 - s.roz * x10);
-      const TB x60 = s.PA[2] * x27 + s.WP[2] * x33 + z1 * (x7 - s.roz * x10);
-      const TB x59 = s.PA[2] * x26 + s.WP[2] * x32;
-      const TB x61 = s.PA[1] * x28 + s.WP[1] * x34 + z2 * (x8 - s.roz * x11);
-      const TB x63 = s.PA[2] * x29 + s.WP[2] * x35 + z1 * (x8 - s.roz * x11);
-      const TB x62 = s.PA[2] * x28 + s.WP[2] * x34;
-      const TB x65 = s.PA[0] * x31 + s.WP[0] * x37 + z2 * (x10 - s.roz * x13);
-      const TB x66 = s.PA[1] * x31 + s.WP[1] * x37;
-      const TB x67 = s.PA[2] * x31 + s.WP[2] * x37;
-      const TB x100 = s.PA[2] * x57 + s.WP[2] * x67 + z1 * (x25 - s.roz * x31);
-      const TB x68 = s.PA[1] * x32 + s.WP[1] * x38 + z1 * (x10 - s.roz * x13);
-      const TB x70 = s.PA[2] * x33 + s.WP[2] * x39 + z1 * (x10 - s.roz * x13);
-      const TB x104 = s.PA[2] * x60 + s.WP[2] * x70 + z2 * (x27 - s.roz * x33);
-      const TB x69 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x103 = s.PA[2] * x59 + s.WP[2] * x69 + z1 * (x26 - s.roz * x32);
-      const TB x101 = s.PA[1] * x58 + s.WP[1] * x68 + z2 * (x26 - s.roz * x32);
-      const TB x102 = s.PA[2] * x58 + s.WP[2] * x68;
-      const TB x71 = s.PA[1] * x34 + s.WP[1] * x40 + z2 * (x11 - s.roz * x14);
-      const TB x73 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x11 - s.roz * x14);
-      const TB x108 = s.PA[2] * x63 + s.WP[2] * x73 + z2 * (x29 - s.roz * x35);
-      const TB x72 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x107 = s.PA[2] * x62 + s.WP[2] * x72 + z1 * (x28 - s.roz * x34);
-      const TB x105 = s.PA[1] * x61 + s.WP[1] * x71 + z3 * (x28 - s.roz * x34);
-      const TB x106 = s.PA[2] * x61 + s.WP[2] * x71;
-      const TB x75 = s.PA[0] * x37 + s.WP[0] * x43 + z2 * (x13 - s.roz * x16);
-      const TB x76 = s.PA[1] * x37 + s.WP[1] * x43;
-      const TB x77 = s.PA[2] * x37 + s.WP[2] * x43;
-      const TB x78 = s.PA[1] * x38 + s.WP[1] * x44 + z1 * (x13 - s.roz * x16);
-      const TB x80 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x13 - s.roz * x16);
-      const TB x134 = s.PA[2] * x80 + s.WP[2] * x90 + z2 * (x39 - s.roz * x45);
-      const TB x119 = s.PA[2] * x70 + s.WP[2] * x80 + z2 * (x33 - s.roz * x39);
-      const TB x79 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x81 = s.PA[1] * x40 + s.WP[1] * x46 + z2 * (x14 - s.roz * x17);
-      const TB x83 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x14 - s.roz * x17);
-      const TB x138 = s.PA[2] * x83 + s.WP[2] * x93 + z2 * (x41 - s.roz * x47);
-      const TB x123 = s.PA[2] * x73 + s.WP[2] * x83 + z2 * (x35 - s.roz * x41);
-      const TB x82 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x137 = s.PA[2] * x82 + s.WP[2] * x92 + z1 * (x40 - s.roz * x46);
-      const TB x135 = s.PA[1] * x81 + s.WP[1] * x91 + z3 * (x40 - s.roz * x46);
-      const TB x136 = s.PA[2] * x81 + s.WP[2] * x91;
-      const TB x85 = s.PA[0] * x43 + s.WP[0] * x49 + z2 * (x16 - s.roz * x19);
-      const TB x88 = s.PA[1] * x44 + s.WP[1] * x50 + z1 * (x16 - s.roz * x19);
-      const TB x89 = s.PA[2] * x44 + s.WP[2] * x50;
-      const TB x133 = s.PA[2] * x79 + s.WP[2] * x89 + z1 * (x38 - s.roz * x44);
-      const TB x131 = s.PA[1] * x78 + s.WP[1] * x88 + z2 * (x38 - s.roz * x44);
-      const TB x132 = s.PA[2] * x78 + s.WP[2] * x88;
-      const TB x86 = s.PA[1] * x43 + s.WP[1] * x49;
-      const TB x87 = s.PA[2] * x43 + s.WP[2] * x49;
-      const TB x130 = s.PA[2] * x77 + s.WP[2] * x87 + z1 * (x37 - s.roz * x43);
-      const TB x95 = s.PA[0] * x55 + s.WP[0] * x65 + z3 * (x25 - s.roz * x31);
-      const TB x98 = s.PA[1] * x56 + s.WP[1] * x66 + z1 * (x25 - s.roz * x31);
-      const TB x99 = s.PA[2] * x56 + s.WP[2] * x66;
-      const TB x96 = s.PA[1] * x55 + s.WP[1] * x65;
-      const TB x97 = s.PA[2] * x55 + s.WP[2] * x65;
-      const TB x110 = s.PA[0] * x65 + s.WP[0] * x75 + z3 * (x31 - s.roz * x37);
-      const TB x111 = s.PA[1] * x65 + s.WP[1] * x75;
-      const TB x112 = s.PA[2] * x65 + s.WP[2] * x75;
-      const TB x113 = s.PA[1] * x66 + s.WP[1] * x76 + z1 * (x31 - s.roz * x37);
-      const TB x115 = s.PA[2] * x67 + s.WP[2] * x77 + z1 * (x31 - s.roz * x37);
-      const TB x114 = s.PA[2] * x66 + s.WP[2] * x76;
-      const TB x116 = s.PA[1] * x68 + s.WP[1] * x78 + z2 * (x32 - s.roz * x38);
-      const TB x118 = s.PA[2] * x69 + s.WP[2] * x79 + z1 * (x32 - s.roz * x38);
-      const TB x117 = s.PA[2] * x68 + s.WP[2] * x78;
-      const TB x120 = s.PA[1] * x71 + s.WP[1] * x81 + z3 * (x34 - s.roz * x40);
-      const TB x122 = s.PA[2] * x72 + s.WP[2] * x82 + z1 * (x34 - s.roz * x40);
-      const TB x121 = s.PA[2] * x71 + s.WP[2] * x81;
-      const TB x124 = s.PA[2] * x74 + s.WP[2] * x84 + z3 * (x36 - s.roz * x42);
-      const TB x125 = s.PA[0] * x75 + s.WP[0] * x85 + z3 * (x37 - s.roz * x43);
-      const TB x128 = s.PA[1] * x76 + s.WP[1] * x86 + z1 * (x37 - s.roz * x43);
-      const TB x129 = s.PA[2] * x76 + s.WP[2] * x86;
-      const TB x126 = s.PA[1] * x75 + s.WP[1] * x85;
-      const TB x127 = s.PA[2] * x75 + s.WP[2] * x85;
-      const TB x156 = s.QC[0] * x65 + s.WQ[0] * x75 + r3 * x37;
-      const TB x158 = s.QC[0] * x66 + s.WQ[0] * x76 + r2 * x38;
-      const TB x160 = s.QC[0] * x67 + s.WQ[0] * x77 + r2 * x39;
-      const TB x162 = s.QC[0] * x68 + s.WQ[0] * x78 + r1 * x40;
-      const TB x164 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x41;
-      const TB x166 = s.QC[0] * x70 + s.WQ[0] * x80 + r1 * x42;
-      const TB x168 = s.QC[0] * x71 + s.WQ[0] * x81;
-      const TB x170 = s.QC[0] * x72 + s.WQ[0] * x82;
-      const TB x172 = s.QC[0] * x73 + s.WQ[0] * x83;
-      const TB x174 = s.QC[0] * x74 + s.WQ[0] * x84;
-      const TB x175 = s.QC[0] * x95 + s.WQ[0] * x110 + r4 * x65;
-      const TB x176 = s.QC[0] * x110 + s.WQ[0] * x125 + r4 * x75;
-      const TB x351 = s.QC[0] * x175 + s.WQ[0] * x176 + e1 * (x95 - s.roe * x110) + r4 * x156;
-      acc16 += static_cast<double>(x351);
-      const TB x382 = s.QC[1] * x175 + s.WQ[1] * x176;
-      acc47 += static_cast<double>(x382);
-      const TB x413 = s.QC[2] * x175 + s.WQ[2] * x176;
-      acc78 += static_cast<double>(x413);
-      const TB x177 = s.QC[0] * x96 + s.WQ[0] * x111 + r3 * x66;
-      const TB x178 = s.QC[0] * x111 + s.WQ[0] * x126 + r3 * x76;
-      const TB x352 = s.QC[0] * x177 + s.WQ[0] * x178 + e1 * (x96 - s.roe * x111) + r3 * x158;
-      acc17 += static_cast<double>(x352);
-      const TB x383 = s.QC[1] * x177 + s.WQ[1] * x178 + r1 * x156;
-      acc48 += static_cast<double>(x383);
-      const TB x414 = s.QC[2] * x177 + s.WQ[2] * x178;
-      acc79 += static_cast<double>(x414);
-      const TB x179 = s.QC[0] * x97 + s.WQ[0] * x112 + r3 * x67;
-      const TB x180 = s.QC[0] * x112 + s.WQ[0] * x127 + r3 * x77;
-      const TB x415 = s.QC[2] * x179 + s.WQ[2] * x180 + r1 * x156;
-      acc80 += static_cast<double>(x415);
-      const TB x353 = s.QC[0] * x179 + s.WQ[0] * x180 + e1 * (x97 - s.roe * x112) + r3 * x160;
-      acc18 += static_cast<double>(x353);
-      const TB x384 = s.QC[1] * x179 + s.WQ[1] * x180;
-      acc49 += static_cast<double>(x384);
-      const TB x181 = s.QC[0] * x98 + s.WQ[0] * x113 + r2 * x68;
-      const TB x182 = s.QC[0] * x113 + s.WQ[0] * x128 + r2 * x78;
-      const TB x354 = s.QC[0] * x181 + s.WQ[0] * x182 + e1 * (x98 - s.roe * x113) + r2 * x162;
-      acc19 += static_cast<double>(x354);
-      const TB x385 = s.QC[1] * x181 + s.WQ[1] * x182 + r2 * x158;
-      acc50 += static_cast<double>(x385);
-      const TB x416 = s.QC[2] * x181 + s.WQ[2] * x182;
-      acc81 += static_cast<double>(x416);
-      const TB x183 = s.QC[0] * x99 + s.WQ[0] * x114 + r2 * x69;
-      const TB x184 = s.QC[0] * x114 + s.WQ[0] * x129 + r2 * x79;
-      const TB x417 = s.QC[2] * x183 + s.WQ[2] * x184 + r1 * x158;
-      acc82 += static_cast<double>(x417);
-      const TB x355 = s.QC[0] * x183 + s.WQ[0] * x184 + e1 * (x99 - s.roe * x114) + r2 * x164;
-      acc20 += static_cast<double>(x355);
-      const TB x386 = s.QC[1] * x183 + s.WQ[1] * x184 + r1 * x160;
-      acc51 += static_cast<double>(x386);
-      const TB x185 = s.QC[0] * x100 + s.WQ[0] * x115 + r2 * x70;
-      const TB x186 = s.QC[0] * x115 + s.WQ[0] * x130 + r2 * x80;
-      const TB x418 = s.QC[2] * x185 + s.WQ[2] * x186 + r2 * x160;
-      acc83 += static_cast<double>(x418);
-      const TB x356 = s.QC[0] * x185 + s.WQ[0] * x186 + e1 * (x100 - s.roe * x115) + r2 * x166;
-      acc21 += static_cast<double>(x356);
-      const TB x387 = s.QC[1] * x185 + s.WQ[1] * x186;
-      acc52 += static_cast<double>(x387);
-      const TB x187 = s.QC[0] * x101 + s.WQ[0] * x116 + r1 * x71;
-      const TB x188 = s.QC[0] * x116 + s.WQ[0] * x131 + r1 * x81;
-      const TB x357 = s.QC[0] * x187 + s.WQ[0] * x188 + e1 * (x101 - s.roe * x116) + r1 * x168;
-      acc22 += static_cast<double>(x357);
-      const TB x388 = s.QC[1] * x187 + s.WQ[1] * x188 + r3 * x162;
-      acc53 += static_cast<double>(x388);
-      const TB x419 = s.QC[2] * x187 + s.WQ[2] * x188;
-      acc84 += static_cast<double>(x419);
-      const TB x189 = s.QC[0] * x102 + s.WQ[0] * x117 + r1 * x72;
-      const TB x190 = s.QC[0] * x117 + s.WQ[0] * x132 + r1 * x82;
-      const TB x420 = s.QC[2] * x189 + s.WQ[2] * x190 + r1 * x162;
-      acc85 += static_cast<double>(x420);
-      const TB x358 = s.QC[0] * x189 + s.WQ[0] * x190 + e1 * (x102 - s.roe * x117) + r1 * x170;
-      acc23 += static_cast<double>(x358);
-      const TB x389 = s.QC[1] * x189 + s.WQ[1] * x190 + r2 * x164;
-      acc54 += static_cast<double>(x389);
-      const TB x191 = s.QC[0] * x103 + s.WQ[0] * x118 + r1 * x73;
-      const TB x192 = s.QC[0] * x118 + s.WQ[0] * x133 + r1 * x83;
-      const TB x421 = s.QC[2] * x191 + s.WQ[2] * x192 + r2 * x164;
-      acc86 += static_cast<double>(x421);
-      const TB x359 = s.QC[0] * x191 + s.WQ[0] * x192 + e1 * (x103 - s.roe * x118) + r1 * x172;
-      acc24 += static_cast<double>(x359);
-      const TB x390 = s.QC[1] * x191 + s.WQ[1] * x192 + r1 * x166;
-      acc55 += static_cast<double>(x390);
-      const TB x193 = s.QC[0] * x104 + s.WQ[0] * x119 + r1 * x74;
-      const TB x194 = s.QC[0] * x119 + s.WQ[0] * x134 + r1 * x84;
-      const TB x422 = s.QC[2] * x193 + s.WQ[2] * x194 + r3 * x166;
-      acc87 += static_cast<double>(x422);
-      const TB x360 = s.QC[0] * x193 + s.WQ[0] * x194 + e1 * (x104 - s.roe * x119) + r1 * x174;
-      acc25 += static_cast<double>(x360);
-      const TB x391 = s.QC[1] * x193 + s.WQ[1] * x194;
-      acc56 += static_cast<double>(x391);
-      const TB x195 = s.QC[0] * x105 + s.WQ[0] * x120;
-      const TB x196 = s.QC[0] * x120 + s.WQ[0] * x135;
-      const TB x361 = s.QC[0] * x195 + s.WQ[0] * x196 + e1 * (x105 - s.roe * x120);
-      acc26 += static_cast<double>(x361);
-      const TB x392 = s.QC[1] * x195 + s.WQ[1] * x196 + r4 * x168;
-      acc57 += static_cast<double>(x392);
-      const TB x423 = s.QC[2] * x195 + s.WQ[2] * x196;
-      acc88 += static_cast<double>(x423);
-      const TB x197 = s.QC[0] * x106 + s.WQ[0] * x121;
-      const TB x198 = s.QC[0] * x121 + s.WQ[0] * x136;
-      const TB x424 = s.QC[2] * x197 + s.WQ[2] * x198 + r1 * x168;
-      acc89 += static_cast<double>(x424);
-      const TB x362 = s.QC[0] * x197 + s.WQ[0] * x198 + e1 * (x106 - s.roe * x121);
-      acc27 += static_cast<double>(x362);
-      const TB x393 = s.QC[1] * x197 + s.WQ[1] * x198 + r3 * x170;
-      acc58 += static_cast<double>(x393);
-      const TB x199 = s.QC[0] * x107 + s.WQ[0] * x122;
-      const TB x200 = s.QC[0] * x122 + s.WQ[0] * x137;
-      const TB x425 = s.QC[2] * x199 + s.WQ[2] * x200 + r2 * x170;
-      acc90 += static_cast<double>(x425);
-      const TB x363 = s.QC[0] * x199 + s.WQ[0] * x200 + e1 * (x107 - s.roe * x122);
-      acc28 += static_cast<double>(x363);
-      const TB x394 = s.QC[1] * x199 + s.WQ[1] * x200 + r2 * x172;
-      acc59 += static_cast<double>(x394);
-      const TB x201 = s.QC[0] * x108 + s.WQ[0] * x123;
-      const TB x202 = s.QC[0] * x123 + s.WQ[0] * x138;
-      const TB x426 = s.QC[2] * x201 + s.WQ[2] * x202 + r3 * x172;
-      acc91 += static_cast<double>(x426);
-      const TB x364 = s.QC[0] * x201 + s.WQ[0] * x202 + e1 * (x108 - s.roe * x123);
-      acc29 += static_cast<double>(x364);
-      const TB x395 = s.QC[1] * x201 + s.WQ[1] * x202 + r1 * x174;
-      acc60 += static_cast<double>(x395);
-      const TB x203 = s.QC[0] * x109 + s.WQ[0] * x124;
-      const TB x204 = s.QC[0] * x124 + s.WQ[0] * x139;
-      const TB x427 = s.QC[2] * x203 + s.WQ[2] * x204 + r4 * x174;
-      acc92 += static_cast<double>(x427);
-      const TB x365 = s.QC[0] * x203 + s.WQ[0] * x204 + e1 * (x109 - s.roe * x124);
-      acc30 += static_cast<double>(x365);
-      const TB x396 = s.QC[1] * x203 + s.WQ[1] * x204;
-      acc61 += static_cast<double>(x396);
-      const TB x219 = s.QC[1] * x36 + s.WQ[1] * x42;
-      const TB x303 = s.QC[2] * x64 + s.WQ[2] * x74 + r3 * x36;
-      const TB x221 = s.QC[1] * x65 + s.WQ[1] * x75;
-      const TB x223 = s.QC[1] * x66 + s.WQ[1] * x76 + r1 * x37;
-      const TB x290 = s.QC[2] * x67 + s.WQ[2] * x77 + r1 * x37;
-      const TB x225 = s.QC[1] * x67 + s.WQ[1] * x77;
-      const TB x227 = s.QC[1] * x68 + s.WQ[1] * x78 + r2 * x38;
-      const TB x294 = s.QC[2] * x69 + s.WQ[2] * x79 + r1 * x38;
-      const TB x229 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x39;
-      const TB x296 = s.QC[2] * x70 + s.WQ[2] * x80 + r2 * x39;
-      const TB x231 = s.QC[1] * x70 + s.WQ[1] * x80;
-      const TB x233 = s.QC[1] * x71 + s.WQ[1] * x81 + r3 * x40;
-      const TB x300 = s.QC[2] * x72 + s.WQ[2] * x82 + r1 * x40;
-      const TB x235 = s.QC[1] * x72 + s.WQ[1] * x82 + r2 * x41;
-      const TB x302 = s.QC[2] * x73 + s.WQ[2] * x83 + r2 * x41;
-      const TB x237 = s.QC[1] * x73 + s.WQ[1] * x83 + r1 * x42;
-      const TB x304 = s.QC[2] * x74 + s.WQ[2] * x84 + r3 * x42;
-      const TB x505 = s.QC[2] * x303 + s.WQ[2] * x304 + e1 * (x64 - s.roe * x74) + r3 * x284;
-      acc170 += static_cast<double>(x505);
-      const TB x238 = s.QC[1] * x64 + s.WQ[1] * x74;
-      const TB x239 = s.QC[1] * x74 + s.WQ[1] * x84;
-      const TB x443 = s.QC[1] * x238 + s.WQ[1] * x239 + e1 * (x64 - s.roe * x74);
-      acc108 += static_cast<double>(x443);
-      const TB x474 = s.QC[2] * x238 + s.WQ[2] * x239 + r3 * x219;
-      acc139 += static_cast<double>(x474);
-      const TB x240 = s.QC[1] * x95 + s.WQ[1] * x110;
-      const TB x241 = s.QC[1] * x110 + s.WQ[1] * x125;
-      const TB x306 = s.QC[2] * x110 + s.WQ[2] * x125;
-      const TB x444 = s.QC[1] * x240 + s.WQ[1] * x241 + e1 * (x95 - s.roe * x110);
-      acc109 += static_cast<double>(x444);
-      const TB x475 = s.QC[2] * x240 + s.WQ[2] * x241;
-      acc140 += static_cast<double>(x475);
-      const TB x242 = s.QC[1] * x96 + s.WQ[1] * x111 + r1 * x65;
-      const TB x243 = s.QC[1] * x111 + s.WQ[1] * x126 + r1 * x75;
-      const TB x308 = s.QC[2] * x111 + s.WQ[2] * x126;
-      const TB x445 = s.QC[1] * x242 + s.WQ[1] * x243 + e1 * (x96 - s.roe * x111) + r1 * x221;
-      acc110 += static_cast<double>(x445);
-      const TB x476 = s.QC[2] * x242 + s.WQ[2] * x243;
-      acc141 += static_cast<double>(x476);
-      const TB x244 = s.QC[1] * x97 + s.WQ[1] * x112;
-      const TB x245 = s.QC[1] * x112 + s.WQ[1] * x127;
-      const TB x477 = s.QC[2] * x244 + s.WQ[2] * x245 + r1 * x221;
-      acc142 += static_cast<double>(x477);
-      const TB x446 = s.QC[1] * x244 + s.WQ[1] * x245 + e1 * (x97 - s.roe * x112);
-      acc111 += static_cast<double>(x446);
-      const TB x310 = s.QC[2] * x112 + s.WQ[2] * x127 + r1 * x75;
-      const TB x286 = s.QC[2] * x65 + s.WQ[2] * x75;
-      const TB x309 = s.QC[2] * x97 + s.WQ[2] * x112 + r1 * x65;
-      const TB x508 = s.QC[2] * x309 + s.WQ[2] * x310 + e1 * (x97 - s.roe * x112) + r1 * x286;
-      acc173 += static_cast<double>(x508);
-      const TB x246 = s.QC[1] * x98 + s.WQ[1] * x113 + r2 * x66;
-      const TB x247 = s.QC[1] * x113 + s.WQ[1] * x128 + r2 * x76;
-      const TB x312 = s.QC[2] * x113 + s.WQ[2] * x128;
-      const TB x447 = s.QC[1] * x246 + s.WQ[1] * x247 + e1 * (x98 - s.roe * x113) + r2 * x223;
-      acc112 += static_cast<double>(x447);
-      const TB x478 = s.QC[2] * x246 + s.WQ[2] * x247;
-      acc143 += static_cast<double>(x478);
-      const TB x248 = s.QC[1] * x99 + s.WQ[1] * x114 + r1 * x67;
-      const TB x315 = s.QC[2] * x100 + s.WQ[2] * x115 + r2 * x67;
-      const TB x249 = s.QC[1] * x114 + s.WQ[1] * x129 + r1 * x77;
-      const TB x479 = s.QC[2] * x248 + s.WQ[2] * x249 + r1 * x223;
-      acc144 += static_cast<double>(x479);
-      const TB x448 = s.QC[1] * x248 + s.WQ[1] * x249 + e1 * (x99 - s.roe * x114) + r1 * x225;
-      acc113 += static_cast<double>(x448);
-      const TB x314 = s.QC[2] * x114 + s.WQ[2] * x129 + r1 * x76;
-      const TB x288 = s.QC[2] * x66 + s.WQ[2] * x76;
-      const TB x313 = s.QC[2] * x99 + s.WQ[2] * x114 + r1 * x66;
-      const TB x510 = s.QC[2] * x313 + s.WQ[2] * x314 + e1 * (x99 - s.roe * x114) + r1 * x288;
-      acc175 += static_cast<double>(x510);
-      const TB x316 = s.QC[2] * x115 + s.WQ[2] * x130 + r2 * x77;
-      const TB x511 = s.QC[2] * x315 + s.WQ[2] * x316 + e1 * (x100 - s.roe * x115) + r2 * x290;
-      acc176 += static_cast<double>(x511);
-      const TB x251 = s.QC[1] * x115 + s.WQ[1] * x130;
-      const TB x250 = s.QC[1] * x100 + s.WQ[1] * x115;
-      const TB x449 = s.QC[1] * x250 + s.WQ[1] * x251 + e1 * (x100 - s.roe * x115);
-      acc114 += static_cast<double>(x449);
-      const TB x480 = s.QC[2] * x250 + s.WQ[2] * x251 + r2 * x225;
-      acc145 += static_cast<double>(x480);
-      const TB x252 = s.QC[1] * x101 + s.WQ[1] * x116 + r3 * x68;
-      const TB x253 = s.QC[1] * x116 + s.WQ[1] * x131 + r3 * x78;
-      const TB x318 = s.QC[2] * x116 + s.WQ[2] * x131;
-      const TB x450 = s.QC[1] * x252 + s.WQ[1] * x253 + e1 * (x101 - s.roe * x116) + r3 * x227;
-      acc115 += static_cast<double>(x450);
-      const TB x481 = s.QC[2] * x252 + s.WQ[2] * x253;
-      acc146 += static_cast<double>(x481);
-      const TB x254 = s.QC[1] * x102 + s.WQ[1] * x117 + r2 * x69;
-      const TB x321 = s.QC[2] * x103 + s.WQ[2] * x118 + r2 * x69;
-      const TB x255 = s.QC[1] * x117 + s.WQ[1] * x132 + r2 * x79;
-      const TB x482 = s.QC[2] * x254 + s.WQ[2] * x255 + r1 * x227;
-      acc147 += static_cast<double>(x482);
-      const TB x451 = s.QC[1] * x254 + s.WQ[1] * x255 + e1 * (x102 - s.roe * x117) + r2 * x229;
-      acc116 += static_cast<double>(x451);
-      const TB x320 = s.QC[2] * x117 + s.WQ[2] * x132 + r1 * x78;
-      const TB x292 = s.QC[2] * x68 + s.WQ[2] * x78;
-      const TB x319 = s.QC[2] * x102 + s.WQ[2] * x117 + r1 * x68;
-      const TB x513 = s.QC[2] * x319 + s.WQ[2] * x320 + e1 * (x102 - s.roe * x117) + r1 * x292;
-      acc178 += static_cast<double>(x513);
-      const TB x322 = s.QC[2] * x118 + s.WQ[2] * x133 + r2 * x79;
-      const TB x514 = s.QC[2] * x321 + s.WQ[2] * x322 + e1 * (x103 - s.roe * x118) + r2 * x294;
-      acc179 += static_cast<double>(x514);
-      const TB x257 = s.QC[1] * x118 + s.WQ[1] * x133 + r1 * x80;
-      const TB x324 = s.QC[2] * x119 + s.WQ[2] * x134 + r3 * x80;
-      const TB x259 = s.QC[1] * x119 + s.WQ[1] * x134;
-      const TB x256 = s.QC[1] * x103 + s.WQ[1] * x118 + r1 * x70;
-      const TB x452 = s.QC[1] * x256 + s.WQ[1] * x257 + e1 * (x103 - s.roe * x118) + r1 * x231;
-      acc117 += static_cast<double>(x452);
-      const TB x483 = s.QC[2] * x256 + s.WQ[2] * x257 + r2 * x229;
-      acc148 += static_cast<double>(x483);
-      const TB x323 = s.QC[2] * x104 + s.WQ[2] * x119 + r3 * x70;
-      const TB x515 = s.QC[2] * x323 + s.WQ[2] * x324 + e1 * (x104 - s.roe * x119) + r3 * x296;
-      acc180 += static_cast<double>(x515);
-      const TB x258 = s.QC[1] * x104 + s.WQ[1] * x119;
-      const TB x453 = s.QC[1] * x258 + s.WQ[1] * x259 + e1 * (x104 - s.roe * x119);
-      acc118 += static_cast<double>(x453);
-      const TB x484 = s.QC[2] * x258 + s.WQ[2] * x259 + r3 * x231;
-      acc149 += static_cast<double>(x484);
-      const TB x260 = s.QC[1] * x105 + s.WQ[1] * x120 + r4 * x71;
-      const TB x261 = s.QC[1] * x120 + s.WQ[1] * x135 + r4 * x81;
-      const TB x326 = s.QC[2] * x120 + s.WQ[2] * x135;
-      const TB x454 = s.QC[1] * x260 + s.WQ[1] * x261 + e1 * (x105 - s.roe * x120) + r4 * x233;
-      acc119 += static_cast<double>(x454);
-      const TB x485 = s.QC[2] * x260 + s.WQ[2] * x261;
-      acc150 += static_cast<double>(x485);
-      const TB x262 = s.QC[1] * x106 + s.WQ[1] * x121 + r3 * x72;
-      const TB x329 = s.QC[2] * x107 + s.WQ[2] * x122 + r2 * x72;
-      const TB x263 = s.QC[1] * x121 + s.WQ[1] * x136 + r3 * x82;
-      const TB x486 = s.QC[2] * x262 + s.WQ[2] * x263 + r1 * x233;
-      acc151 += static_cast<double>(x486);
-      const TB x455 = s.QC[1] * x262 + s.WQ[1] * x263 + e1 * (x106 - s.roe * x121) + r3 * x235;
-      acc120 += static_cast<double>(x455);
-      const TB x328 = s.QC[2] * x121 + s.WQ[2] * x136 + r1 * x81;
-      const TB x298 = s.QC[2] * x71 + s.WQ[2] * x81;
-      const TB x327 = s.QC[2] * x106 + s.WQ[2] * x121 + r1 * x71;
-      const TB x517 = s.QC[2] * x327 + s.WQ[2] * x328 + e1 * (x106 - s.roe * x121) + r1 * x298;
-      acc182 += static_cast<double>(x517);
-      const TB x330 = s.QC[2] * x122 + s.WQ[2] * x137 + r2 * x82;
-      const TB x518 = s.QC[2] * x329 + s.WQ[2] * x330 + e1 * (x107 - s.roe * x122) + r2 * x300;
-      acc183 += static_cast<double>(x518);
-      const TB x265 = s.QC[1] * x122 + s.WQ[1] * x137 + r2 * x83;
-      const TB x332 = s.QC[2] * x123 + s.WQ[2] * x138 + r3 * x83;
-      const TB x267 = s.QC[1] * x123 + s.WQ[1] * x138 + r1 * x84;
-      const TB x334 = s.QC[2] * x124 + s.WQ[2] * x139 + r4 * x84;
-      const TB x269 = s.QC[1] * x124 + s.WQ[1] * x139;
-      const TB x264 = s.QC[1] * x107 + s.WQ[1] * x122 + r2 * x73;
-      const TB x456 = s.QC[1] * x264 + s.WQ[1] * x265 + e1 * (x107 - s.roe * x122) + r2 * x237;
-      acc121 += static_cast<double>(x456);
-      const TB x487 = s.QC[2] * x264 + s.WQ[2] * x265 + r2 * x235;
-      acc152 += static_cast<double>(x487);
-      const TB x331 = s.QC[2] * x108 + s.WQ[2] * x123 + r3 * x73;
-      const TB x519 = s.QC[2] * x331 + s.WQ[2] * x332 + e1 * (x108 - s.roe * x123) + r3 * x302;
-      acc184 += static_cast<double>(x519);
-      const TB x266 = s.QC[1] * x108 + s.WQ[1] * x123 + r1 * x74;
-      const TB x457 = s.QC[1] * x266 + s.WQ[1] * x267 + e1 * (x108 - s.roe * x123) + r1 * x239;
-      acc122 += static_cast<double>(x457);
-      const TB x488 = s.QC[2] * x266 + s.WQ[2] * x267 + r3 * x237;
-      acc153 += static_cast<double>(x488);
-      const TB x333 = s.QC[2] * x109 + s.WQ[2] * x124 + r4 * x74;
-      const TB x520 = s.QC[2] * x333 + s.WQ[2] * x334 + e1 * (x109 - s.roe * x124) + r4 * x304;
-      acc185 += static_cast<double>(x520);
-      const TB x268 = s.QC[1] * x109 + s.WQ[1] * x124;
-      const TB x458 = s.QC[1] * x268 + s.WQ[1] * x269 + e1 * (x109 - s.roe * x124);
-      acc123 += static_cast<double>(x458);
-      const TB x489 = s.QC[2] * x268 + s.WQ[2] * x269 + r4 * x239;
-      acc154 += static_cast<double>(x489);
-      const TB x305 = s.QC[2] * x95 + s.WQ[2] * x110;
-      const TB x506 = s.QC[2] * x305 + s.WQ[2] * x306 + e1 * (x95 - s.roe * x110);
-      acc171 += static_cast<double>(x506);
-      const TB x307 = s.QC[2] * x96 + s.WQ[2] * x111;
-      const TB x507 = s.QC[2] * x307 + s.WQ[2] * x308 + e1 * (x96 - s.roe * x111);
-      acc172 += static_cast<double>(x507);
-      const TB x311 = s.QC[2] * x98 + s.WQ[2] * x113;
-      const TB x509 = s.QC[2] * x311 + s.WQ[2] * x312 + e1 * (x98 - s.roe * x113);
-      acc174 += static_cast<double>(x509);
-      const TB x317 = s.QC[2] * x101 + s.WQ[2] * x116;
-      const TB x512 = s.QC[2] * x317 + s.WQ[2] * x318 + e1 * (x101 - s.roe * x116);
-      acc177 += static_cast<double>(x512);
-      const TB x325 = s.QC[2] * x105 + s.WQ[2] * x120;
-      const TB x516 = s.QC[2] * x325 + s.WQ[2] * x326 + e1 * (x105 - s.roe * x120);
-      acc181 += static_cast<double>(x516);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(108 * R + r) * a.acc_stride + q] = acc108;
-  a.acc[(139 * R + r) * a.acc_stride + q] = acc139;
-  a.acc[(170 * R + r) * a.acc_stride + q] = acc170;
-  a.acc[(16 * R + r) * a.acc_stride + q] = acc16;
-  a.acc[(47 * R + r) * a.acc_stride + q] = acc47;
-  a.acc[(78 * R + r) * a.acc_stride + q] = acc78;
-  a.acc[(109 * R + r) * a.acc_stride + q] = acc109;
-  a.acc[(140 * R + r) * a.acc_stride + q] = acc140;
-  a.acc[(171 * R + r) * a.acc_stride + q] = acc171;
-  a.acc[(17 * R + r) * a.acc_stride + q] = acc17;
-  a.acc[(48 * R + r) * a.acc_stride + q] = acc48;
-  a.acc[(79 * R + r) * a.acc_stride + q] = acc79;
-  a.acc[(110 * R + r) * a.acc_stride + q] = acc110;
-  a.acc[(141 * R + r) * a.acc_stride + q] = acc141;
-  a.acc[(172 * R + r) * a.acc_stride + q] = acc172;
-  a.acc[(18 * R + r) * a.acc_stride + q] = acc18;
-  a.acc[(49 * R + r) * a.acc_stride + q] = acc49;
-  a.acc[(80 * R + r) * a.acc_stride + q] = acc80;
-  a.acc[(111 * R + r) * a.acc_stride + q] = acc111;
-  a.acc[(142 * R + r) * a.acc_stride + q] = acc142;
-  a.acc[(173 * R + r) * a.acc_stride + q] = acc173;
-  a.acc[(19 * R + r) * a.acc_stride + q] = acc19;
-  a.acc[(50 * R + r) * a.acc_stride + q] = acc50;
-  a.acc[(81 * R + r) * a.acc_stride + q] = acc81;
-  a.acc[(112 * R + r) * a.acc_stride + q] = acc112;
-  a.acc[(143 * R + r) * a.acc_stride + q] = acc143;
-  a.acc[(174 * R + r) * a.acc_stride + q] = acc174;
-  a.acc[(20 * R + r) * a.acc_stride + q] = acc20;
-  a.acc[(51 * R + r) * a.acc_stride + q] = acc51;
-  a.acc[(82 * R + r) * a.acc_stride + q] = acc82;
-  a.acc[(113 * R + r) * a.acc_stride + q] = acc113;
-  a.acc[(144 * R + r) * a.acc_stride + q] = acc144;
-  a.acc[(175 * R + r) * a.acc_stride + q] = acc175;
-  a.acc[(21 * R + r) * a.acc_stride + q] = acc21;
-  a.acc[(52 * R + r) * a.acc_stride + q] = acc52;
-  a.acc[(83 * R + r) * a.acc_stride + q] = acc83;
-  a.acc[(114 * R + r) * a.acc_stride + q] = acc114;
-  a.acc[(145 * R + r) * a.acc_stride + q] = acc145;
-  a.acc[(176 * R + r) * a.acc_stride + q] = acc176;
-  a.acc[(22 * R + r) * a.acc_stride + q] = acc22;
-  a.acc[(53 * R + r) * a.acc_stride + q] = acc53;
-  a.acc[(84 * R + r) * a.acc_stride + q] = acc84;
-  a.acc[(115 * R + r) * a.acc_stride + q] = acc115;
-  a.acc[(146 * R + r) * a.acc_stride + q] = acc146;
-  a.acc[(177 * R + r) * a.acc_stride + q] = acc177;
-  a.acc[(23 * R + r) * a.acc_stride + q] = acc23;
-  a.acc[(54 * R + r) * a.acc_stride + q] = acc54;
-  a.acc[(85 * R + r) * a.acc_stride + q] = acc85;
-  a.acc[(116 * R + r) * a.acc_stride + q] = acc116;
-  a.acc[(147 * R + r) * a.acc_stride + q] = acc147;
-  a.acc[(178 * R + r) * a.acc_stride + q] = acc178;
-  a.acc[(24 * R + r) * a.acc_stride + q] = acc24;
-  a.acc[(55 * R + r) * a.acc_stride + q] = acc55;
-  a.acc[(86 * R + r) * a.acc_stride + q] = acc86;
-  a.acc[(117 * R + r) * a.acc_stride + q] = acc117;
-  a.acc[(148 * R + r) * a.acc_stride + q] = acc148;
-  a.acc[(179 * R + r) * a.acc_stride + q] = acc179;
-  a.acc[(25 * R + r) * a.acc_stride + q] = acc25;
-  a.acc[(56 * R + r) * a.acc_stride + q] = acc56;
-  a.acc[(87 * R + r) * a.acc_stride + q] = acc87;
-  a.acc[(118 * R + r) * a.acc_stride + q] = acc118;
-  a.acc[(149 * R + r) * a.acc_stride + q] = acc149;
-  a.acc[(180 * R + r) * a.acc_stride + q] = acc180;
-  a.acc[(26 * R + r) * a.acc_stride + q] = acc26;
-  a.acc[(57 * R + r) * a.acc_stride + q] = acc57;
-  a.acc[(88 * R + r) * a.acc_stride + q] = acc88;
-  a.acc[(119 * R + r) * a.acc_stride + q] = acc119;
-  a.acc[(150 * R + r) * a.acc_stride + q] = acc150;
-  a.acc[(181 * R + r) * a.acc_stride + q] = acc181;
-  a.acc[(27 * R + r) * a.acc_stride + q] = acc27;
-  a.acc[(58 * R + r) * a.acc_stride + q] = acc58;
-  a.acc[(89 * R + r) * a.acc_stride + q] = acc89;
-  a.acc[(120 * R + r) * a.acc_stride + q] = acc120;
-  a.acc[(151 * R + r) * a.acc_stride + q] = acc151;
-  a.acc[(182 * R + r) * a.acc_stride + q] = acc182;
-  a.acc[(28 * R + r) * a.acc_stride + q] = acc28;
-  a.acc[(59 * R + r) * a.acc_stride + q] = acc59;
-  a.acc[(90 * R + r) * a.acc_stride + q] = acc90;
-  a.acc[(121 * R + r) * a.acc_stride + q] = acc121;
-  a.acc[(152 * R + r) * a.acc_stride + q] = acc152;
-  a.acc[(183 * R + r) * a.acc_stride + q] = acc183;
-  a.acc[(29 * R + r) * a.acc_stride + q] = acc29;
-  a.acc[(60 * R + r) * a.acc_stride + q] = acc60;
-  a.acc[(91 * R + r) * a.acc_stride + q] = acc91;
-  a.acc[(122 * R + r) * a.acc_stride + q] = acc122;
-  a.acc[(153 * R + r) * a.acc_stride + q] = acc153;
-  a.acc[(184 * R + r) * a.acc_stride + q] = acc184;
-  a.acc[(30 * R + r) * a.acc_stride + q] = acc30;
-  a.acc[(61 * R + r) * a.acc_stride + q] = acc61;
-  a.acc[(92 * R + r) * a.acc_stride + q] = acc92;
-  a.acc[(123 * R + r) * a.acc_stride + q] = acc123;
-  a.acc[(154 * R + r) * a.acc_stride + q] = acc154;
-  a.acc[(185 * R + r) * a.acc_stride + q] = acc185;
-}
-// class (22|20): horizontal recursion per ket column, 186 accumulators
-__global__ __launch_bounds__(kRegThreads) void eri_hrr_2220(gbx_tables t, RegArgs a) {
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  ColumnStore<6, 6, 6, 1> st(t, a, bra, ket, pb, pk);
-  {
-    const ColumnSlot cl = st.column(0, 0);
-    const double v0 = sum_parts(a, 16, q);
-    const double v1 = sum_parts(a, 6, q);
-    const double v2 = v0 + AB[0] * v1;
-    const double v3 = sum_parts(a, 0, q);
-    const double v4 = v1 + AB[0] * v3;
-    const double v5 = v2 + AB[0] * v4;
-    st.put(cl, 0, 0, v5 * 0.19245008972987535);
-    const double v6 = sum_parts(a, 17, q);
-    const double v7 = sum_parts(a, 7, q);
-    const double v8 = v6 + AB[0] * v7;
-    const double v9 = v8 + AB[1] * v4;
-    st.put(cl, 0, 1, v9 * 0.3333333333333334);
-    const double v10 = sum_parts(a, 18, q);
-    const double v11 = sum_parts(a, 8, q);
-    const double v12 = v10 + AB[0] * v11;
-    const double v13 = v12 + AB[2] * v4;
-    st.put(cl, 0, 2, v13 * 0.3333333333333334);
-    const double v14 = sum_parts(a, 19, q);
-    const double v15 = v14 + AB[1] * v7;
-    const double v16 = v7 + AB[1] * v3;
-    const double v17 = v15 + AB[1] * v16;
-    st.put(cl, 0, 3, v17 * 0.19245008972987535);
-    const double v18 = sum_parts(a, 20, q);
-    const double v19 = v18 + AB[1] * v11;
-    const double v20 = v19 + AB[2] * v16;
-    st.put(cl, 0, 4, v20 * 0.3333333333333334);
-    const double v21 = sum_parts(a, 21, q);
-    const double v22 = v21 + AB[2] * v11;
-    const double v23 = v11 + AB[2] * v3;
-    const double v24 = v22 + AB[2] * v23;
-    st.put(cl, 0, 5, v24 * 0.19245008972987535);
-    const double v25 = sum_parts(a, 1, q);
-    const double v26 = v7 + AB[0] * v25;
-    const double v27 = v8 + AB[0] * v26;
-    st.put(cl, 1, 0, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 9, q);
-    const double v29 = v14 + AB[0] * v28;
-    const double v30 = v29 + AB[1] * v26;
-    st.put(cl, 1, 1, v30 * 0.5773502691896258);
-    const double v31 = sum_parts(a, 10, q);
-    const double v32 = v18 + AB[0] * v31;
-    const double v33 = v32 + AB[2] * v26;
-    st.put(cl, 1, 2, v33 * 0.5773502691896258);
-    const double v34 = sum_parts(a, 22, q);
-    const double v35 = v34 + AB[1] * v28;
-    const double v36 = v28 + AB[1] * v25;
-    const double v37 = v35 + AB[1] * v36;
-    st.put(cl, 1, 3, v37 * 0.3333333333333334);
-    const double v38 = sum_parts(a, 23, q);
-    const double v39 = v38 + AB[1] * v31;
-    const double v40 = v39 + AB[2] * v36;
-    st.put(cl, 1, 4, v40 * 0.5773502691896258);
-    const double v41 = sum_parts(a, 24, q);
-    const double v42 = v41 + AB[2] * v31;
-    const double v43 = v31 + AB[2] * v25;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 1, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 2, q);
-    const double v46 = v11 + AB[0] * v45;
-    const double v47 = v12 + AB[0] * v46;
-    st.put(cl, 2, 0, v47 * 0.3333333333333334);
-    const double v48 = v32 + AB[1] * v46;
-    st.put(cl, 2, 1, v48 * 0.5773502691896258);
-    const double v49 = sum_parts(a, 11, q);
-    const double v50 = v21 + AB[0] * v49;
-    const double v51 = v50 + AB[2] * v46;
-    st.put(cl, 2, 2, v51 * 0.5773502691896258);
-    const double v52 = v31 + AB[1] * v45;
-    const double v53 = v39 + AB[1] * v52;
-    st.put(cl, 2, 3, v53 * 0.3333333333333334);
-    const double v54 = v41 + AB[1] * v49;
-    const double v55 = v54 + AB[2] * v52;
-    st.put(cl, 2, 4, v55 * 0.5773502691896258);
-    const double v56 = sum_parts(a, 25, q);
-    const double v57 = v56 + AB[2] * v49;
-    const double v58 = v49 + AB[2] * v45;
-    const double v59 = v57 + AB[2] * v58;
-    st.put(cl, 2, 5, v59 * 0.3333333333333334);
-    const double v60 = sum_parts(a, 3, q);
-    const double v61 = v28 + AB[0] * v60;
-    const double v62 = v29 + AB[0] * v61;
-    st.put(cl, 3, 0, v62 * 0.19245008972987535);
-    const double v63 = sum_parts(a, 12, q);
-    const double v64 = v34 + AB[0] * v63;
-    const double v65 = v64 + AB[1] * v61;
-    st.put(cl, 3, 1, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 13, q);
-    const double v67 = v38 + AB[0] * v66;
-    const double v68 = v67 + AB[2] * v61;
-    st.put(cl, 3, 2, v68 * 0.3333333333333334);
-    const double v69 = sum_parts(a, 26, q);
-    const double v70 = v69 + AB[1] * v63;
-    const double v71 = v63 + AB[1] * v60;
-    const double v72 = v70 + AB[1] * v71;
-    st.put(cl, 3, 3, v72 * 0.19245008972987535);
-    const double v73 = sum_parts(a, 27, q);
-    const double v74 = v73 + AB[1] * v66;
-    const double v75 = v74 + AB[2] * v71;
-    st.put(cl, 3, 4, v75 * 0.3333333333333334);
-    const double v76 = sum_parts(a, 28, q);
-    const double v77 = v76 + AB[2] * v66;
-    const double v78 = v66 + AB[2] * v60;
-    const double v79 = v77 + AB[2] * v78;
-    st.put(cl, 3, 5, v79 * 0.19245008972987535);
-    const double v80 = sum_parts(a, 4, q);
-    const double v81 = v31 + AB[0] * v80;
-    const double v82 = v32 + AB[0] * v81;
-    st.put(cl, 4, 0, v82 * 0.3333333333333334);
-    const double v83 = v67 + AB[1] * v81;
-    st.put(cl, 4, 1, v83 * 0.5773502691896258);
-    const double v84 = sum_parts(a, 14, q);
-    const double v85 = v41 + AB[0] * v84;
-    const double v86 = v85 + AB[2] * v81;
-    st.put(cl, 4, 2, v86 * 0.5773502691896258);
-    const double v87 = v66 + AB[1] * v80;
-    const double v88 = v74 + AB[1] * v87;
-    st.put(cl, 4, 3, v88 * 0.3333333333333334);
-    const double v89 = v76 + AB[1] * v84;
-    const double v90 = v89 + AB[2] * v87;
-    st.put(cl, 4, 4, v90 * 0.5773502691896258);
-    const double v91 = sum_parts(a, 29, q);
-    const double v92 = v91 + AB[2] * v84;
-    const double v93 = v84 + AB[2] * v80;
-    const double v94 = v92 + AB[2] * v93;
-    st.put(cl, 4, 5, v94 * 0.3333333333333334);
-    const double v95 = sum_parts(a, 5, q);
-    const double v96 = v49 + AB[0] * v95;
-    const double v97 = v50 + AB[0] * v96;
-    st.put(cl, 5, 0, v97 * 0.19245008972987535);
-    const double v98 = v85 + AB[1] * v96;
-    st.put(cl, 5, 1, v98 * 0.3333333333333334);
-    const double v99 = sum_parts(a, 15, q);
-    const double v100 = v56 + AB[0] * v99;
-    const double v101 = v100 + AB[2] * v96;
-    st.put(cl, 5, 2, v101 * 0.3333333333333334);
-    const double v102 = v84 + AB[1] * v95;
-    const double v103 = v89 + AB[1] * v102;
-    st.put(cl, 5, 3, v103 * 0.19245008972987535);
-    const double v104 = v91 + AB[1] * v99;
-    const double v105 = v104 + AB[2] * v102;
-    st.put(cl, 5, 4, v105 * 0.3333333333333334);
-    const double v106 = sum_parts(a, 30, q);
-    const double v107 = v106 + AB[2] * v99;
-    const double v108 = v99 + AB[2] * v95;
-    const double v109 = v107 + AB[2] * v108;
-    st.put(cl, 5, 5, v109 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 0);
-    const double v0 = sum_parts(a, 47, q);
-    const double v1 = sum_parts(a, 37, q);
-    const double v2 = v0 + AB[0] * v1;
-    const double v3 = sum_parts(a, 31, q);
-    const double v4 = v1 + AB[0] * v3;
-    const double v5 = v2 + AB[0] * v4;
-    st.put(cl, 0, 0, v5 * 0.3333333333333334);
-    const double v6 = sum_parts(a, 48, q);
-    const double v7 = sum_parts(a, 38, q);
-    const double v8 = v6 + AB[0] * v7;
-    const double v9 = v8 + AB[1] * v4;
-    st.put(cl, 0, 1, v9 * 0.5773502691896258);
-    const double v10 = sum_parts(a, 49, q);
-    const double v11 = sum_parts(a, 39, q);
-    const double v12 = v10 + AB[0] * v11;
-    const double v13 = v12 + AB[2] * v4;
-    st.put(cl, 0, 2, v13 * 0.5773502691896258);
-    const double v14 = sum_parts(a, 50, q);
-    const double v15 = v14 + AB[1] * v7;
-    const double v16 = v7 + AB[1] * v3;
-    const double v17 = v15 + AB[1] * v16;
-    st.put(cl, 0, 3, v17 * 0.3333333333333334);
-    const double v18 = sum_parts(a, 51, q);
-    const double v19 = v18 + AB[1] * v11;
-    const double v20 = v19 + AB[2] * v16;
-    st.put(cl, 0, 4, v20 * 0.5773502691896258);
-    const double v21 = sum_parts(a, 52, q);
-    const double v22 = v21 + AB[2] * v11;
-    const double v23 = v11 + AB[2] * v3;
-    const double v24 = v22 + AB[2] * v23;
-    st.put(cl, 0, 5, v24 * 0.3333333333333334);
-    const double v25 = sum_parts(a, 32, q);
-    const double v26 = v7 + AB[0] * v25;
-    const double v27 = v8 + AB[0] * v26;
-    st.put(cl, 1, 0, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 40, q);
-    const double v29 = v14 + AB[0] * v28;
-    const double v30 = v29 + AB[1] * v26;
-    st.put(cl, 1, 1, v30);
-    const double v31 = sum_parts(a, 41, q);
-    const double v32 = v18 + AB[0] * v31;
-    const double v33 = v32 + AB[2] * v26;
-    st.put(cl, 1, 2, v33);
-    const double v34 = sum_parts(a, 53, q);
-    const double v35 = v34 + AB[1] * v28;
-    const double v36 = v28 + AB[1] * v25;
-    const double v37 = v35 + AB[1] * v36;
-    st.put(cl, 1, 3, v37 * 0.5773502691896258);
-    const double v38 = sum_parts(a, 54, q);
-    const double v39 = v38 + AB[1] * v31;
-    const double v40 = v39 + AB[2] * v36;
-    st.put(cl, 1, 4, v40);
-    const double v41 = sum_parts(a, 55, q);
-    const double v42 = v41 + AB[2] * v31;
-    const double v43 = v31 + AB[2] * v25;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 1, 5, v44 * 0.5773502691896258);
-    const double v45 = sum_parts(a, 33, q);
-    const double v46 = v11 + AB[0] * v45;
-    const double v47 = v12 + AB[0] * v46;
-    st.put(cl, 2, 0, v47 * 0.5773502691896258);
-    const double v48 = v32 + AB[1] * v46;
-    st.put(cl, 2, 1, v48);
-    const double v49 = sum_parts(a, 42, q);
-    const double v50 = v21 + AB[0] * v49;
-    const double v51 = v50 + AB[2] * v46;
-    st.put(cl, 2, 2, v51);
-    const double v52 = v31 + AB[1] * v45;
-    const double v53 = v39 + AB[1] * v52;
-    st.put(cl, 2, 3, v53 * 0.5773502691896258);
-    const double v54 = v41 + AB[1] * v49;
-    const double v55 = v54 + AB[2] * v52;
-    st.put(cl, 2, 4, v55);
-    const double v56 = sum_parts(a, 56, q);
-    const double v57 = v56 + AB[2] * v49;
-    const double v58 = v49 + AB[2] * v45;
-    const double v59 = v57 + AB[2] * v58;
-    st.put(cl, 2, 5, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 34, q);
-    const double v61 = v28 + AB[0] * v60;
-    const double v62 = v29 + AB[0] * v61;
-    st.put(cl, 3, 0, v62 * 0.3333333333333334);
-    const double v63 = sum_parts(a, 43, q);
-    const double v64 = v34 + AB[0] * v63;
-    const double v65 = v64 + AB[1] * v61;
-    st.put(cl, 3, 1, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 44, q);
-    const double v67 = v38 + AB[0] * v66;
-    const double v68 = v67 + AB[2] * v61;
-    st.put(cl, 3, 2, v68 * 0.5773502691896258);
-    const double v69 = sum_parts(a, 57, q);
-    const double v70 = v69 + AB[1] * v63;
-    const double v71 = v63 + AB[1] * v60;
-    const double v72 = v70 + AB[1] * v71;
-    st.put(cl, 3, 3, v72 * 0.3333333333333334);
-    const double v73 = sum_parts(a, 58, q);
-    const double v74 = v73 + AB[1] * v66;
-    const double v75 = v74 + AB[2] * v71;
-    st.put(cl, 3, 4, v75 * 0.5773502691896258);
-    const double v76 = sum_parts(a, 59, q);
-    const double v77 = v76 + AB[2] * v66;
-    const double v78 = v66 + AB[2] * v60;
-    const double v79 = v77 + AB[2] * v78;
-    st.put(cl, 3, 5, v79 * 0.3333333333333334);
-    const double v80 = sum_parts(a, 35, q);
-    const double v81 = v31 + AB[0] * v80;
-    const double v82 = v32 + AB[0] * v81;
-    st.put(cl, 4, 0, v82 * 0.5773502691896258);
-    const double v83 = v67 + AB[1] * v81;
-    st.put(cl, 4, 1, v83);
-    const double v84 = sum_parts(a, 45, q);
-    const double v85 = v41 + AB[0] * v84;
-    const double v86 = v85 + AB[2] * v81;
-    st.put(cl, 4, 2, v86);
-    const double v87 = v66 + AB[1] * v80;
-    const double v88 = v74 + AB[1] * v87;
-    st.put(cl, 4, 3, v88 * 0.5773502691896258);
-    const double v89 = v76 + AB[1] * v84;
-    const double v90 = v89 + AB[2] * v87;
-    st.put(cl, 4, 4, v90);
-    const double v91 = sum_parts(a, 60, q);
-    const double v92 = v91 + AB[2] * v84;
-    const double v93 = v84 + AB[2] * v80;
-    const double v94 = v92 + AB[2] * v93;
-    st.put(cl, 4, 5, v94 * 0.5773502691896258);
-    const double v95 = sum_parts(a, 36, q);
-    const double v96 = v49 + AB[0] * v95;
-    const double v97 = v50 + AB[0] * v96;
-    st.put(cl, 5, 0, v97 * 0.3333333333333334);
-    const double v98 = v85 + AB[1] * v96;
-    st.put(cl, 5, 1, v98 * 0.5773502691896258);
-    const double v99 = sum_parts(a, 46, q);
-    const double v100 = v56 + AB[0] * v99;
-    const double v101 = v100 + AB[2] * v96;
-    st.put(cl, 5, 2, v101 * 0.5773502691896258);
-    const double v102 = v84 + AB[1] * v95;
-    const double v103 = v89 + AB[1] * v102;
-    st.put(cl, 5, 3, v103 * 0.3333333333333334);
-    const double v104 = v91 + AB[1] * v99;
-    const double v105 = v104 + AB[2] * v102;
-    st.put(cl, 5, 4, v105 * 0.5773502691896258);
-    const double v106 = sum_parts(a, 61, q);
-    const double v107 = v106 + AB[2] * v99;
-    const double v108 = v99 + AB[2] * v95;
-    const double v109 = v107 + AB[2] * v108;
-    st.put(cl, 5, 5, v109 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 0);
-    const double v0 = sum_parts(a, 78, q);
-    const double v1 = sum_parts(a, 68, q);
-    const double v2 = v0 + AB[0] * v1;
-    const double v3 = sum_parts(a, 62, q);
-    const double v4 = v1 + AB[0] * v3;
-    const double v5 = v2 + AB[0] * v4;
-    st.put(cl, 0, 0, v5 * 0.3333333333333334);
-    const double v6 = sum_parts(a, 79, q);
-    const double v7 = sum_parts(a, 69, q);
-    const double v8 = v6 + AB[0] * v7;
-    const double v9 = v8 + AB[1] * v4;
-    st.put(cl, 0, 1, v9 * 0.5773502691896258);
-    const double v10 = sum_parts(a, 80, q);
-    const double v11 = sum_parts(a, 70, q);
-    const double v12 = v10 + AB[0] * v11;
-    const double v13 = v12 + AB[2] * v4;
-    st.put(cl, 0, 2, v13 * 0.5773502691896258);
-    const double v14 = sum_parts(a, 81, q);
-    const double v15 = v14 + AB[1] * v7;
-    const double v16 = v7 + AB[1] * v3;
-    const double v17 = v15 + AB[1] * v16;
-    st.put(cl, 0, 3, v17 * 0.3333333333333334);
-    const double v18 = sum_parts(a, 82, q);
-    const double v19 = v18 + AB[1] * v11;
-    const double v20 = v19 + AB[2] * v16;
-    st.put(cl, 0, 4, v20 * 0.5773502691896258);
-    const double v21 = sum_parts(a, 83, q);
-    const double v22 = v21 + AB[2] * v11;
-    const double v23 = v11 + AB[2] * v3;
-    const double v24 = v22 + AB[2] * v23;
-    st.put(cl, 0, 5, v24 * 0.3333333333333334);
-    const double v25 = sum_parts(a, 63, q);
-    const double v26 = v7 + AB[0] * v25;
-    const double v27 = v8 + AB[0] * v26;
-    st.put(cl, 1, 0, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 71, q);
-    const double v29 = v14 + AB[0] * v28;
-    const double v30 = v29 + AB[1] * v26;
-    st.put(cl, 1, 1, v30);
-    const double v31 = sum_parts(a, 72, q);
-    const double v32 = v18 + AB[0] * v31;
-    const double v33 = v32 + AB[2] * v26;
-    st.put(cl, 1, 2, v33);
-    const double v34 = sum_parts(a, 84, q);
-    const double v35 = v34 + AB[1] * v28;
-    const double v36 = v28 + AB[1] * v25;
-    const double v37 = v35 + AB[1] * v36;
-    st.put(cl, 1, 3, v37 * 0.5773502691896258);
-    const double v38 = sum_parts(a, 85, q);
-    const double v39 = v38 + AB[1] * v31;
-    const double v40 = v39 + AB[2] * v36;
-    st.put(cl, 1, 4, v40);
-    const double v41 = sum_parts(a, 86, q);
-    const double v42 = v41 + AB[2] * v31;
-    const double v43 = v31 + AB[2] * v25;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 1, 5, v44 * 0.5773502691896258);
-    const double v45 = sum_parts(a, 64, q);
-    const double v46 = v11 + AB[0] * v45;
-    const double v47 = v12 + AB[0] * v46;
-    st.put(cl, 2, 0, v47 * 0.5773502691896258);
-    const double v48 = v32 + AB[1] * v46;
-    st.put(cl, 2, 1, v48);
-    const double v49 = sum_parts(a, 73, q);
-    const double v50 = v21 + AB[0] * v49;
-    const double v51 = v50 + AB[2] * v46;
-    st.put(cl, 2, 2, v51);
-    const double v52 = v31 + AB[1] * v45;
-    const double v53 = v39 + AB[1] * v52;
-    st.put(cl, 2, 3, v53 * 0.5773502691896258);
-    const double v54 = v41 + AB[1] * v49;
-    const double v55 = v54 + AB[2] * v52;
-    st.put(cl, 2, 4, v55);
-    const double v56 = sum_parts(a, 87, q);
-    const double v57 = v56 + AB[2] * v49;
-    const double v58 = v49 + AB[2] * v45;
-    const double v59 = v57 + AB[2] * v58;
-    st.put(cl, 2, 5, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 65, q);
-    const double v61 = v28 + AB[0] * v60;
-    const double v62 = v29 + AB[0] * v61;
-    st.put(cl, 3, 0, v62 * 0.3333333333333334);
-    const double v63 = sum_parts(a, 74, q);
-    const double v64 = v34 + AB[0] * v63;
-    const double v65 = v64 + AB[1] * v61;
-    st.put(cl, 3, 1, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 75, q);
-    const double v67 = v38 + AB[0] * v66;
-    const double v68 = v67 + AB[2] * v61;
-    st.put(cl, 3, 2, v68 * 0.5773502691896258);
-    const double v69 = sum_parts(a, 88, q);
-    const double v70 = v69 + AB[1] * v63;
-    const double v71 = v63 + AB[1] * v60;
-    const double v72 = v70 + AB[1] * v71;
-    st.put(cl, 3, 3, v72 * 0.3333333333333334);
-    const double v73 = sum_parts(a, 89, q);
-    const double v74 = v73 + AB[1] * v66;
-    const double v75 = v74 + AB[2] * v71;
-    st.put(cl, 3, 4, v75 * 0.5773502691896258);
-    const double v76 = sum_parts(a, 90, q);
-    const double v77 = v76 + AB[2] * v66;
-    const double v78 = v66 + AB[2] * v60;
-    const double v79 = v77 + AB[2] * v78;
-    st.put(cl, 3, 5, v79 * 0.3333333333333334);
-    const double v80 = sum_parts(a, 66, q);
-    const double v81 = v31 + AB[0] * v80;
-    const double v82 = v32 + AB[0] * v81;
-    st.put(cl, 4, 0, v82 * 0.5773502691896258);
-    const double v83 = v67 + AB[1] * v81;
-    st.put(cl, 4, 1, v83);
-    const double v84 = sum_parts(a, 76, q);
-    const double v85 = v41 + AB[0] * v84;
-    const double v86 = v85 + AB[2] * v81;
-    st.put(cl, 4, 2, v86);
-    const double v87 = v66 + AB[1] * v80;
-    const double v88 = v74 + AB[1] * v87;
-    st.put(cl, 4, 3, v88 * 0.5773502691896258);
-    const double v89 = v76 + AB[1] * v84;
-    const double v90 = v89 + AB[2] * v87;
-    st.put(cl, 4, 4, v90);
-    const double v91 = sum_parts(a, 91, q);
-    const double v92 = v91 + AB[2] * v84;
-    const double v93 = v84 + AB[2] * v80;
-    const double v94 = v92 + AB[2] * v93;
-    st.put(cl, 4, 5, v94 * 0.5773502691896258);
-    const double v95 = sum_parts(a, 67, q);
-    const double v96 = v49 + AB[0] * v95;
-    const double v97 = v50 + AB[0] * v96;
-    st.put(cl, 5, 0, v97 * 0.3333333333333334);
-    const double v98 = v85 + AB[1] * v96;
-    st.put(cl, 5, 1, v98 * 0.5773502691896258);
-    const double v99 = sum_parts(a, 77, q);
-    const double v100 = v56 + AB[0] * v99;
-    const double v101 = v100 + AB[2] * v96;
-    st.put(cl, 5, 2, v101 * 0.5773502691896258);
-    const double v102 = v84 + AB[1] * v95;
-    const double v103 = v89 + AB[1] * v102;
-    st.put(cl, 5, 3, v103 * 0.3333333333333334);
-    const double v104 = v91 + AB[1] * v99;
-    const double v105 = v104 + AB[2] * v102;
-    st.put(cl, 5, 4, v105 * 0.5773502691896258);
-    const double v106 = sum_parts(a, 92, q);
-    const double v107 = v106 + AB[2] * v99;
-    const double v108 = v99 + AB[2] * v95;
-    const double v109 = v107 + AB[2] * v108;
-    st.put(cl, 5, 5, v109 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(3, 0);
-    const double v0 = sum_parts(a, 109, q);
-    const double v1 = sum_parts(a, 99, q);
-    const double v2 = v0 + AB[0] * v1;
-    const double v3 = sum_parts(a, 93, q);
-    const double v4 = v1 + AB[0] * v3;
-    const double v5 = v2 + AB[0] * v4;
-    st.put(cl, 0, 0, v5 * 0.19245008972987535);
-    const double v6 = sum_parts(a, 110, q);
-    const double v7 = sum_parts(a, 100, q);
-    const double v8 = v6 + AB[0] * v7;
-    const double v9 = v8 + AB[1] * v4;
-    st.put(cl, 0, 1, v9 * 0.3333333333333334);
-    const double v10 = sum_parts(a, 111, q);
-    const double v11 = sum_parts(a, 101, q);
-    const double v12 = v10 + AB[0] * v11;
-    const double v13 = v12 + AB[2] * v4;
-    st.put(cl, 0, 2, v13 * 0.3333333333333334);
-    const double v14 = sum_parts(a, 112, q);
-    const double v15 = v14 + AB[1] * v7;
-    const double v16 = v7 + AB[1] * v3;
-    const double v17 = v15 + AB[1] * v16;
-    st.put(cl, 0, 3, v17 * 0.19245008972987535);
-    const double v18 = sum_parts(a, 113, q);
-    const double v19 = v18 + AB[1] * v11;
-    const double v20 = v19 + AB[2] * v16;
-    st.put(cl, 0, 4, v20 * 0.3333333333333334);
-    const double v21 = sum_parts(a, 114, q);
-    const double v22 = v21 + AB[2] * v11;
-    const double v23 = v11 + AB[2] * v3;
-    const double v24 = v22 + AB[2] * v23;
-    st.put(cl, 0, 5, v24 * 0.19245008972987535);
-    const double v25 = sum_parts(a, 94, q);
-    const double v26 = v7 + AB[0] * v25;
-    const double v27 = v8 + AB[0] * v26;
-    st.put(cl, 1, 0, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 102, q);
-    const double v29 = v14 + AB[0] * v28;
-    const double v30 = v29 + AB[1] * v26;
-    st.put(cl, 1, 1, v30 * 0.5773502691896258);
-    const double v31 = sum_parts(a, 103, q);
-    const double v32 = v18 + AB[0] * v31;
-    const double v33 = v32 + AB[2] * v26;
-    st.put(cl, 1, 2, v33 * 0.5773502691896258);
-    const double v34 = sum_parts(a, 115, q);
-    const double v35 = v34 + AB[1] * v28;
-    const double v36 = v28 + AB[1] * v25;
-    const double v37 = v35 + AB[1] * v36;
-    st.put(cl, 1, 3, v37 * 0.3333333333333334);
-    const double v38 = sum_parts(a, 116, q);
-    const double v39 = v38 + AB[1] * v31;
-    const double v40 = v39 + AB[2] * v36;
-    st.put(cl, 1, 4, v40 * 0.5773502691896258);
-    const double v41 = sum_parts(a, 117, q);
-    const double v42 = v41 + AB[2] * v31;
-    const double v43 = v31 + AB[2] * v25;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 1, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 95, q);
-    const double v46 = v11 + AB[0] * v45;
-    const double v47 = v12 + AB[0] * v46;
-    st.put(cl, 2, 0, v47 * 0.3333333333333334);
-    const double v48 = v32 + AB[1] * v46;
-    st.put(cl, 2, 1, v48 * 0.5773502691896258);
-    const double v49 = sum_parts(a, 104, q);
-    const double v50 = v21 + AB[0] * v49;
-    const double v51 = v50 + AB[2] * v46;
-    st.put(cl, 2, 2, v51 * 0.5773502691896258);
-    const double v52 = v31 + AB[1] * v45;
-    const double v53 = v39 + AB[1] * v52;
-    st.put(cl, 2, 3, v53 * 0.3333333333333334);
-    const double v54 = v41 + AB[1] * v49;
-    const double v55 = v54 + AB[2] * v52;
-    st.put(cl, 2, 4, v55 * 0.5773502691896258);
-    const double v56 = sum_parts(a, 118, q);
-    const double v57 = v56 + AB[2] * v49;
-    const double v58 = v49 + AB[2] * v45;
-    const double v59 = v57 + AB[2] * v58;
-    st.put(cl, 2, 5, v59 * 0.3333333333333334);
-    const double v60 = sum_parts(a, 96, q);
-    const double v61 = v28 + AB[0] * v60;
-    const double v62 = v29 + AB[0] * v61;
-    st.put(cl, 3, 0, v62 * 0.19245008972987535);
-    const double v63 = sum_parts(a, 105, q);
-    const double v64 = v34 + AB[0] * v63;
-    const double v65 = v64 + AB[1] * v61;
-    st.put(cl, 3, 1, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 106, q);
-    const double v67 = v38 + AB[0] * v66;
-    const double v68 = v67 + AB[2] * v61;
-    st.put(cl, 3, 2, v68 * 0.3333333333333334);
-    const double v69 = sum_parts(a, 119, q);
-    const double v70 = v69 + AB[1] * v63;
-    const double v71 = v63 + AB[1] * v60;
-    const double v72 = v70 + AB[1] * v71;
-    st.put(cl, 3, 3, v72 * 0.19245008972987535);
-    const double v73 = sum_parts(a, 120, q);
-    const double v74 = v73 + AB[1] * v66;
-    const double v75 = v74 + AB[2] * v71;
-    st.put(cl, 3, 4, v75 * 0.3333333333333334);
-    const double v76 = sum_parts(a, 121, q);
-    const double v77 = v76 + AB[2] * v66;
-    const double v78 = v66 + AB[2] * v60;
-    const double v79 = v77 + AB[2] * v78;
-    st.put(cl, 3, 5, v79 * 0.19245008972987535);
-    const double v80 = sum_parts(a, 97, q);
-    const double v81 = v31 + AB[0] * v80;
-    const double v82 = v32 + AB[0] * v81;
-    st.put(cl, 4, 0, v82 * 0.3333333333333334);
-    const double v83 = v67 + AB[1] * v81;
-    st.put(cl, 4, 1, v83 * 0.5773502691896258);
-    const double v84 = sum_parts(a, 107, q);
-    const double v85 = v41 + AB[0] * v84;
-    const double v86 = v85 + AB[2] * v81;
-    st.put(cl, 4, 2, v86 * 0.5773502691896258);
-    const double v87 = v66 + AB[1] * v80;
-    const double v88 = v74 + AB[1] * v87;
-    st.put(cl, 4, 3, v88 * 0.3333333333333334);
-    const double v89 = v76 + AB[1] * v84;
-    const double v90 = v89 + AB[2] * v87;
-    st.put(cl, 4, 4, v90 * 0.5773502691896258);
-    const double v91 = sum_parts(a, 122, q);
-    const double v92 = v91 + AB[2] * v84;
-    const double v93 = v84 + AB[2] * v80;
-    const double v94 = v92 + AB[2] * v93;
-    st.put(cl, 4, 5, v94 * 0.3333333333333334);
-    const double v95 = sum_parts(a, 98, q);
-    const double v96 = v49 + AB[0] * v95;
-    const double v97 = v50 + AB[0] * v96;
-    st.put(cl, 5, 0, v97 * 0.19245008972987535);
-    const double v98 = v85 + AB[1] * v96;
-    st.put(cl, 5, 1, v98 * 0.3333333333333334);
-    const double v99 = sum_parts(a, 108, q);
-    const double v100 = v56 + AB[0] * v99;
-    const double v101 = v100 + AB[2] * v96;
-    st.put(cl, 5, 2, v101 * 0.3333333333333334);
-    const double v102 = v84 + AB[1] * v95;
-    const double v103 = v89 + AB[1] * v102;
-    st.put(cl, 5, 3, v103 * 0.19245008972987535);
-    const double v104 = v91 + AB[1] * v99;
-    const double v105 = v104 + AB[2] * v102;
-    st.put(cl, 5, 4, v105 * 0.3333333333333334);
-    const double v106 = sum_parts(a, 123, q);
-    const double v107 = v106 + AB[2] * v99;
-    const double v108 = v99 + AB[2] * v95;
-    const double v109 = v107 + AB[2] * v108;
-    st.put(cl, 5, 5, v109 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(4, 0);
-    const double v0 = sum_parts(a, 140, q);
-    const double v1 = sum_parts(a, 130, q);
-    const double v2 = v0 + AB[0] * v1;
-    const double v3 = sum_parts(a, 124, q);
-    const double v4 = v1 + AB[0] * v3;
-    const double v5 = v2 + AB[0] * v4;
-    st.put(cl, 0, 0, v5 * 0.3333333333333334);
-    const double v6 = sum_parts(a, 141, q);
-    const double v7 = sum_parts(a, 131, q);
-    const double v8 = v6 + AB[0] * v7;
-    const double v9 = v8 + AB[1] * v4;
-    st.put(cl, 0, 1, v9 * 0.5773502691896258);
-    const double v10 = sum_parts(a, 142, q);
-    const double v11 = sum_parts(a, 132, q);
-    const double v12 = v10 + AB[0] * v11;
-    const double v13 = v12 + AB[2] * v4;
-    st.put(cl, 0, 2, v13 * 0.5773502691896258);
-    const double v14 = sum_parts(a, 143, q);
-    const double v15 = v14 + AB[1] * v7;
-    const double v16 = v7 + AB[1] * v3;
-    const double v17 = v15 + AB[1] * v16;
-    st.put(cl, 0, 3, v17 * 0.3333333333333334);
-    const double v18 = sum_parts(a, 144, q);
-    const double v19 = v18 + AB[1] * v11;
-    const double v20 = v19 + AB[2] * v16;
-    st.put(cl, 0, 4, v20 * 0.5773502691896258);
-    const double v21 = sum_parts(a, 145, q);
-    const double v22 = v21 + AB[2] * v11;
-    const double v23 = v11 + AB[2] * v3;
-    const double v24 = v22 + AB[2] * v23;
-    st.put(cl, 0, 5, v24 * 0.3333333333333334);
-    const double v25 = sum_parts(a, 125, q);
-    const double v26 = v7 + AB[0] * v25;
-    const double v27 = v8 + AB[0] * v26;
-    st.put(cl, 1, 0, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 133, q);
-    const double v29 = v14 + AB[0] * v28;
-    const double v30 = v29 + AB[1] * v26;
-    st.put(cl, 1, 1, v30);
-    const double v31 = sum_parts(a, 134, q);
-    const double v32 = v18 + AB[0] * v31;
-    const double v33 = v32 + AB[2] * v26;
-    st.put(cl, 1, 2, v33);
-    const double v34 = sum_parts(a, 146, q);
-    const double v35 = v34 + AB[1] * v28;
-    const double v36 = v28 + AB[1] * v25;
-    const double v37 = v35 + AB[1] * v36;
-    st.put(cl, 1, 3, v37 * 0.5773502691896258);
-    const double v38 = sum_parts(a, 147, q);
-    const double v39 = v38 + AB[1] * v31;
-    const double v40 = v39 + AB[2] * v36;
-    st.put(cl, 1, 4, v40);
-    const double v41 = sum_parts(a, 148, q);
-    const double v42 = v41 + AB[2] * v31;
-    const double v43 = v31 + AB[2] * v25;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 1, 5, v44 * 0.5773502691896258);
-    const double v45 = sum_parts(a, 126, q);
-    const double v46 = v11 + AB[0] * v45;
-    const double v47 = v12 + AB[0] * v46;
-    st.put(cl, 2, 0, v47 * 0.5773502691896258);
-    const double v48 = v32 + AB[1] * v46;
-    st.put(cl, 2, 1, v48);
-    const double v49 = sum_parts(a, 135, q);
-    const double v50 = v21 + AB[0] * v49;
-    const double v51 = v50 + AB[2] * v46;
-    st.put(cl, 2, 2, v51);
-    const double v52 = v31 + AB[1] * v45;
-    const double v53 = v39 + AB[1] * v52;
-    st.put(cl, 2, 3, v53 * 0.5773502691896258);
-    const double v54 = v41 + AB[1] * v49;
-    const double v55 = v54 + AB[2] * v52;
-    st.put(cl, 2, 4, v55);
-    const double v56 = sum_parts(a, 149, q);
-    const double v57 = v56 + AB[2] * v49;
-    const double v58 = v49 + AB[2] * v45;
-    const double v59 = v57 + AB[2] * v58;
-    st.put(cl, 2, 5, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 127, q);
-    const double v61 = v28 + AB[0] * v60;
-    const double v62 = v29 + AB[0] * v61;
-    st.put(cl, 3, 0, v62 * 0.3333333333333334);
-    const double v63 = sum_parts(a, 136, q);
-    const double v64 = v34 + AB[0] * v63;
-    const double v65 = v64 + AB[1] * v61;
-    st.put(cl, 3, 1, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 137, q);
-    const double v67 = v38 + AB[0] * v66;
-    const double v68 = v67 + AB[2] * v61;
-    st.put(cl, 3, 2, v68 * 0.5773502691896258);
-    const double v69 = sum_parts(a, 150, q);
-    const double v70 = v69 + AB[1] * v63;
-    const double v71 = v63 + AB[1] * v60;
-    const double v72 = v70 + AB[1] * v71;
-    st.put(cl, 3, 3, v72 * 0.3333333333333334);
-    const double v73 = sum_parts(a, 151, q);
-    const double v74 = v73 + AB[1] * v66;
-    const double v75 = v74 + AB[2] * v71;
-    st.put(cl, 3, 4, v75 * 0.5773502691896258);
-    const double v76 = sum_parts(a, 152, q);
-    const double v77 = v76 + AB[2] * v66;
-    const double v78 = v66 + AB[2] * v60;
-    const double v79 = v77 + AB[2] * v78;
-    st.put(cl, 3, 5, v79 * 0.3333333333333334);
-    const double v80 = sum_parts(a, 128, q);
-    const double v81 = v31 + AB[0] * v80;
-    const double v82 = v32 + AB[0] * v81;
-    st.put(cl, 4, 0, v82 * 0.5773502691896258);
-    const double v83 = v67 + AB[1] * v81;
-    st.put(cl, 4, 1, v83);
-    const double v84 = sum_parts(a, 138, q);
-    const double v85 = v41 + AB[0] * v84;
-    const double v86 = v85 + AB[2] * v81;
-    st.put(cl, 4, 2, v86);
-    const double v87 = v66 + AB[1] * v80;
-    const double v88 = v74 + AB[1] * v87;
-    st.put(cl, 4, 3, v88 * 0.5773502691896258);
-    const double v89 = v76 + AB[1] * v84;
-    const double v90 = v89 + AB[2] * v87;
-    st.put(cl, 4, 4, v90);
-    const double v91 = sum_parts(a, 153, q);
-    const double v92 = v91 + AB[2] * v84;
-    const double v93 = v84 + AB[2] * v80;
-    const double v94 = v92 + AB[2] * v93;
-    st.put(cl, 4, 5, v94 * 0.5773502691896258);
-    const double v95 = sum_parts(a, 129, q);
-    const double v96 = v49 + AB[0] * v95;
-    const double v97 = v50 + AB[0] * v96;
-    st.put(cl, 5, 0, v97 * 0.3333333333333334);
-    const double v98 = v85 + AB[1] * v96;
-    st.put(cl, 5, 1, v98 * 0.5773502691896258);
-    const double v99 = sum_parts(a, 139, q);
-    const double v100 = v56 + AB[0] * v99;
-    const double v101 = v100 + AB[2] * v96;
-    st.put(cl, 5, 2, v101 * 0.5773502691896258);
-    const double v102 = v84 + AB[1] * v95;
-    const double v103 = v89 + AB[1] * v102;
-    st.put(cl, 5, 3, v103 * 0.3333333333333334);
-    const double v104 = v91 + AB[1] * v99;
-    const double v105 = v104 + AB[2] * v102;
-    st.put(cl, 5, 4, v105 * 0.5773502691896258);
-    const double v106 = sum_parts(a, 154, q);
-    const double v107 = v106 + AB[2] * v99;
-    const double v108 = v99 + AB[2] * v95;
-    const double v109 = v107 + AB[2] * v108;
-    st.put(cl, 5, 5, v109 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(5, 0);
-    const double v0 = sum_parts(a, 171, q);
-    const double v1 = sum_parts(a, 161, q);
-    const double v2 = v0 + AB[0] * v1;
-    const double v3 = sum_parts(a, 155, q);
-    const double v4 = v1 + AB[0] * v3;
-    const double v5 = v2 + AB[0] * v4;
-    st.put(cl, 0, 0, v5 * 0.19245008972987535);
-    const double v6 = sum_parts(a, 172, q);
-    const double v7 = sum_parts(a, 162, q);
-    const double v8 = v6 + AB[0] * v7;
-    const double v9 = v8 + AB[1] * v4;
-    st.put(cl, 0, 1, v9 * 0.3333333333333334);
-    const double v10 = sum_parts(a, 173, q);
-    const double v11 = sum_parts(a, 163, q);
-    const double v12 = v10 + AB[0] * v11;
-    const double v13 = v12 + AB[2] * v4;
-    st.put(cl, 0, 2, v13 * 0.3333333333333334);
-    const double v14 = sum_parts(a, 174, q);
-    const double v15 = v14 + AB[1] * v7;
-    const double v16 = v7 + AB[1] * v3;
-    const double v17 = v15 + AB[1] * v16;
-    st.put(cl, 0, 3, v17 * 0.19245008972987535);
-    const double v18 = sum_parts(a, 175, q);
-    const double v19 = v18 + AB[1] * v11;
-    const double v20 = v19 + AB[2] * v16;
-    st.put(cl, 0, 4, v20 * 0.3333333333333334);
-    const double v21 = sum_parts(a, 176, q);
-    const double v22 = v21 + AB[2] * v11;
-    const double v23 = v11 + AB[2] * v3;
-    const double v24 = v22 + AB[2] * v23;
-    st.put(cl, 0, 5, v24 * 0.19245008972987535);
-    const double v25 = sum_parts(a, 156, q);
-    const double v26 = v7 + AB[0] * v25;
-    const double v27 = v8 + AB[0] * v26;
-    st.put(cl, 1, 0, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 164, q);
-    const double v29 = v14 + AB[0] * v28;
-    const double v30 = v29 + AB[1] * v26;
-    st.put(cl, 1, 1, v30 * 0.5773502691896258);
-    const double v31 = sum_parts(a, 165, q);
-    const double v32 = v18 + AB[0] * v31;
-    const double v33 = v32 + AB[2] * v26;
-    st.put(cl, 1, 2, v33 * 0.5773502691896258);
-    const double v34 = sum_parts(a, 177, q);
-    const double v35 = v34 + AB[1] * v28;
-    const double v36 = v28 + AB[1] * v25;
-    const double v37 = v35 + AB[1] * v36;
-    st.put(cl, 1, 3, v37 * 0.3333333333333334);
-    const double v38 = sum_parts(a, 178, q);
-    const double v39 = v38 + AB[1] * v31;
-    const double v40 = v39 + AB[2] * v36;
-    st.put(cl, 1, 4, v40 * 0.5773502691896258);
-    const double v41 = sum_parts(a, 179, q);
-    const double v42 = v41 + AB[2] * v31;
-    const double v43 = v31 + AB[2] * v25;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 1, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 157, q);
-    const double v46 = v11 + AB[0] * v45;
-    const double v47 = v12 + AB[0] * v46;
-    st.put(cl, 2, 0, v47 * 0.3333333333333334);
-    const double v48 = v32 + AB[1] * v46;
-    st.put(cl, 2, 1, v48 * 0.5773502691896258);
-    const double v49 = sum_parts(a, 166, q);
-    const double v50 = v21 + AB[0] * v49;
-    const double v51 = v50 + AB[2] * v46;
-    st.put(cl, 2, 2, v51 * 0.5773502691896258);
-    const double v52 = v31 + AB[1] * v45;
-    const double v53 = v39 + AB[1] * v52;
-    st.put(cl, 2, 3, v53 * 0.3333333333333334);
-    const double v54 = v41 + AB[1] * v49;
-    const double v55 = v54 + AB[2] * v52;
-    st.put(cl, 2, 4, v55 * 0.5773502691896258);
-    const double v56 = sum_parts(a, 180, q);
-    const double v57 = v56 + AB[2] * v49;
-    const double v58 = v49 + AB[2] * v45;
-    const double v59 = v57 + AB[2] * v58;
-    st.put(cl, 2, 5, v59 * 0.3333333333333334);
-    const double v60 = sum_parts(a, 158, q);
-    const double v61 = v28 + AB[0] * v60;
-    const double v62 = v29 + AB[0] * v61;
-    st.put(cl, 3, 0, v62 * 0.19245008972987535);
-    const double v63 = sum_parts(a, 167, q);
-    const double v64 = v34 + AB[0] * v63;
-    const double v65 = v64 + AB[1] * v61;
-    st.put(cl, 3, 1, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 168, q);
-    const double v67 = v38 + AB[0] * v66;
-    const double v68 = v67 + AB[2] * v61;
-    st.put(cl, 3, 2, v68 * 0.3333333333333334);
-    const double v69 = sum_parts(a, 181, q);
-    const double v70 = v69 + AB[1] * v63;
-    const double v71 = v63 + AB[1] * v60;
-    const double v72 = v70 + AB[1] * v71;
-    st.put(cl, 3, 3, v72 * 0.19245008972987535);
-    const double v73 = sum_parts(a, 182, q);
-    const double v74 = v73 + AB[1] * v66;
-    const double v75 = v74 + AB[2] * v71;
-    st.put(cl, 3, 4, v75 * 0.3333333333333334);
-    const double v76 = sum_parts(a, 183, q);
-    const double v77 = v76 + AB[2] * v66;
-    const double v78 = v66 + AB[2] * v60;
-    const double v79 = v77 + AB[2] * v78;
-    st.put(cl, 3, 5, v79 * 0.19245008972987535);
-    const double v80 = sum_parts(a, 159, q);
-    const double v81 = v31 + AB[0] * v80;
-    const double v82 = v32 + AB[0] * v81;
-    st.put(cl, 4, 0, v82 * 0.3333333333333334);
-    const double v83 = v67 + AB[1] * v81;
-    st.put(cl, 4, 1, v83 * 0.5773502691896258);
-    const double v84 = sum_parts(a, 169, q);
-    const double v85 = v41 + AB[0] * v84;
-    const double v86 = v85 + AB[2] * v81;
-    st.put(cl, 4, 2, v86 * 0.5773502691896258);
-    const double v87 = v66 + AB[1] * v80;
-    const double v88 = v74 + AB[1] * v87;
-    st.put(cl, 4, 3, v88 * 0.3333333333333334);
-    const double v89 = v76 + AB[1] * v84;
-    const double v90 = v89 + AB[2] * v87;
-    st.put(cl, 4, 4, v90 * 0.5773502691896258);
-    const double v91 = sum_parts(a, 184, q);
-    const double v92 = v91 + AB[2] * v84;
-    const double v93 = v84 + AB[2] * v80;
-    const double v94 = v92 + AB[2] * v93;
-    st.put(cl, 4, 5, v94 * 0.3333333333333334);
-    const double v95 = sum_parts(a, 160, q);
-    const double v96 = v49 + AB[0] * v95;
-    const double v97 = v50 + AB[0] * v96;
-    st.put(cl, 5, 0, v97 * 0.19245008972987535);
-    const double v98 = v85 + AB[1] * v96;
-    st.put(cl, 5, 1, v98 * 0.3333333333333334);
-    const double v99 = sum_parts(a, 170, q);
-    const double v100 = v56 + AB[0] * v99;
-    const double v101 = v100 + AB[2] * v96;
-    st.put(cl, 5, 2, v101 * 0.3333333333333334);
-    const double v102 = v84 + AB[1] * v95;
-    const double v103 = v89 + AB[1] * v102;
-    st.put(cl, 5, 3, v103 * 0.19245008972987535);
-    const double v104 = v91 + AB[1] * v99;
-    const double v105 = v104 + AB[2] * v102;
-    st.put(cl, 5, 4, v105 * 0.3333333333333334);
-    const double v106 = sum_parts(a, 185, q);
-    const double v107 = v106 + AB[2] * v99;
-    const double v108 = v99 + AB[2] * v95;
-    const double v109 = v107 + AB[2] * v108;
-    st.put(cl, 5, 5, v109 * 0.19245008972987535);
-  }
-}
-
-// class (22|21) split 0/6: 229 recursion values, 82 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2221_0(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc31 = 0.0;
-  double acc62 = 0.0;
-  double acc93 = 0.0;
-  double acc124 = 0.0;
-  double acc155 = 0.0;
-  double acc186 = 0.0;
-  double acc217 = 0.0;
-  double acc248 = 0.0;
-  double acc279 = 0.0;
-  double acc310 = 0.0;
-  double acc341 = 0.0;
-  double acc372 = 0.0;
-  double acc403 = 0.0;
-  double acc434 = 0.0;
-  double acc465 = 0.0;
-  double acc1 = 0.0;
-  double acc32 = 0.0;
-  double acc63 = 0.0;
-  double acc94 = 0.0;
-  double acc125 = 0.0;
-  double acc156 = 0.0;
-  double acc187 = 0.0;
-  double acc218 = 0.0;
-  double acc249 = 0.0;
-  double acc280 = 0.0;
-  double acc311 = 0.0;
-  double acc342 = 0.0;
-  double acc373 = 0.0;
-  double acc404 = 0.0;
-  double acc435 = 0.0;
-  double acc466 = 0.0;
-  double acc2 = 0.0;
-  double acc33 = 0.0;
-  double acc64 = 0.0;
-  double acc95 = 0.0;
-  double acc126 = 0.0;
-  double acc157 = 0.0;
-  double acc188 = 0.0;
-  double acc219 = 0.0;
-  double acc250 = 0.0;
-  double acc281 = 0.0;
-  double acc312 = 0.0;
-  double acc343 = 0.0;
-  double acc374 = 0.0;
-  double acc405 = 0.0;
-  double acc436 = 0.0;
-  double acc467 = 0.0;
-  double acc3 = 0.0;
-  double acc34 = 0.0;
-  double acc65 = 0.0;
-  double acc96 = 0.0;
-  double acc127 = 0.0;
-  double acc158 = 0.0;
-  double acc189 = 0.0;
-  double acc220 = 0.0;
-  double acc251 = 0.0;
-  double acc282 = 0.0;
-  double acc313 = 0.0;
-  double acc344 = 0.0;
-  double acc375 = 0.0;
-  double acc406 = 0.0;
-  double acc437 = 0.0;
-  double acc468 = 0.0;
-  double acc4 = 0.0;
-  double acc35 = 0.0;
-  double acc66 = 0.0;
-  double acc97 = 0.0;
-  double acc128 = 0.0;
-  double acc159 = 0.0;
-  double acc190 = 0.0;
-  double acc221 = 0.0;
-  double acc252 = 0.0;
-  double acc283 = 0.0;
-  double acc314 = 0.0;
-  double acc345 = 0.0;
-  double acc376 = 0.0;
-  double acc407 = 0.0;
-  double acc438 = 0.0;
-  double acc469 = 0.0;
-  double acc5 = 0.0;
-  double acc36 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[8];
-      boys_eval_fixed<7>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x8 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x9 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x10 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x11 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x12 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x13 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x34 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x14 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x15 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x16 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x17 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x18 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x19 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x20 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x21 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x29 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x32 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x33 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x30 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x31 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x35 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x37 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x38 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x40 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x39 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x41 = s.PA[0] * x14 + s.WP[0] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[1] * x14 + s.WP[1] * x17;
-      const TB x43 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x44 = s.PA[1] * x15 + s.WP[1] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x45 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x46 = s.PA[2] * x16 + s.WP[2] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x198 = s.QC[0] * x40 + s.WQ[0] * x46;
-      const TB x47 = s.PA[0] * x17 + s.WP[0] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x50 = s.PA[1] * x18 + s.WP[1] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x51 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x48 = s.PA[1] * x17 + s.WP[1] * x20;
-      const TB x49 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x175 = s.QC[0] * x2 + s.WQ[0] * x3;
-      const TB x176 = s.QC[0] * x11 + s.WQ[0] * x14 + r1 * x2;
-      const TB x177 = s.QC[0] * x14 + s.WQ[0] * x17 + r1 * x3;
-      const TB x178 = s.QC[0] * x12 + s.WQ[0] * x15;
-      const TB x179 = s.QC[0] * x15 + s.WQ[0] * x18;
-      const TB x180 = s.QC[0] * x13 + s.WQ[0] * x16;
-      const TB x181 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x182 = s.QC[0] * x29 + s.WQ[0] * x35 + r2 * x11;
-      const TB x183 = s.QC[0] * x35 + s.WQ[0] * x41 + r2 * x14;
-      const TB x184 = s.QC[0] * x41 + s.WQ[0] * x47 + r2 * x17;
-      const TB x185 = s.QC[0] * x30 + s.WQ[0] * x36 + r1 * x12;
-      const TB x186 = s.QC[0] * x36 + s.WQ[0] * x42 + r1 * x15;
-      const TB x187 = s.QC[0] * x42 + s.WQ[0] * x48 + r1 * x18;
-      const TB x188 = s.QC[0] * x31 + s.WQ[0] * x37 + r1 * x13;
-      const TB x189 = s.QC[0] * x37 + s.WQ[0] * x43 + r1 * x16;
-      const TB x190 = s.QC[0] * x43 + s.WQ[0] * x49 + r1 * x19;
-      const TB x191 = s.QC[0] * x32 + s.WQ[0] * x38;
-      const TB x192 = s.QC[0] * x38 + s.WQ[0] * x44;
-      const TB x193 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x194 = s.QC[0] * x33 + s.WQ[0] * x39;
-      const TB x195 = s.QC[0] * x39 + s.WQ[0] * x45;
-      const TB x196 = s.QC[0] * x45 + s.WQ[0] * x51;
-      const TB x197 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x488 = s.QC[0] * x197 + s.WQ[0] * x198 + e1 * (x34 - s.roe * x40);
-      acc5 += static_cast<double>(x488);
-      const TB x553 = s.QC[1] * x197 + s.WQ[1] * x198;
-      acc36 += static_cast<double>(x553);
-      const TB x275 = s.QC[1] * x2 + s.WQ[1] * x3;
-      const TB x278 = s.QC[1] * x12 + s.WQ[1] * x15 + r1 * x2;
-      const TB x279 = s.QC[1] * x15 + s.WQ[1] * x18 + r1 * x3;
-      const TB x671 = s.QC[1] * x278 + s.WQ[1] * x279 + e1 * (x12 - s.roe * x15) + r1 * x275;
-      const TB x276 = s.QC[1] * x11 + s.WQ[1] * x14;
-      const TB x277 = s.QC[1] * x14 + s.WQ[1] * x17;
-      const TB x280 = s.QC[1] * x13 + s.WQ[1] * x16;
-      const TB x281 = s.QC[1] * x16 + s.WQ[1] * x19;
-      const TB x296 = s.QC[1] * x45 + s.WQ[1] * x51 + r1 * x19;
-      const TB x396 = s.QC[2] * x45 + s.WQ[2] * x51 + r1 * x18;
-      const TB x282 = s.QC[1] * x29 + s.WQ[1] * x35;
-      const TB x283 = s.QC[1] * x35 + s.WQ[1] * x41;
-      const TB x284 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x384 = s.QC[2] * x41 + s.WQ[2] * x47;
-      const TB x285 = s.QC[1] * x30 + s.WQ[1] * x36 + r1 * x11;
-      const TB x286 = s.QC[1] * x36 + s.WQ[1] * x42 + r1 * x14;
-      const TB x287 = s.QC[1] * x42 + s.WQ[1] * x48 + r1 * x17;
-      const TB x387 = s.QC[2] * x42 + s.WQ[2] * x48;
-      const TB x288 = s.QC[1] * x31 + s.WQ[1] * x37;
-      const TB x289 = s.QC[1] * x37 + s.WQ[1] * x43;
-      const TB x290 = s.QC[1] * x43 + s.WQ[1] * x49;
-      const TB x390 = s.QC[2] * x43 + s.WQ[2] * x49 + r1 * x17;
-      const TB x377 = s.QC[2] * x14 + s.WQ[2] * x17;
-      const TB x291 = s.QC[1] * x32 + s.WQ[1] * x38 + r2 * x12;
-      const TB x292 = s.QC[1] * x38 + s.WQ[1] * x44 + r2 * x15;
-      const TB x293 = s.QC[1] * x44 + s.WQ[1] * x50 + r2 * x18;
-      const TB x379 = s.QC[2] * x15 + s.WQ[2] * x18;
-      const TB x393 = s.QC[2] * x44 + s.WQ[2] * x50;
-      const TB x294 = s.QC[1] * x33 + s.WQ[1] * x39 + r1 * x13;
-      const TB x295 = s.QC[1] * x39 + s.WQ[1] * x45 + r1 * x16;
-      const TB x376 = s.QC[2] * x11 + s.WQ[2] * x14;
-      const TB x378 = s.QC[2] * x12 + s.WQ[2] * x15;
-      const TB x382 = s.QC[2] * x29 + s.WQ[2] * x35;
-      const TB x383 = s.QC[2] * x35 + s.WQ[2] * x41;
-      const TB x804 = s.QC[2] * x383 + s.WQ[2] * x384 + e1 * (x35 - s.roe * x41);
-      const TB x385 = s.QC[2] * x30 + s.WQ[2] * x36;
-      const TB x386 = s.QC[2] * x36 + s.WQ[2] * x42;
-      const TB x806 = s.QC[2] * x386 + s.WQ[2] * x387 + e1 * (x36 - s.roe * x42);
-      const TB x388 = s.QC[2] * x31 + s.WQ[2] * x37 + r1 * x11;
-      const TB x389 = s.QC[2] * x37 + s.WQ[2] * x43 + r1 * x14;
-      const TB x808 = s.QC[2] * x389 + s.WQ[2] * x390 + e1 * (x37 - s.roe * x43) + r1 * x377;
-      const TB x800 = s.QC[2] * x376 + s.WQ[2] * x377 + e1 * (x11 - s.roe * x14);
-      const TB x807 = s.QC[2] * x388 + s.WQ[2] * x389 + e1 * (x31 - s.roe * x37) + r1 * x376;
-      acc157 += static_cast<double>(x807);
-      const TB x1146 = s.QC[2] * x807 + s.WQ[2] * x808 + e2 * (x388 - s.roe * x389) + r1 * x800;
-      acc467 += static_cast<double>(x1146);
-      const TB x391 = s.QC[2] * x32 + s.WQ[2] * x38;
-      const TB x392 = s.QC[2] * x38 + s.WQ[2] * x44;
-      const TB x810 = s.QC[2] * x392 + s.WQ[2] * x393 + e1 * (x38 - s.roe * x44);
-      const TB x394 = s.QC[2] * x33 + s.WQ[2] * x39 + r1 * x12;
-      const TB x395 = s.QC[2] * x39 + s.WQ[2] * x45 + r1 * x15;
-      const TB x812 = s.QC[2] * x395 + s.WQ[2] * x396 + e1 * (x39 - s.roe * x45) + r1 * x379;
-      const TB x801 = s.QC[2] * x378 + s.WQ[2] * x379 + e1 * (x12 - s.roe * x15);
-      const TB x476 = s.QC[0] * x178 + s.WQ[0] * x179 + e1 * (x12 - s.roe * x15);
-      const TB x811 = s.QC[2] * x394 + s.WQ[2] * x395 + e1 * (x33 - s.roe * x39) + r1 * x378;
-      acc159 += static_cast<double>(x811);
-      const TB x1148 = s.QC[2] * x811 + s.WQ[2] * x812 + e2 * (x394 - s.roe * x395) + r1 * x801;
-      acc469 += static_cast<double>(x1148);
-      const TB x475 = s.QC[0] * x176 + s.WQ[0] * x177 + e1 * (x11 - s.roe * x14) + r1 * x175;
-      const TB x670 = s.QC[1] * x276 + s.WQ[1] * x277 + e1 * (x11 - s.roe * x14);
-      const TB x541 = s.QC[1] * x178 + s.WQ[1] * x179 + r1 * x175;
-      const TB x477 = s.QC[0] * x180 + s.WQ[0] * x181 + e1 * (x13 - s.roe * x16);
-      const TB x672 = s.QC[1] * x280 + s.WQ[1] * x281 + e1 * (x13 - s.roe * x16);
-      const TB x681 = s.QC[1] * x294 + s.WQ[1] * x295 + e1 * (x33 - s.roe * x39) + r1 * x280;
-      acc97 += static_cast<double>(x681);
-      const TB x486 = s.QC[0] * x194 + s.WQ[0] * x195 + e1 * (x33 - s.roe * x39);
-      acc4 += static_cast<double>(x486);
-      const TB x682 = s.QC[1] * x295 + s.WQ[1] * x296 + e1 * (x39 - s.roe * x45) + r1 * x281;
-      const TB x487 = s.QC[0] * x195 + s.WQ[0] * x196 + e1 * (x39 - s.roe * x45);
-      const TB x1055 = s.QC[1] * x681 + s.WQ[1] * x682 + e2 * (x294 - s.roe * x295) + r1 * x672;
-      acc376 += static_cast<double>(x1055);
-      const TB x1086 = s.QC[2] * x681 + s.WQ[2] * x682 + r1 * x671;
-      acc407 += static_cast<double>(x1086);
-      const TB x747 = s.QC[2] * x295 + s.WQ[2] * x296 + r1 * x279;
-      const TB x869 = s.QC[0] * x486 + s.WQ[0] * x487 + e2 * (x194 - s.roe * x195);
-      acc190 += static_cast<double>(x869);
-      const TB x900 = s.QC[1] * x486 + s.WQ[1] * x487 + r1 * x477;
-      acc221 += static_cast<double>(x900);
-      const TB x931 = s.QC[2] * x486 + s.WQ[2] * x487 + r1 * x476;
-      acc252 += static_cast<double>(x931);
-      const TB x478 = s.QC[0] * x182 + s.WQ[0] * x183 + e1 * (x29 - s.roe * x35) + r2 * x176;
-      acc0 += static_cast<double>(x478);
-      const TB x479 = s.QC[0] * x183 + s.WQ[0] * x184 + e1 * (x35 - s.roe * x41) + r2 * x177;
-      const TB x674 = s.QC[1] * x283 + s.WQ[1] * x284 + e1 * (x35 - s.roe * x41);
-      const TB x739 = s.QC[2] * x283 + s.WQ[2] * x284;
-      const TB x865 = s.QC[0] * x478 + s.WQ[0] * x479 + e2 * (x182 - s.roe * x183) + r2 * x475;
-      acc186 += static_cast<double>(x865);
-      const TB x896 = s.QC[1] * x478 + s.WQ[1] * x479;
-      acc217 += static_cast<double>(x896);
-      const TB x927 = s.QC[2] * x478 + s.WQ[2] * x479;
-      acc248 += static_cast<double>(x927);
-      const TB x480 = s.QC[0] * x185 + s.WQ[0] * x186 + e1 * (x30 - s.roe * x36) + r1 * x178;
-      acc1 += static_cast<double>(x480);
-      const TB x481 = s.QC[0] * x186 + s.WQ[0] * x187 + e1 * (x36 - s.roe * x42) + r1 * x179;
-      const TB x676 = s.QC[1] * x286 + s.WQ[1] * x287 + e1 * (x36 - s.roe * x42) + r1 * x277;
-      const TB x741 = s.QC[2] * x286 + s.WQ[2] * x287;
-      const TB x866 = s.QC[0] * x480 + s.WQ[0] * x481 + e2 * (x185 - s.roe * x186) + r1 * x476;
-      acc187 += static_cast<double>(x866);
-      const TB x897 = s.QC[1] * x480 + s.WQ[1] * x481 + r1 * x475;
-      acc218 += static_cast<double>(x897);
-      const TB x928 = s.QC[2] * x480 + s.WQ[2] * x481;
-      acc249 += static_cast<double>(x928);
-      const TB x482 = s.QC[0] * x188 + s.WQ[0] * x189 + e1 * (x31 - s.roe * x37) + r1 * x180;
-      acc2 += static_cast<double>(x482);
-      const TB x677 = s.QC[1] * x288 + s.WQ[1] * x289 + e1 * (x31 - s.roe * x37);
-      acc95 += static_cast<double>(x677);
-      const TB x483 = s.QC[0] * x189 + s.WQ[0] * x190 + e1 * (x37 - s.roe * x43) + r1 * x181;
-      const TB x678 = s.QC[1] * x289 + s.WQ[1] * x290 + e1 * (x37 - s.roe * x43);
-      const TB x867 = s.QC[0] * x482 + s.WQ[0] * x483 + e2 * (x188 - s.roe * x189) + r1 * x477;
-      acc188 += static_cast<double>(x867);
-      const TB x929 = s.QC[2] * x482 + s.WQ[2] * x483 + r1 * x475;
-      acc250 += static_cast<double>(x929);
-      const TB x898 = s.QC[1] * x482 + s.WQ[1] * x483;
-      acc219 += static_cast<double>(x898);
-      const TB x743 = s.QC[2] * x289 + s.WQ[2] * x290 + r1 * x277;
-      const TB x735 = s.QC[2] * x276 + s.WQ[2] * x277;
-      const TB x1053 = s.QC[1] * x677 + s.WQ[1] * x678 + e2 * (x288 - s.roe * x289);
-      acc374 += static_cast<double>(x1053);
-      const TB x1084 = s.QC[2] * x677 + s.WQ[2] * x678 + r1 * x670;
-      acc405 += static_cast<double>(x1084);
-      const TB x484 = s.QC[0] * x191 + s.WQ[0] * x192 + e1 * (x32 - s.roe * x38);
-      acc3 += static_cast<double>(x484);
-      const TB x485 = s.QC[0] * x192 + s.WQ[0] * x193 + e1 * (x38 - s.roe * x44);
-      const TB x899 = s.QC[1] * x484 + s.WQ[1] * x485 + r2 * x476;
-      acc220 += static_cast<double>(x899);
-      const TB x680 = s.QC[1] * x292 + s.WQ[1] * x293 + e1 * (x38 - s.roe * x44) + r2 * x279;
-      const TB x736 = s.QC[2] * x278 + s.WQ[2] * x279;
-      const TB x745 = s.QC[2] * x292 + s.WQ[2] * x293;
-      const TB x868 = s.QC[0] * x484 + s.WQ[0] * x485 + e2 * (x191 - s.roe * x192);
-      acc189 += static_cast<double>(x868);
-      const TB x930 = s.QC[2] * x484 + s.WQ[2] * x485;
-      acc251 += static_cast<double>(x930);
-      const TB x540 = s.QC[1] * x176 + s.WQ[1] * x177;
-      const TB x542 = s.QC[1] * x180 + s.WQ[1] * x181;
-      const TB x551 = s.QC[1] * x194 + s.WQ[1] * x195 + r1 * x180;
-      acc35 += static_cast<double>(x551);
-      const TB x552 = s.QC[1] * x195 + s.WQ[1] * x196 + r1 * x181;
-      const TB x962 = s.QC[1] * x551 + s.WQ[1] * x552 + e1 * (x194 - s.roe * x195) + r1 * x542;
-      acc283 += static_cast<double>(x962);
-      const TB x993 = s.QC[2] * x551 + s.WQ[2] * x552 + r1 * x541;
-      acc314 += static_cast<double>(x993);
-      const TB x617 = s.QC[2] * x195 + s.WQ[2] * x196 + r1 * x179;
-      const TB x543 = s.QC[1] * x182 + s.WQ[1] * x183;
-      acc31 += static_cast<double>(x543);
-      const TB x544 = s.QC[1] * x183 + s.WQ[1] * x184;
-      const TB x609 = s.QC[2] * x183 + s.WQ[2] * x184;
-      const TB x958 = s.QC[1] * x543 + s.WQ[1] * x544 + e1 * (x182 - s.roe * x183);
-      acc279 += static_cast<double>(x958);
-      const TB x989 = s.QC[2] * x543 + s.WQ[2] * x544;
-      acc310 += static_cast<double>(x989);
-      const TB x545 = s.QC[1] * x185 + s.WQ[1] * x186 + r1 * x176;
-      acc32 += static_cast<double>(x545);
-      const TB x546 = s.QC[1] * x186 + s.WQ[1] * x187 + r1 * x177;
-      const TB x611 = s.QC[2] * x186 + s.WQ[2] * x187;
-      const TB x959 = s.QC[1] * x545 + s.WQ[1] * x546 + e1 * (x185 - s.roe * x186) + r1 * x540;
-      acc280 += static_cast<double>(x959);
-      const TB x990 = s.QC[2] * x545 + s.WQ[2] * x546;
-      acc311 += static_cast<double>(x990);
-      const TB x547 = s.QC[1] * x188 + s.WQ[1] * x189;
-      acc33 += static_cast<double>(x547);
-      const TB x548 = s.QC[1] * x189 + s.WQ[1] * x190;
-      const TB x991 = s.QC[2] * x547 + s.WQ[2] * x548 + r1 * x540;
-      acc312 += static_cast<double>(x991);
-      const TB x960 = s.QC[1] * x547 + s.WQ[1] * x548 + e1 * (x188 - s.roe * x189);
-      acc281 += static_cast<double>(x960);
-      const TB x613 = s.QC[2] * x189 + s.WQ[2] * x190 + r1 * x177;
-      const TB x605 = s.QC[2] * x176 + s.WQ[2] * x177;
-      const TB x612 = s.QC[2] * x188 + s.WQ[2] * x189 + r1 * x176;
-      acc64 += static_cast<double>(x612);
-      const TB x1022 = s.QC[2] * x612 + s.WQ[2] * x613 + e1 * (x188 - s.roe * x189) + r1 * x605;
-      acc343 += static_cast<double>(x1022);
-      const TB x549 = s.QC[1] * x191 + s.WQ[1] * x192 + r2 * x178;
-      acc34 += static_cast<double>(x549);
-      const TB x550 = s.QC[1] * x192 + s.WQ[1] * x193 + r2 * x179;
-      const TB x961 = s.QC[1] * x549 + s.WQ[1] * x550 + e1 * (x191 - s.roe * x192) + r2 * x541;
-      acc282 += static_cast<double>(x961);
-      const TB x992 = s.QC[2] * x549 + s.WQ[2] * x550;
-      acc313 += static_cast<double>(x992);
-      const TB x606 = s.QC[2] * x178 + s.WQ[2] * x179;
-      const TB x615 = s.QC[2] * x192 + s.WQ[2] * x193;
-      const TB x616 = s.QC[2] * x194 + s.WQ[2] * x195 + r1 * x178;
-      acc66 += static_cast<double>(x616);
-      const TB x1024 = s.QC[2] * x616 + s.WQ[2] * x617 + e1 * (x194 - s.roe * x195) + r1 * x606;
-      acc345 += static_cast<double>(x1024);
-      const TB x608 = s.QC[2] * x182 + s.WQ[2] * x183;
-      acc62 += static_cast<double>(x608);
-      const TB x1020 = s.QC[2] * x608 + s.WQ[2] * x609 + e1 * (x182 - s.roe * x183);
-      acc341 += static_cast<double>(x1020);
-      const TB x610 = s.QC[2] * x185 + s.WQ[2] * x186;
-      acc63 += static_cast<double>(x610);
-      const TB x1021 = s.QC[2] * x610 + s.WQ[2] * x611 + e1 * (x185 - s.roe * x186);
-      acc342 += static_cast<double>(x1021);
-      const TB x614 = s.QC[2] * x191 + s.WQ[2] * x192;
-      acc65 += static_cast<double>(x614);
-      const TB x1023 = s.QC[2] * x614 + s.WQ[2] * x615 + e1 * (x191 - s.roe * x192);
-      acc344 += static_cast<double>(x1023);
-      const TB x673 = s.QC[1] * x282 + s.WQ[1] * x283 + e1 * (x29 - s.roe * x35);
-      acc93 += static_cast<double>(x673);
-      const TB x803 = s.QC[2] * x382 + s.WQ[2] * x383 + e1 * (x29 - s.roe * x35);
-      acc155 += static_cast<double>(x803);
-      const TB x1144 = s.QC[2] * x803 + s.WQ[2] * x804 + e2 * (x382 - s.roe * x383);
-      acc465 += static_cast<double>(x1144);
-      const TB x1051 = s.QC[1] * x673 + s.WQ[1] * x674 + e2 * (x282 - s.roe * x283);
-      acc372 += static_cast<double>(x1051);
-      const TB x1082 = s.QC[2] * x673 + s.WQ[2] * x674;
-      acc403 += static_cast<double>(x1082);
-      const TB x675 = s.QC[1] * x285 + s.WQ[1] * x286 + e1 * (x30 - s.roe * x36) + r1 * x276;
-      acc94 += static_cast<double>(x675);
-      const TB x805 = s.QC[2] * x385 + s.WQ[2] * x386 + e1 * (x30 - s.roe * x36);
-      acc156 += static_cast<double>(x805);
-      const TB x1145 = s.QC[2] * x805 + s.WQ[2] * x806 + e2 * (x385 - s.roe * x386);
-      acc466 += static_cast<double>(x1145);
-      const TB x1052 = s.QC[1] * x675 + s.WQ[1] * x676 + e2 * (x285 - s.roe * x286) + r1 * x670;
-      acc373 += static_cast<double>(x1052);
-      const TB x1083 = s.QC[2] * x675 + s.WQ[2] * x676;
-      acc404 += static_cast<double>(x1083);
-      const TB x742 = s.QC[2] * x288 + s.WQ[2] * x289 + r1 * x276;
-      acc126 += static_cast<double>(x742);
-      const TB x1115 = s.QC[2] * x742 + s.WQ[2] * x743 + e1 * (x288 - s.roe * x289) + r1 * x735;
-      acc436 += static_cast<double>(x1115);
-      const TB x679 = s.QC[1] * x291 + s.WQ[1] * x292 + e1 * (x32 - s.roe * x38) + r2 * x278;
-      acc96 += static_cast<double>(x679);
-      const TB x809 = s.QC[2] * x391 + s.WQ[2] * x392 + e1 * (x32 - s.roe * x38);
-      acc158 += static_cast<double>(x809);
-      const TB x1147 = s.QC[2] * x809 + s.WQ[2] * x810 + e2 * (x391 - s.roe * x392);
-      acc468 += static_cast<double>(x1147);
-      const TB x1054 = s.QC[1] * x679 + s.WQ[1] * x680 + e2 * (x291 - s.roe * x292) + r2 * x671;
-      acc375 += static_cast<double>(x1054);
-      const TB x1085 = s.QC[2] * x679 + s.WQ[2] * x680;
-      acc406 += static_cast<double>(x1085);
-      const TB x746 = s.QC[2] * x294 + s.WQ[2] * x295 + r1 * x278;
-      acc128 += static_cast<double>(x746);
-      const TB x1117 = s.QC[2] * x746 + s.WQ[2] * x747 + e1 * (x294 - s.roe * x295) + r1 * x736;
-      acc438 += static_cast<double>(x1117);
-      const TB x738 = s.QC[2] * x282 + s.WQ[2] * x283;
-      acc124 += static_cast<double>(x738);
-      const TB x1113 = s.QC[2] * x738 + s.WQ[2] * x739 + e1 * (x282 - s.roe * x283);
-      acc434 += static_cast<double>(x1113);
-      const TB x740 = s.QC[2] * x285 + s.WQ[2] * x286;
-      acc125 += static_cast<double>(x740);
-      const TB x1114 = s.QC[2] * x740 + s.WQ[2] * x741 + e1 * (x285 - s.roe * x286);
-      acc435 += static_cast<double>(x1114);
-      const TB x744 = s.QC[2] * x291 + s.WQ[2] * x292;
-      acc127 += static_cast<double>(x744);
-      const TB x1116 = s.QC[2] * x744 + s.WQ[2] * x745 + e1 * (x291 - s.roe * x292);
-      acc437 += static_cast<double>(x1116);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(0 * R + r) * a.acc_stride + q] = acc0;
-  a.acc[(31 * R + r) * a.acc_stride + q] = acc31;
-  a.acc[(62 * R + r) * a.acc_stride + q] = acc62;
-  a.acc[(93 * R + r) * a.acc_stride + q] = acc93;
-  a.acc[(124 * R + r) * a.acc_stride + q] = acc124;
-  a.acc[(155 * R + r) * a.acc_stride + q] = acc155;
-  a.acc[(186 * R + r) * a.acc_stride + q] = acc186;
-  a.acc[(217 * R + r) * a.acc_stride + q] = acc217;
-  a.acc[(248 * R + r) * a.acc_stride + q] = acc248;
-  a.acc[(279 * R + r) * a.acc_stride + q] = acc279;
-  a.acc[(310 * R + r) * a.acc_stride + q] = acc310;
-  a.acc[(341 * R + r) * a.acc_stride + q] = acc341;
-  a.acc[(372 * R + r) * a.acc_stride + q] = acc372;
-  a.acc[(403 * R + r) * a.acc_stride + q] = acc403;
-  a.acc[(434 * R + r) * a.acc_stride + q] = acc434;
-  a.acc[(465 * R + r) * a.acc_stride + q] = acc465;
-  a.acc[(1 * R + r) * a.acc_stride + q] = acc1;
-  a.acc[(32 * R + r) * a.acc_stride + q] = acc32;
-  a.acc[(63 * R + r) * a.acc_stride + q] = acc63;
-  a.acc[(94 * R + r) * a.acc_stride + q] = acc94;
-  a.acc[(125 * R + r) * a.acc_stride + q] = acc125;
-  a.acc[(156 * R + r) * a.acc_stride + q] = acc156;
-  a.acc[(187 * R + r) * a.acc_stride + q] = acc187;
-  a.acc[(218 * R + r) * a.acc_stride + q] = acc218;
-  a.acc[(249 * R + r) * a.acc_stride + q] = acc249;
-  a.acc[(280 * R + r) * a.acc_stride + q] = acc280;
-  a.acc[(311 * R + r) * a.acc_stride + q] = acc311;
-  a.acc[(342 * R + r) * a.acc_stride + q] = acc342;
-  a.acc[(373 * R + r) * a.acc_stride + q] = acc373;
-  a.acc[(404 * R + r) * a.acc_stride + q] = acc404;
-  a.acc[(435 * R + r) * a.acc_stride + q] = acc435;
-  a.acc[(466 * R + r) * a.acc_stride + q] = acc466;
-  a.acc[(2 * R + r) * a.acc_stride + q] = acc2;
-  a.acc[(33 * R + r) * a.acc_stride + q] = acc33;
-  a.acc[(64 * R + r) * a.acc_stride + q] = acc64;
-  a.acc[(95 * R + r) * a.acc_stride + q] = acc95;
-  a.acc[(126 * R + r) * a.acc_stride + q] = acc126;
-  a.acc[(157 * R + r) * a.acc_stride + q] = acc157;
-  a.acc[(188 * R + r) * a.acc_stride + q] = acc188;
-  a.acc[(219 * R + r) * a.acc_stride + q] = acc219;
-  a.acc[(250 * R + r) * a.acc_stride + q] = acc250;
-  a.acc[(281 * R + r) * a.acc_stride + q] = acc281;
-  a.acc[(312 * R + r) * a.acc_stride + q] = acc312;
-  a.acc[(343 * R + r) * a.acc_stride + q] = acc343;
-  a.acc[(374 * R + r) * a.acc_stride + q] = acc374;
-  a.acc[(405 * R + r) * a.acc_stride + q] = acc405;
-  a.acc[(436 * R + r) * a.acc_stride + q] = acc436;
-  a.acc[(467 * R + r) * a.acc_stride + q] = acc467;
-  a.acc[(3 * R + r) * a.acc_stride + q] = acc3;
-  a.acc[(34 * R + r) * a.acc_stride + q] = acc34;
-  a.acc[(65 * R + r) * a.acc_stride + q] = acc65;
-  a.acc[(96 * R + r) * a.acc_stride + q] = acc96;
-  a.acc[(127 * R + r) * a.acc_stride + q] = acc127;
-  a.acc[(158 * R + r) * a.acc_stride + q] = acc158;
-  a.acc[(189 * R + r) * a.acc_stride + q] = acc189;
-  a.acc[(220 * R + r) * a.acc_stride + q] = acc220;
-  a.acc[(251 * R + r) * a.acc_stride + q] = acc251;
-  a.acc[(282 * R + r) * a.acc_stride + q] = acc282;
-  a.acc[(313 * R + r) * a.acc_stride + q] = acc313;
-  a.acc[(344 * R + r) * a.acc_stride + q] = acc344;
-  a.acc[(375 * R + r) * a.acc_stride + q] = acc375;
-  a.acc[(406 * R + r) * a.acc_stride + q] = acc406;
-  a.acc[(437 * R + r) * a.acc_stride + q] = acc437;
-  a.acc[(468 * R + r) * a.acc_stride + q] = acc468;
-  a.acc[(4 * R + r) * a.acc_stride + q] = acc4;
-  a.acc[(35 * R + r) * a.acc_stride + q] = acc35;
-  a.acc[(66 * R + r) * a.acc_stride + q] = acc66;
-  a.acc[(97 * R + r) * a.acc_stride + q] = acc97;
-  a.acc[(128 * R + r) * a.acc_stride + q] = acc128;
-  a.acc[(159 * R + r) * a.acc_stride + q] = acc159;
-  a.acc[(190 * R + r) * a.acc_stride + q] = acc190;
-  a.acc[(221 * R + r) * a.acc_stride + q] = acc221;
-  a.acc[(252 * R + r) * a.acc_stride + q] = acc252;
-  a.acc[(283 * R + r) * a.acc_stride + q] = acc283;
-  a.acc[(314 * R + r) * a.acc_stride + q] = acc314;
-  a.acc[(345 * R + r) * a.acc_stride + q] = acc345;
-  a.acc[(376 * R + r) * a.acc_stride + q] = acc376;
-  a.acc[(407 * R + r) * a.acc_stride + q] = acc407;
-  a.acc[(438 * R + r) * a.acc_stride + q] = acc438;
-  a.acc[(469 * R + r) * a.acc_stride + q] = acc469;
-  a.acc[(5 * R + r) * a.acc_stride + q] = acc5;
-  a.acc[(36 * R + r) * a.acc_stride + q] = acc36;
-}
-// class (22|21) split 1/6: 265 recursion values, 83 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2221_1(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc67 = 0.0;
-  double acc98 = 0.0;
-  double acc129 = 0.0;
-  double acc160 = 0.0;
-  double acc191 = 0.0;
-  double acc222 = 0.0;
-  double acc253 = 0.0;
-  double acc284 = 0.0;
-  double acc315 = 0.0;
-  double acc346 = 0.0;
-  double acc377 = 0.0;
-  double acc408 = 0.0;
-  double acc439 = 0.0;
-  double acc470 = 0.0;
-  double acc6 = 0.0;
-  double acc37 = 0.0;
-  double acc68 = 0.0;
-  double acc99 = 0.0;
-  double acc130 = 0.0;
-  double acc161 = 0.0;
-  double acc192 = 0.0;
-  double acc223 = 0.0;
-  double acc254 = 0.0;
-  double acc285 = 0.0;
-  double acc316 = 0.0;
-  double acc347 = 0.0;
-  double acc378 = 0.0;
-  double acc409 = 0.0;
-  double acc440 = 0.0;
-  double acc471 = 0.0;
-  double acc7 = 0.0;
-  double acc38 = 0.0;
-  double acc69 = 0.0;
-  double acc100 = 0.0;
-  double acc131 = 0.0;
-  double acc162 = 0.0;
-  double acc193 = 0.0;
-  double acc224 = 0.0;
-  double acc255 = 0.0;
-  double acc286 = 0.0;
-  double acc317 = 0.0;
-  double acc348 = 0.0;
-  double acc379 = 0.0;
-  double acc410 = 0.0;
-  double acc441 = 0.0;
-  double acc472 = 0.0;
-  double acc8 = 0.0;
-  double acc39 = 0.0;
-  double acc70 = 0.0;
-  double acc101 = 0.0;
-  double acc132 = 0.0;
-  double acc163 = 0.0;
-  double acc194 = 0.0;
-  double acc225 = 0.0;
-  double acc256 = 0.0;
-  double acc287 = 0.0;
-  double acc318 = 0.0;
-  double acc349 = 0.0;
-  double acc380 = 0.0;
-  double acc411 = 0.0;
-  double acc442 = 0.0;
-  double acc473 = 0.0;
-  double acc9 = 0.0;
-  double acc40 = 0.0;
-  double acc71 = 0.0;
-  double acc102 = 0.0;
-  double acc133 = 0.0;
-  double acc164 = 0.0;
-  double acc195 = 0.0;
-  double acc226 = 0.0;
-  double acc257 = 0.0;
-  double acc288 = 0.0;
-  double acc319 = 0.0;
-  double acc350 = 0.0;
-  double acc381 = 0.0;
-  double acc412 = 0.0;
-  double acc443 = 0.0;
-  double acc474 = 0.0;
-  double acc10 = 0.0;
-  double acc41 = 0.0;
-  double acc72 = 0.0;
-  double acc103 = 0.0;
-  double acc134 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[8];
-      boys_eval_fixed<7>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x23 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x8 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x10 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x11 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x12 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x13 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x34 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x29 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x14 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x15 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x16 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x17 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x18 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x19 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x20 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x21 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x50 = s.PA[1] * x18 + s.WP[1] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x22 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x52 = s.PA[2] * x19 + s.WP[2] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x53 = s.PA[0] * x20 + s.WP[0] * x23 + z1 * (x4 - s.roz * x5);
-      const TB x47 = s.PA[0] * x17 + s.WP[0] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x54 = s.PA[1] * x20 + s.WP[1] * x23;
-      const TB x30 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x35 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x37 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x38 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x39 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x40 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x41 = s.PA[0] * x14 + s.WP[0] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[1] * x14 + s.WP[1] * x17;
-      const TB x43 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x44 = s.PA[1] * x15 + s.WP[1] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x45 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x46 = s.PA[2] * x16 + s.WP[2] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x48 = s.PA[1] * x17 + s.WP[1] * x20;
-      const TB x98 = s.PA[1] * x48 + s.WP[1] * x54 + z1 * (x17 - s.roz * x20);
-      const TB x49 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x95 = s.PA[0] * x47 + s.WP[0] * x53 + z2 * (x17 - s.roz * x20);
-      const TB x65 = s.PA[0] * x29 + s.WP[0] * x35 + z2 * (x8 - s.roz * x11);
-      const TB x68 = s.PA[1] * x30 + s.WP[1] * x36 + z1 * (x8 - s.roz * x11);
-      const TB x69 = s.PA[2] * x30 + s.WP[2] * x36;
-      const TB x66 = s.PA[1] * x29 + s.WP[1] * x35;
-      const TB x67 = s.PA[2] * x29 + s.WP[2] * x35;
-      const TB x75 = s.PA[0] * x35 + s.WP[0] * x41 + z2 * (x11 - s.roz * x14);
-      const TB x78 = s.PA[1] * x36 + s.WP[1] * x42 + z1 * (x11 - s.roz * x14);
-      const TB x76 = s.PA[1] * x35 + s.WP[1] * x41;
-      const TB x77 = s.PA[2] * x35 + s.WP[2] * x41;
-      const TB x79 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x85 = s.PA[0] * x41 + s.WP[0] * x47 + z2 * (x14 - s.roz * x17);
-      const TB x86 = s.PA[1] * x41 + s.WP[1] * x47;
-      const TB x87 = s.PA[2] * x41 + s.WP[2] * x47;
-      const TB x88 = s.PA[1] * x42 + s.WP[1] * x48 + z1 * (x14 - s.roz * x17);
-      const TB x89 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x96 = s.PA[1] * x47 + s.WP[1] * x53;
-      const TB x97 = s.PA[2] * x47 + s.WP[2] * x53;
-      const TB x175 = s.QC[0] * x2 + s.WQ[0] * x3;
-      const TB x177 = s.QC[0] * x14 + s.WQ[0] * x17 + r1 * x3;
-      const TB x179 = s.QC[0] * x15 + s.WQ[0] * x18;
-      const TB x186 = s.QC[0] * x36 + s.WQ[0] * x42 + r1 * x15;
-      const TB x187 = s.QC[0] * x42 + s.WQ[0] * x48 + r1 * x18;
-      const TB x481 = s.QC[0] * x186 + s.WQ[0] * x187 + e1 * (x36 - s.roe * x42) + r1 * x179;
-      const TB x180 = s.QC[0] * x13 + s.WQ[0] * x16;
-      const TB x181 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x607 = s.QC[2] * x180 + s.WQ[2] * x181 + r1 * x175;
-      const TB x183 = s.QC[0] * x35 + s.WQ[0] * x41 + r2 * x14;
-      const TB x184 = s.QC[0] * x41 + s.WQ[0] * x47 + r2 * x17;
-      const TB x189 = s.QC[0] * x37 + s.WQ[0] * x43 + r1 * x16;
-      const TB x190 = s.QC[0] * x43 + s.WQ[0] * x49 + r1 * x19;
-      const TB x208 = s.QC[0] * x87 + s.WQ[0] * x97 + r2 * x49;
-      const TB x192 = s.QC[0] * x38 + s.WQ[0] * x44;
-      const TB x193 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x211 = s.QC[0] * x88 + s.WQ[0] * x98 + r1 * x50;
-      const TB x195 = s.QC[0] * x39 + s.WQ[0] * x45;
-      const TB x212 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x39;
-      const TB x213 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x45;
-      const TB x498 = s.QC[0] * x212 + s.WQ[0] * x213 + e1 * (x69 - s.roe * x79) + r1 * x195;
-      acc10 += static_cast<double>(x498);
-      const TB x313 = s.QC[1] * x79 + s.WQ[1] * x89 + r1 * x43;
-      const TB x563 = s.QC[1] * x212 + s.WQ[1] * x213 + r1 * x189;
-      acc41 += static_cast<double>(x563);
-      const TB x628 = s.QC[2] * x212 + s.WQ[2] * x213 + r1 * x186;
-      acc72 += static_cast<double>(x628);
-      const TB x197 = s.QC[0] * x34 + s.WQ[0] * x40;
-      const TB x198 = s.QC[0] * x40 + s.WQ[0] * x46;
-      const TB x199 = s.QC[0] * x46 + s.WQ[0] * x52;
-      const TB x200 = s.QC[0] * x65 + s.WQ[0] * x75 + r3 * x35;
-      const TB x201 = s.QC[0] * x75 + s.WQ[0] * x85 + r3 * x41;
-      const TB x202 = s.QC[0] * x85 + s.WQ[0] * x95 + r3 * x47;
-      const TB x203 = s.QC[0] * x66 + s.WQ[0] * x76 + r2 * x36;
-      const TB x204 = s.QC[0] * x76 + s.WQ[0] * x86 + r2 * x42;
-      const TB x205 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x48;
-      const TB x206 = s.QC[0] * x67 + s.WQ[0] * x77 + r2 * x37;
-      const TB x207 = s.QC[0] * x77 + s.WQ[0] * x87 + r2 * x43;
-      const TB x209 = s.QC[0] * x68 + s.WQ[0] * x78 + r1 * x38;
-      const TB x485 = s.QC[0] * x192 + s.WQ[0] * x193 + e1 * (x38 - s.roe * x44);
-      const TB x210 = s.QC[0] * x78 + s.WQ[0] * x88 + r1 * x44;
-      const TB x496 = s.QC[0] * x209 + s.WQ[0] * x210 + e1 * (x68 - s.roe * x78) + r1 * x192;
-      acc9 += static_cast<double>(x496);
-      const TB x497 = s.QC[0] * x210 + s.WQ[0] * x211 + e1 * (x78 - s.roe * x88) + r1 * x193;
-      const TB x874 = s.QC[0] * x496 + s.WQ[0] * x497 + e2 * (x209 - s.roe * x210) + r1 * x485;
-      acc195 += static_cast<double>(x874);
-      const TB x905 = s.QC[1] * x496 + s.WQ[1] * x497 + r2 * x481;
-      acc226 += static_cast<double>(x905);
-      const TB x936 = s.QC[2] * x496 + s.WQ[2] * x497;
-      acc257 += static_cast<double>(x936);
-      const TB x275 = s.QC[1] * x2 + s.WQ[1] * x3;
-      const TB x277 = s.QC[1] * x14 + s.WQ[1] * x17;
-      const TB x286 = s.QC[1] * x36 + s.WQ[1] * x42 + r1 * x14;
-      const TB x287 = s.QC[1] * x42 + s.WQ[1] * x48 + r1 * x17;
-      const TB x311 = s.QC[1] * x88 + s.WQ[1] * x98 + r2 * x48;
-      const TB x411 = s.QC[2] * x88 + s.WQ[2] * x98;
-      const TB x676 = s.QC[1] * x286 + s.WQ[1] * x287 + e1 * (x36 - s.roe * x42) + r1 * x277;
-      const TB x309 = s.QC[1] * x68 + s.WQ[1] * x78 + r2 * x36;
-      const TB x310 = s.QC[1] * x78 + s.WQ[1] * x88 + r2 * x42;
-      const TB x692 = s.QC[1] * x310 + s.WQ[1] * x311 + e1 * (x78 - s.roe * x88) + r2 * x287;
-      const TB x757 = s.QC[2] * x310 + s.WQ[2] * x311;
-      const TB x280 = s.QC[1] * x13 + s.WQ[1] * x16;
-      const TB x281 = s.QC[1] * x16 + s.WQ[1] * x19;
-      const TB x737 = s.QC[2] * x280 + s.WQ[2] * x281 + r1 * x275;
-      const TB x283 = s.QC[1] * x35 + s.WQ[1] * x41;
-      const TB x284 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x289 = s.QC[1] * x37 + s.WQ[1] * x43;
-      const TB x483 = s.QC[0] * x189 + s.WQ[0] * x190 + e1 * (x37 - s.roe * x43) + r1 * x181;
-      const TB x312 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x37;
-      const TB x693 = s.QC[1] * x312 + s.WQ[1] * x313 + e1 * (x69 - s.roe * x79) + r1 * x289;
-      acc103 += static_cast<double>(x693);
-      const TB x758 = s.QC[2] * x312 + s.WQ[2] * x313 + r1 * x286;
-      acc134 += static_cast<double>(x758);
-      const TB x494 = s.QC[0] * x206 + s.WQ[0] * x207 + e1 * (x67 - s.roe * x77) + r2 * x189;
-      acc8 += static_cast<double>(x494);
-      const TB x495 = s.QC[0] * x207 + s.WQ[0] * x208 + e1 * (x77 - s.roe * x87) + r2 * x190;
-      const TB x873 = s.QC[0] * x494 + s.WQ[0] * x495 + e2 * (x206 - s.roe * x207) + r2 * x483;
-      acc194 += static_cast<double>(x873);
-      const TB x691 = s.QC[1] * x309 + s.WQ[1] * x310 + e1 * (x68 - s.roe * x78) + r2 * x286;
-      acc102 += static_cast<double>(x691);
-      const TB x1060 = s.QC[1] * x691 + s.WQ[1] * x692 + e2 * (x309 - s.roe * x310) + r2 * x676;
-      acc381 += static_cast<double>(x1060);
-      const TB x1091 = s.QC[2] * x691 + s.WQ[2] * x692;
-      acc412 += static_cast<double>(x1091);
-      const TB x904 = s.QC[1] * x494 + s.WQ[1] * x495;
-      acc225 += static_cast<double>(x904);
-      const TB x297 = s.QC[1] * x34 + s.WQ[1] * x40;
-      const TB x298 = s.QC[1] * x40 + s.WQ[1] * x46;
-      const TB x299 = s.QC[1] * x46 + s.WQ[1] * x52;
-      const TB x399 = s.QC[2] * x46 + s.WQ[2] * x52 + r2 * x19;
-      const TB x381 = s.QC[2] * x16 + s.WQ[2] * x19 + r1 * x3;
-      const TB x375 = s.QC[2] * x2 + s.WQ[2] * x3;
-      const TB x380 = s.QC[2] * x13 + s.WQ[2] * x16 + r1 * x2;
-      const TB x802 = s.QC[2] * x380 + s.WQ[2] * x381 + e1 * (x13 - s.roe * x16) + r1 * x375;
-      const TB x300 = s.QC[1] * x65 + s.WQ[1] * x75;
-      const TB x301 = s.QC[1] * x75 + s.WQ[1] * x85;
-      const TB x302 = s.QC[1] * x85 + s.WQ[1] * x95;
-      const TB x402 = s.QC[2] * x85 + s.WQ[2] * x95;
-      const TB x303 = s.QC[1] * x66 + s.WQ[1] * x76 + r1 * x35;
-      const TB x304 = s.QC[1] * x76 + s.WQ[1] * x86 + r1 * x41;
-      const TB x305 = s.QC[1] * x86 + s.WQ[1] * x96 + r1 * x47;
-      const TB x405 = s.QC[2] * x86 + s.WQ[2] * x96;
-      const TB x306 = s.QC[1] * x67 + s.WQ[1] * x77;
-      const TB x307 = s.QC[1] * x77 + s.WQ[1] * x87;
-      const TB x308 = s.QC[1] * x87 + s.WQ[1] * x97;
-      const TB x408 = s.QC[2] * x87 + s.WQ[2] * x97 + r1 * x47;
-      const TB x384 = s.QC[2] * x41 + s.WQ[2] * x47;
-      const TB x383 = s.QC[2] * x35 + s.WQ[2] * x41;
-      const TB x397 = s.QC[2] * x34 + s.WQ[2] * x40 + r2 * x13;
-      const TB x398 = s.QC[2] * x40 + s.WQ[2] * x46 + r2 * x16;
-      const TB x814 = s.QC[2] * x398 + s.WQ[2] * x399 + e1 * (x40 - s.roe * x46) + r2 * x381;
-      const TB x813 = s.QC[2] * x397 + s.WQ[2] * x398 + e1 * (x34 - s.roe * x40) + r2 * x380;
-      acc160 += static_cast<double>(x813);
-      const TB x1149 = s.QC[2] * x813 + s.WQ[2] * x814 + e2 * (x397 - s.roe * x398) + r2 * x802;
-      acc470 += static_cast<double>(x1149);
-      const TB x400 = s.QC[2] * x65 + s.WQ[2] * x75;
-      const TB x401 = s.QC[2] * x75 + s.WQ[2] * x85;
-      const TB x816 = s.QC[2] * x401 + s.WQ[2] * x402 + e1 * (x75 - s.roe * x85);
-      const TB x403 = s.QC[2] * x66 + s.WQ[2] * x76;
-      const TB x404 = s.QC[2] * x76 + s.WQ[2] * x86;
-      const TB x818 = s.QC[2] * x404 + s.WQ[2] * x405 + e1 * (x76 - s.roe * x86);
-      const TB x406 = s.QC[2] * x67 + s.WQ[2] * x77 + r1 * x35;
-      const TB x407 = s.QC[2] * x77 + s.WQ[2] * x87 + r1 * x41;
-      const TB x820 = s.QC[2] * x407 + s.WQ[2] * x408 + e1 * (x77 - s.roe * x87) + r1 * x384;
-      const TB x690 = s.QC[1] * x307 + s.WQ[1] * x308 + e1 * (x77 - s.roe * x87);
-      const TB x755 = s.QC[2] * x307 + s.WQ[2] * x308 + r1 * x284;
-      const TB x804 = s.QC[2] * x383 + s.WQ[2] * x384 + e1 * (x35 - s.roe * x41);
-      const TB x819 = s.QC[2] * x406 + s.WQ[2] * x407 + e1 * (x67 - s.roe * x77) + r1 * x383;
-      acc163 += static_cast<double>(x819);
-      const TB x1152 = s.QC[2] * x819 + s.WQ[2] * x820 + e2 * (x406 - s.roe * x407) + r1 * x804;
-      acc473 += static_cast<double>(x1152);
-      const TB x689 = s.QC[1] * x306 + s.WQ[1] * x307 + e1 * (x67 - s.roe * x77);
-      acc101 += static_cast<double>(x689);
-      const TB x1059 = s.QC[1] * x689 + s.WQ[1] * x690 + e2 * (x306 - s.roe * x307);
-      acc380 += static_cast<double>(x1059);
-      const TB x409 = s.QC[2] * x68 + s.WQ[2] * x78;
-      const TB x410 = s.QC[2] * x78 + s.WQ[2] * x88;
-      const TB x822 = s.QC[2] * x410 + s.WQ[2] * x411 + e1 * (x78 - s.roe * x88);
-      const TB x821 = s.QC[2] * x409 + s.WQ[2] * x410 + e1 * (x68 - s.roe * x78);
-      acc164 += static_cast<double>(x821);
-      const TB x1153 = s.QC[2] * x821 + s.WQ[2] * x822 + e2 * (x409 - s.roe * x410);
-      acc474 += static_cast<double>(x1153);
-      const TB x477 = s.QC[0] * x180 + s.WQ[0] * x181 + e1 * (x13 - s.roe * x16);
-      const TB x672 = s.QC[1] * x280 + s.WQ[1] * x281 + e1 * (x13 - s.roe * x16);
-      const TB x748 = s.QC[2] * x297 + s.WQ[2] * x298 + r2 * x280;
-      acc129 += static_cast<double>(x748);
-      const TB x749 = s.QC[2] * x298 + s.WQ[2] * x299 + r2 * x281;
-      const TB x1118 = s.QC[2] * x748 + s.WQ[2] * x749 + e1 * (x297 - s.roe * x298) + r2 * x737;
-      acc439 += static_cast<double>(x1118);
-      const TB x684 = s.QC[1] * x298 + s.WQ[1] * x299 + e1 * (x40 - s.roe * x46);
-      const TB x489 = s.QC[0] * x198 + s.WQ[0] * x199 + e1 * (x40 - s.roe * x46);
-      const TB x479 = s.QC[0] * x183 + s.WQ[0] * x184 + e1 * (x35 - s.roe * x41) + r2 * x177;
-      const TB x935 = s.QC[2] * x494 + s.WQ[2] * x495 + r1 * x479;
-      acc256 += static_cast<double>(x935);
-      const TB x674 = s.QC[1] * x283 + s.WQ[1] * x284 + e1 * (x35 - s.roe * x41);
-      const TB x1090 = s.QC[2] * x689 + s.WQ[2] * x690 + r1 * x674;
-      acc411 += static_cast<double>(x1090);
-      const TB x546 = s.QC[1] * x186 + s.WQ[1] * x187 + r1 * x177;
-      const TB x488 = s.QC[0] * x197 + s.WQ[0] * x198 + e1 * (x34 - s.roe * x40);
-      const TB x683 = s.QC[1] * x297 + s.WQ[1] * x298 + e1 * (x34 - s.roe * x40);
-      acc98 += static_cast<double>(x683);
-      const TB x1056 = s.QC[1] * x683 + s.WQ[1] * x684 + e2 * (x297 - s.roe * x298);
-      acc377 += static_cast<double>(x1056);
-      const TB x1087 = s.QC[2] * x683 + s.WQ[2] * x684 + r2 * x672;
-      acc408 += static_cast<double>(x1087);
-      const TB x932 = s.QC[2] * x488 + s.WQ[2] * x489 + r2 * x477;
-      acc253 += static_cast<double>(x932);
-      const TB x870 = s.QC[0] * x488 + s.WQ[0] * x489 + e2 * (x197 - s.roe * x198);
-      acc191 += static_cast<double>(x870);
-      const TB x901 = s.QC[1] * x488 + s.WQ[1] * x489;
-      acc222 += static_cast<double>(x901);
-      const TB x490 = s.QC[0] * x200 + s.WQ[0] * x201 + e1 * (x65 - s.roe * x75) + r3 * x183;
-      acc6 += static_cast<double>(x490);
-      const TB x491 = s.QC[0] * x201 + s.WQ[0] * x202 + e1 * (x75 - s.roe * x85) + r3 * x184;
-      const TB x686 = s.QC[1] * x301 + s.WQ[1] * x302 + e1 * (x75 - s.roe * x85);
-      const TB x751 = s.QC[2] * x301 + s.WQ[2] * x302;
-      const TB x871 = s.QC[0] * x490 + s.WQ[0] * x491 + e2 * (x200 - s.roe * x201) + r3 * x479;
-      acc192 += static_cast<double>(x871);
-      const TB x902 = s.QC[1] * x490 + s.WQ[1] * x491;
-      acc223 += static_cast<double>(x902);
-      const TB x933 = s.QC[2] * x490 + s.WQ[2] * x491;
-      acc254 += static_cast<double>(x933);
-      const TB x492 = s.QC[0] * x203 + s.WQ[0] * x204 + e1 * (x66 - s.roe * x76) + r2 * x186;
-      acc7 += static_cast<double>(x492);
-      const TB x561 = s.QC[1] * x209 + s.WQ[1] * x210 + r2 * x186;
-      acc40 += static_cast<double>(x561);
-      const TB x493 = s.QC[0] * x204 + s.WQ[0] * x205 + e1 * (x76 - s.roe * x86) + r2 * x187;
-      const TB x872 = s.QC[0] * x492 + s.WQ[0] * x493 + e2 * (x203 - s.roe * x204) + r2 * x481;
-      acc193 += static_cast<double>(x872);
-      const TB x903 = s.QC[1] * x492 + s.WQ[1] * x493 + r1 * x479;
-      acc224 += static_cast<double>(x903);
-      const TB x934 = s.QC[2] * x492 + s.WQ[2] * x493;
-      acc255 += static_cast<double>(x934);
-      const TB x562 = s.QC[1] * x210 + s.WQ[1] * x211 + r2 * x187;
-      const TB x967 = s.QC[1] * x561 + s.WQ[1] * x562 + e1 * (x209 - s.roe * x210) + r2 * x546;
-      acc288 += static_cast<double>(x967);
-      const TB x998 = s.QC[2] * x561 + s.WQ[2] * x562;
-      acc319 += static_cast<double>(x998);
-      const TB x627 = s.QC[2] * x210 + s.WQ[2] * x211;
-      const TB x688 = s.QC[1] * x304 + s.WQ[1] * x305 + e1 * (x76 - s.roe * x86) + r1 * x284;
-      const TB x739 = s.QC[2] * x283 + s.WQ[2] * x284;
-      const TB x753 = s.QC[2] * x304 + s.WQ[2] * x305;
-      const TB x542 = s.QC[1] * x180 + s.WQ[1] * x181;
-      const TB x618 = s.QC[2] * x197 + s.WQ[2] * x198 + r2 * x180;
-      acc67 += static_cast<double>(x618);
-      const TB x619 = s.QC[2] * x198 + s.WQ[2] * x199 + r2 * x181;
-      const TB x1025 = s.QC[2] * x618 + s.WQ[2] * x619 + e1 * (x197 - s.roe * x198) + r2 * x607;
-      acc346 += static_cast<double>(x1025);
-      const TB x554 = s.QC[1] * x198 + s.WQ[1] * x199;
-      const TB x544 = s.QC[1] * x183 + s.WQ[1] * x184;
-      const TB x553 = s.QC[1] * x197 + s.WQ[1] * x198;
-      const TB x963 = s.QC[1] * x553 + s.WQ[1] * x554 + e1 * (x197 - s.roe * x198);
-      acc284 += static_cast<double>(x963);
-      const TB x994 = s.QC[2] * x553 + s.WQ[2] * x554 + r2 * x542;
-      acc315 += static_cast<double>(x994);
-      const TB x555 = s.QC[1] * x200 + s.WQ[1] * x201;
-      acc37 += static_cast<double>(x555);
-      const TB x556 = s.QC[1] * x201 + s.WQ[1] * x202;
-      const TB x621 = s.QC[2] * x201 + s.WQ[2] * x202;
-      const TB x964 = s.QC[1] * x555 + s.WQ[1] * x556 + e1 * (x200 - s.roe * x201);
-      acc285 += static_cast<double>(x964);
-      const TB x995 = s.QC[2] * x555 + s.WQ[2] * x556;
-      acc316 += static_cast<double>(x995);
-      const TB x557 = s.QC[1] * x203 + s.WQ[1] * x204 + r1 * x183;
-      acc38 += static_cast<double>(x557);
-      const TB x558 = s.QC[1] * x204 + s.WQ[1] * x205 + r1 * x184;
-      const TB x623 = s.QC[2] * x204 + s.WQ[2] * x205;
-      const TB x965 = s.QC[1] * x557 + s.WQ[1] * x558 + e1 * (x203 - s.roe * x204) + r1 * x544;
-      acc286 += static_cast<double>(x965);
-      const TB x996 = s.QC[2] * x557 + s.WQ[2] * x558;
-      acc317 += static_cast<double>(x996);
-      const TB x559 = s.QC[1] * x206 + s.WQ[1] * x207;
-      acc39 += static_cast<double>(x559);
-      const TB x560 = s.QC[1] * x207 + s.WQ[1] * x208;
-      const TB x997 = s.QC[2] * x559 + s.WQ[2] * x560 + r1 * x544;
-      acc318 += static_cast<double>(x997);
-      const TB x966 = s.QC[1] * x559 + s.WQ[1] * x560 + e1 * (x206 - s.roe * x207);
-      acc287 += static_cast<double>(x966);
-      const TB x625 = s.QC[2] * x207 + s.WQ[2] * x208 + r1 * x184;
-      const TB x609 = s.QC[2] * x183 + s.WQ[2] * x184;
-      const TB x624 = s.QC[2] * x206 + s.WQ[2] * x207 + r1 * x183;
-      acc70 += static_cast<double>(x624);
-      const TB x1028 = s.QC[2] * x624 + s.WQ[2] * x625 + e1 * (x206 - s.roe * x207) + r1 * x609;
-      acc349 += static_cast<double>(x1028);
-      const TB x620 = s.QC[2] * x200 + s.WQ[2] * x201;
-      acc68 += static_cast<double>(x620);
-      const TB x1026 = s.QC[2] * x620 + s.WQ[2] * x621 + e1 * (x200 - s.roe * x201);
-      acc347 += static_cast<double>(x1026);
-      const TB x622 = s.QC[2] * x203 + s.WQ[2] * x204;
-      acc69 += static_cast<double>(x622);
-      const TB x1027 = s.QC[2] * x622 + s.WQ[2] * x623 + e1 * (x203 - s.roe * x204);
-      acc348 += static_cast<double>(x1027);
-      const TB x626 = s.QC[2] * x209 + s.WQ[2] * x210;
-      acc71 += static_cast<double>(x626);
-      const TB x1029 = s.QC[2] * x626 + s.WQ[2] * x627 + e1 * (x209 - s.roe * x210);
-      acc350 += static_cast<double>(x1029);
-      const TB x685 = s.QC[1] * x300 + s.WQ[1] * x301 + e1 * (x65 - s.roe * x75);
-      acc99 += static_cast<double>(x685);
-      const TB x815 = s.QC[2] * x400 + s.WQ[2] * x401 + e1 * (x65 - s.roe * x75);
-      acc161 += static_cast<double>(x815);
-      const TB x1150 = s.QC[2] * x815 + s.WQ[2] * x816 + e2 * (x400 - s.roe * x401);
-      acc471 += static_cast<double>(x1150);
-      const TB x1057 = s.QC[1] * x685 + s.WQ[1] * x686 + e2 * (x300 - s.roe * x301);
-      acc378 += static_cast<double>(x1057);
-      const TB x1088 = s.QC[2] * x685 + s.WQ[2] * x686;
-      acc409 += static_cast<double>(x1088);
-      const TB x687 = s.QC[1] * x303 + s.WQ[1] * x304 + e1 * (x66 - s.roe * x76) + r1 * x283;
-      acc100 += static_cast<double>(x687);
-      const TB x817 = s.QC[2] * x403 + s.WQ[2] * x404 + e1 * (x66 - s.roe * x76);
-      acc162 += static_cast<double>(x817);
-      const TB x1151 = s.QC[2] * x817 + s.WQ[2] * x818 + e2 * (x403 - s.roe * x404);
-      acc472 += static_cast<double>(x1151);
-      const TB x1058 = s.QC[1] * x687 + s.WQ[1] * x688 + e2 * (x303 - s.roe * x304) + r1 * x674;
-      acc379 += static_cast<double>(x1058);
-      const TB x1089 = s.QC[2] * x687 + s.WQ[2] * x688;
-      acc410 += static_cast<double>(x1089);
-      const TB x754 = s.QC[2] * x306 + s.WQ[2] * x307 + r1 * x283;
-      acc132 += static_cast<double>(x754);
-      const TB x1121 = s.QC[2] * x754 + s.WQ[2] * x755 + e1 * (x306 - s.roe * x307) + r1 * x739;
-      acc442 += static_cast<double>(x1121);
-      const TB x750 = s.QC[2] * x300 + s.WQ[2] * x301;
-      acc130 += static_cast<double>(x750);
-      const TB x1119 = s.QC[2] * x750 + s.WQ[2] * x751 + e1 * (x300 - s.roe * x301);
-      acc440 += static_cast<double>(x1119);
-      const TB x752 = s.QC[2] * x303 + s.WQ[2] * x304;
-      acc131 += static_cast<double>(x752);
-      const TB x1120 = s.QC[2] * x752 + s.WQ[2] * x753 + e1 * (x303 - s.roe * x304);
-      acc441 += static_cast<double>(x1120);
-      const TB x756 = s.QC[2] * x309 + s.WQ[2] * x310;
-      acc133 += static_cast<double>(x756);
-      const TB x1122 = s.QC[2] * x756 + s.WQ[2] * x757 + e1 * (x309 - s.roe * x310);
-      acc443 += static_cast<double>(x1122);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(67 * R + r) * a.acc_stride + q] = acc67;
-  a.acc[(98 * R + r) * a.acc_stride + q] = acc98;
-  a.acc[(129 * R + r) * a.acc_stride + q] = acc129;
-  a.acc[(160 * R + r) * a.acc_stride + q] = acc160;
-  a.acc[(191 * R + r) * a.acc_stride + q] = acc191;
-  a.acc[(222 * R + r) * a.acc_stride + q] = acc222;
-  a.acc[(253 * R + r) * a.acc_stride + q] = acc253;
-  a.acc[(284 * R + r) * a.acc_stride + q] = acc284;
-  a.acc[(315 * R + r) * a.acc_stride + q] = acc315;
-  a.acc[(346 * R + r) * a.acc_stride + q] = acc346;
-  a.acc[(377 * R + r) * a.acc_stride + q] = acc377;
-  a.acc[(408 * R + r) * a.acc_stride + q] = acc408;
-  a.acc[(439 * R + r) * a.acc_stride + q] = acc439;
-  a.acc[(470 * R + r) * a.acc_stride + q] = acc470;
-  a.acc[(6 * R + r) * a.acc_stride + q] = acc6;
-  a.acc[(37 * R + r) * a.acc_stride + q] = acc37;
-  a.acc[(68 * R + r) * a.acc_stride + q] = acc68;
-  a.acc[(99 * R + r) * a.acc_stride + q] = acc99;
-  a.acc[(130 * R + r) * a.acc_stride + q] = acc130;
-  a.acc[(161 * R + r) * a.acc_stride + q] = acc161;
-  a.acc[(192 * R + r) * a.acc_stride + q] = acc192;
-  a.acc[(223 * R + r) * a.acc_stride + q] = acc223;
-  a.acc[(254 * R + r) * a.acc_stride + q] = acc254;
-  a.acc[(285 * R + r) * a.acc_stride + q] = acc285;
-  a.acc[(316 * R + r) * a.acc_stride + q] = acc316;
-  a.acc[(347 * R + r) * a.acc_stride + q] = acc347;
-  a.acc[(378 * R + r) * a.acc_stride + q] = acc378;
-  a.acc[(409 * R + r) * a.acc_stride + q] = acc409;
-  a.acc[(440 * R + r) * a.acc_stride + q] = acc440;
-  a.acc[(471 * R + r) * a.acc_stride + q] = acc471;
-  a.acc[(7 * R + r) * a.acc_stride + q] = acc7;
-  a.acc[(38 * R + r) * a.acc_stride + q] = acc38;
-  a.acc[(69 * R + r) * a.acc_stride + q] = acc69;
-  a.acc[(100 * R + r) * a.acc_stride + q] = acc100;
-  a.acc[(131 * R + r) * a.acc_stride + q] = acc131;
-  a.acc[(162 * R + r) * a.acc_stride + q] = acc162;
-  a.acc[(193 * R + r) * a.acc_stride + q] = acc193;
-  a.acc[(224 * R + r) * a.acc_stride + q] = acc224;
-  a.acc[(255 * R + r) * a.acc_stride + q] = acc255;
-  a.acc[(286 * R + r) * a.acc_stride + q] = acc286;
-  a.acc[(317 * R + r) * a.acc_stride + q] = acc317;
-  a.acc[(348 * R + r) * a.acc_stride + q] = acc348;
-  a.acc[(379 * R + r) * a.acc_stride + q] = acc379;
-  a.acc[(410 * R + r) * a.acc_stride + q] = acc410;
-  a.acc[(441 * R + r) * a.acc_stride + q] = acc441;
-  a.acc[(472 * R + r) * a.acc_stride + q] = acc472;
-  a.acc[(8 * R + r) * a.acc_stride + q] = acc8;
-  a.acc[(39 * R + r) * a.acc_stride + q] = acc39;
-  a.acc[(70 * R + r) * a.acc_stride + q] = acc70;
-  a.acc[(101 * R + r) * a.acc_stride + q] = acc101;
-  a.acc[(132 * R + r) * a.acc_stride + q] = acc132;
-  a.acc[(163 * R + r) * a.acc_stride + q] = acc163;
-  a.acc[(194 * R + r) * a.acc_stride + q] = acc194;
-  a.acc[(225 * R + r) * a.acc_stride + q] = acc225;
-  a.acc[(256 * R + r) * a.acc_stride + q] = acc256;
-  a.acc[(287 * R + r) * a.acc_stride + q] = acc287;
-  a.acc[(318 * R + r) * a.acc_stride + q] = acc318;
-  a.acc[(349 * R + r) * a.acc_stride + q] = acc349;
-  a.acc[(380 * R + r) * a.acc_stride + q] = acc380;
-  a.acc[(411 * R + r) * a.acc_stride + q] = acc411;
-  a.acc[(442 * R + r) * a.acc_stride + q] = acc442;
-  a.acc[(473 * R + r) * a.acc_stride + q] = acc473;
-  a.acc[(9 * R + r) * a.acc_stride + q] = acc9;
-  a.acc[(40 * R + r) * a.acc_stride + q] = acc40;
-  a.acc[(71 * R + r) * a.acc_stride + q] = acc71;
-  a.acc[(102 * R + r) * a.acc_stride + q] = acc102;
-  a.acc[(133 * R + r) * a.acc_stride + q] = acc133;
-  a.acc[(164 * R + r) * a.acc_stride + q] = acc164;
-  a.acc[(195 * R + r) * a.acc_stride + q] = acc195;
-  a.acc[(226 * R + r) * a.acc_stride + q] = acc226;
-  a.acc[(257 * R + r) * a.acc_stride + q] = acc257;
-  a.acc[(288 * R + r) * a.acc_stride + q] = acc288;
-  a.acc[(319 * R + r) * a.acc_stride + q] = acc319;
-  a.acc[(350 * R + r) * a.acc_stride + q] = acc350;
-  a.acc[(381 * R + r) * a.acc_stride + q] = acc381;
-  a.acc[(412 * R + r) * a.acc_stride + q] = acc412;
-  a.acc[(443 * R + r) * a.acc_stride + q] = acc443;
-  a.acc[(474 * R + r) * a.acc_stride + q] = acc474;
-  a.acc[(10 * R + r) * a.acc_stride + q] = acc10;
-  a.acc[(41 * R + r) * a.acc_stride + q] = acc41;
-  a.acc[(72 * R + r) * a.acc_stride + q] = acc72;
-  a.acc[(103 * R + r) * a.acc_stride + q] = acc103;
-  a.acc[(134 * R + r) * a.acc_stride + q] = acc134;
-}
-// class (22|21) split 2/6: 302 recursion values, 83 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2221_2(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc165 = 0.0;
-  double acc196 = 0.0;
-  double acc227 = 0.0;
-  double acc258 = 0.0;
-  double acc289 = 0.0;
-  double acc320 = 0.0;
-  double acc351 = 0.0;
-  double acc382 = 0.0;
-  double acc413 = 0.0;
-  double acc444 = 0.0;
-  double acc475 = 0.0;
-  double acc11 = 0.0;
-  double acc42 = 0.0;
-  double acc73 = 0.0;
-  double acc104 = 0.0;
-  double acc135 = 0.0;
-  double acc166 = 0.0;
-  double acc197 = 0.0;
-  double acc228 = 0.0;
-  double acc259 = 0.0;
-  double acc290 = 0.0;
-  double acc321 = 0.0;
-  double acc352 = 0.0;
-  double acc383 = 0.0;
-  double acc414 = 0.0;
-  double acc445 = 0.0;
-  double acc476 = 0.0;
-  double acc12 = 0.0;
-  double acc43 = 0.0;
-  double acc74 = 0.0;
-  double acc105 = 0.0;
-  double acc136 = 0.0;
-  double acc167 = 0.0;
-  double acc198 = 0.0;
-  double acc229 = 0.0;
-  double acc260 = 0.0;
-  double acc291 = 0.0;
-  double acc322 = 0.0;
-  double acc353 = 0.0;
-  double acc384 = 0.0;
-  double acc415 = 0.0;
-  double acc446 = 0.0;
-  double acc477 = 0.0;
-  double acc13 = 0.0;
-  double acc44 = 0.0;
-  double acc75 = 0.0;
-  double acc106 = 0.0;
-  double acc137 = 0.0;
-  double acc168 = 0.0;
-  double acc199 = 0.0;
-  double acc230 = 0.0;
-  double acc261 = 0.0;
-  double acc292 = 0.0;
-  double acc323 = 0.0;
-  double acc354 = 0.0;
-  double acc385 = 0.0;
-  double acc416 = 0.0;
-  double acc447 = 0.0;
-  double acc478 = 0.0;
-  double acc14 = 0.0;
-  double acc45 = 0.0;
-  double acc76 = 0.0;
-  double acc107 = 0.0;
-  double acc138 = 0.0;
-  double acc169 = 0.0;
-  double acc200 = 0.0;
-  double acc231 = 0.0;
-  double acc262 = 0.0;
-  double acc293 = 0.0;
-  double acc324 = 0.0;
-  double acc355 = 0.0;
-  double acc386 = 0.0;
-  double acc417 = 0.0;
-  double acc448 = 0.0;
-  double acc479 = 0.0;
-  double acc15 = 0.0;
-  double acc46 = 0.0;
-  double acc77 = 0.0;
-  double acc108 = 0.0;
-  double acc139 = 0.0;
-  double acc170 = 0.0;
-  double acc201 = 0.0;
-  double acc232 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[8];
-      boys_eval_fixed<7>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x8 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x9 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x10 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x11 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x12 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x13 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x14 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x15 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x16 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x17 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x18 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x19 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x20 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x21 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x22 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x23 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x24 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x25 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x58 = s.PA[2] * x22 + s.WP[2] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x56 = s.PA[1] * x21 + s.WP[1] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x57 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x30 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x31 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x32 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x34 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x33 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x36 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x69 = s.PA[2] * x30 + s.WP[2] * x36;
-      const TB x37 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x70 = s.PA[2] * x31 + s.WP[2] * x37 + z1 * (x8 - s.roz * x11);
-      const TB x38 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x40 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x74 = s.PA[2] * x34 + s.WP[2] * x40 + z2 * (x10 - s.roz * x13);
-      const TB x39 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x73 = s.PA[2] * x33 + s.WP[2] * x39 + z1 * (x9 - s.roz * x12);
-      const TB x71 = s.PA[1] * x32 + s.WP[1] * x38 + z2 * (x9 - s.roz * x12);
-      const TB x72 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x42 = s.PA[1] * x14 + s.WP[1] * x17;
-      const TB x43 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x80 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x11 - s.roz * x14);
-      const TB x44 = s.PA[1] * x15 + s.WP[1] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x46 = s.PA[2] * x16 + s.WP[2] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x84 = s.PA[2] * x40 + s.WP[2] * x46 + z2 * (x13 - s.roz * x16);
-      const TB x45 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x48 = s.PA[1] * x17 + s.WP[1] * x20;
-      const TB x49 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x50 = s.PA[1] * x18 + s.WP[1] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[2] * x19 + s.WP[2] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x104 = s.PA[2] * x52 + s.WP[2] * x58 + z2 * (x19 - s.roz * x22);
-      const TB x51 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x103 = s.PA[2] * x51 + s.WP[2] * x57 + z1 * (x18 - s.roz * x21);
-      const TB x101 = s.PA[1] * x50 + s.WP[1] * x56 + z2 * (x18 - s.roz * x21);
-      const TB x102 = s.PA[2] * x50 + s.WP[2] * x56;
-      const TB x54 = s.PA[1] * x20 + s.WP[1] * x23;
-      const TB x55 = s.PA[2] * x20 + s.WP[2] * x23;
-      const TB x100 = s.PA[2] * x49 + s.WP[2] * x55 + z1 * (x17 - s.roz * x20);
-      const TB x99 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x79 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x81 = s.PA[1] * x38 + s.WP[1] * x44 + z2 * (x12 - s.roz * x15);
-      const TB x83 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x12 - s.roz * x15);
-      const TB x82 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x89 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x90 = s.PA[2] * x43 + s.WP[2] * x49 + z1 * (x14 - s.roz * x17);
-      const TB x91 = s.PA[1] * x44 + s.WP[1] * x50 + z2 * (x15 - s.roz * x18);
-      const TB x92 = s.PA[2] * x44 + s.WP[2] * x50;
-      const TB x93 = s.PA[2] * x45 + s.WP[2] * x51 + z1 * (x15 - s.roz * x18);
-      const TB x94 = s.PA[2] * x46 + s.WP[2] * x52 + z2 * (x16 - s.roz * x19);
-      const TB x229 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x177 = s.QC[0] * x14 + s.WQ[0] * x17 + r1 * x3;
-      const TB x279 = s.QC[1] * x15 + s.WQ[1] * x18 + r1 * x3;
-      const TB x179 = s.QC[0] * x15 + s.WQ[0] * x18;
-      const TB x181 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x186 = s.QC[0] * x36 + s.WQ[0] * x42 + r1 * x15;
-      const TB x187 = s.QC[0] * x42 + s.WQ[0] * x48 + r1 * x18;
-      const TB x189 = s.QC[0] * x37 + s.WQ[0] * x43 + r1 * x16;
-      const TB x190 = s.QC[0] * x43 + s.WQ[0] * x49 + r1 * x19;
-      const TB x192 = s.QC[0] * x38 + s.WQ[0] * x44;
-      const TB x193 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x195 = s.QC[0] * x39 + s.WQ[0] * x45;
-      const TB x196 = s.QC[0] * x45 + s.WQ[0] * x51;
-      const TB x198 = s.QC[0] * x40 + s.WQ[0] * x46;
-      const TB x199 = s.QC[0] * x46 + s.WQ[0] * x52;
-      const TB x212 = s.QC[0] * x69 + s.WQ[0] * x79 + r1 * x39;
-      const TB x213 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x45;
-      const TB x214 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x51;
-      const TB x215 = s.QC[0] * x70 + s.WQ[0] * x80 + r1 * x40;
-      const TB x216 = s.QC[0] * x80 + s.WQ[0] * x90 + r1 * x46;
-      const TB x217 = s.QC[0] * x90 + s.WQ[0] * x100 + r1 * x52;
-      const TB x218 = s.QC[0] * x71 + s.WQ[0] * x81;
-      const TB x219 = s.QC[0] * x81 + s.WQ[0] * x91;
-      const TB x220 = s.QC[0] * x91 + s.WQ[0] * x101;
-      const TB x221 = s.QC[0] * x72 + s.WQ[0] * x82;
-      const TB x222 = s.QC[0] * x82 + s.WQ[0] * x92;
-      const TB x223 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x224 = s.QC[0] * x73 + s.WQ[0] * x83;
-      const TB x225 = s.QC[0] * x83 + s.WQ[0] * x93;
-      const TB x226 = s.QC[0] * x93 + s.WQ[0] * x103;
-      const TB x227 = s.QC[0] * x74 + s.WQ[0] * x84;
-      const TB x228 = s.QC[0] * x84 + s.WQ[0] * x94;
-      const TB x509 = s.QC[0] * x228 + s.WQ[0] * x229 + e1 * (x84 - s.roe * x94);
-      const TB x573 = s.QC[1] * x227 + s.WQ[1] * x228;
-      acc46 += static_cast<double>(x573);
-      const TB x638 = s.QC[2] * x227 + s.WQ[2] * x228 + r3 * x198;
-      acc77 += static_cast<double>(x638);
-      const TB x277 = s.QC[1] * x14 + s.WQ[1] * x17;
-      const TB x281 = s.QC[1] * x16 + s.WQ[1] * x19;
-      const TB x296 = s.QC[1] * x45 + s.WQ[1] * x51 + r1 * x19;
-      const TB x286 = s.QC[1] * x36 + s.WQ[1] * x42 + r1 * x14;
-      const TB x287 = s.QC[1] * x42 + s.WQ[1] * x48 + r1 * x17;
-      const TB x289 = s.QC[1] * x37 + s.WQ[1] * x43;
-      const TB x290 = s.QC[1] * x43 + s.WQ[1] * x49;
-      const TB x292 = s.QC[1] * x38 + s.WQ[1] * x44 + r2 * x15;
-      const TB x293 = s.QC[1] * x44 + s.WQ[1] * x50 + r2 * x18;
-      const TB x295 = s.QC[1] * x39 + s.WQ[1] * x45 + r1 * x16;
-      const TB x398 = s.QC[2] * x40 + s.WQ[2] * x46 + r2 * x16;
-      const TB x682 = s.QC[1] * x295 + s.WQ[1] * x296 + e1 * (x39 - s.roe * x45) + r1 * x281;
-      const TB x298 = s.QC[1] * x40 + s.WQ[1] * x46;
-      const TB x299 = s.QC[1] * x46 + s.WQ[1] * x52;
-      const TB x326 = s.QC[1] * x93 + s.WQ[1] * x103 + r1 * x52;
-      const TB x426 = s.QC[2] * x93 + s.WQ[2] * x103 + r2 * x51;
-      const TB x312 = s.QC[1] * x69 + s.WQ[1] * x79 + r1 * x37;
-      const TB x313 = s.QC[1] * x79 + s.WQ[1] * x89 + r1 * x43;
-      const TB x314 = s.QC[1] * x89 + s.WQ[1] * x99 + r1 * x49;
-      const TB x414 = s.QC[2] * x89 + s.WQ[2] * x99 + r1 * x48;
-      const TB x387 = s.QC[2] * x42 + s.WQ[2] * x48;
-      const TB x315 = s.QC[1] * x70 + s.WQ[1] * x80;
-      const TB x316 = s.QC[1] * x80 + s.WQ[1] * x90;
-      const TB x317 = s.QC[1] * x90 + s.WQ[1] * x100;
-      const TB x417 = s.QC[2] * x90 + s.WQ[2] * x100 + r2 * x49;
-      const TB x390 = s.QC[2] * x43 + s.WQ[2] * x49 + r1 * x17;
-      const TB x377 = s.QC[2] * x14 + s.WQ[2] * x17;
-      const TB x389 = s.QC[2] * x37 + s.WQ[2] * x43 + r1 * x14;
-      const TB x808 = s.QC[2] * x389 + s.WQ[2] * x390 + e1 * (x37 - s.roe * x43) + r1 * x377;
-      const TB x318 = s.QC[1] * x71 + s.WQ[1] * x81 + r3 * x38;
-      const TB x319 = s.QC[1] * x81 + s.WQ[1] * x91 + r3 * x44;
-      const TB x320 = s.QC[1] * x91 + s.WQ[1] * x101 + r3 * x50;
-      const TB x420 = s.QC[2] * x91 + s.WQ[2] * x101;
-      const TB x321 = s.QC[1] * x72 + s.WQ[1] * x82 + r2 * x39;
-      const TB x322 = s.QC[1] * x82 + s.WQ[1] * x92 + r2 * x45;
-      const TB x323 = s.QC[1] * x92 + s.WQ[1] * x102 + r2 * x51;
-      const TB x396 = s.QC[2] * x45 + s.WQ[2] * x51 + r1 * x18;
-      const TB x379 = s.QC[2] * x15 + s.WQ[2] * x18;
-      const TB x395 = s.QC[2] * x39 + s.WQ[2] * x45 + r1 * x15;
-      const TB x423 = s.QC[2] * x92 + s.WQ[2] * x102 + r1 * x50;
-      const TB x393 = s.QC[2] * x44 + s.WQ[2] * x50;
-      const TB x812 = s.QC[2] * x395 + s.WQ[2] * x396 + e1 * (x39 - s.roe * x45) + r1 * x379;
-      const TB x324 = s.QC[1] * x73 + s.WQ[1] * x83 + r1 * x40;
-      const TB x325 = s.QC[1] * x83 + s.WQ[1] * x93 + r1 * x46;
-      const TB x327 = s.QC[1] * x74 + s.WQ[1] * x84;
-      const TB x328 = s.QC[1] * x84 + s.WQ[1] * x94;
-      const TB x428 = s.QC[2] * x84 + s.WQ[2] * x94 + r3 * x46;
-      const TB x703 = s.QC[1] * x327 + s.WQ[1] * x328 + e1 * (x74 - s.roe * x84);
-      acc108 += static_cast<double>(x703);
-      const TB x768 = s.QC[2] * x327 + s.WQ[2] * x328 + r3 * x298;
-      acc139 += static_cast<double>(x768);
-      const TB x386 = s.QC[2] * x36 + s.WQ[2] * x42;
-      const TB x392 = s.QC[2] * x38 + s.WQ[2] * x44;
-      const TB x412 = s.QC[2] * x69 + s.WQ[2] * x79 + r1 * x36;
-      const TB x413 = s.QC[2] * x79 + s.WQ[2] * x89 + r1 * x42;
-      const TB x824 = s.QC[2] * x413 + s.WQ[2] * x414 + e1 * (x79 - s.roe * x89) + r1 * x387;
-      const TB x806 = s.QC[2] * x386 + s.WQ[2] * x387 + e1 * (x36 - s.roe * x42);
-      const TB x823 = s.QC[2] * x412 + s.WQ[2] * x413 + e1 * (x69 - s.roe * x79) + r1 * x386;
-      acc165 += static_cast<double>(x823);
-      const TB x1154 = s.QC[2] * x823 + s.WQ[2] * x824 + e2 * (x412 - s.roe * x413) + r1 * x806;
-      acc475 += static_cast<double>(x1154);
-      const TB x415 = s.QC[2] * x70 + s.WQ[2] * x80 + r2 * x37;
-      const TB x416 = s.QC[2] * x80 + s.WQ[2] * x90 + r2 * x43;
-      const TB x826 = s.QC[2] * x416 + s.WQ[2] * x417 + e1 * (x80 - s.roe * x90) + r2 * x390;
-      const TB x825 = s.QC[2] * x415 + s.WQ[2] * x416 + e1 * (x70 - s.roe * x80) + r2 * x389;
-      acc166 += static_cast<double>(x825);
-      const TB x1155 = s.QC[2] * x825 + s.WQ[2] * x826 + e2 * (x415 - s.roe * x416) + r2 * x808;
-      acc476 += static_cast<double>(x1155);
-      const TB x418 = s.QC[2] * x71 + s.WQ[2] * x81;
-      const TB x419 = s.QC[2] * x81 + s.WQ[2] * x91;
-      const TB x828 = s.QC[2] * x419 + s.WQ[2] * x420 + e1 * (x81 - s.roe * x91);
-      const TB x421 = s.QC[2] * x72 + s.WQ[2] * x82 + r1 * x38;
-      const TB x422 = s.QC[2] * x82 + s.WQ[2] * x92 + r1 * x44;
-      const TB x830 = s.QC[2] * x422 + s.WQ[2] * x423 + e1 * (x82 - s.roe * x92) + r1 * x393;
-      const TB x810 = s.QC[2] * x392 + s.WQ[2] * x393 + e1 * (x38 - s.roe * x44);
-      const TB x829 = s.QC[2] * x421 + s.WQ[2] * x422 + e1 * (x72 - s.roe * x82) + r1 * x392;
-      acc168 += static_cast<double>(x829);
-      const TB x1157 = s.QC[2] * x829 + s.WQ[2] * x830 + e2 * (x421 - s.roe * x422) + r1 * x810;
-      acc478 += static_cast<double>(x1157);
-      const TB x424 = s.QC[2] * x73 + s.WQ[2] * x83 + r2 * x39;
-      const TB x487 = s.QC[0] * x195 + s.WQ[0] * x196 + e1 * (x39 - s.roe * x45);
-      const TB x425 = s.QC[2] * x83 + s.WQ[2] * x93 + r2 * x45;
-      const TB x832 = s.QC[2] * x425 + s.WQ[2] * x426 + e1 * (x83 - s.roe * x93) + r2 * x396;
-      const TB x831 = s.QC[2] * x424 + s.WQ[2] * x425 + e1 * (x73 - s.roe * x83) + r2 * x395;
-      acc169 += static_cast<double>(x831);
-      const TB x1158 = s.QC[2] * x831 + s.WQ[2] * x832 + e2 * (x424 - s.roe * x425) + r2 * x812;
-      acc479 += static_cast<double>(x1158);
-      const TB x427 = s.QC[2] * x74 + s.WQ[2] * x84 + r3 * x40;
-      const TB x833 = s.QC[2] * x427 + s.WQ[2] * x428 + e1 * (x74 - s.roe * x84) + r3 * x398;
-      acc170 += static_cast<double>(x833);
-      const TB x508 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x74 - s.roe * x84);
-      acc15 += static_cast<double>(x508);
-      const TB x880 = s.QC[0] * x508 + s.WQ[0] * x509 + e2 * (x227 - s.roe * x228);
-      acc201 += static_cast<double>(x880);
-      const TB x911 = s.QC[1] * x508 + s.WQ[1] * x509;
-      acc232 += static_cast<double>(x911);
-      const TB x481 = s.QC[0] * x186 + s.WQ[0] * x187 + e1 * (x36 - s.roe * x42) + r1 * x179;
-      const TB x676 = s.QC[1] * x286 + s.WQ[1] * x287 + e1 * (x36 - s.roe * x42) + r1 * x277;
-      const TB x743 = s.QC[2] * x289 + s.WQ[2] * x290 + r1 * x277;
-      const TB x483 = s.QC[0] * x189 + s.WQ[0] * x190 + e1 * (x37 - s.roe * x43) + r1 * x181;
-      const TB x678 = s.QC[1] * x289 + s.WQ[1] * x290 + e1 * (x37 - s.roe * x43);
-      const TB x552 = s.QC[1] * x195 + s.WQ[1] * x196 + r1 * x181;
-      const TB x485 = s.QC[0] * x192 + s.WQ[0] * x193 + e1 * (x38 - s.roe * x44);
-      const TB x680 = s.QC[1] * x292 + s.WQ[1] * x293 + e1 * (x38 - s.roe * x44) + r2 * x279;
-      const TB x747 = s.QC[2] * x295 + s.WQ[2] * x296 + r1 * x279;
-      const TB x489 = s.QC[0] * x198 + s.WQ[0] * x199 + e1 * (x40 - s.roe * x46);
-      const TB x684 = s.QC[1] * x298 + s.WQ[1] * x299 + e1 * (x40 - s.roe * x46);
-      const TB x701 = s.QC[1] * x324 + s.WQ[1] * x325 + e1 * (x73 - s.roe * x83) + r1 * x298;
-      acc107 += static_cast<double>(x701);
-      const TB x506 = s.QC[0] * x224 + s.WQ[0] * x225 + e1 * (x73 - s.roe * x83);
-      acc14 += static_cast<double>(x506);
-      const TB x702 = s.QC[1] * x325 + s.WQ[1] * x326 + e1 * (x83 - s.roe * x93) + r1 * x299;
-      const TB x507 = s.QC[0] * x225 + s.WQ[0] * x226 + e1 * (x83 - s.roe * x93);
-      const TB x1065 = s.QC[1] * x701 + s.WQ[1] * x702 + e2 * (x324 - s.roe * x325) + r1 * x684;
-      acc386 += static_cast<double>(x1065);
-      const TB x1096 = s.QC[2] * x701 + s.WQ[2] * x702 + r2 * x682;
-      acc417 += static_cast<double>(x1096);
-      const TB x767 = s.QC[2] * x325 + s.WQ[2] * x326 + r2 * x296;
-      const TB x700 = s.QC[1] * x322 + s.WQ[1] * x323 + e1 * (x82 - s.roe * x92) + r2 * x296;
-      const TB x505 = s.QC[0] * x222 + s.WQ[0] * x223 + e1 * (x82 - s.roe * x92);
-      const TB x765 = s.QC[2] * x322 + s.WQ[2] * x323 + r1 * x293;
-      const TB x879 = s.QC[0] * x506 + s.WQ[0] * x507 + e2 * (x224 - s.roe * x225);
-      acc200 += static_cast<double>(x879);
-      const TB x910 = s.QC[1] * x506 + s.WQ[1] * x507 + r1 * x489;
-      acc231 += static_cast<double>(x910);
-      const TB x941 = s.QC[2] * x506 + s.WQ[2] * x507 + r2 * x487;
-      acc262 += static_cast<double>(x941);
-      const TB x498 = s.QC[0] * x212 + s.WQ[0] * x213 + e1 * (x69 - s.roe * x79) + r1 * x195;
-      const TB x693 = s.QC[1] * x312 + s.WQ[1] * x313 + e1 * (x69 - s.roe * x79) + r1 * x289;
-      const TB x760 = s.QC[2] * x315 + s.WQ[2] * x316 + r2 * x289;
-      acc135 += static_cast<double>(x760);
-      const TB x499 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x79 - s.roe * x89) + r1 * x196;
-      const TB x694 = s.QC[1] * x313 + s.WQ[1] * x314 + e1 * (x79 - s.roe * x89) + r1 * x290;
-      const TB x937 = s.QC[2] * x498 + s.WQ[2] * x499 + r1 * x481;
-      acc258 += static_cast<double>(x937);
-      const TB x1092 = s.QC[2] * x693 + s.WQ[2] * x694 + r1 * x676;
-      acc413 += static_cast<double>(x1092);
-      const TB x1061 = s.QC[1] * x693 + s.WQ[1] * x694 + e2 * (x312 - s.roe * x313) + r1 * x678;
-      acc382 += static_cast<double>(x1061);
-      const TB x759 = s.QC[2] * x313 + s.WQ[2] * x314 + r1 * x287;
-      const TB x741 = s.QC[2] * x286 + s.WQ[2] * x287;
-      const TB x758 = s.QC[2] * x312 + s.WQ[2] * x313 + r1 * x286;
-      const TB x1123 = s.QC[2] * x758 + s.WQ[2] * x759 + e1 * (x312 - s.roe * x313) + r1 * x741;
-      acc444 += static_cast<double>(x1123);
-      const TB x761 = s.QC[2] * x316 + s.WQ[2] * x317 + r2 * x290;
-      const TB x1124 = s.QC[2] * x760 + s.WQ[2] * x761 + e1 * (x315 - s.roe * x316) + r2 * x743;
-      acc445 += static_cast<double>(x1124);
-      const TB x696 = s.QC[1] * x316 + s.WQ[1] * x317 + e1 * (x80 - s.roe * x90);
-      const TB x501 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x80 - s.roe * x90) + r1 * x199;
-      const TB x875 = s.QC[0] * x498 + s.WQ[0] * x499 + e2 * (x212 - s.roe * x213) + r1 * x487;
-      acc196 += static_cast<double>(x875);
-      const TB x906 = s.QC[1] * x498 + s.WQ[1] * x499 + r1 * x483;
-      acc227 += static_cast<double>(x906);
-      const TB x500 = s.QC[0] * x215 + s.WQ[0] * x216 + e1 * (x70 - s.roe * x80) + r1 * x198;
-      acc11 += static_cast<double>(x500);
-      const TB x695 = s.QC[1] * x315 + s.WQ[1] * x316 + e1 * (x70 - s.roe * x80);
-      acc104 += static_cast<double>(x695);
-      const TB x1062 = s.QC[1] * x695 + s.WQ[1] * x696 + e2 * (x315 - s.roe * x316);
-      acc383 += static_cast<double>(x1062);
-      const TB x1093 = s.QC[2] * x695 + s.WQ[2] * x696 + r2 * x678;
-      acc414 += static_cast<double>(x1093);
-      const TB x876 = s.QC[0] * x500 + s.WQ[0] * x501 + e2 * (x215 - s.roe * x216) + r1 * x489;
-      acc197 += static_cast<double>(x876);
-      const TB x938 = s.QC[2] * x500 + s.WQ[2] * x501 + r2 * x483;
-      acc259 += static_cast<double>(x938);
-      const TB x907 = s.QC[1] * x500 + s.WQ[1] * x501;
-      acc228 += static_cast<double>(x907);
-      const TB x502 = s.QC[0] * x218 + s.WQ[0] * x219 + e1 * (x71 - s.roe * x81);
-      acc12 += static_cast<double>(x502);
-      const TB x503 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x81 - s.roe * x91);
-      const TB x698 = s.QC[1] * x319 + s.WQ[1] * x320 + e1 * (x81 - s.roe * x91) + r3 * x293;
-      const TB x745 = s.QC[2] * x292 + s.WQ[2] * x293;
-      const TB x763 = s.QC[2] * x319 + s.WQ[2] * x320;
-      const TB x877 = s.QC[0] * x502 + s.WQ[0] * x503 + e2 * (x218 - s.roe * x219);
-      acc198 += static_cast<double>(x877);
-      const TB x908 = s.QC[1] * x502 + s.WQ[1] * x503 + r3 * x485;
-      acc229 += static_cast<double>(x908);
-      const TB x939 = s.QC[2] * x502 + s.WQ[2] * x503;
-      acc260 += static_cast<double>(x939);
-      const TB x504 = s.QC[0] * x221 + s.WQ[0] * x222 + e1 * (x72 - s.roe * x82);
-      acc13 += static_cast<double>(x504);
-      const TB x699 = s.QC[1] * x321 + s.WQ[1] * x322 + e1 * (x72 - s.roe * x82) + r2 * x295;
-      acc106 += static_cast<double>(x699);
-      const TB x909 = s.QC[1] * x504 + s.WQ[1] * x505 + r2 * x487;
-      acc230 += static_cast<double>(x909);
-      const TB x940 = s.QC[2] * x504 + s.WQ[2] * x505 + r1 * x485;
-      acc261 += static_cast<double>(x940);
-      const TB x878 = s.QC[0] * x504 + s.WQ[0] * x505 + e2 * (x221 - s.roe * x222);
-      acc199 += static_cast<double>(x878);
-      const TB x1064 = s.QC[1] * x699 + s.WQ[1] * x700 + e2 * (x321 - s.roe * x322) + r2 * x682;
-      acc385 += static_cast<double>(x1064);
-      const TB x1095 = s.QC[2] * x699 + s.WQ[2] * x700 + r1 * x680;
-      acc416 += static_cast<double>(x1095);
-      const TB x766 = s.QC[2] * x324 + s.WQ[2] * x325 + r2 * x295;
-      acc138 += static_cast<double>(x766);
-      const TB x1127 = s.QC[2] * x766 + s.WQ[2] * x767 + e1 * (x324 - s.roe * x325) + r2 * x747;
-      acc448 += static_cast<double>(x1127);
-      const TB x546 = s.QC[1] * x186 + s.WQ[1] * x187 + r1 * x177;
-      const TB x613 = s.QC[2] * x189 + s.WQ[2] * x190 + r1 * x177;
-      const TB x548 = s.QC[1] * x189 + s.WQ[1] * x190;
-      const TB x550 = s.QC[1] * x192 + s.WQ[1] * x193 + r2 * x179;
-      const TB x617 = s.QC[2] * x195 + s.WQ[2] * x196 + r1 * x179;
-      const TB x554 = s.QC[1] * x198 + s.WQ[1] * x199;
-      const TB x571 = s.QC[1] * x224 + s.WQ[1] * x225 + r1 * x198;
-      acc45 += static_cast<double>(x571);
-      const TB x572 = s.QC[1] * x225 + s.WQ[1] * x226 + r1 * x199;
-      const TB x972 = s.QC[1] * x571 + s.WQ[1] * x572 + e1 * (x224 - s.roe * x225) + r1 * x554;
-      acc293 += static_cast<double>(x972);
-      const TB x1003 = s.QC[2] * x571 + s.WQ[2] * x572 + r2 * x552;
-      acc324 += static_cast<double>(x1003);
-      const TB x637 = s.QC[2] * x225 + s.WQ[2] * x226 + r2 * x196;
-      const TB x570 = s.QC[1] * x222 + s.WQ[1] * x223 + r2 * x196;
-      const TB x635 = s.QC[2] * x222 + s.WQ[2] * x223 + r1 * x193;
-      const TB x563 = s.QC[1] * x212 + s.WQ[1] * x213 + r1 * x189;
-      const TB x630 = s.QC[2] * x215 + s.WQ[2] * x216 + r2 * x189;
-      acc73 += static_cast<double>(x630);
-      const TB x564 = s.QC[1] * x213 + s.WQ[1] * x214 + r1 * x190;
-      const TB x999 = s.QC[2] * x563 + s.WQ[2] * x564 + r1 * x546;
-      acc320 += static_cast<double>(x999);
-      const TB x968 = s.QC[1] * x563 + s.WQ[1] * x564 + e1 * (x212 - s.roe * x213) + r1 * x548;
-      acc289 += static_cast<double>(x968);
-      const TB x629 = s.QC[2] * x213 + s.WQ[2] * x214 + r1 * x187;
-      const TB x611 = s.QC[2] * x186 + s.WQ[2] * x187;
-      const TB x628 = s.QC[2] * x212 + s.WQ[2] * x213 + r1 * x186;
-      const TB x1030 = s.QC[2] * x628 + s.WQ[2] * x629 + e1 * (x212 - s.roe * x213) + r1 * x611;
-      acc351 += static_cast<double>(x1030);
-      const TB x631 = s.QC[2] * x216 + s.WQ[2] * x217 + r2 * x190;
-      const TB x1031 = s.QC[2] * x630 + s.WQ[2] * x631 + e1 * (x215 - s.roe * x216) + r2 * x613;
-      acc352 += static_cast<double>(x1031);
-      const TB x566 = s.QC[1] * x216 + s.WQ[1] * x217;
-      const TB x565 = s.QC[1] * x215 + s.WQ[1] * x216;
-      acc42 += static_cast<double>(x565);
-      const TB x969 = s.QC[1] * x565 + s.WQ[1] * x566 + e1 * (x215 - s.roe * x216);
-      acc290 += static_cast<double>(x969);
-      const TB x1000 = s.QC[2] * x565 + s.WQ[2] * x566 + r2 * x548;
-      acc321 += static_cast<double>(x1000);
-      const TB x567 = s.QC[1] * x218 + s.WQ[1] * x219 + r3 * x192;
-      acc43 += static_cast<double>(x567);
-      const TB x568 = s.QC[1] * x219 + s.WQ[1] * x220 + r3 * x193;
-      const TB x615 = s.QC[2] * x192 + s.WQ[2] * x193;
-      const TB x633 = s.QC[2] * x219 + s.WQ[2] * x220;
-      const TB x634 = s.QC[2] * x221 + s.WQ[2] * x222 + r1 * x192;
-      acc75 += static_cast<double>(x634);
-      const TB x1033 = s.QC[2] * x634 + s.WQ[2] * x635 + e1 * (x221 - s.roe * x222) + r1 * x615;
-      acc354 += static_cast<double>(x1033);
-      const TB x970 = s.QC[1] * x567 + s.WQ[1] * x568 + e1 * (x218 - s.roe * x219) + r3 * x550;
-      acc291 += static_cast<double>(x970);
-      const TB x1001 = s.QC[2] * x567 + s.WQ[2] * x568;
-      acc322 += static_cast<double>(x1001);
-      const TB x569 = s.QC[1] * x221 + s.WQ[1] * x222 + r2 * x195;
-      acc44 += static_cast<double>(x569);
-      const TB x971 = s.QC[1] * x569 + s.WQ[1] * x570 + e1 * (x221 - s.roe * x222) + r2 * x552;
-      acc292 += static_cast<double>(x971);
-      const TB x1002 = s.QC[2] * x569 + s.WQ[2] * x570 + r1 * x550;
-      acc323 += static_cast<double>(x1002);
-      const TB x636 = s.QC[2] * x224 + s.WQ[2] * x225 + r2 * x195;
-      acc76 += static_cast<double>(x636);
-      const TB x1034 = s.QC[2] * x636 + s.WQ[2] * x637 + e1 * (x224 - s.roe * x225) + r2 * x617;
-      acc355 += static_cast<double>(x1034);
-      const TB x632 = s.QC[2] * x218 + s.WQ[2] * x219;
-      acc74 += static_cast<double>(x632);
-      const TB x1032 = s.QC[2] * x632 + s.WQ[2] * x633 + e1 * (x218 - s.roe * x219);
-      acc353 += static_cast<double>(x1032);
-      const TB x697 = s.QC[1] * x318 + s.WQ[1] * x319 + e1 * (x71 - s.roe * x81) + r3 * x292;
-      acc105 += static_cast<double>(x697);
-      const TB x827 = s.QC[2] * x418 + s.WQ[2] * x419 + e1 * (x71 - s.roe * x81);
-      acc167 += static_cast<double>(x827);
-      const TB x1156 = s.QC[2] * x827 + s.WQ[2] * x828 + e2 * (x418 - s.roe * x419);
-      acc477 += static_cast<double>(x1156);
-      const TB x1063 = s.QC[1] * x697 + s.WQ[1] * x698 + e2 * (x318 - s.roe * x319) + r3 * x680;
-      acc384 += static_cast<double>(x1063);
-      const TB x1094 = s.QC[2] * x697 + s.WQ[2] * x698;
-      acc415 += static_cast<double>(x1094);
-      const TB x764 = s.QC[2] * x321 + s.WQ[2] * x322 + r1 * x292;
-      acc137 += static_cast<double>(x764);
-      const TB x1126 = s.QC[2] * x764 + s.WQ[2] * x765 + e1 * (x321 - s.roe * x322) + r1 * x745;
-      acc447 += static_cast<double>(x1126);
-      const TB x762 = s.QC[2] * x318 + s.WQ[2] * x319;
-      acc136 += static_cast<double>(x762);
-      const TB x1125 = s.QC[2] * x762 + s.WQ[2] * x763 + e1 * (x318 - s.roe * x319);
-      acc446 += static_cast<double>(x1125);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(165 * R + r) * a.acc_stride + q] = acc165;
-  a.acc[(196 * R + r) * a.acc_stride + q] = acc196;
-  a.acc[(227 * R + r) * a.acc_stride + q] = acc227;
-  a.acc[(258 * R + r) * a.acc_stride + q] = acc258;
-  a.acc[(289 * R + r) * a.acc_stride + q] = acc289;
-  a.acc[(320 * R + r) * a.acc_stride + q] = acc320;
-  a.acc[(351 * R + r) * a.acc_stride + q] = acc351;
-  a.acc[(382 * R + r) * a.acc_stride + q] = acc382;
-  a.acc[(413 * R + r) * a.acc_stride + q] = acc413;
-  a.acc[(444 * R + r) * a.acc_stride + q] = acc444;
-  a.acc[(475 * R + r) * a.acc_stride + q] = acc475;
-  a.acc[(11 * R + r) * a.acc_stride + q] = acc11;
-  a.acc[(42 * R + r) * a.acc_stride + q] = acc42;
-  a.acc[(73 * R + r) * a.acc_stride + q] = acc73;
-  a.acc[(104 * R + r) * a.acc_stride + q] = acc104;
-  a.acc[(135 * R + r) * a.acc_stride + q] = acc135;
-  a.acc[(166 * R + r) * a.acc_stride + q] = acc166;
-  a.acc[(197 * R + r) * a.acc_stride + q] = acc197;
-  a.acc[(228 * R + r) * a.acc_stride + q] = acc228;
-  a.acc[(259 * R + r) * a.acc_stride + q] = acc259;
-  a.acc[(290 * R + r) * a.acc_stride + q] = acc290;
-  a.acc[(321 * R + r) * a.acc_stride + q] = acc321;
-  a.acc[(352 * R + r) * a.acc_stride + q] = acc352;
-  a.acc[(383 * R + r) * a.acc_stride + q] = acc383;
-  a.acc[(414 * R + r) * a.acc_stride + q] = acc414;
-  a.acc[(445 * R + r) * a.acc_stride + q] = acc445;
-  a.acc[(476 * R + r) * a.acc_stride + q] = acc476;
-  a.acc[(12 * R + r) * a.acc_stride + q] = acc12;
-  a.acc[(43 * R + r) * a.acc_stride + q] = acc43;
-  a.acc[(74 * R + r) * a.acc_stride + q] = acc74;
-  a.acc[(105 * R + r) * a.acc_stride + q] = acc105;
-  a.acc[(136 * R + r) * a.acc_stride + q] = acc136;
-  a.acc[(167 * R + r) * a.acc_stride + q] = acc167;
-  a.acc[(198 * R + r) * a.acc_stride + q] = acc198;
-  a.acc[(229 * R + r) * a.acc_stride + q] = acc229;
-  a.acc[(260 * R + r) * a.acc_stride + q] = acc260;
-  a.acc[(291 * R + r) * a.acc_stride + q] = acc291;
-  a.acc[(322 * R + r) * a.acc_stride + q] = acc322;
-  a.acc[(353 * R + r) * a.acc_stride + q] = acc353;
-  a.acc[(384 * R + r) * a.acc_stride + q] = acc384;
-  a.acc[(415 * R + r) * a.acc_stride + q] = acc415;
-  a.acc[(446 * R + r) * a.acc_stride + q] = acc446;
-  a.acc[(477 * R + r) * a.acc_stride + q] = acc477;
-  a.acc[(13 * R + r) * a.acc_stride + q] = acc13;
-  a.acc[(44 * R + r) * a.acc_stride + q] = acc44;
-  a.acc[(75 * R + r) * a.acc_stride + q] = acc75;
-  a.acc[(106 * R + r) * a.acc_stride + q] = acc106;
-  a.acc[(137 * R + r) * a.acc_stride + q] = acc137;
-  a.acc[(168 * R + r) * a.acc_stride + q] = acc168;
-  a.acc[(199 * R + r) * a.acc_stride + q] = acc199;
-  a.acc[(230 * R + r) * a.acc_stride + q] = acc230;
-  a.acc[(261 * R + r) * a.acc_stride + q] = acc261;
-  a.acc[(292 * R + r) * a.acc_stride + q] = acc292;
-  a.acc[(323 * R + r) * a.acc_stride + q] = acc323;
-  a.acc[(354 * R + r) * a.acc_stride + q] = acc354;
-  a.acc[(385 * R + r) * a.acc_stride + q] = acc385;
-  a.acc[(416 * R + r) * a.acc_stride + q] = acc416;
-  a.acc[(447 * R + r) * a.acc_stride + q] = acc447;
-  a.acc[(478 * R + r) * a.acc_stride + q] = acc478;
-  a.acc[(14 * R + r) * a.acc_stride + q] = acc14;
-  a.acc[(45 * R + r) * a.acc_stride + q] = acc45;
-  a.acc[(76 * R + r) * a.acc_stride + q] = acc76;
-  a.acc[(107 * R + r) * a.acc_stride + q] = acc107;
-  a.acc[(138 * R + r) * a.acc_stride + q] = acc138;
-  a.acc[(169 * R + r) * a.acc_stride + q] = acc169;
-  a.acc[(200 * R + r) * a.acc_stride + q] = acc200;
-  a.acc[(231 * R + r) * a.acc_stride + q] = acc231;
-  a.acc[(262 * R + r) * a.acc_stride + q] = acc262;
-  a.acc[(293 * R + r) * a.acc_stride + q] = acc293;
-  a.acc[(324 * R + r) * a.acc_stride + q] = acc324;
-  a.acc[(355 * R + r) * a.acc_stride + q] = acc355;
-  a.acc[(386 * R + r) * a.acc_stride + q] = acc386;
-  a.acc[(417 * R + r) * a.acc_stride + q] = acc417;
-  a.acc[(448 * R + r) * a.acc_stride + q] = acc448;
-  a.acc[(479 * R + r) * a.acc_stride + q] = acc479;
-  a.acc[(15 * R + r) * a.acc_stride + q] = acc15;
-  a.acc[(46 * R + r) * a.acc_stride + q] = acc46;
-  a.acc[(77 * R + r) * a.acc_stride + q] = acc77;
-  a.acc[(108 * R + r) * a.acc_stride + q] = acc108;
-  a.acc[(139 * R + r) * a.acc_stride + q] = acc139;
-  a.acc[(170 * R + r) * a.acc_stride + q] = acc170;
-  a.acc[(201 * R + r) * a.acc_stride + q] = acc201;
-  a.acc[(232 * R + r) * a.acc_stride + q] = acc232;
-}
-// class (22|21) split 3/6: 304 recursion values, 82 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2221_3(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc263 = 0.0;
-  double acc294 = 0.0;
-  double acc325 = 0.0;
-  double acc356 = 0.0;
-  double acc387 = 0.0;
-  double acc418 = 0.0;
-  double acc449 = 0.0;
-  double acc480 = 0.0;
-  double acc16 = 0.0;
-  double acc47 = 0.0;
-  double acc78 = 0.0;
-  double acc109 = 0.0;
-  double acc140 = 0.0;
-  double acc171 = 0.0;
-  double acc202 = 0.0;
-  double acc233 = 0.0;
-  double acc264 = 0.0;
-  double acc295 = 0.0;
-  double acc326 = 0.0;
-  double acc357 = 0.0;
-  double acc388 = 0.0;
-  double acc419 = 0.0;
-  double acc450 = 0.0;
-  double acc481 = 0.0;
-  double acc17 = 0.0;
-  double acc48 = 0.0;
-  double acc79 = 0.0;
-  double acc110 = 0.0;
-  double acc141 = 0.0;
-  double acc172 = 0.0;
-  double acc203 = 0.0;
-  double acc234 = 0.0;
-  double acc265 = 0.0;
-  double acc296 = 0.0;
-  double acc327 = 0.0;
-  double acc358 = 0.0;
-  double acc389 = 0.0;
-  double acc420 = 0.0;
-  double acc451 = 0.0;
-  double acc482 = 0.0;
-  double acc18 = 0.0;
-  double acc49 = 0.0;
-  double acc80 = 0.0;
-  double acc111 = 0.0;
-  double acc142 = 0.0;
-  double acc173 = 0.0;
-  double acc204 = 0.0;
-  double acc235 = 0.0;
-  double acc266 = 0.0;
-  double acc297 = 0.0;
-  double acc328 = 0.0;
-  double acc359 = 0.0;
-  double acc390 = 0.0;
-  double acc421 = 0.0;
-  double acc452 = 0.0;
-  double acc483 = 0.0;
-  double acc19 = 0.0;
-  double acc50 = 0.0;
-  double acc81 = 0.0;
-  double acc112 = 0.0;
-  double acc143 = 0.0;
-  double acc174 = 0.0;
-  double acc205 = 0.0;
-  double acc236 = 0.0;
-  double acc267 = 0.0;
-  double acc298 = 0.0;
-  double acc329 = 0.0;
-  double acc360 = 0.0;
-  double acc391 = 0.0;
-  double acc422 = 0.0;
-  double acc453 = 0.0;
-  double acc484 = 0.0;
-  double acc20 = 0.0;
-  double acc51 = 0.0;
-  double acc82 = 0.0;
-  double acc113 = 0.0;
-  double acc144 = 0.0;
-  double acc175 = 0.0;
-  double acc206 = 0.0;
-  double acc237 = 0.0;
-  double acc268 = 0.0;
-  double acc299 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[8];
-      boys_eval_fixed<7>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x26 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x8 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x10 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x11 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x13 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x14 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x15 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x16 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x17 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x18 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x19 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x20 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x21 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x22 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x23 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x59 = s.PA[0] * x23 + s.WP[0] * x26 + z1 * (x5 - s.roz * x6);
-      const TB x25 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x58 = s.PA[2] * x22 + s.WP[2] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x53 = s.PA[0] * x20 + s.WP[0] * x23 + z1 * (x4 - s.roz * x5);
-      const TB x29 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x34 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x35 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x40 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x74 = s.PA[2] * x34 + s.WP[2] * x40 + z2 * (x10 - s.roz * x13);
-      const TB x65 = s.PA[0] * x29 + s.WP[0] * x35 + z2 * (x8 - s.roz * x11);
-      const TB x36 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x41 = s.PA[0] * x14 + s.WP[0] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[1] * x14 + s.WP[1] * x17;
-      const TB x43 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x44 = s.PA[1] * x15 + s.WP[1] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x45 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x46 = s.PA[2] * x16 + s.WP[2] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x84 = s.PA[2] * x40 + s.WP[2] * x46 + z2 * (x13 - s.roz * x16);
-      const TB x47 = s.PA[0] * x17 + s.WP[0] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x48 = s.PA[1] * x17 + s.WP[1] * x20;
-      const TB x49 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x50 = s.PA[1] * x18 + s.WP[1] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x51 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x52 = s.PA[2] * x19 + s.WP[2] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x104 = s.PA[2] * x52 + s.WP[2] * x58 + z2 * (x19 - s.roz * x22);
-      const TB x187 = s.QC[0] * x42 + s.WQ[0] * x48 + r1 * x18;
-      const TB x381 = s.QC[2] * x16 + s.WQ[2] * x19 + r1 * x3;
-      const TB x54 = s.PA[1] * x20 + s.WP[1] * x23;
-      const TB x105 = s.PA[0] * x53 + s.WP[0] * x59 + z2 * (x20 - s.roz * x23);
-      const TB x106 = s.PA[1] * x53 + s.WP[1] * x59;
-      const TB x66 = s.PA[1] * x29 + s.WP[1] * x35;
-      const TB x75 = s.PA[0] * x35 + s.WP[0] * x41 + z2 * (x11 - s.roz * x14);
-      const TB x78 = s.PA[1] * x36 + s.WP[1] * x42 + z1 * (x11 - s.roz * x14);
-      const TB x79 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x76 = s.PA[1] * x35 + s.WP[1] * x41;
-      const TB x77 = s.PA[2] * x35 + s.WP[2] * x41;
-      const TB x85 = s.PA[0] * x41 + s.WP[0] * x47 + z2 * (x14 - s.roz * x17);
-      const TB x88 = s.PA[1] * x42 + s.WP[1] * x48 + z1 * (x14 - s.roz * x17);
-      const TB x86 = s.PA[1] * x41 + s.WP[1] * x47;
-      const TB x87 = s.PA[2] * x41 + s.WP[2] * x47;
-      const TB x89 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x204 = s.QC[0] * x76 + s.WQ[0] * x86 + r2 * x42;
-      const TB x94 = s.PA[2] * x46 + s.WP[2] * x52 + z2 * (x16 - s.roz * x19);
-      const TB x95 = s.PA[0] * x47 + s.WP[0] * x53 + z2 * (x17 - s.roz * x20);
-      const TB x98 = s.PA[1] * x48 + s.WP[1] * x54 + z1 * (x17 - s.roz * x20);
-      const TB x99 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x184 = s.QC[0] * x41 + s.WQ[0] * x47 + r2 * x17;
-      const TB x96 = s.PA[1] * x47 + s.WP[1] * x53;
-      const TB x205 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x48;
-      const TB x493 = s.QC[0] * x204 + s.WQ[0] * x205 + e1 * (x76 - s.roe * x86) + r2 * x187;
-      const TB x97 = s.PA[2] * x47 + s.WP[2] * x53;
-      const TB x115 = s.PA[0] * x65 + s.WP[0] * x75 + z3 * (x29 - s.roz * x35);
-      const TB x118 = s.PA[1] * x66 + s.WP[1] * x76 + z1 * (x29 - s.roz * x35);
-      const TB x119 = s.PA[2] * x66 + s.WP[2] * x76;
-      const TB x116 = s.PA[1] * x65 + s.WP[1] * x75;
-      const TB x117 = s.PA[2] * x65 + s.WP[2] * x75;
-      const TB x130 = s.PA[0] * x75 + s.WP[0] * x85 + z3 * (x35 - s.roz * x41);
-      const TB x133 = s.PA[1] * x76 + s.WP[1] * x86 + z1 * (x35 - s.roz * x41);
-      const TB x131 = s.PA[1] * x75 + s.WP[1] * x85;
-      const TB x132 = s.PA[2] * x75 + s.WP[2] * x85;
-      const TB x134 = s.PA[2] * x76 + s.WP[2] * x86;
-      const TB x145 = s.PA[0] * x85 + s.WP[0] * x95 + z3 * (x41 - s.roz * x47);
-      const TB x146 = s.PA[1] * x85 + s.WP[1] * x95;
-      const TB x147 = s.PA[2] * x85 + s.WP[2] * x95;
-      const TB x148 = s.PA[1] * x86 + s.WP[1] * x96 + z1 * (x41 - s.roz * x47);
-      const TB x149 = s.PA[2] * x86 + s.WP[2] * x96;
-      const TB x160 = s.PA[0] * x95 + s.WP[0] * x105 + z3 * (x47 - s.roz * x53);
-      const TB x163 = s.PA[1] * x96 + s.WP[1] * x106 + z1 * (x47 - s.roz * x53);
-      const TB x164 = s.PA[2] * x96 + s.WP[2] * x106;
-      const TB x244 = s.QC[0] * x149 + s.WQ[0] * x164 + r2 * x99;
-      const TB x214 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x51;
-      const TB x196 = s.QC[0] * x45 + s.WQ[0] * x51;
-      const TB x213 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x45;
-      const TB x499 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x79 - s.roe * x89) + r1 * x196;
-      const TB x242 = s.QC[0] * x119 + s.WQ[0] * x134 + r2 * x79;
-      const TB x243 = s.QC[0] * x134 + s.WQ[0] * x149 + r2 * x89;
-      const TB x518 = s.QC[0] * x242 + s.WQ[0] * x243 + e1 * (x119 - s.roe * x134) + r2 * x213;
-      acc20 += static_cast<double>(x518);
-      const TB x519 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x134 - s.roe * x149) + r2 * x214;
-      const TB x885 = s.QC[0] * x518 + s.WQ[0] * x519 + e2 * (x242 - s.roe * x243) + r2 * x499;
-      acc206 += static_cast<double>(x885);
-      const TB x648 = s.QC[2] * x242 + s.WQ[2] * x243 + r1 * x204;
-      acc82 += static_cast<double>(x648);
-      const TB x947 = s.QC[2] * x518 + s.WQ[2] * x519 + r1 * x493;
-      acc268 += static_cast<double>(x947);
-      const TB x161 = s.PA[1] * x95 + s.WP[1] * x105;
-      const TB x162 = s.PA[2] * x95 + s.WP[2] * x105;
-      const TB x181 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x190 = s.QC[0] * x43 + s.WQ[0] * x49 + r1 * x19;
-      const TB x207 = s.QC[0] * x77 + s.WQ[0] * x87 + r2 * x43;
-      const TB x208 = s.QC[0] * x87 + s.WQ[0] * x97 + r2 * x49;
-      const TB x238 = s.QC[0] * x147 + s.WQ[0] * x162 + r3 * x97;
-      const TB x495 = s.QC[0] * x207 + s.WQ[0] * x208 + e1 * (x77 - s.roe * x87) + r2 * x190;
-      const TB x916 = s.QC[1] * x518 + s.WQ[1] * x519 + r1 * x495;
-      acc237 += static_cast<double>(x916);
-      const TB x584 = s.QC[1] * x243 + s.WQ[1] * x244 + r1 * x208;
-      const TB x193 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x210 = s.QC[0] * x78 + s.WQ[0] * x88 + r1 * x44;
-      const TB x211 = s.QC[0] * x88 + s.WQ[0] * x98 + r1 * x50;
-      const TB x241 = s.QC[0] * x148 + s.WQ[0] * x163 + r2 * x98;
-      const TB x497 = s.QC[0] * x210 + s.WQ[0] * x211 + e1 * (x78 - s.roe * x88) + r1 * x193;
-      const TB x239 = s.QC[0] * x118 + s.WQ[0] * x133 + r2 * x78;
-      const TB x240 = s.QC[0] * x133 + s.WQ[0] * x148 + r2 * x88;
-      const TB x516 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x118 - s.roe * x133) + r2 * x210;
-      acc19 += static_cast<double>(x516);
-      const TB x517 = s.QC[0] * x240 + s.WQ[0] * x241 + e1 * (x133 - s.roe * x148) + r2 * x211;
-      const TB x884 = s.QC[0] * x516 + s.WQ[0] * x517 + e2 * (x239 - s.roe * x240) + r2 * x497;
-      acc205 += static_cast<double>(x884);
-      const TB x915 = s.QC[1] * x516 + s.WQ[1] * x517 + r2 * x493;
-      acc236 += static_cast<double>(x915);
-      const TB x946 = s.QC[2] * x516 + s.WQ[2] * x517;
-      acc267 += static_cast<double>(x946);
-      const TB x198 = s.QC[0] * x40 + s.WQ[0] * x46;
-      const TB x199 = s.QC[0] * x46 + s.WQ[0] * x52;
-      const TB x619 = s.QC[2] * x198 + s.WQ[2] * x199 + r2 * x181;
-      const TB x201 = s.QC[0] * x75 + s.WQ[0] * x85 + r3 * x41;
-      const TB x202 = s.QC[0] * x85 + s.WQ[0] * x95 + r3 * x47;
-      const TB x227 = s.QC[0] * x74 + s.WQ[0] * x84;
-      const TB x228 = s.QC[0] * x84 + s.WQ[0] * x94;
-      const TB x229 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x230 = s.QC[0] * x115 + s.WQ[0] * x130 + r4 * x75;
-      const TB x231 = s.QC[0] * x130 + s.WQ[0] * x145 + r4 * x85;
-      const TB x232 = s.QC[0] * x145 + s.WQ[0] * x160 + r4 * x95;
-      const TB x233 = s.QC[0] * x116 + s.WQ[0] * x131 + r3 * x76;
-      const TB x234 = s.QC[0] * x131 + s.WQ[0] * x146 + r3 * x86;
-      const TB x235 = s.QC[0] * x146 + s.WQ[0] * x161 + r3 * x96;
-      const TB x236 = s.QC[0] * x117 + s.WQ[0] * x132 + r3 * x77;
-      const TB x237 = s.QC[0] * x132 + s.WQ[0] * x147 + r3 * x87;
-      const TB x281 = s.QC[1] * x16 + s.WQ[1] * x19;
-      const TB x398 = s.QC[2] * x40 + s.WQ[2] * x46 + r2 * x16;
-      const TB x399 = s.QC[2] * x46 + s.WQ[2] * x52 + r2 * x19;
-      const TB x814 = s.QC[2] * x398 + s.WQ[2] * x399 + e1 * (x40 - s.roe * x46) + r2 * x381;
-      const TB x284 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x304 = s.QC[1] * x76 + s.WQ[1] * x86 + r1 * x41;
-      const TB x305 = s.QC[1] * x86 + s.WQ[1] * x96 + r1 * x47;
-      const TB x341 = s.QC[1] * x148 + s.WQ[1] * x163 + r2 * x96;
-      const TB x441 = s.QC[2] * x148 + s.WQ[2] * x163;
-      const TB x688 = s.QC[1] * x304 + s.WQ[1] * x305 + e1 * (x76 - s.roe * x86) + r1 * x284;
-      const TB x298 = s.QC[1] * x40 + s.WQ[1] * x46;
-      const TB x299 = s.QC[1] * x46 + s.WQ[1] * x52;
-      const TB x429 = s.QC[2] * x94 + s.WQ[2] * x104 + r3 * x52;
-      const TB x329 = s.QC[1] * x94 + s.WQ[1] * x104;
-      const TB x749 = s.QC[2] * x298 + s.WQ[2] * x299 + r2 * x281;
-      const TB x301 = s.QC[1] * x75 + s.WQ[1] * x85;
-      const TB x302 = s.QC[1] * x85 + s.WQ[1] * x95;
-      const TB x307 = s.QC[1] * x77 + s.WQ[1] * x87;
-      const TB x342 = s.QC[1] * x119 + s.WQ[1] * x134 + r1 * x77;
-      const TB x343 = s.QC[1] * x134 + s.WQ[1] * x149 + r1 * x87;
-      const TB x713 = s.QC[1] * x342 + s.WQ[1] * x343 + e1 * (x119 - s.roe * x134) + r1 * x307;
-      acc113 += static_cast<double>(x713);
-      const TB x778 = s.QC[2] * x342 + s.WQ[2] * x343 + r1 * x304;
-      acc144 += static_cast<double>(x778);
-      const TB x443 = s.QC[2] * x134 + s.WQ[2] * x149 + r1 * x86;
-      const TB x327 = s.QC[1] * x74 + s.WQ[1] * x84;
-      const TB x328 = s.QC[1] * x84 + s.WQ[1] * x94;
-      const TB x330 = s.QC[1] * x115 + s.WQ[1] * x130;
-      const TB x331 = s.QC[1] * x130 + s.WQ[1] * x145;
-      const TB x332 = s.QC[1] * x145 + s.WQ[1] * x160;
-      const TB x432 = s.QC[2] * x145 + s.WQ[2] * x160;
-      const TB x333 = s.QC[1] * x116 + s.WQ[1] * x131 + r1 * x75;
-      const TB x334 = s.QC[1] * x131 + s.WQ[1] * x146 + r1 * x85;
-      const TB x335 = s.QC[1] * x146 + s.WQ[1] * x161 + r1 * x95;
-      const TB x435 = s.QC[2] * x146 + s.WQ[2] * x161;
-      const TB x336 = s.QC[1] * x117 + s.WQ[1] * x132;
-      const TB x337 = s.QC[1] * x132 + s.WQ[1] * x147;
-      const TB x338 = s.QC[1] * x147 + s.WQ[1] * x162;
-      const TB x438 = s.QC[2] * x147 + s.WQ[2] * x162 + r1 * x95;
-      const TB x402 = s.QC[2] * x85 + s.WQ[2] * x95;
-      const TB x339 = s.QC[1] * x118 + s.WQ[1] * x133 + r2 * x76;
-      const TB x340 = s.QC[1] * x133 + s.WQ[1] * x148 + r2 * x86;
-      const TB x404 = s.QC[2] * x76 + s.WQ[2] * x86;
-      const TB x442 = s.QC[2] * x119 + s.WQ[2] * x134 + r1 * x76;
-      const TB x843 = s.QC[2] * x442 + s.WQ[2] * x443 + e1 * (x119 - s.roe * x134) + r1 * x404;
-      acc175 += static_cast<double>(x843);
-      const TB x711 = s.QC[1] * x339 + s.WQ[1] * x340 + e1 * (x118 - s.roe * x133) + r2 * x304;
-      acc112 += static_cast<double>(x711);
-      const TB x712 = s.QC[1] * x340 + s.WQ[1] * x341 + e1 * (x133 - s.roe * x148) + r2 * x305;
-      const TB x1070 = s.QC[1] * x711 + s.WQ[1] * x712 + e2 * (x339 - s.roe * x340) + r2 * x688;
-      acc391 += static_cast<double>(x1070);
-      const TB x1101 = s.QC[2] * x711 + s.WQ[2] * x712;
-      acc422 += static_cast<double>(x1101);
-      const TB x777 = s.QC[2] * x340 + s.WQ[2] * x341;
-      const TB x401 = s.QC[2] * x75 + s.WQ[2] * x85;
-      const TB x427 = s.QC[2] * x74 + s.WQ[2] * x84 + r3 * x40;
-      const TB x428 = s.QC[2] * x84 + s.WQ[2] * x94 + r3 * x46;
-      const TB x834 = s.QC[2] * x428 + s.WQ[2] * x429 + e1 * (x84 - s.roe * x94) + r3 * x399;
-      const TB x833 = s.QC[2] * x427 + s.WQ[2] * x428 + e1 * (x74 - s.roe * x84) + r3 * x398;
-      const TB x1159 = s.QC[2] * x833 + s.WQ[2] * x834 + e2 * (x427 - s.roe * x428) + r3 * x814;
-      acc480 += static_cast<double>(x1159);
-      const TB x430 = s.QC[2] * x115 + s.WQ[2] * x130;
-      const TB x431 = s.QC[2] * x130 + s.WQ[2] * x145;
-      const TB x836 = s.QC[2] * x431 + s.WQ[2] * x432 + e1 * (x130 - s.roe * x145);
-      const TB x433 = s.QC[2] * x116 + s.WQ[2] * x131;
-      const TB x434 = s.QC[2] * x131 + s.WQ[2] * x146;
-      const TB x838 = s.QC[2] * x434 + s.WQ[2] * x435 + e1 * (x131 - s.roe * x146);
-      const TB x436 = s.QC[2] * x117 + s.WQ[2] * x132 + r1 * x75;
-      const TB x437 = s.QC[2] * x132 + s.WQ[2] * x147 + r1 * x85;
-      const TB x840 = s.QC[2] * x437 + s.WQ[2] * x438 + e1 * (x132 - s.roe * x147) + r1 * x402;
-      const TB x816 = s.QC[2] * x401 + s.WQ[2] * x402 + e1 * (x75 - s.roe * x85);
-      const TB x839 = s.QC[2] * x436 + s.WQ[2] * x437 + e1 * (x117 - s.roe * x132) + r1 * x401;
-      acc173 += static_cast<double>(x839);
-      const TB x1162 = s.QC[2] * x839 + s.WQ[2] * x840 + e2 * (x436 - s.roe * x437) + r1 * x816;
-      acc483 += static_cast<double>(x1162);
-      const TB x439 = s.QC[2] * x118 + s.WQ[2] * x133;
-      const TB x440 = s.QC[2] * x133 + s.WQ[2] * x148;
-      const TB x842 = s.QC[2] * x440 + s.WQ[2] * x441 + e1 * (x133 - s.roe * x148);
-      const TB x841 = s.QC[2] * x439 + s.WQ[2] * x440 + e1 * (x118 - s.roe * x133);
-      acc174 += static_cast<double>(x841);
-      const TB x1163 = s.QC[2] * x841 + s.WQ[2] * x842 + e2 * (x439 - s.roe * x440);
-      acc484 += static_cast<double>(x1163);
-      const TB x489 = s.QC[0] * x198 + s.WQ[0] * x199 + e1 * (x40 - s.roe * x46);
-      const TB x684 = s.QC[1] * x298 + s.WQ[1] * x299 + e1 * (x40 - s.roe * x46);
-      const TB x768 = s.QC[2] * x327 + s.WQ[2] * x328 + r3 * x298;
-      const TB x769 = s.QC[2] * x328 + s.WQ[2] * x329 + r3 * x299;
-      const TB x1128 = s.QC[2] * x768 + s.WQ[2] * x769 + e1 * (x327 - s.roe * x328) + r3 * x749;
-      acc449 += static_cast<double>(x1128);
-      const TB x704 = s.QC[1] * x328 + s.WQ[1] * x329 + e1 * (x84 - s.roe * x94);
-      const TB x509 = s.QC[0] * x228 + s.WQ[0] * x229 + e1 * (x84 - s.roe * x94);
-      const TB x491 = s.QC[0] * x201 + s.WQ[0] * x202 + e1 * (x75 - s.roe * x85) + r3 * x184;
-      const TB x686 = s.QC[1] * x301 + s.WQ[1] * x302 + e1 * (x75 - s.roe * x85);
-      const TB x558 = s.QC[1] * x204 + s.WQ[1] * x205 + r1 * x184;
-      const TB x508 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x74 - s.roe * x84);
-      const TB x942 = s.QC[2] * x508 + s.WQ[2] * x509 + r3 * x489;
-      acc263 += static_cast<double>(x942);
-      const TB x703 = s.QC[1] * x327 + s.WQ[1] * x328 + e1 * (x74 - s.roe * x84);
-      const TB x1066 = s.QC[1] * x703 + s.WQ[1] * x704 + e2 * (x327 - s.roe * x328);
-      acc387 += static_cast<double>(x1066);
-      const TB x1097 = s.QC[2] * x703 + s.WQ[2] * x704 + r3 * x684;
-      acc418 += static_cast<double>(x1097);
-      const TB x510 = s.QC[0] * x230 + s.WQ[0] * x231 + e1 * (x115 - s.roe * x130) + r4 * x201;
-      acc16 += static_cast<double>(x510);
-      const TB x511 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x130 - s.roe * x145) + r4 * x202;
-      const TB x706 = s.QC[1] * x331 + s.WQ[1] * x332 + e1 * (x130 - s.roe * x145);
-      const TB x771 = s.QC[2] * x331 + s.WQ[2] * x332;
-      const TB x881 = s.QC[0] * x510 + s.WQ[0] * x511 + e2 * (x230 - s.roe * x231) + r4 * x491;
-      acc202 += static_cast<double>(x881);
-      const TB x912 = s.QC[1] * x510 + s.WQ[1] * x511;
-      acc233 += static_cast<double>(x912);
-      const TB x943 = s.QC[2] * x510 + s.WQ[2] * x511;
-      acc264 += static_cast<double>(x943);
-      const TB x512 = s.QC[0] * x233 + s.WQ[0] * x234 + e1 * (x116 - s.roe * x131) + r3 * x204;
-      acc17 += static_cast<double>(x512);
-      const TB x581 = s.QC[1] * x239 + s.WQ[1] * x240 + r2 * x204;
-      acc50 += static_cast<double>(x581);
-      const TB x513 = s.QC[0] * x234 + s.WQ[0] * x235 + e1 * (x131 - s.roe * x146) + r3 * x205;
-      const TB x882 = s.QC[0] * x512 + s.WQ[0] * x513 + e2 * (x233 - s.roe * x234) + r3 * x493;
-      acc203 += static_cast<double>(x882);
-      const TB x582 = s.QC[1] * x240 + s.WQ[1] * x241 + r2 * x205;
-      const TB x977 = s.QC[1] * x581 + s.WQ[1] * x582 + e1 * (x239 - s.roe * x240) + r2 * x558;
-      acc298 += static_cast<double>(x977);
-      const TB x1008 = s.QC[2] * x581 + s.WQ[2] * x582;
-      acc329 += static_cast<double>(x1008);
-      const TB x647 = s.QC[2] * x240 + s.WQ[2] * x241;
-      const TB x708 = s.QC[1] * x334 + s.WQ[1] * x335 + e1 * (x131 - s.roe * x146) + r1 * x302;
-      const TB x773 = s.QC[2] * x334 + s.WQ[2] * x335;
-      const TB x913 = s.QC[1] * x512 + s.WQ[1] * x513 + r1 * x491;
-      acc234 += static_cast<double>(x913);
-      const TB x944 = s.QC[2] * x512 + s.WQ[2] * x513;
-      acc265 += static_cast<double>(x944);
-      const TB x514 = s.QC[0] * x236 + s.WQ[0] * x237 + e1 * (x117 - s.roe * x132) + r3 * x207;
-      acc18 += static_cast<double>(x514);
-      const TB x709 = s.QC[1] * x336 + s.WQ[1] * x337 + e1 * (x117 - s.roe * x132);
-      acc111 += static_cast<double>(x709);
-      const TB x515 = s.QC[0] * x237 + s.WQ[0] * x238 + e1 * (x132 - s.roe * x147) + r3 * x208;
-      const TB x710 = s.QC[1] * x337 + s.WQ[1] * x338 + e1 * (x132 - s.roe * x147);
-      const TB x883 = s.QC[0] * x514 + s.WQ[0] * x515 + e2 * (x236 - s.roe * x237) + r3 * x495;
-      acc204 += static_cast<double>(x883);
-      const TB x945 = s.QC[2] * x514 + s.WQ[2] * x515 + r1 * x491;
-      acc266 += static_cast<double>(x945);
-      const TB x914 = s.QC[1] * x514 + s.WQ[1] * x515;
-      acc235 += static_cast<double>(x914);
-      const TB x560 = s.QC[1] * x207 + s.WQ[1] * x208;
-      const TB x583 = s.QC[1] * x242 + s.WQ[1] * x243 + r1 * x207;
-      acc51 += static_cast<double>(x583);
-      const TB x978 = s.QC[1] * x583 + s.WQ[1] * x584 + e1 * (x242 - s.roe * x243) + r1 * x560;
-      acc299 += static_cast<double>(x978);
-      const TB x775 = s.QC[2] * x337 + s.WQ[2] * x338 + r1 * x302;
-      const TB x751 = s.QC[2] * x301 + s.WQ[2] * x302;
-      const TB x1069 = s.QC[1] * x709 + s.WQ[1] * x710 + e2 * (x336 - s.roe * x337);
-      acc390 += static_cast<double>(x1069);
-      const TB x1100 = s.QC[2] * x709 + s.WQ[2] * x710 + r1 * x686;
-      acc421 += static_cast<double>(x1100);
-      const TB x554 = s.QC[1] * x198 + s.WQ[1] * x199;
-      const TB x638 = s.QC[2] * x227 + s.WQ[2] * x228 + r3 * x198;
-      const TB x639 = s.QC[2] * x228 + s.WQ[2] * x229 + r3 * x199;
-      const TB x1035 = s.QC[2] * x638 + s.WQ[2] * x639 + e1 * (x227 - s.roe * x228) + r3 * x619;
-      acc356 += static_cast<double>(x1035);
-      const TB x574 = s.QC[1] * x228 + s.WQ[1] * x229;
-      const TB x556 = s.QC[1] * x201 + s.WQ[1] * x202;
-      const TB x573 = s.QC[1] * x227 + s.WQ[1] * x228;
-      const TB x973 = s.QC[1] * x573 + s.WQ[1] * x574 + e1 * (x227 - s.roe * x228);
-      acc294 += static_cast<double>(x973);
-      const TB x1004 = s.QC[2] * x573 + s.WQ[2] * x574 + r3 * x554;
-      acc325 += static_cast<double>(x1004);
-      const TB x575 = s.QC[1] * x230 + s.WQ[1] * x231;
-      acc47 += static_cast<double>(x575);
-      const TB x576 = s.QC[1] * x231 + s.WQ[1] * x232;
-      const TB x641 = s.QC[2] * x231 + s.WQ[2] * x232;
-      const TB x974 = s.QC[1] * x575 + s.WQ[1] * x576 + e1 * (x230 - s.roe * x231);
-      acc295 += static_cast<double>(x974);
-      const TB x1005 = s.QC[2] * x575 + s.WQ[2] * x576;
-      acc326 += static_cast<double>(x1005);
-      const TB x577 = s.QC[1] * x233 + s.WQ[1] * x234 + r1 * x201;
-      acc48 += static_cast<double>(x577);
-      const TB x578 = s.QC[1] * x234 + s.WQ[1] * x235 + r1 * x202;
-      const TB x643 = s.QC[2] * x234 + s.WQ[2] * x235;
-      const TB x975 = s.QC[1] * x577 + s.WQ[1] * x578 + e1 * (x233 - s.roe * x234) + r1 * x556;
-      acc296 += static_cast<double>(x975);
-      const TB x1006 = s.QC[2] * x577 + s.WQ[2] * x578;
-      acc327 += static_cast<double>(x1006);
-      const TB x579 = s.QC[1] * x236 + s.WQ[1] * x237;
-      acc49 += static_cast<double>(x579);
-      const TB x580 = s.QC[1] * x237 + s.WQ[1] * x238;
-      const TB x1007 = s.QC[2] * x579 + s.WQ[2] * x580 + r1 * x556;
-      acc328 += static_cast<double>(x1007);
-      const TB x976 = s.QC[1] * x579 + s.WQ[1] * x580 + e1 * (x236 - s.roe * x237);
-      acc297 += static_cast<double>(x976);
-      const TB x645 = s.QC[2] * x237 + s.WQ[2] * x238 + r1 * x202;
-      const TB x621 = s.QC[2] * x201 + s.WQ[2] * x202;
-      const TB x644 = s.QC[2] * x236 + s.WQ[2] * x237 + r1 * x201;
-      acc80 += static_cast<double>(x644);
-      const TB x1038 = s.QC[2] * x644 + s.WQ[2] * x645 + e1 * (x236 - s.roe * x237) + r1 * x621;
-      acc359 += static_cast<double>(x1038);
-      const TB x640 = s.QC[2] * x230 + s.WQ[2] * x231;
-      acc78 += static_cast<double>(x640);
-      const TB x1036 = s.QC[2] * x640 + s.WQ[2] * x641 + e1 * (x230 - s.roe * x231);
-      acc357 += static_cast<double>(x1036);
-      const TB x642 = s.QC[2] * x233 + s.WQ[2] * x234;
-      acc79 += static_cast<double>(x642);
-      const TB x1037 = s.QC[2] * x642 + s.WQ[2] * x643 + e1 * (x233 - s.roe * x234);
-      acc358 += static_cast<double>(x1037);
-      const TB x646 = s.QC[2] * x239 + s.WQ[2] * x240;
-      acc81 += static_cast<double>(x646);
-      const TB x1039 = s.QC[2] * x646 + s.WQ[2] * x647 + e1 * (x239 - s.roe * x240);
-      acc360 += static_cast<double>(x1039);
-      const TB x705 = s.QC[1] * x330 + s.WQ[1] * x331 + e1 * (x115 - s.roe * x130);
-      acc109 += static_cast<double>(x705);
-      const TB x835 = s.QC[2] * x430 + s.WQ[2] * x431 + e1 * (x115 - s.roe * x130);
-      acc171 += static_cast<double>(x835);
-      const TB x1160 = s.QC[2] * x835 + s.WQ[2] * x836 + e2 * (x430 - s.roe * x431);
-      acc481 += static_cast<double>(x1160);
-      const TB x1067 = s.QC[1] * x705 + s.WQ[1] * x706 + e2 * (x330 - s.roe * x331);
-      acc388 += static_cast<double>(x1067);
-      const TB x1098 = s.QC[2] * x705 + s.WQ[2] * x706;
-      acc419 += static_cast<double>(x1098);
-      const TB x707 = s.QC[1] * x333 + s.WQ[1] * x334 + e1 * (x116 - s.roe * x131) + r1 * x301;
-      acc110 += static_cast<double>(x707);
-      const TB x837 = s.QC[2] * x433 + s.WQ[2] * x434 + e1 * (x116 - s.roe * x131);
-      acc172 += static_cast<double>(x837);
-      const TB x1161 = s.QC[2] * x837 + s.WQ[2] * x838 + e2 * (x433 - s.roe * x434);
-      acc482 += static_cast<double>(x1161);
-      const TB x1068 = s.QC[1] * x707 + s.WQ[1] * x708 + e2 * (x333 - s.roe * x334) + r1 * x686;
-      acc389 += static_cast<double>(x1068);
-      const TB x1099 = s.QC[2] * x707 + s.WQ[2] * x708;
-      acc420 += static_cast<double>(x1099);
-      const TB x774 = s.QC[2] * x336 + s.WQ[2] * x337 + r1 * x301;
-      acc142 += static_cast<double>(x774);
-      const TB x1131 = s.QC[2] * x774 + s.WQ[2] * x775 + e1 * (x336 - s.roe * x337) + r1 * x751;
-      acc452 += static_cast<double>(x1131);
-      const TB x770 = s.QC[2] * x330 + s.WQ[2] * x331;
-      acc140 += static_cast<double>(x770);
-      const TB x1129 = s.QC[2] * x770 + s.WQ[2] * x771 + e1 * (x330 - s.roe * x331);
-      acc450 += static_cast<double>(x1129);
-      const TB x772 = s.QC[2] * x333 + s.WQ[2] * x334;
-      acc141 += static_cast<double>(x772);
-      const TB x1130 = s.QC[2] * x772 + s.WQ[2] * x773 + e1 * (x333 - s.roe * x334);
-      acc451 += static_cast<double>(x1130);
-      const TB x776 = s.QC[2] * x339 + s.WQ[2] * x340;
-      acc143 += static_cast<double>(x776);
-      const TB x1132 = s.QC[2] * x776 + s.WQ[2] * x777 + e1 * (x339 - s.roe * x340);
-      acc453 += static_cast<double>(x1132);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(263 * R + r) * a.acc_stride + q] = acc263;
-  a.acc[(294 * R + r) * a.acc_stride + q] = acc294;
-  a.acc[(325 * R + r) * a.acc_stride + q] = acc325;
-  a.acc[(356 * R + r) * a.acc_stride + q] = acc356;
-  a.acc[(387 * R + r) * a.acc_stride + q] = acc387;
-  a.acc[(418 * R + r) * a.acc_stride + q] = acc418;
-  a.acc[(449 * R + r) * a.acc_stride + q] = acc449;
-  a.acc[(480 * R + r) * a.acc_stride + q] = acc480;
-  a.acc[(16 * R + r) * a.acc_stride + q] = acc16;
-  a.acc[(47 * R + r) * a.acc_stride + q] = acc47;
-  a.acc[(78 * R + r) * a.acc_stride + q] = acc78;
-  a.acc[(109 * R + r) * a.acc_stride + q] = acc109;
-  a.acc[(140 * R + r) * a.acc_stride + q] = acc140;
-  a.acc[(171 * R + r) * a.acc_stride + q] = acc171;
-  a.acc[(202 * R + r) * a.acc_stride + q] = acc202;
-  a.acc[(233 * R + r) * a.acc_stride + q] = acc233;
-  a.acc[(264 * R + r) * a.acc_stride + q] = acc264;
-  a.acc[(295 * R + r) * a.acc_stride + q] = acc295;
-  a.acc[(326 * R + r) * a.acc_stride + q] = acc326;
-  a.acc[(357 * R + r) * a.acc_stride + q] = acc357;
-  a.acc[(388 * R + r) * a.acc_stride + q] = acc388;
-  a.acc[(419 * R + r) * a.acc_stride + q] = acc419;
-  a.acc[(450 * R + r) * a.acc_stride + q] = acc450;
-  a.acc[(481 * R + r) * a.acc_stride + q] = acc481;
-  a.acc[(17 * R + r) * a.acc_stride + q] = acc17;
-  a.acc[(48 * R + r) * a.acc_stride + q] = acc48;
-  a.acc[(79 * R + r) * a.acc_stride + q] = acc79;
-  a.acc[(110 * R + r) * a.acc_stride + q] = acc110;
-  a.acc[(141 * R + r) * a.acc_stride + q] = acc141;
-  a.acc[(172 * R + r) * a.acc_stride + q] = acc172;
-  a.acc[(203 * R + r) * a.acc_stride + q] = acc203;
-  a.acc[(234 * R + r) * a.acc_stride + q] = acc234;
-  a.acc[(265 * R + r) * a.acc_stride + q] = acc265;
-  a.acc[(296 * R + r) * a.acc_stride + q] = acc296;
-  a.acc[(327 * R + r) * a.acc_stride + q] = acc327;
-  a.acc[(358 * R + r) * a.acc_stride + q] = acc358;
-  a.acc[(389 * R + r) * a.acc_stride + q] = acc389;
-  a.acc[(420 * R + r) * a.acc_stride + q] = acc420;
-  a.acc[(451 * R + r) * a.acc_stride + q] = acc451;
-  a.acc[(482 * R + r) * a.acc_stride + q] = acc482;
-  a.acc[(18 * R + r) * a.acc_stride + q] = acc18;
-  a.acc[(49 * R + r) * a.acc_stride + q] = acc49;
-  a.acc[(80 * R + r) * a.acc_stride + q] = acc80;
-  a.acc[(111 * R + r) * a.acc_stride + q] = acc111;
-  a.acc[(142 * R + r) * a.acc_stride + q] = acc142;
-  a.acc[(173 * R + r) * a.acc_stride + q] = acc173;
-  a.acc[(204 * R + r) * a.acc_stride + q] = acc204;
-  a.acc[(235 * R + r) * a.acc_stride + q] = acc235;
-  a.acc[(266 * R + r) * a.acc_stride + q] = acc266;
-  a.acc[(297 * R + r) * a.acc_stride + q] = acc297;
-  a.acc[(328 * R + r) * a.acc_stride + q] = acc328;
-  a.acc[(359 * R + r) * a.acc_stride + q] = acc359;
-  a.acc[(390 * R + r) * a.acc_stride + q] = acc390;
-  a.acc[(421 * R + r) * a.acc_stride + q] = acc421;
-  a.acc[(452 * R + r) * a.acc_stride + q] = acc452;
-  a.acc[(483 * R + r) * a.acc_stride + q] = acc483;
-  a.acc[(19 * R + r) * a.acc_stride + q] = acc19;
-  a.acc[(50 * R + r) * a.acc_stride + q] = acc50;
-  a.acc[(81 * R + r) * a.acc_stride + q] = acc81;
-  a.acc[(112 * R + r) * a.acc_stride + q] = acc112;
-  a.acc[(143 * R + r) * a.acc_stride + q] = acc143;
-  a.acc[(174 * R + r) * a.acc_stride + q] = acc174;
-  a.acc[(205 * R + r) * a.acc_stride + q] = acc205;
-  a.acc[(236 * R + r) * a.acc_stride + q] = acc236;
-  a.acc[(267 * R + r) * a.acc_stride + q] = acc267;
-  a.acc[(298 * R + r) * a.acc_stride + q] = acc298;
-  a.acc[(329 * R + r) * a.acc_stride + q] = acc329;
-  a.acc[(360 * R + r) * a.acc_stride + q] = acc360;
-  a.acc[(391 * R + r) * a.acc_stride + q] = acc391;
-  a.acc[(422 * R + r) * a.acc_stride + q] = acc422;
-  a.acc[(453 * R + r) * a.acc_stride + q] = acc453;
-  a.acc[(484 * R + r) * a.acc_stride + q] = acc484;
-  a.acc[(20 * R + r) * a.acc_stride + q] = acc20;
-  a.acc[(51 * R + r) * a.acc_stride + q] = acc51;
-  a.acc[(82 * R + r) * a.acc_stride + q] = acc82;
-  a.acc[(113 * R + r) * a.acc_stride + q] = acc113;
-  a.acc[(144 * R + r) * a.acc_stride + q] = acc144;
-  a.acc[(175 * R + r) * a.acc_stride + q] = acc175;
-  a.acc[(206 * R + r) * a.acc_stride + q] = acc206;
-  a.acc[(237 * R + r) * a.acc_stride + q] = acc237;
-  a.acc[(268 * R + r) * a.acc_stride + q] = acc268;
-  a.acc[(299 * R + r) * a.acc_stride + q] = acc299;
-}
-// class (22|21) split 4/6: 361 recursion values, 83 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2221_4(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc330 = 0.0;
-  double acc361 = 0.0;
-  double acc392 = 0.0;
-  double acc423 = 0.0;
-  double acc454 = 0.0;
-  double acc485 = 0.0;
-  double acc21 = 0.0;
-  double acc52 = 0.0;
-  double acc83 = 0.0;
-  double acc114 = 0.0;
-  double acc145 = 0.0;
-  double acc176 = 0.0;
-  double acc207 = 0.0;
-  double acc238 = 0.0;
-  double acc269 = 0.0;
-  double acc300 = 0.0;
-  double acc331 = 0.0;
-  double acc362 = 0.0;
-  double acc393 = 0.0;
-  double acc424 = 0.0;
-  double acc455 = 0.0;
-  double acc486 = 0.0;
-  double acc22 = 0.0;
-  double acc53 = 0.0;
-  double acc84 = 0.0;
-  double acc115 = 0.0;
-  double acc146 = 0.0;
-  double acc177 = 0.0;
-  double acc208 = 0.0;
-  double acc239 = 0.0;
-  double acc270 = 0.0;
-  double acc301 = 0.0;
-  double acc332 = 0.0;
-  double acc363 = 0.0;
-  double acc394 = 0.0;
-  double acc425 = 0.0;
-  double acc456 = 0.0;
-  double acc487 = 0.0;
-  double acc23 = 0.0;
-  double acc54 = 0.0;
-  double acc85 = 0.0;
-  double acc116 = 0.0;
-  double acc147 = 0.0;
-  double acc178 = 0.0;
-  double acc209 = 0.0;
-  double acc240 = 0.0;
-  double acc271 = 0.0;
-  double acc302 = 0.0;
-  double acc333 = 0.0;
-  double acc364 = 0.0;
-  double acc395 = 0.0;
-  double acc426 = 0.0;
-  double acc457 = 0.0;
-  double acc488 = 0.0;
-  double acc24 = 0.0;
-  double acc55 = 0.0;
-  double acc86 = 0.0;
-  double acc117 = 0.0;
-  double acc148 = 0.0;
-  double acc179 = 0.0;
-  double acc210 = 0.0;
-  double acc241 = 0.0;
-  double acc272 = 0.0;
-  double acc303 = 0.0;
-  double acc334 = 0.0;
-  double acc365 = 0.0;
-  double acc396 = 0.0;
-  double acc427 = 0.0;
-  double acc458 = 0.0;
-  double acc489 = 0.0;
-  double acc25 = 0.0;
-  double acc56 = 0.0;
-  double acc87 = 0.0;
-  double acc118 = 0.0;
-  double acc149 = 0.0;
-  double acc180 = 0.0;
-  double acc211 = 0.0;
-  double acc242 = 0.0;
-  double acc273 = 0.0;
-  double acc304 = 0.0;
-  double acc335 = 0.0;
-  double acc366 = 0.0;
-  double acc397 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[8];
-      boys_eval_fixed<7>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x26 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x8 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x11 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x29 = s.PA[0] * x8 + s.WP[0] * x11 + z1 * (x0 - s.roz * x1);
-      const TB x12 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x13 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x14 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x15 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x16 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x17 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x18 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x19 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x20 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x21 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x22 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x23 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x24 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x25 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x58 = s.PA[2] * x22 + s.WP[2] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x59 = s.PA[0] * x23 + s.WP[0] * x26 + z1 * (x5 - s.roz * x6);
-      const TB x30 = s.PA[1] * x8 + s.WP[1] * x11;
-      const TB x31 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x35 = s.PA[0] * x11 + s.WP[0] * x14 + z1 * (x1 - s.roz * x2);
-      const TB x36 = s.PA[1] * x11 + s.WP[1] * x14;
-      const TB x37 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x38 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x40 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x39 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x41 = s.PA[0] * x14 + s.WP[0] * x17 + z1 * (x2 - s.roz * x3);
-      const TB x42 = s.PA[1] * x14 + s.WP[1] * x17;
-      const TB x43 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x44 = s.PA[1] * x15 + s.WP[1] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x46 = s.PA[2] * x16 + s.WP[2] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x84 = s.PA[2] * x40 + s.WP[2] * x46 + z2 * (x13 - s.roz * x16);
-      const TB x45 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x83 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x12 - s.roz * x15);
-      const TB x81 = s.PA[1] * x38 + s.WP[1] * x44 + z2 * (x12 - s.roz * x15);
-      const TB x82 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x47 = s.PA[0] * x17 + s.WP[0] * x20 + z1 * (x3 - s.roz * x4);
-      const TB x48 = s.PA[1] * x17 + s.WP[1] * x20;
-      const TB x49 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x50 = s.PA[1] * x18 + s.WP[1] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[2] * x19 + s.WP[2] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x104 = s.PA[2] * x52 + s.WP[2] * x58 + z2 * (x19 - s.roz * x22);
-      const TB x94 = s.PA[2] * x46 + s.WP[2] * x52 + z2 * (x16 - s.roz * x19);
-      const TB x190 = s.QC[0] * x43 + s.WQ[0] * x49 + r1 * x19;
-      const TB x51 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x53 = s.PA[0] * x20 + s.WP[0] * x23 + z1 * (x4 - s.roz * x5);
-      const TB x56 = s.PA[1] * x21 + s.WP[1] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x57 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x103 = s.PA[2] * x51 + s.WP[2] * x57 + z1 * (x18 - s.roz * x21);
-      const TB x101 = s.PA[1] * x50 + s.WP[1] * x56 + z2 * (x18 - s.roz * x21);
-      const TB x102 = s.PA[2] * x50 + s.WP[2] * x56;
-      const TB x54 = s.PA[1] * x20 + s.WP[1] * x23;
-      const TB x55 = s.PA[2] * x20 + s.WP[2] * x23;
-      const TB x60 = s.PA[1] * x23 + s.WP[1] * x26;
-      const TB x61 = s.PA[2] * x23 + s.WP[2] * x26;
-      const TB x110 = s.PA[2] * x55 + s.WP[2] * x61 + z1 * (x20 - s.roz * x23);
-      const TB x108 = s.PA[1] * x54 + s.WP[1] * x60 + z1 * (x20 - s.roz * x23);
-      const TB x109 = s.PA[2] * x54 + s.WP[2] * x60;
-      const TB x66 = s.PA[1] * x29 + s.WP[1] * x35;
-      const TB x67 = s.PA[2] * x29 + s.WP[2] * x35;
-      const TB x68 = s.PA[1] * x30 + s.WP[1] * x36 + z1 * (x8 - s.roz * x11);
-      const TB x70 = s.PA[2] * x31 + s.WP[2] * x37 + z1 * (x8 - s.roz * x11);
-      const TB x69 = s.PA[2] * x30 + s.WP[2] * x36;
-      const TB x76 = s.PA[1] * x35 + s.WP[1] * x41;
-      const TB x119 = s.PA[2] * x66 + s.WP[2] * x76;
-      const TB x77 = s.PA[2] * x35 + s.WP[2] * x41;
-      const TB x120 = s.PA[2] * x67 + s.WP[2] * x77 + z1 * (x29 - s.roz * x35);
-      const TB x78 = s.PA[1] * x36 + s.WP[1] * x42 + z1 * (x11 - s.roz * x14);
-      const TB x80 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x11 - s.roz * x14);
-      const TB x124 = s.PA[2] * x70 + s.WP[2] * x80 + z2 * (x31 - s.roz * x37);
-      const TB x79 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x123 = s.PA[2] * x69 + s.WP[2] * x79 + z1 * (x30 - s.roz * x36);
-      const TB x121 = s.PA[1] * x68 + s.WP[1] * x78 + z2 * (x30 - s.roz * x36);
-      const TB x122 = s.PA[2] * x68 + s.WP[2] * x78;
-      const TB x86 = s.PA[1] * x41 + s.WP[1] * x47;
-      const TB x87 = s.PA[2] * x41 + s.WP[2] * x47;
-      const TB x135 = s.PA[2] * x77 + s.WP[2] * x87 + z1 * (x35 - s.roz * x41);
-      const TB x88 = s.PA[1] * x42 + s.WP[1] * x48 + z1 * (x14 - s.roz * x17);
-      const TB x90 = s.PA[2] * x43 + s.WP[2] * x49 + z1 * (x14 - s.roz * x17);
-      const TB x139 = s.PA[2] * x80 + s.WP[2] * x90 + z2 * (x37 - s.roz * x43);
-      const TB x89 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x91 = s.PA[1] * x44 + s.WP[1] * x50 + z2 * (x15 - s.roz * x18);
-      const TB x93 = s.PA[2] * x45 + s.WP[2] * x51 + z1 * (x15 - s.roz * x18);
-      const TB x187 = s.QC[0] * x42 + s.WQ[0] * x48 + r1 * x18;
-      const TB x92 = s.PA[2] * x44 + s.WP[2] * x50;
-      const TB x96 = s.PA[1] * x47 + s.WP[1] * x53;
-      const TB x97 = s.PA[2] * x47 + s.WP[2] * x53;
-      const TB x98 = s.PA[1] * x48 + s.WP[1] * x54 + z1 * (x17 - s.roz * x20);
-      const TB x100 = s.PA[2] * x49 + s.WP[2] * x55 + z1 * (x17 - s.roz * x20);
-      const TB x169 = s.PA[2] * x100 + s.WP[2] * x110 + z2 * (x49 - s.roz * x55);
-      const TB x99 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x168 = s.PA[2] * x99 + s.WP[2] * x109 + z1 * (x48 - s.roz * x54);
-      const TB x166 = s.PA[1] * x98 + s.WP[1] * x108 + z2 * (x48 - s.roz * x54);
-      const TB x167 = s.PA[2] * x98 + s.WP[2] * x108;
-      const TB x106 = s.PA[1] * x53 + s.WP[1] * x59;
-      const TB x107 = s.PA[2] * x53 + s.WP[2] * x59;
-      const TB x165 = s.PA[2] * x97 + s.WP[2] * x107 + z1 * (x47 - s.roz * x53);
-      const TB x164 = s.PA[2] * x96 + s.WP[2] * x106;
-      const TB x134 = s.PA[2] * x76 + s.WP[2] * x86;
-      const TB x136 = s.PA[1] * x78 + s.WP[1] * x88 + z2 * (x36 - s.roz * x42);
-      const TB x138 = s.PA[2] * x79 + s.WP[2] * x89 + z1 * (x36 - s.roz * x42);
-      const TB x137 = s.PA[2] * x78 + s.WP[2] * x88;
-      const TB x149 = s.PA[2] * x86 + s.WP[2] * x96;
-      const TB x150 = s.PA[2] * x87 + s.WP[2] * x97 + z1 * (x41 - s.roz * x47);
-      const TB x151 = s.PA[1] * x88 + s.WP[1] * x98 + z2 * (x42 - s.roz * x48);
-      const TB x152 = s.PA[2] * x88 + s.WP[2] * x98;
-      const TB x153 = s.PA[2] * x89 + s.WP[2] * x99 + z1 * (x42 - s.roz * x48);
-      const TB x154 = s.PA[2] * x90 + s.WP[2] * x100 + z2 * (x43 - s.roz * x49);
-      const TB x184 = s.QC[0] * x41 + s.WQ[0] * x47 + r2 * x17;
-      const TB x287 = s.QC[1] * x42 + s.WQ[1] * x48 + r1 * x17;
-      const TB x193 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x210 = s.QC[0] * x78 + s.WQ[0] * x88 + r1 * x44;
-      const TB x211 = s.QC[0] * x88 + s.WQ[0] * x98 + r1 * x50;
-      const TB x497 = s.QC[0] * x210 + s.WQ[0] * x211 + e1 * (x78 - s.roe * x88) + r1 * x193;
-      const TB x196 = s.QC[0] * x45 + s.WQ[0] * x51;
-      const TB x213 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x45;
-      const TB x214 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x51;
-      const TB x499 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x79 - s.roe * x89) + r1 * x196;
-      const TB x199 = s.QC[0] * x46 + s.WQ[0] * x52;
-      const TB x216 = s.QC[0] * x80 + s.WQ[0] * x90 + r1 * x46;
-      const TB x217 = s.QC[0] * x90 + s.WQ[0] * x100 + r1 * x52;
-      const TB x501 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x80 - s.roe * x90) + r1 * x199;
-      const TB x204 = s.QC[0] * x76 + s.WQ[0] * x86 + r2 * x42;
-      const TB x205 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x48;
-      const TB x207 = s.QC[0] * x77 + s.WQ[0] * x87 + r2 * x43;
-      const TB x208 = s.QC[0] * x87 + s.WQ[0] * x97 + r2 * x49;
-      const TB x219 = s.QC[0] * x81 + s.WQ[0] * x91;
-      const TB x220 = s.QC[0] * x91 + s.WQ[0] * x101;
-      const TB x250 = s.QC[0] * x151 + s.WQ[0] * x166 + r1 * x101;
-      const TB x222 = s.QC[0] * x82 + s.WQ[0] * x92;
-      const TB x223 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x253 = s.QC[0] * x152 + s.WQ[0] * x167 + r1 * x102;
-      const TB x225 = s.QC[0] * x83 + s.WQ[0] * x93;
-      const TB x226 = s.QC[0] * x93 + s.WQ[0] * x103;
-      const TB x256 = s.QC[0] * x153 + s.WQ[0] * x168 + r1 * x103;
-      const TB x228 = s.QC[0] * x84 + s.WQ[0] * x94;
-      const TB x229 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x259 = s.QC[0] * x154 + s.WQ[0] * x169 + r1 * x104;
-      const TB x359 = s.QC[1] * x154 + s.WQ[1] * x169;
-      const TB x242 = s.QC[0] * x119 + s.WQ[0] * x134 + r2 * x79;
-      const TB x243 = s.QC[0] * x134 + s.WQ[0] * x149 + r2 * x89;
-      const TB x244 = s.QC[0] * x149 + s.WQ[0] * x164 + r2 * x99;
-      const TB x245 = s.QC[0] * x120 + s.WQ[0] * x135 + r2 * x80;
-      const TB x246 = s.QC[0] * x135 + s.WQ[0] * x150 + r2 * x90;
-      const TB x247 = s.QC[0] * x150 + s.WQ[0] * x165 + r2 * x100;
-      const TB x248 = s.QC[0] * x121 + s.WQ[0] * x136 + r1 * x81;
-      const TB x503 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x81 - s.roe * x91);
-      const TB x249 = s.QC[0] * x136 + s.WQ[0] * x151 + r1 * x91;
-      const TB x522 = s.QC[0] * x248 + s.WQ[0] * x249 + e1 * (x121 - s.roe * x136) + r1 * x219;
-      acc22 += static_cast<double>(x522);
-      const TB x523 = s.QC[0] * x249 + s.WQ[0] * x250 + e1 * (x136 - s.roe * x151) + r1 * x220;
-      const TB x887 = s.QC[0] * x522 + s.WQ[0] * x523 + e2 * (x248 - s.roe * x249) + r1 * x503;
-      acc208 += static_cast<double>(x887);
-      const TB x918 = s.QC[1] * x522 + s.WQ[1] * x523 + r3 * x497;
-      acc239 += static_cast<double>(x918);
-      const TB x949 = s.QC[2] * x522 + s.WQ[2] * x523;
-      acc270 += static_cast<double>(x949);
-      const TB x251 = s.QC[0] * x122 + s.WQ[0] * x137 + r1 * x82;
-      const TB x505 = s.QC[0] * x222 + s.WQ[0] * x223 + e1 * (x82 - s.roe * x92);
-      const TB x252 = s.QC[0] * x137 + s.WQ[0] * x152 + r1 * x92;
-      const TB x524 = s.QC[0] * x251 + s.WQ[0] * x252 + e1 * (x122 - s.roe * x137) + r1 * x222;
-      acc23 += static_cast<double>(x524);
-      const TB x525 = s.QC[0] * x252 + s.WQ[0] * x253 + e1 * (x137 - s.roe * x152) + r1 * x223;
-      const TB x888 = s.QC[0] * x524 + s.WQ[0] * x525 + e2 * (x251 - s.roe * x252) + r1 * x505;
-      acc209 += static_cast<double>(x888);
-      const TB x950 = s.QC[2] * x524 + s.WQ[2] * x525 + r1 * x497;
-      acc271 += static_cast<double>(x950);
-      const TB x919 = s.QC[1] * x524 + s.WQ[1] * x525 + r2 * x499;
-      acc240 += static_cast<double>(x919);
-      const TB x254 = s.QC[0] * x123 + s.WQ[0] * x138 + r1 * x83;
-      const TB x507 = s.QC[0] * x225 + s.WQ[0] * x226 + e1 * (x83 - s.roe * x93);
-      const TB x255 = s.QC[0] * x138 + s.WQ[0] * x153 + r1 * x93;
-      const TB x526 = s.QC[0] * x254 + s.WQ[0] * x255 + e1 * (x123 - s.roe * x138) + r1 * x225;
-      acc24 += static_cast<double>(x526);
-      const TB x527 = s.QC[0] * x255 + s.WQ[0] * x256 + e1 * (x138 - s.roe * x153) + r1 * x226;
-      const TB x889 = s.QC[0] * x526 + s.WQ[0] * x527 + e2 * (x254 - s.roe * x255) + r1 * x507;
-      acc210 += static_cast<double>(x889);
-      const TB x951 = s.QC[2] * x526 + s.WQ[2] * x527 + r2 * x499;
-      acc272 += static_cast<double>(x951);
-      const TB x920 = s.QC[1] * x526 + s.WQ[1] * x527 + r1 * x501;
-      acc241 += static_cast<double>(x920);
-      const TB x257 = s.QC[0] * x124 + s.WQ[0] * x139 + r1 * x84;
-      const TB x509 = s.QC[0] * x228 + s.WQ[0] * x229 + e1 * (x84 - s.roe * x94);
-      const TB x258 = s.QC[0] * x139 + s.WQ[0] * x154 + r1 * x94;
-      const TB x528 = s.QC[0] * x257 + s.WQ[0] * x258 + e1 * (x124 - s.roe * x139) + r1 * x228;
-      acc25 += static_cast<double>(x528);
-      const TB x529 = s.QC[0] * x258 + s.WQ[0] * x259 + e1 * (x139 - s.roe * x154) + r1 * x229;
-      const TB x890 = s.QC[0] * x528 + s.WQ[0] * x529 + e2 * (x257 - s.roe * x258) + r1 * x509;
-      acc211 += static_cast<double>(x890);
-      const TB x921 = s.QC[1] * x528 + s.WQ[1] * x529;
-      acc242 += static_cast<double>(x921);
-      const TB x952 = s.QC[2] * x528 + s.WQ[2] * x529 + r3 * x501;
-      acc273 += static_cast<double>(x952);
-      const TB x284 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x290 = s.QC[1] * x43 + s.WQ[1] * x49;
-      const TB x314 = s.QC[1] * x89 + s.WQ[1] * x99 + r1 * x49;
-      const TB x304 = s.QC[1] * x76 + s.WQ[1] * x86 + r1 * x41;
-      const TB x305 = s.QC[1] * x86 + s.WQ[1] * x96 + r1 * x47;
-      const TB x307 = s.QC[1] * x77 + s.WQ[1] * x87;
-      const TB x308 = s.QC[1] * x87 + s.WQ[1] * x97;
-      const TB x310 = s.QC[1] * x78 + s.WQ[1] * x88 + r2 * x42;
-      const TB x311 = s.QC[1] * x88 + s.WQ[1] * x98 + r2 * x48;
-      const TB x313 = s.QC[1] * x79 + s.WQ[1] * x89 + r1 * x43;
-      const TB x416 = s.QC[2] * x80 + s.WQ[2] * x90 + r2 * x43;
-      const TB x694 = s.QC[1] * x313 + s.WQ[1] * x314 + e1 * (x79 - s.roe * x89) + r1 * x290;
-      const TB x316 = s.QC[1] * x80 + s.WQ[1] * x90;
-      const TB x317 = s.QC[1] * x90 + s.WQ[1] * x100;
-      const TB x356 = s.QC[1] * x153 + s.WQ[1] * x168 + r1 * x100;
-      const TB x456 = s.QC[2] * x153 + s.WQ[2] * x168 + r2 * x99;
-      const TB x342 = s.QC[1] * x119 + s.WQ[1] * x134 + r1 * x77;
-      const TB x343 = s.QC[1] * x134 + s.WQ[1] * x149 + r1 * x87;
-      const TB x344 = s.QC[1] * x149 + s.WQ[1] * x164 + r1 * x97;
-      const TB x444 = s.QC[2] * x149 + s.WQ[2] * x164 + r1 * x96;
-      const TB x405 = s.QC[2] * x86 + s.WQ[2] * x96;
-      const TB x345 = s.QC[1] * x120 + s.WQ[1] * x135;
-      const TB x346 = s.QC[1] * x135 + s.WQ[1] * x150;
-      const TB x347 = s.QC[1] * x150 + s.WQ[1] * x165;
-      const TB x447 = s.QC[2] * x150 + s.WQ[2] * x165 + r2 * x97;
-      const TB x408 = s.QC[2] * x87 + s.WQ[2] * x97 + r1 * x47;
-      const TB x384 = s.QC[2] * x41 + s.WQ[2] * x47;
-      const TB x407 = s.QC[2] * x77 + s.WQ[2] * x87 + r1 * x41;
-      const TB x820 = s.QC[2] * x407 + s.WQ[2] * x408 + e1 * (x77 - s.roe * x87) + r1 * x384;
-      const TB x348 = s.QC[1] * x121 + s.WQ[1] * x136 + r3 * x78;
-      const TB x349 = s.QC[1] * x136 + s.WQ[1] * x151 + r3 * x88;
-      const TB x350 = s.QC[1] * x151 + s.WQ[1] * x166 + r3 * x98;
-      const TB x450 = s.QC[2] * x151 + s.WQ[2] * x166;
-      const TB x351 = s.QC[1] * x122 + s.WQ[1] * x137 + r2 * x79;
-      const TB x352 = s.QC[1] * x137 + s.WQ[1] * x152 + r2 * x89;
-      const TB x353 = s.QC[1] * x152 + s.WQ[1] * x167 + r2 * x99;
-      const TB x414 = s.QC[2] * x89 + s.WQ[2] * x99 + r1 * x48;
-      const TB x387 = s.QC[2] * x42 + s.WQ[2] * x48;
-      const TB x413 = s.QC[2] * x79 + s.WQ[2] * x89 + r1 * x42;
-      const TB x453 = s.QC[2] * x152 + s.WQ[2] * x167 + r1 * x98;
-      const TB x411 = s.QC[2] * x88 + s.WQ[2] * x98;
-      const TB x824 = s.QC[2] * x413 + s.WQ[2] * x414 + e1 * (x79 - s.roe * x89) + r1 * x387;
-      const TB x454 = s.QC[2] * x123 + s.WQ[2] * x138 + r2 * x79;
-      const TB x455 = s.QC[2] * x138 + s.WQ[2] * x153 + r2 * x89;
-      const TB x852 = s.QC[2] * x455 + s.WQ[2] * x456 + e1 * (x138 - s.roe * x153) + r2 * x414;
-      const TB x851 = s.QC[2] * x454 + s.WQ[2] * x455 + e1 * (x123 - s.roe * x138) + r2 * x413;
-      acc179 += static_cast<double>(x851);
-      const TB x1168 = s.QC[2] * x851 + s.WQ[2] * x852 + e2 * (x454 - s.roe * x455) + r2 * x824;
-      acc489 += static_cast<double>(x1168);
-      const TB x354 = s.QC[1] * x123 + s.WQ[1] * x138 + r1 * x80;
-      const TB x355 = s.QC[1] * x138 + s.WQ[1] * x153 + r1 * x90;
-      const TB x721 = s.QC[1] * x354 + s.WQ[1] * x355 + e1 * (x123 - s.roe * x138) + r1 * x316;
-      acc117 += static_cast<double>(x721);
-      const TB x722 = s.QC[1] * x355 + s.WQ[1] * x356 + e1 * (x138 - s.roe * x153) + r1 * x317;
-      const TB x696 = s.QC[1] * x316 + s.WQ[1] * x317 + e1 * (x80 - s.roe * x90);
-      const TB x1075 = s.QC[1] * x721 + s.WQ[1] * x722 + e2 * (x354 - s.roe * x355) + r1 * x696;
-      acc396 += static_cast<double>(x1075);
-      const TB x1106 = s.QC[2] * x721 + s.WQ[2] * x722 + r2 * x694;
-      acc427 += static_cast<double>(x1106);
-      const TB x457 = s.QC[2] * x124 + s.WQ[2] * x139 + r3 * x80;
-      const TB x458 = s.QC[2] * x139 + s.WQ[2] * x154 + r3 * x90;
-      const TB x853 = s.QC[2] * x457 + s.WQ[2] * x458 + e1 * (x124 - s.roe * x139) + r3 * x416;
-      acc180 += static_cast<double>(x853);
-      const TB x787 = s.QC[2] * x355 + s.WQ[2] * x356 + r2 * x314;
-      const TB x357 = s.QC[1] * x124 + s.WQ[1] * x139;
-      const TB x358 = s.QC[1] * x139 + s.WQ[1] * x154;
-      const TB x724 = s.QC[1] * x358 + s.WQ[1] * x359 + e1 * (x139 - s.roe * x154);
-      const TB x723 = s.QC[1] * x357 + s.WQ[1] * x358 + e1 * (x124 - s.roe * x139);
-      acc118 += static_cast<double>(x723);
-      const TB x1076 = s.QC[1] * x723 + s.WQ[1] * x724 + e2 * (x357 - s.roe * x358);
-      acc397 += static_cast<double>(x1076);
-      const TB x788 = s.QC[2] * x357 + s.WQ[2] * x358 + r3 * x316;
-      acc149 += static_cast<double>(x788);
-      const TB x404 = s.QC[2] * x76 + s.WQ[2] * x86;
-      const TB x410 = s.QC[2] * x78 + s.WQ[2] * x88;
-      const TB x442 = s.QC[2] * x119 + s.WQ[2] * x134 + r1 * x76;
-      const TB x443 = s.QC[2] * x134 + s.WQ[2] * x149 + r1 * x86;
-      const TB x844 = s.QC[2] * x443 + s.WQ[2] * x444 + e1 * (x134 - s.roe * x149) + r1 * x405;
-      const TB x714 = s.QC[1] * x343 + s.WQ[1] * x344 + e1 * (x134 - s.roe * x149) + r1 * x308;
-      const TB x779 = s.QC[2] * x343 + s.WQ[2] * x344 + r1 * x305;
-      const TB x818 = s.QC[2] * x404 + s.WQ[2] * x405 + e1 * (x76 - s.roe * x86);
-      const TB x688 = s.QC[1] * x304 + s.WQ[1] * x305 + e1 * (x76 - s.roe * x86) + r1 * x284;
-      const TB x753 = s.QC[2] * x304 + s.WQ[2] * x305;
-      const TB x755 = s.QC[2] * x307 + s.WQ[2] * x308 + r1 * x284;
-      const TB x778 = s.QC[2] * x342 + s.WQ[2] * x343 + r1 * x304;
-      const TB x1133 = s.QC[2] * x778 + s.WQ[2] * x779 + e1 * (x342 - s.roe * x343) + r1 * x753;
-      acc454 += static_cast<double>(x1133);
-      const TB x843 = s.QC[2] * x442 + s.WQ[2] * x443 + e1 * (x119 - s.roe * x134) + r1 * x404;
-      const TB x1164 = s.QC[2] * x843 + s.WQ[2] * x844 + e2 * (x442 - s.roe * x443) + r1 * x818;
-      acc485 += static_cast<double>(x1164);
-      const TB x713 = s.QC[1] * x342 + s.WQ[1] * x343 + e1 * (x119 - s.roe * x134) + r1 * x307;
-      const TB x1102 = s.QC[2] * x713 + s.WQ[2] * x714 + r1 * x688;
-      acc423 += static_cast<double>(x1102);
-      const TB x445 = s.QC[2] * x120 + s.WQ[2] * x135 + r2 * x77;
-      const TB x446 = s.QC[2] * x135 + s.WQ[2] * x150 + r2 * x87;
-      const TB x846 = s.QC[2] * x446 + s.WQ[2] * x447 + e1 * (x135 - s.roe * x150) + r2 * x408;
-      const TB x845 = s.QC[2] * x445 + s.WQ[2] * x446 + e1 * (x120 - s.roe * x135) + r2 * x407;
-      acc176 += static_cast<double>(x845);
-      const TB x1165 = s.QC[2] * x845 + s.WQ[2] * x846 + e2 * (x445 - s.roe * x446) + r2 * x820;
-      acc486 += static_cast<double>(x1165);
-      const TB x448 = s.QC[2] * x121 + s.WQ[2] * x136;
-      const TB x449 = s.QC[2] * x136 + s.WQ[2] * x151;
-      const TB x848 = s.QC[2] * x449 + s.WQ[2] * x450 + e1 * (x136 - s.roe * x151);
-      const TB x718 = s.QC[1] * x349 + s.WQ[1] * x350 + e1 * (x136 - s.roe * x151) + r3 * x311;
-      const TB x783 = s.QC[2] * x349 + s.WQ[2] * x350;
-      const TB x451 = s.QC[2] * x122 + s.WQ[2] * x137 + r1 * x78;
-      const TB x452 = s.QC[2] * x137 + s.WQ[2] * x152 + r1 * x88;
-      const TB x850 = s.QC[2] * x452 + s.WQ[2] * x453 + e1 * (x137 - s.roe * x152) + r1 * x411;
-      const TB x720 = s.QC[1] * x352 + s.WQ[1] * x353 + e1 * (x137 - s.roe * x152) + r2 * x314;
-      const TB x759 = s.QC[2] * x313 + s.WQ[2] * x314 + r1 * x287;
-      const TB x692 = s.QC[1] * x310 + s.WQ[1] * x311 + e1 * (x78 - s.roe * x88) + r2 * x287;
-      const TB x822 = s.QC[2] * x410 + s.WQ[2] * x411 + e1 * (x78 - s.roe * x88);
-      const TB x785 = s.QC[2] * x352 + s.WQ[2] * x353 + r1 * x311;
-      const TB x757 = s.QC[2] * x310 + s.WQ[2] * x311;
-      const TB x849 = s.QC[2] * x451 + s.WQ[2] * x452 + e1 * (x122 - s.roe * x137) + r1 * x410;
-      acc178 += static_cast<double>(x849);
-      const TB x1167 = s.QC[2] * x849 + s.WQ[2] * x850 + e2 * (x451 - s.roe * x452) + r1 * x822;
-      acc488 += static_cast<double>(x1167);
-      const TB x719 = s.QC[1] * x351 + s.WQ[1] * x352 + e1 * (x122 - s.roe * x137) + r2 * x313;
-      acc116 += static_cast<double>(x719);
-      const TB x1074 = s.QC[1] * x719 + s.WQ[1] * x720 + e2 * (x351 - s.roe * x352) + r2 * x694;
-      acc395 += static_cast<double>(x1074);
-      const TB x1105 = s.QC[2] * x719 + s.WQ[2] * x720 + r1 * x692;
-      acc426 += static_cast<double>(x1105);
-      const TB x786 = s.QC[2] * x354 + s.WQ[2] * x355 + r2 * x313;
-      acc148 += static_cast<double>(x786);
-      const TB x1137 = s.QC[2] * x786 + s.WQ[2] * x787 + e1 * (x354 - s.roe * x355) + r2 * x759;
-      acc458 += static_cast<double>(x1137);
-      const TB x495 = s.QC[0] * x207 + s.WQ[0] * x208 + e1 * (x77 - s.roe * x87) + r2 * x190;
-      const TB x690 = s.QC[1] * x307 + s.WQ[1] * x308 + e1 * (x77 - s.roe * x87);
-      const TB x1071 = s.QC[1] * x713 + s.WQ[1] * x714 + e2 * (x342 - s.roe * x343) + r1 * x690;
-      acc392 += static_cast<double>(x1071);
-      const TB x780 = s.QC[2] * x345 + s.WQ[2] * x346 + r2 * x307;
-      acc145 += static_cast<double>(x780);
-      const TB x781 = s.QC[2] * x346 + s.WQ[2] * x347 + r2 * x308;
-      const TB x1134 = s.QC[2] * x780 + s.WQ[2] * x781 + e1 * (x345 - s.roe * x346) + r2 * x755;
-      acc455 += static_cast<double>(x1134);
-      const TB x716 = s.QC[1] * x346 + s.WQ[1] * x347 + e1 * (x135 - s.roe * x150);
-      const TB x521 = s.QC[0] * x246 + s.WQ[0] * x247 + e1 * (x135 - s.roe * x150) + r2 * x217;
-      const TB x520 = s.QC[0] * x245 + s.WQ[0] * x246 + e1 * (x120 - s.roe * x135) + r2 * x216;
-      acc21 += static_cast<double>(x520);
-      const TB x715 = s.QC[1] * x345 + s.WQ[1] * x346 + e1 * (x120 - s.roe * x135);
-      acc114 += static_cast<double>(x715);
-      const TB x1072 = s.QC[1] * x715 + s.WQ[1] * x716 + e2 * (x345 - s.roe * x346);
-      acc393 += static_cast<double>(x1072);
-      const TB x1103 = s.QC[2] * x715 + s.WQ[2] * x716 + r2 * x690;
-      acc424 += static_cast<double>(x1103);
-      const TB x886 = s.QC[0] * x520 + s.WQ[0] * x521 + e2 * (x245 - s.roe * x246) + r2 * x501;
-      acc207 += static_cast<double>(x886);
-      const TB x948 = s.QC[2] * x520 + s.WQ[2] * x521 + r2 * x495;
-      acc269 += static_cast<double>(x948);
-      const TB x917 = s.QC[1] * x520 + s.WQ[1] * x521;
-      acc238 += static_cast<double>(x917);
-      const TB x558 = s.QC[1] * x204 + s.WQ[1] * x205 + r1 * x184;
-      const TB x625 = s.QC[2] * x207 + s.WQ[2] * x208 + r1 * x184;
-      const TB x560 = s.QC[1] * x207 + s.WQ[1] * x208;
-      const TB x562 = s.QC[1] * x210 + s.WQ[1] * x211 + r2 * x187;
-      const TB x629 = s.QC[2] * x213 + s.WQ[2] * x214 + r1 * x187;
-      const TB x564 = s.QC[1] * x213 + s.WQ[1] * x214 + r1 * x190;
-      const TB x631 = s.QC[2] * x216 + s.WQ[2] * x217 + r2 * x190;
-      const TB x566 = s.QC[1] * x216 + s.WQ[1] * x217;
-      const TB x583 = s.QC[1] * x242 + s.WQ[1] * x243 + r1 * x207;
-      const TB x650 = s.QC[2] * x245 + s.WQ[2] * x246 + r2 * x207;
-      acc83 += static_cast<double>(x650);
-      const TB x584 = s.QC[1] * x243 + s.WQ[1] * x244 + r1 * x208;
-      const TB x1009 = s.QC[2] * x583 + s.WQ[2] * x584 + r1 * x558;
-      acc330 += static_cast<double>(x1009);
-      const TB x649 = s.QC[2] * x243 + s.WQ[2] * x244 + r1 * x205;
-      const TB x623 = s.QC[2] * x204 + s.WQ[2] * x205;
-      const TB x648 = s.QC[2] * x242 + s.WQ[2] * x243 + r1 * x204;
-      const TB x1040 = s.QC[2] * x648 + s.WQ[2] * x649 + e1 * (x242 - s.roe * x243) + r1 * x623;
-      acc361 += static_cast<double>(x1040);
-      const TB x651 = s.QC[2] * x246 + s.WQ[2] * x247 + r2 * x208;
-      const TB x1041 = s.QC[2] * x650 + s.WQ[2] * x651 + e1 * (x245 - s.roe * x246) + r2 * x625;
-      acc362 += static_cast<double>(x1041);
-      const TB x586 = s.QC[1] * x246 + s.WQ[1] * x247;
-      const TB x585 = s.QC[1] * x245 + s.WQ[1] * x246;
-      acc52 += static_cast<double>(x585);
-      const TB x979 = s.QC[1] * x585 + s.WQ[1] * x586 + e1 * (x245 - s.roe * x246);
-      acc300 += static_cast<double>(x979);
-      const TB x1010 = s.QC[2] * x585 + s.WQ[2] * x586 + r2 * x560;
-      acc331 += static_cast<double>(x1010);
-      const TB x587 = s.QC[1] * x248 + s.WQ[1] * x249 + r3 * x210;
-      acc53 += static_cast<double>(x587);
-      const TB x588 = s.QC[1] * x249 + s.WQ[1] * x250 + r3 * x211;
-      const TB x653 = s.QC[2] * x249 + s.WQ[2] * x250;
-      const TB x980 = s.QC[1] * x587 + s.WQ[1] * x588 + e1 * (x248 - s.roe * x249) + r3 * x562;
-      acc301 += static_cast<double>(x980);
-      const TB x1011 = s.QC[2] * x587 + s.WQ[2] * x588;
-      acc332 += static_cast<double>(x1011);
-      const TB x589 = s.QC[1] * x251 + s.WQ[1] * x252 + r2 * x213;
-      acc54 += static_cast<double>(x589);
-      const TB x656 = s.QC[2] * x254 + s.WQ[2] * x255 + r2 * x213;
-      acc86 += static_cast<double>(x656);
-      const TB x590 = s.QC[1] * x252 + s.WQ[1] * x253 + r2 * x214;
-      const TB x1012 = s.QC[2] * x589 + s.WQ[2] * x590 + r1 * x562;
-      acc333 += static_cast<double>(x1012);
-      const TB x981 = s.QC[1] * x589 + s.WQ[1] * x590 + e1 * (x251 - s.roe * x252) + r2 * x564;
-      acc302 += static_cast<double>(x981);
-      const TB x655 = s.QC[2] * x252 + s.WQ[2] * x253 + r1 * x211;
-      const TB x627 = s.QC[2] * x210 + s.WQ[2] * x211;
-      const TB x654 = s.QC[2] * x251 + s.WQ[2] * x252 + r1 * x210;
-      acc85 += static_cast<double>(x654);
-      const TB x1043 = s.QC[2] * x654 + s.WQ[2] * x655 + e1 * (x251 - s.roe * x252) + r1 * x627;
-      acc364 += static_cast<double>(x1043);
-      const TB x657 = s.QC[2] * x255 + s.WQ[2] * x256 + r2 * x214;
-      const TB x1044 = s.QC[2] * x656 + s.WQ[2] * x657 + e1 * (x254 - s.roe * x255) + r2 * x629;
-      acc365 += static_cast<double>(x1044);
-      const TB x592 = s.QC[1] * x255 + s.WQ[1] * x256 + r1 * x217;
-      const TB x659 = s.QC[2] * x258 + s.WQ[2] * x259 + r3 * x217;
-      const TB x594 = s.QC[1] * x258 + s.WQ[1] * x259;
-      const TB x591 = s.QC[1] * x254 + s.WQ[1] * x255 + r1 * x216;
-      acc55 += static_cast<double>(x591);
-      const TB x982 = s.QC[1] * x591 + s.WQ[1] * x592 + e1 * (x254 - s.roe * x255) + r1 * x566;
-      acc303 += static_cast<double>(x982);
-      const TB x1013 = s.QC[2] * x591 + s.WQ[2] * x592 + r2 * x564;
-      acc334 += static_cast<double>(x1013);
-      const TB x658 = s.QC[2] * x257 + s.WQ[2] * x258 + r3 * x216;
-      acc87 += static_cast<double>(x658);
-      const TB x1045 = s.QC[2] * x658 + s.WQ[2] * x659 + e1 * (x257 - s.roe * x258) + r3 * x631;
-      acc366 += static_cast<double>(x1045);
-      const TB x593 = s.QC[1] * x257 + s.WQ[1] * x258;
-      acc56 += static_cast<double>(x593);
-      const TB x983 = s.QC[1] * x593 + s.WQ[1] * x594 + e1 * (x257 - s.roe * x258);
-      acc304 += static_cast<double>(x983);
-      const TB x1014 = s.QC[2] * x593 + s.WQ[2] * x594 + r3 * x566;
-      acc335 += static_cast<double>(x1014);
-      const TB x652 = s.QC[2] * x248 + s.WQ[2] * x249;
-      acc84 += static_cast<double>(x652);
-      const TB x1042 = s.QC[2] * x652 + s.WQ[2] * x653 + e1 * (x248 - s.roe * x249);
-      acc363 += static_cast<double>(x1042);
-      const TB x717 = s.QC[1] * x348 + s.WQ[1] * x349 + e1 * (x121 - s.roe * x136) + r3 * x310;
-      acc115 += static_cast<double>(x717);
-      const TB x847 = s.QC[2] * x448 + s.WQ[2] * x449 + e1 * (x121 - s.roe * x136);
-      acc177 += static_cast<double>(x847);
-      const TB x1166 = s.QC[2] * x847 + s.WQ[2] * x848 + e2 * (x448 - s.roe * x449);
-      acc487 += static_cast<double>(x1166);
-      const TB x1073 = s.QC[1] * x717 + s.WQ[1] * x718 + e2 * (x348 - s.roe * x349) + r3 * x692;
-      acc394 += static_cast<double>(x1073);
-      const TB x1104 = s.QC[2] * x717 + s.WQ[2] * x718;
-      acc425 += static_cast<double>(x1104);
-      const TB x784 = s.QC[2] * x351 + s.WQ[2] * x352 + r1 * x310;
-      acc147 += static_cast<double>(x784);
-      const TB x1136 = s.QC[2] * x784 + s.WQ[2] * x785 + e1 * (x351 - s.roe * x352) + r1 * x757;
-      acc457 += static_cast<double>(x1136);
-      const TB x782 = s.QC[2] * x348 + s.WQ[2] * x349;
-      acc146 += static_cast<double>(x782);
-      const TB x1135 = s.QC[2] * x782 + s.WQ[2] * x783 + e1 * (x348 - s.roe * x349);
-      acc456 += static_cast<double>(x1135);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(330 * R + r) * a.acc_stride + q] = acc330;
-  a.acc[(361 * R + r) * a.acc_stride + q] = acc361;
-  a.acc[(392 * R + r) * a.acc_stride + q] = acc392;
-  a.acc[(423 * R + r) * a.acc_stride + q] = acc423;
-  a.acc[(454 * R + r) * a.acc_stride + q] = acc454;
-  a.acc[(485 * R + r) * a.acc_stride + q] = acc485;
-  a.acc[(21 * R + r) * a.acc_stride + q] = acc21;
-  a.acc[(52 * R + r) * a.acc_stride + q] = acc52;
-  a.acc[(83 * R + r) * a.acc_stride + q] = acc83;
-  a.acc[(114 * R + r) * a.acc_stride + q] = acc114;
-  a.acc[(145 * R + r) * a.acc_stride + q] = acc145;
-  a.acc[(176 * R + r) * a.acc_stride + q] = acc176;
-  a.acc[(207 * R + r) * a.acc_stride + q] = acc207;
-  a.acc[(238 * R + r) * a.acc_stride + q] = acc238;
-  a.acc[(269 * R + r) * a.acc_stride + q] = acc269;
-  a.acc[(300 * R + r) * a.acc_stride + q] = acc300;
-  a.acc[(331 * R + r) * a.acc_stride + q] = acc331;
-  a.acc[(362 * R + r) * a.acc_stride + q] = acc362;
-  a.acc[(393 * R + r) * a.acc_stride + q] = acc393;
-  a.acc[(424 * R + r) * a.acc_stride + q] = acc424;
-  a.acc[(455 * R + r) * a.acc_stride + q] = acc455;
-  a.acc[(486 * R + r) * a.acc_stride + q] = acc486;
-  a.acc[(22 * R + r) * a.acc_stride + q] = acc22;
-  a.acc[(53 * R + r) * a.acc_stride + q] = acc53;
-  a.acc[(84 * R + r) * a.acc_stride + q] = acc84;
-  a.acc[(115 * R + r) * a.acc_stride + q] = acc115;
-  a.acc[(146 * R + r) * a.acc_stride + q] = acc146;
-  a.acc[(177 * R + r) * a.acc_stride + q] = acc177;
-  a.acc[(208 * R + r) * a.acc_stride + q] = acc208;
-  a.acc[(239 * R + r) * a.acc_stride + q] = acc239;
-  a.acc[(270 * R + r) * a.acc_stride + q] = acc270;
-  a.acc[(301 * R + r) * a.acc_stride + q] = acc301;
-  a.acc[(332 * R + r) * a.acc_stride + q] = acc332;
-  a.acc[(363 * R + r) * a.acc_stride + q] = acc363;
-  a.acc[(394 * R + r) * a.acc_stride + q] = acc394;
-  a.acc[(425 * R + r) * a.acc_stride + q] = acc425;
-  a.acc[(456 * R + r) * a.acc_stride + q] = acc456;
-  a.acc[(487 * R + r) * a.acc_stride + q] = acc487;
-  a.acc[(23 * R + r) * a.acc_stride + q] = acc23;
-  a.acc[(54 * R + r) * a.acc_stride + q] = acc54;
-  a.acc[(85 * R + r) * a.acc_stride + q] = acc85;
-  a.acc[(116 * R + r) * a.acc_stride + q] = acc116;
-  a.acc[(147 * R + r) * a.acc_stride + q] = acc147;
-  a.acc[(178 * R + r) * a.acc_stride + q] = acc178;
-  a.acc[(209 * R + r) * a.acc_stride + q] = acc209;
-  a.acc[(240 * R + r) * a.acc_stride + q] = acc240;
-  a.acc[(271 * R + r) * a.acc_stride + q] = acc271;
-  a.acc[(302 * R + r) * a.acc_stride + q] = acc302;
-  a.acc[(333 * R + r) * a.acc_stride + q] = acc333;
-  a.acc[(364 * R + r) * a.acc_stride + q] = acc364;
-  a.acc[(395 * R + r) * a.acc_stride + q] = acc395;
-  a.acc[(426 * R + r) * a.acc_stride + q] = acc426;
-  a.acc[(457 * R + r) * a.acc_stride + q] = acc457;
-  a.acc[(488 * R + r) * a.acc_stride + q] = acc488;
-  a.acc[(24 * R + r) * a.acc_stride + q] = acc24;
-  a.acc[(55 * R + r) * a.acc_stride + q] = acc55;
-  a.acc[(86 * R + r) * a.acc_stride + q] = acc86;
-  a.acc[(117 * R + r) * a.acc_stride + q] = acc117;
-  a.acc[(148 * R + r) * a.acc_stride + q] = acc148;
-  a.acc[(179 * R + r) * a.acc_stride + q] = acc179;
-  a.acc[(210 * R + r) * a.acc_stride + q] = acc210;
-  a.acc[(241 * R + r) * a.acc_stride + q] = acc241;
-  a.acc[(272 * R + r) * a.acc_stride + q] = acc272;
-  a.acc[(303 * R + r) * a.acc_stride + q] = acc303;
-  a.acc[(334 * R + r) * a.acc_stride + q] = acc334;
-  a.acc[(365 * R + r) * a.acc_stride + q] = acc365;
-  a.acc[(396 * R + r) * a.acc_stride + q] = acc396;
-  a.acc[(427 * R + r) * a.acc_stride + q] = acc427;
-  a.acc[(458 * R + r) * a.acc_stride + q] = acc458;
-  a.acc[(489 * R + r) * a.acc_stride + q] = acc489;
-  a.acc[(25 * R + r) * a.acc_stride + q] = acc25;
-  a.acc[(56 * R + r) * a.acc_stride + q] = acc56;
-  a.acc[(87 * R + r) * a.acc_stride + q] = acc87;
-  a.acc[(118 * R + r) * a.acc_stride + q] = acc118;
-  a.acc[(149 * R + r) * a.acc_stride + q] = acc149;
-  a.acc[(180 * R + r) * a.acc_stride + q] = acc180;
-  a.acc[(211 * R + r) * a.acc_stride + q] = acc211;
-  a.acc[(242 * R + r) * a.acc_stride + q] = acc242;
-  a.acc[(273 * R + r) * a.acc_stride + q] = acc273;
-  a.acc[(304 * R + r) * a.acc_stride + q] = acc304;
-  a.acc[(335 * R + r) * a.acc_stride + q] = acc335;
-  a.acc[(366 * R + r) * a.acc_stride + q] = acc366;
-  a.acc[(397 * R + r) * a.acc_stride + q] = acc397;
-}
-// class (22|21) split 5/6: 330 recursion values, 83 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2221_5(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc428 = 0.0;
-  double acc459 = 0.0;
-  double acc490 = 0.0;
-  double acc26 = 0.0;
-  double acc57 = 0.0;
-  double acc88 = 0.0;
-  double acc119 = 0.0;
-  double acc150 = 0.0;
-  double acc181 = 0.0;
-  double acc212 = 0.0;
-  double acc243 = 0.0;
-  double acc274 = 0.0;
-  double acc305 = 0.0;
-  double acc336 = 0.0;
-  double acc367 = 0.0;
-  double acc398 = 0.0;
-  double acc429 = 0.0;
-  double acc460 = 0.0;
-  double acc491 = 0.0;
-  double acc27 = 0.0;
-  double acc58 = 0.0;
-  double acc89 = 0.0;
-  double acc120 = 0.0;
-  double acc151 = 0.0;
-  double acc182 = 0.0;
-  double acc213 = 0.0;
-  double acc244 = 0.0;
-  double acc275 = 0.0;
-  double acc306 = 0.0;
-  double acc337 = 0.0;
-  double acc368 = 0.0;
-  double acc399 = 0.0;
-  double acc430 = 0.0;
-  double acc461 = 0.0;
-  double acc492 = 0.0;
-  double acc28 = 0.0;
-  double acc59 = 0.0;
-  double acc90 = 0.0;
-  double acc121 = 0.0;
-  double acc152 = 0.0;
-  double acc183 = 0.0;
-  double acc214 = 0.0;
-  double acc245 = 0.0;
-  double acc276 = 0.0;
-  double acc307 = 0.0;
-  double acc338 = 0.0;
-  double acc369 = 0.0;
-  double acc400 = 0.0;
-  double acc431 = 0.0;
-  double acc462 = 0.0;
-  double acc493 = 0.0;
-  double acc29 = 0.0;
-  double acc60 = 0.0;
-  double acc91 = 0.0;
-  double acc122 = 0.0;
-  double acc153 = 0.0;
-  double acc184 = 0.0;
-  double acc215 = 0.0;
-  double acc246 = 0.0;
-  double acc277 = 0.0;
-  double acc308 = 0.0;
-  double acc339 = 0.0;
-  double acc370 = 0.0;
-  double acc401 = 0.0;
-  double acc432 = 0.0;
-  double acc463 = 0.0;
-  double acc494 = 0.0;
-  double acc30 = 0.0;
-  double acc61 = 0.0;
-  double acc92 = 0.0;
-  double acc123 = 0.0;
-  double acc154 = 0.0;
-  double acc185 = 0.0;
-  double acc216 = 0.0;
-  double acc247 = 0.0;
-  double acc278 = 0.0;
-  double acc309 = 0.0;
-  double acc340 = 0.0;
-  double acc371 = 0.0;
-  double acc402 = 0.0;
-  double acc433 = 0.0;
-  double acc464 = 0.0;
-  double acc495 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[8];
-      boys_eval_fixed<7>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x8 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x9 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x10 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x11 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x12 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x13 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x14 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x15 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x16 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x17 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x18 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x19 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x20 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x21 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x22 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x23 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x24 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x25 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x26 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x61 = s.PA[2] * x23 + s.WP[2] * x26;
-      const TB x27 = s.PA[1] * x6 + s.WP[1] * x7;
-      const TB x28 = s.PA[2] * x6 + s.WP[2] * x7;
-      const TB x64 = s.PA[2] * x25 + s.WP[2] * x28 + z1 * (x5 - s.roz * x6);
-      const TB x62 = s.PA[1] * x24 + s.WP[1] * x27 + z1 * (x5 - s.roz * x6);
-      const TB x63 = s.PA[2] * x24 + s.WP[2] * x27;
-      const TB x31 = s.PA[2] * x8 + s.WP[2] * x11;
-      const TB x32 = s.PA[1] * x9 + s.WP[1] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x34 = s.PA[2] * x10 + s.WP[2] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x33 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x37 = s.PA[2] * x11 + s.WP[2] * x14;
-      const TB x70 = s.PA[2] * x31 + s.WP[2] * x37 + z1 * (x8 - s.roz * x11);
-      const TB x38 = s.PA[1] * x12 + s.WP[1] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x40 = s.PA[2] * x13 + s.WP[2] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x74 = s.PA[2] * x34 + s.WP[2] * x40 + z2 * (x10 - s.roz * x13);
-      const TB x39 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x43 = s.PA[2] * x14 + s.WP[2] * x17;
-      const TB x80 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x11 - s.roz * x14);
-      const TB x124 = s.PA[2] * x70 + s.WP[2] * x80 + z2 * (x31 - s.roz * x37);
-      const TB x44 = s.PA[1] * x15 + s.WP[1] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x46 = s.PA[2] * x16 + s.WP[2] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x84 = s.PA[2] * x40 + s.WP[2] * x46 + z2 * (x13 - s.roz * x16);
-      const TB x129 = s.PA[2] * x74 + s.WP[2] * x84 + z3 * (x34 - s.roz * x40);
-      const TB x45 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x49 = s.PA[2] * x17 + s.WP[2] * x20;
-      const TB x90 = s.PA[2] * x43 + s.WP[2] * x49 + z1 * (x14 - s.roz * x17);
-      const TB x139 = s.PA[2] * x80 + s.WP[2] * x90 + z2 * (x37 - s.roz * x43);
-      const TB x50 = s.PA[1] * x18 + s.WP[1] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[2] * x19 + s.WP[2] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x94 = s.PA[2] * x46 + s.WP[2] * x52 + z2 * (x16 - s.roz * x19);
-      const TB x144 = s.PA[2] * x84 + s.WP[2] * x94 + z3 * (x40 - s.roz * x46);
-      const TB x51 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x55 = s.PA[2] * x20 + s.WP[2] * x23;
-      const TB x110 = s.PA[2] * x55 + s.WP[2] * x61 + z1 * (x20 - s.roz * x23);
-      const TB x100 = s.PA[2] * x49 + s.WP[2] * x55 + z1 * (x17 - s.roz * x20);
-      const TB x169 = s.PA[2] * x100 + s.WP[2] * x110 + z2 * (x49 - s.roz * x55);
-      const TB x390 = s.QC[2] * x43 + s.WQ[2] * x49 + r1 * x17;
-      const TB x56 = s.PA[1] * x21 + s.WP[1] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x58 = s.PA[2] * x22 + s.WP[2] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x114 = s.PA[2] * x58 + s.WP[2] * x64 + z2 * (x22 - s.roz * x25);
-      const TB x104 = s.PA[2] * x52 + s.WP[2] * x58 + z2 * (x19 - s.roz * x22);
-      const TB x174 = s.PA[2] * x104 + s.WP[2] * x114 + z3 * (x52 - s.roz * x58);
-      const TB x57 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x113 = s.PA[2] * x57 + s.WP[2] * x63 + z1 * (x21 - s.roz * x24);
-      const TB x111 = s.PA[1] * x56 + s.WP[1] * x62 + z2 * (x21 - s.roz * x24);
-      const TB x112 = s.PA[2] * x56 + s.WP[2] * x62;
-      const TB x71 = s.PA[1] * x32 + s.WP[1] * x38 + z2 * (x9 - s.roz * x12);
-      const TB x73 = s.PA[2] * x33 + s.WP[2] * x39 + z1 * (x9 - s.roz * x12);
-      const TB x72 = s.PA[2] * x32 + s.WP[2] * x38;
-      const TB x81 = s.PA[1] * x38 + s.WP[1] * x44 + z2 * (x12 - s.roz * x15);
-      const TB x83 = s.PA[2] * x39 + s.WP[2] * x45 + z1 * (x12 - s.roz * x15);
-      const TB x128 = s.PA[2] * x73 + s.WP[2] * x83 + z2 * (x33 - s.roz * x39);
-      const TB x82 = s.PA[2] * x38 + s.WP[2] * x44;
-      const TB x127 = s.PA[2] * x72 + s.WP[2] * x82 + z1 * (x32 - s.roz * x38);
-      const TB x125 = s.PA[1] * x71 + s.WP[1] * x81 + z3 * (x32 - s.roz * x38);
-      const TB x126 = s.PA[2] * x71 + s.WP[2] * x81;
-      const TB x91 = s.PA[1] * x44 + s.WP[1] * x50 + z2 * (x15 - s.roz * x18);
-      const TB x93 = s.PA[2] * x45 + s.WP[2] * x51 + z1 * (x15 - s.roz * x18);
-      const TB x143 = s.PA[2] * x83 + s.WP[2] * x93 + z2 * (x39 - s.roz * x45);
-      const TB x92 = s.PA[2] * x44 + s.WP[2] * x50;
-      const TB x101 = s.PA[1] * x50 + s.WP[1] * x56 + z2 * (x18 - s.roz * x21);
-      const TB x103 = s.PA[2] * x51 + s.WP[2] * x57 + z1 * (x18 - s.roz * x21);
-      const TB x173 = s.PA[2] * x103 + s.WP[2] * x113 + z2 * (x51 - s.roz * x57);
-      const TB x102 = s.PA[2] * x50 + s.WP[2] * x56;
-      const TB x172 = s.PA[2] * x102 + s.WP[2] * x112 + z1 * (x50 - s.roz * x56);
-      const TB x170 = s.PA[1] * x101 + s.WP[1] * x111 + z3 * (x50 - s.roz * x56);
-      const TB x171 = s.PA[2] * x101 + s.WP[2] * x111;
-      const TB x140 = s.PA[1] * x81 + s.WP[1] * x91 + z3 * (x38 - s.roz * x44);
-      const TB x142 = s.PA[2] * x82 + s.WP[2] * x92 + z1 * (x38 - s.roz * x44);
-      const TB x141 = s.PA[2] * x81 + s.WP[2] * x91;
-      const TB x154 = s.PA[2] * x90 + s.WP[2] * x100 + z2 * (x43 - s.roz * x49);
-      const TB x155 = s.PA[1] * x91 + s.WP[1] * x101 + z3 * (x44 - s.roz * x50);
-      const TB x156 = s.PA[2] * x91 + s.WP[2] * x101;
-      const TB x157 = s.PA[2] * x92 + s.WP[2] * x102 + z1 * (x44 - s.roz * x50);
-      const TB x158 = s.PA[2] * x93 + s.WP[2] * x103 + z2 * (x45 - s.roz * x51);
-      const TB x159 = s.PA[2] * x94 + s.WP[2] * x104 + z3 * (x46 - s.roz * x52);
-      const TB x193 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x196 = s.QC[0] * x45 + s.WQ[0] * x51;
-      const TB x199 = s.QC[0] * x46 + s.WQ[0] * x52;
-      const TB x219 = s.QC[0] * x81 + s.WQ[0] * x91;
-      const TB x220 = s.QC[0] * x91 + s.WQ[0] * x101;
-      const TB x222 = s.QC[0] * x82 + s.WQ[0] * x92;
-      const TB x223 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x225 = s.QC[0] * x83 + s.WQ[0] * x93;
-      const TB x226 = s.QC[0] * x93 + s.WQ[0] * x103;
-      const TB x228 = s.QC[0] * x84 + s.WQ[0] * x94;
-      const TB x229 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x260 = s.QC[0] * x125 + s.WQ[0] * x140;
-      const TB x261 = s.QC[0] * x140 + s.WQ[0] * x155;
-      const TB x262 = s.QC[0] * x155 + s.WQ[0] * x170;
-      const TB x263 = s.QC[0] * x126 + s.WQ[0] * x141;
-      const TB x264 = s.QC[0] * x141 + s.WQ[0] * x156;
-      const TB x265 = s.QC[0] * x156 + s.WQ[0] * x171;
-      const TB x266 = s.QC[0] * x127 + s.WQ[0] * x142;
-      const TB x267 = s.QC[0] * x142 + s.WQ[0] * x157;
-      const TB x268 = s.QC[0] * x157 + s.WQ[0] * x172;
-      const TB x269 = s.QC[0] * x128 + s.WQ[0] * x143;
-      const TB x270 = s.QC[0] * x143 + s.WQ[0] * x158;
-      const TB x271 = s.QC[0] * x158 + s.WQ[0] * x173;
-      const TB x272 = s.QC[0] * x129 + s.WQ[0] * x144;
-      const TB x273 = s.QC[0] * x144 + s.WQ[0] * x159;
-      const TB x274 = s.QC[0] * x159 + s.WQ[0] * x174;
-      const TB x290 = s.QC[1] * x43 + s.WQ[1] * x49;
-      const TB x416 = s.QC[2] * x80 + s.WQ[2] * x90 + r2 * x43;
-      const TB x417 = s.QC[2] * x90 + s.WQ[2] * x100 + r2 * x49;
-      const TB x826 = s.QC[2] * x416 + s.WQ[2] * x417 + e1 * (x80 - s.roe * x90) + r2 * x390;
-      const TB x293 = s.QC[1] * x44 + s.WQ[1] * x50 + r2 * x18;
-      const TB x396 = s.QC[2] * x45 + s.WQ[2] * x51 + r1 * x18;
-      const TB x296 = s.QC[1] * x45 + s.WQ[1] * x51 + r1 * x19;
-      const TB x399 = s.QC[2] * x46 + s.WQ[2] * x52 + r2 * x19;
-      const TB x299 = s.QC[1] * x46 + s.WQ[1] * x52;
-      const TB x316 = s.QC[1] * x80 + s.WQ[1] * x90;
-      const TB x317 = s.QC[1] * x90 + s.WQ[1] * x100;
-      const TB x459 = s.QC[2] * x154 + s.WQ[2] * x169 + r3 * x100;
-      const TB x359 = s.QC[1] * x154 + s.WQ[1] * x169;
-      const TB x761 = s.QC[2] * x316 + s.WQ[2] * x317 + r2 * x290;
-      const TB x319 = s.QC[1] * x81 + s.WQ[1] * x91 + r3 * x44;
-      const TB x320 = s.QC[1] * x91 + s.WQ[1] * x101 + r3 * x50;
-      const TB x322 = s.QC[1] * x82 + s.WQ[1] * x92 + r2 * x45;
-      const TB x425 = s.QC[2] * x83 + s.WQ[2] * x93 + r2 * x45;
-      const TB x323 = s.QC[1] * x92 + s.WQ[1] * x102 + r2 * x51;
-      const TB x426 = s.QC[2] * x93 + s.WQ[2] * x103 + r2 * x51;
-      const TB x832 = s.QC[2] * x425 + s.WQ[2] * x426 + e1 * (x83 - s.roe * x93) + r2 * x396;
-      const TB x325 = s.QC[1] * x83 + s.WQ[1] * x93 + r1 * x46;
-      const TB x428 = s.QC[2] * x84 + s.WQ[2] * x94 + r3 * x46;
-      const TB x326 = s.QC[1] * x93 + s.WQ[1] * x103 + r1 * x52;
-      const TB x429 = s.QC[2] * x94 + s.WQ[2] * x104 + r3 * x52;
-      const TB x834 = s.QC[2] * x428 + s.WQ[2] * x429 + e1 * (x84 - s.roe * x94) + r3 * x399;
-      const TB x328 = s.QC[1] * x84 + s.WQ[1] * x94;
-      const TB x329 = s.QC[1] * x94 + s.WQ[1] * x104;
-      const TB x357 = s.QC[1] * x124 + s.WQ[1] * x139;
-      const TB x358 = s.QC[1] * x139 + s.WQ[1] * x154;
-      const TB x360 = s.QC[1] * x125 + s.WQ[1] * x140 + r4 * x81;
-      const TB x361 = s.QC[1] * x140 + s.WQ[1] * x155 + r4 * x91;
-      const TB x362 = s.QC[1] * x155 + s.WQ[1] * x170 + r4 * x101;
-      const TB x462 = s.QC[2] * x155 + s.WQ[2] * x170;
-      const TB x363 = s.QC[1] * x126 + s.WQ[1] * x141 + r3 * x82;
-      const TB x364 = s.QC[1] * x141 + s.WQ[1] * x156 + r3 * x92;
-      const TB x365 = s.QC[1] * x156 + s.WQ[1] * x171 + r3 * x102;
-      const TB x465 = s.QC[2] * x156 + s.WQ[2] * x171 + r1 * x101;
-      const TB x420 = s.QC[2] * x91 + s.WQ[2] * x101;
-      const TB x366 = s.QC[1] * x127 + s.WQ[1] * x142 + r2 * x83;
-      const TB x367 = s.QC[1] * x142 + s.WQ[1] * x157 + r2 * x93;
-      const TB x368 = s.QC[1] * x157 + s.WQ[1] * x172 + r2 * x103;
-      const TB x468 = s.QC[2] * x157 + s.WQ[2] * x172 + r2 * x102;
-      const TB x423 = s.QC[2] * x92 + s.WQ[2] * x102 + r1 * x50;
-      const TB x393 = s.QC[2] * x44 + s.WQ[2] * x50;
-      const TB x422 = s.QC[2] * x82 + s.WQ[2] * x92 + r1 * x44;
-      const TB x471 = s.QC[2] * x158 + s.WQ[2] * x173 + r3 * x103;
-      const TB x371 = s.QC[1] * x158 + s.WQ[1] * x173 + r1 * x104;
-      const TB x474 = s.QC[2] * x159 + s.WQ[2] * x174 + r4 * x104;
-      const TB x374 = s.QC[1] * x159 + s.WQ[1] * x174;
-      const TB x830 = s.QC[2] * x422 + s.WQ[2] * x423 + e1 * (x82 - s.roe * x92) + r1 * x393;
-      const TB x369 = s.QC[1] * x128 + s.WQ[1] * x143 + r1 * x84;
-      const TB x370 = s.QC[1] * x143 + s.WQ[1] * x158 + r1 * x94;
-      const TB x372 = s.QC[1] * x129 + s.WQ[1] * x144;
-      const TB x373 = s.QC[1] * x144 + s.WQ[1] * x159;
-      const TB x419 = s.QC[2] * x81 + s.WQ[2] * x91;
-      const TB x457 = s.QC[2] * x124 + s.WQ[2] * x139 + r3 * x80;
-      const TB x696 = s.QC[1] * x316 + s.WQ[1] * x317 + e1 * (x80 - s.roe * x90);
-      const TB x458 = s.QC[2] * x139 + s.WQ[2] * x154 + r3 * x90;
-      const TB x854 = s.QC[2] * x458 + s.WQ[2] * x459 + e1 * (x139 - s.roe * x154) + r3 * x417;
-      const TB x724 = s.QC[1] * x358 + s.WQ[1] * x359 + e1 * (x139 - s.roe * x154);
-      const TB x789 = s.QC[2] * x358 + s.WQ[2] * x359 + r3 * x317;
-      const TB x788 = s.QC[2] * x357 + s.WQ[2] * x358 + r3 * x316;
-      const TB x1138 = s.QC[2] * x788 + s.WQ[2] * x789 + e1 * (x357 - s.roe * x358) + r3 * x761;
-      acc459 += static_cast<double>(x1138);
-      const TB x723 = s.QC[1] * x357 + s.WQ[1] * x358 + e1 * (x124 - s.roe * x139);
-      const TB x1107 = s.QC[2] * x723 + s.WQ[2] * x724 + r3 * x696;
-      acc428 += static_cast<double>(x1107);
-      const TB x853 = s.QC[2] * x457 + s.WQ[2] * x458 + e1 * (x124 - s.roe * x139) + r3 * x416;
-      const TB x1169 = s.QC[2] * x853 + s.WQ[2] * x854 + e2 * (x457 - s.roe * x458) + r3 * x826;
-      acc490 += static_cast<double>(x1169);
-      const TB x460 = s.QC[2] * x125 + s.WQ[2] * x140;
-      const TB x461 = s.QC[2] * x140 + s.WQ[2] * x155;
-      const TB x856 = s.QC[2] * x461 + s.WQ[2] * x462 + e1 * (x140 - s.roe * x155);
-      const TB x463 = s.QC[2] * x126 + s.WQ[2] * x141 + r1 * x81;
-      const TB x464 = s.QC[2] * x141 + s.WQ[2] * x156 + r1 * x91;
-      const TB x858 = s.QC[2] * x464 + s.WQ[2] * x465 + e1 * (x141 - s.roe * x156) + r1 * x420;
-      const TB x828 = s.QC[2] * x419 + s.WQ[2] * x420 + e1 * (x81 - s.roe * x91);
-      const TB x857 = s.QC[2] * x463 + s.WQ[2] * x464 + e1 * (x126 - s.roe * x141) + r1 * x419;
-      acc182 += static_cast<double>(x857);
-      const TB x1171 = s.QC[2] * x857 + s.WQ[2] * x858 + e2 * (x463 - s.roe * x464) + r1 * x828;
-      acc492 += static_cast<double>(x1171);
-      const TB x466 = s.QC[2] * x127 + s.WQ[2] * x142 + r2 * x82;
-      const TB x467 = s.QC[2] * x142 + s.WQ[2] * x157 + r2 * x92;
-      const TB x860 = s.QC[2] * x467 + s.WQ[2] * x468 + e1 * (x142 - s.roe * x157) + r2 * x423;
-      const TB x859 = s.QC[2] * x466 + s.WQ[2] * x467 + e1 * (x127 - s.roe * x142) + r2 * x422;
-      acc183 += static_cast<double>(x859);
-      const TB x1172 = s.QC[2] * x859 + s.WQ[2] * x860 + e2 * (x466 - s.roe * x467) + r2 * x830;
-      acc493 += static_cast<double>(x1172);
-      const TB x469 = s.QC[2] * x128 + s.WQ[2] * x143 + r3 * x83;
-      const TB x470 = s.QC[2] * x143 + s.WQ[2] * x158 + r3 * x93;
-      const TB x862 = s.QC[2] * x470 + s.WQ[2] * x471 + e1 * (x143 - s.roe * x158) + r3 * x426;
-      const TB x861 = s.QC[2] * x469 + s.WQ[2] * x470 + e1 * (x128 - s.roe * x143) + r3 * x425;
-      acc184 += static_cast<double>(x861);
-      const TB x1173 = s.QC[2] * x861 + s.WQ[2] * x862 + e2 * (x469 - s.roe * x470) + r3 * x832;
-      acc494 += static_cast<double>(x1173);
-      const TB x472 = s.QC[2] * x129 + s.WQ[2] * x144 + r4 * x84;
-      const TB x473 = s.QC[2] * x144 + s.WQ[2] * x159 + r4 * x94;
-      const TB x864 = s.QC[2] * x473 + s.WQ[2] * x474 + e1 * (x144 - s.roe * x159) + r4 * x429;
-      const TB x863 = s.QC[2] * x472 + s.WQ[2] * x473 + e1 * (x129 - s.roe * x144) + r4 * x428;
-      acc185 += static_cast<double>(x863);
-      const TB x1174 = s.QC[2] * x863 + s.WQ[2] * x864 + e2 * (x472 - s.roe * x473) + r4 * x834;
-      acc495 += static_cast<double>(x1174);
-      const TB x503 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x81 - s.roe * x91);
-      const TB x698 = s.QC[1] * x319 + s.WQ[1] * x320 + e1 * (x81 - s.roe * x91) + r3 * x293;
-      const TB x765 = s.QC[2] * x322 + s.WQ[2] * x323 + r1 * x293;
-      const TB x505 = s.QC[0] * x222 + s.WQ[0] * x223 + e1 * (x82 - s.roe * x92);
-      const TB x700 = s.QC[1] * x322 + s.WQ[1] * x323 + e1 * (x82 - s.roe * x92) + r2 * x296;
-      const TB x767 = s.QC[2] * x325 + s.WQ[2] * x326 + r2 * x296;
-      const TB x507 = s.QC[0] * x225 + s.WQ[0] * x226 + e1 * (x83 - s.roe * x93);
-      const TB x702 = s.QC[1] * x325 + s.WQ[1] * x326 + e1 * (x83 - s.roe * x93) + r1 * x299;
-      const TB x769 = s.QC[2] * x328 + s.WQ[2] * x329 + r3 * x299;
-      const TB x509 = s.QC[0] * x228 + s.WQ[0] * x229 + e1 * (x84 - s.roe * x94);
-      const TB x704 = s.QC[1] * x328 + s.WQ[1] * x329 + e1 * (x84 - s.roe * x94);
-      const TB x530 = s.QC[0] * x260 + s.WQ[0] * x261 + e1 * (x125 - s.roe * x140);
-      acc26 += static_cast<double>(x530);
-      const TB x531 = s.QC[0] * x261 + s.WQ[0] * x262 + e1 * (x140 - s.roe * x155);
-      const TB x726 = s.QC[1] * x361 + s.WQ[1] * x362 + e1 * (x140 - s.roe * x155) + r4 * x320;
-      const TB x791 = s.QC[2] * x361 + s.WQ[2] * x362;
-      const TB x891 = s.QC[0] * x530 + s.WQ[0] * x531 + e2 * (x260 - s.roe * x261);
-      acc212 += static_cast<double>(x891);
-      const TB x922 = s.QC[1] * x530 + s.WQ[1] * x531 + r4 * x503;
-      acc243 += static_cast<double>(x922);
-      const TB x953 = s.QC[2] * x530 + s.WQ[2] * x531;
-      acc274 += static_cast<double>(x953);
-      const TB x532 = s.QC[0] * x263 + s.WQ[0] * x264 + e1 * (x126 - s.roe * x141);
-      acc27 += static_cast<double>(x532);
-      const TB x727 = s.QC[1] * x363 + s.WQ[1] * x364 + e1 * (x126 - s.roe * x141) + r3 * x322;
-      acc120 += static_cast<double>(x727);
-      const TB x794 = s.QC[2] * x366 + s.WQ[2] * x367 + r2 * x322;
-      acc152 += static_cast<double>(x794);
-      const TB x533 = s.QC[0] * x264 + s.WQ[0] * x265 + e1 * (x141 - s.roe * x156);
-      const TB x728 = s.QC[1] * x364 + s.WQ[1] * x365 + e1 * (x141 - s.roe * x156) + r3 * x323;
-      const TB x954 = s.QC[2] * x532 + s.WQ[2] * x533 + r1 * x503;
-      acc275 += static_cast<double>(x954);
-      const TB x793 = s.QC[2] * x364 + s.WQ[2] * x365 + r1 * x320;
-      const TB x763 = s.QC[2] * x319 + s.WQ[2] * x320;
-      const TB x795 = s.QC[2] * x367 + s.WQ[2] * x368 + r2 * x323;
-      const TB x1141 = s.QC[2] * x794 + s.WQ[2] * x795 + e1 * (x366 - s.roe * x367) + r2 * x765;
-      acc462 += static_cast<double>(x1141);
-      const TB x730 = s.QC[1] * x367 + s.WQ[1] * x368 + e1 * (x142 - s.roe * x157) + r2 * x326;
-      const TB x535 = s.QC[0] * x267 + s.WQ[0] * x268 + e1 * (x142 - s.roe * x157);
-      const TB x797 = s.QC[2] * x370 + s.WQ[2] * x371 + r3 * x326;
-      const TB x732 = s.QC[1] * x370 + s.WQ[1] * x371 + e1 * (x143 - s.roe * x158) + r1 * x329;
-      const TB x537 = s.QC[0] * x270 + s.WQ[0] * x271 + e1 * (x143 - s.roe * x158);
-      const TB x799 = s.QC[2] * x373 + s.WQ[2] * x374 + r4 * x329;
-      const TB x734 = s.QC[1] * x373 + s.WQ[1] * x374 + e1 * (x144 - s.roe * x159);
-      const TB x539 = s.QC[0] * x273 + s.WQ[0] * x274 + e1 * (x144 - s.roe * x159);
-      const TB x892 = s.QC[0] * x532 + s.WQ[0] * x533 + e2 * (x263 - s.roe * x264);
-      acc213 += static_cast<double>(x892);
-      const TB x923 = s.QC[1] * x532 + s.WQ[1] * x533 + r3 * x505;
-      acc244 += static_cast<double>(x923);
-      const TB x1078 = s.QC[1] * x727 + s.WQ[1] * x728 + e2 * (x363 - s.roe * x364) + r3 * x700;
-      acc399 += static_cast<double>(x1078);
-      const TB x1109 = s.QC[2] * x727 + s.WQ[2] * x728 + r1 * x698;
-      acc430 += static_cast<double>(x1109);
-      const TB x534 = s.QC[0] * x266 + s.WQ[0] * x267 + e1 * (x127 - s.roe * x142);
-      acc28 += static_cast<double>(x534);
-      const TB x729 = s.QC[1] * x366 + s.WQ[1] * x367 + e1 * (x127 - s.roe * x142) + r2 * x325;
-      acc121 += static_cast<double>(x729);
-      const TB x1079 = s.QC[1] * x729 + s.WQ[1] * x730 + e2 * (x366 - s.roe * x367) + r2 * x702;
-      acc400 += static_cast<double>(x1079);
-      const TB x1110 = s.QC[2] * x729 + s.WQ[2] * x730 + r2 * x700;
-      acc431 += static_cast<double>(x1110);
-      const TB x955 = s.QC[2] * x534 + s.WQ[2] * x535 + r2 * x505;
-      acc276 += static_cast<double>(x955);
-      const TB x796 = s.QC[2] * x369 + s.WQ[2] * x370 + r3 * x325;
-      acc153 += static_cast<double>(x796);
-      const TB x1142 = s.QC[2] * x796 + s.WQ[2] * x797 + e1 * (x369 - s.roe * x370) + r3 * x767;
-      acc463 += static_cast<double>(x1142);
-      const TB x893 = s.QC[0] * x534 + s.WQ[0] * x535 + e2 * (x266 - s.roe * x267);
-      acc214 += static_cast<double>(x893);
-      const TB x924 = s.QC[1] * x534 + s.WQ[1] * x535 + r2 * x507;
-      acc245 += static_cast<double>(x924);
-      const TB x536 = s.QC[0] * x269 + s.WQ[0] * x270 + e1 * (x128 - s.roe * x143);
-      acc29 += static_cast<double>(x536);
-      const TB x731 = s.QC[1] * x369 + s.WQ[1] * x370 + e1 * (x128 - s.roe * x143) + r1 * x328;
-      acc122 += static_cast<double>(x731);
-      const TB x1080 = s.QC[1] * x731 + s.WQ[1] * x732 + e2 * (x369 - s.roe * x370) + r1 * x704;
-      acc401 += static_cast<double>(x1080);
-      const TB x1111 = s.QC[2] * x731 + s.WQ[2] * x732 + r3 * x702;
-      acc432 += static_cast<double>(x1111);
-      const TB x956 = s.QC[2] * x536 + s.WQ[2] * x537 + r3 * x507;
-      acc277 += static_cast<double>(x956);
-      const TB x798 = s.QC[2] * x372 + s.WQ[2] * x373 + r4 * x328;
-      acc154 += static_cast<double>(x798);
-      const TB x1143 = s.QC[2] * x798 + s.WQ[2] * x799 + e1 * (x372 - s.roe * x373) + r4 * x769;
-      acc464 += static_cast<double>(x1143);
-      const TB x894 = s.QC[0] * x536 + s.WQ[0] * x537 + e2 * (x269 - s.roe * x270);
-      acc215 += static_cast<double>(x894);
-      const TB x925 = s.QC[1] * x536 + s.WQ[1] * x537 + r1 * x509;
-      acc246 += static_cast<double>(x925);
-      const TB x538 = s.QC[0] * x272 + s.WQ[0] * x273 + e1 * (x129 - s.roe * x144);
-      acc30 += static_cast<double>(x538);
-      const TB x733 = s.QC[1] * x372 + s.WQ[1] * x373 + e1 * (x129 - s.roe * x144);
-      acc123 += static_cast<double>(x733);
-      const TB x1081 = s.QC[1] * x733 + s.WQ[1] * x734 + e2 * (x372 - s.roe * x373);
-      acc402 += static_cast<double>(x1081);
-      const TB x1112 = s.QC[2] * x733 + s.WQ[2] * x734 + r4 * x704;
-      acc433 += static_cast<double>(x1112);
-      const TB x957 = s.QC[2] * x538 + s.WQ[2] * x539 + r4 * x509;
-      acc278 += static_cast<double>(x957);
-      const TB x895 = s.QC[0] * x538 + s.WQ[0] * x539 + e2 * (x272 - s.roe * x273);
-      acc216 += static_cast<double>(x895);
-      const TB x926 = s.QC[1] * x538 + s.WQ[1] * x539;
-      acc247 += static_cast<double>(x926);
-      const TB x568 = s.QC[1] * x219 + s.WQ[1] * x220 + r3 * x193;
-      const TB x635 = s.QC[2] * x222 + s.WQ[2] * x223 + r1 * x193;
-      const TB x570 = s.QC[1] * x222 + s.WQ[1] * x223 + r2 * x196;
-      const TB x637 = s.QC[2] * x225 + s.WQ[2] * x226 + r2 * x196;
-      const TB x572 = s.QC[1] * x225 + s.WQ[1] * x226 + r1 * x199;
-      const TB x639 = s.QC[2] * x228 + s.WQ[2] * x229 + r3 * x199;
-      const TB x574 = s.QC[1] * x228 + s.WQ[1] * x229;
-      const TB x595 = s.QC[1] * x260 + s.WQ[1] * x261 + r4 * x219;
-      acc57 += static_cast<double>(x595);
-      const TB x596 = s.QC[1] * x261 + s.WQ[1] * x262 + r4 * x220;
-      const TB x661 = s.QC[2] * x261 + s.WQ[2] * x262;
-      const TB x984 = s.QC[1] * x595 + s.WQ[1] * x596 + e1 * (x260 - s.roe * x261) + r4 * x568;
-      acc305 += static_cast<double>(x984);
-      const TB x1015 = s.QC[2] * x595 + s.WQ[2] * x596;
-      acc336 += static_cast<double>(x1015);
-      const TB x597 = s.QC[1] * x263 + s.WQ[1] * x264 + r3 * x222;
-      acc58 += static_cast<double>(x597);
-      const TB x664 = s.QC[2] * x266 + s.WQ[2] * x267 + r2 * x222;
-      acc90 += static_cast<double>(x664);
-      const TB x598 = s.QC[1] * x264 + s.WQ[1] * x265 + r3 * x223;
-      const TB x1016 = s.QC[2] * x597 + s.WQ[2] * x598 + r1 * x568;
-      acc337 += static_cast<double>(x1016);
-      const TB x985 = s.QC[1] * x597 + s.WQ[1] * x598 + e1 * (x263 - s.roe * x264) + r3 * x570;
-      acc306 += static_cast<double>(x985);
-      const TB x663 = s.QC[2] * x264 + s.WQ[2] * x265 + r1 * x220;
-      const TB x633 = s.QC[2] * x219 + s.WQ[2] * x220;
-      const TB x662 = s.QC[2] * x263 + s.WQ[2] * x264 + r1 * x219;
-      acc89 += static_cast<double>(x662);
-      const TB x1047 = s.QC[2] * x662 + s.WQ[2] * x663 + e1 * (x263 - s.roe * x264) + r1 * x633;
-      acc368 += static_cast<double>(x1047);
-      const TB x665 = s.QC[2] * x267 + s.WQ[2] * x268 + r2 * x223;
-      const TB x1048 = s.QC[2] * x664 + s.WQ[2] * x665 + e1 * (x266 - s.roe * x267) + r2 * x635;
-      acc369 += static_cast<double>(x1048);
-      const TB x600 = s.QC[1] * x267 + s.WQ[1] * x268 + r2 * x226;
-      const TB x667 = s.QC[2] * x270 + s.WQ[2] * x271 + r3 * x226;
-      const TB x602 = s.QC[1] * x270 + s.WQ[1] * x271 + r1 * x229;
-      const TB x669 = s.QC[2] * x273 + s.WQ[2] * x274 + r4 * x229;
-      const TB x604 = s.QC[1] * x273 + s.WQ[1] * x274;
-      const TB x599 = s.QC[1] * x266 + s.WQ[1] * x267 + r2 * x225;
-      acc59 += static_cast<double>(x599);
-      const TB x986 = s.QC[1] * x599 + s.WQ[1] * x600 + e1 * (x266 - s.roe * x267) + r2 * x572;
-      acc307 += static_cast<double>(x986);
-      const TB x1017 = s.QC[2] * x599 + s.WQ[2] * x600 + r2 * x570;
-      acc338 += static_cast<double>(x1017);
-      const TB x666 = s.QC[2] * x269 + s.WQ[2] * x270 + r3 * x225;
-      acc91 += static_cast<double>(x666);
-      const TB x1049 = s.QC[2] * x666 + s.WQ[2] * x667 + e1 * (x269 - s.roe * x270) + r3 * x637;
-      acc370 += static_cast<double>(x1049);
-      const TB x601 = s.QC[1] * x269 + s.WQ[1] * x270 + r1 * x228;
-      acc60 += static_cast<double>(x601);
-      const TB x987 = s.QC[1] * x601 + s.WQ[1] * x602 + e1 * (x269 - s.roe * x270) + r1 * x574;
-      acc308 += static_cast<double>(x987);
-      const TB x1018 = s.QC[2] * x601 + s.WQ[2] * x602 + r3 * x572;
-      acc339 += static_cast<double>(x1018);
-      const TB x668 = s.QC[2] * x272 + s.WQ[2] * x273 + r4 * x228;
-      acc92 += static_cast<double>(x668);
-      const TB x1050 = s.QC[2] * x668 + s.WQ[2] * x669 + e1 * (x272 - s.roe * x273) + r4 * x639;
-      acc371 += static_cast<double>(x1050);
-      const TB x603 = s.QC[1] * x272 + s.WQ[1] * x273;
-      acc61 += static_cast<double>(x603);
-      const TB x988 = s.QC[1] * x603 + s.WQ[1] * x604 + e1 * (x272 - s.roe * x273);
-      acc309 += static_cast<double>(x988);
-      const TB x1019 = s.QC[2] * x603 + s.WQ[2] * x604 + r4 * x574;
-      acc340 += static_cast<double>(x1019);
-      const TB x660 = s.QC[2] * x260 + s.WQ[2] * x261;
-      acc88 += static_cast<double>(x660);
-      const TB x1046 = s.QC[2] * x660 + s.WQ[2] * x661 + e1 * (x260 - s.roe * x261);
-      acc367 += static_cast<double>(x1046);
-      const TB x725 = s.QC[1] * x360 + s.WQ[1] * x361 + e1 * (x125 - s.roe * x140) + r4 * x319;
-      acc119 += static_cast<double>(x725);
-      const TB x855 = s.QC[2] * x460 + s.WQ[2] * x461 + e1 * (x125 - s.roe * x140);
-      acc181 += static_cast<double>(x855);
-      const TB x1170 = s.QC[2] * x855 + s.WQ[2] * x856 + e2 * (x460 - s.roe * x461);
-      acc491 += static_cast<double>(x1170);
-      const TB x1077 = s.QC[1] * x725 + s.WQ[1] * x726 + e2 * (x360 - s.roe * x361) + r4 * x698;
-      acc398 += static_cast<double>(x1077);
-      const TB x1108 = s.QC[2] * x725 + s.WQ[2] * x726;
-      acc429 += static_cast<double>(x1108);
-      const TB x792 = s.QC[2] * x363 + s.WQ[2] * x364 + r1 * x319;
-      acc151 += static_cast<double>(x792);
-      const TB x1140 = s.QC[2] * x792 + s.WQ[2] * x793 + e1 * (x363 - s.roe * x364) + r1 * x763;
-      acc461 += static_cast<double>(x1140);
-      const TB x790 = s.QC[2] * x360 + s.WQ[2] * x361;
-      acc150 += static_cast<double>(x790);
-      const TB x1139 = s.QC[2] * x790 + s.WQ[2] * x791 + e1 * (x360 - s.roe * x361);
-      acc460 += static_cast<double>(x1139);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(428 * R + r) * a.acc_stride + q] = acc428;
-  a.acc[(459 * R + r) * a.acc_stride + q] = acc459;
-  a.acc[(490 * R + r) * a.acc_stride + q] = acc490;
-  a.acc[(26 * R + r) * a.acc_stride + q] = acc26;
-  a.acc[(57 * R + r) * a.acc_stride + q] = acc57;
-  a.acc[(88 * R + r) * a.acc_stride + q] = acc88;
-  a.acc[(119 * R + r) * a.acc_stride + q] = acc119;
-  a.acc[(150 * R + r) * a.acc_stride + q] = acc150;
-  a.acc[(181 * R + r) * a.acc_stride + q] = acc181;
-  a.acc[(212 * R + r) * a.acc_stride + q] = acc212;
-  a.acc[(243 * R + r) * a.acc_stride + q] = acc243;
-  a.acc[(274 * R + r) * a.acc_stride + q] = acc274;
-  a.acc[(305 * R + r) * a.acc_stride + q] = acc305;
-  a.acc[(336 * R + r) * a.acc_stride + q] = acc336;
-  a.acc[(367 * R + r) * a.acc_stride + q] = acc367;
-  a.acc[(398 * R + r) * a.acc_stride + q] = acc398;
-  a.acc[(429 * R + r) * a.acc_stride + q] = acc429;
-  a.acc[(460 * R + r) * a.acc_stride + q] = acc460;
-  a.acc[(491 * R + r) * a.acc_stride + q] = acc491;
-  a.acc[(27 * R + r) * a.acc_stride + q] = acc27;
-  a.acc[(58 * R + r) * a.acc_stride + q] = acc58;
-  a.acc[(89 * R + r) * a.acc_stride + q] = acc89;
-  a.acc[(120 * R + r) * a.acc_stride + q] = acc120;
-  a.acc[(151 * R + r) * a.acc_stride + q] = acc151;
-  a.acc[(182 * R + r) * a.acc_stride + q] = acc182;
-  a.acc[(213 * R + r) * a.acc_stride + q] = acc213;
-  a.acc[(244 * R + r) * a.acc_stride + q] = acc244;
-  a.acc[(275 * R + r) * a.acc_stride + q] = acc275;
-  a.acc[(306 * R + r) * a.acc_stride + q] = acc306;
-  a.acc[(337 * R + r) * a.acc_stride + q] = acc337;
-  a.acc[(368 * R + r) * a.acc_stride + q] = acc368;
-  a.acc[(399 * R + r) * a.acc_stride + q] = acc399;
-  a.acc[(430 * R + r) * a.acc_stride + q] = acc430;
-  a.acc[(461 * R + r) * a.acc_stride + q] = acc461;
-  a.acc[(492 * R + r) * a.acc_stride + q] = acc492;
-  a.acc[(28 * R + r) * a.acc_stride + q] = acc28;
-  a.acc[(59 * R + r) * a.acc_stride + q] = acc59;
-  a.acc[(90 * R + r) * a.acc_stride + q] = acc90;
-  a.acc[(121 * R + r) * a.acc_stride + q] = acc121;
-  a.acc[(152 * R + r) * a.acc_stride + q] = acc152;
-  a.acc[(183 * R + r) * a.acc_stride + q] = acc183;
-  a.acc[(214 * R + r) * a.acc_stride + q] = acc214;
-  a.acc[(245 * R + r) * a.acc_stride + q] = acc245;
-  a.acc[(276 * R + r) * a.acc_stride + q] = acc276;
-  a.acc[(307 * R + r) * a.acc_stride + q] = acc307;
-  a.acc[(338 * R + r) * a.acc_stride + q] = acc338;
-  a.acc[(369 * R + r) * a.acc_stride + q] = acc369;
-  a.acc[(400 * R + r) * a.acc_stride + q] = acc400;
-  a.acc[(431 * R + r) * a.acc_stride + q] = acc431;
-  a.acc[(462 * R + r) * a.acc_stride + q] = acc462;
-  a.acc[(493 * R + r) * a.acc_stride + q] = acc493;
-  a.acc[(29 * R + r) * a.acc_stride + q] = acc29;
-  a.acc[(60 * R + r) * a.acc_stride + q] = acc60;
-  a.acc[(91 * R + r) * a.acc_stride + q] = acc91;
-  a.acc[(122 * R + r) * a.acc_stride + q] = acc122;
-  a.acc[(153 * R + r) * a.acc_stride + q] = acc153;
-  a.acc[(184 * R + r) * a.acc_stride + q] = acc184;
-  a.acc[(215 * R + r) * a.acc_stride + q] = acc215;
-  a.acc[(246 * R + r) * a.acc_stride + q] = acc246;
-  a.acc[(277 * R + r) * a.acc_stride + q] = acc277;
-  a.acc[(308 * R + r) * a.acc_stride + q] = acc308;
-  a.acc[(339 * R + r) * a.acc_stride + q] = acc339;
-  a.acc[(370 * R + r) * a.acc_stride + q] = acc370;
-  a.acc[(401 * R + r) * a.acc_stride + q] = acc401;
-  a.acc[(432 * R + r) * a.acc_stride + q] = acc432;
-  a.acc[(463 * R + r) * a.acc_stride + q] = acc463;
-  a.acc[(494 * R + r) * a.acc_stride + q] = acc494;
-  a.acc[(30 * R + r) * a.acc_stride + q] = acc30;
-  a.acc[(61 * R + r) * a.acc_stride + q] = acc61;
-  a.acc[(92 * R + r) * a.acc_stride + q] = acc92;
-  a.acc[(123 * R + r) * a.acc_stride + q] = acc123;
-  a.acc[(154 * R + r) * a.acc_stride + q] = acc154;
-  a.acc[(185 * R + r) * a.acc_stride + q] = acc185;
-  a.acc[(216 * R + r) * a.acc_stride + q] = acc216;
-  a.acc[(247 * R + r) * a.acc_stride + q] = acc247;
-  a.acc[(278 * R + r) * a.acc_stride + q] = acc278;
-  a.acc[(309 * R + r) * a.acc_stride + q] = acc309;
-  a.acc[(340 * R + r) * a.acc_stride + q] = acc340;
-  a.acc[(371 * R + r) * a.acc_stride + q] = acc371;
-  a.acc[(402 * R + r) * a.acc_stride + q] = acc402;
-  a.acc[(433 * R + r) * a.acc_stride + q] = acc433;
-  a.acc[(464 * R + r) * a.acc_stride + q] = acc464;
-  a.acc[(495 * R + r) * a.acc_stride + q] = acc495;
-}
-// class (22|21): horizontal recursion per ket column, 496 accumulators
-__global__ __launch_bounds__(kRegThreads) void eri_hrr_2221(gbx_tables t, RegArgs a) {
-  const int64_t q = lane_quartet(a, static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x);
-  if (q < 0) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  const double AB[3] = {t.pair_geom[4 * bra], t.pair_geom[4 * bra + 1], t.pair_geom[4 * bra + 2]};
-  const double CD[3] = {t.pair_geom[4 * ket], t.pair_geom[4 * ket + 1], t.pair_geom[4 * ket + 2]};
-  ColumnStore<6, 6, 6, 3> st(t, a, bra, ket, pb, pk);
-  {
-    const ColumnSlot cl = st.column(0, 0);
-    const double v0 = sum_parts(a, 202, q);
-    const double v1 = sum_parts(a, 16, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 192, q);
-    const double v4 = sum_parts(a, 6, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 186, q);
-    const double v8 = sum_parts(a, 0, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 203, q);
-    const double v13 = sum_parts(a, 17, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 193, q);
-    const double v16 = sum_parts(a, 7, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 204, q);
-    const double v21 = sum_parts(a, 18, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 194, q);
-    const double v24 = sum_parts(a, 8, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 205, q);
-    const double v29 = sum_parts(a, 19, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 206, q);
-    const double v35 = sum_parts(a, 20, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 207, q);
-    const double v40 = sum_parts(a, 21, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 187, q);
-    const double v46 = sum_parts(a, 1, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 195, q);
-    const double v51 = sum_parts(a, 9, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 196, q);
-    const double v56 = sum_parts(a, 10, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 208, q);
-    const double v61 = sum_parts(a, 22, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 209, q);
-    const double v67 = sum_parts(a, 23, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 210, q);
-    const double v72 = sum_parts(a, 24, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 188, q);
-    const double v78 = sum_parts(a, 2, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 197, q);
-    const double v84 = sum_parts(a, 11, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 211, q);
-    const double v93 = sum_parts(a, 25, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 189, q);
-    const double v99 = sum_parts(a, 3, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 198, q);
-    const double v104 = sum_parts(a, 12, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 199, q);
-    const double v109 = sum_parts(a, 13, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 212, q);
-    const double v114 = sum_parts(a, 26, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 213, q);
-    const double v120 = sum_parts(a, 27, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 214, q);
-    const double v125 = sum_parts(a, 28, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 190, q);
-    const double v131 = sum_parts(a, 4, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 200, q);
-    const double v137 = sum_parts(a, 14, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 215, q);
-    const double v146 = sum_parts(a, 29, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 191, q);
-    const double v152 = sum_parts(a, 5, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 201, q);
-    const double v158 = sum_parts(a, 15, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 216, q);
-    const double v167 = sum_parts(a, 30, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(0, 1);
-    const double v0 = sum_parts(a, 233, q);
-    const double v1 = sum_parts(a, 16, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 223, q);
-    const double v4 = sum_parts(a, 6, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 217, q);
-    const double v8 = sum_parts(a, 0, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 234, q);
-    const double v13 = sum_parts(a, 17, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 224, q);
-    const double v16 = sum_parts(a, 7, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 235, q);
-    const double v21 = sum_parts(a, 18, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 225, q);
-    const double v24 = sum_parts(a, 8, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 236, q);
-    const double v29 = sum_parts(a, 19, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 237, q);
-    const double v35 = sum_parts(a, 20, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 238, q);
-    const double v40 = sum_parts(a, 21, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 218, q);
-    const double v46 = sum_parts(a, 1, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 226, q);
-    const double v51 = sum_parts(a, 9, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 227, q);
-    const double v56 = sum_parts(a, 10, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 239, q);
-    const double v61 = sum_parts(a, 22, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 240, q);
-    const double v67 = sum_parts(a, 23, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 241, q);
-    const double v72 = sum_parts(a, 24, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 219, q);
-    const double v78 = sum_parts(a, 2, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 228, q);
-    const double v84 = sum_parts(a, 11, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 242, q);
-    const double v93 = sum_parts(a, 25, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 220, q);
-    const double v99 = sum_parts(a, 3, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 229, q);
-    const double v104 = sum_parts(a, 12, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 230, q);
-    const double v109 = sum_parts(a, 13, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 243, q);
-    const double v114 = sum_parts(a, 26, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 244, q);
-    const double v120 = sum_parts(a, 27, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 245, q);
-    const double v125 = sum_parts(a, 28, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 221, q);
-    const double v131 = sum_parts(a, 4, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 231, q);
-    const double v137 = sum_parts(a, 14, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 246, q);
-    const double v146 = sum_parts(a, 29, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 222, q);
-    const double v152 = sum_parts(a, 5, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 232, q);
-    const double v158 = sum_parts(a, 15, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 247, q);
-    const double v167 = sum_parts(a, 30, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(0, 2);
-    const double v0 = sum_parts(a, 264, q);
-    const double v1 = sum_parts(a, 16, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 254, q);
-    const double v4 = sum_parts(a, 6, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 248, q);
-    const double v8 = sum_parts(a, 0, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 265, q);
-    const double v13 = sum_parts(a, 17, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 255, q);
-    const double v16 = sum_parts(a, 7, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 266, q);
-    const double v21 = sum_parts(a, 18, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 256, q);
-    const double v24 = sum_parts(a, 8, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 267, q);
-    const double v29 = sum_parts(a, 19, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 268, q);
-    const double v35 = sum_parts(a, 20, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 269, q);
-    const double v40 = sum_parts(a, 21, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 249, q);
-    const double v46 = sum_parts(a, 1, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 257, q);
-    const double v51 = sum_parts(a, 9, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 258, q);
-    const double v56 = sum_parts(a, 10, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 270, q);
-    const double v61 = sum_parts(a, 22, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 271, q);
-    const double v67 = sum_parts(a, 23, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 272, q);
-    const double v72 = sum_parts(a, 24, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 250, q);
-    const double v78 = sum_parts(a, 2, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 259, q);
-    const double v84 = sum_parts(a, 11, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 273, q);
-    const double v93 = sum_parts(a, 25, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 251, q);
-    const double v99 = sum_parts(a, 3, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 260, q);
-    const double v104 = sum_parts(a, 12, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 261, q);
-    const double v109 = sum_parts(a, 13, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 274, q);
-    const double v114 = sum_parts(a, 26, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 275, q);
-    const double v120 = sum_parts(a, 27, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 276, q);
-    const double v125 = sum_parts(a, 28, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 252, q);
-    const double v131 = sum_parts(a, 4, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 262, q);
-    const double v137 = sum_parts(a, 14, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 277, q);
-    const double v146 = sum_parts(a, 29, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 253, q);
-    const double v152 = sum_parts(a, 5, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 263, q);
-    const double v158 = sum_parts(a, 15, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 278, q);
-    const double v167 = sum_parts(a, 30, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 0);
-    const double v0 = sum_parts(a, 233, q);
-    const double v1 = sum_parts(a, 47, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 223, q);
-    const double v4 = sum_parts(a, 37, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 217, q);
-    const double v8 = sum_parts(a, 31, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 234, q);
-    const double v13 = sum_parts(a, 48, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 224, q);
-    const double v16 = sum_parts(a, 38, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 235, q);
-    const double v21 = sum_parts(a, 49, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 225, q);
-    const double v24 = sum_parts(a, 39, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 236, q);
-    const double v29 = sum_parts(a, 50, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 237, q);
-    const double v35 = sum_parts(a, 51, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 238, q);
-    const double v40 = sum_parts(a, 52, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 218, q);
-    const double v46 = sum_parts(a, 32, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 226, q);
-    const double v51 = sum_parts(a, 40, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 227, q);
-    const double v56 = sum_parts(a, 41, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 239, q);
-    const double v61 = sum_parts(a, 53, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 240, q);
-    const double v67 = sum_parts(a, 54, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 241, q);
-    const double v72 = sum_parts(a, 55, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 219, q);
-    const double v78 = sum_parts(a, 33, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 228, q);
-    const double v84 = sum_parts(a, 42, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 242, q);
-    const double v93 = sum_parts(a, 56, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 220, q);
-    const double v99 = sum_parts(a, 34, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 229, q);
-    const double v104 = sum_parts(a, 43, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 230, q);
-    const double v109 = sum_parts(a, 44, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 243, q);
-    const double v114 = sum_parts(a, 57, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 244, q);
-    const double v120 = sum_parts(a, 58, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 245, q);
-    const double v125 = sum_parts(a, 59, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 221, q);
-    const double v131 = sum_parts(a, 35, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 231, q);
-    const double v137 = sum_parts(a, 45, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 246, q);
-    const double v146 = sum_parts(a, 60, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 222, q);
-    const double v152 = sum_parts(a, 36, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 232, q);
-    const double v158 = sum_parts(a, 46, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 247, q);
-    const double v167 = sum_parts(a, 61, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 1);
-    const double v0 = sum_parts(a, 295, q);
-    const double v1 = sum_parts(a, 47, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 285, q);
-    const double v4 = sum_parts(a, 37, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 279, q);
-    const double v8 = sum_parts(a, 31, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 296, q);
-    const double v13 = sum_parts(a, 48, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 286, q);
-    const double v16 = sum_parts(a, 38, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 297, q);
-    const double v21 = sum_parts(a, 49, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 287, q);
-    const double v24 = sum_parts(a, 39, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 298, q);
-    const double v29 = sum_parts(a, 50, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 299, q);
-    const double v35 = sum_parts(a, 51, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 300, q);
-    const double v40 = sum_parts(a, 52, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 280, q);
-    const double v46 = sum_parts(a, 32, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 288, q);
-    const double v51 = sum_parts(a, 40, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 289, q);
-    const double v56 = sum_parts(a, 41, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 301, q);
-    const double v61 = sum_parts(a, 53, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 302, q);
-    const double v67 = sum_parts(a, 54, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 303, q);
-    const double v72 = sum_parts(a, 55, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 281, q);
-    const double v78 = sum_parts(a, 33, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 290, q);
-    const double v84 = sum_parts(a, 42, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 304, q);
-    const double v93 = sum_parts(a, 56, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 282, q);
-    const double v99 = sum_parts(a, 34, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 291, q);
-    const double v104 = sum_parts(a, 43, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 292, q);
-    const double v109 = sum_parts(a, 44, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 305, q);
-    const double v114 = sum_parts(a, 57, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 306, q);
-    const double v120 = sum_parts(a, 58, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 307, q);
-    const double v125 = sum_parts(a, 59, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 283, q);
-    const double v131 = sum_parts(a, 35, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 293, q);
-    const double v137 = sum_parts(a, 45, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 308, q);
-    const double v146 = sum_parts(a, 60, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 284, q);
-    const double v152 = sum_parts(a, 36, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 294, q);
-    const double v158 = sum_parts(a, 46, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 309, q);
-    const double v167 = sum_parts(a, 61, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(1, 2);
-    const double v0 = sum_parts(a, 326, q);
-    const double v1 = sum_parts(a, 47, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 316, q);
-    const double v4 = sum_parts(a, 37, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 310, q);
-    const double v8 = sum_parts(a, 31, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 327, q);
-    const double v13 = sum_parts(a, 48, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 317, q);
-    const double v16 = sum_parts(a, 38, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 328, q);
-    const double v21 = sum_parts(a, 49, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 318, q);
-    const double v24 = sum_parts(a, 39, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 329, q);
-    const double v29 = sum_parts(a, 50, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 330, q);
-    const double v35 = sum_parts(a, 51, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 331, q);
-    const double v40 = sum_parts(a, 52, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 311, q);
-    const double v46 = sum_parts(a, 32, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 319, q);
-    const double v51 = sum_parts(a, 40, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 320, q);
-    const double v56 = sum_parts(a, 41, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 332, q);
-    const double v61 = sum_parts(a, 53, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 333, q);
-    const double v67 = sum_parts(a, 54, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 334, q);
-    const double v72 = sum_parts(a, 55, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 312, q);
-    const double v78 = sum_parts(a, 33, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 321, q);
-    const double v84 = sum_parts(a, 42, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 335, q);
-    const double v93 = sum_parts(a, 56, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 313, q);
-    const double v99 = sum_parts(a, 34, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 322, q);
-    const double v104 = sum_parts(a, 43, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 323, q);
-    const double v109 = sum_parts(a, 44, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 336, q);
-    const double v114 = sum_parts(a, 57, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 337, q);
-    const double v120 = sum_parts(a, 58, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 338, q);
-    const double v125 = sum_parts(a, 59, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 314, q);
-    const double v131 = sum_parts(a, 35, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 324, q);
-    const double v137 = sum_parts(a, 45, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 339, q);
-    const double v146 = sum_parts(a, 60, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 315, q);
-    const double v152 = sum_parts(a, 36, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 325, q);
-    const double v158 = sum_parts(a, 46, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 340, q);
-    const double v167 = sum_parts(a, 61, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 0);
-    const double v0 = sum_parts(a, 264, q);
-    const double v1 = sum_parts(a, 78, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 254, q);
-    const double v4 = sum_parts(a, 68, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 248, q);
-    const double v8 = sum_parts(a, 62, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 265, q);
-    const double v13 = sum_parts(a, 79, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 255, q);
-    const double v16 = sum_parts(a, 69, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 266, q);
-    const double v21 = sum_parts(a, 80, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 256, q);
-    const double v24 = sum_parts(a, 70, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 267, q);
-    const double v29 = sum_parts(a, 81, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 268, q);
-    const double v35 = sum_parts(a, 82, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 269, q);
-    const double v40 = sum_parts(a, 83, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 249, q);
-    const double v46 = sum_parts(a, 63, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 257, q);
-    const double v51 = sum_parts(a, 71, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 258, q);
-    const double v56 = sum_parts(a, 72, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 270, q);
-    const double v61 = sum_parts(a, 84, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 271, q);
-    const double v67 = sum_parts(a, 85, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 272, q);
-    const double v72 = sum_parts(a, 86, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 250, q);
-    const double v78 = sum_parts(a, 64, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 259, q);
-    const double v84 = sum_parts(a, 73, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 273, q);
-    const double v93 = sum_parts(a, 87, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 251, q);
-    const double v99 = sum_parts(a, 65, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 260, q);
-    const double v104 = sum_parts(a, 74, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 261, q);
-    const double v109 = sum_parts(a, 75, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 274, q);
-    const double v114 = sum_parts(a, 88, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 275, q);
-    const double v120 = sum_parts(a, 89, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 276, q);
-    const double v125 = sum_parts(a, 90, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 252, q);
-    const double v131 = sum_parts(a, 66, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 262, q);
-    const double v137 = sum_parts(a, 76, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 277, q);
-    const double v146 = sum_parts(a, 91, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 253, q);
-    const double v152 = sum_parts(a, 67, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 263, q);
-    const double v158 = sum_parts(a, 77, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 278, q);
-    const double v167 = sum_parts(a, 92, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 1);
-    const double v0 = sum_parts(a, 326, q);
-    const double v1 = sum_parts(a, 78, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 316, q);
-    const double v4 = sum_parts(a, 68, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 310, q);
-    const double v8 = sum_parts(a, 62, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 327, q);
-    const double v13 = sum_parts(a, 79, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 317, q);
-    const double v16 = sum_parts(a, 69, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 328, q);
-    const double v21 = sum_parts(a, 80, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 318, q);
-    const double v24 = sum_parts(a, 70, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 329, q);
-    const double v29 = sum_parts(a, 81, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 330, q);
-    const double v35 = sum_parts(a, 82, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 331, q);
-    const double v40 = sum_parts(a, 83, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 311, q);
-    const double v46 = sum_parts(a, 63, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 319, q);
-    const double v51 = sum_parts(a, 71, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 320, q);
-    const double v56 = sum_parts(a, 72, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 332, q);
-    const double v61 = sum_parts(a, 84, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 333, q);
-    const double v67 = sum_parts(a, 85, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 334, q);
-    const double v72 = sum_parts(a, 86, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 312, q);
-    const double v78 = sum_parts(a, 64, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 321, q);
-    const double v84 = sum_parts(a, 73, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 335, q);
-    const double v93 = sum_parts(a, 87, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 313, q);
-    const double v99 = sum_parts(a, 65, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 322, q);
-    const double v104 = sum_parts(a, 74, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 323, q);
-    const double v109 = sum_parts(a, 75, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 336, q);
-    const double v114 = sum_parts(a, 88, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 337, q);
-    const double v120 = sum_parts(a, 89, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 338, q);
-    const double v125 = sum_parts(a, 90, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 314, q);
-    const double v131 = sum_parts(a, 66, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 324, q);
-    const double v137 = sum_parts(a, 76, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 339, q);
-    const double v146 = sum_parts(a, 91, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 315, q);
-    const double v152 = sum_parts(a, 67, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 325, q);
-    const double v158 = sum_parts(a, 77, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 340, q);
-    const double v167 = sum_parts(a, 92, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(2, 2);
-    const double v0 = sum_parts(a, 357, q);
-    const double v1 = sum_parts(a, 78, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 347, q);
-    const double v4 = sum_parts(a, 68, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 341, q);
-    const double v8 = sum_parts(a, 62, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 358, q);
-    const double v13 = sum_parts(a, 79, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 348, q);
-    const double v16 = sum_parts(a, 69, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 359, q);
-    const double v21 = sum_parts(a, 80, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 349, q);
-    const double v24 = sum_parts(a, 70, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 360, q);
-    const double v29 = sum_parts(a, 81, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 361, q);
-    const double v35 = sum_parts(a, 82, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 362, q);
-    const double v40 = sum_parts(a, 83, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 342, q);
-    const double v46 = sum_parts(a, 63, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 350, q);
-    const double v51 = sum_parts(a, 71, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 351, q);
-    const double v56 = sum_parts(a, 72, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 363, q);
-    const double v61 = sum_parts(a, 84, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 364, q);
-    const double v67 = sum_parts(a, 85, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 365, q);
-    const double v72 = sum_parts(a, 86, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 343, q);
-    const double v78 = sum_parts(a, 64, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 352, q);
-    const double v84 = sum_parts(a, 73, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 366, q);
-    const double v93 = sum_parts(a, 87, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 344, q);
-    const double v99 = sum_parts(a, 65, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 353, q);
-    const double v104 = sum_parts(a, 74, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 354, q);
-    const double v109 = sum_parts(a, 75, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 367, q);
-    const double v114 = sum_parts(a, 88, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 368, q);
-    const double v120 = sum_parts(a, 89, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 369, q);
-    const double v125 = sum_parts(a, 90, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 345, q);
-    const double v131 = sum_parts(a, 66, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 355, q);
-    const double v137 = sum_parts(a, 76, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 370, q);
-    const double v146 = sum_parts(a, 91, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 346, q);
-    const double v152 = sum_parts(a, 67, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 356, q);
-    const double v158 = sum_parts(a, 77, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 371, q);
-    const double v167 = sum_parts(a, 92, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(3, 0);
-    const double v0 = sum_parts(a, 295, q);
-    const double v1 = sum_parts(a, 109, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 285, q);
-    const double v4 = sum_parts(a, 99, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 279, q);
-    const double v8 = sum_parts(a, 93, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 296, q);
-    const double v13 = sum_parts(a, 110, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 286, q);
-    const double v16 = sum_parts(a, 100, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 297, q);
-    const double v21 = sum_parts(a, 111, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 287, q);
-    const double v24 = sum_parts(a, 101, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 298, q);
-    const double v29 = sum_parts(a, 112, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 299, q);
-    const double v35 = sum_parts(a, 113, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 300, q);
-    const double v40 = sum_parts(a, 114, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 280, q);
-    const double v46 = sum_parts(a, 94, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 288, q);
-    const double v51 = sum_parts(a, 102, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 289, q);
-    const double v56 = sum_parts(a, 103, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 301, q);
-    const double v61 = sum_parts(a, 115, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 302, q);
-    const double v67 = sum_parts(a, 116, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 303, q);
-    const double v72 = sum_parts(a, 117, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 281, q);
-    const double v78 = sum_parts(a, 95, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 290, q);
-    const double v84 = sum_parts(a, 104, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 304, q);
-    const double v93 = sum_parts(a, 118, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 282, q);
-    const double v99 = sum_parts(a, 96, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 291, q);
-    const double v104 = sum_parts(a, 105, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 292, q);
-    const double v109 = sum_parts(a, 106, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 305, q);
-    const double v114 = sum_parts(a, 119, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 306, q);
-    const double v120 = sum_parts(a, 120, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 307, q);
-    const double v125 = sum_parts(a, 121, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 283, q);
-    const double v131 = sum_parts(a, 97, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 293, q);
-    const double v137 = sum_parts(a, 107, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 308, q);
-    const double v146 = sum_parts(a, 122, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 284, q);
-    const double v152 = sum_parts(a, 98, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 294, q);
-    const double v158 = sum_parts(a, 108, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 309, q);
-    const double v167 = sum_parts(a, 123, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(3, 1);
-    const double v0 = sum_parts(a, 388, q);
-    const double v1 = sum_parts(a, 109, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 378, q);
-    const double v4 = sum_parts(a, 99, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 372, q);
-    const double v8 = sum_parts(a, 93, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 389, q);
-    const double v13 = sum_parts(a, 110, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 379, q);
-    const double v16 = sum_parts(a, 100, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 390, q);
-    const double v21 = sum_parts(a, 111, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 380, q);
-    const double v24 = sum_parts(a, 101, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 391, q);
-    const double v29 = sum_parts(a, 112, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 392, q);
-    const double v35 = sum_parts(a, 113, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 393, q);
-    const double v40 = sum_parts(a, 114, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 373, q);
-    const double v46 = sum_parts(a, 94, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 381, q);
-    const double v51 = sum_parts(a, 102, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 382, q);
-    const double v56 = sum_parts(a, 103, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 394, q);
-    const double v61 = sum_parts(a, 115, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 395, q);
-    const double v67 = sum_parts(a, 116, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 396, q);
-    const double v72 = sum_parts(a, 117, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 374, q);
-    const double v78 = sum_parts(a, 95, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 383, q);
-    const double v84 = sum_parts(a, 104, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 397, q);
-    const double v93 = sum_parts(a, 118, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 375, q);
-    const double v99 = sum_parts(a, 96, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 384, q);
-    const double v104 = sum_parts(a, 105, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 385, q);
-    const double v109 = sum_parts(a, 106, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 398, q);
-    const double v114 = sum_parts(a, 119, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 399, q);
-    const double v120 = sum_parts(a, 120, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 400, q);
-    const double v125 = sum_parts(a, 121, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 376, q);
-    const double v131 = sum_parts(a, 97, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 386, q);
-    const double v137 = sum_parts(a, 107, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 401, q);
-    const double v146 = sum_parts(a, 122, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 377, q);
-    const double v152 = sum_parts(a, 98, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 387, q);
-    const double v158 = sum_parts(a, 108, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 402, q);
-    const double v167 = sum_parts(a, 123, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(3, 2);
-    const double v0 = sum_parts(a, 419, q);
-    const double v1 = sum_parts(a, 109, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 409, q);
-    const double v4 = sum_parts(a, 99, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 403, q);
-    const double v8 = sum_parts(a, 93, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 420, q);
-    const double v13 = sum_parts(a, 110, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 410, q);
-    const double v16 = sum_parts(a, 100, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 421, q);
-    const double v21 = sum_parts(a, 111, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 411, q);
-    const double v24 = sum_parts(a, 101, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 422, q);
-    const double v29 = sum_parts(a, 112, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 423, q);
-    const double v35 = sum_parts(a, 113, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 424, q);
-    const double v40 = sum_parts(a, 114, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 404, q);
-    const double v46 = sum_parts(a, 94, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 412, q);
-    const double v51 = sum_parts(a, 102, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 413, q);
-    const double v56 = sum_parts(a, 103, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 425, q);
-    const double v61 = sum_parts(a, 115, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 426, q);
-    const double v67 = sum_parts(a, 116, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 427, q);
-    const double v72 = sum_parts(a, 117, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 405, q);
-    const double v78 = sum_parts(a, 95, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 414, q);
-    const double v84 = sum_parts(a, 104, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 428, q);
-    const double v93 = sum_parts(a, 118, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 406, q);
-    const double v99 = sum_parts(a, 96, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 415, q);
-    const double v104 = sum_parts(a, 105, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 416, q);
-    const double v109 = sum_parts(a, 106, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 429, q);
-    const double v114 = sum_parts(a, 119, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 430, q);
-    const double v120 = sum_parts(a, 120, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 431, q);
-    const double v125 = sum_parts(a, 121, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 407, q);
-    const double v131 = sum_parts(a, 97, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 417, q);
-    const double v137 = sum_parts(a, 107, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 432, q);
-    const double v146 = sum_parts(a, 122, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 408, q);
-    const double v152 = sum_parts(a, 98, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 418, q);
-    const double v158 = sum_parts(a, 108, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 433, q);
-    const double v167 = sum_parts(a, 123, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(4, 0);
-    const double v0 = sum_parts(a, 326, q);
-    const double v1 = sum_parts(a, 140, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 316, q);
-    const double v4 = sum_parts(a, 130, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 310, q);
-    const double v8 = sum_parts(a, 124, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 327, q);
-    const double v13 = sum_parts(a, 141, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 317, q);
-    const double v16 = sum_parts(a, 131, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 328, q);
-    const double v21 = sum_parts(a, 142, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 318, q);
-    const double v24 = sum_parts(a, 132, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 329, q);
-    const double v29 = sum_parts(a, 143, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 330, q);
-    const double v35 = sum_parts(a, 144, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 331, q);
-    const double v40 = sum_parts(a, 145, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 311, q);
-    const double v46 = sum_parts(a, 125, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 319, q);
-    const double v51 = sum_parts(a, 133, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 320, q);
-    const double v56 = sum_parts(a, 134, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 332, q);
-    const double v61 = sum_parts(a, 146, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 333, q);
-    const double v67 = sum_parts(a, 147, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 334, q);
-    const double v72 = sum_parts(a, 148, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 312, q);
-    const double v78 = sum_parts(a, 126, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 321, q);
-    const double v84 = sum_parts(a, 135, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 335, q);
-    const double v93 = sum_parts(a, 149, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 313, q);
-    const double v99 = sum_parts(a, 127, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 322, q);
-    const double v104 = sum_parts(a, 136, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 323, q);
-    const double v109 = sum_parts(a, 137, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 336, q);
-    const double v114 = sum_parts(a, 150, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 337, q);
-    const double v120 = sum_parts(a, 151, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 338, q);
-    const double v125 = sum_parts(a, 152, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 314, q);
-    const double v131 = sum_parts(a, 128, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 324, q);
-    const double v137 = sum_parts(a, 138, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 339, q);
-    const double v146 = sum_parts(a, 153, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 315, q);
-    const double v152 = sum_parts(a, 129, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 325, q);
-    const double v158 = sum_parts(a, 139, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 340, q);
-    const double v167 = sum_parts(a, 154, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(4, 1);
-    const double v0 = sum_parts(a, 419, q);
-    const double v1 = sum_parts(a, 140, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 409, q);
-    const double v4 = sum_parts(a, 130, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 403, q);
-    const double v8 = sum_parts(a, 124, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 420, q);
-    const double v13 = sum_parts(a, 141, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 410, q);
-    const double v16 = sum_parts(a, 131, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 421, q);
-    const double v21 = sum_parts(a, 142, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 411, q);
-    const double v24 = sum_parts(a, 132, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 422, q);
-    const double v29 = sum_parts(a, 143, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 423, q);
-    const double v35 = sum_parts(a, 144, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 424, q);
-    const double v40 = sum_parts(a, 145, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 404, q);
-    const double v46 = sum_parts(a, 125, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 412, q);
-    const double v51 = sum_parts(a, 133, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 413, q);
-    const double v56 = sum_parts(a, 134, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 425, q);
-    const double v61 = sum_parts(a, 146, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 426, q);
-    const double v67 = sum_parts(a, 147, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 427, q);
-    const double v72 = sum_parts(a, 148, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 405, q);
-    const double v78 = sum_parts(a, 126, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 414, q);
-    const double v84 = sum_parts(a, 135, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 428, q);
-    const double v93 = sum_parts(a, 149, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 406, q);
-    const double v99 = sum_parts(a, 127, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 415, q);
-    const double v104 = sum_parts(a, 136, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 416, q);
-    const double v109 = sum_parts(a, 137, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 429, q);
-    const double v114 = sum_parts(a, 150, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 430, q);
-    const double v120 = sum_parts(a, 151, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 431, q);
-    const double v125 = sum_parts(a, 152, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 407, q);
-    const double v131 = sum_parts(a, 128, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 417, q);
-    const double v137 = sum_parts(a, 138, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 432, q);
-    const double v146 = sum_parts(a, 153, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 408, q);
-    const double v152 = sum_parts(a, 129, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 418, q);
-    const double v158 = sum_parts(a, 139, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 433, q);
-    const double v167 = sum_parts(a, 154, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(4, 2);
-    const double v0 = sum_parts(a, 450, q);
-    const double v1 = sum_parts(a, 140, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 440, q);
-    const double v4 = sum_parts(a, 130, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 434, q);
-    const double v8 = sum_parts(a, 124, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.3333333333333334);
-    const double v12 = sum_parts(a, 451, q);
-    const double v13 = sum_parts(a, 141, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 441, q);
-    const double v16 = sum_parts(a, 131, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.5773502691896258);
-    const double v20 = sum_parts(a, 452, q);
-    const double v21 = sum_parts(a, 142, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 442, q);
-    const double v24 = sum_parts(a, 132, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.5773502691896258);
-    const double v28 = sum_parts(a, 453, q);
-    const double v29 = sum_parts(a, 143, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.3333333333333334);
-    const double v34 = sum_parts(a, 454, q);
-    const double v35 = sum_parts(a, 144, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.5773502691896258);
-    const double v39 = sum_parts(a, 455, q);
-    const double v40 = sum_parts(a, 145, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.3333333333333334);
-    const double v45 = sum_parts(a, 435, q);
-    const double v46 = sum_parts(a, 125, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.5773502691896258);
-    const double v50 = sum_parts(a, 443, q);
-    const double v51 = sum_parts(a, 133, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54);
-    const double v55 = sum_parts(a, 444, q);
-    const double v56 = sum_parts(a, 134, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59);
-    const double v60 = sum_parts(a, 456, q);
-    const double v61 = sum_parts(a, 146, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.5773502691896258);
-    const double v66 = sum_parts(a, 457, q);
-    const double v67 = sum_parts(a, 147, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70);
-    const double v71 = sum_parts(a, 458, q);
-    const double v72 = sum_parts(a, 148, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.5773502691896258);
-    const double v77 = sum_parts(a, 436, q);
-    const double v78 = sum_parts(a, 126, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.5773502691896258);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82);
-    const double v83 = sum_parts(a, 445, q);
-    const double v84 = sum_parts(a, 135, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.5773502691896258);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91);
-    const double v92 = sum_parts(a, 459, q);
-    const double v93 = sum_parts(a, 149, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.5773502691896258);
-    const double v98 = sum_parts(a, 437, q);
-    const double v99 = sum_parts(a, 127, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.3333333333333334);
-    const double v103 = sum_parts(a, 446, q);
-    const double v104 = sum_parts(a, 136, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.5773502691896258);
-    const double v108 = sum_parts(a, 447, q);
-    const double v109 = sum_parts(a, 137, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.5773502691896258);
-    const double v113 = sum_parts(a, 460, q);
-    const double v114 = sum_parts(a, 150, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.3333333333333334);
-    const double v119 = sum_parts(a, 461, q);
-    const double v120 = sum_parts(a, 151, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.5773502691896258);
-    const double v124 = sum_parts(a, 462, q);
-    const double v125 = sum_parts(a, 152, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.3333333333333334);
-    const double v130 = sum_parts(a, 438, q);
-    const double v131 = sum_parts(a, 128, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.5773502691896258);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135);
-    const double v136 = sum_parts(a, 448, q);
-    const double v137 = sum_parts(a, 138, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.5773502691896258);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144);
-    const double v145 = sum_parts(a, 463, q);
-    const double v146 = sum_parts(a, 153, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.5773502691896258);
-    const double v151 = sum_parts(a, 439, q);
-    const double v152 = sum_parts(a, 129, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.3333333333333334);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.5773502691896258);
-    const double v157 = sum_parts(a, 449, q);
-    const double v158 = sum_parts(a, 139, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.5773502691896258);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.3333333333333334);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.5773502691896258);
-    const double v166 = sum_parts(a, 464, q);
-    const double v167 = sum_parts(a, 154, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.3333333333333334);
-  }
-  {
-    const ColumnSlot cl = st.column(5, 0);
-    const double v0 = sum_parts(a, 357, q);
-    const double v1 = sum_parts(a, 171, q);
-    const double v2 = v0 + CD[0] * v1;
-    const double v3 = sum_parts(a, 347, q);
-    const double v4 = sum_parts(a, 161, q);
-    const double v5 = v3 + CD[0] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 341, q);
-    const double v8 = sum_parts(a, 155, q);
-    const double v9 = v7 + CD[0] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 358, q);
-    const double v13 = sum_parts(a, 172, q);
-    const double v14 = v12 + CD[0] * v13;
-    const double v15 = sum_parts(a, 348, q);
-    const double v16 = sum_parts(a, 162, q);
-    const double v17 = v15 + CD[0] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 359, q);
-    const double v21 = sum_parts(a, 173, q);
-    const double v22 = v20 + CD[0] * v21;
-    const double v23 = sum_parts(a, 349, q);
-    const double v24 = sum_parts(a, 163, q);
-    const double v25 = v23 + CD[0] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 360, q);
-    const double v29 = sum_parts(a, 174, q);
-    const double v30 = v28 + CD[0] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 361, q);
-    const double v35 = sum_parts(a, 175, q);
-    const double v36 = v34 + CD[0] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 362, q);
-    const double v40 = sum_parts(a, 176, q);
-    const double v41 = v39 + CD[0] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 342, q);
-    const double v46 = sum_parts(a, 156, q);
-    const double v47 = v45 + CD[0] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 350, q);
-    const double v51 = sum_parts(a, 164, q);
-    const double v52 = v50 + CD[0] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 351, q);
-    const double v56 = sum_parts(a, 165, q);
-    const double v57 = v55 + CD[0] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 363, q);
-    const double v61 = sum_parts(a, 177, q);
-    const double v62 = v60 + CD[0] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 364, q);
-    const double v67 = sum_parts(a, 178, q);
-    const double v68 = v66 + CD[0] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 365, q);
-    const double v72 = sum_parts(a, 179, q);
-    const double v73 = v71 + CD[0] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 343, q);
-    const double v78 = sum_parts(a, 157, q);
-    const double v79 = v77 + CD[0] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 352, q);
-    const double v84 = sum_parts(a, 166, q);
-    const double v85 = v83 + CD[0] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 366, q);
-    const double v93 = sum_parts(a, 180, q);
-    const double v94 = v92 + CD[0] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 344, q);
-    const double v99 = sum_parts(a, 158, q);
-    const double v100 = v98 + CD[0] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 353, q);
-    const double v104 = sum_parts(a, 167, q);
-    const double v105 = v103 + CD[0] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 354, q);
-    const double v109 = sum_parts(a, 168, q);
-    const double v110 = v108 + CD[0] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 367, q);
-    const double v114 = sum_parts(a, 181, q);
-    const double v115 = v113 + CD[0] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 368, q);
-    const double v120 = sum_parts(a, 182, q);
-    const double v121 = v119 + CD[0] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 369, q);
-    const double v125 = sum_parts(a, 183, q);
-    const double v126 = v124 + CD[0] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 345, q);
-    const double v131 = sum_parts(a, 159, q);
-    const double v132 = v130 + CD[0] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 355, q);
-    const double v137 = sum_parts(a, 169, q);
-    const double v138 = v136 + CD[0] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 370, q);
-    const double v146 = sum_parts(a, 184, q);
-    const double v147 = v145 + CD[0] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 346, q);
-    const double v152 = sum_parts(a, 160, q);
-    const double v153 = v151 + CD[0] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 356, q);
-    const double v158 = sum_parts(a, 170, q);
-    const double v159 = v157 + CD[0] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 371, q);
-    const double v167 = sum_parts(a, 185, q);
-    const double v168 = v166 + CD[0] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(5, 1);
-    const double v0 = sum_parts(a, 450, q);
-    const double v1 = sum_parts(a, 171, q);
-    const double v2 = v0 + CD[1] * v1;
-    const double v3 = sum_parts(a, 440, q);
-    const double v4 = sum_parts(a, 161, q);
-    const double v5 = v3 + CD[1] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 434, q);
-    const double v8 = sum_parts(a, 155, q);
-    const double v9 = v7 + CD[1] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 451, q);
-    const double v13 = sum_parts(a, 172, q);
-    const double v14 = v12 + CD[1] * v13;
-    const double v15 = sum_parts(a, 441, q);
-    const double v16 = sum_parts(a, 162, q);
-    const double v17 = v15 + CD[1] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 452, q);
-    const double v21 = sum_parts(a, 173, q);
-    const double v22 = v20 + CD[1] * v21;
-    const double v23 = sum_parts(a, 442, q);
-    const double v24 = sum_parts(a, 163, q);
-    const double v25 = v23 + CD[1] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 453, q);
-    const double v29 = sum_parts(a, 174, q);
-    const double v30 = v28 + CD[1] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 454, q);
-    const double v35 = sum_parts(a, 175, q);
-    const double v36 = v34 + CD[1] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 455, q);
-    const double v40 = sum_parts(a, 176, q);
-    const double v41 = v39 + CD[1] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 435, q);
-    const double v46 = sum_parts(a, 156, q);
-    const double v47 = v45 + CD[1] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 443, q);
-    const double v51 = sum_parts(a, 164, q);
-    const double v52 = v50 + CD[1] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 444, q);
-    const double v56 = sum_parts(a, 165, q);
-    const double v57 = v55 + CD[1] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 456, q);
-    const double v61 = sum_parts(a, 177, q);
-    const double v62 = v60 + CD[1] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 457, q);
-    const double v67 = sum_parts(a, 178, q);
-    const double v68 = v66 + CD[1] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 458, q);
-    const double v72 = sum_parts(a, 179, q);
-    const double v73 = v71 + CD[1] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 436, q);
-    const double v78 = sum_parts(a, 157, q);
-    const double v79 = v77 + CD[1] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 445, q);
-    const double v84 = sum_parts(a, 166, q);
-    const double v85 = v83 + CD[1] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 459, q);
-    const double v93 = sum_parts(a, 180, q);
-    const double v94 = v92 + CD[1] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 437, q);
-    const double v99 = sum_parts(a, 158, q);
-    const double v100 = v98 + CD[1] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 446, q);
-    const double v104 = sum_parts(a, 167, q);
-    const double v105 = v103 + CD[1] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 447, q);
-    const double v109 = sum_parts(a, 168, q);
-    const double v110 = v108 + CD[1] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 460, q);
-    const double v114 = sum_parts(a, 181, q);
-    const double v115 = v113 + CD[1] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 461, q);
-    const double v120 = sum_parts(a, 182, q);
-    const double v121 = v119 + CD[1] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 462, q);
-    const double v125 = sum_parts(a, 183, q);
-    const double v126 = v124 + CD[1] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 438, q);
-    const double v131 = sum_parts(a, 159, q);
-    const double v132 = v130 + CD[1] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 448, q);
-    const double v137 = sum_parts(a, 169, q);
-    const double v138 = v136 + CD[1] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 463, q);
-    const double v146 = sum_parts(a, 184, q);
-    const double v147 = v145 + CD[1] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 439, q);
-    const double v152 = sum_parts(a, 160, q);
-    const double v153 = v151 + CD[1] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 449, q);
-    const double v158 = sum_parts(a, 170, q);
-    const double v159 = v157 + CD[1] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 464, q);
-    const double v167 = sum_parts(a, 185, q);
-    const double v168 = v166 + CD[1] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-  {
-    const ColumnSlot cl = st.column(5, 2);
-    const double v0 = sum_parts(a, 481, q);
-    const double v1 = sum_parts(a, 171, q);
-    const double v2 = v0 + CD[2] * v1;
-    const double v3 = sum_parts(a, 471, q);
-    const double v4 = sum_parts(a, 161, q);
-    const double v5 = v3 + CD[2] * v4;
-    const double v6 = v2 + AB[0] * v5;
-    const double v7 = sum_parts(a, 465, q);
-    const double v8 = sum_parts(a, 155, q);
-    const double v9 = v7 + CD[2] * v8;
-    const double v10 = v5 + AB[0] * v9;
-    const double v11 = v6 + AB[0] * v10;
-    st.put(cl, 0, 0, v11 * 0.19245008972987535);
-    const double v12 = sum_parts(a, 482, q);
-    const double v13 = sum_parts(a, 172, q);
-    const double v14 = v12 + CD[2] * v13;
-    const double v15 = sum_parts(a, 472, q);
-    const double v16 = sum_parts(a, 162, q);
-    const double v17 = v15 + CD[2] * v16;
-    const double v18 = v14 + AB[0] * v17;
-    const double v19 = v18 + AB[1] * v10;
-    st.put(cl, 0, 1, v19 * 0.3333333333333334);
-    const double v20 = sum_parts(a, 483, q);
-    const double v21 = sum_parts(a, 173, q);
-    const double v22 = v20 + CD[2] * v21;
-    const double v23 = sum_parts(a, 473, q);
-    const double v24 = sum_parts(a, 163, q);
-    const double v25 = v23 + CD[2] * v24;
-    const double v26 = v22 + AB[0] * v25;
-    const double v27 = v26 + AB[2] * v10;
-    st.put(cl, 0, 2, v27 * 0.3333333333333334);
-    const double v28 = sum_parts(a, 484, q);
-    const double v29 = sum_parts(a, 174, q);
-    const double v30 = v28 + CD[2] * v29;
-    const double v31 = v30 + AB[1] * v17;
-    const double v32 = v17 + AB[1] * v9;
-    const double v33 = v31 + AB[1] * v32;
-    st.put(cl, 0, 3, v33 * 0.19245008972987535);
-    const double v34 = sum_parts(a, 485, q);
-    const double v35 = sum_parts(a, 175, q);
-    const double v36 = v34 + CD[2] * v35;
-    const double v37 = v36 + AB[1] * v25;
-    const double v38 = v37 + AB[2] * v32;
-    st.put(cl, 0, 4, v38 * 0.3333333333333334);
-    const double v39 = sum_parts(a, 486, q);
-    const double v40 = sum_parts(a, 176, q);
-    const double v41 = v39 + CD[2] * v40;
-    const double v42 = v41 + AB[2] * v25;
-    const double v43 = v25 + AB[2] * v9;
-    const double v44 = v42 + AB[2] * v43;
-    st.put(cl, 0, 5, v44 * 0.19245008972987535);
-    const double v45 = sum_parts(a, 466, q);
-    const double v46 = sum_parts(a, 156, q);
-    const double v47 = v45 + CD[2] * v46;
-    const double v48 = v17 + AB[0] * v47;
-    const double v49 = v18 + AB[0] * v48;
-    st.put(cl, 1, 0, v49 * 0.3333333333333334);
-    const double v50 = sum_parts(a, 474, q);
-    const double v51 = sum_parts(a, 164, q);
-    const double v52 = v50 + CD[2] * v51;
-    const double v53 = v30 + AB[0] * v52;
-    const double v54 = v53 + AB[1] * v48;
-    st.put(cl, 1, 1, v54 * 0.5773502691896258);
-    const double v55 = sum_parts(a, 475, q);
-    const double v56 = sum_parts(a, 165, q);
-    const double v57 = v55 + CD[2] * v56;
-    const double v58 = v36 + AB[0] * v57;
-    const double v59 = v58 + AB[2] * v48;
-    st.put(cl, 1, 2, v59 * 0.5773502691896258);
-    const double v60 = sum_parts(a, 487, q);
-    const double v61 = sum_parts(a, 177, q);
-    const double v62 = v60 + CD[2] * v61;
-    const double v63 = v62 + AB[1] * v52;
-    const double v64 = v52 + AB[1] * v47;
-    const double v65 = v63 + AB[1] * v64;
-    st.put(cl, 1, 3, v65 * 0.3333333333333334);
-    const double v66 = sum_parts(a, 488, q);
-    const double v67 = sum_parts(a, 178, q);
-    const double v68 = v66 + CD[2] * v67;
-    const double v69 = v68 + AB[1] * v57;
-    const double v70 = v69 + AB[2] * v64;
-    st.put(cl, 1, 4, v70 * 0.5773502691896258);
-    const double v71 = sum_parts(a, 489, q);
-    const double v72 = sum_parts(a, 179, q);
-    const double v73 = v71 + CD[2] * v72;
-    const double v74 = v73 + AB[2] * v57;
-    const double v75 = v57 + AB[2] * v47;
-    const double v76 = v74 + AB[2] * v75;
-    st.put(cl, 1, 5, v76 * 0.3333333333333334);
-    const double v77 = sum_parts(a, 467, q);
-    const double v78 = sum_parts(a, 157, q);
-    const double v79 = v77 + CD[2] * v78;
-    const double v80 = v25 + AB[0] * v79;
-    const double v81 = v26 + AB[0] * v80;
-    st.put(cl, 2, 0, v81 * 0.3333333333333334);
-    const double v82 = v58 + AB[1] * v80;
-    st.put(cl, 2, 1, v82 * 0.5773502691896258);
-    const double v83 = sum_parts(a, 476, q);
-    const double v84 = sum_parts(a, 166, q);
-    const double v85 = v83 + CD[2] * v84;
-    const double v86 = v41 + AB[0] * v85;
-    const double v87 = v86 + AB[2] * v80;
-    st.put(cl, 2, 2, v87 * 0.5773502691896258);
-    const double v88 = v57 + AB[1] * v79;
-    const double v89 = v69 + AB[1] * v88;
-    st.put(cl, 2, 3, v89 * 0.3333333333333334);
-    const double v90 = v73 + AB[1] * v85;
-    const double v91 = v90 + AB[2] * v88;
-    st.put(cl, 2, 4, v91 * 0.5773502691896258);
-    const double v92 = sum_parts(a, 490, q);
-    const double v93 = sum_parts(a, 180, q);
-    const double v94 = v92 + CD[2] * v93;
-    const double v95 = v94 + AB[2] * v85;
-    const double v96 = v85 + AB[2] * v79;
-    const double v97 = v95 + AB[2] * v96;
-    st.put(cl, 2, 5, v97 * 0.3333333333333334);
-    const double v98 = sum_parts(a, 468, q);
-    const double v99 = sum_parts(a, 158, q);
-    const double v100 = v98 + CD[2] * v99;
-    const double v101 = v52 + AB[0] * v100;
-    const double v102 = v53 + AB[0] * v101;
-    st.put(cl, 3, 0, v102 * 0.19245008972987535);
-    const double v103 = sum_parts(a, 477, q);
-    const double v104 = sum_parts(a, 167, q);
-    const double v105 = v103 + CD[2] * v104;
-    const double v106 = v62 + AB[0] * v105;
-    const double v107 = v106 + AB[1] * v101;
-    st.put(cl, 3, 1, v107 * 0.3333333333333334);
-    const double v108 = sum_parts(a, 478, q);
-    const double v109 = sum_parts(a, 168, q);
-    const double v110 = v108 + CD[2] * v109;
-    const double v111 = v68 + AB[0] * v110;
-    const double v112 = v111 + AB[2] * v101;
-    st.put(cl, 3, 2, v112 * 0.3333333333333334);
-    const double v113 = sum_parts(a, 491, q);
-    const double v114 = sum_parts(a, 181, q);
-    const double v115 = v113 + CD[2] * v114;
-    const double v116 = v115 + AB[1] * v105;
-    const double v117 = v105 + AB[1] * v100;
-    const double v118 = v116 + AB[1] * v117;
-    st.put(cl, 3, 3, v118 * 0.19245008972987535);
-    const double v119 = sum_parts(a, 492, q);
-    const double v120 = sum_parts(a, 182, q);
-    const double v121 = v119 + CD[2] * v120;
-    const double v122 = v121 + AB[1] * v110;
-    const double v123 = v122 + AB[2] * v117;
-    st.put(cl, 3, 4, v123 * 0.3333333333333334);
-    const double v124 = sum_parts(a, 493, q);
-    const double v125 = sum_parts(a, 183, q);
-    const double v126 = v124 + CD[2] * v125;
-    const double v127 = v126 + AB[2] * v110;
-    const double v128 = v110 + AB[2] * v100;
-    const double v129 = v127 + AB[2] * v128;
-    st.put(cl, 3, 5, v129 * 0.19245008972987535);
-    const double v130 = sum_parts(a, 469, q);
-    const double v131 = sum_parts(a, 159, q);
-    const double v132 = v130 + CD[2] * v131;
-    const double v133 = v57 + AB[0] * v132;
-    const double v134 = v58 + AB[0] * v133;
-    st.put(cl, 4, 0, v134 * 0.3333333333333334);
-    const double v135 = v111 + AB[1] * v133;
-    st.put(cl, 4, 1, v135 * 0.5773502691896258);
-    const double v136 = sum_parts(a, 479, q);
-    const double v137 = sum_parts(a, 169, q);
-    const double v138 = v136 + CD[2] * v137;
-    const double v139 = v73 + AB[0] * v138;
-    const double v140 = v139 + AB[2] * v133;
-    st.put(cl, 4, 2, v140 * 0.5773502691896258);
-    const double v141 = v110 + AB[1] * v132;
-    const double v142 = v122 + AB[1] * v141;
-    st.put(cl, 4, 3, v142 * 0.3333333333333334);
-    const double v143 = v126 + AB[1] * v138;
-    const double v144 = v143 + AB[2] * v141;
-    st.put(cl, 4, 4, v144 * 0.5773502691896258);
-    const double v145 = sum_parts(a, 494, q);
-    const double v146 = sum_parts(a, 184, q);
-    const double v147 = v145 + CD[2] * v146;
-    const double v148 = v147 + AB[2] * v138;
-    const double v149 = v138 + AB[2] * v132;
-    const double v150 = v148 + AB[2] * v149;
-    st.put(cl, 4, 5, v150 * 0.3333333333333334);
-    const double v151 = sum_parts(a, 470, q);
-    const double v152 = sum_parts(a, 160, q);
-    const double v153 = v151 + CD[2] * v152;
-    const double v154 = v85 + AB[0] * v153;
-    const double v155 = v86 + AB[0] * v154;
-    st.put(cl, 5, 0, v155 * 0.19245008972987535);
-    const double v156 = v139 + AB[1] * v154;
-    st.put(cl, 5, 1, v156 * 0.3333333333333334);
-    const double v157 = sum_parts(a, 480, q);
-    const double v158 = sum_parts(a, 170, q);
-    const double v159 = v157 + CD[2] * v158;
-    const double v160 = v94 + AB[0] * v159;
-    const double v161 = v160 + AB[2] * v154;
-    st.put(cl, 5, 2, v161 * 0.3333333333333334);
-    const double v162 = v138 + AB[1] * v153;
-    const double v163 = v143 + AB[1] * v162;
-    st.put(cl, 5, 3, v163 * 0.19245008972987535);
-    const double v164 = v147 + AB[1] * v159;
-    const double v165 = v164 + AB[2] * v162;
-    st.put(cl, 5, 4, v165 * 0.3333333333333334);
-    const double v166 = sum_parts(a, 495, q);
-    const double v167 = sum_parts(a, 185, q);
-    const double v168 = v166 + CD[2] * v167;
-    const double v169 = v168 + AB[2] * v159;
-    const double v170 = v159 + AB[2] * v153;
-    const double v171 = v169 + AB[2] * v170;
-    st.put(cl, 5, 5, v171 * 0.19245008972987535);
-  }
-}
-
-// class (22|22) split 0/12: 231 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_0(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc0 = 0.0;
-  double acc31 = 0.0;
-  double acc62 = 0.0;
-  double acc93 = 0.0;
-  double acc124 = 0.0;
-  double acc155 = 0.0;
-  double acc186 = 0.0;
-  double acc217 = 0.0;
-  double acc248 = 0.0;
-  double acc279 = 0.0;
-  double acc310 = 0.0;
-  double acc341 = 0.0;
-  double acc372 = 0.0;
-  double acc403 = 0.0;
-  double acc434 = 0.0;
-  double acc465 = 0.0;
-  double acc496 = 0.0;
-  double acc527 = 0.0;
-  double acc558 = 0.0;
-  double acc589 = 0.0;
-  double acc620 = 0.0;
-  double acc651 = 0.0;
-  double acc682 = 0.0;
-  double acc713 = 0.0;
-  double acc744 = 0.0;
-  double acc775 = 0.0;
-  double acc806 = 0.0;
-  double acc837 = 0.0;
-  double acc868 = 0.0;
-  double acc899 = 0.0;
-  double acc930 = 0.0;
-  double acc1 = 0.0;
-  double acc32 = 0.0;
-  double acc63 = 0.0;
-  double acc94 = 0.0;
-  double acc125 = 0.0;
-  double acc156 = 0.0;
-  double acc187 = 0.0;
-  double acc218 = 0.0;
-  double acc249 = 0.0;
-  double acc280 = 0.0;
-  double acc311 = 0.0;
-  double acc342 = 0.0;
-  double acc373 = 0.0;
-  double acc404 = 0.0;
-  double acc435 = 0.0;
-  double acc466 = 0.0;
-  double acc497 = 0.0;
-  double acc528 = 0.0;
-  double acc559 = 0.0;
-  double acc590 = 0.0;
-  double acc621 = 0.0;
-  double acc652 = 0.0;
-  double acc683 = 0.0;
-  double acc714 = 0.0;
-  double acc745 = 0.0;
-  double acc776 = 0.0;
-  double acc807 = 0.0;
-  double acc838 = 0.0;
-  double acc869 = 0.0;
-  double acc900 = 0.0;
-  double acc931 = 0.0;
-  double acc2 = 0.0;
-  double acc33 = 0.0;
-  double acc64 = 0.0;
-  double acc95 = 0.0;
-  double acc126 = 0.0;
-  double acc157 = 0.0;
-  double acc188 = 0.0;
-  double acc219 = 0.0;
-  double acc250 = 0.0;
-  double acc281 = 0.0;
-  double acc312 = 0.0;
-  double acc343 = 0.0;
-  double acc374 = 0.0;
-  double acc405 = 0.0;
-  double acc436 = 0.0;
-  double acc467 = 0.0;
-  double acc498 = 0.0;
-  double acc529 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x33 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x39 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x57 = s.PA[0] * x21 + s.WP[0] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x34 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x35 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x40 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x41 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x45 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x51 = s.PA[0] * x18 + s.WP[0] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x210 = s.QC[0] * x2 + s.WQ[0] * x3;
-      const TB x211 = s.QC[0] * x3 + s.WQ[0] * x4;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x212 = s.QC[0] * x12 + s.WQ[0] * x15 + r1 * x2;
-      const TB x615 = s.QC[0] * x210 + s.WQ[0] * x211 + e1 * (x2 - s.roe * x3);
-      const TB x213 = s.QC[0] * x15 + s.WQ[0] * x18 + r1 * x3;
-      const TB x616 = s.QC[0] * x212 + s.WQ[0] * x213 + e1 * (x12 - s.roe * x15) + r1 * x210;
-      const TB x617 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x15 - s.roe * x18) + r1 * x211;
-      const TB x1215 = s.QC[0] * x616 + s.WQ[0] * x617 + e2 * (x212 - s.roe * x213) + r1 * x615;
-      const TB x215 = s.QC[0] * x13 + s.WQ[0] * x16;
-      const TB x216 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x218 = s.QC[0] * x14 + s.WQ[0] * x17;
-      const TB x219 = s.QC[0] * x17 + s.WQ[0] * x20;
-      const TB x221 = s.QC[0] * x33 + s.WQ[0] * x39 + r2 * x12;
-      const TB x222 = s.QC[0] * x39 + s.WQ[0] * x45 + r2 * x15;
-      const TB x223 = s.QC[0] * x45 + s.WQ[0] * x51 + r2 * x18;
-      const TB x224 = s.QC[0] * x51 + s.WQ[0] * x57 + r2 * x21;
-      const TB x225 = s.QC[0] * x34 + s.WQ[0] * x40 + r1 * x13;
-      const TB x618 = s.QC[0] * x215 + s.WQ[0] * x216 + e1 * (x13 - s.roe * x16);
-      const TB x226 = s.QC[0] * x40 + s.WQ[0] * x46 + r1 * x16;
-      const TB x619 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x16 - s.roe * x19);
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x229 = s.QC[0] * x35 + s.WQ[0] * x41 + r1 * x14;
-      const TB x620 = s.QC[0] * x218 + s.WQ[0] * x219 + e1 * (x14 - s.roe * x17);
-      const TB x230 = s.QC[0] * x41 + s.WQ[0] * x47 + r1 * x17;
-      const TB x621 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x17 - s.roe * x20);
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x630 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x47 - s.roe * x53) + r1 * x220;
-      const TB x347 = s.QC[1] * x12 + s.WQ[1] * x15;
-      const TB x348 = s.QC[1] * x15 + s.WQ[1] * x18;
-      const TB x349 = s.QC[1] * x18 + s.WQ[1] * x21;
-      const TB x363 = s.QC[1] * x52 + s.WQ[1] * x58 + r1 * x21;
-      const TB x498 = s.QC[2] * x52 + s.WQ[2] * x58;
-      const TB x356 = s.QC[1] * x33 + s.WQ[1] * x39;
-      const TB x357 = s.QC[1] * x39 + s.WQ[1] * x45;
-      const TB x358 = s.QC[1] * x45 + s.WQ[1] * x51;
-      const TB x359 = s.QC[1] * x51 + s.WQ[1] * x57;
-      const TB x494 = s.QC[2] * x51 + s.WQ[2] * x57;
-      const TB x360 = s.QC[1] * x34 + s.WQ[1] * x40 + r1 * x12;
-      const TB x361 = s.QC[1] * x40 + s.WQ[1] * x46 + r1 * x15;
-      const TB x362 = s.QC[1] * x46 + s.WQ[1] * x52 + r1 * x18;
-      const TB x364 = s.QC[1] * x35 + s.WQ[1] * x41;
-      const TB x365 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x366 = s.QC[1] * x47 + s.WQ[1] * x53;
-      const TB x501 = s.QC[2] * x47 + s.WQ[2] * x53 + r1 * x18;
-      const TB x482 = s.QC[2] * x12 + s.WQ[2] * x15;
-      const TB x483 = s.QC[2] * x15 + s.WQ[2] * x18;
-      const TB x917 = s.QC[1] * x348 + s.WQ[1] * x349 + e1 * (x15 - s.roe * x18);
-      const TB x927 = s.QC[1] * x362 + s.WQ[1] * x363 + e1 * (x46 - s.roe * x52) + r1 * x349;
-      const TB x1027 = s.QC[2] * x362 + s.WQ[2] * x363;
-      const TB x491 = s.QC[2] * x33 + s.WQ[2] * x39;
-      const TB x492 = s.QC[2] * x39 + s.WQ[2] * x45;
-      const TB x493 = s.QC[2] * x45 + s.WQ[2] * x51;
-      const TB x1124 = s.QC[2] * x493 + s.WQ[2] * x494 + e1 * (x45 - s.roe * x51);
-      const TB x495 = s.QC[2] * x34 + s.WQ[2] * x40;
-      const TB x496 = s.QC[2] * x40 + s.WQ[2] * x46;
-      const TB x497 = s.QC[2] * x46 + s.WQ[2] * x52;
-      const TB x1127 = s.QC[2] * x497 + s.WQ[2] * x498 + e1 * (x46 - s.roe * x52);
-      const TB x499 = s.QC[2] * x35 + s.WQ[2] * x41 + r1 * x12;
-      const TB x500 = s.QC[2] * x41 + s.WQ[2] * x47 + r1 * x15;
-      const TB x1129 = s.QC[2] * x500 + s.WQ[2] * x501 + e1 * (x41 - s.roe * x47) + r1 * x483;
-      const TB x1116 = s.QC[2] * x482 + s.WQ[2] * x483 + e1 * (x12 - s.roe * x15);
-      const TB x916 = s.QC[1] * x347 + s.WQ[1] * x348 + e1 * (x12 - s.roe * x15);
-      const TB x1128 = s.QC[2] * x499 + s.WQ[2] * x500 + e1 * (x35 - s.roe * x41) + r1 * x482;
-      acc157 += static_cast<double>(x1128);
-      const TB x1807 = s.QC[2] * x1128 + s.WQ[2] * x1129 + e2 * (x499 - s.roe * x500) + r1 * x1116;
-      acc467 += static_cast<double>(x1807);
-      const TB x622 = s.QC[0] * x221 + s.WQ[0] * x222 + e1 * (x33 - s.roe * x39) + r2 * x212;
-      acc0 += static_cast<double>(x622);
-      const TB x623 = s.QC[0] * x222 + s.WQ[0] * x223 + e1 * (x39 - s.roe * x45) + r2 * x213;
-      const TB x624 = s.QC[0] * x223 + s.WQ[0] * x224 + e1 * (x45 - s.roe * x51) + r2 * x214;
-      const TB x924 = s.QC[1] * x358 + s.WQ[1] * x359 + e1 * (x45 - s.roe * x51);
-      const TB x1024 = s.QC[2] * x358 + s.WQ[2] * x359;
-      const TB x625 = s.QC[0] * x225 + s.WQ[0] * x226 + e1 * (x34 - s.roe * x40) + r1 * x215;
-      acc1 += static_cast<double>(x625);
-      const TB x1216 = s.QC[0] * x618 + s.WQ[0] * x619 + e2 * (x215 - s.roe * x216);
-      const TB x626 = s.QC[0] * x226 + s.WQ[0] * x227 + e1 * (x40 - s.roe * x46) + r1 * x216;
-      const TB x1220 = s.QC[0] * x625 + s.WQ[0] * x626 + e2 * (x225 - s.roe * x226) + r1 * x618;
-      acc187 += static_cast<double>(x1220);
-      const TB x1221 = s.QC[0] * x626 + s.WQ[0] * x627 + e2 * (x226 - s.roe * x227) + r1 * x619;
-      const TB x1866 = s.QC[0] * x1220 + s.WQ[0] * x1221 + e3 * (x625 - s.roe * x626) + r1 * x1216;
-      acc497 += static_cast<double>(x1866);
-      const TB x1897 = s.QC[1] * x1220 + s.WQ[1] * x1221 + r1 * x1215;
-      acc528 += static_cast<double>(x1897);
-      const TB x1928 = s.QC[2] * x1220 + s.WQ[2] * x1221;
-      acc559 += static_cast<double>(x1928);
-      const TB x628 = s.QC[0] * x229 + s.WQ[0] * x230 + e1 * (x35 - s.roe * x41) + r1 * x218;
-      acc2 += static_cast<double>(x628);
-      const TB x928 = s.QC[1] * x364 + s.WQ[1] * x365 + e1 * (x35 - s.roe * x41);
-      acc95 += static_cast<double>(x928);
-      const TB x1217 = s.QC[0] * x620 + s.WQ[0] * x621 + e2 * (x218 - s.roe * x219);
-      const TB x629 = s.QC[0] * x230 + s.WQ[0] * x231 + e1 * (x41 - s.roe * x47) + r1 * x219;
-      const TB x929 = s.QC[1] * x365 + s.WQ[1] * x366 + e1 * (x41 - s.roe * x47);
-      const TB x1223 = s.QC[0] * x629 + s.WQ[0] * x630 + e2 * (x230 - s.roe * x231) + r1 * x621;
-      const TB x1029 = s.QC[2] * x365 + s.WQ[2] * x366 + r1 * x348;
-      const TB x1222 = s.QC[0] * x628 + s.WQ[0] * x629 + e2 * (x229 - s.roe * x230) + r1 * x620;
-      acc188 += static_cast<double>(x1222);
-      const TB x1867 = s.QC[0] * x1222 + s.WQ[0] * x1223 + e3 * (x628 - s.roe * x629) + r1 * x1217;
-      acc498 += static_cast<double>(x1867);
-      const TB x1898 = s.QC[1] * x1222 + s.WQ[1] * x1223;
-      acc529 += static_cast<double>(x1898);
-      const TB x1287 = s.QC[1] * x628 + s.WQ[1] * x629;
-      acc219 += static_cast<double>(x1287);
-      const TB x1352 = s.QC[2] * x628 + s.WQ[2] * x629 + r1 * x616;
-      acc250 += static_cast<double>(x1352);
-      const TB x1612 = s.QC[1] * x928 + s.WQ[1] * x929 + e2 * (x364 - s.roe * x365);
-      acc374 += static_cast<double>(x1612);
-      const TB x1677 = s.QC[2] * x928 + s.WQ[2] * x929 + r1 * x916;
-      acc405 += static_cast<double>(x1677);
-      const TB x716 = s.QC[1] * x212 + s.WQ[1] * x213;
-      const TB x717 = s.QC[1] * x213 + s.WQ[1] * x214;
-      const TB x727 = s.QC[1] * x227 + s.WQ[1] * x228 + r1 * x214;
-      const TB x827 = s.QC[2] * x227 + s.WQ[2] * x228;
-      const TB x722 = s.QC[1] * x221 + s.WQ[1] * x222;
-      acc31 += static_cast<double>(x722);
-      const TB x723 = s.QC[1] * x222 + s.WQ[1] * x223;
-      const TB x724 = s.QC[1] * x223 + s.WQ[1] * x224;
-      const TB x824 = s.QC[2] * x223 + s.WQ[2] * x224;
-      const TB x725 = s.QC[1] * x225 + s.WQ[1] * x226 + r1 * x212;
-      acc32 += static_cast<double>(x725);
-      const TB x726 = s.QC[1] * x226 + s.WQ[1] * x227 + r1 * x213;
-      const TB x728 = s.QC[1] * x229 + s.WQ[1] * x230;
-      acc33 += static_cast<double>(x728);
-      const TB x729 = s.QC[1] * x230 + s.WQ[1] * x231;
-      const TB x829 = s.QC[2] * x230 + s.WQ[2] * x231 + r1 * x213;
-      const TB x1417 = s.QC[1] * x728 + s.WQ[1] * x729 + e1 * (x229 - s.roe * x230);
-      acc281 += static_cast<double>(x1417);
-      const TB x1482 = s.QC[2] * x728 + s.WQ[2] * x729 + r1 * x716;
-      acc312 += static_cast<double>(x1482);
-      const TB x816 = s.QC[2] * x212 + s.WQ[2] * x213;
-      const TB x1410 = s.QC[1] * x716 + s.WQ[1] * x717 + e1 * (x212 - s.roe * x213);
-      const TB x828 = s.QC[2] * x229 + s.WQ[2] * x230 + r1 * x212;
-      acc64 += static_cast<double>(x828);
-      const TB x1547 = s.QC[2] * x828 + s.WQ[2] * x829 + e1 * (x229 - s.roe * x230) + r1 * x816;
-      acc343 += static_cast<double>(x1547);
-      const TB x1415 = s.QC[1] * x725 + s.WQ[1] * x726 + e1 * (x225 - s.roe * x226) + r1 * x716;
-      acc280 += static_cast<double>(x1415);
-      const TB x1416 = s.QC[1] * x726 + s.WQ[1] * x727 + e1 * (x226 - s.roe * x227) + r1 * x717;
-      const TB x2052 = s.QC[1] * x1415 + s.WQ[1] * x1416 + e2 * (x725 - s.roe * x726) + r1 * x1410;
-      acc683 += static_cast<double>(x2052);
-      const TB x2083 = s.QC[2] * x1415 + s.WQ[2] * x1416;
-      acc714 += static_cast<double>(x2083);
-      const TB x1481 = s.QC[2] * x726 + s.WQ[2] * x727;
-      const TB x822 = s.QC[2] * x221 + s.WQ[2] * x222;
-      acc62 += static_cast<double>(x822);
-      const TB x823 = s.QC[2] * x222 + s.WQ[2] * x223;
-      const TB x1544 = s.QC[2] * x823 + s.WQ[2] * x824 + e1 * (x222 - s.roe * x223);
-      const TB x825 = s.QC[2] * x225 + s.WQ[2] * x226;
-      acc63 += static_cast<double>(x825);
-      const TB x826 = s.QC[2] * x226 + s.WQ[2] * x227;
-      const TB x1546 = s.QC[2] * x826 + s.WQ[2] * x827 + e1 * (x226 - s.roe * x227);
-      const TB x1545 = s.QC[2] * x825 + s.WQ[2] * x826 + e1 * (x225 - s.roe * x226);
-      acc342 += static_cast<double>(x1545);
-      const TB x2145 = s.QC[2] * x1545 + s.WQ[2] * x1546 + e2 * (x825 - s.roe * x826);
-      acc776 += static_cast<double>(x2145);
-      const TB x922 = s.QC[1] * x356 + s.WQ[1] * x357 + e1 * (x33 - s.roe * x39);
-      acc93 += static_cast<double>(x922);
-      const TB x1122 = s.QC[2] * x491 + s.WQ[2] * x492 + e1 * (x33 - s.roe * x39);
-      acc155 += static_cast<double>(x1122);
-      const TB x923 = s.QC[1] * x357 + s.WQ[1] * x358 + e1 * (x39 - s.roe * x45);
-      const TB x1123 = s.QC[2] * x492 + s.WQ[2] * x493 + e1 * (x39 - s.roe * x45);
-      const TB x1804 = s.QC[2] * x1123 + s.WQ[2] * x1124 + e2 * (x492 - s.roe * x493);
-      const TB x1803 = s.QC[2] * x1122 + s.WQ[2] * x1123 + e2 * (x491 - s.roe * x492);
-      acc465 += static_cast<double>(x1803);
-      const TB x2299 = s.QC[2] * x1803 + s.WQ[2] * x1804 + e3 * (x1122 - s.roe * x1123);
-      acc930 += static_cast<double>(x2299);
-      const TB x925 = s.QC[1] * x360 + s.WQ[1] * x361 + e1 * (x34 - s.roe * x40) + r1 * x347;
-      acc94 += static_cast<double>(x925);
-      const TB x1125 = s.QC[2] * x495 + s.WQ[2] * x496 + e1 * (x34 - s.roe * x40);
-      acc156 += static_cast<double>(x1125);
-      const TB x926 = s.QC[1] * x361 + s.WQ[1] * x362 + e1 * (x40 - s.roe * x46) + r1 * x348;
-      const TB x1126 = s.QC[2] * x496 + s.WQ[2] * x497 + e1 * (x40 - s.roe * x46);
-      const TB x1806 = s.QC[2] * x1126 + s.WQ[2] * x1127 + e2 * (x496 - s.roe * x497);
-      const TB x1805 = s.QC[2] * x1125 + s.WQ[2] * x1126 + e2 * (x495 - s.roe * x496);
-      acc466 += static_cast<double>(x1805);
-      const TB x2300 = s.QC[2] * x1805 + s.WQ[2] * x1806 + e3 * (x1125 - s.roe * x1126);
-      acc931 += static_cast<double>(x2300);
-      const TB x1016 = s.QC[2] * x347 + s.WQ[2] * x348;
-      const TB x1605 = s.QC[1] * x916 + s.WQ[1] * x917 + e2 * (x347 - s.roe * x348);
-      const TB x1028 = s.QC[2] * x364 + s.WQ[2] * x365 + r1 * x347;
-      acc126 += static_cast<double>(x1028);
-      const TB x1742 = s.QC[2] * x1028 + s.WQ[2] * x1029 + e1 * (x364 - s.roe * x365) + r1 * x1016;
-      acc436 += static_cast<double>(x1742);
-      const TB x1610 = s.QC[1] * x925 + s.WQ[1] * x926 + e2 * (x360 - s.roe * x361) + r1 * x916;
-      acc373 += static_cast<double>(x1610);
-      const TB x1611 = s.QC[1] * x926 + s.WQ[1] * x927 + e2 * (x361 - s.roe * x362) + r1 * x917;
-      const TB x2176 = s.QC[1] * x1610 + s.WQ[1] * x1611 + e3 * (x925 - s.roe * x926) + r1 * x1605;
-      acc807 += static_cast<double>(x2176);
-      const TB x2207 = s.QC[2] * x1610 + s.WQ[2] * x1611;
-      acc838 += static_cast<double>(x2207);
-      const TB x1676 = s.QC[2] * x926 + s.WQ[2] * x927;
-      const TB x1022 = s.QC[2] * x356 + s.WQ[2] * x357;
-      acc124 += static_cast<double>(x1022);
-      const TB x1023 = s.QC[2] * x357 + s.WQ[2] * x358;
-      const TB x1739 = s.QC[2] * x1023 + s.WQ[2] * x1024 + e1 * (x357 - s.roe * x358);
-      const TB x1609 = s.QC[1] * x923 + s.WQ[1] * x924 + e2 * (x357 - s.roe * x358);
-      const TB x1674 = s.QC[2] * x923 + s.WQ[2] * x924;
-      const TB x1025 = s.QC[2] * x360 + s.WQ[2] * x361;
-      acc125 += static_cast<double>(x1025);
-      const TB x1026 = s.QC[2] * x361 + s.WQ[2] * x362;
-      const TB x1741 = s.QC[2] * x1026 + s.WQ[2] * x1027 + e1 * (x361 - s.roe * x362);
-      const TB x1740 = s.QC[2] * x1025 + s.WQ[2] * x1026 + e1 * (x360 - s.roe * x361);
-      acc435 += static_cast<double>(x1740);
-      const TB x2269 = s.QC[2] * x1740 + s.WQ[2] * x1741 + e2 * (x1025 - s.roe * x1026);
-      acc900 += static_cast<double>(x2269);
-      const TB x1218 = s.QC[0] * x622 + s.WQ[0] * x623 + e2 * (x221 - s.roe * x222) + r2 * x616;
-      acc186 += static_cast<double>(x1218);
-      const TB x1219 = s.QC[0] * x623 + s.WQ[0] * x624 + e2 * (x222 - s.roe * x223) + r2 * x617;
-      const TB x1865 = s.QC[0] * x1218 + s.WQ[0] * x1219 + e3 * (x622 - s.roe * x623) + r2 * x1215;
-      acc496 += static_cast<double>(x1865);
-      const TB x1414 = s.QC[1] * x723 + s.WQ[1] * x724 + e1 * (x222 - s.roe * x223);
-      const TB x1479 = s.QC[2] * x723 + s.WQ[2] * x724;
-      const TB x1896 = s.QC[1] * x1218 + s.WQ[1] * x1219;
-      acc527 += static_cast<double>(x1896);
-      const TB x1927 = s.QC[2] * x1218 + s.WQ[2] * x1219;
-      acc558 += static_cast<double>(x1927);
-      const TB x1280 = s.QC[1] * x616 + s.WQ[1] * x617;
-      const TB x1285 = s.QC[1] * x625 + s.WQ[1] * x626 + r1 * x616;
-      acc218 += static_cast<double>(x1285);
-      const TB x1286 = s.QC[1] * x626 + s.WQ[1] * x627 + r1 * x617;
-      const TB x1959 = s.QC[1] * x1285 + s.WQ[1] * x1286 + e1 * (x625 - s.roe * x626) + r1 * x1280;
-      acc590 += static_cast<double>(x1959);
-      const TB x1990 = s.QC[2] * x1285 + s.WQ[2] * x1286;
-      acc621 += static_cast<double>(x1990);
-      const TB x1351 = s.QC[2] * x626 + s.WQ[2] * x627;
-      const TB x1283 = s.QC[1] * x622 + s.WQ[1] * x623;
-      acc217 += static_cast<double>(x1283);
-      const TB x1284 = s.QC[1] * x623 + s.WQ[1] * x624;
-      const TB x1349 = s.QC[2] * x623 + s.WQ[2] * x624;
-      const TB x1958 = s.QC[1] * x1283 + s.WQ[1] * x1284 + e1 * (x622 - s.roe * x623);
-      acc589 += static_cast<double>(x1958);
-      const TB x1989 = s.QC[2] * x1283 + s.WQ[2] * x1284;
-      acc620 += static_cast<double>(x1989);
-      const TB x1348 = s.QC[2] * x622 + s.WQ[2] * x623;
-      acc248 += static_cast<double>(x1348);
-      const TB x2020 = s.QC[2] * x1348 + s.WQ[2] * x1349 + e1 * (x622 - s.roe * x623);
-      acc651 += static_cast<double>(x2020);
-      const TB x1350 = s.QC[2] * x625 + s.WQ[2] * x626;
-      acc249 += static_cast<double>(x1350);
-      const TB x2021 = s.QC[2] * x1350 + s.WQ[2] * x1351 + e1 * (x625 - s.roe * x626);
-      acc652 += static_cast<double>(x2021);
-      const TB x1413 = s.QC[1] * x722 + s.WQ[1] * x723 + e1 * (x221 - s.roe * x222);
-      acc279 += static_cast<double>(x1413);
-      const TB x1543 = s.QC[2] * x822 + s.WQ[2] * x823 + e1 * (x221 - s.roe * x222);
-      acc341 += static_cast<double>(x1543);
-      const TB x2144 = s.QC[2] * x1543 + s.WQ[2] * x1544 + e2 * (x822 - s.roe * x823);
-      acc775 += static_cast<double>(x2144);
-      const TB x2051 = s.QC[1] * x1413 + s.WQ[1] * x1414 + e2 * (x722 - s.roe * x723);
-      acc682 += static_cast<double>(x2051);
-      const TB x2082 = s.QC[2] * x1413 + s.WQ[2] * x1414;
-      acc713 += static_cast<double>(x2082);
-      const TB x1478 = s.QC[2] * x722 + s.WQ[2] * x723;
-      acc310 += static_cast<double>(x1478);
-      const TB x2113 = s.QC[2] * x1478 + s.WQ[2] * x1479 + e1 * (x722 - s.roe * x723);
-      acc744 += static_cast<double>(x2113);
-      const TB x1480 = s.QC[2] * x725 + s.WQ[2] * x726;
-      acc311 += static_cast<double>(x1480);
-      const TB x2114 = s.QC[2] * x1480 + s.WQ[2] * x1481 + e1 * (x725 - s.roe * x726);
-      acc745 += static_cast<double>(x2114);
-      const TB x1608 = s.QC[1] * x922 + s.WQ[1] * x923 + e2 * (x356 - s.roe * x357);
-      acc372 += static_cast<double>(x1608);
-      const TB x1738 = s.QC[2] * x1022 + s.WQ[2] * x1023 + e1 * (x356 - s.roe * x357);
-      acc434 += static_cast<double>(x1738);
-      const TB x2268 = s.QC[2] * x1738 + s.WQ[2] * x1739 + e2 * (x1022 - s.roe * x1023);
-      acc899 += static_cast<double>(x2268);
-      const TB x2175 = s.QC[1] * x1608 + s.WQ[1] * x1609 + e3 * (x922 - s.roe * x923);
-      acc806 += static_cast<double>(x2175);
-      const TB x2206 = s.QC[2] * x1608 + s.WQ[2] * x1609;
-      acc837 += static_cast<double>(x2206);
-      const TB x1673 = s.QC[2] * x922 + s.WQ[2] * x923;
-      acc403 += static_cast<double>(x1673);
-      const TB x2237 = s.QC[2] * x1673 + s.WQ[2] * x1674 + e1 * (x922 - s.roe * x923);
-      acc868 += static_cast<double>(x2237);
-      const TB x1675 = s.QC[2] * x925 + s.WQ[2] * x926;
-      acc404 += static_cast<double>(x1675);
-      const TB x2238 = s.QC[2] * x1675 + s.WQ[2] * x1676 + e1 * (x925 - s.roe * x926);
-      acc869 += static_cast<double>(x2238);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(0 * R + r) * a.acc_stride + q] = acc0;
-  a.acc[(31 * R + r) * a.acc_stride + q] = acc31;
-  a.acc[(62 * R + r) * a.acc_stride + q] = acc62;
-  a.acc[(93 * R + r) * a.acc_stride + q] = acc93;
-  a.acc[(124 * R + r) * a.acc_stride + q] = acc124;
-  a.acc[(155 * R + r) * a.acc_stride + q] = acc155;
-  a.acc[(186 * R + r) * a.acc_stride + q] = acc186;
-  a.acc[(217 * R + r) * a.acc_stride + q] = acc217;
-  a.acc[(248 * R + r) * a.acc_stride + q] = acc248;
-  a.acc[(279 * R + r) * a.acc_stride + q] = acc279;
-  a.acc[(310 * R + r) * a.acc_stride + q] = acc310;
-  a.acc[(341 * R + r) * a.acc_stride + q] = acc341;
-  a.acc[(372 * R + r) * a.acc_stride + q] = acc372;
-  a.acc[(403 * R + r) * a.acc_stride + q] = acc403;
-  a.acc[(434 * R + r) * a.acc_stride + q] = acc434;
-  a.acc[(465 * R + r) * a.acc_stride + q] = acc465;
-  a.acc[(496 * R + r) * a.acc_stride + q] = acc496;
-  a.acc[(527 * R + r) * a.acc_stride + q] = acc527;
-  a.acc[(558 * R + r) * a.acc_stride + q] = acc558;
-  a.acc[(589 * R + r) * a.acc_stride + q] = acc589;
-  a.acc[(620 * R + r) * a.acc_stride + q] = acc620;
-  a.acc[(651 * R + r) * a.acc_stride + q] = acc651;
-  a.acc[(682 * R + r) * a.acc_stride + q] = acc682;
-  a.acc[(713 * R + r) * a.acc_stride + q] = acc713;
-  a.acc[(744 * R + r) * a.acc_stride + q] = acc744;
-  a.acc[(775 * R + r) * a.acc_stride + q] = acc775;
-  a.acc[(806 * R + r) * a.acc_stride + q] = acc806;
-  a.acc[(837 * R + r) * a.acc_stride + q] = acc837;
-  a.acc[(868 * R + r) * a.acc_stride + q] = acc868;
-  a.acc[(899 * R + r) * a.acc_stride + q] = acc899;
-  a.acc[(930 * R + r) * a.acc_stride + q] = acc930;
-  a.acc[(1 * R + r) * a.acc_stride + q] = acc1;
-  a.acc[(32 * R + r) * a.acc_stride + q] = acc32;
-  a.acc[(63 * R + r) * a.acc_stride + q] = acc63;
-  a.acc[(94 * R + r) * a.acc_stride + q] = acc94;
-  a.acc[(125 * R + r) * a.acc_stride + q] = acc125;
-  a.acc[(156 * R + r) * a.acc_stride + q] = acc156;
-  a.acc[(187 * R + r) * a.acc_stride + q] = acc187;
-  a.acc[(218 * R + r) * a.acc_stride + q] = acc218;
-  a.acc[(249 * R + r) * a.acc_stride + q] = acc249;
-  a.acc[(280 * R + r) * a.acc_stride + q] = acc280;
-  a.acc[(311 * R + r) * a.acc_stride + q] = acc311;
-  a.acc[(342 * R + r) * a.acc_stride + q] = acc342;
-  a.acc[(373 * R + r) * a.acc_stride + q] = acc373;
-  a.acc[(404 * R + r) * a.acc_stride + q] = acc404;
-  a.acc[(435 * R + r) * a.acc_stride + q] = acc435;
-  a.acc[(466 * R + r) * a.acc_stride + q] = acc466;
-  a.acc[(497 * R + r) * a.acc_stride + q] = acc497;
-  a.acc[(528 * R + r) * a.acc_stride + q] = acc528;
-  a.acc[(559 * R + r) * a.acc_stride + q] = acc559;
-  a.acc[(590 * R + r) * a.acc_stride + q] = acc590;
-  a.acc[(621 * R + r) * a.acc_stride + q] = acc621;
-  a.acc[(652 * R + r) * a.acc_stride + q] = acc652;
-  a.acc[(683 * R + r) * a.acc_stride + q] = acc683;
-  a.acc[(714 * R + r) * a.acc_stride + q] = acc714;
-  a.acc[(745 * R + r) * a.acc_stride + q] = acc745;
-  a.acc[(776 * R + r) * a.acc_stride + q] = acc776;
-  a.acc[(807 * R + r) * a.acc_stride + q] = acc807;
-  a.acc[(838 * R + r) * a.acc_stride + q] = acc838;
-  a.acc[(869 * R + r) * a.acc_stride + q] = acc869;
-  a.acc[(900 * R + r) * a.acc_stride + q] = acc900;
-  a.acc[(931 * R + r) * a.acc_stride + q] = acc931;
-  a.acc[(2 * R + r) * a.acc_stride + q] = acc2;
-  a.acc[(33 * R + r) * a.acc_stride + q] = acc33;
-  a.acc[(64 * R + r) * a.acc_stride + q] = acc64;
-  a.acc[(95 * R + r) * a.acc_stride + q] = acc95;
-  a.acc[(126 * R + r) * a.acc_stride + q] = acc126;
-  a.acc[(157 * R + r) * a.acc_stride + q] = acc157;
-  a.acc[(188 * R + r) * a.acc_stride + q] = acc188;
-  a.acc[(219 * R + r) * a.acc_stride + q] = acc219;
-  a.acc[(250 * R + r) * a.acc_stride + q] = acc250;
-  a.acc[(281 * R + r) * a.acc_stride + q] = acc281;
-  a.acc[(312 * R + r) * a.acc_stride + q] = acc312;
-  a.acc[(343 * R + r) * a.acc_stride + q] = acc343;
-  a.acc[(374 * R + r) * a.acc_stride + q] = acc374;
-  a.acc[(405 * R + r) * a.acc_stride + q] = acc405;
-  a.acc[(436 * R + r) * a.acc_stride + q] = acc436;
-  a.acc[(467 * R + r) * a.acc_stride + q] = acc467;
-  a.acc[(498 * R + r) * a.acc_stride + q] = acc498;
-  a.acc[(529 * R + r) * a.acc_stride + q] = acc529;
-}
-// class (22|22) split 1/12: 322 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_1(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc560 = 0.0;
-  double acc591 = 0.0;
-  double acc622 = 0.0;
-  double acc653 = 0.0;
-  double acc684 = 0.0;
-  double acc715 = 0.0;
-  double acc746 = 0.0;
-  double acc777 = 0.0;
-  double acc808 = 0.0;
-  double acc839 = 0.0;
-  double acc870 = 0.0;
-  double acc901 = 0.0;
-  double acc932 = 0.0;
-  double acc3 = 0.0;
-  double acc34 = 0.0;
-  double acc65 = 0.0;
-  double acc96 = 0.0;
-  double acc127 = 0.0;
-  double acc158 = 0.0;
-  double acc189 = 0.0;
-  double acc220 = 0.0;
-  double acc251 = 0.0;
-  double acc282 = 0.0;
-  double acc313 = 0.0;
-  double acc344 = 0.0;
-  double acc375 = 0.0;
-  double acc406 = 0.0;
-  double acc437 = 0.0;
-  double acc468 = 0.0;
-  double acc499 = 0.0;
-  double acc530 = 0.0;
-  double acc561 = 0.0;
-  double acc592 = 0.0;
-  double acc623 = 0.0;
-  double acc654 = 0.0;
-  double acc685 = 0.0;
-  double acc716 = 0.0;
-  double acc747 = 0.0;
-  double acc778 = 0.0;
-  double acc809 = 0.0;
-  double acc840 = 0.0;
-  double acc871 = 0.0;
-  double acc902 = 0.0;
-  double acc933 = 0.0;
-  double acc4 = 0.0;
-  double acc35 = 0.0;
-  double acc66 = 0.0;
-  double acc97 = 0.0;
-  double acc128 = 0.0;
-  double acc159 = 0.0;
-  double acc190 = 0.0;
-  double acc221 = 0.0;
-  double acc252 = 0.0;
-  double acc283 = 0.0;
-  double acc314 = 0.0;
-  double acc345 = 0.0;
-  double acc376 = 0.0;
-  double acc407 = 0.0;
-  double acc438 = 0.0;
-  double acc469 = 0.0;
-  double acc500 = 0.0;
-  double acc531 = 0.0;
-  double acc562 = 0.0;
-  double acc593 = 0.0;
-  double acc624 = 0.0;
-  double acc655 = 0.0;
-  double acc686 = 0.0;
-  double acc717 = 0.0;
-  double acc748 = 0.0;
-  double acc779 = 0.0;
-  double acc810 = 0.0;
-  double acc841 = 0.0;
-  double acc872 = 0.0;
-  double acc903 = 0.0;
-  double acc934 = 0.0;
-  double acc5 = 0.0;
-  double acc36 = 0.0;
-  double acc67 = 0.0;
-  double acc98 = 0.0;
-  double acc129 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x10 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x11 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x35 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x38 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x0 - s.roz * x1);
-      const TB x36 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x37 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x41 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x43 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x210 = s.QC[0] * x2 + s.WQ[0] * x3;
-      const TB x211 = s.QC[0] * x3 + s.WQ[0] * x4;
-      const TB x212 = s.QC[0] * x12 + s.WQ[0] * x15 + r1 * x2;
-      const TB x213 = s.QC[0] * x15 + s.WQ[0] * x18 + r1 * x3;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x215 = s.QC[0] * x13 + s.WQ[0] * x16;
-      const TB x216 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x218 = s.QC[0] * x14 + s.WQ[0] * x17;
-      const TB x219 = s.QC[0] * x17 + s.WQ[0] * x20;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x229 = s.QC[0] * x35 + s.WQ[0] * x41 + r1 * x14;
-      const TB x230 = s.QC[0] * x41 + s.WQ[0] * x47 + r1 * x17;
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x233 = s.QC[0] * x36 + s.WQ[0] * x42;
-      const TB x234 = s.QC[0] * x42 + s.WQ[0] * x48;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x237 = s.QC[0] * x37 + s.WQ[0] * x43;
-      const TB x238 = s.QC[0] * x43 + s.WQ[0] * x49;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x240 = s.QC[0] * x55 + s.WQ[0] * x61;
-      const TB x241 = s.QC[0] * x38 + s.WQ[0] * x44;
-      const TB x242 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x377 = s.QC[1] * x44 + s.WQ[1] * x50;
-      const TB x637 = s.QC[0] * x241 + s.WQ[0] * x242 + e1 * (x38 - s.roe * x44);
-      acc5 += static_cast<double>(x637);
-      const TB x737 = s.QC[1] * x241 + s.WQ[1] * x242;
-      acc36 += static_cast<double>(x737);
-      const TB x837 = s.QC[2] * x241 + s.WQ[2] * x242 + r2 * x218;
-      acc67 += static_cast<double>(x837);
-      const TB x345 = s.QC[1] * x2 + s.WQ[1] * x3;
-      const TB x346 = s.QC[1] * x3 + s.WQ[1] * x4;
-      const TB x352 = s.QC[1] * x19 + s.WQ[1] * x22 + r1 * x4;
-      const TB x347 = s.QC[1] * x12 + s.WQ[1] * x15;
-      const TB x348 = s.QC[1] * x15 + s.WQ[1] * x18;
-      const TB x349 = s.QC[1] * x18 + s.WQ[1] * x21;
-      const TB x350 = s.QC[1] * x13 + s.WQ[1] * x16 + r1 * x2;
-      const TB x351 = s.QC[1] * x16 + s.WQ[1] * x19 + r1 * x3;
-      const TB x353 = s.QC[1] * x14 + s.WQ[1] * x17;
-      const TB x354 = s.QC[1] * x17 + s.WQ[1] * x20;
-      const TB x355 = s.QC[1] * x20 + s.WQ[1] * x23;
-      const TB x375 = s.QC[1] * x55 + s.WQ[1] * x61 + r1 * x23;
-      const TB x510 = s.QC[2] * x55 + s.WQ[2] * x61 + r1 * x22;
-      const TB x364 = s.QC[1] * x35 + s.WQ[1] * x41;
-      const TB x365 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x366 = s.QC[1] * x47 + s.WQ[1] * x53;
-      const TB x367 = s.QC[1] * x53 + s.WQ[1] * x59;
-      const TB x502 = s.QC[2] * x53 + s.WQ[2] * x59 + r1 * x21;
-      const TB x484 = s.QC[2] * x18 + s.WQ[2] * x21;
-      const TB x368 = s.QC[1] * x36 + s.WQ[1] * x42 + r2 * x13;
-      const TB x369 = s.QC[1] * x42 + s.WQ[1] * x48 + r2 * x16;
-      const TB x370 = s.QC[1] * x48 + s.WQ[1] * x54 + r2 * x19;
-      const TB x371 = s.QC[1] * x54 + s.WQ[1] * x60 + r2 * x22;
-      const TB x487 = s.QC[2] * x19 + s.WQ[2] * x22;
-      const TB x506 = s.QC[2] * x54 + s.WQ[2] * x60;
-      const TB x372 = s.QC[1] * x37 + s.WQ[1] * x43 + r1 * x14;
-      const TB x373 = s.QC[1] * x43 + s.WQ[1] * x49 + r1 * x17;
-      const TB x374 = s.QC[1] * x49 + s.WQ[1] * x55 + r1 * x20;
-      const TB x376 = s.QC[1] * x38 + s.WQ[1] * x44;
-      const TB x937 = s.QC[1] * x376 + s.WQ[1] * x377 + e1 * (x38 - s.roe * x44);
-      acc98 += static_cast<double>(x937);
-      const TB x1037 = s.QC[2] * x376 + s.WQ[2] * x377 + r2 * x353;
-      acc129 += static_cast<double>(x1037);
-      const TB x482 = s.QC[2] * x12 + s.WQ[2] * x15;
-      const TB x483 = s.QC[2] * x15 + s.WQ[2] * x18;
-      const TB x485 = s.QC[2] * x13 + s.WQ[2] * x16;
-      const TB x486 = s.QC[2] * x16 + s.WQ[2] * x19;
-      const TB x499 = s.QC[2] * x35 + s.WQ[2] * x41 + r1 * x12;
-      const TB x500 = s.QC[2] * x41 + s.WQ[2] * x47 + r1 * x15;
-      const TB x501 = s.QC[2] * x47 + s.WQ[2] * x53 + r1 * x18;
-      const TB x1130 = s.QC[2] * x501 + s.WQ[2] * x502 + e1 * (x47 - s.roe * x53) + r1 * x484;
-      const TB x1117 = s.QC[2] * x483 + s.WQ[2] * x484 + e1 * (x15 - s.roe * x18);
-      const TB x503 = s.QC[2] * x36 + s.WQ[2] * x42;
-      const TB x504 = s.QC[2] * x42 + s.WQ[2] * x48;
-      const TB x505 = s.QC[2] * x48 + s.WQ[2] * x54;
-      const TB x1133 = s.QC[2] * x505 + s.WQ[2] * x506 + e1 * (x48 - s.roe * x54);
-      const TB x507 = s.QC[2] * x37 + s.WQ[2] * x43 + r1 * x13;
-      const TB x508 = s.QC[2] * x43 + s.WQ[2] * x49 + r1 * x16;
-      const TB x509 = s.QC[2] * x49 + s.WQ[2] * x55 + r1 * x19;
-      const TB x1136 = s.QC[2] * x509 + s.WQ[2] * x510 + e1 * (x49 - s.roe * x55) + r1 * x487;
-      const TB x1119 = s.QC[2] * x486 + s.WQ[2] * x487 + e1 * (x16 - s.roe * x19);
-      const TB x615 = s.QC[0] * x210 + s.WQ[0] * x211 + e1 * (x2 - s.roe * x3);
-      const TB x915 = s.QC[1] * x345 + s.WQ[1] * x346 + e1 * (x2 - s.roe * x3);
-      const TB x918 = s.QC[1] * x350 + s.WQ[1] * x351 + e1 * (x13 - s.roe * x16) + r1 * x345;
-      const TB x919 = s.QC[1] * x351 + s.WQ[1] * x352 + e1 * (x16 - s.roe * x19) + r1 * x346;
-      const TB x619 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x16 - s.roe * x19);
-      const TB x1606 = s.QC[1] * x918 + s.WQ[1] * x919 + e2 * (x350 - s.roe * x351) + r1 * x915;
-      const TB x616 = s.QC[0] * x212 + s.WQ[0] * x213 + e1 * (x12 - s.roe * x15) + r1 * x210;
-      const TB x617 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x15 - s.roe * x18) + r1 * x211;
-      const TB x917 = s.QC[1] * x348 + s.WQ[1] * x349 + e1 * (x15 - s.roe * x18);
-      const TB x618 = s.QC[0] * x215 + s.WQ[0] * x216 + e1 * (x13 - s.roe * x16);
-      const TB x1118 = s.QC[2] * x485 + s.WQ[2] * x486 + e1 * (x13 - s.roe * x16);
-      const TB x620 = s.QC[0] * x218 + s.WQ[0] * x219 + e1 * (x14 - s.roe * x17);
-      const TB x920 = s.QC[1] * x353 + s.WQ[1] * x354 + e1 * (x14 - s.roe * x17);
-      const TB x621 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x17 - s.roe * x20);
-      const TB x921 = s.QC[1] * x354 + s.WQ[1] * x355 + e1 * (x17 - s.roe * x20);
-      const TB x936 = s.QC[1] * x374 + s.WQ[1] * x375 + e1 * (x49 - s.roe * x55) + r1 * x355;
-      const TB x636 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x49 - s.roe * x55);
-      const TB x1036 = s.QC[2] * x374 + s.WQ[2] * x375 + r1 * x352;
-      const TB x628 = s.QC[0] * x229 + s.WQ[0] * x230 + e1 * (x35 - s.roe * x41) + r1 * x218;
-      const TB x629 = s.QC[0] * x230 + s.WQ[0] * x231 + e1 * (x41 - s.roe * x47) + r1 * x219;
-      const TB x630 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x47 - s.roe * x53) + r1 * x220;
-      const TB x930 = s.QC[1] * x366 + s.WQ[1] * x367 + e1 * (x47 - s.roe * x53);
-      const TB x1030 = s.QC[2] * x366 + s.WQ[2] * x367 + r1 * x349;
-      const TB x1017 = s.QC[2] * x348 + s.WQ[2] * x349;
-      const TB x631 = s.QC[0] * x233 + s.WQ[0] * x234 + e1 * (x36 - s.roe * x42);
-      acc3 += static_cast<double>(x631);
-      const TB x632 = s.QC[0] * x234 + s.WQ[0] * x235 + e1 * (x42 - s.roe * x48);
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x933 = s.QC[1] * x370 + s.WQ[1] * x371 + e1 * (x48 - s.roe * x54) + r2 * x352;
-      const TB x1019 = s.QC[2] * x351 + s.WQ[2] * x352;
-      const TB x1033 = s.QC[2] * x370 + s.WQ[2] * x371;
-      const TB x634 = s.QC[0] * x237 + s.WQ[0] * x238 + e1 * (x37 - s.roe * x43);
-      acc4 += static_cast<double>(x634);
-      const TB x635 = s.QC[0] * x238 + s.WQ[0] * x239 + e1 * (x43 - s.roe * x49);
-      const TB x715 = s.QC[1] * x210 + s.WQ[1] * x211;
-      const TB x718 = s.QC[1] * x215 + s.WQ[1] * x216 + r1 * x210;
-      const TB x719 = s.QC[1] * x216 + s.WQ[1] * x217 + r1 * x211;
-      const TB x1411 = s.QC[1] * x718 + s.WQ[1] * x719 + e1 * (x215 - s.roe * x216) + r1 * x715;
-      const TB x716 = s.QC[1] * x212 + s.WQ[1] * x213;
-      const TB x717 = s.QC[1] * x213 + s.WQ[1] * x214;
-      const TB x720 = s.QC[1] * x218 + s.WQ[1] * x219;
-      const TB x721 = s.QC[1] * x219 + s.WQ[1] * x220;
-      const TB x736 = s.QC[1] * x239 + s.WQ[1] * x240 + r1 * x220;
-      const TB x836 = s.QC[2] * x239 + s.WQ[2] * x240 + r1 * x217;
-      const TB x728 = s.QC[1] * x229 + s.WQ[1] * x230;
-      const TB x729 = s.QC[1] * x230 + s.WQ[1] * x231;
-      const TB x730 = s.QC[1] * x231 + s.WQ[1] * x232;
-      const TB x830 = s.QC[2] * x231 + s.WQ[2] * x232 + r1 * x214;
-      const TB x817 = s.QC[2] * x213 + s.WQ[2] * x214;
-      const TB x731 = s.QC[1] * x233 + s.WQ[1] * x234 + r2 * x215;
-      acc34 += static_cast<double>(x731);
-      const TB x732 = s.QC[1] * x234 + s.WQ[1] * x235 + r2 * x216;
-      const TB x733 = s.QC[1] * x235 + s.WQ[1] * x236 + r2 * x217;
-      const TB x819 = s.QC[2] * x216 + s.WQ[2] * x217;
-      const TB x833 = s.QC[2] * x235 + s.WQ[2] * x236;
-      const TB x734 = s.QC[1] * x237 + s.WQ[1] * x238 + r1 * x218;
-      acc35 += static_cast<double>(x734);
-      const TB x735 = s.QC[1] * x238 + s.WQ[1] * x239 + r1 * x219;
-      const TB x816 = s.QC[2] * x212 + s.WQ[2] * x213;
-      const TB x818 = s.QC[2] * x215 + s.WQ[2] * x216;
-      const TB x828 = s.QC[2] * x229 + s.WQ[2] * x230 + r1 * x212;
-      const TB x829 = s.QC[2] * x230 + s.WQ[2] * x231 + r1 * x213;
-      const TB x1548 = s.QC[2] * x829 + s.WQ[2] * x830 + e1 * (x230 - s.roe * x231) + r1 * x817;
-      const TB x1540 = s.QC[2] * x816 + s.WQ[2] * x817 + e1 * (x212 - s.roe * x213);
-      const TB x1547 = s.QC[2] * x828 + s.WQ[2] * x829 + e1 * (x229 - s.roe * x230) + r1 * x816;
-      const TB x2146 = s.QC[2] * x1547 + s.WQ[2] * x1548 + e2 * (x828 - s.roe * x829) + r1 * x1540;
-      acc777 += static_cast<double>(x2146);
-      const TB x831 = s.QC[2] * x233 + s.WQ[2] * x234;
-      acc65 += static_cast<double>(x831);
-      const TB x832 = s.QC[2] * x234 + s.WQ[2] * x235;
-      const TB x1550 = s.QC[2] * x832 + s.WQ[2] * x833 + e1 * (x234 - s.roe * x235);
-      const TB x834 = s.QC[2] * x237 + s.WQ[2] * x238 + r1 * x215;
-      acc66 += static_cast<double>(x834);
-      const TB x835 = s.QC[2] * x238 + s.WQ[2] * x239 + r1 * x216;
-      const TB x1552 = s.QC[2] * x835 + s.WQ[2] * x836 + e1 * (x238 - s.roe * x239) + r1 * x819;
-      const TB x1541 = s.QC[2] * x818 + s.WQ[2] * x819 + e1 * (x215 - s.roe * x216);
-      const TB x1216 = s.QC[0] * x618 + s.WQ[0] * x619 + e2 * (x215 - s.roe * x216);
-      const TB x1551 = s.QC[2] * x834 + s.WQ[2] * x835 + e1 * (x237 - s.roe * x238) + r1 * x818;
-      acc345 += static_cast<double>(x1551);
-      const TB x2148 = s.QC[2] * x1551 + s.WQ[2] * x1552 + e2 * (x834 - s.roe * x835) + r1 * x1541;
-      acc779 += static_cast<double>(x2148);
-      const TB x916 = s.QC[1] * x347 + s.WQ[1] * x348 + e1 * (x12 - s.roe * x15);
-      const TB x1116 = s.QC[2] * x482 + s.WQ[2] * x483 + e1 * (x12 - s.roe * x15);
-      const TB x928 = s.QC[1] * x364 + s.WQ[1] * x365 + e1 * (x35 - s.roe * x41);
-      const TB x1128 = s.QC[2] * x499 + s.WQ[2] * x500 + e1 * (x35 - s.roe * x41) + r1 * x482;
-      const TB x1800 = s.QC[2] * x1116 + s.WQ[2] * x1117 + e2 * (x482 - s.roe * x483);
-      const TB x1129 = s.QC[2] * x500 + s.WQ[2] * x501 + e1 * (x41 - s.roe * x47) + r1 * x483;
-      const TB x1808 = s.QC[2] * x1129 + s.WQ[2] * x1130 + e2 * (x500 - s.roe * x501) + r1 * x1117;
-      const TB x1807 = s.QC[2] * x1128 + s.WQ[2] * x1129 + e2 * (x499 - s.roe * x500) + r1 * x1116;
-      const TB x2301 = s.QC[2] * x1807 + s.WQ[2] * x1808 + e3 * (x1128 - s.roe * x1129) + r1 * x1800;
-      acc932 += static_cast<double>(x2301);
-      const TB x929 = s.QC[1] * x365 + s.WQ[1] * x366 + e1 * (x41 - s.roe * x47);
-      const TB x931 = s.QC[1] * x368 + s.WQ[1] * x369 + e1 * (x36 - s.roe * x42) + r2 * x350;
-      acc96 += static_cast<double>(x931);
-      const TB x1131 = s.QC[2] * x503 + s.WQ[2] * x504 + e1 * (x36 - s.roe * x42);
-      acc158 += static_cast<double>(x1131);
-      const TB x932 = s.QC[1] * x369 + s.WQ[1] * x370 + e1 * (x42 - s.roe * x48) + r2 * x351;
-      const TB x1132 = s.QC[2] * x504 + s.WQ[2] * x505 + e1 * (x42 - s.roe * x48);
-      const TB x1810 = s.QC[2] * x1132 + s.WQ[2] * x1133 + e2 * (x504 - s.roe * x505);
-      const TB x1809 = s.QC[2] * x1131 + s.WQ[2] * x1132 + e2 * (x503 - s.roe * x504);
-      acc468 += static_cast<double>(x1809);
-      const TB x2302 = s.QC[2] * x1809 + s.WQ[2] * x1810 + e3 * (x1131 - s.roe * x1132);
-      acc933 += static_cast<double>(x2302);
-      const TB x934 = s.QC[1] * x372 + s.WQ[1] * x373 + e1 * (x37 - s.roe * x43) + r1 * x353;
-      acc97 += static_cast<double>(x934);
-      const TB x1134 = s.QC[2] * x507 + s.WQ[2] * x508 + e1 * (x37 - s.roe * x43) + r1 * x485;
-      acc159 += static_cast<double>(x1134);
-      const TB x1607 = s.QC[1] * x920 + s.WQ[1] * x921 + e2 * (x353 - s.roe * x354);
-      const TB x935 = s.QC[1] * x373 + s.WQ[1] * x374 + e1 * (x43 - s.roe * x49) + r1 * x354;
-      const TB x1135 = s.QC[2] * x508 + s.WQ[2] * x509 + e1 * (x43 - s.roe * x49) + r1 * x486;
-      const TB x1801 = s.QC[2] * x1118 + s.WQ[2] * x1119 + e2 * (x485 - s.roe * x486);
-      const TB x1812 = s.QC[2] * x1135 + s.WQ[2] * x1136 + e2 * (x508 - s.roe * x509) + r1 * x1119;
-      const TB x1811 = s.QC[2] * x1134 + s.WQ[2] * x1135 + e2 * (x507 - s.roe * x508) + r1 * x1118;
-      acc469 += static_cast<double>(x1811);
-      const TB x2303 = s.QC[2] * x1811 + s.WQ[2] * x1812 + e3 * (x1134 - s.roe * x1135) + r1 * x1801;
-      acc934 += static_cast<double>(x2303);
-      const TB x1616 = s.QC[1] * x934 + s.WQ[1] * x935 + e2 * (x372 - s.roe * x373) + r1 * x920;
-      acc376 += static_cast<double>(x1616);
-      const TB x1617 = s.QC[1] * x935 + s.WQ[1] * x936 + e2 * (x373 - s.roe * x374) + r1 * x921;
-      const TB x2179 = s.QC[1] * x1616 + s.WQ[1] * x1617 + e3 * (x934 - s.roe * x935) + r1 * x1607;
-      acc810 += static_cast<double>(x2179);
-      const TB x2210 = s.QC[2] * x1616 + s.WQ[2] * x1617 + r1 * x1606;
-      acc841 += static_cast<double>(x2210);
-      const TB x1682 = s.QC[2] * x935 + s.WQ[2] * x936 + r1 * x919;
-      const TB x1016 = s.QC[2] * x347 + s.WQ[2] * x348;
-      const TB x1018 = s.QC[2] * x350 + s.WQ[2] * x351;
-      const TB x1028 = s.QC[2] * x364 + s.WQ[2] * x365 + r1 * x347;
-      const TB x1029 = s.QC[2] * x365 + s.WQ[2] * x366 + r1 * x348;
-      const TB x1743 = s.QC[2] * x1029 + s.WQ[2] * x1030 + e1 * (x365 - s.roe * x366) + r1 * x1017;
-      const TB x1613 = s.QC[1] * x929 + s.WQ[1] * x930 + e2 * (x365 - s.roe * x366);
-      const TB x1678 = s.QC[2] * x929 + s.WQ[2] * x930 + r1 * x917;
-      const TB x1735 = s.QC[2] * x1016 + s.WQ[2] * x1017 + e1 * (x347 - s.roe * x348);
-      const TB x1605 = s.QC[1] * x916 + s.WQ[1] * x917 + e2 * (x347 - s.roe * x348);
-      const TB x1670 = s.QC[2] * x916 + s.WQ[2] * x917;
-      const TB x1677 = s.QC[2] * x928 + s.WQ[2] * x929 + r1 * x916;
-      const TB x2239 = s.QC[2] * x1677 + s.WQ[2] * x1678 + e1 * (x928 - s.roe * x929) + r1 * x1670;
-      acc870 += static_cast<double>(x2239);
-      const TB x1742 = s.QC[2] * x1028 + s.WQ[2] * x1029 + e1 * (x364 - s.roe * x365) + r1 * x1016;
-      const TB x2270 = s.QC[2] * x1742 + s.WQ[2] * x1743 + e2 * (x1028 - s.roe * x1029) + r1 * x1735;
-      acc901 += static_cast<double>(x2270);
-      const TB x1612 = s.QC[1] * x928 + s.WQ[1] * x929 + e2 * (x364 - s.roe * x365);
-      const TB x2177 = s.QC[1] * x1612 + s.WQ[1] * x1613 + e3 * (x928 - s.roe * x929);
-      acc808 += static_cast<double>(x2177);
-      const TB x2208 = s.QC[2] * x1612 + s.WQ[2] * x1613 + r1 * x1605;
-      acc839 += static_cast<double>(x2208);
-      const TB x1031 = s.QC[2] * x368 + s.WQ[2] * x369;
-      acc127 += static_cast<double>(x1031);
-      const TB x1032 = s.QC[2] * x369 + s.WQ[2] * x370;
-      const TB x1745 = s.QC[2] * x1032 + s.WQ[2] * x1033 + e1 * (x369 - s.roe * x370);
-      const TB x1615 = s.QC[1] * x932 + s.WQ[1] * x933 + e2 * (x369 - s.roe * x370) + r2 * x919;
-      const TB x1671 = s.QC[2] * x918 + s.WQ[2] * x919;
-      const TB x1680 = s.QC[2] * x932 + s.WQ[2] * x933;
-      const TB x1034 = s.QC[2] * x372 + s.WQ[2] * x373 + r1 * x350;
-      acc128 += static_cast<double>(x1034);
-      const TB x1736 = s.QC[2] * x1018 + s.WQ[2] * x1019 + e1 * (x350 - s.roe * x351);
-      const TB x1035 = s.QC[2] * x373 + s.WQ[2] * x374 + r1 * x351;
-      const TB x1747 = s.QC[2] * x1035 + s.WQ[2] * x1036 + e1 * (x373 - s.roe * x374) + r1 * x1019;
-      const TB x1746 = s.QC[2] * x1034 + s.WQ[2] * x1035 + e1 * (x372 - s.roe * x373) + r1 * x1018;
-      acc438 += static_cast<double>(x1746);
-      const TB x2272 = s.QC[2] * x1746 + s.WQ[2] * x1747 + e2 * (x1034 - s.roe * x1035) + r1 * x1736;
-      acc903 += static_cast<double>(x2272);
-      const TB x1215 = s.QC[0] * x616 + s.WQ[0] * x617 + e2 * (x212 - s.roe * x213) + r1 * x615;
-      const TB x1410 = s.QC[1] * x716 + s.WQ[1] * x717 + e1 * (x212 - s.roe * x213);
-      const TB x1281 = s.QC[1] * x618 + s.WQ[1] * x619 + r1 * x615;
-      const TB x1217 = s.QC[0] * x620 + s.WQ[0] * x621 + e2 * (x218 - s.roe * x219);
-      const TB x1412 = s.QC[1] * x720 + s.WQ[1] * x721 + e1 * (x218 - s.roe * x219);
-      const TB x1421 = s.QC[1] * x734 + s.WQ[1] * x735 + e1 * (x237 - s.roe * x238) + r1 * x720;
-      acc283 += static_cast<double>(x1421);
-      const TB x1226 = s.QC[0] * x634 + s.WQ[0] * x635 + e2 * (x237 - s.roe * x238);
-      acc190 += static_cast<double>(x1226);
-      const TB x1422 = s.QC[1] * x735 + s.WQ[1] * x736 + e1 * (x238 - s.roe * x239) + r1 * x721;
-      const TB x1227 = s.QC[0] * x635 + s.WQ[0] * x636 + e2 * (x238 - s.roe * x239);
-      const TB x1900 = s.QC[1] * x1226 + s.WQ[1] * x1227 + r1 * x1217;
-      acc531 += static_cast<double>(x1900);
-      const TB x2055 = s.QC[1] * x1421 + s.WQ[1] * x1422 + e2 * (x734 - s.roe * x735) + r1 * x1412;
-      acc686 += static_cast<double>(x2055);
-      const TB x2086 = s.QC[2] * x1421 + s.WQ[2] * x1422 + r1 * x1411;
-      acc717 += static_cast<double>(x2086);
-      const TB x1487 = s.QC[2] * x735 + s.WQ[2] * x736 + r1 * x719;
-      const TB x1869 = s.QC[0] * x1226 + s.WQ[0] * x1227 + e3 * (x634 - s.roe * x635);
-      acc500 += static_cast<double>(x1869);
-      const TB x1931 = s.QC[2] * x1226 + s.WQ[2] * x1227 + r1 * x1216;
-      acc562 += static_cast<double>(x1931);
-      const TB x1222 = s.QC[0] * x628 + s.WQ[0] * x629 + e2 * (x229 - s.roe * x230) + r1 * x620;
-      const TB x1417 = s.QC[1] * x728 + s.WQ[1] * x729 + e1 * (x229 - s.roe * x230);
-      const TB x1223 = s.QC[0] * x629 + s.WQ[0] * x630 + e2 * (x230 - s.roe * x231) + r1 * x621;
-      const TB x1929 = s.QC[2] * x1222 + s.WQ[2] * x1223 + r1 * x1215;
-      acc560 += static_cast<double>(x1929);
-      const TB x1418 = s.QC[1] * x729 + s.WQ[1] * x730 + e1 * (x230 - s.roe * x231);
-      const TB x2084 = s.QC[2] * x1417 + s.WQ[2] * x1418 + r1 * x1410;
-      acc715 += static_cast<double>(x2084);
-      const TB x2053 = s.QC[1] * x1417 + s.WQ[1] * x1418 + e2 * (x728 - s.roe * x729);
-      acc684 += static_cast<double>(x2053);
-      const TB x1483 = s.QC[2] * x729 + s.WQ[2] * x730 + r1 * x717;
-      const TB x1475 = s.QC[2] * x716 + s.WQ[2] * x717;
-      const TB x1482 = s.QC[2] * x728 + s.WQ[2] * x729 + r1 * x716;
-      const TB x2115 = s.QC[2] * x1482 + s.WQ[2] * x1483 + e1 * (x728 - s.roe * x729) + r1 * x1475;
-      acc746 += static_cast<double>(x2115);
-      const TB x1224 = s.QC[0] * x631 + s.WQ[0] * x632 + e2 * (x233 - s.roe * x234);
-      acc189 += static_cast<double>(x1224);
-      const TB x1225 = s.QC[0] * x632 + s.WQ[0] * x633 + e2 * (x234 - s.roe * x235);
-      const TB x1899 = s.QC[1] * x1224 + s.WQ[1] * x1225 + r2 * x1216;
-      acc530 += static_cast<double>(x1899);
-      const TB x1420 = s.QC[1] * x732 + s.WQ[1] * x733 + e1 * (x234 - s.roe * x235) + r2 * x719;
-      const TB x1476 = s.QC[2] * x718 + s.WQ[2] * x719;
-      const TB x1485 = s.QC[2] * x732 + s.WQ[2] * x733;
-      const TB x1868 = s.QC[0] * x1224 + s.WQ[0] * x1225 + e3 * (x631 - s.roe * x632);
-      acc499 += static_cast<double>(x1868);
-      const TB x1930 = s.QC[2] * x1224 + s.WQ[2] * x1225;
-      acc561 += static_cast<double>(x1930);
-      const TB x1280 = s.QC[1] * x616 + s.WQ[1] * x617;
-      const TB x1282 = s.QC[1] * x620 + s.WQ[1] * x621;
-      const TB x1291 = s.QC[1] * x634 + s.WQ[1] * x635 + r1 * x620;
-      acc221 += static_cast<double>(x1291);
-      const TB x1292 = s.QC[1] * x635 + s.WQ[1] * x636 + r1 * x621;
-      const TB x1962 = s.QC[1] * x1291 + s.WQ[1] * x1292 + e1 * (x634 - s.roe * x635) + r1 * x1282;
-      acc593 += static_cast<double>(x1962);
-      const TB x1993 = s.QC[2] * x1291 + s.WQ[2] * x1292 + r1 * x1281;
-      acc624 += static_cast<double>(x1993);
-      const TB x1357 = s.QC[2] * x635 + s.WQ[2] * x636 + r1 * x619;
-      const TB x1287 = s.QC[1] * x628 + s.WQ[1] * x629;
-      const TB x1288 = s.QC[1] * x629 + s.WQ[1] * x630;
-      const TB x1991 = s.QC[2] * x1287 + s.WQ[2] * x1288 + r1 * x1280;
-      acc622 += static_cast<double>(x1991);
-      const TB x1960 = s.QC[1] * x1287 + s.WQ[1] * x1288 + e1 * (x628 - s.roe * x629);
-      acc591 += static_cast<double>(x1960);
-      const TB x1353 = s.QC[2] * x629 + s.WQ[2] * x630 + r1 * x617;
-      const TB x1345 = s.QC[2] * x616 + s.WQ[2] * x617;
-      const TB x1352 = s.QC[2] * x628 + s.WQ[2] * x629 + r1 * x616;
-      const TB x2022 = s.QC[2] * x1352 + s.WQ[2] * x1353 + e1 * (x628 - s.roe * x629) + r1 * x1345;
-      acc653 += static_cast<double>(x2022);
-      const TB x1289 = s.QC[1] * x631 + s.WQ[1] * x632 + r2 * x618;
-      acc220 += static_cast<double>(x1289);
-      const TB x1290 = s.QC[1] * x632 + s.WQ[1] * x633 + r2 * x619;
-      const TB x1961 = s.QC[1] * x1289 + s.WQ[1] * x1290 + e1 * (x631 - s.roe * x632) + r2 * x1281;
-      acc592 += static_cast<double>(x1961);
-      const TB x1992 = s.QC[2] * x1289 + s.WQ[2] * x1290;
-      acc623 += static_cast<double>(x1992);
-      const TB x1346 = s.QC[2] * x618 + s.WQ[2] * x619;
-      const TB x1355 = s.QC[2] * x632 + s.WQ[2] * x633;
-      const TB x1356 = s.QC[2] * x634 + s.WQ[2] * x635 + r1 * x618;
-      acc252 += static_cast<double>(x1356);
-      const TB x2024 = s.QC[2] * x1356 + s.WQ[2] * x1357 + e1 * (x634 - s.roe * x635) + r1 * x1346;
-      acc655 += static_cast<double>(x2024);
-      const TB x1354 = s.QC[2] * x631 + s.WQ[2] * x632;
-      acc251 += static_cast<double>(x1354);
-      const TB x2023 = s.QC[2] * x1354 + s.WQ[2] * x1355 + e1 * (x631 - s.roe * x632);
-      acc654 += static_cast<double>(x2023);
-      const TB x1419 = s.QC[1] * x731 + s.WQ[1] * x732 + e1 * (x233 - s.roe * x234) + r2 * x718;
-      acc282 += static_cast<double>(x1419);
-      const TB x1549 = s.QC[2] * x831 + s.WQ[2] * x832 + e1 * (x233 - s.roe * x234);
-      acc344 += static_cast<double>(x1549);
-      const TB x2147 = s.QC[2] * x1549 + s.WQ[2] * x1550 + e2 * (x831 - s.roe * x832);
-      acc778 += static_cast<double>(x2147);
-      const TB x2054 = s.QC[1] * x1419 + s.WQ[1] * x1420 + e2 * (x731 - s.roe * x732) + r2 * x1411;
-      acc685 += static_cast<double>(x2054);
-      const TB x2085 = s.QC[2] * x1419 + s.WQ[2] * x1420;
-      acc716 += static_cast<double>(x2085);
-      const TB x1486 = s.QC[2] * x734 + s.WQ[2] * x735 + r1 * x718;
-      acc314 += static_cast<double>(x1486);
-      const TB x2117 = s.QC[2] * x1486 + s.WQ[2] * x1487 + e1 * (x734 - s.roe * x735) + r1 * x1476;
-      acc748 += static_cast<double>(x2117);
-      const TB x1484 = s.QC[2] * x731 + s.WQ[2] * x732;
-      acc313 += static_cast<double>(x1484);
-      const TB x2116 = s.QC[2] * x1484 + s.WQ[2] * x1485 + e1 * (x731 - s.roe * x732);
-      acc747 += static_cast<double>(x2116);
-      const TB x1614 = s.QC[1] * x931 + s.WQ[1] * x932 + e2 * (x368 - s.roe * x369) + r2 * x918;
-      acc375 += static_cast<double>(x1614);
-      const TB x1744 = s.QC[2] * x1031 + s.WQ[2] * x1032 + e1 * (x368 - s.roe * x369);
-      acc437 += static_cast<double>(x1744);
-      const TB x2271 = s.QC[2] * x1744 + s.WQ[2] * x1745 + e2 * (x1031 - s.roe * x1032);
-      acc902 += static_cast<double>(x2271);
-      const TB x2178 = s.QC[1] * x1614 + s.WQ[1] * x1615 + e3 * (x931 - s.roe * x932) + r2 * x1606;
-      acc809 += static_cast<double>(x2178);
-      const TB x2209 = s.QC[2] * x1614 + s.WQ[2] * x1615;
-      acc840 += static_cast<double>(x2209);
-      const TB x1681 = s.QC[2] * x934 + s.WQ[2] * x935 + r1 * x918;
-      acc407 += static_cast<double>(x1681);
-      const TB x2241 = s.QC[2] * x1681 + s.WQ[2] * x1682 + e1 * (x934 - s.roe * x935) + r1 * x1671;
-      acc872 += static_cast<double>(x2241);
-      const TB x1679 = s.QC[2] * x931 + s.WQ[2] * x932;
-      acc406 += static_cast<double>(x1679);
-      const TB x2240 = s.QC[2] * x1679 + s.WQ[2] * x1680 + e1 * (x931 - s.roe * x932);
-      acc871 += static_cast<double>(x2240);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(560 * R + r) * a.acc_stride + q] = acc560;
-  a.acc[(591 * R + r) * a.acc_stride + q] = acc591;
-  a.acc[(622 * R + r) * a.acc_stride + q] = acc622;
-  a.acc[(653 * R + r) * a.acc_stride + q] = acc653;
-  a.acc[(684 * R + r) * a.acc_stride + q] = acc684;
-  a.acc[(715 * R + r) * a.acc_stride + q] = acc715;
-  a.acc[(746 * R + r) * a.acc_stride + q] = acc746;
-  a.acc[(777 * R + r) * a.acc_stride + q] = acc777;
-  a.acc[(808 * R + r) * a.acc_stride + q] = acc808;
-  a.acc[(839 * R + r) * a.acc_stride + q] = acc839;
-  a.acc[(870 * R + r) * a.acc_stride + q] = acc870;
-  a.acc[(901 * R + r) * a.acc_stride + q] = acc901;
-  a.acc[(932 * R + r) * a.acc_stride + q] = acc932;
-  a.acc[(3 * R + r) * a.acc_stride + q] = acc3;
-  a.acc[(34 * R + r) * a.acc_stride + q] = acc34;
-  a.acc[(65 * R + r) * a.acc_stride + q] = acc65;
-  a.acc[(96 * R + r) * a.acc_stride + q] = acc96;
-  a.acc[(127 * R + r) * a.acc_stride + q] = acc127;
-  a.acc[(158 * R + r) * a.acc_stride + q] = acc158;
-  a.acc[(189 * R + r) * a.acc_stride + q] = acc189;
-  a.acc[(220 * R + r) * a.acc_stride + q] = acc220;
-  a.acc[(251 * R + r) * a.acc_stride + q] = acc251;
-  a.acc[(282 * R + r) * a.acc_stride + q] = acc282;
-  a.acc[(313 * R + r) * a.acc_stride + q] = acc313;
-  a.acc[(344 * R + r) * a.acc_stride + q] = acc344;
-  a.acc[(375 * R + r) * a.acc_stride + q] = acc375;
-  a.acc[(406 * R + r) * a.acc_stride + q] = acc406;
-  a.acc[(437 * R + r) * a.acc_stride + q] = acc437;
-  a.acc[(468 * R + r) * a.acc_stride + q] = acc468;
-  a.acc[(499 * R + r) * a.acc_stride + q] = acc499;
-  a.acc[(530 * R + r) * a.acc_stride + q] = acc530;
-  a.acc[(561 * R + r) * a.acc_stride + q] = acc561;
-  a.acc[(592 * R + r) * a.acc_stride + q] = acc592;
-  a.acc[(623 * R + r) * a.acc_stride + q] = acc623;
-  a.acc[(654 * R + r) * a.acc_stride + q] = acc654;
-  a.acc[(685 * R + r) * a.acc_stride + q] = acc685;
-  a.acc[(716 * R + r) * a.acc_stride + q] = acc716;
-  a.acc[(747 * R + r) * a.acc_stride + q] = acc747;
-  a.acc[(778 * R + r) * a.acc_stride + q] = acc778;
-  a.acc[(809 * R + r) * a.acc_stride + q] = acc809;
-  a.acc[(840 * R + r) * a.acc_stride + q] = acc840;
-  a.acc[(871 * R + r) * a.acc_stride + q] = acc871;
-  a.acc[(902 * R + r) * a.acc_stride + q] = acc902;
-  a.acc[(933 * R + r) * a.acc_stride + q] = acc933;
-  a.acc[(4 * R + r) * a.acc_stride + q] = acc4;
-  a.acc[(35 * R + r) * a.acc_stride + q] = acc35;
-  a.acc[(66 * R + r) * a.acc_stride + q] = acc66;
-  a.acc[(97 * R + r) * a.acc_stride + q] = acc97;
-  a.acc[(128 * R + r) * a.acc_stride + q] = acc128;
-  a.acc[(159 * R + r) * a.acc_stride + q] = acc159;
-  a.acc[(190 * R + r) * a.acc_stride + q] = acc190;
-  a.acc[(221 * R + r) * a.acc_stride + q] = acc221;
-  a.acc[(252 * R + r) * a.acc_stride + q] = acc252;
-  a.acc[(283 * R + r) * a.acc_stride + q] = acc283;
-  a.acc[(314 * R + r) * a.acc_stride + q] = acc314;
-  a.acc[(345 * R + r) * a.acc_stride + q] = acc345;
-  a.acc[(376 * R + r) * a.acc_stride + q] = acc376;
-  a.acc[(407 * R + r) * a.acc_stride + q] = acc407;
-  a.acc[(438 * R + r) * a.acc_stride + q] = acc438;
-  a.acc[(469 * R + r) * a.acc_stride + q] = acc469;
-  a.acc[(500 * R + r) * a.acc_stride + q] = acc500;
-  a.acc[(531 * R + r) * a.acc_stride + q] = acc531;
-  a.acc[(562 * R + r) * a.acc_stride + q] = acc562;
-  a.acc[(593 * R + r) * a.acc_stride + q] = acc593;
-  a.acc[(624 * R + r) * a.acc_stride + q] = acc624;
-  a.acc[(655 * R + r) * a.acc_stride + q] = acc655;
-  a.acc[(686 * R + r) * a.acc_stride + q] = acc686;
-  a.acc[(717 * R + r) * a.acc_stride + q] = acc717;
-  a.acc[(748 * R + r) * a.acc_stride + q] = acc748;
-  a.acc[(779 * R + r) * a.acc_stride + q] = acc779;
-  a.acc[(810 * R + r) * a.acc_stride + q] = acc810;
-  a.acc[(841 * R + r) * a.acc_stride + q] = acc841;
-  a.acc[(872 * R + r) * a.acc_stride + q] = acc872;
-  a.acc[(903 * R + r) * a.acc_stride + q] = acc903;
-  a.acc[(934 * R + r) * a.acc_stride + q] = acc934;
-  a.acc[(5 * R + r) * a.acc_stride + q] = acc5;
-  a.acc[(36 * R + r) * a.acc_stride + q] = acc36;
-  a.acc[(67 * R + r) * a.acc_stride + q] = acc67;
-  a.acc[(98 * R + r) * a.acc_stride + q] = acc98;
-  a.acc[(129 * R + r) * a.acc_stride + q] = acc129;
-}
-// class (22|22) split 2/12: 284 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_2(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc160 = 0.0;
-  double acc191 = 0.0;
-  double acc222 = 0.0;
-  double acc253 = 0.0;
-  double acc284 = 0.0;
-  double acc315 = 0.0;
-  double acc346 = 0.0;
-  double acc377 = 0.0;
-  double acc408 = 0.0;
-  double acc439 = 0.0;
-  double acc470 = 0.0;
-  double acc501 = 0.0;
-  double acc532 = 0.0;
-  double acc563 = 0.0;
-  double acc594 = 0.0;
-  double acc625 = 0.0;
-  double acc656 = 0.0;
-  double acc687 = 0.0;
-  double acc718 = 0.0;
-  double acc749 = 0.0;
-  double acc780 = 0.0;
-  double acc811 = 0.0;
-  double acc842 = 0.0;
-  double acc873 = 0.0;
-  double acc904 = 0.0;
-  double acc935 = 0.0;
-  double acc6 = 0.0;
-  double acc37 = 0.0;
-  double acc68 = 0.0;
-  double acc99 = 0.0;
-  double acc130 = 0.0;
-  double acc161 = 0.0;
-  double acc192 = 0.0;
-  double acc223 = 0.0;
-  double acc254 = 0.0;
-  double acc285 = 0.0;
-  double acc316 = 0.0;
-  double acc347 = 0.0;
-  double acc378 = 0.0;
-  double acc409 = 0.0;
-  double acc440 = 0.0;
-  double acc471 = 0.0;
-  double acc502 = 0.0;
-  double acc533 = 0.0;
-  double acc564 = 0.0;
-  double acc595 = 0.0;
-  double acc626 = 0.0;
-  double acc657 = 0.0;
-  double acc688 = 0.0;
-  double acc719 = 0.0;
-  double acc750 = 0.0;
-  double acc781 = 0.0;
-  double acc812 = 0.0;
-  double acc843 = 0.0;
-  double acc874 = 0.0;
-  double acc905 = 0.0;
-  double acc936 = 0.0;
-  double acc7 = 0.0;
-  double acc38 = 0.0;
-  double acc69 = 0.0;
-  double acc100 = 0.0;
-  double acc131 = 0.0;
-  double acc162 = 0.0;
-  double acc193 = 0.0;
-  double acc224 = 0.0;
-  double acc255 = 0.0;
-  double acc286 = 0.0;
-  double acc317 = 0.0;
-  double acc348 = 0.0;
-  double acc379 = 0.0;
-  double acc410 = 0.0;
-  double acc441 = 0.0;
-  double acc472 = 0.0;
-  double acc503 = 0.0;
-  double acc534 = 0.0;
-  double acc565 = 0.0;
-  double acc596 = 0.0;
-  double acc627 = 0.0;
-  double acc658 = 0.0;
-  double acc689 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x11 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x38 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x0 - s.roz * x1);
-      const TB x33 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x63 = s.PA[0] * x24 + s.WP[0] * x27 + z1 * (x5 - s.roz * x6);
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x57 = s.PA[0] * x21 + s.WP[0] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x39 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x75 = s.PA[0] * x33 + s.WP[0] * x39 + z2 * (x9 - s.roz * x12);
-      const TB x76 = s.PA[1] * x33 + s.WP[1] * x39;
-      const TB x40 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x45 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x85 = s.PA[0] * x39 + s.WP[0] * x45 + z2 * (x12 - s.roz * x15);
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x51 = s.PA[0] * x18 + s.WP[0] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x115 = s.PA[0] * x57 + s.WP[0] * x63 + z2 * (x21 - s.roz * x24);
-      const TB x116 = s.PA[1] * x57 + s.WP[1] * x63;
-      const TB x86 = s.PA[1] * x39 + s.WP[1] * x45;
-      const TB x95 = s.PA[0] * x45 + s.WP[0] * x51 + z2 * (x15 - s.roz * x18);
-      const TB x96 = s.PA[1] * x45 + s.WP[1] * x51;
-      const TB x105 = s.PA[0] * x51 + s.WP[0] * x57 + z2 * (x18 - s.roz * x21);
-      const TB x106 = s.PA[1] * x51 + s.WP[1] * x57;
-      const TB x252 = s.QC[0] * x106 + s.WQ[0] * x116 + r2 * x58;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x210 = s.QC[0] * x2 + s.WQ[0] * x3;
-      const TB x211 = s.QC[0] * x3 + s.WQ[0] * x4;
-      const TB x213 = s.QC[0] * x15 + s.WQ[0] * x18 + r1 * x3;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x224 = s.QC[0] * x51 + s.WQ[0] * x57 + r2 * x21;
-      const TB x248 = s.QC[0] * x105 + s.WQ[0] * x115 + r3 * x57;
-      const TB x216 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x218 = s.QC[0] * x14 + s.WQ[0] * x17;
-      const TB x219 = s.QC[0] * x17 + s.WQ[0] * x20;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x222 = s.QC[0] * x39 + s.WQ[0] * x45 + r2 * x15;
-      const TB x617 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x15 - s.roe * x18) + r1 * x211;
-      const TB x223 = s.QC[0] * x45 + s.WQ[0] * x51 + r2 * x18;
-      const TB x623 = s.QC[0] * x222 + s.WQ[0] * x223 + e1 * (x39 - s.roe * x45) + r2 * x213;
-      const TB x624 = s.QC[0] * x223 + s.WQ[0] * x224 + e1 * (x45 - s.roe * x51) + r2 * x214;
-      const TB x1219 = s.QC[0] * x623 + s.WQ[0] * x624 + e2 * (x222 - s.roe * x223) + r2 * x617;
-      const TB x226 = s.QC[0] * x40 + s.WQ[0] * x46 + r1 * x16;
-      const TB x619 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x16 - s.roe * x19);
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x626 = s.QC[0] * x226 + s.WQ[0] * x227 + e1 * (x40 - s.roe * x46) + r1 * x216;
-      const TB x249 = s.QC[0] * x76 + s.WQ[0] * x86 + r2 * x40;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x250 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x46;
-      const TB x251 = s.QC[0] * x96 + s.WQ[0] * x106 + r2 * x52;
-      const TB x645 = s.QC[0] * x251 + s.WQ[0] * x252 + e1 * (x96 - s.roe * x106) + r2 * x228;
-      const TB x745 = s.QC[1] * x251 + s.WQ[1] * x252 + r1 * x224;
-      const TB x1221 = s.QC[0] * x626 + s.WQ[0] * x627 + e2 * (x226 - s.roe * x227) + r1 * x619;
-      const TB x643 = s.QC[0] * x249 + s.WQ[0] * x250 + e1 * (x76 - s.roe * x86) + r2 * x226;
-      acc7 += static_cast<double>(x643);
-      const TB x644 = s.QC[0] * x250 + s.WQ[0] * x251 + e1 * (x86 - s.roe * x96) + r2 * x227;
-      const TB x1232 = s.QC[0] * x643 + s.WQ[0] * x644 + e2 * (x249 - s.roe * x250) + r2 * x626;
-      acc193 += static_cast<double>(x1232);
-      const TB x1233 = s.QC[0] * x644 + s.WQ[0] * x645 + e2 * (x250 - s.roe * x251) + r2 * x627;
-      const TB x1872 = s.QC[0] * x1232 + s.WQ[0] * x1233 + e3 * (x643 - s.roe * x644) + r2 * x1221;
-      acc503 += static_cast<double>(x1872);
-      const TB x1903 = s.QC[1] * x1232 + s.WQ[1] * x1233 + r1 * x1219;
-      acc534 += static_cast<double>(x1903);
-      const TB x1934 = s.QC[2] * x1232 + s.WQ[2] * x1233;
-      acc565 += static_cast<double>(x1934);
-      const TB x241 = s.QC[0] * x38 + s.WQ[0] * x44;
-      const TB x242 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x244 = s.QC[0] * x56 + s.WQ[0] * x62;
-      const TB x245 = s.QC[0] * x75 + s.WQ[0] * x85 + r3 * x39;
-      const TB x246 = s.QC[0] * x85 + s.WQ[0] * x95 + r3 * x45;
-      const TB x247 = s.QC[0] * x95 + s.WQ[0] * x105 + r3 * x51;
-      const TB x345 = s.QC[1] * x2 + s.WQ[1] * x3;
-      const TB x346 = s.QC[1] * x3 + s.WQ[1] * x4;
-      const TB x353 = s.QC[1] * x14 + s.WQ[1] * x17;
-      const TB x354 = s.QC[1] * x17 + s.WQ[1] * x20;
-      const TB x355 = s.QC[1] * x20 + s.WQ[1] * x23;
-      const TB x357 = s.QC[1] * x39 + s.WQ[1] * x45;
-      const TB x358 = s.QC[1] * x45 + s.WQ[1] * x51;
-      const TB x386 = s.QC[1] * x96 + s.WQ[1] * x106 + r1 * x51;
-      const TB x521 = s.QC[2] * x96 + s.WQ[2] * x106;
-      const TB x376 = s.QC[1] * x38 + s.WQ[1] * x44;
-      const TB x377 = s.QC[1] * x44 + s.WQ[1] * x50;
-      const TB x378 = s.QC[1] * x50 + s.WQ[1] * x56;
-      const TB x379 = s.QC[1] * x56 + s.WQ[1] * x62;
-      const TB x514 = s.QC[2] * x56 + s.WQ[2] * x62 + r2 * x23;
-      const TB x490 = s.QC[2] * x20 + s.WQ[2] * x23 + r1 * x4;
-      const TB x481 = s.QC[2] * x3 + s.WQ[2] * x4;
-      const TB x380 = s.QC[1] * x75 + s.WQ[1] * x85;
-      const TB x381 = s.QC[1] * x85 + s.WQ[1] * x95;
-      const TB x382 = s.QC[1] * x95 + s.WQ[1] * x105;
-      const TB x383 = s.QC[1] * x105 + s.WQ[1] * x115;
-      const TB x518 = s.QC[2] * x105 + s.WQ[2] * x115;
-      const TB x384 = s.QC[1] * x76 + s.WQ[1] * x86 + r1 * x39;
-      const TB x923 = s.QC[1] * x357 + s.WQ[1] * x358 + e1 * (x39 - s.roe * x45);
-      const TB x385 = s.QC[1] * x86 + s.WQ[1] * x96 + r1 * x45;
-      const TB x943 = s.QC[1] * x384 + s.WQ[1] * x385 + e1 * (x76 - s.roe * x86) + r1 * x357;
-      acc100 += static_cast<double>(x943);
-      const TB x944 = s.QC[1] * x385 + s.WQ[1] * x386 + e1 * (x86 - s.roe * x96) + r1 * x358;
-      const TB x1622 = s.QC[1] * x943 + s.WQ[1] * x944 + e2 * (x384 - s.roe * x385) + r1 * x923;
-      acc379 += static_cast<double>(x1622);
-      const TB x1687 = s.QC[2] * x943 + s.WQ[2] * x944;
-      acc410 += static_cast<double>(x1687);
-      const TB x1044 = s.QC[2] * x385 + s.WQ[2] * x386;
-      const TB x480 = s.QC[2] * x2 + s.WQ[2] * x3;
-      const TB x488 = s.QC[2] * x14 + s.WQ[2] * x17 + r1 * x2;
-      const TB x489 = s.QC[2] * x17 + s.WQ[2] * x20 + r1 * x3;
-      const TB x511 = s.QC[2] * x38 + s.WQ[2] * x44 + r2 * x14;
-      const TB x512 = s.QC[2] * x44 + s.WQ[2] * x50 + r2 * x17;
-      const TB x513 = s.QC[2] * x50 + s.WQ[2] * x56 + r2 * x20;
-      const TB x1139 = s.QC[2] * x513 + s.WQ[2] * x514 + e1 * (x50 - s.roe * x56) + r2 * x490;
-      const TB x1121 = s.QC[2] * x489 + s.WQ[2] * x490 + e1 * (x17 - s.roe * x20) + r1 * x481;
-      const TB x1115 = s.QC[2] * x480 + s.WQ[2] * x481 + e1 * (x2 - s.roe * x3);
-      const TB x1120 = s.QC[2] * x488 + s.WQ[2] * x489 + e1 * (x14 - s.roe * x17) + r1 * x480;
-      const TB x1802 = s.QC[2] * x1120 + s.WQ[2] * x1121 + e2 * (x488 - s.roe * x489) + r1 * x1115;
-      const TB x1137 = s.QC[2] * x511 + s.WQ[2] * x512 + e1 * (x38 - s.roe * x44) + r2 * x488;
-      acc160 += static_cast<double>(x1137);
-      const TB x1138 = s.QC[2] * x512 + s.WQ[2] * x513 + e1 * (x44 - s.roe * x50) + r2 * x489;
-      const TB x1814 = s.QC[2] * x1138 + s.WQ[2] * x1139 + e2 * (x512 - s.roe * x513) + r2 * x1121;
-      const TB x1813 = s.QC[2] * x1137 + s.WQ[2] * x1138 + e2 * (x511 - s.roe * x512) + r2 * x1120;
-      acc470 += static_cast<double>(x1813);
-      const TB x2304 = s.QC[2] * x1813 + s.WQ[2] * x1814 + e3 * (x1137 - s.roe * x1138) + r2 * x1802;
-      acc935 += static_cast<double>(x2304);
-      const TB x515 = s.QC[2] * x75 + s.WQ[2] * x85;
-      const TB x516 = s.QC[2] * x85 + s.WQ[2] * x95;
-      const TB x517 = s.QC[2] * x95 + s.WQ[2] * x105;
-      const TB x1142 = s.QC[2] * x517 + s.WQ[2] * x518 + e1 * (x95 - s.roe * x105);
-      const TB x519 = s.QC[2] * x76 + s.WQ[2] * x86;
-      const TB x520 = s.QC[2] * x86 + s.WQ[2] * x96;
-      const TB x1144 = s.QC[2] * x520 + s.WQ[2] * x521 + e1 * (x86 - s.roe * x96);
-      const TB x1143 = s.QC[2] * x519 + s.WQ[2] * x520 + e1 * (x76 - s.roe * x86);
-      acc162 += static_cast<double>(x1143);
-      const TB x1817 = s.QC[2] * x1143 + s.WQ[2] * x1144 + e2 * (x519 - s.roe * x520);
-      acc472 += static_cast<double>(x1817);
-      const TB x615 = s.QC[0] * x210 + s.WQ[0] * x211 + e1 * (x2 - s.roe * x3);
-      const TB x915 = s.QC[1] * x345 + s.WQ[1] * x346 + e1 * (x2 - s.roe * x3);
-      const TB x620 = s.QC[0] * x218 + s.WQ[0] * x219 + e1 * (x14 - s.roe * x17);
-      const TB x920 = s.QC[1] * x353 + s.WQ[1] * x354 + e1 * (x14 - s.roe * x17);
-      const TB x621 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x17 - s.roe * x20);
-      const TB x921 = s.QC[1] * x354 + s.WQ[1] * x355 + e1 * (x17 - s.roe * x20);
-      const TB x1347 = s.QC[2] * x620 + s.WQ[2] * x621 + r1 * x615;
-      const TB x1672 = s.QC[2] * x920 + s.WQ[2] * x921 + r1 * x915;
-      const TB x637 = s.QC[0] * x241 + s.WQ[0] * x242 + e1 * (x38 - s.roe * x44);
-      const TB x937 = s.QC[1] * x376 + s.WQ[1] * x377 + e1 * (x38 - s.roe * x44);
-      const TB x638 = s.QC[0] * x242 + s.WQ[0] * x243 + e1 * (x44 - s.roe * x50);
-      const TB x938 = s.QC[1] * x377 + s.WQ[1] * x378 + e1 * (x44 - s.roe * x50);
-      const TB x639 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x50 - s.roe * x56);
-      const TB x939 = s.QC[1] * x378 + s.WQ[1] * x379 + e1 * (x50 - s.roe * x56);
-      const TB x1039 = s.QC[2] * x378 + s.WQ[2] * x379 + r2 * x355;
-      const TB x1021 = s.QC[2] * x354 + s.WQ[2] * x355 + r1 * x346;
-      const TB x1015 = s.QC[2] * x345 + s.WQ[2] * x346;
-      const TB x1020 = s.QC[2] * x353 + s.WQ[2] * x354 + r1 * x345;
-      const TB x1737 = s.QC[2] * x1020 + s.WQ[2] * x1021 + e1 * (x353 - s.roe * x354) + r1 * x1015;
-      const TB x640 = s.QC[0] * x245 + s.WQ[0] * x246 + e1 * (x75 - s.roe * x85) + r3 * x222;
-      acc6 += static_cast<double>(x640);
-      const TB x641 = s.QC[0] * x246 + s.WQ[0] * x247 + e1 * (x85 - s.roe * x95) + r3 * x223;
-      const TB x642 = s.QC[0] * x247 + s.WQ[0] * x248 + e1 * (x95 - s.roe * x105) + r3 * x224;
-      const TB x724 = s.QC[1] * x223 + s.WQ[1] * x224;
-      const TB x942 = s.QC[1] * x382 + s.WQ[1] * x383 + e1 * (x95 - s.roe * x105);
-      const TB x1042 = s.QC[2] * x382 + s.WQ[2] * x383;
-      const TB x715 = s.QC[1] * x210 + s.WQ[1] * x211;
-      const TB x720 = s.QC[1] * x218 + s.WQ[1] * x219;
-      const TB x721 = s.QC[1] * x219 + s.WQ[1] * x220;
-      const TB x1477 = s.QC[2] * x720 + s.WQ[2] * x721 + r1 * x715;
-      const TB x723 = s.QC[1] * x222 + s.WQ[1] * x223;
-      const TB x737 = s.QC[1] * x241 + s.WQ[1] * x242;
-      const TB x738 = s.QC[1] * x242 + s.WQ[1] * x243;
-      const TB x739 = s.QC[1] * x243 + s.WQ[1] * x244;
-      const TB x839 = s.QC[2] * x243 + s.WQ[2] * x244 + r2 * x220;
-      const TB x821 = s.QC[2] * x219 + s.WQ[2] * x220 + r1 * x211;
-      const TB x815 = s.QC[2] * x210 + s.WQ[2] * x211;
-      const TB x820 = s.QC[2] * x218 + s.WQ[2] * x219 + r1 * x210;
-      const TB x1542 = s.QC[2] * x820 + s.WQ[2] * x821 + e1 * (x218 - s.roe * x219) + r1 * x815;
-      const TB x740 = s.QC[1] * x245 + s.WQ[1] * x246;
-      acc37 += static_cast<double>(x740);
-      const TB x741 = s.QC[1] * x246 + s.WQ[1] * x247;
-      const TB x742 = s.QC[1] * x247 + s.WQ[1] * x248;
-      const TB x842 = s.QC[2] * x247 + s.WQ[2] * x248;
-      const TB x743 = s.QC[1] * x249 + s.WQ[1] * x250 + r1 * x222;
-      acc38 += static_cast<double>(x743);
-      const TB x1414 = s.QC[1] * x723 + s.WQ[1] * x724 + e1 * (x222 - s.roe * x223);
-      const TB x744 = s.QC[1] * x250 + s.WQ[1] * x251 + r1 * x223;
-      const TB x1428 = s.QC[1] * x744 + s.WQ[1] * x745 + e1 * (x250 - s.roe * x251) + r1 * x724;
-      const TB x844 = s.QC[2] * x250 + s.WQ[2] * x251;
-      const TB x1427 = s.QC[1] * x743 + s.WQ[1] * x744 + e1 * (x249 - s.roe * x250) + r1 * x723;
-      acc286 += static_cast<double>(x1427);
-      const TB x2058 = s.QC[1] * x1427 + s.WQ[1] * x1428 + e2 * (x743 - s.roe * x744) + r1 * x1414;
-      acc689 += static_cast<double>(x2058);
-      const TB x1492 = s.QC[2] * x743 + s.WQ[2] * x744;
-      acc317 += static_cast<double>(x1492);
-      const TB x837 = s.QC[2] * x241 + s.WQ[2] * x242 + r2 * x218;
-      const TB x838 = s.QC[2] * x242 + s.WQ[2] * x243 + r2 * x219;
-      const TB x1554 = s.QC[2] * x838 + s.WQ[2] * x839 + e1 * (x242 - s.roe * x243) + r2 * x821;
-      const TB x1553 = s.QC[2] * x837 + s.WQ[2] * x838 + e1 * (x241 - s.roe * x242) + r2 * x820;
-      acc346 += static_cast<double>(x1553);
-      const TB x2149 = s.QC[2] * x1553 + s.WQ[2] * x1554 + e2 * (x837 - s.roe * x838) + r2 * x1542;
-      acc780 += static_cast<double>(x2149);
-      const TB x840 = s.QC[2] * x245 + s.WQ[2] * x246;
-      acc68 += static_cast<double>(x840);
-      const TB x841 = s.QC[2] * x246 + s.WQ[2] * x247;
-      const TB x1556 = s.QC[2] * x841 + s.WQ[2] * x842 + e1 * (x246 - s.roe * x247);
-      const TB x843 = s.QC[2] * x249 + s.WQ[2] * x250;
-      acc69 += static_cast<double>(x843);
-      const TB x1557 = s.QC[2] * x843 + s.WQ[2] * x844 + e1 * (x249 - s.roe * x250);
-      acc348 += static_cast<double>(x1557);
-      const TB x940 = s.QC[1] * x380 + s.WQ[1] * x381 + e1 * (x75 - s.roe * x85);
-      acc99 += static_cast<double>(x940);
-      const TB x1140 = s.QC[2] * x515 + s.WQ[2] * x516 + e1 * (x75 - s.roe * x85);
-      acc161 += static_cast<double>(x1140);
-      const TB x941 = s.QC[1] * x381 + s.WQ[1] * x382 + e1 * (x85 - s.roe * x95);
-      const TB x1141 = s.QC[2] * x516 + s.WQ[2] * x517 + e1 * (x85 - s.roe * x95);
-      const TB x1816 = s.QC[2] * x1141 + s.WQ[2] * x1142 + e2 * (x516 - s.roe * x517);
-      const TB x1815 = s.QC[2] * x1140 + s.WQ[2] * x1141 + e2 * (x515 - s.roe * x516);
-      acc471 += static_cast<double>(x1815);
-      const TB x2305 = s.QC[2] * x1815 + s.WQ[2] * x1816 + e3 * (x1140 - s.roe * x1141);
-      acc936 += static_cast<double>(x2305);
-      const TB x1037 = s.QC[2] * x376 + s.WQ[2] * x377 + r2 * x353;
-      const TB x1607 = s.QC[1] * x920 + s.WQ[1] * x921 + e2 * (x353 - s.roe * x354);
-      const TB x1038 = s.QC[2] * x377 + s.WQ[2] * x378 + r2 * x354;
-      const TB x1749 = s.QC[2] * x1038 + s.WQ[2] * x1039 + e1 * (x377 - s.roe * x378) + r2 * x1021;
-      const TB x1619 = s.QC[1] * x938 + s.WQ[1] * x939 + e2 * (x377 - s.roe * x378);
-      const TB x1684 = s.QC[2] * x938 + s.WQ[2] * x939 + r2 * x921;
-      const TB x1683 = s.QC[2] * x937 + s.WQ[2] * x938 + r2 * x920;
-      acc408 += static_cast<double>(x1683);
-      const TB x2242 = s.QC[2] * x1683 + s.WQ[2] * x1684 + e1 * (x937 - s.roe * x938) + r2 * x1672;
-      acc873 += static_cast<double>(x2242);
-      const TB x1748 = s.QC[2] * x1037 + s.WQ[2] * x1038 + e1 * (x376 - s.roe * x377) + r2 * x1020;
-      acc439 += static_cast<double>(x1748);
-      const TB x2273 = s.QC[2] * x1748 + s.WQ[2] * x1749 + e2 * (x1037 - s.roe * x1038) + r2 * x1737;
-      acc904 += static_cast<double>(x2273);
-      const TB x1618 = s.QC[1] * x937 + s.WQ[1] * x938 + e2 * (x376 - s.roe * x377);
-      acc377 += static_cast<double>(x1618);
-      const TB x2180 = s.QC[1] * x1618 + s.WQ[1] * x1619 + e3 * (x937 - s.roe * x938);
-      acc811 += static_cast<double>(x2180);
-      const TB x2211 = s.QC[2] * x1618 + s.WQ[2] * x1619 + r2 * x1607;
-      acc842 += static_cast<double>(x2211);
-      const TB x1040 = s.QC[2] * x380 + s.WQ[2] * x381;
-      acc130 += static_cast<double>(x1040);
-      const TB x1041 = s.QC[2] * x381 + s.WQ[2] * x382;
-      const TB x1751 = s.QC[2] * x1041 + s.WQ[2] * x1042 + e1 * (x381 - s.roe * x382);
-      const TB x1621 = s.QC[1] * x941 + s.WQ[1] * x942 + e2 * (x381 - s.roe * x382);
-      const TB x1686 = s.QC[2] * x941 + s.WQ[2] * x942;
-      const TB x1043 = s.QC[2] * x384 + s.WQ[2] * x385;
-      acc131 += static_cast<double>(x1043);
-      const TB x1752 = s.QC[2] * x1043 + s.WQ[2] * x1044 + e1 * (x384 - s.roe * x385);
-      acc441 += static_cast<double>(x1752);
-      const TB x1217 = s.QC[0] * x620 + s.WQ[0] * x621 + e2 * (x218 - s.roe * x219);
-      const TB x1412 = s.QC[1] * x720 + s.WQ[1] * x721 + e1 * (x218 - s.roe * x219);
-      const TB x1488 = s.QC[2] * x737 + s.WQ[2] * x738 + r2 * x720;
-      acc315 += static_cast<double>(x1488);
-      const TB x1489 = s.QC[2] * x738 + s.WQ[2] * x739 + r2 * x721;
-      const TB x2118 = s.QC[2] * x1488 + s.WQ[2] * x1489 + e1 * (x737 - s.roe * x738) + r2 * x1477;
-      acc749 += static_cast<double>(x2118);
-      const TB x1424 = s.QC[1] * x738 + s.WQ[1] * x739 + e1 * (x242 - s.roe * x243);
-      const TB x1229 = s.QC[0] * x638 + s.WQ[0] * x639 + e2 * (x242 - s.roe * x243);
-      const TB x1228 = s.QC[0] * x637 + s.WQ[0] * x638 + e2 * (x241 - s.roe * x242);
-      acc191 += static_cast<double>(x1228);
-      const TB x1423 = s.QC[1] * x737 + s.WQ[1] * x738 + e1 * (x241 - s.roe * x242);
-      acc284 += static_cast<double>(x1423);
-      const TB x2056 = s.QC[1] * x1423 + s.WQ[1] * x1424 + e2 * (x737 - s.roe * x738);
-      acc687 += static_cast<double>(x2056);
-      const TB x2087 = s.QC[2] * x1423 + s.WQ[2] * x1424 + r2 * x1412;
-      acc718 += static_cast<double>(x2087);
-      const TB x1932 = s.QC[2] * x1228 + s.WQ[2] * x1229 + r2 * x1217;
-      acc563 += static_cast<double>(x1932);
-      const TB x1870 = s.QC[0] * x1228 + s.WQ[0] * x1229 + e3 * (x637 - s.roe * x638);
-      acc501 += static_cast<double>(x1870);
-      const TB x1901 = s.QC[1] * x1228 + s.WQ[1] * x1229;
-      acc532 += static_cast<double>(x1901);
-      const TB x1230 = s.QC[0] * x640 + s.WQ[0] * x641 + e2 * (x245 - s.roe * x246) + r3 * x623;
-      acc192 += static_cast<double>(x1230);
-      const TB x1231 = s.QC[0] * x641 + s.WQ[0] * x642 + e2 * (x246 - s.roe * x247) + r3 * x624;
-      const TB x1871 = s.QC[0] * x1230 + s.WQ[0] * x1231 + e3 * (x640 - s.roe * x641) + r3 * x1219;
-      acc502 += static_cast<double>(x1871);
-      const TB x1426 = s.QC[1] * x741 + s.WQ[1] * x742 + e1 * (x246 - s.roe * x247);
-      const TB x1491 = s.QC[2] * x741 + s.WQ[2] * x742;
-      const TB x1902 = s.QC[1] * x1230 + s.WQ[1] * x1231;
-      acc533 += static_cast<double>(x1902);
-      const TB x1933 = s.QC[2] * x1230 + s.WQ[2] * x1231;
-      acc564 += static_cast<double>(x1933);
-      const TB x1282 = s.QC[1] * x620 + s.WQ[1] * x621;
-      const TB x1358 = s.QC[2] * x637 + s.WQ[2] * x638 + r2 * x620;
-      acc253 += static_cast<double>(x1358);
-      const TB x1359 = s.QC[2] * x638 + s.WQ[2] * x639 + r2 * x621;
-      const TB x2025 = s.QC[2] * x1358 + s.WQ[2] * x1359 + e1 * (x637 - s.roe * x638) + r2 * x1347;
-      acc656 += static_cast<double>(x2025);
-      const TB x1294 = s.QC[1] * x638 + s.WQ[1] * x639;
-      const TB x1284 = s.QC[1] * x623 + s.WQ[1] * x624;
-      const TB x1297 = s.QC[1] * x643 + s.WQ[1] * x644 + r1 * x623;
-      acc224 += static_cast<double>(x1297);
-      const TB x1298 = s.QC[1] * x644 + s.WQ[1] * x645 + r1 * x624;
-      const TB x1965 = s.QC[1] * x1297 + s.WQ[1] * x1298 + e1 * (x643 - s.roe * x644) + r1 * x1284;
-      acc596 += static_cast<double>(x1965);
-      const TB x1996 = s.QC[2] * x1297 + s.WQ[2] * x1298;
-      acc627 += static_cast<double>(x1996);
-      const TB x1363 = s.QC[2] * x644 + s.WQ[2] * x645;
-      const TB x1293 = s.QC[1] * x637 + s.WQ[1] * x638;
-      acc222 += static_cast<double>(x1293);
-      const TB x1963 = s.QC[1] * x1293 + s.WQ[1] * x1294 + e1 * (x637 - s.roe * x638);
-      acc594 += static_cast<double>(x1963);
-      const TB x1994 = s.QC[2] * x1293 + s.WQ[2] * x1294 + r2 * x1282;
-      acc625 += static_cast<double>(x1994);
-      const TB x1295 = s.QC[1] * x640 + s.WQ[1] * x641;
-      acc223 += static_cast<double>(x1295);
-      const TB x1296 = s.QC[1] * x641 + s.WQ[1] * x642;
-      const TB x1361 = s.QC[2] * x641 + s.WQ[2] * x642;
-      const TB x1964 = s.QC[1] * x1295 + s.WQ[1] * x1296 + e1 * (x640 - s.roe * x641);
-      acc595 += static_cast<double>(x1964);
-      const TB x1995 = s.QC[2] * x1295 + s.WQ[2] * x1296;
-      acc626 += static_cast<double>(x1995);
-      const TB x1360 = s.QC[2] * x640 + s.WQ[2] * x641;
-      acc254 += static_cast<double>(x1360);
-      const TB x2026 = s.QC[2] * x1360 + s.WQ[2] * x1361 + e1 * (x640 - s.roe * x641);
-      acc657 += static_cast<double>(x2026);
-      const TB x1362 = s.QC[2] * x643 + s.WQ[2] * x644;
-      acc255 += static_cast<double>(x1362);
-      const TB x2027 = s.QC[2] * x1362 + s.WQ[2] * x1363 + e1 * (x643 - s.roe * x644);
-      acc658 += static_cast<double>(x2027);
-      const TB x1425 = s.QC[1] * x740 + s.WQ[1] * x741 + e1 * (x245 - s.roe * x246);
-      acc285 += static_cast<double>(x1425);
-      const TB x1555 = s.QC[2] * x840 + s.WQ[2] * x841 + e1 * (x245 - s.roe * x246);
-      acc347 += static_cast<double>(x1555);
-      const TB x2150 = s.QC[2] * x1555 + s.WQ[2] * x1556 + e2 * (x840 - s.roe * x841);
-      acc781 += static_cast<double>(x2150);
-      const TB x2057 = s.QC[1] * x1425 + s.WQ[1] * x1426 + e2 * (x740 - s.roe * x741);
-      acc688 += static_cast<double>(x2057);
-      const TB x2088 = s.QC[2] * x1425 + s.WQ[2] * x1426;
-      acc719 += static_cast<double>(x2088);
-      const TB x1490 = s.QC[2] * x740 + s.WQ[2] * x741;
-      acc316 += static_cast<double>(x1490);
-      const TB x2119 = s.QC[2] * x1490 + s.WQ[2] * x1491 + e1 * (x740 - s.roe * x741);
-      acc750 += static_cast<double>(x2119);
-      const TB x1620 = s.QC[1] * x940 + s.WQ[1] * x941 + e2 * (x380 - s.roe * x381);
-      acc378 += static_cast<double>(x1620);
-      const TB x1750 = s.QC[2] * x1040 + s.WQ[2] * x1041 + e1 * (x380 - s.roe * x381);
-      acc440 += static_cast<double>(x1750);
-      const TB x2274 = s.QC[2] * x1750 + s.WQ[2] * x1751 + e2 * (x1040 - s.roe * x1041);
-      acc905 += static_cast<double>(x2274);
-      const TB x2181 = s.QC[1] * x1620 + s.WQ[1] * x1621 + e3 * (x940 - s.roe * x941);
-      acc812 += static_cast<double>(x2181);
-      const TB x2212 = s.QC[2] * x1620 + s.WQ[2] * x1621;
-      acc843 += static_cast<double>(x2212);
-      const TB x1685 = s.QC[2] * x940 + s.WQ[2] * x941;
-      acc409 += static_cast<double>(x1685);
-      const TB x2243 = s.QC[2] * x1685 + s.WQ[2] * x1686 + e1 * (x940 - s.roe * x941);
-      acc874 += static_cast<double>(x2243);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(160 * R + r) * a.acc_stride + q] = acc160;
-  a.acc[(191 * R + r) * a.acc_stride + q] = acc191;
-  a.acc[(222 * R + r) * a.acc_stride + q] = acc222;
-  a.acc[(253 * R + r) * a.acc_stride + q] = acc253;
-  a.acc[(284 * R + r) * a.acc_stride + q] = acc284;
-  a.acc[(315 * R + r) * a.acc_stride + q] = acc315;
-  a.acc[(346 * R + r) * a.acc_stride + q] = acc346;
-  a.acc[(377 * R + r) * a.acc_stride + q] = acc377;
-  a.acc[(408 * R + r) * a.acc_stride + q] = acc408;
-  a.acc[(439 * R + r) * a.acc_stride + q] = acc439;
-  a.acc[(470 * R + r) * a.acc_stride + q] = acc470;
-  a.acc[(501 * R + r) * a.acc_stride + q] = acc501;
-  a.acc[(532 * R + r) * a.acc_stride + q] = acc532;
-  a.acc[(563 * R + r) * a.acc_stride + q] = acc563;
-  a.acc[(594 * R + r) * a.acc_stride + q] = acc594;
-  a.acc[(625 * R + r) * a.acc_stride + q] = acc625;
-  a.acc[(656 * R + r) * a.acc_stride + q] = acc656;
-  a.acc[(687 * R + r) * a.acc_stride + q] = acc687;
-  a.acc[(718 * R + r) * a.acc_stride + q] = acc718;
-  a.acc[(749 * R + r) * a.acc_stride + q] = acc749;
-  a.acc[(780 * R + r) * a.acc_stride + q] = acc780;
-  a.acc[(811 * R + r) * a.acc_stride + q] = acc811;
-  a.acc[(842 * R + r) * a.acc_stride + q] = acc842;
-  a.acc[(873 * R + r) * a.acc_stride + q] = acc873;
-  a.acc[(904 * R + r) * a.acc_stride + q] = acc904;
-  a.acc[(935 * R + r) * a.acc_stride + q] = acc935;
-  a.acc[(6 * R + r) * a.acc_stride + q] = acc6;
-  a.acc[(37 * R + r) * a.acc_stride + q] = acc37;
-  a.acc[(68 * R + r) * a.acc_stride + q] = acc68;
-  a.acc[(99 * R + r) * a.acc_stride + q] = acc99;
-  a.acc[(130 * R + r) * a.acc_stride + q] = acc130;
-  a.acc[(161 * R + r) * a.acc_stride + q] = acc161;
-  a.acc[(192 * R + r) * a.acc_stride + q] = acc192;
-  a.acc[(223 * R + r) * a.acc_stride + q] = acc223;
-  a.acc[(254 * R + r) * a.acc_stride + q] = acc254;
-  a.acc[(285 * R + r) * a.acc_stride + q] = acc285;
-  a.acc[(316 * R + r) * a.acc_stride + q] = acc316;
-  a.acc[(347 * R + r) * a.acc_stride + q] = acc347;
-  a.acc[(378 * R + r) * a.acc_stride + q] = acc378;
-  a.acc[(409 * R + r) * a.acc_stride + q] = acc409;
-  a.acc[(440 * R + r) * a.acc_stride + q] = acc440;
-  a.acc[(471 * R + r) * a.acc_stride + q] = acc471;
-  a.acc[(502 * R + r) * a.acc_stride + q] = acc502;
-  a.acc[(533 * R + r) * a.acc_stride + q] = acc533;
-  a.acc[(564 * R + r) * a.acc_stride + q] = acc564;
-  a.acc[(595 * R + r) * a.acc_stride + q] = acc595;
-  a.acc[(626 * R + r) * a.acc_stride + q] = acc626;
-  a.acc[(657 * R + r) * a.acc_stride + q] = acc657;
-  a.acc[(688 * R + r) * a.acc_stride + q] = acc688;
-  a.acc[(719 * R + r) * a.acc_stride + q] = acc719;
-  a.acc[(750 * R + r) * a.acc_stride + q] = acc750;
-  a.acc[(781 * R + r) * a.acc_stride + q] = acc781;
-  a.acc[(812 * R + r) * a.acc_stride + q] = acc812;
-  a.acc[(843 * R + r) * a.acc_stride + q] = acc843;
-  a.acc[(874 * R + r) * a.acc_stride + q] = acc874;
-  a.acc[(905 * R + r) * a.acc_stride + q] = acc905;
-  a.acc[(936 * R + r) * a.acc_stride + q] = acc936;
-  a.acc[(7 * R + r) * a.acc_stride + q] = acc7;
-  a.acc[(38 * R + r) * a.acc_stride + q] = acc38;
-  a.acc[(69 * R + r) * a.acc_stride + q] = acc69;
-  a.acc[(100 * R + r) * a.acc_stride + q] = acc100;
-  a.acc[(131 * R + r) * a.acc_stride + q] = acc131;
-  a.acc[(162 * R + r) * a.acc_stride + q] = acc162;
-  a.acc[(193 * R + r) * a.acc_stride + q] = acc193;
-  a.acc[(224 * R + r) * a.acc_stride + q] = acc224;
-  a.acc[(255 * R + r) * a.acc_stride + q] = acc255;
-  a.acc[(286 * R + r) * a.acc_stride + q] = acc286;
-  a.acc[(317 * R + r) * a.acc_stride + q] = acc317;
-  a.acc[(348 * R + r) * a.acc_stride + q] = acc348;
-  a.acc[(379 * R + r) * a.acc_stride + q] = acc379;
-  a.acc[(410 * R + r) * a.acc_stride + q] = acc410;
-  a.acc[(441 * R + r) * a.acc_stride + q] = acc441;
-  a.acc[(472 * R + r) * a.acc_stride + q] = acc472;
-  a.acc[(503 * R + r) * a.acc_stride + q] = acc503;
-  a.acc[(534 * R + r) * a.acc_stride + q] = acc534;
-  a.acc[(565 * R + r) * a.acc_stride + q] = acc565;
-  a.acc[(596 * R + r) * a.acc_stride + q] = acc596;
-  a.acc[(627 * R + r) * a.acc_stride + q] = acc627;
-  a.acc[(658 * R + r) * a.acc_stride + q] = acc658;
-  a.acc[(689 * R + r) * a.acc_stride + q] = acc689;
-}
-// class (22|22) split 3/12: 334 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_3(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc720 = 0.0;
-  double acc751 = 0.0;
-  double acc782 = 0.0;
-  double acc813 = 0.0;
-  double acc844 = 0.0;
-  double acc875 = 0.0;
-  double acc906 = 0.0;
-  double acc937 = 0.0;
-  double acc8 = 0.0;
-  double acc39 = 0.0;
-  double acc70 = 0.0;
-  double acc101 = 0.0;
-  double acc132 = 0.0;
-  double acc163 = 0.0;
-  double acc194 = 0.0;
-  double acc225 = 0.0;
-  double acc256 = 0.0;
-  double acc287 = 0.0;
-  double acc318 = 0.0;
-  double acc349 = 0.0;
-  double acc380 = 0.0;
-  double acc411 = 0.0;
-  double acc442 = 0.0;
-  double acc473 = 0.0;
-  double acc504 = 0.0;
-  double acc535 = 0.0;
-  double acc566 = 0.0;
-  double acc597 = 0.0;
-  double acc628 = 0.0;
-  double acc659 = 0.0;
-  double acc690 = 0.0;
-  double acc721 = 0.0;
-  double acc752 = 0.0;
-  double acc783 = 0.0;
-  double acc814 = 0.0;
-  double acc845 = 0.0;
-  double acc876 = 0.0;
-  double acc907 = 0.0;
-  double acc938 = 0.0;
-  double acc9 = 0.0;
-  double acc40 = 0.0;
-  double acc71 = 0.0;
-  double acc102 = 0.0;
-  double acc133 = 0.0;
-  double acc164 = 0.0;
-  double acc195 = 0.0;
-  double acc226 = 0.0;
-  double acc257 = 0.0;
-  double acc288 = 0.0;
-  double acc319 = 0.0;
-  double acc350 = 0.0;
-  double acc381 = 0.0;
-  double acc412 = 0.0;
-  double acc443 = 0.0;
-  double acc474 = 0.0;
-  double acc505 = 0.0;
-  double acc536 = 0.0;
-  double acc567 = 0.0;
-  double acc598 = 0.0;
-  double acc629 = 0.0;
-  double acc660 = 0.0;
-  double acc691 = 0.0;
-  double acc722 = 0.0;
-  double acc753 = 0.0;
-  double acc784 = 0.0;
-  double acc815 = 0.0;
-  double acc846 = 0.0;
-  double acc877 = 0.0;
-  double acc908 = 0.0;
-  double acc939 = 0.0;
-  double acc10 = 0.0;
-  double acc41 = 0.0;
-  double acc72 = 0.0;
-  double acc103 = 0.0;
-  double acc134 = 0.0;
-  double acc165 = 0.0;
-  double acc196 = 0.0;
-  double acc227 = 0.0;
-  double acc258 = 0.0;
-  double acc289 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x33 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x63 = s.PA[0] * x24 + s.WP[0] * x27 + z1 * (x5 - s.roz * x6);
-      const TB x57 = s.PA[0] * x21 + s.WP[0] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x64 = s.PA[1] * x24 + s.WP[1] * x27;
-      const TB x34 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x39 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x43 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x40 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x78 = s.PA[1] * x34 + s.WP[1] * x40 + z1 * (x9 - s.roz * x12);
-      const TB x79 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x41 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x45 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x88 = s.PA[1] * x40 + s.WP[1] * x46 + z1 * (x12 - s.roz * x15);
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x51 = s.PA[0] * x18 + s.WP[0] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x118 = s.PA[1] * x58 + s.WP[1] * x64 + z1 * (x21 - s.roz * x24);
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x76 = s.PA[1] * x33 + s.WP[1] * x39;
-      const TB x77 = s.PA[2] * x33 + s.WP[2] * x39;
-      const TB x86 = s.PA[1] * x39 + s.WP[1] * x45;
-      const TB x87 = s.PA[2] * x39 + s.WP[2] * x45;
-      const TB x89 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x96 = s.PA[1] * x45 + s.WP[1] * x51;
-      const TB x97 = s.PA[2] * x45 + s.WP[2] * x51;
-      const TB x98 = s.PA[1] * x46 + s.WP[1] * x52 + z1 * (x15 - s.roz * x18);
-      const TB x99 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x106 = s.PA[1] * x51 + s.WP[1] * x57;
-      const TB x107 = s.PA[2] * x51 + s.WP[2] * x57;
-      const TB x108 = s.PA[1] * x52 + s.WP[1] * x58 + z1 * (x18 - s.roz * x21);
-      const TB x109 = s.PA[2] * x52 + s.WP[2] * x58;
-      const TB x263 = s.QC[0] * x99 + s.WQ[0] * x109 + r1 * x55;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x116 = s.PA[1] * x57 + s.WP[1] * x63;
-      const TB x117 = s.PA[2] * x57 + s.WP[2] * x63;
-      const TB x211 = s.QC[0] * x3 + s.WQ[0] * x4;
-      const TB x213 = s.QC[0] * x15 + s.WQ[0] * x18 + r1 * x3;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x617 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x15 - s.roe * x18) + r1 * x211;
-      const TB x216 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x219 = s.QC[0] * x17 + s.WQ[0] * x20;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x256 = s.QC[0] * x107 + s.WQ[0] * x117 + r2 * x59;
-      const TB x222 = s.QC[0] * x39 + s.WQ[0] * x45 + r2 * x15;
-      const TB x223 = s.QC[0] * x45 + s.WQ[0] * x51 + r2 * x18;
-      const TB x224 = s.QC[0] * x51 + s.WQ[0] * x57 + r2 * x21;
-      const TB x226 = s.QC[0] * x40 + s.WQ[0] * x46 + r1 * x16;
-      const TB x619 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x16 - s.roe * x19);
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x626 = s.QC[0] * x226 + s.WQ[0] * x227 + e1 * (x40 - s.roe * x46) + r1 * x216;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x1221 = s.QC[0] * x626 + s.WQ[0] * x627 + e2 * (x226 - s.roe * x227) + r1 * x619;
-      const TB x230 = s.QC[0] * x41 + s.WQ[0] * x47 + r1 * x17;
-      const TB x621 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x17 - s.roe * x20);
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x629 = s.QC[0] * x230 + s.WQ[0] * x231 + e1 * (x41 - s.roe * x47) + r1 * x219;
-      const TB x630 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x47 - s.roe * x53) + r1 * x220;
-      const TB x255 = s.QC[0] * x97 + s.WQ[0] * x107 + r2 * x53;
-      const TB x648 = s.QC[0] * x255 + s.WQ[0] * x256 + e1 * (x97 - s.roe * x107) + r2 * x232;
-      const TB x1223 = s.QC[0] * x629 + s.WQ[0] * x630 + e2 * (x230 - s.roe * x231) + r1 * x621;
-      const TB x234 = s.QC[0] * x42 + s.WQ[0] * x48;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x260 = s.QC[0] * x108 + s.WQ[0] * x118 + r1 * x60;
-      const TB x238 = s.QC[0] * x43 + s.WQ[0] * x49;
-      const TB x249 = s.QC[0] * x76 + s.WQ[0] * x86 + r2 * x40;
-      const TB x250 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x46;
-      const TB x251 = s.QC[0] * x96 + s.WQ[0] * x106 + r2 * x52;
-      const TB x252 = s.QC[0] * x106 + s.WQ[0] * x116 + r2 * x58;
-      const TB x253 = s.QC[0] * x77 + s.WQ[0] * x87 + r2 * x41;
-      const TB x254 = s.QC[0] * x87 + s.WQ[0] * x97 + r2 * x47;
-      const TB x257 = s.QC[0] * x78 + s.WQ[0] * x88 + r1 * x42;
-      const TB x632 = s.QC[0] * x234 + s.WQ[0] * x235 + e1 * (x42 - s.roe * x48);
-      const TB x258 = s.QC[0] * x88 + s.WQ[0] * x98 + r1 * x48;
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x259 = s.QC[0] * x98 + s.WQ[0] * x108 + r1 * x54;
-      const TB x651 = s.QC[0] * x259 + s.WQ[0] * x260 + e1 * (x98 - s.roe * x108) + r1 * x236;
-      const TB x261 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x43;
-      const TB x635 = s.QC[0] * x238 + s.WQ[0] * x239 + e1 * (x43 - s.roe * x49);
-      const TB x262 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x49;
-      const TB x652 = s.QC[0] * x261 + s.WQ[0] * x262 + e1 * (x79 - s.roe * x89) + r1 * x238;
-      acc10 += static_cast<double>(x652);
-      const TB x653 = s.QC[0] * x262 + s.WQ[0] * x263 + e1 * (x89 - s.roe * x99) + r1 * x239;
-      const TB x1238 = s.QC[0] * x652 + s.WQ[0] * x653 + e2 * (x261 - s.roe * x262) + r1 * x635;
-      acc196 += static_cast<double>(x1238);
-      const TB x753 = s.QC[1] * x262 + s.WQ[1] * x263 + r1 * x231;
-      const TB x852 = s.QC[2] * x261 + s.WQ[2] * x262 + r1 * x226;
-      acc72 += static_cast<double>(x852);
-      const TB x1303 = s.QC[1] * x652 + s.WQ[1] * x653 + r1 * x629;
-      acc227 += static_cast<double>(x1303);
-      const TB x1368 = s.QC[2] * x652 + s.WQ[2] * x653 + r1 * x626;
-      acc258 += static_cast<double>(x1368);
-      const TB x348 = s.QC[1] * x15 + s.WQ[1] * x18;
-      const TB x349 = s.QC[1] * x18 + s.WQ[1] * x21;
-      const TB x363 = s.QC[1] * x52 + s.WQ[1] * x58 + r1 * x21;
-      const TB x395 = s.QC[1] * x108 + s.WQ[1] * x118 + r2 * x58;
-      const TB x530 = s.QC[2] * x108 + s.WQ[2] * x118;
-      const TB x357 = s.QC[1] * x39 + s.WQ[1] * x45;
-      const TB x358 = s.QC[1] * x45 + s.WQ[1] * x51;
-      const TB x359 = s.QC[1] * x51 + s.WQ[1] * x57;
-      const TB x361 = s.QC[1] * x40 + s.WQ[1] * x46 + r1 * x15;
-      const TB x917 = s.QC[1] * x348 + s.WQ[1] * x349 + e1 * (x15 - s.roe * x18);
-      const TB x362 = s.QC[1] * x46 + s.WQ[1] * x52 + r1 * x18;
-      const TB x926 = s.QC[1] * x361 + s.WQ[1] * x362 + e1 * (x40 - s.roe * x46) + r1 * x348;
-      const TB x927 = s.QC[1] * x362 + s.WQ[1] * x363 + e1 * (x46 - s.roe * x52) + r1 * x349;
-      const TB x394 = s.QC[1] * x98 + s.WQ[1] * x108 + r2 * x52;
-      const TB x951 = s.QC[1] * x394 + s.WQ[1] * x395 + e1 * (x98 - s.roe * x108) + r2 * x363;
-      const TB x1051 = s.QC[2] * x394 + s.WQ[2] * x395;
-      const TB x1611 = s.QC[1] * x926 + s.WQ[1] * x927 + e2 * (x361 - s.roe * x362) + r1 * x917;
-      const TB x365 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x396 = s.QC[1] * x79 + s.WQ[1] * x89 + r1 * x41;
-      const TB x397 = s.QC[1] * x89 + s.WQ[1] * x99 + r1 * x47;
-      const TB x952 = s.QC[1] * x396 + s.WQ[1] * x397 + e1 * (x79 - s.roe * x89) + r1 * x365;
-      acc103 += static_cast<double>(x952);
-      const TB x1052 = s.QC[2] * x396 + s.WQ[2] * x397 + r1 * x361;
-      acc134 += static_cast<double>(x1052);
-      const TB x532 = s.QC[2] * x89 + s.WQ[2] * x99 + r1 * x46;
-      const TB x384 = s.QC[1] * x76 + s.WQ[1] * x86 + r1 * x39;
-      const TB x385 = s.QC[1] * x86 + s.WQ[1] * x96 + r1 * x45;
-      const TB x386 = s.QC[1] * x96 + s.WQ[1] * x106 + r1 * x51;
-      const TB x387 = s.QC[1] * x106 + s.WQ[1] * x116 + r1 * x57;
-      const TB x522 = s.QC[2] * x106 + s.WQ[2] * x116;
-      const TB x388 = s.QC[1] * x77 + s.WQ[1] * x87;
-      const TB x389 = s.QC[1] * x87 + s.WQ[1] * x97;
-      const TB x390 = s.QC[1] * x97 + s.WQ[1] * x107;
-      const TB x391 = s.QC[1] * x107 + s.WQ[1] * x117;
-      const TB x526 = s.QC[2] * x107 + s.WQ[2] * x117 + r1 * x57;
-      const TB x494 = s.QC[2] * x51 + s.WQ[2] * x57;
-      const TB x392 = s.QC[1] * x78 + s.WQ[1] * x88 + r2 * x40;
-      const TB x393 = s.QC[1] * x88 + s.WQ[1] * x98 + r2 * x46;
-      const TB x496 = s.QC[2] * x40 + s.WQ[2] * x46;
-      const TB x531 = s.QC[2] * x79 + s.WQ[2] * x89 + r1 * x40;
-      const TB x1152 = s.QC[2] * x531 + s.WQ[2] * x532 + e1 * (x79 - s.roe * x89) + r1 * x496;
-      acc165 += static_cast<double>(x1152);
-      const TB x949 = s.QC[1] * x392 + s.WQ[1] * x393 + e1 * (x78 - s.roe * x88) + r2 * x361;
-      acc102 += static_cast<double>(x949);
-      const TB x950 = s.QC[1] * x393 + s.WQ[1] * x394 + e1 * (x88 - s.roe * x98) + r2 * x362;
-      const TB x1626 = s.QC[1] * x949 + s.WQ[1] * x950 + e2 * (x392 - s.roe * x393) + r2 * x926;
-      acc381 += static_cast<double>(x1626);
-      const TB x1627 = s.QC[1] * x950 + s.WQ[1] * x951 + e2 * (x393 - s.roe * x394) + r2 * x927;
-      const TB x2184 = s.QC[1] * x1626 + s.WQ[1] * x1627 + e3 * (x949 - s.roe * x950) + r2 * x1611;
-      acc815 += static_cast<double>(x2184);
-      const TB x2215 = s.QC[2] * x1626 + s.WQ[2] * x1627;
-      acc846 += static_cast<double>(x2215);
-      const TB x1692 = s.QC[2] * x950 + s.WQ[2] * x951;
-      const TB x492 = s.QC[2] * x39 + s.WQ[2] * x45;
-      const TB x493 = s.QC[2] * x45 + s.WQ[2] * x51;
-      const TB x519 = s.QC[2] * x76 + s.WQ[2] * x86;
-      const TB x520 = s.QC[2] * x86 + s.WQ[2] * x96;
-      const TB x521 = s.QC[2] * x96 + s.WQ[2] * x106;
-      const TB x1145 = s.QC[2] * x521 + s.WQ[2] * x522 + e1 * (x96 - s.roe * x106);
-      const TB x945 = s.QC[1] * x386 + s.WQ[1] * x387 + e1 * (x96 - s.roe * x106) + r1 * x359;
-      const TB x1045 = s.QC[2] * x386 + s.WQ[2] * x387;
-      const TB x523 = s.QC[2] * x77 + s.WQ[2] * x87 + r1 * x39;
-      const TB x524 = s.QC[2] * x87 + s.WQ[2] * x97 + r1 * x45;
-      const TB x525 = s.QC[2] * x97 + s.WQ[2] * x107 + r1 * x51;
-      const TB x1148 = s.QC[2] * x525 + s.WQ[2] * x526 + e1 * (x97 - s.roe * x107) + r1 * x494;
-      const TB x948 = s.QC[1] * x390 + s.WQ[1] * x391 + e1 * (x97 - s.roe * x107);
-      const TB x1048 = s.QC[2] * x390 + s.WQ[2] * x391 + r1 * x359;
-      const TB x1124 = s.QC[2] * x493 + s.WQ[2] * x494 + e1 * (x45 - s.roe * x51);
-      const TB x527 = s.QC[2] * x78 + s.WQ[2] * x88;
-      const TB x528 = s.QC[2] * x88 + s.WQ[2] * x98;
-      const TB x529 = s.QC[2] * x98 + s.WQ[2] * x108;
-      const TB x1151 = s.QC[2] * x529 + s.WQ[2] * x530 + e1 * (x98 - s.roe * x108);
-      const TB x623 = s.QC[0] * x222 + s.WQ[0] * x223 + e1 * (x39 - s.roe * x45) + r2 * x213;
-      const TB x624 = s.QC[0] * x223 + s.WQ[0] * x224 + e1 * (x45 - s.roe * x51) + r2 * x214;
-      const TB x924 = s.QC[1] * x358 + s.WQ[1] * x359 + e1 * (x45 - s.roe * x51);
-      const TB x1024 = s.QC[2] * x358 + s.WQ[2] * x359;
-      const TB x646 = s.QC[0] * x253 + s.WQ[0] * x254 + e1 * (x77 - s.roe * x87) + r2 * x230;
-      acc8 += static_cast<double>(x646);
-      const TB x647 = s.QC[0] * x254 + s.WQ[0] * x255 + e1 * (x87 - s.roe * x97) + r2 * x231;
-      const TB x729 = s.QC[1] * x230 + s.WQ[1] * x231;
-      const TB x752 = s.QC[1] * x261 + s.WQ[1] * x262 + r1 * x230;
-      acc41 += static_cast<double>(x752);
-      const TB x1433 = s.QC[1] * x752 + s.WQ[1] * x753 + e1 * (x261 - s.roe * x262) + r1 * x729;
-      acc289 += static_cast<double>(x1433);
-      const TB x1234 = s.QC[0] * x646 + s.WQ[0] * x647 + e2 * (x253 - s.roe * x254) + r2 * x629;
-      acc194 += static_cast<double>(x1234);
-      const TB x1235 = s.QC[0] * x647 + s.WQ[0] * x648 + e2 * (x254 - s.roe * x255) + r2 * x630;
-      const TB x1873 = s.QC[0] * x1234 + s.WQ[0] * x1235 + e3 * (x646 - s.roe * x647) + r2 * x1223;
-      acc504 += static_cast<double>(x1873);
-      const TB x1904 = s.QC[1] * x1234 + s.WQ[1] * x1235;
-      acc535 += static_cast<double>(x1904);
-      const TB x649 = s.QC[0] * x257 + s.WQ[0] * x258 + e1 * (x78 - s.roe * x88) + r1 * x234;
-      acc9 += static_cast<double>(x649);
-      const TB x1149 = s.QC[2] * x527 + s.WQ[2] * x528 + e1 * (x78 - s.roe * x88);
-      acc164 += static_cast<double>(x1149);
-      const TB x1225 = s.QC[0] * x632 + s.WQ[0] * x633 + e2 * (x234 - s.roe * x235);
-      const TB x650 = s.QC[0] * x258 + s.WQ[0] * x259 + e1 * (x88 - s.roe * x98) + r1 * x235;
-      const TB x1150 = s.QC[2] * x528 + s.WQ[2] * x529 + e1 * (x88 - s.roe * x98);
-      const TB x1822 = s.QC[2] * x1150 + s.WQ[2] * x1151 + e2 * (x528 - s.roe * x529);
-      const TB x1821 = s.QC[2] * x1149 + s.WQ[2] * x1150 + e2 * (x527 - s.roe * x528);
-      acc474 += static_cast<double>(x1821);
-      const TB x2308 = s.QC[2] * x1821 + s.WQ[2] * x1822 + e3 * (x1149 - s.roe * x1150);
-      acc939 += static_cast<double>(x2308);
-      const TB x1236 = s.QC[0] * x649 + s.WQ[0] * x650 + e2 * (x257 - s.roe * x258) + r1 * x632;
-      acc195 += static_cast<double>(x1236);
-      const TB x1237 = s.QC[0] * x650 + s.WQ[0] * x651 + e2 * (x258 - s.roe * x259) + r1 * x633;
-      const TB x1874 = s.QC[0] * x1236 + s.WQ[0] * x1237 + e3 * (x649 - s.roe * x650) + r1 * x1225;
-      acc505 += static_cast<double>(x1874);
-      const TB x1905 = s.QC[1] * x1236 + s.WQ[1] * x1237 + r2 * x1221;
-      acc536 += static_cast<double>(x1905);
-      const TB x1936 = s.QC[2] * x1236 + s.WQ[2] * x1237;
-      acc567 += static_cast<double>(x1936);
-      const TB x717 = s.QC[1] * x213 + s.WQ[1] * x214;
-      const TB x726 = s.QC[1] * x226 + s.WQ[1] * x227 + r1 * x213;
-      const TB x727 = s.QC[1] * x227 + s.WQ[1] * x228 + r1 * x214;
-      const TB x751 = s.QC[1] * x259 + s.WQ[1] * x260 + r2 * x228;
-      const TB x851 = s.QC[2] * x259 + s.WQ[2] * x260;
-      const TB x1416 = s.QC[1] * x726 + s.WQ[1] * x727 + e1 * (x226 - s.roe * x227) + r1 * x717;
-      const TB x749 = s.QC[1] * x257 + s.WQ[1] * x258 + r2 * x226;
-      acc40 += static_cast<double>(x749);
-      const TB x750 = s.QC[1] * x258 + s.WQ[1] * x259 + r2 * x227;
-      const TB x1431 = s.QC[1] * x749 + s.WQ[1] * x750 + e1 * (x257 - s.roe * x258) + r2 * x726;
-      acc288 += static_cast<double>(x1431);
-      const TB x1432 = s.QC[1] * x750 + s.WQ[1] * x751 + e1 * (x258 - s.roe * x259) + r2 * x727;
-      const TB x2060 = s.QC[1] * x1431 + s.WQ[1] * x1432 + e2 * (x749 - s.roe * x750) + r2 * x1416;
-      acc691 += static_cast<double>(x2060);
-      const TB x2091 = s.QC[2] * x1431 + s.WQ[2] * x1432;
-      acc722 += static_cast<double>(x2091);
-      const TB x1497 = s.QC[2] * x750 + s.WQ[2] * x751;
-      const TB x723 = s.QC[1] * x222 + s.WQ[1] * x223;
-      const TB x724 = s.QC[1] * x223 + s.WQ[1] * x224;
-      const TB x743 = s.QC[1] * x249 + s.WQ[1] * x250 + r1 * x222;
-      const TB x744 = s.QC[1] * x250 + s.WQ[1] * x251 + r1 * x223;
-      const TB x745 = s.QC[1] * x251 + s.WQ[1] * x252 + r1 * x224;
-      const TB x845 = s.QC[2] * x251 + s.WQ[2] * x252;
-      const TB x746 = s.QC[1] * x253 + s.WQ[1] * x254;
-      acc39 += static_cast<double>(x746);
-      const TB x747 = s.QC[1] * x254 + s.WQ[1] * x255;
-      const TB x748 = s.QC[1] * x255 + s.WQ[1] * x256;
-      const TB x848 = s.QC[2] * x255 + s.WQ[2] * x256 + r1 * x224;
-      const TB x824 = s.QC[2] * x223 + s.WQ[2] * x224;
-      const TB x823 = s.QC[2] * x222 + s.WQ[2] * x223;
-      const TB x843 = s.QC[2] * x249 + s.WQ[2] * x250;
-      const TB x844 = s.QC[2] * x250 + s.WQ[2] * x251;
-      const TB x1558 = s.QC[2] * x844 + s.WQ[2] * x845 + e1 * (x250 - s.roe * x251);
-      const TB x1428 = s.QC[1] * x744 + s.WQ[1] * x745 + e1 * (x250 - s.roe * x251) + r1 * x724;
-      const TB x1493 = s.QC[2] * x744 + s.WQ[2] * x745;
-      const TB x846 = s.QC[2] * x253 + s.WQ[2] * x254 + r1 * x222;
-      acc70 += static_cast<double>(x846);
-      const TB x847 = s.QC[2] * x254 + s.WQ[2] * x255 + r1 * x223;
-      const TB x1560 = s.QC[2] * x847 + s.WQ[2] * x848 + e1 * (x254 - s.roe * x255) + r1 * x824;
-      const TB x1430 = s.QC[1] * x747 + s.WQ[1] * x748 + e1 * (x254 - s.roe * x255);
-      const TB x1495 = s.QC[2] * x747 + s.WQ[2] * x748 + r1 * x724;
-      const TB x1544 = s.QC[2] * x823 + s.WQ[2] * x824 + e1 * (x222 - s.roe * x223);
-      const TB x1559 = s.QC[2] * x846 + s.WQ[2] * x847 + e1 * (x253 - s.roe * x254) + r1 * x823;
-      acc349 += static_cast<double>(x1559);
-      const TB x2152 = s.QC[2] * x1559 + s.WQ[2] * x1560 + e2 * (x846 - s.roe * x847) + r1 * x1544;
-      acc783 += static_cast<double>(x2152);
-      const TB x1429 = s.QC[1] * x746 + s.WQ[1] * x747 + e1 * (x253 - s.roe * x254);
-      acc287 += static_cast<double>(x1429);
-      const TB x2059 = s.QC[1] * x1429 + s.WQ[1] * x1430 + e2 * (x746 - s.roe * x747);
-      acc690 += static_cast<double>(x2059);
-      const TB x849 = s.QC[2] * x257 + s.WQ[2] * x258;
-      acc71 += static_cast<double>(x849);
-      const TB x850 = s.QC[2] * x258 + s.WQ[2] * x259;
-      const TB x1562 = s.QC[2] * x850 + s.WQ[2] * x851 + e1 * (x258 - s.roe * x259);
-      const TB x1561 = s.QC[2] * x849 + s.WQ[2] * x850 + e1 * (x257 - s.roe * x258);
-      acc350 += static_cast<double>(x1561);
-      const TB x2153 = s.QC[2] * x1561 + s.WQ[2] * x1562 + e2 * (x849 - s.roe * x850);
-      acc784 += static_cast<double>(x2153);
-      const TB x923 = s.QC[1] * x357 + s.WQ[1] * x358 + e1 * (x39 - s.roe * x45);
-      const TB x1123 = s.QC[2] * x492 + s.WQ[2] * x493 + e1 * (x39 - s.roe * x45);
-      const TB x943 = s.QC[1] * x384 + s.WQ[1] * x385 + e1 * (x76 - s.roe * x86) + r1 * x357;
-      const TB x1143 = s.QC[2] * x519 + s.WQ[2] * x520 + e1 * (x76 - s.roe * x86);
-      const TB x944 = s.QC[1] * x385 + s.WQ[1] * x386 + e1 * (x86 - s.roe * x96) + r1 * x358;
-      const TB x1144 = s.QC[2] * x520 + s.WQ[2] * x521 + e1 * (x86 - s.roe * x96);
-      const TB x1818 = s.QC[2] * x1144 + s.WQ[2] * x1145 + e2 * (x520 - s.roe * x521);
-      const TB x1817 = s.QC[2] * x1143 + s.WQ[2] * x1144 + e2 * (x519 - s.roe * x520);
-      const TB x2306 = s.QC[2] * x1817 + s.WQ[2] * x1818 + e3 * (x1143 - s.roe * x1144);
-      acc937 += static_cast<double>(x2306);
-      const TB x946 = s.QC[1] * x388 + s.WQ[1] * x389 + e1 * (x77 - s.roe * x87);
-      acc101 += static_cast<double>(x946);
-      const TB x1146 = s.QC[2] * x523 + s.WQ[2] * x524 + e1 * (x77 - s.roe * x87) + r1 * x492;
-      acc163 += static_cast<double>(x1146);
-      const TB x1804 = s.QC[2] * x1123 + s.WQ[2] * x1124 + e2 * (x492 - s.roe * x493);
-      const TB x1147 = s.QC[2] * x524 + s.WQ[2] * x525 + e1 * (x87 - s.roe * x97) + r1 * x493;
-      const TB x1820 = s.QC[2] * x1147 + s.WQ[2] * x1148 + e2 * (x524 - s.roe * x525) + r1 * x1124;
-      const TB x1819 = s.QC[2] * x1146 + s.WQ[2] * x1147 + e2 * (x523 - s.roe * x524) + r1 * x1123;
-      acc473 += static_cast<double>(x1819);
-      const TB x2307 = s.QC[2] * x1819 + s.WQ[2] * x1820 + e3 * (x1146 - s.roe * x1147) + r1 * x1804;
-      acc938 += static_cast<double>(x2307);
-      const TB x947 = s.QC[1] * x389 + s.WQ[1] * x390 + e1 * (x87 - s.roe * x97);
-      const TB x1023 = s.QC[2] * x357 + s.WQ[2] * x358;
-      const TB x1043 = s.QC[2] * x384 + s.WQ[2] * x385;
-      const TB x1044 = s.QC[2] * x385 + s.WQ[2] * x386;
-      const TB x1753 = s.QC[2] * x1044 + s.WQ[2] * x1045 + e1 * (x385 - s.roe * x386);
-      const TB x1623 = s.QC[1] * x944 + s.WQ[1] * x945 + e2 * (x385 - s.roe * x386) + r1 * x924;
-      const TB x1688 = s.QC[2] * x944 + s.WQ[2] * x945;
-      const TB x1046 = s.QC[2] * x388 + s.WQ[2] * x389 + r1 * x357;
-      acc132 += static_cast<double>(x1046);
-      const TB x1047 = s.QC[2] * x389 + s.WQ[2] * x390 + r1 * x358;
-      const TB x1755 = s.QC[2] * x1047 + s.WQ[2] * x1048 + e1 * (x389 - s.roe * x390) + r1 * x1024;
-      const TB x1625 = s.QC[1] * x947 + s.WQ[1] * x948 + e2 * (x389 - s.roe * x390);
-      const TB x1690 = s.QC[2] * x947 + s.WQ[2] * x948 + r1 * x924;
-      const TB x1739 = s.QC[2] * x1023 + s.WQ[2] * x1024 + e1 * (x357 - s.roe * x358);
-      const TB x1609 = s.QC[1] * x923 + s.WQ[1] * x924 + e2 * (x357 - s.roe * x358);
-      const TB x1674 = s.QC[2] * x923 + s.WQ[2] * x924;
-      const TB x1754 = s.QC[2] * x1046 + s.WQ[2] * x1047 + e1 * (x388 - s.roe * x389) + r1 * x1023;
-      acc442 += static_cast<double>(x1754);
-      const TB x2276 = s.QC[2] * x1754 + s.WQ[2] * x1755 + e2 * (x1046 - s.roe * x1047) + r1 * x1739;
-      acc907 += static_cast<double>(x2276);
-      const TB x1624 = s.QC[1] * x946 + s.WQ[1] * x947 + e2 * (x388 - s.roe * x389);
-      acc380 += static_cast<double>(x1624);
-      const TB x2183 = s.QC[1] * x1624 + s.WQ[1] * x1625 + e3 * (x946 - s.roe * x947);
-      acc814 += static_cast<double>(x2183);
-      const TB x2214 = s.QC[2] * x1624 + s.WQ[2] * x1625 + r1 * x1609;
-      acc845 += static_cast<double>(x2214);
-      const TB x1049 = s.QC[2] * x392 + s.WQ[2] * x393;
-      acc133 += static_cast<double>(x1049);
-      const TB x1050 = s.QC[2] * x393 + s.WQ[2] * x394;
-      const TB x1757 = s.QC[2] * x1050 + s.WQ[2] * x1051 + e1 * (x393 - s.roe * x394);
-      const TB x1756 = s.QC[2] * x1049 + s.WQ[2] * x1050 + e1 * (x392 - s.roe * x393);
-      acc443 += static_cast<double>(x1756);
-      const TB x2277 = s.QC[2] * x1756 + s.WQ[2] * x1757 + e2 * (x1049 - s.roe * x1050);
-      acc908 += static_cast<double>(x2277);
-      const TB x1219 = s.QC[0] * x623 + s.WQ[0] * x624 + e2 * (x222 - s.roe * x223) + r2 * x617;
-      const TB x1935 = s.QC[2] * x1234 + s.WQ[2] * x1235 + r1 * x1219;
-      acc566 += static_cast<double>(x1935);
-      const TB x1414 = s.QC[1] * x723 + s.WQ[1] * x724 + e1 * (x222 - s.roe * x223);
-      const TB x2090 = s.QC[2] * x1429 + s.WQ[2] * x1430 + r1 * x1414;
-      acc721 += static_cast<double>(x2090);
-      const TB x1286 = s.QC[1] * x626 + s.WQ[1] * x627 + r1 * x617;
-      const TB x1301 = s.QC[1] * x649 + s.WQ[1] * x650 + r2 * x626;
-      acc226 += static_cast<double>(x1301);
-      const TB x1302 = s.QC[1] * x650 + s.WQ[1] * x651 + r2 * x627;
-      const TB x1967 = s.QC[1] * x1301 + s.WQ[1] * x1302 + e1 * (x649 - s.roe * x650) + r2 * x1286;
-      acc598 += static_cast<double>(x1967);
-      const TB x1998 = s.QC[2] * x1301 + s.WQ[2] * x1302;
-      acc629 += static_cast<double>(x1998);
-      const TB x1367 = s.QC[2] * x650 + s.WQ[2] * x651;
-      const TB x1479 = s.QC[2] * x723 + s.WQ[2] * x724;
-      const TB x1284 = s.QC[1] * x623 + s.WQ[1] * x624;
-      const TB x1299 = s.QC[1] * x646 + s.WQ[1] * x647;
-      acc225 += static_cast<double>(x1299);
-      const TB x1300 = s.QC[1] * x647 + s.WQ[1] * x648;
-      const TB x1997 = s.QC[2] * x1299 + s.WQ[2] * x1300 + r1 * x1284;
-      acc628 += static_cast<double>(x1997);
-      const TB x1966 = s.QC[1] * x1299 + s.WQ[1] * x1300 + e1 * (x646 - s.roe * x647);
-      acc597 += static_cast<double>(x1966);
-      const TB x1365 = s.QC[2] * x647 + s.WQ[2] * x648 + r1 * x624;
-      const TB x1349 = s.QC[2] * x623 + s.WQ[2] * x624;
-      const TB x1364 = s.QC[2] * x646 + s.WQ[2] * x647 + r1 * x623;
-      acc256 += static_cast<double>(x1364);
-      const TB x2028 = s.QC[2] * x1364 + s.WQ[2] * x1365 + e1 * (x646 - s.roe * x647) + r1 * x1349;
-      acc659 += static_cast<double>(x2028);
-      const TB x1366 = s.QC[2] * x649 + s.WQ[2] * x650;
-      acc257 += static_cast<double>(x1366);
-      const TB x2029 = s.QC[2] * x1366 + s.WQ[2] * x1367 + e1 * (x649 - s.roe * x650);
-      acc660 += static_cast<double>(x2029);
-      const TB x1427 = s.QC[1] * x743 + s.WQ[1] * x744 + e1 * (x249 - s.roe * x250) + r1 * x723;
-      const TB x2089 = s.QC[2] * x1427 + s.WQ[2] * x1428;
-      acc720 += static_cast<double>(x2089);
-      const TB x1557 = s.QC[2] * x843 + s.WQ[2] * x844 + e1 * (x249 - s.roe * x250);
-      const TB x2151 = s.QC[2] * x1557 + s.WQ[2] * x1558 + e2 * (x843 - s.roe * x844);
-      acc782 += static_cast<double>(x2151);
-      const TB x1494 = s.QC[2] * x746 + s.WQ[2] * x747 + r1 * x723;
-      acc318 += static_cast<double>(x1494);
-      const TB x2121 = s.QC[2] * x1494 + s.WQ[2] * x1495 + e1 * (x746 - s.roe * x747) + r1 * x1479;
-      acc752 += static_cast<double>(x2121);
-      const TB x1492 = s.QC[2] * x743 + s.WQ[2] * x744;
-      const TB x2120 = s.QC[2] * x1492 + s.WQ[2] * x1493 + e1 * (x743 - s.roe * x744);
-      acc751 += static_cast<double>(x2120);
-      const TB x1496 = s.QC[2] * x749 + s.WQ[2] * x750;
-      acc319 += static_cast<double>(x1496);
-      const TB x2122 = s.QC[2] * x1496 + s.WQ[2] * x1497 + e1 * (x749 - s.roe * x750);
-      acc753 += static_cast<double>(x2122);
-      const TB x1622 = s.QC[1] * x943 + s.WQ[1] * x944 + e2 * (x384 - s.roe * x385) + r1 * x923;
-      const TB x1752 = s.QC[2] * x1043 + s.WQ[2] * x1044 + e1 * (x384 - s.roe * x385);
-      const TB x2275 = s.QC[2] * x1752 + s.WQ[2] * x1753 + e2 * (x1043 - s.roe * x1044);
-      acc906 += static_cast<double>(x2275);
-      const TB x2182 = s.QC[1] * x1622 + s.WQ[1] * x1623 + e3 * (x943 - s.roe * x944) + r1 * x1609;
-      acc813 += static_cast<double>(x2182);
-      const TB x2213 = s.QC[2] * x1622 + s.WQ[2] * x1623;
-      acc844 += static_cast<double>(x2213);
-      const TB x1689 = s.QC[2] * x946 + s.WQ[2] * x947 + r1 * x923;
-      acc411 += static_cast<double>(x1689);
-      const TB x2245 = s.QC[2] * x1689 + s.WQ[2] * x1690 + e1 * (x946 - s.roe * x947) + r1 * x1674;
-      acc876 += static_cast<double>(x2245);
-      const TB x1687 = s.QC[2] * x943 + s.WQ[2] * x944;
-      const TB x2244 = s.QC[2] * x1687 + s.WQ[2] * x1688 + e1 * (x943 - s.roe * x944);
-      acc875 += static_cast<double>(x2244);
-      const TB x1691 = s.QC[2] * x949 + s.WQ[2] * x950;
-      acc412 += static_cast<double>(x1691);
-      const TB x2246 = s.QC[2] * x1691 + s.WQ[2] * x1692 + e1 * (x949 - s.roe * x950);
-      acc877 += static_cast<double>(x2246);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(720 * R + r) * a.acc_stride + q] = acc720;
-  a.acc[(751 * R + r) * a.acc_stride + q] = acc751;
-  a.acc[(782 * R + r) * a.acc_stride + q] = acc782;
-  a.acc[(813 * R + r) * a.acc_stride + q] = acc813;
-  a.acc[(844 * R + r) * a.acc_stride + q] = acc844;
-  a.acc[(875 * R + r) * a.acc_stride + q] = acc875;
-  a.acc[(906 * R + r) * a.acc_stride + q] = acc906;
-  a.acc[(937 * R + r) * a.acc_stride + q] = acc937;
-  a.acc[(8 * R + r) * a.acc_stride + q] = acc8;
-  a.acc[(39 * R + r) * a.acc_stride + q] = acc39;
-  a.acc[(70 * R + r) * a.acc_stride + q] = acc70;
-  a.acc[(101 * R + r) * a.acc_stride + q] = acc101;
-  a.acc[(132 * R + r) * a.acc_stride + q] = acc132;
-  a.acc[(163 * R + r) * a.acc_stride + q] = acc163;
-  a.acc[(194 * R + r) * a.acc_stride + q] = acc194;
-  a.acc[(225 * R + r) * a.acc_stride + q] = acc225;
-  a.acc[(256 * R + r) * a.acc_stride + q] = acc256;
-  a.acc[(287 * R + r) * a.acc_stride + q] = acc287;
-  a.acc[(318 * R + r) * a.acc_stride + q] = acc318;
-  a.acc[(349 * R + r) * a.acc_stride + q] = acc349;
-  a.acc[(380 * R + r) * a.acc_stride + q] = acc380;
-  a.acc[(411 * R + r) * a.acc_stride + q] = acc411;
-  a.acc[(442 * R + r) * a.acc_stride + q] = acc442;
-  a.acc[(473 * R + r) * a.acc_stride + q] = acc473;
-  a.acc[(504 * R + r) * a.acc_stride + q] = acc504;
-  a.acc[(535 * R + r) * a.acc_stride + q] = acc535;
-  a.acc[(566 * R + r) * a.acc_stride + q] = acc566;
-  a.acc[(597 * R + r) * a.acc_stride + q] = acc597;
-  a.acc[(628 * R + r) * a.acc_stride + q] = acc628;
-  a.acc[(659 * R + r) * a.acc_stride + q] = acc659;
-  a.acc[(690 * R + r) * a.acc_stride + q] = acc690;
-  a.acc[(721 * R + r) * a.acc_stride + q] = acc721;
-  a.acc[(752 * R + r) * a.acc_stride + q] = acc752;
-  a.acc[(783 * R + r) * a.acc_stride + q] = acc783;
-  a.acc[(814 * R + r) * a.acc_stride + q] = acc814;
-  a.acc[(845 * R + r) * a.acc_stride + q] = acc845;
-  a.acc[(876 * R + r) * a.acc_stride + q] = acc876;
-  a.acc[(907 * R + r) * a.acc_stride + q] = acc907;
-  a.acc[(938 * R + r) * a.acc_stride + q] = acc938;
-  a.acc[(9 * R + r) * a.acc_stride + q] = acc9;
-  a.acc[(40 * R + r) * a.acc_stride + q] = acc40;
-  a.acc[(71 * R + r) * a.acc_stride + q] = acc71;
-  a.acc[(102 * R + r) * a.acc_stride + q] = acc102;
-  a.acc[(133 * R + r) * a.acc_stride + q] = acc133;
-  a.acc[(164 * R + r) * a.acc_stride + q] = acc164;
-  a.acc[(195 * R + r) * a.acc_stride + q] = acc195;
-  a.acc[(226 * R + r) * a.acc_stride + q] = acc226;
-  a.acc[(257 * R + r) * a.acc_stride + q] = acc257;
-  a.acc[(288 * R + r) * a.acc_stride + q] = acc288;
-  a.acc[(319 * R + r) * a.acc_stride + q] = acc319;
-  a.acc[(350 * R + r) * a.acc_stride + q] = acc350;
-  a.acc[(381 * R + r) * a.acc_stride + q] = acc381;
-  a.acc[(412 * R + r) * a.acc_stride + q] = acc412;
-  a.acc[(443 * R + r) * a.acc_stride + q] = acc443;
-  a.acc[(474 * R + r) * a.acc_stride + q] = acc474;
-  a.acc[(505 * R + r) * a.acc_stride + q] = acc505;
-  a.acc[(536 * R + r) * a.acc_stride + q] = acc536;
-  a.acc[(567 * R + r) * a.acc_stride + q] = acc567;
-  a.acc[(598 * R + r) * a.acc_stride + q] = acc598;
-  a.acc[(629 * R + r) * a.acc_stride + q] = acc629;
-  a.acc[(660 * R + r) * a.acc_stride + q] = acc660;
-  a.acc[(691 * R + r) * a.acc_stride + q] = acc691;
-  a.acc[(722 * R + r) * a.acc_stride + q] = acc722;
-  a.acc[(753 * R + r) * a.acc_stride + q] = acc753;
-  a.acc[(784 * R + r) * a.acc_stride + q] = acc784;
-  a.acc[(815 * R + r) * a.acc_stride + q] = acc815;
-  a.acc[(846 * R + r) * a.acc_stride + q] = acc846;
-  a.acc[(877 * R + r) * a.acc_stride + q] = acc877;
-  a.acc[(908 * R + r) * a.acc_stride + q] = acc908;
-  a.acc[(939 * R + r) * a.acc_stride + q] = acc939;
-  a.acc[(10 * R + r) * a.acc_stride + q] = acc10;
-  a.acc[(41 * R + r) * a.acc_stride + q] = acc41;
-  a.acc[(72 * R + r) * a.acc_stride + q] = acc72;
-  a.acc[(103 * R + r) * a.acc_stride + q] = acc103;
-  a.acc[(134 * R + r) * a.acc_stride + q] = acc134;
-  a.acc[(165 * R + r) * a.acc_stride + q] = acc165;
-  a.acc[(196 * R + r) * a.acc_stride + q] = acc196;
-  a.acc[(227 * R + r) * a.acc_stride + q] = acc227;
-  a.acc[(258 * R + r) * a.acc_stride + q] = acc258;
-  a.acc[(289 * R + r) * a.acc_stride + q] = acc289;
-}
-// class (22|22) split 4/12: 360 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_4(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc320 = 0.0;
-  double acc351 = 0.0;
-  double acc382 = 0.0;
-  double acc413 = 0.0;
-  double acc444 = 0.0;
-  double acc475 = 0.0;
-  double acc506 = 0.0;
-  double acc537 = 0.0;
-  double acc568 = 0.0;
-  double acc599 = 0.0;
-  double acc630 = 0.0;
-  double acc661 = 0.0;
-  double acc692 = 0.0;
-  double acc723 = 0.0;
-  double acc754 = 0.0;
-  double acc785 = 0.0;
-  double acc816 = 0.0;
-  double acc847 = 0.0;
-  double acc878 = 0.0;
-  double acc909 = 0.0;
-  double acc940 = 0.0;
-  double acc11 = 0.0;
-  double acc42 = 0.0;
-  double acc73 = 0.0;
-  double acc104 = 0.0;
-  double acc135 = 0.0;
-  double acc166 = 0.0;
-  double acc197 = 0.0;
-  double acc228 = 0.0;
-  double acc259 = 0.0;
-  double acc290 = 0.0;
-  double acc321 = 0.0;
-  double acc352 = 0.0;
-  double acc383 = 0.0;
-  double acc414 = 0.0;
-  double acc445 = 0.0;
-  double acc476 = 0.0;
-  double acc507 = 0.0;
-  double acc538 = 0.0;
-  double acc569 = 0.0;
-  double acc600 = 0.0;
-  double acc631 = 0.0;
-  double acc662 = 0.0;
-  double acc693 = 0.0;
-  double acc724 = 0.0;
-  double acc755 = 0.0;
-  double acc786 = 0.0;
-  double acc817 = 0.0;
-  double acc848 = 0.0;
-  double acc879 = 0.0;
-  double acc910 = 0.0;
-  double acc941 = 0.0;
-  double acc12 = 0.0;
-  double acc43 = 0.0;
-  double acc74 = 0.0;
-  double acc105 = 0.0;
-  double acc136 = 0.0;
-  double acc167 = 0.0;
-  double acc198 = 0.0;
-  double acc229 = 0.0;
-  double acc260 = 0.0;
-  double acc291 = 0.0;
-  double acc322 = 0.0;
-  double acc353 = 0.0;
-  double acc384 = 0.0;
-  double acc415 = 0.0;
-  double acc446 = 0.0;
-  double acc477 = 0.0;
-  double acc508 = 0.0;
-  double acc539 = 0.0;
-  double acc570 = 0.0;
-  double acc601 = 0.0;
-  double acc632 = 0.0;
-  double acc663 = 0.0;
-  double acc694 = 0.0;
-  double acc725 = 0.0;
-  double acc756 = 0.0;
-  double acc787 = 0.0;
-  double acc818 = 0.0;
-  double acc849 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x10 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x36 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x81 = s.PA[1] * x36 + s.WP[1] * x42 + z2 * (x10 - s.roz * x13);
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x28 = s.PA[1] * x6 + s.WP[1] * x7;
-      const TB x66 = s.PA[1] * x25 + s.WP[1] * x28 + z1 * (x5 - s.roz * x6);
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x121 = s.PA[1] * x60 + s.WP[1] * x66 + z2 * (x22 - s.roz * x25);
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x34 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x35 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x40 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x79 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x41 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x80 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x9 - s.roz * x12);
-      const TB x43 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x90 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x12 - s.roz * x15);
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x91 = s.PA[1] * x42 + s.WP[1] * x48 + z2 * (x13 - s.roz * x16);
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x64 = s.PA[1] * x24 + s.WP[1] * x27;
-      const TB x65 = s.PA[2] * x24 + s.WP[2] * x27;
-      const TB x120 = s.PA[2] * x59 + s.WP[2] * x65 + z1 * (x21 - s.roz * x24);
-      const TB x119 = s.PA[2] * x58 + s.WP[2] * x64;
-      const TB x89 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x99 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x100 = s.PA[2] * x47 + s.WP[2] * x53 + z1 * (x15 - s.roz * x18);
-      const TB x101 = s.PA[1] * x48 + s.WP[1] * x54 + z2 * (x16 - s.roz * x19);
-      const TB x109 = s.PA[2] * x52 + s.WP[2] * x58;
-      const TB x110 = s.PA[2] * x53 + s.WP[2] * x59 + z1 * (x18 - s.roz * x21);
-      const TB x111 = s.PA[1] * x54 + s.WP[1] * x60 + z2 * (x19 - s.roz * x22);
-      const TB x211 = s.QC[0] * x3 + s.WQ[0] * x4;
-      const TB x213 = s.QC[0] * x15 + s.WQ[0] * x18 + r1 * x3;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x216 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x219 = s.QC[0] * x17 + s.WQ[0] * x20;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x226 = s.QC[0] * x40 + s.WQ[0] * x46 + r1 * x16;
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x230 = s.QC[0] * x41 + s.WQ[0] * x47 + r1 * x17;
-      const TB x621 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x17 - s.roe * x20);
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x629 = s.QC[0] * x230 + s.WQ[0] * x231 + e1 * (x41 - s.roe * x47) + r1 * x219;
-      const TB x630 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x47 - s.roe * x53) + r1 * x220;
-      const TB x1223 = s.QC[0] * x629 + s.WQ[0] * x630 + e2 * (x230 - s.roe * x231) + r1 * x621;
-      const TB x234 = s.QC[0] * x42 + s.WQ[0] * x48;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x238 = s.QC[0] * x43 + s.WQ[0] * x49;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x240 = s.QC[0] * x55 + s.WQ[0] * x61;
-      const TB x264 = s.QC[0] * x109 + s.WQ[0] * x119 + r1 * x61;
-      const TB x242 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x244 = s.QC[0] * x56 + s.WQ[0] * x62;
-      const TB x268 = s.QC[0] * x110 + s.WQ[0] * x120 + r1 * x62;
-      const TB x261 = s.QC[0] * x79 + s.WQ[0] * x89 + r1 * x43;
-      const TB x635 = s.QC[0] * x238 + s.WQ[0] * x239 + e1 * (x43 - s.roe * x49);
-      const TB x262 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x49;
-      const TB x636 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x49 - s.roe * x55);
-      const TB x263 = s.QC[0] * x99 + s.WQ[0] * x109 + r1 * x55;
-      const TB x654 = s.QC[0] * x263 + s.WQ[0] * x264 + e1 * (x99 - s.roe * x109) + r1 * x240;
-      const TB x265 = s.QC[0] * x80 + s.WQ[0] * x90 + r1 * x44;
-      const TB x638 = s.QC[0] * x242 + s.WQ[0] * x243 + e1 * (x44 - s.roe * x50);
-      const TB x266 = s.QC[0] * x90 + s.WQ[0] * x100 + r1 * x50;
-      const TB x639 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x50 - s.roe * x56);
-      const TB x267 = s.QC[0] * x100 + s.WQ[0] * x110 + r1 * x56;
-      const TB x657 = s.QC[0] * x267 + s.WQ[0] * x268 + e1 * (x100 - s.roe * x110) + r1 * x244;
-      const TB x269 = s.QC[0] * x81 + s.WQ[0] * x91;
-      const TB x270 = s.QC[0] * x91 + s.WQ[0] * x101;
-      const TB x271 = s.QC[0] * x101 + s.WQ[0] * x111;
-      const TB x272 = s.QC[0] * x111 + s.WQ[0] * x121;
-      const TB x407 = s.QC[1] * x111 + s.WQ[1] * x121 + r3 * x60;
-      const TB x371 = s.QC[1] * x54 + s.WQ[1] * x60 + r2 * x22;
-      const TB x352 = s.QC[1] * x19 + s.WQ[1] * x22 + r1 * x4;
-      const TB x346 = s.QC[1] * x3 + s.WQ[1] * x4;
-      const TB x351 = s.QC[1] * x16 + s.WQ[1] * x19 + r1 * x3;
-      const TB x919 = s.QC[1] * x351 + s.WQ[1] * x352 + e1 * (x16 - s.roe * x19) + r1 * x346;
-      const TB x348 = s.QC[1] * x15 + s.WQ[1] * x18;
-      const TB x349 = s.QC[1] * x18 + s.WQ[1] * x21;
-      const TB x361 = s.QC[1] * x40 + s.WQ[1] * x46 + r1 * x15;
-      const TB x362 = s.QC[1] * x46 + s.WQ[1] * x52 + r1 * x18;
-      const TB x363 = s.QC[1] * x52 + s.WQ[1] * x58 + r1 * x21;
-      const TB x365 = s.QC[1] * x41 + s.WQ[1] * x47;
-      const TB x366 = s.QC[1] * x47 + s.WQ[1] * x53;
-      const TB x367 = s.QC[1] * x53 + s.WQ[1] * x59;
-      const TB x369 = s.QC[1] * x42 + s.WQ[1] * x48 + r2 * x16;
-      const TB x619 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x16 - s.roe * x19);
-      const TB x370 = s.QC[1] * x48 + s.WQ[1] * x54 + r2 * x19;
-      const TB x932 = s.QC[1] * x369 + s.WQ[1] * x370 + e1 * (x42 - s.roe * x48) + r2 * x351;
-      const TB x933 = s.QC[1] * x370 + s.WQ[1] * x371 + e1 * (x48 - s.roe * x54) + r2 * x352;
-      const TB x1615 = s.QC[1] * x932 + s.WQ[1] * x933 + e2 * (x369 - s.roe * x370) + r2 * x919;
-      const TB x396 = s.QC[1] * x79 + s.WQ[1] * x89 + r1 * x41;
-      const TB x397 = s.QC[1] * x89 + s.WQ[1] * x99 + r1 * x47;
-      const TB x398 = s.QC[1] * x99 + s.WQ[1] * x109 + r1 * x53;
-      const TB x399 = s.QC[1] * x109 + s.WQ[1] * x119 + r1 * x59;
-      const TB x534 = s.QC[2] * x109 + s.WQ[2] * x119 + r1 * x58;
-      const TB x498 = s.QC[2] * x52 + s.WQ[2] * x58;
-      const TB x400 = s.QC[1] * x80 + s.WQ[1] * x90;
-      const TB x401 = s.QC[1] * x90 + s.WQ[1] * x100;
-      const TB x402 = s.QC[1] * x100 + s.WQ[1] * x110;
-      const TB x403 = s.QC[1] * x110 + s.WQ[1] * x120;
-      const TB x538 = s.QC[2] * x110 + s.WQ[2] * x120 + r2 * x59;
-      const TB x502 = s.QC[2] * x53 + s.WQ[2] * x59 + r1 * x21;
-      const TB x484 = s.QC[2] * x18 + s.WQ[2] * x21;
-      const TB x404 = s.QC[1] * x81 + s.WQ[1] * x91 + r3 * x42;
-      const TB x632 = s.QC[0] * x234 + s.WQ[0] * x235 + e1 * (x42 - s.roe * x48);
-      const TB x405 = s.QC[1] * x91 + s.WQ[1] * x101 + r3 * x48;
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x406 = s.QC[1] * x101 + s.WQ[1] * x111 + r3 * x54;
-      const TB x960 = s.QC[1] * x406 + s.WQ[1] * x407 + e1 * (x101 - s.roe * x111) + r3 * x371;
-      const TB x958 = s.QC[1] * x404 + s.WQ[1] * x405 + e1 * (x81 - s.roe * x91) + r3 * x369;
-      acc105 += static_cast<double>(x958);
-      const TB x959 = s.QC[1] * x405 + s.WQ[1] * x406 + e1 * (x91 - s.roe * x101) + r3 * x370;
-      const TB x1633 = s.QC[1] * x959 + s.WQ[1] * x960 + e2 * (x405 - s.roe * x406) + r3 * x933;
-      const TB x1059 = s.QC[2] * x405 + s.WQ[2] * x406;
-      const TB x1632 = s.QC[1] * x958 + s.WQ[1] * x959 + e2 * (x404 - s.roe * x405) + r3 * x932;
-      acc384 += static_cast<double>(x1632);
-      const TB x2187 = s.QC[1] * x1632 + s.WQ[1] * x1633 + e3 * (x958 - s.roe * x959) + r3 * x1615;
-      acc818 += static_cast<double>(x2187);
-      const TB x1697 = s.QC[2] * x958 + s.WQ[2] * x959;
-      acc415 += static_cast<double>(x1697);
-      const TB x2218 = s.QC[2] * x1632 + s.WQ[2] * x1633;
-      acc849 += static_cast<double>(x2218);
-      const TB x483 = s.QC[2] * x15 + s.WQ[2] * x18;
-      const TB x496 = s.QC[2] * x40 + s.WQ[2] * x46;
-      const TB x497 = s.QC[2] * x46 + s.WQ[2] * x52;
-      const TB x500 = s.QC[2] * x41 + s.WQ[2] * x47 + r1 * x15;
-      const TB x501 = s.QC[2] * x47 + s.WQ[2] * x53 + r1 * x18;
-      const TB x531 = s.QC[2] * x79 + s.WQ[2] * x89 + r1 * x40;
-      const TB x532 = s.QC[2] * x89 + s.WQ[2] * x99 + r1 * x46;
-      const TB x533 = s.QC[2] * x99 + s.WQ[2] * x109 + r1 * x52;
-      const TB x1154 = s.QC[2] * x533 + s.WQ[2] * x534 + e1 * (x99 - s.roe * x109) + r1 * x498;
-      const TB x954 = s.QC[1] * x398 + s.WQ[1] * x399 + e1 * (x99 - s.roe * x109) + r1 * x367;
-      const TB x1054 = s.QC[2] * x398 + s.WQ[2] * x399 + r1 * x363;
-      const TB x1127 = s.QC[2] * x497 + s.WQ[2] * x498 + e1 * (x46 - s.roe * x52);
-      const TB x535 = s.QC[2] * x80 + s.WQ[2] * x90 + r2 * x41;
-      const TB x536 = s.QC[2] * x90 + s.WQ[2] * x100 + r2 * x47;
-      const TB x537 = s.QC[2] * x100 + s.WQ[2] * x110 + r2 * x53;
-      const TB x1157 = s.QC[2] * x537 + s.WQ[2] * x538 + e1 * (x100 - s.roe * x110) + r2 * x502;
-      const TB x957 = s.QC[1] * x402 + s.WQ[1] * x403 + e1 * (x100 - s.roe * x110);
-      const TB x1057 = s.QC[2] * x402 + s.WQ[2] * x403 + r2 * x367;
-      const TB x1130 = s.QC[2] * x501 + s.WQ[2] * x502 + e1 * (x47 - s.roe * x53) + r1 * x484;
-      const TB x930 = s.QC[1] * x366 + s.WQ[1] * x367 + e1 * (x47 - s.roe * x53);
-      const TB x1030 = s.QC[2] * x366 + s.WQ[2] * x367 + r1 * x349;
-      const TB x1117 = s.QC[2] * x483 + s.WQ[2] * x484 + e1 * (x15 - s.roe * x18);
-      const TB x1129 = s.QC[2] * x500 + s.WQ[2] * x501 + e1 * (x41 - s.roe * x47) + r1 * x483;
-      const TB x929 = s.QC[1] * x365 + s.WQ[1] * x366 + e1 * (x41 - s.roe * x47);
-      const TB x1808 = s.QC[2] * x1129 + s.WQ[2] * x1130 + e2 * (x500 - s.roe * x501) + r1 * x1117;
-      const TB x1155 = s.QC[2] * x535 + s.WQ[2] * x536 + e1 * (x80 - s.roe * x90) + r2 * x500;
-      acc166 += static_cast<double>(x1155);
-      const TB x1156 = s.QC[2] * x536 + s.WQ[2] * x537 + e1 * (x90 - s.roe * x100) + r2 * x501;
-      const TB x1826 = s.QC[2] * x1156 + s.WQ[2] * x1157 + e2 * (x536 - s.roe * x537) + r2 * x1130;
-      const TB x1825 = s.QC[2] * x1155 + s.WQ[2] * x1156 + e2 * (x535 - s.roe * x536) + r2 * x1129;
-      acc476 += static_cast<double>(x1825);
-      const TB x2310 = s.QC[2] * x1825 + s.WQ[2] * x1826 + e3 * (x1155 - s.roe * x1156) + r2 * x1808;
-      acc941 += static_cast<double>(x2310);
-      const TB x539 = s.QC[2] * x81 + s.WQ[2] * x91;
-      const TB x540 = s.QC[2] * x91 + s.WQ[2] * x101;
-      const TB x541 = s.QC[2] * x101 + s.WQ[2] * x111;
-      const TB x660 = s.QC[0] * x271 + s.WQ[0] * x272 + e1 * (x101 - s.roe * x111);
-      const TB x1159 = s.QC[2] * x540 + s.WQ[2] * x541 + e1 * (x91 - s.roe * x101);
-      const TB x659 = s.QC[0] * x270 + s.WQ[0] * x271 + e1 * (x91 - s.roe * x101);
-      const TB x617 = s.QC[0] * x213 + s.WQ[0] * x214 + e1 * (x15 - s.roe * x18) + r1 * x211;
-      const TB x917 = s.QC[1] * x348 + s.WQ[1] * x349 + e1 * (x15 - s.roe * x18);
-      const TB x719 = s.QC[1] * x216 + s.WQ[1] * x217 + r1 * x211;
-      const TB x626 = s.QC[0] * x226 + s.WQ[0] * x227 + e1 * (x40 - s.roe * x46) + r1 * x216;
-      const TB x732 = s.QC[1] * x234 + s.WQ[1] * x235 + r2 * x216;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x733 = s.QC[1] * x235 + s.WQ[1] * x236 + r2 * x217;
-      const TB x760 = s.QC[1] * x271 + s.WQ[1] * x272 + r3 * x236;
-      const TB x860 = s.QC[2] * x271 + s.WQ[2] * x272;
-      const TB x927 = s.QC[1] * x362 + s.WQ[1] * x363 + e1 * (x46 - s.roe * x52) + r1 * x349;
-      const TB x1017 = s.QC[2] * x348 + s.WQ[2] * x349;
-      const TB x1027 = s.QC[2] * x362 + s.WQ[2] * x363;
-      const TB x1420 = s.QC[1] * x732 + s.WQ[1] * x733 + e1 * (x234 - s.roe * x235) + r2 * x719;
-      const TB x652 = s.QC[0] * x261 + s.WQ[0] * x262 + e1 * (x79 - s.roe * x89) + r1 * x238;
-      const TB x1227 = s.QC[0] * x635 + s.WQ[0] * x636 + e2 * (x238 - s.roe * x239);
-      const TB x653 = s.QC[0] * x262 + s.WQ[0] * x263 + e1 * (x89 - s.roe * x99) + r1 * x239;
-      const TB x1238 = s.QC[0] * x652 + s.WQ[0] * x653 + e2 * (x261 - s.roe * x262) + r1 * x635;
-      const TB x1239 = s.QC[0] * x653 + s.WQ[0] * x654 + e2 * (x262 - s.roe * x263) + r1 * x636;
-      const TB x1875 = s.QC[0] * x1238 + s.WQ[0] * x1239 + e3 * (x652 - s.roe * x653) + r1 * x1227;
-      acc506 += static_cast<double>(x1875);
-      const TB x1906 = s.QC[1] * x1238 + s.WQ[1] * x1239 + r1 * x1223;
-      acc537 += static_cast<double>(x1906);
-      const TB x655 = s.QC[0] * x265 + s.WQ[0] * x266 + e1 * (x80 - s.roe * x90) + r1 * x242;
-      acc11 += static_cast<double>(x655);
-      const TB x955 = s.QC[1] * x400 + s.WQ[1] * x401 + e1 * (x80 - s.roe * x90);
-      acc104 += static_cast<double>(x955);
-      const TB x1229 = s.QC[0] * x638 + s.WQ[0] * x639 + e2 * (x242 - s.roe * x243);
-      const TB x656 = s.QC[0] * x266 + s.WQ[0] * x267 + e1 * (x90 - s.roe * x100) + r1 * x243;
-      const TB x956 = s.QC[1] * x401 + s.WQ[1] * x402 + e1 * (x90 - s.roe * x100);
-      const TB x1240 = s.QC[0] * x655 + s.WQ[0] * x656 + e2 * (x265 - s.roe * x266) + r1 * x638;
-      acc197 += static_cast<double>(x1240);
-      const TB x1241 = s.QC[0] * x656 + s.WQ[0] * x657 + e2 * (x266 - s.roe * x267) + r1 * x639;
-      const TB x1876 = s.QC[0] * x1240 + s.WQ[0] * x1241 + e3 * (x655 - s.roe * x656) + r1 * x1229;
-      acc507 += static_cast<double>(x1876);
-      const TB x1938 = s.QC[2] * x1240 + s.WQ[2] * x1241 + r2 * x1223;
-      acc569 += static_cast<double>(x1938);
-      const TB x1907 = s.QC[1] * x1240 + s.WQ[1] * x1241;
-      acc538 += static_cast<double>(x1907);
-      const TB x658 = s.QC[0] * x269 + s.WQ[0] * x270 + e1 * (x81 - s.roe * x91);
-      acc12 += static_cast<double>(x658);
-      const TB x1158 = s.QC[2] * x539 + s.WQ[2] * x540 + e1 * (x81 - s.roe * x91);
-      acc167 += static_cast<double>(x1158);
-      const TB x1827 = s.QC[2] * x1158 + s.WQ[2] * x1159 + e2 * (x539 - s.roe * x540);
-      acc477 += static_cast<double>(x1827);
-      const TB x717 = s.QC[1] * x213 + s.WQ[1] * x214;
-      const TB x726 = s.QC[1] * x226 + s.WQ[1] * x227 + r1 * x213;
-      const TB x727 = s.QC[1] * x227 + s.WQ[1] * x228 + r1 * x214;
-      const TB x729 = s.QC[1] * x230 + s.WQ[1] * x231;
-      const TB x730 = s.QC[1] * x231 + s.WQ[1] * x232;
-      const TB x752 = s.QC[1] * x261 + s.WQ[1] * x262 + r1 * x230;
-      const TB x753 = s.QC[1] * x262 + s.WQ[1] * x263 + r1 * x231;
-      const TB x754 = s.QC[1] * x263 + s.WQ[1] * x264 + r1 * x232;
-      const TB x854 = s.QC[2] * x263 + s.WQ[2] * x264 + r1 * x228;
-      const TB x827 = s.QC[2] * x227 + s.WQ[2] * x228;
-      const TB x755 = s.QC[1] * x265 + s.WQ[1] * x266;
-      acc42 += static_cast<double>(x755);
-      const TB x756 = s.QC[1] * x266 + s.WQ[1] * x267;
-      const TB x757 = s.QC[1] * x267 + s.WQ[1] * x268;
-      const TB x857 = s.QC[2] * x267 + s.WQ[2] * x268 + r2 * x232;
-      const TB x830 = s.QC[2] * x231 + s.WQ[2] * x232 + r1 * x214;
-      const TB x817 = s.QC[2] * x213 + s.WQ[2] * x214;
-      const TB x829 = s.QC[2] * x230 + s.WQ[2] * x231 + r1 * x213;
-      const TB x1548 = s.QC[2] * x829 + s.WQ[2] * x830 + e1 * (x230 - s.roe * x231) + r1 * x817;
-      const TB x758 = s.QC[1] * x269 + s.WQ[1] * x270 + r3 * x234;
-      acc43 += static_cast<double>(x758);
-      const TB x1225 = s.QC[0] * x632 + s.WQ[0] * x633 + e2 * (x234 - s.roe * x235);
-      const TB x759 = s.QC[1] * x270 + s.WQ[1] * x271 + r3 * x235;
-      const TB x1437 = s.QC[1] * x758 + s.WQ[1] * x759 + e1 * (x269 - s.roe * x270) + r3 * x732;
-      acc291 += static_cast<double>(x1437);
-      const TB x1438 = s.QC[1] * x759 + s.WQ[1] * x760 + e1 * (x270 - s.roe * x271) + r3 * x733;
-      const TB x2063 = s.QC[1] * x1437 + s.WQ[1] * x1438 + e2 * (x758 - s.roe * x759) + r3 * x1420;
-      acc694 += static_cast<double>(x2063);
-      const TB x2094 = s.QC[2] * x1437 + s.WQ[2] * x1438;
-      acc725 += static_cast<double>(x2094);
-      const TB x1503 = s.QC[2] * x759 + s.WQ[2] * x760;
-      const TB x826 = s.QC[2] * x226 + s.WQ[2] * x227;
-      const TB x852 = s.QC[2] * x261 + s.WQ[2] * x262 + r1 * x226;
-      const TB x853 = s.QC[2] * x262 + s.WQ[2] * x263 + r1 * x227;
-      const TB x1564 = s.QC[2] * x853 + s.WQ[2] * x854 + e1 * (x262 - s.roe * x263) + r1 * x827;
-      const TB x1434 = s.QC[1] * x753 + s.WQ[1] * x754 + e1 * (x262 - s.roe * x263) + r1 * x730;
-      const TB x1499 = s.QC[2] * x753 + s.WQ[2] * x754 + r1 * x727;
-      const TB x1546 = s.QC[2] * x826 + s.WQ[2] * x827 + e1 * (x226 - s.roe * x227);
-      const TB x1563 = s.QC[2] * x852 + s.WQ[2] * x853 + e1 * (x261 - s.roe * x262) + r1 * x826;
-      acc351 += static_cast<double>(x1563);
-      const TB x2154 = s.QC[2] * x1563 + s.WQ[2] * x1564 + e2 * (x852 - s.roe * x853) + r1 * x1546;
-      acc785 += static_cast<double>(x2154);
-      const TB x1433 = s.QC[1] * x752 + s.WQ[1] * x753 + e1 * (x261 - s.roe * x262) + r1 * x729;
-      const TB x855 = s.QC[2] * x265 + s.WQ[2] * x266 + r2 * x230;
-      acc73 += static_cast<double>(x855);
-      const TB x1418 = s.QC[1] * x729 + s.WQ[1] * x730 + e1 * (x230 - s.roe * x231);
-      const TB x856 = s.QC[2] * x266 + s.WQ[2] * x267 + r2 * x231;
-      const TB x1566 = s.QC[2] * x856 + s.WQ[2] * x857 + e1 * (x266 - s.roe * x267) + r2 * x830;
-      const TB x1436 = s.QC[1] * x756 + s.WQ[1] * x757 + e1 * (x266 - s.roe * x267);
-      const TB x1501 = s.QC[2] * x756 + s.WQ[2] * x757 + r2 * x730;
-      const TB x1483 = s.QC[2] * x729 + s.WQ[2] * x730 + r1 * x717;
-      const TB x1416 = s.QC[1] * x726 + s.WQ[1] * x727 + e1 * (x226 - s.roe * x227) + r1 * x717;
-      const TB x1221 = s.QC[0] * x626 + s.WQ[0] * x627 + e2 * (x226 - s.roe * x227) + r1 * x619;
-      const TB x1937 = s.QC[2] * x1238 + s.WQ[2] * x1239 + r1 * x1221;
-      acc568 += static_cast<double>(x1937);
-      const TB x2092 = s.QC[2] * x1433 + s.WQ[2] * x1434 + r1 * x1416;
-      acc723 += static_cast<double>(x2092);
-      const TB x2061 = s.QC[1] * x1433 + s.WQ[1] * x1434 + e2 * (x752 - s.roe * x753) + r1 * x1418;
-      acc692 += static_cast<double>(x2061);
-      const TB x1290 = s.QC[1] * x632 + s.WQ[1] * x633 + r2 * x619;
-      const TB x1307 = s.QC[1] * x658 + s.WQ[1] * x659 + r3 * x632;
-      acc229 += static_cast<double>(x1307);
-      const TB x1308 = s.QC[1] * x659 + s.WQ[1] * x660 + r3 * x633;
-      const TB x1970 = s.QC[1] * x1307 + s.WQ[1] * x1308 + e1 * (x658 - s.roe * x659) + r3 * x1290;
-      acc601 += static_cast<double>(x1970);
-      const TB x2001 = s.QC[2] * x1307 + s.WQ[2] * x1308;
-      acc632 += static_cast<double>(x2001);
-      const TB x1481 = s.QC[2] * x726 + s.WQ[2] * x727;
-      const TB x1498 = s.QC[2] * x752 + s.WQ[2] * x753 + r1 * x726;
-      acc320 += static_cast<double>(x1498);
-      const TB x2123 = s.QC[2] * x1498 + s.WQ[2] * x1499 + e1 * (x752 - s.roe * x753) + r1 * x1481;
-      acc754 += static_cast<double>(x2123);
-      const TB x1500 = s.QC[2] * x755 + s.WQ[2] * x756 + r2 * x729;
-      acc321 += static_cast<double>(x1500);
-      const TB x2124 = s.QC[2] * x1500 + s.WQ[2] * x1501 + e1 * (x755 - s.roe * x756) + r2 * x1483;
-      acc755 += static_cast<double>(x2124);
-      const TB x1565 = s.QC[2] * x855 + s.WQ[2] * x856 + e1 * (x265 - s.roe * x266) + r2 * x829;
-      acc352 += static_cast<double>(x1565);
-      const TB x2155 = s.QC[2] * x1565 + s.WQ[2] * x1566 + e2 * (x855 - s.roe * x856) + r2 * x1548;
-      acc786 += static_cast<double>(x2155);
-      const TB x1435 = s.QC[1] * x755 + s.WQ[1] * x756 + e1 * (x265 - s.roe * x266);
-      acc290 += static_cast<double>(x1435);
-      const TB x2062 = s.QC[1] * x1435 + s.WQ[1] * x1436 + e2 * (x755 - s.roe * x756);
-      acc693 += static_cast<double>(x2062);
-      const TB x2093 = s.QC[2] * x1435 + s.WQ[2] * x1436 + r2 * x1418;
-      acc724 += static_cast<double>(x2093);
-      const TB x858 = s.QC[2] * x269 + s.WQ[2] * x270;
-      acc74 += static_cast<double>(x858);
-      const TB x859 = s.QC[2] * x270 + s.WQ[2] * x271;
-      const TB x1568 = s.QC[2] * x859 + s.WQ[2] * x860 + e1 * (x270 - s.roe * x271);
-      const TB x1243 = s.QC[0] * x659 + s.WQ[0] * x660 + e2 * (x270 - s.roe * x271);
-      const TB x1373 = s.QC[2] * x659 + s.WQ[2] * x660;
-      const TB x926 = s.QC[1] * x361 + s.WQ[1] * x362 + e1 * (x40 - s.roe * x46) + r1 * x348;
-      const TB x1126 = s.QC[2] * x496 + s.WQ[2] * x497 + e1 * (x40 - s.roe * x46);
-      const TB x1029 = s.QC[2] * x365 + s.WQ[2] * x366 + r1 * x348;
-      const TB x1743 = s.QC[2] * x1029 + s.WQ[2] * x1030 + e1 * (x365 - s.roe * x366) + r1 * x1017;
-      const TB x952 = s.QC[1] * x396 + s.WQ[1] * x397 + e1 * (x79 - s.roe * x89) + r1 * x365;
-      const TB x1152 = s.QC[2] * x531 + s.WQ[2] * x532 + e1 * (x79 - s.roe * x89) + r1 * x496;
-      const TB x1806 = s.QC[2] * x1126 + s.WQ[2] * x1127 + e2 * (x496 - s.roe * x497);
-      const TB x1153 = s.QC[2] * x532 + s.WQ[2] * x533 + e1 * (x89 - s.roe * x99) + r1 * x497;
-      const TB x1824 = s.QC[2] * x1153 + s.WQ[2] * x1154 + e2 * (x532 - s.roe * x533) + r1 * x1127;
-      const TB x1823 = s.QC[2] * x1152 + s.WQ[2] * x1153 + e2 * (x531 - s.roe * x532) + r1 * x1126;
-      acc475 += static_cast<double>(x1823);
-      const TB x2309 = s.QC[2] * x1823 + s.WQ[2] * x1824 + e3 * (x1152 - s.roe * x1153) + r1 * x1806;
-      acc940 += static_cast<double>(x2309);
-      const TB x953 = s.QC[1] * x397 + s.WQ[1] * x398 + e1 * (x89 - s.roe * x99) + r1 * x366;
-      const TB x1026 = s.QC[2] * x361 + s.WQ[2] * x362;
-      const TB x1052 = s.QC[2] * x396 + s.WQ[2] * x397 + r1 * x361;
-      const TB x1053 = s.QC[2] * x397 + s.WQ[2] * x398 + r1 * x362;
-      const TB x1759 = s.QC[2] * x1053 + s.WQ[2] * x1054 + e1 * (x397 - s.roe * x398) + r1 * x1027;
-      const TB x1629 = s.QC[1] * x953 + s.WQ[1] * x954 + e2 * (x397 - s.roe * x398) + r1 * x930;
-      const TB x1694 = s.QC[2] * x953 + s.WQ[2] * x954 + r1 * x927;
-      const TB x1741 = s.QC[2] * x1026 + s.WQ[2] * x1027 + e1 * (x361 - s.roe * x362);
-      const TB x1611 = s.QC[1] * x926 + s.WQ[1] * x927 + e2 * (x361 - s.roe * x362) + r1 * x917;
-      const TB x1676 = s.QC[2] * x926 + s.WQ[2] * x927;
-      const TB x1678 = s.QC[2] * x929 + s.WQ[2] * x930 + r1 * x917;
-      const TB x1693 = s.QC[2] * x952 + s.WQ[2] * x953 + r1 * x926;
-      acc413 += static_cast<double>(x1693);
-      const TB x2247 = s.QC[2] * x1693 + s.WQ[2] * x1694 + e1 * (x952 - s.roe * x953) + r1 * x1676;
-      acc878 += static_cast<double>(x2247);
-      const TB x1758 = s.QC[2] * x1052 + s.WQ[2] * x1053 + e1 * (x396 - s.roe * x397) + r1 * x1026;
-      acc444 += static_cast<double>(x1758);
-      const TB x2278 = s.QC[2] * x1758 + s.WQ[2] * x1759 + e2 * (x1052 - s.roe * x1053) + r1 * x1741;
-      acc909 += static_cast<double>(x2278);
-      const TB x1628 = s.QC[1] * x952 + s.WQ[1] * x953 + e2 * (x396 - s.roe * x397) + r1 * x929;
-      acc382 += static_cast<double>(x1628);
-      const TB x2216 = s.QC[2] * x1628 + s.WQ[2] * x1629 + r1 * x1611;
-      acc847 += static_cast<double>(x2216);
-      const TB x1055 = s.QC[2] * x400 + s.WQ[2] * x401 + r2 * x365;
-      acc135 += static_cast<double>(x1055);
-      const TB x1613 = s.QC[1] * x929 + s.WQ[1] * x930 + e2 * (x365 - s.roe * x366);
-      const TB x2185 = s.QC[1] * x1628 + s.WQ[1] * x1629 + e3 * (x952 - s.roe * x953) + r1 * x1613;
-      acc816 += static_cast<double>(x2185);
-      const TB x1056 = s.QC[2] * x401 + s.WQ[2] * x402 + r2 * x366;
-      const TB x1761 = s.QC[2] * x1056 + s.WQ[2] * x1057 + e1 * (x401 - s.roe * x402) + r2 * x1030;
-      const TB x1631 = s.QC[1] * x956 + s.WQ[1] * x957 + e2 * (x401 - s.roe * x402);
-      const TB x1696 = s.QC[2] * x956 + s.WQ[2] * x957 + r2 * x930;
-      const TB x1695 = s.QC[2] * x955 + s.WQ[2] * x956 + r2 * x929;
-      acc414 += static_cast<double>(x1695);
-      const TB x2248 = s.QC[2] * x1695 + s.WQ[2] * x1696 + e1 * (x955 - s.roe * x956) + r2 * x1678;
-      acc879 += static_cast<double>(x2248);
-      const TB x1760 = s.QC[2] * x1055 + s.WQ[2] * x1056 + e1 * (x400 - s.roe * x401) + r2 * x1029;
-      acc445 += static_cast<double>(x1760);
-      const TB x2279 = s.QC[2] * x1760 + s.WQ[2] * x1761 + e2 * (x1055 - s.roe * x1056) + r2 * x1743;
-      acc910 += static_cast<double>(x2279);
-      const TB x1630 = s.QC[1] * x955 + s.WQ[1] * x956 + e2 * (x400 - s.roe * x401);
-      acc383 += static_cast<double>(x1630);
-      const TB x2186 = s.QC[1] * x1630 + s.WQ[1] * x1631 + e3 * (x955 - s.roe * x956);
-      acc817 += static_cast<double>(x2186);
-      const TB x2217 = s.QC[2] * x1630 + s.WQ[2] * x1631 + r2 * x1613;
-      acc848 += static_cast<double>(x2217);
-      const TB x1058 = s.QC[2] * x404 + s.WQ[2] * x405;
-      acc136 += static_cast<double>(x1058);
-      const TB x1762 = s.QC[2] * x1058 + s.WQ[2] * x1059 + e1 * (x404 - s.roe * x405);
-      acc446 += static_cast<double>(x1762);
-      const TB x1242 = s.QC[0] * x658 + s.WQ[0] * x659 + e2 * (x269 - s.roe * x270);
-      acc198 += static_cast<double>(x1242);
-      const TB x1567 = s.QC[2] * x858 + s.WQ[2] * x859 + e1 * (x269 - s.roe * x270);
-      acc353 += static_cast<double>(x1567);
-      const TB x2156 = s.QC[2] * x1567 + s.WQ[2] * x1568 + e2 * (x858 - s.roe * x859);
-      acc787 += static_cast<double>(x2156);
-      const TB x1908 = s.QC[1] * x1242 + s.WQ[1] * x1243 + r3 * x1225;
-      acc539 += static_cast<double>(x1908);
-      const TB x1877 = s.QC[0] * x1242 + s.WQ[0] * x1243 + e3 * (x658 - s.roe * x659);
-      acc508 += static_cast<double>(x1877);
-      const TB x1939 = s.QC[2] * x1242 + s.WQ[2] * x1243;
-      acc570 += static_cast<double>(x1939);
-      const TB x1286 = s.QC[1] * x626 + s.WQ[1] * x627 + r1 * x617;
-      const TB x1353 = s.QC[2] * x629 + s.WQ[2] * x630 + r1 * x617;
-      const TB x1288 = s.QC[1] * x629 + s.WQ[1] * x630;
-      const TB x1303 = s.QC[1] * x652 + s.WQ[1] * x653 + r1 * x629;
-      const TB x1370 = s.QC[2] * x655 + s.WQ[2] * x656 + r2 * x629;
-      acc259 += static_cast<double>(x1370);
-      const TB x1304 = s.QC[1] * x653 + s.WQ[1] * x654 + r1 * x630;
-      const TB x1999 = s.QC[2] * x1303 + s.WQ[2] * x1304 + r1 * x1286;
-      acc630 += static_cast<double>(x1999);
-      const TB x1968 = s.QC[1] * x1303 + s.WQ[1] * x1304 + e1 * (x652 - s.roe * x653) + r1 * x1288;
-      acc599 += static_cast<double>(x1968);
-      const TB x1369 = s.QC[2] * x653 + s.WQ[2] * x654 + r1 * x627;
-      const TB x1351 = s.QC[2] * x626 + s.WQ[2] * x627;
-      const TB x1368 = s.QC[2] * x652 + s.WQ[2] * x653 + r1 * x626;
-      const TB x2030 = s.QC[2] * x1368 + s.WQ[2] * x1369 + e1 * (x652 - s.roe * x653) + r1 * x1351;
-      acc661 += static_cast<double>(x2030);
-      const TB x1371 = s.QC[2] * x656 + s.WQ[2] * x657 + r2 * x630;
-      const TB x2031 = s.QC[2] * x1370 + s.WQ[2] * x1371 + e1 * (x655 - s.roe * x656) + r2 * x1353;
-      acc662 += static_cast<double>(x2031);
-      const TB x1306 = s.QC[1] * x656 + s.WQ[1] * x657;
-      const TB x1305 = s.QC[1] * x655 + s.WQ[1] * x656;
-      acc228 += static_cast<double>(x1305);
-      const TB x1969 = s.QC[1] * x1305 + s.WQ[1] * x1306 + e1 * (x655 - s.roe * x656);
-      acc600 += static_cast<double>(x1969);
-      const TB x2000 = s.QC[2] * x1305 + s.WQ[2] * x1306 + r2 * x1288;
-      acc631 += static_cast<double>(x2000);
-      const TB x1372 = s.QC[2] * x658 + s.WQ[2] * x659;
-      acc260 += static_cast<double>(x1372);
-      const TB x2032 = s.QC[2] * x1372 + s.WQ[2] * x1373 + e1 * (x658 - s.roe * x659);
-      acc663 += static_cast<double>(x2032);
-      const TB x1502 = s.QC[2] * x758 + s.WQ[2] * x759;
-      acc322 += static_cast<double>(x1502);
-      const TB x2125 = s.QC[2] * x1502 + s.WQ[2] * x1503 + e1 * (x758 - s.roe * x759);
-      acc756 += static_cast<double>(x2125);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(320 * R + r) * a.acc_stride + q] = acc320;
-  a.acc[(351 * R + r) * a.acc_stride + q] = acc351;
-  a.acc[(382 * R + r) * a.acc_stride + q] = acc382;
-  a.acc[(413 * R + r) * a.acc_stride + q] = acc413;
-  a.acc[(444 * R + r) * a.acc_stride + q] = acc444;
-  a.acc[(475 * R + r) * a.acc_stride + q] = acc475;
-  a.acc[(506 * R + r) * a.acc_stride + q] = acc506;
-  a.acc[(537 * R + r) * a.acc_stride + q] = acc537;
-  a.acc[(568 * R + r) * a.acc_stride + q] = acc568;
-  a.acc[(599 * R + r) * a.acc_stride + q] = acc599;
-  a.acc[(630 * R + r) * a.acc_stride + q] = acc630;
-  a.acc[(661 * R + r) * a.acc_stride + q] = acc661;
-  a.acc[(692 * R + r) * a.acc_stride + q] = acc692;
-  a.acc[(723 * R + r) * a.acc_stride + q] = acc723;
-  a.acc[(754 * R + r) * a.acc_stride + q] = acc754;
-  a.acc[(785 * R + r) * a.acc_stride + q] = acc785;
-  a.acc[(816 * R + r) * a.acc_stride + q] = acc816;
-  a.acc[(847 * R + r) * a.acc_stride + q] = acc847;
-  a.acc[(878 * R + r) * a.acc_stride + q] = acc878;
-  a.acc[(909 * R + r) * a.acc_stride + q] = acc909;
-  a.acc[(940 * R + r) * a.acc_stride + q] = acc940;
-  a.acc[(11 * R + r) * a.acc_stride + q] = acc11;
-  a.acc[(42 * R + r) * a.acc_stride + q] = acc42;
-  a.acc[(73 * R + r) * a.acc_stride + q] = acc73;
-  a.acc[(104 * R + r) * a.acc_stride + q] = acc104;
-  a.acc[(135 * R + r) * a.acc_stride + q] = acc135;
-  a.acc[(166 * R + r) * a.acc_stride + q] = acc166;
-  a.acc[(197 * R + r) * a.acc_stride + q] = acc197;
-  a.acc[(228 * R + r) * a.acc_stride + q] = acc228;
-  a.acc[(259 * R + r) * a.acc_stride + q] = acc259;
-  a.acc[(290 * R + r) * a.acc_stride + q] = acc290;
-  a.acc[(321 * R + r) * a.acc_stride + q] = acc321;
-  a.acc[(352 * R + r) * a.acc_stride + q] = acc352;
-  a.acc[(383 * R + r) * a.acc_stride + q] = acc383;
-  a.acc[(414 * R + r) * a.acc_stride + q] = acc414;
-  a.acc[(445 * R + r) * a.acc_stride + q] = acc445;
-  a.acc[(476 * R + r) * a.acc_stride + q] = acc476;
-  a.acc[(507 * R + r) * a.acc_stride + q] = acc507;
-  a.acc[(538 * R + r) * a.acc_stride + q] = acc538;
-  a.acc[(569 * R + r) * a.acc_stride + q] = acc569;
-  a.acc[(600 * R + r) * a.acc_stride + q] = acc600;
-  a.acc[(631 * R + r) * a.acc_stride + q] = acc631;
-  a.acc[(662 * R + r) * a.acc_stride + q] = acc662;
-  a.acc[(693 * R + r) * a.acc_stride + q] = acc693;
-  a.acc[(724 * R + r) * a.acc_stride + q] = acc724;
-  a.acc[(755 * R + r) * a.acc_stride + q] = acc755;
-  a.acc[(786 * R + r) * a.acc_stride + q] = acc786;
-  a.acc[(817 * R + r) * a.acc_stride + q] = acc817;
-  a.acc[(848 * R + r) * a.acc_stride + q] = acc848;
-  a.acc[(879 * R + r) * a.acc_stride + q] = acc879;
-  a.acc[(910 * R + r) * a.acc_stride + q] = acc910;
-  a.acc[(941 * R + r) * a.acc_stride + q] = acc941;
-  a.acc[(12 * R + r) * a.acc_stride + q] = acc12;
-  a.acc[(43 * R + r) * a.acc_stride + q] = acc43;
-  a.acc[(74 * R + r) * a.acc_stride + q] = acc74;
-  a.acc[(105 * R + r) * a.acc_stride + q] = acc105;
-  a.acc[(136 * R + r) * a.acc_stride + q] = acc136;
-  a.acc[(167 * R + r) * a.acc_stride + q] = acc167;
-  a.acc[(198 * R + r) * a.acc_stride + q] = acc198;
-  a.acc[(229 * R + r) * a.acc_stride + q] = acc229;
-  a.acc[(260 * R + r) * a.acc_stride + q] = acc260;
-  a.acc[(291 * R + r) * a.acc_stride + q] = acc291;
-  a.acc[(322 * R + r) * a.acc_stride + q] = acc322;
-  a.acc[(353 * R + r) * a.acc_stride + q] = acc353;
-  a.acc[(384 * R + r) * a.acc_stride + q] = acc384;
-  a.acc[(415 * R + r) * a.acc_stride + q] = acc415;
-  a.acc[(446 * R + r) * a.acc_stride + q] = acc446;
-  a.acc[(477 * R + r) * a.acc_stride + q] = acc477;
-  a.acc[(508 * R + r) * a.acc_stride + q] = acc508;
-  a.acc[(539 * R + r) * a.acc_stride + q] = acc539;
-  a.acc[(570 * R + r) * a.acc_stride + q] = acc570;
-  a.acc[(601 * R + r) * a.acc_stride + q] = acc601;
-  a.acc[(632 * R + r) * a.acc_stride + q] = acc632;
-  a.acc[(663 * R + r) * a.acc_stride + q] = acc663;
-  a.acc[(694 * R + r) * a.acc_stride + q] = acc694;
-  a.acc[(725 * R + r) * a.acc_stride + q] = acc725;
-  a.acc[(756 * R + r) * a.acc_stride + q] = acc756;
-  a.acc[(787 * R + r) * a.acc_stride + q] = acc787;
-  a.acc[(818 * R + r) * a.acc_stride + q] = acc818;
-  a.acc[(849 * R + r) * a.acc_stride + q] = acc849;
-}
-// class (22|22) split 5/12: 335 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_5(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc880 = 0.0;
-  double acc911 = 0.0;
-  double acc942 = 0.0;
-  double acc13 = 0.0;
-  double acc44 = 0.0;
-  double acc75 = 0.0;
-  double acc106 = 0.0;
-  double acc137 = 0.0;
-  double acc168 = 0.0;
-  double acc199 = 0.0;
-  double acc230 = 0.0;
-  double acc261 = 0.0;
-  double acc292 = 0.0;
-  double acc323 = 0.0;
-  double acc354 = 0.0;
-  double acc385 = 0.0;
-  double acc416 = 0.0;
-  double acc447 = 0.0;
-  double acc478 = 0.0;
-  double acc509 = 0.0;
-  double acc540 = 0.0;
-  double acc571 = 0.0;
-  double acc602 = 0.0;
-  double acc633 = 0.0;
-  double acc664 = 0.0;
-  double acc695 = 0.0;
-  double acc726 = 0.0;
-  double acc757 = 0.0;
-  double acc788 = 0.0;
-  double acc819 = 0.0;
-  double acc850 = 0.0;
-  double acc881 = 0.0;
-  double acc912 = 0.0;
-  double acc943 = 0.0;
-  double acc14 = 0.0;
-  double acc45 = 0.0;
-  double acc76 = 0.0;
-  double acc107 = 0.0;
-  double acc138 = 0.0;
-  double acc169 = 0.0;
-  double acc200 = 0.0;
-  double acc231 = 0.0;
-  double acc262 = 0.0;
-  double acc293 = 0.0;
-  double acc324 = 0.0;
-  double acc355 = 0.0;
-  double acc386 = 0.0;
-  double acc417 = 0.0;
-  double acc448 = 0.0;
-  double acc479 = 0.0;
-  double acc510 = 0.0;
-  double acc541 = 0.0;
-  double acc572 = 0.0;
-  double acc603 = 0.0;
-  double acc634 = 0.0;
-  double acc665 = 0.0;
-  double acc696 = 0.0;
-  double acc727 = 0.0;
-  double acc758 = 0.0;
-  double acc789 = 0.0;
-  double acc820 = 0.0;
-  double acc851 = 0.0;
-  double acc882 = 0.0;
-  double acc913 = 0.0;
-  double acc944 = 0.0;
-  double acc15 = 0.0;
-  double acc46 = 0.0;
-  double acc77 = 0.0;
-  double acc108 = 0.0;
-  double acc139 = 0.0;
-  double acc170 = 0.0;
-  double acc201 = 0.0;
-  double acc232 = 0.0;
-  double acc263 = 0.0;
-  double acc294 = 0.0;
-  double acc325 = 0.0;
-  double acc356 = 0.0;
-  double acc387 = 0.0;
-  double acc418 = 0.0;
-  double acc449 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x28 = s.PA[1] * x6 + s.WP[1] * x7;
-      const TB x10 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x11 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x66 = s.PA[1] * x25 + s.WP[1] * x28 + z1 * (x5 - s.roz * x6);
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x67 = s.PA[2] * x25 + s.WP[2] * x28;
-      const TB x36 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x38 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x0 - s.roz * x1);
-      const TB x37 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x84 = s.PA[2] * x38 + s.WP[2] * x44 + z2 * (x11 - s.roz * x14);
-      const TB x43 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x83 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x10 - s.roz * x13);
-      const TB x81 = s.PA[1] * x36 + s.WP[1] * x42 + z2 * (x10 - s.roz * x13);
-      const TB x82 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x94 = s.PA[2] * x44 + s.WP[2] * x50 + z2 * (x14 - s.roz * x17);
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x123 = s.PA[2] * x61 + s.WP[2] * x67 + z1 * (x22 - s.roz * x25);
-      const TB x121 = s.PA[1] * x60 + s.WP[1] * x66 + z2 * (x22 - s.roz * x25);
-      const TB x122 = s.PA[2] * x60 + s.WP[2] * x66;
-      const TB x91 = s.PA[1] * x42 + s.WP[1] * x48 + z2 * (x13 - s.roz * x16);
-      const TB x93 = s.PA[2] * x43 + s.WP[2] * x49 + z1 * (x13 - s.roz * x16);
-      const TB x92 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x101 = s.PA[1] * x48 + s.WP[1] * x54 + z2 * (x16 - s.roz * x19);
-      const TB x102 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x103 = s.PA[2] * x49 + s.WP[2] * x55 + z1 * (x16 - s.roz * x19);
-      const TB x104 = s.PA[2] * x50 + s.WP[2] * x56 + z2 * (x17 - s.roz * x20);
-      const TB x111 = s.PA[1] * x54 + s.WP[1] * x60 + z2 * (x19 - s.roz * x22);
-      const TB x112 = s.PA[2] * x54 + s.WP[2] * x60;
-      const TB x113 = s.PA[2] * x55 + s.WP[2] * x61 + z1 * (x19 - s.roz * x22);
-      const TB x114 = s.PA[2] * x56 + s.WP[2] * x62 + z2 * (x20 - s.roz * x23);
-      const TB x211 = s.QC[0] * x3 + s.WQ[0] * x4;
-      const TB x216 = s.QC[0] * x16 + s.WQ[0] * x19;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x719 = s.QC[1] * x216 + s.WQ[1] * x217 + r1 * x211;
-      const TB x219 = s.QC[0] * x17 + s.WQ[0] * x20;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x234 = s.QC[0] * x42 + s.WQ[0] * x48;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x238 = s.QC[0] * x43 + s.WQ[0] * x49;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x240 = s.QC[0] * x55 + s.WQ[0] * x61;
-      const TB x242 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x244 = s.QC[0] * x56 + s.WQ[0] * x62;
-      const TB x273 = s.QC[0] * x82 + s.WQ[0] * x92;
-      const TB x274 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x275 = s.QC[0] * x102 + s.WQ[0] * x112;
-      const TB x276 = s.QC[0] * x112 + s.WQ[0] * x122;
-      const TB x277 = s.QC[0] * x83 + s.WQ[0] * x93;
-      const TB x278 = s.QC[0] * x93 + s.WQ[0] * x103;
-      const TB x279 = s.QC[0] * x103 + s.WQ[0] * x113;
-      const TB x280 = s.QC[0] * x113 + s.WQ[0] * x123;
-      const TB x281 = s.QC[0] * x84 + s.WQ[0] * x94;
-      const TB x282 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x283 = s.QC[0] * x104 + s.WQ[0] * x114;
-      const TB x418 = s.QC[1] * x104 + s.WQ[1] * x114;
-      const TB x346 = s.QC[1] * x3 + s.WQ[1] * x4;
-      const TB x351 = s.QC[1] * x16 + s.WQ[1] * x19 + r1 * x3;
-      const TB x352 = s.QC[1] * x19 + s.WQ[1] * x22 + r1 * x4;
-      const TB x919 = s.QC[1] * x351 + s.WQ[1] * x352 + e1 * (x16 - s.roe * x19) + r1 * x346;
-      const TB x354 = s.QC[1] * x17 + s.WQ[1] * x20;
-      const TB x355 = s.QC[1] * x20 + s.WQ[1] * x23;
-      const TB x375 = s.QC[1] * x55 + s.WQ[1] * x61 + r1 * x23;
-      const TB x369 = s.QC[1] * x42 + s.WQ[1] * x48 + r2 * x16;
-      const TB x370 = s.QC[1] * x48 + s.WQ[1] * x54 + r2 * x19;
-      const TB x371 = s.QC[1] * x54 + s.WQ[1] * x60 + r2 * x22;
-      const TB x373 = s.QC[1] * x43 + s.WQ[1] * x49 + r1 * x17;
-      const TB x374 = s.QC[1] * x49 + s.WQ[1] * x55 + r1 * x20;
-      const TB x377 = s.QC[1] * x44 + s.WQ[1] * x50;
-      const TB x378 = s.QC[1] * x50 + s.WQ[1] * x56;
-      const TB x379 = s.QC[1] * x56 + s.WQ[1] * x62;
-      const TB x415 = s.QC[1] * x113 + s.WQ[1] * x123 + r1 * x62;
-      const TB x550 = s.QC[2] * x113 + s.WQ[2] * x123 + r2 * x61;
-      const TB x404 = s.QC[1] * x81 + s.WQ[1] * x91 + r3 * x42;
-      const TB x405 = s.QC[1] * x91 + s.WQ[1] * x101 + r3 * x48;
-      const TB x406 = s.QC[1] * x101 + s.WQ[1] * x111 + r3 * x54;
-      const TB x407 = s.QC[1] * x111 + s.WQ[1] * x121 + r3 * x60;
-      const TB x542 = s.QC[2] * x111 + s.WQ[2] * x121;
-      const TB x408 = s.QC[1] * x82 + s.WQ[1] * x92 + r2 * x43;
-      const TB x409 = s.QC[1] * x92 + s.WQ[1] * x102 + r2 * x49;
-      const TB x410 = s.QC[1] * x102 + s.WQ[1] * x112 + r2 * x55;
-      const TB x411 = s.QC[1] * x112 + s.WQ[1] * x122 + r2 * x61;
-      const TB x510 = s.QC[2] * x55 + s.WQ[2] * x61 + r1 * x22;
-      const TB x487 = s.QC[2] * x19 + s.WQ[2] * x22;
-      const TB x546 = s.QC[2] * x112 + s.WQ[2] * x122 + r1 * x60;
-      const TB x506 = s.QC[2] * x54 + s.WQ[2] * x60;
-      const TB x412 = s.QC[1] * x83 + s.WQ[1] * x93 + r1 * x44;
-      const TB x413 = s.QC[1] * x93 + s.WQ[1] * x103 + r1 * x50;
-      const TB x414 = s.QC[1] * x103 + s.WQ[1] * x113 + r1 * x56;
-      const TB x416 = s.QC[1] * x84 + s.WQ[1] * x94;
-      const TB x417 = s.QC[1] * x94 + s.WQ[1] * x104;
-      const TB x486 = s.QC[2] * x16 + s.WQ[2] * x19;
-      const TB x504 = s.QC[2] * x42 + s.WQ[2] * x48;
-      const TB x505 = s.QC[2] * x48 + s.WQ[2] * x54;
-      const TB x508 = s.QC[2] * x43 + s.WQ[2] * x49 + r1 * x16;
-      const TB x509 = s.QC[2] * x49 + s.WQ[2] * x55 + r1 * x19;
-      const TB x512 = s.QC[2] * x44 + s.WQ[2] * x50 + r2 * x17;
-      const TB x539 = s.QC[2] * x81 + s.WQ[2] * x91;
-      const TB x540 = s.QC[2] * x91 + s.WQ[2] * x101;
-      const TB x541 = s.QC[2] * x101 + s.WQ[2] * x111;
-      const TB x1160 = s.QC[2] * x541 + s.WQ[2] * x542 + e1 * (x101 - s.roe * x111);
-      const TB x960 = s.QC[1] * x406 + s.WQ[1] * x407 + e1 * (x101 - s.roe * x111) + r3 * x371;
-      const TB x1060 = s.QC[2] * x406 + s.WQ[2] * x407;
-      const TB x543 = s.QC[2] * x82 + s.WQ[2] * x92 + r1 * x42;
-      const TB x544 = s.QC[2] * x92 + s.WQ[2] * x102 + r1 * x48;
-      const TB x545 = s.QC[2] * x102 + s.WQ[2] * x112 + r1 * x54;
-      const TB x1163 = s.QC[2] * x545 + s.WQ[2] * x546 + e1 * (x102 - s.roe * x112) + r1 * x506;
-      const TB x1133 = s.QC[2] * x505 + s.WQ[2] * x506 + e1 * (x48 - s.roe * x54);
-      const TB x547 = s.QC[2] * x83 + s.WQ[2] * x93 + r2 * x43;
-      const TB x548 = s.QC[2] * x93 + s.WQ[2] * x103 + r2 * x49;
-      const TB x549 = s.QC[2] * x103 + s.WQ[2] * x113 + r2 * x55;
-      const TB x1166 = s.QC[2] * x549 + s.WQ[2] * x550 + e1 * (x103 - s.roe * x113) + r2 * x510;
-      const TB x1136 = s.QC[2] * x509 + s.WQ[2] * x510 + e1 * (x49 - s.roe * x55) + r1 * x487;
-      const TB x1119 = s.QC[2] * x486 + s.WQ[2] * x487 + e1 * (x16 - s.roe * x19);
-      const TB x619 = s.QC[0] * x216 + s.WQ[0] * x217 + e1 * (x16 - s.roe * x19);
-      const TB x1135 = s.QC[2] * x508 + s.WQ[2] * x509 + e1 * (x43 - s.roe * x49) + r1 * x486;
-      const TB x1812 = s.QC[2] * x1135 + s.WQ[2] * x1136 + e2 * (x508 - s.roe * x509) + r1 * x1119;
-      const TB x1164 = s.QC[2] * x547 + s.WQ[2] * x548 + e1 * (x83 - s.roe * x93) + r2 * x508;
-      acc169 += static_cast<double>(x1164);
-      const TB x1165 = s.QC[2] * x548 + s.WQ[2] * x549 + e1 * (x93 - s.roe * x103) + r2 * x509;
-      const TB x1832 = s.QC[2] * x1165 + s.WQ[2] * x1166 + e2 * (x548 - s.roe * x549) + r2 * x1136;
-      const TB x1831 = s.QC[2] * x1164 + s.WQ[2] * x1165 + e2 * (x547 - s.roe * x548) + r2 * x1135;
-      acc479 += static_cast<double>(x1831);
-      const TB x2313 = s.QC[2] * x1831 + s.WQ[2] * x1832 + e3 * (x1164 - s.roe * x1165) + r2 * x1812;
-      acc944 += static_cast<double>(x2313);
-      const TB x551 = s.QC[2] * x84 + s.WQ[2] * x94 + r3 * x44;
-      const TB x552 = s.QC[2] * x94 + s.WQ[2] * x104 + r3 * x50;
-      const TB x1167 = s.QC[2] * x551 + s.WQ[2] * x552 + e1 * (x84 - s.roe * x94) + r3 * x512;
-      acc170 += static_cast<double>(x1167);
-      const TB x621 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x17 - s.roe * x20);
-      const TB x921 = s.QC[1] * x354 + s.WQ[1] * x355 + e1 * (x17 - s.roe * x20);
-      const TB x936 = s.QC[1] * x374 + s.WQ[1] * x375 + e1 * (x49 - s.roe * x55) + r1 * x355;
-      const TB x636 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x49 - s.roe * x55);
-      const TB x632 = s.QC[0] * x234 + s.WQ[0] * x235 + e1 * (x42 - s.roe * x48);
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x933 = s.QC[1] * x370 + s.WQ[1] * x371 + e1 * (x48 - s.roe * x54) + r2 * x352;
-      const TB x635 = s.QC[0] * x238 + s.WQ[0] * x239 + e1 * (x43 - s.roe * x49);
-      const TB x935 = s.QC[1] * x373 + s.WQ[1] * x374 + e1 * (x43 - s.roe * x49) + r1 * x354;
-      const TB x1038 = s.QC[2] * x377 + s.WQ[2] * x378 + r2 * x354;
-      const TB x1292 = s.QC[1] * x635 + s.WQ[1] * x636 + r1 * x621;
-      const TB x1617 = s.QC[1] * x935 + s.WQ[1] * x936 + e2 * (x373 - s.roe * x374) + r1 * x921;
-      const TB x638 = s.QC[0] * x242 + s.WQ[0] * x243 + e1 * (x44 - s.roe * x50);
-      const TB x938 = s.QC[1] * x377 + s.WQ[1] * x378 + e1 * (x44 - s.roe * x50);
-      const TB x639 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x50 - s.roe * x56);
-      const TB x939 = s.QC[1] * x378 + s.WQ[1] * x379 + e1 * (x50 - s.roe * x56);
-      const TB x966 = s.QC[1] * x414 + s.WQ[1] * x415 + e1 * (x103 - s.roe * x113) + r1 * x379;
-      const TB x666 = s.QC[0] * x279 + s.WQ[0] * x280 + e1 * (x103 - s.roe * x113);
-      const TB x1066 = s.QC[2] * x414 + s.WQ[2] * x415 + r2 * x375;
-      const TB x661 = s.QC[0] * x273 + s.WQ[0] * x274 + e1 * (x82 - s.roe * x92);
-      acc13 += static_cast<double>(x661);
-      const TB x662 = s.QC[0] * x274 + s.WQ[0] * x275 + e1 * (x92 - s.roe * x102);
-      const TB x663 = s.QC[0] * x275 + s.WQ[0] * x276 + e1 * (x102 - s.roe * x112);
-      const TB x963 = s.QC[1] * x410 + s.WQ[1] * x411 + e1 * (x102 - s.roe * x112) + r2 * x375;
-      const TB x1036 = s.QC[2] * x374 + s.WQ[2] * x375 + r1 * x352;
-      const TB x1019 = s.QC[2] * x351 + s.WQ[2] * x352;
-      const TB x1063 = s.QC[2] * x410 + s.WQ[2] * x411 + r1 * x371;
-      const TB x1033 = s.QC[2] * x370 + s.WQ[2] * x371;
-      const TB x664 = s.QC[0] * x277 + s.WQ[0] * x278 + e1 * (x83 - s.roe * x93);
-      acc14 += static_cast<double>(x664);
-      const TB x964 = s.QC[1] * x412 + s.WQ[1] * x413 + e1 * (x83 - s.roe * x93) + r1 * x377;
-      acc107 += static_cast<double>(x964);
-      const TB x665 = s.QC[0] * x278 + s.WQ[0] * x279 + e1 * (x93 - s.roe * x103);
-      const TB x965 = s.QC[1] * x413 + s.WQ[1] * x414 + e1 * (x93 - s.roe * x103) + r1 * x378;
-      const TB x667 = s.QC[0] * x281 + s.WQ[0] * x282 + e1 * (x84 - s.roe * x94);
-      acc15 += static_cast<double>(x667);
-      const TB x967 = s.QC[1] * x416 + s.WQ[1] * x417 + e1 * (x84 - s.roe * x94);
-      acc108 += static_cast<double>(x967);
-      const TB x668 = s.QC[0] * x282 + s.WQ[0] * x283 + e1 * (x94 - s.roe * x104);
-      const TB x968 = s.QC[1] * x417 + s.WQ[1] * x418 + e1 * (x94 - s.roe * x104);
-      const TB x1068 = s.QC[2] * x417 + s.WQ[2] * x418 + r3 * x378;
-      const TB x1248 = s.QC[0] * x667 + s.WQ[0] * x668 + e2 * (x281 - s.roe * x282);
-      acc201 += static_cast<double>(x1248);
-      const TB x1313 = s.QC[1] * x667 + s.WQ[1] * x668;
-      acc232 += static_cast<double>(x1313);
-      const TB x1378 = s.QC[2] * x667 + s.WQ[2] * x668 + r3 * x638;
-      acc263 += static_cast<double>(x1378);
-      const TB x1619 = s.QC[1] * x938 + s.WQ[1] * x939 + e2 * (x377 - s.roe * x378);
-      const TB x1067 = s.QC[2] * x416 + s.WQ[2] * x417 + r3 * x377;
-      acc139 += static_cast<double>(x1067);
-      const TB x1768 = s.QC[2] * x1067 + s.WQ[2] * x1068 + e1 * (x416 - s.roe * x417) + r3 * x1038;
-      acc449 += static_cast<double>(x1768);
-      const TB x1638 = s.QC[1] * x967 + s.WQ[1] * x968 + e2 * (x416 - s.roe * x417);
-      acc387 += static_cast<double>(x1638);
-      const TB x1703 = s.QC[2] * x967 + s.WQ[2] * x968 + r3 * x938;
-      acc418 += static_cast<double>(x1703);
-      const TB x1636 = s.QC[1] * x964 + s.WQ[1] * x965 + e2 * (x412 - s.roe * x413) + r1 * x938;
-      acc386 += static_cast<double>(x1636);
-      const TB x1637 = s.QC[1] * x965 + s.WQ[1] * x966 + e2 * (x413 - s.roe * x414) + r1 * x939;
-      const TB x2189 = s.QC[1] * x1636 + s.WQ[1] * x1637 + e3 * (x964 - s.roe * x965) + r1 * x1619;
-      acc820 += static_cast<double>(x2189);
-      const TB x2220 = s.QC[2] * x1636 + s.WQ[2] * x1637 + r2 * x1617;
-      acc851 += static_cast<double>(x2220);
-      const TB x1702 = s.QC[2] * x965 + s.WQ[2] * x966 + r2 * x936;
-      const TB x721 = s.QC[1] * x219 + s.WQ[1] * x220;
-      const TB x736 = s.QC[1] * x239 + s.WQ[1] * x240 + r1 * x220;
-      const TB x732 = s.QC[1] * x234 + s.WQ[1] * x235 + r2 * x216;
-      const TB x733 = s.QC[1] * x235 + s.WQ[1] * x236 + r2 * x217;
-      const TB x735 = s.QC[1] * x238 + s.WQ[1] * x239 + r1 * x219;
-      const TB x838 = s.QC[2] * x242 + s.WQ[2] * x243 + r2 * x219;
-      const TB x1422 = s.QC[1] * x735 + s.WQ[1] * x736 + e1 * (x238 - s.roe * x239) + r1 * x721;
-      const TB x738 = s.QC[1] * x242 + s.WQ[1] * x243;
-      const TB x739 = s.QC[1] * x243 + s.WQ[1] * x244;
-      const TB x766 = s.QC[1] * x279 + s.WQ[1] * x280 + r1 * x244;
-      const TB x866 = s.QC[2] * x279 + s.WQ[2] * x280 + r2 * x240;
-      const TB x761 = s.QC[1] * x273 + s.WQ[1] * x274 + r2 * x238;
-      acc44 += static_cast<double>(x761);
-      const TB x762 = s.QC[1] * x274 + s.WQ[1] * x275 + r2 * x239;
-      const TB x763 = s.QC[1] * x275 + s.WQ[1] * x276 + r2 * x240;
-      const TB x836 = s.QC[2] * x239 + s.WQ[2] * x240 + r1 * x217;
-      const TB x819 = s.QC[2] * x216 + s.WQ[2] * x217;
-      const TB x835 = s.QC[2] * x238 + s.WQ[2] * x239 + r1 * x216;
-      const TB x863 = s.QC[2] * x275 + s.WQ[2] * x276 + r1 * x236;
-      const TB x833 = s.QC[2] * x235 + s.WQ[2] * x236;
-      const TB x1552 = s.QC[2] * x835 + s.WQ[2] * x836 + e1 * (x238 - s.roe * x239) + r1 * x819;
-      const TB x764 = s.QC[1] * x277 + s.WQ[1] * x278 + r1 * x242;
-      acc45 += static_cast<double>(x764);
-      const TB x765 = s.QC[1] * x278 + s.WQ[1] * x279 + r1 * x243;
-      const TB x767 = s.QC[1] * x281 + s.WQ[1] * x282;
-      acc46 += static_cast<double>(x767);
-      const TB x768 = s.QC[1] * x282 + s.WQ[1] * x283;
-      const TB x868 = s.QC[2] * x282 + s.WQ[2] * x283 + r3 * x243;
-      const TB x1443 = s.QC[1] * x767 + s.WQ[1] * x768 + e1 * (x281 - s.roe * x282);
-      acc294 += static_cast<double>(x1443);
-      const TB x1508 = s.QC[2] * x767 + s.WQ[2] * x768 + r3 * x738;
-      acc325 += static_cast<double>(x1508);
-      const TB x832 = s.QC[2] * x234 + s.WQ[2] * x235;
-      const TB x861 = s.QC[2] * x273 + s.WQ[2] * x274 + r1 * x234;
-      acc75 += static_cast<double>(x861);
-      const TB x862 = s.QC[2] * x274 + s.WQ[2] * x275 + r1 * x235;
-      const TB x1570 = s.QC[2] * x862 + s.WQ[2] * x863 + e1 * (x274 - s.roe * x275) + r1 * x833;
-      const TB x1550 = s.QC[2] * x832 + s.WQ[2] * x833 + e1 * (x234 - s.roe * x235);
-      const TB x1569 = s.QC[2] * x861 + s.WQ[2] * x862 + e1 * (x273 - s.roe * x274) + r1 * x832;
-      acc354 += static_cast<double>(x1569);
-      const TB x2157 = s.QC[2] * x1569 + s.WQ[2] * x1570 + e2 * (x861 - s.roe * x862) + r1 * x1550;
-      acc788 += static_cast<double>(x2157);
-      const TB x864 = s.QC[2] * x277 + s.WQ[2] * x278 + r2 * x238;
-      acc76 += static_cast<double>(x864);
-      const TB x1227 = s.QC[0] * x635 + s.WQ[0] * x636 + e2 * (x238 - s.roe * x239);
-      const TB x865 = s.QC[2] * x278 + s.WQ[2] * x279 + r2 * x239;
-      const TB x1572 = s.QC[2] * x865 + s.WQ[2] * x866 + e1 * (x278 - s.roe * x279) + r2 * x836;
-      const TB x1571 = s.QC[2] * x864 + s.WQ[2] * x865 + e1 * (x277 - s.roe * x278) + r2 * x835;
-      acc355 += static_cast<double>(x1571);
-      const TB x2158 = s.QC[2] * x1571 + s.WQ[2] * x1572 + e2 * (x864 - s.roe * x865) + r2 * x1552;
-      acc789 += static_cast<double>(x2158);
-      const TB x867 = s.QC[2] * x281 + s.WQ[2] * x282 + r3 * x242;
-      acc77 += static_cast<double>(x867);
-      const TB x1573 = s.QC[2] * x867 + s.WQ[2] * x868 + e1 * (x281 - s.roe * x282) + r3 * x838;
-      acc356 += static_cast<double>(x1573);
-      const TB x932 = s.QC[1] * x369 + s.WQ[1] * x370 + e1 * (x42 - s.roe * x48) + r2 * x351;
-      const TB x1132 = s.QC[2] * x504 + s.WQ[2] * x505 + e1 * (x42 - s.roe * x48);
-      const TB x1035 = s.QC[2] * x373 + s.WQ[2] * x374 + r1 * x351;
-      const TB x1747 = s.QC[2] * x1035 + s.WQ[2] * x1036 + e1 * (x373 - s.roe * x374) + r1 * x1019;
-      const TB x958 = s.QC[1] * x404 + s.WQ[1] * x405 + e1 * (x81 - s.roe * x91) + r3 * x369;
-      const TB x1158 = s.QC[2] * x539 + s.WQ[2] * x540 + e1 * (x81 - s.roe * x91);
-      const TB x959 = s.QC[1] * x405 + s.WQ[1] * x406 + e1 * (x91 - s.roe * x101) + r3 * x370;
-      const TB x1159 = s.QC[2] * x540 + s.WQ[2] * x541 + e1 * (x91 - s.roe * x101);
-      const TB x1828 = s.QC[2] * x1159 + s.WQ[2] * x1160 + e2 * (x540 - s.roe * x541);
-      const TB x1827 = s.QC[2] * x1158 + s.WQ[2] * x1159 + e2 * (x539 - s.roe * x540);
-      const TB x2311 = s.QC[2] * x1827 + s.WQ[2] * x1828 + e3 * (x1158 - s.roe * x1159);
-      acc942 += static_cast<double>(x2311);
-      const TB x1698 = s.QC[2] * x959 + s.WQ[2] * x960;
-      const TB x961 = s.QC[1] * x408 + s.WQ[1] * x409 + e1 * (x82 - s.roe * x92) + r2 * x373;
-      acc106 += static_cast<double>(x961);
-      const TB x1064 = s.QC[2] * x412 + s.WQ[2] * x413 + r2 * x373;
-      acc138 += static_cast<double>(x1064);
-      const TB x1161 = s.QC[2] * x543 + s.WQ[2] * x544 + e1 * (x82 - s.roe * x92) + r1 * x504;
-      acc168 += static_cast<double>(x1161);
-      const TB x1810 = s.QC[2] * x1132 + s.WQ[2] * x1133 + e2 * (x504 - s.roe * x505);
-      const TB x1162 = s.QC[2] * x544 + s.WQ[2] * x545 + e1 * (x92 - s.roe * x102) + r1 * x505;
-      const TB x1830 = s.QC[2] * x1162 + s.WQ[2] * x1163 + e2 * (x544 - s.roe * x545) + r1 * x1133;
-      const TB x1829 = s.QC[2] * x1161 + s.WQ[2] * x1162 + e2 * (x543 - s.roe * x544) + r1 * x1132;
-      acc478 += static_cast<double>(x1829);
-      const TB x2312 = s.QC[2] * x1829 + s.WQ[2] * x1830 + e3 * (x1161 - s.roe * x1162) + r1 * x1810;
-      acc943 += static_cast<double>(x2312);
-      const TB x962 = s.QC[1] * x409 + s.WQ[1] * x410 + e1 * (x92 - s.roe * x102) + r2 * x374;
-      const TB x1065 = s.QC[2] * x413 + s.WQ[2] * x414 + r2 * x374;
-      const TB x1767 = s.QC[2] * x1065 + s.WQ[2] * x1066 + e1 * (x413 - s.roe * x414) + r2 * x1036;
-      const TB x1766 = s.QC[2] * x1064 + s.WQ[2] * x1065 + e1 * (x412 - s.roe * x413) + r2 * x1035;
-      acc448 += static_cast<double>(x1766);
-      const TB x2282 = s.QC[2] * x1766 + s.WQ[2] * x1767 + e2 * (x1064 - s.roe * x1065) + r2 * x1747;
-      acc913 += static_cast<double>(x2282);
-      const TB x1032 = s.QC[2] * x369 + s.WQ[2] * x370;
-      const TB x1058 = s.QC[2] * x404 + s.WQ[2] * x405;
-      const TB x1059 = s.QC[2] * x405 + s.WQ[2] * x406;
-      const TB x1763 = s.QC[2] * x1059 + s.WQ[2] * x1060 + e1 * (x405 - s.roe * x406);
-      const TB x1762 = s.QC[2] * x1058 + s.WQ[2] * x1059 + e1 * (x404 - s.roe * x405);
-      const TB x2280 = s.QC[2] * x1762 + s.WQ[2] * x1763 + e2 * (x1058 - s.roe * x1059);
-      acc911 += static_cast<double>(x2280);
-      const TB x1061 = s.QC[2] * x408 + s.WQ[2] * x409 + r1 * x369;
-      acc137 += static_cast<double>(x1061);
-      const TB x1062 = s.QC[2] * x409 + s.WQ[2] * x410 + r1 * x370;
-      const TB x1765 = s.QC[2] * x1062 + s.WQ[2] * x1063 + e1 * (x409 - s.roe * x410) + r1 * x1033;
-      const TB x1635 = s.QC[1] * x962 + s.WQ[1] * x963 + e2 * (x409 - s.roe * x410) + r2 * x936;
-      const TB x1682 = s.QC[2] * x935 + s.WQ[2] * x936 + r1 * x919;
-      const TB x1615 = s.QC[1] * x932 + s.WQ[1] * x933 + e2 * (x369 - s.roe * x370) + r2 * x919;
-      const TB x1745 = s.QC[2] * x1032 + s.WQ[2] * x1033 + e1 * (x369 - s.roe * x370);
-      const TB x1700 = s.QC[2] * x962 + s.WQ[2] * x963 + r1 * x933;
-      const TB x1680 = s.QC[2] * x932 + s.WQ[2] * x933;
-      const TB x1699 = s.QC[2] * x961 + s.WQ[2] * x962 + r1 * x932;
-      acc416 += static_cast<double>(x1699);
-      const TB x2250 = s.QC[2] * x1699 + s.WQ[2] * x1700 + e1 * (x961 - s.roe * x962) + r1 * x1680;
-      acc881 += static_cast<double>(x2250);
-      const TB x1764 = s.QC[2] * x1061 + s.WQ[2] * x1062 + e1 * (x408 - s.roe * x409) + r1 * x1032;
-      acc447 += static_cast<double>(x1764);
-      const TB x2281 = s.QC[2] * x1764 + s.WQ[2] * x1765 + e2 * (x1061 - s.roe * x1062) + r1 * x1745;
-      acc912 += static_cast<double>(x2281);
-      const TB x1634 = s.QC[1] * x961 + s.WQ[1] * x962 + e2 * (x408 - s.roe * x409) + r2 * x935;
-      acc385 += static_cast<double>(x1634);
-      const TB x2188 = s.QC[1] * x1634 + s.WQ[1] * x1635 + e3 * (x961 - s.roe * x962) + r2 * x1617;
-      acc819 += static_cast<double>(x2188);
-      const TB x2219 = s.QC[2] * x1634 + s.WQ[2] * x1635 + r1 * x1615;
-      acc850 += static_cast<double>(x2219);
-      const TB x1701 = s.QC[2] * x964 + s.WQ[2] * x965 + r2 * x935;
-      acc417 += static_cast<double>(x1701);
-      const TB x2251 = s.QC[2] * x1701 + s.WQ[2] * x1702 + e1 * (x964 - s.roe * x965) + r2 * x1682;
-      acc882 += static_cast<double>(x2251);
-      const TB x1225 = s.QC[0] * x632 + s.WQ[0] * x633 + e2 * (x234 - s.roe * x235);
-      const TB x1420 = s.QC[1] * x732 + s.WQ[1] * x733 + e1 * (x234 - s.roe * x235) + r2 * x719;
-      const TB x1487 = s.QC[2] * x735 + s.WQ[2] * x736 + r1 * x719;
-      const TB x1229 = s.QC[0] * x638 + s.WQ[0] * x639 + e2 * (x242 - s.roe * x243);
-      const TB x1424 = s.QC[1] * x738 + s.WQ[1] * x739 + e1 * (x242 - s.roe * x243);
-      const TB x1441 = s.QC[1] * x764 + s.WQ[1] * x765 + e1 * (x277 - s.roe * x278) + r1 * x738;
-      acc293 += static_cast<double>(x1441);
-      const TB x1246 = s.QC[0] * x664 + s.WQ[0] * x665 + e2 * (x277 - s.roe * x278);
-      acc200 += static_cast<double>(x1246);
-      const TB x1442 = s.QC[1] * x765 + s.WQ[1] * x766 + e1 * (x278 - s.roe * x279) + r1 * x739;
-      const TB x1247 = s.QC[0] * x665 + s.WQ[0] * x666 + e2 * (x278 - s.roe * x279);
-      const TB x1910 = s.QC[1] * x1246 + s.WQ[1] * x1247 + r1 * x1229;
-      acc541 += static_cast<double>(x1910);
-      const TB x2065 = s.QC[1] * x1441 + s.WQ[1] * x1442 + e2 * (x764 - s.roe * x765) + r1 * x1424;
-      acc696 += static_cast<double>(x2065);
-      const TB x2096 = s.QC[2] * x1441 + s.WQ[2] * x1442 + r2 * x1422;
-      acc727 += static_cast<double>(x2096);
-      const TB x1507 = s.QC[2] * x765 + s.WQ[2] * x766 + r2 * x736;
-      const TB x1440 = s.QC[1] * x762 + s.WQ[1] * x763 + e1 * (x274 - s.roe * x275) + r2 * x736;
-      const TB x1245 = s.QC[0] * x662 + s.WQ[0] * x663 + e2 * (x274 - s.roe * x275);
-      const TB x1505 = s.QC[2] * x762 + s.WQ[2] * x763 + r1 * x733;
-      const TB x1485 = s.QC[2] * x732 + s.WQ[2] * x733;
-      const TB x1504 = s.QC[2] * x761 + s.WQ[2] * x762 + r1 * x732;
-      acc323 += static_cast<double>(x1504);
-      const TB x2126 = s.QC[2] * x1504 + s.WQ[2] * x1505 + e1 * (x761 - s.roe * x762) + r1 * x1485;
-      acc757 += static_cast<double>(x2126);
-      const TB x1879 = s.QC[0] * x1246 + s.WQ[0] * x1247 + e3 * (x664 - s.roe * x665);
-      acc510 += static_cast<double>(x1879);
-      const TB x1941 = s.QC[2] * x1246 + s.WQ[2] * x1247 + r2 * x1227;
-      acc572 += static_cast<double>(x1941);
-      const TB x1244 = s.QC[0] * x661 + s.WQ[0] * x662 + e2 * (x273 - s.roe * x274);
-      acc199 += static_cast<double>(x1244);
-      const TB x1439 = s.QC[1] * x761 + s.WQ[1] * x762 + e1 * (x273 - s.roe * x274) + r2 * x735;
-      acc292 += static_cast<double>(x1439);
-      const TB x2064 = s.QC[1] * x1439 + s.WQ[1] * x1440 + e2 * (x761 - s.roe * x762) + r2 * x1422;
-      acc695 += static_cast<double>(x2064);
-      const TB x2095 = s.QC[2] * x1439 + s.WQ[2] * x1440 + r1 * x1420;
-      acc726 += static_cast<double>(x2095);
-      const TB x1909 = s.QC[1] * x1244 + s.WQ[1] * x1245 + r2 * x1227;
-      acc540 += static_cast<double>(x1909);
-      const TB x1940 = s.QC[2] * x1244 + s.WQ[2] * x1245 + r1 * x1225;
-      acc571 += static_cast<double>(x1940);
-      const TB x1878 = s.QC[0] * x1244 + s.WQ[0] * x1245 + e3 * (x661 - s.roe * x662);
-      acc509 += static_cast<double>(x1878);
-      const TB x1506 = s.QC[2] * x764 + s.WQ[2] * x765 + r2 * x735;
-      acc324 += static_cast<double>(x1506);
-      const TB x2127 = s.QC[2] * x1506 + s.WQ[2] * x1507 + e1 * (x764 - s.roe * x765) + r2 * x1487;
-      acc758 += static_cast<double>(x2127);
-      const TB x1290 = s.QC[1] * x632 + s.WQ[1] * x633 + r2 * x619;
-      const TB x1357 = s.QC[2] * x635 + s.WQ[2] * x636 + r1 * x619;
-      const TB x1294 = s.QC[1] * x638 + s.WQ[1] * x639;
-      const TB x1311 = s.QC[1] * x664 + s.WQ[1] * x665 + r1 * x638;
-      acc231 += static_cast<double>(x1311);
-      const TB x1312 = s.QC[1] * x665 + s.WQ[1] * x666 + r1 * x639;
-      const TB x1972 = s.QC[1] * x1311 + s.WQ[1] * x1312 + e1 * (x664 - s.roe * x665) + r1 * x1294;
-      acc603 += static_cast<double>(x1972);
-      const TB x2003 = s.QC[2] * x1311 + s.WQ[2] * x1312 + r2 * x1292;
-      acc634 += static_cast<double>(x2003);
-      const TB x1377 = s.QC[2] * x665 + s.WQ[2] * x666 + r2 * x636;
-      const TB x1310 = s.QC[1] * x662 + s.WQ[1] * x663 + r2 * x636;
-      const TB x1375 = s.QC[2] * x662 + s.WQ[2] * x663 + r1 * x633;
-      const TB x1355 = s.QC[2] * x632 + s.WQ[2] * x633;
-      const TB x1374 = s.QC[2] * x661 + s.WQ[2] * x662 + r1 * x632;
-      acc261 += static_cast<double>(x1374);
-      const TB x2033 = s.QC[2] * x1374 + s.WQ[2] * x1375 + e1 * (x661 - s.roe * x662) + r1 * x1355;
-      acc664 += static_cast<double>(x2033);
-      const TB x1309 = s.QC[1] * x661 + s.WQ[1] * x662 + r2 * x635;
-      acc230 += static_cast<double>(x1309);
-      const TB x1971 = s.QC[1] * x1309 + s.WQ[1] * x1310 + e1 * (x661 - s.roe * x662) + r2 * x1292;
-      acc602 += static_cast<double>(x1971);
-      const TB x2002 = s.QC[2] * x1309 + s.WQ[2] * x1310 + r1 * x1290;
-      acc633 += static_cast<double>(x2002);
-      const TB x1376 = s.QC[2] * x664 + s.WQ[2] * x665 + r2 * x635;
-      acc262 += static_cast<double>(x1376);
-      const TB x2034 = s.QC[2] * x1376 + s.WQ[2] * x1377 + e1 * (x664 - s.roe * x665) + r2 * x1357;
-      acc665 += static_cast<double>(x2034);
-      const TB x1697 = s.QC[2] * x958 + s.WQ[2] * x959;
-      const TB x2249 = s.QC[2] * x1697 + s.WQ[2] * x1698 + e1 * (x958 - s.roe * x959);
-      acc880 += static_cast<double>(x2249);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(880 * R + r) * a.acc_stride + q] = acc880;
-  a.acc[(911 * R + r) * a.acc_stride + q] = acc911;
-  a.acc[(942 * R + r) * a.acc_stride + q] = acc942;
-  a.acc[(13 * R + r) * a.acc_stride + q] = acc13;
-  a.acc[(44 * R + r) * a.acc_stride + q] = acc44;
-  a.acc[(75 * R + r) * a.acc_stride + q] = acc75;
-  a.acc[(106 * R + r) * a.acc_stride + q] = acc106;
-  a.acc[(137 * R + r) * a.acc_stride + q] = acc137;
-  a.acc[(168 * R + r) * a.acc_stride + q] = acc168;
-  a.acc[(199 * R + r) * a.acc_stride + q] = acc199;
-  a.acc[(230 * R + r) * a.acc_stride + q] = acc230;
-  a.acc[(261 * R + r) * a.acc_stride + q] = acc261;
-  a.acc[(292 * R + r) * a.acc_stride + q] = acc292;
-  a.acc[(323 * R + r) * a.acc_stride + q] = acc323;
-  a.acc[(354 * R + r) * a.acc_stride + q] = acc354;
-  a.acc[(385 * R + r) * a.acc_stride + q] = acc385;
-  a.acc[(416 * R + r) * a.acc_stride + q] = acc416;
-  a.acc[(447 * R + r) * a.acc_stride + q] = acc447;
-  a.acc[(478 * R + r) * a.acc_stride + q] = acc478;
-  a.acc[(509 * R + r) * a.acc_stride + q] = acc509;
-  a.acc[(540 * R + r) * a.acc_stride + q] = acc540;
-  a.acc[(571 * R + r) * a.acc_stride + q] = acc571;
-  a.acc[(602 * R + r) * a.acc_stride + q] = acc602;
-  a.acc[(633 * R + r) * a.acc_stride + q] = acc633;
-  a.acc[(664 * R + r) * a.acc_stride + q] = acc664;
-  a.acc[(695 * R + r) * a.acc_stride + q] = acc695;
-  a.acc[(726 * R + r) * a.acc_stride + q] = acc726;
-  a.acc[(757 * R + r) * a.acc_stride + q] = acc757;
-  a.acc[(788 * R + r) * a.acc_stride + q] = acc788;
-  a.acc[(819 * R + r) * a.acc_stride + q] = acc819;
-  a.acc[(850 * R + r) * a.acc_stride + q] = acc850;
-  a.acc[(881 * R + r) * a.acc_stride + q] = acc881;
-  a.acc[(912 * R + r) * a.acc_stride + q] = acc912;
-  a.acc[(943 * R + r) * a.acc_stride + q] = acc943;
-  a.acc[(14 * R + r) * a.acc_stride + q] = acc14;
-  a.acc[(45 * R + r) * a.acc_stride + q] = acc45;
-  a.acc[(76 * R + r) * a.acc_stride + q] = acc76;
-  a.acc[(107 * R + r) * a.acc_stride + q] = acc107;
-  a.acc[(138 * R + r) * a.acc_stride + q] = acc138;
-  a.acc[(169 * R + r) * a.acc_stride + q] = acc169;
-  a.acc[(200 * R + r) * a.acc_stride + q] = acc200;
-  a.acc[(231 * R + r) * a.acc_stride + q] = acc231;
-  a.acc[(262 * R + r) * a.acc_stride + q] = acc262;
-  a.acc[(293 * R + r) * a.acc_stride + q] = acc293;
-  a.acc[(324 * R + r) * a.acc_stride + q] = acc324;
-  a.acc[(355 * R + r) * a.acc_stride + q] = acc355;
-  a.acc[(386 * R + r) * a.acc_stride + q] = acc386;
-  a.acc[(417 * R + r) * a.acc_stride + q] = acc417;
-  a.acc[(448 * R + r) * a.acc_stride + q] = acc448;
-  a.acc[(479 * R + r) * a.acc_stride + q] = acc479;
-  a.acc[(510 * R + r) * a.acc_stride + q] = acc510;
-  a.acc[(541 * R + r) * a.acc_stride + q] = acc541;
-  a.acc[(572 * R + r) * a.acc_stride + q] = acc572;
-  a.acc[(603 * R + r) * a.acc_stride + q] = acc603;
-  a.acc[(634 * R + r) * a.acc_stride + q] = acc634;
-  a.acc[(665 * R + r) * a.acc_stride + q] = acc665;
-  a.acc[(696 * R + r) * a.acc_stride + q] = acc696;
-  a.acc[(727 * R + r) * a.acc_stride + q] = acc727;
-  a.acc[(758 * R + r) * a.acc_stride + q] = acc758;
-  a.acc[(789 * R + r) * a.acc_stride + q] = acc789;
-  a.acc[(820 * R + r) * a.acc_stride + q] = acc820;
-  a.acc[(851 * R + r) * a.acc_stride + q] = acc851;
-  a.acc[(882 * R + r) * a.acc_stride + q] = acc882;
-  a.acc[(913 * R + r) * a.acc_stride + q] = acc913;
-  a.acc[(944 * R + r) * a.acc_stride + q] = acc944;
-  a.acc[(15 * R + r) * a.acc_stride + q] = acc15;
-  a.acc[(46 * R + r) * a.acc_stride + q] = acc46;
-  a.acc[(77 * R + r) * a.acc_stride + q] = acc77;
-  a.acc[(108 * R + r) * a.acc_stride + q] = acc108;
-  a.acc[(139 * R + r) * a.acc_stride + q] = acc139;
-  a.acc[(170 * R + r) * a.acc_stride + q] = acc170;
-  a.acc[(201 * R + r) * a.acc_stride + q] = acc201;
-  a.acc[(232 * R + r) * a.acc_stride + q] = acc232;
-  a.acc[(263 * R + r) * a.acc_stride + q] = acc263;
-  a.acc[(294 * R + r) * a.acc_stride + q] = acc294;
-  a.acc[(325 * R + r) * a.acc_stride + q] = acc325;
-  a.acc[(356 * R + r) * a.acc_stride + q] = acc356;
-  a.acc[(387 * R + r) * a.acc_stride + q] = acc387;
-  a.acc[(418 * R + r) * a.acc_stride + q] = acc418;
-  a.acc[(449 * R + r) * a.acc_stride + q] = acc449;
-}
-// class (22|22) split 6/12: 340 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_6(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc480 = 0.0;
-  double acc511 = 0.0;
-  double acc542 = 0.0;
-  double acc573 = 0.0;
-  double acc604 = 0.0;
-  double acc635 = 0.0;
-  double acc666 = 0.0;
-  double acc697 = 0.0;
-  double acc728 = 0.0;
-  double acc759 = 0.0;
-  double acc790 = 0.0;
-  double acc821 = 0.0;
-  double acc852 = 0.0;
-  double acc883 = 0.0;
-  double acc914 = 0.0;
-  double acc945 = 0.0;
-  double acc16 = 0.0;
-  double acc47 = 0.0;
-  double acc78 = 0.0;
-  double acc109 = 0.0;
-  double acc140 = 0.0;
-  double acc171 = 0.0;
-  double acc202 = 0.0;
-  double acc233 = 0.0;
-  double acc264 = 0.0;
-  double acc295 = 0.0;
-  double acc326 = 0.0;
-  double acc357 = 0.0;
-  double acc388 = 0.0;
-  double acc419 = 0.0;
-  double acc450 = 0.0;
-  double acc481 = 0.0;
-  double acc512 = 0.0;
-  double acc543 = 0.0;
-  double acc574 = 0.0;
-  double acc605 = 0.0;
-  double acc636 = 0.0;
-  double acc667 = 0.0;
-  double acc698 = 0.0;
-  double acc729 = 0.0;
-  double acc760 = 0.0;
-  double acc791 = 0.0;
-  double acc822 = 0.0;
-  double acc853 = 0.0;
-  double acc884 = 0.0;
-  double acc915 = 0.0;
-  double acc946 = 0.0;
-  double acc17 = 0.0;
-  double acc48 = 0.0;
-  double acc79 = 0.0;
-  double acc110 = 0.0;
-  double acc141 = 0.0;
-  double acc172 = 0.0;
-  double acc203 = 0.0;
-  double acc234 = 0.0;
-  double acc265 = 0.0;
-  double acc296 = 0.0;
-  double acc327 = 0.0;
-  double acc358 = 0.0;
-  double acc389 = 0.0;
-  double acc420 = 0.0;
-  double acc451 = 0.0;
-  double acc482 = 0.0;
-  double acc513 = 0.0;
-  double acc544 = 0.0;
-  double acc575 = 0.0;
-  double acc606 = 0.0;
-  double acc637 = 0.0;
-  double acc668 = 0.0;
-  double acc699 = 0.0;
-  double acc730 = 0.0;
-  double acc761 = 0.0;
-  double acc792 = 0.0;
-  double acc823 = 0.0;
-  double acc854 = 0.0;
-  double acc885 = 0.0;
-  double acc916 = 0.0;
-  double acc947 = 0.0;
-  double acc18 = 0.0;
-  double acc49 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB x8 = static_cast<TB>(F[8] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x30 = s.PA[0] * x7 + s.WP[0] * x8;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x11 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x69 = s.PA[0] * x27 + s.WP[0] * x30 + z1 * (x6 - s.roz * x7);
-      const TB x29 = s.PA[2] * x6 + s.WP[2] * x7;
-      const TB x68 = s.PA[2] * x26 + s.WP[2] * x29 + z1 * (x5 - s.roz * x6);
-      const TB x63 = s.PA[0] * x24 + s.WP[0] * x27 + z1 * (x5 - s.roz * x6);
-      const TB x125 = s.PA[0] * x63 + s.WP[0] * x69 + z2 * (x24 - s.roz * x27);
-      const TB x33 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x38 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x0 - s.roz * x1);
-      const TB x39 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x84 = s.PA[2] * x38 + s.WP[2] * x44 + z2 * (x11 - s.roz * x14);
-      const TB x75 = s.PA[0] * x33 + s.WP[0] * x39 + z2 * (x9 - s.roz * x12);
-      const TB x45 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x85 = s.PA[0] * x39 + s.WP[0] * x45 + z2 * (x12 - s.roz * x15);
-      const TB x94 = s.PA[2] * x44 + s.WP[2] * x50 + z2 * (x14 - s.roz * x17);
-      const TB x135 = s.PA[0] * x75 + s.WP[0] * x85 + z3 * (x33 - s.roz * x39);
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x51 = s.PA[0] * x18 + s.WP[0] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x95 = s.PA[0] * x45 + s.WP[0] * x51 + z2 * (x15 - s.roz * x18);
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x57 = s.PA[0] * x21 + s.WP[0] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x124 = s.PA[2] * x62 + s.WP[2] * x68 + z2 * (x23 - s.roz * x26);
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x115 = s.PA[0] * x57 + s.WP[0] * x63 + z2 * (x21 - s.roz * x24);
-      const TB x86 = s.PA[1] * x39 + s.WP[1] * x45;
-      const TB x87 = s.PA[2] * x39 + s.WP[2] * x45;
-      const TB x150 = s.PA[0] * x85 + s.WP[0] * x95 + z3 * (x39 - s.roz * x45);
-      const TB x96 = s.PA[1] * x45 + s.WP[1] * x51;
-      const TB x97 = s.PA[2] * x45 + s.WP[2] * x51;
-      const TB x254 = s.QC[0] * x87 + s.WQ[0] * x97 + r2 * x47;
-      const TB x104 = s.PA[2] * x50 + s.WP[2] * x56 + z2 * (x17 - s.roz * x20);
-      const TB x105 = s.PA[0] * x51 + s.WP[0] * x57 + z2 * (x18 - s.roz * x21);
-      const TB x106 = s.PA[1] * x51 + s.WP[1] * x57;
-      const TB x114 = s.PA[2] * x56 + s.WP[2] * x62 + z2 * (x20 - s.roz * x23);
-      const TB x116 = s.PA[1] * x57 + s.WP[1] * x63;
-      const TB x195 = s.PA[0] * x115 + s.WP[0] * x125 + z3 * (x57 - s.roz * x63);
-      const TB x196 = s.PA[1] * x115 + s.WP[1] * x125;
-      const TB x136 = s.PA[1] * x75 + s.WP[1] * x85;
-      const TB x137 = s.PA[2] * x75 + s.WP[2] * x85;
-      const TB x151 = s.PA[1] * x85 + s.WP[1] * x95;
-      const TB x152 = s.PA[2] * x85 + s.WP[2] * x95;
-      const TB x293 = s.QC[0] * x137 + s.WQ[0] * x152 + r3 * x87;
-      const TB x165 = s.PA[0] * x95 + s.WP[0] * x105 + z3 * (x45 - s.roz * x51);
-      const TB x166 = s.PA[1] * x95 + s.WP[1] * x105;
-      const TB x167 = s.PA[2] * x95 + s.WP[2] * x105;
-      const TB x294 = s.QC[0] * x152 + s.WQ[0] * x167 + r3 * x97;
-      const TB x676 = s.QC[0] * x293 + s.WQ[0] * x294 + e1 * (x137 - s.roe * x152) + r3 * x254;
-      acc18 += static_cast<double>(x676);
-      const TB x776 = s.QC[1] * x293 + s.WQ[1] * x294;
-      acc49 += static_cast<double>(x776);
-      const TB x180 = s.PA[0] * x105 + s.WP[0] * x115 + z3 * (x51 - s.roz * x57);
-      const TB x181 = s.PA[1] * x105 + s.WP[1] * x115;
-      const TB x211 = s.QC[0] * x3 + s.WQ[0] * x4;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x223 = s.QC[0] * x45 + s.WQ[0] * x51 + r2 * x18;
-      const TB x224 = s.QC[0] * x51 + s.WQ[0] * x57 + r2 * x21;
-      const TB x248 = s.QC[0] * x105 + s.WQ[0] * x115 + r3 * x57;
-      const TB x624 = s.QC[0] * x223 + s.WQ[0] * x224 + e1 * (x45 - s.roe * x51) + r2 * x214;
-      const TB x246 = s.QC[0] * x85 + s.WQ[0] * x95 + r3 * x45;
-      const TB x247 = s.QC[0] * x95 + s.WQ[0] * x105 + r3 * x51;
-      const TB x641 = s.QC[0] * x246 + s.WQ[0] * x247 + e1 * (x85 - s.roe * x95) + r3 * x223;
-      const TB x642 = s.QC[0] * x247 + s.WQ[0] * x248 + e1 * (x95 - s.roe * x105) + r3 * x224;
-      const TB x1231 = s.QC[0] * x641 + s.WQ[0] * x642 + e2 * (x246 - s.roe * x247) + r3 * x624;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x252 = s.QC[0] * x106 + s.WQ[0] * x116 + r2 * x58;
-      const TB x292 = s.QC[0] * x181 + s.WQ[0] * x196 + r3 * x116;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x250 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x46;
-      const TB x251 = s.QC[0] * x96 + s.WQ[0] * x106 + r2 * x52;
-      const TB x644 = s.QC[0] * x250 + s.WQ[0] * x251 + e1 * (x86 - s.roe * x96) + r2 * x227;
-      const TB x289 = s.QC[0] * x136 + s.WQ[0] * x151 + r3 * x86;
-      const TB x645 = s.QC[0] * x251 + s.WQ[0] * x252 + e1 * (x96 - s.roe * x106) + r2 * x228;
-      const TB x290 = s.QC[0] * x151 + s.WQ[0] * x166 + r3 * x96;
-      const TB x291 = s.QC[0] * x166 + s.WQ[0] * x181 + r3 * x106;
-      const TB x675 = s.QC[0] * x291 + s.WQ[0] * x292 + e1 * (x166 - s.roe * x181) + r3 * x252;
-      const TB x1233 = s.QC[0] * x644 + s.WQ[0] * x645 + e2 * (x250 - s.roe * x251) + r2 * x627;
-      const TB x673 = s.QC[0] * x289 + s.WQ[0] * x290 + e1 * (x136 - s.roe * x151) + r3 * x250;
-      acc17 += static_cast<double>(x673);
-      const TB x674 = s.QC[0] * x290 + s.WQ[0] * x291 + e1 * (x151 - s.roe * x166) + r3 * x251;
-      const TB x1252 = s.QC[0] * x673 + s.WQ[0] * x674 + e2 * (x289 - s.roe * x290) + r3 * x644;
-      acc203 += static_cast<double>(x1252);
-      const TB x1253 = s.QC[0] * x674 + s.WQ[0] * x675 + e2 * (x290 - s.roe * x291) + r3 * x645;
-      const TB x1882 = s.QC[0] * x1252 + s.WQ[0] * x1253 + e3 * (x673 - s.roe * x674) + r3 * x1233;
-      acc513 += static_cast<double>(x1882);
-      const TB x1913 = s.QC[1] * x1252 + s.WQ[1] * x1253 + r1 * x1231;
-      acc544 += static_cast<double>(x1913);
-      const TB x1944 = s.QC[2] * x1252 + s.WQ[2] * x1253;
-      acc575 += static_cast<double>(x1944);
-      const TB x219 = s.QC[0] * x17 + s.WQ[0] * x20;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x821 = s.QC[2] * x219 + s.WQ[2] * x220 + r1 * x211;
-      const TB x242 = s.QC[0] * x44 + s.WQ[0] * x50;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x244 = s.QC[0] * x56 + s.WQ[0] * x62;
-      const TB x281 = s.QC[0] * x84 + s.WQ[0] * x94;
-      const TB x282 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x283 = s.QC[0] * x104 + s.WQ[0] * x114;
-      const TB x284 = s.QC[0] * x114 + s.WQ[0] * x124;
-      const TB x285 = s.QC[0] * x135 + s.WQ[0] * x150 + r4 * x85;
-      const TB x286 = s.QC[0] * x150 + s.WQ[0] * x165 + r4 * x95;
-      const TB x287 = s.QC[0] * x165 + s.WQ[0] * x180 + r4 * x105;
-      const TB x288 = s.QC[0] * x180 + s.WQ[0] * x195 + r4 * x115;
-      const TB x346 = s.QC[1] * x3 + s.WQ[1] * x4;
-      const TB x354 = s.QC[1] * x17 + s.WQ[1] * x20;
-      const TB x355 = s.QC[1] * x20 + s.WQ[1] * x23;
-      const TB x1021 = s.QC[2] * x354 + s.WQ[2] * x355 + r1 * x346;
-      const TB x377 = s.QC[1] * x44 + s.WQ[1] * x50;
-      const TB x378 = s.QC[1] * x50 + s.WQ[1] * x56;
-      const TB x379 = s.QC[1] * x56 + s.WQ[1] * x62;
-      const TB x381 = s.QC[1] * x85 + s.WQ[1] * x95;
-      const TB x382 = s.QC[1] * x95 + s.WQ[1] * x105;
-      const TB x383 = s.QC[1] * x105 + s.WQ[1] * x115;
-      const TB x427 = s.QC[1] * x181 + s.WQ[1] * x196 + r1 * x115;
-      const TB x562 = s.QC[2] * x181 + s.WQ[2] * x196;
-      const TB x416 = s.QC[1] * x84 + s.WQ[1] * x94;
-      const TB x417 = s.QC[1] * x94 + s.WQ[1] * x104;
-      const TB x418 = s.QC[1] * x104 + s.WQ[1] * x114;
-      const TB x419 = s.QC[1] * x114 + s.WQ[1] * x124;
-      const TB x554 = s.QC[2] * x114 + s.WQ[2] * x124 + r3 * x62;
-      const TB x514 = s.QC[2] * x56 + s.WQ[2] * x62 + r2 * x23;
-      const TB x490 = s.QC[2] * x20 + s.WQ[2] * x23 + r1 * x4;
-      const TB x481 = s.QC[2] * x3 + s.WQ[2] * x4;
-      const TB x489 = s.QC[2] * x17 + s.WQ[2] * x20 + r1 * x3;
-      const TB x1121 = s.QC[2] * x489 + s.WQ[2] * x490 + e1 * (x17 - s.roe * x20) + r1 * x481;
-      const TB x420 = s.QC[1] * x135 + s.WQ[1] * x150;
-      const TB x421 = s.QC[1] * x150 + s.WQ[1] * x165;
-      const TB x422 = s.QC[1] * x165 + s.WQ[1] * x180;
-      const TB x423 = s.QC[1] * x180 + s.WQ[1] * x195;
-      const TB x558 = s.QC[2] * x180 + s.WQ[2] * x195;
-      const TB x424 = s.QC[1] * x136 + s.WQ[1] * x151 + r1 * x85;
-      const TB x941 = s.QC[1] * x381 + s.WQ[1] * x382 + e1 * (x85 - s.roe * x95);
-      const TB x425 = s.QC[1] * x151 + s.WQ[1] * x166 + r1 * x95;
-      const TB x942 = s.QC[1] * x382 + s.WQ[1] * x383 + e1 * (x95 - s.roe * x105);
-      const TB x426 = s.QC[1] * x166 + s.WQ[1] * x181 + r1 * x105;
-      const TB x975 = s.QC[1] * x426 + s.WQ[1] * x427 + e1 * (x166 - s.roe * x181) + r1 * x383;
-      const TB x1075 = s.QC[2] * x426 + s.WQ[2] * x427;
-      const TB x512 = s.QC[2] * x44 + s.WQ[2] * x50 + r2 * x17;
-      const TB x513 = s.QC[2] * x50 + s.WQ[2] * x56 + r2 * x20;
-      const TB x1138 = s.QC[2] * x512 + s.WQ[2] * x513 + e1 * (x44 - s.roe * x50) + r2 * x489;
-      const TB x1139 = s.QC[2] * x513 + s.WQ[2] * x514 + e1 * (x50 - s.roe * x56) + r2 * x490;
-      const TB x1814 = s.QC[2] * x1138 + s.WQ[2] * x1139 + e2 * (x512 - s.roe * x513) + r2 * x1121;
-      const TB x551 = s.QC[2] * x84 + s.WQ[2] * x94 + r3 * x44;
-      const TB x552 = s.QC[2] * x94 + s.WQ[2] * x104 + r3 * x50;
-      const TB x1167 = s.QC[2] * x551 + s.WQ[2] * x552 + e1 * (x84 - s.roe * x94) + r3 * x512;
-      const TB x553 = s.QC[2] * x104 + s.WQ[2] * x114 + r3 * x56;
-      const TB x1169 = s.QC[2] * x553 + s.WQ[2] * x554 + e1 * (x104 - s.roe * x114) + r3 * x514;
-      const TB x1168 = s.QC[2] * x552 + s.WQ[2] * x553 + e1 * (x94 - s.roe * x104) + r3 * x513;
-      const TB x1834 = s.QC[2] * x1168 + s.WQ[2] * x1169 + e2 * (x552 - s.roe * x553) + r3 * x1139;
-      const TB x1833 = s.QC[2] * x1167 + s.WQ[2] * x1168 + e2 * (x551 - s.roe * x552) + r3 * x1138;
-      acc480 += static_cast<double>(x1833);
-      const TB x2314 = s.QC[2] * x1833 + s.WQ[2] * x1834 + e3 * (x1167 - s.roe * x1168) + r3 * x1814;
-      acc945 += static_cast<double>(x2314);
-      const TB x555 = s.QC[2] * x135 + s.WQ[2] * x150;
-      const TB x556 = s.QC[2] * x150 + s.WQ[2] * x165;
-      const TB x557 = s.QC[2] * x165 + s.WQ[2] * x180;
-      const TB x1172 = s.QC[2] * x557 + s.WQ[2] * x558 + e1 * (x165 - s.roe * x180);
-      const TB x559 = s.QC[2] * x136 + s.WQ[2] * x151;
-      const TB x560 = s.QC[2] * x151 + s.WQ[2] * x166;
-      const TB x561 = s.QC[2] * x166 + s.WQ[2] * x181;
-      const TB x1175 = s.QC[2] * x561 + s.WQ[2] * x562 + e1 * (x166 - s.roe * x181);
-      const TB x621 = s.QC[0] * x219 + s.WQ[0] * x220 + e1 * (x17 - s.roe * x20);
-      const TB x921 = s.QC[1] * x354 + s.WQ[1] * x355 + e1 * (x17 - s.roe * x20);
-      const TB x1038 = s.QC[2] * x377 + s.WQ[2] * x378 + r2 * x354;
-      const TB x1039 = s.QC[2] * x378 + s.WQ[2] * x379 + r2 * x355;
-      const TB x1749 = s.QC[2] * x1038 + s.WQ[2] * x1039 + e1 * (x377 - s.roe * x378) + r2 * x1021;
-      const TB x638 = s.QC[0] * x242 + s.WQ[0] * x243 + e1 * (x44 - s.roe * x50);
-      const TB x938 = s.QC[1] * x377 + s.WQ[1] * x378 + e1 * (x44 - s.roe * x50);
-      const TB x639 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x50 - s.roe * x56);
-      const TB x939 = s.QC[1] * x378 + s.WQ[1] * x379 + e1 * (x50 - s.roe * x56);
-      const TB x1069 = s.QC[2] * x418 + s.WQ[2] * x419 + r3 * x379;
-      const TB x969 = s.QC[1] * x418 + s.WQ[1] * x419 + e1 * (x104 - s.roe * x114);
-      const TB x669 = s.QC[0] * x283 + s.WQ[0] * x284 + e1 * (x104 - s.roe * x114);
-      const TB x1359 = s.QC[2] * x638 + s.WQ[2] * x639 + r2 * x621;
-      const TB x1684 = s.QC[2] * x938 + s.WQ[2] * x939 + r2 * x921;
-      const TB x667 = s.QC[0] * x281 + s.WQ[0] * x282 + e1 * (x84 - s.roe * x94);
-      const TB x967 = s.QC[1] * x416 + s.WQ[1] * x417 + e1 * (x84 - s.roe * x94);
-      const TB x668 = s.QC[0] * x282 + s.WQ[0] * x283 + e1 * (x94 - s.roe * x104);
-      const TB x968 = s.QC[1] * x417 + s.WQ[1] * x418 + e1 * (x94 - s.roe * x104);
-      const TB x670 = s.QC[0] * x285 + s.WQ[0] * x286 + e1 * (x135 - s.roe * x150) + r4 * x246;
-      acc16 += static_cast<double>(x670);
-      const TB x671 = s.QC[0] * x286 + s.WQ[0] * x287 + e1 * (x150 - s.roe * x165) + r4 * x247;
-      const TB x672 = s.QC[0] * x287 + s.WQ[0] * x288 + e1 * (x165 - s.roe * x180) + r4 * x248;
-      const TB x972 = s.QC[1] * x422 + s.WQ[1] * x423 + e1 * (x165 - s.roe * x180);
-      const TB x1072 = s.QC[2] * x422 + s.WQ[2] * x423;
-      const TB x721 = s.QC[1] * x219 + s.WQ[1] * x220;
-      const TB x838 = s.QC[2] * x242 + s.WQ[2] * x243 + r2 * x219;
-      const TB x839 = s.QC[2] * x243 + s.WQ[2] * x244 + r2 * x220;
-      const TB x1554 = s.QC[2] * x838 + s.WQ[2] * x839 + e1 * (x242 - s.roe * x243) + r2 * x821;
-      const TB x738 = s.QC[1] * x242 + s.WQ[1] * x243;
-      const TB x739 = s.QC[1] * x243 + s.WQ[1] * x244;
-      const TB x869 = s.QC[2] * x283 + s.WQ[2] * x284 + r3 * x244;
-      const TB x769 = s.QC[1] * x283 + s.WQ[1] * x284;
-      const TB x1489 = s.QC[2] * x738 + s.WQ[2] * x739 + r2 * x721;
-      const TB x741 = s.QC[1] * x246 + s.WQ[1] * x247;
-      const TB x742 = s.QC[1] * x247 + s.WQ[1] * x248;
-      const TB x775 = s.QC[1] * x291 + s.WQ[1] * x292 + r1 * x248;
-      const TB x875 = s.QC[2] * x291 + s.WQ[2] * x292;
-      const TB x767 = s.QC[1] * x281 + s.WQ[1] * x282;
-      const TB x768 = s.QC[1] * x282 + s.WQ[1] * x283;
-      const TB x770 = s.QC[1] * x285 + s.WQ[1] * x286;
-      acc47 += static_cast<double>(x770);
-      const TB x771 = s.QC[1] * x286 + s.WQ[1] * x287;
-      const TB x772 = s.QC[1] * x287 + s.WQ[1] * x288;
-      const TB x872 = s.QC[2] * x287 + s.WQ[2] * x288;
-      const TB x773 = s.QC[1] * x289 + s.WQ[1] * x290 + r1 * x246;
-      acc48 += static_cast<double>(x773);
-      const TB x1426 = s.QC[1] * x741 + s.WQ[1] * x742 + e1 * (x246 - s.roe * x247);
-      const TB x774 = s.QC[1] * x290 + s.WQ[1] * x291 + r1 * x247;
-      const TB x1447 = s.QC[1] * x773 + s.WQ[1] * x774 + e1 * (x289 - s.roe * x290) + r1 * x741;
-      acc296 += static_cast<double>(x1447);
-      const TB x1448 = s.QC[1] * x774 + s.WQ[1] * x775 + e1 * (x290 - s.roe * x291) + r1 * x742;
-      const TB x2068 = s.QC[1] * x1447 + s.WQ[1] * x1448 + e2 * (x773 - s.roe * x774) + r1 * x1426;
-      acc699 += static_cast<double>(x2068);
-      const TB x2099 = s.QC[2] * x1447 + s.WQ[2] * x1448;
-      acc730 += static_cast<double>(x2099);
-      const TB x1513 = s.QC[2] * x774 + s.WQ[2] * x775;
-      const TB x867 = s.QC[2] * x281 + s.WQ[2] * x282 + r3 * x242;
-      const TB x868 = s.QC[2] * x282 + s.WQ[2] * x283 + r3 * x243;
-      const TB x1574 = s.QC[2] * x868 + s.WQ[2] * x869 + e1 * (x282 - s.roe * x283) + r3 * x839;
-      const TB x1573 = s.QC[2] * x867 + s.WQ[2] * x868 + e1 * (x281 - s.roe * x282) + r3 * x838;
-      const TB x2159 = s.QC[2] * x1573 + s.WQ[2] * x1574 + e2 * (x867 - s.roe * x868) + r3 * x1554;
-      acc790 += static_cast<double>(x2159);
-      const TB x870 = s.QC[2] * x285 + s.WQ[2] * x286;
-      acc78 += static_cast<double>(x870);
-      const TB x871 = s.QC[2] * x286 + s.WQ[2] * x287;
-      const TB x1576 = s.QC[2] * x871 + s.WQ[2] * x872 + e1 * (x286 - s.roe * x287);
-      const TB x873 = s.QC[2] * x289 + s.WQ[2] * x290;
-      acc79 += static_cast<double>(x873);
-      const TB x874 = s.QC[2] * x290 + s.WQ[2] * x291;
-      const TB x1578 = s.QC[2] * x874 + s.WQ[2] * x875 + e1 * (x290 - s.roe * x291);
-      const TB x1577 = s.QC[2] * x873 + s.WQ[2] * x874 + e1 * (x289 - s.roe * x290);
-      acc358 += static_cast<double>(x1577);
-      const TB x2161 = s.QC[2] * x1577 + s.WQ[2] * x1578 + e2 * (x873 - s.roe * x874);
-      acc792 += static_cast<double>(x2161);
-      const TB x970 = s.QC[1] * x420 + s.WQ[1] * x421 + e1 * (x135 - s.roe * x150);
-      acc109 += static_cast<double>(x970);
-      const TB x1170 = s.QC[2] * x555 + s.WQ[2] * x556 + e1 * (x135 - s.roe * x150);
-      acc171 += static_cast<double>(x1170);
-      const TB x971 = s.QC[1] * x421 + s.WQ[1] * x422 + e1 * (x150 - s.roe * x165);
-      const TB x1171 = s.QC[2] * x556 + s.WQ[2] * x557 + e1 * (x150 - s.roe * x165);
-      const TB x1836 = s.QC[2] * x1171 + s.WQ[2] * x1172 + e2 * (x556 - s.roe * x557);
-      const TB x1835 = s.QC[2] * x1170 + s.WQ[2] * x1171 + e2 * (x555 - s.roe * x556);
-      acc481 += static_cast<double>(x1835);
-      const TB x2315 = s.QC[2] * x1835 + s.WQ[2] * x1836 + e3 * (x1170 - s.roe * x1171);
-      acc946 += static_cast<double>(x2315);
-      const TB x973 = s.QC[1] * x424 + s.WQ[1] * x425 + e1 * (x136 - s.roe * x151) + r1 * x381;
-      acc110 += static_cast<double>(x973);
-      const TB x1173 = s.QC[2] * x559 + s.WQ[2] * x560 + e1 * (x136 - s.roe * x151);
-      acc172 += static_cast<double>(x1173);
-      const TB x1621 = s.QC[1] * x941 + s.WQ[1] * x942 + e2 * (x381 - s.roe * x382);
-      const TB x974 = s.QC[1] * x425 + s.WQ[1] * x426 + e1 * (x151 - s.roe * x166) + r1 * x382;
-      const TB x1174 = s.QC[2] * x560 + s.WQ[2] * x561 + e1 * (x151 - s.roe * x166);
-      const TB x1838 = s.QC[2] * x1174 + s.WQ[2] * x1175 + e2 * (x560 - s.roe * x561);
-      const TB x1837 = s.QC[2] * x1173 + s.WQ[2] * x1174 + e2 * (x559 - s.roe * x560);
-      acc482 += static_cast<double>(x1837);
-      const TB x2316 = s.QC[2] * x1837 + s.WQ[2] * x1838 + e3 * (x1173 - s.roe * x1174);
-      acc947 += static_cast<double>(x2316);
-      const TB x1642 = s.QC[1] * x973 + s.WQ[1] * x974 + e2 * (x424 - s.roe * x425) + r1 * x941;
-      acc389 += static_cast<double>(x1642);
-      const TB x1643 = s.QC[1] * x974 + s.WQ[1] * x975 + e2 * (x425 - s.roe * x426) + r1 * x942;
-      const TB x2192 = s.QC[1] * x1642 + s.WQ[1] * x1643 + e3 * (x973 - s.roe * x974) + r1 * x1621;
-      acc823 += static_cast<double>(x2192);
-      const TB x2223 = s.QC[2] * x1642 + s.WQ[2] * x1643;
-      acc854 += static_cast<double>(x2223);
-      const TB x1708 = s.QC[2] * x974 + s.WQ[2] * x975;
-      const TB x1067 = s.QC[2] * x416 + s.WQ[2] * x417 + r3 * x377;
-      const TB x1619 = s.QC[1] * x938 + s.WQ[1] * x939 + e2 * (x377 - s.roe * x378);
-      const TB x1068 = s.QC[2] * x417 + s.WQ[2] * x418 + r3 * x378;
-      const TB x1769 = s.QC[2] * x1068 + s.WQ[2] * x1069 + e1 * (x417 - s.roe * x418) + r3 * x1039;
-      const TB x1639 = s.QC[1] * x968 + s.WQ[1] * x969 + e2 * (x417 - s.roe * x418);
-      const TB x1704 = s.QC[2] * x968 + s.WQ[2] * x969 + r3 * x939;
-      const TB x1703 = s.QC[2] * x967 + s.WQ[2] * x968 + r3 * x938;
-      const TB x2252 = s.QC[2] * x1703 + s.WQ[2] * x1704 + e1 * (x967 - s.roe * x968) + r3 * x1684;
-      acc883 += static_cast<double>(x2252);
-      const TB x1768 = s.QC[2] * x1067 + s.WQ[2] * x1068 + e1 * (x416 - s.roe * x417) + r3 * x1038;
-      const TB x2283 = s.QC[2] * x1768 + s.WQ[2] * x1769 + e2 * (x1067 - s.roe * x1068) + r3 * x1749;
-      acc914 += static_cast<double>(x2283);
-      const TB x1638 = s.QC[1] * x967 + s.WQ[1] * x968 + e2 * (x416 - s.roe * x417);
-      const TB x2190 = s.QC[1] * x1638 + s.WQ[1] * x1639 + e3 * (x967 - s.roe * x968);
-      acc821 += static_cast<double>(x2190);
-      const TB x2221 = s.QC[2] * x1638 + s.WQ[2] * x1639 + r3 * x1619;
-      acc852 += static_cast<double>(x2221);
-      const TB x1070 = s.QC[2] * x420 + s.WQ[2] * x421;
-      acc140 += static_cast<double>(x1070);
-      const TB x1071 = s.QC[2] * x421 + s.WQ[2] * x422;
-      const TB x1771 = s.QC[2] * x1071 + s.WQ[2] * x1072 + e1 * (x421 - s.roe * x422);
-      const TB x1641 = s.QC[1] * x971 + s.WQ[1] * x972 + e2 * (x421 - s.roe * x422);
-      const TB x1706 = s.QC[2] * x971 + s.WQ[2] * x972;
-      const TB x1073 = s.QC[2] * x424 + s.WQ[2] * x425;
-      acc141 += static_cast<double>(x1073);
-      const TB x1074 = s.QC[2] * x425 + s.WQ[2] * x426;
-      const TB x1773 = s.QC[2] * x1074 + s.WQ[2] * x1075 + e1 * (x425 - s.roe * x426);
-      const TB x1772 = s.QC[2] * x1073 + s.WQ[2] * x1074 + e1 * (x424 - s.roe * x425);
-      acc451 += static_cast<double>(x1772);
-      const TB x2285 = s.QC[2] * x1772 + s.WQ[2] * x1773 + e2 * (x1073 - s.roe * x1074);
-      acc916 += static_cast<double>(x2285);
-      const TB x1229 = s.QC[0] * x638 + s.WQ[0] * x639 + e2 * (x242 - s.roe * x243);
-      const TB x1424 = s.QC[1] * x738 + s.WQ[1] * x739 + e1 * (x242 - s.roe * x243);
-      const TB x1508 = s.QC[2] * x767 + s.WQ[2] * x768 + r3 * x738;
-      const TB x1509 = s.QC[2] * x768 + s.WQ[2] * x769 + r3 * x739;
-      const TB x2128 = s.QC[2] * x1508 + s.WQ[2] * x1509 + e1 * (x767 - s.roe * x768) + r3 * x1489;
-      acc759 += static_cast<double>(x2128);
-      const TB x1444 = s.QC[1] * x768 + s.WQ[1] * x769 + e1 * (x282 - s.roe * x283);
-      const TB x1249 = s.QC[0] * x668 + s.WQ[0] * x669 + e2 * (x282 - s.roe * x283);
-      const TB x1248 = s.QC[0] * x667 + s.WQ[0] * x668 + e2 * (x281 - s.roe * x282);
-      const TB x1443 = s.QC[1] * x767 + s.WQ[1] * x768 + e1 * (x281 - s.roe * x282);
-      const TB x2066 = s.QC[1] * x1443 + s.WQ[1] * x1444 + e2 * (x767 - s.roe * x768);
-      acc697 += static_cast<double>(x2066);
-      const TB x2097 = s.QC[2] * x1443 + s.WQ[2] * x1444 + r3 * x1424;
-      acc728 += static_cast<double>(x2097);
-      const TB x1942 = s.QC[2] * x1248 + s.WQ[2] * x1249 + r3 * x1229;
-      acc573 += static_cast<double>(x1942);
-      const TB x1880 = s.QC[0] * x1248 + s.WQ[0] * x1249 + e3 * (x667 - s.roe * x668);
-      acc511 += static_cast<double>(x1880);
-      const TB x1911 = s.QC[1] * x1248 + s.WQ[1] * x1249;
-      acc542 += static_cast<double>(x1911);
-      const TB x1250 = s.QC[0] * x670 + s.WQ[0] * x671 + e2 * (x285 - s.roe * x286) + r4 * x641;
-      acc202 += static_cast<double>(x1250);
-      const TB x1251 = s.QC[0] * x671 + s.WQ[0] * x672 + e2 * (x286 - s.roe * x287) + r4 * x642;
-      const TB x1881 = s.QC[0] * x1250 + s.WQ[0] * x1251 + e3 * (x670 - s.roe * x671) + r4 * x1231;
-      acc512 += static_cast<double>(x1881);
-      const TB x1446 = s.QC[1] * x771 + s.WQ[1] * x772 + e1 * (x286 - s.roe * x287);
-      const TB x1511 = s.QC[2] * x771 + s.WQ[2] * x772;
-      const TB x1912 = s.QC[1] * x1250 + s.WQ[1] * x1251;
-      acc543 += static_cast<double>(x1912);
-      const TB x1943 = s.QC[2] * x1250 + s.WQ[2] * x1251;
-      acc574 += static_cast<double>(x1943);
-      const TB x1294 = s.QC[1] * x638 + s.WQ[1] * x639;
-      const TB x1378 = s.QC[2] * x667 + s.WQ[2] * x668 + r3 * x638;
-      const TB x1379 = s.QC[2] * x668 + s.WQ[2] * x669 + r3 * x639;
-      const TB x2035 = s.QC[2] * x1378 + s.WQ[2] * x1379 + e1 * (x667 - s.roe * x668) + r3 * x1359;
-      acc666 += static_cast<double>(x2035);
-      const TB x1314 = s.QC[1] * x668 + s.WQ[1] * x669;
-      const TB x1296 = s.QC[1] * x641 + s.WQ[1] * x642;
-      const TB x1317 = s.QC[1] * x673 + s.WQ[1] * x674 + r1 * x641;
-      acc234 += static_cast<double>(x1317);
-      const TB x1318 = s.QC[1] * x674 + s.WQ[1] * x675 + r1 * x642;
-      const TB x1975 = s.QC[1] * x1317 + s.WQ[1] * x1318 + e1 * (x673 - s.roe * x674) + r1 * x1296;
-      acc606 += static_cast<double>(x1975);
-      const TB x2006 = s.QC[2] * x1317 + s.WQ[2] * x1318;
-      acc637 += static_cast<double>(x2006);
-      const TB x1383 = s.QC[2] * x674 + s.WQ[2] * x675;
-      const TB x1313 = s.QC[1] * x667 + s.WQ[1] * x668;
-      const TB x1973 = s.QC[1] * x1313 + s.WQ[1] * x1314 + e1 * (x667 - s.roe * x668);
-      acc604 += static_cast<double>(x1973);
-      const TB x2004 = s.QC[2] * x1313 + s.WQ[2] * x1314 + r3 * x1294;
-      acc635 += static_cast<double>(x2004);
-      const TB x1315 = s.QC[1] * x670 + s.WQ[1] * x671;
-      acc233 += static_cast<double>(x1315);
-      const TB x1316 = s.QC[1] * x671 + s.WQ[1] * x672;
-      const TB x1381 = s.QC[2] * x671 + s.WQ[2] * x672;
-      const TB x1974 = s.QC[1] * x1315 + s.WQ[1] * x1316 + e1 * (x670 - s.roe * x671);
-      acc605 += static_cast<double>(x1974);
-      const TB x2005 = s.QC[2] * x1315 + s.WQ[2] * x1316;
-      acc636 += static_cast<double>(x2005);
-      const TB x1380 = s.QC[2] * x670 + s.WQ[2] * x671;
-      acc264 += static_cast<double>(x1380);
-      const TB x2036 = s.QC[2] * x1380 + s.WQ[2] * x1381 + e1 * (x670 - s.roe * x671);
-      acc667 += static_cast<double>(x2036);
-      const TB x1382 = s.QC[2] * x673 + s.WQ[2] * x674;
-      acc265 += static_cast<double>(x1382);
-      const TB x2037 = s.QC[2] * x1382 + s.WQ[2] * x1383 + e1 * (x673 - s.roe * x674);
-      acc668 += static_cast<double>(x2037);
-      const TB x1445 = s.QC[1] * x770 + s.WQ[1] * x771 + e1 * (x285 - s.roe * x286);
-      acc295 += static_cast<double>(x1445);
-      const TB x1575 = s.QC[2] * x870 + s.WQ[2] * x871 + e1 * (x285 - s.roe * x286);
-      acc357 += static_cast<double>(x1575);
-      const TB x2160 = s.QC[2] * x1575 + s.WQ[2] * x1576 + e2 * (x870 - s.roe * x871);
-      acc791 += static_cast<double>(x2160);
-      const TB x2067 = s.QC[1] * x1445 + s.WQ[1] * x1446 + e2 * (x770 - s.roe * x771);
-      acc698 += static_cast<double>(x2067);
-      const TB x2098 = s.QC[2] * x1445 + s.WQ[2] * x1446;
-      acc729 += static_cast<double>(x2098);
-      const TB x1510 = s.QC[2] * x770 + s.WQ[2] * x771;
-      acc326 += static_cast<double>(x1510);
-      const TB x2129 = s.QC[2] * x1510 + s.WQ[2] * x1511 + e1 * (x770 - s.roe * x771);
-      acc760 += static_cast<double>(x2129);
-      const TB x1512 = s.QC[2] * x773 + s.WQ[2] * x774;
-      acc327 += static_cast<double>(x1512);
-      const TB x2130 = s.QC[2] * x1512 + s.WQ[2] * x1513 + e1 * (x773 - s.roe * x774);
-      acc761 += static_cast<double>(x2130);
-      const TB x1640 = s.QC[1] * x970 + s.WQ[1] * x971 + e2 * (x420 - s.roe * x421);
-      acc388 += static_cast<double>(x1640);
-      const TB x1770 = s.QC[2] * x1070 + s.WQ[2] * x1071 + e1 * (x420 - s.roe * x421);
-      acc450 += static_cast<double>(x1770);
-      const TB x2284 = s.QC[2] * x1770 + s.WQ[2] * x1771 + e2 * (x1070 - s.roe * x1071);
-      acc915 += static_cast<double>(x2284);
-      const TB x2191 = s.QC[1] * x1640 + s.WQ[1] * x1641 + e3 * (x970 - s.roe * x971);
-      acc822 += static_cast<double>(x2191);
-      const TB x2222 = s.QC[2] * x1640 + s.WQ[2] * x1641;
-      acc853 += static_cast<double>(x2222);
-      const TB x1705 = s.QC[2] * x970 + s.WQ[2] * x971;
-      acc419 += static_cast<double>(x1705);
-      const TB x2253 = s.QC[2] * x1705 + s.WQ[2] * x1706 + e1 * (x970 - s.roe * x971);
-      acc884 += static_cast<double>(x2253);
-      const TB x1707 = s.QC[2] * x973 + s.WQ[2] * x974;
-      acc420 += static_cast<double>(x1707);
-      const TB x2254 = s.QC[2] * x1707 + s.WQ[2] * x1708 + e1 * (x973 - s.roe * x974);
-      acc885 += static_cast<double>(x2254);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(480 * R + r) * a.acc_stride + q] = acc480;
-  a.acc[(511 * R + r) * a.acc_stride + q] = acc511;
-  a.acc[(542 * R + r) * a.acc_stride + q] = acc542;
-  a.acc[(573 * R + r) * a.acc_stride + q] = acc573;
-  a.acc[(604 * R + r) * a.acc_stride + q] = acc604;
-  a.acc[(635 * R + r) * a.acc_stride + q] = acc635;
-  a.acc[(666 * R + r) * a.acc_stride + q] = acc666;
-  a.acc[(697 * R + r) * a.acc_stride + q] = acc697;
-  a.acc[(728 * R + r) * a.acc_stride + q] = acc728;
-  a.acc[(759 * R + r) * a.acc_stride + q] = acc759;
-  a.acc[(790 * R + r) * a.acc_stride + q] = acc790;
-  a.acc[(821 * R + r) * a.acc_stride + q] = acc821;
-  a.acc[(852 * R + r) * a.acc_stride + q] = acc852;
-  a.acc[(883 * R + r) * a.acc_stride + q] = acc883;
-  a.acc[(914 * R + r) * a.acc_stride + q] = acc914;
-  a.acc[(945 * R + r) * a.acc_stride + q] = acc945;
-  a.acc[(16 * R + r) * a.acc_stride + q] = acc16;
-  a.acc[(47 * R + r) * a.acc_stride + q] = acc47;
-  a.acc[(78 * R + r) * a.acc_stride + q] = acc78;
-  a.acc[(109 * R + r) * a.acc_stride + q] = acc109;
-  a.acc[(140 * R + r) * a.acc_stride + q] = acc140;
-  a.acc[(171 * R + r) * a.acc_stride + q] = acc171;
-  a.acc[(202 * R + r) * a.acc_stride + q] = acc202;
-  a.acc[(233 * R + r) * a.acc_stride + q] = acc233;
-  a.acc[(264 * R + r) * a.acc_stride + q] = acc264;
-  a.acc[(295 * R + r) * a.acc_stride + q] = acc295;
-  a.acc[(326 * R + r) * a.acc_stride + q] = acc326;
-  a.acc[(357 * R + r) * a.acc_stride + q] = acc357;
-  a.acc[(388 * R + r) * a.acc_stride + q] = acc388;
-  a.acc[(419 * R + r) * a.acc_stride + q] = acc419;
-  a.acc[(450 * R + r) * a.acc_stride + q] = acc450;
-  a.acc[(481 * R + r) * a.acc_stride + q] = acc481;
-  a.acc[(512 * R + r) * a.acc_stride + q] = acc512;
-  a.acc[(543 * R + r) * a.acc_stride + q] = acc543;
-  a.acc[(574 * R + r) * a.acc_stride + q] = acc574;
-  a.acc[(605 * R + r) * a.acc_stride + q] = acc605;
-  a.acc[(636 * R + r) * a.acc_stride + q] = acc636;
-  a.acc[(667 * R + r) * a.acc_stride + q] = acc667;
-  a.acc[(698 * R + r) * a.acc_stride + q] = acc698;
-  a.acc[(729 * R + r) * a.acc_stride + q] = acc729;
-  a.acc[(760 * R + r) * a.acc_stride + q] = acc760;
-  a.acc[(791 * R + r) * a.acc_stride + q] = acc791;
-  a.acc[(822 * R + r) * a.acc_stride + q] = acc822;
-  a.acc[(853 * R + r) * a.acc_stride + q] = acc853;
-  a.acc[(884 * R + r) * a.acc_stride + q] = acc884;
-  a.acc[(915 * R + r) * a.acc_stride + q] = acc915;
-  a.acc[(946 * R + r) * a.acc_stride + q] = acc946;
-  a.acc[(17 * R + r) * a.acc_stride + q] = acc17;
-  a.acc[(48 * R + r) * a.acc_stride + q] = acc48;
-  a.acc[(79 * R + r) * a.acc_stride + q] = acc79;
-  a.acc[(110 * R + r) * a.acc_stride + q] = acc110;
-  a.acc[(141 * R + r) * a.acc_stride + q] = acc141;
-  a.acc[(172 * R + r) * a.acc_stride + q] = acc172;
-  a.acc[(203 * R + r) * a.acc_stride + q] = acc203;
-  a.acc[(234 * R + r) * a.acc_stride + q] = acc234;
-  a.acc[(265 * R + r) * a.acc_stride + q] = acc265;
-  a.acc[(296 * R + r) * a.acc_stride + q] = acc296;
-  a.acc[(327 * R + r) * a.acc_stride + q] = acc327;
-  a.acc[(358 * R + r) * a.acc_stride + q] = acc358;
-  a.acc[(389 * R + r) * a.acc_stride + q] = acc389;
-  a.acc[(420 * R + r) * a.acc_stride + q] = acc420;
-  a.acc[(451 * R + r) * a.acc_stride + q] = acc451;
-  a.acc[(482 * R + r) * a.acc_stride + q] = acc482;
-  a.acc[(513 * R + r) * a.acc_stride + q] = acc513;
-  a.acc[(544 * R + r) * a.acc_stride + q] = acc544;
-  a.acc[(575 * R + r) * a.acc_stride + q] = acc575;
-  a.acc[(606 * R + r) * a.acc_stride + q] = acc606;
-  a.acc[(637 * R + r) * a.acc_stride + q] = acc637;
-  a.acc[(668 * R + r) * a.acc_stride + q] = acc668;
-  a.acc[(699 * R + r) * a.acc_stride + q] = acc699;
-  a.acc[(730 * R + r) * a.acc_stride + q] = acc730;
-  a.acc[(761 * R + r) * a.acc_stride + q] = acc761;
-  a.acc[(792 * R + r) * a.acc_stride + q] = acc792;
-  a.acc[(823 * R + r) * a.acc_stride + q] = acc823;
-  a.acc[(854 * R + r) * a.acc_stride + q] = acc854;
-  a.acc[(885 * R + r) * a.acc_stride + q] = acc885;
-  a.acc[(916 * R + r) * a.acc_stride + q] = acc916;
-  a.acc[(947 * R + r) * a.acc_stride + q] = acc947;
-  a.acc[(18 * R + r) * a.acc_stride + q] = acc18;
-  a.acc[(49 * R + r) * a.acc_stride + q] = acc49;
-}
-// class (22|22) split 7/12: 340 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_7(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc80 = 0.0;
-  double acc111 = 0.0;
-  double acc142 = 0.0;
-  double acc173 = 0.0;
-  double acc204 = 0.0;
-  double acc235 = 0.0;
-  double acc266 = 0.0;
-  double acc297 = 0.0;
-  double acc328 = 0.0;
-  double acc359 = 0.0;
-  double acc390 = 0.0;
-  double acc421 = 0.0;
-  double acc452 = 0.0;
-  double acc483 = 0.0;
-  double acc514 = 0.0;
-  double acc545 = 0.0;
-  double acc576 = 0.0;
-  double acc607 = 0.0;
-  double acc638 = 0.0;
-  double acc669 = 0.0;
-  double acc700 = 0.0;
-  double acc731 = 0.0;
-  double acc762 = 0.0;
-  double acc793 = 0.0;
-  double acc824 = 0.0;
-  double acc855 = 0.0;
-  double acc886 = 0.0;
-  double acc917 = 0.0;
-  double acc948 = 0.0;
-  double acc19 = 0.0;
-  double acc50 = 0.0;
-  double acc81 = 0.0;
-  double acc112 = 0.0;
-  double acc143 = 0.0;
-  double acc174 = 0.0;
-  double acc205 = 0.0;
-  double acc236 = 0.0;
-  double acc267 = 0.0;
-  double acc298 = 0.0;
-  double acc329 = 0.0;
-  double acc360 = 0.0;
-  double acc391 = 0.0;
-  double acc422 = 0.0;
-  double acc453 = 0.0;
-  double acc484 = 0.0;
-  double acc515 = 0.0;
-  double acc546 = 0.0;
-  double acc577 = 0.0;
-  double acc608 = 0.0;
-  double acc639 = 0.0;
-  double acc670 = 0.0;
-  double acc701 = 0.0;
-  double acc732 = 0.0;
-  double acc763 = 0.0;
-  double acc794 = 0.0;
-  double acc825 = 0.0;
-  double acc856 = 0.0;
-  double acc887 = 0.0;
-  double acc918 = 0.0;
-  double acc949 = 0.0;
-  double acc20 = 0.0;
-  double acc51 = 0.0;
-  double acc82 = 0.0;
-  double acc113 = 0.0;
-  double acc144 = 0.0;
-  double acc175 = 0.0;
-  double acc206 = 0.0;
-  double acc237 = 0.0;
-  double acc268 = 0.0;
-  double acc299 = 0.0;
-  double acc330 = 0.0;
-  double acc361 = 0.0;
-  double acc392 = 0.0;
-  double acc423 = 0.0;
-  double acc454 = 0.0;
-  double acc485 = 0.0;
-  double acc516 = 0.0;
-  double acc547 = 0.0;
-  double acc578 = 0.0;
-  double acc609 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB x8 = static_cast<TB>(F[8] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x30 = s.PA[0] * x7 + s.WP[0] * x8;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x33 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x39 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x75 = s.PA[0] * x33 + s.WP[0] * x39 + z2 * (x9 - s.roz * x12);
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x69 = s.PA[0] * x27 + s.WP[0] * x30 + z1 * (x6 - s.roz * x7);
-      const TB x63 = s.PA[0] * x24 + s.WP[0] * x27 + z1 * (x5 - s.roz * x6);
-      const TB x40 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x45 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x51 = s.PA[0] * x18 + s.WP[0] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x57 = s.PA[0] * x21 + s.WP[0] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x64 = s.PA[1] * x24 + s.WP[1] * x27;
-      const TB x125 = s.PA[0] * x63 + s.WP[0] * x69 + z2 * (x24 - s.roz * x27);
-      const TB x126 = s.PA[1] * x63 + s.WP[1] * x69;
-      const TB x76 = s.PA[1] * x33 + s.WP[1] * x39;
-      const TB x85 = s.PA[0] * x39 + s.WP[0] * x45 + z2 * (x12 - s.roz * x15);
-      const TB x88 = s.PA[1] * x40 + s.WP[1] * x46 + z1 * (x12 - s.roz * x15);
-      const TB x89 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x137 = s.PA[2] * x75 + s.WP[2] * x85;
-      const TB x86 = s.PA[1] * x39 + s.WP[1] * x45;
-      const TB x138 = s.PA[1] * x76 + s.WP[1] * x86 + z1 * (x33 - s.roz * x39);
-      const TB x139 = s.PA[2] * x76 + s.WP[2] * x86;
-      const TB x87 = s.PA[2] * x39 + s.WP[2] * x45;
-      const TB x95 = s.PA[0] * x45 + s.WP[0] * x51 + z2 * (x15 - s.roz * x18);
-      const TB x98 = s.PA[1] * x46 + s.WP[1] * x52 + z1 * (x15 - s.roz * x18);
-      const TB x96 = s.PA[1] * x45 + s.WP[1] * x51;
-      const TB x153 = s.PA[1] * x86 + s.WP[1] * x96 + z1 * (x39 - s.roz * x45);
-      const TB x97 = s.PA[2] * x45 + s.WP[2] * x51;
-      const TB x99 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x105 = s.PA[0] * x51 + s.WP[0] * x57 + z2 * (x18 - s.roz * x21);
-      const TB x106 = s.PA[1] * x51 + s.WP[1] * x57;
-      const TB x107 = s.PA[2] * x51 + s.WP[2] * x57;
-      const TB x108 = s.PA[1] * x52 + s.WP[1] * x58 + z1 * (x18 - s.roz * x21);
-      const TB x223 = s.QC[0] * x45 + s.WQ[0] * x51 + r2 * x18;
-      const TB x109 = s.PA[2] * x52 + s.WP[2] * x58;
-      const TB x115 = s.PA[0] * x57 + s.WP[0] * x63 + z2 * (x21 - s.roz * x24);
-      const TB x118 = s.PA[1] * x58 + s.WP[1] * x64 + z1 * (x21 - s.roz * x24);
-      const TB x119 = s.PA[2] * x58 + s.WP[2] * x64;
-      const TB x197 = s.PA[2] * x115 + s.WP[2] * x125;
-      const TB x224 = s.QC[0] * x51 + s.WQ[0] * x57 + r2 * x21;
-      const TB x624 = s.QC[0] * x223 + s.WQ[0] * x224 + e1 * (x45 - s.roe * x51) + r2 * x214;
-      const TB x116 = s.PA[1] * x57 + s.WP[1] * x63;
-      const TB x117 = s.PA[2] * x57 + s.WP[2] * x63;
-      const TB x198 = s.PA[1] * x116 + s.WP[1] * x126 + z1 * (x57 - s.roz * x63);
-      const TB x199 = s.PA[2] * x116 + s.WP[2] * x126;
-      const TB x152 = s.PA[2] * x85 + s.WP[2] * x95;
-      const TB x154 = s.PA[2] * x86 + s.WP[2] * x96;
-      const TB x167 = s.PA[2] * x95 + s.WP[2] * x105;
-      const TB x168 = s.PA[1] * x96 + s.WP[1] * x106 + z1 * (x45 - s.roz * x51);
-      const TB x169 = s.PA[2] * x96 + s.WP[2] * x106;
-      const TB x182 = s.PA[2] * x105 + s.WP[2] * x115;
-      const TB x183 = s.PA[1] * x106 + s.WP[1] * x116 + z1 * (x51 - s.roz * x57);
-      const TB x184 = s.PA[2] * x106 + s.WP[2] * x116;
-      const TB x304 = s.QC[0] * x184 + s.WQ[0] * x199 + r2 * x119;
-      const TB x264 = s.QC[0] * x109 + s.WQ[0] * x119 + r1 * x61;
-      const TB x240 = s.QC[0] * x55 + s.WQ[0] * x61;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x252 = s.QC[0] * x106 + s.WQ[0] * x116 + r2 * x58;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x250 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x46;
-      const TB x251 = s.QC[0] * x96 + s.WQ[0] * x106 + r2 * x52;
-      const TB x644 = s.QC[0] * x250 + s.WQ[0] * x251 + e1 * (x86 - s.roe * x96) + r2 * x227;
-      const TB x645 = s.QC[0] * x251 + s.WQ[0] * x252 + e1 * (x96 - s.roe * x106) + r2 * x228;
-      const TB x1233 = s.QC[0] * x644 + s.WQ[0] * x645 + e2 * (x250 - s.roe * x251) + r2 * x627;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x256 = s.QC[0] * x107 + s.WQ[0] * x117 + r2 * x59;
-      const TB x296 = s.QC[0] * x182 + s.WQ[0] * x197 + r3 * x117;
-      const TB x630 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x47 - s.roe * x53) + r1 * x220;
-      const TB x254 = s.QC[0] * x87 + s.WQ[0] * x97 + r2 * x47;
-      const TB x255 = s.QC[0] * x97 + s.WQ[0] * x107 + r2 * x53;
-      const TB x647 = s.QC[0] * x254 + s.WQ[0] * x255 + e1 * (x87 - s.roe * x97) + r2 * x231;
-      const TB x648 = s.QC[0] * x255 + s.WQ[0] * x256 + e1 * (x97 - s.roe * x107) + r2 * x232;
-      const TB x1235 = s.QC[0] * x647 + s.WQ[0] * x648 + e2 * (x254 - s.roe * x255) + r2 * x630;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x260 = s.QC[0] * x108 + s.WQ[0] * x118 + r1 * x60;
-      const TB x300 = s.QC[0] * x183 + s.WQ[0] * x198 + r2 * x118;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x246 = s.QC[0] * x85 + s.WQ[0] * x95 + r3 * x45;
-      const TB x247 = s.QC[0] * x95 + s.WQ[0] * x105 + r3 * x51;
-      const TB x248 = s.QC[0] * x105 + s.WQ[0] * x115 + r3 * x57;
-      const TB x258 = s.QC[0] * x88 + s.WQ[0] * x98 + r1 * x48;
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x259 = s.QC[0] * x98 + s.WQ[0] * x108 + r1 * x54;
-      const TB x650 = s.QC[0] * x258 + s.WQ[0] * x259 + e1 * (x88 - s.roe * x98) + r1 * x235;
-      const TB x297 = s.QC[0] * x138 + s.WQ[0] * x153 + r2 * x88;
-      const TB x651 = s.QC[0] * x259 + s.WQ[0] * x260 + e1 * (x98 - s.roe * x108) + r1 * x236;
-      const TB x298 = s.QC[0] * x153 + s.WQ[0] * x168 + r2 * x98;
-      const TB x299 = s.QC[0] * x168 + s.WQ[0] * x183 + r2 * x108;
-      const TB x681 = s.QC[0] * x299 + s.WQ[0] * x300 + e1 * (x168 - s.roe * x183) + r2 * x260;
-      const TB x1237 = s.QC[0] * x650 + s.WQ[0] * x651 + e2 * (x258 - s.roe * x259) + r1 * x633;
-      const TB x679 = s.QC[0] * x297 + s.WQ[0] * x298 + e1 * (x138 - s.roe * x153) + r2 * x258;
-      acc19 += static_cast<double>(x679);
-      const TB x680 = s.QC[0] * x298 + s.WQ[0] * x299 + e1 * (x153 - s.roe * x168) + r2 * x259;
-      const TB x1256 = s.QC[0] * x679 + s.WQ[0] * x680 + e2 * (x297 - s.roe * x298) + r2 * x650;
-      acc205 += static_cast<double>(x1256);
-      const TB x1257 = s.QC[0] * x680 + s.WQ[0] * x681 + e2 * (x298 - s.roe * x299) + r2 * x651;
-      const TB x1884 = s.QC[0] * x1256 + s.WQ[0] * x1257 + e3 * (x679 - s.roe * x680) + r2 * x1237;
-      acc515 += static_cast<double>(x1884);
-      const TB x1915 = s.QC[1] * x1256 + s.WQ[1] * x1257 + r2 * x1233;
-      acc546 += static_cast<double>(x1915);
-      const TB x1946 = s.QC[2] * x1256 + s.WQ[2] * x1257;
-      acc577 += static_cast<double>(x1946);
-      const TB x262 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x49;
-      const TB x636 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x49 - s.roe * x55);
-      const TB x263 = s.QC[0] * x99 + s.WQ[0] * x109 + r1 * x55;
-      const TB x653 = s.QC[0] * x262 + s.WQ[0] * x263 + e1 * (x89 - s.roe * x99) + r1 * x239;
-      const TB x301 = s.QC[0] * x139 + s.WQ[0] * x154 + r2 * x89;
-      const TB x654 = s.QC[0] * x263 + s.WQ[0] * x264 + e1 * (x99 - s.roe * x109) + r1 * x240;
-      const TB x302 = s.QC[0] * x154 + s.WQ[0] * x169 + r2 * x99;
-      const TB x303 = s.QC[0] * x169 + s.WQ[0] * x184 + r2 * x109;
-      const TB x684 = s.QC[0] * x303 + s.WQ[0] * x304 + e1 * (x169 - s.roe * x184) + r2 * x264;
-      const TB x1239 = s.QC[0] * x653 + s.WQ[0] * x654 + e2 * (x262 - s.roe * x263) + r1 * x636;
-      const TB x682 = s.QC[0] * x301 + s.WQ[0] * x302 + e1 * (x139 - s.roe * x154) + r2 * x262;
-      acc20 += static_cast<double>(x682);
-      const TB x683 = s.QC[0] * x302 + s.WQ[0] * x303 + e1 * (x154 - s.roe * x169) + r2 * x263;
-      const TB x1258 = s.QC[0] * x682 + s.WQ[0] * x683 + e2 * (x301 - s.roe * x302) + r2 * x653;
-      acc206 += static_cast<double>(x1258);
-      const TB x1259 = s.QC[0] * x683 + s.WQ[0] * x684 + e2 * (x302 - s.roe * x303) + r2 * x654;
-      const TB x1885 = s.QC[0] * x1258 + s.WQ[0] * x1259 + e3 * (x682 - s.roe * x683) + r2 * x1239;
-      acc516 += static_cast<double>(x1885);
-      const TB x1947 = s.QC[2] * x1258 + s.WQ[2] * x1259 + r1 * x1233;
-      acc578 += static_cast<double>(x1947);
-      const TB x1916 = s.QC[1] * x1258 + s.WQ[1] * x1259 + r1 * x1235;
-      acc547 += static_cast<double>(x1916);
-      const TB x1324 = s.QC[1] * x683 + s.WQ[1] * x684 + r1 * x648;
-      const TB x1388 = s.QC[2] * x682 + s.WQ[2] * x683 + r1 * x644;
-      acc268 += static_cast<double>(x1388);
-      const TB x293 = s.QC[0] * x137 + s.WQ[0] * x152 + r3 * x87;
-      const TB x294 = s.QC[0] * x152 + s.WQ[0] * x167 + r3 * x97;
-      const TB x295 = s.QC[0] * x167 + s.WQ[0] * x182 + r3 * x107;
-      const TB x678 = s.QC[0] * x295 + s.WQ[0] * x296 + e1 * (x167 - s.roe * x182) + r3 * x256;
-      const TB x358 = s.QC[1] * x45 + s.WQ[1] * x51;
-      const TB x359 = s.QC[1] * x51 + s.WQ[1] * x57;
-      const TB x387 = s.QC[1] * x106 + s.WQ[1] * x116 + r1 * x57;
-      const TB x435 = s.QC[1] * x183 + s.WQ[1] * x198 + r2 * x116;
-      const TB x570 = s.QC[2] * x183 + s.WQ[2] * x198;
-      const TB x381 = s.QC[1] * x85 + s.WQ[1] * x95;
-      const TB x382 = s.QC[1] * x95 + s.WQ[1] * x105;
-      const TB x383 = s.QC[1] * x105 + s.WQ[1] * x115;
-      const TB x385 = s.QC[1] * x86 + s.WQ[1] * x96 + r1 * x45;
-      const TB x924 = s.QC[1] * x358 + s.WQ[1] * x359 + e1 * (x45 - s.roe * x51);
-      const TB x386 = s.QC[1] * x96 + s.WQ[1] * x106 + r1 * x51;
-      const TB x944 = s.QC[1] * x385 + s.WQ[1] * x386 + e1 * (x86 - s.roe * x96) + r1 * x358;
-      const TB x945 = s.QC[1] * x386 + s.WQ[1] * x387 + e1 * (x96 - s.roe * x106) + r1 * x359;
-      const TB x1623 = s.QC[1] * x944 + s.WQ[1] * x945 + e2 * (x385 - s.roe * x386) + r1 * x924;
-      const TB x389 = s.QC[1] * x87 + s.WQ[1] * x97;
-      const TB x390 = s.QC[1] * x97 + s.WQ[1] * x107;
-      const TB x438 = s.QC[1] * x169 + s.WQ[1] * x184 + r1 * x107;
-      const TB x573 = s.QC[2] * x169 + s.WQ[2] * x184 + r1 * x106;
-      const TB x428 = s.QC[1] * x137 + s.WQ[1] * x152;
-      const TB x429 = s.QC[1] * x152 + s.WQ[1] * x167;
-      const TB x430 = s.QC[1] * x167 + s.WQ[1] * x182;
-      const TB x431 = s.QC[1] * x182 + s.WQ[1] * x197;
-      const TB x566 = s.QC[2] * x182 + s.WQ[2] * x197 + r1 * x115;
-      const TB x518 = s.QC[2] * x105 + s.WQ[2] * x115;
-      const TB x432 = s.QC[1] * x138 + s.WQ[1] * x153 + r2 * x86;
-      const TB x433 = s.QC[1] * x153 + s.WQ[1] * x168 + r2 * x96;
-      const TB x434 = s.QC[1] * x168 + s.WQ[1] * x183 + r2 * x106;
-      const TB x521 = s.QC[2] * x96 + s.WQ[2] * x106;
-      const TB x981 = s.QC[1] * x434 + s.WQ[1] * x435 + e1 * (x168 - s.roe * x183) + r2 * x387;
-      const TB x1081 = s.QC[2] * x434 + s.WQ[2] * x435;
-      const TB x436 = s.QC[1] * x139 + s.WQ[1] * x154 + r1 * x87;
-      const TB x947 = s.QC[1] * x389 + s.WQ[1] * x390 + e1 * (x87 - s.roe * x97);
-      const TB x437 = s.QC[1] * x154 + s.WQ[1] * x169 + r1 * x97;
-      const TB x982 = s.QC[1] * x436 + s.WQ[1] * x437 + e1 * (x139 - s.roe * x154) + r1 * x389;
-      acc113 += static_cast<double>(x982);
-      const TB x983 = s.QC[1] * x437 + s.WQ[1] * x438 + e1 * (x154 - s.roe * x169) + r1 * x390;
-      const TB x1648 = s.QC[1] * x982 + s.WQ[1] * x983 + e2 * (x436 - s.roe * x437) + r1 * x947;
-      acc392 += static_cast<double>(x1648);
-      const TB x1713 = s.QC[2] * x982 + s.WQ[2] * x983 + r1 * x944;
-      acc423 += static_cast<double>(x1713);
-      const TB x1083 = s.QC[2] * x437 + s.WQ[2] * x438 + r1 * x386;
-      const TB x516 = s.QC[2] * x85 + s.WQ[2] * x95;
-      const TB x517 = s.QC[2] * x95 + s.WQ[2] * x105;
-      const TB x520 = s.QC[2] * x86 + s.WQ[2] * x96;
-      const TB x563 = s.QC[2] * x137 + s.WQ[2] * x152 + r1 * x85;
-      const TB x564 = s.QC[2] * x152 + s.WQ[2] * x167 + r1 * x95;
-      const TB x565 = s.QC[2] * x167 + s.WQ[2] * x182 + r1 * x105;
-      const TB x1178 = s.QC[2] * x565 + s.WQ[2] * x566 + e1 * (x167 - s.roe * x182) + r1 * x518;
-      const TB x978 = s.QC[1] * x430 + s.WQ[1] * x431 + e1 * (x167 - s.roe * x182);
-      const TB x1078 = s.QC[2] * x430 + s.WQ[2] * x431 + r1 * x383;
-      const TB x1142 = s.QC[2] * x517 + s.WQ[2] * x518 + e1 * (x95 - s.roe * x105);
-      const TB x567 = s.QC[2] * x138 + s.WQ[2] * x153;
-      const TB x568 = s.QC[2] * x153 + s.WQ[2] * x168;
-      const TB x569 = s.QC[2] * x168 + s.WQ[2] * x183;
-      const TB x1181 = s.QC[2] * x569 + s.WQ[2] * x570 + e1 * (x168 - s.roe * x183);
-      const TB x571 = s.QC[2] * x139 + s.WQ[2] * x154 + r1 * x86;
-      const TB x1144 = s.QC[2] * x520 + s.WQ[2] * x521 + e1 * (x86 - s.roe * x96);
-      const TB x572 = s.QC[2] * x154 + s.WQ[2] * x169 + r1 * x96;
-      const TB x1183 = s.QC[2] * x572 + s.WQ[2] * x573 + e1 * (x154 - s.roe * x169) + r1 * x521;
-      const TB x1182 = s.QC[2] * x571 + s.WQ[2] * x572 + e1 * (x139 - s.roe * x154) + r1 * x520;
-      acc175 += static_cast<double>(x1182);
-      const TB x1843 = s.QC[2] * x1182 + s.WQ[2] * x1183 + e2 * (x571 - s.roe * x572) + r1 * x1144;
-      acc485 += static_cast<double>(x1843);
-      const TB x641 = s.QC[0] * x246 + s.WQ[0] * x247 + e1 * (x85 - s.roe * x95) + r3 * x223;
-      const TB x642 = s.QC[0] * x247 + s.WQ[0] * x248 + e1 * (x95 - s.roe * x105) + r3 * x224;
-      const TB x942 = s.QC[1] * x382 + s.WQ[1] * x383 + e1 * (x95 - s.roe * x105);
-      const TB x1042 = s.QC[2] * x382 + s.WQ[2] * x383;
-      const TB x676 = s.QC[0] * x293 + s.WQ[0] * x294 + e1 * (x137 - s.roe * x152) + r3 * x254;
-      const TB x677 = s.QC[0] * x294 + s.WQ[0] * x295 + e1 * (x152 - s.roe * x167) + r3 * x255;
-      const TB x724 = s.QC[1] * x223 + s.WQ[1] * x224;
-      const TB x744 = s.QC[1] * x250 + s.WQ[1] * x251 + r1 * x223;
-      const TB x745 = s.QC[1] * x251 + s.WQ[1] * x252 + r1 * x224;
-      const TB x781 = s.QC[1] * x299 + s.WQ[1] * x300 + r2 * x252;
-      const TB x881 = s.QC[2] * x299 + s.WQ[2] * x300;
-      const TB x1428 = s.QC[1] * x744 + s.WQ[1] * x745 + e1 * (x250 - s.roe * x251) + r1 * x724;
-      const TB x741 = s.QC[1] * x246 + s.WQ[1] * x247;
-      const TB x742 = s.QC[1] * x247 + s.WQ[1] * x248;
-      const TB x747 = s.QC[1] * x254 + s.WQ[1] * x255;
-      const TB x782 = s.QC[1] * x301 + s.WQ[1] * x302 + r1 * x254;
-      acc51 += static_cast<double>(x782);
-      const TB x783 = s.QC[1] * x302 + s.WQ[1] * x303 + r1 * x255;
-      const TB x1453 = s.QC[1] * x782 + s.WQ[1] * x783 + e1 * (x301 - s.roe * x302) + r1 * x747;
-      acc299 += static_cast<double>(x1453);
-      const TB x1518 = s.QC[2] * x782 + s.WQ[2] * x783 + r1 * x744;
-      acc330 += static_cast<double>(x1518);
-      const TB x883 = s.QC[2] * x302 + s.WQ[2] * x303 + r1 * x251;
-      const TB x776 = s.QC[1] * x293 + s.WQ[1] * x294;
-      const TB x777 = s.QC[1] * x294 + s.WQ[1] * x295;
-      const TB x778 = s.QC[1] * x295 + s.WQ[1] * x296;
-      const TB x878 = s.QC[2] * x295 + s.WQ[2] * x296 + r1 * x248;
-      const TB x842 = s.QC[2] * x247 + s.WQ[2] * x248;
-      const TB x779 = s.QC[1] * x297 + s.WQ[1] * x298 + r2 * x250;
-      acc50 += static_cast<double>(x779);
-      const TB x780 = s.QC[1] * x298 + s.WQ[1] * x299 + r2 * x251;
-      const TB x844 = s.QC[2] * x250 + s.WQ[2] * x251;
-      const TB x882 = s.QC[2] * x301 + s.WQ[2] * x302 + r1 * x250;
-      acc82 += static_cast<double>(x882);
-      const TB x1583 = s.QC[2] * x882 + s.WQ[2] * x883 + e1 * (x301 - s.roe * x302) + r1 * x844;
-      acc361 += static_cast<double>(x1583);
-      const TB x1451 = s.QC[1] * x779 + s.WQ[1] * x780 + e1 * (x297 - s.roe * x298) + r2 * x744;
-      acc298 += static_cast<double>(x1451);
-      const TB x1452 = s.QC[1] * x780 + s.WQ[1] * x781 + e1 * (x298 - s.roe * x299) + r2 * x745;
-      const TB x2070 = s.QC[1] * x1451 + s.WQ[1] * x1452 + e2 * (x779 - s.roe * x780) + r2 * x1428;
-      acc701 += static_cast<double>(x2070);
-      const TB x2101 = s.QC[2] * x1451 + s.WQ[2] * x1452;
-      acc732 += static_cast<double>(x2101);
-      const TB x1517 = s.QC[2] * x780 + s.WQ[2] * x781;
-      const TB x841 = s.QC[2] * x246 + s.WQ[2] * x247;
-      const TB x876 = s.QC[2] * x293 + s.WQ[2] * x294 + r1 * x246;
-      acc80 += static_cast<double>(x876);
-      const TB x877 = s.QC[2] * x294 + s.WQ[2] * x295 + r1 * x247;
-      const TB x1580 = s.QC[2] * x877 + s.WQ[2] * x878 + e1 * (x294 - s.roe * x295) + r1 * x842;
-      const TB x1556 = s.QC[2] * x841 + s.WQ[2] * x842 + e1 * (x246 - s.roe * x247);
-      const TB x1579 = s.QC[2] * x876 + s.WQ[2] * x877 + e1 * (x293 - s.roe * x294) + r1 * x841;
-      acc359 += static_cast<double>(x1579);
-      const TB x2162 = s.QC[2] * x1579 + s.WQ[2] * x1580 + e2 * (x876 - s.roe * x877) + r1 * x1556;
-      acc793 += static_cast<double>(x2162);
-      const TB x879 = s.QC[2] * x297 + s.WQ[2] * x298;
-      acc81 += static_cast<double>(x879);
-      const TB x880 = s.QC[2] * x298 + s.WQ[2] * x299;
-      const TB x1582 = s.QC[2] * x880 + s.WQ[2] * x881 + e1 * (x298 - s.roe * x299);
-      const TB x1581 = s.QC[2] * x879 + s.WQ[2] * x880 + e1 * (x297 - s.roe * x298);
-      acc360 += static_cast<double>(x1581);
-      const TB x2163 = s.QC[2] * x1581 + s.WQ[2] * x1582 + e2 * (x879 - s.roe * x880);
-      acc794 += static_cast<double>(x2163);
-      const TB x941 = s.QC[1] * x381 + s.WQ[1] * x382 + e1 * (x85 - s.roe * x95);
-      const TB x1141 = s.QC[2] * x516 + s.WQ[2] * x517 + e1 * (x85 - s.roe * x95);
-      const TB x976 = s.QC[1] * x428 + s.WQ[1] * x429 + e1 * (x137 - s.roe * x152);
-      acc111 += static_cast<double>(x976);
-      const TB x1176 = s.QC[2] * x563 + s.WQ[2] * x564 + e1 * (x137 - s.roe * x152) + r1 * x516;
-      acc173 += static_cast<double>(x1176);
-      const TB x1816 = s.QC[2] * x1141 + s.WQ[2] * x1142 + e2 * (x516 - s.roe * x517);
-      const TB x1177 = s.QC[2] * x564 + s.WQ[2] * x565 + e1 * (x152 - s.roe * x167) + r1 * x517;
-      const TB x1840 = s.QC[2] * x1177 + s.WQ[2] * x1178 + e2 * (x564 - s.roe * x565) + r1 * x1142;
-      const TB x1839 = s.QC[2] * x1176 + s.WQ[2] * x1177 + e2 * (x563 - s.roe * x564) + r1 * x1141;
-      acc483 += static_cast<double>(x1839);
-      const TB x2317 = s.QC[2] * x1839 + s.WQ[2] * x1840 + e3 * (x1176 - s.roe * x1177) + r1 * x1816;
-      acc948 += static_cast<double>(x2317);
-      const TB x977 = s.QC[1] * x429 + s.WQ[1] * x430 + e1 * (x152 - s.roe * x167);
-      const TB x979 = s.QC[1] * x432 + s.WQ[1] * x433 + e1 * (x138 - s.roe * x153) + r2 * x385;
-      acc112 += static_cast<double>(x979);
-      const TB x1179 = s.QC[2] * x567 + s.WQ[2] * x568 + e1 * (x138 - s.roe * x153);
-      acc174 += static_cast<double>(x1179);
-      const TB x980 = s.QC[1] * x433 + s.WQ[1] * x434 + e1 * (x153 - s.roe * x168) + r2 * x386;
-      const TB x1180 = s.QC[2] * x568 + s.WQ[2] * x569 + e1 * (x153 - s.roe * x168);
-      const TB x1842 = s.QC[2] * x1180 + s.WQ[2] * x1181 + e2 * (x568 - s.roe * x569);
-      const TB x1841 = s.QC[2] * x1179 + s.WQ[2] * x1180 + e2 * (x567 - s.roe * x568);
-      acc484 += static_cast<double>(x1841);
-      const TB x2318 = s.QC[2] * x1841 + s.WQ[2] * x1842 + e3 * (x1179 - s.roe * x1180);
-      acc949 += static_cast<double>(x2318);
-      const TB x1044 = s.QC[2] * x385 + s.WQ[2] * x386;
-      const TB x1082 = s.QC[2] * x436 + s.WQ[2] * x437 + r1 * x385;
-      acc144 += static_cast<double>(x1082);
-      const TB x1778 = s.QC[2] * x1082 + s.WQ[2] * x1083 + e1 * (x436 - s.roe * x437) + r1 * x1044;
-      acc454 += static_cast<double>(x1778);
-      const TB x1646 = s.QC[1] * x979 + s.WQ[1] * x980 + e2 * (x432 - s.roe * x433) + r2 * x944;
-      acc391 += static_cast<double>(x1646);
-      const TB x1647 = s.QC[1] * x980 + s.WQ[1] * x981 + e2 * (x433 - s.roe * x434) + r2 * x945;
-      const TB x2194 = s.QC[1] * x1646 + s.WQ[1] * x1647 + e3 * (x979 - s.roe * x980) + r2 * x1623;
-      acc825 += static_cast<double>(x2194);
-      const TB x2225 = s.QC[2] * x1646 + s.WQ[2] * x1647;
-      acc856 += static_cast<double>(x2225);
-      const TB x1712 = s.QC[2] * x980 + s.WQ[2] * x981;
-      const TB x1041 = s.QC[2] * x381 + s.WQ[2] * x382;
-      const TB x1076 = s.QC[2] * x428 + s.WQ[2] * x429 + r1 * x381;
-      acc142 += static_cast<double>(x1076);
-      const TB x1077 = s.QC[2] * x429 + s.WQ[2] * x430 + r1 * x382;
-      const TB x1775 = s.QC[2] * x1077 + s.WQ[2] * x1078 + e1 * (x429 - s.roe * x430) + r1 * x1042;
-      const TB x1645 = s.QC[1] * x977 + s.WQ[1] * x978 + e2 * (x429 - s.roe * x430);
-      const TB x1710 = s.QC[2] * x977 + s.WQ[2] * x978 + r1 * x942;
-      const TB x1751 = s.QC[2] * x1041 + s.WQ[2] * x1042 + e1 * (x381 - s.roe * x382);
-      const TB x1621 = s.QC[1] * x941 + s.WQ[1] * x942 + e2 * (x381 - s.roe * x382);
-      const TB x1686 = s.QC[2] * x941 + s.WQ[2] * x942;
-      const TB x1709 = s.QC[2] * x976 + s.WQ[2] * x977 + r1 * x941;
-      acc421 += static_cast<double>(x1709);
-      const TB x2255 = s.QC[2] * x1709 + s.WQ[2] * x1710 + e1 * (x976 - s.roe * x977) + r1 * x1686;
-      acc886 += static_cast<double>(x2255);
-      const TB x1774 = s.QC[2] * x1076 + s.WQ[2] * x1077 + e1 * (x428 - s.roe * x429) + r1 * x1041;
-      acc452 += static_cast<double>(x1774);
-      const TB x2286 = s.QC[2] * x1774 + s.WQ[2] * x1775 + e2 * (x1076 - s.roe * x1077) + r1 * x1751;
-      acc917 += static_cast<double>(x2286);
-      const TB x1644 = s.QC[1] * x976 + s.WQ[1] * x977 + e2 * (x428 - s.roe * x429);
-      acc390 += static_cast<double>(x1644);
-      const TB x2193 = s.QC[1] * x1644 + s.WQ[1] * x1645 + e3 * (x976 - s.roe * x977);
-      acc824 += static_cast<double>(x2193);
-      const TB x2224 = s.QC[2] * x1644 + s.WQ[2] * x1645 + r1 * x1621;
-      acc855 += static_cast<double>(x2224);
-      const TB x1079 = s.QC[2] * x432 + s.WQ[2] * x433;
-      acc143 += static_cast<double>(x1079);
-      const TB x1080 = s.QC[2] * x433 + s.WQ[2] * x434;
-      const TB x1777 = s.QC[2] * x1080 + s.WQ[2] * x1081 + e1 * (x433 - s.roe * x434);
-      const TB x1776 = s.QC[2] * x1079 + s.WQ[2] * x1080 + e1 * (x432 - s.roe * x433);
-      acc453 += static_cast<double>(x1776);
-      const TB x2287 = s.QC[2] * x1776 + s.WQ[2] * x1777 + e2 * (x1079 - s.roe * x1080);
-      acc918 += static_cast<double>(x2287);
-      const TB x1231 = s.QC[0] * x641 + s.WQ[0] * x642 + e2 * (x246 - s.roe * x247) + r3 * x624;
-      const TB x1426 = s.QC[1] * x741 + s.WQ[1] * x742 + e1 * (x246 - s.roe * x247);
-      const TB x1298 = s.QC[1] * x644 + s.WQ[1] * x645 + r1 * x624;
-      const TB x1321 = s.QC[1] * x679 + s.WQ[1] * x680 + r2 * x644;
-      acc236 += static_cast<double>(x1321);
-      const TB x1322 = s.QC[1] * x680 + s.WQ[1] * x681 + r2 * x645;
-      const TB x1977 = s.QC[1] * x1321 + s.WQ[1] * x1322 + e1 * (x679 - s.roe * x680) + r2 * x1298;
-      acc608 += static_cast<double>(x1977);
-      const TB x2008 = s.QC[2] * x1321 + s.WQ[2] * x1322;
-      acc639 += static_cast<double>(x2008);
-      const TB x1387 = s.QC[2] * x680 + s.WQ[2] * x681;
-      const TB x1254 = s.QC[0] * x676 + s.WQ[0] * x677 + e2 * (x293 - s.roe * x294) + r3 * x647;
-      acc204 += static_cast<double>(x1254);
-      const TB x1449 = s.QC[1] * x776 + s.WQ[1] * x777 + e1 * (x293 - s.roe * x294);
-      acc297 += static_cast<double>(x1449);
-      const TB x1255 = s.QC[0] * x677 + s.WQ[0] * x678 + e2 * (x294 - s.roe * x295) + r3 * x648;
-      const TB x1450 = s.QC[1] * x777 + s.WQ[1] * x778 + e1 * (x294 - s.roe * x295);
-      const TB x1883 = s.QC[0] * x1254 + s.WQ[0] * x1255 + e3 * (x676 - s.roe * x677) + r3 * x1235;
-      acc514 += static_cast<double>(x1883);
-      const TB x1945 = s.QC[2] * x1254 + s.WQ[2] * x1255 + r1 * x1231;
-      acc576 += static_cast<double>(x1945);
-      const TB x1914 = s.QC[1] * x1254 + s.WQ[1] * x1255;
-      acc545 += static_cast<double>(x1914);
-      const TB x2100 = s.QC[2] * x1449 + s.WQ[2] * x1450 + r1 * x1426;
-      acc731 += static_cast<double>(x2100);
-      const TB x2069 = s.QC[1] * x1449 + s.WQ[1] * x1450 + e2 * (x776 - s.roe * x777);
-      acc700 += static_cast<double>(x2069);
-      const TB x1300 = s.QC[1] * x647 + s.WQ[1] * x648;
-      const TB x1323 = s.QC[1] * x682 + s.WQ[1] * x683 + r1 * x647;
-      acc237 += static_cast<double>(x1323);
-      const TB x1978 = s.QC[1] * x1323 + s.WQ[1] * x1324 + e1 * (x682 - s.roe * x683) + r1 * x1300;
-      acc609 += static_cast<double>(x1978);
-      const TB x1515 = s.QC[2] * x777 + s.WQ[2] * x778 + r1 * x742;
-      const TB x1491 = s.QC[2] * x741 + s.WQ[2] * x742;
-      const TB x1514 = s.QC[2] * x776 + s.WQ[2] * x777 + r1 * x741;
-      acc328 += static_cast<double>(x1514);
-      const TB x2131 = s.QC[2] * x1514 + s.WQ[2] * x1515 + e1 * (x776 - s.roe * x777) + r1 * x1491;
-      acc762 += static_cast<double>(x2131);
-      const TB x1296 = s.QC[1] * x641 + s.WQ[1] * x642;
-      const TB x1319 = s.QC[1] * x676 + s.WQ[1] * x677;
-      acc235 += static_cast<double>(x1319);
-      const TB x1320 = s.QC[1] * x677 + s.WQ[1] * x678;
-      const TB x2007 = s.QC[2] * x1319 + s.WQ[2] * x1320 + r1 * x1296;
-      acc638 += static_cast<double>(x2007);
-      const TB x1976 = s.QC[1] * x1319 + s.WQ[1] * x1320 + e1 * (x676 - s.roe * x677);
-      acc607 += static_cast<double>(x1976);
-      const TB x1385 = s.QC[2] * x677 + s.WQ[2] * x678 + r1 * x642;
-      const TB x1361 = s.QC[2] * x641 + s.WQ[2] * x642;
-      const TB x1384 = s.QC[2] * x676 + s.WQ[2] * x677 + r1 * x641;
-      acc266 += static_cast<double>(x1384);
-      const TB x2038 = s.QC[2] * x1384 + s.WQ[2] * x1385 + e1 * (x676 - s.roe * x677) + r1 * x1361;
-      acc669 += static_cast<double>(x2038);
-      const TB x1386 = s.QC[2] * x679 + s.WQ[2] * x680;
-      acc267 += static_cast<double>(x1386);
-      const TB x2039 = s.QC[2] * x1386 + s.WQ[2] * x1387 + e1 * (x679 - s.roe * x680);
-      acc670 += static_cast<double>(x2039);
-      const TB x1516 = s.QC[2] * x779 + s.WQ[2] * x780;
-      acc329 += static_cast<double>(x1516);
-      const TB x2132 = s.QC[2] * x1516 + s.WQ[2] * x1517 + e1 * (x779 - s.roe * x780);
-      acc763 += static_cast<double>(x2132);
-      const TB x1711 = s.QC[2] * x979 + s.WQ[2] * x980;
-      acc422 += static_cast<double>(x1711);
-      const TB x2256 = s.QC[2] * x1711 + s.WQ[2] * x1712 + e1 * (x979 - s.roe * x980);
-      acc887 += static_cast<double>(x2256);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(80 * R + r) * a.acc_stride + q] = acc80;
-  a.acc[(111 * R + r) * a.acc_stride + q] = acc111;
-  a.acc[(142 * R + r) * a.acc_stride + q] = acc142;
-  a.acc[(173 * R + r) * a.acc_stride + q] = acc173;
-  a.acc[(204 * R + r) * a.acc_stride + q] = acc204;
-  a.acc[(235 * R + r) * a.acc_stride + q] = acc235;
-  a.acc[(266 * R + r) * a.acc_stride + q] = acc266;
-  a.acc[(297 * R + r) * a.acc_stride + q] = acc297;
-  a.acc[(328 * R + r) * a.acc_stride + q] = acc328;
-  a.acc[(359 * R + r) * a.acc_stride + q] = acc359;
-  a.acc[(390 * R + r) * a.acc_stride + q] = acc390;
-  a.acc[(421 * R + r) * a.acc_stride + q] = acc421;
-  a.acc[(452 * R + r) * a.acc_stride + q] = acc452;
-  a.acc[(483 * R + r) * a.acc_stride + q] = acc483;
-  a.acc[(514 * R + r) * a.acc_stride + q] = acc514;
-  a.acc[(545 * R + r) * a.acc_stride + q] = acc545;
-  a.acc[(576 * R + r) * a.acc_stride + q] = acc576;
-  a.acc[(607 * R + r) * a.acc_stride + q] = acc607;
-  a.acc[(638 * R + r) * a.acc_stride + q] = acc638;
-  a.acc[(669 * R + r) * a.acc_stride + q] = acc669;
-  a.acc[(700 * R + r) * a.acc_stride + q] = acc700;
-  a.acc[(731 * R + r) * a.acc_stride + q] = acc731;
-  a.acc[(762 * R + r) * a.acc_stride + q] = acc762;
-  a.acc[(793 * R + r) * a.acc_stride + q] = acc793;
-  a.acc[(824 * R + r) * a.acc_stride + q] = acc824;
-  a.acc[(855 * R + r) * a.acc_stride + q] = acc855;
-  a.acc[(886 * R + r) * a.acc_stride + q] = acc886;
-  a.acc[(917 * R + r) * a.acc_stride + q] = acc917;
-  a.acc[(948 * R + r) * a.acc_stride + q] = acc948;
-  a.acc[(19 * R + r) * a.acc_stride + q] = acc19;
-  a.acc[(50 * R + r) * a.acc_stride + q] = acc50;
-  a.acc[(81 * R + r) * a.acc_stride + q] = acc81;
-  a.acc[(112 * R + r) * a.acc_stride + q] = acc112;
-  a.acc[(143 * R + r) * a.acc_stride + q] = acc143;
-  a.acc[(174 * R + r) * a.acc_stride + q] = acc174;
-  a.acc[(205 * R + r) * a.acc_stride + q] = acc205;
-  a.acc[(236 * R + r) * a.acc_stride + q] = acc236;
-  a.acc[(267 * R + r) * a.acc_stride + q] = acc267;
-  a.acc[(298 * R + r) * a.acc_stride + q] = acc298;
-  a.acc[(329 * R + r) * a.acc_stride + q] = acc329;
-  a.acc[(360 * R + r) * a.acc_stride + q] = acc360;
-  a.acc[(391 * R + r) * a.acc_stride + q] = acc391;
-  a.acc[(422 * R + r) * a.acc_stride + q] = acc422;
-  a.acc[(453 * R + r) * a.acc_stride + q] = acc453;
-  a.acc[(484 * R + r) * a.acc_stride + q] = acc484;
-  a.acc[(515 * R + r) * a.acc_stride + q] = acc515;
-  a.acc[(546 * R + r) * a.acc_stride + q] = acc546;
-  a.acc[(577 * R + r) * a.acc_stride + q] = acc577;
-  a.acc[(608 * R + r) * a.acc_stride + q] = acc608;
-  a.acc[(639 * R + r) * a.acc_stride + q] = acc639;
-  a.acc[(670 * R + r) * a.acc_stride + q] = acc670;
-  a.acc[(701 * R + r) * a.acc_stride + q] = acc701;
-  a.acc[(732 * R + r) * a.acc_stride + q] = acc732;
-  a.acc[(763 * R + r) * a.acc_stride + q] = acc763;
-  a.acc[(794 * R + r) * a.acc_stride + q] = acc794;
-  a.acc[(825 * R + r) * a.acc_stride + q] = acc825;
-  a.acc[(856 * R + r) * a.acc_stride + q] = acc856;
-  a.acc[(887 * R + r) * a.acc_stride + q] = acc887;
-  a.acc[(918 * R + r) * a.acc_stride + q] = acc918;
-  a.acc[(949 * R + r) * a.acc_stride + q] = acc949;
-  a.acc[(20 * R + r) * a.acc_stride + q] = acc20;
-  a.acc[(51 * R + r) * a.acc_stride + q] = acc51;
-  a.acc[(82 * R + r) * a.acc_stride + q] = acc82;
-  a.acc[(113 * R + r) * a.acc_stride + q] = acc113;
-  a.acc[(144 * R + r) * a.acc_stride + q] = acc144;
-  a.acc[(175 * R + r) * a.acc_stride + q] = acc175;
-  a.acc[(206 * R + r) * a.acc_stride + q] = acc206;
-  a.acc[(237 * R + r) * a.acc_stride + q] = acc237;
-  a.acc[(268 * R + r) * a.acc_stride + q] = acc268;
-  a.acc[(299 * R + r) * a.acc_stride + q] = acc299;
-  a.acc[(330 * R + r) * a.acc_stride + q] = acc330;
-  a.acc[(361 * R + r) * a.acc_stride + q] = acc361;
-  a.acc[(392 * R + r) * a.acc_stride + q] = acc392;
-  a.acc[(423 * R + r) * a.acc_stride + q] = acc423;
-  a.acc[(454 * R + r) * a.acc_stride + q] = acc454;
-  a.acc[(485 * R + r) * a.acc_stride + q] = acc485;
-  a.acc[(516 * R + r) * a.acc_stride + q] = acc516;
-  a.acc[(547 * R + r) * a.acc_stride + q] = acc547;
-  a.acc[(578 * R + r) * a.acc_stride + q] = acc578;
-  a.acc[(609 * R + r) * a.acc_stride + q] = acc609;
-}
-// class (22|22) split 8/12: 433 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_8(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc640 = 0.0;
-  double acc671 = 0.0;
-  double acc702 = 0.0;
-  double acc733 = 0.0;
-  double acc764 = 0.0;
-  double acc795 = 0.0;
-  double acc826 = 0.0;
-  double acc857 = 0.0;
-  double acc888 = 0.0;
-  double acc919 = 0.0;
-  double acc950 = 0.0;
-  double acc21 = 0.0;
-  double acc52 = 0.0;
-  double acc83 = 0.0;
-  double acc114 = 0.0;
-  double acc145 = 0.0;
-  double acc176 = 0.0;
-  double acc207 = 0.0;
-  double acc238 = 0.0;
-  double acc269 = 0.0;
-  double acc300 = 0.0;
-  double acc331 = 0.0;
-  double acc362 = 0.0;
-  double acc393 = 0.0;
-  double acc424 = 0.0;
-  double acc455 = 0.0;
-  double acc486 = 0.0;
-  double acc517 = 0.0;
-  double acc548 = 0.0;
-  double acc579 = 0.0;
-  double acc610 = 0.0;
-  double acc641 = 0.0;
-  double acc672 = 0.0;
-  double acc703 = 0.0;
-  double acc734 = 0.0;
-  double acc765 = 0.0;
-  double acc796 = 0.0;
-  double acc827 = 0.0;
-  double acc858 = 0.0;
-  double acc889 = 0.0;
-  double acc920 = 0.0;
-  double acc951 = 0.0;
-  double acc22 = 0.0;
-  double acc53 = 0.0;
-  double acc84 = 0.0;
-  double acc115 = 0.0;
-  double acc146 = 0.0;
-  double acc177 = 0.0;
-  double acc208 = 0.0;
-  double acc239 = 0.0;
-  double acc270 = 0.0;
-  double acc301 = 0.0;
-  double acc332 = 0.0;
-  double acc363 = 0.0;
-  double acc394 = 0.0;
-  double acc425 = 0.0;
-  double acc456 = 0.0;
-  double acc487 = 0.0;
-  double acc518 = 0.0;
-  double acc549 = 0.0;
-  double acc580 = 0.0;
-  double acc611 = 0.0;
-  double acc642 = 0.0;
-  double acc673 = 0.0;
-  double acc704 = 0.0;
-  double acc735 = 0.0;
-  double acc766 = 0.0;
-  double acc797 = 0.0;
-  double acc828 = 0.0;
-  double acc859 = 0.0;
-  double acc890 = 0.0;
-  double acc921 = 0.0;
-  double acc952 = 0.0;
-  double acc23 = 0.0;
-  double acc54 = 0.0;
-  double acc85 = 0.0;
-  double acc116 = 0.0;
-  double acc147 = 0.0;
-  double acc178 = 0.0;
-  double acc209 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB x8 = static_cast<TB>(F[8] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x30 = s.PA[0] * x7 + s.WP[0] * x8;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x33 = s.PA[0] * x9 + s.WP[0] * x12 + z1 * (x0 - s.roz * x1);
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x28 = s.PA[1] * x6 + s.WP[1] * x7;
-      const TB x66 = s.PA[1] * x25 + s.WP[1] * x28 + z1 * (x5 - s.roz * x6);
-      const TB x69 = s.PA[0] * x27 + s.WP[0] * x30 + z1 * (x6 - s.roz * x7);
-      const TB x63 = s.PA[0] * x24 + s.WP[0] * x27 + z1 * (x5 - s.roz * x6);
-      const TB x70 = s.PA[1] * x27 + s.WP[1] * x30;
-      const TB x34 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x39 = s.PA[0] * x12 + s.WP[0] * x15 + z1 * (x1 - s.roz * x2);
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x40 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x78 = s.PA[1] * x34 + s.WP[1] * x40 + z1 * (x9 - s.roz * x12);
-      const TB x41 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x45 = s.PA[0] * x15 + s.WP[0] * x18 + z1 * (x2 - s.roz * x3);
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x91 = s.PA[1] * x42 + s.WP[1] * x48 + z2 * (x13 - s.roz * x16);
-      const TB x92 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x90 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x12 - s.roz * x15);
-      const TB x88 = s.PA[1] * x40 + s.WP[1] * x46 + z1 * (x12 - s.roz * x15);
-      const TB x141 = s.PA[1] * x78 + s.WP[1] * x88 + z2 * (x34 - s.roz * x40);
-      const TB x142 = s.PA[2] * x78 + s.WP[2] * x88;
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x51 = s.PA[0] * x18 + s.WP[0] * x21 + z1 * (x3 - s.roz * x4);
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x101 = s.PA[1] * x48 + s.WP[1] * x54 + z2 * (x16 - s.roz * x19);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x57 = s.PA[0] * x21 + s.WP[0] * x24 + z1 * (x4 - s.roz * x5);
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x121 = s.PA[1] * x60 + s.WP[1] * x66 + z2 * (x22 - s.roz * x25);
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x64 = s.PA[1] * x24 + s.WP[1] * x27;
-      const TB x128 = s.PA[1] * x64 + s.WP[1] * x70 + z1 * (x24 - s.roz * x27);
-      const TB x65 = s.PA[2] * x24 + s.WP[2] * x27;
-      const TB x120 = s.PA[2] * x59 + s.WP[2] * x65 + z1 * (x21 - s.roz * x24);
-      const TB x118 = s.PA[1] * x58 + s.WP[1] * x64 + z1 * (x21 - s.roz * x24);
-      const TB x201 = s.PA[1] * x118 + s.WP[1] * x128 + z2 * (x58 - s.roz * x64);
-      const TB x119 = s.PA[2] * x58 + s.WP[2] * x64;
-      const TB x76 = s.PA[1] * x33 + s.WP[1] * x39;
-      const TB x77 = s.PA[2] * x33 + s.WP[2] * x39;
-      const TB x86 = s.PA[1] * x39 + s.WP[1] * x45;
-      const TB x139 = s.PA[2] * x76 + s.WP[2] * x86;
-      const TB x87 = s.PA[2] * x39 + s.WP[2] * x45;
-      const TB x140 = s.PA[2] * x77 + s.WP[2] * x87 + z1 * (x33 - s.roz * x39);
-      const TB x89 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x96 = s.PA[1] * x45 + s.WP[1] * x51;
-      const TB x97 = s.PA[2] * x45 + s.WP[2] * x51;
-      const TB x155 = s.PA[2] * x87 + s.WP[2] * x97 + z1 * (x39 - s.roz * x45);
-      const TB x98 = s.PA[1] * x46 + s.WP[1] * x52 + z1 * (x15 - s.roz * x18);
-      const TB x100 = s.PA[2] * x47 + s.WP[2] * x53 + z1 * (x15 - s.roz * x18);
-      const TB x156 = s.PA[1] * x88 + s.WP[1] * x98 + z2 * (x40 - s.roz * x46);
-      const TB x99 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x262 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x49;
-      const TB x102 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x106 = s.PA[1] * x51 + s.WP[1] * x57;
-      const TB x107 = s.PA[2] * x51 + s.WP[2] * x57;
-      const TB x108 = s.PA[1] * x52 + s.WP[1] * x58 + z1 * (x18 - s.roz * x21);
-      const TB x109 = s.PA[2] * x52 + s.WP[2] * x58;
-      const TB x263 = s.QC[0] * x99 + s.WQ[0] * x109 + r1 * x55;
-      const TB x264 = s.QC[0] * x109 + s.WQ[0] * x119 + r1 * x61;
-      const TB x110 = s.PA[2] * x53 + s.WP[2] * x59 + z1 * (x18 - s.roz * x21);
-      const TB x111 = s.PA[1] * x54 + s.WP[1] * x60 + z2 * (x19 - s.roz * x22);
-      const TB x112 = s.PA[2] * x54 + s.WP[2] * x60;
-      const TB x116 = s.PA[1] * x57 + s.WP[1] * x63;
-      const TB x117 = s.PA[2] * x57 + s.WP[2] * x63;
-      const TB x126 = s.PA[1] * x63 + s.WP[1] * x69;
-      const TB x127 = s.PA[2] * x63 + s.WP[2] * x69;
-      const TB x200 = s.PA[2] * x117 + s.WP[2] * x127 + z1 * (x57 - s.roz * x63);
-      const TB x199 = s.PA[2] * x116 + s.WP[2] * x126;
-      const TB x154 = s.PA[2] * x86 + s.WP[2] * x96;
-      const TB x157 = s.PA[2] * x88 + s.WP[2] * x98;
-      const TB x169 = s.PA[2] * x96 + s.WP[2] * x106;
-      const TB x170 = s.PA[2] * x97 + s.WP[2] * x107 + z1 * (x45 - s.roz * x51);
-      const TB x171 = s.PA[1] * x98 + s.WP[1] * x108 + z2 * (x46 - s.roz * x52);
-      const TB x172 = s.PA[2] * x98 + s.WP[2] * x108;
-      const TB x184 = s.PA[2] * x106 + s.WP[2] * x116;
-      const TB x303 = s.QC[0] * x169 + s.WQ[0] * x184 + r2 * x109;
-      const TB x304 = s.QC[0] * x184 + s.WQ[0] * x199 + r2 * x119;
-      const TB x684 = s.QC[0] * x303 + s.WQ[0] * x304 + e1 * (x169 - s.roe * x184) + r2 * x264;
-      const TB x185 = s.PA[2] * x107 + s.WP[2] * x117 + z1 * (x51 - s.roz * x57);
-      const TB x186 = s.PA[1] * x108 + s.WP[1] * x118 + z2 * (x52 - s.roz * x58);
-      const TB x187 = s.PA[2] * x108 + s.WP[2] * x118;
-      const TB x315 = s.QC[0] * x172 + s.WQ[0] * x187 + r1 * x112;
-      const TB x275 = s.QC[0] * x102 + s.WQ[0] * x112;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x256 = s.QC[0] * x107 + s.WQ[0] * x117 + r2 * x59;
-      const TB x630 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x47 - s.roe * x53) + r1 * x220;
-      const TB x255 = s.QC[0] * x97 + s.WQ[0] * x107 + r2 * x53;
-      const TB x648 = s.QC[0] * x255 + s.WQ[0] * x256 + e1 * (x97 - s.roe * x107) + r2 * x232;
-      const TB x223 = s.QC[0] * x45 + s.WQ[0] * x51 + r2 * x18;
-      const TB x224 = s.QC[0] * x51 + s.WQ[0] * x57 + r2 * x21;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x260 = s.QC[0] * x108 + s.WQ[0] * x118 + r1 * x60;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x244 = s.QC[0] * x56 + s.WQ[0] * x62;
-      const TB x268 = s.QC[0] * x110 + s.WQ[0] * x120 + r1 * x62;
-      const TB x308 = s.QC[0] * x185 + s.WQ[0] * x200 + r2 * x120;
-      const TB x250 = s.QC[0] * x86 + s.WQ[0] * x96 + r2 * x46;
-      const TB x251 = s.QC[0] * x96 + s.WQ[0] * x106 + r2 * x52;
-      const TB x252 = s.QC[0] * x106 + s.WQ[0] * x116 + r2 * x58;
-      const TB x254 = s.QC[0] * x87 + s.WQ[0] * x97 + r2 * x47;
-      const TB x397 = s.QC[1] * x89 + s.WQ[1] * x99 + r1 * x47;
-      const TB x647 = s.QC[0] * x254 + s.WQ[0] * x255 + e1 * (x87 - s.roe * x97) + r2 * x231;
-      const TB x1235 = s.QC[0] * x647 + s.WQ[0] * x648 + e2 * (x254 - s.roe * x255) + r2 * x630;
-      const TB x258 = s.QC[0] * x88 + s.WQ[0] * x98 + r1 * x48;
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x259 = s.QC[0] * x98 + s.WQ[0] * x108 + r1 * x54;
-      const TB x650 = s.QC[0] * x258 + s.WQ[0] * x259 + e1 * (x88 - s.roe * x98) + r1 * x235;
-      const TB x651 = s.QC[0] * x259 + s.WQ[0] * x260 + e1 * (x98 - s.roe * x108) + r1 * x236;
-      const TB x1237 = s.QC[0] * x650 + s.WQ[0] * x651 + e2 * (x258 - s.roe * x259) + r1 * x633;
-      const TB x1302 = s.QC[1] * x650 + s.WQ[1] * x651 + r2 * x627;
-      const TB x266 = s.QC[0] * x90 + s.WQ[0] * x100 + r1 * x50;
-      const TB x639 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x50 - s.roe * x56);
-      const TB x267 = s.QC[0] * x100 + s.WQ[0] * x110 + r1 * x56;
-      const TB x656 = s.QC[0] * x266 + s.WQ[0] * x267 + e1 * (x90 - s.roe * x100) + r1 * x243;
-      const TB x305 = s.QC[0] * x140 + s.WQ[0] * x155 + r2 * x90;
-      const TB x657 = s.QC[0] * x267 + s.WQ[0] * x268 + e1 * (x100 - s.roe * x110) + r1 * x244;
-      const TB x306 = s.QC[0] * x155 + s.WQ[0] * x170 + r2 * x100;
-      const TB x307 = s.QC[0] * x170 + s.WQ[0] * x185 + r2 * x110;
-      const TB x687 = s.QC[0] * x307 + s.WQ[0] * x308 + e1 * (x170 - s.roe * x185) + r2 * x268;
-      const TB x1241 = s.QC[0] * x656 + s.WQ[0] * x657 + e2 * (x266 - s.roe * x267) + r1 * x639;
-      const TB x685 = s.QC[0] * x305 + s.WQ[0] * x306 + e1 * (x140 - s.roe * x155) + r2 * x266;
-      acc21 += static_cast<double>(x685);
-      const TB x686 = s.QC[0] * x306 + s.WQ[0] * x307 + e1 * (x155 - s.roe * x170) + r2 * x267;
-      const TB x1260 = s.QC[0] * x685 + s.WQ[0] * x686 + e2 * (x305 - s.roe * x306) + r2 * x656;
-      acc207 += static_cast<double>(x1260);
-      const TB x1261 = s.QC[0] * x686 + s.WQ[0] * x687 + e2 * (x306 - s.roe * x307) + r2 * x657;
-      const TB x1886 = s.QC[0] * x1260 + s.WQ[0] * x1261 + e3 * (x685 - s.roe * x686) + r2 * x1241;
-      acc517 += static_cast<double>(x1886);
-      const TB x1948 = s.QC[2] * x1260 + s.WQ[2] * x1261 + r2 * x1235;
-      acc579 += static_cast<double>(x1948);
-      const TB x1917 = s.QC[1] * x1260 + s.WQ[1] * x1261;
-      acc548 += static_cast<double>(x1917);
-      const TB x270 = s.QC[0] * x91 + s.WQ[0] * x101;
-      const TB x271 = s.QC[0] * x101 + s.WQ[0] * x111;
-      const TB x272 = s.QC[0] * x111 + s.WQ[0] * x121;
-      const TB x312 = s.QC[0] * x186 + s.WQ[0] * x201 + r1 * x121;
-      const TB x274 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x301 = s.QC[0] * x139 + s.WQ[0] * x154 + r2 * x89;
-      const TB x448 = s.QC[1] * x142 + s.WQ[1] * x157 + r2 * x89;
-      const TB x302 = s.QC[0] * x154 + s.WQ[0] * x169 + r2 * x99;
-      const TB x449 = s.QC[1] * x157 + s.WQ[1] * x172 + r2 * x99;
-      const TB x991 = s.QC[1] * x448 + s.WQ[1] * x449 + e1 * (x142 - s.roe * x157) + r2 * x397;
-      acc116 += static_cast<double>(x991);
-      const TB x683 = s.QC[0] * x302 + s.WQ[0] * x303 + e1 * (x154 - s.roe * x169) + r2 * x263;
-      const TB x309 = s.QC[0] * x141 + s.WQ[0] * x156 + r1 * x91;
-      const TB x659 = s.QC[0] * x270 + s.WQ[0] * x271 + e1 * (x91 - s.roe * x101);
-      const TB x310 = s.QC[0] * x156 + s.WQ[0] * x171 + r1 * x101;
-      const TB x660 = s.QC[0] * x271 + s.WQ[0] * x272 + e1 * (x101 - s.roe * x111);
-      const TB x311 = s.QC[0] * x171 + s.WQ[0] * x186 + r1 * x111;
-      const TB x690 = s.QC[0] * x311 + s.WQ[0] * x312 + e1 * (x171 - s.roe * x186) + r1 * x272;
-      const TB x313 = s.QC[0] * x142 + s.WQ[0] * x157 + r1 * x92;
-      const TB x662 = s.QC[0] * x274 + s.WQ[0] * x275 + e1 * (x92 - s.roe * x102);
-      const TB x314 = s.QC[0] * x157 + s.WQ[0] * x172 + r1 * x102;
-      const TB x692 = s.QC[0] * x314 + s.WQ[0] * x315 + e1 * (x157 - s.roe * x172) + r1 * x275;
-      const TB x584 = s.QC[2] * x157 + s.WQ[2] * x172 + r1 * x98;
-      const TB x691 = s.QC[0] * x313 + s.WQ[0] * x314 + e1 * (x142 - s.roe * x157) + r1 * x274;
-      acc23 += static_cast<double>(x691);
-      const TB x1264 = s.QC[0] * x691 + s.WQ[0] * x692 + e2 * (x313 - s.roe * x314) + r1 * x662;
-      acc209 += static_cast<double>(x1264);
-      const TB x791 = s.QC[1] * x313 + s.WQ[1] * x314 + r2 * x262;
-      acc54 += static_cast<double>(x791);
-      const TB x891 = s.QC[2] * x313 + s.WQ[2] * x314 + r1 * x258;
-      acc85 += static_cast<double>(x891);
-      const TB x682 = s.QC[0] * x301 + s.WQ[0] * x302 + e1 * (x139 - s.roe * x154) + r2 * x262;
-      const TB x349 = s.QC[1] * x18 + s.WQ[1] * x21;
-      const TB x362 = s.QC[1] * x46 + s.WQ[1] * x52 + r1 * x18;
-      const TB x363 = s.QC[1] * x52 + s.WQ[1] * x58 + r1 * x21;
-      const TB x395 = s.QC[1] * x108 + s.WQ[1] * x118 + r2 * x58;
-      const TB x447 = s.QC[1] * x186 + s.WQ[1] * x201 + r3 * x118;
-      const TB x582 = s.QC[2] * x186 + s.WQ[2] * x201;
-      const TB x927 = s.QC[1] * x362 + s.WQ[1] * x363 + e1 * (x46 - s.roe * x52) + r1 * x349;
-      const TB x393 = s.QC[1] * x88 + s.WQ[1] * x98 + r2 * x46;
-      const TB x1091 = s.QC[2] * x448 + s.WQ[2] * x449 + r1 * x393;
-      acc147 += static_cast<double>(x1091);
-      const TB x394 = s.QC[1] * x98 + s.WQ[1] * x108 + r2 * x52;
-      const TB x950 = s.QC[1] * x393 + s.WQ[1] * x394 + e1 * (x88 - s.roe * x98) + r2 * x362;
-      const TB x951 = s.QC[1] * x394 + s.WQ[1] * x395 + e1 * (x98 - s.roe * x108) + r2 * x363;
-      const TB x446 = s.QC[1] * x171 + s.WQ[1] * x186 + r3 * x108;
-      const TB x990 = s.QC[1] * x446 + s.WQ[1] * x447 + e1 * (x171 - s.roe * x186) + r3 * x395;
-      const TB x1090 = s.QC[2] * x446 + s.WQ[2] * x447;
-      const TB x1627 = s.QC[1] * x950 + s.WQ[1] * x951 + e2 * (x393 - s.roe * x394) + r2 * x927;
-      const TB x358 = s.QC[1] * x45 + s.WQ[1] * x51;
-      const TB x359 = s.QC[1] * x51 + s.WQ[1] * x57;
-      const TB x385 = s.QC[1] * x86 + s.WQ[1] * x96 + r1 * x45;
-      const TB x386 = s.QC[1] * x96 + s.WQ[1] * x106 + r1 * x51;
-      const TB x387 = s.QC[1] * x106 + s.WQ[1] * x116 + r1 * x57;
-      const TB x389 = s.QC[1] * x87 + s.WQ[1] * x97;
-      const TB x390 = s.QC[1] * x97 + s.WQ[1] * x107;
-      const TB x391 = s.QC[1] * x107 + s.WQ[1] * x117;
-      const TB x436 = s.QC[1] * x139 + s.WQ[1] * x154 + r1 * x87;
-      const TB x437 = s.QC[1] * x154 + s.WQ[1] * x169 + r1 * x97;
-      const TB x438 = s.QC[1] * x169 + s.WQ[1] * x184 + r1 * x107;
-      const TB x439 = s.QC[1] * x184 + s.WQ[1] * x199 + r1 * x117;
-      const TB x574 = s.QC[2] * x184 + s.WQ[2] * x199 + r1 * x116;
-      const TB x522 = s.QC[2] * x106 + s.WQ[2] * x116;
-      const TB x440 = s.QC[1] * x140 + s.WQ[1] * x155;
-      const TB x441 = s.QC[1] * x155 + s.WQ[1] * x170;
-      const TB x442 = s.QC[1] * x170 + s.WQ[1] * x185;
-      const TB x443 = s.QC[1] * x185 + s.WQ[1] * x200;
-      const TB x578 = s.QC[2] * x185 + s.WQ[2] * x200 + r2 * x117;
-      const TB x526 = s.QC[2] * x107 + s.WQ[2] * x117 + r1 * x57;
-      const TB x494 = s.QC[2] * x51 + s.WQ[2] * x57;
-      const TB x444 = s.QC[1] * x141 + s.WQ[1] * x156 + r3 * x88;
-      const TB x445 = s.QC[1] * x156 + s.WQ[1] * x171 + r3 * x98;
-      const TB x528 = s.QC[2] * x88 + s.WQ[2] * x98;
-      const TB x583 = s.QC[2] * x142 + s.WQ[2] * x157 + r1 * x88;
-      const TB x1191 = s.QC[2] * x583 + s.WQ[2] * x584 + e1 * (x142 - s.roe * x157) + r1 * x528;
-      acc178 += static_cast<double>(x1191);
-      const TB x988 = s.QC[1] * x444 + s.WQ[1] * x445 + e1 * (x141 - s.roe * x156) + r3 * x393;
-      acc115 += static_cast<double>(x988);
-      const TB x989 = s.QC[1] * x445 + s.WQ[1] * x446 + e1 * (x156 - s.roe * x171) + r3 * x394;
-      const TB x1652 = s.QC[1] * x988 + s.WQ[1] * x989 + e2 * (x444 - s.roe * x445) + r3 * x950;
-      acc394 += static_cast<double>(x1652);
-      const TB x1653 = s.QC[1] * x989 + s.WQ[1] * x990 + e2 * (x445 - s.roe * x446) + r3 * x951;
-      const TB x2197 = s.QC[1] * x1652 + s.WQ[1] * x1653 + e3 * (x988 - s.roe * x989) + r3 * x1627;
-      acc828 += static_cast<double>(x2197);
-      const TB x2228 = s.QC[2] * x1652 + s.WQ[2] * x1653;
-      acc859 += static_cast<double>(x2228);
-      const TB x1718 = s.QC[2] * x989 + s.WQ[2] * x990;
-      const TB x493 = s.QC[2] * x45 + s.WQ[2] * x51;
-      const TB x520 = s.QC[2] * x86 + s.WQ[2] * x96;
-      const TB x521 = s.QC[2] * x96 + s.WQ[2] * x106;
-      const TB x524 = s.QC[2] * x87 + s.WQ[2] * x97 + r1 * x45;
-      const TB x525 = s.QC[2] * x97 + s.WQ[2] * x107 + r1 * x51;
-      const TB x571 = s.QC[2] * x139 + s.WQ[2] * x154 + r1 * x86;
-      const TB x572 = s.QC[2] * x154 + s.WQ[2] * x169 + r1 * x96;
-      const TB x573 = s.QC[2] * x169 + s.WQ[2] * x184 + r1 * x106;
-      const TB x1184 = s.QC[2] * x573 + s.WQ[2] * x574 + e1 * (x169 - s.roe * x184) + r1 * x522;
-      const TB x984 = s.QC[1] * x438 + s.WQ[1] * x439 + e1 * (x169 - s.roe * x184) + r1 * x391;
-      const TB x1084 = s.QC[2] * x438 + s.WQ[2] * x439 + r1 * x387;
-      const TB x1145 = s.QC[2] * x521 + s.WQ[2] * x522 + e1 * (x96 - s.roe * x106);
-      const TB x575 = s.QC[2] * x140 + s.WQ[2] * x155 + r2 * x87;
-      const TB x576 = s.QC[2] * x155 + s.WQ[2] * x170 + r2 * x97;
-      const TB x577 = s.QC[2] * x170 + s.WQ[2] * x185 + r2 * x107;
-      const TB x1187 = s.QC[2] * x577 + s.WQ[2] * x578 + e1 * (x170 - s.roe * x185) + r2 * x526;
-      const TB x987 = s.QC[1] * x442 + s.WQ[1] * x443 + e1 * (x170 - s.roe * x185);
-      const TB x1087 = s.QC[2] * x442 + s.WQ[2] * x443 + r2 * x391;
-      const TB x1148 = s.QC[2] * x525 + s.WQ[2] * x526 + e1 * (x97 - s.roe * x107) + r1 * x494;
-      const TB x948 = s.QC[1] * x390 + s.WQ[1] * x391 + e1 * (x97 - s.roe * x107);
-      const TB x1048 = s.QC[2] * x390 + s.WQ[2] * x391 + r1 * x359;
-      const TB x1124 = s.QC[2] * x493 + s.WQ[2] * x494 + e1 * (x45 - s.roe * x51);
-      const TB x1147 = s.QC[2] * x524 + s.WQ[2] * x525 + e1 * (x87 - s.roe * x97) + r1 * x493;
-      const TB x947 = s.QC[1] * x389 + s.WQ[1] * x390 + e1 * (x87 - s.roe * x97);
-      const TB x1820 = s.QC[2] * x1147 + s.WQ[2] * x1148 + e2 * (x524 - s.roe * x525) + r1 * x1124;
-      const TB x1185 = s.QC[2] * x575 + s.WQ[2] * x576 + e1 * (x140 - s.roe * x155) + r2 * x524;
-      acc176 += static_cast<double>(x1185);
-      const TB x985 = s.QC[1] * x440 + s.WQ[1] * x441 + e1 * (x140 - s.roe * x155);
-      acc114 += static_cast<double>(x985);
-      const TB x1186 = s.QC[2] * x576 + s.WQ[2] * x577 + e1 * (x155 - s.roe * x170) + r2 * x525;
-      const TB x1846 = s.QC[2] * x1186 + s.WQ[2] * x1187 + e2 * (x576 - s.roe * x577) + r2 * x1148;
-      const TB x1845 = s.QC[2] * x1185 + s.WQ[2] * x1186 + e2 * (x575 - s.roe * x576) + r2 * x1147;
-      acc486 += static_cast<double>(x1845);
-      const TB x2320 = s.QC[2] * x1845 + s.WQ[2] * x1846 + e3 * (x1185 - s.roe * x1186) + r2 * x1820;
-      acc951 += static_cast<double>(x2320);
-      const TB x986 = s.QC[1] * x441 + s.WQ[1] * x442 + e1 * (x155 - s.roe * x170);
-      const TB x579 = s.QC[2] * x141 + s.WQ[2] * x156;
-      const TB x580 = s.QC[2] * x156 + s.WQ[2] * x171;
-      const TB x581 = s.QC[2] * x171 + s.WQ[2] * x186;
-      const TB x1190 = s.QC[2] * x581 + s.WQ[2] * x582 + e1 * (x171 - s.roe * x186);
-      const TB x624 = s.QC[0] * x223 + s.WQ[0] * x224 + e1 * (x45 - s.roe * x51) + r2 * x214;
-      const TB x924 = s.QC[1] * x358 + s.WQ[1] * x359 + e1 * (x45 - s.roe * x51);
-      const TB x727 = s.QC[1] * x227 + s.WQ[1] * x228 + r1 * x214;
-      const TB x644 = s.QC[0] * x250 + s.WQ[0] * x251 + e1 * (x86 - s.roe * x96) + r2 * x227;
-      const TB x750 = s.QC[1] * x258 + s.WQ[1] * x259 + r2 * x227;
-      const TB x645 = s.QC[0] * x251 + s.WQ[0] * x252 + e1 * (x96 - s.roe * x106) + r2 * x228;
-      const TB x751 = s.QC[1] * x259 + s.WQ[1] * x260 + r2 * x228;
-      const TB x790 = s.QC[1] * x311 + s.WQ[1] * x312 + r3 * x260;
-      const TB x890 = s.QC[2] * x311 + s.WQ[2] * x312;
-      const TB x945 = s.QC[1] * x386 + s.WQ[1] * x387 + e1 * (x96 - s.roe * x106) + r1 * x359;
-      const TB x1024 = s.QC[2] * x358 + s.WQ[2] * x359;
-      const TB x1045 = s.QC[2] * x386 + s.WQ[2] * x387;
-      const TB x1432 = s.QC[1] * x750 + s.WQ[1] * x751 + e1 * (x258 - s.roe * x259) + r2 * x727;
-      const TB x788 = s.QC[1] * x309 + s.WQ[1] * x310 + r3 * x258;
-      acc53 += static_cast<double>(x788);
-      const TB x789 = s.QC[1] * x310 + s.WQ[1] * x311 + r3 * x259;
-      const TB x1457 = s.QC[1] * x788 + s.WQ[1] * x789 + e1 * (x309 - s.roe * x310) + r3 * x750;
-      acc301 += static_cast<double>(x1457);
-      const TB x1458 = s.QC[1] * x789 + s.WQ[1] * x790 + e1 * (x310 - s.roe * x311) + r3 * x751;
-      const TB x2073 = s.QC[1] * x1457 + s.WQ[1] * x1458 + e2 * (x788 - s.roe * x789) + r3 * x1432;
-      acc704 += static_cast<double>(x2073);
-      const TB x2104 = s.QC[2] * x1457 + s.WQ[2] * x1458;
-      acc735 += static_cast<double>(x2104);
-      const TB x1523 = s.QC[2] * x789 + s.WQ[2] * x790;
-      const TB x688 = s.QC[0] * x309 + s.WQ[0] * x310 + e1 * (x141 - s.roe * x156) + r1 * x270;
-      acc22 += static_cast<double>(x688);
-      const TB x1188 = s.QC[2] * x579 + s.WQ[2] * x580 + e1 * (x141 - s.roe * x156);
-      acc177 += static_cast<double>(x1188);
-      const TB x1243 = s.QC[0] * x659 + s.WQ[0] * x660 + e2 * (x270 - s.roe * x271);
-      const TB x689 = s.QC[0] * x310 + s.WQ[0] * x311 + e1 * (x156 - s.roe * x171) + r1 * x271;
-      const TB x1189 = s.QC[2] * x580 + s.WQ[2] * x581 + e1 * (x156 - s.roe * x171);
-      const TB x1848 = s.QC[2] * x1189 + s.WQ[2] * x1190 + e2 * (x580 - s.roe * x581);
-      const TB x1847 = s.QC[2] * x1188 + s.WQ[2] * x1189 + e2 * (x579 - s.roe * x580);
-      acc487 += static_cast<double>(x1847);
-      const TB x2321 = s.QC[2] * x1847 + s.WQ[2] * x1848 + e3 * (x1188 - s.roe * x1189);
-      acc952 += static_cast<double>(x2321);
-      const TB x1262 = s.QC[0] * x688 + s.WQ[0] * x689 + e2 * (x309 - s.roe * x310) + r1 * x659;
-      acc208 += static_cast<double>(x1262);
-      const TB x1263 = s.QC[0] * x689 + s.WQ[0] * x690 + e2 * (x310 - s.roe * x311) + r1 * x660;
-      const TB x1887 = s.QC[0] * x1262 + s.WQ[0] * x1263 + e3 * (x688 - s.roe * x689) + r1 * x1243;
-      acc518 += static_cast<double>(x1887);
-      const TB x1918 = s.QC[1] * x1262 + s.WQ[1] * x1263 + r3 * x1237;
-      acc549 += static_cast<double>(x1918);
-      const TB x1949 = s.QC[2] * x1262 + s.WQ[2] * x1263;
-      acc580 += static_cast<double>(x1949);
-      const TB x1327 = s.QC[1] * x688 + s.WQ[1] * x689 + r3 * x650;
-      acc239 += static_cast<double>(x1327);
-      const TB x1328 = s.QC[1] * x689 + s.WQ[1] * x690 + r3 * x651;
-      const TB x1980 = s.QC[1] * x1327 + s.WQ[1] * x1328 + e1 * (x688 - s.roe * x689) + r3 * x1302;
-      acc611 += static_cast<double>(x1980);
-      const TB x2011 = s.QC[2] * x1327 + s.WQ[2] * x1328;
-      acc642 += static_cast<double>(x2011);
-      const TB x1393 = s.QC[2] * x689 + s.WQ[2] * x690;
-      const TB x724 = s.QC[1] * x223 + s.WQ[1] * x224;
-      const TB x744 = s.QC[1] * x250 + s.WQ[1] * x251 + r1 * x223;
-      const TB x745 = s.QC[1] * x251 + s.WQ[1] * x252 + r1 * x224;
-      const TB x747 = s.QC[1] * x254 + s.WQ[1] * x255;
-      const TB x748 = s.QC[1] * x255 + s.WQ[1] * x256;
-      const TB x782 = s.QC[1] * x301 + s.WQ[1] * x302 + r1 * x254;
-      const TB x783 = s.QC[1] * x302 + s.WQ[1] * x303 + r1 * x255;
-      const TB x784 = s.QC[1] * x303 + s.WQ[1] * x304 + r1 * x256;
-      const TB x884 = s.QC[2] * x303 + s.WQ[2] * x304 + r1 * x252;
-      const TB x845 = s.QC[2] * x251 + s.WQ[2] * x252;
-      const TB x785 = s.QC[1] * x305 + s.WQ[1] * x306;
-      acc52 += static_cast<double>(x785);
-      const TB x786 = s.QC[1] * x306 + s.WQ[1] * x307;
-      const TB x787 = s.QC[1] * x307 + s.WQ[1] * x308;
-      const TB x887 = s.QC[2] * x307 + s.WQ[2] * x308 + r2 * x256;
-      const TB x848 = s.QC[2] * x255 + s.WQ[2] * x256 + r1 * x224;
-      const TB x824 = s.QC[2] * x223 + s.WQ[2] * x224;
-      const TB x847 = s.QC[2] * x254 + s.WQ[2] * x255 + r1 * x223;
-      const TB x1560 = s.QC[2] * x847 + s.WQ[2] * x848 + e1 * (x254 - s.roe * x255) + r1 * x824;
-      const TB x844 = s.QC[2] * x250 + s.WQ[2] * x251;
-      const TB x882 = s.QC[2] * x301 + s.WQ[2] * x302 + r1 * x250;
-      const TB x883 = s.QC[2] * x302 + s.WQ[2] * x303 + r1 * x251;
-      const TB x1584 = s.QC[2] * x883 + s.WQ[2] * x884 + e1 * (x302 - s.roe * x303) + r1 * x845;
-      const TB x1454 = s.QC[1] * x783 + s.WQ[1] * x784 + e1 * (x302 - s.roe * x303) + r1 * x748;
-      const TB x1519 = s.QC[2] * x783 + s.WQ[2] * x784 + r1 * x745;
-      const TB x1558 = s.QC[2] * x844 + s.WQ[2] * x845 + e1 * (x250 - s.roe * x251);
-      const TB x1428 = s.QC[1] * x744 + s.WQ[1] * x745 + e1 * (x250 - s.roe * x251) + r1 * x724;
-      const TB x1493 = s.QC[2] * x744 + s.WQ[2] * x745;
-      const TB x1495 = s.QC[2] * x747 + s.WQ[2] * x748 + r1 * x724;
-      const TB x1518 = s.QC[2] * x782 + s.WQ[2] * x783 + r1 * x744;
-      const TB x2133 = s.QC[2] * x1518 + s.WQ[2] * x1519 + e1 * (x782 - s.roe * x783) + r1 * x1493;
-      acc764 += static_cast<double>(x2133);
-      const TB x1583 = s.QC[2] * x882 + s.WQ[2] * x883 + e1 * (x301 - s.roe * x302) + r1 * x844;
-      const TB x2164 = s.QC[2] * x1583 + s.WQ[2] * x1584 + e2 * (x882 - s.roe * x883) + r1 * x1558;
-      acc795 += static_cast<double>(x2164);
-      const TB x1453 = s.QC[1] * x782 + s.WQ[1] * x783 + e1 * (x301 - s.roe * x302) + r1 * x747;
-      const TB x2102 = s.QC[2] * x1453 + s.WQ[2] * x1454 + r1 * x1428;
-      acc733 += static_cast<double>(x2102);
-      const TB x885 = s.QC[2] * x305 + s.WQ[2] * x306 + r2 * x254;
-      acc83 += static_cast<double>(x885);
-      const TB x1430 = s.QC[1] * x747 + s.WQ[1] * x748 + e1 * (x254 - s.roe * x255);
-      const TB x2071 = s.QC[1] * x1453 + s.WQ[1] * x1454 + e2 * (x782 - s.roe * x783) + r1 * x1430;
-      acc702 += static_cast<double>(x2071);
-      const TB x886 = s.QC[2] * x306 + s.WQ[2] * x307 + r2 * x255;
-      const TB x1586 = s.QC[2] * x886 + s.WQ[2] * x887 + e1 * (x306 - s.roe * x307) + r2 * x848;
-      const TB x1456 = s.QC[1] * x786 + s.WQ[1] * x787 + e1 * (x306 - s.roe * x307);
-      const TB x1521 = s.QC[2] * x786 + s.WQ[2] * x787 + r2 * x748;
-      const TB x1520 = s.QC[2] * x785 + s.WQ[2] * x786 + r2 * x747;
-      acc331 += static_cast<double>(x1520);
-      const TB x2134 = s.QC[2] * x1520 + s.WQ[2] * x1521 + e1 * (x785 - s.roe * x786) + r2 * x1495;
-      acc765 += static_cast<double>(x2134);
-      const TB x1585 = s.QC[2] * x885 + s.WQ[2] * x886 + e1 * (x305 - s.roe * x306) + r2 * x847;
-      acc362 += static_cast<double>(x1585);
-      const TB x2165 = s.QC[2] * x1585 + s.WQ[2] * x1586 + e2 * (x885 - s.roe * x886) + r2 * x1560;
-      acc796 += static_cast<double>(x2165);
-      const TB x1455 = s.QC[1] * x785 + s.WQ[1] * x786 + e1 * (x305 - s.roe * x306);
-      acc300 += static_cast<double>(x1455);
-      const TB x2072 = s.QC[1] * x1455 + s.WQ[1] * x1456 + e2 * (x785 - s.roe * x786);
-      acc703 += static_cast<double>(x2072);
-      const TB x2103 = s.QC[2] * x1455 + s.WQ[2] * x1456 + r2 * x1430;
-      acc734 += static_cast<double>(x2103);
-      const TB x888 = s.QC[2] * x309 + s.WQ[2] * x310;
-      acc84 += static_cast<double>(x888);
-      const TB x889 = s.QC[2] * x310 + s.WQ[2] * x311;
-      const TB x1588 = s.QC[2] * x889 + s.WQ[2] * x890 + e1 * (x310 - s.roe * x311);
-      const TB x1587 = s.QC[2] * x888 + s.WQ[2] * x889 + e1 * (x309 - s.roe * x310);
-      acc363 += static_cast<double>(x1587);
-      const TB x2166 = s.QC[2] * x1587 + s.WQ[2] * x1588 + e2 * (x888 - s.roe * x889);
-      acc797 += static_cast<double>(x2166);
-      const TB x944 = s.QC[1] * x385 + s.WQ[1] * x386 + e1 * (x86 - s.roe * x96) + r1 * x358;
-      const TB x1144 = s.QC[2] * x520 + s.WQ[2] * x521 + e1 * (x86 - s.roe * x96);
-      const TB x1047 = s.QC[2] * x389 + s.WQ[2] * x390 + r1 * x358;
-      const TB x1755 = s.QC[2] * x1047 + s.WQ[2] * x1048 + e1 * (x389 - s.roe * x390) + r1 * x1024;
-      const TB x982 = s.QC[1] * x436 + s.WQ[1] * x437 + e1 * (x139 - s.roe * x154) + r1 * x389;
-      const TB x1182 = s.QC[2] * x571 + s.WQ[2] * x572 + e1 * (x139 - s.roe * x154) + r1 * x520;
-      const TB x1818 = s.QC[2] * x1144 + s.WQ[2] * x1145 + e2 * (x520 - s.roe * x521);
-      const TB x1183 = s.QC[2] * x572 + s.WQ[2] * x573 + e1 * (x154 - s.roe * x169) + r1 * x521;
-      const TB x1844 = s.QC[2] * x1183 + s.WQ[2] * x1184 + e2 * (x572 - s.roe * x573) + r1 * x1145;
-      const TB x1843 = s.QC[2] * x1182 + s.WQ[2] * x1183 + e2 * (x571 - s.roe * x572) + r1 * x1144;
-      const TB x2319 = s.QC[2] * x1843 + s.WQ[2] * x1844 + e3 * (x1182 - s.roe * x1183) + r1 * x1818;
-      acc950 += static_cast<double>(x2319);
-      const TB x983 = s.QC[1] * x437 + s.WQ[1] * x438 + e1 * (x154 - s.roe * x169) + r1 * x390;
-      const TB x1044 = s.QC[2] * x385 + s.WQ[2] * x386;
-      const TB x1082 = s.QC[2] * x436 + s.WQ[2] * x437 + r1 * x385;
-      const TB x1083 = s.QC[2] * x437 + s.WQ[2] * x438 + r1 * x386;
-      const TB x1779 = s.QC[2] * x1083 + s.WQ[2] * x1084 + e1 * (x437 - s.roe * x438) + r1 * x1045;
-      const TB x1649 = s.QC[1] * x983 + s.WQ[1] * x984 + e2 * (x437 - s.roe * x438) + r1 * x948;
-      const TB x1714 = s.QC[2] * x983 + s.WQ[2] * x984 + r1 * x945;
-      const TB x1753 = s.QC[2] * x1044 + s.WQ[2] * x1045 + e1 * (x385 - s.roe * x386);
-      const TB x1623 = s.QC[1] * x944 + s.WQ[1] * x945 + e2 * (x385 - s.roe * x386) + r1 * x924;
-      const TB x1688 = s.QC[2] * x944 + s.WQ[2] * x945;
-      const TB x1690 = s.QC[2] * x947 + s.WQ[2] * x948 + r1 * x924;
-      const TB x1713 = s.QC[2] * x982 + s.WQ[2] * x983 + r1 * x944;
-      const TB x2257 = s.QC[2] * x1713 + s.WQ[2] * x1714 + e1 * (x982 - s.roe * x983) + r1 * x1688;
-      acc888 += static_cast<double>(x2257);
-      const TB x1778 = s.QC[2] * x1082 + s.WQ[2] * x1083 + e1 * (x436 - s.roe * x437) + r1 * x1044;
-      const TB x2288 = s.QC[2] * x1778 + s.WQ[2] * x1779 + e2 * (x1082 - s.roe * x1083) + r1 * x1753;
-      acc919 += static_cast<double>(x2288);
-      const TB x1648 = s.QC[1] * x982 + s.WQ[1] * x983 + e2 * (x436 - s.roe * x437) + r1 * x947;
-      const TB x2226 = s.QC[2] * x1648 + s.WQ[2] * x1649 + r1 * x1623;
-      acc857 += static_cast<double>(x2226);
-      const TB x1085 = s.QC[2] * x440 + s.WQ[2] * x441 + r2 * x389;
-      acc145 += static_cast<double>(x1085);
-      const TB x1625 = s.QC[1] * x947 + s.WQ[1] * x948 + e2 * (x389 - s.roe * x390);
-      const TB x2195 = s.QC[1] * x1648 + s.WQ[1] * x1649 + e3 * (x982 - s.roe * x983) + r1 * x1625;
-      acc826 += static_cast<double>(x2195);
-      const TB x1086 = s.QC[2] * x441 + s.WQ[2] * x442 + r2 * x390;
-      const TB x1781 = s.QC[2] * x1086 + s.WQ[2] * x1087 + e1 * (x441 - s.roe * x442) + r2 * x1048;
-      const TB x1651 = s.QC[1] * x986 + s.WQ[1] * x987 + e2 * (x441 - s.roe * x442);
-      const TB x1716 = s.QC[2] * x986 + s.WQ[2] * x987 + r2 * x948;
-      const TB x1715 = s.QC[2] * x985 + s.WQ[2] * x986 + r2 * x947;
-      acc424 += static_cast<double>(x1715);
-      const TB x2258 = s.QC[2] * x1715 + s.WQ[2] * x1716 + e1 * (x985 - s.roe * x986) + r2 * x1690;
-      acc889 += static_cast<double>(x2258);
-      const TB x1780 = s.QC[2] * x1085 + s.WQ[2] * x1086 + e1 * (x440 - s.roe * x441) + r2 * x1047;
-      acc455 += static_cast<double>(x1780);
-      const TB x2289 = s.QC[2] * x1780 + s.WQ[2] * x1781 + e2 * (x1085 - s.roe * x1086) + r2 * x1755;
-      acc920 += static_cast<double>(x2289);
-      const TB x1650 = s.QC[1] * x985 + s.WQ[1] * x986 + e2 * (x440 - s.roe * x441);
-      acc393 += static_cast<double>(x1650);
-      const TB x2196 = s.QC[1] * x1650 + s.WQ[1] * x1651 + e3 * (x985 - s.roe * x986);
-      acc827 += static_cast<double>(x2196);
-      const TB x2227 = s.QC[2] * x1650 + s.WQ[2] * x1651 + r2 * x1625;
-      acc858 += static_cast<double>(x2227);
-      const TB x1088 = s.QC[2] * x444 + s.WQ[2] * x445;
-      acc146 += static_cast<double>(x1088);
-      const TB x1089 = s.QC[2] * x445 + s.WQ[2] * x446;
-      const TB x1783 = s.QC[2] * x1089 + s.WQ[2] * x1090 + e1 * (x445 - s.roe * x446);
-      const TB x1782 = s.QC[2] * x1088 + s.WQ[2] * x1089 + e1 * (x444 - s.roe * x445);
-      acc456 += static_cast<double>(x1782);
-      const TB x2290 = s.QC[2] * x1782 + s.WQ[2] * x1783 + e2 * (x1088 - s.roe * x1089);
-      acc921 += static_cast<double>(x2290);
-      const TB x1298 = s.QC[1] * x644 + s.WQ[1] * x645 + r1 * x624;
-      const TB x1365 = s.QC[2] * x647 + s.WQ[2] * x648 + r1 * x624;
-      const TB x1300 = s.QC[1] * x647 + s.WQ[1] * x648;
-      const TB x1323 = s.QC[1] * x682 + s.WQ[1] * x683 + r1 * x647;
-      const TB x1390 = s.QC[2] * x685 + s.WQ[2] * x686 + r2 * x647;
-      acc269 += static_cast<double>(x1390);
-      const TB x1324 = s.QC[1] * x683 + s.WQ[1] * x684 + r1 * x648;
-      const TB x2009 = s.QC[2] * x1323 + s.WQ[2] * x1324 + r1 * x1298;
-      acc640 += static_cast<double>(x2009);
-      const TB x1389 = s.QC[2] * x683 + s.WQ[2] * x684 + r1 * x645;
-      const TB x1363 = s.QC[2] * x644 + s.WQ[2] * x645;
-      const TB x1388 = s.QC[2] * x682 + s.WQ[2] * x683 + r1 * x644;
-      const TB x2040 = s.QC[2] * x1388 + s.WQ[2] * x1389 + e1 * (x682 - s.roe * x683) + r1 * x1363;
-      acc671 += static_cast<double>(x2040);
-      const TB x1391 = s.QC[2] * x686 + s.WQ[2] * x687 + r2 * x648;
-      const TB x2041 = s.QC[2] * x1390 + s.WQ[2] * x1391 + e1 * (x685 - s.roe * x686) + r2 * x1365;
-      acc672 += static_cast<double>(x2041);
-      const TB x1326 = s.QC[1] * x686 + s.WQ[1] * x687;
-      const TB x1325 = s.QC[1] * x685 + s.WQ[1] * x686;
-      acc238 += static_cast<double>(x1325);
-      const TB x1979 = s.QC[1] * x1325 + s.WQ[1] * x1326 + e1 * (x685 - s.roe * x686);
-      acc610 += static_cast<double>(x1979);
-      const TB x2010 = s.QC[2] * x1325 + s.WQ[2] * x1326 + r2 * x1300;
-      acc641 += static_cast<double>(x2010);
-      const TB x1392 = s.QC[2] * x688 + s.WQ[2] * x689;
-      acc270 += static_cast<double>(x1392);
-      const TB x2042 = s.QC[2] * x1392 + s.WQ[2] * x1393 + e1 * (x688 - s.roe * x689);
-      acc673 += static_cast<double>(x2042);
-      const TB x1522 = s.QC[2] * x788 + s.WQ[2] * x789;
-      acc332 += static_cast<double>(x1522);
-      const TB x2135 = s.QC[2] * x1522 + s.WQ[2] * x1523 + e1 * (x788 - s.roe * x789);
-      acc766 += static_cast<double>(x2135);
-      const TB x1717 = s.QC[2] * x988 + s.WQ[2] * x989;
-      acc425 += static_cast<double>(x1717);
-      const TB x2259 = s.QC[2] * x1717 + s.WQ[2] * x1718 + e1 * (x988 - s.roe * x989);
-      acc890 += static_cast<double>(x2259);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(640 * R + r) * a.acc_stride + q] = acc640;
-  a.acc[(671 * R + r) * a.acc_stride + q] = acc671;
-  a.acc[(702 * R + r) * a.acc_stride + q] = acc702;
-  a.acc[(733 * R + r) * a.acc_stride + q] = acc733;
-  a.acc[(764 * R + r) * a.acc_stride + q] = acc764;
-  a.acc[(795 * R + r) * a.acc_stride + q] = acc795;
-  a.acc[(826 * R + r) * a.acc_stride + q] = acc826;
-  a.acc[(857 * R + r) * a.acc_stride + q] = acc857;
-  a.acc[(888 * R + r) * a.acc_stride + q] = acc888;
-  a.acc[(919 * R + r) * a.acc_stride + q] = acc919;
-  a.acc[(950 * R + r) * a.acc_stride + q] = acc950;
-  a.acc[(21 * R + r) * a.acc_stride + q] = acc21;
-  a.acc[(52 * R + r) * a.acc_stride + q] = acc52;
-  a.acc[(83 * R + r) * a.acc_stride + q] = acc83;
-  a.acc[(114 * R + r) * a.acc_stride + q] = acc114;
-  a.acc[(145 * R + r) * a.acc_stride + q] = acc145;
-  a.acc[(176 * R + r) * a.acc_stride + q] = acc176;
-  a.acc[(207 * R + r) * a.acc_stride + q] = acc207;
-  a.acc[(238 * R + r) * a.acc_stride + q] = acc238;
-  a.acc[(269 * R + r) * a.acc_stride + q] = acc269;
-  a.acc[(300 * R + r) * a.acc_stride + q] = acc300;
-  a.acc[(331 * R + r) * a.acc_stride + q] = acc331;
-  a.acc[(362 * R + r) * a.acc_stride + q] = acc362;
-  a.acc[(393 * R + r) * a.acc_stride + q] = acc393;
-  a.acc[(424 * R + r) * a.acc_stride + q] = acc424;
-  a.acc[(455 * R + r) * a.acc_stride + q] = acc455;
-  a.acc[(486 * R + r) * a.acc_stride + q] = acc486;
-  a.acc[(517 * R + r) * a.acc_stride + q] = acc517;
-  a.acc[(548 * R + r) * a.acc_stride + q] = acc548;
-  a.acc[(579 * R + r) * a.acc_stride + q] = acc579;
-  a.acc[(610 * R + r) * a.acc_stride + q] = acc610;
-  a.acc[(641 * R + r) * a.acc_stride + q] = acc641;
-  a.acc[(672 * R + r) * a.acc_stride + q] = acc672;
-  a.acc[(703 * R + r) * a.acc_stride + q] = acc703;
-  a.acc[(734 * R + r) * a.acc_stride + q] = acc734;
-  a.acc[(765 * R + r) * a.acc_stride + q] = acc765;
-  a.acc[(796 * R + r) * a.acc_stride + q] = acc796;
-  a.acc[(827 * R + r) * a.acc_stride + q] = acc827;
-  a.acc[(858 * R + r) * a.acc_stride + q] = acc858;
-  a.acc[(889 * R + r) * a.acc_stride + q] = acc889;
-  a.acc[(920 * R + r) * a.acc_stride + q] = acc920;
-  a.acc[(951 * R + r) * a.acc_stride + q] = acc951;
-  a.acc[(22 * R + r) * a.acc_stride + q] = acc22;
-  a.acc[(53 * R + r) * a.acc_stride + q] = acc53;
-  a.acc[(84 * R + r) * a.acc_stride + q] = acc84;
-  a.acc[(115 * R + r) * a.acc_stride + q] = acc115;
-  a.acc[(146 * R + r) * a.acc_stride + q] = acc146;
-  a.acc[(177 * R + r) * a.acc_stride + q] = acc177;
-  a.acc[(208 * R + r) * a.acc_stride + q] = acc208;
-  a.acc[(239 * R + r) * a.acc_stride + q] = acc239;
-  a.acc[(270 * R + r) * a.acc_stride + q] = acc270;
-  a.acc[(301 * R + r) * a.acc_stride + q] = acc301;
-  a.acc[(332 * R + r) * a.acc_stride + q] = acc332;
-  a.acc[(363 * R + r) * a.acc_stride + q] = acc363;
-  a.acc[(394 * R + r) * a.acc_stride + q] = acc394;
-  a.acc[(425 * R + r) * a.acc_stride + q] = acc425;
-  a.acc[(456 * R + r) * a.acc_stride + q] = acc456;
-  a.acc[(487 * R + r) * a.acc_stride + q] = acc487;
-  a.acc[(518 * R + r) * a.acc_stride + q] = acc518;
-  a.acc[(549 * R + r) * a.acc_stride + q] = acc549;
-  a.acc[(580 * R + r) * a.acc_stride + q] = acc580;
-  a.acc[(611 * R + r) * a.acc_stride + q] = acc611;
-  a.acc[(642 * R + r) * a.acc_stride + q] = acc642;
-  a.acc[(673 * R + r) * a.acc_stride + q] = acc673;
-  a.acc[(704 * R + r) * a.acc_stride + q] = acc704;
-  a.acc[(735 * R + r) * a.acc_stride + q] = acc735;
-  a.acc[(766 * R + r) * a.acc_stride + q] = acc766;
-  a.acc[(797 * R + r) * a.acc_stride + q] = acc797;
-  a.acc[(828 * R + r) * a.acc_stride + q] = acc828;
-  a.acc[(859 * R + r) * a.acc_stride + q] = acc859;
-  a.acc[(890 * R + r) * a.acc_stride + q] = acc890;
-  a.acc[(921 * R + r) * a.acc_stride + q] = acc921;
-  a.acc[(952 * R + r) * a.acc_stride + q] = acc952;
-  a.acc[(23 * R + r) * a.acc_stride + q] = acc23;
-  a.acc[(54 * R + r) * a.acc_stride + q] = acc54;
-  a.acc[(85 * R + r) * a.acc_stride + q] = acc85;
-  a.acc[(116 * R + r) * a.acc_stride + q] = acc116;
-  a.acc[(147 * R + r) * a.acc_stride + q] = acc147;
-  a.acc[(178 * R + r) * a.acc_stride + q] = acc178;
-  a.acc[(209 * R + r) * a.acc_stride + q] = acc209;
-}
-// class (22|22) split 9/12: 408 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_9(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc240 = 0.0;
-  double acc271 = 0.0;
-  double acc302 = 0.0;
-  double acc333 = 0.0;
-  double acc364 = 0.0;
-  double acc395 = 0.0;
-  double acc426 = 0.0;
-  double acc457 = 0.0;
-  double acc488 = 0.0;
-  double acc519 = 0.0;
-  double acc550 = 0.0;
-  double acc581 = 0.0;
-  double acc612 = 0.0;
-  double acc643 = 0.0;
-  double acc674 = 0.0;
-  double acc705 = 0.0;
-  double acc736 = 0.0;
-  double acc767 = 0.0;
-  double acc798 = 0.0;
-  double acc829 = 0.0;
-  double acc860 = 0.0;
-  double acc891 = 0.0;
-  double acc922 = 0.0;
-  double acc953 = 0.0;
-  double acc24 = 0.0;
-  double acc55 = 0.0;
-  double acc86 = 0.0;
-  double acc117 = 0.0;
-  double acc148 = 0.0;
-  double acc179 = 0.0;
-  double acc210 = 0.0;
-  double acc241 = 0.0;
-  double acc272 = 0.0;
-  double acc303 = 0.0;
-  double acc334 = 0.0;
-  double acc365 = 0.0;
-  double acc396 = 0.0;
-  double acc427 = 0.0;
-  double acc458 = 0.0;
-  double acc489 = 0.0;
-  double acc520 = 0.0;
-  double acc551 = 0.0;
-  double acc582 = 0.0;
-  double acc613 = 0.0;
-  double acc644 = 0.0;
-  double acc675 = 0.0;
-  double acc706 = 0.0;
-  double acc737 = 0.0;
-  double acc768 = 0.0;
-  double acc799 = 0.0;
-  double acc830 = 0.0;
-  double acc861 = 0.0;
-  double acc892 = 0.0;
-  double acc923 = 0.0;
-  double acc954 = 0.0;
-  double acc25 = 0.0;
-  double acc56 = 0.0;
-  double acc87 = 0.0;
-  double acc118 = 0.0;
-  double acc149 = 0.0;
-  double acc180 = 0.0;
-  double acc211 = 0.0;
-  double acc242 = 0.0;
-  double acc273 = 0.0;
-  double acc304 = 0.0;
-  double acc335 = 0.0;
-  double acc366 = 0.0;
-  double acc397 = 0.0;
-  double acc428 = 0.0;
-  double acc459 = 0.0;
-  double acc490 = 0.0;
-  double acc521 = 0.0;
-  double acc552 = 0.0;
-  double acc583 = 0.0;
-  double acc614 = 0.0;
-  double acc645 = 0.0;
-  double acc676 = 0.0;
-  double acc707 = 0.0;
-  double acc738 = 0.0;
-  double acc769 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB x8 = static_cast<TB>(F[8] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x30 = s.PA[0] * x7 + s.WP[0] * x8;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x28 = s.PA[1] * x6 + s.WP[1] * x7;
-      const TB x29 = s.PA[2] * x6 + s.WP[2] * x7;
-      const TB x68 = s.PA[2] * x26 + s.WP[2] * x29 + z1 * (x5 - s.roz * x6);
-      const TB x66 = s.PA[1] * x25 + s.WP[1] * x28 + z1 * (x5 - s.roz * x6);
-      const TB x67 = s.PA[2] * x25 + s.WP[2] * x28;
-      const TB x34 = s.PA[1] * x9 + s.WP[1] * x12;
-      const TB x35 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x40 = s.PA[1] * x12 + s.WP[1] * x15;
-      const TB x41 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x43 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x46 = s.PA[1] * x15 + s.WP[1] * x18;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x94 = s.PA[2] * x44 + s.WP[2] * x50 + z2 * (x14 - s.roz * x17);
-      const TB x92 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x93 = s.PA[2] * x43 + s.WP[2] * x49 + z1 * (x13 - s.roz * x16);
-      const TB x52 = s.PA[1] * x18 + s.WP[1] * x21;
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x104 = s.PA[2] * x50 + s.WP[2] * x56 + z2 * (x17 - s.roz * x20);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x103 = s.PA[2] * x49 + s.WP[2] * x55 + z1 * (x16 - s.roz * x19);
-      const TB x58 = s.PA[1] * x21 + s.WP[1] * x24;
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x124 = s.PA[2] * x62 + s.WP[2] * x68 + z2 * (x23 - s.roz * x26);
-      const TB x122 = s.PA[2] * x60 + s.WP[2] * x66;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x123 = s.PA[2] * x61 + s.WP[2] * x67 + z1 * (x22 - s.roz * x25);
-      const TB x64 = s.PA[1] * x24 + s.WP[1] * x27;
-      const TB x65 = s.PA[2] * x24 + s.WP[2] * x27;
-      const TB x70 = s.PA[1] * x27 + s.WP[1] * x30;
-      const TB x71 = s.PA[2] * x27 + s.WP[2] * x30;
-      const TB x130 = s.PA[2] * x65 + s.WP[2] * x71 + z1 * (x24 - s.roz * x27);
-      const TB x128 = s.PA[1] * x64 + s.WP[1] * x70 + z1 * (x24 - s.roz * x27);
-      const TB x129 = s.PA[2] * x64 + s.WP[2] * x70;
-      const TB x78 = s.PA[1] * x34 + s.WP[1] * x40 + z1 * (x9 - s.roz * x12);
-      const TB x80 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x9 - s.roz * x12);
-      const TB x79 = s.PA[2] * x34 + s.WP[2] * x40;
-      const TB x88 = s.PA[1] * x40 + s.WP[1] * x46 + z1 * (x12 - s.roz * x15);
-      const TB x90 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x12 - s.roz * x15);
-      const TB x144 = s.PA[2] * x80 + s.WP[2] * x90 + z2 * (x35 - s.roz * x41);
-      const TB x142 = s.PA[2] * x78 + s.WP[2] * x88;
-      const TB x89 = s.PA[2] * x40 + s.WP[2] * x46;
-      const TB x143 = s.PA[2] * x79 + s.WP[2] * x89 + z1 * (x34 - s.roz * x40);
-      const TB x98 = s.PA[1] * x46 + s.WP[1] * x52 + z1 * (x15 - s.roz * x18);
-      const TB x100 = s.PA[2] * x47 + s.WP[2] * x53 + z1 * (x15 - s.roz * x18);
-      const TB x159 = s.PA[2] * x90 + s.WP[2] * x100 + z2 * (x41 - s.roz * x47);
-      const TB x99 = s.PA[2] * x46 + s.WP[2] * x52;
-      const TB x158 = s.PA[2] * x89 + s.WP[2] * x99 + z1 * (x40 - s.roz * x46);
-      const TB x102 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x108 = s.PA[1] * x52 + s.WP[1] * x58 + z1 * (x18 - s.roz * x21);
-      const TB x109 = s.PA[2] * x52 + s.WP[2] * x58;
-      const TB x110 = s.PA[2] * x53 + s.WP[2] * x59 + z1 * (x18 - s.roz * x21);
-      const TB x112 = s.PA[2] * x54 + s.WP[2] * x60;
-      const TB x113 = s.PA[2] * x55 + s.WP[2] * x61 + z1 * (x19 - s.roz * x22);
-      const TB x114 = s.PA[2] * x56 + s.WP[2] * x62 + z2 * (x20 - s.roz * x23);
-      const TB x118 = s.PA[1] * x58 + s.WP[1] * x64 + z1 * (x21 - s.roz * x24);
-      const TB x120 = s.PA[2] * x59 + s.WP[2] * x65 + z1 * (x21 - s.roz * x24);
-      const TB x204 = s.PA[2] * x120 + s.WP[2] * x130 + z2 * (x59 - s.roz * x65);
-      const TB x202 = s.PA[2] * x118 + s.WP[2] * x128;
-      const TB x119 = s.PA[2] * x58 + s.WP[2] * x64;
-      const TB x203 = s.PA[2] * x119 + s.WP[2] * x129 + z1 * (x58 - s.roz * x64);
-      const TB x157 = s.PA[2] * x88 + s.WP[2] * x98;
-      const TB x172 = s.PA[2] * x98 + s.WP[2] * x108;
-      const TB x173 = s.PA[2] * x99 + s.WP[2] * x109 + z1 * (x46 - s.roz * x52);
-      const TB x174 = s.PA[2] * x100 + s.WP[2] * x110 + z2 * (x47 - s.roz * x53);
-      const TB x187 = s.PA[2] * x108 + s.WP[2] * x118;
-      const TB x188 = s.PA[2] * x109 + s.WP[2] * x119 + z1 * (x52 - s.roz * x58);
-      const TB x189 = s.PA[2] * x110 + s.WP[2] * x120 + z2 * (x53 - s.roz * x59);
-      const TB x324 = s.QC[0] * x189 + s.WQ[0] * x204 + r1 * x124;
-      const TB x284 = s.QC[0] * x114 + s.WQ[0] * x124;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x227 = s.QC[0] * x46 + s.WQ[0] * x52 + r1 * x19;
-      const TB x228 = s.QC[0] * x52 + s.WQ[0] * x58 + r1 * x22;
-      const TB x627 = s.QC[0] * x227 + s.WQ[0] * x228 + e1 * (x46 - s.roe * x52) + r1 * x217;
-      const TB x727 = s.QC[1] * x227 + s.WQ[1] * x228 + r1 * x214;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x630 = s.QC[0] * x231 + s.WQ[0] * x232 + e1 * (x47 - s.roe * x53) + r1 * x220;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x260 = s.QC[0] * x108 + s.WQ[0] * x118 + r1 * x60;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x240 = s.QC[0] * x55 + s.WQ[0] * x61;
-      const TB x264 = s.QC[0] * x109 + s.WQ[0] * x119 + r1 * x61;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x244 = s.QC[0] * x56 + s.WQ[0] * x62;
-      const TB x268 = s.QC[0] * x110 + s.WQ[0] * x120 + r1 * x62;
-      const TB x258 = s.QC[0] * x88 + s.WQ[0] * x98 + r1 * x48;
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x259 = s.QC[0] * x98 + s.WQ[0] * x108 + r1 * x54;
-      const TB x650 = s.QC[0] * x258 + s.WQ[0] * x259 + e1 * (x88 - s.roe * x98) + r1 * x235;
-      const TB x651 = s.QC[0] * x259 + s.WQ[0] * x260 + e1 * (x98 - s.roe * x108) + r1 * x236;
-      const TB x1237 = s.QC[0] * x650 + s.WQ[0] * x651 + e2 * (x258 - s.roe * x259) + r1 * x633;
-      const TB x262 = s.QC[0] * x89 + s.WQ[0] * x99 + r1 * x49;
-      const TB x636 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x49 - s.roe * x55);
-      const TB x263 = s.QC[0] * x99 + s.WQ[0] * x109 + r1 * x55;
-      const TB x653 = s.QC[0] * x262 + s.WQ[0] * x263 + e1 * (x89 - s.roe * x99) + r1 * x239;
-      const TB x654 = s.QC[0] * x263 + s.WQ[0] * x264 + e1 * (x99 - s.roe * x109) + r1 * x240;
-      const TB x1239 = s.QC[0] * x653 + s.WQ[0] * x654 + e2 * (x262 - s.roe * x263) + r1 * x636;
-      const TB x266 = s.QC[0] * x90 + s.WQ[0] * x100 + r1 * x50;
-      const TB x639 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x50 - s.roe * x56);
-      const TB x267 = s.QC[0] * x100 + s.WQ[0] * x110 + r1 * x56;
-      const TB x656 = s.QC[0] * x266 + s.WQ[0] * x267 + e1 * (x90 - s.roe * x100) + r1 * x243;
-      const TB x657 = s.QC[0] * x267 + s.WQ[0] * x268 + e1 * (x100 - s.roe * x110) + r1 * x244;
-      const TB x1241 = s.QC[0] * x656 + s.WQ[0] * x657 + e2 * (x266 - s.roe * x267) + r1 * x639;
-      const TB x274 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x275 = s.QC[0] * x102 + s.WQ[0] * x112;
-      const TB x276 = s.QC[0] * x112 + s.WQ[0] * x122;
-      const TB x316 = s.QC[0] * x187 + s.WQ[0] * x202 + r1 * x122;
-      const TB x278 = s.QC[0] * x93 + s.WQ[0] * x103;
-      const TB x279 = s.QC[0] * x103 + s.WQ[0] * x113;
-      const TB x280 = s.QC[0] * x113 + s.WQ[0] * x123;
-      const TB x320 = s.QC[0] * x188 + s.WQ[0] * x203 + r1 * x123;
-      const TB x282 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x283 = s.QC[0] * x104 + s.WQ[0] * x114;
-      const TB x313 = s.QC[0] * x142 + s.WQ[0] * x157 + r1 * x92;
-      const TB x662 = s.QC[0] * x274 + s.WQ[0] * x275 + e1 * (x92 - s.roe * x102);
-      const TB x314 = s.QC[0] * x157 + s.WQ[0] * x172 + r1 * x102;
-      const TB x663 = s.QC[0] * x275 + s.WQ[0] * x276 + e1 * (x102 - s.roe * x112);
-      const TB x315 = s.QC[0] * x172 + s.WQ[0] * x187 + r1 * x112;
-      const TB x693 = s.QC[0] * x315 + s.WQ[0] * x316 + e1 * (x172 - s.roe * x187) + r1 * x276;
-      const TB x317 = s.QC[0] * x143 + s.WQ[0] * x158 + r1 * x93;
-      const TB x665 = s.QC[0] * x278 + s.WQ[0] * x279 + e1 * (x93 - s.roe * x103);
-      const TB x318 = s.QC[0] * x158 + s.WQ[0] * x173 + r1 * x103;
-      const TB x666 = s.QC[0] * x279 + s.WQ[0] * x280 + e1 * (x103 - s.roe * x113);
-      const TB x319 = s.QC[0] * x173 + s.WQ[0] * x188 + r1 * x113;
-      const TB x696 = s.QC[0] * x319 + s.WQ[0] * x320 + e1 * (x173 - s.roe * x188) + r1 * x280;
-      const TB x321 = s.QC[0] * x144 + s.WQ[0] * x159 + r1 * x94;
-      const TB x668 = s.QC[0] * x282 + s.WQ[0] * x283 + e1 * (x94 - s.roe * x104);
-      const TB x322 = s.QC[0] * x159 + s.WQ[0] * x174 + r1 * x104;
-      const TB x669 = s.QC[0] * x283 + s.WQ[0] * x284 + e1 * (x104 - s.roe * x114);
-      const TB x323 = s.QC[0] * x174 + s.WQ[0] * x189 + r1 * x114;
-      const TB x699 = s.QC[0] * x323 + s.WQ[0] * x324 + e1 * (x174 - s.roe * x189) + r1 * x284;
-      const TB x799 = s.QC[1] * x323 + s.WQ[1] * x324;
-      const TB x349 = s.QC[1] * x18 + s.WQ[1] * x21;
-      const TB x363 = s.QC[1] * x52 + s.WQ[1] * x58 + r1 * x21;
-      const TB x362 = s.QC[1] * x46 + s.WQ[1] * x52 + r1 * x18;
-      const TB x501 = s.QC[2] * x47 + s.WQ[2] * x53 + r1 * x18;
-      const TB x927 = s.QC[1] * x362 + s.WQ[1] * x363 + e1 * (x46 - s.roe * x52) + r1 * x349;
-      const TB x366 = s.QC[1] * x47 + s.WQ[1] * x53;
-      const TB x367 = s.QC[1] * x53 + s.WQ[1] * x59;
-      const TB x399 = s.QC[1] * x109 + s.WQ[1] * x119 + r1 * x59;
-      const TB x393 = s.QC[1] * x88 + s.WQ[1] * x98 + r2 * x46;
-      const TB x394 = s.QC[1] * x98 + s.WQ[1] * x108 + r2 * x52;
-      const TB x395 = s.QC[1] * x108 + s.WQ[1] * x118 + r2 * x58;
-      const TB x397 = s.QC[1] * x89 + s.WQ[1] * x99 + r1 * x47;
-      const TB x398 = s.QC[1] * x99 + s.WQ[1] * x109 + r1 * x53;
-      const TB x401 = s.QC[1] * x90 + s.WQ[1] * x100;
-      const TB x402 = s.QC[1] * x100 + s.WQ[1] * x110;
-      const TB x403 = s.QC[1] * x110 + s.WQ[1] * x120;
-      const TB x455 = s.QC[1] * x188 + s.WQ[1] * x203 + r1 * x120;
-      const TB x590 = s.QC[2] * x188 + s.WQ[2] * x203 + r2 * x119;
-      const TB x448 = s.QC[1] * x142 + s.WQ[1] * x157 + r2 * x89;
-      const TB x449 = s.QC[1] * x157 + s.WQ[1] * x172 + r2 * x99;
-      const TB x450 = s.QC[1] * x172 + s.WQ[1] * x187 + r2 * x109;
-      const TB x451 = s.QC[1] * x187 + s.WQ[1] * x202 + r2 * x119;
-      const TB x534 = s.QC[2] * x109 + s.WQ[2] * x119 + r1 * x58;
-      const TB x498 = s.QC[2] * x52 + s.WQ[2] * x58;
-      const TB x586 = s.QC[2] * x187 + s.WQ[2] * x202 + r1 * x118;
-      const TB x530 = s.QC[2] * x108 + s.WQ[2] * x118;
-      const TB x452 = s.QC[1] * x143 + s.WQ[1] * x158 + r1 * x90;
-      const TB x453 = s.QC[1] * x158 + s.WQ[1] * x173 + r1 * x100;
-      const TB x454 = s.QC[1] * x173 + s.WQ[1] * x188 + r1 * x110;
-      const TB x456 = s.QC[1] * x144 + s.WQ[1] * x159;
-      const TB x457 = s.QC[1] * x159 + s.WQ[1] * x174;
-      const TB x458 = s.QC[1] * x174 + s.WQ[1] * x189;
-      const TB x593 = s.QC[2] * x174 + s.WQ[2] * x189 + r3 * x110;
-      const TB x497 = s.QC[2] * x46 + s.WQ[2] * x52;
-      const TB x528 = s.QC[2] * x88 + s.WQ[2] * x98;
-      const TB x529 = s.QC[2] * x98 + s.WQ[2] * x108;
-      const TB x532 = s.QC[2] * x89 + s.WQ[2] * x99 + r1 * x46;
-      const TB x1127 = s.QC[2] * x497 + s.WQ[2] * x498 + e1 * (x46 - s.roe * x52);
-      const TB x533 = s.QC[2] * x99 + s.WQ[2] * x109 + r1 * x52;
-      const TB x1153 = s.QC[2] * x532 + s.WQ[2] * x533 + e1 * (x89 - s.roe * x99) + r1 * x497;
-      const TB x1154 = s.QC[2] * x533 + s.WQ[2] * x534 + e1 * (x99 - s.roe * x109) + r1 * x498;
-      const TB x1824 = s.QC[2] * x1153 + s.WQ[2] * x1154 + e2 * (x532 - s.roe * x533) + r1 * x1127;
-      const TB x536 = s.QC[2] * x90 + s.WQ[2] * x100 + r2 * x47;
-      const TB x930 = s.QC[1] * x366 + s.WQ[1] * x367 + e1 * (x47 - s.roe * x53);
-      const TB x537 = s.QC[2] * x100 + s.WQ[2] * x110 + r2 * x53;
-      const TB x954 = s.QC[1] * x398 + s.WQ[1] * x399 + e1 * (x99 - s.roe * x109) + r1 * x367;
-      const TB x589 = s.QC[2] * x173 + s.WQ[2] * x188 + r2 * x109;
-      const TB x1196 = s.QC[2] * x589 + s.WQ[2] * x590 + e1 * (x173 - s.roe * x188) + r2 * x534;
-      const TB x957 = s.QC[1] * x402 + s.WQ[1] * x403 + e1 * (x100 - s.roe * x110);
-      const TB x996 = s.QC[1] * x454 + s.WQ[1] * x455 + e1 * (x173 - s.roe * x188) + r1 * x403;
-      const TB x1096 = s.QC[2] * x454 + s.WQ[2] * x455 + r2 * x399;
-      const TB x1156 = s.QC[2] * x536 + s.WQ[2] * x537 + e1 * (x90 - s.roe * x100) + r2 * x501;
-      const TB x583 = s.QC[2] * x142 + s.WQ[2] * x157 + r1 * x88;
-      const TB x584 = s.QC[2] * x157 + s.WQ[2] * x172 + r1 * x98;
-      const TB x585 = s.QC[2] * x172 + s.WQ[2] * x187 + r1 * x108;
-      const TB x1193 = s.QC[2] * x585 + s.WQ[2] * x586 + e1 * (x172 - s.roe * x187) + r1 * x530;
-      const TB x993 = s.QC[1] * x450 + s.WQ[1] * x451 + e1 * (x172 - s.roe * x187) + r2 * x399;
-      const TB x1054 = s.QC[2] * x398 + s.WQ[2] * x399 + r1 * x363;
-      const TB x1093 = s.QC[2] * x450 + s.WQ[2] * x451 + r1 * x395;
-      const TB x1151 = s.QC[2] * x529 + s.WQ[2] * x530 + e1 * (x98 - s.roe * x108);
-      const TB x951 = s.QC[1] * x394 + s.WQ[1] * x395 + e1 * (x98 - s.roe * x108) + r2 * x363;
-      const TB x1027 = s.QC[2] * x362 + s.WQ[2] * x363;
-      const TB x1051 = s.QC[2] * x394 + s.WQ[2] * x395;
-      const TB x587 = s.QC[2] * x143 + s.WQ[2] * x158 + r2 * x89;
-      const TB x953 = s.QC[1] * x397 + s.WQ[1] * x398 + e1 * (x89 - s.roe * x99) + r1 * x366;
-      const TB x588 = s.QC[2] * x158 + s.WQ[2] * x173 + r2 * x99;
-      const TB x1056 = s.QC[2] * x401 + s.WQ[2] * x402 + r2 * x366;
-      const TB x1194 = s.QC[2] * x587 + s.WQ[2] * x588 + e1 * (x143 - s.roe * x158) + r2 * x532;
-      acc179 += static_cast<double>(x1194);
-      const TB x1195 = s.QC[2] * x588 + s.WQ[2] * x589 + e1 * (x158 - s.roe * x173) + r2 * x533;
-      const TB x1852 = s.QC[2] * x1195 + s.WQ[2] * x1196 + e2 * (x588 - s.roe * x589) + r2 * x1154;
-      const TB x1851 = s.QC[2] * x1194 + s.WQ[2] * x1195 + e2 * (x587 - s.roe * x588) + r2 * x1153;
-      acc489 += static_cast<double>(x1851);
-      const TB x2323 = s.QC[2] * x1851 + s.WQ[2] * x1852 + e3 * (x1194 - s.roe * x1195) + r2 * x1824;
-      acc954 += static_cast<double>(x2323);
-      const TB x1629 = s.QC[1] * x953 + s.WQ[1] * x954 + e2 * (x397 - s.roe * x398) + r1 * x930;
-      const TB x591 = s.QC[2] * x144 + s.WQ[2] * x159 + r3 * x90;
-      const TB x956 = s.QC[1] * x401 + s.WQ[1] * x402 + e1 * (x90 - s.roe * x100);
-      const TB x592 = s.QC[2] * x159 + s.WQ[2] * x174 + r3 * x100;
-      const TB x1198 = s.QC[2] * x592 + s.WQ[2] * x593 + e1 * (x159 - s.roe * x174) + r3 * x537;
-      const TB x1197 = s.QC[2] * x591 + s.WQ[2] * x592 + e1 * (x144 - s.roe * x159) + r3 * x536;
-      acc180 += static_cast<double>(x1197);
-      const TB x1853 = s.QC[2] * x1197 + s.WQ[2] * x1198 + e2 * (x591 - s.roe * x592) + r3 * x1156;
-      acc490 += static_cast<double>(x1853);
-      const TB x691 = s.QC[0] * x313 + s.WQ[0] * x314 + e1 * (x142 - s.roe * x157) + r1 * x274;
-      const TB x1245 = s.QC[0] * x662 + s.WQ[0] * x663 + e2 * (x274 - s.roe * x275);
-      const TB x692 = s.QC[0] * x314 + s.WQ[0] * x315 + e1 * (x157 - s.roe * x172) + r1 * x275;
-      const TB x1264 = s.QC[0] * x691 + s.WQ[0] * x692 + e2 * (x313 - s.roe * x314) + r1 * x662;
-      const TB x1265 = s.QC[0] * x692 + s.WQ[0] * x693 + e2 * (x314 - s.roe * x315) + r1 * x663;
-      const TB x1888 = s.QC[0] * x1264 + s.WQ[0] * x1265 + e3 * (x691 - s.roe * x692) + r1 * x1245;
-      acc519 += static_cast<double>(x1888);
-      const TB x1950 = s.QC[2] * x1264 + s.WQ[2] * x1265 + r1 * x1237;
-      acc581 += static_cast<double>(x1950);
-      const TB x1919 = s.QC[1] * x1264 + s.WQ[1] * x1265 + r2 * x1239;
-      acc550 += static_cast<double>(x1919);
-      const TB x694 = s.QC[0] * x317 + s.WQ[0] * x318 + e1 * (x143 - s.roe * x158) + r1 * x278;
-      acc24 += static_cast<double>(x694);
-      const TB x994 = s.QC[1] * x452 + s.WQ[1] * x453 + e1 * (x143 - s.roe * x158) + r1 * x401;
-      acc117 += static_cast<double>(x994);
-      const TB x1247 = s.QC[0] * x665 + s.WQ[0] * x666 + e2 * (x278 - s.roe * x279);
-      const TB x695 = s.QC[0] * x318 + s.WQ[0] * x319 + e1 * (x158 - s.roe * x173) + r1 * x279;
-      const TB x995 = s.QC[1] * x453 + s.WQ[1] * x454 + e1 * (x158 - s.roe * x173) + r1 * x402;
-      const TB x1266 = s.QC[0] * x694 + s.WQ[0] * x695 + e2 * (x317 - s.roe * x318) + r1 * x665;
-      acc210 += static_cast<double>(x1266);
-      const TB x1267 = s.QC[0] * x695 + s.WQ[0] * x696 + e2 * (x318 - s.roe * x319) + r1 * x666;
-      const TB x1889 = s.QC[0] * x1266 + s.WQ[0] * x1267 + e3 * (x694 - s.roe * x695) + r1 * x1247;
-      acc520 += static_cast<double>(x1889);
-      const TB x1951 = s.QC[2] * x1266 + s.WQ[2] * x1267 + r2 * x1239;
-      acc582 += static_cast<double>(x1951);
-      const TB x1920 = s.QC[1] * x1266 + s.WQ[1] * x1267 + r1 * x1241;
-      acc551 += static_cast<double>(x1920);
-      const TB x697 = s.QC[0] * x321 + s.WQ[0] * x322 + e1 * (x144 - s.roe * x159) + r1 * x282;
-      acc25 += static_cast<double>(x697);
-      const TB x997 = s.QC[1] * x456 + s.WQ[1] * x457 + e1 * (x144 - s.roe * x159);
-      acc118 += static_cast<double>(x997);
-      const TB x1249 = s.QC[0] * x668 + s.WQ[0] * x669 + e2 * (x282 - s.roe * x283);
-      const TB x698 = s.QC[0] * x322 + s.WQ[0] * x323 + e1 * (x159 - s.roe * x174) + r1 * x283;
-      const TB x998 = s.QC[1] * x457 + s.WQ[1] * x458 + e1 * (x159 - s.roe * x174);
-      const TB x1098 = s.QC[2] * x457 + s.WQ[2] * x458 + r3 * x402;
-      const TB x1268 = s.QC[0] * x697 + s.WQ[0] * x698 + e2 * (x321 - s.roe * x322) + r1 * x668;
-      acc211 += static_cast<double>(x1268);
-      const TB x1269 = s.QC[0] * x698 + s.WQ[0] * x699 + e2 * (x322 - s.roe * x323) + r1 * x669;
-      const TB x1890 = s.QC[0] * x1268 + s.WQ[0] * x1269 + e3 * (x697 - s.roe * x698) + r1 * x1249;
-      acc521 += static_cast<double>(x1890);
-      const TB x1952 = s.QC[2] * x1268 + s.WQ[2] * x1269 + r3 * x1241;
-      acc583 += static_cast<double>(x1952);
-      const TB x1921 = s.QC[1] * x1268 + s.WQ[1] * x1269;
-      acc552 += static_cast<double>(x1921);
-      const TB x1631 = s.QC[1] * x956 + s.WQ[1] * x957 + e2 * (x401 - s.roe * x402);
-      const TB x1097 = s.QC[2] * x456 + s.WQ[2] * x457 + r3 * x401;
-      acc149 += static_cast<double>(x1097);
-      const TB x1788 = s.QC[2] * x1097 + s.WQ[2] * x1098 + e1 * (x456 - s.roe * x457) + r3 * x1056;
-      acc459 += static_cast<double>(x1788);
-      const TB x1658 = s.QC[1] * x997 + s.WQ[1] * x998 + e2 * (x456 - s.roe * x457);
-      acc397 += static_cast<double>(x1658);
-      const TB x1723 = s.QC[2] * x997 + s.WQ[2] * x998 + r3 * x956;
-      acc428 += static_cast<double>(x1723);
-      const TB x1656 = s.QC[1] * x994 + s.WQ[1] * x995 + e2 * (x452 - s.roe * x453) + r1 * x956;
-      acc396 += static_cast<double>(x1656);
-      const TB x1657 = s.QC[1] * x995 + s.WQ[1] * x996 + e2 * (x453 - s.roe * x454) + r1 * x957;
-      const TB x2199 = s.QC[1] * x1656 + s.WQ[1] * x1657 + e3 * (x994 - s.roe * x995) + r1 * x1631;
-      acc830 += static_cast<double>(x2199);
-      const TB x2230 = s.QC[2] * x1656 + s.WQ[2] * x1657 + r2 * x1629;
-      acc861 += static_cast<double>(x2230);
-      const TB x1722 = s.QC[2] * x995 + s.WQ[2] * x996 + r2 * x954;
-      const TB x730 = s.QC[1] * x231 + s.WQ[1] * x232;
-      const TB x754 = s.QC[1] * x263 + s.WQ[1] * x264 + r1 * x232;
-      const TB x750 = s.QC[1] * x258 + s.WQ[1] * x259 + r2 * x227;
-      const TB x751 = s.QC[1] * x259 + s.WQ[1] * x260 + r2 * x228;
-      const TB x753 = s.QC[1] * x262 + s.WQ[1] * x263 + r1 * x231;
-      const TB x856 = s.QC[2] * x266 + s.WQ[2] * x267 + r2 * x231;
-      const TB x756 = s.QC[1] * x266 + s.WQ[1] * x267;
-      const TB x757 = s.QC[1] * x267 + s.WQ[1] * x268;
-      const TB x796 = s.QC[1] * x319 + s.WQ[1] * x320 + r1 * x268;
-      const TB x896 = s.QC[2] * x319 + s.WQ[2] * x320 + r2 * x264;
-      const TB x791 = s.QC[1] * x313 + s.WQ[1] * x314 + r2 * x262;
-      const TB x792 = s.QC[1] * x314 + s.WQ[1] * x315 + r2 * x263;
-      const TB x793 = s.QC[1] * x315 + s.WQ[1] * x316 + r2 * x264;
-      const TB x854 = s.QC[2] * x263 + s.WQ[2] * x264 + r1 * x228;
-      const TB x827 = s.QC[2] * x227 + s.WQ[2] * x228;
-      const TB x853 = s.QC[2] * x262 + s.WQ[2] * x263 + r1 * x227;
-      const TB x893 = s.QC[2] * x315 + s.WQ[2] * x316 + r1 * x260;
-      const TB x851 = s.QC[2] * x259 + s.WQ[2] * x260;
-      const TB x1564 = s.QC[2] * x853 + s.WQ[2] * x854 + e1 * (x262 - s.roe * x263) + r1 * x827;
-      const TB x794 = s.QC[1] * x317 + s.WQ[1] * x318 + r1 * x266;
-      acc55 += static_cast<double>(x794);
-      const TB x795 = s.QC[1] * x318 + s.WQ[1] * x319 + r1 * x267;
-      const TB x797 = s.QC[1] * x321 + s.WQ[1] * x322;
-      acc56 += static_cast<double>(x797);
-      const TB x798 = s.QC[1] * x322 + s.WQ[1] * x323;
-      const TB x850 = s.QC[2] * x258 + s.WQ[2] * x259;
-      const TB x891 = s.QC[2] * x313 + s.WQ[2] * x314 + r1 * x258;
-      const TB x892 = s.QC[2] * x314 + s.WQ[2] * x315 + r1 * x259;
-      const TB x1590 = s.QC[2] * x892 + s.WQ[2] * x893 + e1 * (x314 - s.roe * x315) + r1 * x851;
-      const TB x1460 = s.QC[1] * x792 + s.WQ[1] * x793 + e1 * (x314 - s.roe * x315) + r2 * x754;
-      const TB x1525 = s.QC[2] * x792 + s.WQ[2] * x793 + r1 * x751;
-      const TB x1562 = s.QC[2] * x850 + s.WQ[2] * x851 + e1 * (x258 - s.roe * x259);
-      const TB x1432 = s.QC[1] * x750 + s.WQ[1] * x751 + e1 * (x258 - s.roe * x259) + r2 * x727;
-      const TB x1497 = s.QC[2] * x750 + s.WQ[2] * x751;
-      const TB x1499 = s.QC[2] * x753 + s.WQ[2] * x754 + r1 * x727;
-      const TB x1524 = s.QC[2] * x791 + s.WQ[2] * x792 + r1 * x750;
-      acc333 += static_cast<double>(x1524);
-      const TB x2136 = s.QC[2] * x1524 + s.WQ[2] * x1525 + e1 * (x791 - s.roe * x792) + r1 * x1497;
-      acc767 += static_cast<double>(x2136);
-      const TB x1589 = s.QC[2] * x891 + s.WQ[2] * x892 + e1 * (x313 - s.roe * x314) + r1 * x850;
-      acc364 += static_cast<double>(x1589);
-      const TB x2167 = s.QC[2] * x1589 + s.WQ[2] * x1590 + e2 * (x891 - s.roe * x892) + r1 * x1562;
-      acc798 += static_cast<double>(x2167);
-      const TB x1459 = s.QC[1] * x791 + s.WQ[1] * x792 + e1 * (x313 - s.roe * x314) + r2 * x753;
-      acc302 += static_cast<double>(x1459);
-      const TB x2105 = s.QC[2] * x1459 + s.WQ[2] * x1460 + r1 * x1432;
-      acc736 += static_cast<double>(x2105);
-      const TB x894 = s.QC[2] * x317 + s.WQ[2] * x318 + r2 * x262;
-      acc86 += static_cast<double>(x894);
-      const TB x1434 = s.QC[1] * x753 + s.WQ[1] * x754 + e1 * (x262 - s.roe * x263) + r1 * x730;
-      const TB x2074 = s.QC[1] * x1459 + s.WQ[1] * x1460 + e2 * (x791 - s.roe * x792) + r2 * x1434;
-      acc705 += static_cast<double>(x2074);
-      const TB x895 = s.QC[2] * x318 + s.WQ[2] * x319 + r2 * x263;
-      const TB x1592 = s.QC[2] * x895 + s.WQ[2] * x896 + e1 * (x318 - s.roe * x319) + r2 * x854;
-      const TB x1462 = s.QC[1] * x795 + s.WQ[1] * x796 + e1 * (x318 - s.roe * x319) + r1 * x757;
-      const TB x1527 = s.QC[2] * x795 + s.WQ[2] * x796 + r2 * x754;
-      const TB x1501 = s.QC[2] * x756 + s.WQ[2] * x757 + r2 * x730;
-      const TB x1526 = s.QC[2] * x794 + s.WQ[2] * x795 + r2 * x753;
-      acc334 += static_cast<double>(x1526);
-      const TB x2137 = s.QC[2] * x1526 + s.WQ[2] * x1527 + e1 * (x794 - s.roe * x795) + r2 * x1499;
-      acc768 += static_cast<double>(x2137);
-      const TB x1591 = s.QC[2] * x894 + s.WQ[2] * x895 + e1 * (x317 - s.roe * x318) + r2 * x853;
-      acc365 += static_cast<double>(x1591);
-      const TB x2168 = s.QC[2] * x1591 + s.WQ[2] * x1592 + e2 * (x894 - s.roe * x895) + r2 * x1564;
-      acc799 += static_cast<double>(x2168);
-      const TB x1461 = s.QC[1] * x794 + s.WQ[1] * x795 + e1 * (x317 - s.roe * x318) + r1 * x756;
-      acc303 += static_cast<double>(x1461);
-      const TB x2106 = s.QC[2] * x1461 + s.WQ[2] * x1462 + r2 * x1434;
-      acc737 += static_cast<double>(x2106);
-      const TB x897 = s.QC[2] * x321 + s.WQ[2] * x322 + r3 * x266;
-      acc87 += static_cast<double>(x897);
-      const TB x1436 = s.QC[1] * x756 + s.WQ[1] * x757 + e1 * (x266 - s.roe * x267);
-      const TB x2075 = s.QC[1] * x1461 + s.WQ[1] * x1462 + e2 * (x794 - s.roe * x795) + r1 * x1436;
-      acc706 += static_cast<double>(x2075);
-      const TB x898 = s.QC[2] * x322 + s.WQ[2] * x323 + r3 * x267;
-      const TB x1593 = s.QC[2] * x897 + s.WQ[2] * x898 + e1 * (x321 - s.roe * x322) + r3 * x856;
-      acc366 += static_cast<double>(x1593);
-      const TB x1463 = s.QC[1] * x797 + s.WQ[1] * x798 + e1 * (x321 - s.roe * x322);
-      acc304 += static_cast<double>(x1463);
-      const TB x1464 = s.QC[1] * x798 + s.WQ[1] * x799 + e1 * (x322 - s.roe * x323);
-      const TB x1529 = s.QC[2] * x798 + s.WQ[2] * x799 + r3 * x757;
-      const TB x2107 = s.QC[2] * x1463 + s.WQ[2] * x1464 + r3 * x1436;
-      acc738 += static_cast<double>(x2107);
-      const TB x2076 = s.QC[1] * x1463 + s.WQ[1] * x1464 + e2 * (x797 - s.roe * x798);
-      acc707 += static_cast<double>(x2076);
-      const TB x1528 = s.QC[2] * x797 + s.WQ[2] * x798 + r3 * x756;
-      acc335 += static_cast<double>(x1528);
-      const TB x2138 = s.QC[2] * x1528 + s.WQ[2] * x1529 + e1 * (x797 - s.roe * x798) + r3 * x1501;
-      acc769 += static_cast<double>(x2138);
-      const TB x950 = s.QC[1] * x393 + s.WQ[1] * x394 + e1 * (x88 - s.roe * x98) + r2 * x362;
-      const TB x1150 = s.QC[2] * x528 + s.WQ[2] * x529 + e1 * (x88 - s.roe * x98);
-      const TB x1053 = s.QC[2] * x397 + s.WQ[2] * x398 + r1 * x362;
-      const TB x1759 = s.QC[2] * x1053 + s.WQ[2] * x1054 + e1 * (x397 - s.roe * x398) + r1 * x1027;
-      const TB x991 = s.QC[1] * x448 + s.WQ[1] * x449 + e1 * (x142 - s.roe * x157) + r2 * x397;
-      const TB x1094 = s.QC[2] * x452 + s.WQ[2] * x453 + r2 * x397;
-      acc148 += static_cast<double>(x1094);
-      const TB x1191 = s.QC[2] * x583 + s.WQ[2] * x584 + e1 * (x142 - s.roe * x157) + r1 * x528;
-      const TB x1822 = s.QC[2] * x1150 + s.WQ[2] * x1151 + e2 * (x528 - s.roe * x529);
-      const TB x1192 = s.QC[2] * x584 + s.WQ[2] * x585 + e1 * (x157 - s.roe * x172) + r1 * x529;
-      const TB x1850 = s.QC[2] * x1192 + s.WQ[2] * x1193 + e2 * (x584 - s.roe * x585) + r1 * x1151;
-      const TB x1849 = s.QC[2] * x1191 + s.WQ[2] * x1192 + e2 * (x583 - s.roe * x584) + r1 * x1150;
-      acc488 += static_cast<double>(x1849);
-      const TB x2322 = s.QC[2] * x1849 + s.WQ[2] * x1850 + e3 * (x1191 - s.roe * x1192) + r1 * x1822;
-      acc953 += static_cast<double>(x2322);
-      const TB x992 = s.QC[1] * x449 + s.WQ[1] * x450 + e1 * (x157 - s.roe * x172) + r2 * x398;
-      const TB x1095 = s.QC[2] * x453 + s.WQ[2] * x454 + r2 * x398;
-      const TB x1787 = s.QC[2] * x1095 + s.WQ[2] * x1096 + e1 * (x453 - s.roe * x454) + r2 * x1054;
-      const TB x1786 = s.QC[2] * x1094 + s.WQ[2] * x1095 + e1 * (x452 - s.roe * x453) + r2 * x1053;
-      acc458 += static_cast<double>(x1786);
-      const TB x2292 = s.QC[2] * x1786 + s.WQ[2] * x1787 + e2 * (x1094 - s.roe * x1095) + r2 * x1759;
-      acc923 += static_cast<double>(x2292);
-      const TB x1050 = s.QC[2] * x393 + s.WQ[2] * x394;
-      const TB x1091 = s.QC[2] * x448 + s.WQ[2] * x449 + r1 * x393;
-      const TB x1092 = s.QC[2] * x449 + s.WQ[2] * x450 + r1 * x394;
-      const TB x1785 = s.QC[2] * x1092 + s.WQ[2] * x1093 + e1 * (x449 - s.roe * x450) + r1 * x1051;
-      const TB x1655 = s.QC[1] * x992 + s.WQ[1] * x993 + e2 * (x449 - s.roe * x450) + r2 * x954;
-      const TB x1694 = s.QC[2] * x953 + s.WQ[2] * x954 + r1 * x927;
-      const TB x1627 = s.QC[1] * x950 + s.WQ[1] * x951 + e2 * (x393 - s.roe * x394) + r2 * x927;
-      const TB x1757 = s.QC[2] * x1050 + s.WQ[2] * x1051 + e1 * (x393 - s.roe * x394);
-      const TB x1720 = s.QC[2] * x992 + s.WQ[2] * x993 + r1 * x951;
-      const TB x1692 = s.QC[2] * x950 + s.WQ[2] * x951;
-      const TB x1719 = s.QC[2] * x991 + s.WQ[2] * x992 + r1 * x950;
-      acc426 += static_cast<double>(x1719);
-      const TB x2260 = s.QC[2] * x1719 + s.WQ[2] * x1720 + e1 * (x991 - s.roe * x992) + r1 * x1692;
-      acc891 += static_cast<double>(x2260);
-      const TB x1784 = s.QC[2] * x1091 + s.WQ[2] * x1092 + e1 * (x448 - s.roe * x449) + r1 * x1050;
-      acc457 += static_cast<double>(x1784);
-      const TB x2291 = s.QC[2] * x1784 + s.WQ[2] * x1785 + e2 * (x1091 - s.roe * x1092) + r1 * x1757;
-      acc922 += static_cast<double>(x2291);
-      const TB x1654 = s.QC[1] * x991 + s.WQ[1] * x992 + e2 * (x448 - s.roe * x449) + r2 * x953;
-      acc395 += static_cast<double>(x1654);
-      const TB x2198 = s.QC[1] * x1654 + s.WQ[1] * x1655 + e3 * (x991 - s.roe * x992) + r2 * x1629;
-      acc829 += static_cast<double>(x2198);
-      const TB x2229 = s.QC[2] * x1654 + s.WQ[2] * x1655 + r1 * x1627;
-      acc860 += static_cast<double>(x2229);
-      const TB x1721 = s.QC[2] * x994 + s.WQ[2] * x995 + r2 * x953;
-      acc427 += static_cast<double>(x1721);
-      const TB x2261 = s.QC[2] * x1721 + s.WQ[2] * x1722 + e1 * (x994 - s.roe * x995) + r2 * x1694;
-      acc892 += static_cast<double>(x2261);
-      const TB x1302 = s.QC[1] * x650 + s.WQ[1] * x651 + r2 * x627;
-      const TB x1369 = s.QC[2] * x653 + s.WQ[2] * x654 + r1 * x627;
-      const TB x1304 = s.QC[1] * x653 + s.WQ[1] * x654 + r1 * x630;
-      const TB x1371 = s.QC[2] * x656 + s.WQ[2] * x657 + r2 * x630;
-      const TB x1306 = s.QC[1] * x656 + s.WQ[1] * x657;
-      const TB x1329 = s.QC[1] * x691 + s.WQ[1] * x692 + r2 * x653;
-      acc240 += static_cast<double>(x1329);
-      const TB x1396 = s.QC[2] * x694 + s.WQ[2] * x695 + r2 * x653;
-      acc272 += static_cast<double>(x1396);
-      const TB x1330 = s.QC[1] * x692 + s.WQ[1] * x693 + r2 * x654;
-      const TB x2012 = s.QC[2] * x1329 + s.WQ[2] * x1330 + r1 * x1302;
-      acc643 += static_cast<double>(x2012);
-      const TB x1981 = s.QC[1] * x1329 + s.WQ[1] * x1330 + e1 * (x691 - s.roe * x692) + r2 * x1304;
-      acc612 += static_cast<double>(x1981);
-      const TB x1395 = s.QC[2] * x692 + s.WQ[2] * x693 + r1 * x651;
-      const TB x1367 = s.QC[2] * x650 + s.WQ[2] * x651;
-      const TB x1394 = s.QC[2] * x691 + s.WQ[2] * x692 + r1 * x650;
-      acc271 += static_cast<double>(x1394);
-      const TB x2043 = s.QC[2] * x1394 + s.WQ[2] * x1395 + e1 * (x691 - s.roe * x692) + r1 * x1367;
-      acc674 += static_cast<double>(x2043);
-      const TB x1397 = s.QC[2] * x695 + s.WQ[2] * x696 + r2 * x654;
-      const TB x2044 = s.QC[2] * x1396 + s.WQ[2] * x1397 + e1 * (x694 - s.roe * x695) + r2 * x1369;
-      acc675 += static_cast<double>(x2044);
-      const TB x1332 = s.QC[1] * x695 + s.WQ[1] * x696 + r1 * x657;
-      const TB x1399 = s.QC[2] * x698 + s.WQ[2] * x699 + r3 * x657;
-      const TB x1334 = s.QC[1] * x698 + s.WQ[1] * x699;
-      const TB x1331 = s.QC[1] * x694 + s.WQ[1] * x695 + r1 * x656;
-      acc241 += static_cast<double>(x1331);
-      const TB x1982 = s.QC[1] * x1331 + s.WQ[1] * x1332 + e1 * (x694 - s.roe * x695) + r1 * x1306;
-      acc613 += static_cast<double>(x1982);
-      const TB x2013 = s.QC[2] * x1331 + s.WQ[2] * x1332 + r2 * x1304;
-      acc644 += static_cast<double>(x2013);
-      const TB x1398 = s.QC[2] * x697 + s.WQ[2] * x698 + r3 * x656;
-      acc273 += static_cast<double>(x1398);
-      const TB x2045 = s.QC[2] * x1398 + s.WQ[2] * x1399 + e1 * (x697 - s.roe * x698) + r3 * x1371;
-      acc676 += static_cast<double>(x2045);
-      const TB x1333 = s.QC[1] * x697 + s.WQ[1] * x698;
-      acc242 += static_cast<double>(x1333);
-      const TB x1983 = s.QC[1] * x1333 + s.WQ[1] * x1334 + e1 * (x697 - s.roe * x698);
-      acc614 += static_cast<double>(x1983);
-      const TB x2014 = s.QC[2] * x1333 + s.WQ[2] * x1334 + r3 * x1306;
-      acc645 += static_cast<double>(x2014);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(240 * R + r) * a.acc_stride + q] = acc240;
-  a.acc[(271 * R + r) * a.acc_stride + q] = acc271;
-  a.acc[(302 * R + r) * a.acc_stride + q] = acc302;
-  a.acc[(333 * R + r) * a.acc_stride + q] = acc333;
-  a.acc[(364 * R + r) * a.acc_stride + q] = acc364;
-  a.acc[(395 * R + r) * a.acc_stride + q] = acc395;
-  a.acc[(426 * R + r) * a.acc_stride + q] = acc426;
-  a.acc[(457 * R + r) * a.acc_stride + q] = acc457;
-  a.acc[(488 * R + r) * a.acc_stride + q] = acc488;
-  a.acc[(519 * R + r) * a.acc_stride + q] = acc519;
-  a.acc[(550 * R + r) * a.acc_stride + q] = acc550;
-  a.acc[(581 * R + r) * a.acc_stride + q] = acc581;
-  a.acc[(612 * R + r) * a.acc_stride + q] = acc612;
-  a.acc[(643 * R + r) * a.acc_stride + q] = acc643;
-  a.acc[(674 * R + r) * a.acc_stride + q] = acc674;
-  a.acc[(705 * R + r) * a.acc_stride + q] = acc705;
-  a.acc[(736 * R + r) * a.acc_stride + q] = acc736;
-  a.acc[(767 * R + r) * a.acc_stride + q] = acc767;
-  a.acc[(798 * R + r) * a.acc_stride + q] = acc798;
-  a.acc[(829 * R + r) * a.acc_stride + q] = acc829;
-  a.acc[(860 * R + r) * a.acc_stride + q] = acc860;
-  a.acc[(891 * R + r) * a.acc_stride + q] = acc891;
-  a.acc[(922 * R + r) * a.acc_stride + q] = acc922;
-  a.acc[(953 * R + r) * a.acc_stride + q] = acc953;
-  a.acc[(24 * R + r) * a.acc_stride + q] = acc24;
-  a.acc[(55 * R + r) * a.acc_stride + q] = acc55;
-  a.acc[(86 * R + r) * a.acc_stride + q] = acc86;
-  a.acc[(117 * R + r) * a.acc_stride + q] = acc117;
-  a.acc[(148 * R + r) * a.acc_stride + q] = acc148;
-  a.acc[(179 * R + r) * a.acc_stride + q] = acc179;
-  a.acc[(210 * R + r) * a.acc_stride + q] = acc210;
-  a.acc[(241 * R + r) * a.acc_stride + q] = acc241;
-  a.acc[(272 * R + r) * a.acc_stride + q] = acc272;
-  a.acc[(303 * R + r) * a.acc_stride + q] = acc303;
-  a.acc[(334 * R + r) * a.acc_stride + q] = acc334;
-  a.acc[(365 * R + r) * a.acc_stride + q] = acc365;
-  a.acc[(396 * R + r) * a.acc_stride + q] = acc396;
-  a.acc[(427 * R + r) * a.acc_stride + q] = acc427;
-  a.acc[(458 * R + r) * a.acc_stride + q] = acc458;
-  a.acc[(489 * R + r) * a.acc_stride + q] = acc489;
-  a.acc[(520 * R + r) * a.acc_stride + q] = acc520;
-  a.acc[(551 * R + r) * a.acc_stride + q] = acc551;
-  a.acc[(582 * R + r) * a.acc_stride + q] = acc582;
-  a.acc[(613 * R + r) * a.acc_stride + q] = acc613;
-  a.acc[(644 * R + r) * a.acc_stride + q] = acc644;
-  a.acc[(675 * R + r) * a.acc_stride + q] = acc675;
-  a.acc[(706 * R + r) * a.acc_stride + q] = acc706;
-  a.acc[(737 * R + r) * a.acc_stride + q] = acc737;
-  a.acc[(768 * R + r) * a.acc_stride + q] = acc768;
-  a.acc[(799 * R + r) * a.acc_stride + q] = acc799;
-  a.acc[(830 * R + r) * a.acc_stride + q] = acc830;
-  a.acc[(861 * R + r) * a.acc_stride + q] = acc861;
-  a.acc[(892 * R + r) * a.acc_stride + q] = acc892;
-  a.acc[(923 * R + r) * a.acc_stride + q] = acc923;
-  a.acc[(954 * R + r) * a.acc_stride + q] = acc954;
-  a.acc[(25 * R + r) * a.acc_stride + q] = acc25;
-  a.acc[(56 * R + r) * a.acc_stride + q] = acc56;
-  a.acc[(87 * R + r) * a.acc_stride + q] = acc87;
-  a.acc[(118 * R + r) * a.acc_stride + q] = acc118;
-  a.acc[(149 * R + r) * a.acc_stride + q] = acc149;
-  a.acc[(180 * R + r) * a.acc_stride + q] = acc180;
-  a.acc[(211 * R + r) * a.acc_stride + q] = acc211;
-  a.acc[(242 * R + r) * a.acc_stride + q] = acc242;
-  a.acc[(273 * R + r) * a.acc_stride + q] = acc273;
-  a.acc[(304 * R + r) * a.acc_stride + q] = acc304;
-  a.acc[(335 * R + r) * a.acc_stride + q] = acc335;
-  a.acc[(366 * R + r) * a.acc_stride + q] = acc366;
-  a.acc[(397 * R + r) * a.acc_stride + q] = acc397;
-  a.acc[(428 * R + r) * a.acc_stride + q] = acc428;
-  a.acc[(459 * R + r) * a.acc_stride + q] = acc459;
-  a.acc[(490 * R + r) * a.acc_stride + q] = acc490;
-  a.acc[(521 * R + r) * a.acc_stride + q] = acc521;
-  a.acc[(552 * R + r) * a.acc_stride + q] = acc552;
-  a.acc[(583 * R + r) * a.acc_stride + q] = acc583;
-  a.acc[(614 * R + r) * a.acc_stride + q] = acc614;
-  a.acc[(645 * R + r) * a.acc_stride + q] = acc645;
-  a.acc[(676 * R + r) * a.acc_stride + q] = acc676;
-  a.acc[(707 * R + r) * a.acc_stride + q] = acc707;
-  a.acc[(738 * R + r) * a.acc_stride + q] = acc738;
-  a.acc[(769 * R + r) * a.acc_stride + q] = acc769;
-}
-// class (22|22) split 10/12: 388 recursion values, 80 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_10(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc800 = 0.0;
-  double acc831 = 0.0;
-  double acc862 = 0.0;
-  double acc893 = 0.0;
-  double acc924 = 0.0;
-  double acc955 = 0.0;
-  double acc26 = 0.0;
-  double acc57 = 0.0;
-  double acc88 = 0.0;
-  double acc119 = 0.0;
-  double acc150 = 0.0;
-  double acc181 = 0.0;
-  double acc212 = 0.0;
-  double acc243 = 0.0;
-  double acc274 = 0.0;
-  double acc305 = 0.0;
-  double acc336 = 0.0;
-  double acc367 = 0.0;
-  double acc398 = 0.0;
-  double acc429 = 0.0;
-  double acc460 = 0.0;
-  double acc491 = 0.0;
-  double acc522 = 0.0;
-  double acc553 = 0.0;
-  double acc584 = 0.0;
-  double acc615 = 0.0;
-  double acc646 = 0.0;
-  double acc677 = 0.0;
-  double acc708 = 0.0;
-  double acc739 = 0.0;
-  double acc770 = 0.0;
-  double acc801 = 0.0;
-  double acc832 = 0.0;
-  double acc863 = 0.0;
-  double acc894 = 0.0;
-  double acc925 = 0.0;
-  double acc956 = 0.0;
-  double acc27 = 0.0;
-  double acc58 = 0.0;
-  double acc89 = 0.0;
-  double acc120 = 0.0;
-  double acc151 = 0.0;
-  double acc182 = 0.0;
-  double acc213 = 0.0;
-  double acc244 = 0.0;
-  double acc275 = 0.0;
-  double acc306 = 0.0;
-  double acc337 = 0.0;
-  double acc368 = 0.0;
-  double acc399 = 0.0;
-  double acc430 = 0.0;
-  double acc461 = 0.0;
-  double acc492 = 0.0;
-  double acc523 = 0.0;
-  double acc554 = 0.0;
-  double acc585 = 0.0;
-  double acc616 = 0.0;
-  double acc647 = 0.0;
-  double acc678 = 0.0;
-  double acc709 = 0.0;
-  double acc740 = 0.0;
-  double acc771 = 0.0;
-  double acc802 = 0.0;
-  double acc833 = 0.0;
-  double acc864 = 0.0;
-  double acc895 = 0.0;
-  double acc926 = 0.0;
-  double acc957 = 0.0;
-  double acc28 = 0.0;
-  double acc59 = 0.0;
-  double acc90 = 0.0;
-  double acc121 = 0.0;
-  double acc152 = 0.0;
-  double acc183 = 0.0;
-  double acc214 = 0.0;
-  double acc245 = 0.0;
-  double acc276 = 0.0;
-  double acc307 = 0.0;
-  double acc338 = 0.0;
-  double acc369 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB x8 = static_cast<TB>(F[8] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x9 = s.PA[0] * x0 + s.WP[0] * x1;
-      const TB x10 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x12 = s.PA[0] * x1 + s.WP[0] * x2;
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x36 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x15 = s.PA[0] * x2 + s.WP[0] * x3;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x18 = s.PA[0] * x3 + s.WP[0] * x4;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x21 = s.PA[0] * x4 + s.WP[0] * x5;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x24 = s.PA[0] * x5 + s.WP[0] * x6;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x27 = s.PA[0] * x6 + s.WP[0] * x7;
-      const TB x28 = s.PA[1] * x6 + s.WP[1] * x7;
-      const TB x29 = s.PA[2] * x6 + s.WP[2] * x7;
-      const TB x68 = s.PA[2] * x26 + s.WP[2] * x29 + z1 * (x5 - s.roz * x6);
-      const TB x30 = s.PA[0] * x7 + s.WP[0] * x8;
-      const TB x31 = s.PA[1] * x7 + s.WP[1] * x8;
-      const TB x72 = s.PA[1] * x28 + s.WP[1] * x31 + z1 * (x6 - s.roz * x7);
-      const TB x66 = s.PA[1] * x25 + s.WP[1] * x28 + z1 * (x5 - s.roz * x6);
-      const TB x131 = s.PA[1] * x66 + s.WP[1] * x72 + z2 * (x25 - s.roz * x28);
-      const TB x71 = s.PA[2] * x27 + s.WP[2] * x30;
-      const TB x35 = s.PA[2] * x9 + s.WP[2] * x12;
-      const TB x41 = s.PA[2] * x12 + s.WP[2] * x15;
-      const TB x80 = s.PA[2] * x35 + s.WP[2] * x41 + z1 * (x9 - s.roz * x12);
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x81 = s.PA[1] * x36 + s.WP[1] * x42 + z2 * (x10 - s.roz * x13);
-      const TB x43 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x47 = s.PA[2] * x15 + s.WP[2] * x18;
-      const TB x90 = s.PA[2] * x41 + s.WP[2] * x47 + z1 * (x12 - s.roz * x15);
-      const TB x144 = s.PA[2] * x80 + s.WP[2] * x90 + z2 * (x35 - s.roz * x41);
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x94 = s.PA[2] * x44 + s.WP[2] * x50 + z2 * (x14 - s.roz * x17);
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x93 = s.PA[2] * x43 + s.WP[2] * x49 + z1 * (x13 - s.roz * x16);
-      const TB x91 = s.PA[1] * x42 + s.WP[1] * x48 + z2 * (x13 - s.roz * x16);
-      const TB x53 = s.PA[2] * x18 + s.WP[2] * x21;
-      const TB x100 = s.PA[2] * x47 + s.WP[2] * x53 + z1 * (x15 - s.roz * x18);
-      const TB x159 = s.PA[2] * x90 + s.WP[2] * x100 + z2 * (x41 - s.roz * x47);
-      const TB x321 = s.QC[0] * x144 + s.WQ[0] * x159 + r1 * x94;
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x104 = s.PA[2] * x50 + s.WP[2] * x56 + z2 * (x17 - s.roz * x20);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x59 = s.PA[2] * x21 + s.WP[2] * x24;
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x124 = s.PA[2] * x62 + s.WP[2] * x68 + z2 * (x23 - s.roz * x26);
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x121 = s.PA[1] * x60 + s.WP[1] * x66 + z2 * (x22 - s.roz * x25);
-      const TB x65 = s.PA[2] * x24 + s.WP[2] * x27;
-      const TB x130 = s.PA[2] * x65 + s.WP[2] * x71 + z1 * (x24 - s.roz * x27);
-      const TB x120 = s.PA[2] * x59 + s.WP[2] * x65 + z1 * (x21 - s.roz * x24);
-      const TB x204 = s.PA[2] * x120 + s.WP[2] * x130 + z2 * (x59 - s.roz * x65);
-      const TB x82 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x92 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x147 = s.PA[2] * x82 + s.WP[2] * x92 + z1 * (x36 - s.roz * x42);
-      const TB x145 = s.PA[1] * x81 + s.WP[1] * x91 + z3 * (x36 - s.roz * x42);
-      const TB x146 = s.PA[2] * x81 + s.WP[2] * x91;
-      const TB x101 = s.PA[1] * x48 + s.WP[1] * x54 + z2 * (x16 - s.roz * x19);
-      const TB x103 = s.PA[2] * x49 + s.WP[2] * x55 + z1 * (x16 - s.roz * x19);
-      const TB x102 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x110 = s.PA[2] * x53 + s.WP[2] * x59 + z1 * (x18 - s.roz * x21);
-      const TB x111 = s.PA[1] * x54 + s.WP[1] * x60 + z2 * (x19 - s.roz * x22);
-      const TB x112 = s.PA[2] * x54 + s.WP[2] * x60;
-      const TB x113 = s.PA[2] * x55 + s.WP[2] * x61 + z1 * (x19 - s.roz * x22);
-      const TB x279 = s.QC[0] * x103 + s.WQ[0] * x113;
-      const TB x114 = s.PA[2] * x56 + s.WP[2] * x62 + z2 * (x20 - s.roz * x23);
-      const TB x268 = s.QC[0] * x110 + s.WQ[0] * x120 + r1 * x62;
-      const TB x122 = s.PA[2] * x60 + s.WP[2] * x66;
-      const TB x205 = s.PA[1] * x121 + s.WP[1] * x131 + z3 * (x60 - s.roz * x66);
-      const TB x206 = s.PA[2] * x121 + s.WP[2] * x131;
-      const TB x160 = s.PA[1] * x91 + s.WP[1] * x101 + z3 * (x42 - s.roz * x48);
-      const TB x162 = s.PA[2] * x92 + s.WP[2] * x102 + z1 * (x42 - s.roz * x48);
-      const TB x161 = s.PA[2] * x91 + s.WP[2] * x101;
-      const TB x174 = s.PA[2] * x100 + s.WP[2] * x110 + z2 * (x47 - s.roz * x53);
-      const TB x322 = s.QC[0] * x159 + s.WQ[0] * x174 + r1 * x104;
-      const TB x175 = s.PA[1] * x101 + s.WP[1] * x111 + z3 * (x48 - s.roz * x54);
-      const TB x176 = s.PA[2] * x101 + s.WP[2] * x111;
-      const TB x177 = s.PA[2] * x102 + s.WP[2] * x112 + z1 * (x48 - s.roz * x54);
-      const TB x189 = s.PA[2] * x110 + s.WP[2] * x120 + z2 * (x53 - s.roz * x59);
-      const TB x323 = s.QC[0] * x174 + s.WQ[0] * x189 + r1 * x114;
-      const TB x324 = s.QC[0] * x189 + s.WQ[0] * x204 + r1 * x124;
-      const TB x899 = s.QC[2] * x323 + s.WQ[2] * x324 + r3 * x268;
-      const TB x190 = s.PA[1] * x111 + s.WP[1] * x121 + z3 * (x54 - s.roz * x60);
-      const TB x191 = s.PA[2] * x111 + s.WP[2] * x121;
-      const TB x192 = s.PA[2] * x112 + s.WP[2] * x122 + z1 * (x54 - s.roz * x60);
-      const TB x335 = s.QC[0] * x177 + s.WQ[0] * x192;
-      const TB x214 = s.QC[0] * x18 + s.WQ[0] * x21 + r1 * x4;
-      const TB x352 = s.QC[1] * x19 + s.WQ[1] * x22 + r1 * x4;
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x370 = s.QC[1] * x48 + s.WQ[1] * x54 + r2 * x19;
-      const TB x371 = s.QC[1] * x54 + s.WQ[1] * x60 + r2 * x22;
-      const TB x933 = s.QC[1] * x370 + s.WQ[1] * x371 + e1 * (x48 - s.roe * x54) + r2 * x352;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x231 = s.QC[0] * x47 + s.WQ[0] * x53 + r1 * x20;
-      const TB x232 = s.QC[0] * x53 + s.WQ[0] * x59 + r1 * x23;
-      const TB x830 = s.QC[2] * x231 + s.WQ[2] * x232 + r1 * x214;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x407 = s.QC[1] * x111 + s.WQ[1] * x121 + r3 * x60;
-      const TB x733 = s.QC[1] * x235 + s.WQ[1] * x236 + r2 * x217;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x240 = s.QC[0] * x55 + s.WQ[0] * x61;
-      const TB x736 = s.QC[1] * x239 + s.WQ[1] * x240 + r1 * x220;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x267 = s.QC[0] * x100 + s.WQ[0] * x110 + r1 * x56;
-      const TB x857 = s.QC[2] * x267 + s.WQ[2] * x268 + r2 * x232;
-      const TB x266 = s.QC[0] * x90 + s.WQ[0] * x100 + r1 * x50;
-      const TB x413 = s.QC[1] * x93 + s.WQ[1] * x103 + r1 * x50;
-      const TB x856 = s.QC[2] * x266 + s.WQ[2] * x267 + r2 * x231;
-      const TB x1566 = s.QC[2] * x856 + s.WQ[2] * x857 + e1 * (x266 - s.roe * x267) + r2 * x830;
-      const TB x897 = s.QC[2] * x321 + s.WQ[2] * x322 + r3 * x266;
-      const TB x898 = s.QC[2] * x322 + s.WQ[2] * x323 + r3 * x267;
-      const TB x1594 = s.QC[2] * x898 + s.WQ[2] * x899 + e1 * (x322 - s.roe * x323) + r3 * x857;
-      const TB x1593 = s.QC[2] * x897 + s.WQ[2] * x898 + e1 * (x321 - s.roe * x322) + r3 * x856;
-      const TB x2169 = s.QC[2] * x1593 + s.WQ[2] * x1594 + e2 * (x897 - s.roe * x898) + r3 * x1566;
-      acc800 += static_cast<double>(x2169);
-      const TB x270 = s.QC[0] * x91 + s.WQ[0] * x101;
-      const TB x271 = s.QC[0] * x101 + s.WQ[0] * x111;
-      const TB x272 = s.QC[0] * x111 + s.WQ[0] * x121;
-      const TB x274 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x275 = s.QC[0] * x102 + s.WQ[0] * x112;
-      const TB x276 = s.QC[0] * x112 + s.WQ[0] * x122;
-      const TB x278 = s.QC[0] * x93 + s.WQ[0] * x103;
-      const TB x765 = s.QC[1] * x278 + s.WQ[1] * x279 + r1 * x243;
-      const TB x325 = s.QC[0] * x145 + s.WQ[0] * x160;
-      const TB x326 = s.QC[0] * x160 + s.WQ[0] * x175;
-      const TB x327 = s.QC[0] * x175 + s.WQ[0] * x190;
-      const TB x328 = s.QC[0] * x190 + s.WQ[0] * x205;
-      const TB x329 = s.QC[0] * x146 + s.WQ[0] * x161;
-      const TB x330 = s.QC[0] * x161 + s.WQ[0] * x176;
-      const TB x331 = s.QC[0] * x176 + s.WQ[0] * x191;
-      const TB x332 = s.QC[0] * x191 + s.WQ[0] * x206;
-      const TB x333 = s.QC[0] * x147 + s.WQ[0] * x162;
-      const TB x334 = s.QC[0] * x162 + s.WQ[0] * x177;
-      const TB x349 = s.QC[1] * x18 + s.WQ[1] * x21;
-      const TB x355 = s.QC[1] * x20 + s.WQ[1] * x23;
-      const TB x374 = s.QC[1] * x49 + s.WQ[1] * x55 + r1 * x20;
-      const TB x375 = s.QC[1] * x55 + s.WQ[1] * x61 + r1 * x23;
-      const TB x411 = s.QC[1] * x112 + s.WQ[1] * x122 + r2 * x61;
-      const TB x467 = s.QC[1] * x191 + s.WQ[1] * x206 + r3 * x122;
-      const TB x602 = s.QC[2] * x191 + s.WQ[2] * x206 + r1 * x121;
-      const TB x936 = s.QC[1] * x374 + s.WQ[1] * x375 + e1 * (x49 - s.roe * x55) + r1 * x355;
-      const TB x366 = s.QC[1] * x47 + s.WQ[1] * x53;
-      const TB x367 = s.QC[1] * x53 + s.WQ[1] * x59;
-      const TB x1030 = s.QC[2] * x366 + s.WQ[2] * x367 + r1 * x349;
-      const TB x401 = s.QC[1] * x90 + s.WQ[1] * x100;
-      const TB x402 = s.QC[1] * x100 + s.WQ[1] * x110;
-      const TB x403 = s.QC[1] * x110 + s.WQ[1] * x120;
-      const TB x405 = s.QC[1] * x91 + s.WQ[1] * x101 + r3 * x48;
-      const TB x406 = s.QC[1] * x101 + s.WQ[1] * x111 + r3 * x54;
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x544 = s.QC[2] * x92 + s.WQ[2] * x102 + r1 * x48;
-      const TB x760 = s.QC[1] * x271 + s.WQ[1] * x272 + r3 * x236;
-      const TB x959 = s.QC[1] * x405 + s.WQ[1] * x406 + e1 * (x91 - s.roe * x101) + r3 * x370;
-      const TB x960 = s.QC[1] * x406 + s.WQ[1] * x407 + e1 * (x101 - s.roe * x111) + r3 * x371;
-      const TB x1633 = s.QC[1] * x959 + s.WQ[1] * x960 + e2 * (x405 - s.roe * x406) + r3 * x933;
-      const TB x409 = s.QC[1] * x92 + s.WQ[1] * x102 + r2 * x49;
-      const TB x636 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x49 - s.roe * x55);
-      const TB x410 = s.QC[1] * x102 + s.WQ[1] * x112 + r2 * x55;
-      const TB x762 = s.QC[1] * x274 + s.WQ[1] * x275 + r2 * x239;
-      const TB x763 = s.QC[1] * x275 + s.WQ[1] * x276 + r2 * x240;
-      const TB x962 = s.QC[1] * x409 + s.WQ[1] * x410 + e1 * (x92 - s.roe * x102) + r2 * x374;
-      const TB x963 = s.QC[1] * x410 + s.WQ[1] * x411 + e1 * (x102 - s.roe * x112) + r2 * x375;
-      const TB x1440 = s.QC[1] * x762 + s.WQ[1] * x763 + e1 * (x274 - s.roe * x275) + r2 * x736;
-      const TB x1635 = s.QC[1] * x962 + s.WQ[1] * x963 + e2 * (x409 - s.roe * x410) + r2 * x936;
-      const TB x456 = s.QC[1] * x144 + s.WQ[1] * x159;
-      const TB x457 = s.QC[1] * x159 + s.WQ[1] * x174;
-      const TB x458 = s.QC[1] * x174 + s.WQ[1] * x189;
-      const TB x459 = s.QC[1] * x189 + s.WQ[1] * x204;
-      const TB x594 = s.QC[2] * x189 + s.WQ[2] * x204 + r3 * x120;
-      const TB x538 = s.QC[2] * x110 + s.WQ[2] * x120 + r2 * x59;
-      const TB x502 = s.QC[2] * x53 + s.WQ[2] * x59 + r1 * x21;
-      const TB x484 = s.QC[2] * x18 + s.WQ[2] * x21;
-      const TB x501 = s.QC[2] * x47 + s.WQ[2] * x53 + r1 * x18;
-      const TB x1130 = s.QC[2] * x501 + s.WQ[2] * x502 + e1 * (x47 - s.roe * x53) + r1 * x484;
-      const TB x460 = s.QC[1] * x145 + s.WQ[1] * x160 + r4 * x91;
-      const TB x461 = s.QC[1] * x160 + s.WQ[1] * x175 + r4 * x101;
-      const TB x462 = s.QC[1] * x175 + s.WQ[1] * x190 + r4 * x111;
-      const TB x463 = s.QC[1] * x190 + s.WQ[1] * x205 + r4 * x121;
-      const TB x542 = s.QC[2] * x111 + s.WQ[2] * x121;
-      const TB x598 = s.QC[2] * x190 + s.WQ[2] * x205;
-      const TB x464 = s.QC[1] * x146 + s.WQ[1] * x161 + r3 * x92;
-      const TB x465 = s.QC[1] * x161 + s.WQ[1] * x176 + r3 * x102;
-      const TB x466 = s.QC[1] * x176 + s.WQ[1] * x191 + r3 * x112;
-      const TB x663 = s.QC[0] * x275 + s.WQ[0] * x276 + e1 * (x102 - s.roe * x112);
-      const TB x805 = s.QC[1] * x331 + s.WQ[1] * x332 + r3 * x276;
-      const TB x1004 = s.QC[1] * x465 + s.WQ[1] * x466 + e1 * (x161 - s.roe * x176) + r3 * x410;
-      const TB x1005 = s.QC[1] * x466 + s.WQ[1] * x467 + e1 * (x176 - s.roe * x191) + r3 * x411;
-      const TB x1105 = s.QC[2] * x466 + s.WQ[2] * x467 + r1 * x407;
-      const TB x1663 = s.QC[1] * x1004 + s.WQ[1] * x1005 + e2 * (x465 - s.roe * x466) + r3 * x963;
-      const TB x1728 = s.QC[2] * x1004 + s.WQ[2] * x1005 + r1 * x960;
-      const TB x468 = s.QC[1] * x147 + s.WQ[1] * x162 + r2 * x93;
-      const TB x665 = s.QC[0] * x278 + s.WQ[0] * x279 + e1 * (x93 - s.roe * x103);
-      const TB x469 = s.QC[1] * x162 + s.WQ[1] * x177 + r2 * x103;
-      const TB x1006 = s.QC[1] * x468 + s.WQ[1] * x469 + e1 * (x147 - s.roe * x162) + r2 * x413;
-      acc121 += static_cast<double>(x1006);
-      const TB x1106 = s.QC[2] * x468 + s.WQ[2] * x469 + r2 * x409;
-      acc152 += static_cast<double>(x1106);
-      const TB x806 = s.QC[1] * x333 + s.WQ[1] * x334 + r2 * x278;
-      acc59 += static_cast<double>(x806);
-      const TB x807 = s.QC[1] * x334 + s.WQ[1] * x335 + r2 * x279;
-      const TB x1469 = s.QC[1] * x806 + s.WQ[1] * x807 + e1 * (x333 - s.roe * x334) + r2 * x765;
-      acc307 += static_cast<double>(x1469);
-      const TB x1534 = s.QC[2] * x806 + s.WQ[2] * x807 + r2 * x762;
-      acc338 += static_cast<double>(x1534);
-      const TB x1003 = s.QC[1] * x464 + s.WQ[1] * x465 + e1 * (x146 - s.roe * x161) + r3 * x409;
-      acc120 += static_cast<double>(x1003);
-      const TB x1662 = s.QC[1] * x1003 + s.WQ[1] * x1004 + e2 * (x464 - s.roe * x465) + r3 * x962;
-      acc399 += static_cast<double>(x1662);
-      const TB x2202 = s.QC[1] * x1662 + s.WQ[1] * x1663 + e3 * (x1003 - s.roe * x1004) + r3 * x1635;
-      acc833 += static_cast<double>(x2202);
-      const TB x2233 = s.QC[2] * x1662 + s.WQ[2] * x1663 + r1 * x1633;
-      acc864 += static_cast<double>(x2233);
-      const TB x536 = s.QC[2] * x90 + s.WQ[2] * x100 + r2 * x47;
-      const TB x930 = s.QC[1] * x366 + s.WQ[1] * x367 + e1 * (x47 - s.roe * x53);
-      const TB x537 = s.QC[2] * x100 + s.WQ[2] * x110 + r2 * x53;
-      const TB x1056 = s.QC[2] * x401 + s.WQ[2] * x402 + r2 * x366;
-      const TB x1057 = s.QC[2] * x402 + s.WQ[2] * x403 + r2 * x367;
-      const TB x1156 = s.QC[2] * x536 + s.WQ[2] * x537 + e1 * (x90 - s.roe * x100) + r2 * x501;
-      const TB x1157 = s.QC[2] * x537 + s.WQ[2] * x538 + e1 * (x100 - s.roe * x110) + r2 * x502;
-      const TB x1761 = s.QC[2] * x1056 + s.WQ[2] * x1057 + e1 * (x401 - s.roe * x402) + r2 * x1030;
-      const TB x1826 = s.QC[2] * x1156 + s.WQ[2] * x1157 + e2 * (x536 - s.roe * x537) + r2 * x1130;
-      const TB x540 = s.QC[2] * x91 + s.WQ[2] * x101;
-      const TB x541 = s.QC[2] * x101 + s.WQ[2] * x111;
-      const TB x591 = s.QC[2] * x144 + s.WQ[2] * x159 + r3 * x90;
-      const TB x956 = s.QC[1] * x401 + s.WQ[1] * x402 + e1 * (x90 - s.roe * x100);
-      const TB x592 = s.QC[2] * x159 + s.WQ[2] * x174 + r3 * x100;
-      const TB x957 = s.QC[1] * x402 + s.WQ[1] * x403 + e1 * (x100 - s.roe * x110);
-      const TB x593 = s.QC[2] * x174 + s.WQ[2] * x189 + r3 * x110;
-      const TB x1199 = s.QC[2] * x593 + s.WQ[2] * x594 + e1 * (x174 - s.roe * x189) + r3 * x538;
-      const TB x999 = s.QC[1] * x458 + s.WQ[1] * x459 + e1 * (x174 - s.roe * x189);
-      const TB x1099 = s.QC[2] * x458 + s.WQ[2] * x459 + r3 * x403;
-      const TB x1197 = s.QC[2] * x591 + s.WQ[2] * x592 + e1 * (x144 - s.roe * x159) + r3 * x536;
-      const TB x997 = s.QC[1] * x456 + s.WQ[1] * x457 + e1 * (x144 - s.roe * x159);
-      const TB x1198 = s.QC[2] * x592 + s.WQ[2] * x593 + e1 * (x159 - s.roe * x174) + r3 * x537;
-      const TB x1854 = s.QC[2] * x1198 + s.WQ[2] * x1199 + e2 * (x592 - s.roe * x593) + r3 * x1157;
-      const TB x1853 = s.QC[2] * x1197 + s.WQ[2] * x1198 + e2 * (x591 - s.roe * x592) + r3 * x1156;
-      const TB x2324 = s.QC[2] * x1853 + s.WQ[2] * x1854 + e3 * (x1197 - s.roe * x1198) + r3 * x1826;
-      acc955 += static_cast<double>(x2324);
-      const TB x998 = s.QC[1] * x457 + s.WQ[1] * x458 + e1 * (x159 - s.roe * x174);
-      const TB x1696 = s.QC[2] * x956 + s.WQ[2] * x957 + r2 * x930;
-      const TB x595 = s.QC[2] * x145 + s.WQ[2] * x160;
-      const TB x596 = s.QC[2] * x160 + s.WQ[2] * x175;
-      const TB x597 = s.QC[2] * x175 + s.WQ[2] * x190;
-      const TB x1202 = s.QC[2] * x597 + s.WQ[2] * x598 + e1 * (x175 - s.roe * x190);
-      const TB x599 = s.QC[2] * x146 + s.WQ[2] * x161 + r1 * x91;
-      const TB x600 = s.QC[2] * x161 + s.WQ[2] * x176 + r1 * x101;
-      const TB x601 = s.QC[2] * x176 + s.WQ[2] * x191 + r1 * x111;
-      const TB x1205 = s.QC[2] * x601 + s.WQ[2] * x602 + e1 * (x176 - s.roe * x191) + r1 * x542;
-      const TB x705 = s.QC[0] * x331 + s.WQ[0] * x332 + e1 * (x176 - s.roe * x191);
-      const TB x905 = s.QC[2] * x331 + s.WQ[2] * x332 + r1 * x272;
-      const TB x1160 = s.QC[2] * x541 + s.WQ[2] * x542 + e1 * (x101 - s.roe * x111);
-      const TB x660 = s.QC[0] * x271 + s.WQ[0] * x272 + e1 * (x101 - s.roe * x111);
-      const TB x603 = s.QC[2] * x147 + s.WQ[2] * x162 + r2 * x92;
-      const TB x662 = s.QC[0] * x274 + s.WQ[0] * x275 + e1 * (x92 - s.roe * x102);
-      const TB x604 = s.QC[2] * x162 + s.WQ[2] * x177 + r2 * x102;
-      const TB x1206 = s.QC[2] * x603 + s.WQ[2] * x604 + e1 * (x147 - s.roe * x162) + r2 * x544;
-      acc183 += static_cast<double>(x1206);
-      const TB x706 = s.QC[0] * x333 + s.WQ[0] * x334 + e1 * (x147 - s.roe * x162);
-      acc28 += static_cast<double>(x706);
-      const TB x707 = s.QC[0] * x334 + s.WQ[0] * x335 + e1 * (x162 - s.roe * x177);
-      const TB x1339 = s.QC[1] * x706 + s.WQ[1] * x707 + r2 * x665;
-      acc245 += static_cast<double>(x1339);
-      const TB x907 = s.QC[2] * x334 + s.WQ[2] * x335 + r2 * x275;
-      const TB x1274 = s.QC[0] * x706 + s.WQ[0] * x707 + e2 * (x333 - s.roe * x334);
-      acc214 += static_cast<double>(x1274);
-      const TB x1404 = s.QC[2] * x706 + s.WQ[2] * x707 + r2 * x662;
-      acc276 += static_cast<double>(x1404);
-      const TB x1310 = s.QC[1] * x662 + s.WQ[1] * x663 + r2 * x636;
-      const TB x659 = s.QC[0] * x270 + s.WQ[0] * x271 + e1 * (x91 - s.roe * x101);
-      const TB x1159 = s.QC[2] * x540 + s.WQ[2] * x541 + e1 * (x91 - s.roe * x101);
-      const TB x1308 = s.QC[1] * x659 + s.WQ[1] * x660 + r3 * x633;
-      const TB x700 = s.QC[0] * x325 + s.WQ[0] * x326 + e1 * (x145 - s.roe * x160);
-      acc26 += static_cast<double>(x700);
-      const TB x701 = s.QC[0] * x326 + s.WQ[0] * x327 + e1 * (x160 - s.roe * x175);
-      const TB x702 = s.QC[0] * x327 + s.WQ[0] * x328 + e1 * (x175 - s.roe * x190);
-      const TB x1002 = s.QC[1] * x462 + s.WQ[1] * x463 + e1 * (x175 - s.roe * x190) + r4 * x407;
-      const TB x1060 = s.QC[2] * x406 + s.WQ[2] * x407;
-      const TB x1102 = s.QC[2] * x462 + s.WQ[2] * x463;
-      const TB x703 = s.QC[0] * x329 + s.WQ[0] * x330 + e1 * (x146 - s.roe * x161);
-      acc27 += static_cast<double>(x703);
-      const TB x1203 = s.QC[2] * x599 + s.WQ[2] * x600 + e1 * (x146 - s.roe * x161) + r1 * x540;
-      acc182 += static_cast<double>(x1203);
-      const TB x1828 = s.QC[2] * x1159 + s.WQ[2] * x1160 + e2 * (x540 - s.roe * x541);
-      const TB x1204 = s.QC[2] * x600 + s.WQ[2] * x601 + e1 * (x161 - s.roe * x176) + r1 * x541;
-      const TB x1858 = s.QC[2] * x1204 + s.WQ[2] * x1205 + e2 * (x600 - s.roe * x601) + r1 * x1160;
-      const TB x1857 = s.QC[2] * x1203 + s.WQ[2] * x1204 + e2 * (x599 - s.roe * x600) + r1 * x1159;
-      acc492 += static_cast<double>(x1857);
-      const TB x2326 = s.QC[2] * x1857 + s.WQ[2] * x1858 + e3 * (x1203 - s.roe * x1204) + r1 * x1828;
-      acc957 += static_cast<double>(x2326);
-      const TB x704 = s.QC[0] * x330 + s.WQ[0] * x331 + e1 * (x161 - s.roe * x176);
-      const TB x759 = s.QC[1] * x270 + s.WQ[1] * x271 + r3 * x235;
-      const TB x862 = s.QC[2] * x274 + s.WQ[2] * x275 + r1 * x235;
-      const TB x1438 = s.QC[1] * x759 + s.WQ[1] * x760 + e1 * (x270 - s.roe * x271) + r3 * x733;
-      const TB x800 = s.QC[1] * x325 + s.WQ[1] * x326 + r4 * x270;
-      acc57 += static_cast<double>(x800);
-      const TB x801 = s.QC[1] * x326 + s.WQ[1] * x327 + r4 * x271;
-      const TB x802 = s.QC[1] * x327 + s.WQ[1] * x328 + r4 * x272;
-      const TB x860 = s.QC[2] * x271 + s.WQ[2] * x272;
-      const TB x902 = s.QC[2] * x327 + s.WQ[2] * x328;
-      const TB x803 = s.QC[1] * x329 + s.WQ[1] * x330 + r3 * x274;
-      acc58 += static_cast<double>(x803);
-      const TB x804 = s.QC[1] * x330 + s.WQ[1] * x331 + r3 * x275;
-      const TB x1245 = s.QC[0] * x662 + s.WQ[0] * x663 + e2 * (x274 - s.roe * x275);
-      const TB x906 = s.QC[2] * x333 + s.WQ[2] * x334 + r2 * x274;
-      acc90 += static_cast<double>(x906);
-      const TB x1599 = s.QC[2] * x906 + s.WQ[2] * x907 + e1 * (x333 - s.roe * x334) + r2 * x862;
-      acc369 += static_cast<double>(x1599);
-      const TB x1337 = s.QC[1] * x703 + s.WQ[1] * x704 + r3 * x662;
-      acc244 += static_cast<double>(x1337);
-      const TB x1338 = s.QC[1] * x704 + s.WQ[1] * x705 + r3 * x663;
-      const TB x1985 = s.QC[1] * x1337 + s.WQ[1] * x1338 + e1 * (x703 - s.roe * x704) + r3 * x1310;
-      acc616 += static_cast<double>(x1985);
-      const TB x2016 = s.QC[2] * x1337 + s.WQ[2] * x1338 + r1 * x1308;
-      acc647 += static_cast<double>(x2016);
-      const TB x1467 = s.QC[1] * x803 + s.WQ[1] * x804 + e1 * (x329 - s.roe * x330) + r3 * x762;
-      acc306 += static_cast<double>(x1467);
-      const TB x1468 = s.QC[1] * x804 + s.WQ[1] * x805 + e1 * (x330 - s.roe * x331) + r3 * x763;
-      const TB x2078 = s.QC[1] * x1467 + s.WQ[1] * x1468 + e2 * (x803 - s.roe * x804) + r3 * x1440;
-      acc709 += static_cast<double>(x2078);
-      const TB x2109 = s.QC[2] * x1467 + s.WQ[2] * x1468 + r1 * x1438;
-      acc740 += static_cast<double>(x2109);
-      const TB x1533 = s.QC[2] * x804 + s.WQ[2] * x805 + r1 * x760;
-      const TB x859 = s.QC[2] * x270 + s.WQ[2] * x271;
-      const TB x900 = s.QC[2] * x325 + s.WQ[2] * x326;
-      acc88 += static_cast<double>(x900);
-      const TB x901 = s.QC[2] * x326 + s.WQ[2] * x327;
-      const TB x1596 = s.QC[2] * x901 + s.WQ[2] * x902 + e1 * (x326 - s.roe * x327);
-      const TB x903 = s.QC[2] * x329 + s.WQ[2] * x330 + r1 * x270;
-      acc89 += static_cast<double>(x903);
-      const TB x904 = s.QC[2] * x330 + s.WQ[2] * x331 + r1 * x271;
-      const TB x1598 = s.QC[2] * x904 + s.WQ[2] * x905 + e1 * (x330 - s.roe * x331) + r1 * x860;
-      const TB x1273 = s.QC[0] * x704 + s.WQ[0] * x705 + e2 * (x330 - s.roe * x331);
-      const TB x1403 = s.QC[2] * x704 + s.WQ[2] * x705 + r1 * x660;
-      const TB x1568 = s.QC[2] * x859 + s.WQ[2] * x860 + e1 * (x270 - s.roe * x271);
-      const TB x1243 = s.QC[0] * x659 + s.WQ[0] * x660 + e2 * (x270 - s.roe * x271);
-      const TB x1597 = s.QC[2] * x903 + s.WQ[2] * x904 + e1 * (x329 - s.roe * x330) + r1 * x859;
-      acc368 += static_cast<double>(x1597);
-      const TB x2171 = s.QC[2] * x1597 + s.WQ[2] * x1598 + e2 * (x903 - s.roe * x904) + r1 * x1568;
-      acc802 += static_cast<double>(x2171);
-      const TB x1272 = s.QC[0] * x703 + s.WQ[0] * x704 + e2 * (x329 - s.roe * x330);
-      acc213 += static_cast<double>(x1272);
-      const TB x1923 = s.QC[1] * x1272 + s.WQ[1] * x1273 + r3 * x1245;
-      acc554 += static_cast<double>(x1923);
-      const TB x1892 = s.QC[0] * x1272 + s.WQ[0] * x1273 + e3 * (x703 - s.roe * x704);
-      acc523 += static_cast<double>(x1892);
-      const TB x1954 = s.QC[2] * x1272 + s.WQ[2] * x1273 + r1 * x1243;
-      acc585 += static_cast<double>(x1954);
-      const TB x1000 = s.QC[1] * x460 + s.WQ[1] * x461 + e1 * (x145 - s.roe * x160) + r4 * x405;
-      acc119 += static_cast<double>(x1000);
-      const TB x1200 = s.QC[2] * x595 + s.WQ[2] * x596 + e1 * (x145 - s.roe * x160);
-      acc181 += static_cast<double>(x1200);
-      const TB x1001 = s.QC[1] * x461 + s.WQ[1] * x462 + e1 * (x160 - s.roe * x175) + r4 * x406;
-      const TB x1201 = s.QC[2] * x596 + s.WQ[2] * x597 + e1 * (x160 - s.roe * x175);
-      const TB x1856 = s.QC[2] * x1201 + s.WQ[2] * x1202 + e2 * (x596 - s.roe * x597);
-      const TB x1855 = s.QC[2] * x1200 + s.WQ[2] * x1201 + e2 * (x595 - s.roe * x596);
-      acc491 += static_cast<double>(x1855);
-      const TB x2325 = s.QC[2] * x1855 + s.WQ[2] * x1856 + e3 * (x1200 - s.roe * x1201);
-      acc956 += static_cast<double>(x2325);
-      const TB x1059 = s.QC[2] * x405 + s.WQ[2] * x406;
-      const TB x1097 = s.QC[2] * x456 + s.WQ[2] * x457 + r3 * x401;
-      const TB x1631 = s.QC[1] * x956 + s.WQ[1] * x957 + e2 * (x401 - s.roe * x402);
-      const TB x1098 = s.QC[2] * x457 + s.WQ[2] * x458 + r3 * x402;
-      const TB x1789 = s.QC[2] * x1098 + s.WQ[2] * x1099 + e1 * (x457 - s.roe * x458) + r3 * x1057;
-      const TB x1659 = s.QC[1] * x998 + s.WQ[1] * x999 + e2 * (x457 - s.roe * x458);
-      const TB x1724 = s.QC[2] * x998 + s.WQ[2] * x999 + r3 * x957;
-      const TB x1723 = s.QC[2] * x997 + s.WQ[2] * x998 + r3 * x956;
-      const TB x2262 = s.QC[2] * x1723 + s.WQ[2] * x1724 + e1 * (x997 - s.roe * x998) + r3 * x1696;
-      acc893 += static_cast<double>(x2262);
-      const TB x1788 = s.QC[2] * x1097 + s.WQ[2] * x1098 + e1 * (x456 - s.roe * x457) + r3 * x1056;
-      const TB x2293 = s.QC[2] * x1788 + s.WQ[2] * x1789 + e2 * (x1097 - s.roe * x1098) + r3 * x1761;
-      acc924 += static_cast<double>(x2293);
-      const TB x1658 = s.QC[1] * x997 + s.WQ[1] * x998 + e2 * (x456 - s.roe * x457);
-      const TB x2200 = s.QC[1] * x1658 + s.WQ[1] * x1659 + e3 * (x997 - s.roe * x998);
-      acc831 += static_cast<double>(x2200);
-      const TB x2231 = s.QC[2] * x1658 + s.WQ[2] * x1659 + r3 * x1631;
-      acc862 += static_cast<double>(x2231);
-      const TB x1100 = s.QC[2] * x460 + s.WQ[2] * x461;
-      acc150 += static_cast<double>(x1100);
-      const TB x1101 = s.QC[2] * x461 + s.WQ[2] * x462;
-      const TB x1791 = s.QC[2] * x1101 + s.WQ[2] * x1102 + e1 * (x461 - s.roe * x462);
-      const TB x1661 = s.QC[1] * x1001 + s.WQ[1] * x1002 + e2 * (x461 - s.roe * x462) + r4 * x960;
-      const TB x1698 = s.QC[2] * x959 + s.WQ[2] * x960;
-      const TB x1726 = s.QC[2] * x1001 + s.WQ[2] * x1002;
-      const TB x1103 = s.QC[2] * x464 + s.WQ[2] * x465 + r1 * x405;
-      acc151 += static_cast<double>(x1103);
-      const TB x1763 = s.QC[2] * x1059 + s.WQ[2] * x1060 + e1 * (x405 - s.roe * x406);
-      const TB x1104 = s.QC[2] * x465 + s.WQ[2] * x466 + r1 * x406;
-      const TB x1793 = s.QC[2] * x1104 + s.WQ[2] * x1105 + e1 * (x465 - s.roe * x466) + r1 * x1060;
-      const TB x1792 = s.QC[2] * x1103 + s.WQ[2] * x1104 + e1 * (x464 - s.roe * x465) + r1 * x1059;
-      acc461 += static_cast<double>(x1792);
-      const TB x2295 = s.QC[2] * x1792 + s.WQ[2] * x1793 + e2 * (x1103 - s.roe * x1104) + r1 * x1763;
-      acc926 += static_cast<double>(x2295);
-      const TB x1270 = s.QC[0] * x700 + s.WQ[0] * x701 + e2 * (x325 - s.roe * x326);
-      acc212 += static_cast<double>(x1270);
-      const TB x1271 = s.QC[0] * x701 + s.WQ[0] * x702 + e2 * (x326 - s.roe * x327);
-      const TB x1922 = s.QC[1] * x1270 + s.WQ[1] * x1271 + r4 * x1243;
-      acc553 += static_cast<double>(x1922);
-      const TB x1466 = s.QC[1] * x801 + s.WQ[1] * x802 + e1 * (x326 - s.roe * x327) + r4 * x760;
-      const TB x1503 = s.QC[2] * x759 + s.WQ[2] * x760;
-      const TB x1531 = s.QC[2] * x801 + s.WQ[2] * x802;
-      const TB x1891 = s.QC[0] * x1270 + s.WQ[0] * x1271 + e3 * (x700 - s.roe * x701);
-      acc522 += static_cast<double>(x1891);
-      const TB x1953 = s.QC[2] * x1270 + s.WQ[2] * x1271;
-      acc584 += static_cast<double>(x1953);
-      const TB x1335 = s.QC[1] * x700 + s.WQ[1] * x701 + r4 * x659;
-      acc243 += static_cast<double>(x1335);
-      const TB x1336 = s.QC[1] * x701 + s.WQ[1] * x702 + r4 * x660;
-      const TB x1984 = s.QC[1] * x1335 + s.WQ[1] * x1336 + e1 * (x700 - s.roe * x701) + r4 * x1308;
-      acc615 += static_cast<double>(x1984);
-      const TB x2015 = s.QC[2] * x1335 + s.WQ[2] * x1336;
-      acc646 += static_cast<double>(x2015);
-      const TB x1373 = s.QC[2] * x659 + s.WQ[2] * x660;
-      const TB x1401 = s.QC[2] * x701 + s.WQ[2] * x702;
-      const TB x1402 = s.QC[2] * x703 + s.WQ[2] * x704 + r1 * x659;
-      acc275 += static_cast<double>(x1402);
-      const TB x2047 = s.QC[2] * x1402 + s.WQ[2] * x1403 + e1 * (x703 - s.roe * x704) + r1 * x1373;
-      acc678 += static_cast<double>(x2047);
-      const TB x1400 = s.QC[2] * x700 + s.WQ[2] * x701;
-      acc274 += static_cast<double>(x1400);
-      const TB x2046 = s.QC[2] * x1400 + s.WQ[2] * x1401 + e1 * (x700 - s.roe * x701);
-      acc677 += static_cast<double>(x2046);
-      const TB x1465 = s.QC[1] * x800 + s.WQ[1] * x801 + e1 * (x325 - s.roe * x326) + r4 * x759;
-      acc305 += static_cast<double>(x1465);
-      const TB x1595 = s.QC[2] * x900 + s.WQ[2] * x901 + e1 * (x325 - s.roe * x326);
-      acc367 += static_cast<double>(x1595);
-      const TB x2170 = s.QC[2] * x1595 + s.WQ[2] * x1596 + e2 * (x900 - s.roe * x901);
-      acc801 += static_cast<double>(x2170);
-      const TB x2077 = s.QC[1] * x1465 + s.WQ[1] * x1466 + e2 * (x800 - s.roe * x801) + r4 * x1438;
-      acc708 += static_cast<double>(x2077);
-      const TB x2108 = s.QC[2] * x1465 + s.WQ[2] * x1466;
-      acc739 += static_cast<double>(x2108);
-      const TB x1532 = s.QC[2] * x803 + s.WQ[2] * x804 + r1 * x759;
-      acc337 += static_cast<double>(x1532);
-      const TB x2140 = s.QC[2] * x1532 + s.WQ[2] * x1533 + e1 * (x803 - s.roe * x804) + r1 * x1503;
-      acc771 += static_cast<double>(x2140);
-      const TB x1530 = s.QC[2] * x800 + s.WQ[2] * x801;
-      acc336 += static_cast<double>(x1530);
-      const TB x2139 = s.QC[2] * x1530 + s.WQ[2] * x1531 + e1 * (x800 - s.roe * x801);
-      acc770 += static_cast<double>(x2139);
-      const TB x1660 = s.QC[1] * x1000 + s.WQ[1] * x1001 + e2 * (x460 - s.roe * x461) + r4 * x959;
-      acc398 += static_cast<double>(x1660);
-      const TB x1790 = s.QC[2] * x1100 + s.WQ[2] * x1101 + e1 * (x460 - s.roe * x461);
-      acc460 += static_cast<double>(x1790);
-      const TB x2294 = s.QC[2] * x1790 + s.WQ[2] * x1791 + e2 * (x1100 - s.roe * x1101);
-      acc925 += static_cast<double>(x2294);
-      const TB x2201 = s.QC[1] * x1660 + s.WQ[1] * x1661 + e3 * (x1000 - s.roe * x1001) + r4 * x1633;
-      acc832 += static_cast<double>(x2201);
-      const TB x2232 = s.QC[2] * x1660 + s.WQ[2] * x1661;
-      acc863 += static_cast<double>(x2232);
-      const TB x1727 = s.QC[2] * x1003 + s.WQ[2] * x1004 + r1 * x959;
-      acc430 += static_cast<double>(x1727);
-      const TB x2264 = s.QC[2] * x1727 + s.WQ[2] * x1728 + e1 * (x1003 - s.roe * x1004) + r1 * x1698;
-      acc895 += static_cast<double>(x2264);
-      const TB x1725 = s.QC[2] * x1000 + s.WQ[2] * x1001;
-      acc429 += static_cast<double>(x1725);
-      const TB x2263 = s.QC[2] * x1725 + s.WQ[2] * x1726 + e1 * (x1000 - s.roe * x1001);
-      acc894 += static_cast<double>(x2263);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(800 * R + r) * a.acc_stride + q] = acc800;
-  a.acc[(831 * R + r) * a.acc_stride + q] = acc831;
-  a.acc[(862 * R + r) * a.acc_stride + q] = acc862;
-  a.acc[(893 * R + r) * a.acc_stride + q] = acc893;
-  a.acc[(924 * R + r) * a.acc_stride + q] = acc924;
-  a.acc[(955 * R + r) * a.acc_stride + q] = acc955;
-  a.acc[(26 * R + r) * a.acc_stride + q] = acc26;
-  a.acc[(57 * R + r) * a.acc_stride + q] = acc57;
-  a.acc[(88 * R + r) * a.acc_stride + q] = acc88;
-  a.acc[(119 * R + r) * a.acc_stride + q] = acc119;
-  a.acc[(150 * R + r) * a.acc_stride + q] = acc150;
-  a.acc[(181 * R + r) * a.acc_stride + q] = acc181;
-  a.acc[(212 * R + r) * a.acc_stride + q] = acc212;
-  a.acc[(243 * R + r) * a.acc_stride + q] = acc243;
-  a.acc[(274 * R + r) * a.acc_stride + q] = acc274;
-  a.acc[(305 * R + r) * a.acc_stride + q] = acc305;
-  a.acc[(336 * R + r) * a.acc_stride + q] = acc336;
-  a.acc[(367 * R + r) * a.acc_stride + q] = acc367;
-  a.acc[(398 * R + r) * a.acc_stride + q] = acc398;
-  a.acc[(429 * R + r) * a.acc_stride + q] = acc429;
-  a.acc[(460 * R + r) * a.acc_stride + q] = acc460;
-  a.acc[(491 * R + r) * a.acc_stride + q] = acc491;
-  a.acc[(522 * R + r) * a.acc_stride + q] = acc522;
-  a.acc[(553 * R + r) * a.acc_stride + q] = acc553;
-  a.acc[(584 * R + r) * a.acc_stride + q] = acc584;
-  a.acc[(615 * R + r) * a.acc_stride + q] = acc615;
-  a.acc[(646 * R + r) * a.acc_stride + q] = acc646;
-  a.acc[(677 * R + r) * a.acc_stride + q] = acc677;
-  a.acc[(708 * R + r) * a.acc_stride + q] = acc708;
-  a.acc[(739 * R + r) * a.acc_stride + q] = acc739;
-  a.acc[(770 * R + r) * a.acc_stride + q] = acc770;
-  a.acc[(801 * R + r) * a.acc_stride + q] = acc801;
-  a.acc[(832 * R + r) * a.acc_stride + q] = acc832;
-  a.acc[(863 * R + r) * a.acc_stride + q] = acc863;
-  a.acc[(894 * R + r) * a.acc_stride + q] = acc894;
-  a.acc[(925 * R + r) * a.acc_stride + q] = acc925;
-  a.acc[(956 * R + r) * a.acc_stride + q] = acc956;
-  a.acc[(27 * R + r) * a.acc_stride + q] = acc27;
-  a.acc[(58 * R + r) * a.acc_stride + q] = acc58;
-  a.acc[(89 * R + r) * a.acc_stride + q] = acc89;
-  a.acc[(120 * R + r) * a.acc_stride + q] = acc120;
-  a.acc[(151 * R + r) * a.acc_stride + q] = acc151;
-  a.acc[(182 * R + r) * a.acc_stride + q] = acc182;
-  a.acc[(213 * R + r) * a.acc_stride + q] = acc213;
-  a.acc[(244 * R + r) * a.acc_stride + q] = acc244;
-  a.acc[(275 * R + r) * a.acc_stride + q] = acc275;
-  a.acc[(306 * R + r) * a.acc_stride + q] = acc306;
-  a.acc[(337 * R + r) * a.acc_stride + q] = acc337;
-  a.acc[(368 * R + r) * a.acc_stride + q] = acc368;
-  a.acc[(399 * R + r) * a.acc_stride + q] = acc399;
-  a.acc[(430 * R + r) * a.acc_stride + q] = acc430;
-  a.acc[(461 * R + r) * a.acc_stride + q] = acc461;
-  a.acc[(492 * R + r) * a.acc_stride + q] = acc492;
-  a.acc[(523 * R + r) * a.acc_stride + q] = acc523;
-  a.acc[(554 * R + r) * a.acc_stride + q] = acc554;
-  a.acc[(585 * R + r) * a.acc_stride + q] = acc585;
-  a.acc[(616 * R + r) * a.acc_stride + q] = acc616;
-  a.acc[(647 * R + r) * a.acc_stride + q] = acc647;
-  a.acc[(678 * R + r) * a.acc_stride + q] = acc678;
-  a.acc[(709 * R + r) * a.acc_stride + q] = acc709;
-  a.acc[(740 * R + r) * a.acc_stride + q] = acc740;
-  a.acc[(771 * R + r) * a.acc_stride + q] = acc771;
-  a.acc[(802 * R + r) * a.acc_stride + q] = acc802;
-  a.acc[(833 * R + r) * a.acc_stride + q] = acc833;
-  a.acc[(864 * R + r) * a.acc_stride + q] = acc864;
-  a.acc[(895 * R + r) * a.acc_stride + q] = acc895;
-  a.acc[(926 * R + r) * a.acc_stride + q] = acc926;
-  a.acc[(957 * R + r) * a.acc_stride + q] = acc957;
-  a.acc[(28 * R + r) * a.acc_stride + q] = acc28;
-  a.acc[(59 * R + r) * a.acc_stride + q] = acc59;
-  a.acc[(90 * R + r) * a.acc_stride + q] = acc90;
-  a.acc[(121 * R + r) * a.acc_stride + q] = acc121;
-  a.acc[(152 * R + r) * a.acc_stride + q] = acc152;
-  a.acc[(183 * R + r) * a.acc_stride + q] = acc183;
-  a.acc[(214 * R + r) * a.acc_stride + q] = acc214;
-  a.acc[(245 * R + r) * a.acc_stride + q] = acc245;
-  a.acc[(276 * R + r) * a.acc_stride + q] = acc276;
-  a.acc[(307 * R + r) * a.acc_stride + q] = acc307;
-  a.acc[(338 * R + r) * a.acc_stride + q] = acc338;
-  a.acc[(369 * R + r) * a.acc_stride + q] = acc369;
-}
-// class (22|22) split 11/12: 400 recursion values, 81 accumulators
-template <typename TB>
-__global__ __launch_bounds__(kRegThreads) void eri_split_2222_11(gbx_tables t, RegArgs a) {
-  const int64_t gid = static_cast<int64_t>(blockIdx.x) * kRegThreads + threadIdx.x;
-  if (gid >= a.nquart * a.lanes_per_quartet) return;
-  const int r = static_cast<int>(gid / a.nquart);
-  const int64_t q = gid - static_cast<int64_t>(r) * a.nquart;
-  if (a.count && q >= *a.count) return;
-  const int bra = a.quart[2 * q], ket = a.quart[2 * q + 1];
-  const int4 pb = reinterpret_cast<const int4*>(t.pair_info)[bra];
-  const int4 pk = reinterpret_cast<const int4*>(t.pair_info)[ket];
-  double acc400 = 0.0;
-  double acc431 = 0.0;
-  double acc462 = 0.0;
-  double acc493 = 0.0;
-  double acc524 = 0.0;
-  double acc555 = 0.0;
-  double acc586 = 0.0;
-  double acc617 = 0.0;
-  double acc648 = 0.0;
-  double acc679 = 0.0;
-  double acc710 = 0.0;
-  double acc741 = 0.0;
-  double acc772 = 0.0;
-  double acc803 = 0.0;
-  double acc834 = 0.0;
-  double acc865 = 0.0;
-  double acc896 = 0.0;
-  double acc927 = 0.0;
-  double acc958 = 0.0;
-  double acc29 = 0.0;
-  double acc60 = 0.0;
-  double acc91 = 0.0;
-  double acc122 = 0.0;
-  double acc153 = 0.0;
-  double acc184 = 0.0;
-  double acc215 = 0.0;
-  double acc246 = 0.0;
-  double acc277 = 0.0;
-  double acc308 = 0.0;
-  double acc339 = 0.0;
-  double acc370 = 0.0;
-  double acc401 = 0.0;
-  double acc432 = 0.0;
-  double acc463 = 0.0;
-  double acc494 = 0.0;
-  double acc525 = 0.0;
-  double acc556 = 0.0;
-  double acc587 = 0.0;
-  double acc618 = 0.0;
-  double acc649 = 0.0;
-  double acc680 = 0.0;
-  double acc711 = 0.0;
-  double acc742 = 0.0;
-  double acc773 = 0.0;
-  double acc804 = 0.0;
-  double acc835 = 0.0;
-  double acc866 = 0.0;
-  double acc897 = 0.0;
-  double acc928 = 0.0;
-  double acc959 = 0.0;
-  double acc30 = 0.0;
-  double acc61 = 0.0;
-  double acc92 = 0.0;
-  double acc123 = 0.0;
-  double acc154 = 0.0;
-  double acc185 = 0.0;
-  double acc216 = 0.0;
-  double acc247 = 0.0;
-  double acc278 = 0.0;
-  double acc309 = 0.0;
-  double acc340 = 0.0;
-  double acc371 = 0.0;
-  double acc402 = 0.0;
-  double acc433 = 0.0;
-  double acc464 = 0.0;
-  double acc495 = 0.0;
-  double acc526 = 0.0;
-  double acc557 = 0.0;
-  double acc588 = 0.0;
-  double acc619 = 0.0;
-  double acc650 = 0.0;
-  double acc681 = 0.0;
-  double acc712 = 0.0;
-  double acc743 = 0.0;
-  double acc774 = 0.0;
-  double acc805 = 0.0;
-  double acc836 = 0.0;
-  double acc867 = 0.0;
-  double acc898 = 0.0;
-  double acc929 = 0.0;
-  double acc960 = 0.0;
-  const int nprimq = pb.w * pk.w;
-  for (int pq = r; pq < nprimq; pq += a.lanes_per_quartet) {
-    const int ib = pq / pk.w, ik = pq - (pq / pk.w) * pk.w;
-    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-    {
-      const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-      PrimQuartet<TB> s;
-      const double T = prim_quartet_setup(rb, rk, s);
-      double F[9];
-      boys_eval_fixed<8>(T, t, F);
-      const TB x0 = static_cast<TB>(F[0] * s.pref);
-      const TB x1 = static_cast<TB>(F[1] * s.pref);
-      const TB x2 = static_cast<TB>(F[2] * s.pref);
-      const TB x3 = static_cast<TB>(F[3] * s.pref);
-      const TB x4 = static_cast<TB>(F[4] * s.pref);
-      const TB x5 = static_cast<TB>(F[5] * s.pref);
-      const TB x6 = static_cast<TB>(F[6] * s.pref);
-      const TB x7 = static_cast<TB>(F[7] * s.pref);
-      const TB x8 = static_cast<TB>(F[8] * s.pref);
-      const TB z1 = static_cast<TB>(1) * s.i2z;
-      const TB z2 = static_cast<TB>(2) * s.i2z;
-      const TB z3 = static_cast<TB>(3) * s.i2z;
-      const TB e1 = static_cast<TB>(1) * s.i2e;
-      const TB e2 = static_cast<TB>(2) * s.i2e;
-      const TB e3 = static_cast<TB>(3) * s.i2e;
-      const TB r1 = static_cast<TB>(1) * s.i2s;
-      const TB r2 = static_cast<TB>(2) * s.i2s;
-      const TB r3 = static_cast<TB>(3) * s.i2s;
-      const TB r4 = static_cast<TB>(4) * s.i2s;
-      const TB x10 = s.PA[1] * x0 + s.WP[1] * x1;
-      const TB x11 = s.PA[2] * x0 + s.WP[2] * x1;
-      const TB x13 = s.PA[1] * x1 + s.WP[1] * x2;
-      const TB x14 = s.PA[2] * x1 + s.WP[2] * x2;
-      const TB x16 = s.PA[1] * x2 + s.WP[1] * x3;
-      const TB x17 = s.PA[2] * x2 + s.WP[2] * x3;
-      const TB x19 = s.PA[1] * x3 + s.WP[1] * x4;
-      const TB x20 = s.PA[2] * x3 + s.WP[2] * x4;
-      const TB x22 = s.PA[1] * x4 + s.WP[1] * x5;
-      const TB x23 = s.PA[2] * x4 + s.WP[2] * x5;
-      const TB x25 = s.PA[1] * x5 + s.WP[1] * x6;
-      const TB x26 = s.PA[2] * x5 + s.WP[2] * x6;
-      const TB x28 = s.PA[1] * x6 + s.WP[1] * x7;
-      const TB x29 = s.PA[2] * x6 + s.WP[2] * x7;
-      const TB x31 = s.PA[1] * x7 + s.WP[1] * x8;
-      const TB x32 = s.PA[2] * x7 + s.WP[2] * x8;
-      const TB x74 = s.PA[2] * x29 + s.WP[2] * x32 + z1 * (x6 - s.roz * x7);
-      const TB x72 = s.PA[1] * x28 + s.WP[1] * x31 + z1 * (x6 - s.roz * x7);
-      const TB x73 = s.PA[2] * x28 + s.WP[2] * x31;
-      const TB x36 = s.PA[1] * x10 + s.WP[1] * x13 + z1 * (x0 - s.roz * x1);
-      const TB x38 = s.PA[2] * x11 + s.WP[2] * x14 + z1 * (x0 - s.roz * x1);
-      const TB x37 = s.PA[2] * x10 + s.WP[2] * x13;
-      const TB x42 = s.PA[1] * x13 + s.WP[1] * x16 + z1 * (x1 - s.roz * x2);
-      const TB x44 = s.PA[2] * x14 + s.WP[2] * x17 + z1 * (x1 - s.roz * x2);
-      const TB x84 = s.PA[2] * x38 + s.WP[2] * x44 + z2 * (x11 - s.roz * x14);
-      const TB x43 = s.PA[2] * x13 + s.WP[2] * x16;
-      const TB x83 = s.PA[2] * x37 + s.WP[2] * x43 + z1 * (x10 - s.roz * x13);
-      const TB x48 = s.PA[1] * x16 + s.WP[1] * x19 + z1 * (x2 - s.roz * x3);
-      const TB x50 = s.PA[2] * x17 + s.WP[2] * x20 + z1 * (x2 - s.roz * x3);
-      const TB x94 = s.PA[2] * x44 + s.WP[2] * x50 + z2 * (x14 - s.roz * x17);
-      const TB x149 = s.PA[2] * x84 + s.WP[2] * x94 + z3 * (x38 - s.roz * x44);
-      const TB x49 = s.PA[2] * x16 + s.WP[2] * x19;
-      const TB x93 = s.PA[2] * x43 + s.WP[2] * x49 + z1 * (x13 - s.roz * x16);
-      const TB x148 = s.PA[2] * x83 + s.WP[2] * x93 + z2 * (x37 - s.roz * x43);
-      const TB x54 = s.PA[1] * x19 + s.WP[1] * x22 + z1 * (x3 - s.roz * x4);
-      const TB x56 = s.PA[2] * x20 + s.WP[2] * x23 + z1 * (x3 - s.roz * x4);
-      const TB x104 = s.PA[2] * x50 + s.WP[2] * x56 + z2 * (x17 - s.roz * x20);
-      const TB x164 = s.PA[2] * x94 + s.WP[2] * x104 + z3 * (x44 - s.roz * x50);
-      const TB x55 = s.PA[2] * x19 + s.WP[2] * x22;
-      const TB x103 = s.PA[2] * x49 + s.WP[2] * x55 + z1 * (x16 - s.roz * x19);
-      const TB x163 = s.PA[2] * x93 + s.WP[2] * x103 + z2 * (x43 - s.roz * x49);
-      const TB x60 = s.PA[1] * x22 + s.WP[1] * x25 + z1 * (x4 - s.roz * x5);
-      const TB x61 = s.PA[2] * x22 + s.WP[2] * x25;
-      const TB x62 = s.PA[2] * x23 + s.WP[2] * x26 + z1 * (x4 - s.roz * x5);
-      const TB x66 = s.PA[1] * x25 + s.WP[1] * x28 + z1 * (x5 - s.roz * x6);
-      const TB x68 = s.PA[2] * x26 + s.WP[2] * x29 + z1 * (x5 - s.roz * x6);
-      const TB x134 = s.PA[2] * x68 + s.WP[2] * x74 + z2 * (x26 - s.roz * x29);
-      const TB x124 = s.PA[2] * x62 + s.WP[2] * x68 + z2 * (x23 - s.roz * x26);
-      const TB x209 = s.PA[2] * x124 + s.WP[2] * x134 + z3 * (x62 - s.roz * x68);
-      const TB x132 = s.PA[2] * x66 + s.WP[2] * x72;
-      const TB x67 = s.PA[2] * x25 + s.WP[2] * x28;
-      const TB x133 = s.PA[2] * x67 + s.WP[2] * x73 + z1 * (x25 - s.roz * x28);
-      const TB x123 = s.PA[2] * x61 + s.WP[2] * x67 + z1 * (x22 - s.roz * x25);
-      const TB x208 = s.PA[2] * x123 + s.WP[2] * x133 + z2 * (x61 - s.roz * x67);
-      const TB x82 = s.PA[2] * x36 + s.WP[2] * x42;
-      const TB x92 = s.PA[2] * x42 + s.WP[2] * x48;
-      const TB x147 = s.PA[2] * x82 + s.WP[2] * x92 + z1 * (x36 - s.roz * x42);
-      const TB x102 = s.PA[2] * x48 + s.WP[2] * x54;
-      const TB x162 = s.PA[2] * x92 + s.WP[2] * x102 + z1 * (x42 - s.roz * x48);
-      const TB x112 = s.PA[2] * x54 + s.WP[2] * x60;
-      const TB x113 = s.PA[2] * x55 + s.WP[2] * x61 + z1 * (x19 - s.roz * x22);
-      const TB x114 = s.PA[2] * x56 + s.WP[2] * x62 + z2 * (x20 - s.roz * x23);
-      const TB x122 = s.PA[2] * x60 + s.WP[2] * x66;
-      const TB x207 = s.PA[2] * x122 + s.WP[2] * x132 + z1 * (x60 - s.roz * x66);
-      const TB x177 = s.PA[2] * x102 + s.WP[2] * x112 + z1 * (x48 - s.roz * x54);
-      const TB x178 = s.PA[2] * x103 + s.WP[2] * x113 + z2 * (x49 - s.roz * x55);
-      const TB x179 = s.PA[2] * x104 + s.WP[2] * x114 + z3 * (x50 - s.roz * x56);
-      const TB x192 = s.PA[2] * x112 + s.WP[2] * x122 + z1 * (x54 - s.roz * x60);
-      const TB x193 = s.PA[2] * x113 + s.WP[2] * x123 + z2 * (x55 - s.roz * x61);
-      const TB x194 = s.PA[2] * x114 + s.WP[2] * x124 + z3 * (x56 - s.roz * x62);
-      const TB x217 = s.QC[0] * x19 + s.WQ[0] * x22;
-      const TB x220 = s.QC[0] * x20 + s.WQ[0] * x23;
-      const TB x235 = s.QC[0] * x48 + s.WQ[0] * x54;
-      const TB x236 = s.QC[0] * x54 + s.WQ[0] * x60;
-      const TB x239 = s.QC[0] * x49 + s.WQ[0] * x55;
-      const TB x240 = s.QC[0] * x55 + s.WQ[0] * x61;
-      const TB x243 = s.QC[0] * x50 + s.WQ[0] * x56;
-      const TB x244 = s.QC[0] * x56 + s.WQ[0] * x62;
-      const TB x274 = s.QC[0] * x92 + s.WQ[0] * x102;
-      const TB x275 = s.QC[0] * x102 + s.WQ[0] * x112;
-      const TB x276 = s.QC[0] * x112 + s.WQ[0] * x122;
-      const TB x278 = s.QC[0] * x93 + s.WQ[0] * x103;
-      const TB x279 = s.QC[0] * x103 + s.WQ[0] * x113;
-      const TB x280 = s.QC[0] * x113 + s.WQ[0] * x123;
-      const TB x282 = s.QC[0] * x94 + s.WQ[0] * x104;
-      const TB x283 = s.QC[0] * x104 + s.WQ[0] * x114;
-      const TB x284 = s.QC[0] * x114 + s.WQ[0] * x124;
-      const TB x333 = s.QC[0] * x147 + s.WQ[0] * x162;
-      const TB x334 = s.QC[0] * x162 + s.WQ[0] * x177;
-      const TB x335 = s.QC[0] * x177 + s.WQ[0] * x192;
-      const TB x336 = s.QC[0] * x192 + s.WQ[0] * x207;
-      const TB x337 = s.QC[0] * x148 + s.WQ[0] * x163;
-      const TB x338 = s.QC[0] * x163 + s.WQ[0] * x178;
-      const TB x339 = s.QC[0] * x178 + s.WQ[0] * x193;
-      const TB x340 = s.QC[0] * x193 + s.WQ[0] * x208;
-      const TB x341 = s.QC[0] * x149 + s.WQ[0] * x164;
-      const TB x342 = s.QC[0] * x164 + s.WQ[0] * x179;
-      const TB x343 = s.QC[0] * x179 + s.WQ[0] * x194;
-      const TB x344 = s.QC[0] * x194 + s.WQ[0] * x209;
-      const TB x352 = s.QC[1] * x19 + s.WQ[1] * x22 + r1 * x4;
-      const TB x490 = s.QC[2] * x20 + s.WQ[2] * x23 + r1 * x4;
-      const TB x355 = s.QC[1] * x20 + s.WQ[1] * x23;
-      const TB x370 = s.QC[1] * x48 + s.WQ[1] * x54 + r2 * x19;
-      const TB x371 = s.QC[1] * x54 + s.WQ[1] * x60 + r2 * x22;
-      const TB x374 = s.QC[1] * x49 + s.WQ[1] * x55 + r1 * x20;
-      const TB x513 = s.QC[2] * x50 + s.WQ[2] * x56 + r2 * x20;
-      const TB x375 = s.QC[1] * x55 + s.WQ[1] * x61 + r1 * x23;
-      const TB x514 = s.QC[2] * x56 + s.WQ[2] * x62 + r2 * x23;
-      const TB x1139 = s.QC[2] * x513 + s.WQ[2] * x514 + e1 * (x50 - s.roe * x56) + r2 * x490;
-      const TB x378 = s.QC[1] * x50 + s.WQ[1] * x56;
-      const TB x379 = s.QC[1] * x56 + s.WQ[1] * x62;
-      const TB x409 = s.QC[1] * x92 + s.WQ[1] * x102 + r2 * x49;
-      const TB x410 = s.QC[1] * x102 + s.WQ[1] * x112 + r2 * x55;
-      const TB x411 = s.QC[1] * x112 + s.WQ[1] * x122 + r2 * x61;
-      const TB x413 = s.QC[1] * x93 + s.WQ[1] * x103 + r1 * x50;
-      const TB x414 = s.QC[1] * x103 + s.WQ[1] * x113 + r1 * x56;
-      const TB x415 = s.QC[1] * x113 + s.WQ[1] * x123 + r1 * x62;
-      const TB x554 = s.QC[2] * x114 + s.WQ[2] * x124 + r3 * x62;
-      const TB x417 = s.QC[1] * x94 + s.WQ[1] * x104;
-      const TB x418 = s.QC[1] * x104 + s.WQ[1] * x114;
-      const TB x419 = s.QC[1] * x114 + s.WQ[1] * x124;
-      const TB x468 = s.QC[1] * x147 + s.WQ[1] * x162 + r2 * x93;
-      const TB x469 = s.QC[1] * x162 + s.WQ[1] * x177 + r2 * x103;
-      const TB x470 = s.QC[1] * x177 + s.WQ[1] * x192 + r2 * x113;
-      const TB x471 = s.QC[1] * x192 + s.WQ[1] * x207 + r2 * x123;
-      const TB x606 = s.QC[2] * x192 + s.WQ[2] * x207 + r2 * x122;
-      const TB x546 = s.QC[2] * x112 + s.WQ[2] * x122 + r1 * x60;
-      const TB x506 = s.QC[2] * x54 + s.WQ[2] * x60;
-      const TB x472 = s.QC[1] * x148 + s.WQ[1] * x163 + r1 * x94;
-      const TB x473 = s.QC[1] * x163 + s.WQ[1] * x178 + r1 * x104;
-      const TB x474 = s.QC[1] * x178 + s.WQ[1] * x193 + r1 * x114;
-      const TB x475 = s.QC[1] * x193 + s.WQ[1] * x208 + r1 * x124;
-      const TB x610 = s.QC[2] * x193 + s.WQ[2] * x208 + r3 * x123;
-      const TB x550 = s.QC[2] * x113 + s.WQ[2] * x123 + r2 * x61;
-      const TB x510 = s.QC[2] * x55 + s.WQ[2] * x61 + r1 * x22;
-      const TB x487 = s.QC[2] * x19 + s.WQ[2] * x22;
-      const TB x509 = s.QC[2] * x49 + s.WQ[2] * x55 + r1 * x19;
-      const TB x614 = s.QC[2] * x194 + s.WQ[2] * x209 + r4 * x124;
-      const TB x479 = s.QC[1] * x194 + s.WQ[1] * x209;
-      const TB x1136 = s.QC[2] * x509 + s.WQ[2] * x510 + e1 * (x49 - s.roe * x55) + r1 * x487;
-      const TB x476 = s.QC[1] * x149 + s.WQ[1] * x164;
-      const TB x477 = s.QC[1] * x164 + s.WQ[1] * x179;
-      const TB x478 = s.QC[1] * x179 + s.WQ[1] * x194;
-      const TB x505 = s.QC[2] * x48 + s.WQ[2] * x54;
-      const TB x544 = s.QC[2] * x92 + s.WQ[2] * x102 + r1 * x48;
-      const TB x545 = s.QC[2] * x102 + s.WQ[2] * x112 + r1 * x54;
-      const TB x548 = s.QC[2] * x93 + s.WQ[2] * x103 + r2 * x49;
-      const TB x549 = s.QC[2] * x103 + s.WQ[2] * x113 + r2 * x55;
-      const TB x1165 = s.QC[2] * x548 + s.WQ[2] * x549 + e1 * (x93 - s.roe * x103) + r2 * x509;
-      const TB x1166 = s.QC[2] * x549 + s.WQ[2] * x550 + e1 * (x103 - s.roe * x113) + r2 * x510;
-      const TB x1832 = s.QC[2] * x1165 + s.WQ[2] * x1166 + e2 * (x548 - s.roe * x549) + r2 * x1136;
-      const TB x552 = s.QC[2] * x94 + s.WQ[2] * x104 + r3 * x50;
-      const TB x553 = s.QC[2] * x104 + s.WQ[2] * x114 + r3 * x56;
-      const TB x1168 = s.QC[2] * x552 + s.WQ[2] * x553 + e1 * (x94 - s.roe * x104) + r3 * x513;
-      const TB x1169 = s.QC[2] * x553 + s.WQ[2] * x554 + e1 * (x104 - s.roe * x114) + r3 * x514;
-      const TB x1834 = s.QC[2] * x1168 + s.WQ[2] * x1169 + e2 * (x552 - s.roe * x553) + r3 * x1139;
-      const TB x603 = s.QC[2] * x147 + s.WQ[2] * x162 + r2 * x92;
-      const TB x604 = s.QC[2] * x162 + s.WQ[2] * x177 + r2 * x102;
-      const TB x605 = s.QC[2] * x177 + s.WQ[2] * x192 + r2 * x112;
-      const TB x1208 = s.QC[2] * x605 + s.WQ[2] * x606 + e1 * (x177 - s.roe * x192) + r2 * x546;
-      const TB x1163 = s.QC[2] * x545 + s.WQ[2] * x546 + e1 * (x102 - s.roe * x112) + r1 * x506;
-      const TB x1133 = s.QC[2] * x505 + s.WQ[2] * x506 + e1 * (x48 - s.roe * x54);
-      const TB x1162 = s.QC[2] * x544 + s.WQ[2] * x545 + e1 * (x92 - s.roe * x102) + r1 * x505;
-      const TB x1830 = s.QC[2] * x1162 + s.WQ[2] * x1163 + e2 * (x544 - s.roe * x545) + r1 * x1133;
-      const TB x1206 = s.QC[2] * x603 + s.WQ[2] * x604 + e1 * (x147 - s.roe * x162) + r2 * x544;
-      const TB x1207 = s.QC[2] * x604 + s.WQ[2] * x605 + e1 * (x162 - s.roe * x177) + r2 * x545;
-      const TB x1860 = s.QC[2] * x1207 + s.WQ[2] * x1208 + e2 * (x604 - s.roe * x605) + r2 * x1163;
-      const TB x1859 = s.QC[2] * x1206 + s.WQ[2] * x1207 + e2 * (x603 - s.roe * x604) + r2 * x1162;
-      acc493 += static_cast<double>(x1859);
-      const TB x2327 = s.QC[2] * x1859 + s.WQ[2] * x1860 + e3 * (x1206 - s.roe * x1207) + r2 * x1830;
-      acc958 += static_cast<double>(x2327);
-      const TB x607 = s.QC[2] * x148 + s.WQ[2] * x163 + r3 * x93;
-      const TB x608 = s.QC[2] * x163 + s.WQ[2] * x178 + r3 * x103;
-      const TB x1209 = s.QC[2] * x607 + s.WQ[2] * x608 + e1 * (x148 - s.roe * x163) + r3 * x548;
-      acc184 += static_cast<double>(x1209);
-      const TB x609 = s.QC[2] * x178 + s.WQ[2] * x193 + r3 * x113;
-      const TB x1211 = s.QC[2] * x609 + s.WQ[2] * x610 + e1 * (x178 - s.roe * x193) + r3 * x550;
-      const TB x1210 = s.QC[2] * x608 + s.WQ[2] * x609 + e1 * (x163 - s.roe * x178) + r3 * x549;
-      const TB x1862 = s.QC[2] * x1210 + s.WQ[2] * x1211 + e2 * (x608 - s.roe * x609) + r3 * x1166;
-      const TB x1861 = s.QC[2] * x1209 + s.WQ[2] * x1210 + e2 * (x607 - s.roe * x608) + r3 * x1165;
-      acc494 += static_cast<double>(x1861);
-      const TB x2328 = s.QC[2] * x1861 + s.WQ[2] * x1862 + e3 * (x1209 - s.roe * x1210) + r3 * x1832;
-      acc959 += static_cast<double>(x2328);
-      const TB x611 = s.QC[2] * x149 + s.WQ[2] * x164 + r4 * x94;
-      const TB x612 = s.QC[2] * x164 + s.WQ[2] * x179 + r4 * x104;
-      const TB x1212 = s.QC[2] * x611 + s.WQ[2] * x612 + e1 * (x149 - s.roe * x164) + r4 * x552;
-      acc185 += static_cast<double>(x1212);
-      const TB x613 = s.QC[2] * x179 + s.WQ[2] * x194 + r4 * x114;
-      const TB x1214 = s.QC[2] * x613 + s.WQ[2] * x614 + e1 * (x179 - s.roe * x194) + r4 * x554;
-      const TB x1213 = s.QC[2] * x612 + s.WQ[2] * x613 + e1 * (x164 - s.roe * x179) + r4 * x553;
-      const TB x1864 = s.QC[2] * x1213 + s.WQ[2] * x1214 + e2 * (x612 - s.roe * x613) + r4 * x1169;
-      const TB x1863 = s.QC[2] * x1212 + s.WQ[2] * x1213 + e2 * (x611 - s.roe * x612) + r4 * x1168;
-      acc495 += static_cast<double>(x1863);
-      const TB x2329 = s.QC[2] * x1863 + s.WQ[2] * x1864 + e3 * (x1212 - s.roe * x1213) + r4 * x1834;
-      acc960 += static_cast<double>(x2329);
-      const TB x633 = s.QC[0] * x235 + s.WQ[0] * x236 + e1 * (x48 - s.roe * x54);
-      const TB x933 = s.QC[1] * x370 + s.WQ[1] * x371 + e1 * (x48 - s.roe * x54) + r2 * x352;
-      const TB x1036 = s.QC[2] * x374 + s.WQ[2] * x375 + r1 * x352;
-      const TB x636 = s.QC[0] * x239 + s.WQ[0] * x240 + e1 * (x49 - s.roe * x55);
-      const TB x936 = s.QC[1] * x374 + s.WQ[1] * x375 + e1 * (x49 - s.roe * x55) + r1 * x355;
-      const TB x1039 = s.QC[2] * x378 + s.WQ[2] * x379 + r2 * x355;
-      const TB x639 = s.QC[0] * x243 + s.WQ[0] * x244 + e1 * (x50 - s.roe * x56);
-      const TB x939 = s.QC[1] * x378 + s.WQ[1] * x379 + e1 * (x50 - s.roe * x56);
-      const TB x662 = s.QC[0] * x274 + s.WQ[0] * x275 + e1 * (x92 - s.roe * x102);
-      const TB x962 = s.QC[1] * x409 + s.WQ[1] * x410 + e1 * (x92 - s.roe * x102) + r2 * x374;
-      const TB x1065 = s.QC[2] * x413 + s.WQ[2] * x414 + r2 * x374;
-      const TB x663 = s.QC[0] * x275 + s.WQ[0] * x276 + e1 * (x102 - s.roe * x112);
-      const TB x963 = s.QC[1] * x410 + s.WQ[1] * x411 + e1 * (x102 - s.roe * x112) + r2 * x375;
-      const TB x1066 = s.QC[2] * x414 + s.WQ[2] * x415 + r2 * x375;
-      const TB x1375 = s.QC[2] * x662 + s.WQ[2] * x663 + r1 * x633;
-      const TB x1700 = s.QC[2] * x962 + s.WQ[2] * x963 + r1 * x933;
-      const TB x1767 = s.QC[2] * x1065 + s.WQ[2] * x1066 + e1 * (x413 - s.roe * x414) + r2 * x1036;
-      const TB x665 = s.QC[0] * x278 + s.WQ[0] * x279 + e1 * (x93 - s.roe * x103);
-      const TB x965 = s.QC[1] * x413 + s.WQ[1] * x414 + e1 * (x93 - s.roe * x103) + r1 * x378;
-      const TB x1068 = s.QC[2] * x417 + s.WQ[2] * x418 + r3 * x378;
-      const TB x666 = s.QC[0] * x279 + s.WQ[0] * x280 + e1 * (x103 - s.roe * x113);
-      const TB x966 = s.QC[1] * x414 + s.WQ[1] * x415 + e1 * (x103 - s.roe * x113) + r1 * x379;
-      const TB x1069 = s.QC[2] * x418 + s.WQ[2] * x419 + r3 * x379;
-      const TB x1769 = s.QC[2] * x1068 + s.WQ[2] * x1069 + e1 * (x417 - s.roe * x418) + r3 * x1039;
-      const TB x668 = s.QC[0] * x282 + s.WQ[0] * x283 + e1 * (x94 - s.roe * x104);
-      const TB x968 = s.QC[1] * x417 + s.WQ[1] * x418 + e1 * (x94 - s.roe * x104);
-      const TB x669 = s.QC[0] * x283 + s.WQ[0] * x284 + e1 * (x104 - s.roe * x114);
-      const TB x969 = s.QC[1] * x418 + s.WQ[1] * x419 + e1 * (x104 - s.roe * x114);
-      const TB x706 = s.QC[0] * x333 + s.WQ[0] * x334 + e1 * (x147 - s.roe * x162);
-      const TB x1006 = s.QC[1] * x468 + s.WQ[1] * x469 + e1 * (x147 - s.roe * x162) + r2 * x413;
-      const TB x707 = s.QC[0] * x334 + s.WQ[0] * x335 + e1 * (x162 - s.roe * x177);
-      const TB x1007 = s.QC[1] * x469 + s.WQ[1] * x470 + e1 * (x162 - s.roe * x177) + r2 * x414;
-      const TB x708 = s.QC[0] * x335 + s.WQ[0] * x336 + e1 * (x177 - s.roe * x192);
-      const TB x1008 = s.QC[1] * x470 + s.WQ[1] * x471 + e1 * (x177 - s.roe * x192) + r2 * x415;
-      const TB x1108 = s.QC[2] * x470 + s.WQ[2] * x471 + r2 * x411;
-      const TB x1063 = s.QC[2] * x410 + s.WQ[2] * x411 + r1 * x371;
-      const TB x1033 = s.QC[2] * x370 + s.WQ[2] * x371;
-      const TB x1062 = s.QC[2] * x409 + s.WQ[2] * x410 + r1 * x370;
-      const TB x1111 = s.QC[2] * x474 + s.WQ[2] * x475 + r3 * x415;
-      const TB x1011 = s.QC[1] * x474 + s.WQ[1] * x475 + e1 * (x178 - s.roe * x193) + r1 * x419;
-      const TB x711 = s.QC[0] * x339 + s.WQ[0] * x340 + e1 * (x178 - s.roe * x193);
-      const TB x1114 = s.QC[2] * x478 + s.WQ[2] * x479 + r4 * x419;
-      const TB x1014 = s.QC[1] * x478 + s.WQ[1] * x479 + e1 * (x179 - s.roe * x194);
-      const TB x714 = s.QC[0] * x343 + s.WQ[0] * x344 + e1 * (x179 - s.roe * x194);
-      const TB x1765 = s.QC[2] * x1062 + s.WQ[2] * x1063 + e1 * (x409 - s.roe * x410) + r1 * x1033;
-      const TB x709 = s.QC[0] * x337 + s.WQ[0] * x338 + e1 * (x148 - s.roe * x163);
-      acc29 += static_cast<double>(x709);
-      const TB x1009 = s.QC[1] * x472 + s.WQ[1] * x473 + e1 * (x148 - s.roe * x163) + r1 * x417;
-      acc122 += static_cast<double>(x1009);
-      const TB x710 = s.QC[0] * x338 + s.WQ[0] * x339 + e1 * (x163 - s.roe * x178);
-      const TB x1010 = s.QC[1] * x473 + s.WQ[1] * x474 + e1 * (x163 - s.roe * x178) + r1 * x418;
-      const TB x712 = s.QC[0] * x341 + s.WQ[0] * x342 + e1 * (x149 - s.roe * x164);
-      acc30 += static_cast<double>(x712);
-      const TB x1012 = s.QC[1] * x476 + s.WQ[1] * x477 + e1 * (x149 - s.roe * x164);
-      acc123 += static_cast<double>(x1012);
-      const TB x713 = s.QC[0] * x342 + s.WQ[0] * x343 + e1 * (x164 - s.roe * x179);
-      const TB x1013 = s.QC[1] * x477 + s.WQ[1] * x478 + e1 * (x164 - s.roe * x179);
-      const TB x733 = s.QC[1] * x235 + s.WQ[1] * x236 + r2 * x217;
-      const TB x836 = s.QC[2] * x239 + s.WQ[2] * x240 + r1 * x217;
-      const TB x736 = s.QC[1] * x239 + s.WQ[1] * x240 + r1 * x220;
-      const TB x839 = s.QC[2] * x243 + s.WQ[2] * x244 + r2 * x220;
-      const TB x739 = s.QC[1] * x243 + s.WQ[1] * x244;
-      const TB x762 = s.QC[1] * x274 + s.WQ[1] * x275 + r2 * x239;
-      const TB x865 = s.QC[2] * x278 + s.WQ[2] * x279 + r2 * x239;
-      const TB x763 = s.QC[1] * x275 + s.WQ[1] * x276 + r2 * x240;
-      const TB x866 = s.QC[2] * x279 + s.WQ[2] * x280 + r2 * x240;
-      const TB x1505 = s.QC[2] * x762 + s.WQ[2] * x763 + r1 * x733;
-      const TB x1572 = s.QC[2] * x865 + s.WQ[2] * x866 + e1 * (x278 - s.roe * x279) + r2 * x836;
-      const TB x765 = s.QC[1] * x278 + s.WQ[1] * x279 + r1 * x243;
-      const TB x868 = s.QC[2] * x282 + s.WQ[2] * x283 + r3 * x243;
-      const TB x766 = s.QC[1] * x279 + s.WQ[1] * x280 + r1 * x244;
-      const TB x869 = s.QC[2] * x283 + s.WQ[2] * x284 + r3 * x244;
-      const TB x1574 = s.QC[2] * x868 + s.WQ[2] * x869 + e1 * (x282 - s.roe * x283) + r3 * x839;
-      const TB x768 = s.QC[1] * x282 + s.WQ[1] * x283;
-      const TB x769 = s.QC[1] * x283 + s.WQ[1] * x284;
-      const TB x806 = s.QC[1] * x333 + s.WQ[1] * x334 + r2 * x278;
-      const TB x807 = s.QC[1] * x334 + s.WQ[1] * x335 + r2 * x279;
-      const TB x808 = s.QC[1] * x335 + s.WQ[1] * x336 + r2 * x280;
-      const TB x908 = s.QC[2] * x335 + s.WQ[2] * x336 + r2 * x276;
-      const TB x863 = s.QC[2] * x275 + s.WQ[2] * x276 + r1 * x236;
-      const TB x833 = s.QC[2] * x235 + s.WQ[2] * x236;
-      const TB x862 = s.QC[2] * x274 + s.WQ[2] * x275 + r1 * x235;
-      const TB x911 = s.QC[2] * x339 + s.WQ[2] * x340 + r3 * x280;
-      const TB x811 = s.QC[1] * x339 + s.WQ[1] * x340 + r1 * x284;
-      const TB x914 = s.QC[2] * x343 + s.WQ[2] * x344 + r4 * x284;
-      const TB x814 = s.QC[1] * x343 + s.WQ[1] * x344;
-      const TB x1570 = s.QC[2] * x862 + s.WQ[2] * x863 + e1 * (x274 - s.roe * x275) + r1 * x833;
-      const TB x809 = s.QC[1] * x337 + s.WQ[1] * x338 + r1 * x282;
-      acc60 += static_cast<double>(x809);
-      const TB x810 = s.QC[1] * x338 + s.WQ[1] * x339 + r1 * x283;
-      const TB x812 = s.QC[1] * x341 + s.WQ[1] * x342;
-      acc61 += static_cast<double>(x812);
-      const TB x813 = s.QC[1] * x342 + s.WQ[1] * x343;
-      const TB x906 = s.QC[2] * x333 + s.WQ[2] * x334 + r2 * x274;
-      const TB x907 = s.QC[2] * x334 + s.WQ[2] * x335 + r2 * x275;
-      const TB x1600 = s.QC[2] * x907 + s.WQ[2] * x908 + e1 * (x334 - s.roe * x335) + r2 * x863;
-      const TB x1599 = s.QC[2] * x906 + s.WQ[2] * x907 + e1 * (x333 - s.roe * x334) + r2 * x862;
-      const TB x2172 = s.QC[2] * x1599 + s.WQ[2] * x1600 + e2 * (x906 - s.roe * x907) + r2 * x1570;
-      acc803 += static_cast<double>(x2172);
-      const TB x909 = s.QC[2] * x337 + s.WQ[2] * x338 + r3 * x278;
-      acc91 += static_cast<double>(x909);
-      const TB x910 = s.QC[2] * x338 + s.WQ[2] * x339 + r3 * x279;
-      const TB x1602 = s.QC[2] * x910 + s.WQ[2] * x911 + e1 * (x338 - s.roe * x339) + r3 * x866;
-      const TB x1601 = s.QC[2] * x909 + s.WQ[2] * x910 + e1 * (x337 - s.roe * x338) + r3 * x865;
-      acc370 += static_cast<double>(x1601);
-      const TB x2173 = s.QC[2] * x1601 + s.WQ[2] * x1602 + e2 * (x909 - s.roe * x910) + r3 * x1572;
-      acc804 += static_cast<double>(x2173);
-      const TB x912 = s.QC[2] * x341 + s.WQ[2] * x342 + r4 * x282;
-      acc92 += static_cast<double>(x912);
-      const TB x913 = s.QC[2] * x342 + s.WQ[2] * x343 + r4 * x283;
-      const TB x1604 = s.QC[2] * x913 + s.WQ[2] * x914 + e1 * (x342 - s.roe * x343) + r4 * x869;
-      const TB x1603 = s.QC[2] * x912 + s.WQ[2] * x913 + e1 * (x341 - s.roe * x342) + r4 * x868;
-      acc371 += static_cast<double>(x1603);
-      const TB x2174 = s.QC[2] * x1603 + s.WQ[2] * x1604 + e2 * (x912 - s.roe * x913) + r4 * x1574;
-      acc805 += static_cast<double>(x2174);
-      const TB x1106 = s.QC[2] * x468 + s.WQ[2] * x469 + r2 * x409;
-      const TB x1635 = s.QC[1] * x962 + s.WQ[1] * x963 + e2 * (x409 - s.roe * x410) + r2 * x936;
-      const TB x1107 = s.QC[2] * x469 + s.WQ[2] * x470 + r2 * x410;
-      const TB x1795 = s.QC[2] * x1107 + s.WQ[2] * x1108 + e1 * (x469 - s.roe * x470) + r2 * x1063;
-      const TB x1665 = s.QC[1] * x1007 + s.WQ[1] * x1008 + e2 * (x469 - s.roe * x470) + r2 * x966;
-      const TB x1730 = s.QC[2] * x1007 + s.WQ[2] * x1008 + r2 * x963;
-      const TB x1702 = s.QC[2] * x965 + s.WQ[2] * x966 + r2 * x936;
-      const TB x1729 = s.QC[2] * x1006 + s.WQ[2] * x1007 + r2 * x962;
-      acc431 += static_cast<double>(x1729);
-      const TB x2265 = s.QC[2] * x1729 + s.WQ[2] * x1730 + e1 * (x1006 - s.roe * x1007) + r2 * x1700;
-      acc896 += static_cast<double>(x2265);
-      const TB x1794 = s.QC[2] * x1106 + s.WQ[2] * x1107 + e1 * (x468 - s.roe * x469) + r2 * x1062;
-      acc462 += static_cast<double>(x1794);
-      const TB x2296 = s.QC[2] * x1794 + s.WQ[2] * x1795 + e2 * (x1106 - s.roe * x1107) + r2 * x1765;
-      acc927 += static_cast<double>(x2296);
-      const TB x1664 = s.QC[1] * x1006 + s.WQ[1] * x1007 + e2 * (x468 - s.roe * x469) + r2 * x965;
-      acc400 += static_cast<double>(x1664);
-      const TB x2234 = s.QC[2] * x1664 + s.WQ[2] * x1665 + r2 * x1635;
-      acc865 += static_cast<double>(x2234);
-      const TB x1109 = s.QC[2] * x472 + s.WQ[2] * x473 + r3 * x413;
-      acc153 += static_cast<double>(x1109);
-      const TB x1637 = s.QC[1] * x965 + s.WQ[1] * x966 + e2 * (x413 - s.roe * x414) + r1 * x939;
-      const TB x2203 = s.QC[1] * x1664 + s.WQ[1] * x1665 + e3 * (x1006 - s.roe * x1007) + r2 * x1637;
-      acc834 += static_cast<double>(x2203);
-      const TB x1110 = s.QC[2] * x473 + s.WQ[2] * x474 + r3 * x414;
-      const TB x1797 = s.QC[2] * x1110 + s.WQ[2] * x1111 + e1 * (x473 - s.roe * x474) + r3 * x1066;
-      const TB x1667 = s.QC[1] * x1010 + s.WQ[1] * x1011 + e2 * (x473 - s.roe * x474) + r1 * x969;
-      const TB x1732 = s.QC[2] * x1010 + s.WQ[2] * x1011 + r3 * x966;
-      const TB x1704 = s.QC[2] * x968 + s.WQ[2] * x969 + r3 * x939;
-      const TB x1731 = s.QC[2] * x1009 + s.WQ[2] * x1010 + r3 * x965;
-      acc432 += static_cast<double>(x1731);
-      const TB x2266 = s.QC[2] * x1731 + s.WQ[2] * x1732 + e1 * (x1009 - s.roe * x1010) + r3 * x1702;
-      acc897 += static_cast<double>(x2266);
-      const TB x1796 = s.QC[2] * x1109 + s.WQ[2] * x1110 + e1 * (x472 - s.roe * x473) + r3 * x1065;
-      acc463 += static_cast<double>(x1796);
-      const TB x2297 = s.QC[2] * x1796 + s.WQ[2] * x1797 + e2 * (x1109 - s.roe * x1110) + r3 * x1767;
-      acc928 += static_cast<double>(x2297);
-      const TB x1666 = s.QC[1] * x1009 + s.WQ[1] * x1010 + e2 * (x472 - s.roe * x473) + r1 * x968;
-      acc401 += static_cast<double>(x1666);
-      const TB x2235 = s.QC[2] * x1666 + s.WQ[2] * x1667 + r3 * x1637;
-      acc866 += static_cast<double>(x2235);
-      const TB x1112 = s.QC[2] * x476 + s.WQ[2] * x477 + r4 * x417;
-      acc154 += static_cast<double>(x1112);
-      const TB x1639 = s.QC[1] * x968 + s.WQ[1] * x969 + e2 * (x417 - s.roe * x418);
-      const TB x2204 = s.QC[1] * x1666 + s.WQ[1] * x1667 + e3 * (x1009 - s.roe * x1010) + r1 * x1639;
-      acc835 += static_cast<double>(x2204);
-      const TB x1113 = s.QC[2] * x477 + s.WQ[2] * x478 + r4 * x418;
-      const TB x1799 = s.QC[2] * x1113 + s.WQ[2] * x1114 + e1 * (x477 - s.roe * x478) + r4 * x1069;
-      const TB x1669 = s.QC[1] * x1013 + s.WQ[1] * x1014 + e2 * (x477 - s.roe * x478);
-      const TB x1734 = s.QC[2] * x1013 + s.WQ[2] * x1014 + r4 * x969;
-      const TB x1733 = s.QC[2] * x1012 + s.WQ[2] * x1013 + r4 * x968;
-      acc433 += static_cast<double>(x1733);
-      const TB x2267 = s.QC[2] * x1733 + s.WQ[2] * x1734 + e1 * (x1012 - s.roe * x1013) + r4 * x1704;
-      acc898 += static_cast<double>(x2267);
-      const TB x1798 = s.QC[2] * x1112 + s.WQ[2] * x1113 + e1 * (x476 - s.roe * x477) + r4 * x1068;
-      acc464 += static_cast<double>(x1798);
-      const TB x2298 = s.QC[2] * x1798 + s.WQ[2] * x1799 + e2 * (x1112 - s.roe * x1113) + r4 * x1769;
-      acc929 += static_cast<double>(x2298);
-      const TB x1668 = s.QC[1] * x1012 + s.WQ[1] * x1013 + e2 * (x476 - s.roe * x477);
-      acc402 += static_cast<double>(x1668);
-      const TB x2205 = s.QC[1] * x1668 + s.WQ[1] * x1669 + e3 * (x1012 - s.roe * x1013);
-      acc836 += static_cast<double>(x2205);
-      const TB x2236 = s.QC[2] * x1668 + s.WQ[2] * x1669 + r4 * x1639;
-      acc867 += static_cast<double>(x2236);
-      const TB x1245 = s.QC[0] * x662 + s.WQ[0] * x663 + e2 * (x274 - s.roe * x275);
-      const TB x1440 = s.QC[1] * x762 + s.WQ[1] * x763 + e1 * (x274 - s.roe * x275) + r2 * x736;
-      const TB x1507 = s.QC[2] * x765 + s.WQ[2] * x766 + r2 * x736;
-      const TB x1534 = s.QC[2] * x806 + s.WQ[2] * x807 + r2 * x762;
-      const TB x1535 = s.QC[2] * x807 + s.WQ[2] * x808 + r2 * x763;
-      const TB x2141 = s.QC[2] * x1534 + s.WQ[2] * x1535 + e1 * (x806 - s.roe * x807) + r2 * x1505;
-      acc772 += static_cast<double>(x2141);
-      const TB x1470 = s.QC[1] * x807 + s.WQ[1] * x808 + e1 * (x334 - s.roe * x335) + r2 * x766;
-      const TB x1275 = s.QC[0] * x707 + s.WQ[0] * x708 + e2 * (x334 - s.roe * x335);
-      const TB x1247 = s.QC[0] * x665 + s.WQ[0] * x666 + e2 * (x278 - s.roe * x279);
-      const TB x1442 = s.QC[1] * x765 + s.WQ[1] * x766 + e1 * (x278 - s.roe * x279) + r1 * x739;
-      const TB x1509 = s.QC[2] * x768 + s.WQ[2] * x769 + r3 * x739;
-      const TB x1537 = s.QC[2] * x810 + s.WQ[2] * x811 + r3 * x766;
-      const TB x1472 = s.QC[1] * x810 + s.WQ[1] * x811 + e1 * (x338 - s.roe * x339) + r1 * x769;
-      const TB x1277 = s.QC[0] * x710 + s.WQ[0] * x711 + e2 * (x338 - s.roe * x339);
-      const TB x1249 = s.QC[0] * x668 + s.WQ[0] * x669 + e2 * (x282 - s.roe * x283);
-      const TB x1444 = s.QC[1] * x768 + s.WQ[1] * x769 + e1 * (x282 - s.roe * x283);
-      const TB x1539 = s.QC[2] * x813 + s.WQ[2] * x814 + r4 * x769;
-      const TB x1474 = s.QC[1] * x813 + s.WQ[1] * x814 + e1 * (x342 - s.roe * x343);
-      const TB x1279 = s.QC[0] * x713 + s.WQ[0] * x714 + e2 * (x342 - s.roe * x343);
-      const TB x1274 = s.QC[0] * x706 + s.WQ[0] * x707 + e2 * (x333 - s.roe * x334);
-      const TB x1469 = s.QC[1] * x806 + s.WQ[1] * x807 + e1 * (x333 - s.roe * x334) + r2 * x765;
-      const TB x2079 = s.QC[1] * x1469 + s.WQ[1] * x1470 + e2 * (x806 - s.roe * x807) + r2 * x1442;
-      acc710 += static_cast<double>(x2079);
-      const TB x2110 = s.QC[2] * x1469 + s.WQ[2] * x1470 + r2 * x1440;
-      acc741 += static_cast<double>(x2110);
-      const TB x1955 = s.QC[2] * x1274 + s.WQ[2] * x1275 + r2 * x1245;
-      acc586 += static_cast<double>(x1955);
-      const TB x1536 = s.QC[2] * x809 + s.WQ[2] * x810 + r3 * x765;
-      acc339 += static_cast<double>(x1536);
-      const TB x2142 = s.QC[2] * x1536 + s.WQ[2] * x1537 + e1 * (x809 - s.roe * x810) + r3 * x1507;
-      acc773 += static_cast<double>(x2142);
-      const TB x1893 = s.QC[0] * x1274 + s.WQ[0] * x1275 + e3 * (x706 - s.roe * x707);
-      acc524 += static_cast<double>(x1893);
-      const TB x1924 = s.QC[1] * x1274 + s.WQ[1] * x1275 + r2 * x1247;
-      acc555 += static_cast<double>(x1924);
-      const TB x1276 = s.QC[0] * x709 + s.WQ[0] * x710 + e2 * (x337 - s.roe * x338);
-      acc215 += static_cast<double>(x1276);
-      const TB x1471 = s.QC[1] * x809 + s.WQ[1] * x810 + e1 * (x337 - s.roe * x338) + r1 * x768;
-      acc308 += static_cast<double>(x1471);
-      const TB x2080 = s.QC[1] * x1471 + s.WQ[1] * x1472 + e2 * (x809 - s.roe * x810) + r1 * x1444;
-      acc711 += static_cast<double>(x2080);
-      const TB x2111 = s.QC[2] * x1471 + s.WQ[2] * x1472 + r3 * x1442;
-      acc742 += static_cast<double>(x2111);
-      const TB x1956 = s.QC[2] * x1276 + s.WQ[2] * x1277 + r3 * x1247;
-      acc587 += static_cast<double>(x1956);
-      const TB x1538 = s.QC[2] * x812 + s.WQ[2] * x813 + r4 * x768;
-      acc340 += static_cast<double>(x1538);
-      const TB x2143 = s.QC[2] * x1538 + s.WQ[2] * x1539 + e1 * (x812 - s.roe * x813) + r4 * x1509;
-      acc774 += static_cast<double>(x2143);
-      const TB x1894 = s.QC[0] * x1276 + s.WQ[0] * x1277 + e3 * (x709 - s.roe * x710);
-      acc525 += static_cast<double>(x1894);
-      const TB x1925 = s.QC[1] * x1276 + s.WQ[1] * x1277 + r1 * x1249;
-      acc556 += static_cast<double>(x1925);
-      const TB x1278 = s.QC[0] * x712 + s.WQ[0] * x713 + e2 * (x341 - s.roe * x342);
-      acc216 += static_cast<double>(x1278);
-      const TB x1473 = s.QC[1] * x812 + s.WQ[1] * x813 + e1 * (x341 - s.roe * x342);
-      acc309 += static_cast<double>(x1473);
-      const TB x2081 = s.QC[1] * x1473 + s.WQ[1] * x1474 + e2 * (x812 - s.roe * x813);
-      acc712 += static_cast<double>(x2081);
-      const TB x2112 = s.QC[2] * x1473 + s.WQ[2] * x1474 + r4 * x1444;
-      acc743 += static_cast<double>(x2112);
-      const TB x1957 = s.QC[2] * x1278 + s.WQ[2] * x1279 + r4 * x1249;
-      acc588 += static_cast<double>(x1957);
-      const TB x1895 = s.QC[0] * x1278 + s.WQ[0] * x1279 + e3 * (x712 - s.roe * x713);
-      acc526 += static_cast<double>(x1895);
-      const TB x1926 = s.QC[1] * x1278 + s.WQ[1] * x1279;
-      acc557 += static_cast<double>(x1926);
-      const TB x1310 = s.QC[1] * x662 + s.WQ[1] * x663 + r2 * x636;
-      const TB x1377 = s.QC[2] * x665 + s.WQ[2] * x666 + r2 * x636;
-      const TB x1404 = s.QC[2] * x706 + s.WQ[2] * x707 + r2 * x662;
-      const TB x1405 = s.QC[2] * x707 + s.WQ[2] * x708 + r2 * x663;
-      const TB x2048 = s.QC[2] * x1404 + s.WQ[2] * x1405 + e1 * (x706 - s.roe * x707) + r2 * x1375;
-      acc679 += static_cast<double>(x2048);
-      const TB x1340 = s.QC[1] * x707 + s.WQ[1] * x708 + r2 * x666;
-      const TB x1312 = s.QC[1] * x665 + s.WQ[1] * x666 + r1 * x639;
-      const TB x1379 = s.QC[2] * x668 + s.WQ[2] * x669 + r3 * x639;
-      const TB x1407 = s.QC[2] * x710 + s.WQ[2] * x711 + r3 * x666;
-      const TB x1342 = s.QC[1] * x710 + s.WQ[1] * x711 + r1 * x669;
-      const TB x1314 = s.QC[1] * x668 + s.WQ[1] * x669;
-      const TB x1409 = s.QC[2] * x713 + s.WQ[2] * x714 + r4 * x669;
-      const TB x1344 = s.QC[1] * x713 + s.WQ[1] * x714;
-      const TB x1339 = s.QC[1] * x706 + s.WQ[1] * x707 + r2 * x665;
-      const TB x1986 = s.QC[1] * x1339 + s.WQ[1] * x1340 + e1 * (x706 - s.roe * x707) + r2 * x1312;
-      acc617 += static_cast<double>(x1986);
-      const TB x2017 = s.QC[2] * x1339 + s.WQ[2] * x1340 + r2 * x1310;
-      acc648 += static_cast<double>(x2017);
-      const TB x1406 = s.QC[2] * x709 + s.WQ[2] * x710 + r3 * x665;
-      acc277 += static_cast<double>(x1406);
-      const TB x2049 = s.QC[2] * x1406 + s.WQ[2] * x1407 + e1 * (x709 - s.roe * x710) + r3 * x1377;
-      acc680 += static_cast<double>(x2049);
-      const TB x1341 = s.QC[1] * x709 + s.WQ[1] * x710 + r1 * x668;
-      acc246 += static_cast<double>(x1341);
-      const TB x1987 = s.QC[1] * x1341 + s.WQ[1] * x1342 + e1 * (x709 - s.roe * x710) + r1 * x1314;
-      acc618 += static_cast<double>(x1987);
-      const TB x2018 = s.QC[2] * x1341 + s.WQ[2] * x1342 + r3 * x1312;
-      acc649 += static_cast<double>(x2018);
-      const TB x1408 = s.QC[2] * x712 + s.WQ[2] * x713 + r4 * x668;
-      acc278 += static_cast<double>(x1408);
-      const TB x2050 = s.QC[2] * x1408 + s.WQ[2] * x1409 + e1 * (x712 - s.roe * x713) + r4 * x1379;
-      acc681 += static_cast<double>(x2050);
-      const TB x1343 = s.QC[1] * x712 + s.WQ[1] * x713;
-      acc247 += static_cast<double>(x1343);
-      const TB x1988 = s.QC[1] * x1343 + s.WQ[1] * x1344 + e1 * (x712 - s.roe * x713);
-      acc619 += static_cast<double>(x1988);
-      const TB x2019 = s.QC[2] * x1343 + s.WQ[2] * x1344 + r4 * x1314;
-      acc650 += static_cast<double>(x2019);
-    }
-  }
-  const int64_t R = a.lanes_per_quartet;
-  a.acc[(400 * R + r) * a.acc_stride + q] = acc400;
-  a.acc[(431 * R + r) * a.acc_stride + q] = acc431;
-  a.acc[(462 * R + r) * a.acc_stride + q] = acc462;
-  a.acc[(493 * R + r) * a.acc_stride + q] = acc493;
-  a.acc[(524 * R + r) * a.acc_stride + q] = acc524;
-  a.acc[(555 * R + r) * a.acc_stride + q] = acc555;
-  a.acc[(586 * R + r) * a.acc_stride + q] = acc586;
-  a.acc[(617 * R + r) * a.acc_stride + q] = acc617;
-  a.acc[(648 * R + r) * a.acc_stride + q] = acc648;
-  a.acc[(679 * R + r) * a.acc_stride + q] = acc679;
-  a.acc[(710 * R + r) * a.acc_stride + q] = acc710;
-  a.acc[(741 * R + r) * a.acc_stride + q] = acc741;
-  a.acc[(772 * R + r) * a.acc_stride + q] = acc772;
-  a.acc[(803 * R + r) * a.acc_stride + q] = acc803;
-  a.acc[(834 * R + r) * a.acc_stride + q] = acc834;
-  a.acc[(865 * R + r) * a.acc_stride + q] = acc865;
-  a.acc[(896 * R + r) * a.acc_stride + q] = acc896;
-  a.acc[(927 * R + r) * a.acc_stride + q] = acc927;
-  a.acc[(958 * R + r) * a.acc_stride + q] = acc958;
-  a.acc[(29 * R + r) * a.acc_stride + q] = acc29;
-  a.acc[(60 * R + r) * a.acc_stride + q] = acc60;
-  a.acc[(91 * R + r) * a.acc_stride + q] = acc91;
-  a.acc[(122 * R + r) * a.acc_stride + q] = acc122;
-  a.acc[(153 * R + r) * a.acc_stride + q] = acc153;
-  a.acc[(184 * R + r) * a.acc_stride + q] = acc184;
-  a.acc[(215 * R + r) * a.acc_stride + q] = acc215;
-  a.acc[(246 * R + r) * a.acc_stride + q] = acc246;
-  a.acc[(277 * R + r) * a.acc_stride + q] = acc277;
-  a.acc[(308 * R + r) * a.acc_stride + q] = acc308;
-  a.acc[(339 * R + r) * a.acc_stride + q] = acc339;
-  a.acc[(370 * R + r) * a.acc_stride + q] = acc370;
-  a.acc[(401 * R + r) * a.acc_stride + q] = acc401;
-  a.acc[(432 * R + r) * a.acc_stride + q] = acc432;
-  a.acc[(463 * R + r) * a.acc_stride + q] = acc463;
-  a.acc[(494 * R + r) * a.acc_stride + q] = acc494;
-  a.acc[(525 * R + r) * a.acc_stride + q] = acc525;
-  a.acc[(556 * R + r) * a.acc_stride + q] = acc556;
-  a.acc[(587 * R + r) * a.acc_stride + q] = acc587;
-  a.acc[(618 * R + r) * a.acc_stride + q] = acc618;
-  a.acc[(649 * R + r) * a.acc_stride + q] = acc649;
-  a.acc[(680 * R + r) * a.acc_stride + q] = acc680;
-  a.acc[(711 * R + r) * a.acc_stride + q] = acc711;
-  a.acc[(742 * R + r) * a.acc_stride + q] = acc742;
-  a.acc[(773 * R + r) * a.acc_stride + q] = acc773;
-  a.acc[(804 * R + r) * a.acc_stride + q] = acc804;
-  a.acc[(835 * R + r) * a.acc_stride + q] = acc835;
-  a.acc[(866 * R + r) * a.acc_stride + q] = acc866;
-  a.acc[(897 * R + r) * a.acc_stride + q] = acc897;
-  a.acc[(928 * R + r) * a.acc_stride + q] = acc928;
-  a.acc[(959 * R + r) * a.acc_stride + q] = acc959;
-  a.acc[(30 * R + r) * a.acc_stride + q] = acc30;
-  a.acc[(61 * R + r) * a.acc_stride + q] = acc61;
-  a.acc[(92 * R + r) * a.acc_stride + q] = acc92;
-  a.acc[(123 * R + r) * a.acc_stride + q] = acc123;
-  a.acc[(154 * R + r) * a.acc_stride + q] = acc154;
-  a.acc[(185 * R + r) * a.acc_stride + q] = acc185;
-  a.acc[(216 * R + r) * a.acc_stride + q] = acc216;
-  a.acc[(247 * R + r) * a.acc_stride + q] = acc247;
-  a.acc[(278 * R + r) * a.acc_stride + q] = acc278;
-  a.acc[(309 * R + r) * a.acc_stride + q] = acc309;
-  a.acc[(340 * R + r) * a.acc_stride + q] = acc340;
-  a.acc[(371 * R + r) * a.acc_stride + q] = acc371;
-  a.acc[(402 * R + r) * a.acc_stride + q] = acc402;
-  a.acc[(433 * R + r) * a.acc_stride + q] = acc433;
-  a.acc[(464 * R + r) * a.acc_stride + q] = acc464;
-  a.acc[(495 * R + r) * a.acc_stride + q] = acc495;
-  a.acc[(526 * R + r) * a.acc_stride + q] = acc526;
-  a.acc[(557 * R + r) * a.acc_stride + q] = acc557;
-  a.acc[(588 * R + r) * a.acc_stride + q] = acc588;
-  a.acc[(619 * R + r) * a.acc_stride + q] = acc619;
-  a.acc[(650 * R + r) * a.acc_stride + q] = acc650;
-  a.acc[(681 * R + r) * a.acc_stride + q] = acc681;
-  a.acc[(712 * R + r) * a.acc_stride + q] = acc712;
-  a.acc[(743 * R + r) * a.acc_stride + q] = acc743;
-  a.acc[(774 * R + r) * a.acc_stride + q] = acc774;
-  a.acc[(805 * R + r) * a.acc_stride + q] = acc805;
-  a.acc[(836 * R + r) * a.acc_stride + q] = acc836;
-  a.acc[(867 * R + r) * a.acc_stride + q] = acc867;
-  a.acc[(898 * R + r) * a.acc_stride + q] = acc898;
-  a.acc[(929 * R + r) * a.acc_stride + q] = acc929;
-  a.acc[(960 * R + r) * a.acc_stride + q] = acc960;
-}
+int launch_reg_part0(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part0(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part0(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
+int launch_reg_part1(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part1(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part1(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
+int launch_reg_part2(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part2(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part2(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
+int launch_reg_part3(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part3(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part3(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
+int launch_reg_part4(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part4(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part4(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
+int launch_reg_part5(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part5(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part5(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
+int launch_reg_part6(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part6(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part6(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
+int launch_reg_part7(const gbx_tables* t, int key, const RegArgs& a, int precision, hipStream_t s);
+int launch_split_part7(const gbx_tables* t, int key, int split, const RegArgs& a, int precision, hipStream_t s);
+int launch_hrr_part7(const gbx_tables* t, int key, const RegArgs& a, hipStream_t s);
 
 int launch_reg_generated(const gbx_tables* t, int la, int lb, int lc, int ld, const RegArgs& a, int precision, hipStream_t s) {
-  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / kRegThreads);
-  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {
-    case 0:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_0000<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_0000<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_0000");
-    case 512:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_1000<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_1000<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_1000");
-    case 520:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_1010<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_1010<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_1010");
-    case 576:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_1100<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_1100<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_1100");
-    case 584:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_1110<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_1110<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_1110");
-    case 585:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_1111<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_1111<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_1111");
-    case 1024:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2000<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2000<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2000");
-    case 1032:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2010<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2010<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2010");
-    case 1033:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2011<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2011<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2011");
-    case 1040:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2020<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2020<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2020");
-    case 1088:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2100<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2100<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2100");
-    case 1096:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2110<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2110<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2110");
-    case 1097:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2111<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2111<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2111");
-    case 1104:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2120<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2120<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2120");
-    case 1152:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2200<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2200<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2200");
-    case 1160:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_reg_2210<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_reg_2210<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_reg_2210");
-    default:
-      return fail(GBX_ERR_UNSUPPORTED, "no fused register kernel for this class");
-  }
+  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;
+  int rc;
+  if ((rc = launch_reg_part0(t, key, a, precision, s)) != 1) return rc;
+  if ((rc = launch_reg_part1(t, key, a, precision, s)) != 1) return rc;
+  if ((rc = launch_reg_part2(t, key, a, precision, s)) != 1) return rc;
+  if ((rc = launch_reg_part3(t, key, a, precision, s)) != 1) return rc;
+  if ((rc = launch_reg_part4(t, key, a, precision, s)) != 1) return rc;
+  if ((rc = launch_reg_part5(t, key, a, precision, s)) != 1) return rc;
+  if ((rc = launch_reg_part6(t, key, a, precision, s)) != 1) return rc;
+  if ((rc = launch_reg_part7(t, key, a, precision, s)) != 1) return rc;
+  return fail(GBX_ERR_UNSUPPORTED, "no fused register kernel for this class");
 }
 
 int launch_split_generated(const gbx_tables* t, int la, int lb, int lc, int ld, int k, const RegArgs& a, int precision, hipStream_t s) {
-  const unsigned grid = static_cast<unsigned>((a.nquart * a.lanes_per_quartet + kRegThreads - 1) / kRegThreads);
-  switch ((((la * 8 + lb) * 8 + lc) * 8 + ld) * 64 + k) {
-    case 70720:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2121_0<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2121_0<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2121_0");
-    case 70721:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2121_1<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2121_1<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2121_1");
-    case 70722:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2121_2<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2121_2<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2121_2");
-    case 74304:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2211_0<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2211_0<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2211_0");
-    case 74305:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2211_1<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2211_1<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2211_1");
-    case 74306:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2211_2<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2211_2<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2211_2");
-    case 74752:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2220_0<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2220_0<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2220_0");
-    case 74753:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2220_1<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2220_1<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2220_1");
-    case 74816:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2221_0<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2221_0<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2221_0");
-    case 74817:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2221_1<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2221_1<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2221_1");
-    case 74818:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2221_2<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2221_2<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2221_2");
-    case 74819:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2221_3<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2221_3<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2221_3");
-    case 74820:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2221_4<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2221_4<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2221_4");
-    case 74821:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2221_5<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2221_5<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2221_5");
-    case 74880:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_0<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_0<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_0");
-    case 74881:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_1<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_1<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_1");
-    case 74882:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_2<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_2<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_2");
-    case 74883:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_3<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_3<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_3");
-    case 74884:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_4<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_4<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_4");
-    case 74885:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_5<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_5<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_5");
-    case 74886:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_6<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_6<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_6");
-    case 74887:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_7<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_7<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_7");
-    case 74888:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_8<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_8<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_8");
-    case 74889:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_9<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_9<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_9");
-    case 74890:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_10<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_10<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_10");
-    case 74891:
-      if (precision == GBX_FP64) hipLaunchKernelGGL((eri_split_2222_11<double>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      else hipLaunchKernelGGL((eri_split_2222_11<float>), dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_split_2222_11");
-    default:
-      return fail(GBX_ERR_UNSUPPORTED, "no split register kernel for this class");
-  }
+  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;
+  int rc;
+  if ((rc = launch_split_part0(t, key, k, a, precision, s)) != 1) return rc;
+  if ((rc = launch_split_part1(t, key, k, a, precision, s)) != 1) return rc;
+  if ((rc = launch_split_part2(t, key, k, a, precision, s)) != 1) return rc;
+  if ((rc = launch_split_part3(t, key, k, a, precision, s)) != 1) return rc;
+  if ((rc = launch_split_part4(t, key, k, a, precision, s)) != 1) return rc;
+  if ((rc = launch_split_part5(t, key, k, a, precision, s)) != 1) return rc;
+  if ((rc = launch_split_part6(t, key, k, a, precision, s)) != 1) return rc;
+  if ((rc = launch_split_part7(t, key, k, a, precision, s)) != 1) return rc;
+  return fail(GBX_ERR_UNSUPPORTED, "no split register kernel for this class");
 }
 
 int launch_hrr_generated(const gbx_tables* t, int la, int lb, int lc, int ld, const RegArgs& a, hipStream_t s) {
-  const unsigned grid = static_cast<unsigned>((a.nquart + kRegThreads - 1) / kRegThreads);
-  switch (((la * 8 + lb) * 8 + lc) * 8 + ld) {
-    case 1105:
-      hipLaunchKernelGGL(eri_hrr_2121, dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_hrr_2121");
-    case 1161:
-      hipLaunchKernelGGL(eri_hrr_2211, dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_hrr_2211");
-    case 1168:
-      hipLaunchKernelGGL(eri_hrr_2220, dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_hrr_2220");
-    case 1169:
-      hipLaunchKernelGGL(eri_hrr_2221, dim3(grid), dim3(kRegThreads), 0, s, *t, a);
-      return check_launch("eri_hrr_2221");
-    default:
-      return fail(GBX_ERR_UNSUPPORTED, "no register HRR kernel for this class");
-  }
+  const int key = ((la * 8 + lb) * 8 + lc) * 8 + ld;
+  int rc;
+  if ((rc = launch_hrr_part0(t, key, a, s)) != 1) return rc;
+  if ((rc = launch_hrr_part1(t, key, a, s)) != 1) return rc;
+  if ((rc = launch_hrr_part2(t, key, a, s)) != 1) return rc;
+  if ((rc = launch_hrr_part3(t, key, a, s)) != 1) return rc;
+  if ((rc = launch_hrr_part4(t, key, a, s)) != 1) return rc;
+  if ((rc = launch_hrr_part5(t, key, a, s)) != 1) return rc;
+  if ((rc = launch_hrr_part6(t, key, a, s)) != 1) return rc;
+  if ((rc = launch_hrr_part7(t, key, a, s)) != 1) return rc;
+  return fail(GBX_ERR_UNSUPPORTED, "no register HRR kernel for this class");
 }
 
 bool has_hrr_generated(int la, int lb, int lc, int ld) {
@@ -30576,6 +77,11 @@ bool has_hrr_generated(int la, int lb, int lc, int ld) {
     case 1161:
     case 1168:
     case 1169:
+    case 649:
+    case 650:
+    case 657:
+    case 1098:
+    case 1162:
       return true;
     default:
       return false;
@@ -30600,6 +106,9 @@ bool has_reg_generated(int la, int lb, int lc, int ld) {
     case 1104:
     case 1152:
     case 1160:
+    case 640:
+    case 648:
+    case 656:
       return true;
     default:
       return false;
@@ -30618,6 +127,16 @@ int split_count_generated(int la, int lb, int lc, int ld) {
       return 6;
     case 1170:
       return 12;
+    case 649:
+      return 2;
+    case 650:
+      return 4;
+    case 657:
+      return 4;
+    case 1098:
+      return 4;
+    case 1162:
+      return 7;
     default:
       return 0;
   }

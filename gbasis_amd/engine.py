@@ -23,12 +23,13 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_SCHWARZ, GBX_POT_COULOMB,
-                   GBX_POT_ERF, GBX_POT_ERFC, GbxProgram, GbxTables, check, ptr)
+from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_PACKED_ADD,
+                   GBX_MODE_SCHWARZ, GBX_POT_COULOMB, GBX_POT_ERF, GBX_POT_ERFC, GbxProgram,
+                   GbxTables, check, ptr)
 from .boys import BOYS_H, boys_table
-from .programs import encoded_program
+from .programs import canonical_class, encoded_program
 from .roofline import class_flops, quartet_flops
-from .tables import BatchTables, MoleculeTables, f2norm_table
+from .tables import NUM_BUCKETS, BatchTables, MoleculeTables, decode_head, f2norm_table
 
 DEFAULT_SCRATCH_BYTES = 2 << 30
 
@@ -52,6 +53,12 @@ class Bucket:
     count: object = None      # device int32 count (n is then the capacity); None: n exact
     key: tuple = None         # ("head", class/dense/precision head) for a fused launch,
                               # ("key", bucket key) otherwise
+    phase: int = 0            # sub-quartet phase: 0 stores the packed slots, 1-3 add to
+                              # them and run after the phases before
+
+    def mode(self):
+        """Packed output mode of this bucket's launch."""
+        return GBX_MODE_PACKED if self.phase == 0 else GBX_MODE_PACKED_ADD
 
     def launch_precision(self):
         return self.precision if self.run_precision < 0 else self.run_precision
@@ -154,6 +161,7 @@ class DeviceTables:
             "prim_pairs": batch.prim_pairs.astype(np.float64),
             "f2norm": f2norm_table(8),
             "pair_key": batch.pair_key.astype(np.int32),
+            "pair_sub": batch.pair_sub.astype(np.int32),
             "pair_npf": batch.pair_npf.astype(np.int32),
             "mol_cand_off": batch.mol_cand_off.astype(np.int64),
             "mol_pair_off": np.asarray(batch.mol_pair_off, dtype=np.int32),
@@ -374,11 +382,14 @@ class EriEngine:
             self._reg[cls] = bool(self.lib.gbx_has_reg_kernel(*cls))
         return self._reg[cls]
 
-    def launch_bucket(self, dev, bk, out, mode=GBX_MODE_PACKED, precision=None, lanes=None,
+    def launch_bucket(self, dev, bk, out, mode=None, precision=None, lanes=None,
                       hrr=None, path=None):
+        """Launch one bucket; ``mode`` defaults to the bucket's own (store in phase 0,
+        add in the later phases)."""
         prec = bk.launch_precision() if precision is None else precision
         if path is None:
             path = "coop" if bk.variant and bk.variant[0] == "coop" else self.path
+        mode = bk.mode() if mode is None else mode
         return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
                                  path, lanes if lanes is not None else bk.lanes(), bk.seg,
                                  bk.count, hrr if hrr is not None else bk.hrr())
@@ -393,8 +404,9 @@ class EriEngine:
         kernel.  ``count``: device int32 holding the number of quartets (n is then
         the capacity).
         """
-        tab = dev.struct if mode == GBX_MODE_PACKED else dev.struct_coulomb
-        if dev.potential != "coulomb" and mode == GBX_MODE_PACKED:
+        packed = mode in (GBX_MODE_PACKED, GBX_MODE_PACKED_ADD)
+        tab = dev.struct if packed else dev.struct_coulomb
+        if dev.potential != "coulomb" and packed:
             # range-separated operators: cooperative kernels, FP64
             path = "coop"
             if precision != GBX_FP64:
@@ -430,9 +442,9 @@ class EriEngine:
                                              ptr(count), precision, R, ptr(acc), nq,
                                              self._stream()), f"gbx_eri_split{cls}[{k}]")
             use_reg = self.hrr_reg if hrr is None else hrr
-            if mode == GBX_MODE_PACKED and not dense and use_reg and self.has_hrr(cls):
+            if packed and not dense and use_reg and self.has_hrr(cls):
                 check(self.lib.gbx_eri_hrr_reg(ctypes.byref(tab), *cls, ptr(quartets), nq,
-                                               ptr(count), ptr(acc), nq, R, ptr(out),
+                                               ptr(count), ptr(acc), nq, R, mode, ptr(out),
                                                self._stream()), f"gbx_eri_hrr_reg{cls}")
                 return "split"
             sc = self.scratch()
@@ -491,10 +503,11 @@ class EriEngine:
         return q
 
     def plan_quartets(self, dev: DeviceTables, q_pair=None, threshold=0.0, fp32_cutoff=0.0):
-        """Screened, bucketed quartet lists on the device.
+        """Screened, bucketed sub-quartet lists enumerated on the host (the device
+        queue builder's result, for checks).
 
-        Returns a list of Bucket.  Quartets whose bound Q_ij Q_kl is below
-        ``fp32_cutoff`` run with an FP32 recursion bulk; all others in FP64.
+        Returns a list of Bucket, phase by phase.  Quartets whose bound Q_ij Q_kl is
+        below ``fp32_cutoff`` run with an FP32 recursion bulk; all others in FP64.
         """
         b = dev.batch
         if q_pair is not None:
@@ -521,13 +534,15 @@ class EriEngine:
         for prec, mask in groups:
             if not mask.any():
                 continue
-            for cls, nprimq, dense, quart in b.class_buckets(bra[mask], ket[mask]):
-                qb, qk = quart[:, 0].astype(np.int64), quart[:, 1].astype(np.int64)
-                eris = int(b.unique_counts(qb, qk).sum())
-                flops = float(quartet_flops(cls, nprim[qb] * nprim[qk], pure=dense).sum())
-                plans.append(Bucket(cls, nprimq, prec, dense,
-                                    torch.from_numpy(quart).to(self.device), quart.shape[0],
-                                    eris, flops))
+            for phase, xb, xk in b.sub_quartets(bra[mask], ket[mask]):
+                for cls, nprimq, dense, quart in b.class_buckets(xb, xk):
+                    qb, qk = quart[:, 0].astype(np.int64), quart[:, 1].astype(np.int64)
+                    eris = int(b.unique_counts(qb, qk).sum()) if phase == 0 else 0
+                    flops = float(quartet_flops(cls, nprim[qb] * nprim[qk], pure=dense,
+                                                contracted=phase == 0).sum())
+                    plans.append(Bucket(cls, nprimq, prec, dense,
+                                        torch.from_numpy(quart).to(self.device), quart.shape[0],
+                                        eris, flops, phase=phase))
         return plans
 
     def pair_bounds(self, dev: DeviceTables, q_pair):
@@ -566,7 +581,7 @@ class EriEngine:
             ones = torch.ones(b.npair, dtype=torch.float64, device=self.device)
             bound = (ones, ones)
         qb, pbd = bound
-        nb = 262144
+        nb = NUM_BUCKETS
         st = self._stream()
         mask = self.fp32_ok
         mcut = dev.mol_fp32_cutoff
@@ -576,7 +591,8 @@ class EriEngine:
             primq = torch.zeros(nb, dtype=torch.int64, device=self.device)
             check(self.lib.gbx_quartets_count(len(b.mols), ptr(dev.mol_cand_off),
                                               ptr(dev.mol_pair_off), ptr(dev.pair_key),
-                                              ptr(dev.pair_npf), ptr(qb), ptr(pbd),
+                                              ptr(dev.pair_sub), ptr(dev.pair_npf), ptr(qb),
+                                              ptr(pbd),
                                               float(threshold), float(fp32_cutoff), ptr(mcut),
                                               ptr(mask), b.n_cand, ptr(counts), ptr(eris),
                                               ptr(primq), st), "gbx_quartets_count")
@@ -592,11 +608,11 @@ class EriEngine:
                 cap[:, 1, :] = cap[:, 0, :]
                 cap = cap.reshape(-1)
         if layout not in dev.queue:
-            offs = np.concatenate([[0], np.cumsum(cap)[:-1]]).astype(np.int64)
+            offs = np.concatenate([[0], np.cumsum(cap)]).astype(np.int64)   # nb + 1 entries
             total = max(int(cap.sum()), 1)
             if self.stream_workspaces:
                 quart = self.workspace("queue", 2 * total, torch.int32).view(total, 2)
-                d_offs = self.workspace("queue_offs", nb, torch.int64)
+                d_offs = self.workspace("queue_offs", nb + 1, torch.int64)
                 d_offs.copy_(torch.from_numpy(offs).pin_memory(), non_blocking=True)
                 cursor = self.workspace("queue_cursor", nb, torch.int32)
             else:
@@ -609,7 +625,8 @@ class EriEngine:
         quart, d_offs, cursor, offs = qc["quart"], qc["d_offs"], qc["cursor"], qc["offs"]
         cursor.zero_()
         check(self.lib.gbx_quartets_fill(len(b.mols), ptr(dev.mol_cand_off), ptr(dev.mol_pair_off),
-                                         ptr(dev.pair_key), ptr(qb), ptr(pbd), float(threshold),
+                                         ptr(dev.pair_key), ptr(dev.pair_sub), ptr(qb), ptr(pbd),
+                                         float(threshold),
                                          float(fp32_cutoff), ptr(mcut), ptr(mask), b.n_cand,
                                          ptr(d_offs), ptr(cursor), ptr(quart), st),
               "gbx_quartets_fill")
@@ -617,6 +634,9 @@ class EriEngine:
             ch = cap
         else:
             ch = cursor.cpu().numpy().astype(np.int64)
+            if np.any(ch > cap):
+                raise RuntimeError("work-queue bucket capacity exceeded (host capacity model "
+                                   "out of step with the device queue builder)")
         eh = eris.cpu().numpy() if stats else None
         ph = primq.cpu().numpy() if stats else None
         plans = []
@@ -624,24 +644,24 @@ class EriEngine:
         heads = {}
         for key in keys.tolist():
             heads.setdefault(key >> 4, []).append(key)
-        for head, ks in heads.items():
-            fp32 = head & 1
-            dense = bool((head >> 1) & 1)
-            c = head >> 2
-            cls = ((c >> 9) & 7, (c >> 6) & 7, (c >> 3) & 7, c & 7)
+        for head in sorted(heads):
+            ks = heads[head]
+            phase, cls, dense, fp32 = decode_head(head)
             prec = GBX_FP32_BULK if fp32 else GBX_FP64
             fused = ((not dense) and self.path != "coop" and dev.potential == "coulomb"
                      and self.has_reg(cls))
             fpr, fco = class_flops(*cls, pure=dense) if stats else (0, 0)
+            if phase > 0:
+                fco = 0   # a shell quartet's contracted part is counted in phase 0
             if fused:
                 base = head << 4
                 n = int(ch[ks].sum())
                 flops = float(ph[ks].sum()) * fpr + n * fco if stats else 0.0
                 eris_n = int(eh[ks].sum()) if stats else 0
-                seg = (d_offs[base:base + 16], cursor[base:base + 16],
+                seg = (d_offs[base:base + 17], cursor[base:base + 16],
                        int(cap[base:base + 16].sum()))
                 plans.append(Bucket(cls, 1 << (ks[-1] & 15), prec, dense, quart, n, eris_n, flops,
-                                    seg=seg, key=("head", head)))
+                                    seg=seg, key=("head", head), phase=phase))
                 continue
             for key in ks:
                 n = int(ch[key])
@@ -650,7 +670,7 @@ class EriEngine:
                 plans.append(Bucket(cls, 1 << (key & 15), prec, dense, view, n,
                                     int(eh[key]) if stats else 0, flops,
                                     count=cursor[key:key + 1] if device_counts else None,
-                                    key=("key", key)))
+                                    key=("key", key), phase=phase))
         return plans
 
     def capture(self, dev: DeviceTables, bound=None, threshold=0.0, fp32_cutoff=0.0,
@@ -707,32 +727,37 @@ class EriEngine:
         return lst[:n]
 
     def run(self, dev: DeviceTables, plans, out=None, n_streams=None):
-        """Execute the plans into a zero-initialised packed array (allocated if None).
+        """Execute the plans into the packed array (allocated if None).
 
-        With ``n_streams`` > 1 (default: ``self.launch_streams``) the buckets are spread
-        round-robin over that many side streams forked from and joined back into the
-        current stream: every bucket writes its own packed slots, so small launches of
-        different classes overlap on the device instead of running one after another."""
+        Phase 0 stores every packed slot of every kept quartet; phases 1-3 add to
+        them, one phase after the other.  Within a phase every packed slot belongs to
+        one sub-quartet, so with ``n_streams`` > 1 (default: ``self.launch_streams``)
+        a phase's buckets are spread round-robin over that many side streams forked
+        from and joined back into the current stream: small launches of different
+        classes overlap on the device instead of running one after another."""
         if out is None:
             out = torch.zeros(dev.batch.n_slots, dtype=torch.float64, device=self.device)
         ns = self.launch_streams if n_streams is None else n_streams
-        if ns <= 1 or len(plans) <= 1:
-            for bk in plans:
-                bk.path = self.launch_bucket(dev, bk, out)
-            return out
-        cur = torch.cuda.current_stream(self.device)
-        side = self._side_streams(ns)
-        fork = torch.cuda.Event()
-        fork.record(cur)
-        for st in side:
-            st.wait_event(fork)
-        for i, bk in enumerate(plans):
-            with torch.cuda.stream(side[i % ns]):
-                bk.path = self.launch_bucket(dev, bk, out)
-        for st in side:
-            join = torch.cuda.Event()
-            join.record(st)
-            cur.wait_event(join)
+        phases = sorted({bk.phase for bk in plans})
+        for ph in phases:
+            group = [bk for bk in plans if bk.phase == ph]
+            if ns <= 1 or len(group) <= 1:
+                for bk in group:
+                    bk.path = self.launch_bucket(dev, bk, out)
+                continue
+            cur = torch.cuda.current_stream(self.device)
+            side = self._side_streams(ns)
+            fork = torch.cuda.Event()
+            fork.record(cur)
+            for st in side:
+                st.wait_event(fork)
+            for i, bk in enumerate(group):
+                with torch.cuda.stream(side[i % ns]):
+                    bk.path = self.launch_bucket(dev, bk, out)
+            for st in side:
+                join = torch.cuda.Event()
+                join.record(st)
+                cur.wait_event(join)
         return out
 
     FP32_MAX_L = 2

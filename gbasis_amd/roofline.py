@@ -67,7 +67,9 @@ def class_flops(la, lb, lc, ld, pure=False):
     return f_prim, f_cont
 
 
-def quartet_flops(cls, nprimq, pure=False):
-    """Algorithmic FLOPs of quartets of one class; nprimq may be an array."""
+def quartet_flops(cls, nprimq, pure=False, contracted=True):
+    """Algorithmic FLOPs of quartets of one class; nprimq may be an array.
+    ``contracted=False``: the primitive part only (a sub-quartet of a later phase,
+    whose shell quartet's contracted part is counted once, in phase 0)."""
     fp, fc = class_flops(*cls, pure=pure)
-    return np.asarray(nprimq, dtype=np.float64) * fp + fc
+    return np.asarray(nprimq, dtype=np.float64) * fp + (fc if contracted else 0)

@@ -62,9 +62,11 @@ class EriRequest:
         self.tables = MoleculeTables(basis)
         m = self.tables
         n_cand = m.npair * (m.npair + 1) // 2
-        # packed output + queue room (both precision variants) + pair records
-        self.footprint = int(8 * m.n_slots + 16 * n_cand + 80 * m.prim_pairs.shape[0]
-                             + 64 * m.npair + (1 << 20))
+        # packed output + queue room (up to four sub-quartets per candidate, both
+        # precision variants) + pair records (first- and second-centre copies)
+        self.footprint = int(8 * m.n_slots + 64 * n_cand
+                             + 80 * (m.prim_pairs.shape[0] + m.sub_prims.shape[0])
+                             + 3 * 64 * m.npair + (1 << 20))
         self.t_launch = None
         self.t_done = None
         self.result = None

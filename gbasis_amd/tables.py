@@ -110,6 +110,46 @@ def expand_shells(basis, coord_types):
     return out, func
 
 
+# Split threshold of second_centre_group: a primitive pair whose horizontal recursion
+# from the first centre would multiply rounding by more than this per unit of angular
+# momentum of the second shell is built on the second centre.  With 8, plain FP64
+# (same operation order as the kernels) stays within 2.3e-14 max(1, |value|) of
+# binary128 on all 293 sampled quartets of the dense s,s,p,d, the s,p,d,f and the
+# s-g workloads (exponents up to 1e5), where the first-centre build alone is off by
+# up to 5.2e-4 (23 s-g quartets above 1e-12); 32 already lets one quartet exceed
+# 1e-12.
+SPLIT_TAU = 8.0
+
+
+def second_centre_group(l_second, e_first, e_second, ab):
+    """True for the primitive pairs of a shell pair (first, second) that are built on
+    the second centre.
+
+    The Head-Gordon-Pople transfer [a, b+1_i| = [a+1_i, b| + AB_i [a, b| expands the
+    second shell's polynomial about the first centre.  When the product Gaussian of a
+    primitive pair sits near the second centre (tight exponent there, diffuse on the
+    first), the terms it sums exceed their result by about
+    r^(l_second), r = (|AB| + max(|PA|, s)) / max(|PB|, s), s = (2 zeta)^(-1/2) the
+    width of the product Gaussian, and the horizontal recursion after contraction
+    (gbasis/integrals/_two_elec_int_improved.py:364-588) passes the rounding of its
+    inputs on multiplied by that factor: for g shells with exponents up to 1e5 it
+    reaches 1e-4 Eh in FP64.  Such pairs are built on the second centre and transferred
+    to the first by the mirrored recursion, where the same factor is at most 2 per
+    unit of angular momentum.  Pairs are moved only when the second exponent is the
+    larger, the second shell has l > 0, and r exceeds SPLIT_TAU.
+    """
+    l_second = np.asarray(l_second)
+    e_first = np.asarray(e_first, dtype=np.float64)
+    e_second = np.asarray(e_second, dtype=np.float64)
+    ab = np.asarray(ab, dtype=np.float64)
+    zeta = e_first + e_second
+    s = 1.0 / np.sqrt(2.0 * zeta)
+    pa = e_second / zeta * ab
+    pb = e_first / zeta * ab
+    r = (ab + np.maximum(pa, s)) / np.maximum(pb, s)
+    return (l_second > 0) & (e_second > e_first) & (r > SPLIT_TAU)
+
+
 def prim_norm(exps, l):
     return (2.0 * exps / np.pi) ** 0.75 * (4.0 * exps) ** (l / 2.0)
 
@@ -177,7 +217,21 @@ class MoleculeTables:
         recs[:, 2:5] = P
         recs[:, 5:8] = P - A
         recs[:, 8] = K
+        # ---- build centre of each primitive pair (second_centre_group): the records
+        # of a pair are ordered with the first-centre group first; the second-centre
+        # group is also stored with P - B in place of P - A, for the sub-pair (b, a) ----
+        grp = second_centre_group(self.pair_lb[rp], ea, eb, np.sqrt(ab2))
+        order = np.lexsort((np.arange(rp.size), grp, rp))
+        recs = recs[order]
+        grp = grp[order]
         self.prim_pairs = recs
+        self.pair_nB = np.bincount(rp[grp], minlength=self.npair).astype(np.int32)
+        self.pair_nA = (self.pair_nprim - self.pair_nB).astype(np.int32)
+        sub = recs[grp].copy()
+        Bs = coords[sb][rp][order][grp]
+        sub[:, 5:8] = sub[:, 2:5] - Bs
+        self.sub_prims = sub
+        self.pair_subB_off = np.concatenate([[0], np.cumsum(self.pair_nB)[:-1]]).astype(np.int64)
         self.n_slots = _n_slots(self.nfunc)
 
     # ------------------------------------------------------------------
@@ -256,8 +310,41 @@ def _n_slots(nfunc):
     return npair * (npair + 1) // 2
 
 
+NUM_PHASES = 4
+NUM_BUCKETS = NUM_PHASES * 4096 * 2 * 2 * 16
+
+
+def queue_key(phase, cls, dense, fp32, nprimq):
+    """Bucket key of the device work queue (gbx_queue.hip):
+    ((((phase * 4096 + cls) * 2 + dense) * 2 + fp32) * 16 + prim_bin(nprimq)),
+    cls = ((la * 8 + lb) * 8 + lc) * 8 + ld in the sub-quartet's own orientation."""
+    return ((((np.asarray(phase, dtype=np.int64) * 4096 + cls) * 2 + dense) * 2 + fp32) * 16
+            + prim_bin(nprimq))
+
+
+def decode_head(head):
+    """(phase, cls tuple, dense, fp32) of key >> 4."""
+    fp32 = head & 1
+    dense = (head >> 1) & 1
+    c = head >> 2
+    cls = c & 4095
+    return c >> 12, ((cls >> 9) & 7, (cls >> 6) & 7, (cls >> 3) & 7, cls & 7), bool(dense), fp32
+
+
 class BatchTables:
-    """Tables of several molecules concatenated into one device-resident set."""
+    """Tables of several molecules concatenated into one device-resident set.
+
+    Pair entries 0 .. npair-1 are the shell pairs of the molecules (bra orientation
+    l_a >= l_b) with all their primitive pairs: the Schwarz pass and the work-queue
+    candidates use them.  A pair some of whose primitive pairs are built on the
+    second centre (second_centre_group) has two more entries after all of those:
+    its first-centre part (a, b) and its second-centre part (b, a), whose records
+    hold P - B.  ``pair_sub[p]`` = (first-centre entry, second-centre entry or -1);
+    the first-centre entry of an unsplit pair is the pair itself.  A shell quartet
+    (p|q) is the sum of up to four sub-quartets, one per phase:
+    (A_p|A_q), (B_p|A_q), (A_p|B_q), (B_p|B_q); phase 0 always runs and stores,
+    the others add into the same packed slots.
+    """
 
     def __init__(self, molecules):
         self.mols = molecules
@@ -267,6 +354,7 @@ class BatchTables:
         pinfo, pgeom, pbase, prims = [], [], [], []
         self.mol_pair_off, self.mol_slot_off = [], []
         coff = moff = 0
+        split_rows = []   # per molecule: (global pair ids, nA, nB, sub record offsets)
         for m in molecules:
             info, cmap, cscale, mat = m.shell_arrays()
             info = info.copy()
@@ -293,24 +381,73 @@ class BatchTables:
             pair_off += m.npair
             prim_off += m.prim_pairs.shape[0]
             slot_off += m.n_slots
-        if prim_off >= 2 ** 31 or pair_off >= 2 ** 31:
+        # second-centre records after all first-centre ones
+        sub_off = prim_off
+        for mi, m in enumerate(molecules):
+            split = np.flatnonzero(m.pair_nB > 0)
+            split_rows.append((split + self.mol_pair_off[mi], split, sub_off))
+            prims.append(m.sub_prims)
+            sub_off += m.sub_prims.shape[0]
+        npair = pair_off
+        pinfo = np.concatenate(pinfo)
+        pgeom = np.concatenate(pgeom)
+        pbase = np.concatenate(pbase)
+        pair_la = np.concatenate([m.pair_la for m in molecules])
+        pair_lb = np.concatenate([m.pair_lb for m in molecules])
+        pair_sub = np.stack([np.arange(npair), np.full(npair, -1)], axis=1).astype(np.int64)
+        ext_info, ext_geom, ext_base, ext_la, ext_lb = [], [], [], [], []
+        nxt = npair
+        for mi, (gids, local, soff) in enumerate(split_rows):
+            m = molecules[mi]
+            n = gids.size
+            if n == 0:
+                continue
+            a_ids = nxt + 2 * np.arange(n)
+            pair_sub[gids, 0] = a_ids
+            pair_sub[gids, 1] = a_ids + 1
+            ia = pinfo[gids].copy()
+            ia[:, 3] = m.pair_nA[local]
+            ib = np.zeros_like(ia)
+            ib[:, 0] = pinfo[gids, 1]
+            ib[:, 1] = pinfo[gids, 0]
+            ib[:, 2] = soff + m.pair_subB_off[local]
+            ib[:, 3] = m.pair_nB[local]
+            ext_info.append(np.stack([ia, ib], axis=1).reshape(-1, 4))
+            ga = pgeom[gids]
+            gb = ga.copy()
+            gb[:, :3] = -gb[:, :3]
+            ext_geom.append(np.stack([ga, gb], axis=1).reshape(-1, 4))
+            ext_base.append(np.repeat(pbase[gids], 2))
+            ext_la.append(np.stack([pair_la[gids], pair_lb[gids]], axis=1).reshape(-1))
+            ext_lb.append(np.stack([pair_lb[gids], pair_la[gids]], axis=1).reshape(-1))
+            nxt += 2 * n
+        if ext_info:
+            pinfo = np.concatenate([pinfo] + ext_info)
+            pgeom = np.concatenate([pgeom] + ext_geom)
+            pbase = np.concatenate([pbase] + ext_base)
+            pair_la = np.concatenate([pair_la] + ext_la)
+            pair_lb = np.concatenate([pair_lb] + ext_lb)
+        if sub_off >= 2 ** 31 or nxt >= 2 ** 31:
             raise ValueError("batch too large for 32-bit pair / primitive indices")
         self.shell_info = np.concatenate(infos)
         self.comp_map = np.concatenate(cmaps).astype(np.int32)
         self.comp_scale = np.concatenate(cscales)
         self.out_mat = np.concatenate(mats)
-        self.pair_info = np.concatenate(pinfo).astype(np.int32)
-        self.pair_geom = np.concatenate(pgeom)
-        self.pair_out_base = np.concatenate(pbase)
+        self.pair_info = pinfo.astype(np.int32)
+        self.pair_geom = pgeom
+        self.pair_out_base = pbase
         self.prim_pairs = np.concatenate(prims)
         self.n_slots = slot_off
-        self.npair = pair_off
-        self.pair_la = np.concatenate([m.pair_la for m in molecules])
-        self.pair_lb = np.concatenate([m.pair_lb for m in molecules])
+        self.npair = npair            # shell pairs (Schwarz pass, queue candidates)
+        self.npair_all = nxt          # plus the sub-pair entries
+        self.pair_sub = pair_sub.astype(np.int32)
+        self.pair_la = pair_la
+        self.pair_lb = pair_lb
         kind = self.shell_info[:, 2]
         # pairs touching a pure shell with l >= 2 need the dense output transform
         self.pair_dense = (kind[self.pair_info[:, 0]] == 1) | (kind[self.pair_info[:, 1]] == 1)
-        # per-pair key for the device queue builder: la | lb << 3 | dense << 6 | nprim << 8
+        # per pair entry, key for the device queue builder: la | lb << 3 | dense << 6 |
+        # nprim << 8 (la, lb in the entry's own orientation)
         self.pair_key = (self.pair_la.astype(np.int64) | (self.pair_lb.astype(np.int64) << 3)
                          | (self.pair_dense.astype(np.int64) << 6)
                          | (self.pair_info[:, 3].astype(np.int64) << 8)).astype(np.int32)
@@ -326,32 +463,61 @@ class BatchTables:
         self.generalized = any(m.nshell != m.n_orig for m in molecules)
         self._capacity = None
 
+    def sub_quartets(self, bra, ket):
+        """Sub-quartets of shell quartets (bra, ket) (pair entries < npair, oriented as
+        the work queue orients them): a list of (phase, bra entries, ket entries)."""
+        bra = np.asarray(bra, dtype=np.int64)
+        ket = np.asarray(ket, dtype=np.int64)
+        ab, bb = self.pair_sub[bra, 0], self.pair_sub[bra, 1]
+        ak, bk = self.pair_sub[ket, 0], self.pair_sub[ket, 1]
+        out = [(0, ab, ak)]
+        for phase, (x, y) in ((1, (bb, ak)), (2, (ab, bk)), (3, (bb, bk))):
+            sel = (x >= 0) & (y >= 0)
+            if sel.any():
+                out.append((phase, x[sel], y[sel]))
+        return out
+
     def bucket_capacity(self):
-        """Unscreened candidate quartets per bucket key (precision bit 0): an upper bound
-        of every bucket the device queue builder can fill, computed from the pair-key
-        histogram of each molecule."""
+        """Sub-quartets per bucket key (precision bit 0) over all unscreened candidates:
+        an upper bound of every bucket the device queue builder can fill, computed from
+        the histogram of the pairs' sub-pair keys in each molecule (pairs of equal
+        weight (la + lb, la) are counted in both bra/ket orders, since the device orders
+        them by pair index)."""
         if self._capacity is not None:
             return self._capacity
-        cap = np.zeros(262144, dtype=np.int64)
+        cap = np.zeros(NUM_BUCKETS, dtype=np.int64)
         for mi, m in enumerate(self.mols):
             po = self.mol_pair_off[mi]
-            keys, counts = np.unique(self.pair_key[po:po + m.npair], return_counts=True)
-            keys = keys.astype(np.int64)
+            g = np.arange(po, po + m.npair)
+            ka = self.pair_key[self.pair_sub[g, 0]].astype(np.int64)
+            sb_ = self.pair_sub[g, 1]
+            kb_ = np.where(sb_ >= 0, self.pair_key[np.maximum(sb_, 0)].astype(np.int64), -1)
+            tup, counts = np.unique(np.stack([ka, kb_], axis=1), axis=0, return_counts=True)
             counts = counts.astype(np.int64)
-            ki, kj = np.meshgrid(np.arange(keys.size), np.arange(keys.size), indexing="ij")
+            ki, kj = np.meshgrid(np.arange(len(tup)), np.arange(len(tup)), indexing="ij")
             sel = ki >= kj
             ki, kj = ki[sel], kj[sel]
             n = np.where(ki == kj, counts[ki] * (counts[ki] + 1) // 2, counts[ki] * counts[kj])
-            kp, kq = keys[ki], keys[kj]
-            la_p, lb_p, la_q, lb_q = kp & 7, (kp >> 3) & 7, kq & 7, (kq >> 3) & 7
-            swap = ((la_p + lb_p) * 16 + la_p) < ((la_q + lb_q) * 16 + la_q)
-            kb = np.where(swap, kq, kp)
-            kk = np.where(swap, kp, kq)
-            dense = ((kb >> 6) & 1) | ((kk >> 6) & 1)
-            npq = (kb >> 8) * (kk >> 8)
-            cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7)
-            key = ((cls * 2 + dense) * 2) * 16 + prim_bin(npq)
-            np.add.at(cap, key, n)
+            ti, tj = tup[ki], tup[kj]
+            la_p, lb_p = ti[:, 0] & 7, (ti[:, 0] >> 3) & 7
+            la_q, lb_q = tj[:, 0] & 7, (tj[:, 0] >> 3) & 7
+            sp, sq = (la_p + lb_p) * 16 + la_p, (la_q + lb_q) * 16 + la_q
+            swap = sp < sq
+            tb = np.where(swap[:, None], tj, ti)
+            tk = np.where(swap[:, None], ti, tj)
+            # two different tuples of equal weight: the device puts the pair of higher
+            # index in the bra, so either tuple may be the bra -- count both orders
+            tie = (sp == sq) & (ki != kj)
+            tb, tk = np.concatenate([tb, tj[tie]]), np.concatenate([tk, ti[tie]])
+            n = np.concatenate([n, n[tie]])
+            dense = ((tb[:, 0] >> 6) & 1) | ((tk[:, 0] >> 6) & 1)
+            for phase, (xb, xk) in enumerate(((0, 0), (1, 0), (0, 1), (1, 1))):
+                kb, kk = tb[:, xb], tk[:, xk]
+                ok = (kb >= 0) & (kk >= 0)
+                kb, kk = kb[ok], kk[ok]
+                cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7)
+                key = queue_key(phase, cls, dense[ok], 0, (kb >> 8) * (kk >> 8))
+                np.add.at(cap, key, n[ok])
         self._capacity = cap
         return cap
 

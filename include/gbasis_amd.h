@@ -31,9 +31,11 @@ extern "C" {
 #define GBX_FP32_BULK 1
 
 /* output mode of the ERI kernels */
-#define GBX_MODE_PACKED 0  /* write canonical 8-fold packed slots        */
-#define GBX_MODE_SCHWARZ 1 /* per bra pair p: d_out[2p] = sqrt(max|(ij|ij)|),
-                              d_out[2p+1] = sqrt(max|contracted [e0|f0]|)  */
+#define GBX_MODE_PACKED 0     /* write canonical 8-fold packed slots        */
+#define GBX_MODE_SCHWARZ 1    /* per bra pair p: d_out[2p] = sqrt(max|(ij|ij)|),
+                                 d_out[2p+1] = sqrt(max|contracted [e0|f0]|)  */
+#define GBX_MODE_PACKED_ADD 2 /* add into the canonical 8-fold packed slots
+                                 (sub-quartets of phases 1-3, see the work queue) */
 
 /* two-electron operator (gbx_tables.potential); the range-separated ones use
  * gbx_tables.omega, as gbasis/integrals/boys_functions.py:145-223 */
@@ -50,12 +52,19 @@ typedef struct gbx_tables {
   const int32_t *comp_map;       /* per Cartesian component: position in its shell  */
   const double *comp_scale;      /* per Cartesian component: output scale           */
   const double *out_mat;         /* dense per-shell matrices, row-major nout x ncart */
-  const int32_t *pair_info;      /* 4 x int32 per shell pair: shell_a, shell_b,
-                                    prim_off, nprim (bra orientation la >= lb)       */
-  const double *pair_geom;       /* 4 x double per pair: A-B (x,y,z), 0              */
+  const int32_t *pair_info;      /* 4 x int32 per pair entry: shell_a, shell_b,
+                                    prim_off, nprim.  Entries below the batch's pair
+                                    count are the shell pairs (la >= lb) with all
+                                    their primitive pairs; after them come the
+                                    sub-pair entries of split pairs: the primitive
+                                    pairs built on the first centre (a, b) and those
+                                    built on the second (b, a, records with P-B in
+                                    place of P-A)                                    */
+  const double *pair_geom;       /* 4 x double per pair entry: A-B (x,y,z), 0        */
   const uint64_t *pair_out_base; /* per pair: first packed slot of its molecule      */
   const double *prim_pairs;      /* 10 x double per primitive pair: zeta, 1/(2 zeta),
-                                    P(3), P-A(3), K', 0; K' = c_a c_b N_a N_b
+                                    P(3), P-A(3) (A: the entry's first shell),
+                                    K', 0; K' = c_a c_b N_a N_b
                                     exp(-ab/zeta |AB|^2) sqrt(2) pi^(5/4) / zeta     */
   const double *f2norm;          /* 1/sqrt(prod (2a_i-1)!!) by l, then component     */
   const double *boys_table;      /* boys_nt x boys_nord, row k: e^{-k h}, then
@@ -108,8 +117,10 @@ int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t 
                        int64_t *lds_bytes, int64_t *global_bytes_per_block);
 
 /* One angular-momentum class of shell quartets with the cooperative
- * (one workgroup of 256-1024 lanes per shell quartet) kernel.  d_quartets holds nquart (bra pair, ket pair)
- * int32 pairs in canonical orientation.  Replaces, for a batch of quartets,
+ * (one workgroup of 256-1024 lanes per shell quartet) kernel.  d_quartets holds
+ * nquart (bra, ket) int32 pair entries whose angular momenta are (la, lb) and
+ * (lc, ld) in the entries' own order (a sub-pair built on its second centre has
+ * la < lb).  Replaces, for a batch of quartets,
  * ElectronRepulsionIntegralImproved.construct_array_contraction
  * (gbasis/integrals/electron_repulsion.py:305-412) ->
  * compute_two_electron_integrals_os_hgp (gbasis/integrals/_two_elec_int_improved.py:591-732)
@@ -118,7 +129,10 @@ int gbx_coop_workspace(const gbx_program *prog, int32_t la, int32_t lb, int32_t 
  * writes d_out[2 bra] = sqrt(max|(ij|ij)|) without norm_cont, as
  * compute_schwarz_bound_shell_pair (gbasis/integrals/_schwarz_screening.py:14-59),
  * and d_out[2 bra + 1] = sqrt of the largest contracted input of the horizontal
- * recursion, the bound that selects the FP32 recursion bulk.
+ * recursion, the bound that selects the FP32 recursion bulk.  GBX_MODE_PACKED_ADD
+ * adds the block to the packed slots instead of storing it.  Packed placement
+ * writes each unique slot once: a quartet whose bra and ket entries cover the same
+ * two shells writes only its ij >= kl half.
  * d_scratch (scratch_bytes) is only used when the class does not fit in LDS.
  * d_count: NULL, or a device int32 holding the number of quartets to evaluate
  * (at most nquart, which is then only the capacity the launch is sized for), so
@@ -166,21 +180,34 @@ int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t
  * second value GBX_MODE_SCHWARZ writes per pair) and d_fp32_ok (NULL, or one
  * int8 per class*2+dense, class = ((la*8+lb)*8+lc)*8+ld) allows it;
  * d_mol_fp32_cutoff (NULL, or one cutoff per molecule) replaces fp32_cutoff
- * per molecule, so FP64 and mixed-precision requests can share one batch.  Buckets
- * are keyed (((((la*8+lb)*8+lc)*8+ld)*2+dense)*2+fp32)*16 + ceil(log2 nprimq),
- * GBX_NUM_BUCKETS of them.  count: quartets per bucket (and, if d_bucket_eris is
- * not NULL, packed slots and primitive quartets per bucket).  fill: writes the
- * (bra, ket) pairs at d_bucket_offset[key] + position (d_bucket_cursor zeroed). */
-#define GBX_NUM_BUCKETS 262144
+ * per molecule, so FP64 and mixed-precision requests can share one batch.  A kept
+ * quartet (p|q) becomes the sub-quartets (A_p|A_q) (phase 0), (B_p|A_q) (1),
+ * (A_p|B_q) (2), (B_p|B_q) (3) of its pairs' entries d_pair_sub[2p] = A_p (the
+ * pair itself if unsplit), d_pair_sub[2p+1] = B_p or -1; their integrals sum to the
+ * quartet's.  Buckets are keyed
+ * ((((phase*4096+cls)*2+dense)*2+fp32)*16 + min(ceil(log2 nprimq), 15),
+ * cls = ((la*8+lb)*8+lc)*8+ld of the sub-quartet's entries, GBX_NUM_BUCKETS of
+ * them.  d_pair_key per pair entry: la | lb << 3 | dense << 6 | nprim << 8.
+ * count: sub-quartets per bucket (and, if d_bucket_eris is not NULL, packed slots,
+ * counted in phase 0, and primitive quartets per bucket).  fill: writes the
+ * (bra, ket) entries at d_bucket_offset[key] + position (d_bucket_cursor zeroed;
+ * d_bucket_offset has GBX_NUM_BUCKETS + 1 entries, the last the end of the queue, and
+ * entries beyond a bucket's end, d_bucket_offset[key + 1], are not written: the
+ * bucket's cursor then exceeds its capacity).
+ * Launch phase 0 with GBX_MODE_PACKED and phases 1-3 after it, in order, with
+ * GBX_MODE_PACKED_ADD. */
+#define GBX_NUM_BUCKETS 1048576
 int gbx_quartets_count(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
-                       const int32_t *d_pair_key, const int32_t *d_pair_npf,
+                       const int32_t *d_pair_key, const int32_t *d_pair_sub,
+                       const int32_t *d_pair_npf,
                        const double *d_pair_bound, const double *d_pair_pbound,
                        double threshold, double fp32_cutoff, const double *d_mol_fp32_cutoff,
                        const int8_t *d_fp32_ok, int64_t n_cand, int32_t *d_bucket_count,
                        uint64_t *d_bucket_eris,
                        uint64_t *d_bucket_primq, void *stream);
 int gbx_quartets_fill(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_t *d_mol_pair_off,
-                      const int32_t *d_pair_key, const double *d_pair_bound,
+                      const int32_t *d_pair_key, const int32_t *d_pair_sub,
+                      const double *d_pair_bound,
                       const double *d_pair_pbound, double threshold, double fp32_cutoff,
                       const double *d_mol_fp32_cutoff, const int8_t *d_fp32_ok, int64_t n_cand,
                       const int64_t *d_bucket_offset,
@@ -188,8 +215,8 @@ int gbx_quartets_fill(int32_t n_mol, const int64_t *d_mol_cand_off, const int32_
 
 /* Segmented form of gbx_eri_reg: one grid of max_quartets lanes covers up to 16
  * quartet segments of one class (the primitive-count bins of the device work
- * queue); segment s holds d_seg_cnt[s] quartets starting at
- * d_quartets[2 * d_seg_off[s]].  Counts are read on the device, so the launch
+ * queue); segment s holds min(d_seg_cnt[s], d_seg_off[s + 1] - d_seg_off[s])
+ * quartets starting at d_quartets[2 * d_seg_off[s]] (d_seg_off: 17 entries).  Counts are read on the device, so the launch
  * needs no host round trip (max_quartets: an upper bound of their sum). */
 int gbx_eri_reg_segments(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                          const int32_t *d_quartets, const int64_t *d_seg_off,
@@ -224,15 +251,15 @@ int gbx_eri_hrr(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_
                 void *d_scratch, int64_t scratch_bytes, void *stream);
 
 /* Register form of gbx_eri_hrr for the split classes gbx_has_hrr_kernel lists, packed output
- * and shells with a scaled-permutation output map (Cartesian, pure l <= 1): one
+ * (mode GBX_MODE_PACKED or GBX_MODE_PACKED_ADD) and shells with a scaled-permutation output map (Cartesian, pure l <= 1): one
  * lane per quartet sums the acc_parts partials and runs the horizontal
  * recursion one ket column at a time (gbasis/integrals/_two_elec_int_improved.py:364-588,
  * gbasis/base_four_symm.py:160-244). */
 int gbx_has_hrr_kernel(int32_t la, int32_t lb, int32_t lc, int32_t ld);
 int gbx_eri_hrr_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
                     const int32_t *d_quartets, int64_t nquart, const int32_t *d_count,
-                    const double *d_acc, int64_t acc_stride, int32_t acc_parts, double *d_out,
-                    void *stream);
+                    const double *d_acc, int64_t acc_stride, int32_t acc_parts, int32_t mode,
+                    double *d_out, void *stream);
 
 /* Expand the canonical 8-fold packed array of one molecule (nfunc output
  * functions) to the dense (nfunc)^4 tensor; notation 0 = chemist (ab|cd),

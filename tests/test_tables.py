@@ -121,10 +121,12 @@ def test_bucket_capacity_deep_contractions():
     b = BatchTables([m])
     cap = b.bucket_capacity()
     keys = np.flatnonzero(cap)
-    assert int(cap.sum()) == b.n_cand
-    binid, fp32, dense = keys & 15, (keys >> 4) & 1, (keys >> 5) & 1
-    cls = keys >> 6
-    assert np.all(binid == MAX_PRIM_BIN) and not fp32.any() and not dense.any()
-    got = {(int(c >> 9) & 7, int(c >> 6) & 7, int(c >> 3) & 7, int(c) & 7) for c in cls}
+    assert int(cap[:int(cap.size) // 4].sum()) == b.n_cand   # phase 0: one per candidate
+    from gbasis_amd.tables import decode_head
+
+    heads = [decode_head(int(k) >> 4) for k in keys]
+    assert np.all((keys & 15) == MAX_PRIM_BIN)
+    assert not any(fp32 or dense for _, _, dense, fp32 in heads)
+    got = {cls for phase, cls, _, _ in heads if phase == 0}
     assert got == {(0, 0, 0, 0), (1, 0, 0, 0), (1, 0, 1, 0), (1, 1, 0, 0), (1, 1, 1, 0),
                    (1, 1, 1, 1)}
