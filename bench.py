@@ -316,18 +316,24 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
     algorithmic FLOP of the sampled ones."""
     from oracle.assembly import contraction_block
 
+    from gbasis_amd.programs import canonical_class
+
     rng = np.random.default_rng(0)
     pair_mol = np.zeros(batch.npair, dtype=np.int64)
     for mi, m in enumerate(batch.mols):
         pair_mol[batch.mol_pair_off[mi]:batch.mol_pair_off[mi] + m.npair] = mi
-    by_cls = {}
+    # shell quartets by canonical class: sampled from the phase-0 buckets (one sub-quartet
+    # per shell quartet, mapped back to the shell pairs), FLOPs of every phase
+    by_cls, flops_cls = {}, {}
     for bk in plans_fp64:
-        if bk.n > 0:
-            by_cls.setdefault(bk.cls, []).append(bk)
+        cc = canonical_class(*bk.cls)[0]
+        flops_cls[cc] = flops_cls.get(cc, 0.0) + bk.flops
+        if bk.n > 0 and bk.phase == 0:
+            by_cls.setdefault(cc, []).append(bk)
     stats = []
     for cls, bks in by_cls.items():
         n = sum(bk.n for bk in bks)
-        stats.append((sum(bk.flops for bk in bks) / n, cls, bks, n))
+        stats.append((flops_cls[cls] / n, cls, bks, n))
     stats.sort(key=lambda t: t[0])
     per_cls_budget = budget_s / max(1, len(stats))
     est = spent = 0.0
@@ -340,7 +346,7 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
             pending_flops += fpq * n_cls
             skipped += 1
             continue
-        quart = np.concatenate([bk.host_quartets() for bk in bks])
+        quart = batch.entry_pair[np.concatenate([bk.host_quartets() for bk in bks])]
         times = []
         for _ in range(256):
             q = quart[rng.integers(0, quart.shape[0])]
@@ -415,7 +421,7 @@ def _canon_cls(ls):
 def sample_errors(name, mol, basis, q_orig, threshold, outs):
     """Per-class errors of the GPU outputs ``outs`` {label: packed device array} at the
     oracle's sampled quartets (tests/golden/config_samples.npz), against the
-    extended-precision oracle; also the reference algorithm's own FP64 error there."""
+    binary128 oracle; also the reference algorithm's own FP64 error there."""
     import torch
 
     if not os.path.exists(SAMPLES_FILE):
@@ -424,7 +430,7 @@ def sample_errors(name, mol, basis, q_orig, threshold, outs):
     if f"{name}_idx" not in z:
         return None
     idx, shapes = z[f"{name}_idx"], z[f"{name}_shape"]
-    exact, ref64 = z[f"{name}_exact"], z[f"{name}_fp64"]
+    exact, ref64 = z[f"{name}_quad"], z[f"{name}_fp64"]
     first = {}
     for es in mol.shells:
         first.setdefault(es.orig, es.func_off)
@@ -463,11 +469,16 @@ def per_class_mixed_error(eng, dev, plans64, plansmx, n_slots):
     alone into zeroed buffers)."""
     import torch
 
+    from gbasis_amd.programs import canonical_class
+
+    def canon(bk):
+        return canonical_class(*bk.cls)[0]
+
     a = torch.zeros(n_slots, dtype=torch.float64, device="cuda")
     b = torch.zeros_like(a)
     out = {}
-    fp32_classes = {bk.cls for bk in plansmx if bk.launch_precision() == 1 and bk.n > 0}
-    for cls in sorted({bk.cls for bk in plans64}):
+    fp32_classes = {canon(bk) for bk in plansmx if bk.launch_precision() == 1 and bk.n > 0}
+    for cls in sorted({canon(bk) for bk in plans64}):
         if cls not in fp32_classes:
             # no quartet of the class runs the FP32 bulk: mixed == FP64 bit for bit
             out["".join(map(str, cls))] = {"slots": None, "max_abs_err": 0.0, "rms_err": 0.0,
@@ -475,11 +486,12 @@ def per_class_mixed_error(eng, dev, plans64, plansmx, n_slots):
             continue
         a.zero_()
         b.zero_()
+        # every phase of the class's shell quartets, in phase order
         for bk in plans64:
-            if bk.cls == cls:
+            if canon(bk) == cls:
                 eng.launch_bucket(dev, bk, a)
         for bk in plansmx:
-            if bk.cls == cls:
+            if canon(bk) == cls:
                 eng.launch_bucket(dev, bk, b)
         nz = a != 0
         d = (b - a).abs_()
@@ -488,7 +500,8 @@ def per_class_mixed_error(eng, dev, plans64, plansmx, n_slots):
             "slots": cnt, "max_abs_err": float(d.max()),
             "rms_err": float(torch.sqrt((d[nz] ** 2).mean())) if cnt else 0.0,
             "fp32_quartets": sum(bk.n for bk in plansmx
-                                 if bk.cls == cls and bk.launch_precision() == 1)}
+                                 if canon(bk) == cls and bk.phase == 0
+                                 and bk.launch_precision() == 1)}
     del a, b
     return out
 
@@ -515,8 +528,10 @@ def config_bench(args):
     q_orig = mol.schwarz_orig(q[:, 0].cpu().numpy())
     plans64 = eng.plan_device(dev, bound, thr, 0.0, stats=True)
     eris = sum(bk.eris for bk in plans64)
-    nquart = sum(bk.n for bk in plans64)
-    log(f"{name}: N={mol.nfunc}, {nquart} shell quartets, {eris} ERIs, {mol.n_slots} slots")
+    nquart = sum(bk.n for bk in plans64 if bk.phase == 0)
+    nsub = sum(bk.n for bk in plans64)
+    log(f"{name}: N={mol.nfunc}, {nquart} shell quartets ({nsub} sub-quartets), {eris} ERIs, "
+        f"{mol.n_slots} slots")
     tuner = None if args.no_tune else Autotuner(eng)
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     t64, q64, _, pb64, last64 = time_steps(eng, dev, bound, thr, 0.0, args.steps, args.warmup,
@@ -565,6 +580,7 @@ def config_bench(args):
         "dtype": "f64+f32",
         "data": "synthetic",
         "config": {"workload": desc, "name": name, "nfunc": mol.nfunc, "shell_quartets": nquart,
+                   "sub_quartets": nsub,
                    "eris": eris, "packed_slots": mol.n_slots, "schwarz_threshold": thr,
                    "fp32_cutoff": args.fp32_cutoff},
         "step_timed_as": "hip graph replay" if gmx is not None else "eager launches",
@@ -647,8 +663,9 @@ def main():
     plans64 = eng.plan_device(dev, bound, args.threshold, 0.0, stats=True)
     plansmx = eng.plan_device(dev, bound, args.threshold, args.fp32_cutoff, stats=True)
     eris = sum(bk.eris for bk in plans64)
-    nquart = sum(bk.n for bk in plans64)
-    n_fp32 = sum(bk.n for bk in plansmx if bk.precision == 1)
+    nquart = sum(bk.n for bk in plans64 if bk.phase == 0)
+    nsub = sum(bk.n for bk in plans64)
+    n_fp32 = sum(bk.n for bk in plansmx if bk.precision == 1 and bk.phase == 0)
     log(f"plans: {nquart} quartets, {eris} ERIs, {len(plans64)} buckets; "
         f"{n_fp32} quartets FP32-bulk in mixed ({time.perf_counter() - t0:.1f} s)")
 
@@ -719,7 +736,9 @@ def main():
                    "fp32_quartets": n_fp32, "kernel_path": args.path,
                    "autotuned": not args.no_tune,
                    "fp32_quartets_run_fp32": sum(bk.n for bk in lastmx
-                                                 if bk.launch_precision() == 1)},
+                                                 if bk.launch_precision() == 1
+                                                 and bk.phase == 0),
+                   "sub_quartets": nsub},
         "step_timed_as": "hip graph replay" if gmx is not None else "eager launches",
         "fp64": {"value": eris / t64_head, "ms_per_step": t64_head * 1e3},
         "eager_ms_per_step": {"fp64": t64 * 1e3, "mixed": tmx * 1e3},

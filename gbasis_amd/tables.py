@@ -441,6 +441,12 @@ class BatchTables:
         self.npair = npair            # shell pairs (Schwarz pass, queue candidates)
         self.npair_all = nxt          # plus the sub-pair entries
         self.pair_sub = pair_sub.astype(np.int32)
+        # shell pair of every entry (a sub-pair entry maps to the pair it splits)
+        entry_pair = np.arange(nxt, dtype=np.int64)
+        split_ids = np.flatnonzero(pair_sub[:, 1] >= 0)
+        entry_pair[pair_sub[split_ids, 0]] = split_ids
+        entry_pair[pair_sub[split_ids, 1]] = split_ids
+        self.entry_pair = entry_pair
         self.pair_la = pair_la
         self.pair_lb = pair_lb
         kind = self.shell_info[:, 2]

@@ -212,8 +212,9 @@ static void FN(group)(const FN(shell_t) * X, const FN(shell_t) * Y, const int* b
  * (the reference's orientation); mode 1: when the second shell has l > 0 and the
  * larger exponent, and the horizontal recursion from the first centre would
  * multiply rounding by more than tau per unit of angular momentum, estimated as
- * (|AB| + max(|PA|, s)) / max(|PB|, s) with s = 1/sqrt(2 zeta) the width of the
- * product Gaussian (tau = 0: every such pair). */
+ * r = (|AB| + max(|PA|, s)) / max(|PB|, s) with s = 1/sqrt(2 zeta) the width of the
+ * product Gaussian (tau = 0: every such pair); mode 2: the same with r^l_second >
+ * tau (the whole amplification of the second shell's transfer). */
 static void FN(orient)(const FN(shell_t) * S1, const FN(shell_t) * S2, int mode, double tau,
                        int* group) {
   double ab2 = 0;
@@ -222,12 +223,12 @@ static void FN(orient)(const FN(shell_t) * S1, const FN(shell_t) * S2, int mode,
   for (int i = 0; i < S1->K; ++i)
     for (int j = 0; j < S2->K; ++j) {
       int g = 0;
-      if (mode == 1 && S2->l > 0 && S2->e[j] > S1->e[i]) {
+      if ((mode == 1 || mode == 2) && S2->l > 0 && S2->e[j] > S1->e[i]) {
         const double z = S1->e[i] + S2->e[j];
         const double s = 1.0 / sqrt(2.0 * z);
         const double pa = S2->e[j] / z * ab, pb = S1->e[i] / z * ab;
         const double ratio = (ab + fmax(pa, s)) / fmax(pb, s);
-        g = ratio > tau;
+        g = mode == 1 ? ratio > tau : pow(ratio, S2->l) > tau;
       }
       group[i * S2->K + j] = g;
     }
