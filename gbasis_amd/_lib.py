@@ -57,6 +57,10 @@ SIGNATURES = {
     "gbx_eri_coop": (ctypes.c_int, [ctypes.POINTER(GbxTables), ctypes.POINTER(GbxProgram), c_i32,
                                     c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32,
                                     c_vp, c_vp, c_i64, c_vp]),
+    "gbx_eri_coop_noalloc": (ctypes.c_int, [ctypes.POINTER(GbxTables),
+                                            ctypes.POINTER(GbxProgram), c_i32, c_i32, c_i32,
+                                            c_i32, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32,
+                                            c_vp, c_vp]),
     "gbx_unpack_dense": (ctypes.c_int, [c_vp, c_i64, c_i32, c_vp, c_vp]),
     "gbx_jk_packed": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "gbx_has_reg_kernel": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i32]),

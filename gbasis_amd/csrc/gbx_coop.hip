@@ -27,6 +27,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 
 #include "gbasis_amd.h"
@@ -696,6 +697,60 @@ extern "C" int gbx_eri_coop(const gbx_tables* t, const gbx_program* prog, int32_
                                         prim_copies, mode, d_out, d_scratch, scratch_bytes, s)
              : launch_coop<float, false>(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count,
                                          prim_copies, mode, d_out, d_scratch, scratch_bytes, s);
+}
+
+// ---- entry point that allocates nothing ----
+// Global scratch of gbx_eri_coop_noalloc: a file-scope device array, one copy per GPU
+// the library runs on, held until the process ends.  Host code never uses its name as
+// an address: the device address of each GPU's copy comes from hipGetSymbolAddress,
+// taken once per device with that device current.
+constexpr int64_t kNoAllocScratchBytes = int64_t(128) << 20;   // 128 MiB
+static __device__ __align__(256) char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (looked up on its first use on
+// that device).
+int noalloc_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices)
+    return fail(GBX_ERR_HIP, "gbx_eri_coop_noalloc: no current device (or device index >= 64)");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+      return fail(GBX_ERR_HIP, std::string("gbx_eri_coop_noalloc: hipGetSymbolAddress of the "
+                                           "scratch array failed: ") + hipGetErrorString(e));
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return GBX_OK;
+}
+}  // namespace
+
+extern "C" int gbx_eri_coop_noalloc(const gbx_tables* t, const gbx_program* prog, int32_t la,
+                                    int32_t lb, int32_t lc, int32_t ld,
+                                    const int32_t* d_quartets, int64_t nquart,
+                                    const int32_t* d_count, int32_t precision,
+                                    int32_t prim_copies, int32_t mode, double* d_out,
+                                    void* stream) {
+  if (!prog || prim_copies < 1 || prim_copies > kCoopThreads)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_noalloc: bad program or prim_copies");
+  if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_noalloc: bad precision");
+  const CoopLayout l = coop_layout(prog->header, precision == GBX_FP64 ? 8 : 4, prim_copies);
+  if (l.total > coop_lds_limit() && l.total > kNoAllocScratchBytes)
+    return fail(GBX_ERR_ARG, "gbx_eri_coop_noalloc: the workspace of one workgroup (" +
+                                 std::to_string(l.total) + " bytes) exceeds the " +
+                                 std::to_string(kNoAllocScratchBytes) + "-byte scratch array");
+  void* sp = nullptr;
+  if (int rc = noalloc_scratch(&sp)) return rc;
+  return gbx_eri_coop(t, prog, la, lb, lc, ld, d_quartets, nquart, d_count, precision,
+                      prim_copies, mode, d_out, sp, kNoAllocScratchBytes, stream);
 }
 
 extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t la, int32_t lb,

@@ -143,6 +143,21 @@ int gbx_eri_coop(const gbx_tables *t, const gbx_program *prog, int32_t la, int32
                  const int32_t *d_count, int32_t precision, int32_t prim_copies, int32_t mode,
                  double *d_out, void *d_scratch, int64_t scratch_bytes, void *stream);
 
+/* gbx_eri_coop without the scratch arguments: allocates nothing.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).  Classes whose
+ * workgroup workspace exceeds the LDS work in a file-scope device array of 128 MiB
+ * (one copy per GPU, held until the process ends; its device address is looked up
+ * once per device), with as many workgroups as slices of it fit; a workspace larger
+ * than the array is rejected with GBX_ERR_ARG.  It calls no hipMalloc / hipFree.  The
+ * array needs no clearing (every workgroup initialises the parts of its slice it
+ * reads), but all calls on one GPU share it: calls that overlap on one GPU (other
+ * streams, other host threads) race and must be serialised by the caller. */
+int gbx_eri_coop_noalloc(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_t lb,
+                         int32_t lc, int32_t ld, const int32_t *d_quartets, int64_t nquart,
+                         const int32_t *d_count, int32_t precision, int32_t prim_copies,
+                         int32_t mode, double *d_out, void *stream);
+
 /* Primitive pass of the cooperative kernel, split over `parts` workgroups per
  * shell quartet: work item (q, r) sums the primitive quartets
  * [r c, (r + 1) c), c = ceil(nprimq / parts), of quartet q -- Boys, vertical
