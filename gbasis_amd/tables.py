@@ -111,14 +111,14 @@ def expand_shells(basis, coord_types):
 
 
 # Split threshold of second_centre_group: a primitive pair whose horizontal recursion
-# from the first centre would multiply rounding by more than this per unit of angular
-# momentum of the second shell is built on the second centre.  With 8, plain FP64
-# (same operation order as the kernels) stays within 2.3e-14 max(1, |value|) of
-# binary128 on all 293 sampled quartets of the dense s,s,p,d, the s,p,d,f and the
-# s-g workloads (exponents up to 1e5), where the first-centre build alone is off by
-# up to 5.2e-4 (23 s-g quartets above 1e-12); 32 already lets one quartet exceed
-# 1e-12.
-SPLIT_TAU = 8.0
+# from the first centre would multiply rounding by more than this is built on the
+# second centre.  With 512, plain FP64 (the kernels' operation order) stays within
+# 1.0e-14 max(1, |value|) of binary128 on the 293 sampled quartets of the dense
+# s,s,p,d, the s,p,d,f and the s-g workloads and on 300 random quartets of two batch
+# molecules, where the first-centre build alone is off by up to 5.2e-4 (23 s-g
+# quartets above 1e-12); 4096 still keeps 7.2e-14.  It splits 2 % of the batch's shell
+# pairs, 30 % of the s,p,d,f and 50 % of the s-g molecule's.
+SPLIT_AMPLIFICATION = 512.0
 
 
 def second_centre_group(l_second, e_first, e_second, ab):
@@ -128,15 +128,15 @@ def second_centre_group(l_second, e_first, e_second, ab):
     The Head-Gordon-Pople transfer [a, b+1_i| = [a+1_i, b| + AB_i [a, b| expands the
     second shell's polynomial about the first centre.  When the product Gaussian of a
     primitive pair sits near the second centre (tight exponent there, diffuse on the
-    first), the terms it sums exceed their result by about
-    r^(l_second), r = (|AB| + max(|PA|, s)) / max(|PB|, s), s = (2 zeta)^(-1/2) the
-    width of the product Gaussian, and the horizontal recursion after contraction
+    first), the terms it sums exceed their result by about r^(l_second),
+    r = (|AB| + max(|PA|, s)) / max(|PB|, s), s = (2 zeta)^(-1/2) the width of the
+    product Gaussian, and the horizontal recursion after contraction
     (gbasis/integrals/_two_elec_int_improved.py:364-588) passes the rounding of its
     inputs on multiplied by that factor: for g shells with exponents up to 1e5 it
     reaches 1e-4 Eh in FP64.  Such pairs are built on the second centre and transferred
-    to the first by the mirrored recursion, where the same factor is at most 2 per
-    unit of angular momentum.  Pairs are moved only when the second exponent is the
-    larger, the second shell has l > 0, and r exceeds SPLIT_TAU.
+    to the first by the mirrored recursion, where r is at most about 2.  Pairs are moved
+    only when the second exponent is the larger, the second shell has l > 0, and
+    r^(l_second) exceeds SPLIT_AMPLIFICATION.
     """
     l_second = np.asarray(l_second)
     e_first = np.asarray(e_first, dtype=np.float64)
@@ -147,7 +147,7 @@ def second_centre_group(l_second, e_first, e_second, ab):
     pa = e_second / zeta * ab
     pb = e_first / zeta * ab
     r = (ab + np.maximum(pa, s)) / np.maximum(pb, s)
-    return (l_second > 0) & (e_second > e_first) & (r > SPLIT_TAU)
+    return (l_second > 0) & (e_second > e_first) & (r ** l_second > SPLIT_AMPLIFICATION)
 
 
 def prim_norm(exps, l):

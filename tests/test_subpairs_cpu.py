@@ -111,13 +111,14 @@ def _wide_basis(seed, ls, k=3):
 
 def test_second_centre_group_rule():
     """Pairs move only with l_second > 0, the larger exponent on the second centre and a
-    large amplification; coincident centres never move."""
-    ab = np.array([2.0, 2.0, 2.0, 0.0, 2.0, 2.0])
-    e1 = np.array([0.1, 50.0, 0.1, 0.1, 0.1, 1.0])
-    e2 = np.array([50.0, 0.1, 50.0, 50.0, 0.12, 1.0])
-    l2 = np.array([1, 1, 0, 1, 1, 2])
+    large amplification r^l_second; coincident centres never move.  For exponents 0.1
+    and 50 at 2 bohr, r = 40: enough for l = 2 (1600), not for l = 1."""
+    ab = np.array([2.0, 2.0, 2.0, 0.0, 2.0, 2.0, 2.0])
+    e1 = np.array([0.1, 50.0, 0.1, 0.1, 0.1, 1.0, 0.1])
+    e2 = np.array([50.0, 0.1, 50.0, 50.0, 0.12, 1.0, 50.0])
+    l2 = np.array([2, 2, 0, 2, 4, 4, 1])
     assert second_centre_group(l2, e1, e2, ab).tolist() == [True, False, False, False, False,
-                                                            False]
+                                                            False, False]
 
 
 def test_sub_pair_entries():
@@ -168,8 +169,7 @@ def _replay_worst(ls):
 def test_replayed_split_quartets_vs_binary128(ls):
     """Quartets of shells with exponents 0.02 .. 1e5 as the device computes them from the
     split tables, against the binary128 oracle: within 1e-14 of the largest value."""
-    worst, b = _replay_worst(ls)
-    assert (b.pair_sub[:, 1] >= 0).any()
+    worst, _ = _replay_worst(ls)
     assert worst <= 1e-14, worst
 
 
@@ -179,7 +179,7 @@ def test_unsplit_tables_miss_the_bar(monkeypatch):
     the test above discriminates."""
     import gbasis_amd.tables as tables
 
-    monkeypatch.setattr(tables, "SPLIT_TAU", np.inf)
+    monkeypatch.setattr(tables, "SPLIT_AMPLIFICATION", np.inf)
     worst, b = _replay_worst([3, 3, 2, 2])
     assert not (b.pair_sub[:, 1] >= 0).any()
     assert worst > 1e-13, worst
@@ -227,4 +227,5 @@ def test_bucket_capacity_bounds_every_bucket(which):
             np.add.at(exact, queue_key(phase, cls, dense, 0, npq), 1)
     cap = b.bucket_capacity()
     assert np.all(cap >= exact), int((exact - cap).clip(0).sum())
-    assert (b.pair_sub[:, 1] >= 0).any()
+    if which in ("high_l", "wide"):
+        assert (b.pair_sub[:, 1] >= 0).any()
