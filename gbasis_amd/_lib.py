@@ -63,6 +63,9 @@ SIGNATURES = {
                                             c_vp, c_vp]),
     "gbx_unpack_dense": (ctypes.c_int, [c_vp, c_i64, c_i32, c_vp, c_vp]),
     "gbx_jk_packed": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+    "gbx_mo_workspace": (ctypes.c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),
+    "gbx_mo_transform": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_i64, c_vp, c_vp, c_i32,
+                                        c_vp]),
     "gbx_has_reg_kernel": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i32]),
     "gbx_split_count": (ctypes.c_int, [c_i32, c_i32, c_i32, c_i32]),
     "gbx_eri_reg_segments": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32,

@@ -53,6 +53,12 @@ for _c in filter(None, os.environ.get("GBX_FORCE_SPLIT", "").split(",")):
 # too many values live per column): they keep the cooperative HRR pass.
 HRR_REG_EXCLUDE = {(2, 2, 2, 2)}
 
+
+def _hrr_reg(cls):
+    """Split class with a register HRR kernel (classes with an f shell use the
+    cooperative HRR pass)."""
+    return cls not in HRR_REG_EXCLUDE and max(cls) <= 2
+
 BOYS_LDS = os.environ.get("GBX_BOYS_LDS", "1") == "1"
 LOOP_SWAP = os.environ.get("GBX_LOOP_SWAP", "1") == "1"
 
@@ -157,25 +163,67 @@ def class_plan(cls):
     raise RuntimeError(f"class {cls}: no split fits the register budget")
 
 
-def _reg_classes():
-    """Classes with l <= 2 per shell that get generated register kernels: the kernel
-    orientations of the shell quartets (la >= lb, lc >= ld, bra heavier), then the
-    orientations of their sub-quartets whose bra or ket entry is built on its second
+# Classes with an f shell get split register kernels (FP64 only, cooperative HRR pass)
+# when their accumulators split into at most F_MAX_PARTS register-sized parts; the
+# others (e.g. (ff|ff), (ff|fd), (fd|fd)) stay on the cooperative kernel.  Classes with
+# more accumulators than F_MAX_NACC are not tried.
+F_SHELLS = os.environ.get("GBX_F_REGISTER", "1") == "1"
+F_MAX_NACC = 2400
+F_MAX_PARTS = 32
+
+
+def _orientations(out):
+    """The orientations of the sub-quartets whose bra or ket entry is built on its second
     centre (entries (lb, la) with lb > 0, and likewise on the ket)."""
-    out = []
-    for la in range(3):
-        for lb in range(la + 1):
-            for lc in range(3):
-                for ld in range(lc + 1):
-                    if (la + lb, la) < (lc + ld, lc):
-                        continue
-                    out.append((la, lb, lc, ld))
     extra = set()
     for la, lb, lc, ld in out:
         for bra in ((la, lb), (lb, la)) if lb > 0 else ((la, lb),):
             for ket in ((lc, ld), (ld, lc)) if ld > 0 else ((lc, ld),):
                 extra.add(bra + ket)
-    return out + sorted(extra - set(out))
+    return sorted(extra - set(out))
+
+
+def _canonical(lmax):
+    out = []
+    for la in range(lmax + 1):
+        for lb in range(la + 1):
+            for lc in range(lmax + 1):
+                for ld in range(lc + 1):
+                    if (la + lb, la) < (lc + ld, lc):
+                        continue
+                    out.append((la, lb, lc, ld))
+    return out
+
+
+def _plannable(cls):
+    """True if the class has a fused or split register plan."""
+    if class_program(*cls).nacc > F_MAX_NACC:
+        return False
+    try:
+        kind, parts = class_plan(cls)
+    except RuntimeError:
+        return False
+    return kind == "fused" or len(parts) <= F_MAX_PARTS
+
+
+def _reg_classes():
+    """Classes that get generated register kernels: with l <= 2 per shell, the kernel
+    orientations of the shell quartets (la >= lb, lc >= ld, bra heavier) and of their
+    sub-quartets; then the classes with an f shell (and their sub-quartet orientations)
+    that have a register plan."""
+    low = _canonical(2)
+    out = low + _orientations(low)
+    if F_SHELLS:
+        f = [c for c in _canonical(3) if max(c) == 3]
+        f = f + _orientations(f)
+        out += [c for c in f if _plannable(c)]
+    return out
+
+
+def fp64_only(cls):
+    """Register kernels of classes with an f shell are instantiated in FP64 only (the
+    mixed form keeps such classes in FP64)."""
+    return max(cls) >= 3
 
 
 def _f2(comp):
@@ -471,6 +519,23 @@ def _key(c):
     return ((c[0] * 8 + c[1]) * 8 + c[2]) * 8 + c[3]
 
 
+def _launch_lines(cls, name):
+    """Launch of a register kernel in the precision asked for (FP64-only classes refuse
+    FP32)."""
+    if fp64_only(cls):
+        return ["      if (precision != GBX_FP64)",
+                '        return fail(GBX_ERR_UNSUPPORTED, "register kernels of classes with '
+                'an f shell are FP64 only");',
+                f"      hipLaunchKernelGGL(({name}<double>), dim3(grid), dim3(kRegThreads), 0, "
+                "s, *t, a);",
+                f'      return check_launch("{name}");']
+    return [f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
+            "dim3(grid), dim3(kRegThreads), 0, s, *t, a);",
+            f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
+            "dim3(kRegThreads), 0, s, *t, a);",
+            f'      return check_launch("{name}");']
+
+
 def _part_sources(classes):
     """Kernel sources of every class, grouped into N_PARTS translation units of similar
     size, each with its own launchers (which return NOT_HERE for other classes)."""
@@ -483,7 +548,7 @@ def _part_sources(classes):
         else:
             for k in range(len(pl)):
                 src.append(split_kernel_source(cls, k))
-            if cls not in HRR_REG_EXCLUDE:
+            if _hrr_reg(cls):
                 src.append(hrr_kernel_source(cls))
         units.append((cls, kind, len(pl), "\n".join(src)))
     bins = [[] for _ in range(N_PARTS)]
@@ -510,11 +575,7 @@ def _part_sources(classes):
                 continue
             name = f"eri_reg_{''.join(map(str, cls))}"
             o.append(f"    case {_key(cls)}:")
-            o.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
-                     "dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
-            o.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
-                     "dim3(kRegThreads), 0, s, *t, a);")
-            o.append(f'      return check_launch("{name}");')
+            o += _launch_lines(cls, name)
         o.append("    default:")
         o.append(f"      return {NOT_HERE};")
         o.append("  }")
@@ -532,11 +593,7 @@ def _part_sources(classes):
             for j in range(n):
                 name = f"eri_split_{''.join(map(str, cls))}_{j}"
                 o.append(f"    case {_key(cls) * 64 + j}:")
-                o.append(f"      if (precision == GBX_FP64) hipLaunchKernelGGL(({name}<double>), "
-                         "dim3(grid), dim3(kRegThreads), 0, s, *t, a);")
-                o.append(f"      else hipLaunchKernelGGL(({name}<float>), dim3(grid), "
-                         "dim3(kRegThreads), 0, s, *t, a);")
-                o.append(f'      return check_launch("{name}");')
+                o += _launch_lines(cls, name)
         o.append("    default:")
         o.append(f"      return {NOT_HERE};")
         o.append("  }")
@@ -549,7 +606,7 @@ def _part_sources(classes):
         o.append("  (void)grid;")
         o.append("  switch (key) {")
         for cls, kind, _, _ in members:
-            if kind == "fused" or cls in HRR_REG_EXCLUDE:
+            if kind == "fused" or not _hrr_reg(cls):
                 continue
             name = f"eri_hrr_{''.join(map(str, cls))}"
             o.append(f"    case {_key(cls)}:")
@@ -573,6 +630,31 @@ def _write(path, src):
         fh.write(src)
 
 
+def _stamp():
+    """Hash of everything the generated sources depend on: this generator, the class
+    programs and the settings read from the environment."""
+    import hashlib
+
+    h = hashlib.sha1()
+    here = os.path.dirname(os.path.abspath(__file__))
+    for name in ("codegen.py", "programs.py"):
+        with open(os.path.join(here, name), "rb") as fh:
+            h.update(fh.read())
+    h.update(repr((REG_BUDGET, SPLIT_BUDGET, sorted(FORCE_SPLIT), BOYS_LDS, LOOP_SWAP, _U,
+                   F_SHELLS)).encode())
+    return h.hexdigest()[:16]
+
+
+def up_to_date(path=OUT):
+    """True if ``path`` and every part were generated from the current inputs."""
+    base = path[:-len(".hip")]
+    if not all(os.path.exists(p) for p in [path] + [f"{base}_{k}.hip" for k in range(N_PARTS)]):
+        return False
+    with open(path) as fh:
+        fh.readline()
+        return fh.readline().strip() == f"// inputs {_stamp()}"
+
+
 def generate(path=OUT):
     """Write the kernel parts gbx_reg_gen_<k>.hip and the dispatch file ``path``."""
     classes = _reg_classes()
@@ -581,8 +663,9 @@ def generate(path=OUT):
         _write(f"{base}_{k}.hip", src)
     fused = [c for c in classes if class_plan(c)[0] == "fused"]
     split = [(c, len(class_plan(c)[1])) for c in classes if class_plan(c)[0] != "fused"]
-    hrr = [c for c, _ in split if c not in HRR_REG_EXCLUDE]
+    hrr = [c for c, _ in split if _hrr_reg(c)]
     m = ["// GENERATED by gbasis_amd/codegen.py -- do not edit.",
+         f"// inputs {_stamp()}",
          f"// Dispatch over the {N_PARTS} parts of the generated register kernels.",
          '#include "gbx_reg.h"', "", "namespace gbx {", ""]
     for k in range(N_PARTS):
@@ -649,5 +732,8 @@ def generate(path=OUT):
 
 
 if __name__ == "__main__":
-    cl = generate()
-    print(f"generated register kernels for {len(cl)} classes -> {OUT}")
+    if "--force" not in sys.argv and up_to_date():
+        print(f"register kernels up to date ({OUT})")
+    else:
+        cl = generate()
+        print(f"generated register kernels for {len(cl)} classes -> {OUT}")

@@ -257,10 +257,12 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
             //   + a_i/(2z) ([a-1_i]^(m) - rho/z [a-1_i]^(m+1))
             // C centre: [a|c+1_i]^(m) = QC_i [a|c]^(m) + WQ_i [a|c]^(m+1)
             //   + c_i/(2e) ([a|c-1_i]^(m) - rho/e [a|c-1_i]^(m+1)) + a_i/(2(z+e)) [a-1_i|c]^(m+1)
-            TB v = pd * S[s0 * PF + c] + wd * S[s1 * PF + c];
-            if (s2 != kNoSlot) v += k1 * r_i2 * (S[s2 * PF + c] - r_ro * S[s3 * PF + c]);
-            if (s4 != kNoSlot) v += static_cast<TB>(code >> 12) * r_i2s * S[s4 * PF + c];
-            S[dst * PF + c] = v;
+            // slot * PF as a shift (PF is a power of two)
+            TB v = pd * S[(s0 << pf_shift) + c] + wd * S[(s1 << pf_shift) + c];
+            if (s2 != kNoSlot)
+              v += k1 * r_i2 * (S[(s2 << pf_shift) + c] - r_ro * S[(s3 << pf_shift) + c]);
+            if (s4 != kNoSlot) v += static_cast<TB>(code >> 12) * r_i2s * S[(s4 << pf_shift) + c];
+            S[(dst << pf_shift) + c] = v;
           }
         }
         __syncthreads();

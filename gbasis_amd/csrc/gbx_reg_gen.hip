@@ -1,4 +1,5 @@
 // GENERATED by gbasis_amd/codegen.py -- do not edit.
+// inputs 6ea9f7d8b661e2e2
 // Dispatch over the 8 parts of the generated register kernels.
 #include "gbx_reg.h"
 
@@ -109,6 +110,19 @@ bool has_reg_generated(int la, int lb, int lc, int ld) {
     case 640:
     case 648:
     case 656:
+    case 1536:
+    case 1544:
+    case 1545:
+    case 1552:
+    case 1600:
+    case 1608:
+    case 1664:
+    case 1672:
+    case 1728:
+    case 704:
+    case 712:
+    case 1216:
+    case 1224:
       return true;
     default:
       return false;
@@ -137,6 +151,94 @@ int split_count_generated(int la, int lb, int lc, int ld) {
       return 4;
     case 1162:
       return 7;
+    case 1176:
+      return 5;
+    case 1553:
+      return 2;
+    case 1560:
+      return 2;
+    case 1609:
+      return 2;
+    case 1616:
+      return 2;
+    case 1617:
+      return 5;
+    case 1618:
+      return 11;
+    case 1624:
+      return 4;
+    case 1625:
+      return 10;
+    case 1673:
+      return 4;
+    case 1680:
+      return 3;
+    case 1681:
+      return 9;
+    case 1682:
+      return 18;
+    case 1688:
+      return 8;
+    case 1689:
+      return 17;
+    case 1736:
+      return 2;
+    case 1737:
+      return 6;
+    case 1744:
+      return 5;
+    case 1745:
+      return 15;
+    case 1746:
+      return 32;
+    case 1752:
+      return 12;
+    case 1753:
+      return 29;
+    case 713:
+      return 3;
+    case 714:
+      return 7;
+    case 715:
+      return 14;
+    case 720:
+      return 2;
+    case 721:
+      return 7;
+    case 722:
+      return 13;
+    case 728:
+      return 6;
+    case 729:
+      return 12;
+    case 1225:
+      return 5;
+    case 1226:
+      return 11;
+    case 1227:
+      return 21;
+    case 1232:
+      return 4;
+    case 1233:
+      return 10;
+    case 1234:
+      return 21;
+    case 1240:
+      return 8;
+    case 1241:
+      return 19;
+    case 1546:
+      return 2;
+    case 1610:
+      return 5;
+    case 1611:
+      return 11;
+    case 1674:
+      return 10;
+    case 1675:
+      return 21;
+    case 1738:
+      return 16;
     default:
       return 0;
   }

@@ -21,7 +21,6 @@ FP32-bulk mixed precision.
 """
 
 import numpy as np
-import torch
 
 from .base_four_symm import BaseFourIndexSymmetric
 from .boys_functions import get_boys_function
@@ -114,18 +113,11 @@ def _dense(basis, coord_types, notation, schwarz_threshold, transform, device="c
         dense = eng.unpack_dense(packed, nfunc, notation)
         out = dense.cpu().numpy()
         return out
-    arr = eng.unpack_dense(packed, nfunc, "chemist")
+    # sum_ijkl T_ai T_bj T_ck T_dl (ij|kl) (gbasis/base_four_symm.py:611-615) by the
+    # library's two-pass transform of the packed array (no dense AO array)
+    arr = eng.mo_transform(packed, nfunc, transform, notation)
     del packed
-    tr = torch.as_tensor(np.asarray(transform, dtype=np.float64), device=arr.device)
-    # sum_ijkl T_ai T_bj T_ck T_dl (ij|kl)  (gbasis/base_four_symm.py:611-615)
-    arr = torch.tensordot(tr, arr, dims=([1], [0]))
-    arr = torch.tensordot(tr, arr, dims=([1], [1]))
-    arr = torch.tensordot(tr, arr, dims=([1], [2]))
-    arr = torch.tensordot(tr, arr, dims=([1], [3]))
-    arr = arr.permute(3, 2, 1, 0)
-    if notation == "physicist":
-        arr = arr.permute(0, 2, 1, 3)
-    return arr.contiguous().cpu().numpy()
+    return arr.cpu().numpy()
 
 
 def electron_repulsion_integral(basis, transform=None, notation="physicist"):

@@ -604,8 +604,10 @@ class EriEngine:
             layout = ("cap", mixed)
             cap = b.bucket_capacity().copy()
             if mixed:
+                # the FP32 variant of a bucket only where the class may run FP32
                 cap = cap.reshape(-1, 2, 16)
-                cap[:, 1, :] = cap[:, 0, :]
+                allow = self._fp32_ok_host[np.arange(cap.shape[0]) % self._fp32_ok_host.size]
+                cap[:, 1, :] = cap[:, 0, :] * allow[:, None]
                 cap = cap.reshape(-1)
         if layout not in dev.queue:
             offs = np.concatenate([[0], np.cumsum(cap)]).astype(np.int64)   # nb + 1 entries
@@ -772,6 +774,7 @@ class EriEngine:
             ci = ((cls[0] * 8 + cls[1]) * 8 + cls[2]) * 8 + cls[3]
             if not ok:
                 mask[ci * 2 + int(dense)] = 0
+        self._fp32_ok_host = mask.astype(np.int64)
         self.fp32_ok = torch.from_numpy(mask).to(self.device)
 
     def unpack_dense(self, packed, nfunc, notation):
@@ -779,6 +782,26 @@ class EriEngine:
         check(self.lib.gbx_unpack_dense(ptr(packed), nfunc, 1 if notation == "physicist" else 0,
                                         ptr(dense), self._stream()), "gbx_unpack_dense")
         return dense
+
+    def mo_transform(self, packed, nfunc, transform, notation):
+        """Dense (m, m, m, m) device array of sum_ijkl T_ai T_bj T_ck T_dl (ij|kl) from the
+        packed ERIs (``transform``: m x nfunc), in chemist or physicist order; the
+        half-transformed workspace (nfunc(nfunc+1)/2 x m(m+1)/2) is the only
+        intermediate."""
+        tr = torch.as_tensor(np.ascontiguousarray(transform, dtype=np.float64),
+                             device=self.device)
+        if tr.ndim != 2 or tr.shape[1] != nfunc:
+            raise ValueError(f"transform must have shape (m, {nfunc}), got {tuple(tr.shape)}")
+        m = int(tr.shape[0])
+        trt = tr.t().contiguous()
+        n_half = ctypes.c_int64()
+        check(self.lib.gbx_mo_workspace(nfunc, m, ctypes.byref(n_half)), "gbx_mo_workspace")
+        half = torch.empty(n_half.value, dtype=torch.float64, device=self.device)
+        out = torch.empty((m,) * 4, dtype=torch.float64, device=self.device)
+        check(self.lib.gbx_mo_transform(ptr(packed), nfunc, ptr(tr), ptr(trt), m, ptr(half),
+                                        ptr(out), 1 if notation == "physicist" else 0,
+                                        self._stream()), "gbx_mo_transform")
+        return out
 
     def jk(self, packed, nfunc, density):
         """Coulomb and exchange matrices (device float64, n x n) of one molecule's
