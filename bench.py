@@ -133,7 +133,8 @@ def _host_busy(srv, span):
     return {k: v / span for k, v in srv.host_seconds.items()}
 
 
-def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_batch=8):
+def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_batch=16,
+                  batch_flops=4e11):
     """Serving workload: requests of 2-30 atoms (seeds 4000 + i, s/p/d, K <= 6),
     precision FP64 or mixed with probability 1/2 each.  First the saturated rate
     (n_sat requests all arriving at once), then Poisson arrivals at 50 % and 80 %
@@ -150,7 +151,7 @@ def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_b
     bases = [serving_request(4000 + i) for i in range(n_requests)]
     precs = np.where(rng.random(n_requests) < 0.5, "mixed", "fp64")
     srv = EriServer(threshold=threshold, fp32_cutoff=cutoff, n_streams=n_streams,
-                    max_batch=max_batch)
+                    max_batch=max_batch, batch_flops=batch_flops)
     clock = srv.clock
     # warm-up: programs and workspaces
     for i in range(4):
@@ -207,7 +208,10 @@ def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_b
     out = {"workload": "requests of 2-30 atoms (s/p/d Cartesian, K 1-6, exps [0.05,1e3]), "
                        "FP64 or mixed with probability 1/2, Schwarz threshold "
                        f"{threshold:g}; result = packed ERIs resident in HBM",
-           "streams": n_streams, "saturated": {"requests": n_sat, "req_per_s": sat_rate,
+           "streams": n_streams,
+           "batching": {"max_requests": max_batch, "max_flops": batch_flops,
+                        "rule": "summed estimated FLOPs of a batch <= max(opener's, max_flops)"},
+           "saturated": {"requests": n_sat, "req_per_s": sat_rate,
                                                "eris_per_s": sat_eris / sat_span}}
     all_eris = None
     out["saturated"]["host_busy"] = _host_busy(srv, sat_span)
@@ -306,7 +310,49 @@ def roofline_entry(plans, per_bucket, prefer_reg=True):
             "all_kernels": {"achieved": total_f / total_t / 1e12, "frac": total_f / total_t / peak}}
 
 
-def cpu_baseline(batch, plans_fp64, budget_s=20.0):
+def time_schwarz(eng, dev, reps=3):
+    """Milliseconds of the Schwarz pass (the diagonal (ij|ij) quartets of every shell pair
+    and the pair bounds), warm, outside the timed step."""
+    import torch
+
+    eng.pair_bounds(dev, eng.schwarz(dev))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        eng.pair_bounds(dev, eng.schwarz(dev))
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) / reps * 1e3
+
+
+def fp32_fraction(plans_mixed):
+    """Fraction of the shell quartets whose recursion bulk actually runs in FP32 in the
+    mixed form (after the autotuner's choices)."""
+    tot = sum(bk.n for bk in plans_mixed if bk.phase == 0)
+    f32 = sum(bk.n for bk in plans_mixed if bk.phase == 0 and bk.launch_precision() == 1)
+    return f32 / tot if tot else 0.0
+
+
+def mixed_detail(plans_mixed, t64, tmx):
+    """What the mixed form is, in numbers."""
+    return (f"f64 Boys, prefactors, accumulation and HRR; f32 recursion bulk for "
+            f"{100 * fp32_fraction(plans_mixed):.2f} % of the shell quartets (those below the "
+            f"f32 cutoff that the autotuner measured faster in f32); mixed step / f64 step = "
+            f"{tmx / t64:.3f}: on this chip non-packed f32 issues at the f64 rate, so the "
+            f"mixed form is an f64 computation in all but those quartets")
+
+
+def cpu_baselines(batch, plans_fp64):
+    """The oracle's CPU rate with BLAS on every host core the process may use (at most
+    16), then with BLAS on one thread."""
+    ncores = max(1, min(16, len(os.sched_getaffinity(0))))
+    with threadpool_limits(limits=ncores):
+        allc = cpu_baseline(batch, plans_fp64, budget_s=12.0, cores=ncores)
+    with threadpool_limits(limits=1):
+        one = cpu_baseline(batch, plans_fp64, budget_s=12.0, cores=1)
+    return allc, one
+
+
+def cpu_baseline(batch, plans_fp64, budget_s=20.0, cores=1):
     """Oracle (NumPy restatement of the reference engine) timed on a stratified sample,
     on one core (BLAS limited to one thread).
 
@@ -368,13 +414,13 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0):
         s_flops += fpq * len(times)
     if pending_flops > 0:
         est += pending_flops * s_time / s_flops
-    return {"value": total_eris / est, "unit": "ERIs/s", "cores": 1, "kind": "port",
+    return {"value": total_eris / est, "unit": "ERIs/s", "cores": cores, "kind": "port",
             "sample": f"{sampled} shell quartets over {len(stats) - skipped} of {len(stats)} "
                       f"classes ({spent:.1f} s of CPU), extrapolated by class counts"
                       + (f" and, for the {skipped} costliest classes, by the algorithmic FLOP "
                          "model at the sampled oracle rate" if skipped else "")
                       + "; oracle = NumPy restatement of the reference OS+HGP engine, one "
-                      "process, BLAS limited to one thread"}
+                      f"process, BLAS limited to {cores} thread" + ("s" if cores > 1 else "")}
 
 
 class _OracleShell:
@@ -507,15 +553,46 @@ def per_class_mixed_error(eng, dev, plans64, plansmx, n_slots):
 
 
 def config_bench(args):
+    """One single-molecule workload (``--workload NAME``): prints its JSON line."""
+    print(json.dumps(run_workload(args.workload, args)), flush=True)
+
+
+def workload_summary(r):
+    """The keys of one single-molecule workload's line that the default bench line
+    carries for it."""
+    s = r.get("oracle_samples") or {}
+    summ = s.get("summary", {})
+    per = s.get("per_class", {})
+    return {
+        "workload": r["config"]["workload"],
+        "eris": r["config"]["eris"], "shell_quartets": r["config"]["shell_quartets"],
+        "fp64": r["fp64"], "mixed": {"value": r["value"], "ms_per_step": r["ms_per_step"]},
+        "step_timed_as": r["step_timed_as"],
+        "schwarz_ms": r.get("schwarz_ms"),
+        "dense_unpack_ms": r.get("dense_unpack_ms"),
+        "mixed_vs_fp64": {k: r["mixed_vs_fp64"][k] for k in ("max_abs_err", "rms_err")},
+        "fp32_quartet_fraction": r.get("fp32_quartet_fraction"),
+        "vs_binary128": {
+            "sampled_quartets": summ.get("sampled_quartets"),
+            "max_gpu_fp64_err": summ.get("max_gpu_fp64_err"),
+            "max_gpu_mixed_err": summ.get("max_gpu_mixed_err"),
+            "max_ref_fp64_err": summ.get("max_ref_fp64_err"),
+            "per_class_gpu_fp64_err": {k: v["gpu_fp64_err"] for k, v in per.items()},
+        },
+        "roofline": r["roofline"],
+        "classes_fp64": r["classes_fp64"][:6],
+    }
+
+
+def run_workload(name, args):
     """One single-molecule workload: FP64 and mixed steps, per-class errors, roofline,
-    CPU baseline.  Prints one JSON line."""
+    CPU baseline.  Returns its JSON record."""
     import torch
     from gbasis_amd import workloads
     from gbasis_amd.autotune import Autotuner
     from gbasis_amd.engine import DeviceTables, get_engine
     from gbasis_amd.tables import BatchTables, MoleculeTables
 
-    name = args.workload
     desc, thr, dense = CONFIG_WORKLOADS[name]
     basis = getattr(workloads, name)()
     mol = MoleculeTables(basis)
@@ -523,6 +600,7 @@ def config_bench(args):
     eng = get_engine("cuda")
     eng.launch_streams = args.launch_streams
     dev = DeviceTables(batch)
+    schwarz_ms = time_schwarz(eng, dev)
     q = eng.schwarz(dev)
     bound = eng.pair_bounds(dev, q)
     q_orig = mol.schwarz_orig(q[:, 0].cpu().numpy())
@@ -578,6 +656,7 @@ def config_bench(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": tmxh * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+f32",
+        "dtype_detail": mixed_detail(plansmx, t64h, tmxh),
         "data": "synthetic",
         "config": {"workload": desc, "name": name, "nfunc": mol.nfunc, "shell_quartets": nquart,
                    "sub_quartets": nsub,
@@ -586,6 +665,8 @@ def config_bench(args):
         "step_timed_as": "hip graph replay" if gmx is not None else "eager launches",
         "fp64": {"value": eris / t64h, "ms_per_step": t64h * 1e3},
         "eager_ms_per_step": {"fp64": t64 * 1e3, "mixed": tmx * 1e3},
+        "schwarz_ms": schwarz_ms,
+        "fp32_quartet_fraction": fp32_fraction(plansmx),
         "dense_unpack_ms": unpack_ms,
         "mixed_vs_fp64": {"max_abs_err": max_err, "rms_err": rms_err, "per_class": per_cls},
         "oracle_samples": samples,
@@ -595,9 +676,8 @@ def config_bench(args):
         "classes_mixed": class_table(lastmx, pbmx),
     }
     if not args.no_cpu:
-        with threadpool_limits(limits=1):
-            result["cpu_baseline"] = cpu_baseline(batch, plans64)
-    print(json.dumps(result), flush=True)
+        result["cpu_baseline"], result["cpu_baseline_one_core"] = cpu_baselines(batch, plans64)
+    return result
 
 
 def main():
@@ -620,8 +700,15 @@ def main():
     ap.add_argument("--serving-requests", type=int, default=2000)
     ap.add_argument("--serving-saturation", type=int, default=300)
     ap.add_argument("--serving-streams", type=int, default=4)
-    ap.add_argument("--serving-max-batch", type=int, default=8)
+    ap.add_argument("--serving-max-batch", type=int, default=16)
+    ap.add_argument("--serving-batch-flops", type=float, default=4e11,
+                    help="cap on a serving batch's summed algorithmic FLOPs (0: size batches "
+                         "by count and the 2x packed-slot rule only)")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-workloads", action="store_true",
+                    help="skip the single-molecule workloads in the default line")
+    ap.add_argument("--workload-steps", type=int, default=2,
+                    help="timed steps of each single-molecule workload in the default line")
     ap.add_argument("--launch-streams", type=int, default=4,
                     help="spread the class launches of a step over this many streams (the "
                          "per-class timings then need --launch-streams 1)")
@@ -657,6 +744,7 @@ def main():
     eng.path = args.path
     eng.launch_streams = args.launch_streams
     dev = DeviceTables(batch)
+    schwarz_ms = time_schwarz(eng, dev)
     q = eng.schwarz(dev)
     bound = eng.pair_bounds(dev, q)
     torch.cuda.synchronize()
@@ -726,8 +814,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": tmx_head * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64+f32",
-        "dtype_detail": "f64 Boys, prefactors, accumulation and HRR; f32 recursion bulk of the "
-                        "low-bound quartets the autotuner keeps in f32",
+        "dtype_detail": mixed_detail(lastmx, t64_head, tmx_head),
         "data": "synthetic",
         "config": {"workload": f"batch of {args.molecules} molecules, 4-24 atoms, s/p/d "
                                "Cartesian, K 1-6, exps [0.05,1e3], packed 8-fold output",
@@ -745,6 +832,8 @@ def main():
         "graph_ms_per_step": {"fp64": None if g64 is None else g64 * 1e3,
                               "mixed": None if gmx is None else gmx * 1e3},
         "queue_build_ms": {"fp64": q64 * 1e3, "mixed": qmx * 1e3},
+        "schwarz_ms": schwarz_ms,
+        "fp32_quartet_fraction": fp32_fraction(lastmx),
         "mixed_vs_fp64": {"max_abs_err": max_err, "rms_err": rms_err},
         "wall_ms_per_step": {"fp64": w64 * 1e3, "mixed": wmx * 1e3},
         "roofline": roof,
@@ -753,15 +842,25 @@ def main():
         "classes_mixed": class_table(lastmx, pbmx),
     }
     if not args.no_cpu:
-        with threadpool_limits(limits=1):
-            result["cpu_baseline"] = cpu_baseline(batch, plans64)
+        result["cpu_baseline"], result["cpu_baseline_one_core"] = cpu_baselines(batch, plans64)
+    del dev, out64
+    torch.cuda.empty_cache()
+    if not args.no_workloads:
+        # the single-molecule workloads, one warm-up and a few timed steps each
+        sub = argparse.Namespace(**vars(args))
+        sub.steps, sub.warmup, sub.no_cpu = args.workload_steps, 1, True
+        result["workloads"] = {}
+        for name in CONFIG_WORKLOADS:
+            t0 = time.perf_counter()
+            result["workloads"][name] = workload_summary(run_workload(name, sub))
+            torch.cuda.empty_cache()
+            log(f"workload {name}: {time.perf_counter() - t0:.0f} s")
     if not args.no_serving:
-        del dev, out64
-        torch.cuda.empty_cache()
         result["serving"] = serving_bench(args.serving_requests, args.serving_saturation,
                                           args.threshold, args.fp32_cutoff,
                                           args.serving_streams,
-                                          max_batch=args.serving_max_batch)
+                                          max_batch=args.serving_max_batch,
+                                          batch_flops=args.serving_batch_flops or None)
     print(json.dumps(result), flush=True)
 
 

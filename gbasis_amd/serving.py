@@ -24,10 +24,12 @@ torch tensor shared between streams).
   device.  FP64 and mixed requests share a batch through per-molecule FP32
   cutoffs.
 * Order: with policy "sjf" (default) the waiting request with the smallest
-  cost / (1 + age / age_tau) opens a batch (cost: packed slots, age: time since
-  arrival), and the batch takes the next requests by the same score whose cost
-  is at most twice the opener's -- small molecules do not wait behind large
-  ones, and a request waiting longer than max_wait (0.5 s) opens the next batch.
+  cost / (1 + age / age_tau) opens a batch (cost: the algorithmic FLOPs of all of
+  the molecule's shell quartets, ``estimated_flops``; age: time since arrival), and
+  the batch takes the next requests by the same score while its summed cost stays
+  within max(opener's cost, batch_flops) -- batches are sized by work, not by
+  count, so that a batch of small molecules and one large molecule take similar
+  times, and a request waiting longer than max_wait (0.5 s) opens the next batch.
   Policy "fifo" takes requests in arrival order.
 * Results do not depend on the company a request keeps: the engine runs in
   deterministic mode (variant choices depend on class and primitive bin only)
@@ -67,6 +69,7 @@ class EriRequest:
         self.footprint = int(8 * m.n_slots + 64 * n_cand
                              + 80 * (m.prim_pairs.shape[0] + m.sub_prims.shape[0])
                              + 3 * 64 * m.npair + (1 << 20))
+        self.cost = estimated_flops(m)
         self.t_launch = None
         self.t_done = None
         self.result = None
@@ -83,6 +86,33 @@ class EriRequest:
 
     def latency(self):
         return None if self.t_done is None else self.t_done - self.t_arrival
+
+
+def estimated_flops(m):
+    """Algorithmic FLOPs (gbasis_amd.roofline) of every unique shell quartet of the
+    molecule tables ``m``, before screening: pairs are grouped by their canonical
+    (l_a, l_b); a group pair (P, Q) holds n_P n_Q quartets (n_P (n_P + 1) / 2 when
+    P = Q) with sum_P K sum_Q K primitive quartets ((sum_P K)^2 + sum_P K^2) / 2
+    when P = Q), K the primitive pairs of a pair."""
+    from .roofline import class_flops
+
+    la = np.maximum(m.pair_la, m.pair_lb)
+    lb = np.minimum(m.pair_la, m.pair_lb)
+    keys, inv = np.unique(la * 8 + lb, return_inverse=True)
+    k = m.pair_nprim.astype(np.float64)
+    n = np.bincount(inv).astype(np.float64)
+    s1 = np.bincount(inv, weights=k)
+    s2 = np.bincount(inv, weights=k * k)
+    total = 0.0
+    for i in range(keys.size):
+        for j in range(i, keys.size):
+            fp, fc = class_flops(keys[i] // 8, keys[i] % 8, keys[j] // 8, keys[j] % 8)
+            if i == j:
+                prim, quart = 0.5 * (s1[i] * s1[i] + s2[i]), 0.5 * n[i] * (n[i] + 1)
+            else:
+                prim, quart = s1[i] * s1[j], n[i] * n[j]
+            total += prim * fp + quart * fc
+    return total
 
 
 class _Batch:
@@ -133,13 +163,14 @@ class EriServer:
     """Admission, continuous batching and completion of ERI requests."""
 
     def __init__(self, device="cuda", threshold=1e-12, fp32_cutoff=1e-2, hbm_budget=None,
-                 n_streams=4, max_batch=8, clock=time.perf_counter, policy="sjf",
-                 age_tau=0.1, max_wait=0.5):
+                 n_streams=4, max_batch=16, clock=time.perf_counter, policy="sjf",
+                 age_tau=0.1, max_wait=0.5, batch_flops=4e11):
         if policy not in ("sjf", "fifo"):
             raise ValueError("policy must be 'sjf' or 'fifo'")
         self.policy = policy
         self.age_tau = float(age_tau)
         self.max_wait = float(max_wait)
+        self.batch_flops = None if batch_flops is None else float(batch_flops)
         # an engine of its own: deterministic variant choices and per-stream workspaces
         self.eng = EriEngine(device)
         self.eng.deterministic = True
@@ -238,7 +269,8 @@ class EriServer:
             if b is not None or not self.pending:
                 continue
             take = select_batch(list(self.pending), self.clock(), self.budget - self.used,
-                                self.max_batch, self.policy, self.age_tau, self.max_wait)
+                                self.max_batch, self.policy, self.age_tau, self.max_wait,
+                                self.batch_flops)
             for r in take:
                 self.pending.remove(r)
             need = sum(r.footprint for r in take)
@@ -284,10 +316,14 @@ class EriServer:
 
 
 def select_batch(pending, now, budget_left, max_batch, policy="sjf", age_tau=0.1,
-                 max_wait=0.5):
+                 max_wait=0.5, batch_cost=None):
     """The requests of the next batch, from ``pending`` (arrival order).  Objects need
-    ``footprint``, ``t_arrival`` and ``tables.n_slots``.  With "sjf" a request that
-    has waited longer than ``max_wait`` seconds opens the batch (oldest first)."""
+    ``footprint``, ``t_arrival`` and ``tables.n_slots``, and ``cost`` when
+    ``batch_cost`` is given.  With "sjf" a request that has waited longer than
+    ``max_wait`` seconds opens the batch (oldest first).  The batch then holds at most
+    ``max_batch`` requests and, with ``batch_cost``, a summed cost of at most
+    max(opener's cost, batch_cost); without it, requests of at most twice the
+    opener's packed slots."""
     take, need = [], 0
     if policy == "fifo":
         for r in pending:
@@ -296,16 +332,22 @@ def select_batch(pending, now, budget_left, max_batch, policy="sjf", age_tau=0.1
             take.append(r)
             need += r.footprint
         return take
-    order = sorted(pending, key=lambda r: r.tables.n_slots /
+    def cost(r):
+        return r.cost if batch_cost is not None else r.tables.n_slots
+
+    order = sorted(pending, key=lambda r: cost(r) /
                    (1.0 + max(0.0, now - r.t_arrival) / age_tau))
     overdue = [r for r in pending if now - r.t_arrival > max_wait]
     if overdue:
         order.remove(overdue[0])
         order.insert(0, overdue[0])
+    spent = 0.0
     for r in order:
         if len(take) >= max_batch:
             break
-        if take and r.tables.n_slots > 2 * take[0].tables.n_slots:
+        if take and batch_cost is None and r.tables.n_slots > 2 * take[0].tables.n_slots:
+            continue
+        if take and batch_cost is not None and spent + r.cost > max(take[0].cost, batch_cost):
             continue
         if need + r.footprint > budget_left:
             if not take:
@@ -313,6 +355,7 @@ def select_batch(pending, now, budget_left, max_batch, policy="sjf", age_tau=0.1
             continue
         take.append(r)
         need += r.footprint
+        spent += cost(r)
     return take
 
 
