@@ -5,7 +5,8 @@ or binary64 with the same operation order (``prec="f64"``), with the primitive p
 each shell pair built on the first centre (``mode=0``, the reference's orientation) or
 each on the centre of the larger exponent where the horizontal recursion from the
 first centre would amplify rounding by more than ``tau`` per unit of angular momentum
-(``mode=1``; ``tau=0``: every pair whose second centre has the larger exponent).
+(``mode=1``; ``tau=0``: every pair whose second centre has the larger exponent) or by
+more than ``tau`` in all (``mode=2``, r^l_second > tau, the device tables' rule).
 The library is built on first use (oracle/build_hp.sh, gcc and libquadmath).
 """
 

@@ -161,14 +161,14 @@ def test_erf_p_shell_from_centre_derivative():
         assert abs(got[ax, 3, 4, 5] - ref) <= 1e-8 * max(1.0, abs(ref)), (ax, got[ax, 3, 4, 5], ref)
 
 
-def _md_primitive(a, b, c, d, ea, eb, ec, ed, A, B, C, D):
+def _md_primitive(a, b, c, d, ea, eb, ec, ed, A, B, C, D, dps=40):
     """Unnormalised Cartesian primitive (ab|cd) by McMurchie-Davidson (Hermite
     expansion coefficients E and Hermite Coulomb integrals R, Helgaker, Jorgensen and
-    Olsen, Molecular Electronic-Structure Theory, ch. 9), in mpmath at 40 digits: an
-    evaluation that shares no recursion with the Obara-Saika oracle."""
+    Olsen, Molecular Electronic-Structure Theory, ch. 9), in mpmath at ``dps`` digits:
+    an evaluation that shares no recursion with the Obara-Saika oracle."""
     import mpmath
 
-    mpmath.mp.dps = 40
+    mpmath.mp.dps = dps
     mpf = mpmath.mpf
 
     def E(i, j, t, Qx, p, mu, XPA, XPB):
@@ -226,7 +226,7 @@ def _md_primitive(a, b, c, d, ea, eb, ec, ed, A, B, C, D):
                         for phi, f3 in enumerate(Ecd[2]):
                             total += ((-1) ** (tau + nu + phi) * et * eu * ev * f1 * f2 * f3
                                       * R(t + tau, u + nu, v + phi, 0, alpha, X, Y, Z))
-    return float(2 * mpmath.pi ** mpf(2.5) / (p * q * mpmath.sqrt(p + q)) * total)
+    return 2 * mpmath.pi ** mpf(2.5) / (p * q * mpmath.sqrt(p + q)) * total
 
 
 @pytest.mark.parametrize("seed", range(10))
@@ -240,7 +240,7 @@ def test_oracle_vs_mcmurchie_davidson(seed):
     comp = [_comps(int(l))[rng.integers(0, (l + 1) * (l + 2) // 2)] for l in ls]
     exps = np.exp(rng.uniform(np.log(0.2), np.log(8.0), 4))
     coords = rng.uniform(-1.5, 1.5, (4, 3))
-    ref = _md_primitive(*comp, *exps, *coords)
+    ref = float(_md_primitive(*comp, *exps, *coords))
     norm = 1.0
     for (x, y, z), e in zip(comp, exps):
         l = x + y + z
@@ -249,3 +249,43 @@ def test_oracle_vs_mcmurchie_davidson(seed):
     for ket in ("etr", "vrr"):
         got = _prim(*comp, exps, coords, ket=ket)
         assert abs(got - ref * norm) <= 1e-11 * max(1e-3, abs(ref * norm)), (ket, comp, got, ref * norm)
+
+
+def _prim_norm(comp, exps):
+    norm = 1.0
+    for (x, y, z), e in zip(comp, exps):
+        l = x + y + z
+        df = np.prod([np.prod(np.arange(2 * k - 1, 0, -2)) if k > 0 else 1.0 for k in (x, y, z)])
+        norm *= (2 * e / np.pi) ** 0.75 * (4 * e) ** (l / 2) / np.sqrt(df)
+    return norm
+
+
+@pytest.mark.parametrize("seed", range(8))
+def test_binary128_oracle_vs_mcmurchie_davidson(seed):
+    """The binary128 oracle (oracle/eri_hp.c, which writes the exact column of the
+    workload samples) on primitive f and g quartets of the hard kind of the s-g
+    workload -- a diffuse exponent (0.02-1) paired with a tight one (1e2-1e5) in the bra,
+    0.02-10 with 10-1e4 in the ket, centres within 1.2 bohr -- against the
+    McMurchie-Davidson brute force at 60 digits (itself stable to 1e-25 against 90
+    digits).  Three components of each block are checked, to 1e-15 relative to the
+    block's largest value; plain FP64 Obara-Saika built on the first centres is off by
+    1e-8 to 1e+2 relative on these quartets."""
+    import mpmath
+
+    from oracle.hp import hp_block_raw
+    from oracle.os_hgp import _comps
+
+    rng = np.random.default_rng(90 + seed)
+    ls = [int(x) for x in rng.integers(3, 5, 4)]
+    exps = np.exp(rng.uniform(np.log([0.02, 1e2, 0.02, 10.0]), np.log([1.0, 1e5, 10.0, 1e4])))
+    coords = rng.uniform(-0.6, 0.6, (4, 3))
+    blk = hp_block_raw(ls, coords, [[e] for e in exps], [[1.0]] * 4, prec="quad")
+    scale = np.abs(blk).max()
+    picks = [tuple(int(rng.integers(0, (l + 1) * (l + 2) // 2)) for l in ls) for _ in range(3)]
+    for idx in picks:
+        comp = [_comps(l)[i] for l, i in zip(ls, idx)]
+        ref = _md_primitive(*comp, *exps, *coords, dps=60)
+        hi = _md_primitive(*comp, *exps, *coords, dps=90)
+        assert abs(ref - hi) <= mpmath.mpf("1e-25") * max(abs(hi), mpmath.mpf("1e-30"))
+        want = float(ref * mpmath.mpf(_prim_norm(comp, exps)))
+        assert abs(blk[idx] - want) <= 1e-15 * scale, (ls, idx, blk[idx], want, scale)
