@@ -760,7 +760,7 @@ extern "C" int gbx_eri_hrr(const gbx_tables* t, const gbx_program* prog, int32_t
                            const int32_t* d_count, int32_t mode, const double* d_acc,
                            int64_t acc_stride, int32_t acc_parts, double* d_out, void* d_scratch,
                            int64_t scratch_bytes, void* stream) {
-  if (!t || !prog || !d_out || !d_acc || (nquart > 0 && !d_quartets))
+  if (!t || !prog || !d_out || (nquart > 0 && (!d_quartets || !d_acc)))
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: null argument");
   if (!class_in_range(la, lb, lc, ld) || prog->header[6] != la + lb + lc + ld)
     return fail(GBX_ERR_ARG, "gbx_eri_hrr: program does not match the class");
@@ -786,7 +786,7 @@ extern "C" int gbx_eri_coop_acc(const gbx_tables* t, const gbx_program* prog, in
                                 int32_t prim_copies, int32_t parts, double* d_acc,
                                 int64_t acc_stride, void* d_scratch, int64_t scratch_bytes,
                                 void* stream) {
-  if (!t || !prog || !d_acc || (nquart > 0 && !d_quartets))
+  if (!t || !prog || (nquart > 0 && (!d_quartets || !d_acc)))
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: null argument");
   if (prim_copies < 1 || prim_copies > kCoopThreads || (prim_copies & (prim_copies - 1)))
     return fail(GBX_ERR_ARG, "gbx_eri_coop_acc: prim_copies must be a power of two in [1, 64]");

@@ -63,7 +63,7 @@ extern "C" int gbx_eri_split(const gbx_tables* t, int32_t la, int32_t lb, int32_
                              const int32_t* d_count, int32_t precision,
                              int32_t lanes_per_quartet, double* d_acc,
                              int64_t acc_stride, void* stream) {
-  if (!t || !d_acc || (nquart > 0 && !d_quartets))
+  if (!t || (nquart > 0 && (!d_quartets || !d_acc)))
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: null argument");
   if (precision != GBX_FP64 && precision != GBX_FP32_BULK)
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_split: bad precision");
@@ -93,7 +93,7 @@ extern "C" int gbx_eri_hrr_reg(const gbx_tables* t, int32_t la, int32_t lb, int3
                                int32_t ld, const int32_t* d_quartets, int64_t nquart,
                                const int32_t* d_count, const double* d_acc, int64_t acc_stride,
                                int32_t acc_parts, int32_t mode, double* d_out, void* stream) {
-  if (!t || !d_out || !d_acc || (nquart > 0 && !d_quartets))
+  if (!t || !d_out || (nquart > 0 && (!d_quartets || !d_acc)))
     return gbx::fail(GBX_ERR_ARG, "gbx_eri_hrr_reg: null argument");
   if (gbx_has_hrr_kernel(la, lb, lc, ld) == 0)
     return gbx::fail(GBX_ERR_UNSUPPORTED, "gbx_eri_hrr_reg: no register HRR kernel for class");

@@ -427,6 +427,9 @@ class EriEngine:
                                            quartets.shape[0], ptr(count), precision, mode,
                                            ptr(out), self._stream()), f"gbx_eri_reg{cls}")
             return "reg"
+        if quartets.shape[0] == 0:
+            # a bucket of capacity 0 (a class and bin with no quartet in this precision)
+            return "empty"
         st, _, _ = self.programs.get(cls)
         nsplit = self.split_count(cls)
         if path != "coop" and nsplit > 0:

@@ -54,6 +54,25 @@ def test_error_path_without_device():
     assert lib.gbx_boys(None, fake, 1, 0, fake, None) == -1
 
 
+def test_empty_launch_accepts_null_workspace():
+    """A launch of capacity 0 (a class and bin with no quartet in this precision) may pass
+    null quartet and accumulator pointers: it succeeds without touching the device."""
+    lib = _lib.load()
+    tab = _lib.GbxTables()
+    tab.boys_nord, tab.boys_nt, tab.boys_h, tab.potential = 33, 401, 0.1, _lib.GBX_POT_COULOMB
+    fake = ctypes.c_void_p(0x1000)       # never dereferenced
+    cls = (2, 2, 2, 2)
+    assert lib.gbx_split_count(*cls) > 0
+    assert lib.gbx_eri_split(ctypes.byref(tab), *cls, 0, None, 0, None, _lib.GBX_FP64, 1,
+                             None, 0, None) == 0
+    assert lib.gbx_eri_split(ctypes.byref(tab), *cls, 0, fake, 4, None, _lib.GBX_FP64, 1,
+                             None, 4, None) == -1
+    assert b"null" in lib.gbx_last_error()
+    if lib.gbx_has_hrr_kernel(2, 2, 2, 1):
+        assert lib.gbx_eri_hrr_reg(ctypes.byref(tab), 2, 2, 2, 1, None, 0, None, None, 0, 1,
+                                   _lib.GBX_MODE_PACKED, fake, None) == 0
+
+
 def test_range_separated_argument_errors():
     """Operator checks happen before any device work (CPU)."""
     import pytest
