@@ -90,6 +90,7 @@ def time_steps(eng, dev, bound, threshold, cutoff, steps, warmup, tuner=None):
     stop.record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
+    eng.check_queue(dev)   # no bucket overflowed (its quartets would be missing)
     per_bucket = np.zeros(len(plans))
     for q0, marks in recs:
         t_queue += q0.elapsed_time(marks[0]) * 1e-3
@@ -327,8 +328,8 @@ def time_schwarz(eng, dev, reps=3):
 def fp32_fraction(plans_mixed):
     """Fraction of the shell quartets whose recursion bulk actually runs in FP32 in the
     mixed form (after the autotuner's choices)."""
-    tot = sum(bk.n for bk in plans_mixed if bk.phase == 0)
-    f32 = sum(bk.n for bk in plans_mixed if bk.phase == 0 and bk.launch_precision() == 1)
+    tot = sum(bk.n for bk in plans_mixed if bk.stores())
+    f32 = sum(bk.n for bk in plans_mixed if bk.stores() and bk.launch_precision() == 1)
     return f32 / tot if tot else 0.0
 
 
@@ -368,13 +369,13 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0, cores=1):
     pair_mol = np.zeros(batch.npair, dtype=np.int64)
     for mi, m in enumerate(batch.mols):
         pair_mol[batch.mol_pair_off[mi]:batch.mol_pair_off[mi] + m.npair] = mi
-    # shell quartets by canonical class: sampled from the phase-0 buckets (one sub-quartet
+    # shell quartets by canonical class: sampled from the storing buckets (one sub-quartet
     # per shell quartet, mapped back to the shell pairs), FLOPs of every phase
     by_cls, flops_cls = {}, {}
     for bk in plans_fp64:
         cc = canonical_class(*bk.cls)[0]
         flops_cls[cc] = flops_cls.get(cc, 0.0) + bk.flops
-        if bk.n > 0 and bk.phase == 0:
+        if bk.n > 0 and bk.stores():
             by_cls.setdefault(cc, []).append(bk)
     stats = []
     for cls, bks in by_cls.items():
@@ -546,7 +547,7 @@ def per_class_mixed_error(eng, dev, plans64, plansmx, n_slots):
             "slots": cnt, "max_abs_err": float(d.max()),
             "rms_err": float(torch.sqrt((d[nz] ** 2).mean())) if cnt else 0.0,
             "fp32_quartets": sum(bk.n for bk in plansmx
-                                 if canon(bk) == cls and bk.phase == 0
+                                 if canon(bk) == cls and bk.stores()
                                  and bk.launch_precision() == 1)}
     del a, b
     return out
@@ -606,7 +607,7 @@ def run_workload(name, args):
     q_orig = mol.schwarz_orig(q[:, 0].cpu().numpy())
     plans64 = eng.plan_device(dev, bound, thr, 0.0, stats=True)
     eris = sum(bk.eris for bk in plans64)
-    nquart = sum(bk.n for bk in plans64 if bk.phase == 0)
+    nquart = sum(bk.n for bk in plans64 if bk.stores())
     nsub = sum(bk.n for bk in plans64)
     log(f"{name}: N={mol.nfunc}, {nquart} shell quartets ({nsub} sub-quartets), {eris} ERIs, "
         f"{mol.n_slots} slots")
@@ -751,9 +752,9 @@ def main():
     plans64 = eng.plan_device(dev, bound, args.threshold, 0.0, stats=True)
     plansmx = eng.plan_device(dev, bound, args.threshold, args.fp32_cutoff, stats=True)
     eris = sum(bk.eris for bk in plans64)
-    nquart = sum(bk.n for bk in plans64 if bk.phase == 0)
+    nquart = sum(bk.n for bk in plans64 if bk.stores())
     nsub = sum(bk.n for bk in plans64)
-    n_fp32 = sum(bk.n for bk in plansmx if bk.precision == 1 and bk.phase == 0)
+    n_fp32 = sum(bk.n for bk in plansmx if bk.precision == 1 and bk.stores())
     log(f"plans: {nquart} quartets, {eris} ERIs, {len(plans64)} buckets; "
         f"{n_fp32} quartets FP32-bulk in mixed ({time.perf_counter() - t0:.1f} s)")
 
@@ -824,7 +825,7 @@ def main():
                    "autotuned": not args.no_tune,
                    "fp32_quartets_run_fp32": sum(bk.n for bk in lastmx
                                                  if bk.launch_precision() == 1
-                                                 and bk.phase == 0),
+                                                 and bk.stores()),
                    "sub_quartets": nsub},
         "step_timed_as": "hip graph replay" if gmx is not None else "eager launches",
         "fp64": {"value": eris / t64_head, "ms_per_step": t64_head * 1e3},

@@ -6,7 +6,8 @@
 // iff Q_p * Q_q >= threshold (gbasis/integrals/_schwarz_screening.py:184-191;
 // threshold <= 0 keeps everything), oriented so the "heavier" pair is the
 // bra, and marked FP32-bulk iff P_p * P_q < fp32_cutoff (P: bound of the
-// contracted [e0|f0], the input of the horizontal recursion).  A kept quartet
+// contracted [e0|f0], the input of the horizontal recursion) and fp32_ok allows
+// the class, checked again for each sub-quartet's own class.  A kept quartet
 // becomes up to four sub-quartets, one per phase, from the pairs' sub-pair
 // entries pair_sub[p] = (first-centre part A_p, second-centre part B_p or -1):
 //   phase 0 (A_p|A_q), always; phase 1 (B_p|A_q); phase 2 (A_p|B_q); phase 3 (B_p|B_q)
@@ -129,7 +130,11 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       const int kb = a.pair_key[eb], kk = a.pair_key[ek];
       const int cls = (((kb & 7) * 8 + ((kb >> 3) & 7)) * 8 + (kk & 7)) * 8 + ((kk >> 3) & 7);
       const int np = (kb >> 8) * (kk >> 8);
-      key = (((phase * 4096 + cls) * 2 + dense) * 2 + fp32) * 16 + prim_bin(np);
+      // a sub-quartet of a later phase has its own class (an entry built on the second
+      // centre swaps its shells): it runs FP32 only if that class may too, so it never
+      // lands in an FP32 bucket the host gave no room
+      const int fp32s = (fp32 && (a.fp32_ok == nullptr || a.fp32_ok[cls * 2 + dense])) ? 1 : 0;
+      key = (((phase * 4096 + cls) * 2 + dense) * 2 + fp32s) * 16 + prim_bin(np);
       primq = static_cast<uint64_t>(np);
     }
     // wavefront-aggregated reservation per distinct key

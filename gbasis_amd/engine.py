@@ -29,7 +29,8 @@ from ._lib import (GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_PACKED_ADD
 from .boys import BOYS_H, boys_table
 from .programs import canonical_class, encoded_program
 from .roofline import class_flops, quartet_flops
-from .tables import NUM_BUCKETS, BatchTables, MoleculeTables, decode_head, f2norm_table
+from .tables import (NUM_BUCKETS, STORE_PHASES, BatchTables, MoleculeTables, decode_head,
+                     f2norm_table)
 
 DEFAULT_SCRATCH_BYTES = 2 << 30
 
@@ -58,7 +59,12 @@ class Bucket:
 
     def mode(self):
         """Packed output mode of this bucket's launch."""
-        return GBX_MODE_PACKED if self.phase == 0 else GBX_MODE_PACKED_ADD
+        return GBX_MODE_PACKED if self.stores() else GBX_MODE_PACKED_ADD
+
+    def stores(self):
+        """True for the first sub-quartet of each of its shell quartets (phase 0): the
+        launch that stores, and the one that counts the shell quartet."""
+        return self.phase in STORE_PHASES
 
     def launch_precision(self):
         return self.precision if self.run_precision < 0 else self.run_precision
@@ -251,6 +257,10 @@ class EriEngine:
         # exponent ranges are not (errors near 1e-7 seen), and the autotuner only ever
         # narrows this set.
         self.fp32_ok = None
+        # every mask the engine has used, by version: a work queue sized for one mask
+        # (its cached layout carries the version) and a HIP graph captured with it keep
+        # reading that mask's device copy, so none of them is ever freed
+        self._masks = []
         self.set_fp32_allowed({})
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
         # deterministic: variant choices depend only on (class, primitive bin, precision),
@@ -540,9 +550,9 @@ class EriEngine:
             for phase, xb, xk in b.sub_quartets(bra[mask], ket[mask]):
                 for cls, nprimq, dense, quart in b.class_buckets(xb, xk):
                     qb, qk = quart[:, 0].astype(np.int64), quart[:, 1].astype(np.int64)
-                    eris = int(b.unique_counts(qb, qk).sum()) if phase == 0 else 0
+                    eris = int(b.unique_counts(qb, qk).sum()) if phase in STORE_PHASES else 0
                     flops = float(quartet_flops(cls, nprim[qb] * nprim[qk], pure=dense,
-                                                contracted=phase == 0).sum())
+                                                contracted=phase in STORE_PHASES).sum())
                     plans.append(Bucket(cls, nprimq, prec, dense,
                                         torch.from_numpy(quart).to(self.device), quart.shape[0],
                                         eris, flops, phase=phase))
@@ -603,8 +613,9 @@ class EriEngine:
             layout = ("stats", cap.tobytes())
         else:
             # room for every unscreened candidate of each bucket (both precision
-            # variants when mixed)
-            layout = ("cap", mixed)
+            # variants when mixed); the FP32 variants follow the current mask, so a
+            # mixed layout is only valid for the mask version it was sized with
+            layout = ("cap", mixed, len(self._masks) if mixed else 0)
             cap = b.bucket_capacity().copy()
             if mixed:
                 # the FP32 variant of a bucket only where the class may run FP32
@@ -656,8 +667,9 @@ class EriEngine:
             fused = ((not dense) and self.path != "coop" and dev.potential == "coulomb"
                      and self.has_reg(cls))
             fpr, fco = class_flops(*cls, pure=dense) if stats else (0, 0)
-            if phase > 0:
-                fco = 0   # a shell quartet's contracted part is counted in phase 0
+            if phase not in STORE_PHASES:
+                fco = 0   # a shell quartet's contracted part is counted with its first
+                          # sub-quartet
             if fused:
                 base = head << 4
                 n = int(ch[ks].sum())
@@ -677,6 +689,19 @@ class EriEngine:
                                     count=cursor[key:key + 1] if device_counts else None,
                                     key=("key", key), phase=phase))
         return plans
+
+    def check_queue(self, dev: DeviceTables):
+        """Raise if the last fill of any of ``dev``'s work queues reserved more entries for
+        a bucket than its capacity (the fill drops those quartets, so the step's result
+        would miss them).  Reads the device cursors: call it outside timed or captured
+        regions."""
+        for layout, qc in dev.queue.items():
+            over = np.flatnonzero(qc["cursor"].cpu().numpy().astype(np.int64) > qc["cap"])
+            if over.size:
+                k = int(over[0])
+                raise RuntimeError(
+                    f"work-queue bucket capacity exceeded in {over.size} bucket(s), e.g. "
+                    f"key {k} ({decode_head(k >> 4)}, bin {k & 15}) of queue {layout[0]}")
 
     def capture(self, dev: DeviceTables, bound=None, threshold=0.0, fp32_cutoff=0.0,
                 out=None, tuner=None):
@@ -739,7 +764,10 @@ class EriEngine:
         one sub-quartet, so with ``n_streams`` > 1 (default: ``self.launch_streams``)
         a phase's buckets are spread round-robin over that many side streams forked
         from and joined back into the current stream: small launches of different
-        classes overlap on the device instead of running one after another."""
+        classes overlap on the device instead of running one after another.  (Giving
+        the quartets with a split pair a phase of their own, so that their later
+        phases could overlap the rest, doubles the launches of the heaviest classes
+        and measured slower.)"""
         if out is None:
             out = torch.zeros(dev.batch.n_slots, dtype=torch.float64, device=self.device)
         ns = self.launch_streams if n_streams is None else n_streams
@@ -777,8 +805,12 @@ class EriEngine:
             ci = ((cls[0] * 8 + cls[1]) * 8 + cls[2]) * 8 + cls[3]
             if not ok:
                 mask[ci * 2 + int(dense)] = 0
-        self._fp32_ok_host = mask.astype(np.int64)
+        host = mask.astype(np.int64)
+        if self._masks and np.array_equal(host, self._fp32_ok_host):
+            return
+        self._fp32_ok_host = host
         self.fp32_ok = torch.from_numpy(mask).to(self.device)
+        self._masks.append(self.fp32_ok)
 
     def unpack_dense(self, packed, nfunc, notation):
         dense = torch.empty((nfunc,) * 4, dtype=torch.float64, device=self.device)

@@ -27,6 +27,7 @@ Packed output: every molecule owns a contiguous slice of canonical slots
 ij >= kl over its output functions.
 """
 
+import os
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -112,13 +113,14 @@ def expand_shells(basis, coord_types):
 
 # Split threshold of second_centre_group: a primitive pair whose horizontal recursion
 # from the first centre would multiply rounding by more than this is built on the
-# second centre.  With 512, plain FP64 (the kernels' operation order) stays within
-# 1.0e-14 max(1, |value|) of binary128 on the 293 sampled quartets of the dense
-# s,s,p,d, the s,p,d,f and the s-g workloads and on 300 random quartets of two batch
-# molecules, where the first-centre build alone is off by up to 5.2e-4 (23 s-g
-# quartets above 1e-12); 4096 still keeps 7.2e-14.  It splits 2 % of the batch's shell
-# pairs, 30 % of the s,p,d,f and 50 % of the s-g molecule's.
-SPLIT_AMPLIFICATION = 512.0
+# second centre.  With 4096, plain FP64 (the kernels' operation order) stays within
+# 7.2e-14 max(1, |value|) of binary128 on the 293 sampled quartets of the dense
+# s,s,p,d, the s,p,d,f and the s-g workloads, where the first-centre build alone is off
+# by up to 5.2e-4 (23 s-g quartets above 1e-12), and within 1.1e-13 on 32275 random
+# quartets of 16 batch molecules (s, p, d shells, exponents up to 1e3), where it is off by
+# up to 8.5e-12 (512: 1.0e-13; 32768: 9.1e-13).  It splits 0.9 % of the batch's shell
+# pairs, 21 % of the s,p,d,f and 32 % of the s-g molecule's.
+SPLIT_AMPLIFICATION = float(os.environ.get("GBX_SPLIT_AMPLIFICATION", "4096"))
 
 
 def second_centre_group(l_second, e_first, e_second, ab):
@@ -312,6 +314,7 @@ def _n_slots(nfunc):
 
 NUM_PHASES = 4
 NUM_BUCKETS = NUM_PHASES * 4096 * 2 * 2 * 16
+STORE_PHASES = (0,)   # phases whose launches store their packed slots; the later ones add
 
 
 def queue_key(phase, cls, dense, fp32, nprimq):

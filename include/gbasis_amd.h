@@ -193,7 +193,8 @@ int gbx_eri_reg(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t
  * kept iff bound_p * bound_q >= threshold (threshold <= 0 keeps all), run with
  * the FP32 recursion bulk iff pbound_p * pbound_q < fp32_cutoff (pbound: the
  * second value GBX_MODE_SCHWARZ writes per pair) and d_fp32_ok (NULL, or one
- * int8 per class*2+dense, class = ((la*8+lb)*8+lc)*8+ld) allows it;
+ * int8 per class*2+dense, class = ((la*8+lb)*8+lc)*8+ld) allows it for the
+ * quartet's class and for the class of the sub-quartet (below) itself;
  * d_mol_fp32_cutoff (NULL, or one cutoff per molecule) replaces fp32_cutoff
  * per molecule, so FP64 and mixed-precision requests can share one batch.  A kept
  * quartet (p|q) becomes the sub-quartets (A_p|A_q) (phase 0), (B_p|A_q) (1),

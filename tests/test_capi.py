@@ -33,6 +33,15 @@ def test_header_symbols_exported_and_bound():
         assert n in _lib.SIGNATURES, f"{n} not bound in gbasis_amd/_lib.py"
 
 
+def test_bucket_count_matches_header():
+    """The host's work-queue key space is the one the device queue builder uses."""
+    from gbasis_amd.tables import NUM_BUCKETS
+
+    m = re.search(r"#define GBX_NUM_BUCKETS (\d+)", open(os.path.join(ROOT, "include",
+                                                                    "gbasis_amd.h")).read())
+    assert m and int(m.group(1)) == NUM_BUCKETS
+
+
 def test_error_path_without_device():
     lib = _lib.load()
     assert lib.gbx_version() >= 1

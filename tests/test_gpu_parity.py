@@ -316,6 +316,41 @@ def test_mixed_precision_error(gpu):
     assert err <= 1e-8
 
 
+def test_mixed_after_fp32_mask_changes(gpu):
+    """Mixed precision stays within 1e-8 Eh of FP64 when the FP32 class mask changes
+    between queue builds on the same tables (autotuners narrow it, a new one starts
+    again from the default): each queue is sized for the mask it is filled with, so no
+    bucket overflows and no quartet is dropped."""
+    import torch
+
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+    from gbasis_amd.workloads import dense_small
+
+    _print_case("mixed after FP32 mask changes")
+    eng = get_engine()
+    mol = MoleculeTables(dense_small())
+    dev = DeviceTables(BatchTables([mol]))
+    bound = eng.pair_bounds(dev, eng.schwarz(dev))
+    ref = eng.run(dev, eng.plan_device(dev, bound, 0.0, 0.0, stats=True))
+    scale = max(1.0, float(ref.abs().max()))
+    stats = eng.plan_device(dev, bound, 0.0, 1e-2, stats=True)
+    fp32_classes = {(bk.cls, bool(bk.dense)) for bk in stats if bk.precision == 1}
+    assert fp32_classes
+    try:
+        for allowed in ({k: False for k in fp32_classes}, {}):
+            eng.set_fp32_allowed(allowed)
+            out = torch.zeros_like(ref)
+            eng.run(dev, eng.plan_device(dev, bound, 0.0, 1e-2, stats=False,
+                                         device_counts=True), out)
+            eng.check_queue(dev)
+            err = float((out - ref).abs().max())
+            print(f"mask {len(allowed)} classes off: mixed max err {err:.3e}", flush=True)
+            assert err <= 1e-8 * scale
+    finally:
+        eng.set_fp32_allowed({})
+
+
 def test_register_vs_cooperative(gpu):
     """Both kernel families give the same packed array (FP64)."""
     from gbasis_amd.engine import DeviceTables, get_engine
@@ -509,6 +544,7 @@ def _eval_sampled_quartets(basis, quartets, precision, fp32_cutoff=1e-2):
     import torch
 
     from gbasis_amd._lib import GBX_FP32_BULK, GBX_FP64, GBX_MODE_PACKED, GBX_MODE_PACKED_ADD
+    from gbasis_amd.tables import STORE_PHASES
     from gbasis_amd.engine import DeviceTables, get_engine
     from gbasis_amd.tables import BatchTables, MoleculeTables
 
@@ -544,7 +580,7 @@ def _eval_sampled_quartets(basis, quartets, precision, fp32_cutoff=1e-2):
         if not sel.any():
             continue
         for phase, xb, xk in batch.sub_quartets(bra[sel], ket[sel]):
-            mode = GBX_MODE_PACKED if phase == 0 else GBX_MODE_PACKED_ADD
+            mode = GBX_MODE_PACKED if phase in STORE_PHASES else GBX_MODE_PACKED_ADD
             for cls, nprimq, dense, quart in batch.class_buckets(xb, xk):
                 eng.launch_class(dev, cls, torch.from_numpy(quart).cuda(), nprimq, int(pr),
                                  mode, out, dense)

@@ -112,11 +112,11 @@ def _wide_basis(seed, ls, k=3):
 def test_second_centre_group_rule():
     """Pairs move only with l_second > 0, the larger exponent on the second centre and a
     large amplification r^l_second; coincident centres never move.  For exponents 0.1
-    and 50 at 2 bohr, r = 40: enough for l = 2 (1600), not for l = 1."""
+    and 50 at 2 bohr, r = 40: enough for l = 3 (64000 > 4096), not for l = 2 (1600)."""
     ab = np.array([2.0, 2.0, 2.0, 0.0, 2.0, 2.0, 2.0])
     e1 = np.array([0.1, 50.0, 0.1, 0.1, 0.1, 1.0, 0.1])
     e2 = np.array([50.0, 0.1, 50.0, 50.0, 0.12, 1.0, 50.0])
-    l2 = np.array([2, 2, 0, 2, 4, 4, 1])
+    l2 = np.array([3, 3, 0, 3, 4, 4, 2])
     assert second_centre_group(l2, e1, e2, ab).tolist() == [True, False, False, False, False,
                                                             False, False]
 
