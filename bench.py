@@ -134,8 +134,8 @@ def _host_busy(srv, span):
     return {k: v / span for k, v in srv.host_seconds.items()}
 
 
-def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_batch=16,
-                  batch_flops=4e11):
+def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_batch=32,
+                  batch_flops=8e12):
     """Serving workload: requests of 2-30 atoms (seeds 4000 + i, s/p/d, K <= 6),
     precision FP64 or mixed with probability 1/2 each.  First the saturated rate
     (n_sat requests all arriving at once), then Poisson arrivals at 50 % and 80 %
@@ -447,6 +447,7 @@ CONFIG_WORKLOADS = {
                     "[0.02,1e5], Schwarz 1e-12, packed output", 1e-12, False),
 }
 SAMPLES_FILE = os.path.join(ROOT, "tests", "golden", "config_samples.npz")
+TUNE_FILE = os.path.join(ROOT, "gbasis_amd", "tuning_mi355x.json")
 
 
 def _packed_index(i, j, k, l):
@@ -585,12 +586,26 @@ def workload_summary(r):
     }
 
 
-def run_workload(name, args):
+def make_tuner(eng, args):
+    """The variant autotuner with the choices of ``--tune-cache`` loaded (by default the
+    MI355X choices shipped in gbasis_amd/tuning_mi355x.json; buckets it does not list
+    are timed when first seen), or None with --no-tune."""
+    from gbasis_amd.autotune import Autotuner
+
+    if args.no_tune:
+        return None
+    tuner = Autotuner(eng)
+    if args.tune_cache and tuner.load(args.tune_cache):
+        log(f"autotuner choices loaded from {args.tune_cache} ({len(tuner.choice)} buckets)")
+    return tuner
+
+
+def run_workload(name, args, tuner=None):
     """One single-molecule workload: FP64 and mixed steps, per-class errors, roofline,
-    CPU baseline.  Returns its JSON record."""
+    CPU baseline.  Returns its JSON record.  ``tuner``: shared autotuner (default: a new
+    one from make_tuner)."""
     import torch
     from gbasis_amd import workloads
-    from gbasis_amd.autotune import Autotuner
     from gbasis_amd.engine import DeviceTables, get_engine
     from gbasis_amd.tables import BatchTables, MoleculeTables
 
@@ -611,7 +626,8 @@ def run_workload(name, args):
     nsub = sum(bk.n for bk in plans64)
     log(f"{name}: N={mol.nfunc}, {nquart} shell quartets ({nsub} sub-quartets), {eris} ERIs, "
         f"{mol.n_slots} slots")
-    tuner = None if args.no_tune else Autotuner(eng)
+    if tuner is None:
+        tuner = make_tuner(eng, args)
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     t64, q64, _, pb64, last64 = time_steps(eng, dev, bound, thr, 0.0, args.steps, args.warmup,
                                            tuner)
@@ -692,17 +708,21 @@ def main():
     ap.add_argument("--path", choices=["auto", "coop"], default="auto")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="skip the variant autotuner")
-    ap.add_argument("--tune-cache", default="", help="load (if present) / save autotuner choices")
+    ap.add_argument("--tune-cache", default=TUNE_FILE,
+                    help="autotuner choices to load (default: the shipped MI355X choices); "
+                         "'' to tune every bucket afresh")
+    ap.add_argument("--tune-save", default="",
+                    help="write the autotuner's choices (loaded and new) here at the end")
     ap.add_argument("--copy-calibration-mb", type=int, default=0,
                     help="run one plain device copy of this size first (HBM counter calibration)")
     ap.add_argument("--only", choices=["both", "fp64"], default="both",
                     help="fp64: time the FP64 form only (for single-precision kernel traces)")
     ap.add_argument("--no-graph", action="store_true", help="skip the HIP-graph replay timing")
-    ap.add_argument("--serving-requests", type=int, default=2000)
+    ap.add_argument("--serving-requests", type=int, default=600)
     ap.add_argument("--serving-saturation", type=int, default=300)
     ap.add_argument("--serving-streams", type=int, default=4)
-    ap.add_argument("--serving-max-batch", type=int, default=16)
-    ap.add_argument("--serving-batch-flops", type=float, default=4e11,
+    ap.add_argument("--serving-max-batch", type=int, default=32)
+    ap.add_argument("--serving-batch-flops", type=float, default=8e12,
                     help="cap on a serving batch's summed algorithmic FLOPs (0: size batches "
                          "by count and the 2x packed-slot rule only)")
     ap.add_argument("--no-serving", action="store_true")
@@ -758,11 +778,7 @@ def main():
     log(f"plans: {nquart} quartets, {eris} ERIs, {len(plans64)} buckets; "
         f"{n_fp32} quartets FP32-bulk in mixed ({time.perf_counter() - t0:.1f} s)")
 
-    from gbasis_amd.autotune import Autotuner
-
-    tuner = None if args.no_tune else Autotuner(eng)
-    if tuner is not None and args.tune_cache and tuner.load(args.tune_cache):
-        log(f"autotuner choices loaded from {args.tune_cache}")
+    tuner = make_tuner(eng, args)
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     t64, q64, w64, pb64, last64 = time_steps(eng, dev, bound, args.threshold, 0.0, args.steps,
                                              args.warmup, tuner)
@@ -774,8 +790,8 @@ def main():
     out64 = dev.out
     attach_stats(last64, plans64)
     if args.only == "fp64":
-        if tuner is not None and args.tune_cache:
-            tuner.save(args.tune_cache)
+        if tuner is not None and args.tune_save:
+            tuner.save(args.tune_save)
         roof = roofline_entry(last64, pb64)
         print(json.dumps({"metric": METRIC, "value": eris / t64, "unit": "ERIs/s", "n_gpus": 1,
                           "steps": args.steps, "warmup": args.warmup, "ms_per_step": t64 * 1e3,
@@ -795,8 +811,6 @@ def main():
         gmx, _ = time_graph(eng, dev, bound, args.threshold, args.fp32_cutoff, args.steps, tuner)
         log(f"mixed graph replay: {gmx * 1e3:.1f} ms/step, {eris / gmx:.3e} ERIs/s")
     outmx = dev.out
-    if tuner is not None and args.tune_cache:
-        tuner.save(args.tune_cache)
     # bookkeeping of the buckets actually launched (after the autotuner's FP32 mask)
     plansmx = eng.plan_device(dev, bound, args.threshold, args.fp32_cutoff, stats=True)
     attach_stats(lastmx, plansmx)
@@ -853,9 +867,11 @@ def main():
         result["workloads"] = {}
         for name in CONFIG_WORKLOADS:
             t0 = time.perf_counter()
-            result["workloads"][name] = workload_summary(run_workload(name, sub))
+            result["workloads"][name] = workload_summary(run_workload(name, sub, tuner))
             torch.cuda.empty_cache()
             log(f"workload {name}: {time.perf_counter() - t0:.0f} s")
+    if tuner is not None and args.tune_save:
+        tuner.save(args.tune_save)
     if not args.no_serving:
         result["serving"] = serving_bench(args.serving_requests, args.serving_saturation,
                                           args.threshold, args.fp32_cutoff,

@@ -163,8 +163,8 @@ class EriServer:
     """Admission, continuous batching and completion of ERI requests."""
 
     def __init__(self, device="cuda", threshold=1e-12, fp32_cutoff=1e-2, hbm_budget=None,
-                 n_streams=4, max_batch=16, clock=time.perf_counter, policy="sjf",
-                 age_tau=0.1, max_wait=0.5, batch_flops=4e11):
+                 n_streams=4, max_batch=32, clock=time.perf_counter, policy="sjf",
+                 age_tau=0.1, max_wait=0.5, batch_flops=8e12):
         if policy not in ("sjf", "fifo"):
             raise ValueError("policy must be 'sjf' or 'fifo'")
         self.policy = policy
