@@ -27,11 +27,13 @@ def _f2(c):
     return 1 / np.sqrt(df(2 * c[0] - 1) * df(2 * c[1] - 1) * df(2 * c[2] - 1))
 
 
-def replay_entry_quartet(b, eb, ek):
+def replay_entry_quartet(b, eb, ek, prim_acc=None):
     """Cartesian block (entry orientation, canonical component order, factorial2 norms,
     no norm_cont) of the sub-quartet (eb|ek) of BatchTables b, as the kernels compute
     it: primitive quartets of the two entries' records, the class program of the
-    entries' angular momenta, horizontal recursion with the entries' A-B and C-D."""
+    entries' angular momenta, horizontal recursion with the entries' A-B and C-D.
+    ``prim_acc(prog, rb, rk)``, if given, replaces the Obara-Saika primitive part: it
+    returns the contracted [e0|f0] in the program's accumulator order."""
     la, lb = int(b.pair_la[eb]), int(b.pair_lb[eb])
     lc, ld = int(b.pair_la[ek]), int(b.pair_lb[ek])
     prog = class_program(la, lb, lc, ld)
@@ -40,7 +42,9 @@ def replay_entry_quartet(b, eb, ek):
     rb = np.repeat(b.prim_pairs[ib:ib + nb], nk, axis=0)
     rk = np.tile(b.prim_pairs[ik:ik + nk], (nb, 1))
     acc = np.zeros(prog.nacc)
-    if rb.shape[0]:
+    if prim_acc is not None:
+        acc = prim_acc(prog, rb, rk)
+    elif rb.shape[0]:
         zeta, eta = rb[:, 0], rk[:, 0]
         inv = 1 / (zeta + eta)
         PQ = rb[:, 2:5] - rk[:, 2:5]
@@ -83,14 +87,14 @@ def replay_entry_quartet(b, eb, ek):
     return vals.reshape([len(comps(x)) for x in (la, lb, lc, ld)])
 
 
-def replay_quartet(b, bra, ket):
+def replay_quartet(b, bra, ket, prim_acc=None):
     """Sum of the sub-quartets of shell quartet (bra|ket), in the orientation of the
     pairs (a, b | c, d)."""
     sa, sb = b.pair_info[bra, :2]
     total = None
     for phase, xb, xk in b.sub_quartets([bra], [ket]):
         eb, ek = int(xb[0]), int(xk[0])
-        blk = replay_entry_quartet(b, eb, ek)
+        blk = replay_entry_quartet(b, eb, ek, prim_acc)
         if b.pair_info[eb, 0] != sa:        # bra entry (b, a)
             blk = blk.transpose(1, 0, 2, 3)
         if b.pair_info[ek, 0] != b.pair_info[ket, 0]:
