@@ -9,7 +9,10 @@ quartets and keeps the fastest:
   error bound, so the faster one is kept.  (On CDNA4 non-packed FP32 FMA runs
   at the FP64 vector rate, so FP32 wins only where it relieves registers or LDS.)
 * split classes: lanes per shell quartet R in {1, 2, 4, ..., 64} with either
-  horizontal-recursion pass, or the cooperative kernel instead;
+  horizontal-recursion pass, or the cooperative kernel instead, or (8 <= L <= 17) the
+  Rys primitive pass;
+* classes of the cooperative kernel with 8 <= L <= 17: its own primitive pass or the
+  Rys primitive pass (both finished by the cooperative horizontal recursion);
 * cooperative classes: primitive quartets in flight PF.
 
 Choices are cached in memory per (class, bin, dense, precision class) and can be
@@ -72,19 +75,23 @@ class Autotuner:
             best = None
             for prec in precisions:
                 variants = [None]
+                rys = (eng.rys_ok(bk.cls) and dev.potential == "coulomb" and eng.path != "coop"
+                       and prec == GBX_FP64)
                 if not bk.dense and eng.has_reg(bk.cls):
                     variants = [None]
                 elif eng.split_count(bk.cls) > 0 and eng.path != "coop":
                     hrrs = (0, 1) if (eng.has_hrr(bk.cls) and not bk.dense) else (0,)
                     variants = [("R", r, h) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq
-                                for h in hrrs] + [("coop",)]
+                                for h in hrrs] + [("coop",)] + ([("rys",)] if rys else [])
+                elif rys:
+                    variants = [("coop",), ("rys",)]
                 for var in variants:
                     def run(prec=prec, var=var):
-                        coop = bool(var) and var[0] == "coop"
+                        named = var[0] if var and var[0] in ("coop", "rys") else None
                         eng.launch_bucket(dev, bk, out, precision=prec,
-                                          lanes=var[1] if var and not coop else None,
+                                          lanes=var[1] if var and not named else None,
                                           hrr=bool(var[2]) if var and len(var) > 2 else None,
-                                          path="coop" if coop else None)
+                                          path=named)
                     t = self._time(run)
                     if prec == GBX_FP32_BULK:
                         # keep FP32 only for a clear win: an FP64 bucket also merges with

@@ -251,6 +251,8 @@ class EriEngine:
         self._hrr = {}
         self._split = {}
         self._pf = {}
+        self._rys_dev = None
+        self._rys_idx = {}
         # int8 device mask per class*2+dense: classes allowed the FP32 recursion bulk.
         # By default only classes with every l <= 2: their FP32 error against FP64 is
         # measured within the 1e-8 bound at the default cutoff; f and g shells with wide
@@ -398,7 +400,8 @@ class EriEngine:
         add in the later phases)."""
         prec = bk.launch_precision() if precision is None else precision
         if path is None:
-            path = "coop" if bk.variant and bk.variant[0] == "coop" else self.path
+            path = (bk.variant[0] if bk.variant and bk.variant[0] in ("coop", "rys")
+                    else self.path)
         mode = bk.mode() if mode is None else mode
         return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
                                  path, lanes if lanes is not None else bk.lanes(), bk.seg,
@@ -426,7 +429,7 @@ class EriEngine:
             # kernels read its segment table
             raise ValueError(f"fused launch of class {cls} planned for a register kernel "
                              "cannot run on the cooperative kernel; plan again")
-        if path != "coop" and not dense and self.has_reg(cls):
+        if path not in ("coop", "rys") and not dense and self.has_reg(cls):
             if seg is not None:
                 check(self.lib.gbx_eri_reg_segments(ctypes.byref(tab), *cls, ptr(quartets),
                                                     ptr(seg[0]), ptr(seg[1]), seg[2], precision,
@@ -440,6 +443,12 @@ class EriEngine:
         if quartets.shape[0] == 0:
             # a bucket of capacity 0 (a class and bin with no quartet in this precision)
             return "empty"
+        if path == "auto" and self.rys_default(cls) and dev.potential == "coulomb":
+            path = "rys"
+        if path == "rys":
+            if not (self.rys_ok(cls) and dev.potential == "coulomb"):
+                raise ValueError(f"class {cls} has no Rys primitive pass")
+            return self._launch_rys(dev, tab, cls, quartets, nprimq, mode, out, count)
         st, _, _ = self.programs.get(cls)
         nsplit = self.split_count(cls)
         if path != "coop" and nsplit > 0:
@@ -500,6 +509,64 @@ class EriEngine:
                                    ptr(out), ptr(sc), sc.numel(), self._stream()),
               f"gbx_eri_hrr{cls}")
         return "coop2"
+
+    # Rys primitive pass (gbx_eri_rys_acc): admissible for 8 <= L <= 17, taken by default
+    # from RYS_MIN_L on (the autotuner times it against the cooperative kernel)
+    RYS_MIN_L = 8
+
+    def rys_ok(self, cls):
+        return 8 <= sum(cls) <= 17
+
+    def rys_default(self, cls):
+        return self.rys_ok(cls) and sum(cls) >= self.RYS_MIN_L and self.split_count(cls) == 0 \
+            and not self.has_reg(cls)
+
+    def _rys_table(self):
+        if self._rys_dev is None:
+            from .rys import rys_table
+
+            tab = rys_table()
+            if tab.size != int(self.lib.gbx_rys_table_size()):
+                raise RuntimeError("Rys node table does not match the library's layout")
+            self._rys_dev = torch.from_numpy(tab).to(self.device)
+        return self._rys_dev
+
+    def _rys_index(self, cls):
+        """Per accumulator of the class program (order (ket c, bra a), as the horizontal
+        recursion pass reads them): the offsets of (a_i, c_i) in the kernel's
+        (la + lb + 1) x (lc + ld + 1) one-dimensional tables, packed x | y << 10 | z << 20."""
+        if cls not in self._rys_idx:
+            from .programs import comps
+
+            la, lb, lc, ld = cls
+            W = lc + ld + 1
+            vals = [(a[0] * W + c[0]) | ((a[1] * W + c[1]) << 10) | ((a[2] * W + c[2]) << 20)
+                    for nc in range(lc, lc + ld + 1) for c in comps(nc)
+                    for na in range(la, la + lb + 1) for a in comps(na)]
+            self._rys_idx[cls] = torch.tensor(vals, dtype=torch.int32, device=self.device)
+        return self._rys_idx[cls]
+
+    def _launch_rys(self, dev, tab, cls, quartets, nprimq, mode, out, count):
+        """Rys primitive pass split over R workgroups per quartet, then the cooperative
+        horizontal recursion and epilogue (FP64)."""
+        nq = quartets.shape[0]
+        R = self._coop_parts(nq, nprimq)
+        st, _, _ = self.programs.get(cls)
+        nacc = st.header[1]
+        if self.stream_workspaces:
+            acc = self.workspace("acc", nacc * R * nq, torch.float64)
+        else:
+            acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
+        check(self.lib.gbx_eri_rys_acc(ctypes.byref(tab), ptr(self._rys_table()),
+                                       ptr(self._rys_index(cls)), *cls, ptr(quartets), nq,
+                                       ptr(count), R, ptr(acc), nq, self._stream()),
+              f"gbx_eri_rys_acc{cls}")
+        sc = self.scratch()
+        check(self.lib.gbx_eri_hrr(ctypes.byref(tab), ctypes.byref(st), *cls,
+                                   ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
+                                   ptr(out), ptr(sc), sc.numel(), self._stream()),
+              f"gbx_eri_hrr{cls}")
+        return "rys"
 
     def schwarz(self, dev: DeviceTables):
         """Bounds per global shell pair, device tensor (npair, 2).

@@ -351,6 +351,42 @@ def test_mixed_after_fp32_mask_changes(gpu):
         eng.set_fp32_allowed({})
 
 
+def test_rys_vs_cooperative(gpu):
+    """The Rys primitive pass (every class with 8 <= L <= 17 forced onto it, sub-quartet
+    phases included) gives the cooperative kernel's packed array within 1e-12 of the
+    largest value, pure and Cartesian shells up to g."""
+    import torch
+
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+
+    _print_case("rys vs coop")
+    eng = get_engine()
+    for seed, ls, coord in ((41, [4, 3, 2, 1], "spherical"), (42, [4, 4, 0, 2], "cartesian")):
+        mol = MoleculeTables(random_basis(seed, ls, kmax=3, coord_type=coord, emin=0.05,
+                                          emax=2e3))
+        dev = DeviceTables(BatchTables([mol]))
+        plans = eng.plan_quartets(dev)
+        eng.path = "coop"
+        try:
+            ref = eng.run(dev, plans)
+        finally:
+            eng.path = "auto"
+        forced = 0
+        for bk in plans:
+            if eng.rys_ok(bk.cls):
+                bk.variant = ("rys",)
+                forced += 1
+        assert forced > 0
+        got = eng.run(dev, plans)
+        assert all(bk.path == "rys" for bk in plans if bk.variant == ("rys",))
+        scale = max(1.0, float(ref.abs().max()))
+        err = float((got - ref).abs().max())
+        print(f"{ls} {coord}: {forced} buckets on the Rys pass, max diff {err:.2e}", flush=True)
+        assert err <= 1e-12 * scale
+        torch.cuda.synchronize()
+
+
 def test_register_vs_cooperative(gpu):
     """Both kernel families give the same packed array (FP64)."""
     from gbasis_amd.engine import DeviceTables, get_engine
