@@ -10,9 +10,10 @@ quartets and keeps the fastest:
   at the FP64 vector rate, so FP32 wins only where it relieves registers or LDS.)
 * split classes: lanes per shell quartet R in {1, 2, 4, ..., 64} with either
   horizontal-recursion pass, or the cooperative kernel instead, or (8 <= L <= 17) the
-  Rys primitive pass;
+  Rys primitive pass in either accumulator layout;
 * classes of the cooperative kernel with 8 <= L <= 17: its own primitive pass or the
-  Rys primitive pass (both finished by the cooperative horizontal recursion);
+  Rys primitive pass in either layout (all finished by the cooperative horizontal
+  recursion);
 * cooperative classes: primitive quartets in flight PF.
 
 Choices are cached in memory per (class, bin, dense, precision class) and can be
@@ -82,12 +83,18 @@ class Autotuner:
                 elif eng.split_count(bk.cls) > 0 and eng.path != "coop":
                     hrrs = (0, 1) if (eng.has_hrr(bk.cls) and not bk.dense) else (0,)
                     variants = [("R", r, h) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq
-                                for h in hrrs] + [("coop",)] + ([("rys",)] if rys else [])
+                                for h in hrrs] + [("coop",)] + (
+                                    [("rys", 0), ("rys", 1)] if rys else [])
                 elif rys:
-                    variants = [("coop",), ("rys",)]
+                    variants = [("coop",), ("rys", 0), ("rys", 1)]
                 for var in variants:
                     def run(prec=prec, var=var):
-                        named = var[0] if var and var[0] in ("coop", "rys") else None
+                        if var and var[0] == "rys":
+                            eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, GBX_FP64,
+                                             bk.mode(), out, bk.dense, "rys", seg=bk.seg,
+                                             count=bk.count, rys_layout=var[1])
+                            return
+                        named = var[0] if var and var[0] == "coop" else None
                         eng.launch_bucket(dev, bk, out, precision=prec,
                                           lanes=var[1] if var and not named else None,
                                           hrr=bool(var[2]) if var and len(var) > 2 else None,

@@ -16,9 +16,14 @@
 //   I(e, f+1) = D00 I(e, f) + f B01 I(e, f-1) + e B00 I(e-1, f)
 //   C00 = (P-A)_i + u (W-P)_i, D00 = (Q-C)_i + u (W-Q)_i,
 //   B10 = (1 - u rho/zeta) / (2 zeta), B01 = (1 - u rho/eta) / (2 eta), B00 = u / (2(zeta+eta)).
-// A batch of kRysPB primitive quartets has its n * 3 tables built in LDS by one thread
-// each; then every thread adds, for each of its accumulators, the products over the
-// batch's nodes (the node loop is unrolled, so the table reads use immediate offsets).
+// A batch of primitive quartets (as many as their tables fit in 64 KiB of LDS) has its
+// n * 3 tables built in LDS by one thread each, column by column in registers; two
+// layouts of the accumulators over the lanes follow, the faster one chosen per class:
+// strided (lane tid holds f = tid + 512 k; three table reads and one FMA per node), or
+// grouped (every lane adds the batch's products into the accumulators of its group:
+// the [a0|c0] that share (ax, ay, cx, cy), and with them one product Ix Iy per node, so
+// an accumulator costs one table read and one FMA per node; groups of one shape fill a
+// wavefront (laid out on the host), and blockIdx.y selects a block of 512 slots).
 // Nodes and weights come from the table of gbasis_amd/rys.py: Chebyshev interpolants
 // on unit intervals of T < kRysTmax, the half-range Hermite rule beyond.
 #include <hip/hip_runtime.h>
@@ -32,8 +37,11 @@
 namespace gbx {
 
 constexpr int kRysThreads = 512;
-constexpr int kRysPB = 2;          // primitive quartets per LDS batch
-constexpr int kRysKmax = 32;       // accumulators per thread and launch
+constexpr int kRysPBmax = 8;       // primitive quartets per LDS batch, at most
+constexpr int kRysMaxE = 9;        // la + lb + 1 <= 9 (l <= 4 per shell)
+constexpr int kRysMaxLb = 4;       // second shell of bra and ket: l <= 4 (groups of <= 5 x 5)
+constexpr int kRysKmax = 24;       // strided layout: accumulators per lane and launch
+constexpr int kRysStrided = 0, kRysGrouped = 1;
 constexpr int kRysNmax = 9;
 constexpr int kRysNseg = 100;
 constexpr int kRysDeg = 13;
@@ -83,27 +91,114 @@ struct RysArgs {
   int64_t nquart;
   const int32_t* count;
   const double* rys;      // node table
-  const int32_t* acc_idx; // per accumulator: ox | oy << 10 | oz << 20, offsets of
-                          //   (ax, cx), (ay, cy), (az, cz) in a (lab+1) x (lcd+1) table
-  int32_t lab, lcd, nacc, kacc;
-  int32_t acc_lo;         // this launch's accumulators: [acc_lo, acc_lo + kacc * threads)
+  const int32_t* groups;  // grouped layout: per lane slot a group descriptor (below) or -1,
+                          //   then per slot MA x MC accumulator rows f of its members (-1:
+                          //   none); strided layout: per accumulator f, ox | oy << 10 |
+                          //   oz << 20 (its three table offsets)
+  int32_t nslots;         // grouped: lane slots; strided: accumulators
+  int32_t lab, lcd;
+  int32_t pb;             // primitive quartets per batch (their tables fill the LDS)
+  int32_t kacc, acc_lo;   // strided layout: accumulators per lane, first of this launch
   int32_t parts;
   double* acc_out;
   int64_t acc_stride;
 };
 
-template <int NR, int KMAX>
-__global__ __launch_bounds__(kRysThreads) void eri_rys_acc_kernel(gbx_tables t, RysArgs a) {
+// One thread's part of a batch's node tables: thread tid < PB NR 3 builds the table of
+// primitive quartet p0 + tid / (3 NR), node (tid / 3) % NR, direction tid % 3, column by
+// column in registers (column f: I(e, f), e <= lab), each column stored once in LDS
+// (entries beyond lab are computed and never stored); an absent primitive's tables are
+// zero.
+__device__ inline void rys_build(const gbx_tables& t, const RysArgs& a, int4 pb, int4 pk, int p0,
+                                 int ncopy, int NR, double* rtab) {
+  const int tid = threadIdx.x;
+  const int W = a.lcd + 1;
+  const int TS = (a.lab + 1) * W;
+  const int nk = pk.w;
+  const int p = tid / (3 * NR);
+  const int rem = tid - p * 3 * NR;
+  const int r = rem / 3;
+  const int d = rem - r * 3;
+  double* I = rtab + ((p * NR + r) * 3 + d) * TS;
+  if (p < ncopy) {
+    const int pp = p0 + p;
+    const int ib = pp / nk, ik = pp - ib * nk;
+    const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
+    const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
+    PrimQuartet<double> s;
+    const double T = prim_quartet_setup(rb, rk, s);
+    double u, wt;
+    rys_node(a.rys, NR, r, T, u, wt);
+    const double pa = d == 0 ? s.PA[0] : (d == 1 ? s.PA[1] : s.PA[2]);
+    const double wp = d == 0 ? s.WP[0] : (d == 1 ? s.WP[1] : s.WP[2]);
+    const double qc = d == 0 ? s.QC[0] : (d == 1 ? s.QC[1] : s.QC[2]);
+    const double wq = d == 0 ? s.WQ[0] : (d == 1 ? s.WQ[1] : s.WQ[2]);
+    const double c00 = pa + u * wp, d00 = qc + u * wq;
+    const double b10 = s.i2z * (1.0 - u * s.roz);
+    const double b01 = s.i2e * (1.0 - u * s.roe);
+    const double b00 = u * s.i2s;
+    // the table column by column in registers (column f: I(e, f), e <= lab), each
+    // column stored once; entries beyond lab are computed and never stored
+    double col[kRysMaxE], prv[kRysMaxE];
+    col[0] = d == 2 ? s.pref * wt : 1.0;
+    prv[0] = 0.0;
+#pragma unroll
+    for (int e = 1; e < kRysMaxE; ++e) {
+      col[e] = c00 * col[e - 1] + (e > 1 ? static_cast<double>(e - 1) * b10 * col[e - 2] : 0.0);
+      prv[e] = 0.0;
+    }
+#pragma unroll
+    for (int e = 0; e < kRysMaxE; ++e)
+      if (e <= a.lab) I[e * W] = col[e];
+    for (int f = 0; f < a.lcd; ++f) {
+      const double fb01 = static_cast<double>(f) * b01;
+      double nxt[kRysMaxE];
+#pragma unroll
+      for (int e = 0; e < kRysMaxE; ++e) {
+        double v = d00 * col[e] + fb01 * prv[e];
+        if (e > 0) v += static_cast<double>(e) * b00 * col[e - 1];
+        nxt[e] = v;
+      }
+#pragma unroll
+      for (int e = 0; e < kRysMaxE; ++e) {
+        if (e <= a.lab) I[e * W + f + 1] = nxt[e];
+        prv[e] = col[e];
+        col[e] = nxt[e];
+      }
+    }
+  } else {
+    for (int e = 0; e < TS; ++e) I[e] = 0.0;  // an absent primitive adds nothing
+  }
+}
+
+// Group descriptor of a lane: the accumulators [a0|c0] with a = (ax, ay, az_lo + i),
+// c = (cx, cy, cz_lo + j), i < alen, j < clen share Ix(ax, cx) Iy(ay, cy):
+//   ox = ax (lcd+1) + cx | oy = ay (lcd+1) + cy << 7 | az_lo << 14 | alen << 18 |
+//   cz_lo << 22 | clen << 26.
+// The lanes of a wavefront hold groups of one shape (alen, clen).
+template <int MA, int MC>
+__global__ __launch_bounds__(kRysThreads) void eri_rys_grouped_kernel(gbx_tables t, RysArgs a) {
+  constexpr int KG = 1;
   extern __shared__ __align__(16) double rtab[];
   const int tid = threadIdx.x;
   const int W = a.lcd + 1;
   const int TS = (a.lab + 1) * W;
-  // this thread's accumulators f = tid + k * kRysThreads and their table offsets
-  int idx[KMAX];
+  const int NR = (a.lab + a.lcd) / 2 + 1;
+  const int PB = a.pb;
+  // the group of this lane: slot blockIdx.y 512 + tid
+  int slot[KG], g[KG], alen[KG], clen[KG], ox[KG], oy[KG], oz0[KG];
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const int f = a.acc_lo + tid + k * kRysThreads;
-    idx[k] = (k < a.kacc && f < a.nacc) ? a.acc_idx[f] : -1;
+  for (int k = 0; k < KG; ++k) {
+    slot[k] = (blockIdx.y * KG + k) * kRysThreads + tid;
+    g[k] = slot[k] < a.nslots ? a.groups[slot[k]] : -1;
+    // shape of this wavefront's groups (the same for every lane of a wavefront)
+    const int gw = __builtin_amdgcn_readfirstlane(g[k]);
+    alen[k] = gw < 0 ? 0 : (gw >> 18) & 15;
+    clen[k] = gw < 0 ? 0 : (gw >> 26) & 15;
+    // an empty slot (g < 0) reads the first entries of the tables and writes nothing
+    ox[k] = g[k] < 0 ? 0 : g[k] & 127;
+    oy[k] = g[k] < 0 ? 0 : (g[k] >> 7) & 127;
+    oz0[k] = g[k] < 0 ? 0 : ((g[k] >> 14) & 15) * W + ((g[k] >> 22) & 15);
   }
   const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
   const int64_t nquart = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
@@ -118,53 +213,108 @@ __global__ __launch_bounds__(kRysThreads) void eri_rys_acc_kernel(gbx_tables t, 
     const int chunk = (nprimq + a.parts - 1) / a.parts;
     const int pbeg = min(nprimq, part * chunk);
     const int pend = min(nprimq, pbeg + chunk);
+    double acc[KG][MA][MC];
+#pragma unroll
+    for (int k = 0; k < KG; ++k)
+#pragma unroll
+      for (int i = 0; i < MA; ++i)
+#pragma unroll
+        for (int j = 0; j < MC; ++j) acc[k][i][j] = 0.0;
+
+    for (int p0 = pbeg; p0 < pend; p0 += PB) {
+      const int ncopy = min(PB, pend - p0);
+      __syncthreads();  // the previous batch's tables are no longer read
+      if (tid < PB * NR * 3) rys_build(t, a, pb, pk, p0, ncopy, NR, rtab);
+      __syncthreads();
+      const int nodes = ncopy * NR;
+#pragma unroll
+      for (int k = 0; k < KG; ++k) {
+        if (alen[k] == 0) continue;
+        for (int pr = 0; pr < nodes; ++pr) {
+          const double* base = rtab + pr * 3 * TS;
+          const double xy = base[ox[k]] * base[TS + oy[k]];
+          const double* zb = base + 2 * TS + oz0[k];
+#pragma unroll
+          for (int i = 0; i < MA; ++i) {
+            if (i < alen[k]) {
+#pragma unroll
+              for (int j = 0; j < MC; ++j)
+                if (j < clen[k]) acc[k][i][j] = fma(xy, zb[i * W + j], acc[k][i][j]);
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < KG; ++k) {
+      if (g[k] < 0) continue;
+      const int32_t* rows = a.groups + a.nslots + static_cast<int64_t>(slot[k]) * MA * MC;
+#pragma unroll
+      for (int i = 0; i < MA; ++i)
+#pragma unroll
+        for (int j = 0; j < MC; ++j) {
+          const int f = (i < alen[k] && j < clen[k]) ? rows[i * MC + j] : -1;
+          if (f >= 0)
+            a.acc_out[(static_cast<int64_t>(f) * a.parts + part) * a.acc_stride + qi] =
+                acc[k][i][j];
+        }
+    }
+  }
+}
+
+template <int MA, int MC>
+int launch_rys_mc(const gbx_tables* t, const RysArgs& a, dim3 grid, size_t shmem,
+                  hipStream_t s) {
+  hipLaunchKernelGGL((eri_rys_grouped_kernel<MA, MC>), grid, dim3(kRysThreads), shmem, s, *t, a);
+  return check_launch("eri_rys_grouped_kernel");
+}
+
+template <int MA>
+int launch_rys_ma(int mc, const gbx_tables* t, const RysArgs& a, dim3 grid, size_t shmem,
+                  hipStream_t s) {
+  switch (mc) {
+    case 1: return launch_rys_mc<MA, 1>(t, a, grid, shmem, s);
+    case 2: return launch_rys_mc<MA, 2>(t, a, grid, shmem, s);
+    case 3: return launch_rys_mc<MA, 3>(t, a, grid, shmem, s);
+    case 4: return launch_rys_mc<MA, 4>(t, a, grid, shmem, s);
+    default: return launch_rys_mc<MA, 5>(t, a, grid, shmem, s);
+  }
+}
+
+// Strided layout: lane tid holds accumulators f = acc_lo + tid + k 512, k < KMAX, each
+// read as three table entries per node.
+template <int NR, int KMAX>
+__global__ __launch_bounds__(kRysThreads) void eri_rys_strided_kernel(gbx_tables t, RysArgs a) {
+  extern __shared__ __align__(16) double rtab[];
+  const int tid = threadIdx.x;
+  const int W = a.lcd + 1;
+  const int TS = (a.lab + 1) * W;
+  const int PB = a.pb;
+  int idx[KMAX];
+#pragma unroll
+  for (int k = 0; k < KMAX; ++k) {
+    const int f = a.acc_lo + tid + k * kRysThreads;
+    idx[k] = (k < a.kacc && f < a.nslots) ? a.groups[f] : -1;
+  }
+  const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
+  const int64_t nquart = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
+  for (int64_t wi = blockIdx.x; wi < nquart * a.parts; wi += gridDim.x) {
+    const int64_t qi = wi % nquart;
+    const int part = static_cast<int>(wi / nquart);
+    const int bra = a.quart[2 * qi], ket = a.quart[2 * qi + 1];
+    const int4 pb = pinfo[bra];
+    const int4 pk = pinfo[ket];
+    const int nprimq = pb.w * pk.w;
+    const int chunk = (nprimq + a.parts - 1) / a.parts;
+    const int pbeg = min(nprimq, part * chunk);
+    const int pend = min(nprimq, pbeg + chunk);
     double acc[KMAX];
 #pragma unroll
     for (int k = 0; k < KMAX; ++k) acc[k] = 0.0;
-
-    for (int p0 = pbeg; p0 < pend; p0 += kRysPB) {
-      const int ncopy = min(kRysPB, pend - p0);
+    for (int p0 = pbeg; p0 < pend; p0 += PB) {
+      const int ncopy = min(PB, pend - p0);
       __syncthreads();  // the previous batch's tables are no longer read
-      if (tid < kRysPB * NR * 3) {
-        const int p = tid / (3 * NR);
-        const int rem = tid - p * 3 * NR;
-        const int r = rem / 3;
-        const int d = rem - r * 3;
-        double* I = rtab + ((p * NR + r) * 3 + d) * TS;
-        if (p < ncopy) {
-          const int pp = p0 + p;
-          const int ib = pp / nk, ik = pp - ib * nk;
-          const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
-          const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-          PrimQuartet<double> s;
-          const double T = prim_quartet_setup(rb, rk, s);
-          double u, wt;
-          rys_node(a.rys, NR, r, T, u, wt);
-          const double pa = d == 0 ? s.PA[0] : (d == 1 ? s.PA[1] : s.PA[2]);
-          const double wp = d == 0 ? s.WP[0] : (d == 1 ? s.WP[1] : s.WP[2]);
-          const double qc = d == 0 ? s.QC[0] : (d == 1 ? s.QC[1] : s.QC[2]);
-          const double wq = d == 0 ? s.WQ[0] : (d == 1 ? s.WQ[1] : s.WQ[2]);
-          const double c00 = pa + u * wp, d00 = qc + u * wq;
-          const double b10 = s.i2z * (1.0 - u * s.roz);
-          const double b01 = s.i2e * (1.0 - u * s.roe);
-          const double b00 = u * s.i2s;
-          I[0] = d == 2 ? s.pref * wt : 1.0;
-          if (a.lab > 0) I[W] = c00 * I[0];
-          for (int e = 1; e < a.lab; ++e)
-            I[(e + 1) * W] = c00 * I[e * W] + static_cast<double>(e) * b10 * I[(e - 1) * W];
-          for (int f = 0; f < a.lcd; ++f) {
-            const double fb01 = static_cast<double>(f) * b01;
-            for (int e = 0; e <= a.lab; ++e) {
-              double v = d00 * I[e * W + f];
-              if (f > 0) v += fb01 * I[e * W + f - 1];
-              if (e > 0) v += static_cast<double>(e) * b00 * I[(e - 1) * W + f];
-              I[e * W + f + 1] = v;
-            }
-          }
-        } else {
-          for (int e = 0; e < TS; ++e) I[e] = 0.0;  // an absent primitive adds nothing
-        }
-      }
+      if (tid < PB * NR * 3) rys_build(t, a, pb, pk, p0, ncopy, NR, rtab);
       __syncthreads();
       // the node stride, opaque to the compiler in each batch: otherwise it hoists the
       // (accumulator, node) addresses out of the batch loop and spills them
@@ -173,17 +323,20 @@ __global__ __launch_bounds__(kRysThreads) void eri_rys_acc_kernel(gbx_tables t, 
 #pragma unroll
       for (int k = 0; k < KMAX; ++k) {
         if (idx[k] < 0) continue;
-        // likewise the table offsets: decoded again in each batch, three registers fewer
-        // per accumulator than keeping the decoded addresses
+        // decoded again in each batch (three registers fewer per accumulator than
+        // keeping the addresses across batches)
         __asm__ volatile("" : "+v"(idx[k]));
         const double* X = rtab + (idx[k] & 1023);
         const double* Y = rtab + TS + ((idx[k] >> 10) & 1023);
         const double* Z = rtab + 2 * TS + (idx[k] >> 20);
         double s = acc[k];
+        for (int p = 0; p < ncopy; ++p) {
+          // the nodes of one primitive quartet unrolled: NR independent reads in flight
 #pragma unroll
-        for (int pr = 0; pr < kRysPB * NR; ++pr) {
-          const int o = pr * stride;
-          s = fma(X[o] * Y[o], Z[o], s);
+          for (int r = 0; r < NR; ++r) {
+            const int o = (p * NR + r) * stride;
+            s = fma(X[o] * Y[o], Z[o], s);
+          }
         }
         acc[k] = s;
       }
@@ -198,18 +351,26 @@ __global__ __launch_bounds__(kRysThreads) void eri_rys_acc_kernel(gbx_tables t, 
 }
 
 template <int NR>
-int launch_rys_nr(const gbx_tables* t, const RysArgs& a, int64_t grid, size_t shmem,
-                  hipStream_t s) {
-  const unsigned g = static_cast<unsigned>(grid);
+int launch_rys_strided_nr(const gbx_tables* t, const RysArgs& a, dim3 grid, size_t shmem,
+                          hipStream_t s) {
   if (a.kacc <= 8)
-    hipLaunchKernelGGL((eri_rys_acc_kernel<NR, 8>), dim3(g), dim3(kRysThreads), shmem, s, *t, a);
+    hipLaunchKernelGGL((eri_rys_strided_kernel<NR, 8>), grid, dim3(kRysThreads), shmem, s, *t, a);
   else if (a.kacc <= 16)
-    hipLaunchKernelGGL((eri_rys_acc_kernel<NR, 16>), dim3(g), dim3(kRysThreads), shmem, s, *t, a);
-  else if (a.kacc <= 24)
-    hipLaunchKernelGGL((eri_rys_acc_kernel<NR, 24>), dim3(g), dim3(kRysThreads), shmem, s, *t, a);
+    hipLaunchKernelGGL((eri_rys_strided_kernel<NR, 16>), grid, dim3(kRysThreads), shmem, s, *t, a);
   else
-    hipLaunchKernelGGL((eri_rys_acc_kernel<NR, 32>), dim3(g), dim3(kRysThreads), shmem, s, *t, a);
-  return check_launch("eri_rys_acc_kernel");
+    hipLaunchKernelGGL((eri_rys_strided_kernel<NR, 24>), grid, dim3(kRysThreads), shmem, s, *t, a);
+  return check_launch("eri_rys_strided_kernel");
+}
+
+int launch_rys_strided(int nr, const gbx_tables* t, const RysArgs& a, dim3 grid, size_t shmem,
+                       hipStream_t s) {
+  switch (nr) {
+    case 5: return launch_rys_strided_nr<5>(t, a, grid, shmem, s);
+    case 6: return launch_rys_strided_nr<6>(t, a, grid, shmem, s);
+    case 7: return launch_rys_strided_nr<7>(t, a, grid, shmem, s);
+    case 8: return launch_rys_strided_nr<8>(t, a, grid, shmem, s);
+    default: return launch_rys_strided_nr<9>(t, a, grid, shmem, s);
+  }
 }
 
 }  // namespace gbx
@@ -218,14 +379,15 @@ using namespace gbx;
 
 extern "C" int64_t gbx_rys_table_size(void) { return kRysTableSize; }
 
-extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, const int32_t* d_acc_idx,
-                               int32_t la, int32_t lb, int32_t lc, int32_t ld,
-                               const int32_t* d_quartets, int64_t nquart, const int32_t* d_count,
-                               int32_t parts, double* d_acc, int64_t acc_stride, void* stream) {
-  if (!t || (nquart > 0 && (!d_quartets || !d_acc || !d_rys || !d_acc_idx)))
+extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t layout,
+                               const int32_t* d_index, int32_t n_index, int32_t la, int32_t lb,
+                               int32_t lc, int32_t ld, const int32_t* d_quartets, int64_t nquart,
+                               const int32_t* d_count, int32_t parts, double* d_acc,
+                               int64_t acc_stride, void* stream) {
+  if (!t || (nquart > 0 && (!d_quartets || !d_acc || !d_rys || !d_index)))
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: null argument");
-  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la > kMaxShellL || lb > kMaxShellL ||
-      lc > kMaxShellL || ld > kMaxShellL)
+  if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la + lb >= kRysMaxE || lb > kRysMaxLb ||
+      lc + ld >= kRysMaxE || ld > kRysMaxLb)
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: angular momentum out of range");
   if (t->potential != GBX_POT_COULOMB)
     return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: the Rys rule integrates 1/r only");
@@ -236,48 +398,62 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, const i
   if (parts < 1 || parts > 4096)
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: parts must be in [1, 4096]");
   if (acc_stride < nquart) return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: acc_stride < nquart");
-  const int lab = la + lb, lcd = lc + ld;
-  // accumulators: S(la, lab) x S(lc, lcd) components
+  if ((la + lb + 1) * (lc + ld + 1) > 128)
+    return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: tables too large for the index encoding");
   auto ncomp = [](int lo, int hi) {
     int n = 0;
     for (int l = lo; l <= hi; ++l) n += (l + 1) * (l + 2) / 2;
     return n;
   };
-  const int nacc = ncomp(la, lab) * ncomp(lc, lcd);
-  // at most kRysKmax accumulators per thread (the registers of one lane): classes with
-  // more run as several launches over consecutive accumulator ranges
-  const int kall = (nacc + kRysThreads - 1) / kRysThreads;
-  const int nsplit = (kall + kRysKmax - 1) / kRysKmax;
-  const int kacc = (kall + nsplit - 1) / nsplit;
-  if (nsplit > 4) return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: too many accumulators");
+  const int nacc = ncomp(la, la + lb) * ncomp(lc, lc + ld);
+  if (layout == kRysGrouped) {
+    if (n_index < 1 || n_index % kRysThreads != 0 || n_index > 64 * kRysThreads)
+      return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: grouped slots must be a multiple of 512");
+  } else if (layout == kRysStrided) {
+    if (n_index != nacc)
+      return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: strided index must hold one entry per accumulator");
+  } else {
+    return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: unknown layout");
+  }
   if (nquart == 0) return GBX_OK;
   RysArgs a{};
   a.quart = d_quartets;
   a.nquart = nquart;
   a.count = d_count;
   a.rys = d_rys;
-  a.acc_idx = d_acc_idx;
-  a.lab = lab;
-  a.lcd = lcd;
-  a.nacc = nacc;
-  a.kacc = kacc;
+  a.groups = d_index;
+  a.nslots = n_index;
+  a.lab = la + lb;
+  a.lcd = lc + ld;
   a.parts = parts;
   a.acc_out = d_acc;
   a.acc_stride = acc_stride;
-  const size_t shmem = sizeof(double) * kRysPB * nr * 3 * (lab + 1) * (lcd + 1);
-  if (shmem > 65536) return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: node tables exceed 64 KiB");
-  const int64_t grid = std::min<int64_t>(nquart * parts, int64_t(1) << 20);
+  const int64_t per_prim = int64_t(sizeof(double)) * nr * 3 * (a.lab + 1) * (a.lcd + 1);
+  a.pb = static_cast<int>(std::min<int64_t>({kRysPBmax, 65536 / per_prim,
+                                             kRysThreads / (3 * nr)}));
+  if (a.pb < 1) return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: node tables exceed 64 KiB");
+  const size_t shmem = static_cast<size_t>(per_prim * a.pb);
+  const unsigned gx =
+      static_cast<unsigned>(std::min<int64_t>(nquart * parts, int64_t(1) << 20));
   hipStream_t s = static_cast<hipStream_t>(stream);
-  for (int sp = 0; sp < nsplit; ++sp) {
-    a.acc_lo = sp * kacc * kRysThreads;
-    int rc = GBX_OK;
-    switch (nr) {
-      case 5: rc = launch_rys_nr<5>(t, a, grid, shmem, s); break;
-      case 6: rc = launch_rys_nr<6>(t, a, grid, shmem, s); break;
-      case 7: rc = launch_rys_nr<7>(t, a, grid, shmem, s); break;
-      case 8: rc = launch_rys_nr<8>(t, a, grid, shmem, s); break;
-      default: rc = launch_rys_nr<9>(t, a, grid, shmem, s); break;
+  if (layout == kRysGrouped) {
+    const dim3 grid(gx, static_cast<unsigned>(n_index / kRysThreads));
+    switch (lb + 1) {
+      case 1: return launch_rys_ma<1>(ld + 1, t, a, grid, shmem, s);
+      case 2: return launch_rys_ma<2>(ld + 1, t, a, grid, shmem, s);
+      case 3: return launch_rys_ma<3>(ld + 1, t, a, grid, shmem, s);
+      case 4: return launch_rys_ma<4>(ld + 1, t, a, grid, shmem, s);
+      default: return launch_rys_ma<5>(ld + 1, t, a, grid, shmem, s);
     }
+  }
+  // strided: at most kRysKmax accumulators per lane and launch; classes with more run as
+  // several launches over consecutive accumulator ranges
+  const int kall = (nacc + kRysThreads - 1) / kRysThreads;
+  const int nsplit = (kall + kRysKmax - 1) / kRysKmax;
+  a.kacc = (kall + nsplit - 1) / nsplit;
+  for (int sp = 0; sp < nsplit; ++sp) {
+    a.acc_lo = sp * a.kacc * kRysThreads;
+    const int rc = launch_rys_strided(nr, t, a, dim3(gx), shmem, s);
     if (rc != GBX_OK) return rc;
   }
   return GBX_OK;

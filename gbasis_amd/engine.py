@@ -399,16 +399,20 @@ class EriEngine:
         """Launch one bucket; ``mode`` defaults to the bucket's own (store in phase 0,
         add in the later phases)."""
         prec = bk.launch_precision() if precision is None else precision
+        rys_layout = None
         if path is None:
             path = (bk.variant[0] if bk.variant and bk.variant[0] in ("coop", "rys")
                     else self.path)
+            if bk.variant and bk.variant[0] == "rys" and len(bk.variant) > 1:
+                rys_layout = bk.variant[1]
         mode = bk.mode() if mode is None else mode
         return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
                                  path, lanes if lanes is not None else bk.lanes(), bk.seg,
-                                 bk.count, hrr if hrr is not None else bk.hrr())
+                                 bk.count, hrr if hrr is not None else bk.hrr(), rys_layout)
 
     def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,
-                     dense=False, path="auto", lanes=None, seg=None, count=None, hrr=None):
+                     dense=False, path="auto", lanes=None, seg=None, count=None, hrr=None,
+                     rys_layout=None):
         """One launch for a (class, bin) bucket; quartets is a device int32 (n,2).
 
         Classes with a generated register kernel and no dense output transform run
@@ -448,7 +452,8 @@ class EriEngine:
         if path == "rys":
             if not (self.rys_ok(cls) and dev.potential == "coulomb"):
                 raise ValueError(f"class {cls} has no Rys primitive pass")
-            return self._launch_rys(dev, tab, cls, quartets, nprimq, mode, out, count)
+            return self._launch_rys(dev, tab, cls, quartets, nprimq, mode, out, count,
+                                    self.rys_layout(cls) if rys_layout is None else rys_layout)
         st, _, _ = self.programs.get(cls)
         nsplit = self.split_count(cls)
         if path != "coop" and nsplit > 0:
@@ -515,7 +520,33 @@ class EriEngine:
     RYS_MIN_L = 8
 
     def rys_ok(self, cls):
-        return 8 <= sum(cls) <= 17
+        la, lb, lc, ld = cls
+        return (8 <= la + lb + lc + ld <= 17 and lb <= 4 and ld <= 4 and la + lb <= 8
+                and lc + ld <= 8 and (la + lb + 1) * (lc + ld + 1) <= 128)
+
+    RYS_STRIDED, RYS_GROUPED = 0, 1
+
+    def rys_layout(self, cls):
+        """Default accumulator layout of the Rys pass: groups sharing Ix Iy where they are
+        small ((l_b + 1)(l_d + 1) < 15), one accumulator per lane slot otherwise (measured
+        faster on the g classes; the autotuner times both)."""
+        return self.RYS_GROUPED if (cls[1] + 1) * (cls[3] + 1) < 15 else self.RYS_STRIDED
+
+    def _rys_strided(self, cls):
+        """Per accumulator of the class program (order (ket c, bra a)): the offsets of
+        (a_i, c_i) in the (l_a + l_b + 1) x (l_c + l_d + 1) tables, x | y << 10 | z << 20."""
+        key = ("strided", cls)
+        if key not in self._rys_idx:
+            from .programs import comps
+
+            la, lb, lc, ld = cls
+            W = lc + ld + 1
+            vals = [(a[0] * W + c[0]) | ((a[1] * W + c[1]) << 10) | ((a[2] * W + c[2]) << 20)
+                    for nc in range(lc, lc + ld + 1) for c in comps(nc)
+                    for na in range(la, la + lb + 1) for a in comps(na)]
+            self._rys_idx[key] = (torch.tensor(vals, dtype=torch.int32, device=self.device),
+                                  len(vals))
+        return self._rys_idx[key]
 
     def rys_default(self, cls):
         return self.rys_ok(cls) and sum(cls) >= self.RYS_MIN_L and self.split_count(cls) == 0 \
@@ -531,22 +562,59 @@ class EriEngine:
             self._rys_dev = torch.from_numpy(tab).to(self.device)
         return self._rys_dev
 
-    def _rys_index(self, cls):
-        """Per accumulator of the class program (order (ket c, bra a), as the horizontal
-        recursion pass reads them): the offsets of (a_i, c_i) in the kernel's
-        (la + lb + 1) x (lc + ld + 1) one-dimensional tables, packed x | y << 10 | z << 20."""
+    def _rys_groups(self, cls):
+        """Lane slots of the Rys pass for a class (gbx_eri_rys_acc): the accumulators
+        [a0|c0] grouped by (a_x, a_y, c_x, c_y) -- a group's members differ in a_z and c_z
+        only and share the product Ix Iy -- each group a lane, groups of one shape
+        (numbers of a_z and c_z) filling whole wavefronts.  Returns (device int32
+        [descriptors, member rows], number of slots)."""
         if cls not in self._rys_idx:
             from .programs import comps
 
             la, lb, lc, ld = cls
-            W = lc + ld + 1
-            vals = [(a[0] * W + c[0]) | ((a[1] * W + c[1]) << 10) | ((a[2] * W + c[2]) << 20)
-                    for nc in range(lc, lc + ld + 1) for c in comps(nc)
-                    for na in range(la, la + lb + 1) for a in comps(na)]
-            self._rys_idx[cls] = torch.tensor(vals, dtype=torch.int32, device=self.device)
+            Lab, Lcd = la + lb, lc + ld
+            W, MA, MC = Lcd + 1, lb + 1, ld + 1
+            fidx = {}
+            for nc in range(lc, Lcd + 1):
+                for c in comps(nc):
+                    for na in range(la, Lab + 1):
+                        for a in comps(na):
+                            fidx[(a, c)] = len(fidx)
+            by_shape = {}
+            for ax in range(Lab + 1):
+                for ay in range(Lab + 1 - ax):
+                    az0, az1 = max(0, la - ax - ay), Lab - ax - ay
+                    for cx in range(Lcd + 1):
+                        for cy in range(Lcd + 1 - cx):
+                            cz0, cz1 = max(0, lc - cx - cy), Lcd - cx - cy
+                            if az1 < az0 or cz1 < cz0:
+                                continue
+                            na_, nc_ = az1 - az0 + 1, cz1 - cz0 + 1
+                            desc = ((ax * W + cx) | ((ay * W + cy) << 7) | (az0 << 14)
+                                    | (na_ << 18) | (cz0 << 22) | (nc_ << 26))
+                            rows = [-1] * (MA * MC)
+                            for i in range(na_):
+                                for j in range(nc_):
+                                    rows[i * MC + j] = fidx[((ax, ay, az0 + i), (cx, cy, cz0 + j))]
+                            by_shape.setdefault((na_, nc_), []).append((desc, rows))
+            descs, rows = [], []
+            for shape in sorted(by_shape, reverse=True):
+                grp = by_shape[shape]
+                pad = -len(grp) % 64
+                grp = grp + [(-1, [-1] * (MA * MC))] * pad
+                descs += [d for d, _ in grp]
+                rows += [r for _, r in grp]
+            pad = -len(descs) % 512
+            descs += [-1] * pad
+            rows += [[-1] * (MA * MC)] * pad
+            members = sum(1 for r in rows for f in r if f >= 0)
+            assert members == len(fidx), (cls, members, len(fidx))
+            arr = np.concatenate([np.array(descs, np.int64),
+                                  np.array(rows, np.int64).reshape(-1)]).astype(np.int32)
+            self._rys_idx[cls] = (torch.from_numpy(arr).to(self.device), len(descs))
         return self._rys_idx[cls]
 
-    def _launch_rys(self, dev, tab, cls, quartets, nprimq, mode, out, count):
+    def _launch_rys(self, dev, tab, cls, quartets, nprimq, mode, out, count, layout):
         """Rys primitive pass split over R workgroups per quartet, then the cooperative
         horizontal recursion and epilogue (FP64)."""
         nq = quartets.shape[0]
@@ -557,8 +625,10 @@ class EriEngine:
             acc = self.workspace("acc", nacc * R * nq, torch.float64)
         else:
             acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
-        check(self.lib.gbx_eri_rys_acc(ctypes.byref(tab), ptr(self._rys_table()),
-                                       ptr(self._rys_index(cls)), *cls, ptr(quartets), nq,
+        index, nindex = (self._rys_groups(cls) if layout == self.RYS_GROUPED
+                         else self._rys_strided(cls))
+        check(self.lib.gbx_eri_rys_acc(ctypes.byref(tab), ptr(self._rys_table()), int(layout),
+                                       ptr(index), nindex, *cls, ptr(quartets), nq,
                                        ptr(count), R, ptr(acc), nq, self._stream()),
               f"gbx_eri_rys_acc{cls}")
         sc = self.scratch()

@@ -176,21 +176,28 @@ int gbx_eri_coop_acc(const gbx_tables *t, const gbx_program *prog, int32_t la, i
                      int64_t scratch_bytes, void *stream);
 
 /* Primitive pass by Rys quadrature for classes with 8 <= L <= 17 (n = L/2 + 1 <= 9
- * nodes) and at most 65536 accumulators, Coulomb operator, FP64: work item (q, r)
- * sums the primitive quartets [r c, (r + 1) c) of quartet q as sum over nodes of
- * pref W_u Ix Iy Iz and writes the partial contracted [e0|f0] to
+ * nodes), l_a + l_b <= 8, l_c + l_d <= 8, l_b <= 4, l_d <= 4, Coulomb operator, FP64:
+ * work item (q, r) sums the primitive quartets [r c, (r + 1) c) of quartet q as sum over
+ * nodes of pref W_u Ix Iy Iz and writes the partial contracted [e0|f0] to
  * d_acc[(f * parts + r) * acc_stride + q], the layout of gbx_eri_coop_acc, which
  * gbx_eri_hrr with acc_parts = parts finishes.  Replaces the Boys function, vertical
  * recursion and transfer to the ket of gbasis/integrals/_two_elec_int_improved.py:87-361
  * with the same integrals (the rule integrates F_m(T), m < 2n, exactly).  d_rys is the
- * node table (gbx_rys_table_size doubles, built by gbasis_amd/rys.py); d_acc_idx holds,
- * per accumulator f in the class program's order, ox | oy << 10 | oz << 20 with
- * ox = a_x (lc + ld + 1) + c_x (likewise y, z) for its bra component a and ket
- * component c. */
-int gbx_eri_rys_acc(const gbx_tables *t, const double *d_rys, const int32_t *d_acc_idx,
-                    int32_t la, int32_t lb, int32_t lc, int32_t ld, const int32_t *d_quartets,
-                    int64_t nquart, const int32_t *d_count, int32_t parts, double *d_acc,
-                    int64_t acc_stride, void *stream);
+ * node table (gbx_rys_table_size doubles, built by gbasis_amd/rys.py).  Accumulator f is
+ * the class program's [a0|c0] (a of degree l_a..l_a+l_b, c of l_c..l_c+l_d); with
+ * ox = a_x (l_c + l_d + 1) + c_x and oy, oz likewise:
+ *  layout 0 (strided): d_index holds n_index = (number of accumulators) entries
+ *    ox | oy << 10 | oz << 20, one per f;
+ *  layout 1 (grouped): d_index holds n_index (a multiple of 512, at most 32768) lane
+ *    descriptors, each -1 or a group of the accumulators sharing (a_x, a_y, c_x, c_y):
+ *    ox | oy << 7 | az_lo << 14 | alen << 18 | cz_lo << 22 | clen << 26 (members
+ *    a_z = az_lo + i, i < alen, c_z = cz_lo + j, j < clen), every descriptor of a 64-lane
+ *    wavefront of the same (alen, clen); then, per slot, (l_b + 1)(l_d + 1) rows f
+ *    (member (i, j) at i (l_d + 1) + j), -1 where none. */
+int gbx_eri_rys_acc(const gbx_tables *t, const double *d_rys, int32_t layout,
+                    const int32_t *d_index, int32_t n_index, int32_t la, int32_t lb, int32_t lc,
+                    int32_t ld, const int32_t *d_quartets, int64_t nquart, const int32_t *d_count,
+                    int32_t parts, double *d_acc, int64_t acc_stride, void *stream);
 
 /* Number of doubles of the Rys node table gbx_eri_rys_acc reads. */
 int64_t gbx_rys_table_size(void);

@@ -352,9 +352,9 @@ def test_mixed_after_fp32_mask_changes(gpu):
 
 
 def test_rys_vs_cooperative(gpu):
-    """The Rys primitive pass (every class with 8 <= L <= 17 forced onto it, sub-quartet
-    phases included) gives the cooperative kernel's packed array within 1e-12 of the
-    largest value, pure and Cartesian shells up to g."""
+    """The Rys primitive pass in both accumulator layouts (every class with 8 <= L <= 17
+    forced onto it, sub-quartet phases included) gives the cooperative kernel's packed
+    array within 1e-12 of the largest value, pure and Cartesian shells up to g."""
     import torch
 
     from gbasis_amd.engine import DeviceTables, get_engine
@@ -372,18 +372,21 @@ def test_rys_vs_cooperative(gpu):
             ref = eng.run(dev, plans)
         finally:
             eng.path = "auto"
-        forced = 0
-        for bk in plans:
-            if eng.rys_ok(bk.cls):
-                bk.variant = ("rys",)
-                forced += 1
-        assert forced > 0
-        got = eng.run(dev, plans)
-        assert all(bk.path == "rys" for bk in plans if bk.variant == ("rys",))
         scale = max(1.0, float(ref.abs().max()))
-        err = float((got - ref).abs().max())
-        print(f"{ls} {coord}: {forced} buckets on the Rys pass, max diff {err:.2e}", flush=True)
-        assert err <= 1e-12 * scale
+        for layout in (eng.RYS_STRIDED, eng.RYS_GROUPED):
+            forced = 0
+            for bk in plans:
+                bk.variant = None
+                if eng.rys_ok(bk.cls):
+                    bk.variant = ("rys", layout)
+                    forced += 1
+            assert forced > 0
+            got = eng.run(dev, plans)
+            assert all(bk.path == "rys" for bk in plans if bk.variant)
+            err = float((got - ref).abs().max())
+            print(f"{ls} {coord} layout {layout}: {forced} buckets on the Rys pass, max diff "
+                  f"{err:.2e}", flush=True)
+            assert err <= 1e-12 * scale
         torch.cuda.synchronize()
 
 

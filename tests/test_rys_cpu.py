@@ -116,3 +116,51 @@ def test_rys_primitive_part_vs_binary128(ls):
                            [s.coeffs for s in sh], "quad", 0)
         worst = max(worst, np.abs(got - ref).max() / max(1.0, np.abs(ref).max()))
     assert worst <= 5e-13, worst
+
+
+@pytest.mark.parametrize("cls", [(4, 4, 4, 4), (4, 4, 4, 3), (3, 3, 2, 2), (4, 0, 4, 0),
+                                 (2, 4, 3, 3), (4, 1, 3, 0)])
+def test_rys_lane_groups(cls):
+    """Lane slots of the Rys pass: every accumulator of the class program exactly once, in
+    a group whose members share (a_x, a_y, c_x, c_y); the groups of a wavefront have one
+    shape and its first lane is occupied; whole 512-lane blocks."""
+    from types import SimpleNamespace
+
+    from gbasis_amd.engine import EriEngine
+    from gbasis_amd.programs import class_program
+
+    stub = SimpleNamespace(_rys_idx={}, device="cpu")
+    arr, n = EriEngine._rys_groups(stub, cls)
+    a = arr.numpy()
+    la, lb, lc, ld = cls
+    W, MA, MC = lc + ld + 1, lb + 1, ld + 1
+    desc, rows = a[:n], a[n:].reshape(n, MA * MC)
+    assert n % 512 == 0
+    prog = class_program(*cls)
+    final = [(c, x) for nc in range(lc, lc + ld + 1) for c in comps(nc)
+             for na in range(la, la + lb + 1) for x in comps(na)]
+    seen = []
+    for w in range(0, n, 64):
+        d = desc[w:w + 64]
+        ok = d >= 0
+        if ok.any():
+            assert d[0] >= 0
+            assert len({(int(x) >> 18) & 15 for x in d[ok]}) == 1
+            assert len({(int(x) >> 26) & 15 for x in d[ok]}) == 1
+    for g, r in zip(desc.tolist(), rows.tolist()):
+        if g < 0:
+            assert all(f < 0 for f in r)
+            continue
+        ox, oy = g & 127, (g >> 7) & 127
+        az0, alen, cz0, clen = (g >> 14) & 15, (g >> 18) & 15, (g >> 22) & 15, (g >> 26) & 15
+        for i in range(MA):
+            for j in range(MC):
+                f = r[i * MC + j]
+                if i < alen and j < clen:
+                    c, x = final[f]
+                    assert (x[0] * W + c[0], x[1] * W + c[1]) == (ox, oy)
+                    assert (x[2], c[2]) == (az0 + i, cz0 + j)
+                    seen.append(f)
+                else:
+                    assert f < 0
+    assert sorted(seen) == list(range(prog.nacc))
