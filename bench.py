@@ -154,8 +154,9 @@ def serving_bench(n_requests, n_sat, threshold, cutoff, n_streams, seed=4, max_b
     srv = EriServer(threshold=threshold, fp32_cutoff=cutoff, n_streams=n_streams,
                     max_batch=max_batch, batch_flops=batch_flops)
     clock = srv.clock
-    # warm-up: programs and workspaces
-    for i in range(4):
+    # warm-up: programs and per-stream workspaces at the sizes of full batches (a
+    # workspace that grows inside a timed run costs that batch a device allocation)
+    for i in range(min(n_requests, 2 * max_batch * n_streams)):
         srv.submit(bases[i], precs[i])
     srv.drain(on_done=srv.release)
 
