@@ -271,7 +271,8 @@ def recorded_traffic(kernel):
     (each scaled by the counter's reading for a plain copy of known size), or None."""
     if not os.path.exists(TRAFFIC_FILE):
         return None, None
-    rec = json.load(open(TRAFFIC_FILE)).get("kernels", {}).get(kernel)
+    kernels = json.load(open(TRAFFIC_FILE)).get("kernels", {})
+    rec = kernels.get(kernel, kernels.get("gbx::" + kernel))
     if rec is None:
         return None, None
     return rec["fetch_bytes"] + rec["write_bytes"], os.path.relpath(TRAFFIC_FILE, ROOT)
@@ -343,35 +344,35 @@ def mixed_detail(plans_mixed, t64, tmx):
             f"mixed form is an f64 computation in all but those quartets")
 
 
-def cpu_baselines(batch, plans_fp64):
-    """The oracle's CPU rate with BLAS on every host core the process may use (at most
-    16), then with BLAS on one thread."""
+def cpu_baselines(batch, plans_fp64, budget_s=12.0):
+    """The oracle's CPU rate on a stratified sample of the step's shell quartets: timed
+    by one worker process per host core (at most 16, BLAS on one thread each, side by
+    side), and in this process on one core.  Returns (all cores, one core)."""
+    stats, total_eris = baseline_sample(batch, plans_fp64)
     ncores = max(1, min(16, len(os.sched_getaffinity(0))))
-    with threadpool_limits(limits=ncores):
-        allc = cpu_baseline(batch, plans_fp64, budget_s=12.0, cores=ncores)
     with threadpool_limits(limits=1):
-        one = cpu_baseline(batch, plans_fp64, budget_s=12.0, cores=1)
+        one = baseline_one_core(stats, total_eris, budget_s)
+    try:
+        allc = baseline_all_cores(stats, total_eris, budget_s, ncores)
+    except (OSError, RuntimeError, ValueError) as exc:
+        print(f"[bench] all-core CPU baseline failed ({exc}); reporting the one-core one",
+              file=sys.stderr, flush=True)
+        allc = one
     return allc, one
 
 
-def cpu_baseline(batch, plans_fp64, budget_s=20.0, cores=1):
-    """Oracle (NumPy restatement of the reference engine) timed on a stratified sample,
-    on one core (BLAS limited to one thread).
-
-    Classes are visited from the cheapest (algorithmic FLOPs per quartet) up; each
-    gets up to 256 random quartets within its share of the budget.  Classes left
-    when the budget runs out are extrapolated with the measured oracle seconds per
-    algorithmic FLOP of the sampled ones."""
-    from oracle.assembly import contraction_block
-
+def baseline_sample(batch, plans_fp64, per_class=256):
+    """Up to ``per_class`` random shell quartets of every canonical class of the step
+    (drawn from the storing buckets: one sub-quartet per shell quartet, mapped back to the
+    shell pairs), as four shells [l, coord, exps, coeffs] each; classes sorted by their
+    algorithmic FLOPs per quartet.  Returns ([(class, quartets in the step, samples)],
+    ERIs of the step)."""
     from gbasis_amd.programs import canonical_class
 
     rng = np.random.default_rng(0)
     pair_mol = np.zeros(batch.npair, dtype=np.int64)
     for mi, m in enumerate(batch.mols):
         pair_mol[batch.mol_pair_off[mi]:batch.mol_pair_off[mi] + m.npair] = mi
-    # shell quartets by canonical class: sampled from the storing buckets (one sub-quartet
-    # per shell quartet, mapped back to the shell pairs), FLOPs of every phase
     by_cls, flops_cls = {}, {}
     for bk in plans_fp64:
         cc = canonical_class(*bk.cls)[0]
@@ -381,61 +382,70 @@ def cpu_baseline(batch, plans_fp64, budget_s=20.0, cores=1):
     stats = []
     for cls, bks in by_cls.items():
         n = sum(bk.n for bk in bks)
-        stats.append((flops_cls[cls] / n, cls, bks, n))
-    stats.sort(key=lambda t: t[0])
-    per_cls_budget = budget_s / max(1, len(stats))
-    est = spent = 0.0
-    sampled = skipped = 0
-    s_time = s_flops = 0.0
-    pending_flops = 0.0
-    total_eris = sum(bk.eris for bk in plans_fp64)
-    for fpq, cls, bks, n_cls in stats:
-        if spent > budget_s:
-            pending_flops += fpq * n_cls
-            skipped += 1
-            continue
         quart = batch.entry_pair[np.concatenate([bk.host_quartets() for bk in bks])]
-        times = []
-        for _ in range(256):
-            q = quart[rng.integers(0, quart.shape[0])]
+        samples = []
+        for q in quart[rng.integers(0, quart.shape[0], size=min(per_class, quart.shape[0]))]:
             mi = pair_mol[q[0]]
             m = batch.mols[mi]
             po = batch.mol_pair_off[mi]
             sh = [m.pair_shell_a[q[0] - po], m.pair_shell_b[q[0] - po],
                   m.pair_shell_a[q[1] - po], m.pair_shell_b[q[1] - po]]
-            shells = [_OracleShell(m.shells[i]) for i in sh]
-            t0 = time.perf_counter()
-            contraction_block(*shells)
-            times.append(time.perf_counter() - t0)
-            if sum(times) > per_cls_budget:
-                break
-        spent += sum(times)
-        sampled += len(times)
-        est += n_cls * float(np.mean(times))
-        s_time += sum(times)
-        s_flops += fpq * len(times)
-    if pending_flops > 0:
-        est += pending_flops * s_time / s_flops
-    return {"value": total_eris / est, "unit": "ERIs/s", "cores": cores, "kind": "port",
-            "sample": f"{sampled} shell quartets over {len(stats) - skipped} of {len(stats)} "
-                      f"classes ({spent:.1f} s of CPU), extrapolated by class counts"
-                      + (f" and, for the {skipped} costliest classes, by the algorithmic FLOP "
-                         "model at the sampled oracle rate" if skipped else "")
-                      + "; oracle = NumPy restatement of the reference OS+HGP engine, one "
-                      f"process, BLAS limited to {cores} thread" + ("s" if cores > 1 else "")}
+            samples.append([[int(m.shells[i].l), m.shells[i].coord.tolist(),
+                             m.shells[i].exps.tolist(), m.shells[i].coeffs.tolist()]
+                            for i in sh])
+        stats.append((flops_cls[cls] / n, cls, n, samples))
+    stats.sort(key=lambda t: t[0])
+    return [(cls, n, samples) for _, cls, n, samples in stats], \
+        sum(bk.eris for bk in plans_fp64)
 
 
-class _OracleShell:
-    """Adapter: an expanded device-table shell seen through the reference's shell attributes."""
+_BASELINE_NOTE = ("; oracle = NumPy restatement of the reference OS+HGP engine, timed per "
+                  "class on random quartets of the step and extrapolated by class counts")
 
-    def __init__(self, es):
-        from gbasis_amd.utils import default_cart_components
 
-        self.coord = es.coord
-        self.angmom = es.l
-        self.exps = es.exps
-        self.coeffs = es.coeffs[:, None]
-        self.angmom_components_cart = default_cart_components(es.l)
+def baseline_one_core(stats, total_eris, budget_s):
+    """The sample timed in this process (BLAS limited to one thread by the caller)."""
+    from oracle.rate import time_classes
+
+    times = time_classes([s for _, _, s in stats], 0, budget_s)
+    est = sum(n * t / k for (_, n, _), (t, k) in zip(stats, times))
+    spent = sum(t for t, _ in times)
+    return {"value": total_eris / est, "unit": "ERIs/s", "cores": 1, "kind": "port",
+            "sample": f"{sum(k for _, k in times)} shell quartets over {len(stats)} classes "
+                      f"({spent:.1f} s of CPU) in one process on one core" + _BASELINE_NOTE}
+
+
+def baseline_all_cores(stats, total_eris, budget_s, ncores):
+    """The sample timed by ``ncores`` worker processes at once (python -m oracle.rate),
+    each on one BLAS thread with its own random draws: the per-class mean time under full
+    load gives the one-core time of the step, divided by ``ncores``."""
+    import subprocess
+    import tempfile
+
+    with tempfile.NamedTemporaryFile("w", suffix=".json", delete=False) as f:
+        json.dump({"classes": [s for _, _, s in stats]}, f)
+        path = f.name
+    env = dict(os.environ, OMP_NUM_THREADS="1", OPENBLAS_NUM_THREADS="1", MKL_NUM_THREADS="1")
+    try:
+        procs = [subprocess.Popen([sys.executable, "-m", "oracle.rate", path, str(1 + w),
+                                   str(budget_s)], cwd=ROOT, env=env, stdout=subprocess.PIPE,
+                                  stderr=subprocess.PIPE, text=True) for w in range(ncores)]
+        results = []
+        for p in procs:
+            out, err = p.communicate(timeout=budget_s * 4 + 300)
+            if p.returncode != 0:
+                raise RuntimeError(f"oracle.rate worker exited {p.returncode}: {err[-300:]}")
+            results.append(json.loads(out.strip().splitlines()[-1]))
+    finally:
+        os.unlink(path)
+    tsum = np.sum([[t for t, _ in r] for r in results], axis=0)
+    ksum = np.sum([[k for _, k in r] for r in results], axis=0)
+    est = sum(n * t / k for (_, n, _), t, k in zip(stats, tsum, ksum))
+    return {"value": float(total_eris * ncores / est), "unit": "ERIs/s", "cores": ncores,
+            "kind": "port",
+            "sample": f"{int(ksum.sum())} shell quartets over {len(stats)} classes "
+                      f"({float(tsum.sum()):.1f} s of CPU) in {ncores} worker processes at "
+                      "once, one core each" + _BASELINE_NOTE}
 
 
 CONFIG_WORKLOADS = {
