@@ -61,9 +61,9 @@ SIGNATURES = {
                                             ctypes.POINTER(GbxProgram), c_i32, c_i32, c_i32,
                                             c_i32, c_vp, c_i64, c_vp, c_i32, c_i32, c_i32,
                                             c_vp, c_vp]),
-    "gbx_eri_rys_acc": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_vp, c_i32, c_vp, c_i32,
-                                       c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp, c_i32,
-                                       c_vp, c_i64, c_vp]),
+    "gbx_eri_rys_acc": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_vp, c_i32, c_i32, c_vp,
+                                       c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp,
+                                       c_i32, c_vp, c_i64, c_vp]),
     "gbx_rys_table_size": (c_i64, []),
     "gbx_unpack_dense": (ctypes.c_int, [c_vp, c_i64, c_i32, c_vp, c_vp]),
     "gbx_jk_packed": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),

@@ -30,6 +30,9 @@ from ._lib import GBX_FP32_BULK, GBX_FP64
 
 class Autotuner:
     FP32_MARGIN = 1.10
+    # Rys pass variants: the two accumulator layouts (a third entry, when present, caps
+    # the primitive quartets per LDS batch; the library's default is its largest batch)
+    RYS = [("rys", 0), ("rys", 1)]
 
     def __init__(self, engine, repeats=3):
         self.eng = engine
@@ -39,7 +42,13 @@ class Autotuner:
 
     @staticmethod
     def key(bk):
-        return (tuple(bk.cls), int(bk.nprimq), bool(bk.dense), int(bk.precision))
+        """(class, primitive bin, dense, precision, phase).  The phase keeps the small
+        buckets of the build-centre sub-quartets (phases 1-3) from inheriting the choice
+        measured on the much fuller phase-0 bucket of the same class and bin; like the
+        other fields it is a property of the sub-quartet, so a request's variants do not
+        depend on the batch it runs in."""
+        return (tuple(bk.cls), int(bk.nprimq), bool(bk.dense), int(bk.precision),
+                int(bk.phase))
 
     def _time(self, fn):
         s = torch.cuda.current_stream()
@@ -83,16 +92,16 @@ class Autotuner:
                 elif eng.split_count(bk.cls) > 0 and eng.path != "coop":
                     hrrs = (0, 1) if (eng.has_hrr(bk.cls) and not bk.dense) else (0,)
                     variants = [("R", r, h) for r in (1, 2, 4, 8, 16, 32, 64) if r <= bk.nprimq
-                                for h in hrrs] + [("coop",)] + (
-                                    [("rys", 0), ("rys", 1)] if rys else [])
+                                for h in hrrs] + [("coop",)] + (self.RYS if rys else [])
                 elif rys:
-                    variants = [("coop",), ("rys", 0), ("rys", 1)]
+                    variants = [("coop",)] + self.RYS
                 for var in variants:
                     def run(prec=prec, var=var):
                         if var and var[0] == "rys":
                             eng.launch_class(dev, bk.cls, bk.quart, bk.nprimq, GBX_FP64,
                                              bk.mode(), out, bk.dense, "rys", seg=bk.seg,
-                                             count=bk.count, rys_layout=var[1])
+                                             count=bk.count, rys_layout=var[1],
+                                             rys_batch=var[2] if len(var) > 2 else None)
                             return
                         named = var[0] if var and var[0] == "coop" else None
                         eng.launch_bucket(dev, bk, out, precision=prec,
@@ -132,8 +141,14 @@ class Autotuner:
         if not os.path.exists(path):
             return False
         d = json.load(open(path))
+        from .tables import NUM_PHASES
+
         for k, prec, var in d["choice"]:
-            self.choice[(tuple(k[0]), k[1], k[2], k[3])] = (prec, tuple(var) if var else None)
+            # choices saved without a phase apply to every phase
+            phases = [k[4]] if len(k) > 4 else range(NUM_PHASES)
+            for ph in phases:
+                self.choice[(tuple(k[0]), k[1], k[2], k[3], ph)] = (prec,
+                                                                   tuple(var) if var else None)
         self.fp32_win = {(tuple(c), dense): v for c, dense, v in d["fp32"]}
         self.eng.set_fp32_allowed(self.fp32_win)
         return True

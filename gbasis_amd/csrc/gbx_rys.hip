@@ -37,7 +37,8 @@
 namespace gbx {
 
 constexpr int kRysThreads = 512;
-constexpr int kRysPBmax = 8;       // primitive quartets per LDS batch, at most
+constexpr int kRysPBmax = 32;      // primitive quartets per LDS batch, at most (the
+                                   // default; a caller may ask for fewer)
 constexpr int kRysMaxE = 9;        // la + lb + 1 <= 9 (l <= 4 per shell)
 constexpr int kRysMaxLb = 4;       // second shell of bra and ket: l <= 4 (groups of <= 5 x 5)
 constexpr int kRysKmax = 24;       // strided layout: accumulators per lane and launch
@@ -125,22 +126,30 @@ __device__ inline void rys_build(const gbx_tables& t, const RysArgs& a, int4 pb,
     const int ib = pp / nk, ik = pp - ib * nk;
     const double* rb = t.prim_pairs + static_cast<int64_t>(pb.z + ib) * kPrimRec;
     const double* rk = t.prim_pairs + static_cast<int64_t>(pk.z + ik) * kPrimRec;
-    PrimQuartet<double> s;
-    const double T = prim_quartet_setup(rb, rk, s);
+    // the quantities of prim_quartet_setup (gbx_common.h) this direction needs, with the
+    // same operations, taken before the node evaluation so that few stay live across it
+    const double zeta = rb[0], eta = rk[0];
+    const double rs = rsqrt(zeta + eta);
+    const double inv_s = rs * rs;
+    const double rho = zeta * eta * inv_s;
+    const double PQx = rb[2] - rk[2], PQy = rb[3] - rk[3], PQz = rb[4] - rk[4];
+    const double PQd = d == 0 ? PQx : (d == 1 ? PQy : PQz);
+    const double roz = eta * inv_s, roe = zeta * inv_s;
+    const double pa = rb[5 + d], qc = rk[5 + d];
+    const double wp = -roz * PQd, wq = roe * PQd;
+    const double i2z = rb[1], i2e = rk[1], i2s = 0.5 * inv_s;
+    const double pref = d == 2 ? rb[8] * rk[8] * rs : 1.0;
+    const double T = rho * (PQx * PQx + PQy * PQy + PQz * PQz);
     double u, wt;
     rys_node(a.rys, NR, r, T, u, wt);
-    const double pa = d == 0 ? s.PA[0] : (d == 1 ? s.PA[1] : s.PA[2]);
-    const double wp = d == 0 ? s.WP[0] : (d == 1 ? s.WP[1] : s.WP[2]);
-    const double qc = d == 0 ? s.QC[0] : (d == 1 ? s.QC[1] : s.QC[2]);
-    const double wq = d == 0 ? s.WQ[0] : (d == 1 ? s.WQ[1] : s.WQ[2]);
     const double c00 = pa + u * wp, d00 = qc + u * wq;
-    const double b10 = s.i2z * (1.0 - u * s.roz);
-    const double b01 = s.i2e * (1.0 - u * s.roe);
-    const double b00 = u * s.i2s;
+    const double b10 = i2z * (1.0 - u * roz);
+    const double b01 = i2e * (1.0 - u * roe);
+    const double b00 = u * i2s;
     // the table column by column in registers (column f: I(e, f), e <= lab), each
     // column stored once; entries beyond lab are computed and never stored
     double col[kRysMaxE], prv[kRysMaxE];
-    col[0] = d == 2 ? s.pref * wt : 1.0;
+    col[0] = d == 2 ? pref * wt : 1.0;
     prv[0] = 0.0;
 #pragma unroll
     for (int e = 1; e < kRysMaxE; ++e) {
@@ -152,18 +161,15 @@ __device__ inline void rys_build(const gbx_tables& t, const RysArgs& a, int4 pb,
       if (e <= a.lab) I[e * W] = col[e];
     for (int f = 0; f < a.lcd; ++f) {
       const double fb01 = static_cast<double>(f) * b01;
-      double nxt[kRysMaxE];
+      // column f + 1 in place, from the top entry down: entry e reads col[e - 1] of
+      // column f, which is still in place
 #pragma unroll
-      for (int e = 0; e < kRysMaxE; ++e) {
+      for (int e = kRysMaxE - 1; e >= 0; --e) {
         double v = d00 * col[e] + fb01 * prv[e];
         if (e > 0) v += static_cast<double>(e) * b00 * col[e - 1];
-        nxt[e] = v;
-      }
-#pragma unroll
-      for (int e = 0; e < kRysMaxE; ++e) {
-        if (e <= a.lab) I[e * W + f + 1] = nxt[e];
         prv[e] = col[e];
-        col[e] = nxt[e];
+        col[e] = v;
+        if (e <= a.lab) I[e * W + f + 1] = v;
       }
     }
   } else {
@@ -176,8 +182,15 @@ __device__ inline void rys_build(const gbx_tables& t, const RysArgs& a, int4 pb,
 //   ox = ax (lcd+1) + cx | oy = ay (lcd+1) + cy << 7 | az_lo << 14 | alen << 18 |
 //   cz_lo << 22 | clen << 26.
 // The lanes of a wavefront hold groups of one shape (alen, clen).
+// Both layouts are compiled for 4 waves per SIMD (two 512-lane workgroups per CU, at
+// most 128 VGPRs).  The lanes with 24 strided or 25 grouped accumulators then keep part
+// of their state in scratch; measured on the g classes, the second workgroup per CU
+// still shortens them by 20-40 % (it runs while the first builds its node tables).
+constexpr int kRysWaves = 4;
+
 template <int MA, int MC>
-__global__ __launch_bounds__(kRysThreads) void eri_rys_grouped_kernel(gbx_tables t, RysArgs a) {
+__global__ __launch_bounds__(kRysThreads, kRysWaves) void eri_rys_grouped_kernel(gbx_tables t,
+                                                                                RysArgs a) {
   constexpr int KG = 1;
   extern __shared__ __align__(16) double rtab[];
   const int tid = threadIdx.x;
@@ -284,7 +297,8 @@ int launch_rys_ma(int mc, const gbx_tables* t, const RysArgs& a, dim3 grid, size
 // Strided layout: lane tid holds accumulators f = acc_lo + tid + k 512, k < KMAX, each
 // read as three table entries per node.
 template <int NR, int KMAX>
-__global__ __launch_bounds__(kRysThreads) void eri_rys_strided_kernel(gbx_tables t, RysArgs a) {
+__global__ __launch_bounds__(kRysThreads, kRysWaves) void eri_rys_strided_kernel(gbx_tables t,
+                                                                                RysArgs a) {
   extern __shared__ __align__(16) double rtab[];
   const int tid = threadIdx.x;
   const int W = a.lcd + 1;
@@ -380,10 +394,10 @@ using namespace gbx;
 extern "C" int64_t gbx_rys_table_size(void) { return kRysTableSize; }
 
 extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t layout,
-                               const int32_t* d_index, int32_t n_index, int32_t la, int32_t lb,
-                               int32_t lc, int32_t ld, const int32_t* d_quartets, int64_t nquart,
-                               const int32_t* d_count, int32_t parts, double* d_acc,
-                               int64_t acc_stride, void* stream) {
+                               int32_t batch_max, const int32_t* d_index, int32_t n_index,
+                               int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                               const int32_t* d_quartets, int64_t nquart, const int32_t* d_count,
+                               int32_t parts, double* d_acc, int64_t acc_stride, void* stream) {
   if (!t || (nquart > 0 && (!d_quartets || !d_acc || !d_rys || !d_index)))
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: null argument");
   if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la + lb >= kRysMaxE || lb > kRysMaxLb ||
@@ -397,6 +411,8 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t
     return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: classes with 8 <= L <= 17 only");
   if (parts < 1 || parts > 4096)
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: parts must be in [1, 4096]");
+  if (batch_max < 0 || batch_max > kRysPBmax)
+    return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: batch_max must be in [0, 32]");
   if (acc_stride < nquart) return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: acc_stride < nquart");
   if ((la + lb + 1) * (lc + ld + 1) > 128)
     return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: tables too large for the index encoding");
@@ -429,8 +445,8 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t
   a.acc_out = d_acc;
   a.acc_stride = acc_stride;
   const int64_t per_prim = int64_t(sizeof(double)) * nr * 3 * (a.lab + 1) * (a.lcd + 1);
-  a.pb = static_cast<int>(std::min<int64_t>({kRysPBmax, 65536 / per_prim,
-                                             kRysThreads / (3 * nr)}));
+  a.pb = static_cast<int>(std::min<int64_t>({batch_max > 0 ? batch_max : kRysPBmax,
+                                             65536 / per_prim, kRysThreads / (3 * nr)}));
   if (a.pb < 1) return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: node tables exceed 64 KiB");
   const size_t shmem = static_cast<size_t>(per_prim * a.pb);
   const unsigned gx =

@@ -399,20 +399,22 @@ class EriEngine:
         """Launch one bucket; ``mode`` defaults to the bucket's own (store in phase 0,
         add in the later phases)."""
         prec = bk.launch_precision() if precision is None else precision
-        rys_layout = None
+        rys_layout = rys_batch = None
         if path is None:
             path = (bk.variant[0] if bk.variant and bk.variant[0] in ("coop", "rys")
                     else self.path)
-            if bk.variant and bk.variant[0] == "rys" and len(bk.variant) > 1:
-                rys_layout = bk.variant[1]
+            if bk.variant and bk.variant[0] == "rys":
+                rys_layout = bk.variant[1] if len(bk.variant) > 1 else None
+                rys_batch = bk.variant[2] if len(bk.variant) > 2 else None
         mode = bk.mode() if mode is None else mode
         return self.launch_class(dev, bk.cls, bk.quart, bk.nprimq, prec, mode, out, bk.dense,
                                  path, lanes if lanes is not None else bk.lanes(), bk.seg,
-                                 bk.count, hrr if hrr is not None else bk.hrr(), rys_layout)
+                                 bk.count, hrr if hrr is not None else bk.hrr(), rys_layout,
+                                 rys_batch)
 
     def launch_class(self, dev: DeviceTables, cls, quartets, nprimq, precision, mode, out,
                      dense=False, path="auto", lanes=None, seg=None, count=None, hrr=None,
-                     rys_layout=None):
+                     rys_layout=None, rys_batch=None):
         """One launch for a (class, bin) bucket; quartets is a device int32 (n,2).
 
         Classes with a generated register kernel and no dense output transform run
@@ -453,7 +455,8 @@ class EriEngine:
             if not (self.rys_ok(cls) and dev.potential == "coulomb"):
                 raise ValueError(f"class {cls} has no Rys primitive pass")
             return self._launch_rys(dev, tab, cls, quartets, nprimq, mode, out, count,
-                                    self.rys_layout(cls) if rys_layout is None else rys_layout)
+                                    self.rys_layout(cls) if rys_layout is None else rys_layout,
+                                    rys_batch)
         st, _, _ = self.programs.get(cls)
         nsplit = self.split_count(cls)
         if path != "coop" and nsplit > 0:
@@ -614,9 +617,11 @@ class EriEngine:
             self._rys_idx[cls] = (torch.from_numpy(arr).to(self.device), len(descs))
         return self._rys_idx[cls]
 
-    def _launch_rys(self, dev, tab, cls, quartets, nprimq, mode, out, count, layout):
+    def _launch_rys(self, dev, tab, cls, quartets, nprimq, mode, out, count, layout,
+                    batch=None):
         """Rys primitive pass split over R workgroups per quartet, then the cooperative
-        horizontal recursion and epilogue (FP64)."""
+        horizontal recursion and epilogue (FP64).  ``batch``: primitive quartets per LDS
+        batch, at most (None: the library's default)."""
         nq = quartets.shape[0]
         R = self._coop_parts(nq, nprimq)
         st, _, _ = self.programs.get(cls)
@@ -628,7 +633,8 @@ class EriEngine:
         index, nindex = (self._rys_groups(cls) if layout == self.RYS_GROUPED
                          else self._rys_strided(cls))
         check(self.lib.gbx_eri_rys_acc(ctypes.byref(tab), ptr(self._rys_table()), int(layout),
-                                       ptr(index), nindex, *cls, ptr(quartets), nq,
+                                       int(batch or 0), ptr(index), nindex, *cls,
+                                       ptr(quartets), nq,
                                        ptr(count), R, ptr(acc), nq, self._stream()),
               f"gbx_eri_rys_acc{cls}")
         sc = self.scratch()

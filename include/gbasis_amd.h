@@ -193,11 +193,15 @@ int gbx_eri_coop_acc(const gbx_tables *t, const gbx_program *prog, int32_t la, i
  *    ox | oy << 7 | az_lo << 14 | alen << 18 | cz_lo << 22 | clen << 26 (members
  *    a_z = az_lo + i, i < alen, c_z = cz_lo + j, j < clen), every descriptor of a 64-lane
  *    wavefront of the same (alen, clen); then, per slot, (l_b + 1)(l_d + 1) rows f
- *    (member (i, j) at i (l_d + 1) + j), -1 where none. */
+ *    (member (i, j) at i (l_d + 1) + j), -1 where none.
+ * batch_max (0: 32, else 1..32) caps the primitive quartets whose node tables a
+ * workgroup builds at once in LDS (further capped by 64 KiB of tables and 512 build
+ * threads); larger batches amortise the table build of the small classes. */
 int gbx_eri_rys_acc(const gbx_tables *t, const double *d_rys, int32_t layout,
-                    const int32_t *d_index, int32_t n_index, int32_t la, int32_t lb, int32_t lc,
-                    int32_t ld, const int32_t *d_quartets, int64_t nquart, const int32_t *d_count,
-                    int32_t parts, double *d_acc, int64_t acc_stride, void *stream);
+                    int32_t batch_max, const int32_t *d_index, int32_t n_index, int32_t la,
+                    int32_t lb, int32_t lc, int32_t ld, const int32_t *d_quartets,
+                    int64_t nquart, const int32_t *d_count, int32_t parts, double *d_acc,
+                    int64_t acc_stride, void *stream);
 
 /* Number of doubles of the Rys node table gbx_eri_rys_acc reads. */
 int64_t gbx_rys_table_size(void);

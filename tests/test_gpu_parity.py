@@ -352,9 +352,11 @@ def test_mixed_after_fp32_mask_changes(gpu):
 
 
 def test_rys_vs_cooperative(gpu):
-    """The Rys primitive pass in both accumulator layouts (every class with 8 <= L <= 17
-    forced onto it, sub-quartet phases included) gives the cooperative kernel's packed
-    array within 1e-12 of the largest value, pure and Cartesian shells up to g."""
+    """The Rys primitive pass in both accumulator layouts, with the default LDS batch (up
+    to 32 primitive quartets) and with batches of at most 8 (every class with
+    8 <= L <= 17 forced onto it, sub-quartet phases included), gives the cooperative
+    kernel's packed array within 1e-12 of the largest value, pure and Cartesian shells
+    up to g."""
     import torch
 
     from gbasis_amd.engine import DeviceTables, get_engine
@@ -373,18 +375,19 @@ def test_rys_vs_cooperative(gpu):
         finally:
             eng.path = "auto"
         scale = max(1.0, float(ref.abs().max()))
-        for layout in (eng.RYS_STRIDED, eng.RYS_GROUPED):
+        for variant in (("rys", eng.RYS_STRIDED), ("rys", eng.RYS_GROUPED),
+                        ("rys", eng.RYS_STRIDED, 8), ("rys", eng.RYS_GROUPED, 8)):
             forced = 0
             for bk in plans:
                 bk.variant = None
                 if eng.rys_ok(bk.cls):
-                    bk.variant = ("rys", layout)
+                    bk.variant = variant
                     forced += 1
             assert forced > 0
             got = eng.run(dev, plans)
             assert all(bk.path == "rys" for bk in plans if bk.variant)
             err = float((got - ref).abs().max())
-            print(f"{ls} {coord} layout {layout}: {forced} buckets on the Rys pass, max diff "
+            print(f"{ls} {coord} {variant}: {forced} buckets on the Rys pass, max diff "
                   f"{err:.2e}", flush=True)
             assert err <= 1e-12 * scale
         torch.cuda.synchronize()
