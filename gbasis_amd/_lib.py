@@ -63,8 +63,11 @@ SIGNATURES = {
                                             c_vp, c_vp]),
     "gbx_eri_rys_acc": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_vp, c_i32, c_i32, c_vp,
                                        c_i32, c_i32, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp,
-                                       c_i32, c_vp, c_i64, c_vp]),
+                                       c_i32, c_vp, c_i64, c_i32, c_vp]),
     "gbx_rys_table_size": (c_i64, []),
+    "gbx_eri_hrr_lds": (ctypes.c_int, [ctypes.POINTER(GbxTables), c_i32, c_i32, c_i32, c_i32,
+                                       c_vp, c_vp, c_i32, c_i32, c_i32, c_vp, c_i64, c_vp,
+                                       c_i32, c_vp, c_i64, c_i32, c_i32, c_vp, c_vp]),
     "gbx_unpack_dense": (ctypes.c_int, [c_vp, c_i64, c_i32, c_vp, c_vp]),
     "gbx_jk_packed": (ctypes.c_int, [c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
     "gbx_mo_workspace": (ctypes.c_int, [c_i64, c_i64, ctypes.POINTER(c_i64)]),

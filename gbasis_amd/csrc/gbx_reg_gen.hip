@@ -1,5 +1,5 @@
 // GENERATED by gbasis_amd/codegen.py -- do not edit.
-// inputs 6ea9f7d8b661e2e2
+// inputs 8536766e4a5b6837
 // Dispatch over the 8 parts of the generated register kernels.
 #include "gbx_reg.h"
 

@@ -6,7 +6,9 @@
 // accumulator of the class program (bra component a of degree la..la+lb, ket
 // component c of degree lc..lc+ld), and writes them to
 // acc_out[(f * parts + r) * acc_stride + qi] -- the layout the cooperative horizontal
-// recursion pass (gbx_eri_hrr) reads.  It replaces the Boys function and the vertical
+// recursion pass (gbx_eri_hrr) reads -- or acc_out[(qi * parts + r) * acc_stride + f],
+// the quartet-major layout of the LDS pass (gbx_eri_hrr_lds), whose writes by
+// consecutive lanes are contiguous.  It replaces the Boys function and the vertical
 // and ket-centre recursions of gbasis/integrals/_two_elec_int_improved.py:87-361 (the
 // same primitive integrals: the Rys rule integrates the Boys moments F_m(T), m < 2n,
 // exactly).
@@ -103,7 +105,14 @@ struct RysArgs {
   int32_t parts;
   double* acc_out;
   int64_t acc_stride;
+  int32_t qmajor;         // 1: acc_out[(qi * parts + r) * acc_stride + f]
 };
+
+// Address of accumulator f of work item (qi, part) in either layout.
+__device__ inline int64_t rys_acc_index(const RysArgs& a, int64_t qi, int part, int f) {
+  return a.qmajor ? (qi * a.parts + part) * a.acc_stride + f
+                  : (static_cast<int64_t>(f) * a.parts + part) * a.acc_stride + qi;
+}
 
 // One thread's part of a batch's node tables: thread tid < PB NR 3 builds the table of
 // primitive quartet p0 + tid / (3 NR), node (tid / 3) % NR, direction tid % 3, column by
@@ -268,7 +277,7 @@ __global__ __launch_bounds__(kRysThreads, kRysWaves) void eri_rys_grouped_kernel
         for (int j = 0; j < MC; ++j) {
           const int f = (i < alen[k] && j < clen[k]) ? rows[i * MC + j] : -1;
           if (f >= 0)
-            a.acc_out[(static_cast<int64_t>(f) * a.parts + part) * a.acc_stride + qi] =
+            a.acc_out[rys_acc_index(a, qi, part, f)] =
                 acc[k][i][j];
         }
     }
@@ -359,7 +368,7 @@ __global__ __launch_bounds__(kRysThreads, kRysWaves) void eri_rys_strided_kernel
     for (int k = 0; k < KMAX; ++k) {
       if (idx[k] < 0) continue;
       const int f = a.acc_lo + tid + k * kRysThreads;
-      a.acc_out[(static_cast<int64_t>(f) * a.parts + part) * a.acc_stride + qi] = acc[k];
+      a.acc_out[rys_acc_index(a, qi, part, f)] = acc[k];
     }
   }
 }
@@ -397,7 +406,8 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t
                                int32_t batch_max, const int32_t* d_index, int32_t n_index,
                                int32_t la, int32_t lb, int32_t lc, int32_t ld,
                                const int32_t* d_quartets, int64_t nquart, const int32_t* d_count,
-                               int32_t parts, double* d_acc, int64_t acc_stride, void* stream) {
+                               int32_t parts, double* d_acc, int64_t acc_stride,
+                               int32_t acc_layout, void* stream) {
   if (!t || (nquart > 0 && (!d_quartets || !d_acc || !d_rys || !d_index)))
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: null argument");
   if (la < 0 || lb < 0 || lc < 0 || ld < 0 || la + lb >= kRysMaxE || lb > kRysMaxLb ||
@@ -413,7 +423,8 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: parts must be in [1, 4096]");
   if (batch_max < 0 || batch_max > kRysPBmax)
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: batch_max must be in [0, 32]");
-  if (acc_stride < nquart) return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: acc_stride < nquart");
+  if (acc_layout != 0 && acc_layout != 1)
+    return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: acc_layout must be 0 or 1");
   if ((la + lb + 1) * (lc + ld + 1) > 128)
     return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: tables too large for the index encoding");
   auto ncomp = [](int lo, int hi) {
@@ -422,6 +433,9 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t
     return n;
   };
   const int nacc = ncomp(la, la + lb) * ncomp(lc, lc + ld);
+  if (acc_layout == 0 ? acc_stride < nquart : acc_stride < nacc)
+    return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: acc_stride below the quartets (layout 0) or "
+                             "the accumulators (layout 1)");
   if (layout == kRysGrouped) {
     if (n_index < 1 || n_index % kRysThreads != 0 || n_index > 64 * kRysThreads)
       return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: grouped slots must be a multiple of 512");
@@ -444,6 +458,7 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t
   a.parts = parts;
   a.acc_out = d_acc;
   a.acc_stride = acc_stride;
+  a.qmajor = acc_layout;
   const int64_t per_prim = int64_t(sizeof(double)) * nr * 3 * (a.lab + 1) * (a.lcd + 1);
   a.pb = static_cast<int>(std::min<int64_t>({batch_max > 0 ? batch_max : kRysPBmax,
                                              65536 / per_prim, kRysThreads / (3 * nr)}));

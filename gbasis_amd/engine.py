@@ -155,6 +155,10 @@ class DeviceTables:
         self.potential = potential
         self.batch = batch
         self.device = torch.device(device)
+        # largest output dimension of the batch's shells per angular momentum
+        info = np.asarray(batch.shell_info).reshape(-1, 8)
+        self.max_out_dim = {int(l): int(info[info[:, 1] == l, 5].max())
+                            for l in np.unique(info[:, 1])}
         tab = boys_table()
         arrays = {
             "shell_info": batch.shell_info.astype(np.int32),
@@ -265,6 +269,10 @@ class EriEngine:
         self._masks = []
         self.set_fp32_allowed({})
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
+        # horizontal-recursion pass in LDS (gbx_eri_hrr_lds): "auto" for the classes whose
+        # cooperative pass would run on global scratch, "always" wherever it fits, "never"
+        self.hrr_lds = "auto"
+        self._recipes = {}
         # deterministic: variant choices depend only on (class, primitive bin, precision),
         # never on how many quartets a launch holds, so the value of every packed slot
         # is independent of the batch it runs in (serving)
@@ -477,11 +485,7 @@ class EriEngine:
                                                ptr(count), ptr(acc), nq, R, mode, ptr(out),
                                                self._stream()), f"gbx_eri_hrr_reg{cls}")
                 return "split"
-            sc = self.scratch()
-            check(self.lib.gbx_eri_hrr(ctypes.byref(tab), ctypes.byref(st), *cls,
-                                       ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
-                                       ptr(out), ptr(sc), sc.numel(), self._stream()),
-                  f"gbx_eri_hrr{cls}")
+            self._hrr_pass(dev, tab, st, cls, quartets, count, mode, acc, R, out)
             return "split"
         nq = quartets.shape[0]
         R = self._coop_parts(nq, nprimq)
@@ -512,10 +516,7 @@ class EriEngine:
                                             ptr(quartets), nq, ptr(count), precision, pf, R,
                                             ptr(acc), nq, ptr(sc), sc.numel(), self._stream()),
                   f"gbx_eri_coop_acc{cls}")
-        check(self.lib.gbx_eri_hrr(ctypes.byref(tab), ctypes.byref(st), *cls,
-                                   ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
-                                   ptr(out), ptr(sc), sc.numel(), self._stream()),
-              f"gbx_eri_hrr{cls}")
+        self._hrr_pass(dev, tab, st, cls, quartets, count, mode, acc, R, out)
         return "coop2"
 
     # Rys primitive pass (gbx_eri_rys_acc): admissible for 8 <= L <= 17, taken by default
@@ -632,17 +633,78 @@ class EriEngine:
             acc = torch.empty(nacc * R * nq, dtype=torch.float64, device=self.device)
         index, nindex = (self._rys_groups(cls) if layout == self.RYS_GROUPED
                          else self._rys_strided(cls))
+        # the LDS recursion pass reads the quartet-major layout, whose writes here are
+        # contiguous across lanes
+        lds = self._lds_hrr_plan(dev, cls, mode)
+        qmajor = lds is not None
         check(self.lib.gbx_eri_rys_acc(ctypes.byref(tab), ptr(self._rys_table()), int(layout),
                                        int(batch or 0), ptr(index), nindex, *cls,
-                                       ptr(quartets), nq,
-                                       ptr(count), R, ptr(acc), nq, self._stream()),
+                                       ptr(quartets), nq, ptr(count), R, ptr(acc),
+                                       nacc if qmajor else nq, 1 if qmajor else 0,
+                                       self._stream()),
               f"gbx_eri_rys_acc{cls}")
+        self._hrr_pass(dev, tab, st, cls, quartets, count, mode, acc, R, out, qmajor=qmajor,
+                       lds=lds)
+        return "rys"
+
+    LDS_BYTES = 160 * 1024   # LDS of one gfx950 workgroup
+
+    def _hrr_recipe(self, cls):
+        """(host int32 recipe, device copy, LDS bytes of W per unit output dimension
+        product, LDS bytes of one row buffer) of the LDS horizontal-recursion pass."""
+        if cls not in self._recipes:
+            from .programs import hrr_recipe, ncart
+
+            la, lb, lc, ld = cls
+            rec = np.ascontiguousarray(hrr_recipe(*cls), dtype=np.int32)
+            na, nc, nlk, nlb = (int(x) for x in rec[:4])
+            half = max([na, nc, ncart(lc) * ncart(ld), ncart(la) * ncart(lb)]
+                       + [int(x) for x in rec[4:4 + nlk + nlb]])
+            half += half & 1
+            self._recipes[cls] = (rec, torch.from_numpy(rec).to(self.device), na * 8,
+                                  2 * half * 8)
+        return self._recipes[cls]
+
+    def _lds_hrr_plan(self, dev, cls, mode):
+        """(host recipe, device recipe, largest C and D output dimensions) when the LDS
+        horizontal-recursion pass runs this class and mode, else None."""
+        if self.hrr_lds == "never" or mode not in (GBX_MODE_PACKED, GBX_MODE_PACKED_ADD):
+            return None
+        rec, drec, w_unit, row = self._hrr_recipe(cls)
+        dc = dev.max_out_dim.get(cls[2], 1)
+        dd = dev.max_out_dim.get(cls[3], 1)
+        if w_unit * dc * dd + row > self.LDS_BYTES:
+            return None
+        if self.hrr_lds == "always" or self._workspace(cls, GBX_FP64, 1, False)[1] > 0:
+            return rec, drec, dc, dd
+        return None
+
+    def _hrr_pass(self, dev, tab, st, cls, quartets, count, mode, acc, R, out, qmajor=False,
+                  lds=None):
+        """Horizontal recursion, norms, transforms and placement of a class from R partial
+        contracted accumulators per quartet: the LDS pass (gbx_eri_hrr_lds) where
+        ``_lds_hrr_plan`` selects it, else the cooperative pass (gbx_eri_hrr).  ``qmajor``:
+        the accumulators are in the quartet-major layout (LDS pass only)."""
+        nq = quartets.shape[0]
+        if lds is None:
+            lds = self._lds_hrr_plan(dev, cls, mode)
+        if lds is not None:
+            rec, drec, dc, dd = lds
+            stride = st.header[1] if qmajor else nq
+            check(self.lib.gbx_eri_hrr_lds(ctypes.byref(tab), *cls, ptr(rec), ptr(drec),
+                                           int(rec.size), dc, dd, ptr(quartets), nq,
+                                           ptr(count), mode, ptr(acc), stride, R,
+                                           1 if qmajor else 0, ptr(out), self._stream()),
+                  f"gbx_eri_hrr_lds{cls}")
+            return
+        if qmajor:
+            raise ValueError("the cooperative horizontal-recursion pass reads the "
+                             "accumulator-major layout only")
         sc = self.scratch()
         check(self.lib.gbx_eri_hrr(ctypes.byref(tab), ctypes.byref(st), *cls,
                                    ptr(quartets), nq, ptr(count), mode, ptr(acc), nq, R,
                                    ptr(out), ptr(sc), sc.numel(), self._stream()),
               f"gbx_eri_hrr{cls}")
-        return "rys"
 
     def schwarz(self, dev: DeviceTables):
         """Bounds per global shell pair, device tensor (npair, 2).

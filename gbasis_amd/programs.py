@@ -447,6 +447,57 @@ def _subprogram(p, finals):
     return q
 
 
+def _hrr_side(lo, span, l_first, l_second):
+    """Levels of one side of the horizontal recursion, [x, y+1_i] = [x+1_i, y] +
+    G_i [x, y] (i = direction(y + 1_i)): level 0 lists the components x of degree
+    lo..lo+span in comps order; level k the (x, y) with deg(y) = k that the final block
+    needs, the last one being [(x, y) for x in comps(l_first) for y in comps(l_second)].
+    Returns the list of levels, each a list of (s1, s2, dir) into the level below."""
+    level0 = [c for n in range(lo, lo + span + 1) for c in comps(n)]
+    final = [(x, y) for x in comps(l_first) for y in comps(l_second)]
+    need = set()
+    stack = list(final)
+    while stack:
+        x, y = stack.pop()
+        if (x, y) in need or y == Z3:
+            continue
+        need.add((x, y))
+        i = direction(y)
+        stack.append((add(x, i), sub(y, i)))
+        stack.append((x, sub(y, i)))
+    index = {(x, Z3): j for j, x in enumerate(level0)}
+    levels = []
+    for k in range(1, l_second + 1):
+        if k == l_second:
+            entries = final
+        else:
+            entries = sorted((g for g in need if deg(g[1]) == k),
+                             key=lambda g: (deg(g[0]), comp_index(g[0]), comp_index(g[1])))
+        rows = []
+        for x, y in entries:
+            i = direction(y)
+            rows.append((index[(add(x, i), sub(y, i))], index[(x, sub(y, i))], i))
+        index = {g: j for j, g in enumerate(entries)}
+        levels.append(rows)
+    return levels
+
+
+@lru_cache(maxsize=None)
+def hrr_recipe(la, lb, lc, ld):
+    """Recipe of the LDS horizontal-recursion pass (gbx_eri_hrr_lds) of a class: int32
+    [NA, NC, l_d, l_b, ket level sizes, bra level sizes, entries s1 | s2 << 12 | dir << 24
+    of the ket levels, then of the bra levels].  The steps are those of the class
+    program's HRR phases (same sources, same direction), grouped by level."""
+    ket = _hrr_side(lc, ld, lc, ld)
+    bra = _hrr_side(la, lb, la, lb)
+    na = sum(ncart(n) for n in range(la, la + lb + 1))
+    nc = sum(ncart(n) for n in range(lc, lc + ld + 1))
+    head = [na, nc, ld, lb] + [len(v) for v in ket] + [len(v) for v in bra]
+    body = [s1 | (s2 << 12) | (i << 24) for v in ket + bra for s1, s2, i in v]
+    assert all(len(v) < 4096 for v in ket + bra)
+    return np.array(head + body, dtype=np.int32)
+
+
 @lru_cache(maxsize=None)
 def class_subprograms(la, lb, lc, ld, parts):
     """The primitive part of a class split into ``parts`` programs over contiguous runs

@@ -179,8 +179,10 @@ int gbx_eri_coop_acc(const gbx_tables *t, const gbx_program *prog, int32_t la, i
  * nodes), l_a + l_b <= 8, l_c + l_d <= 8, l_b <= 4, l_d <= 4, Coulomb operator, FP64:
  * work item (q, r) sums the primitive quartets [r c, (r + 1) c) of quartet q as sum over
  * nodes of pref W_u Ix Iy Iz and writes the partial contracted [e0|f0] to
- * d_acc[(f * parts + r) * acc_stride + q], the layout of gbx_eri_coop_acc, which
- * gbx_eri_hrr with acc_parts = parts finishes.  Replaces the Boys function, vertical
+ * d_acc[(f * parts + r) * acc_stride + q] (acc_layout 0, acc_stride >= nquart), the
+ * layout of gbx_eri_coop_acc, which gbx_eri_hrr with acc_parts = parts finishes, or
+ * d_acc[(q * parts + r) * acc_stride + f] (acc_layout 1, acc_stride >= the number of
+ * accumulators), which gbx_eri_hrr_lds reads.  Replaces the Boys function, vertical
  * recursion and transfer to the ket of gbasis/integrals/_two_elec_int_improved.py:87-361
  * with the same integrals (the rule integrates F_m(T), m < 2n, exactly).  d_rys is the
  * node table (gbx_rys_table_size doubles, built by gbasis_amd/rys.py).  Accumulator f is
@@ -201,7 +203,7 @@ int gbx_eri_rys_acc(const gbx_tables *t, const double *d_rys, int32_t layout,
                     int32_t batch_max, const int32_t *d_index, int32_t n_index, int32_t la,
                     int32_t lb, int32_t lc, int32_t ld, const int32_t *d_quartets,
                     int64_t nquart, const int32_t *d_count, int32_t parts, double *d_acc,
-                    int64_t acc_stride, void *stream);
+                    int64_t acc_stride, int32_t acc_layout, void *stream);
 
 /* Number of doubles of the Rys node table gbx_eri_rys_acc reads. */
 int64_t gbx_rys_table_size(void);
@@ -296,6 +298,26 @@ int gbx_eri_hrr(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_
                 int32_t mode, const double *d_acc, int64_t acc_stride, int32_t acc_parts,
                 double *d_out,
                 void *d_scratch, int64_t scratch_bytes, void *stream);
+
+/* The same pass as gbx_eri_hrr in GBX_MODE_PACKED / GBX_MODE_PACKED_ADD, for classes
+ * whose whole recursion workspace exceeds the LDS (gbx_eri_hrr runs them on global
+ * scratch): one workgroup per quartet runs the ket recursion, the norms of c and d and
+ * the transforms of shells C and D row by row (bra component e) into an LDS array
+ * W[e][c'd'], then the bra recursion, the norms and transforms of A and B and the
+ * placement column by column (gbasis/integrals/_two_elec_int_improved.py:364-588,
+ * gbasis/base_four_symm.py:160-557).  h_recipe / d_recipe: the class's recursion
+ * recipe (gbasis_amd.programs.hrr_recipe, recipe_len int32) on the host and on the
+ * device; max_dim_c / max_dim_d: the largest output dimension of the batch's shells of
+ * l_c / l_d (W is sized for them; a quartet with larger ones is skipped, so the caller
+ * passes the batch's maxima).  acc_layout 0 reads d_acc as gbx_eri_hrr does, 1 as
+ * d_acc[(q * acc_parts + r) * acc_stride + f] (the quartet-major layout of
+ * gbx_eri_rys_acc).  GBX_ERR_UNSUPPORTED when W and one row buffer exceed the LDS. */
+int gbx_eri_hrr_lds(const gbx_tables *t, int32_t la, int32_t lb, int32_t lc, int32_t ld,
+                    const int32_t *h_recipe, const int32_t *d_recipe, int32_t recipe_len,
+                    int32_t max_dim_c, int32_t max_dim_d, const int32_t *d_quartets,
+                    int64_t nquart, const int32_t *d_count, int32_t mode, const double *d_acc,
+                    int64_t acc_stride, int32_t acc_parts, int32_t acc_layout, double *d_out,
+                    void *stream);
 
 /* Register form of gbx_eri_hrr for the split classes gbx_has_hrr_kernel lists, packed output
  * (mode GBX_MODE_PACKED or GBX_MODE_PACKED_ADD) and shells with a scaled-permutation output map (Cartesian, pure l <= 1): one

@@ -393,6 +393,47 @@ def test_rys_vs_cooperative(gpu):
         torch.cuda.synchronize()
 
 
+def test_lds_hrr_vs_cooperative_hrr(gpu):
+    """The horizontal-recursion pass in LDS (gbx_eri_hrr_lds, forced on every class that
+    runs a separate recursion pass and fits) gives the cooperative pass's packed array
+    within 1e-12 of the largest value: pure, Cartesian and mixed shells up to g, with
+    the build-centre sub-quartet phases (stores and adds)."""
+    import torch
+
+    from gbasis_amd.engine import DeviceTables, get_engine
+    from gbasis_amd.tables import BatchTables, MoleculeTables
+
+    _print_case("lds hrr vs coop hrr")
+    eng = get_engine()
+    cases = ((43, [4, 3, 2, 1], "spherical"), (44, [4, 4, 3, 0], "cartesian"),
+             (45, [3, 3, 2, 2], "cartesian"), (46, [4, 2, 3, 1], "mixed"))
+    phases = set()
+    for seed, ls, coord in cases:
+        if coord == "mixed":
+            basis = (random_basis(seed, ls[:2], kmax=3, coord_type="spherical", emin=0.05,
+                                  emax=2e3)
+                     + random_basis(seed + 100, ls[2:], kmax=3, coord_type="cartesian",
+                                    emin=0.05, emax=2e3))
+        else:
+            basis = random_basis(seed, ls, kmax=3, coord_type=coord, emin=0.05, emax=2e3)
+        dev = DeviceTables(BatchTables([MoleculeTables(basis)]))
+        plans = eng.plan_quartets(dev)
+        phases |= {bk.phase for bk in plans}
+        try:
+            eng.hrr_lds = "never"
+            ref = eng.run(dev, plans)
+            eng.hrr_lds = "always"
+            got = eng.run(dev, plans)
+        finally:
+            eng.hrr_lds = "auto"
+        scale = max(1.0, float(ref.abs().max()))
+        err = float((got - ref).abs().max())
+        print(f"{ls} {coord}: max diff {err:.2e} (scale {scale:.2e})", flush=True)
+        assert err <= 1e-12 * scale
+        torch.cuda.synchronize()
+    assert phases == {0, 1, 2, 3}
+
+
 def test_register_vs_cooperative(gpu):
     """Both kernel families give the same packed array (FP64)."""
     from gbasis_amd.engine import DeviceTables, get_engine
