@@ -626,6 +626,7 @@ def run_workload(name, args, tuner=None):
     batch = BatchTables([mol])
     eng = get_engine("cuda")
     eng.launch_streams = args.launch_streams
+    eng.hrr_lds = args.hrr_lds
     dev = DeviceTables(batch)
     schwarz_ms = time_schwarz(eng, dev)
     q = eng.schwarz(dev)
@@ -744,6 +745,9 @@ def main():
     ap.add_argument("--launch-streams", type=int, default=4,
                     help="spread the class launches of a step over this many streams (the "
                          "per-class timings then need --launch-streams 1)")
+    ap.add_argument("--hrr-lds", choices=["auto", "always", "never"], default="auto",
+                    help="horizontal-recursion pass in LDS: for the classes whose cooperative "
+                         "pass needs global scratch (auto), wherever it fits, or never")
     ap.add_argument("--workload", choices=["batch64"] + list(CONFIG_WORKLOADS), default="batch64",
                     help="batch64: the 64-molecule batch plus serving (default); the others "
                          "are single-molecule workloads")
@@ -775,6 +779,7 @@ def main():
     eng = get_engine("cuda")
     eng.path = args.path
     eng.launch_streams = args.launch_streams
+    eng.hrr_lds = args.hrr_lds
     dev = DeviceTables(batch)
     schwarz_ms = time_schwarz(eng, dev)
     q = eng.schwarz(dev)

@@ -33,7 +33,9 @@
 
 namespace gbx {
 
-constexpr int kHrr2Threads = 256;
+// one workgroup per CU (its LDS arrays take most of the 160 KiB): 16 wavefronts keep the
+// level loops and the transforms busy
+constexpr int kHrr2Threads = 1024;
 
 struct Hrr2Args {
   const int32_t* quart;
@@ -122,9 +124,11 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
         const int64_t f = static_cast<int64_t>(c) * NA + e0 + r;
         double s = 0.0;
         if (a.qmajor) {
+#pragma unroll 4
           for (int p = 0; p < a.acc_parts; ++p)
             s += a.acc_in[(qi * a.acc_parts + p) * a.acc_stride + f];
         } else {
+#pragma unroll 4
           for (int p = 0; p < a.acc_parts; ++p)
             s += a.acc_in[(f * a.acc_parts + p) * a.acc_stride + qi];
         }
