@@ -1,23 +1,22 @@
 """Measured choice of kernel variants per work bucket.
 
-For every bucket (class, primitive-count bin, precision class, dense flag) of a
-work queue the tuner times the admissible variants on the bucket's own
-quartets and keeps the fastest:
+For every bucket (class, primitive-count bin, precision class, dense flag,
+sub-quartet phase) of a work queue the tuner times the admissible variants on the
+bucket's own quartets and keeps the fastest:
 
 * mixed precision: a bucket whose quartets may run with the FP32 recursion
   bulk (bound below the cutoff) is timed in FP32 and in FP64.  Both meet the
   error bound, so the faster one is kept.  (On CDNA4 non-packed FP32 FMA runs
   at the FP64 vector rate, so FP32 wins only where it relieves registers or LDS.)
 * split classes: lanes per shell quartet R in {1, 2, 4, ..., 64} with either
-  horizontal-recursion pass, or the cooperative kernel instead, or (8 <= L <= 17) the
+  horizontal-recursion pass, or the cooperative kernel instead, or (6 <= L <= 17) the
   Rys primitive pass in either accumulator layout;
-* classes of the cooperative kernel with 8 <= L <= 17: its own primitive pass or the
-  Rys primitive pass in either layout (all finished by the cooperative horizontal
-  recursion);
+* classes of the cooperative kernel with 6 <= L <= 17: its own primitive pass or the
+  Rys primitive pass in either layout (all finished by a horizontal-recursion pass);
 * cooperative classes: primitive quartets in flight PF.
 
-Choices are cached in memory per (class, bin, dense, precision class) and can be
-saved to / loaded from a JSON file.
+Choices are cached in memory per (class, bin, dense, precision class, phase) and can
+be saved to / loaded from a JSON file.
 """
 
 import json

@@ -388,6 +388,7 @@ int launch_rys_strided_nr(const gbx_tables* t, const RysArgs& a, dim3 grid, size
 int launch_rys_strided(int nr, const gbx_tables* t, const RysArgs& a, dim3 grid, size_t shmem,
                        hipStream_t s) {
   switch (nr) {
+    case 4: return launch_rys_strided_nr<4>(t, a, grid, shmem, s);
     case 5: return launch_rys_strided_nr<5>(t, a, grid, shmem, s);
     case 6: return launch_rys_strided_nr<6>(t, a, grid, shmem, s);
     case 7: return launch_rys_strided_nr<7>(t, a, grid, shmem, s);
@@ -417,8 +418,8 @@ extern "C" int gbx_eri_rys_acc(const gbx_tables* t, const double* d_rys, int32_t
     return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: the Rys rule integrates 1/r only");
   const int L = la + lb + lc + ld;
   const int nr = L / 2 + 1;
-  if (nr < 5 || nr > kRysNmax)
-    return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: classes with 8 <= L <= 17 only");
+  if (nr < 4 || nr > kRysNmax)
+    return fail(GBX_ERR_UNSUPPORTED, "gbx_eri_rys_acc: classes with 6 <= L <= 17 only");
   if (parts < 1 || parts > 4096)
     return fail(GBX_ERR_ARG, "gbx_eri_rys_acc: parts must be in [1, 4096]");
   if (batch_max < 0 || batch_max > kRysPBmax)

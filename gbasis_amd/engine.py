@@ -519,13 +519,14 @@ class EriEngine:
         self._hrr_pass(dev, tab, st, cls, quartets, count, mode, acc, R, out)
         return "coop2"
 
-    # Rys primitive pass (gbx_eri_rys_acc): admissible for 8 <= L <= 17, taken by default
-    # from RYS_MIN_L on (the autotuner times it against the cooperative kernel)
+    # Rys primitive pass (gbx_eri_rys_acc): admissible for 6 <= L <= 17 (the autotuner
+    # times it against the other variants), taken by default from RYS_MIN_L on for the
+    # classes without register kernels
     RYS_MIN_L = 8
 
     def rys_ok(self, cls):
         la, lb, lc, ld = cls
-        return (8 <= la + lb + lc + ld <= 17 and lb <= 4 and ld <= 4 and la + lb <= 8
+        return (6 <= la + lb + lc + ld <= 17 and lb <= 4 and ld <= 4 and la + lb <= 8
                 and lc + ld <= 8 and (la + lb + 1) * (lc + ld + 1) <= 128)
 
     RYS_STRIDED, RYS_GROUPED = 0, 1

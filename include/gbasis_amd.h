@@ -175,7 +175,7 @@ int gbx_eri_coop_acc(const gbx_tables *t, const gbx_program *prog, int32_t la, i
                      int32_t parts, double *d_acc, int64_t acc_stride, void *d_scratch,
                      int64_t scratch_bytes, void *stream);
 
-/* Primitive pass by Rys quadrature for classes with 8 <= L <= 17 (n = L/2 + 1 <= 9
+/* Primitive pass by Rys quadrature for classes with 6 <= L <= 17 (n = L/2 + 1 <= 9
  * nodes), l_a + l_b <= 8, l_c + l_d <= 8, l_b <= 4, l_d <= 4, Coulomb operator, FP64:
  * work item (q, r) sums the primitive quartets [r c, (r + 1) c) of quartet q as sum over
  * nodes of pref W_u Ix Iy Iz and writes the partial contracted [e0|f0] to

@@ -354,7 +354,7 @@ def test_mixed_after_fp32_mask_changes(gpu):
 def test_rys_vs_cooperative(gpu):
     """The Rys primitive pass in both accumulator layouts, with the default LDS batch (up
     to 32 primitive quartets) and with batches of at most 8 (every class with
-    8 <= L <= 17 forced onto it, sub-quartet phases included), gives the cooperative
+    6 <= L <= 17 forced onto it, sub-quartet phases included), gives the cooperative
     kernel's packed array within 1e-12 of the largest value, pure and Cartesian shells
     up to g."""
     import torch
