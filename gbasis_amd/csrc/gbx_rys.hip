@@ -197,10 +197,40 @@ __device__ inline void rys_build(const gbx_tables& t, const RysArgs& a, int4 pb,
 // still shortens them by 20-40 % (it runs while the first builds its node tables).
 constexpr int kRysWaves = 4;
 
+// The nodes of one LDS batch for a group of exactly A x C members: straight-line table
+// reads and FMAs (no per-member branch, so the reads of a node are all in flight before
+// its FMAs wait on them).
+template <int A, int C, int MA, int MC>
+__device__ inline void rys_group_nodes(const double* rtab, int nodes, int TS, int W, int ox,
+                                       int oy, int oz0, double (&acc)[MA][MC]) {
+#pragma unroll 2
+  for (int pr = 0; pr < nodes; ++pr) {
+    const double* base = rtab + pr * 3 * TS;
+    const double xy = base[ox] * base[TS + oy];
+    const double* zb = base + 2 * TS + oz0;
+#pragma unroll
+    for (int i = 0; i < A; ++i)
+#pragma unroll
+      for (int j = 0; j < C; ++j) acc[i][j] = fma(xy, zb[i * W + j], acc[i][j]);
+  }
+}
+
+// Shape S = (alen - 1) MC + (clen - 1) of this wavefront's groups -> its exact loop.
+template <int MA, int MC, int S = 0>
+__device__ inline void rys_group_dispatch(int shape, const double* rtab, int nodes, int TS, int W,
+                                          int ox, int oy, int oz0, double (&acc)[MA][MC]) {
+  if constexpr (S < MA * MC) {
+    if (shape == S) {
+      rys_group_nodes<S / MC + 1, S % MC + 1, MA, MC>(rtab, nodes, TS, W, ox, oy, oz0, acc);
+      return;
+    }
+    rys_group_dispatch<MA, MC, S + 1>(shape, rtab, nodes, TS, W, ox, oy, oz0, acc);
+  }
+}
+
 template <int MA, int MC>
 __global__ __launch_bounds__(kRysThreads, kRysWaves) void eri_rys_grouped_kernel(gbx_tables t,
                                                                                 RysArgs a) {
-  constexpr int KG = 1;
   extern __shared__ __align__(16) double rtab[];
   const int tid = threadIdx.x;
   const int W = a.lcd + 1;
@@ -208,20 +238,19 @@ __global__ __launch_bounds__(kRysThreads, kRysWaves) void eri_rys_grouped_kernel
   const int NR = (a.lab + a.lcd) / 2 + 1;
   const int PB = a.pb;
   // the group of this lane: slot blockIdx.y 512 + tid
-  int slot[KG], g[KG], alen[KG], clen[KG], ox[KG], oy[KG], oz0[KG];
-#pragma unroll
-  for (int k = 0; k < KG; ++k) {
-    slot[k] = (blockIdx.y * KG + k) * kRysThreads + tid;
-    g[k] = slot[k] < a.nslots ? a.groups[slot[k]] : -1;
-    // shape of this wavefront's groups (the same for every lane of a wavefront)
-    const int gw = __builtin_amdgcn_readfirstlane(g[k]);
-    alen[k] = gw < 0 ? 0 : (gw >> 18) & 15;
-    clen[k] = gw < 0 ? 0 : (gw >> 26) & 15;
-    // an empty slot (g < 0) reads the first entries of the tables and writes nothing
-    ox[k] = g[k] < 0 ? 0 : g[k] & 127;
-    oy[k] = g[k] < 0 ? 0 : (g[k] >> 7) & 127;
-    oz0[k] = g[k] < 0 ? 0 : ((g[k] >> 14) & 15) * W + ((g[k] >> 22) & 15);
-  }
+  const int slot = blockIdx.y * kRysThreads + tid;
+  const int g = slot < a.nslots ? a.groups[slot] : -1;
+  // shape of this wavefront's groups (the same for every lane of a wavefront; a wavefront
+  // of empty slots has none)
+  const int gw = __builtin_amdgcn_readfirstlane(g);
+  const int alen = gw < 0 ? 0 : (gw >> 18) & 15;
+  const int clen = gw < 0 ? 0 : (gw >> 26) & 15;
+  const int shape = alen > 0 && clen > 0 ? (alen - 1) * MC + (clen - 1) : -1;
+  // an empty slot (g < 0) in a wavefront of groups reads the first entries of the tables
+  // and writes nothing
+  const int ox = g < 0 ? 0 : g & 127;
+  const int oy = g < 0 ? 0 : (g >> 7) & 127;
+  const int oz0 = g < 0 ? 0 : ((g >> 14) & 15) * W + ((g >> 22) & 15);
   const int4* pinfo = reinterpret_cast<const int4*>(t.pair_info);
   const int64_t nquart = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
   for (int64_t wi = blockIdx.x; wi < nquart * a.parts; wi += gridDim.x) {
@@ -235,50 +264,28 @@ __global__ __launch_bounds__(kRysThreads, kRysWaves) void eri_rys_grouped_kernel
     const int chunk = (nprimq + a.parts - 1) / a.parts;
     const int pbeg = min(nprimq, part * chunk);
     const int pend = min(nprimq, pbeg + chunk);
-    double acc[KG][MA][MC];
+    double acc[MA][MC];
 #pragma unroll
-    for (int k = 0; k < KG; ++k)
+    for (int i = 0; i < MA; ++i)
 #pragma unroll
-      for (int i = 0; i < MA; ++i)
-#pragma unroll
-        for (int j = 0; j < MC; ++j) acc[k][i][j] = 0.0;
+      for (int j = 0; j < MC; ++j) acc[i][j] = 0.0;
 
     for (int p0 = pbeg; p0 < pend; p0 += PB) {
       const int ncopy = min(PB, pend - p0);
       __syncthreads();  // the previous batch's tables are no longer read
       if (tid < PB * NR * 3) rys_build(t, a, pb, pk, p0, ncopy, NR, rtab);
       __syncthreads();
-      const int nodes = ncopy * NR;
-#pragma unroll
-      for (int k = 0; k < KG; ++k) {
-        if (alen[k] == 0) continue;
-        for (int pr = 0; pr < nodes; ++pr) {
-          const double* base = rtab + pr * 3 * TS;
-          const double xy = base[ox[k]] * base[TS + oy[k]];
-          const double* zb = base + 2 * TS + oz0[k];
-#pragma unroll
-          for (int i = 0; i < MA; ++i) {
-            if (i < alen[k]) {
-#pragma unroll
-              for (int j = 0; j < MC; ++j)
-                if (j < clen[k]) acc[k][i][j] = fma(xy, zb[i * W + j], acc[k][i][j]);
-            }
-          }
-        }
-      }
+      if (shape >= 0)
+        rys_group_dispatch<MA, MC>(shape, rtab, ncopy * NR, TS, W, ox, oy, oz0, acc);
     }
-#pragma unroll
-    for (int k = 0; k < KG; ++k) {
-      if (g[k] < 0) continue;
-      const int32_t* rows = a.groups + a.nslots + static_cast<int64_t>(slot[k]) * MA * MC;
+    if (g >= 0) {
+      const int32_t* rows = a.groups + a.nslots + static_cast<int64_t>(slot) * MA * MC;
 #pragma unroll
       for (int i = 0; i < MA; ++i)
 #pragma unroll
         for (int j = 0; j < MC; ++j) {
-          const int f = (i < alen[k] && j < clen[k]) ? rows[i * MC + j] : -1;
-          if (f >= 0)
-            a.acc_out[rys_acc_index(a, qi, part, f)] =
-                acc[k][i][j];
+          const int f = (i < alen && j < clen) ? rows[i * MC + j] : -1;
+          if (f >= 0) a.acc_out[rys_acc_index(a, qi, part, f)] = acc[i][j];
         }
     }
   }
