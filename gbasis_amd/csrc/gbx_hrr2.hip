@@ -61,13 +61,47 @@ __device__ inline uint64_t hrr2_tri(uint64_t i, uint64_t j) {
   return i >= j ? i * (i + 1) / 2 + j : j * (j + 1) / 2 + i;
 }
 
+// The flat index (r * A + a) * B + b of a [nr][A][B] loop, visited by a thread from
+// threadIdx.x in steps of kHrr2Threads: the steps are decomposed once, so the loops need
+// no integer division per element (B = 1 or A = 1 for two-level loops).
+struct Hrr2Iter {
+  int r, a, b;
+  int dr, da, db;
+  int A, B;
+  __device__ Hrr2Iter(int A_, int B_) : A(A_), B(B_) {
+    const int q = threadIdx.x / B;
+    b = threadIdx.x - q * B;
+    r = q / A;
+    a = q - r * A;
+    const int qs = kHrr2Threads / B;
+    db = kHrr2Threads - qs * B;
+    dr = qs / A;
+    da = qs - dr * A;
+  }
+  __device__ void next() {
+    b += db;
+    int carry = 0;
+    if (b >= B) {
+      b -= B;
+      carry = 1;
+    }
+    a += da + carry;
+    carry = 0;
+    if (a >= A) {
+      a -= A;
+      carry = 1;
+    }
+    r += dr + carry;
+  }
+};
+
 // One level of a recursion for rows [0, nr) of the row buffers:
 // dst[j] = src[s1] + g_dir src[s2], g = (gx, gy, gz).
 __device__ inline void hrr2_level(double* buf, int srow, int src_half, int dst_half, int half,
                                   int nr, int size, const int32_t* ent, double gx, double gy,
                                   double gz) {
-  for (int idx = threadIdx.x; idx < nr * size; idx += kHrr2Threads) {
-    const int r = idx / size, j = idx - r * size;
+  for (Hrr2Iter it(size, 1); it.r < nr; it.next()) {
+    const int r = it.r, j = it.a;
     const int e = ent[j];
     const int dir = e >> 24;
     const double g = dir == 0 ? gx : (dir == 1 ? gy : gz);
@@ -119,8 +153,8 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
     for (int e0 = 0; e0 < NA; e0 += a.rb) {
       const int nr = min(a.rb, NA - e0);
       __syncthreads();  // the buffers of the previous group (or quartet) are free
-      for (int idx = threadIdx.x; idx < nr * NC; idx += kHrr2Threads) {
-        const int r = idx / NC, c = idx - r * NC;
+      for (Hrr2Iter it(NC, 1); it.r < nr; it.next()) {
+        const int r = it.r, c = it.a;
         const int64_t f = static_cast<int64_t>(c) * NA + e0 + r;
         double s = 0.0;
         if (a.qmajor) {
@@ -145,12 +179,10 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       // of C into half 1 - h as [c'][d], then that of D into W
       const int h = nlk & 1;
       {
-        const int n = nr * dim[2] * nc[3];
         const double* M = t.out_mat + si1[2].x;
-        for (int idx = threadIdx.x; idx < n; idx += kHrr2Threads) {
-          const int r = idx / (dim[2] * nc[3]);
-          const int rem = idx - r * dim[2] * nc[3];
-          const int jc = rem / nc[3], id = rem - jc * nc[3];
+        for (Hrr2Iter it(dim[2], nc[3]); it.r < nr; it.next()) {
+          const int r = it.r, jc = it.a, id = it.b;
+          const int rem = jc * nc[3] + id;
           const double* F = buf + r * srow + h * a.half;
           double v;
           if (si0[2].z == 1) {
@@ -165,12 +197,10 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       }
       __syncthreads();
       {
-        const int n = nr * NJ;
         const double* M = t.out_mat + si1[3].x;
-        for (int idx = threadIdx.x; idx < n; idx += kHrr2Threads) {
-          const int r = idx / NJ;
-          const int j = idx - r * NJ;
-          const int jc = j / dim[3], jd = j - jc * dim[3];
+        for (Hrr2Iter it(dim[2], dim[3]); it.r < nr; it.next()) {
+          const int r = it.r, jc = it.a, jd = it.b;
+          const int j = jc * dim[3] + jd;
           const double* T = buf + r * srow + (1 - h) * a.half + jc * nc[3];
           double v;
           if (si0[3].z == 1) {
@@ -191,8 +221,8 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
     for (int j0 = 0; j0 < NJ; j0 += a.rb) {
       const int nr = min(a.rb, NJ - j0);
       __syncthreads();  // W is complete; the buffers of the previous group are free
-      for (int idx = threadIdx.x; idx < nr * NA; idx += kHrr2Threads) {
-        const int e = idx / nr, r = idx - e * nr;
+      for (Hrr2Iter it(nr, 1); it.r < NA; it.next()) {
+        const int e = it.r, r = it.a;
         buf[r * srow + e] = W[static_cast<int64_t>(e) * NJ + j0 + r];
       }
       __syncthreads();
@@ -204,12 +234,10 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       }
       const int h = nlb & 1;
       {
-        const int n = nr * dim[0] * nc[1];
         const double* M = t.out_mat + si1[0].x;
-        for (int idx = threadIdx.x; idx < n; idx += kHrr2Threads) {
-          const int r = idx / (dim[0] * nc[1]);
-          const int rem = idx - r * dim[0] * nc[1];
-          const int ja = rem / nc[1], ib = rem - ja * nc[1];
+        for (Hrr2Iter it(dim[0], nc[1]); it.r < nr; it.next()) {
+          const int r = it.r, ja = it.a, ib = it.b;
+          const int rem = ja * nc[1] + ib;
           const double* F = buf + r * srow + h * a.half;
           double v;
           if (si0[0].z == 1) {
@@ -224,12 +252,15 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       }
       __syncthreads();
       {
-        const int nab = dim[0] * dim[1];
         const double* M = t.out_mat + si1[1].x;
-        for (int idx = threadIdx.x; idx < nr * nab; idx += kHrr2Threads) {
-          const int r = idx / nab;
-          const int rem = idx - r * nab;
-          const int ja = rem / dim[1], jb = rem - ja * dim[1];
+        int rr = -1, jc = 0, jd = 0;
+        for (Hrr2Iter it(dim[0], dim[1]); it.r < nr; it.next()) {
+          const int r = it.r, ja = it.a, jb = it.b;
+          if (r != rr) {  // the column (c', d') of this row group
+            rr = r;
+            jc = (j0 + r) / dim[3];
+            jd = j0 + r - jc * dim[3];
+          }
           const double* T = buf + r * srow + (1 - h) * a.half + ja * nc[1];
           double v;
           if (si0[1].z == 1) {
@@ -238,8 +269,6 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
           } else {
             v = T[jb];
           }
-          const int j = j0 + r;
-          const int jc = j / dim[3], jd = j - jc * dim[3];
           const int pos[4] = {ja, jb, jc, jd};
           uint32_t fn[4];
           for (int x = 0; x < 4; ++x) {
