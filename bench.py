@@ -458,6 +458,40 @@ CONFIG_WORKLOADS = {
                     "[0.02,1e5], Schwarz 1e-12, packed output", 1e-12, False),
 }
 SAMPLES_FILE = os.path.join(ROOT, "tests", "golden", "config_samples.npz")
+# workloads whose packed ERIs also go through the four-index transform (square, seeded
+# transform): the dense new-basis array is nfunc^4 doubles (4 GB at N = 150)
+MO_WORKLOADS = ("high_l_pure",)
+
+
+def time_mo_transform(eng, packed, nfunc, reps=3):
+    """Milliseconds and FP64 rate of the four-index transform of the packed ERIs into a
+    basis of nfunc functions (gbx_mo_transform: two passes of Y = T X T^T over n x n
+    matrices, n^2 m + n m^2 / 2 FMAs per matrix; npair(n) matrices in the first pass,
+    npair(m) in the second)."""
+    import torch
+
+    from gbasis_amd.roofline import PEAK_FP64_VECTOR
+
+    rng = np.random.default_rng(11)
+    T = rng.standard_normal((nfunc, nfunc)) / np.sqrt(nfunc)
+    out = eng.mo_transform(packed, nfunc, T, "chemist")
+    del out
+    torch.cuda.synchronize()
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(2)]
+    ev[0].record()
+    for _ in range(reps):
+        out = eng.mo_transform(packed, nfunc, T, "chemist")
+        del out
+    ev[1].record()
+    torch.cuda.synchronize()
+    ms = ev[0].elapsed_time(ev[1]) / reps
+    n = m = nfunc
+    npn, npm = n * (n + 1) // 2, m * (m + 1) // 2
+    flops = 2.0 * (npn + npm) * (n * n * m + n * m * m / 2)
+    log(f"  four-index transform N={n}: {ms:.2f} ms, {flops / ms / 1e9:.2f} TFLOP/s")
+    return {"n": n, "m": m, "ms": ms, "flops": flops, "tflops": flops / ms / 1e9,
+            "frac": flops / ms / 1e9 / (PEAK_FP64_VECTOR / 1e12),
+            "includes": "workspace and output allocation (torch caching allocator)"}
 TUNE_FILE = os.path.join(ROOT, "gbasis_amd", "tuning_mi355x.json")
 
 
@@ -658,6 +692,7 @@ def run_workload(name, args, tuner=None):
         torch.cuda.synchronize()
         unpack_ms = ev[0].elapsed_time(ev[1]) / args.steps
         del d
+    mo = time_mo_transform(eng, out64, mol.nfunc) if name in MO_WORKLOADS else None
     dev.out = torch.zeros(batch.n_slots, dtype=torch.float64, device="cuda")
     tmx, qmx, _, pbmx, lastmx = time_steps(eng, dev, bound, thr, args.fp32_cutoff, args.steps,
                                            args.warmup, tuner)
@@ -697,6 +732,7 @@ def run_workload(name, args, tuner=None):
         "schwarz_ms": schwarz_ms,
         "fp32_quartet_fraction": fp32_fraction(plansmx),
         "dense_unpack_ms": unpack_ms,
+        "mo_transform": mo,
         "mixed_vs_fp64": {"max_abs_err": max_err, "rms_err": rms_err, "per_class": per_cls},
         "oracle_samples": samples,
         "roofline": roofline_entry(last64, pb64, prefer_reg=False),

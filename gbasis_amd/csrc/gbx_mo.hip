@@ -6,17 +6,24 @@
 // (gbasis/base_four_symm.py:559-616, four tensordot passes over the dense array in
 // the reference).  Both halves are the same operation on a batch of symmetric
 // matrices, Y = T X T^T with X symmetric n x n and T m x n:
-//   pass 1, one matrix per AO pair ij (i >= j): X_ij[k][l] = (ij|kl) gathered from the
-//     packed array; its lower triangle Y[c][d] (c >= d) goes to the half-transformed
-//     array H[cd][ij] (Mpair x Npair, Npair = n(n+1)/2, Mpair = m(m+1)/2);
-//   pass 2, one matrix per new pair cd: X_cd[i][j] = H[cd][tri(i, j)]; its lower
-//     triangle gives (ab|cd) for a >= b, written to the four dense positions of
+//   pass 1, one matrix per AO pair r = ij (i >= j): X_r[k][l] = (ij|kl) from the packed
+//     array; the lower triangle Y_r[c][d] (c >= d) goes to the half-transformed array
+//     H[cd][ij] (Mpair x Npair, Npair = n(n+1)/2, Mpair = m(m+1)/2);
+//   pass 2, one matrix per new pair r = cd: X_r[i][j] = H[cd][tri(i, j)]; the lower
+//     triangle of Y_r gives (ab|cd) for a >= b, written to the four dense positions of
 //     (ab|cd) = (ba|cd) = (ab|dc) = (ba|dc) in chemist or physicist order.
-// A workgroup owns one matrix and a tile of kMoTC rows c of Y: V = T[c-tile] X is
-// accumulated over kMoKC-row chunks of X staged in LDS (each element of X is
-// gathered once per tile), then Y[c][d] = sum_l V[c][l] T[d][l] with T^T read
-// coalesced.  FP64 vector FMAs throughout (FP64 MFMA has the same peak on
-// gfx950); the passes cost n^2 m + n m^2 / 2 FMAs per matrix.
+// Each pass runs, over chunks of rc matrices, as two GEMMs on the FP64 matrix cores:
+//   GEMM 1, rows (k, q):  U[k][d][q] = sum_l X_q[k][l] T[d][l]   (rows gathered from
+//           the packed array or H);
+//   GEMM 2, rows (d, q):  Z[d][q][c] = sum_k U[k][d][q] T[c][k] = Y_q[c][d],
+//           whose epilogue keeps c >= d and places it in H or the dense output.
+// The matrix q of the chunk is the fastest index of both GEMMs' rows, so that the
+// U traffic, the H rows and the dense output (whose last index is the pair's second
+// function) are written and read along consecutive addresses.
+// The GEMM tile: 64 x 64 outputs per 256-lane workgroup, K in steps of 16 staged in
+// LDS, each wavefront a 32 x 32 block of four 16 x 16 v_mfma_f64_16x16x4f64
+// accumulators.  Operand and result lanes of that instruction: A[i][k] in lane
+// i + 16 k, B[k][j] in lane j + 16 k, C[i][j] in lane j + 16 (i mod 4), register i / 4.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -28,16 +35,23 @@
 namespace gbx {
 
 constexpr int kMoThreads = 256;
-constexpr int kMoTC = 16;   // rows of Y per workgroup
-constexpr int kMoKC = 16;   // rows of X per LDS chunk
+constexpr int kMoBM = 64, kMoBN = 64, kMoBK = 16;
+constexpr int kMoPadA = kMoBK + 1;   // LDS row strides in doubles, odd against bank
+constexpr int kMoPadB = kMoBN + 1;   // conflicts of the fragment reads
+constexpr int64_t kMoChunkDoubles = int64_t(1) << 27;  // U of one chunk: at most 1 GiB
+constexpr int kMoXcds = 8;           // workgroups are dealt round-robin to the XCDs
+
+typedef double mo_v4d __attribute__((ext_vector_type(4)));
 
 struct MoArgs {
   const double* packed;  // pass 1 source
   double* half;          // H: pass 1 output, pass 2 source
   double* out;           // dense m^4 output
-  const double* T;       // m x n, row-major
   const double* TT;      // n x m (T transposed), row-major
+  double* u;             // U of the current chunk: [k][d][q]
   int64_t n, m, npair_n;
+  int64_t r0, rc;        // matrices [r0, r0 + rc) of this chunk
+  int64_t tiles;         // row tiles x column tiles of the GEMM
   int notation;          // 0 chemist, 1 physicist
 };
 
@@ -54,129 +68,214 @@ __device__ inline void mo_tri_split(uint64_t t, uint64_t& r, uint64_t& c) {
   c = t - x * (x + 1) / 2;
 }
 
-template <int kPass>
-__device__ inline double mo_source(const MoArgs& a, uint64_t r, uint64_t k, uint64_t l) {
-  if constexpr (kPass == 1) return a.packed[mo_tri(r, mo_tri(k, l))];
-  else return a.half[r * static_cast<uint64_t>(a.npair_n) + mo_tri(k, l)];
-}
-
-template <int kPass>
-__global__ __launch_bounds__(kMoThreads) void mo_sandwich_kernel(MoArgs a) {
-  extern __shared__ __align__(16) char smem[];
-  const int64_t n = a.n, m = a.m;
-  double* V = reinterpret_cast<double*>(smem);  // kMoTC x n
-  double* Xs = V + kMoTC * n;                   // kMoKC x n
-  double* Ts = Xs + kMoKC * n;                  // kMoTC x kMoKC
-  const uint64_t r = blockIdx.x;
-  const int64_t c0 = static_cast<int64_t>(blockIdx.y) * kMoTC;
-  const int tc = static_cast<int>(std::min<int64_t>(kMoTC, m - c0));
+// One GEMM of a pass (kStage 1 or 2); K = n and N = m for both, M = n rc (stage 1) or
+// m rc (stage 2).  The grid is 1-D, padded to a multiple of the XCD count; the logical
+// tile of a workgroup makes the consecutive workgroups of one XCD neighbours along the
+// columns, so that they share A rows in that XCD's L2.
+template <int kPass, int kStage>
+__global__ __launch_bounds__(kMoThreads) void mo_gemm_kernel(MoArgs a) {
+  __shared__ double As[kMoBM * kMoPadA];
+  __shared__ double Bs[kMoBK * kMoPadB];
+  const int64_t n = a.n, m = a.m, rc = a.rc;
+  const int64_t M = rc * (kStage == 1 ? n : m);
+  const int64_t per_xcd = (a.tiles + kMoXcds - 1) / kMoXcds;
+  const int64_t tile = (blockIdx.x % kMoXcds) * per_xcd + blockIdx.x / kMoXcds;
+  if (tile >= a.tiles) return;  // the whole workgroup leaves before any barrier
+  const int64_t col_tiles = (m + kMoBN - 1) / kMoBN;
+  const int64_t row0 = (tile / col_tiles) * kMoBM, col0 = (tile % col_tiles) * kMoBN;
   const int tid = threadIdx.x;
-  for (int64_t e = tid; e < kMoTC * n; e += kMoThreads) V[e] = 0.0;
-  for (int64_t k0 = 0; k0 < n; k0 += kMoKC) {
-    const int kc = static_cast<int>(std::min<int64_t>(kMoKC, n - k0));
-    __syncthreads();  // previous chunk consumed (and V zeroed)
-    for (int64_t e = tid; e < kc * n; e += kMoThreads)
-      Xs[e] = mo_source<kPass>(a, r, static_cast<uint64_t>(k0 + e / n),
-                               static_cast<uint64_t>(e % n));
-    for (int e = tid; e < kMoTC * kMoKC; e += kMoThreads) {
-      const int c = e / kMoKC, kk = e % kMoKC;
-      Ts[e] = (c < tc && kk < kc) ? a.T[(c0 + c) * n + k0 + kk] : 0.0;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  // A loader.  Stage 1: 16 lanes along l in each of the rows tid / 16 + 16 u, row
+  // (k, q) = X_{r0+q}[k][:].  Stage 2: the 64 lanes along the rows (d, q), which are
+  // consecutive in U, K columns 4 (tid / 64) .. + 3.
+  bool a_ok[4];
+  uint64_t a_r[4], a_k[4];  // stage 1: matrix and row index per loaded row
+  int64_t a_base = 0;       // stage 2: offset of U[0][d][q]
+  if constexpr (kStage == 1) {
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t row = row0 + (tid >> 4) + 16 * u;
+      a_ok[u] = row < M;
+      const int64_t k = a_ok[u] ? row / rc : 0;
+      a_k[u] = static_cast<uint64_t>(k);
+      a_r[u] = static_cast<uint64_t>(a.r0 + (a_ok[u] ? row - k * rc : 0));
+    }
+  } else {
+    const int64_t row = row0 + lane;
+    a_ok[0] = row < M;
+    a_base = row;  // U[k][d][q] = u[(k m + d) rc + q] = u[k m rc + row]
+  }
+  const int b_col = tid & 63, b_row = (tid >> 6) * 4;
+  const bool b_ok = col0 + b_col < m;
+
+  mo_v4d acc[2][2];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) acc[x][y] = mo_v4d{0.0, 0.0, 0.0, 0.0};
+
+  for (int64_t k0 = 0; k0 < n; k0 += kMoBK) {
+    __syncthreads();  // the previous K step's tiles are consumed
+    if constexpr (kStage == 1) {
+      const int64_t l = k0 + (tid & 15);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        double v = 0.0;
+        if (a_ok[u] && l < n) {
+          const uint64_t kl = mo_tri(a_k[u], static_cast<uint64_t>(l));
+          if constexpr (kPass == 1)
+            v = a.packed[mo_tri(a_r[u], kl)];
+          else
+            v = a.half[a_r[u] * static_cast<uint64_t>(a.npair_n) + kl];
+        }
+        As[((tid >> 4) + 16 * u) * kMoPadA + (tid & 15)] = v;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const int64_t k = k0 + (tid >> 6) * 4 + u;
+        As[lane * kMoPadA + (tid >> 6) * 4 + u] =
+            (a_ok[0] && k < n) ? a.u[k * m * rc + a_base] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int64_t k = k0 + b_row + u;
+      Bs[(b_row + u) * kMoPadB + b_col] = (b_ok && k < n) ? a.TT[k * m + col0 + b_col] : 0.0;
     }
     __syncthreads();
-    for (int64_t e = tid; e < tc * n; e += kMoThreads) {
-      const int c = static_cast<int>(e / n);
-      const int64_t l = e % n;
-      double s = V[e];
-      for (int kk = 0; kk < kc; ++kk) s += Ts[c * kMoKC + kk] * Xs[kk * n + l];
-      V[e] = s;
+#pragma unroll
+    for (int ks = 0; ks < kMoBK; ks += 4) {
+      double af[2], bf[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+        af[x] = As[(wm + x * 16 + (lane & 15)) * kMoPadA + ks + (lane >> 4)];
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+        bf[y] = Bs[(ks + (lane >> 4)) * kMoPadB + wn + y * 16 + (lane & 15)];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y)
+          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
     }
   }
-  __syncthreads();
-  uint64_t pc = 0, pd = 0;
-  if constexpr (kPass == 2) mo_tri_split(r, pc, pd);
-  // lower triangle of the tile: Y[c][d], d <= c
-  for (int64_t e = tid; e < tc * m; e += kMoThreads) {
-    const int c = static_cast<int>(e / m);
-    const int64_t d = e % m;
-    const int64_t cg = c0 + c;
-    if (d > cg) continue;
-    const double* vr = V + c * n;
-    double s = 0.0;
-    for (int64_t l = 0; l < n; ++l) s += vr[l] * a.TT[l * m + d];
-    if constexpr (kPass == 1) {
-      a.half[mo_tri(cg, d) * static_cast<uint64_t>(a.npair_n) + r] = s;
-    } else {
-      // (ab|cd) with (a, b) = (cg, d) and (c, d) = (pc, pd)
-      const uint64_t M = m, A = cg, B = d;
-      auto put = [&](uint64_t i, uint64_t j, uint64_t k, uint64_t l) {
-        // chemist (ij|kl) at [i][j][k][l]; physicist <ik|jl> = (ij|kl) at [i][k][j][l]
-        const uint64_t idx = a.notation == 0 ? ((i * M + j) * M + k) * M + l
-                                             : ((i * M + k) * M + j) * M + l;
-        a.out[idx] = s;
-      };
-      put(A, B, pc, pd);
-      put(B, A, pc, pd);
-      put(A, B, pd, pc);
-      put(B, A, pd, pc);
+
+  // epilogue: element (i, j) of a 16 x 16 tile sits in lane j + 16 (i mod 4), register i / 4
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int reg = 0; reg < 4; ++reg) {
+      const int64_t row = row0 + wm + x * 16 + (lane >> 4) + 4 * reg;
+      if (row >= M) continue;
+      const int64_t i = row / rc;       // k (stage 1) or d (stage 2)
+      const int64_t q = row - i * rc;
+#pragma unroll
+      for (int y = 0; y < 2; ++y) {
+        const int64_t col = col0 + wn + y * 16 + (lane & 15);  // d (stage 1) or c (stage 2)
+        if (col >= m) continue;
+        const double v = acc[x][y][reg];
+        if constexpr (kStage == 1) {
+          a.u[(i * m + col) * rc + q] = v;
+        } else {
+          const int64_t c = col, d = i;
+          if (c < d) continue;  // Y_q is symmetric: the lower triangle only
+          const uint64_t r = static_cast<uint64_t>(a.r0 + q);
+          if constexpr (kPass == 1) {
+            a.half[mo_tri(c, d) * static_cast<uint64_t>(a.npair_n) + r] = v;
+          } else {
+            // (ab|ef) with (a, b) = (c, d) of Y_q and (e, f) the new pair r
+            uint64_t e = 0, f = 0;
+            mo_tri_split(r, e, f);
+            const uint64_t mm = m, A = c, B = d;
+            auto put = [&](uint64_t i0, uint64_t j0, uint64_t k0, uint64_t l0) {
+              // chemist (ij|kl) at [i][j][k][l]; physicist <ik|jl> = (ij|kl) at [i][k][j][l]
+              const uint64_t idx = a.notation == 0 ? ((i0 * mm + j0) * mm + k0) * mm + l0
+                                                   : ((i0 * mm + k0) * mm + j0) * mm + l0;
+              a.out[idx] = v;
+            };
+            put(A, B, e, f);
+            put(B, A, e, f);
+            put(A, B, f, e);
+            put(B, A, f, e);
+          }
+        }
+      }
     }
-  }
 }
 
-size_t mo_lds_bytes(int64_t n) {
-  return static_cast<size_t>(2 * kMoTC * n + kMoTC * kMoKC) * sizeof(double);
+// Matrices per chunk: their U takes at most kMoChunkDoubles.
+inline int64_t mo_chunk(int64_t n, int64_t m, int64_t nmat) {
+  return std::max<int64_t>(1, std::min<int64_t>(nmat, kMoChunkDoubles / (n * m)));
+}
+
+template <int kPass, int kStage>
+int mo_gemm(MoArgs a, hipStream_t s) {
+  const int64_t rows = a.rc * (kStage == 1 ? a.n : a.m);
+  a.tiles = ((rows + kMoBM - 1) / kMoBM) * ((a.m + kMoBN - 1) / kMoBN);
+  const int64_t blocks = (a.tiles + kMoXcds - 1) / kMoXcds * kMoXcds;
+  hipLaunchKernelGGL((mo_gemm_kernel<kPass, kStage>), dim3(static_cast<unsigned>(blocks)),
+                     dim3(kMoThreads), 0, s, a);
+  return check_launch(kStage == 1 ? "mo_gemm_kernel (stage 1)" : "mo_gemm_kernel (stage 2)");
 }
 
 template <int kPass>
-int mo_launch(const MoArgs& a, uint64_t nmat, int lds_limit, int dev, hipStream_t s) {
-  const size_t shmem = mo_lds_bytes(a.n);
-  if (shmem > 65536) {
-    static bool raised[64] = {};  // per device: the attribute is per-device state
-    if (dev < 0 || dev >= 64 || !raised[dev]) {
-      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&mo_sandwich_kernel<kPass>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize,
-                              lds_limit) != hipSuccess)
-        return fail(GBX_ERR_HIP, "gbx_mo_transform: could not raise the LDS limit");
-      if (dev >= 0 && dev < 64) raised[dev] = true;
-    }
+int mo_pass(MoArgs a, int64_t nmat, hipStream_t s) {
+  const int64_t rc_max = mo_chunk(a.n, a.m, nmat);
+  for (int64_t r0 = 0; r0 < nmat; r0 += rc_max) {
+    a.r0 = r0;
+    a.rc = std::min(rc_max, nmat - r0);
+    if (int rc = mo_gemm<kPass, 1>(a, s)) return rc;
+    if (int rc = mo_gemm<kPass, 2>(a, s)) return rc;
   }
-  const dim3 grid(static_cast<unsigned>(nmat), static_cast<unsigned>((a.m + kMoTC - 1) / kMoTC));
-  hipLaunchKernelGGL(mo_sandwich_kernel<kPass>, grid, dim3(kMoThreads), shmem, s, a);
-  return check_launch(kPass == 1 ? "mo_sandwich_kernel<1>" : "mo_sandwich_kernel<2>");
+  return GBX_OK;
+}
+
+// Doubles of the workspace: H, then the U of one chunk (the larger of the two passes').
+inline int64_t mo_workspace_doubles(int64_t n, int64_t m, int64_t* half_doubles) {
+  const int64_t npn = n * (n + 1) / 2, npm = m * (m + 1) / 2;
+  const int64_t u1 = mo_chunk(n, m, npn) * n * m, u2 = mo_chunk(n, m, npm) * n * m;
+  if (half_doubles) *half_doubles = npn * npm;
+  return npn * npm + std::max(u1, u2);
 }
 
 }  // namespace gbx
 
 using namespace gbx;
 
-extern "C" int gbx_mo_workspace(int64_t nfunc, int64_t nnew, int64_t* half_elems) {
-  if (!half_elems) return fail(GBX_ERR_ARG, "gbx_mo_workspace: null argument");
-  if (nfunc < 1 || nnew < 1) return fail(GBX_ERR_ARG, "gbx_mo_workspace: sizes must be >= 1");
-  *half_elems = (nfunc * (nfunc + 1) / 2) * (nnew * (nnew + 1) / 2);
+extern "C" int gbx_mo_workspace(int64_t nfunc, int64_t nnew, int64_t* workspace_elems) {
+  if (!workspace_elems) return fail(GBX_ERR_ARG, "gbx_mo_workspace: null argument");
+  if (nfunc < 1 || nnew < 1 || nfunc > 32768 || nnew > 32768)
+    return fail(GBX_ERR_ARG, "gbx_mo_workspace: sizes must be in [1, 32768]");
+  *workspace_elems = mo_workspace_doubles(nfunc, nnew, nullptr);
   return GBX_OK;
 }
 
 extern "C" int gbx_mo_transform(const double* d_packed, int64_t nfunc, const double* d_T,
-                                const double* d_TT, int64_t nnew, double* d_half,
+                                const double* d_TT, int64_t nnew, double* d_work,
                                 double* d_out, int32_t notation, void* stream) {
-  if (!d_packed || !d_T || !d_TT || !d_half || !d_out)
+  if (!d_packed || !d_T || !d_TT || !d_work || !d_out)
     return fail(GBX_ERR_ARG, "gbx_mo_transform: null argument");
   if (notation != 0 && notation != 1)
     return fail(GBX_ERR_ARG, "gbx_mo_transform: notation must be 0 (chemist) or 1 (physicist)");
-  // the pair counts index the grid (< 2^31) and every dense index stays below 2^63
+  // rows of one chunk stay below 2^27 (or one matrix's n): the 1-D grids stay in range
   if (nfunc < 1 || nfunc > 32768 || nnew < 1 || nnew > 32768)
     return fail(GBX_ERR_ARG, "gbx_mo_transform: nfunc and nnew must be in [1, 32768]");
-  const int64_t npair_n = nfunc * (nfunc + 1) / 2, npair_m = nnew * (nnew + 1) / 2;
-  if (npair_n >= (int64_t(1) << 31) || npair_m >= (int64_t(1) << 31))
-    return fail(GBX_ERR_ARG, "gbx_mo_transform: too many pairs for one grid");
-  int dev = 0, lds = 0;
-  if (hipGetDevice(&dev) != hipSuccess ||
-      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-    return fail(GBX_ERR_HIP, "gbx_mo_transform: device query failed");
-  if (mo_lds_bytes(nfunc) > static_cast<size_t>(lds))
-    return fail(GBX_ERR_ARG, "gbx_mo_transform: nfunc too large for the LDS tiles");
+  int64_t half_doubles = 0;
+  mo_workspace_doubles(nfunc, nnew, &half_doubles);
   hipStream_t s = static_cast<hipStream_t>(stream);
-  MoArgs a{d_packed, d_half, d_out, d_T, d_TT, nfunc, nnew, npair_n, notation};
-  int rc = mo_launch<1>(a, static_cast<uint64_t>(npair_n), lds, dev, s);
-  if (rc != GBX_OK) return rc;
-  return mo_launch<2>(a, static_cast<uint64_t>(npair_m), lds, dev, s);
+  MoArgs a{};
+  a.packed = d_packed;
+  a.half = d_work;
+  a.out = d_out;
+  a.TT = d_TT;
+  a.u = d_work + half_doubles;
+  a.n = nfunc;
+  a.m = nnew;
+  a.npair_n = nfunc * (nfunc + 1) / 2;
+  a.notation = notation;
+  if (int rc = mo_pass<1>(a, a.npair_n, s)) return rc;
+  return mo_pass<2>(a, nnew * (nnew + 1) / 2, s);
 }

@@ -1027,19 +1027,19 @@ class EriEngine:
     def mo_transform(self, packed, nfunc, transform, notation):
         """Dense (m, m, m, m) device array of sum_ijkl T_ai T_bj T_ck T_dl (ij|kl) from the
         packed ERIs (``transform``: m x nfunc), in chemist or physicist order; the
-        half-transformed workspace (nfunc(nfunc+1)/2 x m(m+1)/2) is the only
-        intermediate."""
+        workspace (the half-transformed nfunc(nfunc+1)/2 x m(m+1)/2 array and one
+        chunk of GEMM staging) is the only intermediate."""
         tr = torch.as_tensor(np.ascontiguousarray(transform, dtype=np.float64),
                              device=self.device)
         if tr.ndim != 2 or tr.shape[1] != nfunc:
             raise ValueError(f"transform must have shape (m, {nfunc}), got {tuple(tr.shape)}")
         m = int(tr.shape[0])
         trt = tr.t().contiguous()
-        n_half = ctypes.c_int64()
-        check(self.lib.gbx_mo_workspace(nfunc, m, ctypes.byref(n_half)), "gbx_mo_workspace")
-        half = torch.empty(n_half.value, dtype=torch.float64, device=self.device)
+        n_work = ctypes.c_int64()
+        check(self.lib.gbx_mo_workspace(nfunc, m, ctypes.byref(n_work)), "gbx_mo_workspace")
+        work = torch.empty(n_work.value, dtype=torch.float64, device=self.device)
         out = torch.empty((m,) * 4, dtype=torch.float64, device=self.device)
-        check(self.lib.gbx_mo_transform(ptr(packed), nfunc, ptr(tr), ptr(trt), m, ptr(half),
+        check(self.lib.gbx_mo_transform(ptr(packed), nfunc, ptr(tr), ptr(trt), m, ptr(work),
                                         ptr(out), 1 if notation == "physicist" else 0,
                                         self._stream()), "gbx_mo_transform")
         return out

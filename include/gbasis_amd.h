@@ -349,24 +349,25 @@ int gbx_unpack_dense(const double *d_packed, int64_t nfunc, int32_t notation, do
 int gbx_jk_packed(const double *d_packed, int64_t nfunc, const double *d_dens, double *d_J,
                   double *d_K, void *stream);
 
-/* Size (in doubles) of the half-transformed workspace gbx_mo_transform needs for
- * nfunc basis functions transformed to nnew new functions:
- * nfunc(nfunc+1)/2 * nnew(nnew+1)/2. */
-int gbx_mo_workspace(int64_t nfunc, int64_t nnew, int64_t *half_elems);
+/* Size (in doubles) of the workspace gbx_mo_transform needs for nfunc basis
+ * functions transformed to nnew new functions: the half-transformed array,
+ * nfunc(nfunc+1)/2 * nnew(nnew+1)/2, followed by the first GEMM's output for one
+ * chunk of matrices (at most max(2^27, nfunc * nnew) doubles). */
+int gbx_mo_workspace(int64_t nfunc, int64_t nnew, int64_t *workspace_elems);
 
 /* Four-index transform of one molecule's canonical packed ERIs into a new basis,
  * (ab|cd) = sum_ijkl T_ai T_bj T_ck T_dl (ij|kl), written to the dense nnew^4
  * array d_out in chemist (notation 0) or physicist (notation 1) order.  d_T is the
- * row-major nnew x nfunc transform and d_TT its transpose (nfunc x nnew); d_half is
+ * row-major nnew x nfunc transform and d_TT its transpose (nfunc x nnew); d_work is
  * a caller-owned workspace of gbx_mo_workspace doubles (the half-transformed
- * array, indexed [cd][ij]).  No dense nfunc^4 array is formed.  nfunc and nnew in
- * [1, 32768] and nfunc small enough for the LDS tiles (about 600 on gfx950).
+ * array, indexed [cd][ij], then GEMM staging).  No dense nfunc^4 array is formed;
+ * both halves run as GEMMs on the FP64 matrix cores.  nfunc and nnew in [1, 32768].
  * Replaces the four tensordot passes of
  * BaseFourIndexSymmetric.construct_array_lincomb (gbasis/base_four_symm.py:559-616)
  * that electron_repulsion_integral(..., transform=T) runs
  * (gbasis/integrals/electron_repulsion.py:270-271). */
 int gbx_mo_transform(const double *d_packed, int64_t nfunc, const double *d_T,
-                     const double *d_TT, int64_t nnew, double *d_half, double *d_out,
+                     const double *d_TT, int64_t nnew, double *d_work, double *d_out,
                      int32_t notation, void *stream);
 
 #ifdef __cplusplus

@@ -4,8 +4,9 @@ CPU: argument validation and the workspace size of the C ABI.
 GPU: the two-pass transform against einsum of the dense array with the transform,
 (ab|cd) = sum_ijkl T_ai T_bj T_ck T_dl (ij|kl) (gbasis/base_four_symm.py:611-615), on
 random 8-fold-symmetric arrays (square, narrowing and widening transforms, sizes that
-cover several LDS chunks and row tiles, both notations) and on a real basis against
-the oracle's dense ERIs.
+cover several K steps and row and column tiles of the GEMMs, both notations), on a
+real basis against the oracle's dense ERIs, and at a size whose first pass runs in two
+chunks against FP64 tensordots of the unpacked array on the device.
 """
 import numpy as np
 import pytest
@@ -23,7 +24,11 @@ def test_mo_transform_argument_errors_without_device():
     assert b"null" in lib.gbx_last_error()
     n = ctypes.c_int64()
     assert lib.gbx_mo_workspace(5, 3, ctypes.byref(n)) == 0
-    assert n.value == 15 * 6
+    # the half-transformed array (15 x 6 pairs), then the larger chunk of GEMM staging:
+    # all 15 AO-pair matrices of the first pass at 5 x 3 doubles each
+    assert n.value == 15 * 6 + 15 * 5 * 3
+    assert lib.gbx_mo_workspace(400, 400, ctypes.byref(n)) == 0
+    assert n.value == (400 * 401 // 2) ** 2 + (2 ** 27 // 160000) * 160000
     assert lib.gbx_mo_workspace(0, 3, ctypes.byref(n)) == -1
 
 
@@ -33,7 +38,7 @@ def _reference(eri, T, notation):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("n,m", [(1, 1), (6, 4), (20, 20), (37, 40)])
+@pytest.mark.parametrize("n,m", [(1, 1), (6, 4), (20, 20), (37, 40), (66, 70)])
 @pytest.mark.parametrize("notation", ["chemist", "physicist"])
 def test_mo_transform_random_vs_einsum(gpu, n, m, notation):
     import torch
@@ -65,3 +70,33 @@ def test_mo_transform_basis_vs_oracle(gpu):
     for notation in ("chemist", "physicist"):
         got = electron_repulsion_integral(basis, transform=T, notation=notation)
         assert np.abs(got - _reference(ref, T, notation)).max() <= 1e-11
+
+
+@pytest.mark.gpu
+def test_mo_transform_chunked_vs_tensordot(gpu):
+    """n = m = 130: the first pass's 8515 matrices of 130 x 130 outnumber one chunk of
+    GEMM staging (2^27 doubles), so the chunk loop runs twice."""
+    import torch
+
+    from gbasis_amd.engine import get_engine
+
+    print("[gpu case] mo transform chunked n=m=130", flush=True)
+    n = m = 130
+    eng = get_engine("cuda")
+    npair = n * (n + 1) // 2
+    g = torch.Generator(device=gpu).manual_seed(5)
+    # any packed vector is the canonical part of an 8-fold-symmetric array
+    packed = torch.rand(npair * (npair + 1) // 2, dtype=torch.float64, device=gpu,
+                        generator=g) - 0.5
+    T = np.random.default_rng(9).standard_normal((m, n)) / np.sqrt(n)
+    got = eng.mo_transform(packed, n, T, "chemist")
+    dense = eng.unpack_dense(packed, n, "chemist")
+    Td = torch.as_tensor(T, device=gpu)
+    ref = dense
+    for _ in range(4):  # contract the leading AO index, rotating it to the back
+        ref = torch.tensordot(ref, Td, dims=([0], [1]))
+    del dense
+    assert torch.isfinite(got).all()
+    # |T| row sums ~ sqrt(n) * 0.8: a bound on the magnitude of each output element
+    scale = 0.5 * float(np.abs(T).sum(axis=1).max()) ** 4
+    assert float((got - ref).abs().max()) <= 1e-13 * scale
