@@ -100,3 +100,29 @@ def test_mo_transform_chunked_vs_tensordot(gpu):
     # |T| row sums ~ sqrt(n) * 0.8: a bound on the magnitude of each output element
     scale = 0.5 * float(np.abs(T).sum(axis=1).max()) ** 4
     assert float((got - ref).abs().max()) <= 1e-13 * scale
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("notation", ["chemist", "physicist"])
+def test_mo_transform_wide_sampled(gpu, notation):
+    """m = 200 > 192 new functions: two column tiles of the GEMMs.  The 200^4 output is
+    checked at 3000 random positions against the quadruple sum over the 5^4 array."""
+    import torch
+
+    from gbasis_amd.engine import get_engine
+
+    print(f"[gpu case] mo transform wide n=5 m=200 {notation}", flush=True)
+    n, m = 5, 200
+    eri = _random_symmetric_eri(n, 17)
+    T = np.random.default_rng(21).standard_normal((m, n))
+    packed = torch.as_tensor(_packed_from_dense(eri), device=gpu)
+    got = get_engine("cuda").mo_transform(packed, n, T, notation)
+    idx = np.random.default_rng(4).integers(0, m, (3000, 4))
+    vals = got[tuple(torch.as_tensor(idx[:, j], device=gpu) for j in range(4))].cpu().numpy()
+    del got
+    if notation == "physicist":  # <ac|bd> at [a][c][b][d] = (ab|cd)
+        idx = idx[:, [0, 2, 1, 3]]
+    ref = np.einsum("pi,pj,pk,pl,ijkl->p", T[idx[:, 0]], T[idx[:, 1]], T[idx[:, 2]],
+                    T[idx[:, 3]], eri, optimize=True)
+    scale = np.abs(eri).max() * np.abs(T).sum(axis=1).max() ** 4
+    assert np.abs(vals - ref).max() <= 1e-14 * scale
