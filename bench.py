@@ -466,8 +466,9 @@ MO_WORKLOADS = ("high_l_pure",)
 def time_mo_transform(eng, packed, nfunc, reps=3):
     """Milliseconds and FP64 rate of the four-index transform of the packed ERIs into a
     basis of nfunc functions (gbx_mo_transform: two passes of Y = T X T^T over n x n
-    matrices, n^2 m + n m^2 / 2 FMAs per matrix; npair(n) matrices in the first pass,
-    npair(m) in the second)."""
+    matrices, n^2 m + n m^2 / 2 algorithmic FMAs per matrix, on the FP64 matrix cores,
+    whose MI355X peak equals the FP64 vector peak of 78.6 TFLOP/s; npair(n) matrices in
+    the first pass, npair(m) in the second)."""
     import torch
 
     from gbasis_amd.roofline import PEAK_FP64_VECTOR
@@ -492,6 +493,8 @@ def time_mo_transform(eng, packed, nfunc, reps=3):
     return {"n": n, "m": m, "ms": ms, "flops": flops, "tflops": flops / ms / 1e9,
             "frac": flops / ms / 1e9 / (PEAK_FP64_VECTOR / 1e12),
             "includes": "workspace and output allocation (torch caching allocator)"}
+
+
 TUNE_FILE = os.path.join(ROOT, "gbasis_amd", "tuning_mi355x.json")
 
 

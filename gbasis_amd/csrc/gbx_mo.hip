@@ -103,8 +103,16 @@ __global__ __launch_bounds__(kMoThreads, 2) void mo_gemm_kernel(MoArgs a) {
   const int64_t col_tiles = (m + kBN - 1) / kBN;
   const int64_t row0 = (tile / col_tiles) * kMoBM, col0 = (tile % col_tiles) * kBN;
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 16 * kNT;
+  // GEMM 2 keeps c >= d only: a 16-column MFMA tile whose columns all lie below the
+  // smallest d of the wavefront's rows is skipped (a wave-uniform test)
+  bool live[kNT];
+  {
+    const int64_t d_lo = kStage == 2 ? (row0 + wm) / rc : 0;
+#pragma unroll
+    for (int y = 0; y < kNT; ++y) live[y] = col0 + wn + 16 * y + 15 >= d_lo;
+  }
 
   // A loader.  Stage 1: 16 lanes along l in each of the rows tid / 16 + 16 u, row
   // (k, q) = X_{r0+q}[k][:].  Stage 2: the 64 lanes along the rows (d, q), which are
@@ -199,7 +207,8 @@ __global__ __launch_bounds__(kMoThreads, 2) void mo_gemm_kernel(MoArgs a) {
       for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int y = 0; y < kNT; ++y)
-          acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
+          if (live[y])
+            acc[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(af[x], bf[y], acc[x][y], 0, 0, 0);
     }
   }
 
