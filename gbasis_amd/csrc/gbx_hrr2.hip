@@ -33,9 +33,12 @@
 
 namespace gbx {
 
-// one workgroup per CU (its LDS arrays take most of the 160 KiB): 16 wavefronts keep the
-// level loops and the transforms busy
+// Workgroup sizes: 1024 lanes where the LDS arrays take most of the 160 KiB (one
+// workgroup per CU; 16 wavefronts keep the level loops and the transforms busy), 256
+// where they are small enough for several workgroups, and so quartets, per CU.
 constexpr int kHrr2Threads = 1024;
+constexpr int kHrr2SmallThreads = 256;
+constexpr int64_t kHrr2SmallLds = 32 * 1024;  // LDS bytes up to which the 256-lane form runs
 
 struct Hrr2Args {
   const int32_t* quart;
@@ -62,8 +65,9 @@ __device__ inline uint64_t hrr2_tri(uint64_t i, uint64_t j) {
 }
 
 // The flat index (r * A + a) * B + b of a [nr][A][B] loop, visited by a thread from
-// threadIdx.x in steps of kHrr2Threads: the steps are decomposed once, so the loops need
-// no integer division per element (B = 1 or A = 1 for two-level loops).
+// threadIdx.x in steps of kT (the workgroup size): the steps are decomposed once, so the
+// loops need no integer division per element (B = 1 or A = 1 for two-level loops).
+template <int kT>
 struct Hrr2Iter {
   int r, a, b;
   int dr, da, db;
@@ -73,8 +77,8 @@ struct Hrr2Iter {
     b = threadIdx.x - q * B;
     r = q / A;
     a = q - r * A;
-    const int qs = kHrr2Threads / B;
-    db = kHrr2Threads - qs * B;
+    const int qs = kT / B;
+    db = kT - qs * B;
     dr = qs / A;
     da = qs - dr * A;
   }
@@ -97,10 +101,11 @@ struct Hrr2Iter {
 
 // One level of a recursion for rows [0, nr) of the row buffers:
 // dst[j] = src[s1] + g_dir src[s2], g = (gx, gy, gz).
+template <int kT>
 __device__ inline void hrr2_level(double* buf, int srow, int src_half, int dst_half, int half,
                                   int nr, int size, const int32_t* ent, double gx, double gy,
                                   double gz) {
-  for (Hrr2Iter it(size, 1); it.r < nr; it.next()) {
+  for (Hrr2Iter<kT> it(size, 1); it.r < nr; it.next()) {
     const int r = it.r, j = it.a;
     const int e = ent[j];
     const int dir = e >> 24;
@@ -110,7 +115,8 @@ __device__ inline void hrr2_level(double* buf, int srow, int src_half, int dst_h
   }
 }
 
-__global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hrr2Args a) {
+template <int kT>
+__global__ __launch_bounds__(kT) void eri_hrr2_kernel(gbx_tables t, Hrr2Args a) {
   extern __shared__ __align__(16) double lds[];
   double* buf = lds;
   double* W = lds + a.w_off;
@@ -153,7 +159,7 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
     for (int e0 = 0; e0 < NA; e0 += a.rb) {
       const int nr = min(a.rb, NA - e0);
       __syncthreads();  // the buffers of the previous group (or quartet) are free
-      for (Hrr2Iter it(NC, 1); it.r < nr; it.next()) {
+      for (Hrr2Iter<kT> it(NC, 1); it.r < nr; it.next()) {
         const int r = it.r, c = it.a;
         const int64_t f = static_cast<int64_t>(c) * NA + e0 + r;
         double s = 0.0;
@@ -171,7 +177,8 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       __syncthreads();
       for (int k = 1, o = 4 + nlk + nlb; k <= nlk; ++k) {
         const int size = a.rec[3 + k];
-        hrr2_level(buf, srow, (k - 1) & 1, k & 1, a.half, nr, size, a.rec + o, CDx, CDy, CDz);
+        hrr2_level<kT>(buf, srow, (k - 1) & 1, k & 1, a.half, nr, size, a.rec + o, CDx, CDy,
+                       CDz);
         o += size;
         __syncthreads();
       }
@@ -180,7 +187,7 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       const int h = nlk & 1;
       {
         const double* M = t.out_mat + si1[2].x;
-        for (Hrr2Iter it(dim[2], nc[3]); it.r < nr; it.next()) {
+        for (Hrr2Iter<kT> it(dim[2], nc[3]); it.r < nr; it.next()) {
           const int r = it.r, jc = it.a, id = it.b;
           const int rem = jc * nc[3] + id;
           const double* F = buf + r * srow + h * a.half;
@@ -198,7 +205,7 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       __syncthreads();
       {
         const double* M = t.out_mat + si1[3].x;
-        for (Hrr2Iter it(dim[2], dim[3]); it.r < nr; it.next()) {
+        for (Hrr2Iter<kT> it(dim[2], dim[3]); it.r < nr; it.next()) {
           const int r = it.r, jc = it.a, jd = it.b;
           const int j = jc * dim[3] + jd;
           const double* T = buf + r * srow + (1 - h) * a.half + jc * nc[3];
@@ -221,21 +228,22 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
     for (int j0 = 0; j0 < NJ; j0 += a.rb) {
       const int nr = min(a.rb, NJ - j0);
       __syncthreads();  // W is complete; the buffers of the previous group are free
-      for (Hrr2Iter it(nr, 1); it.r < NA; it.next()) {
+      for (Hrr2Iter<kT> it(nr, 1); it.r < NA; it.next()) {
         const int e = it.r, r = it.a;
         buf[r * srow + e] = W[static_cast<int64_t>(e) * NJ + j0 + r];
       }
       __syncthreads();
       for (int k = 1, o = off_b0; k <= nlb; ++k) {
         const int size = a.rec[3 + nlk + k];
-        hrr2_level(buf, srow, (k - 1) & 1, k & 1, a.half, nr, size, a.rec + o, ABx, ABy, ABz);
+        hrr2_level<kT>(buf, srow, (k - 1) & 1, k & 1, a.half, nr, size, a.rec + o, ABx, ABy,
+                       ABz);
         o += size;
         __syncthreads();
       }
       const int h = nlb & 1;
       {
         const double* M = t.out_mat + si1[0].x;
-        for (Hrr2Iter it(dim[0], nc[1]); it.r < nr; it.next()) {
+        for (Hrr2Iter<kT> it(dim[0], nc[1]); it.r < nr; it.next()) {
           const int r = it.r, ja = it.a, ib = it.b;
           const int rem = ja * nc[1] + ib;
           const double* F = buf + r * srow + h * a.half;
@@ -254,7 +262,7 @@ __global__ __launch_bounds__(kHrr2Threads) void eri_hrr2_kernel(gbx_tables t, Hr
       {
         const double* M = t.out_mat + si1[1].x;
         int rr = -1, jc = 0, jd = 0;
-        for (Hrr2Iter it(dim[0], dim[1]); it.r < nr; it.next()) {
+        for (Hrr2Iter<kT> it(dim[0], dim[1]); it.r < nr; it.next()) {
           const int r = it.r, ja = it.a, jb = it.b;
           if (r != rr) {  // the column (c', d') of this row group
             rr = r;
@@ -387,20 +395,25 @@ extern "C" int gbx_eri_hrr_lds(const gbx_tables* t, int32_t la, int32_t lb, int3
   a.max_dc = max_dim_c;
   a.max_dd = max_dim_d;
   const size_t shmem = static_cast<size_t>((rb * 2 * half + w_doubles) * 8);
+  const unsigned grid = static_cast<unsigned>(std::min<int64_t>(nquart, int64_t(1) << 20));
+  if (static_cast<int64_t>(shmem) <= kHrr2SmallLds) {
+    hipLaunchKernelGGL(eri_hrr2_kernel<kHrr2SmallThreads>, dim3(grid), dim3(kHrr2SmallThreads),
+                       shmem, static_cast<hipStream_t>(stream), *t, a);
+    return check_launch("eri_hrr2_kernel");
+  }
   if (shmem > 65536) {
     static bool raised[64] = {};
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail(GBX_ERR_HIP, "eri_hrr2_kernel: no device");
     if (dev < 0 || dev >= 64 || !raised[dev]) {
-      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&eri_hrr2_kernel),
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(&eri_hrr2_kernel<kHrr2Threads>),
                               hipFuncAttributeMaxDynamicSharedMemorySize,
                               static_cast<int>(limit)) != hipSuccess)
         return fail(GBX_ERR_HIP, "eri_hrr2_kernel: could not raise the LDS limit");
       if (dev >= 0 && dev < 64) raised[dev] = true;
     }
   }
-  const unsigned grid = static_cast<unsigned>(std::min<int64_t>(nquart, int64_t(1) << 20));
-  hipLaunchKernelGGL(eri_hrr2_kernel, dim3(grid), dim3(kHrr2Threads), shmem,
+  hipLaunchKernelGGL(eri_hrr2_kernel<kHrr2Threads>, dim3(grid), dim3(kHrr2Threads), shmem,
                      static_cast<hipStream_t>(stream), *t, a);
   return check_launch("eri_hrr2_kernel");
 }

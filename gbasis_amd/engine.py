@@ -269,8 +269,10 @@ class EriEngine:
         self._masks = []
         self.set_fp32_allowed({})
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
-        # horizontal-recursion pass in LDS (gbx_eri_hrr_lds): "auto" for the classes whose
-        # cooperative pass would run on global scratch, "always" wherever it fits, "never"
+        # horizontal-recursion pass in LDS (gbx_eri_hrr_lds): "auto" after every Rys pass
+        # (whose accumulator writes are contiguous in the quartet-major layout only that
+        # pass reads) and for the other classes whose cooperative pass would run on global
+        # scratch, "always" wherever it fits, "never"
         self.hrr_lds = "auto"
         self._recipes = {}
         # deterministic: variant choices depend only on (class, primitive bin, precision),
@@ -636,7 +638,7 @@ class EriEngine:
                          else self._rys_strided(cls))
         # the LDS recursion pass reads the quartet-major layout, whose writes here are
         # contiguous across lanes
-        lds = self._lds_hrr_plan(dev, cls, mode)
+        lds = self._lds_hrr_plan(dev, cls, mode, rys=True)
         qmajor = lds is not None
         check(self.lib.gbx_eri_rys_acc(ctypes.byref(tab), ptr(self._rys_table()), int(layout),
                                        int(batch or 0), ptr(index), nindex, *cls,
@@ -666,9 +668,10 @@ class EriEngine:
                                   2 * half * 8)
         return self._recipes[cls]
 
-    def _lds_hrr_plan(self, dev, cls, mode):
+    def _lds_hrr_plan(self, dev, cls, mode, rys=False):
         """(host recipe, device recipe, largest C and D output dimensions) when the LDS
-        horizontal-recursion pass runs this class and mode, else None."""
+        horizontal-recursion pass runs this class and mode (``rys``: after the Rys
+        primitive pass), else None."""
         if self.hrr_lds == "never" or mode not in (GBX_MODE_PACKED, GBX_MODE_PACKED_ADD):
             return None
         rec, drec, w_unit, row = self._hrr_recipe(cls)
@@ -676,7 +679,8 @@ class EriEngine:
         dd = dev.max_out_dim.get(cls[3], 1)
         if w_unit * dc * dd + row > self.LDS_BYTES:
             return None
-        if self.hrr_lds == "always" or self._workspace(cls, GBX_FP64, 1, False)[1] > 0:
+        if (self.hrr_lds == "always" or rys
+                or self._workspace(cls, GBX_FP64, 1, False)[1] > 0):
             return rec, drec, dc, dd
         return None
 
