@@ -15,6 +15,21 @@ constexpr int kPrimRec = 10;          // doubles per primitive-pair record
 constexpr int kTaylorTerms = 8;       // Taylor terms of the Boys table expansion
 constexpr int kMaxShellL = 6;         // largest l per shell the host may ask for
 constexpr int64_t kCoopLdsLimit = 65536;
+constexpr int kXcdCount = 8;          // XCDs of an MI355X; workgroups go round-robin to them
+constexpr int kXcdChunk = 16;         // consecutive work items per XCD in xcd_chunked()
+
+// The hardware deals workgroup b of a G-workgroup grid to XCD b mod 8.  This maps b to a
+// work item so that each XCD runs chunks of 16 consecutive items (chunk c on XCD c mod 8):
+// the items of one chunk share the 128-byte lines of item-minor arrays (accumulators
+// [f][quartet]) in that XCD's L2, and the chunks stay interleaved over the XCDs, so a
+// cost-sorted queue still loads them evenly.  A bijection on [0, G) (the last G mod 128
+// workgroups keep their own index).
+__device__ inline int64_t xcd_chunked(int64_t b, int64_t G) {
+  const int64_t full = G / (kXcdCount * kXcdChunk) * (kXcdCount * kXcdChunk);
+  if (b >= full) return b;
+  const int64_t x = b % kXcdCount, i = b / kXcdCount;
+  return ((i / kXcdChunk) * kXcdCount + x) * kXcdChunk + i % kXcdChunk;
+}
 
 // host: record an error message for gbx_last_error() and return the code
 int fail(int code, const std::string& msg);

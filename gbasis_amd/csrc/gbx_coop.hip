@@ -136,7 +136,7 @@ __global__ __launch_bounds__(NT) void eri_coop_kernel(gbx_tables t, CoopArgs a) 
 
   const int64_t nquart = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
   const int parts = a.acc_out ? a.parts : 1;
-  for (int64_t wi = blockIdx.x; wi < nquart * parts; wi += gridDim.x) {
+  for (int64_t wi = xcd_chunked(blockIdx.x, gridDim.x); wi < nquart * parts; wi += gridDim.x) {
     const int64_t qi = wi % nquart;
     const int part = static_cast<int>(wi / nquart);
     const int bra = a.quart[2 * qi], ket = a.quart[2 * qi + 1];

@@ -134,7 +134,7 @@ __global__ __launch_bounds__(kT) void eri_hrr2_kernel(gbx_tables t, Hrr2Args a) 
   const int4* sinfo4 = reinterpret_cast<const int4*>(t.shell_info);
   const int64_t nquart = a.count ? min(a.nquart, static_cast<int64_t>(*a.count)) : a.nquart;
 
-  for (int64_t qi = blockIdx.x; qi < nquart; qi += gridDim.x) {
+  for (int64_t qi = xcd_chunked(blockIdx.x, gridDim.x); qi < nquart; qi += gridDim.x) {
     const int bra = a.quart[2 * qi], ket = a.quart[2 * qi + 1];
     const int4 pb = pinfo[bra];
     const int4 pk = pinfo[ket];
