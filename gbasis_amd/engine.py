@@ -271,8 +271,9 @@ class EriEngine:
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
         # horizontal-recursion pass in LDS (gbx_eri_hrr_lds): "auto" after every Rys pass
         # (whose accumulator writes are contiguous in the quartet-major layout only that
-        # pass reads) and for the other classes whose cooperative pass would run on global
-        # scratch, "always" wherever it fits, "never"
+        # pass reads), after the split kernels where its 256-lane form runs, and for the
+        # other classes whose cooperative pass would run on global scratch; "always"
+        # wherever it fits, "never"
         self.hrr_lds = "auto"
         self._recipes = {}
         # deterministic: variant choices depend only on (class, primitive bin, precision),
@@ -487,7 +488,8 @@ class EriEngine:
                                                ptr(count), ptr(acc), nq, R, mode, ptr(out),
                                                self._stream()), f"gbx_eri_hrr_reg{cls}")
                 return "split"
-            self._hrr_pass(dev, tab, st, cls, quartets, count, mode, acc, R, out)
+            self._hrr_pass(dev, tab, st, cls, quartets, count, mode, acc, R, out,
+                           lds=self._lds_hrr_plan(dev, cls, mode, split=True))
             return "split"
         nq = quartets.shape[0]
         R = self._coop_parts(nq, nprimq)
@@ -651,6 +653,7 @@ class EriEngine:
         return "rys"
 
     LDS_BYTES = 160 * 1024   # LDS of one gfx950 workgroup
+    HRR_SMALL_LDS = 32 * 1024  # LDS bytes up to which gbx_eri_hrr_lds runs 256 lanes
 
     def _hrr_recipe(self, cls):
         """(host int32 recipe, device copy, LDS bytes of W per unit output dimension
@@ -668,10 +671,10 @@ class EriEngine:
                                   2 * half * 8)
         return self._recipes[cls]
 
-    def _lds_hrr_plan(self, dev, cls, mode, rys=False):
+    def _lds_hrr_plan(self, dev, cls, mode, rys=False, split=False):
         """(host recipe, device recipe, largest C and D output dimensions) when the LDS
         horizontal-recursion pass runs this class and mode (``rys``: after the Rys
-        primitive pass), else None."""
+        primitive pass; ``split``: after the split register kernels), else None."""
         if self.hrr_lds == "never" or mode not in (GBX_MODE_PACKED, GBX_MODE_PACKED_ADD):
             return None
         rec, drec, w_unit, row = self._hrr_recipe(cls)
@@ -681,6 +684,9 @@ class EriEngine:
             return None
         if (self.hrr_lds == "always" or rys
                 or self._workspace(cls, GBX_FP64, 1, False)[1] > 0):
+            return rec, drec, dc, dd
+        w = w_unit * dc * dd  # the row groups and W, as gbx_eri_hrr_lds sizes them
+        if split and min(16, (self.LDS_BYTES - w) // row) * row + w <= self.HRR_SMALL_LDS:
             return rec, drec, dc, dd
         return None
 
