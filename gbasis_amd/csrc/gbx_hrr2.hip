@@ -34,11 +34,13 @@
 namespace gbx {
 
 // Workgroup sizes: 1024 lanes where the LDS arrays take most of the 160 KiB (one
-// workgroup per CU; 16 wavefronts keep the level loops and the transforms busy), 256
-// where they are small enough for several workgroups, and so quartets, per CU.
+// workgroup per CU; 16 wavefronts keep the level loops and the transforms busy), 512 and
+// 256 where they are small enough for several workgroups, and so quartets, per CU.
 constexpr int kHrr2Threads = 1024;
+constexpr int kHrr2MidThreads = 512;
 constexpr int kHrr2SmallThreads = 256;
 constexpr int64_t kHrr2SmallLds = 32 * 1024;  // LDS bytes up to which the 256-lane form runs
+constexpr int64_t kHrr2MidLds = 64 * 1024;    // and the 512-lane form
 
 struct Hrr2Args {
   const int32_t* quart;
@@ -398,6 +400,11 @@ extern "C" int gbx_eri_hrr_lds(const gbx_tables* t, int32_t la, int32_t lb, int3
   const unsigned grid = static_cast<unsigned>(std::min<int64_t>(nquart, int64_t(1) << 20));
   if (static_cast<int64_t>(shmem) <= kHrr2SmallLds) {
     hipLaunchKernelGGL(eri_hrr2_kernel<kHrr2SmallThreads>, dim3(grid), dim3(kHrr2SmallThreads),
+                       shmem, static_cast<hipStream_t>(stream), *t, a);
+    return check_launch("eri_hrr2_kernel");
+  }
+  if (static_cast<int64_t>(shmem) <= kHrr2MidLds) {  // within the default 64 KiB limit
+    hipLaunchKernelGGL(eri_hrr2_kernel<kHrr2MidThreads>, dim3(grid), dim3(kHrr2MidThreads),
                        shmem, static_cast<hipStream_t>(stream), *t, a);
     return check_launch("eri_hrr2_kernel");
   }

@@ -271,9 +271,9 @@ class EriEngine:
         self.path = "auto"   # "coop" forces the cooperative kernel for every class
         # horizontal-recursion pass in LDS (gbx_eri_hrr_lds): "auto" after every Rys pass
         # (whose accumulator writes are contiguous in the quartet-major layout only that
-        # pass reads), after the split kernels where its 256-lane form runs, and for the
-        # other classes whose cooperative pass would run on global scratch; "always"
-        # wherever it fits, "never"
+        # pass reads), after the split kernels where its 256- or 512-lane forms run, and
+        # for the other classes whose cooperative pass would run on global scratch;
+        # "always" wherever it fits, "never"
         self.hrr_lds = "auto"
         self._recipes = {}
         # deterministic: variant choices depend only on (class, primitive bin, precision),
@@ -653,7 +653,7 @@ class EriEngine:
         return "rys"
 
     LDS_BYTES = 160 * 1024   # LDS of one gfx950 workgroup
-    HRR_SMALL_LDS = 32 * 1024  # LDS bytes up to which gbx_eri_hrr_lds runs 256 lanes
+    HRR_SPLIT_LDS = 64 * 1024  # LDS bytes up to which gbx_eri_hrr_lds runs 256 or 512 lanes
 
     def _hrr_recipe(self, cls):
         """(host int32 recipe, device copy, LDS bytes of W per unit output dimension
@@ -686,7 +686,7 @@ class EriEngine:
                 or self._workspace(cls, GBX_FP64, 1, False)[1] > 0):
             return rec, drec, dc, dd
         w = w_unit * dc * dd  # the row groups and W, as gbx_eri_hrr_lds sizes them
-        if split and min(16, (self.LDS_BYTES - w) // row) * row + w <= self.HRR_SMALL_LDS:
+        if split and min(16, (self.LDS_BYTES - w) // row) * row + w <= self.HRR_SPLIT_LDS:
             return rec, drec, dc, dd
         return None
 
