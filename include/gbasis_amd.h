@@ -303,9 +303,8 @@ int gbx_eri_hrr(const gbx_tables *t, const gbx_program *prog, int32_t la, int32_
  * whose row buffers and W fit the LDS, including those whose whole recursion workspace
  * exceeds it (gbx_eri_hrr runs them on global scratch): one workgroup per quartet (256
  * lanes when the LDS arrays take at most 32 KiB, 512 up to 64 KiB, else 1024) runs the
- * ket recursion, the norms of c and d and
- * the transforms of shells C and D row by row (bra component e) into an LDS array
- * W[e][c'd'], then the bra recursion, the norms and transforms of A and B and the
+ * ket recursion, the norms of c and d and the transforms of shells C and D row by row
+ * (bra component e) into an LDS array W[e][c'd'], then the bra recursion, the norms and transforms of A and B and the
  * placement column by column (gbasis/integrals/_two_elec_int_improved.py:364-588,
  * gbasis/base_four_symm.py:160-557).  h_recipe / d_recipe: the class's recursion
  * recipe (gbasis_amd.programs.hrr_recipe, recipe_len int32) on the host and on the
