@@ -40,7 +40,6 @@ constexpr int kMoPadA = kMoBK + 1;   // LDS row stride of the A tile (doubles), 
                                      // bank conflicts of the fragment reads
 constexpr int kMoGroup = 8;          // GEMM 1 rows: groups of 8 matrices per row k
 constexpr int64_t kMoChunkDoubles = int64_t(1) << 27;  // U of one chunk: at most 1 GiB
-constexpr int kMoXcds = 8;           // workgroups are dealt round-robin to the XCDs
 constexpr int kMoFill = 32;          // tile edge of the ket-transpose fill
 
 typedef double mo_v4d __attribute__((ext_vector_type(4)));
@@ -97,8 +96,8 @@ __global__ __launch_bounds__(kMoThreads, 2) void mo_gemm_kernel(MoArgs a) {
   __shared__ double Bs[kMoBK * kPadB];
   const int64_t n = a.n, m = a.m, rc = a.rc;
   const int64_t M = kStage == 1 ? n * ((rc + kMoGroup - 1) / kMoGroup * kMoGroup) : m * rc;
-  const int64_t per_xcd = (a.tiles + kMoXcds - 1) / kMoXcds;
-  const int64_t tile = (blockIdx.x % kMoXcds) * per_xcd + blockIdx.x / kMoXcds;
+  const int64_t per_xcd = (a.tiles + kXcdCount - 1) / kXcdCount;
+  const int64_t tile = (blockIdx.x % kXcdCount) * per_xcd + blockIdx.x / kXcdCount;
   if (tile >= a.tiles) return;  // the whole workgroup leaves before any barrier
   const int64_t col_tiles = (m + kBN - 1) / kBN;
   const int64_t row0 = (tile / col_tiles) * kMoBM, col0 = (tile % col_tiles) * kBN;
@@ -317,7 +316,7 @@ int mo_gemm_nt(MoArgs a, hipStream_t s) {
   const int64_t rows = kStage == 1 ? a.n * ((a.rc + kMoGroup - 1) / kMoGroup * kMoGroup)
                                    : a.m * a.rc;
   a.tiles = ((rows + kMoBM - 1) / kMoBM) * ((a.m + 32 * kNT - 1) / (32 * kNT));
-  const int64_t blocks = (a.tiles + kMoXcds - 1) / kMoXcds * kMoXcds;
+  const int64_t blocks = (a.tiles + kXcdCount - 1) / kXcdCount * kXcdCount;
   hipLaunchKernelGGL((mo_gemm_kernel<kPass, kStage, kNT>), dim3(static_cast<unsigned>(blocks)),
                      dim3(kMoThreads), 0, s, a);
   return check_launch(kStage == 1 ? "mo_gemm_kernel (stage 1)" : "mo_gemm_kernel (stage 2)");
