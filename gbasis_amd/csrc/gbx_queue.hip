@@ -35,6 +35,7 @@
 namespace gbx {
 
 constexpr int kQueueThreads = 256;
+constexpr int kQueueSlots = 64;  // LDS keys per workgroup and phase of the fill pass (2^k)
 
 struct QueueArgs {
   int32_t n_mol;
@@ -121,6 +122,14 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
     }
   }
   const int lane = threadIdx.x & 63;
+  // fill: the positions are reserved per workgroup and key, not per wavefront and key.
+  // The wavefronts add their counts into an LDS table of up to kQueueSlots keys (open
+  // addressing on the key); one lane per occupied slot then reserves the workgroup's
+  // block with one device atomic.  A key that finds no slot reserves its wavefront's
+  // block directly.
+  __shared__ int s_key[kQueueSlots];
+  __shared__ int s_cnt[kQueueSlots];
+  __shared__ int s_base[kQueueSlots];
   for (int phase = 0; phase < 4; ++phase) {
     const int eb = sb[phase & 1], ek = sk[phase >> 1];
     const bool emit = valid && eb >= 0 && ek >= 0;
@@ -137,6 +146,14 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       key = (((phase * 4096 + cls) * 2 + dense) * 2 + fp32s) * 16 + prim_bin(np);
       primq = static_cast<uint64_t>(np);
     }
+    if (kFill) {
+      if (threadIdx.x < kQueueSlots) {
+        s_key[threadIdx.x] = -1;
+        s_cnt[threadIdx.x] = 0;
+      }
+      __syncthreads();
+    }
+    int my_slot = -1, my_off = 0;  // fill: this lane's slot and offset in its block
     // wavefront-aggregated reservation per distinct key
     uint64_t pending = __ballot(emit);
     while (pending) {
@@ -147,17 +164,27 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       const int n = __popcll(same);
       const int rank = __popcll(same & ((uint64_t(1) << lane) - 1));
       if (kFill) {
-        int base = 0;
-        if (lane == leader) base = atomicAdd(a.bucket_cursor + lkey, n);
-        base = __shfl(base, leader, 64);
-        if (mine) {
-          // a bucket sized too small never writes into its neighbour: its cursor then
-          // exceeds its capacity, which the host can detect
-          const int64_t pos = a.bucket_offset[lkey] + base + rank;
-          if (pos < a.bucket_offset[lkey + 1]) {
-            a.out[2 * pos] = eb;
-            a.out[2 * pos + 1] = ek;
+        int slot = -1, off = 0;
+        if (lane == leader) {
+          int h = lkey & (kQueueSlots - 1);
+          for (int probe = 0; probe < kQueueSlots; ++probe) {
+            const int prev = atomicCAS(&s_key[h], -1, lkey);
+            if (prev == -1 || prev == lkey) {
+              slot = h;
+              break;
+            }
+            h = (h + 1) & (kQueueSlots - 1);
           }
+          if (slot >= 0)
+            off = atomicAdd(&s_cnt[slot], n);
+          else
+            off = atomicAdd(a.bucket_cursor + lkey, n);  // the table is full
+        }
+        slot = __shfl(slot, leader, 64);
+        off = __shfl(off, leader, 64);
+        if (mine) {
+          my_slot = slot;
+          my_off = off + rank;
         }
       } else {
         if (lane == leader) atomicAdd(a.bucket_count + lkey, n);
@@ -178,6 +205,24 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
         }
       }
       pending &= ~same;
+    }
+    if (kFill) {
+      __syncthreads();  // the table holds the workgroup's counts
+      if (threadIdx.x < kQueueSlots && s_key[threadIdx.x] >= 0)
+        s_base[threadIdx.x] = atomicAdd(a.bucket_cursor + s_key[threadIdx.x],
+                                        s_cnt[threadIdx.x]);
+      __syncthreads();
+      if (emit) {
+        // a bucket sized too small never writes into its neighbour: its cursor then
+        // exceeds its capacity, which the host can detect
+        const int64_t pos =
+            a.bucket_offset[key] + (my_slot >= 0 ? s_base[my_slot] : 0) + my_off;
+        if (pos < a.bucket_offset[key + 1]) {
+          a.out[2 * pos] = eb;
+          a.out[2 * pos + 1] = ek;
+        }
+      }
+      __syncthreads();  // s_base is read before the next phase clears the table
     }
   }
 }
