@@ -35,7 +35,7 @@
 namespace gbx {
 
 constexpr int kQueueThreads = 256;
-constexpr int kQueueSlots = 64;  // LDS keys per workgroup and phase of the fill pass (2^k)
+constexpr int kQueueSlots = 64;  // LDS keys per workgroup and phase (a power of two)
 
 struct QueueArgs {
   int32_t n_mol;
@@ -65,6 +65,18 @@ __device__ inline int prim_bin(int x) {
   int b = 0;
   while (b < kMaxBin && (1 << b) < x) ++b;
   return b;
+}
+
+// The slot of key in an LDS table of kQueueSlots keys (-1 = free), claimed by open
+// addressing; -1 when the table is full.
+__device__ inline int queue_slot(int* s_key, int key) {
+  int h = key & (kQueueSlots - 1);
+  for (int probe = 0; probe < kQueueSlots; ++probe) {
+    const int prev = atomicCAS(&s_key[h], -1, key);
+    if (prev == -1 || prev == key) return h;
+    h = (h + 1) & (kQueueSlots - 1);
+  }
+  return -1;
 }
 
 // candidate t of a molecule with P pairs -> (p >= q), t = p (p + 1) / 2 + q
@@ -122,14 +134,16 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
     }
   }
   const int lane = threadIdx.x & 63;
-  // fill: the positions are reserved per workgroup and key, not per wavefront and key.
-  // The wavefronts add their counts into an LDS table of up to kQueueSlots keys (open
-  // addressing on the key); one lane per occupied slot then reserves the workgroup's
-  // block with one device atomic.  A key that finds no slot reserves its wavefront's
-  // block directly.
+  // Counts and positions go to the buckets per workgroup and key, not per wavefront and
+  // key: the wavefronts add into an LDS table of up to kQueueSlots keys, then one lane
+  // per occupied slot makes the workgroup's device atomics (count pass: the count, ERIs
+  // and primitive quartets; fill pass: the reservation of the workgroup's block).  A
+  // key that finds no slot goes to the device atomics per wavefront.
   __shared__ int s_key[kQueueSlots];
   __shared__ int s_cnt[kQueueSlots];
   __shared__ int s_base[kQueueSlots];
+  __shared__ unsigned long long s_eris[kQueueSlots];
+  __shared__ unsigned long long s_pq[kQueueSlots];
   for (int phase = 0; phase < 4; ++phase) {
     const int eb = sb[phase & 1], ek = sk[phase >> 1];
     const bool emit = valid && eb >= 0 && ek >= 0;
@@ -146,13 +160,13 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       key = (((phase * 4096 + cls) * 2 + dense) * 2 + fp32s) * 16 + prim_bin(np);
       primq = static_cast<uint64_t>(np);
     }
-    if (kFill) {
-      if (threadIdx.x < kQueueSlots) {
-        s_key[threadIdx.x] = -1;
-        s_cnt[threadIdx.x] = 0;
-      }
-      __syncthreads();
+    if (threadIdx.x < kQueueSlots) {
+      s_key[threadIdx.x] = -1;
+      s_cnt[threadIdx.x] = 0;
+      s_eris[threadIdx.x] = 0ull;
+      s_pq[threadIdx.x] = 0ull;
     }
+    __syncthreads();
     int my_slot = -1, my_off = 0;  // fill: this lane's slot and offset in its block
     // wavefront-aggregated reservation per distinct key
     uint64_t pending = __ballot(emit);
@@ -166,15 +180,7 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
       if (kFill) {
         int slot = -1, off = 0;
         if (lane == leader) {
-          int h = lkey & (kQueueSlots - 1);
-          for (int probe = 0; probe < kQueueSlots; ++probe) {
-            const int prev = atomicCAS(&s_key[h], -1, lkey);
-            if (prev == -1 || prev == lkey) {
-              slot = h;
-              break;
-            }
-            h = (h + 1) & (kQueueSlots - 1);
-          }
+          slot = queue_slot(s_key, lkey);
           if (slot >= 0)
             off = atomicAdd(&s_cnt[slot], n);
           else
@@ -187,20 +193,31 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
           my_off = off + rank;
         }
       } else {
-        if (lane == leader) atomicAdd(a.bucket_count + lkey, n);
+        unsigned long long e = 0ull, pq = 0ull;
         if (a.bucket_eris) {
-          // sums over the lanes of this key, one atomic per key and wavefront; the
-          // packed slots of a quartet count once, in phase 0
-          unsigned long long e = (mine && phase == 0) ? static_cast<unsigned long long>(eris)
-                                                       : 0ull;
-          unsigned long long pq = mine ? static_cast<unsigned long long>(primq) : 0ull;
+          // sums over the lanes of this key; the packed slots of a quartet count once,
+          // in phase 0
+          e = (mine && phase == 0) ? static_cast<unsigned long long>(eris) : 0ull;
+          pq = mine ? static_cast<unsigned long long>(primq) : 0ull;
           for (int off = 32; off > 0; off >>= 1) {
             e += __shfl_xor(e, off, 64);
             pq += __shfl_xor(pq, off, 64);
           }
-          if (lane == leader) {
-            atomicAdd(a.bucket_eris + lkey, e);
-            atomicAdd(a.bucket_primq + lkey, pq);
+        }
+        if (lane == leader) {
+          const int slot = queue_slot(s_key, lkey);
+          if (slot >= 0) {
+            atomicAdd(&s_cnt[slot], n);
+            if (a.bucket_eris) {
+              atomicAdd(&s_eris[slot], e);
+              atomicAdd(&s_pq[slot], pq);
+            }
+          } else {  // the table is full
+            atomicAdd(a.bucket_count + lkey, n);
+            if (a.bucket_eris) {
+              atomicAdd(a.bucket_eris + lkey, e);
+              atomicAdd(a.bucket_primq + lkey, pq);
+            }
           }
         }
       }
@@ -223,6 +240,17 @@ __global__ __launch_bounds__(kQueueThreads) void quartet_queue_kernel(QueueArgs 
         }
       }
       __syncthreads();  // s_base is read before the next phase clears the table
+    } else {
+      __syncthreads();  // the table holds the workgroup's sums
+      if (threadIdx.x < kQueueSlots && s_key[threadIdx.x] >= 0) {
+        const int k = s_key[threadIdx.x];
+        atomicAdd(a.bucket_count + k, s_cnt[threadIdx.x]);
+        if (a.bucket_eris) {
+          atomicAdd(a.bucket_eris + k, s_eris[threadIdx.x]);
+          atomicAdd(a.bucket_primq + k, s_pq[threadIdx.x]);
+        }
+      }
+      __syncthreads();  // the table is read before the next phase clears it
     }
   }
 }
